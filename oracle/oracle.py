@@ -1,0 +1,259 @@
+"""ctypes front-end of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py``, always as the checker.  The product
+(``neopz_amd``) never imports it.
+
+The C restatement follows NeoPZ's own algorithms (file:line citations in
+``oracle/pz_oracle.c``).  The pins of each function against the reference's
+golden vectors, closed forms and invariants live in ``tests/test_oracle.py``.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+
+POISSON, DARCY, ELAST3D, MOCK1D = 0, 1, 2, 3
+LINE, QUAD, HEX = 1, 2, 3
+FORCE_NONE, FORCE_CONST, FORCE_BUMPS = 0, 1, 2
+
+
+class _Problem(C.Structure):
+    _fields_ = [
+        ("material", C.c_int), ("dim", C.c_int), ("p", C.c_int),
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("scale", C.c_double), ("E", C.c_double), ("nu", C.c_double),
+        ("el_coef", C.POINTER(C.c_double)),
+        ("force_kind", C.c_int), ("force_porder", C.c_int),
+        ("force_const", C.c_double * 3),
+        ("nbumps", C.c_int), ("bumps", C.POINTER(C.c_double)),
+        ("bump_width", C.c_double),
+        ("bc_type", C.c_int * 6), ("bc_val", (C.c_double * 3) * 6),
+    ]
+
+
+class _Mesh(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int), ("p", C.c_int), ("nstate", C.c_int),
+        ("nnodes", C.c_int64), ("xyz", C.POINTER(C.c_double)),
+        ("nel", C.c_int64), ("nvol", C.c_int64),
+        ("etype", C.POINTER(C.c_int)), ("bcface", C.POINTER(C.c_int)),
+        ("enodes", C.POINTER(C.c_int64)), ("econn", C.POINTER(C.c_int64)),
+        ("nconn", C.c_int64), ("conn_nshape", C.POINTER(C.c_int)),
+        ("conn_seq", C.POINTER(C.c_int64)), ("blk_pos", C.POINTER(C.c_int64)),
+        ("blk_size", C.POINTER(C.c_int)), ("neq", C.c_int64),
+    ]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = C.CDLL(_LIB)
+        P, M = C.POINTER(_Problem), C.POINTER(_Mesh)
+        dp, i64p = C.POINTER(C.c_double), C.POINTER(C.c_int64)
+        L.orc_gauss_legendre.argtypes = [C.c_int, dp, dp]
+        L.orc_mesh_build.argtypes = [P, M]
+        L.orc_mesh_free.argtypes = [M]
+        L.orc_calcstiff.argtypes = [P, M, C.c_int64, dp, dp, i64p]
+        L.orc_shape.argtypes = [C.c_int, C.c_int, dp, dp, dp]
+        L.orc_contribute.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, dp, dp,
+                                     dp, C.c_double, C.c_double, C.c_double, dp, dp]
+        L.orc_setup_pattern.argtypes = [M, C.POINTER(i64p), C.POINTER(i64p)]
+        L.orc_setup_pattern.restype = C.c_int64
+        L.orc_assemble.argtypes = [P, M, i64p, i64p, dp, dp]
+        L.orc_assemble_range.argtypes = [P, M, i64p, i64p, dp, dp, C.c_int64, C.c_int64]
+        L.orc_addkel.argtypes = [i64p, i64p, dp, C.c_int, dp, i64p]
+        L.orc_multadd.argtypes = [C.c_int64, i64p, i64p, dp, dp, dp, dp, C.c_double, C.c_double]
+        L.orc_pcg.argtypes = [C.c_int64, i64p, i64p, dp, dp, dp, i64p, dp, C.c_int, dp]
+        L.orc_nnz_closed_form.argtypes = [C.c_int] * 6
+        L.orc_nnz_closed_form.restype = C.c_int64
+        L.free = C.CDLL(None).free
+        L.free.argtypes = [C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+def gauss_legendre(order):
+    loc = np.zeros(64)
+    w = np.zeros(64)
+    n = lib().orc_gauss_legendre(order, _dp(loc), _dp(w))
+    return loc[:n].copy(), w[:n].copy()
+
+
+def shape(etype, p, xi):
+    xi = np.ascontiguousarray(xi, dtype=np.float64)
+    phi = np.zeros(27)
+    dphi = np.zeros(81)
+    ns = lib().orc_shape(etype, p, _dp(np.r_[xi, np.zeros(3 - len(xi))]), _dp(phi), _dp(dphi))
+    dim = {LINE: 1, QUAD: 2, HEX: 3}[etype]
+    return phi[:ns].copy(), dphi[: dim * ns].reshape(dim, ns).copy()
+
+
+def contribute(material, dim, weight, phi, dphix, f=(0.0, 0.0, 0.0), kappa=1.0, E=1.0, nu=0.3,
+               ek=None, ef=None):
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    dphix = np.ascontiguousarray(dphix, dtype=np.float64)
+    nsh = phi.shape[0]
+    ne = nsh * (3 if material == ELAST3D else 1)
+    if ek is None:
+        ek = np.zeros((ne, ne))
+    if ef is None:
+        ef = np.zeros(ne)
+    fa = np.ascontiguousarray(f, dtype=np.float64)
+    lib().orc_contribute(material, dim, nsh, ne, weight, _dp(phi), _dp(dphix), _dp(fa), kappa,
+                         E, nu, _dp(ek), _dp(ef))
+    return ek, ef
+
+
+class Oracle:
+    """A problem instance: structured mesh + first-touch numbering on the CPU.
+
+    ``spec`` is a :class:`neopz_amd.problems.Problem`-like object (attribute
+    access) describing material, mesh sizes, coefficients, loads and BCs.
+    """
+
+    def __init__(self, spec):
+        self.spec = spec
+        self._keep = []
+        pb = _Problem()
+        pb.material, pb.dim, pb.p = spec.material, spec.dim, spec.p
+        pb.nx, pb.ny, pb.nz = spec.nx, spec.ny, spec.nz
+        pb.scale, pb.E, pb.nu = spec.scale, spec.E, spec.nu
+        if spec.el_coef is not None:
+            c = np.ascontiguousarray(spec.el_coef, dtype=np.float64)
+            self._keep.append(c)
+            pb.el_coef = _dp(c)
+        pb.force_kind, pb.force_porder = spec.force_kind, spec.force_porder
+        for k in range(3):
+            pb.force_const[k] = spec.force_const[k]
+        if spec.bumps is not None and len(spec.bumps):
+            b = np.ascontiguousarray(spec.bumps, dtype=np.float64).reshape(-1, 6)
+            self._keep.append(b)
+            pb.nbumps = b.shape[0]
+            pb.bumps = _dp(b)
+        pb.bump_width = spec.bump_width
+        for f in range(6):
+            pb.bc_type[f] = spec.bc_type[f]
+            for k in range(3):
+                pb.bc_val[f][k] = spec.bc_val[f][k]
+        self.pb = pb
+        self.m = _Mesh()
+        if lib().orc_mesh_build(C.byref(pb), C.byref(self.m)) != 0:
+            raise RuntimeError("oracle mesh build failed")
+        self.neq = self.m.neq
+        self.ia = self.ja = None
+
+    def __del__(self):
+        try:
+            lib().orc_mesh_free(C.byref(self.m))
+        except Exception:
+            pass
+
+    # --- numbering ---------------------------------------------------------
+    def connects(self):
+        m = self.m
+        econn = np.ctypeslib.as_array(m.econn, shape=(27 * m.nel,)).reshape(m.nel, 27).copy()
+        blk_pos = np.ctypeslib.as_array(m.blk_pos, shape=(m.nconn,)).copy()
+        blk_size = np.ctypeslib.as_array(m.blk_size, shape=(m.nconn,)).copy()
+        return econn, blk_pos, blk_size
+
+    def pattern(self):
+        if self.ia is None:
+            ia = C.POINTER(C.c_int64)()
+            ja = C.POINTER(C.c_int64)()
+            nnz = lib().orc_setup_pattern(C.byref(self.m), C.byref(ia), C.byref(ja))
+            self.ia = np.ctypeslib.as_array(ia, shape=(self.neq + 1,)).copy()
+            self.ja = np.ctypeslib.as_array(ja, shape=(max(nnz, 1),))[:nnz].copy()
+            lib().free(ia)
+            lib().free(ja)
+        return self.ia, self.ja
+
+    def calcstiff(self, iel):
+        ek = np.zeros(81 * 81)
+        ef = np.zeros(81)
+        dest = np.zeros(81, dtype=np.int64)
+        ne = lib().orc_calcstiff(C.byref(self.pb), C.byref(self.m), iel, _dp(ek), _dp(ef),
+                                 _ip(dest))
+        return ek[: ne * ne].reshape(ne, ne).copy(), ef[:ne].copy(), dest[:ne].copy()
+
+    def assemble(self):
+        ia, ja = self.pattern()
+        a = np.zeros(len(ja))
+        rhs = np.zeros(self.neq)
+        lib().orc_assemble(C.byref(self.pb), C.byref(self.m), _ip(ia), _ip(ja), _dp(a), _dp(rhs))
+        return a, rhs
+
+    def assemble_range(self, e0, e1, a=None, rhs=None):
+        ia, ja = self.pattern()
+        if a is None:
+            a = np.zeros(len(ja))
+        if rhs is None:
+            rhs = np.zeros(self.neq)
+        lib().orc_assemble_range(C.byref(self.pb), C.byref(self.m), _ip(ia), _ip(ja), _dp(a),
+                                 _dp(rhs), e0, e1)
+        return a, rhs
+
+
+def addkel(ia, ja, a, ek, dest):
+    ek = np.ascontiguousarray(ek, dtype=np.float64)
+    dest = np.ascontiguousarray(dest, dtype=np.int64)
+    lib().orc_addkel(_ip(ia), _ip(ja), _dp(a), ek.shape[0], _dp(ek), _ip(dest))
+
+
+def multadd(ia, ja, a, x, y=None, alpha=1.0, beta=0.0):
+    n = len(ia) - 1
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    z = np.zeros(n)
+    yy = np.ascontiguousarray(y if y is not None else np.zeros(n), dtype=np.float64)
+    lib().orc_multadd(n, _ip(ia), _ip(ja), _dp(a), _dp(x), _dp(yy), _dp(z), alpha, beta)
+    return z
+
+
+def pcg(ia, ja, a, b, maxit=100000, tol=1e-10, wasted_spmv=True, history=False):
+    n = len(ia) - 1
+    x = np.zeros(n)
+    it = C.c_int64(maxit)
+    t = C.c_double(tol)
+    hist = np.zeros(maxit) if history else None
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    ret = lib().orc_pcg(n, _ip(ia), _ip(ja), _dp(a), _dp(b), _dp(x), C.byref(it), C.byref(t),
+                        1 if wasted_spmv else 0, _dp(hist) if history else None)
+    out = {"x": x, "iters": it.value, "tol": t.value, "converged": ret == 0}
+    if history:
+        out["history"] = hist[: it.value].copy()
+    return out
+
+
+def nnz_closed_form(dim, p, nstate, nx, ny, nz):
+    return lib().orc_nnz_closed_form(dim, p, nstate, nx, ny, nz)
+
+
+def upper_to_dense(ia, ja, a):
+    n = len(ia) - 1
+    K = np.zeros((n, n))
+    for r in range(n):
+        for k in range(ia[r], ia[r + 1]):
+            K[r, ja[k]] = a[k]
+            K[ja[k], r] = a[k]
+    return K

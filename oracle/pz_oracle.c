@@ -1,0 +1,884 @@
+/*
+ * pz_oracle.c -- CPU restatement of NeoPZ's linear H1 FEM path (checker only).
+ *
+ * TEST INFRASTRUCTURE: loaded only by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py.  The product path never links or calls it.
+ *
+ * Every function cites the reference file:line whose algorithm it restates.
+ * Pinning of each function (golden vectors / closed forms / invariants) is in
+ * tests/test_oracle.py; see the header comment there.
+ */
+#include "pz_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------ */
+/* Topology tables (Topology/tpzquadrilateral.h:323-324, Topology/tpzcube.h: */
+/* 271-275, Topology/tpzcube.cpp:49-82).  These have to be the reference's:  */
+/* they fix side order, hence connect creation order and shape order.        */
+/* ------------------------------------------------------------------------ */
+static const int QUAD_EDGE[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
+static const int HEX_EDGE[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {0, 4}, {1, 5},
+                                    {2, 6}, {3, 7}, {4, 5}, {5, 6}, {6, 7}, {7, 4}};
+static const int HEX_FACE[6][4] = {{0, 1, 2, 3}, {0, 1, 5, 4}, {1, 2, 6, 5},
+                                   {3, 2, 6, 7}, {0, 3, 7, 4}, {4, 5, 6, 7}};
+
+static int etype_ncorner(int t) { return t == ORC_LINE ? 2 : t == ORC_QUAD ? 4 : 8; }
+static int etype_nsides(int t) { return t == ORC_LINE ? 3 : t == ORC_QUAD ? 9 : 27; }
+static int etype_dim(int t) { return t == ORC_LINE ? 1 : t == ORC_QUAD ? 2 : 3; }
+
+/* dimension of side s of element type t */
+static int side_dim(int t, int s) {
+  int nc = etype_ncorner(t);
+  if (s < nc) return 0;
+  if (t == ORC_LINE) return 1;
+  if (t == ORC_QUAD) return s < 8 ? 1 : 2;
+  if (s < 20) return 1;
+  if (s < 26) return 2;
+  return 3;
+}
+
+/* TPZShape*::NConnectShapeF (Shape/pzshapequad.cpp:468-482,
+ * Shape/pzshapecube.cpp:573-584): p-1 per edge, (p-1)^2 per quad face,
+ * (p-1)^3 per hex interior. */
+static int nconnect_shape(int t, int s, int p) {
+  int d = side_dim(t, s);
+  if (d == 0) return 1;
+  int q = p - 1;
+  if (q <= 0) return 0;
+  if (d == 1) return q;
+  if (d == 2) return q * q;
+  return q * q * q;
+}
+
+/* local corner nodes of side s */
+static int side_corners(int t, int s, int *loc) {
+  int nc = etype_ncorner(t);
+  if (s < nc) { loc[0] = s; return 1; }
+  if (t == ORC_LINE) { loc[0] = 0; loc[1] = 1; return 2; }
+  if (t == ORC_QUAD) {
+    if (s < 8) { loc[0] = QUAD_EDGE[s - 4][0]; loc[1] = QUAD_EDGE[s - 4][1]; return 2; }
+    for (int i = 0; i < 4; i++) loc[i] = i;
+    return 4;
+  }
+  if (s < 20) { loc[0] = HEX_EDGE[s - 8][0]; loc[1] = HEX_EDGE[s - 8][1]; return 2; }
+  if (s < 26) { for (int i = 0; i < 4; i++) loc[i] = HEX_FACE[s - 20][i]; return 4; }
+  for (int i = 0; i < 8; i++) loc[i] = i;
+  return 8;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Gauss-Legendre (Integral/tpzgaussrule.cpp:170-240): n = (int)(0.51*(order */
+/* +2)), Newton iteration in long double from cos(pi(i+0.75)/(n+0.5)),       */
+/* stored as (-z_i, +z_i) pairs.                                              */
+/* ------------------------------------------------------------------------ */
+int orc_gauss_legendre(int order, double *loc, double *w) {
+  int n = (int)(0.51 * (order + 2));
+  if (n < 1) n = 1;
+  long double tol = 1.0e-18L; /* below long double eps*|z|; Newton stalls at roundoff */
+  int m = (n + 1) / 2;
+  for (int i = 0; i < m; i++) {
+    long double p1 = (long double)i + 0.75L, p2 = (long double)n + 0.5L, p3, pp = 1, z1, z;
+    z = cosl((3.14159265358979323846264338327950288L * p1) / p2);
+    for (int it = 0; it < 100; it++) {
+      p1 = 1.0L;
+      p2 = 0.0L;
+      for (int j = 0; j < n; j++) {
+        p3 = p2;
+        p2 = p1;
+        p1 = ((2.0L * (long double)j + 1.0L) * z * p2 - ((long double)j) * p3) / ((long double)j + 1.0L);
+      }
+      long double den = z * z - 1.0L;
+      pp = (long double)n * (z * p1 - p2) / den;
+      z1 = z;
+      z = z1 - p1 / pp;
+      if (fabsl(z - z1) <= tol) break;
+    }
+    long double weight = 2.0L / ((1.0L - z * z) * pp * pp);
+    loc[2 * i] = (double)(-z);
+    w[2 * i] = (double)weight;
+    if (2 * i + 1 < n) {
+      loc[2 * i + 1] = (double)z;
+      w[2 * i + 1] = (double)weight;
+    }
+  }
+  return n;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Shape functions, p <= 2 (Shape/TPZShapeH1.cpp:40-116 + ShapeCorner /     */
+/* ShapeGenerating of Shape/pzshapelinear.cpp:203-274,                       */
+/* Shape/pzshapequad.cpp:36-98, Shape/pzshapecube.cpp:36-160).               */
+/* phi[ns], dphi[d*ns + i] (d-th reference derivative of shape i).           */
+/* ------------------------------------------------------------------------ */
+int orc_shape(int t, int p, const double *xi, double *phi, double *dphi) {
+  int dim = etype_dim(t), nc = etype_ncorner(t), nsides = etype_nsides(t);
+  /* raw (unscaled) generating functions on all sides, dims: [side], [d][side] */
+  double ph[27], dph[3][27];
+  if (t == ORC_LINE) {
+    ph[0] = (1. - xi[0]) / 2.; ph[1] = (1. + xi[0]) / 2.;
+    dph[0][0] = -0.5; dph[0][1] = 0.5;
+  } else if (t == ORC_QUAD) {
+    double x[2] = {(1. - xi[0]) / 2., (1. + xi[0]) / 2.}, dx[2] = {-0.5, 0.5};
+    double y[2] = {(1. - xi[1]) / 2., (1. + xi[1]) / 2.}, dy[2] = {-0.5, 0.5};
+    const int cx[4] = {0, 1, 1, 0}, cy[4] = {0, 0, 1, 1};
+    for (int n = 0; n < 4; n++) {
+      ph[n] = x[cx[n]] * y[cy[n]];
+      dph[0][n] = dx[cx[n]] * y[cy[n]];
+      dph[1][n] = x[cx[n]] * dy[cy[n]];
+    }
+  } else {
+    double x[2] = {(1. - xi[0]) / 2., (1. + xi[0]) / 2.};
+    double y[2] = {(1. - xi[1]) / 2., (1. + xi[1]) / 2.};
+    double z[2] = {(1. - xi[2]) / 2., (1. + xi[2]) / 2.};
+    const double dd[2] = {-0.5, 0.5};
+    const int cx[8] = {0, 1, 1, 0, 0, 1, 1, 0}, cy[8] = {0, 0, 1, 1, 0, 0, 1, 1},
+              cz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+    for (int n = 0; n < 8; n++) {
+      ph[n] = x[cx[n]] * y[cy[n]] * z[cz[n]];
+      dph[0][n] = dd[cx[n]] * y[cy[n]] * z[cz[n]];
+      dph[1][n] = x[cx[n]] * dd[cy[n]] * z[cz[n]];
+      dph[2][n] = x[cx[n]] * y[cy[n]] * dd[cz[n]];
+    }
+  }
+  int ns = nc;
+  for (int i = 0; i < nc; i++) {
+    phi[i] = ph[i];
+  }
+  if (p < 2) {
+    for (int d = 0; d < dim; d++)
+      for (int i = 0; i < nc; i++) dphi[d * nc + i] = dph[d][i];
+    return nc;
+  }
+  /* raw products: edge = c_a*c_b, face = c0*c2 of the face, volume = c0*c6 */
+  for (int s = nc; s < nsides; s++) {
+    int a, b;
+    if (t == ORC_LINE) { a = 0; b = 1; }
+    else if (t == ORC_QUAD) {
+      if (s < 8) { a = QUAD_EDGE[s - 4][0]; b = QUAD_EDGE[s - 4][1]; }
+      else { a = 0; b = 2; }
+    } else {
+      if (s < 20) { a = HEX_EDGE[s - 8][0]; b = HEX_EDGE[s - 8][1]; }
+      else if (s < 26) { a = HEX_FACE[s - 20][0]; b = HEX_FACE[s - 20][2]; }
+      else { a = 0; b = 6; }
+    }
+    ph[s] = ph[a] * ph[b];
+    for (int d = 0; d < dim; d++) dph[d][s] = dph[d][a] * ph[b] + ph[a] * dph[d][b];
+  }
+  /* "make the generating functions linear and unitary": add the raw values of
+   * all higher-dimensional sides containing s (ascending side order, as in
+   * Topology/tpzcube.cpp:49-82 highsides), then scale by 4/16/64. */
+  for (int s = nc; s < nsides; s++) {
+    int sd = side_dim(t, s);
+    if (sd < dim) {
+      int lc[8], nlc = side_corners(t, s, lc);
+      for (int h = s + 1; h < nsides; h++) {
+        if (side_dim(t, h) <= sd) continue;
+        int hc[8], nhc = side_corners(t, h, hc), contained = 1;
+        for (int k = 0; k < nlc && contained; k++) {
+          int f = 0;
+          for (int l = 0; l < nhc; l++) f |= (hc[l] == lc[k]);
+          contained = f;
+        }
+        if (!contained) continue;
+        ph[s] += ph[h];
+        for (int d = 0; d < dim; d++) dph[d][s] += dph[d][h];
+      }
+    }
+    double mult = sd == 1 ? 4. : sd == 2 ? 16. : 64.;
+    ph[s] *= mult;
+    for (int d = 0; d < dim; d++) dph[d][s] *= mult;
+  }
+  for (int s = nc; s < nsides; s++) phi[ns++] = ph[s];
+  for (int d = 0; d < dim; d++)
+    for (int i = 0; i < ns; i++) dphi[d * ns + i] = dph[d][i];
+  return ns;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Hash map: sorted corner-node tuple of a side -> connect index.            */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  int64_t *keys; /* 4 per slot, -1 = empty */
+  int64_t *val;
+  int64_t cap;
+} sidemap;
+
+static uint64_t hash4(const int64_t *k) {
+  uint64_t h = 1469598103934665603ULL;
+  for (int i = 0; i < 4; i++) {
+    h ^= (uint64_t)k[i] + 0x9e3779b97f4a7c15ULL + (h << 6) + (h >> 2);
+    h *= 1099511628211ULL;
+  }
+  return h;
+}
+
+static int64_t sidemap_find_or_insert(sidemap *m, const int64_t *key, int64_t newval, int *inserted) {
+  uint64_t h = hash4(key) & (uint64_t)(m->cap - 1);
+  for (;;) {
+    int64_t *k = m->keys + 4 * h;
+    if (k[0] == -1) {
+      memcpy(k, key, 4 * sizeof(int64_t));
+      m->val[h] = newval;
+      *inserted = 1;
+      return newval;
+    }
+    if (k[0] == key[0] && k[1] == key[1] && k[2] == key[2] && k[3] == key[3]) {
+      *inserted = 0;
+      return m->val[h];
+    }
+    h = (h + 1) & (uint64_t)(m->cap - 1);
+  }
+}
+
+static void sort_small(int64_t *a, int n) {
+  for (int i = 1; i < n; i++) {
+    int64_t v = a[i];
+    int j = i - 1;
+    while (j >= 0 && a[j] > v) { a[j + 1] = a[j]; j--; }
+    a[j + 1] = v;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Structured mesh + first-touch numbering.                                   */
+/* Element order: volume elements (x fastest), then boundary elements face by */
+/* face (faces 0..5 = x=0,x=1,y=0,y=1,z=0,z=1).  Connects are created in     */
+/* element order, side order, when no earlier element shares the side:        */
+/* TPZCreateApproximationSpace::BuildMesh (Mesh/pzcreateapproxspace.cpp:     */
+/* 252-325), TPZCompElH1 ctor (Mesh/TPZCompElH1.cpp:17-24),                   */
+/* TPZInterpolatedElement::CreateMidSideConnect (Mesh/pzintel.cpp:652-703),   */
+/* TPZCompMesh::AllocateNewConnect (Mesh/pzcmesh.h:703-715).                 */
+/* ------------------------------------------------------------------------ */
+int orc_mesh_build(const orc_problem *pb, orc_mesh *m) {
+  memset(m, 0, sizeof(*m));
+  int dim = pb->dim, p = pb->p;
+  int nx = pb->nx, ny = dim >= 2 ? pb->ny : 1, nz = dim >= 3 ? pb->nz : 1;
+  m->dim = dim;
+  m->p = p;
+  m->nstate = pb->material == ORC_ELAST3D ? 3 : 1;
+  int64_t NX = nx + 1, NY = dim >= 2 ? ny + 1 : 1, NZ = dim >= 3 ? nz + 1 : 1;
+  m->nnodes = NX * NY * NZ;
+  m->xyz = (double *)calloc(3 * m->nnodes, sizeof(double));
+  for (int64_t k = 0; k < NZ; k++)
+    for (int64_t j = 0; j < NY; j++)
+      for (int64_t i = 0; i < NX; i++) {
+        int64_t n = i + NX * (j + NY * k);
+        m->xyz[3 * n + 0] = (double)i / nx;
+        m->xyz[3 * n + 1] = dim >= 2 ? (double)j / ny : 0.;
+        m->xyz[3 * n + 2] = dim >= 3 ? (double)k / nz : 0.;
+      }
+#define NODE(i, j, k) ((int64_t)(i) + NX * ((int64_t)(j) + NY * (int64_t)(k)))
+  int64_t nvol = (int64_t)nx * ny * nz;
+  int64_t nbc = 0;
+  if (dim >= 2)
+    for (int f = 0; f < 2 * dim; f++) {
+      if (pb->bc_type[f] < 0) continue;
+      int ax = f / 2;
+      int64_t cnt = 1;
+      for (int a = 0; a < dim; a++)
+        if (a != ax) cnt *= (a == 0 ? nx : a == 1 ? ny : nz);
+      nbc += cnt;
+    }
+  m->nvol = nvol;
+  m->nel = nvol + nbc;
+  m->etype = (int *)calloc(m->nel, sizeof(int));
+  m->bcface = (int *)calloc(m->nel, sizeof(int));
+  m->enodes = (int64_t *)calloc(8 * m->nel, sizeof(int64_t));
+  m->econn = (int64_t *)calloc(27 * m->nel, sizeof(int64_t));
+  int64_t e = 0;
+  int vt = dim == 1 ? ORC_LINE : dim == 2 ? ORC_QUAD : ORC_HEX;
+  for (int iz = 0; iz < nz; iz++)
+    for (int iy = 0; iy < ny; iy++)
+      for (int ix = 0; ix < nx; ix++, e++) {
+        m->etype[e] = vt;
+        m->bcface[e] = -1;
+        int64_t *en = m->enodes + 8 * e;
+        if (dim == 1) { en[0] = ix; en[1] = ix + 1; }
+        else if (dim == 2) {
+          en[0] = NODE(ix, iy, 0); en[1] = NODE(ix + 1, iy, 0);
+          en[2] = NODE(ix + 1, iy + 1, 0); en[3] = NODE(ix, iy + 1, 0);
+        } else {
+          en[0] = NODE(ix, iy, iz); en[1] = NODE(ix + 1, iy, iz);
+          en[2] = NODE(ix + 1, iy + 1, iz); en[3] = NODE(ix, iy + 1, iz);
+          en[4] = NODE(ix, iy, iz + 1); en[5] = NODE(ix + 1, iy, iz + 1);
+          en[6] = NODE(ix + 1, iy + 1, iz + 1); en[7] = NODE(ix, iy + 1, iz + 1);
+        }
+      }
+  if (dim >= 2)
+    for (int f = 0; f < 2 * dim; f++) {
+      if (pb->bc_type[f] < 0) continue;
+      int ax = f / 2, side = f % 2;
+      if (dim == 2) {
+        int nn = ax == 0 ? ny : nx;
+        for (int a = 0; a < nn; a++, e++) {
+          m->etype[e] = ORC_LINE;
+          m->bcface[e] = f;
+          int64_t *en = m->enodes + 8 * e;
+          if (ax == 0) { int x = side ? nx : 0; en[0] = NODE(x, a, 0); en[1] = NODE(x, a + 1, 0); }
+          else { int y = side ? ny : 0; en[0] = NODE(a, y, 0); en[1] = NODE(a + 1, y, 0); }
+        }
+      } else {
+        /* in-face axes (u,v): x-face (y,z), y-face (x,z), z-face (x,y) */
+        int nu_ = ax == 0 ? ny : nx, nv_ = ax == 2 ? ny : nz;
+        int c = side ? (ax == 0 ? nx : ax == 1 ? ny : nz) : 0;
+        for (int b = 0; b < nv_; b++)
+          for (int a = 0; a < nu_; a++, e++) {
+            m->etype[e] = ORC_QUAD;
+            m->bcface[e] = f;
+            int64_t *en = m->enodes + 8 * e;
+            if (ax == 0) {
+              en[0] = NODE(c, a, b); en[1] = NODE(c, a + 1, b);
+              en[2] = NODE(c, a + 1, b + 1); en[3] = NODE(c, a, b + 1);
+            } else if (ax == 1) {
+              en[0] = NODE(a, c, b); en[1] = NODE(a + 1, c, b);
+              en[2] = NODE(a + 1, c, b + 1); en[3] = NODE(a, c, b + 1);
+            } else {
+              en[0] = NODE(a, b, c); en[1] = NODE(a + 1, b, c);
+              en[2] = NODE(a + 1, b + 1, c); en[3] = NODE(a, b + 1, c);
+            }
+          }
+      }
+    }
+#undef NODE
+  /* first-touch numbering */
+  sidemap sm;
+  int64_t maxsides = 27 * m->nel;
+  sm.cap = 1;
+  while (sm.cap < 2 * maxsides) sm.cap <<= 1;
+  sm.keys = (int64_t *)malloc(4 * sm.cap * sizeof(int64_t));
+  memset(sm.keys, 0xff, 4 * sm.cap * sizeof(int64_t));
+  sm.val = (int64_t *)malloc(sm.cap * sizeof(int64_t));
+  int64_t ccap = 1024, nconn = 0;
+  int *cns = (int *)malloc(ccap * sizeof(int));
+  for (e = 0; e < m->nel; e++) {
+    int t = m->etype[e];
+    for (int s = 0; s < etype_nsides(t); s++) {
+      int lc[8], nlc = side_corners(t, s, lc);
+      int64_t key[8];
+      for (int k = 0; k < nlc; k++) key[k] = m->enodes[8 * e + lc[k]];
+      int64_t idx;
+      if (nlc == 8) {
+        idx = nconn; /* hexahedron interior is never shared */
+      } else {
+        sort_small(key, nlc);
+        for (int k = nlc; k < 4; k++) key[k] = -2;
+        int ins;
+        idx = sidemap_find_or_insert(&sm, key, nconn, &ins);
+        if (!ins) { m->econn[27 * e + s] = idx; continue; }
+      }
+      if (nconn == ccap) { ccap *= 2; cns = (int *)realloc(cns, ccap * sizeof(int)); }
+      cns[nconn] = nconnect_shape(t, s, p);
+      m->econn[27 * e + s] = nconn++;
+    }
+  }
+  free(sm.keys);
+  free(sm.val);
+  m->nconn = nconn;
+  m->conn_nshape = cns;
+  m->conn_seq = (int64_t *)malloc(nconn * sizeof(int64_t));
+  m->blk_pos = (int64_t *)malloc(nconn * sizeof(int64_t));
+  m->blk_size = (int *)malloc(nconn * sizeof(int));
+  int64_t pos = 0;
+  for (int64_t c = 0; c < nconn; c++) {
+    m->conn_seq[c] = c;
+    m->blk_size[c] = cns[c] * m->nstate;
+    m->blk_pos[c] = pos;
+    pos += m->blk_size[c];
+  }
+  m->neq = pos;
+  return 0;
+}
+
+void orc_mesh_free(orc_mesh *m) {
+  free(m->xyz); free(m->etype); free(m->bcface); free(m->enodes); free(m->econn);
+  free(m->conn_nshape); free(m->conn_seq); free(m->blk_pos); free(m->blk_size);
+  memset(m, 0, sizeof(*m));
+}
+
+int orc_elem_ndof(const orc_mesh *m, int64_t iel) {
+  int t = m->etype[iel], n = 0;
+  for (int s = 0; s < etype_nsides(t); s++) n += m->blk_size[m->conn_seq[m->econn[27 * iel + s]]];
+  return n;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Jacobian (Mesh/pzgeoel.cpp:1168-1355).  gradx[i][j] = sum_n x_n[i]        */
+/* dphi_corner[j][n].  Returns detjac (signed), fills jacinv[dim][dim] and    */
+/* axes[dim][3].                                                             */
+/* ------------------------------------------------------------------------ */
+static double jacobian(int dim, const double gradx[3][3], double jacinv[3][3], double axes[3][3]) {
+  double detjac = 0.;
+  memset(axes, 0, 9 * sizeof(double));
+  if (dim == 1) {
+    double n1 = 0.;
+    for (int i = 0; i < 3; i++) n1 += gradx[i][0] * gradx[i][0];
+    n1 = sqrt(n1);
+    detjac = n1;
+    jacinv[0][0] = 1.0 / detjac;
+    for (int i = 0; i < 3; i++) axes[0][i] = gradx[i][0] / n1;
+  } else if (dim == 2) {
+    double v1[3], v2[3], v1t[3], v2t[3], n1 = 0., n2 = 0., d12 = 0.;
+    for (int i = 0; i < 3; i++) { v1[i] = gradx[i][0]; v2[i] = gradx[i][1]; }
+    for (int i = 0; i < 3; i++) { n1 += v1[i] * v1[i]; d12 += v1[i] * v2[i]; }
+    n1 = sqrt(n1);
+    for (int i = 0; i < 3; i++) {
+      v1t[i] = v1[i] / n1;
+      v2t[i] = v2[i] - d12 * v1t[i] / n1;
+      n2 += v2t[i] * v2t[i];
+    }
+    n2 = sqrt(n2);
+    double j00 = n1, j01 = d12 / n1, j10 = 0., j11 = n2;
+    detjac = j00 * j11 - j10 * j01;
+    jacinv[0][0] = +j11 / detjac;
+    jacinv[1][1] = +j00 / detjac;
+    jacinv[0][1] = -j01 / detjac;
+    jacinv[1][0] = -j10 / detjac;
+    for (int i = 0; i < 3; i++) {
+      v2t[i] /= n2;
+      axes[0][i] = v1t[i];
+      axes[1][i] = v2t[i];
+    }
+  } else {
+    const double(*j)[3] = gradx;
+    detjac -= j[0][2] * j[1][1] * j[2][0];
+    detjac += j[0][1] * j[1][2] * j[2][0];
+    detjac += j[0][2] * j[1][0] * j[2][1];
+    detjac -= j[0][0] * j[1][2] * j[2][1];
+    detjac -= j[0][1] * j[1][0] * j[2][2];
+    detjac += j[0][0] * j[1][1] * j[2][2];
+    jacinv[0][0] = (-j[1][2] * j[2][1] + j[1][1] * j[2][2]) / detjac;
+    jacinv[0][1] = (j[0][2] * j[2][1] - j[0][1] * j[2][2]) / detjac;
+    jacinv[0][2] = (-j[0][2] * j[1][1] + j[0][1] * j[1][2]) / detjac;
+    jacinv[1][0] = (j[1][2] * j[2][0] - j[1][0] * j[2][2]) / detjac;
+    jacinv[1][1] = (-j[0][2] * j[2][0] + j[0][0] * j[2][2]) / detjac;
+    jacinv[1][2] = (j[0][2] * j[1][0] - j[0][0] * j[1][2]) / detjac;
+    jacinv[2][0] = (-j[1][1] * j[2][0] + j[1][0] * j[2][1]) / detjac;
+    jacinv[2][1] = (j[0][1] * j[2][0] - j[0][0] * j[2][1]) / detjac;
+    jacinv[2][2] = (-j[0][1] * j[1][0] + j[0][0] * j[1][1]) / detjac;
+    for (int i = 0; i < 3; i++) axes[i][i] = 1.;
+  }
+  return detjac;
+}
+
+static void forcing(const orc_problem *pb, const double *x, double *f) {
+  f[0] = f[1] = f[2] = 0.;
+  if (pb->force_kind == ORC_FORCE_CONST) {
+    for (int k = 0; k < 3; k++) f[k] = pb->force_const[k];
+  } else if (pb->force_kind == ORC_FORCE_BUMPS) {
+    double w2 = 2. * pb->bump_width * pb->bump_width;
+    for (int b = 0; b < pb->nbumps; b++) {
+      const double *bp = pb->bumps + 6 * b;
+      double r2 = 0.;
+      for (int d = 0; d < 3; d++) r2 += (x[d] - bp[d]) * (x[d] - bp[d]);
+      double g = exp(-r2 / w2);
+      for (int k = 0; k < 3; k++) f[k] += bp[3 + k] * g;
+    }
+  }
+}
+
+/* TPZMatSingleSpaceT::IntegrationRuleOrder (Material/TPZMatSingleSpace.cpp:
+ * 60-72) and the TPZElasticity3D override 2p (TPZElasticity3D.h:82-85). */
+static int rule_order(const orc_problem *pb, int isbc, int p) {
+  if (pb->material == ORC_ELAST3D || isbc) return 2 * p;
+  int ff = pb->force_kind != ORC_FORCE_NONE ? pb->force_porder : 0;
+  return p < ff ? ff + p : 2 * p;
+}
+
+/* fBigNumber = pow(10, max_digits10)*2/3 (Material/TPZMaterial.h:125) */
+static double big_number(void) { return pow(10, 17) * 2 / 3; }
+
+/* One integration point of the volume weak forms; ek row-major with leading
+ * dimension ne.  kappa = fScale (Poisson) or permeability (Darcy).
+ * TPZMatPoisson::Contribute (Material/Poisson/TPZMatPoisson.cpp:19-42),
+ * TPZDarcyFlow::Contribute (Material/DarcyFlow/TPZDarcyFlow.cpp:19-46),
+ * TPZElasticity3D::Contribute CODE3 (Material/Elasticity/TPZElasticity3D.cpp:
+ * 269-372) with C1..C3 of TPZElasticity3D.h:183-188. */
+void orc_contribute(int material, int dim, int nsh, int ne, double weight, const double *phi,
+                    const double *dphix, const double *f, double kappa, double E, double nu,
+                    double *ek, double *ef) {
+  if (material == ORC_POISSON) {
+    for (int i = 0; i < nsh; i++) {
+      for (int j = 0; j < nsh; j++) {
+        double dd = 0;
+        for (int a = 0; a < dim; a++) dd += dphix[a * nsh + i] * dphix[a * nsh + j];
+        ek[i * ne + j] += weight * kappa * dd;
+      }
+      ef[i] += weight * kappa * phi[i] * f[0];
+    }
+  } else if (material == ORC_DARCY) {
+    double source = -f[0];
+    for (int in = 0; in < nsh; in++) {
+      ef[in] -= weight * source * phi[in];
+      for (int jn = 0; jn < nsh; jn++)
+        for (int kd = 0; kd < dim; kd++)
+          ek[in * ne + jn] += weight * (dphix[kd * nsh + in] * kappa * dphix[kd * nsh + jn]);
+    }
+  } else if (material == ORC_ELAST3D) {
+    double C1 = E / (2. + 2. * nu);
+    double C2 = E * nu / (-1. + nu + 2. * nu * nu);
+    double C3 = E * (nu - 1.) / (-1. + nu + 2. * nu * nu);
+    for (int jn = 0; jn < nsh; jn++) {
+      double dj[3] = {dphix[jn], dphix[nsh + jn], dphix[2 * nsh + jn]};
+      for (int kd = 0; kd < 3; kd++) ef[3 * jn + kd] += weight * (f[kd] * phi[jn]);
+      for (int in = 0; in < nsh; in++) {
+        double D[3][3];
+        for (int vd = 0; vd < 3; vd++)
+          for (int ud = 0; ud < 3; ud++) D[vd][ud] = dphix[vd * nsh + in] * dj[ud];
+        double *r0 = ek + (3 * in + 0) * ne + 3 * jn;
+        double *r1 = ek + (3 * in + 1) * ne + 3 * jn;
+        double *r2 = ek + (3 * in + 2) * ne + 3 * jn;
+        r0[0] += weight * ((D[1][1] + D[2][2]) * C1 + D[0][0] * C3);
+        r1[0] += weight * (D[0][1] * C1 - D[1][0] * C2);
+        r2[0] += weight * (D[0][2] * C1 - D[2][0] * C2);
+        r0[1] += weight * (D[1][0] * C1 - D[0][1] * C2);
+        r1[1] += weight * ((D[0][0] + D[2][2]) * C1 + D[1][1] * C3);
+        r2[1] += weight * (D[1][2] * C1 - D[2][1] * C2);
+        r0[2] += weight * (D[2][0] * C1 - D[0][2] * C2);
+        r1[2] += weight * (D[2][1] * C1 - D[1][2] * C2);
+        r2[2] += weight * ((D[0][0] + D[1][1]) * C1 + D[2][2] * C3);
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* CalcStiff: TPZInterpolationSpace::CalcStiffInternal                        */
+/* (Mesh/pzinterpolationspace.cpp:403-474) with ComputeRequiredDataT (:265-  */
+/* 298) and TPZCompElH1::ComputeShape (Mesh/TPZCompElH1.cpp:139-149).         */
+/* ek row-major, dofs ordered shape-major (index = nstate*shape + state).     */
+/* ------------------------------------------------------------------------ */
+int orc_calcstiff(const orc_problem *pb, const orc_mesh *m, int64_t iel, double *ek,
+                  double *ef, int64_t *dest) {
+  int t = m->etype[iel], dim = etype_dim(t), nc = etype_ncorner(t), ns_ = m->nstate;
+  int isbc = m->bcface[iel] >= 0;
+  /* destination indices (Mesh/pzelmat.cpp:37-70) */
+  int ne = 0;
+  for (int s = 0; s < etype_nsides(t); s++) {
+    int64_t seq = m->conn_seq[m->econn[27 * iel + s]];
+    for (int k = 0; k < m->blk_size[seq]; k++) dest[ne++] = m->blk_pos[seq] + k;
+  }
+  memset(ek, 0, sizeof(double) * ne * ne);
+  memset(ef, 0, sizeof(double) * ne);
+  if (pb->material == ORC_MOCK1D) {
+    /* TPZMatTest in UnitTest_PZ/TestStruct/StructMatrixUnitTest.cpp:333-362 */
+    if (!isbc && ne == 2) { ek[0] = 2; ek[1] = 1; ek[2] = 1; ek[3] = 2; }
+    return ne;
+  }
+  double coord[8][3];
+  for (int n = 0; n < nc; n++)
+    for (int d = 0; d < 3; d++) coord[n][d] = m->xyz[3 * m->enodes[8 * iel + n] + d];
+  double gl[64], gw[64];
+  int ng = orc_gauss_legendre(rule_order(pb, isbc, m->p), gl, gw);
+  int nq = 1;
+  for (int d = 0; d < dim; d++) nq *= ng;
+  double coef = 1.;
+  if (!isbc && pb->el_coef) coef = pb->el_coef[iel];
+  double phi[27], dphi[81], dphix[81];
+  for (int q = 0; q < nq; q++) {
+    double xi[3] = {0, 0, 0}, w;
+    if (dim == 1) { xi[0] = gl[q]; w = gw[q]; }
+    else if (dim == 2) { /* Integral/pzquad.cpp:159-169: ksi slow, eta fast */
+      int ik = q / ng, ie = q % ng;
+      xi[0] = gl[ik]; xi[1] = gl[ie]; w = gw[ik] * gw[ie];
+    } else { /* Integral/pzquad.cpp:274-287: ksi fastest */
+      int ik = q % ng, ie = (q % (ng * ng)) / ng, iz = q / (ng * ng);
+      xi[0] = gl[ik]; xi[1] = gl[ie]; xi[2] = gl[iz]; w = gw[ik] * gw[ie] * gw[iz];
+    }
+    int nsh = orc_shape(t, m->p, xi, phi, dphi);
+    /* corner derivatives for GradX (Geom/TPZGeoCube.h:124, pzgeoquad.h:144) */
+    double cphi[27], cdphi[81];
+    orc_shape(t, 1, xi, cphi, cdphi);
+    double gradx[3][3] = {{0}}, x[3] = {0, 0, 0};
+    for (int n = 0; n < nc; n++)
+      for (int i = 0; i < 3; i++) {
+        x[i] += coord[n][i] * cphi[n];
+        for (int j = 0; j < dim; j++) gradx[i][j] += coord[n][i] * cdphi[j * nc + n];
+      }
+    double jinv[3][3] = {{0}}, axes[3][3];
+    double detjac = fabs(jacobian(dim, gradx, jinv, axes));
+    /* dphix = jacinv^T dphi */
+    for (int i = 0; i < nsh; i++)
+      for (int a = 0; a < dim; a++) {
+        double s = 0.;
+        for (int b = 0; b < dim; b++) s += jinv[b][a] * dphi[b * nsh + i];
+        dphix[a * nsh + i] = s;
+      }
+    double weight = w * detjac;
+    double f[3];
+    forcing(pb, x, f);
+    if (isbc) {
+      int face = m->bcface[iel], type = pb->bc_type[face];
+      const double *v2 = pb->bc_val[face];
+      double big = pb->material == ORC_ELAST3D ? 1.e12 : big_number();
+      if (pb->material == ORC_ELAST3D) {
+        /* Material/Elasticity/TPZElasticity3D.cpp:671-692 */
+        for (int in = 0; in < nsh; in++) {
+          if (type == 0) {
+            for (int k = 0; k < 3; k++) ef[3 * in + k] += big * v2[k] * phi[in] * weight;
+            for (int jn = 0; jn < nsh; jn++)
+              for (int k = 0; k < 3; k++)
+                ek[(3 * in + k) * ne + 3 * jn + k] += big * phi[in] * phi[jn] * weight;
+          } else {
+            for (int k = 0; k < 3; k++) ef[3 * in + k] += v2[k] * phi[in] * weight;
+          }
+        }
+      } else if (pb->material == ORC_POISSON) {
+        /* Material/Poisson/TPZMatPoisson.cpp:76-100 */
+        for (int in = 0; in < nsh; in++) {
+          if (type == 0) {
+            ef[in] += big * v2[0] * phi[in] * weight;
+            for (int jn = 0; jn < nsh; jn++) ek[in * ne + jn] += big * phi[in] * phi[jn] * weight;
+          } else {
+            ef[in] += v2[0] * pb->scale * phi[in] * weight;
+          }
+        }
+      } else {
+        /* Material/DarcyFlow/TPZDarcyFlow.cpp:65-78 */
+        for (int in = 0; in < nsh; in++) {
+          if (type == 0) {
+            ef[in] += (big * phi[in] * weight) * v2[0];
+            for (int jn = 0; jn < nsh; jn++) ek[in * ne + jn] += big * phi[in] * phi[jn] * weight;
+          } else {
+            ef[in] += v2[0] * (phi[in] * weight);
+          }
+        }
+      }
+      continue;
+    }
+    orc_contribute(pb->material, dim, nsh, ne, weight, phi, dphix, f,
+                   pb->material == ORC_DARCY ? (pb->el_coef ? coef : pb->scale) : pb->scale,
+                   pb->el_coef ? coef : pb->E, pb->nu, ek, ef);
+  }
+  return ne;
+}
+
+/* ------------------------------------------------------------------------ */
+/* SetupMatrixData (StrMatrix/TPZSSpStructMatrix.cpp:49-193) on the node     */
+/* graph of TPZRenumbering::ConvertGraph (External/TPZRenumbering.cpp:76-110) */
+/* built from TPZCompMesh::ComputeElGraph (Mesh/pzcmesh.cpp:1223-1266).       */
+/* ------------------------------------------------------------------------ */
+static int cmp_i64(const void *a, const void *b) {
+  int64_t x = *(const int64_t *)a, y = *(const int64_t *)b;
+  return x < y ? -1 : x > y;
+}
+
+int64_t orc_setup_pattern(const orc_mesh *m, int64_t **ia_out, int64_t **ja_out) {
+  int64_t nb = m->nconn;
+  /* node -> element graph */
+  int64_t *cnt = (int64_t *)calloc(nb + 1, sizeof(int64_t));
+  for (int64_t e = 0; e < m->nel; e++)
+    for (int s = 0; s < etype_nsides(m->etype[e]); s++) cnt[m->conn_seq[m->econn[27 * e + s]] + 1]++;
+  for (int64_t i = 0; i < nb; i++) cnt[i + 1] += cnt[i];
+  int64_t *n2e = (int64_t *)malloc(cnt[nb] * sizeof(int64_t));
+  int64_t *fill = (int64_t *)calloc(nb, sizeof(int64_t));
+  for (int64_t e = 0; e < m->nel; e++)
+    for (int s = 0; s < etype_nsides(m->etype[e]); s++) {
+      int64_t c = m->conn_seq[m->econn[27 * e + s]];
+      n2e[cnt[c] + fill[c]++] = e;
+    }
+  free(fill);
+  /* per node: sorted unique neighbour set (std::set semantics), without self */
+  int64_t *ngp = (int64_t *)calloc(nb + 1, sizeof(int64_t));
+  int64_t ngcap = 1 << 20, ngn = 0;
+  int64_t *ng = (int64_t *)malloc(ngcap * sizeof(int64_t));
+  int64_t *tmp = (int64_t *)malloc(4096 * sizeof(int64_t));
+  for (int64_t i = 0; i < nb; i++) {
+    int nt = 0;
+    for (int64_t k = cnt[i]; k < cnt[i + 1]; k++) {
+      int64_t e = n2e[k];
+      for (int s = 0; s < etype_nsides(m->etype[e]); s++) {
+        int64_t c = m->conn_seq[m->econn[27 * e + s]];
+        if (c != i) tmp[nt++] = c;
+      }
+    }
+    qsort(tmp, nt, sizeof(int64_t), cmp_i64);
+    int nu = 0;
+    for (int k = 0; k < nt; k++)
+      if (nu == 0 || tmp[k] != tmp[nu - 1]) tmp[nu++] = tmp[k];
+    while (ngn + nu > ngcap) { ngcap *= 2; ng = (int64_t *)realloc(ng, ngcap * sizeof(int64_t)); }
+    memcpy(ng + ngn, tmp, nu * sizeof(int64_t));
+    ngn += nu;
+    ngp[i + 1] = ngn;
+  }
+  free(tmp);
+  free(n2e);
+  free(cnt);
+  /* count pass */
+  int64_t total = 0;
+  for (int64_t i = 0; i < nb; i++) {
+    int64_t bs = m->blk_size[i];
+    if (!bs) continue;
+    total += bs * (bs + 1) / 2;
+    for (int64_t j = ngp[i]; j < ngp[i + 1]; j++)
+      if (ng[j] > i) total += bs * m->blk_size[ng[j]];
+  }
+  int64_t *ia = (int64_t *)malloc((m->neq + 1) * sizeof(int64_t));
+  int64_t *ja = (int64_t *)malloc((total ? total : 1) * sizeof(int64_t));
+  int64_t pos = 0, ieq = 0;
+  for (int64_t i = 0; i < nb; i++) {
+    int64_t bs = m->blk_size[i], bp = m->blk_pos[i];
+    for (int64_t k = 0; k < bs; k++, ieq++) {
+      ia[ieq] = pos;
+      int diag = 0;
+      for (int64_t j = ngp[i]; j < ngp[i + 1]; j++) {
+        int64_t col = ng[j];
+        if (col < i) continue;
+        if (!diag && col > i) {
+          diag = 1;
+          for (int64_t l = 0; l < bs; l++)
+            if (bp + l >= ieq) ja[pos++] = bp + l;
+        }
+        for (int64_t l = 0; l < m->blk_size[col]; l++)
+          if (m->blk_pos[col] + l >= ieq) ja[pos++] = m->blk_pos[col] + l;
+      }
+      if (!diag)
+        for (int64_t l = 0; l < bs; l++)
+          if (bp + l >= ieq) ja[pos++] = bp + l;
+    }
+  }
+  ia[ieq] = pos;
+  free(ng);
+  free(ngp);
+  *ia_out = ia;
+  *ja_out = ja;
+  return pos;
+}
+
+/* ------------------------------------------------------------------------ */
+/* AddKel (Matrix/pzsysmp.cpp:369-411): upper entries only, skip            */
+/* IsZero(v) = |v| < 1e-12 (Common/pzreal.h:736-741), sequential guess then  */
+/* row search.                                                               */
+/* ------------------------------------------------------------------------ */
+void orc_addkel(const int64_t *ia, const int64_t *ja, double *a, int ne, const double *ek,
+                const int64_t *dest) {
+  int64_t k = 0;
+  for (int i = 0; i < ne; i++)
+    for (int j = 0; j < ne; j++) {
+      int64_t ip = dest[i], jp = dest[j];
+      if (jp < ip) continue;
+      double v = ek[i * ne + j];
+      if (fabs(v) < 1.e-12) continue;
+      k++;
+      if (k >= ia[ip] && k < ia[ip + 1] && ja[k] == jp) {
+        a[k] += v;
+      } else {
+        for (k = ia[ip]; k < ia[ip + 1]; k++)
+          if (ja[k] == jp) { a[k] += v; break; }
+      }
+    }
+}
+
+int orc_assemble_range(const orc_problem *pb, const orc_mesh *m, const int64_t *ia,
+                       const int64_t *ja, double *a, double *rhs, int64_t e0, int64_t e1) {
+  double *ek = (double *)malloc(sizeof(double) * 81 * 81);
+  double ef[81];
+  int64_t dest[81];
+  for (int64_t e = e0; e < e1; e++) {
+    int ne = orc_calcstiff(pb, m, e, ek, ef, dest);
+    orc_addkel(ia, ja, a, ne, ek, dest);
+    for (int i = 0; i < ne; i++) rhs[dest[i]] += ef[i]; /* AddFel, Matrix/pzfmatrix.cpp:284-298 */
+  }
+  free(ek);
+  return 0;
+}
+
+/* Serial_Assemble (StrMatrix/pzstrmatrixor.cpp:104-365), element order. */
+int orc_assemble(const orc_problem *pb, const orc_mesh *m, const int64_t *ia, const int64_t *ja,
+                 double *a, double *rhs) {
+  memset(a, 0, sizeof(double) * ia[m->neq]);
+  memset(rhs, 0, sizeof(double) * m->neq);
+  return orc_assemble_range(pb, m, ia, ja, a, rhs, 0, m->nel);
+}
+
+/* TPZSYsmpMatrix::MultAdd (Matrix/pzsysmp.cpp:189-232) with PrepareZ
+ * (Matrix/pzmatrix.cpp:112-141): z = beta*y, then z += alpha*A*x mirrored. */
+void orc_multadd(int64_t n, const int64_t *ia, const int64_t *ja, const double *a,
+                 const double *x, const double *y, double *z, double alpha, double beta) {
+  for (int64_t i = 0; i < n; i++) z[i] = (beta != 0. && y) ? beta * y[i] : 0.;
+  for (int64_t r = 0; r < n; r++)
+    for (int64_t iv = ia[r]; iv < ia[r + 1]; iv++) {
+      int64_t c = ja[iv];
+      z[r] += alpha * a[iv] * x[c];
+      if (r != c) z[c] += alpha * a[iv] * x[r];
+    }
+}
+
+static double dot(int64_t n, const double *x, const double *y) {
+  double s = 0.;
+  for (int64_t i = 0; i < n; i++) s += x[i] * y[i];
+  return s;
+}
+
+static double getval_diag(const int64_t *ia, const int64_t *ja, const double *a, int64_t i) {
+  for (int64_t k = ia[i]; k < ia[i + 1]; k++)
+    if (ja[k] == i) return a[k];
+  return 0.;
+}
+
+/* CG<> (Solvers/LinearSolvers/cg.h:29-132) preconditioned by a TPZStepSolver
+ * EJacobi(1 iteration, tol 0) = TPZMatrix::SolveJacobi (Matrix/pzmatrix.cpp:
+ * 642-667): z = r / GetVal(i,i), followed by a Residual whose result is
+ * discarded (reproduced when wasted_spmv != 0). */
+int orc_pcg(int64_t n, const int64_t *ia, const int64_t *ja, const double *a, const double *b,
+            double *x, int64_t *iters, double *tol, int wasted_spmv, double *res_history) {
+  double *r = (double *)malloc(n * sizeof(double)), *z = (double *)malloc(n * sizeof(double));
+  double *p = (double *)malloc(n * sizeof(double)), *q = (double *)malloc(n * sizeof(double));
+  double *scr = (double *)malloc(n * sizeof(double));
+  int64_t maxit = *iters;
+  double normb = sqrt(dot(n, b, b)), resid, rho, rho1 = 0., alpha, beta;
+  for (int64_t i = 0; i < n; i++) { x[i] = 0.; r[i] = b[i]; }
+  if (normb == 0.) normb = 1.;
+  int ret = 1;
+  if ((resid = sqrt(dot(n, r, r)) / normb) <= *tol) {
+    *tol = resid;
+    *iters = 0;
+    ret = 0;
+    goto done;
+  }
+  int64_t i;
+  for (i = 1; i <= maxit; i++) {
+    for (int64_t k = 0; k < n; k++) z[k] = r[k] / getval_diag(ia, ja, a, k);
+    if (wasted_spmv) orc_multadd(n, ia, ja, a, z, r, scr, -1., 1.);
+    rho = dot(n, r, z);
+    if (i == 1) memcpy(p, z, n * sizeof(double));
+    else {
+      beta = rho / rho1;
+      for (int64_t k = 0; k < n; k++) p[k] = beta * p[k] + z[k];
+    }
+    orc_multadd(n, ia, ja, a, p, NULL, q, 1., 0.);
+    alpha = rho / dot(n, p, q);
+    for (int64_t k = 0; k < n; k++) x[k] += alpha * p[k];
+    for (int64_t k = 0; k < n; k++) r[k] += -alpha * q[k];
+    resid = sqrt(dot(n, r, r)) / normb;
+    if (res_history) res_history[i - 1] = resid;
+    if (resid <= *tol) {
+      *tol = resid;
+      *iters = i;
+      ret = 0;
+      goto done;
+    }
+    rho1 = rho;
+  }
+  *tol = resid;
+  *iters = maxit;
+done:
+  free(r); free(z); free(p); free(q); free(scr);
+  return ret;
+}
+
+/* nnz_full = S^d s^2, S = 2(p+1) + (n-1)(2p+1) + n(p-1)(p+1) per direction
+ * (1-D coupling count); nnz_upper = (nnz_full + N)/2. */
+int64_t orc_nnz_closed_form(int dim, int p, int nstate, int nx, int ny, int nz) {
+  int64_t n[3] = {nx, ny, nz}, full = 1, N = 1;
+  for (int d = 0; d < dim; d++) {
+    int64_t S = 2 * (p + 1) + (n[d] - 1) * (2 * p + 1) + n[d] * (p - 1) * (p + 1);
+    full *= S;
+    N *= p * n[d] + 1;
+  }
+  full *= (int64_t)nstate * nstate;
+  N *= nstate;
+  return (full + N) / 2;
+}

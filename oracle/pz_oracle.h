@@ -1,0 +1,128 @@
+/*
+ * pz_oracle.h -- CPU restatement of NeoPZ's linear H1 FEM path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
+ * path (neopz_amd/csrc).  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  Nothing in the product links it.
+ *
+ * What it restates (reference file:line in each function comment of
+ * pz_oracle.c):
+ *   - first-touch connect numbering of TPZCompMesh::AutoBuild with
+ *     TPZLinearAnalysis(mesh, mustOptimizeBandwidth=false)
+ *   - Gauss-Legendre rules (TPZGaussRule), quad/cube point order
+ *   - H1 hierarchical shape functions p<=2 on line/quad/hex (TPZShapeH1 +
+ *     TPZShape{Linear,Quad,Cube}::ShapeCorner/ShapeGenerating)
+ *   - TPZGeoEl::Jacobian (1D/2D Gram-Schmidt, 3D cofactor inverse)
+ *   - Contribute / ContributeBC of TPZMatPoisson, TPZDarcyFlow,
+ *     TPZElasticity3D (CODE3 branch)
+ *   - TPZSSpStructMatrix::SetupMatrixData (upper CSR pattern)
+ *   - TPZSYsmpMatrix::AddKel (with the |v|<1e-12 drop rule), AddFel
+ *   - TPZSYsmpMatrix::MultAdd, CG<> with the EJacobi TPZStepSolver
+ */
+#ifndef PZ_ORACLE_H
+#define PZ_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_POISSON = 0, ORC_DARCY = 1, ORC_ELAST3D = 2, ORC_MOCK1D = 3 };
+enum { ORC_LINE = 1, ORC_QUAD = 2, ORC_HEX = 3 };
+enum { ORC_FORCE_NONE = 0, ORC_FORCE_CONST = 1, ORC_FORCE_BUMPS = 2 };
+
+/* Problem description shared with the product's structured mesh builder. */
+typedef struct {
+  int material;      /* ORC_POISSON / ORC_DARCY / ORC_ELAST3D / ORC_MOCK1D   */
+  int dim;           /* 1, 2 or 3                                            */
+  int p;             /* polynomial order (1 or 2)                            */
+  int nx, ny, nz;    /* elements per direction on the unit square/cube       */
+  double scale;      /* TPZMatPoisson fScale / Darcy constant permeability   */
+  double E, nu;      /* TPZElasticity3D constants (E overridden by el_coef)  */
+  const double *el_coef; /* per volume element: Darcy kappa or Elasticity E  */
+  int force_kind;    /* ORC_FORCE_*                                          */
+  int force_porder;  /* fForcingFunctionPOrder                               */
+  double force_const[3];
+  int nbumps;        /* Gaussian bumps: rows (cx,cy,cz,a0,a1,a2)             */
+  const double *bumps;
+  double bump_width; /* f = sum a exp(-|x-c|^2 / (2 w^2))                    */
+  int bc_type[6];    /* per boundary face: -1 none, 0 Dirichlet, 1 Neumann   */
+  double bc_val[6][3];
+} orc_problem;
+
+typedef struct {
+  int dim, p, nstate;
+  int64_t nnodes;
+  double *xyz;            /* 3*nnodes */
+  int64_t nel, nvol;      /* volume elements come first, then BC elements */
+  int *etype;             /* ORC_LINE / ORC_QUAD / ORC_HEX */
+  int *bcface;            /* -1 volume, else boundary face index */
+  int64_t *enodes;        /* stride 8 */
+  int64_t *econn;         /* stride 27: connect index per side */
+  int64_t nconn;
+  int *conn_nshape;
+  int64_t *conn_seq;      /* first-touch: seq == creation index */
+  int64_t *blk_pos;       /* by seq */
+  int *blk_size;          /* by seq */
+  int64_t neq;
+} orc_mesh;
+
+/* Gauss-Legendre points for polynomial order 'order' (TPZGaussRule).
+ * Returns npoints; loc/w must hold >= 64 entries. */
+int orc_gauss_legendre(int order, double *loc, double *w);
+
+/* Builds the structured mesh + first-touch numbering. Returns 0 on success. */
+int orc_mesh_build(const orc_problem *pb, orc_mesh *m);
+void orc_mesh_free(orc_mesh *m);
+
+/* number of element dofs and quadrature points of element iel */
+int orc_elem_ndof(const orc_mesh *m, int64_t iel);
+
+/* CalcStiff of one element: ek row-major ne x ne, ef ne; dest gets global
+ * equation numbers (ComputeDestinationIndices). Returns ne. */
+int orc_calcstiff(const orc_problem *pb, const orc_mesh *m, int64_t iel,
+                  double *ek, double *ef, int64_t *dest);
+
+/* Reference-element shape functions (p <= 2): returns nshape; dphi[d*ns+i]. */
+int orc_shape(int etype, int p, const double *xi, double *phi, double *dphi);
+
+/* One integration point of Contribute (volume materials), ek row-major. */
+void orc_contribute(int material, int dim, int nsh, int ne, double weight, const double *phi,
+                    const double *dphix, const double *f, double kappa, double E, double nu,
+                    double *ek, double *ef);
+
+/* SetupMatrixData: returns nnz; ia (neq+1) and ja (nnz) are malloc'ed. */
+int64_t orc_setup_pattern(const orc_mesh *m, int64_t **ia, int64_t **ja);
+
+/* Full serial assembly (Serial_Assemble + AddKel + AddFel). a/rhs zeroed here. */
+int orc_assemble(const orc_problem *pb, const orc_mesh *m, const int64_t *ia,
+                 const int64_t *ja, double *a, double *rhs);
+
+/* Assembly restricted to elements [e0, e1): used for bounded CPU timing. */
+int orc_assemble_range(const orc_problem *pb, const orc_mesh *m,
+                       const int64_t *ia, const int64_t *ja, double *a,
+                       double *rhs, int64_t e0, int64_t e1);
+
+/* AddKel of a caller-given element matrix (row-major ne x ne). */
+void orc_addkel(const int64_t *ia, const int64_t *ja, double *a, int ne,
+                const double *ek, const int64_t *dest);
+
+/* z = beta*y + alpha*K*x with upper-stored symmetric CSR (TPZSYsmpMatrix::MultAdd) */
+void orc_multadd(int64_t n, const int64_t *ia, const int64_t *ja,
+                 const double *a, const double *x, const double *y, double *z,
+                 double alpha, double beta);
+
+/* Jacobi-preconditioned CG (CG<> + EJacobi(1)); x starts at 0.
+ * wasted_spmv != 0 reproduces the reference's discarded residual SpMV.
+ * On return *iters and *tol hold the achieved values. Returns 0 if converged. */
+int orc_pcg(int64_t n, const int64_t *ia, const int64_t *ja, const double *a,
+            const double *b, double *x, int64_t *iters, double *tol,
+            int wasted_spmv, double *res_history);
+
+/* Number of CSR entries for structured meshes, closed form (for checks). */
+int64_t orc_nnz_closed_form(int dim, int p, int nstate, int nx, int ny, int nz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
