@@ -1,0 +1,218 @@
+/*
+ * pzg.h -- C ABI of the MI355X (gfx950) linear FEM hot path.
+ *
+ * The entry points are what a NeoPZ drop-in pair of subclasses binds:
+ *   TPZSSpStructMatrixHIP : TPZSSpStructMatrix<STATE, TPZStructMatrixOR<STATE>>
+ *   TPZSYsmpMatrixHIP     : TPZSYsmpMatrix<STATE>
+ * so that TPZLinearAnalysis::Assemble()/Solve() stay unchanged.  Each function
+ * names the reference interface it replaces (paths relative to the NeoPZ
+ * source tree).
+ *
+ * Conventions
+ *  - Plain C types only; host pointers are borrowed for the duration of the
+ *    call; device memory is owned by the context.
+ *  - Every function returns PZG_OK (0) or an error code; pzg_last_error()
+ *    gives the message of the last failure on the calling thread.  The C++
+ *    adapter maps a non-zero code to PZError + DebugStop(), which is the
+ *    reference's error behaviour (e.g. Matrix/pzsysmp.cpp:99-101).
+ *  - One context = one GPU, one HIP stream; a context is used from one host
+ *    thread at a time (the reference calls the struct matrix from one thread).
+ *  - Equation numbering and the upper-triangular CSR pattern are exactly the
+ *    reference's (StrMatrix/TPZSSpStructMatrix.cpp:49-193): IA/JA are int64,
+ *    values double, rows sorted, diagonal first.
+ */
+#ifndef PZG_H
+#define PZG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define PZG_OK 0
+#define PZG_EINVAL 1   /* bad argument / inconsistent mesh description      */
+#define PZG_EHIP 2     /* HIP runtime failure                               */
+#define PZG_ENOCONV 3  /* solver hit max_iter before reaching tol           */
+#define PZG_ESTATE 4   /* call out of order (e.g. assemble before create)   */
+#define PZG_ENOMEM 5   /* device allocation failed                          */
+#define PZG_EUNSUP 6   /* element/material combination not implemented     */
+
+/* ---- element types, materials, loads ----------------------------------- */
+#define PZG_LINE 1
+#define PZG_QUAD 2
+#define PZG_HEX 3
+
+#define PZG_MAT_POISSON 0  /* TPZMatPoisson   (Material/Poisson/TPZMatPoisson.cpp) */
+#define PZG_MAT_DARCY 1    /* TPZDarcyFlow    (Material/DarcyFlow/TPZDarcyFlow.cpp) */
+#define PZG_MAT_ELAST3D 2  /* TPZElasticity3D (Material/Elasticity/TPZElasticity3D.cpp) */
+
+#define PZG_FORCE_NONE 0
+#define PZG_FORCE_CONST 1
+#define PZG_FORCE_BUMPS 2  /* f(x) = sum_k a_k exp(-|x-c_k|^2/(2 w^2)), rows (cx,cy,cz,a0,a1,a2) */
+
+#define PZG_BC_DIRICHLET 0
+#define PZG_BC_NEUMANN 1
+
+typedef struct pzg_ctx pzg_ctx;
+
+/*
+ * Flattened computational mesh: what TPZSSpStructMatrixHIP reads once from
+ * TPZCompMesh (connect sequence numbers, Block() position/size, element
+ * connect lists from TPZCompEl::BuildConnectList, geometric node coordinates,
+ * material parameters and boundary conditions).
+ */
+typedef struct {
+  int32_t dim;        /* 2 or 3                                               */
+  int32_t material;   /* PZG_MAT_*                                            */
+  int32_t nstate;     /* 1 (Poisson/Darcy) or 3 (Elasticity3D)                */
+  int32_t porder;     /* uniform H1 order, 1 or 2                             */
+  int64_t nnodes;
+  const double *xyz;  /* 3 * nnodes                                           */
+  /* volume elements: PZG_QUAD (dim 2) or PZG_HEX (dim 3)                     */
+  int64_t nvol;
+  const int64_t *vol_nodes; /* nvol * ncorner (4 or 8), NeoPZ corner order  */
+  const int64_t *vol_conn;  /* nvol * nsides (9 or 27): connect seq per side */
+  const double *vol_coef;   /* nvol: E (elasticity) or permeability (Darcy); NULL = constant */
+  /* boundary elements: PZG_LINE (dim 2) or PZG_QUAD (dim 3)                  */
+  int64_t nbc;
+  const int64_t *bc_nodes;  /* nbc * ncorner (2 or 4)                        */
+  const int64_t *bc_conn;   /* nbc * nsides (3 or 9)                          */
+  const int32_t *bc_id;     /* nbc: index into the bc tables below            */
+  int32_t nbcid;
+  const int32_t *bc_type;   /* nbcid: PZG_BC_*                                */
+  const double *bc_val;     /* 3 * nbcid: TPZBndCondT::Val2                   */
+  /* connects by sequence number: TPZCompMesh::Block() (Matrix/pzblock.h)     */
+  int64_t nconn;
+  const int64_t *blk_pos;
+  const int32_t *blk_size;
+  int64_t neq;
+  /* material constants and loads                                             */
+  double scale;       /* TPZMatPoisson fScale / constant Darcy permeability   */
+  double E, nu;       /* TPZElasticity3D (E used when vol_coef == NULL)       */
+  int32_t rule_order_vol; /* TPZMatSingleSpace::IntegrationRuleOrder result   */
+  int32_t rule_order_bc;
+  int32_t force_kind;
+  double force_const[3];
+  int32_t nbumps;
+  const double *bumps;    /* 6 * nbumps                                       */
+  double bump_width;
+  double big_number;      /* Dirichlet penalty (TPZMaterial::fBigNumber or 1e12) */
+} pzg_mesh_desc;
+
+/* ---- solver selection -------------------------------------------------- */
+#define PZG_SOLVER_PCG64 0   /* FP64 CG + Jacobi, reference semantics:
+                                CG<> (Solvers/LinearSolvers/cg.h:29-132),
+                                stop on the recursive ||r||/||b|| <= tol      */
+#define PZG_SOLVER_MIXED 1   /* FP32 flexible CG on D^-1/2 K D^-1/2 inside an
+                                FP64 iterative refinement (the shape of IR<>,
+                                Solvers/LinearSolvers/ir.h:23-66); stop on the
+                                true FP64 ||f - K u|| / ||f|| <= tol          */
+
+#define PZG_PRECOND_JACOBI 0       /* point Jacobi                              */
+#define PZG_PRECOND_BLOCK_JACOBI 1 /* node-block (nstate x nstate) Jacobi       */
+#define PZG_PRECOND_LEARNED 2      /* learned message-passing preconditioner    */
+
+typedef struct {
+  int32_t solver;        /* PZG_SOLVER_*                                       */
+  int32_t precond;       /* PZG_PRECOND_* (mixed solver only)                  */
+  int64_t max_iter;      /* total inner iterations allowed                     */
+  double tol;            /* relative residual target                           */
+  double inner_tol;      /* mixed: relative reduction per refinement step      */
+  int32_t max_outer;     /* mixed: refinement steps allowed                    */
+  int32_t check_every;   /* iterations per device batch between host checks   */
+  int32_t use_graph;     /* capture the iteration batch in a hipGraph          */
+} pzg_solve_params;
+
+typedef struct {
+  int64_t iterations;     /* CG iterations (inner total for the mixed solver)  */
+  int32_t outer;          /* refinement steps                                   */
+  int32_t converged;
+  double achieved;        /* PCG64: recursive ||r||/||b|| (TPZStepSolver fTol)  */
+  double true_residual;   /* ||f - K64 u|| / ||f|| in FP64                      */
+  double solve_ms;        /* device time of the solve                           */
+  int32_t fallback;       /* 1 if the learned preconditioner was replaced       */
+} pzg_solve_info;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+const char *pzg_last_error(void);
+int pzg_create(int device, pzg_ctx **out);
+void pzg_destroy(pzg_ctx *ctx);
+
+/* Flattens the computational mesh into device arrays (geometry, connect
+ * blocks, coefficients, shape tables at the quadrature points).
+ * Replaces the element loop set-up of TPZStructMatrixOR::Serial_Assemble
+ * (StrMatrix/pzstrmatrixor.cpp:104-180). */
+int pzg_mesh_set(pzg_ctx *ctx, const pzg_mesh_desc *mesh);
+
+/* Symbolic phase: the reference's upper-triangular CSR pattern, computed from
+ * the connect graph.  Replaces TPZSSpStructMatrix::Create / SetupMatrixData
+ * (StrMatrix/TPZSSpStructMatrix.cpp:30-193) with
+ * TPZRenumbering::ConvertGraph (External/TPZRenumbering.cpp:76-110). */
+int pzg_struct_create(pzg_ctx *ctx, int64_t *neq, int64_t *nnz);
+
+/* Copies the pattern to host arrays ia[neq+1], ja[nnz] (TPZSYsmpMatrix::IA/JA,
+ * Matrix/pzsysmp.h:104-127). */
+int pzg_struct_get_pattern(pzg_ctx *ctx, int64_t *ia, int64_t *ja);
+
+/* Numeric assembly into the pattern: element integration + deterministic
+ * coloured scatter.  Replaces TPZStructMatrixOR::Assemble (StrMatrix/
+ * pzstrmatrixor.cpp:41-71) -> CalcStiff (Mesh/pzinterpolationspace.cpp:403-474)
+ * -> TPZSYsmpMatrix::AddKel (Matrix/pzsysmp.cpp:369-411) and
+ * TPZFMatrix::AddFel (Matrix/pzfmatrix.cpp:284-298), then
+ * EndCreateAssemble -> ComputeDiagonal (StrMatrix/TPZSSpStructMatrix.cpp:24-28). */
+int pzg_assemble(pzg_ctx *ctx);
+
+/* Copies the assembled values a[nnz] and rhs[neq] to the host (either may be
+ * NULL).  TPZSYsmpMatrix::A (Matrix/pzsysmp.h:104-127) and the TPZFMatrix rhs. */
+int pzg_get_system(pzg_ctx *ctx, double *a, double *rhs);
+
+/* Installs a caller-built matrix (and optional rhs): TPZSYsmpMatrix::SetData
+ * (Matrix/pzsysmp.h:232-240). */
+int pzg_csr_set(pzg_ctx *ctx, int64_t neq, const int64_t *ia, const int64_t *ja,
+                const double *a, const double *rhs);
+
+/* AddKel of a batch of caller-given element matrices (row-major ne x ne each,
+ * with dest[nel*ne] global equations) into the current pattern, on the
+ * device, including the |v| < 1e-12 drop rule (Matrix/pzsysmp.cpp:381,
+ * Common/pzreal.h:736-741). */
+int pzg_add_kel_batch(pzg_ctx *ctx, int64_t nel, int32_t ne, const int64_t *dest,
+                      const double *ek);
+
+/* z = beta*y + alpha*K*x on host vectors of length neq.
+ * TPZSYsmpMatrix::MultAdd (Matrix/pzsysmp.cpp:189-232); y may be NULL when
+ * beta == 0. */
+int pzg_multadd(pzg_ctx *ctx, const double *x, const double *y, double *z, double alpha,
+                double beta);
+
+/* Solve K u = rhs.  rhs: host vector or NULL (use the assembled rhs on the
+ * device).  u: host output or NULL (keep on the device).
+ * PZG_SOLVER_PCG64 replaces TPZMatrix::SolveCG (Matrix/pzmatrix.cpp:750-765)
+ * with the EJacobi TPZStepSolver preconditioner (Solvers/pzstepsolver.cpp:
+ * 65-122, Matrix/pzmatrix.cpp:642-667), x starting at zero. */
+int pzg_solve(pzg_ctx *ctx, const pzg_solve_params *params, const double *rhs, double *u,
+              pzg_solve_info *info);
+
+/* FP64 true relative residual ||rhs - K u|| / ||rhs|| of the device solution. */
+int pzg_true_residual(pzg_ctx *ctx, double *rel);
+
+/* Average device duration (HIP events on the context stream) of nrep launches
+ * of the SpMV kernel of the given format: 0 = FP64 full CSR (plain PCG path),
+ * 1 = FP32 block CSR of the mixed path, 2 = element assembly (whole pass). */
+int pzg_time_kernel(pzg_ctx *ctx, int32_t which, int32_t nrep, double *avg_ms, double *bytes);
+
+/* Sizes of the device formats (for roofline accounting). */
+typedef struct {
+  int64_t neq, nnz_upper, nnz_full, nblocks, nnz_blocks, nvol, ncolors;
+  int32_t bs;  /* block size of the mixed-path block CSR */
+} pzg_sizes;
+int pzg_get_sizes(pzg_ctx *ctx, pzg_sizes *s);
+
+/* Synchronise the context stream. */
+int pzg_sync(pzg_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,747 @@
+// pzg_assemble.hip -- gfx950 kernels of the assembly phase.
+//
+//  * k_upper_ja      : equation-level JA of the reference upper CSR from the
+//                      block graph (SetupMatrixData, StrMatrix/
+//                      TPZSSpStructMatrix.cpp:134-188: diagonal block first,
+//                      then every coupled block in ascending order).
+//  * k_elem_maps     : per element and side pair, the index of the coupled
+//                      block inside the upper block row (replaces AddKel's row
+//                      search, Matrix/pzsysmp.cpp:369-411).
+//  * k_vol<...>      : one workgroup per element: geometry at the quadrature
+//                      points, gradient Gram blocks P^{ab} = sum_q w|J| dphi_a
+//                      dphi_b^T, the material's element matrix from P, and the
+//                      scatter of its upper part into the CSR with the
+//                      |v| < 1e-12 drop rule.  Elements of one colour share no
+//                      connect, so the read-modify-write needs no atomics and
+//                      the result is deterministic.
+//  * k_bc<...>       : boundary elements (penalty Dirichlet / Neumann loads).
+//  * k_expand_full / k_build_bsr : FP64 full CSR (plain PCG path) and the
+//                      FP32 scaled block CSR (mixed path) from the upper CSR.
+#include <hip/hip_runtime.h>
+
+#include "pzg_internal.h"
+
+namespace pzg {
+
+constexpr int TPB = 256;
+
+// --------------------------------------------------------------------------
+__global__ void k_upper_ja(int64_t nb, int bs, const int64_t *__restrict__ ubia,
+                           const int32_t *__restrict__ ubja, const int64_t *__restrict__ ia,
+                           int64_t *__restrict__ ja) {
+  int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  int lane = threadIdx.x & 63;
+  if (wave >= nb) return;
+  int64_t a = wave, u0 = ubia[a], nup = ubia[a + 1] - u0;
+  for (int k = 0; k < bs; k++) {
+    int64_t r = a * bs + k, base = ia[r], len = (bs - k) + bs * nup;
+    for (int64_t e = lane; e < len; e += 64) {
+      int64_t col;
+      if (e < bs - k) col = a * bs + k + e;
+      else {
+        int64_t e2 = e - (bs - k);
+        col = (int64_t)ubja[u0 + e2 / bs] * bs + e2 % bs;
+      }
+      ja[base + e] = col;
+    }
+  }
+}
+
+int launch_expand_upper_ja(Ctx &c) {
+  if (c.nb == 0) return PZG_OK;
+  int64_t threads = c.nb * 64;
+  hipLaunchKernelGGL(k_upper_ja, dim3((threads + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
+                     c.bs, c.d_ubia.as<int64_t>(), c.d_ubja.as<int32_t>(), c.d_ia.as<int64_t>(),
+                     c.d_ja.as<int64_t>());
+  PZG_HIP(hipGetLastError());
+  return PZG_OK;
+}
+
+// --------------------------------------------------------------------------
+// map[e][s1][s2]: -2 inactive pair, -1 same block, t >= 0 index of the larger
+// block inside the upper block row of the smaller one.
+__global__ void k_elem_maps(int64_t nel, int ns, const int64_t *__restrict__ conn,
+                            const int64_t *__restrict__ aidx, const int64_t *__restrict__ ubia,
+                            const int32_t *__restrict__ ubja, int32_t *__restrict__ map) {
+  int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t total = nel * ns * ns;
+  if (gid >= total) return;
+  int64_t e = gid / (ns * ns);
+  int rem = (int)(gid % (ns * ns)), s1 = rem / ns, s2 = rem % ns;
+  int64_t a = aidx[conn[e * ns + s1]], b = aidx[conn[e * ns + s2]];
+  int32_t v;
+  if (a < 0 || b < 0) v = -2;
+  else if (a == b) v = -1;
+  else {
+    int64_t lo = a < b ? a : b, hi = a < b ? b : a;
+    int64_t l = ubia[lo], h = ubia[lo + 1] - 1, pos = -1;
+    while (l <= h) {
+      int64_t m = (l + h) >> 1;
+      int64_t vm = ubja[m];
+      if (vm == hi) { pos = m; break; }
+      if (vm < hi) l = m + 1;
+      else h = m - 1;
+    }
+    v = pos < 0 ? -3 : (int32_t)(pos - ubia[lo]);
+  }
+  map[gid] = v;
+}
+
+int launch_elem_maps(Ctx &c) {
+  int ns = c.vol_nsides;
+  PZG_TRY(dev_alloc(c.d_vmap, sizeof(int32_t) * c.nvol * ns * ns));
+  int64_t tot = c.nvol * ns * ns;
+  hipLaunchKernelGGL(k_elem_maps, dim3((tot + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nvol, ns,
+                     c.d_vconn.as<int64_t>(), c.d_aidx.as<int64_t>(), c.d_ubia.as<int64_t>(),
+                     c.d_ubja.as<int32_t>(), c.d_vmap.as<int32_t>());
+  PZG_HIP(hipGetLastError());
+  if (c.nbc > 0) {
+    int nb = c.bc_nsides;
+    PZG_TRY(dev_alloc(c.d_bmap, sizeof(int32_t) * c.nbc * nb * nb));
+    int64_t t2 = c.nbc * nb * nb;
+    hipLaunchKernelGGL(k_elem_maps, dim3((t2 + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nbc, nb,
+                       c.d_bconn.as<int64_t>(), c.d_aidx.as<int64_t>(), c.d_ubia.as<int64_t>(),
+                       c.d_ubja.as<int32_t>(), c.d_bmap.as<int32_t>());
+    PZG_HIP(hipGetLastError());
+  }
+  return PZG_OK;
+}
+
+// --------------------------------------------------------------------------
+template <class T> __global__ void k_fill(T *p, int64_t n, T v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+template <class T> static void fill(Ctx &c, T *p, int64_t n, T v) {
+  if (n <= 0) return;
+  int64_t blocks = (n + TPB - 1) / TPB;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_fill<T>, dim3(blocks), dim3(TPB), 0, c.stream, p, n, v);
+}
+
+struct ElemArgs {
+  const double *xyz;
+  const int64_t *nodes;
+  const int64_t *conn;
+  const double *coef;
+  const int64_t *aidx;
+  const int32_t *map;
+  const int64_t *ia;
+  double *a;
+  double *rhs;
+  const ElemTables *tab;
+  const double *bumps;
+  const int32_t *bcid;
+  const int32_t *bctype;
+  const double *bcval;
+  int nbumps;
+  double bw;
+  double fconst[3];
+  int force_kind;
+  double scale, E, nu, big;
+  int material;
+  int bs;
+};
+
+__device__ inline void forcing(const ElemArgs &g, const double *x, double *f) {
+  f[0] = f[1] = f[2] = 0.;
+  if (g.force_kind == PZG_FORCE_CONST) {
+    f[0] = g.fconst[0];
+    f[1] = g.fconst[1];
+    f[2] = g.fconst[2];
+  } else if (g.force_kind == PZG_FORCE_BUMPS) {
+    double w2 = 2. * g.bw * g.bw;
+    for (int b = 0; b < g.nbumps; b++) {
+      const double *bp = g.bumps + 6 * b;
+      double r2 = (x[0] - bp[0]) * (x[0] - bp[0]) + (x[1] - bp[1]) * (x[1] - bp[1]) +
+                  (x[2] - bp[2]) * (x[2] - bp[2]);
+      double e = exp(-r2 / w2);
+      f[0] += bp[3] * e;
+      f[1] += bp[4] * e;
+      f[2] += bp[5] * e;
+    }
+  }
+}
+
+// Volume elements.  DIM: 2 (quad) / 3 (hex); NC corners; NSD sides; NSH shapes;
+// NS states.  One workgroup per element of the current colour.
+template <int DIM, int NC, int NSD, int NSH, int NS>
+__global__ __launch_bounds__(TPB) void k_vol(ElemArgs g, const int32_t *__restrict__ list) {
+  constexpr int NE = NSH * NS;
+  constexpr int NP = (NS == 3) ? 6 : 1;  // Gram blocks P^{ab}, a <= b
+  __shared__ double s_coord[NC * 3];
+  __shared__ double s_jinv[MAXQ][DIM][DIM];
+  __shared__ double s_wdet[MAXQ];
+  __shared__ double s_f[MAXQ][3];
+  __shared__ double s_G[MAXQ][DIM][NSH];
+  __shared__ double s_Gw[MAXQ][DIM][NSH];
+  __shared__ double s_P[NP][NSH][NSH];
+  __shared__ int64_t s_row[NE];
+  __shared__ int64_t s_blk[NSH];
+  __shared__ int32_t s_map[NSD * NSD];
+  const int tid = threadIdx.x;
+  const int64_t e = list[blockIdx.x];
+  const ElemTables &T = *g.tab;
+  const int nq = T.nq;
+
+  if (tid < NC * 3) s_coord[tid] = g.xyz[3 * g.nodes[e * NC + tid / 3] + tid % 3];
+  for (int k = tid; k < NSD * NSD; k += TPB) s_map[k] = g.map[e * NSD * NSD + k];
+  if (tid < NSH) s_blk[tid] = g.aidx[g.conn[e * NSD + T.shape_side[tid]]];
+  __syncthreads();
+  // geometry per quadrature point (TPZGeoEl::Jacobian, Mesh/pzgeoel.cpp:1168-1355)
+  if (tid < nq) {
+    const int q = tid;
+    double gx[3][DIM], x[3] = {0, 0, 0};
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < DIM; j++) gx[i][j] = 0.;
+    for (int n = 0; n < NC; n++)
+      for (int i = 0; i < 3; i++) {
+        x[i] += s_coord[3 * n + i] * T.cphi[q][n];
+        for (int j = 0; j < DIM; j++) gx[i][j] += s_coord[3 * n + i] * T.cdphi[q][j][n];
+      }
+    double det;
+    if (DIM == 3) {
+      double(*j)[DIM] = gx;
+      det = -j[0][2] * j[1][1] * j[2][0] + j[0][1] * j[1][2] * j[2][0] + j[0][2] * j[1][0] * j[2][1] -
+            j[0][0] * j[1][2] * j[2][1] - j[0][1] * j[1][0] * j[2][2] + j[0][0] * j[1][1] * j[2][2];
+      double inv[3][3];
+      inv[0][0] = (-j[1][2] * j[2][1] + j[1][1] * j[2][2]) / det;
+      inv[0][1] = (j[0][2] * j[2][1] - j[0][1] * j[2][2]) / det;
+      inv[0][2] = (-j[0][2] * j[1][1] + j[0][1] * j[1][2]) / det;
+      inv[1][0] = (j[1][2] * j[2][0] - j[1][0] * j[2][2]) / det;
+      inv[1][1] = (-j[0][2] * j[2][0] + j[0][0] * j[2][2]) / det;
+      inv[1][2] = (j[0][2] * j[1][0] - j[0][0] * j[1][2]) / det;
+      inv[2][0] = (-j[1][1] * j[2][0] + j[1][0] * j[2][1]) / det;
+      inv[2][1] = (j[0][1] * j[2][0] - j[0][0] * j[2][1]) / det;
+      inv[2][2] = (-j[0][1] * j[1][0] + j[0][0] * j[1][1]) / det;
+      for (int a = 0; a < DIM; a++)
+        for (int b = 0; b < DIM; b++) s_jinv[q][a][b] = inv[a][b];
+    } else {
+      // 2D element in 3D space: Gram-Schmidt frame, jac upper triangular
+      double n1 = 0, d12 = 0, n2 = 0;
+      for (int i = 0; i < 3; i++) { n1 += gx[i][0] * gx[i][0]; d12 += gx[i][0] * gx[i][1]; }
+      n1 = sqrt(n1);
+      for (int i = 0; i < 3; i++) {
+        double v2 = gx[i][1] - d12 * (gx[i][0] / n1) / n1;
+        n2 += v2 * v2;
+      }
+      n2 = sqrt(n2);
+      double j01 = d12 / n1;
+      det = n1 * n2;
+      s_jinv[q][0][0] = n2 / det;
+      s_jinv[q][1][1] = n1 / det;
+      s_jinv[q][0][1] = -j01 / det;
+      s_jinv[q][1][0] = 0.;
+    }
+    s_wdet[q] = T.w[q] * fabs(det);
+    double f[3];
+    forcing(g, x, f);
+    s_f[q][0] = f[0];
+    s_f[q][1] = f[1];
+    s_f[q][2] = f[2];
+  }
+  __syncthreads();
+  // dphix = jacinv^T dphi (Mesh/TPZCompElH1.cpp:139-149)
+  for (int k = tid; k < nq * NSH; k += TPB) {
+    int q = k / NSH, i = k % NSH;
+    for (int a = 0; a < DIM; a++) {
+      double s = 0.;
+      for (int b = 0; b < DIM; b++) s += s_jinv[q][b][a] * T.dphi[q][b][i];
+      s_G[q][a][i] = s;
+      s_Gw[q][a][i] = s * s_wdet[q];
+    }
+  }
+  // row starts of the element's equations in the upper CSR
+  for (int k = tid; k < NE; k += TPB) {
+    int64_t b = s_blk[k / NS];
+    s_row[k] = b >= 0 ? g.ia[b * g.bs + k % NS] : -1;
+  }
+  __syncthreads();
+  // Gram blocks
+  for (int k = tid; k < NP * NSH * NSH; k += TPB) {
+    int pb = k / (NSH * NSH), ij = k % (NSH * NSH), i = ij / NSH, j = ij % NSH;
+    double s = 0.;
+    if (NS == 3) {
+      const int PA[6] = {0, 0, 0, 1, 1, 2}, PB[6] = {0, 1, 2, 1, 2, 2};
+      int a = PA[pb], b = PB[pb];
+      for (int q = 0; q < nq; q++) s += s_Gw[q][a][i] * s_G[q][b][j];
+    } else {
+      for (int q = 0; q < nq; q++)
+        for (int a = 0; a < DIM; a++) s += s_Gw[q][a][i] * s_G[q][a][j];
+    }
+    s_P[pb][i][j] = s;
+  }
+  __syncthreads();
+  // material constants
+  double coef = g.coef ? g.coef[e] : (g.material == PZG_MAT_ELAST3D ? g.E : g.scale);
+  double C1 = 0, C2 = 0, C3 = 0;
+  if (NS == 3) {
+    double E = coef, nu = g.nu;
+    C1 = E / (2. + 2. * nu);
+    C2 = E * nu / (-1. + nu + 2. * nu * nu);
+    C3 = E * (nu - 1.) / (-1. + nu + 2. * nu * nu);
+  }
+  // element matrix entries, upper part by global equation, scatter
+  for (int k = tid; k < NE * NE; k += TPB) {
+    int I = k / NE, J = k % NE;
+    int i = I / NS, ci = I % NS, j = J / NS, cj = J % NS;
+    int64_t bi = s_blk[i], bj = s_blk[j];
+    if (bi < 0 || bj < 0) continue;
+    if (bi > bj || (bi == bj && (ci > cj || (ci == cj && i > j)))) continue;
+    double v;
+    if (NS == 3) {
+      // P^{xy}[i][j] = sum_q w dphi_x(i) dphi_y(j)
+      auto P = [&](int x, int y, int ii, int jj) -> double {
+        const int idx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+        return x <= y ? s_P[idx[x][y]][ii][jj] : s_P[idx[y][x]][jj][ii];
+      };
+      if (ci == cj) {
+        double tr = 0.;
+        for (int cc = 0; cc < 3; cc++)
+          if (cc != ci) tr += P(cc, cc, i, j);
+        v = tr * C1 + P(ci, ci, i, j) * C3;
+      } else {
+        v = P(cj, ci, i, j) * C1 - P(ci, cj, i, j) * C2;
+      }
+    } else {
+      v = coef * s_P[0][i][j];
+    }
+    if (fabs(v) < 1.e-12) continue;  // IsZero drop rule of AddKel
+    int64_t slot;
+    if (bi == bj) slot = s_row[I] + (cj - ci);
+    else {
+      int t = s_map[T.shape_side[i] * NSD + T.shape_side[j]];
+      slot = s_row[I] + g.bs + (int64_t)t * g.bs - ci + cj;
+    }
+    g.a[slot] += v;
+  }
+  // load vector
+  for (int I = tid; I < NE; I += TPB) {
+    int i = I / NS, ci = I % NS;
+    if (s_blk[i] < 0) continue;
+    double s = 0.;
+    for (int q = 0; q < nq; q++) s += s_wdet[q] * T.phi[q][i] * s_f[q][ci];
+    if (g.material == PZG_MAT_POISSON) s *= g.scale;
+    g.rhs[s_blk[i] * g.bs + ci] += s;
+  }
+}
+
+// Boundary elements (dimension DIMB = DIM-1 embedded in 3D).
+template <int DIMB, int NC, int NSD, int NSH, int NS>
+__global__ __launch_bounds__(TPB) void k_bc(ElemArgs g, const int32_t *__restrict__ list) {
+  constexpr int NE = NSH * NS;
+  __shared__ double s_coord[NC * 3];
+  __shared__ double s_wdet[MAXQ];
+  __shared__ double s_M[NSH][NSH];
+  __shared__ int64_t s_row[NE];
+  __shared__ int64_t s_blk[NSH];
+  __shared__ int32_t s_map[NSD * NSD];
+  const int tid = threadIdx.x;
+  const int64_t e = list[blockIdx.x];
+  const ElemTables &T = *g.tab;
+  const int nq = T.nq;
+  const int id = g.bcid[e];
+  const int type = g.bctype[id];
+  const double *val = g.bcval + 3 * id;
+  if (tid < NC * 3) s_coord[tid] = g.xyz[3 * g.nodes[e * NC + tid / 3] + tid % 3];
+  for (int k = tid; k < NSD * NSD; k += TPB) s_map[k] = g.map[e * NSD * NSD + k];
+  if (tid < NSH) s_blk[tid] = g.aidx[g.conn[e * NSD + T.shape_side[tid]]];
+  __syncthreads();
+  if (tid < nq) {
+    const int q = tid;
+    double gx[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    for (int n = 0; n < NC; n++)
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < DIMB; j++) gx[i][j] += s_coord[3 * n + i] * T.cdphi[q][j][n];
+    double det;
+    if (DIMB == 1) {
+      double n1 = 0;
+      for (int i = 0; i < 3; i++) n1 += gx[i][0] * gx[i][0];
+      det = sqrt(n1);
+    } else {
+      double n1 = 0, d12 = 0, n2 = 0;
+      for (int i = 0; i < 3; i++) { n1 += gx[i][0] * gx[i][0]; d12 += gx[i][0] * gx[i][1]; }
+      n1 = sqrt(n1);
+      for (int i = 0; i < 3; i++) {
+        double v2 = gx[i][1] - d12 * (gx[i][0] / n1) / n1;
+        n2 += v2 * v2;
+      }
+      det = n1 * sqrt(n2);
+    }
+    s_wdet[q] = T.w[q] * fabs(det);
+  }
+  for (int k = tid; k < NE; k += TPB) {
+    int64_t b = s_blk[k / NS];
+    s_row[k] = b >= 0 ? g.ia[b * g.bs + k % NS] : -1;
+  }
+  __syncthreads();
+  for (int k = tid; k < NSH * NSH; k += TPB) {
+    int i = k / NSH, j = k % NSH;
+    double s = 0.;
+    for (int q = 0; q < nq; q++) s += T.phi[q][i] * T.phi[q][j] * s_wdet[q];
+    s_M[i][j] = s;
+  }
+  __syncthreads();
+  if (type == PZG_BC_DIRICHLET) {
+    for (int k = tid; k < NE * NE; k += TPB) {
+      int I = k / NE, J = k % NE;
+      int i = I / NS, ci = I % NS, j = J / NS, cj = J % NS;
+      if (ci != cj) continue;  // the penalty couples equal components only
+      int64_t bi = s_blk[i], bj = s_blk[j];
+      if (bi < 0 || bj < 0) continue;
+      if (bi > bj || (bi == bj && i > j)) continue;
+      double v = g.big * s_M[i][j];
+      if (fabs(v) < 1.e-12) continue;
+      int64_t slot;
+      if (bi == bj) slot = s_row[I];
+      else {
+        int t = s_map[T.shape_side[i] * NSD + T.shape_side[j]];
+        slot = s_row[I] + g.bs + (int64_t)t * g.bs;
+      }
+      g.a[slot] += v;
+    }
+  }
+  for (int I = tid; I < NE; I += TPB) {
+    int i = I / NS, ci = I % NS;
+    if (s_blk[i] < 0) continue;
+    double s = 0.;
+    for (int q = 0; q < nq; q++) s += T.phi[q][i] * s_wdet[q];
+    double f;
+    if (type == PZG_BC_DIRICHLET) f = g.big * val[ci] * s;
+    else f = val[ci] * s * (g.material == PZG_MAT_POISSON ? g.scale : 1.0);
+    g.rhs[s_blk[i] * g.bs + ci] += f;
+  }
+}
+
+static ElemArgs make_args(Ctx &c, bool vol) {
+  ElemArgs g;
+  g.xyz = c.d_xyz.as<double>();
+  g.nodes = vol ? c.d_vnodes.as<int64_t>() : c.d_bnodes.as<int64_t>();
+  g.conn = vol ? c.d_vconn.as<int64_t>() : c.d_bconn.as<int64_t>();
+  g.coef = (vol && !c.vol_coef.empty()) ? c.d_vcoef.as<double>() : nullptr;
+  g.aidx = c.d_aidx.as<int64_t>();
+  g.map = vol ? c.d_vmap.as<int32_t>() : c.d_bmap.as<int32_t>();
+  g.ia = c.d_ia.as<int64_t>();
+  g.a = c.d_a.as<double>();
+  g.rhs = c.d_rhs.as<double>();
+  g.tab = vol ? c.d_tvol.as<ElemTables>() : c.d_tbc.as<ElemTables>();
+  g.bumps = c.d_bumps.as<double>();
+  g.bcid = c.d_bcid.as<int32_t>();
+  g.bctype = c.d_bctype.as<int32_t>();
+  g.bcval = c.d_bcval.as<double>();
+  g.nbumps = c.nbumps;
+  g.bw = c.bump_width;
+  for (int k = 0; k < 3; k++) g.fconst[k] = c.force_const[k];
+  g.force_kind = c.force_kind;
+  g.scale = c.scale;
+  g.E = c.E;
+  g.nu = c.nu;
+  g.big = c.big;
+  g.material = c.material;
+  g.bs = c.bs;
+  return g;
+}
+
+template <int DIM, int NC, int NSD, int NSH, int NS>
+static int run_vol(Ctx &c) {
+  ElemArgs g = make_args(c, true);
+  for (size_t k = 0; k + 1 < c.vcolor_ptr.size(); k++) {
+    int64_t n = c.vcolor_ptr[k + 1] - c.vcolor_ptr[k];
+    if (!n) continue;
+    hipLaunchKernelGGL((k_vol<DIM, NC, NSD, NSH, NS>), dim3(n), dim3(TPB), 0, c.stream, g,
+                       c.d_vcolor_list.as<int32_t>() + c.vcolor_ptr[k]);
+    PZG_HIP(hipGetLastError());
+  }
+  return PZG_OK;
+}
+
+template <int DIMB, int NC, int NSD, int NSH, int NS>
+static int run_bc(Ctx &c) {
+  if (c.nbc == 0) return PZG_OK;
+  ElemArgs g = make_args(c, false);
+  for (size_t k = 0; k + 1 < c.bcolor_ptr.size(); k++) {
+    int64_t n = c.bcolor_ptr[k + 1] - c.bcolor_ptr[k];
+    if (!n) continue;
+    hipLaunchKernelGGL((k_bc<DIMB, NC, NSD, NSH, NS>), dim3(n), dim3(TPB), 0, c.stream, g,
+                       c.d_bcolor_list.as<int32_t>() + c.bcolor_ptr[k]);
+    PZG_HIP(hipGetLastError());
+  }
+  return PZG_OK;
+}
+
+int launch_assemble(Ctx &c) {
+  fill<double>(c, c.d_a.as<double>(), c.nnz_u, 0.);
+  fill<double>(c, c.d_rhs.as<double>(), c.neq, 0.);
+  if (c.tvol.nq > MAXQ || c.tbc.nq > MAXQ) return PZG_EUNSUP;
+  int p = c.porder;
+  if (c.dim == 3 && c.nstate == 3) {
+    if (p == 2) { PZG_TRY((run_vol<3, 8, 27, 27, 3>(c))); PZG_TRY((run_bc<2, 4, 9, 9, 3>(c))); }
+    else { PZG_TRY((run_vol<3, 8, 27, 8, 3>(c))); PZG_TRY((run_bc<2, 4, 9, 4, 3>(c))); }
+  } else if (c.dim == 3 && c.nstate == 1) {
+    if (p == 2) { PZG_TRY((run_vol<3, 8, 27, 27, 1>(c))); PZG_TRY((run_bc<2, 4, 9, 9, 1>(c))); }
+    else { PZG_TRY((run_vol<3, 8, 27, 8, 1>(c))); PZG_TRY((run_bc<2, 4, 9, 4, 1>(c))); }
+  } else if (c.dim == 2 && c.nstate == 1) {
+    if (p == 2) { PZG_TRY((run_vol<2, 4, 9, 9, 1>(c))); PZG_TRY((run_bc<1, 2, 3, 3, 1>(c))); }
+    else { PZG_TRY((run_vol<2, 4, 9, 4, 1>(c))); PZG_TRY((run_bc<1, 2, 3, 2, 1>(c))); }
+  } else {
+    set_error("pzg_assemble: unsupported dimension/state combination");
+    return PZG_EUNSUP;
+  }
+  return PZG_OK;
+}
+
+// --------------------------------------------------------------------------
+__global__ void k_diag(int64_t n, const int64_t *__restrict__ ia, const double *__restrict__ a,
+                       double *__restrict__ d) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = a[ia[i]];  // diagonal is the first entry of every row
+}
+
+int launch_diag(Ctx &c) {
+  if (!c.neq) return PZG_OK;
+  hipLaunchKernelGGL(k_diag, dim3((c.neq + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.neq,
+                     c.d_ia.as<int64_t>(), c.d_a.as<double>(), c.d_diag.as<double>());
+  PZG_HIP(hipGetLastError());
+  return PZG_OK;
+}
+
+// --------------------------------------------------------------------------
+// Full CSR column indices (once per pattern).
+__global__ void k_full_ja(int64_t nb, int bs, const int64_t *__restrict__ fbia,
+                          const int32_t *__restrict__ fbja, const int64_t *__restrict__ fia,
+                          int32_t *__restrict__ fja) {
+  int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  int lane = threadIdx.x & 63;
+  if (wave >= nb) return;
+  int64_t b0 = fbia[wave], nbk = fbia[wave + 1] - b0;
+  for (int k = 0; k < bs; k++) {
+    int64_t base = fia[wave * bs + k];
+    for (int64_t e = lane; e < nbk * bs; e += 64)
+      fja[base + e] = fbja[b0 + e / bs] * bs + (int32_t)(e % bs);
+  }
+}
+
+// Upper CSR values -> full CSR values: diagonal blocks (k_expand_diag) and
+// coupled blocks with their transposes (k_expand_off).
+__global__ void k_nlow(int64_t nb, const int64_t *__restrict__ fbia, const int64_t *__restrict__ ubia,
+                       int32_t *__restrict__ nlow) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nb) nlow[i] = (int32_t)((fbia[i + 1] - fbia[i]) - 1 - (ubia[i + 1] - ubia[i]));
+}
+
+// number of lower blocks of every block row (= rank of the diagonal block)
+__global__ void k_expand_diag(int64_t nb, int bs, const int64_t *__restrict__ ia,
+                              const double *__restrict__ a, const int32_t *__restrict__ nlow,
+                              const int64_t *__restrict__ fia, double *__restrict__ fa) {
+  int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= nb * bs * bs) return;
+  int64_t A = gid / (bs * bs);
+  int k = (int)((gid % (bs * bs)) / bs), l = (int)(gid % bs);
+  if (l < k) return;
+  double v = a[ia[A * bs + k] + (l - k)];
+  int64_t nl = nlow[A];
+  fa[fia[A * bs + k] + nl * bs + l] = v;
+  if (l != k) fa[fia[A * bs + l] + nl * bs + k] = v;
+}
+
+__global__ void k_expand_off(int64_t nb, int bs, const int64_t *__restrict__ ia,
+                             const double *__restrict__ a, const int64_t *__restrict__ ubia,
+                             const int32_t *__restrict__ ubja, const int32_t *__restrict__ rfwd,
+                             const int32_t *__restrict__ rbwd, const int64_t *__restrict__ fia,
+                             double *__restrict__ fa) {
+  int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  int lane = threadIdx.x & 63;
+  if (wave >= nb) return;
+  const int64_t A = wave, u0 = ubia[A], nup = ubia[A + 1] - u0;
+  const int bb = bs * bs;
+  for (int64_t e = lane; e < nup * bb; e += 64) {
+    int64_t t = e / bb;
+    int k = (int)((e % bb) / bs), l = (int)(e % bs);
+    int64_t B = ubja[u0 + t];
+    double v = a[ia[A * bs + k] + bs + t * bs - k + l];
+    fa[fia[A * bs + k] + (int64_t)rfwd[u0 + t] * bs + l] = v;
+    fa[fia[B * bs + l] + (int64_t)rbwd[u0 + t] * bs + k] = v;
+  }
+}
+
+int launch_expand_full(Ctx &c) {
+  PZG_TRY(dev_alloc(c.d_fja, sizeof(int32_t) * c.nnz_full));
+  PZG_TRY(dev_alloc(c.d_fa, sizeof(double) * c.nnz_full));
+  if (!c.fja_valid) {
+    int64_t threads = c.nb * 64;
+    hipLaunchKernelGGL(k_full_ja, dim3((threads + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
+                       c.bs, c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(), c.d_fia.as<int64_t>(),
+                       c.d_fja.as<int32_t>());
+    PZG_HIP(hipGetLastError());
+    c.fja_valid = true;
+  }
+  DevBuf nl;
+  PZG_TRY(dev_alloc(nl, sizeof(int32_t) * c.nb));
+  hipLaunchKernelGGL(k_nlow, dim3((c.nb + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
+                     c.d_fbia.as<int64_t>(), c.d_ubia.as<int64_t>(), nl.as<int32_t>());
+  int64_t nd = c.nb * c.bs * c.bs;
+  hipLaunchKernelGGL(k_expand_diag, dim3((nd + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb, c.bs,
+                     c.d_ia.as<int64_t>(), c.d_a.as<double>(), nl.as<int32_t>(),
+                     c.d_fia.as<int64_t>(), c.d_fa.as<double>());
+  int64_t threads = c.nb * 64;
+  hipLaunchKernelGGL(k_expand_off, dim3((threads + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
+                     c.bs, c.d_ia.as<int64_t>(), c.d_a.as<double>(), c.d_ubia.as<int64_t>(),
+                     c.d_ubja.as<int32_t>(), c.d_rank_fwd.as<int32_t>(),
+                     c.d_rank_bwd.as<int32_t>(), c.d_fia.as<int64_t>(), c.d_fa.as<double>());
+  PZG_HIP(hipGetLastError());
+  PZG_HIP(hipStreamSynchronize(c.stream));
+  dev_free(nl);
+  c.full_valid = true;
+  return PZG_OK;
+}
+
+// --------------------------------------------------------------------------
+// Scaled FP32 block CSR S = D^-1/2 K D^-1/2 (unit diagonal) and the inverses
+// of its diagonal blocks.
+__global__ void k_dinvsqrt(int64_t n, const double *__restrict__ d, double *__restrict__ s) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) s[i] = 1.0 / sqrt(fabs(d[i]));
+}
+
+template <int BS>
+__global__ void k_bsr_diag(int64_t nb, const int64_t *__restrict__ ia, const double *__restrict__ a,
+                           const int32_t *__restrict__ nlow, const int64_t *__restrict__ fbia,
+                           const double *__restrict__ dis, float *__restrict__ bsr,
+                           float *__restrict__ binv) {
+  int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (A >= nb) return;
+  double m[BS][BS];
+  for (int k = 0; k < BS; k++)
+    for (int l = k; l < BS; l++) {
+      double v = a[ia[A * BS + k] + (l - k)] * dis[A * BS + k] * dis[A * BS + l];
+      m[k][l] = v;
+      m[l][k] = v;
+    }
+  float *blk = bsr + (fbia[A] + nlow[A]) * BS * BS;
+  for (int k = 0; k < BS; k++)
+    for (int l = 0; l < BS; l++) blk[k * BS + l] = (float)m[k][l];
+  float *inv = binv + A * BS * BS;
+  if (BS == 1) {
+    inv[0] = (float)(1.0 / m[0][0]);
+  } else {
+    // 3x3 inverse by cofactors
+    double c00 = m[1][1] * m[2][2] - m[1][2] * m[2][1];
+    double c01 = m[1][2] * m[2][0] - m[1][0] * m[2][2];
+    double c02 = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+    double det = m[0][0] * c00 + m[0][1] * c01 + m[0][2] * c02;
+    double id = 1.0 / det;
+    double r[3][3];
+    r[0][0] = c00 * id;
+    r[1][0] = c01 * id;
+    r[2][0] = c02 * id;
+    r[0][1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) * id;
+    r[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * id;
+    r[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) * id;
+    r[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * id;
+    r[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * id;
+    r[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * id;
+    for (int k = 0; k < 3; k++)
+      for (int l = 0; l < 3; l++) inv[k * 3 + l] = (float)r[k][l];
+  }
+}
+
+template <int BS>
+__global__ void k_bsr_off(int64_t nb, const int64_t *__restrict__ ia, const double *__restrict__ a,
+                          const int64_t *__restrict__ ubia, const int32_t *__restrict__ ubja,
+                          const int32_t *__restrict__ rfwd, const int32_t *__restrict__ rbwd,
+                          const int64_t *__restrict__ fbia, const double *__restrict__ dis,
+                          float *__restrict__ bsr) {
+  int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  int lane = threadIdx.x & 63;
+  if (wave >= nb) return;
+  const int64_t A = wave, u0 = ubia[A], nup = ubia[A + 1] - u0;
+  constexpr int BB = BS * BS;
+  for (int64_t e = lane; e < nup * BB; e += 64) {
+    int64_t t = e / BB;
+    int k = (int)((e % BB) / BS), l = (int)(e % BS);
+    int64_t B = ubja[u0 + t];
+    double v = a[ia[A * BS + k] + BS + t * BS - k + l] * dis[A * BS + k] * dis[B * BS + l];
+    bsr[(fbia[A] + rfwd[u0 + t]) * BB + k * BS + l] = (float)v;
+    bsr[(fbia[B] + rbwd[u0 + t]) * BB + l * BS + k] = (float)v;
+  }
+}
+
+int launch_build_bsr(Ctx &c) {
+  const int bs = c.bs;
+  PZG_TRY(dev_alloc(c.d_bsr, sizeof(float) * c.nbf * bs * bs));
+  PZG_TRY(dev_alloc(c.d_bjinv, sizeof(float) * c.nb * bs * bs));
+  PZG_TRY(dev_alloc(c.d_dinvsqrt, sizeof(double) * c.neq));
+  DevBuf nl;
+  PZG_TRY(dev_alloc(nl, sizeof(int32_t) * c.nb));
+  hipLaunchKernelGGL(k_nlow, dim3((c.nb + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
+                     c.d_fbia.as<int64_t>(), c.d_ubia.as<int64_t>(), nl.as<int32_t>());
+  hipLaunchKernelGGL(k_dinvsqrt, dim3((c.neq + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.neq,
+                     c.d_diag.as<double>(), c.d_dinvsqrt.as<double>());
+  int64_t threads = c.nb * 64;
+  if (bs == 3) {
+    hipLaunchKernelGGL(k_bsr_diag<3>, dim3((c.nb + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
+                       c.d_ia.as<int64_t>(), c.d_a.as<double>(), nl.as<int32_t>(),
+                       c.d_fbia.as<int64_t>(), c.d_dinvsqrt.as<double>(), c.d_bsr.as<float>(),
+                       c.d_bjinv.as<float>());
+    hipLaunchKernelGGL(k_bsr_off<3>, dim3((threads + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
+                       c.d_ia.as<int64_t>(), c.d_a.as<double>(), c.d_ubia.as<int64_t>(),
+                       c.d_ubja.as<int32_t>(), c.d_rank_fwd.as<int32_t>(),
+                       c.d_rank_bwd.as<int32_t>(), c.d_fbia.as<int64_t>(),
+                       c.d_dinvsqrt.as<double>(), c.d_bsr.as<float>());
+  } else if (bs == 1) {
+    hipLaunchKernelGGL(k_bsr_diag<1>, dim3((c.nb + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
+                       c.d_ia.as<int64_t>(), c.d_a.as<double>(), nl.as<int32_t>(),
+                       c.d_fbia.as<int64_t>(), c.d_dinvsqrt.as<double>(), c.d_bsr.as<float>(),
+                       c.d_bjinv.as<float>());
+    hipLaunchKernelGGL(k_bsr_off<1>, dim3((threads + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
+                       c.d_ia.as<int64_t>(), c.d_a.as<double>(), c.d_ubia.as<int64_t>(),
+                       c.d_ubja.as<int32_t>(), c.d_rank_fwd.as<int32_t>(),
+                       c.d_rank_bwd.as<int32_t>(), c.d_fbia.as<int64_t>(),
+                       c.d_dinvsqrt.as<double>(), c.d_bsr.as<float>());
+  } else {
+    dev_free(nl);
+    set_error("block CSR: unsupported block size");
+    return PZG_EUNSUP;
+  }
+  PZG_HIP(hipGetLastError());
+  PZG_HIP(hipStreamSynchronize(c.stream));
+  dev_free(nl);
+  c.bsr_valid = true;
+  return PZG_OK;
+}
+
+// --------------------------------------------------------------------------
+// AddKel of caller-given element matrices (one workgroup per element).
+__global__ void k_addkel(int ne, const int64_t *__restrict__ dest, const double *__restrict__ ek,
+                         const int32_t *__restrict__ list, const int64_t *__restrict__ ia,
+                         const int64_t *__restrict__ ja, double *__restrict__ a) {
+  int64_t e = list[blockIdx.x];
+  for (int k = threadIdx.x; k < ne * ne; k += blockDim.x) {
+    int i = k / ne, j = k % ne;
+    int64_t ip = dest[e * ne + i], jp = dest[e * ne + j];
+    if (jp < ip) continue;
+    double v = ek[(e * ne + i) * (int64_t)ne + j];
+    if (fabs(v) < 1.e-12) continue;
+    int64_t l = ia[ip], h = ia[ip + 1] - 1;
+    while (l <= h) {
+      int64_t m = (l + h) >> 1;
+      if (ja[m] == jp) { a[m] += v; break; }
+      if (ja[m] < jp) l = m + 1;
+      else h = m - 1;
+    }
+  }
+}
+
+int launch_addkel_batch(Ctx &c, int64_t nel, int ne, const int64_t *d_dest, const double *d_ek,
+                        const int32_t *d_list, int64_t nlist) {
+  (void)nel;
+  if (!nlist) return PZG_OK;
+  hipLaunchKernelGGL(k_addkel, dim3(nlist), dim3(TPB), 0, c.stream, ne, d_dest, d_ek, d_list,
+                     c.d_ia.as<int64_t>(), c.d_ja.as<int64_t>(), c.d_a.as<double>());
+  PZG_HIP(hipGetLastError());
+  return PZG_OK;
+}
+
+}  // namespace pzg

@@ -1,0 +1,780 @@
+// pzg_host.cpp -- C ABI of the HIP linear FEM path: context, mesh flattening,
+// symbolic phase (pattern + maps + colouring) and the entry points that
+// dispatch to the device kernels in pzg_assemble.hip / pzg_solve.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <thread>
+
+#include "pzg_internal.h"
+
+namespace pzg {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+int dev_alloc(DevBuf &b, size_t bytes) {
+  if (b.p && b.bytes >= bytes && bytes > 0) return PZG_OK;
+  dev_free(b);
+  if (bytes == 0) return PZG_OK;
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    set_error("hipMalloc(" + std::to_string(bytes) + " bytes): " + hipGetErrorString(e));
+    return PZG_ENOMEM;
+  }
+  b.bytes = bytes;
+  return PZG_OK;
+}
+
+void dev_free(DevBuf &b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+static int nthreads() {
+  unsigned n = std::thread::hardware_concurrency();
+  if (n == 0) n = 4;
+  return (int)std::min(16u, n);
+}
+
+template <class F> static void parallel_for(int64_t n, F f) {
+  int T = nthreads();
+  if (n < 4096) T = 1;
+  std::vector<std::thread> th;
+  for (int t = 0; t < T; t++)
+    th.emplace_back([=]() {
+      int64_t lo = n * t / T, hi = n * (t + 1) / T;
+      for (int64_t i = lo; i < hi; i++) f(i);
+    });
+  for (auto &x : th) x.join();
+}
+
+static void free_all(Ctx &c) {
+  DevBuf *bufs[] = {&c.d_ia, &c.d_ja, &c.d_a, &c.d_rhs, &c.d_diag, &c.d_ubia, &c.d_ubja,
+                    &c.d_fbia, &c.d_fbja, &c.d_rank_fwd, &c.d_rank_bwd, &c.d_fia, &c.d_fja,
+                    &c.d_fa, &c.d_bsr, &c.d_dinvsqrt, &c.d_bjinv, &c.d_xyz, &c.d_vnodes,
+                    &c.d_vconn, &c.d_vcoef, &c.d_bnodes, &c.d_bconn, &c.d_bcid, &c.d_bctype,
+                    &c.d_bcval, &c.d_aidx, &c.d_bumps, &c.d_vmap, &c.d_bmap, &c.d_tvol,
+                    &c.d_tbc, &c.d_vcolor_list, &c.d_bcolor_list, &c.d_x, &c.d_r, &c.d_z,
+                    &c.d_p, &c.d_p2, &c.d_q, &c.d_u, &c.d_f, &c.d_w, &c.d_xs, &c.d_rs,
+                    &c.d_zs, &c.d_ps, &c.d_ps2, &c.d_qs, &c.d_bs, &c.d_scal, &c.d_partials,
+                    &c.d_counter, &c.d_nn_w, &c.d_nn_h0, &c.d_nn_h1, &c.d_nn_agg};
+  for (DevBuf *b : bufs) dev_free(*b);
+}
+
+// Greedy first-fit colouring over connects (the idea of
+// TPZStructMatrix::ComputeElementColors, StrMatrix/TPZStructMatrix.cpp:104-161):
+// elements of one colour share no active connect, so their scatter is race-free.
+static int color_elements(int64_t nel, int nsides, const std::vector<int64_t> &conn,
+                          const std::vector<int64_t> &aidx, int64_t nb,
+                          std::vector<int64_t> &ptr, std::vector<int32_t> &list) {
+  std::vector<uint64_t> used(nb, 0);
+  std::vector<int> col(nel);
+  int ncol = 0;
+  for (int64_t e = 0; e < nel; e++) {
+    uint64_t m = 0;
+    for (int s = 0; s < nsides; s++) {
+      int64_t a = aidx[conn[e * nsides + s]];
+      if (a >= 0) m |= used[a];
+    }
+    if (m == ~0ULL) {
+      set_error("element colouring needs more than 64 colours");
+      return PZG_EUNSUP;
+    }
+    int k = __builtin_ctzll(~m);
+    col[e] = k;
+    ncol = std::max(ncol, k + 1);
+    for (int s = 0; s < nsides; s++) {
+      int64_t a = aidx[conn[e * nsides + s]];
+      if (a >= 0) used[a] |= (1ULL << k);
+    }
+  }
+  ptr.assign(ncol + 1, 0);
+  for (int64_t e = 0; e < nel; e++) ptr[col[e] + 1]++;
+  for (int k = 0; k < ncol; k++) ptr[k + 1] += ptr[k];
+  list.resize(nel);
+  std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
+  for (int64_t e = 0; e < nel; e++) list[fill[col[e]]++] = (int32_t)e;
+  return PZG_OK;
+}
+
+static int etype_nc(int t) { return t == PZG_LINE ? 2 : t == PZG_QUAD ? 4 : 8; }
+static int etype_ns(int t) { return t == PZG_LINE ? 3 : t == PZG_QUAD ? 9 : 27; }
+
+static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
+  if (!m || (m->dim != 2 && m->dim != 3) || m->porder < 1 || m->porder > 2 ||
+      (m->nstate != 1 && m->nstate != 3) || m->material < 0 || m->material > 2 ||
+      (m->material == PZG_MAT_ELAST3D && (m->nstate != 3 || m->dim != 3)) ||
+      (m->material != PZG_MAT_ELAST3D && m->nstate != 1) || m->nvol <= 0 || !m->xyz ||
+      !m->vol_nodes || !m->vol_conn || !m->blk_pos || !m->blk_size || m->nconn <= 0) {
+    set_error("pzg_mesh_set: invalid or unsupported mesh description");
+    return PZG_EINVAL;
+  }
+  if (m->nbc > 0 && (!m->bc_nodes || !m->bc_conn || !m->bc_id || !m->bc_type || !m->bc_val)) {
+    set_error("pzg_mesh_set: boundary element arrays missing");
+    return PZG_EINVAL;
+  }
+  c.dim = m->dim;
+  c.material = m->material;
+  c.nstate = m->nstate;
+  c.porder = m->porder;
+  c.nnodes = m->nnodes;
+  c.nvol = m->nvol;
+  c.nbc = m->nbc;
+  c.nconn = m->nconn;
+  c.neq = m->neq;
+  int vt = m->dim == 2 ? PZG_QUAD : PZG_HEX, bt = m->dim == 2 ? PZG_LINE : PZG_QUAD;
+  c.vol_ncorner = etype_nc(vt);
+  c.vol_nsides = etype_ns(vt);
+  c.bc_ncorner = etype_nc(bt);
+  c.bc_nsides = etype_ns(bt);
+  c.xyz.assign(m->xyz, m->xyz + 3 * m->nnodes);
+  c.vol_nodes.assign(m->vol_nodes, m->vol_nodes + m->nvol * c.vol_ncorner);
+  c.vol_conn.assign(m->vol_conn, m->vol_conn + m->nvol * c.vol_nsides);
+  c.vol_coef.clear();
+  if (m->vol_coef) c.vol_coef.assign(m->vol_coef, m->vol_coef + m->nvol);
+  c.bc_nodes.clear();
+  c.bc_conn.clear();
+  c.bc_id.clear();
+  if (m->nbc > 0) {
+    c.bc_nodes.assign(m->bc_nodes, m->bc_nodes + m->nbc * c.bc_ncorner);
+    c.bc_conn.assign(m->bc_conn, m->bc_conn + m->nbc * c.bc_nsides);
+    c.bc_id.assign(m->bc_id, m->bc_id + m->nbc);
+  }
+  c.nbcid = m->nbcid;
+  c.bc_type.assign(m->bc_type, m->bc_type + std::max(0, m->nbcid));
+  c.bc_val.assign(m->bc_val, m->bc_val + 3 * std::max(0, m->nbcid));
+  c.blk_pos.assign(m->blk_pos, m->blk_pos + m->nconn);
+  c.blk_size.assign(m->blk_size, m->blk_size + m->nconn);
+  c.scale = m->scale;
+  c.E = m->E;
+  c.nu = m->nu;
+  c.big = m->big_number;
+  c.rule_vol = m->rule_order_vol;
+  c.rule_bc = m->rule_order_bc;
+  c.force_kind = m->force_kind;
+  for (int k = 0; k < 3; k++) c.force_const[k] = m->force_const[k];
+  c.nbumps = m->force_kind == PZG_FORCE_BUMPS ? m->nbumps : 0;
+  c.bumps.clear();
+  if (c.nbumps > 0) c.bumps.assign(m->bumps, m->bumps + 6 * c.nbumps);
+  c.bump_width = m->bump_width;
+
+  // Uniform active block size: every connect carries 0 or nstate equations
+  // (true for H1 p<=2), block positions must be consecutive in seq order.
+  c.bs = c.nstate;
+  c.aidx.assign(c.nconn, -1);
+  int64_t nb = 0, pos = 0;
+  for (int64_t s = 0; s < c.nconn; s++) {
+    int sz = c.blk_size[s];
+    if (sz != 0 && sz != c.bs) {
+      set_error("pzg_mesh_set: block sizes other than 0 or nstate are not supported");
+      return PZG_EUNSUP;
+    }
+    if (c.blk_pos[s] != pos) {
+      set_error("pzg_mesh_set: Block() positions are not cumulative in sequence order");
+      return PZG_EINVAL;
+    }
+    if (sz) c.aidx[s] = nb++;
+    pos += sz;
+  }
+  if (pos != c.neq) {
+    set_error("pzg_mesh_set: neq does not match the block sizes");
+    return PZG_EINVAL;
+  }
+  c.nb = nb;
+  for (int64_t v : c.vol_conn)
+    if (v < 0 || v >= c.nconn) { set_error("pzg_mesh_set: connect index out of range"); return PZG_EINVAL; }
+  for (int64_t v : c.bc_conn)
+    if (v < 0 || v >= c.nconn) { set_error("pzg_mesh_set: connect index out of range"); return PZG_EINVAL; }
+  for (int64_t v : c.vol_nodes)
+    if (v < 0 || v >= c.nnodes) { set_error("pzg_mesh_set: node index out of range"); return PZG_EINVAL; }
+  for (int64_t v : c.bc_nodes)
+    if (v < 0 || v >= c.nnodes) { set_error("pzg_mesh_set: node index out of range"); return PZG_EINVAL; }
+  for (int32_t v : c.bc_id)
+    if (v < 0 || v >= c.nbcid) { set_error("pzg_mesh_set: bc id out of range"); return PZG_EINVAL; }
+
+  PZG_TRY(build_tables(vt, c.porder, c.rule_vol, c.tvol));
+  PZG_TRY(build_tables(bt, c.porder, c.rule_bc, c.tbc));
+
+  hipStream_t s = c.stream;
+  PZG_TRY(dev_upload(c.d_xyz, c.xyz.data(), c.xyz.size(), s));
+  PZG_TRY(dev_upload(c.d_vnodes, c.vol_nodes.data(), c.vol_nodes.size(), s));
+  PZG_TRY(dev_upload(c.d_vconn, c.vol_conn.data(), c.vol_conn.size(), s));
+  if (!c.vol_coef.empty()) PZG_TRY(dev_upload(c.d_vcoef, c.vol_coef.data(), c.vol_coef.size(), s));
+  else dev_free(c.d_vcoef);
+  if (c.nbc > 0) {
+    PZG_TRY(dev_upload(c.d_bnodes, c.bc_nodes.data(), c.bc_nodes.size(), s));
+    PZG_TRY(dev_upload(c.d_bconn, c.bc_conn.data(), c.bc_conn.size(), s));
+    PZG_TRY(dev_upload(c.d_bcid, c.bc_id.data(), c.bc_id.size(), s));
+  }
+  if (c.nbcid > 0) {
+    PZG_TRY(dev_upload(c.d_bctype, c.bc_type.data(), c.bc_type.size(), s));
+    PZG_TRY(dev_upload(c.d_bcval, c.bc_val.data(), c.bc_val.size(), s));
+  }
+  if (c.nbumps > 0) PZG_TRY(dev_upload(c.d_bumps, c.bumps.data(), c.bumps.size(), s));
+  PZG_TRY(dev_upload(c.d_aidx, c.aidx.data(), c.aidx.size(), s));
+  PZG_TRY(dev_upload(c.d_tvol, reinterpret_cast<const char *>(&c.tvol), sizeof(ElemTables), s));
+  PZG_TRY(dev_upload(c.d_tbc, reinterpret_cast<const char *>(&c.tbc), sizeof(ElemTables), s));
+  PZG_HIP(hipStreamSynchronize(s));
+  c.has_mesh = true;
+  c.has_struct = false;
+  c.assembled = false;
+  c.external_csr = false;
+  c.has_solution = false;
+  return PZG_OK;
+}
+
+// Node graph of the active blocks (TPZRenumbering::ConvertGraph semantics:
+// union of the connects of all elements that touch a block, sorted, self
+// excluded), split into the upper part (SetupMatrixData's column order) and
+// the full symmetric part.
+static int struct_create(Ctx &c) {
+  if (!c.has_mesh) {
+    set_error("pzg_struct_create: call pzg_mesh_set first");
+    return PZG_ESTATE;
+  }
+  const int64_t nb = c.nb, bs = c.bs;
+  const int nsv = c.vol_nsides, nsb = c.bc_nsides;
+  const int64_t nel = c.nvol + c.nbc;
+  auto elem_conn = [&](int64_t e, int s) -> int64_t {
+    return e < c.nvol ? c.aidx[c.vol_conn[e * nsv + s]] : c.aidx[c.bc_conn[(e - c.nvol) * nsb + s]];
+  };
+  auto elem_ns = [&](int64_t e) { return e < c.nvol ? nsv : nsb; };
+  // block -> element incidence
+  std::vector<int64_t> bptr(nb + 1, 0);
+  for (int64_t e = 0; e < nel; e++)
+    for (int s = 0; s < elem_ns(e); s++) {
+      int64_t a = elem_conn(e, s);
+      if (a >= 0) bptr[a + 1]++;
+    }
+  for (int64_t a = 0; a < nb; a++) bptr[a + 1] += bptr[a];
+  std::vector<int64_t> belem(bptr[nb]);
+  {
+    std::vector<int64_t> fill(bptr.begin(), bptr.end() - 1);
+    for (int64_t e = 0; e < nel; e++)
+      for (int s = 0; s < elem_ns(e); s++) {
+        int64_t a = elem_conn(e, s);
+        if (a >= 0) belem[fill[a]++] = e;
+      }
+  }
+  auto neighbours = [&](int64_t a, std::vector<int64_t> &buf) {
+    buf.clear();
+    for (int64_t k = bptr[a]; k < bptr[a + 1]; k++) {
+      int64_t e = belem[k];
+      for (int s = 0; s < elem_ns(e); s++) {
+        int64_t b = elem_conn(e, s);
+        if (b >= 0 && b != a) buf.push_back(b);
+      }
+    }
+    std::sort(buf.begin(), buf.end());
+    buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+  };
+  std::vector<int32_t> nup(nb), nlow(nb);
+  {
+    int T = nthreads();
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; t++)
+      th.emplace_back([&, t]() {
+        std::vector<int64_t> buf;
+        for (int64_t a = nb * t / T; a < nb * (t + 1) / T; a++) {
+          neighbours(a, buf);
+          int64_t lo = std::lower_bound(buf.begin(), buf.end(), a) - buf.begin();
+          nlow[a] = (int32_t)lo;
+          nup[a] = (int32_t)(buf.size() - lo);
+        }
+      });
+    for (auto &x : th) x.join();
+  }
+  std::vector<int64_t> ubia(nb + 1, 0), fbia(nb + 1, 0);
+  for (int64_t a = 0; a < nb; a++) {
+    ubia[a + 1] = ubia[a] + nup[a];
+    fbia[a + 1] = fbia[a] + nlow[a] + 1 + nup[a];
+  }
+  c.nbu = ubia[nb];
+  c.nbf = fbia[nb];
+  std::vector<int32_t> ubja(std::max<int64_t>(c.nbu, 1)), fbja(std::max<int64_t>(c.nbf, 1));
+  {
+    int T = nthreads();
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; t++)
+      th.emplace_back([&, t]() {
+        std::vector<int64_t> buf;
+        for (int64_t a = nb * t / T; a < nb * (t + 1) / T; a++) {
+          neighbours(a, buf);
+          int64_t f = fbia[a], u = ubia[a];
+          for (int64_t k = 0; k < nlow[a]; k++) fbja[f++] = (int32_t)buf[k];
+          fbja[f++] = (int32_t)a;
+          for (int64_t k = nlow[a]; k < (int64_t)buf.size(); k++) {
+            fbja[f++] = (int32_t)buf[k];
+            ubja[u++] = (int32_t)buf[k];
+          }
+        }
+      });
+    for (auto &x : th) x.join();
+  }
+  std::vector<int32_t> rank_fwd(std::max<int64_t>(c.nbu, 1)), rank_bwd(std::max<int64_t>(c.nbu, 1));
+  parallel_for(nb, [&](int64_t a) {
+    for (int64_t k = ubia[a]; k < ubia[a + 1]; k++) {
+      int64_t b = ubja[k];
+      rank_fwd[k] = (int32_t)(nlow[a] + 1 + (k - ubia[a]));
+      const int32_t *row = fbja.data() + fbia[b];
+      rank_bwd[k] = (int32_t)(std::lower_bound(row, row + nlow[b], (int32_t)a) - row);
+    }
+  });
+  // equation-level row pointers: reference upper CSR and full CSR
+  const int64_t N = c.neq;
+  std::vector<int64_t> ia(N + 1), fia(N + 1);
+  int64_t ub = 0, fb = 0;
+  for (int64_t a = 0; a < nb; a++) {
+    for (int64_t k = 0; k < bs; k++) {
+      ia[a * bs + k] = ub;
+      ub += (bs - k) + bs * (int64_t)nup[a];
+      fia[a * bs + k] = fb;
+      fb += bs * (int64_t)(nlow[a] + 1 + nup[a]);
+    }
+  }
+  ia[N] = ub;
+  fia[N] = fb;
+  c.nnz_u = ub;
+  c.nnz_full = fb;
+  if (c.nnz_full >= (int64_t)1 << 31 && false) {
+    set_error("full CSR exceeds int32 indexing");
+    return PZG_EUNSUP;
+  }
+  hipStream_t s = c.stream;
+  PZG_TRY(dev_upload(c.d_ubia, ubia.data(), ubia.size(), s));
+  PZG_TRY(dev_upload(c.d_ubja, ubja.data(), ubja.size(), s));
+  PZG_TRY(dev_upload(c.d_fbia, fbia.data(), fbia.size(), s));
+  PZG_TRY(dev_upload(c.d_fbja, fbja.data(), fbja.size(), s));
+  PZG_TRY(dev_upload(c.d_rank_fwd, rank_fwd.data(), rank_fwd.size(), s));
+  PZG_TRY(dev_upload(c.d_rank_bwd, rank_bwd.data(), rank_bwd.size(), s));
+  PZG_TRY(dev_upload(c.d_ia, ia.data(), ia.size(), s));
+  PZG_TRY(dev_upload(c.d_fia, fia.data(), fia.size(), s));
+  PZG_TRY(dev_alloc(c.d_ja, sizeof(int64_t) * std::max<int64_t>(c.nnz_u, 1)));
+  PZG_TRY(dev_alloc(c.d_a, sizeof(double) * std::max<int64_t>(c.nnz_u, 1)));
+  PZG_TRY(dev_alloc(c.d_rhs, sizeof(double) * std::max<int64_t>(N, 1)));
+  PZG_TRY(dev_alloc(c.d_diag, sizeof(double) * std::max<int64_t>(N, 1)));
+  PZG_TRY(launch_expand_upper_ja(c));
+  // colourings and element -> upper-block maps
+  std::vector<int64_t> vptr, bptr2;
+  std::vector<int32_t> vlist, blist;
+  PZG_TRY(color_elements(c.nvol, nsv, c.vol_conn, c.aidx, nb, vptr, vlist));
+  c.vcolor_ptr = vptr;
+  PZG_TRY(dev_upload(c.d_vcolor_list, vlist.data(), vlist.size(), s));
+  if (c.nbc > 0) {
+    PZG_TRY(color_elements(c.nbc, nsb, c.bc_conn, c.aidx, nb, bptr2, blist));
+    c.bcolor_ptr = bptr2;
+    PZG_TRY(dev_upload(c.d_bcolor_list, blist.data(), blist.size(), s));
+  } else {
+    c.bcolor_ptr.assign(1, 0);
+  }
+  PZG_TRY(launch_elem_maps(c));
+  PZG_HIP(hipStreamSynchronize(s));
+  c.has_struct = true;
+  c.external_csr = false;
+  c.assembled = false;
+  c.fja_valid = false;
+  c.full_valid = c.bsr_valid = false;
+  c.has_solution = false;
+  return PZG_OK;
+}
+
+}  // namespace pzg
+
+using namespace pzg;
+
+struct pzg_ctx {
+  Ctx c;
+};
+
+extern "C" {
+
+const char *pzg_last_error(void) { return g_err.c_str(); }
+
+int pzg_create(int device, pzg_ctx **out) {
+  if (!out) {
+    set_error("pzg_create: null output pointer");
+    return PZG_EINVAL;
+  }
+  *out = nullptr;
+  int n = 0;
+  PZG_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) {
+    set_error("pzg_create: no HIP device " + std::to_string(device));
+    return PZG_EINVAL;
+  }
+  PZG_HIP(hipSetDevice(device));
+  pzg_ctx *p = new pzg_ctx();
+  p->c.device = device;
+  hipError_t e = hipStreamCreateWithFlags(&p->c.stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&p->c.ev0);
+  if (e == hipSuccess) e = hipEventCreate(&p->c.ev1);
+  if (e != hipSuccess) {
+    set_error(std::string("pzg_create: ") + hipGetErrorString(e));
+    delete p;
+    return PZG_EHIP;
+  }
+  *out = p;
+  return PZG_OK;
+}
+
+void pzg_destroy(pzg_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->c.device);
+  (void)hipStreamSynchronize(ctx->c.stream);
+  free_all(ctx->c);
+  if (ctx->c.ev0) (void)hipEventDestroy(ctx->c.ev0);
+  if (ctx->c.ev1) (void)hipEventDestroy(ctx->c.ev1);
+  if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
+  delete ctx;
+}
+
+#define CTX_CHECK()                                      \
+  do {                                                   \
+    if (!ctx) {                                          \
+      set_error("null context");                         \
+      return PZG_EINVAL;                                 \
+    }                                                    \
+    PZG_HIP(hipSetDevice(ctx->c.device));                \
+  } while (0)
+
+int pzg_mesh_set(pzg_ctx *ctx, const pzg_mesh_desc *mesh) {
+  CTX_CHECK();
+  return mesh_set(ctx->c, mesh);
+}
+
+int pzg_struct_create(pzg_ctx *ctx, int64_t *neq, int64_t *nnz) {
+  CTX_CHECK();
+  PZG_TRY(struct_create(ctx->c));
+  if (neq) *neq = ctx->c.neq;
+  if (nnz) *nnz = ctx->c.nnz_u;
+  return PZG_OK;
+}
+
+int pzg_struct_get_pattern(pzg_ctx *ctx, int64_t *ia, int64_t *ja) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (!c.has_struct) {
+    set_error("pzg_struct_get_pattern: no pattern");
+    return PZG_ESTATE;
+  }
+  if (ia) PZG_HIP(hipMemcpyAsync(ia, c.d_ia.p, sizeof(int64_t) * (c.neq + 1), hipMemcpyDeviceToHost, c.stream));
+  if (ja && c.nnz_u) PZG_HIP(hipMemcpyAsync(ja, c.d_ja.p, sizeof(int64_t) * c.nnz_u, hipMemcpyDeviceToHost, c.stream));
+  PZG_HIP(hipStreamSynchronize(c.stream));
+  return PZG_OK;
+}
+
+int pzg_assemble(pzg_ctx *ctx) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (!c.has_struct || c.external_csr || !c.has_mesh) {
+    set_error("pzg_assemble: call pzg_mesh_set and pzg_struct_create first");
+    return PZG_ESTATE;
+  }
+  PZG_TRY(launch_assemble(c));
+  PZG_TRY(launch_diag(c));
+  c.assembled = true;
+  c.full_valid = c.bsr_valid = false;
+  c.has_solution = false;
+  return PZG_OK;
+}
+
+int pzg_get_system(pzg_ctx *ctx, double *a, double *rhs) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (!c.assembled) {
+    set_error("pzg_get_system: nothing assembled");
+    return PZG_ESTATE;
+  }
+  if (a && c.nnz_u) PZG_HIP(hipMemcpyAsync(a, c.d_a.p, sizeof(double) * c.nnz_u, hipMemcpyDeviceToHost, c.stream));
+  if (rhs) PZG_HIP(hipMemcpyAsync(rhs, c.d_rhs.p, sizeof(double) * c.neq, hipMemcpyDeviceToHost, c.stream));
+  PZG_HIP(hipStreamSynchronize(c.stream));
+  return PZG_OK;
+}
+
+int pzg_csr_set(pzg_ctx *ctx, int64_t neq, const int64_t *ia, const int64_t *ja, const double *a,
+                const double *rhs) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (neq <= 0 || !ia || !ja || !a || ia[0] != 0) {
+    set_error("pzg_csr_set: invalid CSR");
+    return PZG_EINVAL;
+  }
+  // validate: rows sorted, upper (col >= row), diagonal first
+  for (int64_t r = 0; r < neq; r++) {
+    if (ia[r + 1] <= ia[r] || ja[ia[r]] != r) {
+      set_error("pzg_csr_set: row " + std::to_string(r) + " has no leading diagonal");
+      return PZG_EINVAL;
+    }
+    for (int64_t k = ia[r] + 1; k < ia[r + 1]; k++)
+      if (ja[k] <= ja[k - 1] || ja[k] >= neq) {
+        set_error("pzg_csr_set: row " + std::to_string(r) + " not sorted/upper");
+        return PZG_EINVAL;
+      }
+  }
+  // A caller matrix is handled as a scalar problem (block size 1).
+  c.has_mesh = false;
+  c.neq = neq;
+  c.bs = 1;
+  c.nb = neq;
+  c.nnz_u = ia[neq];
+  std::vector<int32_t> nlow(neq, 0);
+  std::vector<int64_t> rowof(c.nnz_u);
+  for (int64_t r = 0; r < neq; r++)
+    for (int64_t k = ia[r]; k < ia[r + 1]; k++) {
+      rowof[k] = r;
+      if (ja[k] != r) nlow[ja[k]]++;
+    }
+  std::vector<int64_t> ubia(neq + 1), fbia(neq + 1);
+  ubia[0] = fbia[0] = 0;
+  for (int64_t r = 0; r < neq; r++) {
+    int64_t nup = ia[r + 1] - ia[r] - 1;
+    ubia[r + 1] = ubia[r] + nup;
+    fbia[r + 1] = fbia[r] + nlow[r] + 1 + nup;
+  }
+  c.nbu = ubia[neq];
+  c.nbf = fbia[neq];
+  c.nnz_full = c.nbf;
+  std::vector<int32_t> ubja(std::max<int64_t>(c.nbu, 1)), fbja(std::max<int64_t>(c.nbf, 1));
+  std::vector<int32_t> rank_fwd(std::max<int64_t>(c.nbu, 1)), rank_bwd(std::max<int64_t>(c.nbu, 1));
+  std::vector<int64_t> lowfill(neq, 0);
+  for (int64_t r = 0; r < neq; r++) {
+    int64_t u = ubia[r];
+    for (int64_t k = ia[r] + 1; k < ia[r + 1]; k++) {
+      int64_t col = ja[k];
+      ubja[u] = (int32_t)col;
+      rank_fwd[u] = (int32_t)(nlow[r] + 1 + (u - ubia[r]));
+      rank_bwd[u] = (int32_t)lowfill[col];
+      fbja[fbia[col] + lowfill[col]] = (int32_t)r;
+      lowfill[col]++;
+      u++;
+    }
+    fbja[fbia[r] + nlow[r]] = (int32_t)r;
+    for (int64_t k = ia[r] + 1; k < ia[r + 1]; k++) fbja[fbia[r] + nlow[r] + (k - ia[r])] = (int32_t)ja[k];
+  }
+  hipStream_t s = c.stream;
+  PZG_TRY(dev_upload(c.d_ubia, ubia.data(), ubia.size(), s));
+  PZG_TRY(dev_upload(c.d_ubja, ubja.data(), ubja.size(), s));
+  PZG_TRY(dev_upload(c.d_fbia, fbia.data(), fbia.size(), s));
+  PZG_TRY(dev_upload(c.d_fbja, fbja.data(), fbja.size(), s));
+  PZG_TRY(dev_upload(c.d_rank_fwd, rank_fwd.data(), rank_fwd.size(), s));
+  PZG_TRY(dev_upload(c.d_rank_bwd, rank_bwd.data(), rank_bwd.size(), s));
+  PZG_TRY(dev_upload(c.d_ia, ia, neq + 1, s));
+  PZG_TRY(dev_upload(c.d_ja, ja, c.nnz_u, s));
+  PZG_TRY(dev_upload(c.d_a, a, c.nnz_u, s));
+  PZG_TRY(dev_upload(c.d_fia, fbia.data(), fbia.size(), s));
+  PZG_TRY(dev_alloc(c.d_rhs, sizeof(double) * neq));
+  if (rhs) PZG_HIP(hipMemcpyAsync(c.d_rhs.p, rhs, sizeof(double) * neq, hipMemcpyHostToDevice, s));
+  else PZG_HIP(hipMemsetAsync(c.d_rhs.p, 0, sizeof(double) * neq, s));
+  PZG_TRY(dev_alloc(c.d_diag, sizeof(double) * neq));
+  PZG_TRY(launch_diag(c));
+  PZG_HIP(hipStreamSynchronize(s));
+  c.has_struct = true;
+  c.external_csr = true;
+  c.assembled = true;
+  c.fja_valid = false;
+  c.full_valid = c.bsr_valid = false;
+  c.has_solution = false;
+  return PZG_OK;
+}
+
+int pzg_add_kel_batch(pzg_ctx *ctx, int64_t nel, int32_t ne, const int64_t *dest, const double *ek) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (!c.has_struct) {
+    set_error("pzg_add_kel_batch: no pattern (pzg_struct_create or pzg_csr_set first)");
+    return PZG_ESTATE;
+  }
+  if (nel < 0 || ne <= 0 || ne > 1024 || (nel > 0 && (!dest || !ek))) {
+    set_error("pzg_add_kel_batch: invalid arguments");
+    return PZG_EINVAL;
+  }
+  if (nel == 0) return PZG_OK;
+  for (int64_t i = 0; i < nel * ne; i++)
+    if (dest[i] < 0 || dest[i] >= c.neq) {
+      set_error("pzg_add_kel_batch: destination equation out of range");
+      return PZG_EINVAL;
+    }
+  // colour by equations so that one launch never writes a row twice
+  std::vector<uint64_t> used(c.neq, 0);
+  std::vector<int> col(nel);
+  int ncol = 0;
+  for (int64_t e = 0; e < nel; e++) {
+    uint64_t m = 0;
+    for (int i = 0; i < ne; i++) m |= used[dest[e * ne + i]];
+    if (m == ~0ULL) {
+      set_error("pzg_add_kel_batch: more than 64 colours");
+      return PZG_EUNSUP;
+    }
+    int k = __builtin_ctzll(~m);
+    col[e] = k;
+    ncol = std::max(ncol, k + 1);
+    for (int i = 0; i < ne; i++) used[dest[e * ne + i]] |= 1ULL << k;
+  }
+  std::vector<int64_t> ptr(ncol + 1, 0);
+  for (int64_t e = 0; e < nel; e++) ptr[col[e] + 1]++;
+  for (int k = 0; k < ncol; k++) ptr[k + 1] += ptr[k];
+  std::vector<int32_t> list(nel);
+  std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
+  for (int64_t e = 0; e < nel; e++) list[fill[col[e]]++] = (int32_t)e;
+  DevBuf d_dest, d_ek, d_list;
+  int rc = dev_upload(d_dest, dest, nel * ne, c.stream);
+  if (rc == PZG_OK) rc = dev_upload(d_ek, ek, nel * ne * (int64_t)ne, c.stream);
+  if (rc == PZG_OK) rc = dev_upload(d_list, list.data(), list.size(), c.stream);
+  for (int k = 0; k < ncol && rc == PZG_OK; k++)
+    rc = launch_addkel_batch(c, nel, ne, d_dest.as<int64_t>(), d_ek.as<double>(),
+                             d_list.as<int32_t>() + ptr[k], ptr[k + 1] - ptr[k]);
+  if (rc == PZG_OK) rc = launch_diag(c);
+  if (rc == PZG_OK && hipStreamSynchronize(c.stream) != hipSuccess) rc = PZG_EHIP;
+  dev_free(d_dest);
+  dev_free(d_ek);
+  dev_free(d_list);
+  c.assembled = true;
+  c.full_valid = c.bsr_valid = false;
+  return rc;
+}
+
+int pzg_multadd(pzg_ctx *ctx, const double *x, const double *y, double *z, double alpha, double beta) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (!c.assembled) {
+    set_error("pzg_multadd: no matrix");
+    return PZG_ESTATE;
+  }
+  if (!x || !z || (beta != 0. && !y)) {
+    set_error("pzg_multadd: null vector");
+    return PZG_EINVAL;
+  }
+  int64_t n = c.neq;
+  PZG_TRY(dev_alloc(c.d_x, n * sizeof(double)));
+  PZG_TRY(dev_alloc(c.d_w, n * sizeof(double)));
+  PZG_TRY(dev_alloc(c.d_z, n * sizeof(double)));
+  PZG_HIP(hipMemcpyAsync(c.d_x.p, x, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  if (beta != 0.) PZG_HIP(hipMemcpyAsync(c.d_w.p, y, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  PZG_TRY(launch_multadd_upper(c, c.d_x.as<double>(), beta != 0. ? c.d_w.as<double>() : nullptr,
+                               c.d_z.as<double>(), alpha, beta));
+  PZG_HIP(hipMemcpyAsync(z, c.d_z.p, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  PZG_HIP(hipStreamSynchronize(c.stream));
+  return PZG_OK;
+}
+
+int pzg_solve(pzg_ctx *ctx, const pzg_solve_params *params, const double *rhs, double *u,
+              pzg_solve_info *info) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (!c.assembled) {
+    set_error("pzg_solve: no assembled system");
+    return PZG_ESTATE;
+  }
+  if (!params || params->tol < 0 || params->max_iter < 0) {
+    set_error("pzg_solve: invalid parameters");
+    return PZG_EINVAL;
+  }
+  pzg_solve_info tmp;
+  pzg_solve_info &inf = info ? *info : tmp;
+  std::memset(&inf, 0, sizeof(inf));
+  int64_t n = c.neq;
+  PZG_TRY(dev_alloc(c.d_f, n * sizeof(double)));
+  if (rhs) PZG_HIP(hipMemcpyAsync(c.d_f.p, rhs, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  else PZG_HIP(hipMemcpyAsync(c.d_f.p, c.d_rhs.p, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  int rc;
+  if (params->solver == PZG_SOLVER_PCG64) rc = solve_pcg64(c, *params, inf);
+  else if (params->solver == PZG_SOLVER_MIXED) rc = solve_mixed(c, *params, inf);
+  else {
+    set_error("pzg_solve: unknown solver");
+    return PZG_EINVAL;
+  }
+  if (rc != PZG_OK && rc != PZG_ENOCONV) return rc;
+  c.has_solution = true;
+  if (u) {
+    PZG_HIP(hipMemcpyAsync(u, c.d_u.p, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    PZG_HIP(hipStreamSynchronize(c.stream));
+  }
+  if (rc == PZG_ENOCONV) set_error("pzg_solve: no convergence within max_iter");
+  return rc;
+}
+
+int pzg_true_residual(pzg_ctx *ctx, double *rel) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (!c.has_solution || !rel) {
+    set_error("pzg_true_residual: no solution on the device");
+    return PZG_ESTATE;
+  }
+  return true_residual(c, c.d_u.as<double>(), rel);
+}
+
+int pzg_get_sizes(pzg_ctx *ctx, pzg_sizes *s) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (!s) return PZG_EINVAL;
+  s->neq = c.neq;
+  s->nnz_upper = c.nnz_u;
+  s->nnz_full = c.nnz_full;
+  s->nblocks = c.nb;
+  s->nnz_blocks = c.nbf;
+  s->nvol = c.nvol;
+  s->ncolors = c.vcolor_ptr.empty() ? 0 : (int64_t)c.vcolor_ptr.size() - 1;
+  s->bs = c.bs;
+  return PZG_OK;
+}
+
+int pzg_sync(pzg_ctx *ctx) {
+  CTX_CHECK();
+  PZG_HIP(hipStreamSynchronize(ctx->c.stream));
+  return PZG_OK;
+}
+
+int pzg_time_kernel(pzg_ctx *ctx, int32_t which, int32_t nrep, double *avg_ms, double *bytes) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (!c.assembled || nrep <= 0 || !avg_ms) {
+    set_error("pzg_time_kernel: needs an assembled system");
+    return PZG_ESTATE;
+  }
+  int64_t n = c.neq;
+  double b = 0;
+  if (which == 0) {
+    if (!c.full_valid) PZG_TRY(launch_expand_full(c));
+    PZG_TRY(dev_alloc(c.d_x, n * sizeof(double)));
+    PZG_TRY(dev_alloc(c.d_w, n * sizeof(double)));
+    PZG_HIP(hipMemsetAsync(c.d_x.p, 0, n * sizeof(double), c.stream));
+    PZG_TRY(launch_spmv64(c, c.d_x.as<double>(), c.d_w.as<double>()));
+    PZG_HIP(hipEventRecord(c.ev0, c.stream));
+    for (int i = 0; i < nrep; i++) PZG_TRY(launch_spmv64(c, c.d_x.as<double>(), c.d_w.as<double>()));
+    PZG_HIP(hipEventRecord(c.ev1, c.stream));
+    b = (double)c.nnz_full * 12.0 + (double)(n + 1) * 8.0 + 2.0 * n * 8.0;
+  } else if (which == 1) {
+    if (!c.bsr_valid) PZG_TRY(launch_build_bsr(c));
+    PZG_TRY(dev_alloc(c.d_xs, n * sizeof(float)));
+    PZG_TRY(dev_alloc(c.d_qs, n * sizeof(float)));
+    PZG_HIP(hipMemsetAsync(c.d_xs.p, 0, n * sizeof(float), c.stream));
+    PZG_TRY(launch_spmv_bsr(c, c.d_xs.as<float>(), c.d_qs.as<float>()));
+    PZG_HIP(hipEventRecord(c.ev0, c.stream));
+    for (int i = 0; i < nrep; i++) PZG_TRY(launch_spmv_bsr(c, c.d_xs.as<float>(), c.d_qs.as<float>()));
+    PZG_HIP(hipEventRecord(c.ev1, c.stream));
+    b = (double)c.nbf * (c.bs * c.bs * 4.0 + 4.0) + (double)(c.nb + 1) * 8.0 + 2.0 * n * 4.0;
+  } else if (which == 2) {
+    if (c.external_csr) {
+      set_error("pzg_time_kernel: no mesh to assemble");
+      return PZG_ESTATE;
+    }
+    PZG_HIP(hipEventRecord(c.ev0, c.stream));
+    for (int i = 0; i < nrep; i++) PZG_TRY(launch_assemble(c));
+    PZG_HIP(hipEventRecord(c.ev1, c.stream));
+    b = 0;
+  } else {
+    set_error("pzg_time_kernel: unknown kernel");
+    return PZG_EINVAL;
+  }
+  PZG_HIP(hipEventSynchronize(c.ev1));
+  float ms = 0;
+  PZG_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+  *avg_ms = ms / nrep;
+  if (bytes) *bytes = b;
+  return PZG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,147 @@
+// pzg_internal.h -- shared declarations of the HIP linear FEM path.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/pzg.h"
+
+namespace pzg {
+
+void set_error(const std::string &msg);
+
+#define PZG_HIP(call)                                                                 \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess) {                                                           \
+      ::pzg::set_error(std::string(#call) + ": " + hipGetErrorString(e_));            \
+      return PZG_EHIP;                                                                \
+    }                                                                                 \
+  } while (0)
+
+#define PZG_TRY(call)              \
+  do {                             \
+    int rc_ = (call);              \
+    if (rc_ != PZG_OK) return rc_; \
+  } while (0)
+
+// Device buffer owned by the context.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+int dev_alloc(DevBuf &b, size_t bytes);
+void dev_free(DevBuf &b);
+template <class T> int dev_upload(DevBuf &b, const T *h, size_t n, hipStream_t s) {
+  PZG_TRY(dev_alloc(b, n * sizeof(T)));
+  if (n) PZG_HIP(hipMemcpyAsync(b.p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+  return PZG_OK;
+}
+
+// Maximum sizes of the element tables (hexahedron p=2).
+constexpr int MAXQ = 27;      // quadrature points per element
+constexpr int MAXSH = 27;     // shape functions per element
+constexpr int MAXSIDES = 27;  // sides per element
+
+// Reference-element tables at the quadrature points (host-computed once).
+struct ElemTables {
+  int etype = 0, dim = 0, ncorner = 0, nsides = 0, nsh = 0, nq = 0;
+  int shape_side[MAXSH];       // side that owns shape i (p <= 2: one shape per side)
+  double w[MAXQ];              // quadrature weights
+  double phi[MAXQ][MAXSH];     // shape values
+  double dphi[MAXQ][3][MAXSH]; // reference derivatives
+  double cphi[MAXQ][8];        // corner (geometric) shape values
+  double cdphi[MAXQ][3][8];    // corner reference derivatives
+};
+
+// Device-side scalar state of an iterative solve.
+struct SolveScalars {
+  double rho, rho_old, pq, alpha, beta, rr, normb, resid, tol, pr, zq;
+  int64_t iter, maxit;
+  int32_t done, pad;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  // ---- mesh (host copies of the description) --------------------------
+  bool has_mesh = false;
+  int dim = 0, material = 0, nstate = 1, porder = 1, bs = 1;
+  int64_t nnodes = 0, nvol = 0, nbc = 0, nconn = 0, neq = 0;
+  std::vector<double> xyz, vol_coef, bc_val, bumps;
+  std::vector<int64_t> vol_nodes, vol_conn, bc_nodes, bc_conn, blk_pos;
+  std::vector<int32_t> blk_size, bc_id, bc_type;
+  double scale = 1, E = 1, nu = 0.3, big = 1e12, bump_width = 0.1, force_const[3] = {0, 0, 0};
+  int force_kind = 0, nbumps = 0, rule_vol = 0, rule_bc = 0, nbcid = 0;
+  ElemTables tvol, tbc;
+  int vol_ncorner = 0, vol_nsides = 0, bc_ncorner = 0, bc_nsides = 0;
+
+  // connect -> active block index (-1 for size-0 connects)
+  std::vector<int64_t> aidx;
+  int64_t nb = 0;  // active blocks
+
+  // ---- pattern (device) -------------------------------------------------
+  bool has_struct = false, external_csr = false;
+  int64_t nnz_u = 0, nnz_full = 0, nbu = 0, nbf = 0;
+  DevBuf d_ia, d_ja;          // reference upper CSR: int64 IA[neq+1], JA[nnz_u]
+  DevBuf d_a, d_rhs, d_diag;  // values (fp64), rhs, diagonal (ComputeDiagonal)
+  DevBuf d_ubia, d_ubja;      // upper block graph: int64 [nb+1], int32 [nbu]
+  DevBuf d_fbia, d_fbja;      // full block graph:  int64 [nb+1], int32 [nbf]
+  DevBuf d_rank_fwd, d_rank_bwd; // per upper block entry: rank of (a,b) in full row a, (b,a) in row b
+  DevBuf d_fia, d_fja;        // full scalar CSR: int64 [neq+1], int32 [nnz_full]
+  DevBuf d_fa;                // full scalar CSR values fp64
+  DevBuf d_bsr;               // full BSR fp32 values, scaled D^-1/2 K D^-1/2
+  DevBuf d_dinvsqrt;          // fp64 D^-1/2
+  DevBuf d_bjinv;             // fp32 inverse diagonal blocks of the scaled matrix
+  bool full_valid = false, bsr_valid = false;
+  bool fja_valid = false;     // full CSR column indices match the current pattern
+
+  // ---- assembly (device) -------------------------------------------------
+  DevBuf d_xyz, d_vnodes, d_vconn, d_vcoef, d_bnodes, d_bconn, d_bcid, d_bctype, d_bcval;
+  DevBuf d_aidx, d_bumps, d_vmap, d_bmap, d_tvol, d_tbc;
+  DevBuf d_vcolor_list, d_bcolor_list;
+  std::vector<int64_t> vcolor_ptr, bcolor_ptr;  // color c: list[ptr[c] .. ptr[c+1])
+  bool assembled = false;
+
+  // ---- solver workspace -----------------------------------------------------
+  DevBuf d_x, d_r, d_z, d_p, d_p2, d_q, d_u, d_f, d_w;        // fp64
+  DevBuf d_xs, d_rs, d_zs, d_ps, d_ps2, d_qs, d_bs;          // fp32
+  DevBuf d_scal, d_partials, d_counter;
+  int reduce_blocks = 1024;
+  bool has_solution = false;
+
+  // learned preconditioner
+  DevBuf d_nn_w;            // weights (fp32)
+  bool nn_ready = false;
+  int nn_layers = 3, nn_hidden = 32;
+  DevBuf d_nn_h0, d_nn_h1, d_nn_agg;  // node features workspace
+};
+
+// ---- element tables (pzg_shape.cpp) ----------------------------------------
+int build_tables(int etype, int p, int rule_order, ElemTables &t);
+int gauss_legendre(int order, double *loc, double *w);
+
+// ---- kernels launchers (pzg_assemble.hip / pzg_solve.hip) --------------------
+int launch_expand_upper_ja(Ctx &c);
+int launch_elem_maps(Ctx &c);
+int launch_assemble(Ctx &c);
+int launch_diag(Ctx &c);
+int launch_expand_full(Ctx &c);
+int launch_build_bsr(Ctx &c);
+int launch_spmv64(Ctx &c, const double *x, double *y);
+int launch_spmv_bsr(Ctx &c, const float *x, float *y);
+int launch_multadd_upper(Ctx &c, const double *x, const double *y, double *z, double alpha,
+                         double beta);
+int launch_addkel_batch(Ctx &c, int64_t nel, int ne, const int64_t *d_dest, const double *d_ek,
+                        const int32_t *d_list, int64_t nlist);
+int solve_pcg64(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info);
+int solve_mixed(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info);
+int true_residual(Ctx &c, const double *u, double *rel);
+
+}  // namespace pzg

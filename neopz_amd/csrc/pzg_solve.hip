@@ -1,0 +1,706 @@
+// pzg_solve.hip -- gfx950 kernels and drivers of the solve phase.
+//
+//  PCG64 (plain path, reference semantics).  CG<> of
+//  Solvers/LinearSolvers/cg.h:29-132 with the EJacobi TPZStepSolver
+//  (z = r / diag, Matrix/pzmatrix.cpp:642-667), x0 = 0, stop on the recursive
+//  ||r||/||b|| <= tol.  The reference's Jacobi step also runs a residual SpMV
+//  whose result it discards; it has no effect on the iterates and is not run.
+//  Matrix: full CSR, FP64 values, int32 columns (expanded once per assembly
+//  from the reference upper CSR).
+//
+//  MIXED.  FP32 flexible CG (Notay's FCG(1)) on S = D^-1/2 K D^-1/2 stored as
+//  FP32 block CSR, preconditioned by the inverse diagonal blocks, inside an
+//  FP64 iterative refinement u += D^-1/2 S^-1 D^-1/2 (f - K u) (the shape of
+//  IR<>, Solvers/LinearSolvers/ir.h:23-66); the stop test is the true FP64
+//  residual.
+//
+//  All scalars live on the device; reductions are two-level and
+//  deterministic (per-block partials, the last block sums them in a fixed
+//  order), so iterations need no host round trip.  The host only polls the
+//  'done' flag once per batch of iterations; a batch can be captured once in
+//  a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "pzg_internal.h"
+
+namespace pzg {
+
+constexpr int TPB = 256;
+constexpr int MAXBLK = 2048;  // partial-sum slots per reduction
+constexpr int NSLOT = 4;
+
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Sum of v over the block; result valid in thread 0.
+__device__ inline double block_sum(double v, double *sm) {
+  v = wave_sum(v);
+  int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) sm[w] = v;
+  __syncthreads();
+  double s = 0.;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) s += sm[k];
+  return s;
+}
+
+// Stores this block's partials; returns true in the (single) last block once
+// every block has stored, after which 'tot' holds the fixed-order totals.
+template <int NV>
+__device__ inline bool reduce_last(const double (&v)[NV], double *partials, unsigned *counter,
+                                   double (&tot)[NV]) {
+  __shared__ double sm[TPB / 64];
+  __shared__ bool last;
+  for (int k = 0; k < NV; k++) {
+    double s = block_sum(v[k], sm);
+    if (threadIdx.x == 0) partials[k * MAXBLK + blockIdx.x] = s;
+  }
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned prev = atomicAdd(counter, 1u);
+    last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+  for (int k = 0; k < NV; k++) {
+    double s = 0.;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
+      s += __builtin_nontemporal_load(&partials[k * MAXBLK + b]);
+    s = block_sum(s, sm);
+    if (threadIdx.x == 0) tot[k] = s;
+  }
+  if (threadIdx.x == 0) *counter = 0u;
+  return true;
+}
+
+static int grid_for(int64_t n, int per_block) {
+  int64_t g = (n + per_block - 1) / per_block;
+  if (g > MAXBLK) g = MAXBLK;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+// ----------------------------------------------------------------------------
+// FP64 full CSR SpMV; VL lanes per row.
+template <int VL>
+__device__ inline double row_dot64(int64_t r, const int64_t *__restrict__ ia,
+                                   const int32_t *__restrict__ ja, const double *__restrict__ a,
+                                   const double *__restrict__ x, int sub) {
+  double s = 0.;
+  for (int64_t k = ia[r] + sub; k < ia[r + 1]; k += VL) s += a[k] * x[ja[k]];
+  for (int o = VL / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, VL);
+  return s;
+}
+
+template <int VL>
+__global__ __launch_bounds__(TPB) void k_spmv64(int64_t n, const int64_t *__restrict__ ia,
+                                                const int32_t *__restrict__ ja,
+                                                const double *__restrict__ a,
+                                                const double *__restrict__ x,
+                                                double *__restrict__ y) {
+  const int sub = threadIdx.x % VL;
+  int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / VL;
+  const int64_t ng = (int64_t)gridDim.x * blockDim.x / VL;
+  for (int64_t r = g; r < n; r += ng) {
+    double s = row_dot64<VL>(r, ia, ja, a, x, sub);
+    if (sub == 0) y[r] = s;
+  }
+}
+
+// q = A p, then alpha = rho / (p.q)
+template <int VL>
+__global__ __launch_bounds__(TPB) void k_spmv64_pq(int64_t n, const int64_t *__restrict__ ia,
+                                                   const int32_t *__restrict__ ja,
+                                                   const double *__restrict__ a,
+                                                   const double *__restrict__ p,
+                                                   double *__restrict__ q, SolveScalars *sc,
+                                                   double *partials, unsigned *counter) {
+  if (sc->done) return;
+  const int sub = threadIdx.x % VL;
+  int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / VL;
+  const int64_t ng = (int64_t)gridDim.x * blockDim.x / VL;
+  double acc = 0.;
+  for (int64_t r = g; r < n; r += ng) {
+    double s = row_dot64<VL>(r, ia, ja, a, p, sub);
+    if (sub == 0) {
+      q[r] = s;
+      acc += s * p[r];
+    }
+  }
+  double v[1] = {acc}, tot[1];
+  if (reduce_last<1>(v, partials, counter, tot) && threadIdx.x == 0) {
+    sc->pq = tot[0];
+    sc->alpha = sc->rho / tot[0];
+  }
+}
+
+// p = z + beta p  (p = z on the first iteration)
+__global__ void k_p_update64(int64_t n, const double *__restrict__ z, double *__restrict__ p,
+                             const SolveScalars *sc) {
+  if (sc->done) return;
+  const double beta = sc->iter == 0 ? 0. : sc->beta;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  if (sc->iter == 0)
+    for (; i < n; i += st) p[i] = z[i];
+  else
+    for (; i < n; i += st) p[i] = beta * p[i] + z[i];
+}
+
+// x += alpha p; r -= alpha q; resid = |r|/|b|; z = r/diag; beta = (r.z)/rho
+__global__ __launch_bounds__(TPB) void k_cg_update64(int64_t n, double *__restrict__ x,
+                                                     double *__restrict__ r,
+                                                     double *__restrict__ z,
+                                                     const double *__restrict__ p,
+                                                     const double *__restrict__ q,
+                                                     const double *__restrict__ diag,
+                                                     SolveScalars *sc, double *hist,
+                                                     double *partials, unsigned *counter) {
+  if (sc->done) return;
+  const double alpha = sc->alpha;
+  double rr = 0., rz = 0.;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) {
+    x[i] += alpha * p[i];
+    double ri = r[i] + (-alpha) * q[i];
+    r[i] = ri;
+    double zi = ri / diag[i];
+    z[i] = zi;
+    rr += ri * ri;
+    rz += ri * zi;
+  }
+  double v[2] = {rr, rz}, tot[2];
+  if (reduce_last<2>(v, partials, counter, tot) && threadIdx.x == 0) {
+    int64_t it = sc->iter + 1;
+    sc->iter = it;
+    double resid = sqrt(tot[0]) / sc->normb;
+    sc->resid = resid;
+    if (hist) hist[it - 1] = resid;
+    if (resid <= sc->tol) sc->done = 1;
+    else if (it >= sc->maxit) sc->done = 2;
+    sc->beta = tot[1] / sc->rho;
+    sc->rho_old = sc->rho;
+    sc->rho = tot[1];
+  }
+}
+
+// x = 0; r = b; z = r/diag; rho = r.z; normb = |b|
+__global__ __launch_bounds__(TPB) void k_cg_init64(int64_t n, const double *__restrict__ b,
+                                                   double *__restrict__ x, double *__restrict__ r,
+                                                   double *__restrict__ z, double *__restrict__ p,
+                                                   const double *__restrict__ diag,
+                                                   SolveScalars *sc, double *partials,
+                                                   unsigned *counter) {
+  double bb = 0., rz = 0.;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) {
+    double bi = b[i];
+    x[i] = 0.;
+    r[i] = bi;
+    p[i] = 0.;
+    double zi = bi / diag[i];
+    z[i] = zi;
+    bb += bi * bi;
+    rz += bi * zi;
+  }
+  double v[2] = {bb, rz}, tot[2];
+  if (reduce_last<2>(v, partials, counter, tot) && threadIdx.x == 0) {
+    double normb = sqrt(tot[0]);
+    double resid = normb == 0. ? 0. : 1.0;
+    if (normb == 0.) normb = 1.;
+    sc->normb = normb;
+    sc->resid = resid;
+    sc->iter = 0;
+    sc->rho = tot[1];
+    sc->beta = 0.;
+    sc->done = resid <= sc->tol ? 1 : 0;
+  }
+}
+
+// r = f - A u (FP64), |r|^2 and |f|^2
+template <int VL>
+__global__ __launch_bounds__(TPB) void k_residual64(int64_t n, const int64_t *__restrict__ ia,
+                                                    const int32_t *__restrict__ ja,
+                                                    const double *__restrict__ a,
+                                                    const double *__restrict__ u,
+                                                    const double *__restrict__ f,
+                                                    double *__restrict__ r, double *out,
+                                                    double *partials, unsigned *counter) {
+  const int sub = threadIdx.x % VL;
+  int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / VL;
+  const int64_t ng = (int64_t)gridDim.x * blockDim.x / VL;
+  double rr = 0., ff = 0.;
+  for (int64_t i = g; i < n; i += ng) {
+    double s = row_dot64<VL>(i, ia, ja, a, u, sub);
+    if (sub == 0) {
+      double ri = f[i] - s;
+      if (r) r[i] = ri;
+      rr += ri * ri;
+      ff += f[i] * f[i];
+    }
+  }
+  double v[2] = {rr, ff}, tot[2];
+  if (reduce_last<2>(v, partials, counter, tot) && threadIdx.x == 0) {
+    out[0] = tot[0];
+    out[1] = tot[1];
+  }
+}
+
+__global__ void k_axpby64(int64_t n, double alpha, const double *__restrict__ x, double beta,
+                          const double *__restrict__ y, double *__restrict__ z) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) z[i] = alpha * x[i] + (y ? beta * y[i] : 0.);
+}
+
+static int pick_vl(int64_t nnz, int64_t n) {
+  double avg = n ? (double)nnz / n : 1;
+  if (avg > 96) return 64;
+  if (avg > 40) return 32;
+  if (avg > 20) return 16;
+  if (avg > 10) return 8;
+  return 4;
+}
+
+#define DISPATCH_VL(VLV, CALL)   \
+  switch (VLV) {                 \
+    case 64: { constexpr int VL = 64; CALL; } break; \
+    case 32: { constexpr int VL = 32; CALL; } break; \
+    case 16: { constexpr int VL = 16; CALL; } break; \
+    case 8:  { constexpr int VL = 8;  CALL; } break; \
+    default: { constexpr int VL = 4;  CALL; } break; \
+  }
+
+static int spmv_grid(int64_t n, int vl) {
+  int rows_per_block = TPB / vl;
+  int64_t g = (n + rows_per_block - 1) / rows_per_block;
+  int64_t cap = 8192;
+  if (g > cap) g = cap;
+  return (int)std::max<int64_t>(g, 1);
+}
+
+int launch_spmv64(Ctx &c, const double *x, double *y) {
+  int vl = pick_vl(c.nnz_full, c.neq);
+  int g = spmv_grid(c.neq, vl);
+  DISPATCH_VL(vl, hipLaunchKernelGGL(k_spmv64<VL>, dim3(g), dim3(TPB), 0, c.stream, c.neq,
+                                     c.d_fia.as<int64_t>(), c.d_fja.as<int32_t>(),
+                                     c.d_fa.as<double>(), x, y));
+  PZG_HIP(hipGetLastError());
+  return PZG_OK;
+}
+
+static int ensure_reduce(Ctx &c) {
+  if (!c.d_partials.p) {
+    PZG_TRY(dev_alloc(c.d_partials, sizeof(double) * NSLOT * MAXBLK));
+    PZG_TRY(dev_alloc(c.d_counter, sizeof(unsigned) * 16));
+    PZG_HIP(hipMemsetAsync(c.d_counter.p, 0, sizeof(unsigned) * 16, c.stream));
+  }
+  PZG_TRY(dev_alloc(c.d_scal, sizeof(SolveScalars) * 2));
+  return PZG_OK;
+}
+
+int launch_multadd_upper(Ctx &c, const double *x, const double *y, double *z, double alpha,
+                         double beta) {
+  if (!c.full_valid) PZG_TRY(launch_expand_full(c));
+  PZG_TRY(launch_spmv64(c, x, z));
+  int64_t n = c.neq;
+  hipLaunchKernelGGL(k_axpby64, dim3(grid_for(n, TPB)), dim3(TPB), 0, c.stream, n, alpha, z, beta,
+                     y, z);
+  PZG_HIP(hipGetLastError());
+  return PZG_OK;
+}
+
+int true_residual(Ctx &c, const double *u, double *rel) {
+  if (!c.full_valid) PZG_TRY(launch_expand_full(c));
+  PZG_TRY(ensure_reduce(c));
+  DevBuf out;
+  PZG_TRY(dev_alloc(out, 2 * sizeof(double)));
+  int vl = pick_vl(c.nnz_full, c.neq);
+  int g = grid_for(c.neq, TPB / vl);
+  DISPATCH_VL(vl, hipLaunchKernelGGL(k_residual64<VL>, dim3(g), dim3(TPB), 0, c.stream, c.neq,
+                                     c.d_fia.as<int64_t>(), c.d_fja.as<int32_t>(),
+                                     c.d_fa.as<double>(), u, c.d_f.as<double>(), nullptr,
+                                     out.as<double>(), c.d_partials.as<double>(),
+                                     c.d_counter.as<unsigned>()));
+  PZG_HIP(hipGetLastError());
+  double h[2];
+  PZG_HIP(hipMemcpyAsync(h, out.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+  PZG_HIP(hipStreamSynchronize(c.stream));
+  dev_free(out);
+  *rel = h[1] > 0 ? std::sqrt(h[0] / h[1]) : std::sqrt(h[0]);
+  return PZG_OK;
+}
+
+// Runs 'batch' iterations of the given enqueue function, optionally through a
+// captured hipGraph, until the device 'done' flag is set.
+template <class F>
+static int run_batches(Ctx &c, int batch, bool use_graph, F enqueue_iter, SolveScalars &hs) {
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  if (use_graph) {
+    PZG_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < batch; k++) enqueue_iter();
+    PZG_HIP(hipStreamEndCapture(c.stream, &graph));
+    PZG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  }
+  for (;;) {
+    if (use_graph) PZG_HIP(hipGraphLaunch(exec, c.stream));
+    else
+      for (int k = 0; k < batch; k++) enqueue_iter();
+    PZG_HIP(hipMemcpyAsync(&hs, c.d_scal.p, sizeof(SolveScalars), hipMemcpyDeviceToHost, c.stream));
+    PZG_HIP(hipStreamSynchronize(c.stream));
+    if (hs.done) break;
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  return PZG_OK;
+}
+
+int solve_pcg64(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info) {
+  const int64_t n = c.neq;
+  if (!c.full_valid) PZG_TRY(launch_expand_full(c));
+  PZG_TRY(ensure_reduce(c));
+  for (DevBuf *b : {&c.d_u, &c.d_r, &c.d_z, &c.d_p, &c.d_q}) PZG_TRY(dev_alloc(*b, n * sizeof(double)));
+  SolveScalars hs;
+  std::memset(&hs, 0, sizeof(hs));
+  hs.tol = prm.tol;
+  hs.maxit = prm.max_iter;
+  PZG_HIP(hipMemcpyAsync(c.d_scal.p, &hs, sizeof(hs), hipMemcpyHostToDevice, c.stream));
+  PZG_HIP(hipEventRecord(c.ev0, c.stream));
+  SolveScalars *sc = c.d_scal.as<SolveScalars>();
+  double *part = c.d_partials.as<double>();
+  unsigned *cnt = c.d_counter.as<unsigned>();
+  const int gv = grid_for(n, TPB);
+  hipLaunchKernelGGL(k_cg_init64, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_f.as<double>(),
+                     c.d_u.as<double>(), c.d_r.as<double>(), c.d_z.as<double>(),
+                     c.d_p.as<double>(), c.d_diag.as<double>(), sc, part, cnt);
+  PZG_HIP(hipGetLastError());
+  const int vl = pick_vl(c.nnz_full, n);
+  const int gs = grid_for(n, TPB / vl);
+  auto iter = [&]() {
+    hipLaunchKernelGGL(k_p_update64, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_z.as<double>(),
+                       c.d_p.as<double>(), sc);
+    DISPATCH_VL(vl, hipLaunchKernelGGL(k_spmv64_pq<VL>, dim3(gs), dim3(TPB), 0, c.stream, n,
+                                       c.d_fia.as<int64_t>(), c.d_fja.as<int32_t>(),
+                                       c.d_fa.as<double>(), c.d_p.as<double>(),
+                                       c.d_q.as<double>(), sc, part, cnt));
+    hipLaunchKernelGGL(k_cg_update64, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_u.as<double>(),
+                       c.d_r.as<double>(), c.d_z.as<double>(), c.d_p.as<double>(),
+                       c.d_q.as<double>(), c.d_diag.as<double>(), sc, (double *)nullptr, part,
+                       cnt);
+  };
+  int batch = prm.check_every > 0 ? prm.check_every : 50;
+  PZG_TRY(run_batches(c, batch, prm.use_graph != 0, iter, hs));
+  PZG_HIP(hipEventRecord(c.ev1, c.stream));
+  PZG_HIP(hipEventSynchronize(c.ev1));
+  float ms = 0;
+  PZG_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+  info.solve_ms = ms;
+  info.iterations = hs.iter;
+  info.outer = 1;
+  info.achieved = hs.resid;
+  info.converged = hs.done == 1;
+  double tr = 0;
+  PZG_TRY(true_residual(c, c.d_u.as<double>(), &tr));
+  info.true_residual = tr;
+  return hs.done == 1 ? PZG_OK : PZG_ENOCONV;
+}
+
+// ----------------------------------------------------------------------------
+// Mixed path: FP32 block CSR SpMV (block size BS, one wave per block row).
+template <int BS>
+__device__ inline void bsr_row(int64_t A, const int64_t *__restrict__ bia,
+                               const int32_t *__restrict__ bja, const float *__restrict__ v,
+                               const float *__restrict__ x, int lane, float (&out)[BS]) {
+  float acc[BS];
+  for (int k = 0; k < BS; k++) acc[k] = 0.f;
+  const int64_t b0 = bia[A], b1 = bia[A + 1];
+  for (int64_t t = b0 + lane; t < b1; t += 64) {
+    const int64_t col = bja[t];
+    const float *blk = v + t * (BS * BS);
+    float xv[BS];
+    for (int l = 0; l < BS; l++) xv[l] = x[col * BS + l];
+    for (int k = 0; k < BS; k++)
+      for (int l = 0; l < BS; l++) acc[k] += blk[k * BS + l] * xv[l];
+  }
+  for (int k = 0; k < BS; k++) {
+    float s = acc[k];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    out[k] = s;
+  }
+}
+
+template <int BS>
+__global__ __launch_bounds__(TPB) void k_spmv_bsr(int64_t nb, const int64_t *__restrict__ bia,
+                                                  const int32_t *__restrict__ bja,
+                                                  const float *__restrict__ v,
+                                                  const float *__restrict__ x,
+                                                  float *__restrict__ y) {
+  const int lane = threadIdx.x & 63;
+  int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t A = w; A < nb; A += nw) {
+    float out[BS];
+    bsr_row<BS>(A, bia, bja, v, x, lane, out);
+    if (lane < BS) {
+      float o = out[0];
+      for (int k = 1; k < BS; k++)
+        if (lane == k) o = out[k];
+      y[A * BS + lane] = o;
+    }
+  }
+}
+
+// q = S p; pq = p.q; pr = p.r; alpha = pr/pq
+template <int BS>
+__global__ __launch_bounds__(TPB) void k_fcg_spmv(int64_t nb, const int64_t *__restrict__ bia,
+                                                  const int32_t *__restrict__ bja,
+                                                  const float *__restrict__ v,
+                                                  const float *__restrict__ p,
+                                                  const float *__restrict__ r,
+                                                  float *__restrict__ q, SolveScalars *sc,
+                                                  double *partials, unsigned *counter) {
+  if (sc->done) return;
+  const int lane = threadIdx.x & 63;
+  int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  double pq = 0., pr = 0.;
+  for (int64_t A = w; A < nb; A += nw) {
+    float out[BS];
+    bsr_row<BS>(A, bia, bja, v, p, lane, out);
+    if (lane < BS) {
+      float o = out[0];
+      for (int k = 1; k < BS; k++)
+        if (lane == k) o = out[k];
+      const int64_t i = A * BS + lane;
+      q[i] = o;
+      pq += (double)o * p[i];
+      pr += (double)p[i] * r[i];
+    }
+  }
+  double vv[2] = {pq, pr}, tot[2];
+  if (reduce_last<2>(vv, partials, counter, tot) && threadIdx.x == 0) {
+    sc->pq = tot[0];
+    sc->pr = tot[1];
+    sc->alpha = tot[1] / tot[0];
+  }
+}
+
+// x += alpha p; r -= alpha q; z = Binv r (block Jacobi); rr, zq; beta = -zq/pq
+template <int BS>
+__global__ __launch_bounds__(TPB) void k_fcg_update(int64_t nb, float *__restrict__ x,
+                                                    float *__restrict__ r, float *__restrict__ z,
+                                                    const float *__restrict__ p,
+                                                    const float *__restrict__ q,
+                                                    const float *__restrict__ binv,
+                                                    SolveScalars *sc, double *partials,
+                                                    unsigned *counter) {
+  if (sc->done) return;
+  const float alpha = (float)sc->alpha;
+  double rr = 0., zq = 0.;
+  int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (; A < nb; A += st) {
+    float rv[BS];
+    for (int k = 0; k < BS; k++) {
+      const int64_t i = A * BS + k;
+      x[i] += alpha * p[i];
+      rv[k] = r[i] - alpha * q[i];
+      r[i] = rv[k];
+      rr += (double)rv[k] * rv[k];
+    }
+    const float *bi = binv + A * BS * BS;
+    for (int k = 0; k < BS; k++) {
+      float s = 0.f;
+      for (int l = 0; l < BS; l++) s += bi[k * BS + l] * rv[l];
+      z[A * BS + k] = s;
+      zq += (double)s * q[A * BS + k];
+    }
+  }
+  double vv[2] = {rr, zq}, tot[2];
+  if (reduce_last<2>(vv, partials, counter, tot) && threadIdx.x == 0) {
+    int64_t it = sc->iter + 1;
+    sc->iter = it;
+    double resid = sqrt(tot[0]) / sc->normb;
+    // divergence / stall guard: leave the inner loop, the outer loop decides
+    if (!(resid == resid) || resid > 1e3) sc->done = 3;
+    else if (resid <= sc->tol) sc->done = 1;
+    else if (it >= sc->maxit) sc->done = 2;
+    sc->resid = resid;
+    sc->beta = -tot[1] / sc->pq;
+  }
+}
+
+__global__ void k_p_update32(int64_t n, const float *__restrict__ z, float *__restrict__ p,
+                             const SolveScalars *sc) {
+  if (sc->done) return;
+  const float beta = (float)sc->beta;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) p[i] = z[i] + beta * p[i];
+}
+
+// b = D^-1/2 r64 (fp32); x = 0; r = b; z = Binv r; p = z; normb = |b|
+template <int BS>
+__global__ __launch_bounds__(TPB) void k_fcg_init(int64_t nb, const double *__restrict__ r64,
+                                                  const double *__restrict__ dis,
+                                                  float *__restrict__ x, float *__restrict__ r,
+                                                  float *__restrict__ z, float *__restrict__ p,
+                                                  const float *__restrict__ binv,
+                                                  SolveScalars *sc, double *partials,
+                                                  unsigned *counter) {
+  double bb = 0.;
+  int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (; A < nb; A += st) {
+    float rv[BS];
+    for (int k = 0; k < BS; k++) {
+      const int64_t i = A * BS + k;
+      double bi = r64[i] * dis[i];
+      rv[k] = (float)bi;
+      r[i] = rv[k];
+      x[i] = 0.f;
+      bb += bi * bi;
+    }
+    const float *bi = binv + A * BS * BS;
+    for (int k = 0; k < BS; k++) {
+      float s = 0.f;
+      for (int l = 0; l < BS; l++) s += bi[k * BS + l] * rv[l];
+      z[A * BS + k] = s;
+      p[A * BS + k] = s;
+    }
+  }
+  double vv[1] = {bb}, tot[1];
+  if (reduce_last<1>(vv, partials, counter, tot) && threadIdx.x == 0) {
+    double nbv = sqrt(tot[0]);
+    sc->normb = nbv > 0. ? nbv : 1.;
+    sc->iter = 0;
+    sc->resid = nbv > 0. ? 1. : 0.;
+    sc->done = nbv > 0. ? 0 : 1;
+  }
+}
+
+// u += D^-1/2 y
+__global__ void k_ir_update(int64_t n, const float *__restrict__ y, const double *__restrict__ dis,
+                            double *__restrict__ u) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) u[i] += dis[i] * (double)y[i];
+}
+
+int launch_spmv_bsr(Ctx &c, const float *x, float *y) {
+  int g = grid_for(c.nb, TPB / 64);
+  if (g < 1) g = 1;
+  if (c.bs == 3)
+    hipLaunchKernelGGL(k_spmv_bsr<3>, dim3(g), dim3(TPB), 0, c.stream, c.nb,
+                       c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(), c.d_bsr.as<float>(), x, y);
+  else
+    hipLaunchKernelGGL(k_spmv_bsr<1>, dim3(g), dim3(TPB), 0, c.stream, c.nb,
+                       c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(), c.d_bsr.as<float>(), x, y);
+  PZG_HIP(hipGetLastError());
+  return PZG_OK;
+}
+
+template <int BS>
+static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info) {
+  const int64_t n = c.neq, nb = c.nb;
+  if (!c.full_valid) PZG_TRY(launch_expand_full(c));
+  if (!c.bsr_valid) PZG_TRY(launch_build_bsr(c));
+  PZG_TRY(ensure_reduce(c));
+  for (DevBuf *b : {&c.d_u, &c.d_r}) PZG_TRY(dev_alloc(*b, n * sizeof(double)));
+  for (DevBuf *b : {&c.d_xs, &c.d_rs, &c.d_zs, &c.d_ps, &c.d_qs}) PZG_TRY(dev_alloc(*b, n * sizeof(float)));
+  DevBuf out;
+  PZG_TRY(dev_alloc(out, 2 * sizeof(double)));
+  SolveScalars *sc = c.d_scal.as<SolveScalars>();
+  double *part = c.d_partials.as<double>();
+  unsigned *cnt = c.d_counter.as<unsigned>();
+  const int gv = grid_for(n, TPB), gb = grid_for(nb, TPB), gs = grid_for(nb, TPB / 64);
+  const int vl = pick_vl(c.nnz_full, n), gr = grid_for(n, TPB / vl);
+  hipLaunchKernelGGL(k_axpby64, dim3(gv), dim3(TPB), 0, c.stream, n, 0.0, c.d_f.as<double>(), 0.0,
+                     (const double *)nullptr, c.d_u.as<double>());
+  PZG_HIP(hipEventRecord(c.ev0, c.stream));
+  int64_t total_it = 0;
+  int outer = 0;
+  double rel = 1.0;
+  bool conv = false;
+  const int batch = prm.check_every > 0 ? prm.check_every : 50;
+  for (;;) {
+    // true residual r64 = f - K u
+    DISPATCH_VL(vl, hipLaunchKernelGGL(k_residual64<VL>, dim3(gr), dim3(TPB), 0, c.stream, n,
+                                       c.d_fia.as<int64_t>(), c.d_fja.as<int32_t>(),
+                                       c.d_fa.as<double>(), c.d_u.as<double>(),
+                                       c.d_f.as<double>(), c.d_r.as<double>(), out.as<double>(),
+                                       part, cnt));
+    double h[2];
+    PZG_HIP(hipMemcpyAsync(h, out.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+    PZG_HIP(hipStreamSynchronize(c.stream));
+    rel = h[1] > 0 ? std::sqrt(h[0] / h[1]) : std::sqrt(h[0]);
+    if (rel <= prm.tol) { conv = true; break; }
+    if (outer >= prm.max_outer || total_it >= prm.max_iter) break;
+    outer++;
+    SolveScalars hs;
+    std::memset(&hs, 0, sizeof(hs));
+    hs.tol = prm.inner_tol > 0 ? prm.inner_tol : 1e-4;
+    // never iterate far below what the outer target needs
+    if (hs.tol < 0.5 * prm.tol / rel) hs.tol = 0.5 * prm.tol / rel;
+    hs.maxit = prm.max_iter - total_it;
+    PZG_HIP(hipMemcpyAsync(c.d_scal.p, &hs, sizeof(hs), hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(k_fcg_init<BS>, dim3(gb), dim3(TPB), 0, c.stream, nb, c.d_r.as<double>(),
+                       c.d_dinvsqrt.as<double>(), c.d_xs.as<float>(), c.d_rs.as<float>(),
+                       c.d_zs.as<float>(), c.d_ps.as<float>(), c.d_bjinv.as<float>(), sc, part,
+                       cnt);
+    PZG_HIP(hipGetLastError());
+    auto iter = [&]() {
+      hipLaunchKernelGGL(k_fcg_spmv<BS>, dim3(gs), dim3(TPB), 0, c.stream, nb,
+                         c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(), c.d_bsr.as<float>(),
+                         c.d_ps.as<float>(), c.d_rs.as<float>(), c.d_qs.as<float>(), sc, part,
+                         cnt);
+      hipLaunchKernelGGL(k_fcg_update<BS>, dim3(gb), dim3(TPB), 0, c.stream, nb,
+                         c.d_xs.as<float>(), c.d_rs.as<float>(), c.d_zs.as<float>(),
+                         c.d_ps.as<float>(), c.d_qs.as<float>(), c.d_bjinv.as<float>(), sc, part,
+                         cnt);
+      hipLaunchKernelGGL(k_p_update32, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_zs.as<float>(),
+                         c.d_ps.as<float>(), sc);
+    };
+    PZG_TRY(run_batches(c, batch, prm.use_graph != 0, iter, hs));
+    total_it += hs.iter;
+    hipLaunchKernelGGL(k_ir_update, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_xs.as<float>(),
+                       c.d_dinvsqrt.as<double>(), c.d_u.as<double>());
+    PZG_HIP(hipGetLastError());
+  }
+  PZG_HIP(hipEventRecord(c.ev1, c.stream));
+  PZG_HIP(hipEventSynchronize(c.ev1));
+  float ms = 0;
+  PZG_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+  dev_free(out);
+  info.solve_ms = ms;
+  info.iterations = total_it;
+  info.outer = outer;
+  info.true_residual = rel;
+  info.achieved = rel;
+  info.converged = conv ? 1 : 0;
+  return conv ? PZG_OK : PZG_ENOCONV;
+}
+
+int solve_mixed(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info) {
+  if (prm.precond == PZG_PRECOND_LEARNED) {
+    set_error("pzg_solve: learned preconditioner not trained in this context");
+    return PZG_ESTATE;
+  }
+  if (c.bs == 3) return mixed_impl<3>(c, prm, info);
+  if (c.bs == 1) return mixed_impl<1>(c, prm, info);
+  set_error("pzg_solve: unsupported block size for the mixed solver");
+  return PZG_EUNSUP;
+}
+
+}  // namespace pzg

@@ -1,0 +1,206 @@
+"""Structured computational meshes in NeoPZ's layout (caller side of the C ABI).
+
+In drop-in use the flattened mesh comes from ``TPZCompMesh`` (connect
+sequence numbers, ``Block()`` positions, element connect lists).  Here it is
+built directly for the unit square/cube, reproducing what
+``TPZCompMesh::AutoBuild`` + ``TPZLinearAnalysis(cmesh, false)`` number:
+connects are created in element order and, inside an element, in side order
+(corners, edges, faces, interior) the first time a side is met
+(``Mesh/TPZCompElH1.cpp:17-24``, ``Mesh/pzintel.cpp:652-703``,
+``Mesh/pzcmesh.h:703-715``).  Volume elements come first (x fastest), then the
+boundary elements face by face; boundary elements never create connects.
+
+The numbering is computed in closed form: every side has a structured key
+(node id, edge or face id by direction and lower corner, element id), and the
+sequence number of a key is the rank of its first occurrence in the
+element-major, side-minor walk.
+"""
+import numpy as np
+
+from . import problems as P
+
+QUAD, HEX, LINE = 2, 3, 1
+
+
+def _first_touch(keys):
+    """seq[k] for every entry of the walk = rank of the key's first occurrence."""
+    uniq, first, inv = np.unique(keys, return_index=True, return_inverse=True)
+    order = np.argsort(first, kind="stable")
+    rank = np.empty_like(order)
+    rank[order] = np.arange(len(order))
+    return rank[inv], len(uniq)
+
+
+def build(pb):
+    """Flattened mesh description of a :class:`neopz_amd.problems.Problem`."""
+    if pb.dim == 2:
+        return _build2d(pb)
+    if pb.dim == 3:
+        return _build3d(pb)
+    raise ValueError("structured builder supports dim 2 and 3")
+
+
+def _common(pb, econn_vol, econn_bc, nconn, corner_flags, xyz, vol_nodes, bc_nodes, bc_id):
+    ns = pb.nstate
+    # block size: corners carry nstate, higher sides nstate at p=2, 0 at p=1
+    nshape = np.where(corner_flags, 1, 1 if pb.p == 2 else 0)
+    blk_size = (nshape * ns).astype(np.int32)
+    blk_pos = np.zeros(nconn, dtype=np.int64)
+    if nconn > 1:
+        blk_pos[1:] = np.cumsum(blk_size[:-1].astype(np.int64))
+    neq = int(blk_size.astype(np.int64).sum())
+    ff = pb.force_porder if pb.force_kind != P.FORCE_NONE else 0
+    if pb.material == P.ELAST3D:
+        rule_vol = 2 * pb.p
+    else:
+        rule_vol = ff + pb.p if pb.p < ff else 2 * pb.p
+    big = 1.0e12 if pb.material == P.ELAST3D else 10.0 ** 17 * 2 / 3
+    bc_ids_used = [f for f in range(6) if pb.bc_type[f] >= 0]
+    return dict(
+        dim=pb.dim, material=pb.material, nstate=ns, porder=pb.p, xyz=xyz,
+        vol_nodes=vol_nodes, vol_conn=econn_vol,
+        vol_coef=None if pb.el_coef is None else np.ascontiguousarray(pb.el_coef, dtype=np.float64),
+        bc_nodes=bc_nodes, bc_conn=econn_bc, bc_id=bc_id,
+        bc_type=np.array([pb.bc_type[f] for f in range(6)], dtype=np.int32),
+        bc_val=np.array([pb.bc_val[f] for f in range(6)], dtype=np.float64).ravel(),
+        nconn=nconn, blk_pos=blk_pos, blk_size=blk_size, neq=neq,
+        scale=pb.scale, E=pb.E, nu=pb.nu, rule_order_vol=rule_vol, rule_order_bc=2 * pb.p,
+        force_kind=pb.force_kind, force_const=np.array(pb.force_const, dtype=np.float64),
+        bumps=None if pb.bumps is None else np.ascontiguousarray(pb.bumps, dtype=np.float64).reshape(-1, 6),
+        bump_width=pb.bump_width, big_number=big, bc_faces=bc_ids_used,
+    )
+
+
+def _build2d(pb):
+    nx, ny = pb.nx, pb.ny
+    NX = nx + 1
+    nv = NX * (ny + 1)
+    nxe = nx * (ny + 1)
+    nye = NX * ny
+    ix, iy = np.meshgrid(np.arange(nx), np.arange(ny), indexing="xy")
+    ix = ix.ravel().astype(np.int64)
+    iy = iy.ravel().astype(np.int64)
+    node = lambda i, j: i + NX * j
+    xe = lambda i, j: nv + i + nx * j
+    ye = lambda i, j: nv + nxe + i + NX * j
+    face = lambda e: nv + nxe + nye + e
+    e = np.arange(nx * ny, dtype=np.int64)
+    vol_nodes = np.stack([node(ix, iy), node(ix + 1, iy), node(ix + 1, iy + 1), node(ix, iy + 1)], 1)
+    keys = np.stack([node(ix, iy), node(ix + 1, iy), node(ix + 1, iy + 1), node(ix, iy + 1),
+                     xe(ix, iy), ye(ix + 1, iy), xe(ix, iy + 1), ye(ix, iy), face(e)], 1)
+    ktot = nv + nxe + nye + nx * ny
+    seq_flat, nconn = _first_touch(keys.ravel())
+    key2seq = np.full(ktot, -1, dtype=np.int64)
+    key2seq[keys.ravel()] = seq_flat
+    econn = seq_flat.reshape(-1, 9)
+    corner = np.zeros(nconn, dtype=bool)
+    corner[key2seq[:nv][key2seq[:nv] >= 0]] = True
+    xs = np.arange(NX) / nx
+    ys = np.arange(ny + 1) / ny
+    X, Y = np.meshgrid(xs, ys, indexing="xy")
+    xyz = np.stack([X.ravel(), Y.ravel(), np.zeros(nv)], 1).ravel()
+    bn, bk, bid = [], [], []
+    for f in range(4):
+        if pb.bc_type[f] < 0:
+            continue
+        ax, side = f // 2, f % 2
+        if ax == 0:
+            c = nx if side else 0
+            a = np.arange(ny, dtype=np.int64)
+            bn.append(np.stack([node(c, a), node(c, a + 1)], 1))
+            bk.append(np.stack([node(c, a), node(c, a + 1), ye(c, a)], 1))
+        else:
+            c = ny if side else 0
+            a = np.arange(nx, dtype=np.int64)
+            bn.append(np.stack([node(a, c), node(a + 1, c)], 1))
+            bk.append(np.stack([node(a, c), node(a + 1, c), xe(a, c)], 1))
+        bid.append(np.full(len(a), f, dtype=np.int32))
+    if bn:
+        bc_nodes = np.concatenate(bn).astype(np.int64)
+        bc_conn = key2seq[np.concatenate(bk)]
+        bc_id = np.concatenate(bid)
+    else:
+        bc_nodes = np.zeros((0, 2), np.int64)
+        bc_conn = np.zeros((0, 3), np.int64)
+        bc_id = np.zeros(0, np.int32)
+    return _common(pb, econn.astype(np.int64), bc_conn.astype(np.int64), nconn, corner, xyz,
+                   vol_nodes.astype(np.int64), bc_nodes, bc_id)
+
+
+def _build3d(pb):
+    nx, ny, nz = pb.nx, pb.ny, pb.nz
+    NX, NY, NZ = nx + 1, ny + 1, nz + 1
+    nv = NX * NY * NZ
+    n_xe, n_ye, n_ze = nx * NY * NZ, NX * ny * NZ, NX * NY * nz
+    n_xf, n_yf, n_zf = NX * ny * nz, nx * NY * nz, nx * ny * NZ
+    o_xe = nv
+    o_ye = o_xe + n_xe
+    o_ze = o_ye + n_ye
+    o_xf = o_ze + n_ze
+    o_yf = o_xf + n_xf
+    o_zf = o_yf + n_yf
+    o_v = o_zf + n_zf
+    ktot = o_v + nx * ny * nz
+    node = lambda i, j, k: i + NX * (j + NY * k)
+    xe = lambda i, j, k: o_xe + i + nx * (j + NY * k)
+    ye = lambda i, j, k: o_ye + i + NX * (j + ny * k)
+    ze = lambda i, j, k: o_ze + i + NX * (j + NY * k)
+    xf = lambda i, j, k: o_xf + i + NX * (j + ny * k)
+    yf = lambda i, j, k: o_yf + i + nx * (j + NY * k)
+    zf = lambda i, j, k: o_zf + i + nx * (j + ny * k)
+    iz, iy, ix = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    ix = ix.ravel().astype(np.int64)
+    iy = iy.ravel().astype(np.int64)
+    iz = iz.ravel().astype(np.int64)
+    e = np.arange(nx * ny * nz, dtype=np.int64)
+    c = [node(ix, iy, iz), node(ix + 1, iy, iz), node(ix + 1, iy + 1, iz), node(ix, iy + 1, iz),
+         node(ix, iy, iz + 1), node(ix + 1, iy, iz + 1), node(ix + 1, iy + 1, iz + 1),
+         node(ix, iy + 1, iz + 1)]
+    vol_nodes = np.stack(c, 1)
+    keys = np.stack(c + [
+        xe(ix, iy, iz), ye(ix + 1, iy, iz), xe(ix, iy + 1, iz), ye(ix, iy, iz),
+        ze(ix, iy, iz), ze(ix + 1, iy, iz), ze(ix + 1, iy + 1, iz), ze(ix, iy + 1, iz),
+        xe(ix, iy, iz + 1), ye(ix + 1, iy, iz + 1), xe(ix, iy + 1, iz + 1), ye(ix, iy, iz + 1),
+        zf(ix, iy, iz), yf(ix, iy, iz), xf(ix + 1, iy, iz), yf(ix, iy + 1, iz), xf(ix, iy, iz),
+        zf(ix, iy, iz + 1), o_v + e], 1)
+    seq_flat, nconn = _first_touch(keys.ravel())
+    key2seq = np.full(ktot, -1, dtype=np.int64)
+    key2seq[keys.ravel()] = seq_flat
+    econn = seq_flat.reshape(-1, 27)
+    corner = np.zeros(nconn, dtype=bool)
+    corner[key2seq[:nv][key2seq[:nv] >= 0]] = True
+    gz, gy, gx = np.meshgrid(np.arange(NZ) / nz, np.arange(NY) / ny, np.arange(NX) / nx, indexing="ij")
+    xyz = np.stack([gx.ravel(), gy.ravel(), gz.ravel()], 1).ravel()
+    bn, bk, bid = [], [], []
+    for f in range(6):
+        if pb.bc_type[f] < 0:
+            continue
+        ax, side = f // 2, f % 2
+        nu_ = ny if ax == 0 else nx
+        nv_ = ny if ax == 2 else nz
+        cc = (nx if ax == 0 else ny if ax == 1 else nz) if side else 0
+        b, a = np.meshgrid(np.arange(nv_), np.arange(nu_), indexing="ij")
+        a = a.ravel().astype(np.int64)
+        b = b.ravel().astype(np.int64)
+        if ax == 0:
+            nodes = [node(cc, a, b), node(cc, a + 1, b), node(cc, a + 1, b + 1), node(cc, a, b + 1)]
+            sides = [ye(cc, a, b), ze(cc, a + 1, b), ye(cc, a, b + 1), ze(cc, a, b), xf(cc, a, b)]
+        elif ax == 1:
+            nodes = [node(a, cc, b), node(a + 1, cc, b), node(a + 1, cc, b + 1), node(a, cc, b + 1)]
+            sides = [xe(a, cc, b), ze(a + 1, cc, b), xe(a, cc, b + 1), ze(a, cc, b), yf(a, cc, b)]
+        else:
+            nodes = [node(a, b, cc), node(a + 1, b, cc), node(a + 1, b + 1, cc), node(a, b + 1, cc)]
+            sides = [xe(a, b, cc), ye(a + 1, b, cc), xe(a, b + 1, cc), ye(a, b, cc), zf(a, b, cc)]
+        bn.append(np.stack(nodes, 1))
+        bk.append(np.stack(nodes + sides, 1))
+        bid.append(np.full(len(a), f, dtype=np.int32))
+    if bn:
+        bc_nodes = np.concatenate(bn).astype(np.int64)
+        bc_conn = key2seq[np.concatenate(bk)]
+        bc_id = np.concatenate(bid)
+    else:
+        bc_nodes = np.zeros((0, 4), np.int64)
+        bc_conn = np.zeros((0, 9), np.int64)
+        bc_id = np.zeros(0, np.int32)
+    return _common(pb, econn.astype(np.int64), bc_conn.astype(np.int64), nconn, corner, xyz,
+                   vol_nodes.astype(np.int64), bc_nodes, bc_id)
