@@ -103,7 +103,7 @@ def main():
             st.SetCG(200000, TPZStepSolver.jacobi(), args.tol, 0)
         else:
             pre = L.PRECOND_LEARNED if args.precond == "learned" else L.PRECOND_BLOCK_JACOBI
-            st.SetMixedFCG(200000, args.tol, precond=pre, inner_tol=1e-4)
+            st.SetMixedFCG(200000, args.tol, precond=pre, inner_tol=0.1)
         st.check_every = 100
         return st
 

@@ -212,6 +212,41 @@ int pzg_get_sizes(pzg_ctx *ctx, pzg_sizes *s);
 /* Synchronise the context stream. */
 int pzg_sync(pzg_ctx *ctx);
 
+/* Replaces the per-element coefficients of the current mesh (a new
+ * TPZElasticity3D E / TPZDarcyFlow permeability field on the same mesh);
+ * the next pzg_assemble uses them.  Same role as re-running the element loop
+ * of TPZStructMatrixOR::Assemble (StrMatrix/pzstrmatrixor.cpp:104-365) after
+ * changing material data. */
+int pzg_mesh_set_coef(pzg_ctx *ctx, const double *vol_coef);
+
+/* ---- learned preconditioner (no counterpart in the reference) -------------
+ * A message-passing network on the aggregated block graph that adds a coarse
+ * correction to the block-Jacobi step of the mixed solver
+ * (PZG_PRECOND_LEARNED).  NeoPZ has no learned component; these entry points
+ * serve the mixed solver only. */
+/* Builds aggregates (coarsen x coarsen [x coarsen] elements), the coarse graph
+ * and random initial weights (output layer zero = block Jacobi); reserves
+ * room for max_samples training pairs. */
+int pzg_learned_setup(pzg_ctx *ctx, int32_t coarsen, uint64_t seed, int32_t max_samples);
+/* Generates (residual, error) pairs with the block-Jacobi FCG on the assembled
+ * system: nrhs random solutions, pairs stored at the given iterations. */
+int pzg_learned_collect(pzg_ctx *ctx, int32_t nrhs, const int32_t *record_iters, int32_t nrec,
+                        uint64_t seed);
+/* Full-batch Adam on the stored pairs; mean loss per pair before/after. */
+int pzg_learned_train(pzg_ctx *ctx, int32_t steps, double lr, double *loss_first,
+                      double *loss_last);
+/* Reads (get) and/or writes (set) the nparams FP32 weights; *n = nparams. */
+int pzg_learned_weights(pzg_ctx *ctx, float *get, const float *set, int32_t *n);
+/* Coarse graph (cia[nc+1], cja[cia[nc]]), aggregate of every block (agg[nb])
+ * and the number of stored pairs; arrays may be NULL. */
+int pzg_learned_graph(pzg_ctx *ctx, int64_t *nc, int64_t *cia, int32_t *cja, int32_t *agg,
+                      int64_t *nsamples);
+/* Copies the stored pairs X, T ([sample][aggregate][bs] FP32). */
+int pzg_learned_samples(pzg_ctx *ctx, float *X, float *T);
+/* Loss and gradient of the network on caller data X, T (ns samples). */
+int pzg_learned_loss_grad(pzg_ctx *ctx, int64_t ns, const float *X, const float *T, double *loss,
+                          float *grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,10 +19,13 @@ EXPORTS = [
     "pzg_last_error", "pzg_create", "pzg_destroy", "pzg_mesh_set", "pzg_struct_create",
     "pzg_struct_get_pattern", "pzg_assemble", "pzg_get_system", "pzg_csr_set",
     "pzg_add_kel_batch", "pzg_multadd", "pzg_solve", "pzg_true_residual", "pzg_time_kernel",
-    "pzg_get_sizes", "pzg_sync",
+    "pzg_get_sizes", "pzg_sync", "pzg_mesh_set_coef", "pzg_learned_setup", "pzg_learned_collect",
+    "pzg_learned_train", "pzg_learned_weights", "pzg_learned_graph", "pzg_learned_samples",
+    "pzg_learned_loss_grad",
 ]
 
 dp = C.POINTER(C.c_double)
+fp = C.POINTER(C.c_float)
 i64p = C.POINTER(C.c_int64)
 i32p = C.POINTER(C.c_int32)
 
@@ -103,6 +106,14 @@ def load():
     L.pzg_time_kernel.argtypes = [vp, C.c_int32, C.c_int32, dp, dp]
     L.pzg_get_sizes.argtypes = [vp, C.POINTER(Sizes)]
     L.pzg_sync.argtypes = [vp]
+    L.pzg_mesh_set_coef.argtypes = [vp, dp]
+    L.pzg_learned_setup.argtypes = [vp, C.c_int32, C.c_uint64, C.c_int32]
+    L.pzg_learned_collect.argtypes = [vp, C.c_int32, i32p, C.c_int32, C.c_uint64]
+    L.pzg_learned_train.argtypes = [vp, C.c_int32, C.c_double, dp, dp]
+    L.pzg_learned_weights.argtypes = [vp, fp, fp, i32p]
+    L.pzg_learned_graph.argtypes = [vp, i64p, i64p, i32p, i32p, i64p]
+    L.pzg_learned_samples.argtypes = [vp, fp, fp]
+    L.pzg_learned_loss_grad.argtypes = [vp, C.c_int64, fp, fp, dp, fp]
     _lib = L
     return L
 

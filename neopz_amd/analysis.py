@@ -144,8 +144,8 @@ class TPZStepSolver:
         self.tol = 0.0
         self.pre = None
         self.precond = L.PRECOND_BLOCK_JACOBI
-        self.inner_tol = 1e-4
-        self.max_outer = 50
+        self.inner_tol = 0.1
+        self.max_outer = 200
         self.check_every = 50
         self.use_graph = True
         self.info = None
@@ -177,9 +177,11 @@ class TPZStepSolver:
         self.tol = float(tol)
         self.pre = pre
 
-    def SetMixedFCG(self, numiterations, tol, precond=L.PRECOND_BLOCK_JACOBI, inner_tol=1e-4,
-                    max_outer=50):
-        """FP32 flexible CG inside FP64 iterative refinement; stops on the true residual."""
+    def SetMixedFCG(self, numiterations, tol, precond=L.PRECOND_BLOCK_JACOBI, inner_tol=0.1,
+                    max_outer=200):
+        """FP32 flexible CG with FP64 reliable updates; stops on the true residual.
+        inner_tol is the reduction of the FP32 recursive residual between two
+        true-residual updates."""
         self.kind = self.EMixedFCG
         self.maxit = int(numiterations)
         self.tol = float(tol)

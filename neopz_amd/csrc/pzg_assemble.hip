@@ -605,11 +605,13 @@ __global__ void k_dinvsqrt(int64_t n, const double *__restrict__ d, double *__re
   if (i < n) s[i] = 1.0 / sqrt(fabs(d[i]));
 }
 
+// Block values are stored component-major: entry (k,l) of block t at
+// bsr[(k*BS+l)*nbf + t], so that lanes reading consecutive blocks coalesce.
 template <int BS>
-__global__ void k_bsr_diag(int64_t nb, const int64_t *__restrict__ ia, const double *__restrict__ a,
-                           const int32_t *__restrict__ nlow, const int64_t *__restrict__ fbia,
-                           const double *__restrict__ dis, float *__restrict__ bsr,
-                           float *__restrict__ binv) {
+__global__ void k_bsr_diag(int64_t nb, int64_t nbf, const int64_t *__restrict__ ia,
+                           const double *__restrict__ a, const int32_t *__restrict__ nlow,
+                           const int64_t *__restrict__ fbia, const double *__restrict__ dis,
+                           float *__restrict__ bsr, float *__restrict__ binv) {
   int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (A >= nb) return;
   double m[BS][BS];
@@ -619,9 +621,9 @@ __global__ void k_bsr_diag(int64_t nb, const int64_t *__restrict__ ia, const dou
       m[k][l] = v;
       m[l][k] = v;
     }
-  float *blk = bsr + (fbia[A] + nlow[A]) * BS * BS;
+  const int64_t t = fbia[A] + nlow[A];
   for (int k = 0; k < BS; k++)
-    for (int l = 0; l < BS; l++) blk[k * BS + l] = (float)m[k][l];
+    for (int l = 0; l < BS; l++) bsr[(k * BS + l) * nbf + t] = (float)m[k][l];
   float *inv = binv + A * BS * BS;
   if (BS == 1) {
     inv[0] = (float)(1.0 / m[0][0]);
@@ -648,7 +650,8 @@ __global__ void k_bsr_diag(int64_t nb, const int64_t *__restrict__ ia, const dou
 }
 
 template <int BS>
-__global__ void k_bsr_off(int64_t nb, const int64_t *__restrict__ ia, const double *__restrict__ a,
+__global__ void k_bsr_off(int64_t nb, int64_t nbf, const int64_t *__restrict__ ia,
+                          const double *__restrict__ a,
                           const int64_t *__restrict__ ubia, const int32_t *__restrict__ ubja,
                           const int32_t *__restrict__ rfwd, const int32_t *__restrict__ rbwd,
                           const int64_t *__restrict__ fbia, const double *__restrict__ dis,
@@ -663,8 +666,8 @@ __global__ void k_bsr_off(int64_t nb, const int64_t *__restrict__ ia, const doub
     int k = (int)((e % BB) / BS), l = (int)(e % BS);
     int64_t B = ubja[u0 + t];
     double v = a[ia[A * BS + k] + BS + t * BS - k + l] * dis[A * BS + k] * dis[B * BS + l];
-    bsr[(fbia[A] + rfwd[u0 + t]) * BB + k * BS + l] = (float)v;
-    bsr[(fbia[B] + rbwd[u0 + t]) * BB + l * BS + k] = (float)v;
+    bsr[(k * BS + l) * nbf + fbia[A] + rfwd[u0 + t]] = (float)v;
+    bsr[(l * BS + k) * nbf + fbia[B] + rbwd[u0 + t]] = (float)v;
   }
 }
 
@@ -682,21 +685,21 @@ int launch_build_bsr(Ctx &c) {
   int64_t threads = c.nb * 64;
   if (bs == 3) {
     hipLaunchKernelGGL(k_bsr_diag<3>, dim3((c.nb + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
-                       c.d_ia.as<int64_t>(), c.d_a.as<double>(), nl.as<int32_t>(),
+                       c.nbf, c.d_ia.as<int64_t>(), c.d_a.as<double>(), nl.as<int32_t>(),
                        c.d_fbia.as<int64_t>(), c.d_dinvsqrt.as<double>(), c.d_bsr.as<float>(),
                        c.d_bjinv.as<float>());
     hipLaunchKernelGGL(k_bsr_off<3>, dim3((threads + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
-                       c.d_ia.as<int64_t>(), c.d_a.as<double>(), c.d_ubia.as<int64_t>(),
+                       c.nbf, c.d_ia.as<int64_t>(), c.d_a.as<double>(), c.d_ubia.as<int64_t>(),
                        c.d_ubja.as<int32_t>(), c.d_rank_fwd.as<int32_t>(),
                        c.d_rank_bwd.as<int32_t>(), c.d_fbia.as<int64_t>(),
                        c.d_dinvsqrt.as<double>(), c.d_bsr.as<float>());
   } else if (bs == 1) {
     hipLaunchKernelGGL(k_bsr_diag<1>, dim3((c.nb + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
-                       c.d_ia.as<int64_t>(), c.d_a.as<double>(), nl.as<int32_t>(),
+                       c.nbf, c.d_ia.as<int64_t>(), c.d_a.as<double>(), nl.as<int32_t>(),
                        c.d_fbia.as<int64_t>(), c.d_dinvsqrt.as<double>(), c.d_bsr.as<float>(),
                        c.d_bjinv.as<float>());
     hipLaunchKernelGGL(k_bsr_off<1>, dim3((threads + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
-                       c.d_ia.as<int64_t>(), c.d_a.as<double>(), c.d_ubia.as<int64_t>(),
+                       c.nbf, c.d_ia.as<int64_t>(), c.d_a.as<double>(), c.d_ubia.as<int64_t>(),
                        c.d_ubja.as<int32_t>(), c.d_rank_fwd.as<int32_t>(),
                        c.d_rank_bwd.as<int32_t>(), c.d_fbia.as<int64_t>(),
                        c.d_dinvsqrt.as<double>(), c.d_bsr.as<float>());

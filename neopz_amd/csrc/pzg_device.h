@@ -1,0 +1,62 @@
+// pzg_device.h -- device helpers shared by the solver and learned kernels:
+// wavefront/block sums and the deterministic two-level reduction.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace pzg {
+
+constexpr int RED_TPB = 256;
+constexpr int MAXBLK = 2048;  // partial-sum slots per reduction
+constexpr int NSLOT = 4;
+
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Sum of v over the block; result valid in thread 0.
+__device__ inline double block_sum(double v, double *sm) {
+  v = wave_sum(v);
+  int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) sm[w] = v;
+  __syncthreads();
+  double s = 0.;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) s += sm[k];
+  return s;
+}
+
+// Stores this block's partials; returns true in the (single) last block once
+// every block has stored, after which 'tot' holds the fixed-order totals.
+// Launches must use at most MAXBLK blocks of RED_TPB threads.
+template <int NV>
+__device__ inline bool reduce_last(const double (&v)[NV], double *partials, unsigned *counter,
+                                   double (&tot)[NV]) {
+  __shared__ double sm[RED_TPB / 64];
+  __shared__ bool last;
+  for (int k = 0; k < NV; k++) {
+    double s = block_sum(v[k], sm);
+    if (threadIdx.x == 0) partials[k * MAXBLK + blockIdx.x] = s;
+  }
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned prev = atomicAdd(counter, 1u);
+    last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+  for (int k = 0; k < NV; k++) {
+    double s = 0.;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
+      s += __builtin_nontemporal_load(&partials[k * MAXBLK + b]);
+    s = block_sum(s, sm);
+    if (threadIdx.x == 0) tot[k] = s;
+  }
+  if (threadIdx.x == 0) *counter = 0u;
+  return true;
+}
+
+}  // namespace pzg

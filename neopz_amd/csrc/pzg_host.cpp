@@ -60,7 +60,12 @@ static void free_all(Ctx &c) {
                     &c.d_tbc, &c.d_vcolor_list, &c.d_bcolor_list, &c.d_x, &c.d_r, &c.d_z,
                     &c.d_p, &c.d_p2, &c.d_q, &c.d_u, &c.d_f, &c.d_w, &c.d_xs, &c.d_rs,
                     &c.d_zs, &c.d_ps, &c.d_ps2, &c.d_qs, &c.d_bs, &c.d_scal, &c.d_partials,
-                    &c.d_counter, &c.d_nn_w, &c.d_nn_h0, &c.d_nn_h1, &c.d_nn_agg};
+                    &c.d_counter, &c.d_nn_agg, &c.d_nn_captr, &c.d_nn_calist, &c.d_nn_cia,
+                    &c.d_nn_cja, &c.d_nn_w8, &c.d_nn_w, &c.d_nn_m, &c.d_nn_v, &c.d_nn_g,
+                    &c.d_nn_X, &c.d_nn_T, &c.d_nn_xi, &c.d_nn_zi, &c.d_nn_h0, &c.d_nn_h1,
+                    &c.d_nn_mi, &c.d_nn_trH, &c.d_nn_trM, &c.d_nn_trP, &c.d_nn_dH, &c.d_nn_G,
+                    &c.d_nn_tmp, &c.d_nn_Z, &c.d_nn_dZ, &c.d_nn_den, &c.d_nn_part,
+                    &c.d_nn_loss, &c.d_nn_ystar, &c.d_nn_e};
   for (DevBuf *b : bufs) dev_free(*b);
 }
 
@@ -775,6 +780,132 @@ int pzg_time_kernel(pzg_ctx *ctx, int32_t which, int32_t nrep, double *avg_ms, d
   *avg_ms = ms / nrep;
   if (bytes) *bytes = b;
   return PZG_OK;
+}
+
+int pzg_mesh_set_coef(pzg_ctx *ctx, const double *vol_coef) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (!c.has_mesh || !vol_coef) {
+    set_error("pzg_mesh_set_coef: no mesh or null coefficients");
+    return PZG_ESTATE;
+  }
+  c.vol_coef.assign(vol_coef, vol_coef + c.nvol);
+  PZG_TRY(dev_upload(c.d_vcoef, c.vol_coef.data(), c.vol_coef.size(), c.stream));
+  PZG_HIP(hipStreamSynchronize(c.stream));
+  c.assembled = false;
+  c.full_valid = c.bsr_valid = false;
+  c.has_solution = false;
+  return PZG_OK;
+}
+
+int pzg_learned_setup(pzg_ctx *ctx, int32_t coarsen, uint64_t seed, int32_t max_samples) {
+  CTX_CHECK();
+  return nn_setup(ctx->c, coarsen, seed, max_samples);
+}
+
+int pzg_learned_collect(pzg_ctx *ctx, int32_t nrhs, const int32_t *record_iters, int32_t nrec,
+                        uint64_t seed) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (!c.assembled || nrhs < 0 || nrec <= 0 || !record_iters) {
+    set_error("pzg_learned_collect: needs an assembled system and record iterations");
+    return PZG_ESTATE;
+  }
+  for (int k = 0; k < nrec; k++)
+    if (record_iters[k] < 1 || record_iters[k] > 100000) {
+      set_error("pzg_learned_collect: record iterations must be in [1, 100000]");
+      return PZG_EINVAL;
+    }
+  return nn_collect(c, nrhs, record_iters, nrec, seed);
+}
+
+int pzg_learned_train(pzg_ctx *ctx, int32_t steps, double lr, double *loss_first,
+                      double *loss_last) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (!c.nn_ready || c.nn_nsamp == 0) {
+    set_error("pzg_learned_train: no training pairs (pzg_learned_collect)");
+    return PZG_ESTATE;
+  }
+  double l = 0;
+  for (int t = 0; t < steps; t++) {
+    PZG_TRY(nn_loss_grad(c, c.nn_nsamp, c.d_nn_X.as<float>(), c.d_nn_T.as<float>(),
+                         (t == 0 || t == steps - 1) ? &l : nullptr));
+    if (t == 0 && loss_first) *loss_first = l / c.nn_nsamp;
+    PZG_TRY(nn_adam_step(c, lr));
+  }
+  PZG_TRY(nn_loss_grad(c, c.nn_nsamp, c.d_nn_X.as<float>(), c.d_nn_T.as<float>(), &l));
+  if (loss_last) *loss_last = l / c.nn_nsamp;
+  return PZG_OK;
+}
+
+int pzg_learned_weights(pzg_ctx *ctx, float *get, const float *set, int32_t *n) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  int np = nn_nparams(c.bs);
+  if (n) *n = np;
+  if (!c.nn_ready) {
+    if (get || set) {
+      set_error("pzg_learned_weights: call pzg_learned_setup first");
+      return PZG_ESTATE;
+    }
+    return PZG_OK;
+  }
+  if (get) PZG_HIP(hipMemcpy(get, c.d_nn_w.p, sizeof(float) * np, hipMemcpyDeviceToHost));
+  if (set) PZG_HIP(hipMemcpy(c.d_nn_w.p, set, sizeof(float) * np, hipMemcpyHostToDevice));
+  return PZG_OK;
+}
+
+int pzg_learned_graph(pzg_ctx *ctx, int64_t *nc, int64_t *cia, int32_t *cja, int32_t *agg,
+                      int64_t *nsamples) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (!c.nn_ready) {
+    set_error("pzg_learned_graph: call pzg_learned_setup first");
+    return PZG_ESTATE;
+  }
+  if (nc) *nc = c.nn_nc;
+  if (nsamples) *nsamples = c.nn_nsamp;
+  if (cia) std::memcpy(cia, c.nn_cia_host.data(), sizeof(int64_t) * c.nn_cia_host.size());
+  if (cja) std::memcpy(cja, c.nn_cja_host.data(), sizeof(int32_t) * c.nn_cja_host.size());
+  if (agg) std::memcpy(agg, c.nn_agg_host.data(), sizeof(int32_t) * c.nn_agg_host.size());
+  return PZG_OK;
+}
+
+int pzg_learned_samples(pzg_ctx *ctx, float *X, float *T) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (!c.nn_ready) {
+    set_error("pzg_learned_samples: call pzg_learned_setup first");
+    return PZG_ESTATE;
+  }
+  size_t bytes = sizeof(float) * c.nn_nsamp * c.nn_nc * c.bs;
+  if (X && bytes) PZG_HIP(hipMemcpy(X, c.d_nn_X.p, bytes, hipMemcpyDeviceToHost));
+  if (T && bytes) PZG_HIP(hipMemcpy(T, c.d_nn_T.p, bytes, hipMemcpyDeviceToHost));
+  return PZG_OK;
+}
+
+int pzg_learned_loss_grad(pzg_ctx *ctx, int64_t ns, const float *X, const float *T, double *loss,
+                          float *grad) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (!c.nn_ready || ns <= 0 || !X || !T) {
+    set_error("pzg_learned_loss_grad: needs pzg_learned_setup and data");
+    return PZG_ESTATE;
+  }
+  DevBuf dx, dt;
+  size_t n = (size_t)ns * c.nn_nc * c.bs;
+  int rc = dev_upload(dx, X, n, c.stream);
+  if (rc == PZG_OK) rc = dev_upload(dt, T, n, c.stream);
+  double l = 0;
+  if (rc == PZG_OK) rc = nn_loss_grad(c, ns, dx.as<float>(), dt.as<float>(), &l);
+  if (rc == PZG_OK && grad &&
+      hipMemcpy(grad, c.d_nn_g.p, sizeof(float) * nn_nparams(c.bs), hipMemcpyDeviceToHost) != hipSuccess)
+    rc = PZG_EHIP;
+  if (loss) *loss = l;
+  dev_free(dx);
+  dev_free(dt);
+  return rc;
 }
 
 }  // extern "C"

@@ -42,6 +42,12 @@ template <class T> int dev_upload(DevBuf &b, const T *h, size_t n, hipStream_t s
   return PZG_OK;
 }
 
+// Learned preconditioner network: hidden features, message-passing layers,
+// leaky-ReLU slope.
+constexpr int NNF = 32;
+constexpr int NNL = 3;
+constexpr float NN_SLOPE = 0.1f;
+
 // Maximum sizes of the element tables (hexahedron p=2).
 constexpr int MAXQ = 27;      // quadrature points per element
 constexpr int MAXSH = 27;     // shape functions per element
@@ -61,8 +67,10 @@ struct ElemTables {
 // Device-side scalar state of an iterative solve.
 struct SolveScalars {
   double rho, rho_old, pq, alpha, beta, rr, normb, resid, tol, pr, zq;
+  double rlast;   // mixed: scaled residual norm at the last reliable update
+  double delta;   // mixed: reduction of the recursive residual that triggers an update
   int64_t iter, maxit;
-  int32_t done, pad;
+  int32_t done, pad;  // done: 1 tol, 2 maxit, 3 divergence, 4 reliable update due
 };
 
 struct Ctx {
@@ -116,11 +124,20 @@ struct Ctx {
   int reduce_blocks = 1024;
   bool has_solution = false;
 
-  // learned preconditioner
-  DevBuf d_nn_w;            // weights (fp32)
+  // learned preconditioner (pzg_learned.hip)
   bool nn_ready = false;
-  int nn_layers = 3, nn_hidden = 32;
-  DevBuf d_nn_h0, d_nn_h1, d_nn_agg;  // node features workspace
+  int64_t nn_nc = 0;                   // aggregates (coarse nodes)
+  std::vector<int32_t> nn_agg_host, nn_cja_host;
+  std::vector<int64_t> nn_cia_host;
+  DevBuf d_nn_agg, d_nn_captr, d_nn_calist, d_nn_cia, d_nn_cja, d_nn_w8;
+  DevBuf d_nn_w, d_nn_m, d_nn_v, d_nn_g;          // weights, Adam moments, gradient
+  int nn_adam_t = 0;
+  int64_t nn_nsamp = 0, nn_maxsamp = 0;
+  DevBuf d_nn_X, d_nn_T;                          // training pairs [sample][aggregate][bs]
+  DevBuf d_nn_xi, d_nn_zi, d_nn_h0, d_nn_h1, d_nn_mi;  // inference workspace
+  DevBuf d_nn_trH, d_nn_trM, d_nn_trP, d_nn_dH, d_nn_G, d_nn_tmp, d_nn_Z, d_nn_dZ;
+  DevBuf d_nn_den, d_nn_part, d_nn_loss;
+  DevBuf d_nn_ystar, d_nn_e;                      // pair generation
 };
 
 // ---- element tables (pzg_shape.cpp) ----------------------------------------
@@ -140,6 +157,15 @@ int launch_multadd_upper(Ctx &c, const double *x, const double *y, double *z, do
                          double beta);
 int launch_addkel_batch(Ctx &c, int64_t nel, int ne, const int64_t *d_dest, const double *d_ek,
                         const int32_t *d_list, int64_t nlist);
+int side_corner_list_pub(int etype, int s, int *loc);
+int nn_nparams(int bs);
+int nn_setup(Ctx &c, int coarsen, uint64_t seed, int max_samples);
+int nn_apply(Ctx &c, const float *r, float *z, float *p, const float *q, SolveScalars *sc,
+             double *partials, unsigned *counter);
+int nn_loss_grad(Ctx &c, int64_t ns, const float *X, const float *T, double *loss_out);
+int nn_adam_step(Ctx &c, double lr);
+int nn_store_pair(Ctx &c, const float *r, const float *e_minus_bjr, int64_t slot);
+int nn_collect(Ctx &c, int nrhs, const int32_t *rec, int nrec, uint64_t seed);
 int solve_pcg64(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info);
 int solve_mixed(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info);
 int true_residual(Ctx &c, const double *u, double *rel);

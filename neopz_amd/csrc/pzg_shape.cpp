@@ -43,6 +43,8 @@ static int side_corner_list(int etype, int s, int *loc) {
   return 8;
 }
 
+int side_corner_list_pub(int etype, int s, int *loc) { return side_corner_list(etype, s, loc); }
+
 // Newton iteration on the Legendre recurrence in long double; points stored
 // as (-z_i, +z_i) pairs from the largest root down, which is NeoPZ's order
 // (Integral/tpzgaussrule.cpp:170-240); n = (int)(0.51*(order+2)).

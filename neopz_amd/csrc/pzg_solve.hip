@@ -9,10 +9,14 @@
 //  from the reference upper CSR).
 //
 //  MIXED.  FP32 flexible CG (Notay's FCG(1)) on S = D^-1/2 K D^-1/2 stored as
-//  FP32 block CSR, preconditioned by the inverse diagonal blocks, inside an
-//  FP64 iterative refinement u += D^-1/2 S^-1 D^-1/2 (f - K u) (the shape of
-//  IR<>, Solvers/LinearSolvers/ir.h:23-66); the stop test is the true FP64
-//  residual.
+//  FP32 block CSR, preconditioned by the inverse diagonal blocks, with FP64
+//  reliable updates: whenever the FP32 recursive residual has dropped by a
+//  factor delta, the FP32 correction is folded into the FP64 solution
+//  (u += D^-1/2 x), the true residual f - K u is recomputed in FP64 and
+//  replaces the recursive one, and the iteration continues along the same
+//  search direction.  This is iterative refinement (the shape of IR<>,
+//  Solvers/LinearSolvers/ir.h:23-66) without restarting the Krylov
+//  iteration; the stop test is the true FP64 residual.
 //
 //  All scalars live on the device; reductions are two-level and
 //  deterministic (per-block partials, the last block sums them in a fixed
@@ -24,62 +28,12 @@
 #include <cmath>
 #include <cstring>
 
+#include "pzg_device.h"
 #include "pzg_internal.h"
 
 namespace pzg {
 
-constexpr int TPB = 256;
-constexpr int MAXBLK = 2048;  // partial-sum slots per reduction
-constexpr int NSLOT = 4;
-
-__device__ inline double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-// Sum of v over the block; result valid in thread 0.
-__device__ inline double block_sum(double v, double *sm) {
-  v = wave_sum(v);
-  int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  __syncthreads();
-  if (l == 0) sm[w] = v;
-  __syncthreads();
-  double s = 0.;
-  if (threadIdx.x == 0)
-    for (int k = 0; k < (int)(blockDim.x >> 6); k++) s += sm[k];
-  return s;
-}
-
-// Stores this block's partials; returns true in the (single) last block once
-// every block has stored, after which 'tot' holds the fixed-order totals.
-template <int NV>
-__device__ inline bool reduce_last(const double (&v)[NV], double *partials, unsigned *counter,
-                                   double (&tot)[NV]) {
-  __shared__ double sm[TPB / 64];
-  __shared__ bool last;
-  for (int k = 0; k < NV; k++) {
-    double s = block_sum(v[k], sm);
-    if (threadIdx.x == 0) partials[k * MAXBLK + blockIdx.x] = s;
-  }
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned prev = atomicAdd(counter, 1u);
-    last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  for (int k = 0; k < NV; k++) {
-    double s = 0.;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
-      s += __builtin_nontemporal_load(&partials[k * MAXBLK + b]);
-    s = block_sum(s, sm);
-    if (threadIdx.x == 0) tot[k] = s;
-  }
-  if (threadIdx.x == 0) *counter = 0u;
-  return true;
-}
+constexpr int TPB = RED_TPB;
 
 static int grid_for(int64_t n, int per_block) {
   int64_t g = (n + per_block - 1) / per_block;
@@ -95,7 +49,10 @@ __device__ inline double row_dot64(int64_t r, const int64_t *__restrict__ ia,
                                    const int32_t *__restrict__ ja, const double *__restrict__ a,
                                    const double *__restrict__ x, int sub) {
   double s = 0.;
-  for (int64_t k = ia[r] + sub; k < ia[r + 1]; k += VL) s += a[k] * x[ja[k]];
+  // the matrix is streamed once per product: non-temporal loads keep the
+  // gathered vector resident in L2
+  for (int64_t k = ia[r] + sub; k < ia[r + 1]; k += VL)
+    s += __builtin_nontemporal_load(&a[k]) * x[__builtin_nontemporal_load(&ja[k])];
   for (int o = VL / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, VL);
   return s;
 }
@@ -421,17 +378,19 @@ int solve_pcg64(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info) {
 template <int BS>
 __device__ inline void bsr_row(int64_t A, const int64_t *__restrict__ bia,
                                const int32_t *__restrict__ bja, const float *__restrict__ v,
-                               const float *__restrict__ x, int lane, float (&out)[BS]) {
+                               int64_t nbf, const float *__restrict__ x, int lane,
+                               float (&out)[BS]) {
   float acc[BS];
   for (int k = 0; k < BS; k++) acc[k] = 0.f;
   const int64_t b0 = bia[A], b1 = bia[A + 1];
   for (int64_t t = b0 + lane; t < b1; t += 64) {
-    const int64_t col = bja[t];
-    const float *blk = v + t * (BS * BS);
+    const int64_t col = __builtin_nontemporal_load(&bja[t]);
+    float m[BS * BS];
+    for (int c = 0; c < BS * BS; c++) m[c] = __builtin_nontemporal_load(&v[c * nbf + t]);
     float xv[BS];
     for (int l = 0; l < BS; l++) xv[l] = x[col * BS + l];
     for (int k = 0; k < BS; k++)
-      for (int l = 0; l < BS; l++) acc[k] += blk[k * BS + l] * xv[l];
+      for (int l = 0; l < BS; l++) acc[k] += m[k * BS + l] * xv[l];
   }
   for (int k = 0; k < BS; k++) {
     float s = acc[k];
@@ -443,7 +402,7 @@ __device__ inline void bsr_row(int64_t A, const int64_t *__restrict__ bia,
 template <int BS>
 __global__ __launch_bounds__(TPB) void k_spmv_bsr(int64_t nb, const int64_t *__restrict__ bia,
                                                   const int32_t *__restrict__ bja,
-                                                  const float *__restrict__ v,
+                                                  const float *__restrict__ v, int64_t nbf,
                                                   const float *__restrict__ x,
                                                   float *__restrict__ y) {
   const int lane = threadIdx.x & 63;
@@ -451,7 +410,7 @@ __global__ __launch_bounds__(TPB) void k_spmv_bsr(int64_t nb, const int64_t *__r
   const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
   for (int64_t A = w; A < nb; A += nw) {
     float out[BS];
-    bsr_row<BS>(A, bia, bja, v, x, lane, out);
+    bsr_row<BS>(A, bia, bja, v, nbf, x, lane, out);
     if (lane < BS) {
       float o = out[0];
       for (int k = 1; k < BS; k++)
@@ -465,7 +424,7 @@ __global__ __launch_bounds__(TPB) void k_spmv_bsr(int64_t nb, const int64_t *__r
 template <int BS>
 __global__ __launch_bounds__(TPB) void k_fcg_spmv(int64_t nb, const int64_t *__restrict__ bia,
                                                   const int32_t *__restrict__ bja,
-                                                  const float *__restrict__ v,
+                                                  const float *__restrict__ v, int64_t nbf,
                                                   const float *__restrict__ p,
                                                   const float *__restrict__ r,
                                                   float *__restrict__ q, SolveScalars *sc,
@@ -477,7 +436,7 @@ __global__ __launch_bounds__(TPB) void k_fcg_spmv(int64_t nb, const int64_t *__r
   double pq = 0., pr = 0.;
   for (int64_t A = w; A < nb; A += nw) {
     float out[BS];
-    bsr_row<BS>(A, bia, bja, v, p, lane, out);
+    bsr_row<BS>(A, bia, bja, v, nbf, p, lane, out);
     if (lane < BS) {
       float o = out[0];
       for (int k = 1; k < BS; k++)
@@ -531,19 +490,22 @@ __global__ __launch_bounds__(TPB) void k_fcg_update(int64_t nb, float *__restric
   if (reduce_last<2>(vv, partials, counter, tot) && threadIdx.x == 0) {
     int64_t it = sc->iter + 1;
     sc->iter = it;
-    double resid = sqrt(tot[0]) / sc->normb;
-    // divergence / stall guard: leave the inner loop, the outer loop decides
+    double rn = sqrt(tot[0]);
+    double resid = rn / sc->normb;
+    // divergence guard: leave the loop, the host restarts from the true residual
     if (!(resid == resid) || resid > 1e3) sc->done = 3;
-    else if (resid <= sc->tol) sc->done = 1;
+    else if (rn < sc->delta * sc->rlast) sc->done = 4;
     else if (it >= sc->maxit) sc->done = 2;
     sc->resid = resid;
     sc->beta = -tot[1] / sc->pq;
   }
 }
 
+// runs when the iteration completed normally (done 0) or completed and asked
+// for a reliable update (done 4)
 __global__ void k_p_update32(int64_t n, const float *__restrict__ z, float *__restrict__ p,
                              const SolveScalars *sc) {
-  if (sc->done) return;
+  if (sc->done != 0 && sc->done != 4) return;
   const float beta = (float)sc->beta;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t st = (int64_t)gridDim.x * blockDim.x;
@@ -584,18 +546,90 @@ __global__ __launch_bounds__(TPB) void k_fcg_init(int64_t nb, const double *__re
   if (reduce_last<1>(vv, partials, counter, tot) && threadIdx.x == 0) {
     double nbv = sqrt(tot[0]);
     sc->normb = nbv > 0. ? nbv : 1.;
-    sc->iter = 0;
+    sc->rlast = nbv;
     sc->resid = nbv > 0. ? 1. : 0.;
     sc->done = nbv > 0. ? 0 : 1;
   }
 }
 
-// u += D^-1/2 y
-__global__ void k_ir_update(int64_t n, const float *__restrict__ y, const double *__restrict__ dis,
+// u += D^-1/2 y; y = 0
+__global__ void k_ir_update(int64_t n, float *__restrict__ y, const double *__restrict__ dis,
                             double *__restrict__ u) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t st = (int64_t)gridDim.x * blockDim.x;
-  for (; i < n; i += st) u[i] += dis[i] * (double)y[i];
+  for (; i < n; i += st) {
+    u[i] += dis[i] * (double)y[i];
+    y[i] = 0.f;
+  }
+}
+
+// reliable update: r = D^-1/2 r64 replaces the recursive residual; the search
+// direction p is kept
+__global__ __launch_bounds__(TPB) void k_fcg_reset(int64_t n, const double *__restrict__ r64,
+                                                   const double *__restrict__ dis,
+                                                   float *__restrict__ r, double delta,
+                                                   SolveScalars *sc, double *partials,
+                                                   unsigned *counter) {
+  double bb = 0.;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) {
+    float v = (float)(r64[i] * dis[i]);
+    r[i] = v;
+    bb += (double)v * v;
+  }
+  double vv[1] = {bb}, tot[1];
+  if (reduce_last<1>(vv, partials, counter, tot) && threadIdx.x == 0) {
+    sc->rlast = sqrt(tot[0]);
+    sc->delta = delta;
+    sc->done = 0;
+  }
+}
+
+// x = 0; r = b (fp32); z = Binv r; p = z; normb = |b|
+template <int BS>
+__global__ __launch_bounds__(TPB) void k_fcg_start(int64_t nb, const float *__restrict__ b,
+                                                   float *__restrict__ x, float *__restrict__ r,
+                                                   float *__restrict__ z, float *__restrict__ p,
+                                                   const float *__restrict__ binv,
+                                                   SolveScalars *sc, double *partials,
+                                                   unsigned *counter) {
+  double bb = 0.;
+  int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (; A < nb; A += st) {
+    float rv[BS];
+    for (int k = 0; k < BS; k++) {
+      const int64_t i = A * BS + k;
+      rv[k] = b[i];
+      r[i] = rv[k];
+      x[i] = 0.f;
+      bb += (double)rv[k] * rv[k];
+    }
+    const float *bi = binv + A * BS * BS;
+    for (int k = 0; k < BS; k++) {
+      float s = 0.f;
+      for (int l = 0; l < BS; l++) s += bi[k * BS + l] * rv[l];
+      z[A * BS + k] = s;
+      p[A * BS + k] = s;
+    }
+  }
+  double vv[1] = {bb}, tot[1];
+  if (reduce_last<1>(vv, partials, counter, tot) && threadIdx.x == 0) {
+    double nbv = sqrt(tot[0]);
+    sc->normb = nbv > 0. ? nbv : 1.;
+    sc->rlast = nbv;
+    sc->resid = 1.;
+    sc->done = 0;
+  }
+}
+
+// out = a - b - c
+__global__ void k_sub3(int64_t n, const float *__restrict__ a, const float *__restrict__ b,
+                       const float *__restrict__ c, float *__restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) out[i] = a[i] - b[i] - c[i];
 }
 
 int launch_spmv_bsr(Ctx &c, const float *x, float *y) {
@@ -603,10 +637,10 @@ int launch_spmv_bsr(Ctx &c, const float *x, float *y) {
   if (g < 1) g = 1;
   if (c.bs == 3)
     hipLaunchKernelGGL(k_spmv_bsr<3>, dim3(g), dim3(TPB), 0, c.stream, c.nb,
-                       c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(), c.d_bsr.as<float>(), x, y);
+                       c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(), c.d_bsr.as<float>(), c.nbf, x, y);
   else
     hipLaunchKernelGGL(k_spmv_bsr<1>, dim3(g), dim3(TPB), 0, c.stream, c.nb,
-                       c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(), c.d_bsr.as<float>(), x, y);
+                       c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(), c.d_bsr.as<float>(), c.nbf, x, y);
   PZG_HIP(hipGetLastError());
   return PZG_OK;
 }
@@ -626,16 +660,15 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
   unsigned *cnt = c.d_counter.as<unsigned>();
   const int gv = grid_for(n, TPB), gb = grid_for(nb, TPB), gs = grid_for(nb, TPB / 64);
   const int vl = pick_vl(c.nnz_full, n), gr = grid_for(n, TPB / vl);
+  PZG_HIP(hipEventRecord(c.ev0, c.stream));
   hipLaunchKernelGGL(k_axpby64, dim3(gv), dim3(TPB), 0, c.stream, n, 0.0, c.d_f.as<double>(), 0.0,
                      (const double *)nullptr, c.d_u.as<double>());
-  PZG_HIP(hipEventRecord(c.ev0, c.stream));
   int64_t total_it = 0;
   int outer = 0;
   double rel = 1.0;
   bool conv = false;
   const int batch = prm.check_every > 0 ? prm.check_every : 50;
-  for (;;) {
-    // true residual r64 = f - K u
+  auto true_res = [&]() -> int {
     DISPATCH_VL(vl, hipLaunchKernelGGL(k_residual64<VL>, dim3(gr), dim3(TPB), 0, c.stream, n,
                                        c.d_fia.as<int64_t>(), c.d_fja.as<int32_t>(),
                                        c.d_fa.as<double>(), c.d_u.as<double>(),
@@ -645,37 +678,78 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
     PZG_HIP(hipMemcpyAsync(h, out.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
     PZG_HIP(hipStreamSynchronize(c.stream));
     rel = h[1] > 0 ? std::sqrt(h[0] / h[1]) : std::sqrt(h[0]);
-    if (rel <= prm.tol) { conv = true; break; }
-    if (outer >= prm.max_outer || total_it >= prm.max_iter) break;
-    outer++;
-    SolveScalars hs;
-    std::memset(&hs, 0, sizeof(hs));
-    hs.tol = prm.inner_tol > 0 ? prm.inner_tol : 1e-4;
-    // never iterate far below what the outer target needs
-    if (hs.tol < 0.5 * prm.tol / rel) hs.tol = 0.5 * prm.tol / rel;
-    hs.maxit = prm.max_iter - total_it;
-    PZG_HIP(hipMemcpyAsync(c.d_scal.p, &hs, sizeof(hs), hipMemcpyHostToDevice, c.stream));
-    hipLaunchKernelGGL(k_fcg_init<BS>, dim3(gb), dim3(TPB), 0, c.stream, nb, c.d_r.as<double>(),
-                       c.d_dinvsqrt.as<double>(), c.d_xs.as<float>(), c.d_rs.as<float>(),
-                       c.d_zs.as<float>(), c.d_ps.as<float>(), c.d_bjinv.as<float>(), sc, part,
-                       cnt);
-    PZG_HIP(hipGetLastError());
+    return PZG_OK;
+  };
+  auto next_delta = [&]() {
+    // reduction asked of the FP32 recursion before the next true residual
+    double d = prm.inner_tol > 0 ? prm.inner_tol : 0.1;
+    double need = 0.5 * prm.tol / rel;
+    if (need > d) d = need;
+    return d > 0.5 ? 0.5 : d;
+  };
+  PZG_TRY(true_res());
+  if (rel <= prm.tol) conv = true;
+  bool restart = true;
+  bool learned = prm.precond == PZG_PRECOND_LEARNED;
+  int fallback = 0;
+  double prev_rel = rel;
+  SolveScalars hs;
+  std::memset(&hs, 0, sizeof(hs));
+  while (!conv) {
+    if (restart) {
+      hs.done = 0;
+      hs.iter = total_it;
+      hs.maxit = prm.max_iter;
+      hs.delta = next_delta();
+      PZG_HIP(hipMemcpyAsync(c.d_scal.p, &hs, sizeof(hs), hipMemcpyHostToDevice, c.stream));
+      hipLaunchKernelGGL(k_fcg_init<BS>, dim3(gb), dim3(TPB), 0, c.stream, nb, c.d_r.as<double>(),
+                         c.d_dinvsqrt.as<double>(), c.d_xs.as<float>(), c.d_rs.as<float>(),
+                         c.d_zs.as<float>(), c.d_ps.as<float>(), c.d_bjinv.as<float>(), sc, part,
+                         cnt);
+      PZG_HIP(hipGetLastError());
+      if (learned)
+        PZG_TRY(nn_apply(c, c.d_rs.as<float>(), c.d_zs.as<float>(), c.d_ps.as<float>(), nullptr, sc,
+                         part, cnt));
+      restart = false;
+    }
     auto iter = [&]() {
       hipLaunchKernelGGL(k_fcg_spmv<BS>, dim3(gs), dim3(TPB), 0, c.stream, nb,
                          c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(), c.d_bsr.as<float>(),
-                         c.d_ps.as<float>(), c.d_rs.as<float>(), c.d_qs.as<float>(), sc, part,
+                         c.nbf, c.d_ps.as<float>(), c.d_rs.as<float>(), c.d_qs.as<float>(), sc, part,
                          cnt);
       hipLaunchKernelGGL(k_fcg_update<BS>, dim3(gb), dim3(TPB), 0, c.stream, nb,
                          c.d_xs.as<float>(), c.d_rs.as<float>(), c.d_zs.as<float>(),
                          c.d_ps.as<float>(), c.d_qs.as<float>(), c.d_bjinv.as<float>(), sc, part,
                          cnt);
+      if (learned)
+        (void)nn_apply(c, c.d_rs.as<float>(), c.d_zs.as<float>(), nullptr, c.d_qs.as<float>(), sc,
+                       part, cnt);
       hipLaunchKernelGGL(k_p_update32, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_zs.as<float>(),
                          c.d_ps.as<float>(), sc);
     };
     PZG_TRY(run_batches(c, batch, prm.use_graph != 0, iter, hs));
-    total_it += hs.iter;
+    total_it = hs.iter;
+    // fold the FP32 correction into the FP64 solution and recompute the true residual
     hipLaunchKernelGGL(k_ir_update, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_xs.as<float>(),
                        c.d_dinvsqrt.as<double>(), c.d_u.as<double>());
+    PZG_HIP(hipGetLastError());
+    PZG_TRY(true_res());
+    outer++;
+    if (rel <= prm.tol) { conv = true; break; }
+    if (hs.done == 2 || outer >= prm.max_outer) break;
+    // stall of the learned preconditioner: fall back to block Jacobi and
+    // restart from the true residual
+    if (learned && (hs.done == 3 || rel > 0.9 * prev_rel)) {
+      learned = false;
+      fallback = 1;
+      restart = true;
+      prev_rel = rel;
+      continue;
+    }
+    prev_rel = rel;
+    if (hs.done == 3) { restart = true; continue; }
+    hipLaunchKernelGGL(k_fcg_reset, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_r.as<double>(),
+                       c.d_dinvsqrt.as<double>(), c.d_rs.as<float>(), next_delta(), sc, part, cnt);
     PZG_HIP(hipGetLastError());
   }
   PZG_HIP(hipEventRecord(c.ev1, c.stream));
@@ -689,12 +763,92 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
   info.true_residual = rel;
   info.achieved = rel;
   info.converged = conv ? 1 : 0;
+  info.fallback = fallback;
   return conv ? PZG_OK : PZG_ENOCONV;
 }
 
+// (residual, error) pairs from the solver itself: random scaled solutions y*,
+// b = S y*, block-Jacobi FCG from zero; at the iterations in rec the pair
+// (R r_k, R(y* - y_k - B^-1 r_k)) is stored.
+template <int BS>
+static int collect_impl(Ctx &c, int nrhs, const int32_t *rec, int nrec, uint64_t seed) {
+  const int64_t n = c.neq, nb = c.nb;
+  if (!c.full_valid) PZG_TRY(launch_expand_full(c));
+  if (!c.bsr_valid) PZG_TRY(launch_build_bsr(c));
+  PZG_TRY(ensure_reduce(c));
+  for (DevBuf *b : {&c.d_xs, &c.d_rs, &c.d_zs, &c.d_ps, &c.d_qs, &c.d_bs, &c.d_nn_ystar, &c.d_nn_e})
+    PZG_TRY(dev_alloc(*b, n * sizeof(float)));
+  int kmax = 0;
+  for (int k = 0; k < nrec; k++) kmax = rec[k] > kmax ? rec[k] : kmax;
+  std::vector<float> ys(n);
+  uint64_t st = seed * 6364136223846793005ULL + 1442695040888963407ULL;
+  SolveScalars *sc = c.d_scal.as<SolveScalars>();
+  double *part = c.d_partials.as<double>();
+  unsigned *cnt = c.d_counter.as<unsigned>();
+  const int gv = grid_for(n, TPB), gb = grid_for(nb, TPB), gs = grid_for(nb, TPB / 64);
+  for (int j = 0; j < nrhs; j++) {
+    for (int64_t i = 0; i < n; i++) {  // Box-Muller on a 64-bit LCG
+      st = st * 6364136223846793005ULL + 1442695040888963407ULL;
+      double u1 = ((st >> 11) + 1.0) / 9007199254740993.0;
+      st = st * 6364136223846793005ULL + 1442695040888963407ULL;
+      double u2 = (st >> 11) / 9007199254740992.0;
+      ys[i] = (float)(std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2));
+    }
+    PZG_HIP(hipMemcpyAsync(c.d_nn_ystar.p, ys.data(), n * sizeof(float), hipMemcpyHostToDevice, c.stream));
+    PZG_TRY(launch_spmv_bsr(c, c.d_nn_ystar.as<float>(), c.d_bs.as<float>()));
+    SolveScalars hs;
+    std::memset(&hs, 0, sizeof(hs));
+    hs.maxit = (int64_t)1 << 40;
+    hs.delta = 0.;
+    PZG_HIP(hipMemcpyAsync(c.d_scal.p, &hs, sizeof(hs), hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(k_fcg_start<BS>, dim3(gb), dim3(TPB), 0, c.stream, nb, c.d_bs.as<float>(),
+                       c.d_xs.as<float>(), c.d_rs.as<float>(), c.d_zs.as<float>(), c.d_ps.as<float>(),
+                       c.d_bjinv.as<float>(), sc, part, cnt);
+    for (int k = 1; k <= kmax; k++) {
+      hipLaunchKernelGGL(k_fcg_spmv<BS>, dim3(gs), dim3(TPB), 0, c.stream, nb, c.d_fbia.as<int64_t>(),
+                         c.d_fbja.as<int32_t>(), c.d_bsr.as<float>(), c.nbf, c.d_ps.as<float>(),
+                         c.d_rs.as<float>(), c.d_qs.as<float>(), sc, part, cnt);
+      hipLaunchKernelGGL(k_fcg_update<BS>, dim3(gb), dim3(TPB), 0, c.stream, nb, c.d_xs.as<float>(),
+                         c.d_rs.as<float>(), c.d_zs.as<float>(), c.d_ps.as<float>(), c.d_qs.as<float>(),
+                         c.d_bjinv.as<float>(), sc, part, cnt);
+      hipLaunchKernelGGL(k_p_update32, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_zs.as<float>(),
+                         c.d_ps.as<float>(), sc);
+      for (int t = 0; t < nrec; t++)
+        if (rec[t] == k) {
+          if (c.nn_nsamp >= c.nn_maxsamp) {
+            set_error("learned preconditioner: sample capacity exceeded");
+            return PZG_EINVAL;
+          }
+          hipLaunchKernelGGL(k_sub3, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_nn_ystar.as<float>(),
+                             c.d_xs.as<float>(), c.d_zs.as<float>(), c.d_nn_e.as<float>());
+          PZG_TRY(nn_store_pair(c, c.d_rs.as<float>(), c.d_nn_e.as<float>(), c.nn_nsamp));
+          c.nn_nsamp++;
+        }
+    }
+    PZG_HIP(hipGetLastError());
+    PZG_HIP(hipStreamSynchronize(c.stream));
+    SolveScalars chk;
+    PZG_HIP(hipMemcpy(&chk, c.d_scal.p, sizeof(chk), hipMemcpyDeviceToHost));
+    if (chk.done == 3) {
+      set_error("learned preconditioner: pair generation diverged");
+      return PZG_EINVAL;
+    }
+  }
+  return PZG_OK;
+}
+
+int nn_collect(Ctx &c, int nrhs, const int32_t *rec, int nrec, uint64_t seed) {
+  if (!c.nn_ready) {
+    set_error("learned preconditioner: call pzg_learned_setup first");
+    return PZG_ESTATE;
+  }
+  if (c.bs == 3) return collect_impl<3>(c, nrhs, rec, nrec, seed);
+  return collect_impl<1>(c, nrhs, rec, nrec, seed);
+}
+
 int solve_mixed(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info) {
-  if (prm.precond == PZG_PRECOND_LEARNED) {
-    set_error("pzg_solve: learned preconditioner not trained in this context");
+  if (prm.precond == PZG_PRECOND_LEARNED && !c.nn_ready) {
+    set_error("pzg_solve: learned preconditioner not set up in this context");
     return PZG_ESTATE;
   }
   if (c.bs == 3) return mixed_impl<3>(c, prm, info);
