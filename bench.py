@@ -36,6 +36,9 @@ def parse():
     ap.add_argument("--no-baseline", action="store_true", help="skip the FP64 Jacobi-PCG run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle sample")
     ap.add_argument("--precond", default="block_jacobi", choices=["block_jacobi", "learned"])
+    ap.add_argument("--refine", type=int, default=0, help="0 restart FCG per refinement, 1 reliable update")
+    ap.add_argument("--inner-tol", type=float, default=1e-4)
+    ap.add_argument("--no-graph", action="store_true", help="plain launches instead of hipGraph replay")
     return ap.parse_args()
 
 
@@ -103,8 +106,10 @@ def main():
             st.SetCG(200000, TPZStepSolver.jacobi(), args.tol, 0)
         else:
             pre = L.PRECOND_LEARNED if args.precond == "learned" else L.PRECOND_BLOCK_JACOBI
-            st.SetMixedFCG(200000, args.tol, precond=pre, inner_tol=0.1)
+            st.SetMixedFCG(200000, args.tol, precond=pre, inner_tol=args.inner_tol,
+                           refine=args.refine)
         st.check_every = 100
+        st.use_graph = not args.no_graph
         return st
 
     def step(st):

@@ -123,6 +123,8 @@ typedef struct {
   int32_t max_outer;     /* mixed: refinement steps allowed                    */
   int32_t check_every;   /* iterations per device batch between host checks   */
   int32_t use_graph;     /* capture the iteration batch in a hipGraph          */
+  int32_t refine;        /* mixed: 0 = restart FCG from each true residual,
+                                   1 = reliable update (keep the direction)    */
 } pzg_solve_params;
 
 typedef struct {

@@ -50,7 +50,7 @@ class SolveParams(C.Structure):
     _fields_ = [
         ("solver", C.c_int32), ("precond", C.c_int32), ("max_iter", C.c_int64),
         ("tol", C.c_double), ("inner_tol", C.c_double), ("max_outer", C.c_int32),
-        ("check_every", C.c_int32), ("use_graph", C.c_int32),
+        ("check_every", C.c_int32), ("use_graph", C.c_int32), ("refine", C.c_int32),
     ]
 
 

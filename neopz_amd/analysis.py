@@ -144,7 +144,8 @@ class TPZStepSolver:
         self.tol = 0.0
         self.pre = None
         self.precond = L.PRECOND_BLOCK_JACOBI
-        self.inner_tol = 0.1
+        self.inner_tol = 1e-4
+        self.refine = 0
         self.max_outer = 200
         self.check_every = 50
         self.use_graph = True
@@ -177,12 +178,14 @@ class TPZStepSolver:
         self.tol = float(tol)
         self.pre = pre
 
-    def SetMixedFCG(self, numiterations, tol, precond=L.PRECOND_BLOCK_JACOBI, inner_tol=0.1,
-                    max_outer=200):
-        """FP32 flexible CG with FP64 reliable updates; stops on the true residual.
-        inner_tol is the reduction of the FP32 recursive residual between two
-        true-residual updates."""
+    def SetMixedFCG(self, numiterations, tol, precond=L.PRECOND_BLOCK_JACOBI, inner_tol=1e-4,
+                    max_outer=200, refine=0):
+        """FP32 flexible CG inside FP64 iterative refinement; stops on the true
+        residual.  inner_tol is the reduction of the FP32 recursive residual
+        between two true-residual evaluations; refine 0 restarts FCG from each
+        true residual, 1 keeps the search direction (reliable update)."""
         self.kind = self.EMixedFCG
+        self.refine = refine
         self.maxit = int(numiterations)
         self.tol = float(tol)
         self.precond = precond
@@ -199,6 +202,7 @@ class TPZStepSolver:
         p.max_outer = self.max_outer
         p.check_every = self.check_every
         p.use_graph = 1 if self.use_graph else 0
+        p.refine = self.refine
         return p
 
     def Solve(self, F=None, want_solution=True):

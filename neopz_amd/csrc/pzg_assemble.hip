@@ -605,8 +605,7 @@ __global__ void k_dinvsqrt(int64_t n, const double *__restrict__ d, double *__re
   if (i < n) s[i] = 1.0 / sqrt(fabs(d[i]));
 }
 
-// Block values are stored component-major: entry (k,l) of block t at
-// bsr[(k*BS+l)*nbf + t], so that lanes reading consecutive blocks coalesce.
+// Block values are placed by bsr_index (block-major or component-major).
 template <int BS>
 __global__ void k_bsr_diag(int64_t nb, int64_t nbf, const int64_t *__restrict__ ia,
                            const double *__restrict__ a, const int32_t *__restrict__ nlow,
@@ -623,7 +622,7 @@ __global__ void k_bsr_diag(int64_t nb, int64_t nbf, const int64_t *__restrict__ 
     }
   const int64_t t = fbia[A] + nlow[A];
   for (int k = 0; k < BS; k++)
-    for (int l = 0; l < BS; l++) bsr[(k * BS + l) * nbf + t] = (float)m[k][l];
+    for (int l = 0; l < BS; l++) bsr[bsr_index(t, k * BS + l, nbf, BS * BS)] = (float)m[k][l];
   float *inv = binv + A * BS * BS;
   if (BS == 1) {
     inv[0] = (float)(1.0 / m[0][0]);
@@ -666,8 +665,8 @@ __global__ void k_bsr_off(int64_t nb, int64_t nbf, const int64_t *__restrict__ i
     int k = (int)((e % BB) / BS), l = (int)(e % BS);
     int64_t B = ubja[u0 + t];
     double v = a[ia[A * BS + k] + BS + t * BS - k + l] * dis[A * BS + k] * dis[B * BS + l];
-    bsr[(k * BS + l) * nbf + fbia[A] + rfwd[u0 + t]] = (float)v;
-    bsr[(l * BS + k) * nbf + fbia[B] + rbwd[u0 + t]] = (float)v;
+    bsr[bsr_index(fbia[A] + rfwd[u0 + t], k * BS + l, nbf, BB)] = (float)v;
+    bsr[bsr_index(fbia[B] + rbwd[u0 + t], l * BS + k, nbf, BB)] = (float)v;
   }
 }
 

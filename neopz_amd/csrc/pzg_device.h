@@ -59,4 +59,24 @@ __device__ inline bool reduce_last(const double (&v)[NV], double *partials, unsi
   return true;
 }
 
+// Stores this block's sums of v into partials[k*MAXBLK + blockIdx.x] (plain
+// stores: a following kernel reads them, ordered by the kernel boundary).
+template <int NV>
+__device__ inline void block_partials(const double (&v)[NV], double *partials) {
+  __shared__ double sm[RED_TPB / 64];
+  for (int k = 0; k < NV; k++) {
+    double s = block_sum(v[k], sm);
+    if (threadIdx.x == 0) partials[k * MAXBLK + blockIdx.x] = s;
+  }
+}
+
+// Fixed-order sum of nblk partials of one slot, run by a single block;
+// result valid in thread 0.
+__device__ inline double sum_partials(const double *partials, int slot, int nblk) {
+  __shared__ double sm[RED_TPB / 64];
+  double s = 0.;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += partials[slot * MAXBLK + b];
+  return block_sum(s, sm);
+}
+
 }  // namespace pzg

@@ -42,6 +42,21 @@ template <class T> int dev_upload(DevBuf &b, const T *h, size_t n, hipStream_t s
   return PZG_OK;
 }
 
+// Value layout of the FP32 block CSR: 1 = component-major ((k*BS+l)*nbf + t),
+// 0 = block-major (t*BS*BS + k*BS + l).
+#ifndef PZG_BSR_SOA
+#define PZG_BSR_SOA 0
+#endif
+__host__ __device__ inline int64_t bsr_index(int64_t t, int c, int64_t nbf, int bb) {
+#if PZG_BSR_SOA
+  (void)bb;
+  return (int64_t)c * nbf + t;
+#else
+  (void)nbf;
+  return t * bb + c;
+#endif
+}
+
 // Learned preconditioner network: hidden features, message-passing layers,
 // leaky-ReLU slope.
 constexpr int NNF = 32;
@@ -70,7 +85,10 @@ struct SolveScalars {
   double rlast;   // mixed: scaled residual norm at the last reliable update
   double delta;   // mixed: reduction of the recursive residual that triggers an update
   int64_t iter, maxit;
-  int32_t done, pad;  // done: 1 tol, 2 maxit, 3 divergence, 4 reliable update due
+  // done: 0 running, 1 tol, 2 maxit, 3 divergence, 4 update due (set by the
+  // iteration's last reduction; its completion kernels still run), 5 update
+  // due (completion kernels already ran)
+  int32_t done, pad;
 };
 
 struct Ctx {

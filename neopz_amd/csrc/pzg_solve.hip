@@ -43,16 +43,41 @@ static int grid_for(int64_t n, int per_block) {
 }
 
 // ----------------------------------------------------------------------------
-// FP64 full CSR SpMV; VL lanes per row.
+#ifndef PZG_NT_LOADS
+#define PZG_NT_LOADS 0
+#endif
+template <class T> __device__ inline T ld_stream0(const T *p) {
+#if PZG_NT_LOADS
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+// FP64 full CSR SpMV; VL lanes per row, U entries per lane loaded before use
+// so that a whole row (up to VL*U entries) is in flight at once.
 template <int VL>
 __device__ inline double row_dot64(int64_t r, const int64_t *__restrict__ ia,
                                    const int32_t *__restrict__ ja, const double *__restrict__ a,
                                    const double *__restrict__ x, int sub) {
+  constexpr int U = 4;
   double s = 0.;
-  // the matrix is streamed once per product: non-temporal loads keep the
-  // gathered vector resident in L2
-  for (int64_t k = ia[r] + sub; k < ia[r + 1]; k += VL)
-    s += __builtin_nontemporal_load(&a[k]) * x[__builtin_nontemporal_load(&ja[k])];
+  const int64_t k1 = ia[r + 1];
+  for (int64_t base = ia[r] + sub; base < k1; base += VL * U) {
+    double av[U];
+    int32_t cv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t k = base + u * VL;
+      const bool ok = k < k1;
+      av[u] = ok ? ld_stream0(&a[k]) : 0.;
+      cv[u] = ok ? ld_stream0(&ja[k]) : 0;
+    }
+    double xv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) xv[u] = x[cv[u]];
+#pragma unroll
+    for (int u = 0; u < U; u++) s += av[u] * xv[u];
+  }
   for (int o = VL / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, VL);
   return s;
 }
@@ -92,10 +117,17 @@ __global__ __launch_bounds__(TPB) void k_spmv64_pq(int64_t n, const int64_t *__r
       acc += s * p[r];
     }
   }
-  double v[1] = {acc}, tot[1];
-  if (reduce_last<1>(v, partials, counter, tot) && threadIdx.x == 0) {
-    sc->pq = tot[0];
-    sc->alpha = sc->rho / tot[0];
+  double v[1] = {acc};
+  block_partials<1>(v, partials);
+}
+
+// alpha = rho / (p.q)   (one block)
+__global__ void k_fin_alpha64(SolveScalars *sc, const double *partials, int nblk) {
+  if (sc->done) return;
+  double pq = sum_partials(partials, 0, nblk);
+  if (threadIdx.x == 0) {
+    sc->pq = pq;
+    sc->alpha = sc->rho / pq;
   }
 }
 
@@ -135,18 +167,26 @@ __global__ __launch_bounds__(TPB) void k_cg_update64(int64_t n, double *__restri
     rr += ri * ri;
     rz += ri * zi;
   }
-  double v[2] = {rr, rz}, tot[2];
-  if (reduce_last<2>(v, partials, counter, tot) && threadIdx.x == 0) {
+  double v[2] = {rr, rz};
+  block_partials<2>(v, partials);
+}
+
+// resid = |r|/|b|; stop tests; beta = (r.z)/rho   (one block)
+__global__ void k_fin_cg64(SolveScalars *sc, const double *partials, int nblk, double *hist) {
+  if (sc->done) return;
+  double rr = sum_partials(partials, 0, nblk);
+  double rz = sum_partials(partials, 1, nblk);
+  if (threadIdx.x == 0) {
     int64_t it = sc->iter + 1;
     sc->iter = it;
-    double resid = sqrt(tot[0]) / sc->normb;
+    double resid = sqrt(rr) / sc->normb;
     sc->resid = resid;
     if (hist) hist[it - 1] = resid;
     if (resid <= sc->tol) sc->done = 1;
     else if (it >= sc->maxit) sc->done = 2;
-    sc->beta = tot[1] / sc->rho;
+    sc->beta = rz / sc->rho;
     sc->rho_old = sc->rho;
-    sc->rho = tot[1];
+    sc->rho = rz;
   }
 }
 
@@ -351,10 +391,13 @@ int solve_pcg64(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info) {
                                        c.d_fia.as<int64_t>(), c.d_fja.as<int32_t>(),
                                        c.d_fa.as<double>(), c.d_p.as<double>(),
                                        c.d_q.as<double>(), sc, part, cnt));
+    hipLaunchKernelGGL(k_fin_alpha64, dim3(1), dim3(TPB), 0, c.stream, sc, (const double *)part, gs);
     hipLaunchKernelGGL(k_cg_update64, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_u.as<double>(),
                        c.d_r.as<double>(), c.d_z.as<double>(), c.d_p.as<double>(),
                        c.d_q.as<double>(), c.d_diag.as<double>(), sc, (double *)nullptr, part,
                        cnt);
+    hipLaunchKernelGGL(k_fin_cg64, dim3(1), dim3(TPB), 0, c.stream, sc, (const double *)part, gv,
+                       (double *)nullptr);
   };
   int batch = prm.check_every > 0 ? prm.check_every : 50;
   PZG_TRY(run_batches(c, batch, prm.use_graph != 0, iter, hs));
@@ -375,22 +418,61 @@ int solve_pcg64(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info) {
 
 // ----------------------------------------------------------------------------
 // Mixed path: FP32 block CSR SpMV (block size BS, one wave per block row).
+#ifndef PZG_BSR_VEC
+#define PZG_BSR_VEC 0
+#endif
+template <class T> __device__ inline T ld_stream(const T *p) {
+#if PZG_NT_LOADS
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+
+template <int BS>
+__device__ inline void bsr_load(int64_t t, const float *__restrict__ v, int64_t nbf,
+                                float (&m)[BS * BS]) {
+#if PZG_BSR_SOA
+  for (int c = 0; c < BS * BS; c++) m[c] = ld_stream(&v[c * nbf + t]);
+#else
+  (void)nbf;
+  for (int c = 0; c < BS * BS; c++) m[c] = ld_stream(&v[t * (BS * BS) + c]);
+#endif
+}
+
+// One wavefront per block row; each lane takes U blocks per pass, all loads
+// of a pass issued before the products.
 template <int BS>
 __device__ inline void bsr_row(int64_t A, const int64_t *__restrict__ bia,
                                const int32_t *__restrict__ bja, const float *__restrict__ v,
                                int64_t nbf, const float *__restrict__ x, int lane,
                                float (&out)[BS]) {
+  constexpr int U = 2;
   float acc[BS];
   for (int k = 0; k < BS; k++) acc[k] = 0.f;
   const int64_t b0 = bia[A], b1 = bia[A + 1];
-  for (int64_t t = b0 + lane; t < b1; t += 64) {
-    const int64_t col = __builtin_nontemporal_load(&bja[t]);
-    float m[BS * BS];
-    for (int c = 0; c < BS * BS; c++) m[c] = __builtin_nontemporal_load(&v[c * nbf + t]);
-    float xv[BS];
-    for (int l = 0; l < BS; l++) xv[l] = x[col * BS + l];
-    for (int k = 0; k < BS; k++)
-      for (int l = 0; l < BS; l++) acc[k] += m[k * BS + l] * xv[l];
+  for (int64_t base = b0 + lane; base < b1; base += 64 * U) {
+    float m[U][BS * BS];
+    int64_t col[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t t = base + 64 * u;
+      if (t < b1) {
+        col[u] = ld_stream(&bja[t]);
+        bsr_load<BS>(t, v, nbf, m[u]);
+      } else {
+        col[u] = 0;
+        for (int c = 0; c < BS * BS; c++) m[u][c] = 0.f;
+      }
+    }
+    float xv[U][BS];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      for (int l = 0; l < BS; l++) xv[u][l] = x[col[u] * BS + l];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      for (int k = 0; k < BS; k++)
+        for (int l = 0; l < BS; l++) acc[k] += m[u][k * BS + l] * xv[u][l];
   }
   for (int k = 0; k < BS; k++) {
     float s = acc[k];
@@ -447,11 +529,26 @@ __global__ __launch_bounds__(TPB) void k_fcg_spmv(int64_t nb, const int64_t *__r
       pr += (double)p[i] * r[i];
     }
   }
-  double vv[2] = {pq, pr}, tot[2];
-  if (reduce_last<2>(vv, partials, counter, tot) && threadIdx.x == 0) {
-    sc->pq = tot[0];
-    sc->pr = tot[1];
-    sc->alpha = tot[1] / tot[0];
+  double vv[2] = {pq, pr};
+  block_partials<2>(vv, partials);
+}
+
+// alpha = (p.r)/(p.q)   (one block).  A pending update request (4) raised by
+// the previous iteration becomes 5 here, so that the completion kernels of that
+// iteration (prolongation, p update), which run on 4, do not run again.
+__global__ void k_fin_fcg_alpha(SolveScalars *sc, const double *partials, int nblk) {
+  if (sc->done == 4) {
+    __syncthreads();
+    if (threadIdx.x == 0) sc->done = 5;
+    return;
+  }
+  if (sc->done) return;
+  double pq = sum_partials(partials, 0, nblk);
+  double pr = sum_partials(partials, 1, nblk);
+  if (threadIdx.x == 0) {
+    sc->pq = pq;
+    sc->pr = pr;
+    sc->alpha = pr / pq;
   }
 }
 
@@ -486,18 +583,27 @@ __global__ __launch_bounds__(TPB) void k_fcg_update(int64_t nb, float *__restric
       zq += (double)s * q[A * BS + k];
     }
   }
-  double vv[2] = {rr, zq}, tot[2];
-  if (reduce_last<2>(vv, partials, counter, tot) && threadIdx.x == 0) {
+  double vv[2] = {rr, zq};
+  block_partials<2>(vv, partials);
+}
+
+// stop tests on |r|; beta = -(z.q)/(p.q)   (one block; slot 1 may have been
+// rewritten by the learned preconditioner's prolongation)
+__global__ void k_fin_fcg_beta(SolveScalars *sc, const double *partials, int nblk) {
+  if (sc->done) return;
+  double rr = sum_partials(partials, 0, nblk);
+  double zq = sum_partials(partials, 1, nblk);
+  if (threadIdx.x == 0) {
     int64_t it = sc->iter + 1;
     sc->iter = it;
-    double rn = sqrt(tot[0]);
+    double rn = sqrt(rr);
     double resid = rn / sc->normb;
     // divergence guard: leave the loop, the host restarts from the true residual
     if (!(resid == resid) || resid > 1e3) sc->done = 3;
     else if (rn < sc->delta * sc->rlast) sc->done = 4;
     else if (it >= sc->maxit) sc->done = 2;
     sc->resid = resid;
-    sc->beta = -tot[1] / sc->pq;
+    sc->beta = -zq / sc->pq;
   }
 }
 
@@ -682,7 +788,7 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
   };
   auto next_delta = [&]() {
     // reduction asked of the FP32 recursion before the next true residual
-    double d = prm.inner_tol > 0 ? prm.inner_tol : 0.1;
+    double d = prm.inner_tol > 0 ? prm.inner_tol : 1e-4;
     double need = 0.5 * prm.tol / rel;
     if (need > d) d = need;
     return d > 0.5 ? 0.5 : d;
@@ -717,6 +823,7 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
                          c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(), c.d_bsr.as<float>(),
                          c.nbf, c.d_ps.as<float>(), c.d_rs.as<float>(), c.d_qs.as<float>(), sc, part,
                          cnt);
+      hipLaunchKernelGGL(k_fin_fcg_alpha, dim3(1), dim3(TPB), 0, c.stream, sc, (const double *)part, gs);
       hipLaunchKernelGGL(k_fcg_update<BS>, dim3(gb), dim3(TPB), 0, c.stream, nb,
                          c.d_xs.as<float>(), c.d_rs.as<float>(), c.d_zs.as<float>(),
                          c.d_ps.as<float>(), c.d_qs.as<float>(), c.d_bjinv.as<float>(), sc, part,
@@ -724,6 +831,7 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
       if (learned)
         (void)nn_apply(c, c.d_rs.as<float>(), c.d_zs.as<float>(), nullptr, c.d_qs.as<float>(), sc,
                        part, cnt);
+      hipLaunchKernelGGL(k_fin_fcg_beta, dim3(1), dim3(TPB), 0, c.stream, sc, (const double *)part, gb);
       hipLaunchKernelGGL(k_p_update32, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_zs.as<float>(),
                          c.d_ps.as<float>(), sc);
     };
@@ -747,7 +855,7 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
       continue;
     }
     prev_rel = rel;
-    if (hs.done == 3) { restart = true; continue; }
+    if (hs.done == 3 || prm.refine == 0) { restart = true; continue; }
     hipLaunchKernelGGL(k_fcg_reset, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_r.as<double>(),
                        c.d_dinvsqrt.as<double>(), c.d_rs.as<float>(), next_delta(), sc, part, cnt);
     PZG_HIP(hipGetLastError());
@@ -808,9 +916,11 @@ static int collect_impl(Ctx &c, int nrhs, const int32_t *rec, int nrec, uint64_t
       hipLaunchKernelGGL(k_fcg_spmv<BS>, dim3(gs), dim3(TPB), 0, c.stream, nb, c.d_fbia.as<int64_t>(),
                          c.d_fbja.as<int32_t>(), c.d_bsr.as<float>(), c.nbf, c.d_ps.as<float>(),
                          c.d_rs.as<float>(), c.d_qs.as<float>(), sc, part, cnt);
+      hipLaunchKernelGGL(k_fin_fcg_alpha, dim3(1), dim3(TPB), 0, c.stream, sc, (const double *)part, gs);
       hipLaunchKernelGGL(k_fcg_update<BS>, dim3(gb), dim3(TPB), 0, c.stream, nb, c.d_xs.as<float>(),
                          c.d_rs.as<float>(), c.d_zs.as<float>(), c.d_ps.as<float>(), c.d_qs.as<float>(),
                          c.d_bjinv.as<float>(), sc, part, cnt);
+      hipLaunchKernelGGL(k_fin_fcg_beta, dim3(1), dim3(TPB), 0, c.stream, sc, (const double *)part, gb);
       hipLaunchKernelGGL(k_p_update32, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_zs.as<float>(),
                          c.d_ps.as<float>(), sc);
       for (int t = 0; t < nrec; t++)

@@ -129,7 +129,7 @@ def test_mixed_refinement(name):
     pb = CASES[name]()
     an = TPZLinearAnalysis(pb, False, ctx=ctx())
     st = TPZStepSolver()
-    st.SetMixedFCG(50000, 1e-8, inner_tol=0.1)
+    st.SetMixedFCG(50000, 1e-8, inner_tol=1e-4)
     an.SetSolver(st)
     an.Assemble()
     u = an.Solve()
