@@ -30,7 +30,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=64, help="hexahedra per direction")
+    ap.add_argument("--config", default="C5", choices=["C1", "C2", "C4p1", "C4p2", "C5"],
+                    help="workload (C5 = the headline 64^3 elasticity case)")
+    ap.add_argument("--n", type=int, default=0, help="override elements per direction")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-baseline", action="store_true", help="skip the FP64 Jacobi-PCG run")
@@ -42,21 +44,49 @@ def parse():
     return ap.parse_args()
 
 
+WORKLOADS = {
+    "C5": "3D heterogeneous elasticity {n}^3 hex H1 p=2, E=exp(GRF l=0.1 s=1), nu=0.3, clamped "
+          "base, body load",
+    "C2": "3D elasticity {n}^3 hex H1 p=2, E lognormal iid s=1, nu=0.3, clamped x=0, unit "
+          "traction x=1",
+    "C1": "2D Poisson {n}x{n} quads H1 p=2, 8 Gaussian bumps, homogeneous Dirichlet",
+    "C4p1": "2D Darcy {n}x{n} quads H1 p=1, kappa=exp(GRF l=0.05 s=1.5), f=1",
+    "C4p2": "2D Darcy {n}x{n} quads H1 p=2, kappa=exp(GRF l=0.05 s=1.5), f=1",
+}
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n_target, iters_target, seed):
-    """Times the CPU oracle (a scalar C port of the reference path, 1 thread)
-    on a bounded sample: full assembly + 10 Jacobi-PCG iterations of a 20^3
-    instance of the same workload, scaled per element and per stored entry to
-    the target size."""
+def cpu_baseline(pb, iters_target, seed, tol):
+    """Times the CPU oracle (a scalar C port of the reference path, 1 thread).
+    Small problems (<= 300k equations) run whole: assembly + Jacobi-PCG to the
+    same tolerance.  Larger ones run a bounded sample: full assembly + 10
+    Jacobi-PCG iterations of a 20^3 instance of the same elasticity workload,
+    scaled per element and per stored entry to the target size and to the
+    GPU FP64 PCG iteration count."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from neopz_amd import problems as P
+    if pb.n_equations() <= 300000:
+        o = O.Oracle(pb)
+        t0 = time.perf_counter()
+        ia, ja = o.pattern()
+        a, rhs = o.assemble()
+        t_asm = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        r = O.pcg(ia, ja, a, rhs, maxit=1000000, tol=tol, wasted_spmv=True)
+        t_sol = time.perf_counter() - t0
+        return {"value": t_asm + t_sol, "unit": "s", "cores": 1, "kind": "port",
+                "sample": f"oracle/pz_oracle.c, whole problem: pattern+assembly {t_asm:.2f}s, "
+                          f"Jacobi-PCG {r['iters']} iterations {t_sol:.2f}s (reference semantics incl. "
+                          f"the discarded residual SpMV)",
+                "assembly_s": t_asm, "iterations": r["iters"]}
     ns = 20
-    pb = P.elasticity3d(n=ns, p=2, seed=seed, field_kind="grf", corr_len=0.1, load="body")
-    o = O.Oracle(pb)
+    spb = P.elasticity3d(n=ns, p=2, seed=seed, field_kind="grf", corr_len=0.1,
+                         load="body" if pb.bc_type[4] >= 0 else "traction")
+    o = O.Oracle(spb)
     ia, ja = o.pattern()
     t0 = time.perf_counter()
     a, rhs = o.assemble()
@@ -66,8 +96,8 @@ def cpu_baseline(n_target, iters_target, seed):
     O.pcg(ia, ja, a, rhs, maxit=k, tol=0.0, wasted_spmv=True)
     t_it = (time.perf_counter() - t0) / k
     nel_s, nnz_s = ns ** 3, len(ja)
-    nel_t = n_target ** 3
-    nnz_t = O.nnz_closed_form(3, 2, 3, n_target, n_target, n_target)
+    nel_t = pb.nvol
+    nnz_t = O.nnz_closed_form(3, pb.p, pb.nstate, pb.nx, pb.ny, pb.nz)
     asm_t = t_asm / nel_s * nel_t
     it_t = t_it / nnz_s * nnz_t
     total = asm_t + it_t * iters_target
@@ -75,7 +105,7 @@ def cpu_baseline(n_target, iters_target, seed):
             "sample": f"oracle/pz_oracle.c on a {ns}^3 instance ({nel_s} hexes, {nnz_s} upper "
                       f"entries): assembly {t_asm:.2f}s, Jacobi-PCG {t_it * 1e3:.1f} ms/iteration "
                       f"(reference semantics incl. the discarded residual SpMV); scaled by elements "
-                      f"and stored entries to {n_target}^3 and x {iters_target} iterations",
+                      f"and stored entries to {pb.nx}^3 and x {iters_target} iterations",
             "assembly_s": asm_t, "s_per_iteration": it_t}
 
 
@@ -89,8 +119,9 @@ def main():
     from neopz_amd import problems as P
     from neopz_amd.analysis import Context, TPZSSpStructMatrixHIP, TPZStepSolver
 
-    n = args.n
-    pb = P.elasticity3d(n=n, p=2, seed=args.seed, field_kind="grf", corr_len=0.1, load="body")
+    over = {"n": args.n} if args.n else {}
+    pb = P.config(args.config, seed=args.seed, **over)
+    n = pb.nx
     t0 = time.perf_counter()
     md = M.build(pb)
     log(f"mesh built: neq={md['neq']} ({time.perf_counter() - t0:.1f}s)")
@@ -161,14 +192,15 @@ def main():
     cpu = None
     if not args.no_cpu:
         its = base["iterations"] if base else infos[-1]["iterations"]
-        cpu = cpu_baseline(n, its, args.seed)
+        cpu = cpu_baseline(pb, its, args.seed, args.tol)
     out = {
         "metric": "assembly+solve time to rel. residual 1e-8, learned-FCG vs Jacobi-PCG, 1 GPU",
         "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 inner / f64 refinement+assembly", "data": "synthetic",
-        "config": {"workload": f"3D heterogeneous elasticity {n}^3 hex H1 p=2, E=exp(GRF l=0.1 s=1), "
-                               f"nu=0.3, clamped base, body load; mixed FCG ({args.precond}) + FP64 IR",
+        "config": {"workload": WORKLOADS[args.config].format(n=n) +
+                               f"; mixed FCG ({args.precond}) + FP64 refinement",
+                   "name": args.config,
                    "neq": sizes["neq"], "nnz_upper": sizes["nnz_upper"],
                    "nnz_full": sizes["nnz_full"], "elements": sizes["nvol"], "seed": args.seed},
         "solve": infos[-1],
@@ -177,7 +209,7 @@ def main():
         "assembly_ms": asm_ms,
         "elements_per_s": sizes["nvol"] / (asm_ms * 1e-3),
         "spmv64_full_csr": {"ms": spmv64_ms, "GBps": spmv64_gbs},
-        "roofline": {"bound": "hbm", "kernel": "k_spmv_bsr<3> (FP32 block CSR SpMV)",
+        "roofline": {"bound": "hbm", "kernel": f"k_spmv_bsr<{sizes['bs']}> (FP32 block CSR SpMV)",
                      "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": achieved / 8000.0, "traffic": None},
         "cpu_baseline": cpu,

@@ -1,0 +1,216 @@
+// TPZHIP.h -- C++17 host-side mirror of NeoPZ's interface for the linear FEM
+// path, over the C ABI of include/pzg.h.
+//
+// The classes keep the reference's names, argument meaning and error
+// behaviour (a failing call prints to stderr and stops, as PZError +
+// DebugStop() do), so that code written against
+//   TPZSSpStructMatrix (StrMatrix/TPZSSpStructMatrix.h:17-38),
+//   TPZSYsmpMatrix     (Matrix/pzsysmp.h:20-230),
+//   TPZStepSolver      (Solvers/pzstepsolver.h:34-89),
+//   TPZLinearAnalysis  (Analysis/TPZLinearAnalysis.h:20-45)
+// reads the same.  NeoPZ's own containers are replaced by std::vector; the
+// mesh arrives flattened (pzg_mesh_desc) instead of as a TPZCompMesh.  The
+// subclasses that plug into NeoPZ itself are given in INTEGRATION.md.
+#pragma once
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <memory>
+#include <vector>
+
+#include "../../include/pzg.h"
+
+namespace pzhip {
+
+inline void DebugStop() { std::abort(); }
+
+inline void Check(int rc, const char *where) {
+  if (rc != PZG_OK) {
+    std::fprintf(stderr, "%s: pzg error %d: %s\n", where, rc, pzg_last_error());
+    DebugStop();
+  }
+}
+
+using Ctx = std::shared_ptr<pzg_ctx>;
+
+inline Ctx MakeContext(int device = 0) {
+  pzg_ctx *c = nullptr;
+  Check(pzg_create(device, &c), "pzg_create");
+  return Ctx(c, pzg_destroy);
+}
+
+// TPZSYsmpMatrix: symmetric CSR, upper triangle with the diagonal first.
+class TPZSYsmpMatrixHIP {
+ public:
+  TPZSYsmpMatrixHIP(Ctx ctx, int64_t neq) : fCtx(std::move(ctx)), fNeq(neq) {}
+  int64_t Rows() const { return fNeq; }
+  // TPZSYsmpMatrix::SetData (Matrix/pzsysmp.h:232-240)
+  void SetData(std::vector<int64_t> IA, std::vector<int64_t> JA, std::vector<double> A) {
+    fNeq = (int64_t)IA.size() - 1;
+    Check(pzg_csr_set(fCtx.get(), fNeq, IA.data(), JA.data(), A.data(), nullptr), "SetData");
+    fIA = std::move(IA);
+    fJA = std::move(JA);
+  }
+  const std::vector<int64_t> &IA() {
+    Pattern();
+    return fIA;
+  }
+  const std::vector<int64_t> &JA() {
+    Pattern();
+    return fJA;
+  }
+  std::vector<double> A() {
+    std::vector<double> a(JA().size());
+    Check(pzg_get_system(fCtx.get(), a.data(), nullptr), "A");
+    return a;
+  }
+  // TPZSYsmpMatrix::GetVal (Matrix/pzsysmp.cpp:59-77): lower entries read the transpose
+  double GetVal(int64_t r, int64_t c) {
+    if (r > c) std::swap(r, c);
+    const auto &ia = IA();
+    const auto &ja = JA();
+    for (int64_t k = ia[r]; k < ia[r + 1]; k++)
+      if (ja[k] == c) return A()[k];
+    return 0.;
+  }
+  // batched TPZSYsmpMatrix::AddKel (Matrix/pzsysmp.cpp:369-411)
+  void AddKel(int64_t nel, int32_t ne, const std::vector<int64_t> &dest,
+              const std::vector<double> &ek) {
+    Check(pzg_add_kel_batch(fCtx.get(), nel, ne, dest.data(), ek.data()), "AddKel");
+  }
+  // z = beta*y + alpha*K*x (TPZSYsmpMatrix::MultAdd, Matrix/pzsysmp.cpp:189-232)
+  void MultAdd(const std::vector<double> &x, const std::vector<double> &y, std::vector<double> &z,
+               double alpha, double beta) const {
+    z.resize(fNeq);
+    Check(pzg_multadd(fCtx.get(), x.data(), beta != 0. ? y.data() : nullptr, z.data(), alpha, beta),
+          "MultAdd");
+  }
+  pzg_ctx *Context() const { return fCtx.get(); }
+
+ private:
+  void Pattern() {
+    if (!fIA.empty()) return;
+    pzg_sizes s;
+    Check(pzg_get_sizes(fCtx.get(), &s), "sizes");
+    fIA.resize(fNeq + 1);
+    fJA.resize(s.nnz_upper);
+    Check(pzg_struct_get_pattern(fCtx.get(), fIA.data(), fJA.data()), "IA/JA");
+  }
+  Ctx fCtx;
+  int64_t fNeq;
+  std::vector<int64_t> fIA, fJA;
+};
+
+// TPZStepSolver: ECG with an EJacobi(1) preconditioner, or the mixed solver.
+class TPZStepSolver {
+ public:
+  enum MSolver { ENoSolver, EJacobi, ECG, EMixedFCG };
+  void SetJacobi(int64_t numiterations, double /*tol*/, int FromCurrent) {
+    if (numiterations != 1 || FromCurrent) DebugStop();
+    fSolver = EJacobi;
+  }
+  void SetCG(int64_t numiterations, const TPZStepSolver &pre, double tol, int FromCurrent) {
+    if (pre.fSolver != EJacobi || FromCurrent) DebugStop();
+    fSolver = ECG;
+    fMaxIterations = numiterations;
+    fTol = tol;
+  }
+  void SetMixedFCG(int64_t numiterations, double tol, int precond = PZG_PRECOND_BLOCK_JACOBI) {
+    fSolver = EMixedFCG;
+    fMaxIterations = numiterations;
+    fTol = tol;
+    fPrecond = precond;
+  }
+  void SetMatrix(TPZSYsmpMatrixHIP *K) { fMatrix = K; }
+  MSolver Solver() const { return fSolver; }
+  // TPZStepSolver::Solve (Solvers/pzstepsolver.cpp:65-122): records the
+  // iteration count and overwrites the tolerance with the achieved one.
+  void Solve(const std::vector<double> *F, std::vector<double> &result) {
+    if (!fMatrix || (fSolver != ECG && fSolver != EMixedFCG)) DebugStop();
+    pzg_solve_params p{};
+    p.solver = fSolver == ECG ? PZG_SOLVER_PCG64 : PZG_SOLVER_MIXED;
+    p.precond = fPrecond;
+    p.max_iter = fMaxIterations;
+    p.tol = fTol;
+    p.inner_tol = 1e-4;
+    p.max_outer = 200;
+    p.check_every = 50;
+    p.use_graph = 1;
+    p.refine = 0;
+    result.resize(fMatrix->Rows());
+    int rc = pzg_solve(fMatrix->Context(), &p, F ? F->data() : nullptr, result.data(), &fInfo);
+    if (rc != PZG_OK && rc != PZG_ENOCONV) Check(rc, "Solve");
+    fNumIterations = fInfo.iterations;
+    fTol = fInfo.achieved;
+  }
+  int64_t NumIterations() const { return fNumIterations; }
+  double GetTolerance() const { return fTol; }
+  const pzg_solve_info &Info() const { return fInfo; }
+
+ private:
+  MSolver fSolver = ENoSolver;
+  int64_t fMaxIterations = 0, fNumIterations = 0;
+  double fTol = 0.;
+  int fPrecond = PZG_PRECOND_BLOCK_JACOBI;
+  TPZSYsmpMatrixHIP *fMatrix = nullptr;
+  pzg_solve_info fInfo{};
+};
+
+// TPZSSpStructMatrix over the GPU: Create() -> pattern, Assemble() -> values + rhs.
+class TPZSSpStructMatrixHIP {
+ public:
+  TPZSSpStructMatrixHIP(Ctx ctx, const pzg_mesh_desc &mesh) : fCtx(std::move(ctx)), fMesh(mesh) {}
+  // TPZSSpStructMatrix::Create (StrMatrix/TPZSSpStructMatrix.cpp:30-47)
+  std::unique_ptr<TPZSYsmpMatrixHIP> Create() {
+    if (!fFlattened) {
+      Check(pzg_mesh_set(fCtx.get(), &fMesh), "mesh");
+      fFlattened = true;
+    }
+    int64_t neq = 0, nnz = 0;
+    Check(pzg_struct_create(fCtx.get(), &neq, &nnz), "Create");
+    return std::make_unique<TPZSYsmpMatrixHIP>(fCtx, neq);
+  }
+  // TPZStructMatrixOR::Assemble(K, rhs) (StrMatrix/pzstrmatrixor.cpp:41-71)
+  void Assemble(TPZSYsmpMatrixHIP & /*K*/, std::vector<double> &rhs) {
+    Check(pzg_assemble(fCtx.get()), "Assemble");
+    rhs.resize(fMesh.neq);
+    Check(pzg_get_system(fCtx.get(), nullptr, rhs.data()), "rhs");
+  }
+
+ private:
+  Ctx fCtx;
+  pzg_mesh_desc fMesh;
+  bool fFlattened = false;
+};
+
+// TPZLinearAnalysis(cmesh, mustOptimizeBandwidth=false): Assemble()/Solve().
+class TPZLinearAnalysis {
+ public:
+  TPZLinearAnalysis(Ctx ctx, const pzg_mesh_desc &mesh, bool mustOptimizeBandwidth = false)
+      : fCtx(std::move(ctx)), fStr(fCtx, mesh) {
+    if (mustOptimizeBandwidth) DebugStop();  // first-touch numbering only
+  }
+  void SetSolver(const TPZStepSolver &s) { fSolver = s; }
+  // Analysis/TPZLinearAnalysis.cpp:42-94
+  void Assemble() {
+    if (!fK) fK = fStr.Create();
+    fStr.Assemble(*fK, fRhs);
+    fSolver.SetMatrix(fK.get());
+  }
+  // Analysis/TPZLinearAnalysis.cpp:130-227
+  void Solve() { fSolver.Solve(&fRhs, fSolution); }
+  const std::vector<double> &Solution() const { return fSolution; }
+  const std::vector<double> &Rhs() const { return fRhs; }
+  TPZSYsmpMatrixHIP &Matrix() { return *fK; }
+  TPZStepSolver &Solver() { return fSolver; }
+
+ private:
+  Ctx fCtx;
+  TPZSSpStructMatrixHIP fStr;
+  std::unique_ptr<TPZSYsmpMatrixHIP> fK;
+  TPZStepSolver fSolver;
+  std::vector<double> fRhs, fSolution;
+};
+
+}  // namespace pzhip

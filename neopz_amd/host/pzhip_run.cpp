@@ -1,0 +1,165 @@
+// pzhip_run -- the reference's analysis sequence, written against the C++
+// mirror in TPZHIP.h:
+//
+//   TPZLinearAnalysis an(cmesh, false);
+//   an.SetStructuralMatrix(TPZSSpStructMatrix);     // here: the HIP one
+//   TPZStepSolver jac; jac.SetJacobi(1, 0., 0);
+//   TPZStepSolver cg;  cg.SetCG(maxit, jac, tol, 0);
+//   an.SetSolver(cg); an.Assemble(); an.Solve();
+//
+// (the CG set-up of PerfTests/SubStruct/substruct.cpp:189-203; the
+// struct-matrix use of UnitTest_PZ/TestStruct/StructMatrixUnitTest.cpp:100).
+// The mesh comes from a flat file written by neopz_amd.mesh.save; the results
+// are written as one flat file: neq, nnz, IA, JA, A, rhs, solution
+// (int64/double arrays, each with its count), then iterations and the
+// achieved tolerance.
+//
+// usage: pzhip_run MESH OUT [cg|mixed] [tol] [maxit]
+#include <cstring>
+#include <string>
+
+#include "TPZHIP.h"
+
+namespace {
+
+struct Reader {
+  FILE *f;
+  template <class T>
+  T scalar() {
+    T v;
+    if (std::fread(&v, sizeof(T), 1, f) != 1) fail();
+    return v;
+  }
+  template <class T>
+  std::vector<T> array() {
+    int64_t n = scalar<int64_t>();
+    if (n < 0 || n > (int64_t(1) << 34)) fail();
+    std::vector<T> v(n);
+    if (n && std::fread(v.data(), sizeof(T), n, f) != size_t(n)) fail();
+    return v;
+  }
+  [[noreturn]] static void fail() {
+    std::fprintf(stderr, "pzhip_run: truncated or malformed mesh file\n");
+    std::exit(2);
+  }
+};
+
+template <class T>
+void put(FILE *f, const std::vector<T> &v) {
+  int64_t n = int64_t(v.size());
+  std::fwrite(&n, sizeof n, 1, f);
+  std::fwrite(v.data(), sizeof(T), v.size(), f);
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  if (argc < 3) {
+    std::fprintf(stderr, "usage: %s MESH OUT [cg|mixed] [tol] [maxit]\n", argv[0]);
+    return 2;
+  }
+  const std::string method = argc > 3 ? argv[3] : "cg";
+  const double tol = argc > 4 ? std::atof(argv[4]) : 1e-10;
+  const int64_t maxit = argc > 5 ? std::atoll(argv[5]) : 100000;
+
+  FILE *f = std::fopen(argv[1], "rb");
+  if (!f) {
+    std::fprintf(stderr, "pzhip_run: cannot open %s\n", argv[1]);
+    return 2;
+  }
+  char magic[8];
+  if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, "PZGMESH1", 8) != 0) Reader::fail();
+  Reader r{f};
+  pzg_mesh_desc m{};
+  m.dim = r.scalar<int32_t>();
+  m.material = r.scalar<int32_t>();
+  m.nstate = r.scalar<int32_t>();
+  m.porder = r.scalar<int32_t>();
+  auto xyz = r.array<double>();
+  auto vol_nodes = r.array<int64_t>();
+  auto vol_conn = r.array<int64_t>();
+  auto vol_coef = r.array<double>();
+  auto bc_nodes = r.array<int64_t>();
+  auto bc_conn = r.array<int64_t>();
+  auto bc_id = r.array<int32_t>();
+  auto bc_type = r.array<int32_t>();
+  auto bc_val = r.array<double>();
+  auto blk_pos = r.array<int64_t>();
+  auto blk_size = r.array<int32_t>();
+  m.neq = r.scalar<int64_t>();
+  m.scale = r.scalar<double>();
+  m.E = r.scalar<double>();
+  m.nu = r.scalar<double>();
+  m.rule_order_vol = r.scalar<int32_t>();
+  m.rule_order_bc = r.scalar<int32_t>();
+  m.force_kind = r.scalar<int32_t>();
+  auto force_const = r.array<double>();
+  auto bumps = r.array<double>();
+  m.bump_width = r.scalar<double>();
+  m.big_number = r.scalar<double>();
+  std::fclose(f);
+
+  const int ncorner = m.dim == 2 ? 4 : 8, bcorner = m.dim == 2 ? 2 : 4;
+  if (force_const.size() != 3 || vol_nodes.size() % ncorner || bc_nodes.size() % bcorner ||
+      bumps.size() % 6 || blk_pos.size() != blk_size.size())
+    Reader::fail();
+  m.nnodes = int64_t(xyz.size() / 3);
+  m.xyz = xyz.data();
+  m.nvol = int64_t(vol_nodes.size()) / ncorner;
+  m.vol_nodes = vol_nodes.data();
+  m.vol_conn = vol_conn.data();
+  m.vol_coef = vol_coef.empty() ? nullptr : vol_coef.data();
+  m.nbc = int64_t(bc_nodes.size()) / bcorner;
+  m.bc_nodes = bc_nodes.data();
+  m.bc_conn = bc_conn.data();
+  m.bc_id = bc_id.data();
+  m.nbcid = int32_t(bc_type.size());
+  m.bc_type = bc_type.data();
+  m.bc_val = bc_val.data();
+  m.nconn = int64_t(blk_pos.size());
+  m.blk_pos = blk_pos.data();
+  m.blk_size = blk_size.data();
+  for (int k = 0; k < 3; k++) m.force_const[k] = force_const[k];
+  m.nbumps = int32_t(bumps.size() / 6);
+  m.bumps = bumps.empty() ? nullptr : bumps.data();
+
+  using namespace pzhip;
+  TPZLinearAnalysis an(MakeContext(0), m, /*mustOptimizeBandwidth=*/false);
+  TPZStepSolver step;
+  if (method == "cg") {
+    TPZStepSolver jac;
+    jac.SetJacobi(1, 0., 0);
+    step.SetCG(maxit, jac, tol, 0);
+  } else if (method == "mixed") {
+    step.SetMixedFCG(maxit, tol);
+  } else {
+    std::fprintf(stderr, "pzhip_run: unknown method %s\n", method.c_str());
+    return 2;
+  }
+  an.SetSolver(step);
+  an.Assemble();
+  an.Solve();
+
+  FILE *o = std::fopen(argv[2], "wb");
+  if (!o) {
+    std::fprintf(stderr, "pzhip_run: cannot write %s\n", argv[2]);
+    return 2;
+  }
+  auto &K = an.Matrix();
+  const int64_t neq = K.Rows(), nnz = int64_t(K.JA().size());
+  std::fwrite(&neq, sizeof neq, 1, o);
+  std::fwrite(&nnz, sizeof nnz, 1, o);
+  put(o, K.IA());
+  put(o, K.JA());
+  put(o, K.A());
+  put(o, an.Rhs());
+  put(o, an.Solution());
+  const int64_t its = an.Solver().NumIterations();
+  const double achieved = an.Solver().GetTolerance();
+  std::fwrite(&its, sizeof its, 1, o);
+  std::fwrite(&achieved, sizeof achieved, 1, o);
+  std::fclose(o);
+  std::printf("pzhip_run: neq %lld nnz %lld iterations %lld achieved %.3e\n", (long long)neq,
+              (long long)nnz, (long long)its, achieved);
+  return 0;
+}

@@ -204,3 +204,50 @@ def _build3d(pb):
         bc_id = np.zeros(0, np.int32)
     return _common(pb, econn.astype(np.int64), bc_conn.astype(np.int64), nconn, corner, xyz,
                    vol_nodes.astype(np.int64), bc_nodes, bc_id)
+
+
+# Flat little-endian file of a mesh description, read by the C++ host mirror
+# (neopz_amd/host/pzhip_run.cpp).  Fields in this fixed order; arrays are
+# preceded by their element count (int64).
+_FILE_MAGIC = b"PZGMESH1"
+
+
+def save(m, path):
+    def i32(f, v):
+        f.write(np.int32(v).tobytes())
+
+    def i64(f, v):
+        f.write(np.int64(v).tobytes())
+
+    def f64(f, v):
+        f.write(np.float64(v).tobytes())
+
+    def arr(f, x, dt):
+        a = np.zeros(0, dt) if x is None else np.ascontiguousarray(x, dtype=dt).ravel()
+        i64(f, a.size)
+        f.write(a.tobytes())
+
+    with open(path, "wb") as f:
+        f.write(_FILE_MAGIC)
+        for k in ("dim", "material", "nstate", "porder"):
+            i32(f, m[k])
+        arr(f, m["xyz"], np.float64)
+        arr(f, m["vol_nodes"], np.int64)
+        arr(f, m["vol_conn"], np.int64)
+        arr(f, m["vol_coef"], np.float64)
+        arr(f, m["bc_nodes"], np.int64)
+        arr(f, m["bc_conn"], np.int64)
+        arr(f, m["bc_id"], np.int32)
+        arr(f, m["bc_type"], np.int32)
+        arr(f, m["bc_val"], np.float64)
+        arr(f, m["blk_pos"], np.int64)
+        arr(f, m["blk_size"], np.int32)
+        i64(f, m["neq"])
+        for k in ("scale", "E", "nu"):
+            f64(f, m[k])
+        for k in ("rule_order_vol", "rule_order_bc", "force_kind"):
+            i32(f, m[k])
+        arr(f, m["force_const"], np.float64)
+        arr(f, m["bumps"], np.float64)
+        for k in ("bump_width", "big_number"):
+            f64(f, m[k])

@@ -1,0 +1,76 @@
+"""The C++ host mirror (neopz_amd/host/TPZHIP.h) driven end to end.
+
+``pzhip_run`` runs the reference's analysis sequence (TPZLinearAnalysis with a
+TPZSSpStructMatrix and a Jacobi-preconditioned TPZStepSolver CG) in its own
+process over the C ABI.  Its pattern must equal the oracle's bit for bit, its
+entries and rhs must meet the 1e-12 relative bar, and its CG solution must be
+within 1e-8 of the oracle's CG solution (both converged to 1e-13).
+"""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as O
+from neopz_amd import mesh as M
+from neopz_amd import problems as P
+
+RUN = Path(__file__).resolve().parents[1] / "neopz_amd" / "host" / "pzhip_run"
+
+
+def _read_result(path):
+    buf = path.read_bytes()
+    off = 0
+
+    def take(dt, n):
+        nonlocal off
+        a = np.frombuffer(buf, dt, n, off)
+        off += a.nbytes
+        return a
+
+    neq, nnz = take(np.int64, 2)
+    out = {"neq": int(neq), "nnz": int(nnz)}
+    for name, dt in (("ia", np.int64), ("ja", np.int64), ("a", np.float64), ("rhs", np.float64),
+                     ("u", np.float64)):
+        n = int(take(np.int64, 1)[0])
+        out[name] = take(dt, n)
+    out["iterations"] = int(take(np.int64, 1)[0])
+    out["achieved"] = float(take(np.float64, 1)[0])
+    assert off == len(buf)
+    return out
+
+
+def test_runner_rejects_malformed_mesh(tmp_path):
+    assert RUN.exists(), "build the library first (neopz_amd/csrc make)"
+    bad = tmp_path / "bad.bin"
+    bad.write_bytes(b"PZGMESH1" + b"\0" * 6)
+    r = subprocess.run([str(RUN), str(bad), str(tmp_path / "o.bin")], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 2 and "malformed" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,pb", [
+    ("poisson2d_p2", lambda: P.poisson2d(n=10, p=2, seed=1)),
+    ("elast3d_p2_body", lambda: P.elasticity3d(n=4, p=2, seed=6, field_kind="grf", load="body")),
+])
+def test_cpp_analysis_matches_oracle(tmp_path, name, pb):
+    pb = pb()
+    mfile, ofile = tmp_path / "mesh.bin", tmp_path / "out.bin"
+    M.save(M.build(pb), mfile)
+    print(f"[gpu] pzhip_run {name}", flush=True)
+    r = subprocess.run([str(RUN), str(mfile), str(ofile), "cg", "1e-13", "20000"],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    res = _read_result(ofile)
+    o = O.Oracle(pb)
+    ia, ja = o.pattern()
+    a, rhs = o.assemble()
+    assert np.array_equal(res["ia"], ia) and np.array_equal(res["ja"], ja)
+    assert np.max(np.abs(res["a"] - a) - 1e-12 * np.abs(a)) <= 1e-15 * np.max(np.abs(a))
+    assert np.max(np.abs(res["rhs"] - rhs)) <= 1e-12 * np.max(np.abs(rhs))
+    ref = O.pcg(ia, ja, a, rhs, maxit=20000, tol=1e-13, wasted_spmv=False)
+    assert ref["converged"] and res["achieved"] <= 1e-13
+    assert np.linalg.norm(res["u"] - ref["x"]) <= 1e-8 * np.linalg.norm(ref["x"])
