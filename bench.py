@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--refine", type=int, default=0, help="0 restart FCG per refinement, 1 reliable update")
     ap.add_argument("--inner-tol", type=float, default=1e-4)
     ap.add_argument("--no-graph", action="store_true", help="plain launches instead of hipGraph replay")
+    ap.add_argument("--inner-op", default="auto", choices=["auto", "bsr", "matfree"],
+                    help="FP32 operator of the inner iterations")
     return ap.parse_args()
 
 
@@ -137,8 +139,9 @@ def main():
             st.SetCG(200000, TPZStepSolver.jacobi(), args.tol, 0)
         else:
             pre = L.PRECOND_LEARNED if args.precond == "learned" else L.PRECOND_BLOCK_JACOBI
+            op = {"auto": L.INNER_AUTO, "bsr": L.INNER_BSR, "matfree": L.INNER_MATFREE}[args.inner_op]
             st.SetMixedFCG(200000, args.tol, precond=pre, inner_tol=args.inner_tol,
-                           refine=args.refine)
+                           refine=args.refine, inner_op=op)
         st.check_every = 100
         st.use_graph = not args.no_graph
         return st
@@ -188,6 +191,8 @@ def main():
     spmv64_gbs = nbytes.value / (spmv64_ms * 1e-3) / 1e9
     L.check(ctx.lib.pzg_time_kernel(ctx.h, 2, 3, C.byref(avg), C.byref(nbytes)))
     asm_ms = avg.value
+    L.check(ctx.lib.pzg_time_kernel(ctx.h, 3, 20, C.byref(avg), C.byref(nbytes)))
+    mf_ms, mf_gbs = avg.value, nbytes.value / (avg.value * 1e-3) / 1e9
     sizes = ctx.sizes()
     cpu = None
     if not args.no_cpu:
@@ -199,7 +204,8 @@ def main():
         "ms_per_step": ms_per_step, "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 inner / f64 refinement+assembly", "data": "synthetic",
         "config": {"workload": WORKLOADS[args.config].format(n=n) +
-                               f"; mixed FCG ({args.precond}) + FP64 refinement",
+                               f"; mixed FCG ({args.precond}, {args.inner_op} inner operator) + "
+                               f"FP64 refinement",
                    "name": args.config,
                    "neq": sizes["neq"], "nnz_upper": sizes["nnz_upper"],
                    "nnz_full": sizes["nnz_full"], "elements": sizes["nvol"], "seed": args.seed},
@@ -209,6 +215,8 @@ def main():
         "assembly_ms": asm_ms,
         "elements_per_s": sizes["nvol"] / (asm_ms * 1e-3),
         "spmv64_full_csr": {"ms": spmv64_ms, "GBps": spmv64_gbs},
+        "spmv32_bsr": {"ms": spmv_ms, "GBps": achieved},
+        "matfree_operator": {"ms": mf_ms, "GBps_compulsory": mf_gbs},
         "roofline": {"bound": "hbm", "kernel": f"k_spmv_bsr<{sizes['bs']}> (FP32 block CSR SpMV)",
                      "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": achieved / 8000.0, "traffic": None},

@@ -125,7 +125,15 @@ typedef struct {
   int32_t use_graph;     /* capture the iteration batch in a hipGraph          */
   int32_t refine;        /* mixed: 0 = restart FCG from each true residual,
                                    1 = reliable update (keep the direction)    */
+  int32_t inner_op;      /* mixed: PZG_INNER_* operator of the FP32 iterations */
 } pzg_solve_params;
+
+/* Operator of the mixed solver's FP32 iterations.  Both approximate the
+ * scaled S = D^-1/2 K D^-1/2 in FP32; the FP64 true residual uses the
+ * assembled CSR either way. */
+#define PZG_INNER_AUTO 0     /* matrix-free when the system is mesh-built, else BSR */
+#define PZG_INNER_BSR 1      /* stored FP32 3x3 (or 1x1) block CSR of S             */
+#define PZG_INNER_MATFREE 2  /* element-by-element FP32 application from geometry   */
 
 typedef struct {
   int64_t iterations;     /* CG iterations (inner total for the mixed solver)  */
@@ -201,8 +209,15 @@ int pzg_true_residual(pzg_ctx *ctx, double *rel);
 
 /* Average device duration (HIP events on the context stream) of nrep launches
  * of the SpMV kernel of the given format: 0 = FP64 full CSR (plain PCG path),
- * 1 = FP32 block CSR of the mixed path, 2 = element assembly (whole pass). */
+ * 1 = FP32 block CSR of the mixed path, 2 = element assembly (whole pass),
+ * 3 = matrix-free FP32 operator (element, boundary and row-sum kernels). */
 int pzg_time_kernel(pzg_ctx *ctx, int32_t which, int32_t nrep, double *avg_ms, double *bytes);
+
+/* y = S x with S = D^-1/2 K D^-1/2 applied by the FP32 inner operator op
+ * (PZG_INNER_BSR or PZG_INNER_MATFREE) on host vectors of length neq; D is
+ * the diagonal of the assembled K.  Exposes the operator of the mixed
+ * solver's inner iterations, which has no reference counterpart. */
+int pzg_scaled_apply(pzg_ctx *ctx, int32_t op, const double *x, double *y);
 
 /* Sizes of the device formats (for roofline accounting). */
 typedef struct {

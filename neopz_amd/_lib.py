@@ -14,6 +14,7 @@ LIBPATH = os.path.join(_HERE, "libpzg.so")
 PZG_OK, PZG_EINVAL, PZG_EHIP, PZG_ENOCONV, PZG_ESTATE, PZG_ENOMEM, PZG_EUNSUP = range(7)
 SOLVER_PCG64, SOLVER_MIXED = 0, 1
 PRECOND_JACOBI, PRECOND_BLOCK_JACOBI, PRECOND_LEARNED = 0, 1, 2
+INNER_AUTO, INNER_BSR, INNER_MATFREE = 0, 1, 2
 
 EXPORTS = [
     "pzg_last_error", "pzg_create", "pzg_destroy", "pzg_mesh_set", "pzg_struct_create",
@@ -21,7 +22,7 @@ EXPORTS = [
     "pzg_add_kel_batch", "pzg_multadd", "pzg_solve", "pzg_true_residual", "pzg_time_kernel",
     "pzg_get_sizes", "pzg_sync", "pzg_mesh_set_coef", "pzg_learned_setup", "pzg_learned_collect",
     "pzg_learned_train", "pzg_learned_weights", "pzg_learned_graph", "pzg_learned_samples",
-    "pzg_learned_loss_grad",
+    "pzg_learned_loss_grad", "pzg_scaled_apply",
 ]
 
 dp = C.POINTER(C.c_double)
@@ -51,6 +52,7 @@ class SolveParams(C.Structure):
         ("solver", C.c_int32), ("precond", C.c_int32), ("max_iter", C.c_int64),
         ("tol", C.c_double), ("inner_tol", C.c_double), ("max_outer", C.c_int32),
         ("check_every", C.c_int32), ("use_graph", C.c_int32), ("refine", C.c_int32),
+        ("inner_op", C.c_int32),
     ]
 
 
@@ -114,6 +116,7 @@ def load():
     L.pzg_learned_graph.argtypes = [vp, i64p, i64p, i32p, i32p, i64p]
     L.pzg_learned_samples.argtypes = [vp, fp, fp]
     L.pzg_learned_loss_grad.argtypes = [vp, C.c_int64, fp, fp, dp, fp]
+    L.pzg_scaled_apply.argtypes = [vp, C.c_int32, dp, dp]
     _lib = L
     return L
 

@@ -146,6 +146,7 @@ class TPZStepSolver:
         self.precond = L.PRECOND_BLOCK_JACOBI
         self.inner_tol = 1e-4
         self.refine = 0
+        self.inner_op = L.INNER_AUTO
         self.max_outer = 200
         self.check_every = 50
         self.use_graph = True
@@ -179,13 +180,17 @@ class TPZStepSolver:
         self.pre = pre
 
     def SetMixedFCG(self, numiterations, tol, precond=L.PRECOND_BLOCK_JACOBI, inner_tol=1e-4,
-                    max_outer=200, refine=0):
+                    max_outer=200, refine=0, inner_op=L.INNER_AUTO):
         """FP32 flexible CG inside FP64 iterative refinement; stops on the true
         residual.  inner_tol is the reduction of the FP32 recursive residual
         between two true-residual evaluations; refine 0 restarts FCG from each
-        true residual, 1 keeps the search direction (reliable update)."""
+        true residual, 1 keeps the search direction (reliable update);
+        inner_op selects the FP32 operator (INNER_BSR: stored block CSR,
+        INNER_MATFREE: element-by-element, INNER_AUTO: matrix-free for
+        mesh-built systems)."""
         self.kind = self.EMixedFCG
         self.refine = refine
+        self.inner_op = inner_op
         self.maxit = int(numiterations)
         self.tol = float(tol)
         self.precond = precond
@@ -203,6 +208,7 @@ class TPZStepSolver:
         p.check_every = self.check_every
         p.use_graph = 1 if self.use_graph else 0
         p.refine = self.refine
+        p.inner_op = self.inner_op
         return p
 
     def Solve(self, F=None, want_solution=True):

@@ -711,6 +711,7 @@ int launch_build_bsr(Ctx &c) {
   PZG_HIP(hipStreamSynchronize(c.stream));
   dev_free(nl);
   c.bsr_valid = true;
+  c.mf_vals_valid = false;  // D^-1/2 recomputed
   return PZG_OK;
 }
 

@@ -225,6 +225,7 @@ static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
   PZG_HIP(hipStreamSynchronize(s));
   c.has_mesh = true;
   c.has_struct = false;
+  c.mf_ready = c.mf_vals_valid = false;
   c.assembled = false;
   c.external_csr = false;
   c.has_solution = false;
@@ -380,6 +381,8 @@ static int struct_create(Ctx &c) {
   c.has_struct = true;
   c.external_csr = false;
   c.assembled = false;
+  c.kel_added = false;
+  c.mf_ready = c.mf_vals_valid = false;
   c.fja_valid = false;
   c.full_valid = c.bsr_valid = false;
   c.has_solution = false;
@@ -481,6 +484,7 @@ int pzg_assemble(pzg_ctx *ctx) {
   PZG_TRY(launch_assemble(c));
   PZG_TRY(launch_diag(c));
   c.assembled = true;
+  c.kel_added = false;
   c.full_valid = c.bsr_valid = false;
   c.has_solution = false;
   return PZG_OK;
@@ -578,6 +582,7 @@ int pzg_csr_set(pzg_ctx *ctx, int64_t neq, const int64_t *ia, const int64_t *ja,
   PZG_HIP(hipStreamSynchronize(s));
   c.has_struct = true;
   c.external_csr = true;
+  c.mf_ready = c.mf_vals_valid = false;
   c.assembled = true;
   c.fja_valid = false;
   c.full_valid = c.bsr_valid = false;
@@ -637,6 +642,7 @@ int pzg_add_kel_batch(pzg_ctx *ctx, int64_t nel, int32_t ne, const int64_t *dest
   dev_free(d_ek);
   dev_free(d_list);
   c.assembled = true;
+  c.kel_added = true;
   c.full_valid = c.bsr_valid = false;
   return rc;
 }
@@ -761,6 +767,25 @@ int pzg_time_kernel(pzg_ctx *ctx, int32_t which, int32_t nrep, double *avg_ms, d
     for (int i = 0; i < nrep; i++) PZG_TRY(launch_spmv_bsr(c, c.d_xs.as<float>(), c.d_qs.as<float>()));
     PZG_HIP(hipEventRecord(c.ev1, c.stream));
     b = (double)c.nbf * (c.bs * c.bs * 4.0 + 4.0) + (double)(c.nb + 1) * 8.0 + 2.0 * n * 4.0;
+  } else if (which == 3) {
+    if (!mf_available(c)) {
+      set_error("pzg_time_kernel: matrix-free operator needs a mesh-built system");
+      return PZG_ESTATE;
+    }
+    PZG_TRY(mf_prepare(c));
+    PZG_TRY(dev_alloc(c.d_xs, n * sizeof(float)));
+    PZG_TRY(dev_alloc(c.d_qs, n * sizeof(float)));
+    PZG_HIP(hipMemsetAsync(c.d_xs.p, 0, n * sizeof(float), c.stream));
+    PZG_TRY(mf_apply(c, c.d_xs.as<float>(), nullptr, c.d_qs.as<float>(), nullptr, nullptr, nullptr));
+    PZG_HIP(hipEventRecord(c.ev0, c.stream));
+    for (int i = 0; i < nrep; i++)
+      PZG_TRY(mf_apply(c, c.d_xs.as<float>(), nullptr, c.d_qs.as<float>(), nullptr, nullptr, nullptr));
+    PZG_HIP(hipEventRecord(c.ev1, c.stream));
+    // compulsory traffic: element maps, corners, coefficients, the element
+    // output buffer written and read once, row lists, x, D^-1/2 and y
+    const int64_t nsh = c.tvol.nsh, slots = c.nvol * nsh + c.mf_nbcd * c.tbc.nsh;
+    b = (double)c.nvol * (nsh * 4.0 + c.vol_ncorner * 12.0 + 4.0) + 2.0 * slots * c.nstate * 4.0 +
+        (double)slots * 4.0 + (double)(c.nb + 1) * 8.0 + 3.0 * n * 4.0;
   } else if (which == 2) {
     if (c.external_csr) {
       set_error("pzg_time_kernel: no mesh to assemble");
@@ -779,6 +804,37 @@ int pzg_time_kernel(pzg_ctx *ctx, int32_t which, int32_t nrep, double *avg_ms, d
   PZG_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
   *avg_ms = ms / nrep;
   if (bytes) *bytes = b;
+  return PZG_OK;
+}
+
+int pzg_scaled_apply(pzg_ctx *ctx, int32_t op, const double *x, double *y) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (!c.assembled || !x || !y) {
+    set_error("pzg_scaled_apply: needs an assembled system and host vectors");
+    return PZG_ESTATE;
+  }
+  if (op != PZG_INNER_BSR && op != PZG_INNER_MATFREE) {
+    set_error("pzg_scaled_apply: op must be PZG_INNER_BSR or PZG_INNER_MATFREE");
+    return PZG_EINVAL;
+  }
+  if (op == PZG_INNER_MATFREE && !mf_available(c)) {
+    set_error("pzg_scaled_apply: matrix-free operator needs a mesh-built system");
+    return PZG_ESTATE;
+  }
+  const int64_t n = c.neq;
+  if (!c.bsr_valid) PZG_TRY(launch_build_bsr(c));
+  if (op == PZG_INNER_MATFREE) PZG_TRY(mf_prepare(c));
+  std::vector<float> h(n);
+  for (int64_t i = 0; i < n; i++) h[i] = (float)x[i];
+  PZG_TRY(dev_alloc(c.d_xs, n * sizeof(float)));
+  PZG_TRY(dev_alloc(c.d_qs, n * sizeof(float)));
+  PZG_HIP(hipMemcpyAsync(c.d_xs.p, h.data(), n * sizeof(float), hipMemcpyHostToDevice, c.stream));
+  if (op == PZG_INNER_BSR) PZG_TRY(launch_spmv_bsr(c, c.d_xs.as<float>(), c.d_qs.as<float>()));
+  else PZG_TRY(mf_apply(c, c.d_xs.as<float>(), nullptr, c.d_qs.as<float>(), nullptr, nullptr, nullptr));
+  PZG_HIP(hipMemcpyAsync(h.data(), c.d_qs.p, n * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+  PZG_HIP(hipStreamSynchronize(c.stream));
+  for (int64_t i = 0; i < n; i++) y[i] = h[i];
   return PZG_OK;
 }
 

@@ -142,6 +142,13 @@ struct Ctx {
   int reduce_blocks = 1024;
   bool has_solution = false;
 
+  // matrix-free FP32 inner operator (pzg_matfree.hip)
+  bool mf_ready = false, mf_vals_valid = false;
+  bool kel_added = false;  // AddKel contributions beyond the mesh's elements
+  int64_t mf_nbcd = 0;     // Dirichlet boundary elements
+  DevBuf d_mf_blk, d_mf_xe, d_mf_bblk, d_mf_bm, d_mf_ptr, d_mf_inc, d_mf_tab, d_mf_ye;
+  DevBuf d_mf_coef, d_mf_dis;
+
   // learned preconditioner (pzg_learned.hip)
   bool nn_ready = false;
   int64_t nn_nc = 0;                   // aggregates (coarse nodes)
@@ -184,6 +191,14 @@ int nn_loss_grad(Ctx &c, int64_t ns, const float *X, const float *T, double *los
 int nn_adam_step(Ctx &c, double lr);
 int nn_store_pair(Ctx &c, const float *r, const float *e_minus_bjr, int64_t slot);
 int nn_collect(Ctx &c, int nrhs, const int32_t *rec, int nrec, uint64_t seed);
+int mf_setup(Ctx &c);
+int mf_prepare(Ctx &c);
+int mf_apply(Ctx &c, const float *p, const float *r, float *q, const SolveScalars *sc,
+             double *partials, int *gather_grid);
+// the matrix-free operator reproduces the assembled K (mesh-built, no AddKel extras)
+inline bool mf_available(const Ctx &c) {
+  return c.has_mesh && c.has_struct && !c.external_csr && !c.kel_added && c.assembled;
+}
 int solve_pcg64(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info);
 int solve_mixed(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info);
 int true_residual(Ctx &c, const double *u, double *rel);

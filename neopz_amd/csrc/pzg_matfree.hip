@@ -1,0 +1,511 @@
+// pzg_matfree.hip -- FP32 matrix-free application of the scaled operator
+// S = D^-1/2 K D^-1/2 for the inner iterations of the mixed solver.
+//
+// K is the sum of the element matrices (TPZStructMatrixOR::Serial_Assemble,
+// StrMatrix/pzstrmatrixor.cpp:104-365, with CalcStiff of
+// Mesh/pzinterpolationspace.cpp:403-474).  Instead of streaming the stored
+// matrix, every application recomputes K_e u_e at the element's quadrature
+// points from the corner coordinates, in FP32:
+//   volume:   y_e = sum_q w_q |det J_q| B_q^T C_e B_q u_e, evaluated as
+//             gradient -> flux/stress -> weighted back-projection, with the
+//             material laws of TPZMatPoisson / TPZDarcyFlow
+//             (Material/Poisson/TPZMatPoisson.cpp:19-42,
+//             Material/DarcyFlow/TPZDarcyFlow.cpp:19-46) and the CODE3 form of
+//             TPZElasticity3D (Material/Elasticity/TPZElasticity3D.cpp:269-372):
+//             sigma_kk = C3 e_kk - C2 (sum of the other e_jj), sigma_kd =
+//             C1 (du_k/dx_d + du_d/dx_k);
+//   boundary: Dirichlet penalty big * (phi_i, phi_j) per component
+//             (TPZMatPoisson.cpp:45-121, TPZElasticity3D.cpp:616-773), whose
+//             element matrices depend on geometry only and are stored.
+// Element outputs go to a per-(element, shape) buffer; a second kernel sums
+// them per block row in a fixed order (no atomics: the result is
+// deterministic) and fuses the dot products of the flexible CG step.
+//
+// The operator is an FP32 approximation of S of the same order as the stored
+// FP32 block CSR; the FP64 true residual of the refinement loop (CSR, FP64)
+// still decides convergence.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "pzg_device.h"
+#include "pzg_internal.h"
+
+namespace pzg {
+
+namespace {
+
+constexpr int MF_TPB = 256;
+
+struct MfTab {
+  float w[MAXQ];
+  float4 dphi[MAXQ][MAXSH];  // reference derivatives (d/dxi, d/deta, d/dzeta, 0)
+  float4 cd[MAXQ][8];        // corner (geometric) derivatives
+};
+
+struct MfArgs {
+  int64_t nvol;
+  int nq;
+  float nu;
+  const MfTab *tab;
+  const int32_t *blk;   // [nvol][NSH] active block of each shape (-1: none)
+  const float *xe;      // [nvol][NC*3] corner coordinates
+  const float *coef;    // [nvol] E (elasticity), permeability or scale
+  const float *dis;     // [neq] D^-1/2
+  const float *p;       // input vector (scaled unknowns)
+  float *ye;            // [slot][NS] element outputs, already scaled by D^-1/2
+};
+
+__device__ inline float4 f4(float a, float b, float c) { return make_float4(a, b, c, 0.f); }
+
+// LPE lanes per element (a power of two >= max(nq, NSH)); 256/LPE elements
+// per workgroup; grid-stride loop over element groups.
+template <int DIM, int NC, int NSH, int NS, int LPE>
+__global__ __launch_bounds__(MF_TPB) void k_mf_vol(MfArgs a, const SolveScalars *sc) {
+  if (sc && sc->done) return;
+  constexpr int EPB = MF_TPB / LPE;
+  __shared__ float4 s_dphi[LPE][NSH];
+  __shared__ float4 s_cd[LPE][NC];
+  __shared__ float s_w[LPE];
+  __shared__ float4 s_u[EPB][NSH];
+  __shared__ float s_x[EPB][NC * 3];
+  __shared__ float4 s_P[EPB][LPE][NS];
+  const int tid = threadIdx.x, el = tid / LPE, l = tid % LPE;
+  const int nq = a.nq;
+  for (int t = tid; t < nq * NSH; t += MF_TPB) s_dphi[t / NSH][t % NSH] = a.tab->dphi[t / NSH][t % NSH];
+  for (int t = tid; t < nq * NC; t += MF_TPB) s_cd[t / NC][t % NC] = a.tab->cd[t / NC][t % NC];
+  if (tid < nq) s_w[tid] = a.tab->w[tid];
+  for (int64_t g0 = (int64_t)blockIdx.x * EPB; g0 < a.nvol; g0 += (int64_t)gridDim.x * EPB) {
+    const int64_t e = g0 + el;
+    const bool live = e < a.nvol;
+    int32_t b = -1;
+    if (live && l < NSH) {
+      b = a.blk[e * NSH + l];
+      float u[3] = {0.f, 0.f, 0.f};
+      if (b >= 0)
+        for (int k = 0; k < NS; k++) u[k] = a.p[(int64_t)b * NS + k] * a.dis[(int64_t)b * NS + k];
+      s_u[el][l] = f4(u[0], u[1], u[2]);
+    }
+    if (live)
+      for (int t = l; t < NC * 3; t += LPE) s_x[el][t] = a.xe[e * (NC * 3) + t];
+    __syncthreads();
+    if (live && l < nq) {
+      // J[i][j] = dx_i / dxi_j (TPZGeoEl::Jacobian, Mesh/pzgeoel.cpp:1168-1355)
+      float J[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const float4 d = s_cd[l][c];
+        const float dd[3] = {d.x, d.y, d.z};
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          const float x = s_x[el][3 * c + i];
+#pragma unroll
+          for (int j = 0; j < DIM; j++) J[i][j] += x * dd[j];
+        }
+      }
+      // reference gradient gx[k][m] = d u_k / d xi_m
+      float gx[NS][3];
+#pragma unroll
+      for (int k = 0; k < NS; k++) gx[k][0] = gx[k][1] = gx[k][2] = 0.f;
+#pragma unroll 9
+      for (int s = 0; s < NSH; s++) {
+        const float4 d = s_dphi[l][s];
+        const float4 u4 = s_u[el][s];
+        const float u[3] = {u4.x, u4.y, u4.z};
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+          gx[k][0] += u[k] * d.x;
+          gx[k][1] += u[k] * d.y;
+          if (DIM == 3) gx[k][2] += u[k] * d.z;
+        }
+      }
+      const float coef = a.coef[e];
+      float P[NS][3];
+      if constexpr (DIM == 3) {
+        const float det = -J[0][2] * J[1][1] * J[2][0] + J[0][1] * J[1][2] * J[2][0] +
+                          J[0][2] * J[1][0] * J[2][1] - J[0][0] * J[1][2] * J[2][1] -
+                          J[0][1] * J[1][0] * J[2][2] + J[0][0] * J[1][1] * J[2][2];
+        const float id = 1.f / det;
+        float inv[3][3];  // inv[m][d] = d xi_m / d x_d
+        inv[0][0] = (-J[1][2] * J[2][1] + J[1][1] * J[2][2]) * id;
+        inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+        inv[0][2] = (-J[0][2] * J[1][1] + J[0][1] * J[1][2]) * id;
+        inv[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
+        inv[1][1] = (-J[0][2] * J[2][0] + J[0][0] * J[2][2]) * id;
+        inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+        inv[2][0] = (-J[1][1] * J[2][0] + J[1][0] * J[2][1]) * id;
+        inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+        inv[2][2] = (-J[0][1] * J[1][0] + J[0][0] * J[1][1]) * id;
+        const float wdet = s_w[l] * fabsf(det);
+        float g[NS][3];  // physical gradient
+#pragma unroll
+        for (int k = 0; k < NS; k++)
+#pragma unroll
+          for (int d = 0; d < 3; d++) g[k][d] = gx[k][0] * inv[0][d] + gx[k][1] * inv[1][d] + gx[k][2] * inv[2][d];
+        float S[NS][3];
+        if constexpr (NS == 3) {
+          const float nu = a.nu;
+          const float C1 = coef / (2.f + 2.f * nu);
+          const float den = -1.f + nu + 2.f * nu * nu;
+          const float C2 = coef * nu / den;
+          const float C3 = coef * (nu - 1.f) / den;
+          const float tr = g[0][0] + g[1][1] + g[2][2];
+#pragma unroll
+          for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+              S[k][d] = d == k ? C3 * g[k][k] - C2 * (tr - g[k][k]) : C1 * (g[k][d] + g[d][k]);
+        } else {
+#pragma unroll
+          for (int d = 0; d < 3; d++) S[0][d] = coef * g[0][d];
+        }
+#pragma unroll
+        for (int k = 0; k < NS; k++)
+#pragma unroll
+          for (int m = 0; m < 3; m++)
+            P[k][m] = wdet * (S[k][0] * inv[m][0] + S[k][1] * inv[m][1] + S[k][2] * inv[m][2]);
+      } else {
+        // 2D element in 3D space: grad(a).grad(b) = da^T (J^T J)^-1 db
+        float G00 = 0.f, G01 = 0.f, G11 = 0.f;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          G00 += J[i][0] * J[i][0];
+          G01 += J[i][0] * J[i][1];
+          G11 += J[i][1] * J[i][1];
+        }
+        const float detG = G00 * G11 - G01 * G01;
+        const float wdet = s_w[l] * sqrtf(detG);
+        const float f = wdet * coef / detG;
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+          P[k][0] = f * (G11 * gx[k][0] - G01 * gx[k][1]);
+          P[k][1] = f * (-G01 * gx[k][0] + G00 * gx[k][1]);
+          P[k][2] = 0.f;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < NS; k++) s_P[el][l][k] = f4(P[k][0], P[k][1], P[k][2]);
+    }
+    __syncthreads();
+    if (live && l < NSH) {
+      float y[NS];
+#pragma unroll
+      for (int k = 0; k < NS; k++) y[k] = 0.f;
+#pragma unroll 9
+      for (int q = 0; q < nq; q++) {
+        const float4 d = s_dphi[q][l];
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+          const float4 P = s_P[el][q][k];
+          y[k] += d.x * P.x + d.y * P.y + (DIM == 3 ? d.z * P.z : 0.f);
+        }
+      }
+      const int64_t slot = e * NSH + l;
+#pragma unroll
+      for (int k = 0; k < NS; k++)
+        a.ye[slot * NS + k] = b >= 0 ? y[k] * a.dis[(int64_t)b * NS + k] : 0.f;
+    }
+    __syncthreads();
+  }
+}
+
+// Dirichlet boundary elements: y_i,k = big * sum_j M_ij u_j,k with the stored
+// FP32 matrices big*M; one thread per (element, shape).
+template <int NSHB, int NS>
+__global__ __launch_bounds__(MF_TPB) void k_mf_bc(int64_t nbcd, const int32_t *__restrict__ bblk,
+                                                  const float *__restrict__ bm,
+                                                  const float *__restrict__ dis,
+                                                  const float *__restrict__ p,
+                                                  float *__restrict__ ye, int64_t slot0,
+                                                  const SolveScalars *sc) {
+  if (sc && sc->done) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nbcd * NSHB) return;
+  const int64_t e = t / NSHB;
+  const int i = (int)(t % NSHB);
+  float y[NS];
+  for (int k = 0; k < NS; k++) y[k] = 0.f;
+  for (int j = 0; j < NSHB; j++) {
+    const int32_t bj = bblk[e * NSHB + j];
+    if (bj < 0) continue;
+    const float m = bm[(e * NSHB + i) * NSHB + j];
+    for (int k = 0; k < NS; k++) y[k] += m * p[(int64_t)bj * NS + k] * dis[(int64_t)bj * NS + k];
+  }
+  const int32_t bi = bblk[e * NSHB + i];
+  for (int k = 0; k < NS; k++)
+    ye[(slot0 + t) * NS + k] = bi >= 0 ? y[k] * dis[(int64_t)bi * NS + k] : 0.f;
+}
+
+// q[A] = sum of the element outputs of block row A (fixed order); with
+// partials: slots 0/1 <- per-block sums of p.q and p.r (flexible CG alpha).
+template <int NS>
+__global__ __launch_bounds__(MF_TPB) void k_mf_gather(int64_t nb, const int64_t *__restrict__ ptr,
+                                                      const int32_t *__restrict__ inc,
+                                                      const float *__restrict__ ye,
+                                                      const float *__restrict__ p,
+                                                      const float *__restrict__ r,
+                                                      float *__restrict__ q, const SolveScalars *sc,
+                                                      double *partials) {
+  if (sc && sc->done) return;
+  double pq = 0., pr = 0.;
+  for (int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; A < nb;
+       A += (int64_t)gridDim.x * blockDim.x) {
+    float acc[NS];
+    for (int k = 0; k < NS; k++) acc[k] = 0.f;
+    const int64_t t1 = ptr[A + 1];
+    for (int64_t t = ptr[A]; t < t1; t++) {
+      const int64_t s = inc[t];
+      for (int k = 0; k < NS; k++) acc[k] += ye[s * NS + k];
+    }
+    for (int k = 0; k < NS; k++) {
+      const int64_t i = A * NS + k;
+      q[i] = acc[k];
+      if (partials) {
+        const float pi = p[i];
+        pq += (double)acc[k] * pi;
+        pr += (double)pi * r[i];
+      }
+    }
+  }
+  if (partials) {
+    double vv[2] = {pq, pr};
+    block_partials<2>(vv, partials);
+  }
+}
+
+__global__ void k_d2f(int64_t n, const double *__restrict__ x, float *__restrict__ y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (float)x[i];
+}
+
+int pow2_at_least(int v) {
+  int p = 4;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+template <int DIM, int NC, int NSH, int NS>
+int launch_vol_lpe(Ctx &c, const MfArgs &a, const SolveScalars *sc) {
+  const int lpe = pow2_at_least(std::max(a.nq, NSH));
+  auto go = [&](auto kern, int lpe_) {
+    int64_t groups = (a.nvol + (MF_TPB / lpe_) - 1) / (MF_TPB / lpe_);
+    int grid = (int)std::min<int64_t>(std::max<int64_t>(groups, 1), 4096);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(MF_TPB), 0, c.stream, a, sc);
+    return hipGetLastError() == hipSuccess ? PZG_OK : PZG_EHIP;
+  };
+  if constexpr (NSH <= 4) {
+    if (lpe <= 4) return go(k_mf_vol<DIM, NC, NSH, NS, 4>, 4);
+  }
+  if constexpr (NSH <= 8) {
+    if (lpe <= 8) return go(k_mf_vol<DIM, NC, NSH, NS, 8>, 8);
+  }
+  if constexpr (NSH <= 16) {
+    if (lpe <= 16) return go(k_mf_vol<DIM, NC, NSH, NS, 16>, 16);
+  }
+  if (lpe <= 32) return go(k_mf_vol<DIM, NC, NSH, NS, 32>, 32);
+  set_error("matrix-free operator: too many quadrature points");
+  return PZG_EUNSUP;
+}
+
+}  // namespace
+
+// Geometry-dependent set-up: element block maps, corner coordinates, float
+// tables, Dirichlet boundary element matrices, row incidence lists.
+int mf_setup(Ctx &c) {
+  if (c.mf_ready) return PZG_OK;
+  if (!c.has_mesh || !c.has_struct || c.external_csr) {
+    set_error("matrix-free operator: needs a mesh-built system");
+    return PZG_ESTATE;
+  }
+  const ElemTables &T = c.tvol;
+  const int NSH = T.nsh, NC = c.vol_ncorner, NSD = c.vol_nsides, ns = c.nstate;
+  if (T.nq > 32 || NSH > 27) {
+    set_error("matrix-free operator: element too large");
+    return PZG_EUNSUP;
+  }
+  const int64_t nvol = c.nvol, nb = c.nb;
+  std::vector<int32_t> blk(nvol * NSH);
+  std::vector<float> xe(nvol * NC * 3);
+  for (int64_t e = 0; e < nvol; e++) {
+    for (int s = 0; s < NSH; s++) blk[e * NSH + s] = (int32_t)c.aidx[c.vol_conn[e * NSD + T.shape_side[s]]];
+    for (int k = 0; k < NC; k++)
+      for (int i = 0; i < 3; i++) xe[(e * NC + k) * 3 + i] = (float)c.xyz[3 * c.vol_nodes[e * NC + k] + i];
+  }
+  // Dirichlet boundary elements and their penalty matrices (FP64, then FP32)
+  const ElemTables &B = c.tbc;
+  const int NSHB = B.nsh, NCB = c.bc_ncorner, NSDB = c.bc_nsides, dimb = c.dim - 1;
+  std::vector<int64_t> bcd;
+  for (int64_t e = 0; e < c.nbc; e++)
+    if (c.bc_type[c.bc_id[e]] == PZG_BC_DIRICHLET) bcd.push_back(e);
+  const int64_t nbcd = (int64_t)bcd.size();
+  std::vector<int32_t> bblk(nbcd * NSHB);
+  std::vector<float> bm(nbcd * NSHB * NSHB);
+  for (int64_t t = 0; t < nbcd; t++) {
+    const int64_t e = bcd[t];
+    for (int s = 0; s < NSHB; s++) bblk[t * NSHB + s] = (int32_t)c.aidx[c.bc_conn[e * NSDB + B.shape_side[s]]];
+    double wdet[MAXQ];
+    for (int q = 0; q < B.nq; q++) {
+      double gx[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+      for (int k = 0; k < NCB; k++)
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < dimb; j++) gx[i][j] += c.xyz[3 * c.bc_nodes[e * NCB + k] + i] * B.cdphi[q][j][k];
+      double det;
+      if (dimb == 1) {
+        det = std::sqrt(gx[0][0] * gx[0][0] + gx[1][0] * gx[1][0] + gx[2][0] * gx[2][0]);
+      } else {
+        double n1 = 0, d12 = 0, n2 = 0;
+        for (int i = 0; i < 3; i++) { n1 += gx[i][0] * gx[i][0]; d12 += gx[i][0] * gx[i][1]; }
+        n1 = std::sqrt(n1);
+        for (int i = 0; i < 3; i++) {
+          double v2 = gx[i][1] - d12 * (gx[i][0] / n1) / n1;
+          n2 += v2 * v2;
+        }
+        det = n1 * std::sqrt(n2);
+      }
+      wdet[q] = B.w[q] * std::fabs(det);
+    }
+    for (int i = 0; i < NSHB; i++)
+      for (int j = 0; j < NSHB; j++) {
+        double s = 0;
+        for (int q = 0; q < B.nq; q++) s += B.phi[q][i] * B.phi[q][j] * wdet[q];
+        bm[(t * NSHB + i) * NSHB + j] = (float)(c.big * s);
+      }
+  }
+  // row incidence lists: volume slots e*NSH+s, then boundary slots
+  const int64_t slot0 = nvol * NSH, nslots = slot0 + nbcd * NSHB;
+  if (nslots >= ((int64_t)1 << 31)) {
+    set_error("matrix-free operator: too many element slots");
+    return PZG_EUNSUP;
+  }
+  std::vector<int64_t> ptr(nb + 1, 0);
+  for (int32_t b : blk) if (b >= 0) ptr[b + 1]++;
+  for (int32_t b : bblk) if (b >= 0) ptr[b + 1]++;
+  for (int64_t i = 0; i < nb; i++) ptr[i + 1] += ptr[i];
+  std::vector<int32_t> inc(ptr[nb]);
+  std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
+  for (int64_t s = 0; s < slot0; s++) if (blk[s] >= 0) inc[fill[blk[s]]++] = (int32_t)s;
+  for (int64_t s = 0; s < nbcd * NSHB; s++) if (bblk[s] >= 0) inc[fill[bblk[s]]++] = (int32_t)(slot0 + s);
+  MfTab tab;
+  std::memset(&tab, 0, sizeof(tab));
+  for (int q = 0; q < T.nq; q++) {
+    tab.w[q] = (float)T.w[q];
+    for (int s = 0; s < NSH; s++)
+      tab.dphi[q][s] = make_float4((float)T.dphi[q][0][s], (float)T.dphi[q][1][s],
+                                   c.dim == 3 ? (float)T.dphi[q][2][s] : 0.f, 0.f);
+    for (int k = 0; k < NC; k++)
+      tab.cd[q][k] = make_float4((float)T.cdphi[q][0][k], (float)T.cdphi[q][1][k],
+                                 c.dim == 3 ? (float)T.cdphi[q][2][k] : 0.f, 0.f);
+  }
+  hipStream_t s = c.stream;
+  PZG_TRY(dev_upload(c.d_mf_blk, blk.data(), blk.size(), s));
+  PZG_TRY(dev_upload(c.d_mf_xe, xe.data(), xe.size(), s));
+  PZG_TRY(dev_upload(c.d_mf_bblk, bblk.data(), bblk.size(), s));
+  PZG_TRY(dev_upload(c.d_mf_bm, bm.data(), bm.size(), s));
+  PZG_TRY(dev_upload(c.d_mf_ptr, ptr.data(), ptr.size(), s));
+  PZG_TRY(dev_upload(c.d_mf_inc, inc.data(), inc.size(), s));
+  PZG_TRY(dev_upload(c.d_mf_tab, &tab, 1, s));
+  PZG_TRY(dev_alloc(c.d_mf_ye, sizeof(float) * std::max<int64_t>(nslots, 1) * ns));
+  PZG_TRY(dev_alloc(c.d_mf_coef, sizeof(float) * std::max<int64_t>(nvol, 1)));
+  PZG_TRY(dev_alloc(c.d_mf_dis, sizeof(float) * std::max<int64_t>(c.neq, 1)));
+  PZG_HIP(hipStreamSynchronize(s));
+  c.mf_nbcd = nbcd;
+  c.mf_ready = true;
+  c.mf_vals_valid = false;
+  return PZG_OK;
+}
+
+// Value-dependent set-up (after each assembly): coefficients and D^-1/2 in FP32.
+int mf_prepare(Ctx &c) {
+  PZG_TRY(mf_setup(c));
+  if (c.mf_vals_valid) return PZG_OK;
+  if (!c.bsr_valid) PZG_TRY(launch_build_bsr(c));  // computes D^-1/2
+  std::vector<float> coef(c.nvol);
+  const double dflt = c.material == PZG_MAT_ELAST3D ? c.E : c.scale;
+  for (int64_t e = 0; e < c.nvol; e++) coef[e] = (float)(c.vol_coef.empty() ? dflt : c.vol_coef[e]);
+  if (c.nvol)
+    PZG_HIP(hipMemcpyAsync(c.d_mf_coef.p, coef.data(), coef.size() * sizeof(float),
+                           hipMemcpyHostToDevice, c.stream));
+  if (c.neq) {
+    hipLaunchKernelGGL(k_d2f, dim3((c.neq + MF_TPB - 1) / MF_TPB), dim3(MF_TPB), 0, c.stream, c.neq,
+                       c.d_dinvsqrt.as<double>(), c.d_mf_dis.as<float>());
+    PZG_HIP(hipGetLastError());
+  }
+  PZG_HIP(hipStreamSynchronize(c.stream));
+  c.mf_vals_valid = true;
+  return PZG_OK;
+}
+
+// q = S p.  With partials: per-block sums of p.q (slot 0) and p.r (slot 1);
+// *gather_grid receives the number of blocks that wrote partials.
+int mf_apply(Ctx &c, const float *p, const float *r, float *q, const SolveScalars *sc,
+             double *partials, int *gather_grid) {
+  MfArgs a;
+  a.nvol = c.nvol;
+  a.nq = c.tvol.nq;
+  a.nu = (float)c.nu;
+  a.tab = c.d_mf_tab.as<MfTab>();
+  a.blk = c.d_mf_blk.as<int32_t>();
+  a.xe = c.d_mf_xe.as<float>();
+  a.coef = c.d_mf_coef.as<float>();
+  a.dis = c.d_mf_dis.as<float>();
+  a.p = p;
+  a.ye = c.d_mf_ye.as<float>();
+  const int nsh = c.tvol.nsh;
+  if (c.nvol > 0) {
+    if (c.dim == 3 && c.nstate == 3) {
+      if (nsh == 27) PZG_TRY((launch_vol_lpe<3, 8, 27, 3>(c, a, sc)));
+      else PZG_TRY((launch_vol_lpe<3, 8, 8, 3>(c, a, sc)));
+    } else if (c.dim == 3 && c.nstate == 1) {
+      if (nsh == 27) PZG_TRY((launch_vol_lpe<3, 8, 27, 1>(c, a, sc)));
+      else PZG_TRY((launch_vol_lpe<3, 8, 8, 1>(c, a, sc)));
+    } else if (c.dim == 2 && c.nstate == 1) {
+      if (nsh == 9) PZG_TRY((launch_vol_lpe<2, 4, 9, 1>(c, a, sc)));
+      else PZG_TRY((launch_vol_lpe<2, 4, 4, 1>(c, a, sc)));
+    } else {
+      set_error("matrix-free operator: unsupported dimension/state combination");
+      return PZG_EUNSUP;
+    }
+  }
+  const int64_t nbcd = c.mf_nbcd, slot0 = c.nvol * nsh;
+  const int nshb = c.tbc.nsh;
+  if (nbcd > 0) {
+    const int64_t th = nbcd * nshb;
+    dim3 g((unsigned)((th + MF_TPB - 1) / MF_TPB));
+    const int32_t *bb = c.d_mf_bblk.as<int32_t>();
+    const float *bm = c.d_mf_bm.as<float>();
+    float *ye = c.d_mf_ye.as<float>();
+    const float *dis = c.d_mf_dis.as<float>();
+    if (c.nstate == 3 && nshb == 9)
+      hipLaunchKernelGGL((k_mf_bc<9, 3>), g, dim3(MF_TPB), 0, c.stream, nbcd, bb, bm, dis, p, ye, slot0, sc);
+    else if (c.nstate == 3 && nshb == 4)
+      hipLaunchKernelGGL((k_mf_bc<4, 3>), g, dim3(MF_TPB), 0, c.stream, nbcd, bb, bm, dis, p, ye, slot0, sc);
+    else if (c.nstate == 1 && nshb == 9)
+      hipLaunchKernelGGL((k_mf_bc<9, 1>), g, dim3(MF_TPB), 0, c.stream, nbcd, bb, bm, dis, p, ye, slot0, sc);
+    else if (c.nstate == 1 && nshb == 4)
+      hipLaunchKernelGGL((k_mf_bc<4, 1>), g, dim3(MF_TPB), 0, c.stream, nbcd, bb, bm, dis, p, ye, slot0, sc);
+    else if (c.nstate == 1 && nshb == 3)
+      hipLaunchKernelGGL((k_mf_bc<3, 1>), g, dim3(MF_TPB), 0, c.stream, nbcd, bb, bm, dis, p, ye, slot0, sc);
+    else if (c.nstate == 1 && nshb == 2)
+      hipLaunchKernelGGL((k_mf_bc<2, 1>), g, dim3(MF_TPB), 0, c.stream, nbcd, bb, bm, dis, p, ye, slot0, sc);
+    else {
+      set_error("matrix-free operator: unsupported boundary element");
+      return PZG_EUNSUP;
+    }
+    PZG_HIP(hipGetLastError());
+  }
+  int64_t gg = (c.nb + MF_TPB - 1) / MF_TPB;
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(gg, MAXBLK));
+  if (c.nstate == 3)
+    hipLaunchKernelGGL(k_mf_gather<3>, dim3(grid), dim3(MF_TPB), 0, c.stream, c.nb,
+                       c.d_mf_ptr.as<int64_t>(), c.d_mf_inc.as<int32_t>(), c.d_mf_ye.as<float>(), p, r,
+                       q, sc, partials);
+  else
+    hipLaunchKernelGGL(k_mf_gather<1>, dim3(grid), dim3(MF_TPB), 0, c.stream, c.nb,
+                       c.d_mf_ptr.as<int64_t>(), c.d_mf_inc.as<int32_t>(), c.d_mf_ye.as<float>(), p, r,
+                       q, sc, partials);
+  PZG_HIP(hipGetLastError());
+  if (gather_grid) *gather_grid = grid;
+  return PZG_OK;
+}
+
+}  // namespace pzg

@@ -756,6 +756,21 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
   const int64_t n = c.neq, nb = c.nb;
   if (!c.full_valid) PZG_TRY(launch_expand_full(c));
   if (!c.bsr_valid) PZG_TRY(launch_build_bsr(c));
+  // operator of the FP32 iterations
+  bool use_mf = false;
+  if (prm.inner_op == PZG_INNER_MATFREE) {
+    if (!mf_available(c)) {
+      set_error("pzg_solve: the matrix-free inner operator needs a mesh-built system");
+      return PZG_ESTATE;
+    }
+    use_mf = true;
+  } else if (prm.inner_op == PZG_INNER_AUTO) {
+    use_mf = mf_available(c);
+  } else if (prm.inner_op != PZG_INNER_BSR) {
+    set_error("pzg_solve: unknown inner operator");
+    return PZG_EINVAL;
+  }
+  if (use_mf) PZG_TRY(mf_prepare(c));
   PZG_TRY(ensure_reduce(c));
   for (DevBuf *b : {&c.d_u, &c.d_r}) PZG_TRY(dev_alloc(*b, n * sizeof(double)));
   for (DevBuf *b : {&c.d_xs, &c.d_rs, &c.d_zs, &c.d_ps, &c.d_qs}) PZG_TRY(dev_alloc(*b, n * sizeof(float)));
@@ -819,11 +834,16 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
       restart = false;
     }
     auto iter = [&]() {
-      hipLaunchKernelGGL(k_fcg_spmv<BS>, dim3(gs), dim3(TPB), 0, c.stream, nb,
-                         c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(), c.d_bsr.as<float>(),
-                         c.nbf, c.d_ps.as<float>(), c.d_rs.as<float>(), c.d_qs.as<float>(), sc, part,
-                         cnt);
-      hipLaunchKernelGGL(k_fin_fcg_alpha, dim3(1), dim3(TPB), 0, c.stream, sc, (const double *)part, gs);
+      int ga = gs;
+      if (use_mf) {
+        (void)mf_apply(c, c.d_ps.as<float>(), c.d_rs.as<float>(), c.d_qs.as<float>(), sc, part, &ga);
+      } else {
+        hipLaunchKernelGGL(k_fcg_spmv<BS>, dim3(gs), dim3(TPB), 0, c.stream, nb,
+                           c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(), c.d_bsr.as<float>(),
+                           c.nbf, c.d_ps.as<float>(), c.d_rs.as<float>(), c.d_qs.as<float>(), sc,
+                           part, cnt);
+      }
+      hipLaunchKernelGGL(k_fin_fcg_alpha, dim3(1), dim3(TPB), 0, c.stream, sc, (const double *)part, ga);
       hipLaunchKernelGGL(k_fcg_update<BS>, dim3(gb), dim3(TPB), 0, c.stream, nb,
                          c.d_xs.as<float>(), c.d_rs.as<float>(), c.d_zs.as<float>(),
                          c.d_ps.as<float>(), c.d_qs.as<float>(), c.d_bjinv.as<float>(), sc, part,

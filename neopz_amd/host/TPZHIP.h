@@ -138,6 +138,7 @@ class TPZStepSolver {
     p.check_every = 50;
     p.use_graph = 1;
     p.refine = 0;
+    p.inner_op = PZG_INNER_AUTO;
     result.resize(fMatrix->Rows());
     int rc = pzg_solve(fMatrix->Context(), &p, F ? F->data() : nullptr, result.data(), &fInfo);
     if (rc != PZG_OK && rc != PZG_ENOCONV) Check(rc, "Solve");

@@ -123,13 +123,14 @@ def test_pcg64_matches_oracle(name):
     assert st.fTol <= 1e-13
 
 
+@pytest.mark.parametrize("inner_op", [L.INNER_MATFREE, L.INNER_BSR])
 @pytest.mark.parametrize("name", ["poisson2d_p2", "darcy2d_p1_neumann_h", "elast3d_p2_body",
                                   "elast3d_p1"])
-def test_mixed_refinement(name):
+def test_mixed_refinement(name, inner_op):
     pb = CASES[name]()
     an = TPZLinearAnalysis(pb, False, ctx=ctx())
     st = TPZStepSolver()
-    st.SetMixedFCG(50000, 1e-8, inner_tol=1e-4)
+    st.SetMixedFCG(50000, 1e-8, inner_tol=1e-4, inner_op=inner_op)
     an.SetSolver(st)
     an.Assemble()
     u = an.Solve()
@@ -143,6 +144,51 @@ def test_mixed_refinement(name):
     assert np.linalg.norm(u - ud) <= 1e-5 * np.linalg.norm(ud)
     rel = np.linalg.norm(rhs - Kf @ u) / np.linalg.norm(rhs)
     assert rel <= 2e-8
+
+
+def _scaled_apply(c, op, x):
+    y = np.zeros_like(x)
+    L.check(c.lib.pzg_scaled_apply(c.h, op, L.ptr(x, L.dp), L.ptr(y, L.dp)))
+    return y
+
+
+@pytest.mark.parametrize("name", ["poisson2d_p2", "poisson2d_p1", "darcy2d_p2", "darcy2d_p1_neumann",
+                                  "elast3d_p2_traction", "elast3d_p2_body", "elast3d_p1",
+                                  "poisson3d_p2"])
+def test_inner_operators_match_scaled_matrix(name):
+    """Both FP32 inner operators apply S = D^-1/2 K D^-1/2 of the oracle's K
+    (FP64) to within FP32 accuracy: ||y - S x|| <= 1e-5 ||S x||, and the
+    matrix-free result is bitwise repeatable."""
+    pb = CASES[name]()
+    K, ia, ja, a, rhs = _assemble(pb)
+    o, ia_r, ja_r, a_r, _ = _oracle(pb)
+    rows = np.repeat(np.arange(o.neq), np.diff(ia_r))
+    U = sp.csr_matrix((a_r, (rows, ja_r)), shape=(o.neq, o.neq))
+    Kf = (U + sp.triu(U, 1).T).tocsr()
+    dis = 1.0 / np.sqrt(np.abs(Kf.diagonal()))
+    x = np.random.default_rng(11).standard_normal(o.neq).astype(np.float32).astype(np.float64)
+    ref = dis * (Kf @ (dis * x))
+    c = ctx()
+    for op in (L.INNER_BSR, L.INNER_MATFREE):
+        y = _scaled_apply(c, op, x)
+        err = np.linalg.norm(y - ref) / np.linalg.norm(ref)
+        print(f"{name} op {op}: rel err {err:.2e}", flush=True)
+        assert err <= 1e-5
+    assert np.array_equal(_scaled_apply(c, L.INNER_MATFREE, x), _scaled_apply(c, L.INNER_MATFREE, x))
+
+
+def test_matfree_needs_mesh_built_system():
+    pb = CASES["poisson3d_p2"]()
+    o, ia, ja, a, rhs = _oracle(pb)
+    K = TPZSYsmpMatrixHIP(ctx(), o.neq)
+    K.SetData(ia, ja, a, rhs)
+    x = np.ones(o.neq)
+    with pytest.raises(L.PzgError):
+        _scaled_apply(ctx(), L.INNER_MATFREE, x)
+    st = TPZStepSolver(K)
+    st.SetMixedFCG(10000, 1e-8, inner_op=L.INNER_AUTO)   # falls back to the stored operator
+    st.Solve(rhs)
+    assert st.info["converged"]
 
 
 def test_known_matrix_1d_addkel():
