@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--no-graph", action="store_true", help="plain launches instead of hipGraph replay")
     ap.add_argument("--inner-op", default="auto", choices=["auto", "bsr", "matfree"],
                     help="FP32 operator of the inner iterations")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="assemble once, then time each operator kernel (5 launches) and a 1 GiB "
+                         "device copy (HBM counter calibration); no solve")
     return ap.parse_args()
 
 
@@ -111,6 +114,56 @@ def cpu_baseline(pb, iters_target, seed, tol):
             "assembly_s": asm_t, "s_per_iteration": it_t}
 
 
+def traffic_of(kernel):
+    """HBM bytes per launch of a kernel on the default workload, from the
+    counter summary profiles/hbm_traffic_c5.txt (FETCH_SIZE and WRITE_SIZE in
+    KiB per dispatch, each scaled by what the counter reads for a 1 GiB
+    device copy); None when the summary has no line for the kernel."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_c5.txt")
+    if not os.path.exists(path):
+        return None
+    scale = {"fetch": 1.0, "write": 1.0}
+    found = None
+    for line in open(path):
+        f = dict(kv.split("=", 1) for kv in line.split() if "=" in kv)
+        if "fetch_scale" in f:
+            scale = {"fetch": float(f["fetch_scale"]), "write": float(f["write_scale"])}
+        if f.get("kernel", "").startswith(kernel) and "fetch_KiB" in f:
+            found = f
+    if found is None:
+        return None
+    return 1024.0 * (float(found["fetch_KiB"]) * scale["fetch"] + float(found["write_KiB"]) * scale["write"])
+
+
+def kernels_only(ctx, L):
+    """Per-kernel timings for counter runs: FP32 block CSR SpMV, FP64 CSR SpMV,
+    matrix-free operator, element assembly; then three 1 GiB device-to-device
+    copies whose counters give the bytes-per-count scale of a plain copy."""
+    import ctypes as C
+    L.check(ctx.lib.pzg_assemble(ctx.h))
+    avg, nbytes = C.c_double(), C.c_double()
+    res = {}
+    for which, name in ((1, "spmv_bsr32"), (0, "spmv_bsr64"), (3, "matfree"), (4, "mf_element"),
+                        (2, "assembly")):
+        L.check(ctx.lib.pzg_time_kernel(ctx.h, which, 5, C.byref(avg), C.byref(nbytes)))
+        unit = "alg_flops" if which == 4 else "alg_bytes"
+        res[name] = {"ms": avg.value, unit: nbytes.value}
+        log(f"{name}: {avg.value:.4f} ms, {unit} {nbytes.value:.4e}")
+    hip = C.CDLL("libamdhip64.so")
+    nb = 1 << 30
+    src, dst = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(src), C.c_size_t(nb)) or hip.hipMalloc(C.byref(dst), C.c_size_t(nb)):
+        raise RuntimeError("hipMalloc failed")
+    for _ in range(3):
+        if hip.hipMemcpy(dst, src, C.c_size_t(nb), 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log(f"copy calibration: 3 x {nb} bytes device to device")
+    print(json.dumps({"kernels": res, "copy_bytes": nb}))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -132,6 +185,10 @@ def main():
     t0 = time.perf_counter()
     K = sm.Create()
     log(f"pattern: {ctx.sizes()} ({time.perf_counter() - t0:.1f}s)")
+
+    if args.kernels_only:
+        kernels_only(ctx, L)
+        return
 
     def solver(kind):
         st = TPZStepSolver(K)
@@ -193,7 +250,25 @@ def main():
     asm_ms = avg.value
     L.check(ctx.lib.pzg_time_kernel(ctx.h, 3, 20, C.byref(avg), C.byref(nbytes)))
     mf_ms, mf_gbs = avg.value, nbytes.value / (avg.value * 1e-3) / 1e9
+    L.check(ctx.lib.pzg_time_kernel(ctx.h, 4, 20, C.byref(avg), C.byref(nbytes)))
+    el_ms, el_flops = avg.value, nbytes.value
     sizes = ctx.sizes()
+    # roofline of the dominant kernel of the measured path: the matrix-free
+    # element kernel (FP32 contractions on the matrix cores) when the inner
+    # operator is matrix-free, else the FP32 block CSR SpMV
+    mf_used = args.inner_op != "bsr"
+    if mf_used:
+        kname = (f"k_mf_hex27_mfma<{sizes['bs']}>" if pb.dim == 3 and pb.p == 2
+                 else "k_mf_vol") + " (matrix-free FP32 element operator)"
+        tf = el_flops / (el_ms * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "kernel": kname, "achieved": tf, "peak": 157.3,
+                    "unit": "TFLOP/s", "frac": tf / 157.3, "launch_ms": el_ms,
+                    "flops_per_launch": el_flops, "traffic": traffic_of("k_mf_hex27_mfma")}
+    else:
+        roofline = {"bound": "hbm", "kernel": f"k_spmv_bsr<float,{sizes['bs']}> (FP32 block CSR SpMV)",
+                    "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                    "frac": achieved / 8000.0, "launch_ms": spmv_ms,
+                    "traffic": traffic_of("k_spmv_bsr<float")}
     cpu = None
     if not args.no_cpu:
         its = base["iterations"] if base else infos[-1]["iterations"]
@@ -214,12 +289,12 @@ def main():
         "speedup_vs_pcg64": (base["seconds"] / value) if base else None,
         "assembly_ms": asm_ms,
         "elements_per_s": sizes["nvol"] / (asm_ms * 1e-3),
-        "spmv64_full_csr": {"ms": spmv64_ms, "GBps": spmv64_gbs},
-        "spmv32_bsr": {"ms": spmv_ms, "GBps": achieved},
+        "spmv": {"fp64_block_csr": {"ms": spmv64_ms, "GBps": spmv64_gbs,
+                                    "frac_of_8TBps": spmv64_gbs / 8000.0},
+                 "fp32_block_csr": {"ms": spmv_ms, "GBps": achieved,
+                                    "frac_of_8TBps": achieved / 8000.0}},
         "matfree_operator": {"ms": mf_ms, "GBps_compulsory": mf_gbs},
-        "roofline": {"bound": "hbm", "kernel": f"k_spmv_bsr<{sizes['bs']}> (FP32 block CSR SpMV)",
-                     "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                     "frac": achieved / 8000.0, "traffic": None},
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -210,7 +210,9 @@ int pzg_true_residual(pzg_ctx *ctx, double *rel);
 /* Average device duration (HIP events on the context stream) of nrep launches
  * of the SpMV kernel of the given format: 0 = FP64 full CSR (plain PCG path),
  * 1 = FP32 block CSR of the mixed path, 2 = element assembly (whole pass),
- * 3 = matrix-free FP32 operator (element, boundary and row-sum kernels). */
+ * 3 = matrix-free FP32 operator (element, boundary and row-sum kernels),
+ * 4 = its volume element kernel alone, for which *bytes receives the
+ * algorithmic flops of one launch instead of bytes. */
 int pzg_time_kernel(pzg_ctx *ctx, int32_t which, int32_t nrep, double *avg_ms, double *bytes);
 
 /* y = S x with S = D^-1/2 K D^-1/2 applied by the FP32 inner operator op

@@ -508,88 +508,80 @@ int launch_diag(Ctx &c) {
 }
 
 // --------------------------------------------------------------------------
-// Full CSR column indices (once per pattern).
-__global__ void k_full_ja(int64_t nb, int bs, const int64_t *__restrict__ fbia,
-                          const int32_t *__restrict__ fbja, const int64_t *__restrict__ fia,
-                          int32_t *__restrict__ fja) {
-  int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  int lane = threadIdx.x & 63;
-  if (wave >= nb) return;
-  int64_t b0 = fbia[wave], nbk = fbia[wave + 1] - b0;
-  for (int k = 0; k < bs; k++) {
-    int64_t base = fia[wave * bs + k];
-    for (int64_t e = lane; e < nbk * bs; e += 64)
-      fja[base + e] = fbja[b0 + e / bs] * bs + (int32_t)(e % bs);
-  }
-}
-
-// Upper CSR values -> full CSR values: diagonal blocks (k_expand_diag) and
-// coupled blocks with their transposes (k_expand_off).
+// Full (both triangles) FP64 block CSR of K, from the reference's upper CSR:
+// diagonal blocks (k_bsr64_diag) and coupled blocks with their transposes
+// (k_bsr64_off).  Block values are block-major, row-major inside a block.
 __global__ void k_nlow(int64_t nb, const int64_t *__restrict__ fbia, const int64_t *__restrict__ ubia,
                        int32_t *__restrict__ nlow) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nb) nlow[i] = (int32_t)((fbia[i + 1] - fbia[i]) - 1 - (ubia[i + 1] - ubia[i]));
 }
 
-// number of lower blocks of every block row (= rank of the diagonal block)
-__global__ void k_expand_diag(int64_t nb, int bs, const int64_t *__restrict__ ia,
-                              const double *__restrict__ a, const int32_t *__restrict__ nlow,
-                              const int64_t *__restrict__ fia, double *__restrict__ fa) {
-  int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= nb * bs * bs) return;
-  int64_t A = gid / (bs * bs);
-  int k = (int)((gid % (bs * bs)) / bs), l = (int)(gid % bs);
-  if (l < k) return;
-  double v = a[ia[A * bs + k] + (l - k)];
-  int64_t nl = nlow[A];
-  fa[fia[A * bs + k] + nl * bs + l] = v;
-  if (l != k) fa[fia[A * bs + l] + nl * bs + k] = v;
+// nlow[A] = number of lower blocks of block row A (= rank of its diagonal block)
+template <int BS>
+__global__ void k_bsr64_diag(int64_t nb, const int64_t *__restrict__ ia, const double *__restrict__ a,
+                             const int32_t *__restrict__ nlow, const int64_t *__restrict__ fbia,
+                             double *__restrict__ v) {
+  int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (A >= nb) return;
+  double *blk = v + (fbia[A] + nlow[A]) * (BS * BS);
+  for (int k = 0; k < BS; k++)
+    for (int l = k; l < BS; l++) {
+      const double x = a[ia[A * BS + k] + (l - k)];
+      blk[k * BS + l] = x;
+      blk[l * BS + k] = x;
+    }
 }
 
-__global__ void k_expand_off(int64_t nb, int bs, const int64_t *__restrict__ ia,
-                             const double *__restrict__ a, const int64_t *__restrict__ ubia,
-                             const int32_t *__restrict__ ubja, const int32_t *__restrict__ rfwd,
-                             const int32_t *__restrict__ rbwd, const int64_t *__restrict__ fia,
-                             double *__restrict__ fa) {
+template <int BS>
+__global__ void k_bsr64_off(int64_t nb, const int64_t *__restrict__ ia, const double *__restrict__ a,
+                            const int64_t *__restrict__ ubia, const int32_t *__restrict__ ubja,
+                            const int32_t *__restrict__ rfwd, const int32_t *__restrict__ rbwd,
+                            const int64_t *__restrict__ fbia, double *__restrict__ v) {
   int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   int lane = threadIdx.x & 63;
   if (wave >= nb) return;
   const int64_t A = wave, u0 = ubia[A], nup = ubia[A + 1] - u0;
-  const int bb = bs * bs;
-  for (int64_t e = lane; e < nup * bb; e += 64) {
-    int64_t t = e / bb;
-    int k = (int)((e % bb) / bs), l = (int)(e % bs);
+  constexpr int BB = BS * BS;
+  for (int64_t e = lane; e < nup * BB; e += 64) {
+    int64_t t = e / BB;
+    int k = (int)((e % BB) / BS), l = (int)(e % BS);
     int64_t B = ubja[u0 + t];
-    double v = a[ia[A * bs + k] + bs + t * bs - k + l];
-    fa[fia[A * bs + k] + (int64_t)rfwd[u0 + t] * bs + l] = v;
-    fa[fia[B * bs + l] + (int64_t)rbwd[u0 + t] * bs + k] = v;
+    double x = a[ia[A * BS + k] + BS + t * BS - k + l];
+    v[(fbia[A] + rfwd[u0 + t]) * BB + k * BS + l] = x;
+    v[(fbia[B] + rbwd[u0 + t]) * BB + l * BS + k] = x;
   }
 }
 
 int launch_expand_full(Ctx &c) {
-  PZG_TRY(dev_alloc(c.d_fja, sizeof(int32_t) * c.nnz_full));
-  PZG_TRY(dev_alloc(c.d_fa, sizeof(double) * c.nnz_full));
-  if (!c.fja_valid) {
-    int64_t threads = c.nb * 64;
-    hipLaunchKernelGGL(k_full_ja, dim3((threads + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
-                       c.bs, c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(), c.d_fia.as<int64_t>(),
-                       c.d_fja.as<int32_t>());
-    PZG_HIP(hipGetLastError());
-    c.fja_valid = true;
-  }
+  const int bs = c.bs;
+  PZG_TRY(dev_alloc(c.d_fa, sizeof(double) * c.nbf * bs * bs));
   DevBuf nl;
   PZG_TRY(dev_alloc(nl, sizeof(int32_t) * c.nb));
   hipLaunchKernelGGL(k_nlow, dim3((c.nb + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
                      c.d_fbia.as<int64_t>(), c.d_ubia.as<int64_t>(), nl.as<int32_t>());
-  int64_t nd = c.nb * c.bs * c.bs;
-  hipLaunchKernelGGL(k_expand_diag, dim3((nd + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb, c.bs,
-                     c.d_ia.as<int64_t>(), c.d_a.as<double>(), nl.as<int32_t>(),
-                     c.d_fia.as<int64_t>(), c.d_fa.as<double>());
   int64_t threads = c.nb * 64;
-  hipLaunchKernelGGL(k_expand_off, dim3((threads + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
-                     c.bs, c.d_ia.as<int64_t>(), c.d_a.as<double>(), c.d_ubia.as<int64_t>(),
-                     c.d_ubja.as<int32_t>(), c.d_rank_fwd.as<int32_t>(),
-                     c.d_rank_bwd.as<int32_t>(), c.d_fia.as<int64_t>(), c.d_fa.as<double>());
+  if (bs == 3) {
+    hipLaunchKernelGGL(k_bsr64_diag<3>, dim3((c.nb + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
+                       c.d_ia.as<int64_t>(), c.d_a.as<double>(), nl.as<int32_t>(),
+                       c.d_fbia.as<int64_t>(), c.d_fa.as<double>());
+    hipLaunchKernelGGL(k_bsr64_off<3>, dim3((threads + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
+                       c.d_ia.as<int64_t>(), c.d_a.as<double>(), c.d_ubia.as<int64_t>(),
+                       c.d_ubja.as<int32_t>(), c.d_rank_fwd.as<int32_t>(),
+                       c.d_rank_bwd.as<int32_t>(), c.d_fbia.as<int64_t>(), c.d_fa.as<double>());
+  } else if (bs == 1) {
+    hipLaunchKernelGGL(k_bsr64_diag<1>, dim3((c.nb + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
+                       c.d_ia.as<int64_t>(), c.d_a.as<double>(), nl.as<int32_t>(),
+                       c.d_fbia.as<int64_t>(), c.d_fa.as<double>());
+    hipLaunchKernelGGL(k_bsr64_off<1>, dim3((threads + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
+                       c.d_ia.as<int64_t>(), c.d_a.as<double>(), c.d_ubia.as<int64_t>(),
+                       c.d_ubja.as<int32_t>(), c.d_rank_fwd.as<int32_t>(),
+                       c.d_rank_bwd.as<int32_t>(), c.d_fbia.as<int64_t>(), c.d_fa.as<double>());
+  } else {
+    dev_free(nl);
+    set_error("block CSR: unsupported block size");
+    return PZG_EUNSUP;
+  }
   PZG_HIP(hipGetLastError());
   PZG_HIP(hipStreamSynchronize(c.stream));
   dev_free(nl);
@@ -605,7 +597,7 @@ __global__ void k_dinvsqrt(int64_t n, const double *__restrict__ d, double *__re
   if (i < n) s[i] = 1.0 / sqrt(fabs(d[i]));
 }
 
-// Block values are placed by bsr_index (block-major or component-major).
+// Block values are block-major, row-major inside a block.
 template <int BS>
 __global__ void k_bsr_diag(int64_t nb, int64_t nbf, const int64_t *__restrict__ ia,
                            const double *__restrict__ a, const int32_t *__restrict__ nlow,
@@ -622,7 +614,7 @@ __global__ void k_bsr_diag(int64_t nb, int64_t nbf, const int64_t *__restrict__ 
     }
   const int64_t t = fbia[A] + nlow[A];
   for (int k = 0; k < BS; k++)
-    for (int l = 0; l < BS; l++) bsr[bsr_index(t, k * BS + l, nbf, BS * BS)] = (float)m[k][l];
+    for (int l = 0; l < BS; l++) bsr[t * (BS * BS) + k * BS + l] = (float)m[k][l];
   float *inv = binv + A * BS * BS;
   if (BS == 1) {
     inv[0] = (float)(1.0 / m[0][0]);
@@ -665,8 +657,8 @@ __global__ void k_bsr_off(int64_t nb, int64_t nbf, const int64_t *__restrict__ i
     int k = (int)((e % BB) / BS), l = (int)(e % BS);
     int64_t B = ubja[u0 + t];
     double v = a[ia[A * BS + k] + BS + t * BS - k + l] * dis[A * BS + k] * dis[B * BS + l];
-    bsr[bsr_index(fbia[A] + rfwd[u0 + t], k * BS + l, nbf, BB)] = (float)v;
-    bsr[bsr_index(fbia[B] + rbwd[u0 + t], l * BS + k, nbf, BB)] = (float)v;
+    bsr[(fbia[A] + rfwd[u0 + t]) * BB + k * BS + l] = (float)v;
+    bsr[(fbia[B] + rbwd[u0 + t]) * BB + l * BS + k] = (float)v;
   }
 }
 

@@ -6,6 +6,40 @@
 namespace pzg {
 
 constexpr int RED_TPB = 256;
+
+// Workgroups are dispatched round-robin over the XCDs (workgroup b runs on XCD
+// b % NXCD), and each XCD has its own L2.  xcd_block_range gives workgroup
+// blockIdx.x a contiguous slice [lo, hi) of [0, n) such that the slices of the
+// workgroups of one XCD are adjacent: XCD k covers about [k n/NXCD,
+// (k+1) n/NXCD), so the gathered operands of neighbouring rows stay in that
+// XCD's L2.
+constexpr int NXCD = 8;
+__device__ inline void xcd_block_range(int64_t n, int64_t &lo, int64_t &hi) {
+  const int G = (int)gridDim.x, b = (int)blockIdx.x;
+  const int x = b % NXCD, i = b / NXCD;
+  const int per = G / NXCD, rem = G % NXCD;
+  const int64_t vb = (int64_t)x * per + (x < rem ? x : rem) + i;
+  lo = n * vb / G;
+  hi = n * (vb + 1) / G;
+}
+
+// The slab [lo, hi) of [0, n) owned by this workgroup's XCD, the index i of
+// the workgroup among that XCD's gk workgroups.  Interleaving the XCD's
+// workgroups over its slab (tile i, i + gk, ...) keeps the rows in flight on
+// one XCD in a narrow band, so the gathered operands fit in its L2.
+#ifndef PZG_XCD_INTERLEAVE
+#define PZG_XCD_INTERLEAVE 1
+#endif
+__device__ inline void xcd_slab(int64_t n, int64_t &lo, int64_t &hi, int &i, int &gk) {
+  const int G = (int)gridDim.x, b = (int)blockIdx.x;
+  const int x = b % NXCD;
+  const int per = G / NXCD, rem = G % NXCD;
+  i = b / NXCD;
+  gk = per + (x < rem ? 1 : 0);
+  const int64_t first = (int64_t)x * per + (x < rem ? x : rem);
+  lo = n * first / G;
+  hi = n * (first + gk) / G;
+}
 constexpr int MAXBLK = 2048;  // partial-sum slots per reduction
 constexpr int NSLOT = 4;
 

@@ -53,7 +53,7 @@ template <class F> static void parallel_for(int64_t n, F f) {
 
 static void free_all(Ctx &c) {
   DevBuf *bufs[] = {&c.d_ia, &c.d_ja, &c.d_a, &c.d_rhs, &c.d_diag, &c.d_ubia, &c.d_ubja,
-                    &c.d_fbia, &c.d_fbja, &c.d_rank_fwd, &c.d_rank_bwd, &c.d_fia, &c.d_fja,
+                    &c.d_fbia, &c.d_fbja, &c.d_rank_fwd, &c.d_rank_bwd,
                     &c.d_fa, &c.d_bsr, &c.d_dinvsqrt, &c.d_bjinv, &c.d_xyz, &c.d_vnodes,
                     &c.d_vconn, &c.d_vcoef, &c.d_bnodes, &c.d_bconn, &c.d_bcid, &c.d_bctype,
                     &c.d_bcval, &c.d_aidx, &c.d_bumps, &c.d_vmap, &c.d_bmap, &c.d_tvol,
@@ -331,24 +331,18 @@ static int struct_create(Ctx &c) {
   });
   // equation-level row pointers: reference upper CSR and full CSR
   const int64_t N = c.neq;
-  std::vector<int64_t> ia(N + 1), fia(N + 1);
+  std::vector<int64_t> ia(N + 1);
   int64_t ub = 0, fb = 0;
   for (int64_t a = 0; a < nb; a++) {
     for (int64_t k = 0; k < bs; k++) {
       ia[a * bs + k] = ub;
       ub += (bs - k) + bs * (int64_t)nup[a];
-      fia[a * bs + k] = fb;
       fb += bs * (int64_t)(nlow[a] + 1 + nup[a]);
     }
   }
   ia[N] = ub;
-  fia[N] = fb;
   c.nnz_u = ub;
   c.nnz_full = fb;
-  if (c.nnz_full >= (int64_t)1 << 31 && false) {
-    set_error("full CSR exceeds int32 indexing");
-    return PZG_EUNSUP;
-  }
   hipStream_t s = c.stream;
   PZG_TRY(dev_upload(c.d_ubia, ubia.data(), ubia.size(), s));
   PZG_TRY(dev_upload(c.d_ubja, ubja.data(), ubja.size(), s));
@@ -357,7 +351,6 @@ static int struct_create(Ctx &c) {
   PZG_TRY(dev_upload(c.d_rank_fwd, rank_fwd.data(), rank_fwd.size(), s));
   PZG_TRY(dev_upload(c.d_rank_bwd, rank_bwd.data(), rank_bwd.size(), s));
   PZG_TRY(dev_upload(c.d_ia, ia.data(), ia.size(), s));
-  PZG_TRY(dev_upload(c.d_fia, fia.data(), fia.size(), s));
   PZG_TRY(dev_alloc(c.d_ja, sizeof(int64_t) * std::max<int64_t>(c.nnz_u, 1)));
   PZG_TRY(dev_alloc(c.d_a, sizeof(double) * std::max<int64_t>(c.nnz_u, 1)));
   PZG_TRY(dev_alloc(c.d_rhs, sizeof(double) * std::max<int64_t>(N, 1)));
@@ -383,7 +376,6 @@ static int struct_create(Ctx &c) {
   c.assembled = false;
   c.kel_added = false;
   c.mf_ready = c.mf_vals_valid = false;
-  c.fja_valid = false;
   c.full_valid = c.bsr_valid = false;
   c.has_solution = false;
   return PZG_OK;
@@ -573,7 +565,6 @@ int pzg_csr_set(pzg_ctx *ctx, int64_t neq, const int64_t *ia, const int64_t *ja,
   PZG_TRY(dev_upload(c.d_ia, ia, neq + 1, s));
   PZG_TRY(dev_upload(c.d_ja, ja, c.nnz_u, s));
   PZG_TRY(dev_upload(c.d_a, a, c.nnz_u, s));
-  PZG_TRY(dev_upload(c.d_fia, fbia.data(), fbia.size(), s));
   PZG_TRY(dev_alloc(c.d_rhs, sizeof(double) * neq));
   if (rhs) PZG_HIP(hipMemcpyAsync(c.d_rhs.p, rhs, sizeof(double) * neq, hipMemcpyHostToDevice, s));
   else PZG_HIP(hipMemsetAsync(c.d_rhs.p, 0, sizeof(double) * neq, s));
@@ -584,7 +575,6 @@ int pzg_csr_set(pzg_ctx *ctx, int64_t neq, const int64_t *ia, const int64_t *ja,
   c.external_csr = true;
   c.mf_ready = c.mf_vals_valid = false;
   c.assembled = true;
-  c.fja_valid = false;
   c.full_valid = c.bsr_valid = false;
   c.has_solution = false;
   return PZG_OK;
@@ -756,7 +746,7 @@ int pzg_time_kernel(pzg_ctx *ctx, int32_t which, int32_t nrep, double *avg_ms, d
     PZG_HIP(hipEventRecord(c.ev0, c.stream));
     for (int i = 0; i < nrep; i++) PZG_TRY(launch_spmv64(c, c.d_x.as<double>(), c.d_w.as<double>()));
     PZG_HIP(hipEventRecord(c.ev1, c.stream));
-    b = (double)c.nnz_full * 12.0 + (double)(n + 1) * 8.0 + 2.0 * n * 8.0;
+    b = (double)c.nbf * (c.bs * c.bs * 8.0 + 4.0) + (double)(c.nb + 1) * 8.0 + 2.0 * n * 8.0;
   } else if (which == 1) {
     if (!c.bsr_valid) PZG_TRY(launch_build_bsr(c));
     PZG_TRY(dev_alloc(c.d_xs, n * sizeof(float)));
@@ -783,9 +773,28 @@ int pzg_time_kernel(pzg_ctx *ctx, int32_t which, int32_t nrep, double *avg_ms, d
     PZG_HIP(hipEventRecord(c.ev1, c.stream));
     // compulsory traffic: element maps, corners, coefficients, the element
     // output buffer written and read once, row lists, x, D^-1/2 and y
-    const int64_t nsh = c.tvol.nsh, slots = c.nvol * nsh + c.mf_nbcd * c.tbc.nsh;
-    b = (double)c.nvol * (nsh * 4.0 + c.vol_ncorner * 12.0 + 4.0) + 2.0 * slots * c.nstate * 4.0 +
-        (double)slots * 4.0 + (double)(c.nb + 1) * 8.0 + 3.0 * n * 4.0;
+    // per element: block map and output positions (int32 each), corners,
+    // affine geometry, coefficient; the row lists written and read once; row
+    // pointers; vectors p and D^-1/2 read twice, pt written and read, q written
+    const int64_t nsh = c.tvol.nsh;
+    b = (double)c.nvol * (nsh * 8.0 + c.vol_ncorner * 12.0 + 40.0 + 4.0) +
+        2.0 * c.mf_nout * c.nstate * 4.0 + (double)(c.nb + 1) * 8.0 + 7.0 * n * 4.0;
+  } else if (which == 4) {
+    if (!mf_available(c)) {
+      set_error("pzg_time_kernel: matrix-free operator needs a mesh-built system");
+      return PZG_ESTATE;
+    }
+    PZG_TRY(mf_prepare(c));
+    PZG_TRY(mf_launch_vol(c, nullptr));
+    PZG_HIP(hipEventRecord(c.ev0, c.stream));
+    for (int i = 0; i < nrep; i++) PZG_TRY(mf_launch_vol(c, nullptr));
+    PZG_HIP(hipEventRecord(c.ev1, c.stream));
+    // algorithmic flops (returned in 'bytes'): per element the two
+    // contractions 2 * (2 nq nsh dim ns) and, per point, the gradient and
+    // flux/stress maps 2 * (2 ns dim dim) plus the material law
+    const double nq = c.tvol.nq, nsh = c.tvol.nsh, dim = c.dim, ns = c.nstate;
+    const double law = ns == 3 ? 24.0 : 3.0;
+    b = (double)c.nvol * (2.0 * (2.0 * nq * nsh * dim * ns) + nq * (2.0 * (2.0 * ns * dim * dim) + law));
   } else if (which == 2) {
     if (c.external_csr) {
       set_error("pzg_time_kernel: no mesh to assemble");

@@ -42,21 +42,6 @@ template <class T> int dev_upload(DevBuf &b, const T *h, size_t n, hipStream_t s
   return PZG_OK;
 }
 
-// Value layout of the FP32 block CSR: 1 = component-major ((k*BS+l)*nbf + t),
-// 0 = block-major (t*BS*BS + k*BS + l).
-#ifndef PZG_BSR_SOA
-#define PZG_BSR_SOA 0
-#endif
-__host__ __device__ inline int64_t bsr_index(int64_t t, int c, int64_t nbf, int bb) {
-#if PZG_BSR_SOA
-  (void)bb;
-  return (int64_t)c * nbf + t;
-#else
-  (void)nbf;
-  return t * bb + c;
-#endif
-}
-
 // Learned preconditioner network: hidden features, message-passing layers,
 // leaky-ReLU slope.
 constexpr int NNF = 32;
@@ -120,13 +105,11 @@ struct Ctx {
   DevBuf d_ubia, d_ubja;      // upper block graph: int64 [nb+1], int32 [nbu]
   DevBuf d_fbia, d_fbja;      // full block graph:  int64 [nb+1], int32 [nbf]
   DevBuf d_rank_fwd, d_rank_bwd; // per upper block entry: rank of (a,b) in full row a, (b,a) in row b
-  DevBuf d_fia, d_fja;        // full scalar CSR: int64 [neq+1], int32 [nnz_full]
-  DevBuf d_fa;                // full scalar CSR values fp64
-  DevBuf d_bsr;               // full BSR fp32 values, scaled D^-1/2 K D^-1/2
+  DevBuf d_fa;                // full block CSR values fp64 (K, unscaled), block-major
+  DevBuf d_bsr;               // full block CSR fp32 values, scaled D^-1/2 K D^-1/2
   DevBuf d_dinvsqrt;          // fp64 D^-1/2
   DevBuf d_bjinv;             // fp32 inverse diagonal blocks of the scaled matrix
   bool full_valid = false, bsr_valid = false;
-  bool fja_valid = false;     // full CSR column indices match the current pattern
 
   // ---- assembly (device) -------------------------------------------------
   DevBuf d_xyz, d_vnodes, d_vconn, d_vcoef, d_bnodes, d_bconn, d_bcid, d_bctype, d_bcval;
@@ -146,8 +129,10 @@ struct Ctx {
   bool mf_ready = false, mf_vals_valid = false;
   bool kel_added = false;  // AddKel contributions beyond the mesh's elements
   int64_t mf_nbcd = 0;     // Dirichlet boundary elements
-  DevBuf d_mf_blk, d_mf_xe, d_mf_bblk, d_mf_bm, d_mf_ptr, d_mf_inc, d_mf_tab, d_mf_ye;
-  DevBuf d_mf_coef, d_mf_dis;
+  DevBuf d_mf_blk, d_mf_xe, d_mf_bblk, d_mf_bm, d_mf_ptr, d_mf_pos, d_mf_bpos, d_mf_tab, d_mf_ye;
+  int64_t mf_nout = 0;     // element contributions (entries of the row lists)
+  int mf_resident = 0;     // workgroups of the element kernel resident at once
+  DevBuf d_mf_coef, d_mf_dis, d_mf_mfma, d_mf_geo, d_mf_pt;
 
   // learned preconditioner (pzg_learned.hip)
   bool nn_ready = false;
@@ -193,8 +178,13 @@ int nn_store_pair(Ctx &c, const float *r, const float *e_minus_bjr, int64_t slot
 int nn_collect(Ctx &c, int nrhs, const int32_t *rec, int nrec, uint64_t seed);
 int mf_setup(Ctx &c);
 int mf_prepare(Ctx &c);
+// volume element kernel alone (for timing)
+int mf_launch_vol(Ctx &c, const SolveScalars *sc);
+// pt = D^-1/2 p (input of the element kernels)
+int mf_scale(Ctx &c, const float *p, const SolveScalars *sc);
+// q = S p; scaled_ready: pt already holds D^-1/2 p (fused into the p update)
 int mf_apply(Ctx &c, const float *p, const float *r, float *q, const SolveScalars *sc,
-             double *partials, int *gather_grid);
+             double *partials, int *gather_grid, bool scaled_ready = false);
 // the matrix-free operator reproduces the assembled K (mesh-built, no AddKel extras)
 inline bool mf_available(const Ctx &c) {
   return c.has_mesh && c.has_struct && !c.external_csr && !c.kel_added && c.assembled;

@@ -51,9 +51,10 @@ struct MfArgs {
   const int32_t *blk;   // [nvol][NSH] active block of each shape (-1: none)
   const float *xe;      // [nvol][NC*3] corner coordinates
   const float *coef;    // [nvol] E (elasticity), permeability or scale
-  const float *dis;     // [neq] D^-1/2
-  const float *p;       // input vector (scaled unknowns)
-  float *ye;            // [slot][NS] element outputs, already scaled by D^-1/2
+  const float *geo;     // [nvol][10] affine elements: J^-1 (row m: d xi_m/d x), |det J|; else [9] = -1
+  const float *pt;      // input, already multiplied by D^-1/2
+  const int32_t *pos;   // [nvol][NSH] position of (element, shape) in its row's output list
+  float *ye;            // [position][NS] element outputs K_e pt_e, grouped by block row
 };
 
 __device__ inline float4 f4(float a, float b, float c) { return make_float4(a, b, c, 0.f); }
@@ -83,7 +84,7 @@ __global__ __launch_bounds__(MF_TPB) void k_mf_vol(MfArgs a, const SolveScalars 
       b = a.blk[e * NSH + l];
       float u[3] = {0.f, 0.f, 0.f};
       if (b >= 0)
-        for (int k = 0; k < NS; k++) u[k] = a.p[(int64_t)b * NS + k] * a.dis[(int64_t)b * NS + k];
+        for (int k = 0; k < NS; k++) u[k] = a.pt[(int64_t)b * NS + k];
       s_u[el][l] = f4(u[0], u[1], u[2]);
     }
     if (live)
@@ -200,10 +201,215 @@ __global__ __launch_bounds__(MF_TPB) void k_mf_vol(MfArgs a, const SolveScalars 
           y[k] += d.x * P.x + d.y * P.y + (DIM == 3 ? d.z * P.z : 0.f);
         }
       }
-      const int64_t slot = e * NSH + l;
+      const int32_t t = a.pos[e * NSH + l];
+      if (t >= 0)
+#pragma unroll
+        for (int k = 0; k < NS; k++) a.ye[(int64_t)t * NS + k] = y[k];
+    }
+    __syncthreads();
+  }
+}
+
+// Hexahedron p=2 (27 shapes, 27 points) on the matrix cores.  The two
+// contractions of the element product are GEMMs with the constant shape
+// derivative tables as the left operand:
+//   phase 1: G_m[q][(e,k)] = sum_s dphi_m(q,s) u_e[s][k]      (m = d/dxi_m)
+//   phase 3: y[s][(e,k)]  = sum_(m,q) dphi_m(q,s) P[q][(e,k)][m]
+// as v_mfma_f32_16x16x4f32 tiles (FP32 in, FP32 accumulate): rows = points
+// or shapes (27 -> two 16-row tiles), columns = (element, component) pairs of
+// 16/NS elements per wavefront, k = shapes (7 steps of 4) or (m, point)
+// (21 steps of 4).  The point-wise part (Jacobian, inverse, flux/stress) runs
+// on the vector ALUs between the two, through LDS.
+// Operand layouts of the 16x16x4 instruction (wave64): A: lane l holds
+// A[l%16][l/16]; B: lane l holds B[l/16][l%16]; D: lane l, component r holds
+// D[4*(l/16)+r][l%16].
+#ifndef PZG_MF_MFMA
+#define PZG_MF_MFMA 1
+#endif
+#ifndef PZG_MF_XCD
+#define PZG_MF_XCD 1
+#endif
+
+typedef float mf_f4 __attribute__((ext_vector_type(4)));
+
+struct MfMfmaTab {
+  float a1[3][2][7][64];  // phase 1 A operands per (m, row tile, k step, lane)
+  float a3[2][21][64];    // phase 3 A operands per (row tile, k step, lane)
+};
+
+template <int NS>
+__global__ __launch_bounds__(MF_TPB) void k_mf_hex27_mfma(MfArgs a, const MfMfmaTab *__restrict__ mt,
+                                                         const SolveScalars *sc) {
+  if (sc && sc->done) return;
+  constexpr int EPT = 16 / NS;  // elements per wavefront tile
+  constexpr int NCOL = EPT * NS;
+  constexpr int NW = MF_TPB / 64;
+  constexpr int NG = NS * 3;
+  __shared__ float4 s_cd[27][8];
+  __shared__ float s_w[27];
+  __shared__ float s_g[NW][EPT][27][NG];
+  __shared__ float s_x[NW][EPT][24 + 10];  // corners, then J^-1 and |det J|
+  __shared__ __align__(16) MfMfmaTab s_mt;
+  const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, lr = l >> 4;
+  for (int t = tid; t < 27 * 8; t += MF_TPB) s_cd[t / 8][t % 8] = a.tab->cd[t / 8][t % 8];
+  if (tid < 27) s_w[tid] = a.tab->w[tid];
+  {
+    const float4 *src = reinterpret_cast<const float4 *>(mt);
+    float4 *dst = reinterpret_cast<float4 *>(&s_mt);
+    for (int t = tid; t < (int)(sizeof(MfMfmaTab) / 16); t += MF_TPB) dst[t] = src[t];
+  }
+  __syncthreads();
+  const float(&A1)[3][2][7][64] = s_mt.a1;
+  const float(&A3)[2][21][64] = s_mt.a3;
+  const int j = l & 15, ee = j / NS, kc = j % NS;  // column -> (element, component)
+  const bool colv = j < NCOL;
+#if PZG_MF_XCD
+  int64_t elo, ehi;
+  int xi, xg;
+  xcd_slab(a.nvol, elo, ehi, xi, xg);
+  for (int64_t g0 = elo + (int64_t)xi * (NW * EPT); g0 < ehi; g0 += (int64_t)xg * (NW * EPT)) {
+    const int64_t eend = ehi;
+#else
+  for (int64_t g0 = (int64_t)blockIdx.x * (NW * EPT); g0 < a.nvol; g0 += (int64_t)gridDim.x * (NW * EPT)) {
+    const int64_t eend = a.nvol;
+#endif
+    const int64_t eb = g0 + (int64_t)wv * EPT;
+    for (int t = l; t < EPT * 34; t += 64) {
+      const int el = t / 34, k = t % 34;
+      const int64_t e = eb + el;
+      float v = 0.f;
+      if (e < eend) v = k < 24 ? a.xe[e * 24 + k] : a.geo[e * 10 + (k - 24)];
+      s_x[wv][el][k] = v;
+    }
+    const int64_t e = eb + ee;
+    const bool ev = colv && e < eend;
+    float ub[7];
+#pragma unroll
+    for (int t = 0; t < 7; t++) {
+      const int s = 4 * t + lr;
+      float v = 0.f;
+      if (ev && s < 27) {
+        const int32_t b = a.blk[e * 27 + s];
+        if (b >= 0) v = a.pt[(int64_t)b * NS + kc];
+      }
+      ub[t] = v;
+    }
+    mf_f4 acc[3][2];
+#pragma unroll
+    for (int m = 0; m < 3; m++)
+#pragma unroll
+      for (int r = 0; r < 2; r++) {
+        mf_f4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < 7; t++) c = __builtin_amdgcn_mfma_f32_16x16x4f32(A1[m][r][t][l], ub[t], c, 0, 0, 0);
+        acc[m][r] = c;
+      }
+    if (colv) {
+#pragma unroll
+      for (int m = 0; m < 3; m++)
+#pragma unroll
+        for (int r = 0; r < 2; r++)
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            const int q = 16 * r + 4 * lr + i;
+            if (q < 27) s_g[wv][ee][q][kc * 3 + m] = acc[m][r][i];
+          }
+    }
+    __syncthreads();
+    // point-wise: Jacobian, physical gradient, flux/stress, weighted back-map
+    for (int pi = l; pi < EPT * 27; pi += 64) {
+      const int el = pi / 27, q = pi % 27;
+      if (eb + el >= eend) continue;
+      const float *xg = s_x[wv][el];
+      float inv[3][3], adet;
+      if (xg[24 + 9] >= 0.f) {  // affine element: constant J^-1 and |det J|
+#pragma unroll
+        for (int m = 0; m < 3; m++)
+#pragma unroll
+          for (int d = 0; d < 3; d++) inv[m][d] = xg[24 + 3 * m + d];
+        adet = xg[24 + 9];
+      } else {
+        float J[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+          const float4 d = s_cd[q][c];
+#pragma unroll
+          for (int i = 0; i < 3; i++) {
+            const float x = xg[3 * c + i];
+            J[i][0] += x * d.x;
+            J[i][1] += x * d.y;
+            J[i][2] += x * d.z;
+          }
+        }
+        const float det = -J[0][2] * J[1][1] * J[2][0] + J[0][1] * J[1][2] * J[2][0] +
+                          J[0][2] * J[1][0] * J[2][1] - J[0][0] * J[1][2] * J[2][1] -
+                          J[0][1] * J[1][0] * J[2][2] + J[0][0] * J[1][1] * J[2][2];
+        const float id = 1.f / det;
+        inv[0][0] = (-J[1][2] * J[2][1] + J[1][1] * J[2][2]) * id;
+        inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+        inv[0][2] = (-J[0][2] * J[1][1] + J[0][1] * J[1][2]) * id;
+        inv[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
+        inv[1][1] = (-J[0][2] * J[2][0] + J[0][0] * J[2][2]) * id;
+        inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+        inv[2][0] = (-J[1][1] * J[2][0] + J[1][0] * J[2][1]) * id;
+        inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+        inv[2][2] = (-J[0][1] * J[1][0] + J[0][0] * J[1][1]) * id;
+        adet = fabsf(det);
+      }
+      const float wdet = s_w[q] * adet;
+      const float coef = a.coef[eb + el];
+      float *gp = s_g[wv][el][q];
+      float g[NS][3];
 #pragma unroll
       for (int k = 0; k < NS; k++)
-        a.ye[slot * NS + k] = b >= 0 ? y[k] * a.dis[(int64_t)b * NS + k] : 0.f;
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+          g[k][d] = gp[k * 3 + 0] * inv[0][d] + gp[k * 3 + 1] * inv[1][d] + gp[k * 3 + 2] * inv[2][d];
+      float S[NS][3];
+      if constexpr (NS == 3) {
+        const float nu = a.nu;
+        const float C1 = coef / (2.f + 2.f * nu);
+        const float den = -1.f + nu + 2.f * nu * nu;
+        const float C2 = coef * nu / den;
+        const float C3 = coef * (nu - 1.f) / den;
+        const float tr = g[0][0] + g[1][1] + g[2][2];
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+          for (int d = 0; d < 3; d++)
+            S[k][d] = d == k ? C3 * g[k][k] - C2 * (tr - g[k][k]) : C1 * (g[k][d] + g[d][k]);
+      } else {
+#pragma unroll
+        for (int d = 0; d < 3; d++) S[0][d] = coef * g[0][d];
+      }
+#pragma unroll
+      for (int k = 0; k < NS; k++)
+#pragma unroll
+        for (int m = 0; m < 3; m++)
+          gp[k * 3 + m] = wdet * (S[k][0] * inv[m][0] + S[k][1] * inv[m][1] + S[k][2] * inv[m][2]);
+    }
+    __syncthreads();
+    mf_f4 y[2];
+#pragma unroll
+    for (int r = 0; r < 2; r++) y[r] = mf_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < 21; t++) {
+      const int kk = 4 * t + lr, m = kk / 28, q = kk % 28;
+      const float bv = (colv && q < 27) ? s_g[wv][ee][q][kc * 3 + m] : 0.f;
+#pragma unroll
+      for (int r = 0; r < 2; r++) y[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(A3[r][t][l], bv, y[r], 0, 0, 0);
+    }
+    if (ev) {
+#pragma unroll
+      for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int s = 16 * r + 4 * lr + i;
+          if (s < 27) {
+            const int32_t t = a.pos[e * 27 + s];
+            if (t >= 0) a.ye[(int64_t)t * NS + kc] = y[r][i];
+          }
+        }
     }
     __syncthreads();
   }
@@ -214,10 +420,9 @@ __global__ __launch_bounds__(MF_TPB) void k_mf_vol(MfArgs a, const SolveScalars 
 template <int NSHB, int NS>
 __global__ __launch_bounds__(MF_TPB) void k_mf_bc(int64_t nbcd, const int32_t *__restrict__ bblk,
                                                   const float *__restrict__ bm,
-                                                  const float *__restrict__ dis,
-                                                  const float *__restrict__ p,
-                                                  float *__restrict__ ye, int64_t slot0,
-                                                  const SolveScalars *sc) {
+                                                  const float *__restrict__ pt,
+                                                  const int32_t *__restrict__ bpos,
+                                                  float *__restrict__ ye, const SolveScalars *sc) {
   if (sc && sc->done) return;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nbcd * NSHB) return;
@@ -229,40 +434,46 @@ __global__ __launch_bounds__(MF_TPB) void k_mf_bc(int64_t nbcd, const int32_t *_
     const int32_t bj = bblk[e * NSHB + j];
     if (bj < 0) continue;
     const float m = bm[(e * NSHB + i) * NSHB + j];
-    for (int k = 0; k < NS; k++) y[k] += m * p[(int64_t)bj * NS + k] * dis[(int64_t)bj * NS + k];
+    for (int k = 0; k < NS; k++) y[k] += m * pt[(int64_t)bj * NS + k];
   }
-  const int32_t bi = bblk[e * NSHB + i];
-  for (int k = 0; k < NS; k++)
-    ye[(slot0 + t) * NS + k] = bi >= 0 ? y[k] * dis[(int64_t)bi * NS + k] : 0.f;
+  const int32_t o = bpos[t];
+  if (o >= 0)
+    for (int k = 0; k < NS; k++) ye[(int64_t)o * NS + k] = y[k];
 }
 
 // q[A] = sum of the element outputs of block row A (fixed order); with
 // partials: slots 0/1 <- per-block sums of p.q and p.r (flexible CG alpha).
 template <int NS>
 __global__ __launch_bounds__(MF_TPB) void k_mf_gather(int64_t nb, const int64_t *__restrict__ ptr,
-                                                      const int32_t *__restrict__ inc,
                                                       const float *__restrict__ ye,
+                                                      const float *__restrict__ dis,
                                                       const float *__restrict__ p,
                                                       const float *__restrict__ r,
                                                       float *__restrict__ q, const SolveScalars *sc,
                                                       double *partials) {
   if (sc && sc->done) return;
   double pq = 0., pr = 0.;
+#if PZG_MF_XCD
+  int64_t lo, hi;
+  int xi, xg;
+  xcd_slab(nb, lo, hi, xi, xg);
+  for (int64_t A = lo + (int64_t)xi * MF_TPB + threadIdx.x; A < hi; A += (int64_t)xg * MF_TPB) {
+#else
   for (int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; A < nb;
        A += (int64_t)gridDim.x * blockDim.x) {
+#endif
     float acc[NS];
     for (int k = 0; k < NS; k++) acc[k] = 0.f;
     const int64_t t1 = ptr[A + 1];
-    for (int64_t t = ptr[A]; t < t1; t++) {
-      const int64_t s = inc[t];
-      for (int k = 0; k < NS; k++) acc[k] += ye[s * NS + k];
-    }
+    for (int64_t t = ptr[A]; t < t1; t++)
+      for (int k = 0; k < NS; k++) acc[k] += ye[t * NS + k];
     for (int k = 0; k < NS; k++) {
       const int64_t i = A * NS + k;
-      q[i] = acc[k];
+      const float qi = acc[k] * dis[i];
+      q[i] = qi;
       if (partials) {
         const float pi = p[i];
-        pq += (double)acc[k] * pi;
+        pq += (double)qi * pi;
         pr += (double)pi * r[i];
       }
     }
@@ -271,6 +482,13 @@ __global__ __launch_bounds__(MF_TPB) void k_mf_gather(int64_t nb, const int64_t 
     double vv[2] = {pq, pr};
     block_partials<2>(vv, partials);
   }
+}
+
+__global__ void k_mf_scale(int64_t n, const float *__restrict__ p, const float *__restrict__ dis,
+                           float *__restrict__ pt, const SolveScalars *sc) {
+  if (sc && sc->done) return;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) pt[i] = p[i] * dis[i];
 }
 
 __global__ void k_d2f(int64_t n, const double *__restrict__ x, float *__restrict__ y) {
@@ -325,11 +543,50 @@ int mf_setup(Ctx &c) {
   }
   const int64_t nvol = c.nvol, nb = c.nb;
   std::vector<int32_t> blk(nvol * NSH);
-  std::vector<float> xe(nvol * NC * 3);
+  std::vector<float> xe(nvol * NC * 3), geo(nvol * 10, 0.f);
   for (int64_t e = 0; e < nvol; e++) {
     for (int s = 0; s < NSH; s++) blk[e * NSH + s] = (int32_t)c.aidx[c.vol_conn[e * NSD + T.shape_side[s]]];
     for (int k = 0; k < NC; k++)
       for (int i = 0; i < 3; i++) xe[(e * NC + k) * 3 + i] = (float)c.xyz[3 * c.vol_nodes[e * NC + k] + i];
+    geo[e * 10 + 9] = -1.f;
+    if (c.dim == 3 && NC == 8) {
+      // affine hexahedron: x_c = x_0 + sum_j b_j(c) (x_{e_j} - x_0), e_j = corners 1, 3, 4
+      const double *x[8];
+      for (int k = 0; k < 8; k++) x[k] = &c.xyz[3 * c.vol_nodes[e * 8 + k]];
+      static const int bits[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                     {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+      static const int axis_corner[3] = {1, 3, 4};
+      double J[3][3], scale = 0, dev = 0;
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+          J[i][j] = 0.5 * (x[axis_corner[j]][i] - x[0][i]);
+          scale = std::max(scale, std::fabs(J[i][j]));
+        }
+      for (int k = 0; k < 8; k++)
+        for (int i = 0; i < 3; i++) {
+          double v = x[0][i];
+          for (int j = 0; j < 3; j++) v += 2.0 * J[i][j] * bits[k][j];
+          dev = std::max(dev, std::fabs(v - x[k][i]));
+        }
+      const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                         J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                         J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+      if (dev <= 1e-12 * scale && det != 0.0) {
+        double inv[3][3];  // inv[m][d] = d xi_m / d x_d
+        inv[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+        inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+        inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+        inv[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+        inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+        inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+        inv[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+        inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+        inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+        for (int m = 0; m < 3; m++)
+          for (int d = 0; d < 3; d++) geo[e * 10 + 3 * m + d] = (float)inv[m][d];
+        geo[e * 10 + 9] = (float)std::fabs(det);
+      }
+    }
   }
   // Dirichlet boundary elements and their penalty matrices (FP64, then FP32)
   const ElemTables &B = c.tbc;
@@ -381,10 +638,16 @@ int mf_setup(Ctx &c) {
   for (int32_t b : blk) if (b >= 0) ptr[b + 1]++;
   for (int32_t b : bblk) if (b >= 0) ptr[b + 1]++;
   for (int64_t i = 0; i < nb; i++) ptr[i + 1] += ptr[i];
-  std::vector<int32_t> inc(ptr[nb]);
+  // each (element, shape) writes its output at its place in the row's list:
+  // volume elements in element order, then boundary elements (fixed order)
+  if (ptr[nb] >= ((int64_t)1 << 31)) {
+    set_error("matrix-free operator: too many element contributions");
+    return PZG_EUNSUP;
+  }
+  std::vector<int32_t> pos(slot0, -1), bpos(nbcd * NSHB, -1);
   std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
-  for (int64_t s = 0; s < slot0; s++) if (blk[s] >= 0) inc[fill[blk[s]]++] = (int32_t)s;
-  for (int64_t s = 0; s < nbcd * NSHB; s++) if (bblk[s] >= 0) inc[fill[bblk[s]]++] = (int32_t)(slot0 + s);
+  for (int64_t s = 0; s < slot0; s++) if (blk[s] >= 0) pos[s] = (int32_t)fill[blk[s]]++;
+  for (int64_t s = 0; s < nbcd * NSHB; s++) if (bblk[s] >= 0) bpos[s] = (int32_t)fill[bblk[s]]++;
   MfTab tab;
   std::memset(&tab, 0, sizeof(tab));
   for (int q = 0; q < T.nq; q++) {
@@ -396,19 +659,42 @@ int mf_setup(Ctx &c) {
       tab.cd[q][k] = make_float4((float)T.cdphi[q][0][k], (float)T.cdphi[q][1][k],
                                  c.dim == 3 ? (float)T.cdphi[q][2][k] : 0.f, 0.f);
   }
+  std::vector<MfMfmaTab> mtab(1);
+  std::memset(mtab.data(), 0, sizeof(MfMfmaTab));
+  if (c.dim == 3 && NSH == 27 && T.nq == 27) {
+    MfMfmaTab &M = mtab[0];
+    for (int l = 0; l < 64; l++) {
+      for (int m = 0; m < 3; m++)
+        for (int r = 0; r < 2; r++)
+          for (int t = 0; t < 7; t++) {
+            const int q = 16 * r + l % 16, sh = 4 * t + l / 16;
+            M.a1[m][r][t][l] = (q < 27 && sh < 27) ? (float)T.dphi[q][m][sh] : 0.f;
+          }
+      for (int r = 0; r < 2; r++)
+        for (int t = 0; t < 21; t++) {
+          const int sh = 16 * r + l % 16, kk = 4 * t + l / 16, m = kk / 28, q = kk % 28;
+          M.a3[r][t][l] = (sh < 27 && q < 27) ? (float)T.dphi[q][m][sh] : 0.f;
+        }
+    }
+  }
   hipStream_t s = c.stream;
+  PZG_TRY(dev_upload(c.d_mf_mfma, mtab.data(), 1, s));
   PZG_TRY(dev_upload(c.d_mf_blk, blk.data(), blk.size(), s));
   PZG_TRY(dev_upload(c.d_mf_xe, xe.data(), xe.size(), s));
+  PZG_TRY(dev_upload(c.d_mf_geo, geo.data(), geo.size(), s));
+  PZG_TRY(dev_alloc(c.d_mf_pt, sizeof(float) * std::max<int64_t>(c.neq, 1)));
   PZG_TRY(dev_upload(c.d_mf_bblk, bblk.data(), bblk.size(), s));
   PZG_TRY(dev_upload(c.d_mf_bm, bm.data(), bm.size(), s));
   PZG_TRY(dev_upload(c.d_mf_ptr, ptr.data(), ptr.size(), s));
-  PZG_TRY(dev_upload(c.d_mf_inc, inc.data(), inc.size(), s));
+  PZG_TRY(dev_upload(c.d_mf_pos, pos.data(), pos.size(), s));
+  PZG_TRY(dev_upload(c.d_mf_bpos, bpos.data(), bpos.size(), s));
   PZG_TRY(dev_upload(c.d_mf_tab, &tab, 1, s));
-  PZG_TRY(dev_alloc(c.d_mf_ye, sizeof(float) * std::max<int64_t>(nslots, 1) * ns));
+  PZG_TRY(dev_alloc(c.d_mf_ye, sizeof(float) * std::max<int64_t>(ptr[nb], 1) * ns));
   PZG_TRY(dev_alloc(c.d_mf_coef, sizeof(float) * std::max<int64_t>(nvol, 1)));
   PZG_TRY(dev_alloc(c.d_mf_dis, sizeof(float) * std::max<int64_t>(c.neq, 1)));
   PZG_HIP(hipStreamSynchronize(s));
   c.mf_nbcd = nbcd;
+  c.mf_nout = ptr[nb];
   c.mf_ready = true;
   c.mf_vals_valid = false;
   return PZG_OK;
@@ -437,8 +723,17 @@ int mf_prepare(Ctx &c) {
 
 // q = S p.  With partials: per-block sums of p.q (slot 0) and p.r (slot 1);
 // *gather_grid receives the number of blocks that wrote partials.
-int mf_apply(Ctx &c, const float *p, const float *r, float *q, const SolveScalars *sc,
-             double *partials, int *gather_grid) {
+int mf_scale(Ctx &c, const float *p, const SolveScalars *sc) {
+  if (c.neq > 0) {
+    hipLaunchKernelGGL(k_mf_scale, dim3((unsigned)((c.neq + MF_TPB - 1) / MF_TPB)), dim3(MF_TPB), 0,
+                       c.stream, c.neq, p, c.d_mf_dis.as<float>(), c.d_mf_pt.as<float>(), sc);
+    PZG_HIP(hipGetLastError());
+  }
+  return PZG_OK;
+}
+
+// Volume element kernel only (K_e pt_e into the row lists).
+int mf_launch_vol(Ctx &c, const SolveScalars *sc) {
   MfArgs a;
   a.nvol = c.nvol;
   a.nq = c.tvol.nq;
@@ -447,12 +742,31 @@ int mf_apply(Ctx &c, const float *p, const float *r, float *q, const SolveScalar
   a.blk = c.d_mf_blk.as<int32_t>();
   a.xe = c.d_mf_xe.as<float>();
   a.coef = c.d_mf_coef.as<float>();
-  a.dis = c.d_mf_dis.as<float>();
-  a.p = p;
+  a.geo = c.d_mf_geo.as<float>();
+  a.pt = c.d_mf_pt.as<float>();
+  a.pos = c.d_mf_pos.as<int32_t>();
   a.ye = c.d_mf_ye.as<float>();
   const int nsh = c.tvol.nsh;
   if (c.nvol > 0) {
-    if (c.dim == 3 && c.nstate == 3) {
+    const bool mfma = PZG_MF_MFMA && c.dim == 3 && nsh == 27 && c.tvol.nq == 27;
+    if (mfma) {
+      // persistent grid: as many workgroups as fit on the device at once
+      int64_t groups = (c.nvol + 4 * (16 / c.nstate) - 1) / (4 * (16 / c.nstate));
+      if (c.mf_resident <= 0) {
+        int per_cu = 0, ncu = 0;
+        PZG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device));
+        if (c.nstate == 3)
+          PZG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mf_hex27_mfma<3>, MF_TPB, 0));
+        else
+          PZG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mf_hex27_mfma<1>, MF_TPB, 0));
+        c.mf_resident = std::max(1, per_cu) * std::max(1, ncu);
+      }
+      dim3 grid((unsigned)std::min<int64_t>(std::max<int64_t>(groups, 1), c.mf_resident));
+      const MfMfmaTab *mt = c.d_mf_mfma.as<MfMfmaTab>();
+      if (c.nstate == 3) hipLaunchKernelGGL(k_mf_hex27_mfma<3>, grid, dim3(MF_TPB), 0, c.stream, a, mt, sc);
+      else hipLaunchKernelGGL(k_mf_hex27_mfma<1>, grid, dim3(MF_TPB), 0, c.stream, a, mt, sc);
+      PZG_HIP(hipGetLastError());
+    } else if (c.dim == 3 && c.nstate == 3) {
       if (nsh == 27) PZG_TRY((launch_vol_lpe<3, 8, 27, 3>(c, a, sc)));
       else PZG_TRY((launch_vol_lpe<3, 8, 8, 3>(c, a, sc)));
     } else if (c.dim == 3 && c.nstate == 1) {
@@ -466,7 +780,16 @@ int mf_apply(Ctx &c, const float *p, const float *r, float *q, const SolveScalar
       return PZG_EUNSUP;
     }
   }
-  const int64_t nbcd = c.mf_nbcd, slot0 = c.nvol * nsh;
+  return PZG_OK;
+}
+
+int mf_apply(Ctx &c, const float *p, const float *r, float *q, const SolveScalars *sc,
+             double *partials, int *gather_grid, bool scaled_ready) {
+  const float *dis = c.d_mf_dis.as<float>();
+  if (!scaled_ready) PZG_TRY(mf_scale(c, p, sc));
+  PZG_TRY(mf_launch_vol(c, sc));
+  const int nsh = c.tvol.nsh;
+  const int64_t nbcd = c.mf_nbcd;
   const int nshb = c.tbc.nsh;
   if (nbcd > 0) {
     const int64_t th = nbcd * nshb;
@@ -474,19 +797,20 @@ int mf_apply(Ctx &c, const float *p, const float *r, float *q, const SolveScalar
     const int32_t *bb = c.d_mf_bblk.as<int32_t>();
     const float *bm = c.d_mf_bm.as<float>();
     float *ye = c.d_mf_ye.as<float>();
-    const float *dis = c.d_mf_dis.as<float>();
+    const float *pt = c.d_mf_pt.as<float>();
+    const int32_t *bpos = c.d_mf_bpos.as<int32_t>();
     if (c.nstate == 3 && nshb == 9)
-      hipLaunchKernelGGL((k_mf_bc<9, 3>), g, dim3(MF_TPB), 0, c.stream, nbcd, bb, bm, dis, p, ye, slot0, sc);
+      hipLaunchKernelGGL((k_mf_bc<9, 3>), g, dim3(MF_TPB), 0, c.stream, nbcd, bb, bm, pt, bpos, ye, sc);
     else if (c.nstate == 3 && nshb == 4)
-      hipLaunchKernelGGL((k_mf_bc<4, 3>), g, dim3(MF_TPB), 0, c.stream, nbcd, bb, bm, dis, p, ye, slot0, sc);
+      hipLaunchKernelGGL((k_mf_bc<4, 3>), g, dim3(MF_TPB), 0, c.stream, nbcd, bb, bm, pt, bpos, ye, sc);
     else if (c.nstate == 1 && nshb == 9)
-      hipLaunchKernelGGL((k_mf_bc<9, 1>), g, dim3(MF_TPB), 0, c.stream, nbcd, bb, bm, dis, p, ye, slot0, sc);
+      hipLaunchKernelGGL((k_mf_bc<9, 1>), g, dim3(MF_TPB), 0, c.stream, nbcd, bb, bm, pt, bpos, ye, sc);
     else if (c.nstate == 1 && nshb == 4)
-      hipLaunchKernelGGL((k_mf_bc<4, 1>), g, dim3(MF_TPB), 0, c.stream, nbcd, bb, bm, dis, p, ye, slot0, sc);
+      hipLaunchKernelGGL((k_mf_bc<4, 1>), g, dim3(MF_TPB), 0, c.stream, nbcd, bb, bm, pt, bpos, ye, sc);
     else if (c.nstate == 1 && nshb == 3)
-      hipLaunchKernelGGL((k_mf_bc<3, 1>), g, dim3(MF_TPB), 0, c.stream, nbcd, bb, bm, dis, p, ye, slot0, sc);
+      hipLaunchKernelGGL((k_mf_bc<3, 1>), g, dim3(MF_TPB), 0, c.stream, nbcd, bb, bm, pt, bpos, ye, sc);
     else if (c.nstate == 1 && nshb == 2)
-      hipLaunchKernelGGL((k_mf_bc<2, 1>), g, dim3(MF_TPB), 0, c.stream, nbcd, bb, bm, dis, p, ye, slot0, sc);
+      hipLaunchKernelGGL((k_mf_bc<2, 1>), g, dim3(MF_TPB), 0, c.stream, nbcd, bb, bm, pt, bpos, ye, sc);
     else {
       set_error("matrix-free operator: unsupported boundary element");
       return PZG_EUNSUP;
@@ -497,11 +821,11 @@ int mf_apply(Ctx &c, const float *p, const float *r, float *q, const SolveScalar
   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(gg, MAXBLK));
   if (c.nstate == 3)
     hipLaunchKernelGGL(k_mf_gather<3>, dim3(grid), dim3(MF_TPB), 0, c.stream, c.nb,
-                       c.d_mf_ptr.as<int64_t>(), c.d_mf_inc.as<int32_t>(), c.d_mf_ye.as<float>(), p, r,
+                       c.d_mf_ptr.as<int64_t>(), c.d_mf_ye.as<float>(), dis, p, r,
                        q, sc, partials);
   else
     hipLaunchKernelGGL(k_mf_gather<1>, dim3(grid), dim3(MF_TPB), 0, c.stream, c.nb,
-                       c.d_mf_ptr.as<int64_t>(), c.d_mf_inc.as<int32_t>(), c.d_mf_ye.as<float>(), p, r,
+                       c.d_mf_ptr.as<int64_t>(), c.d_mf_ye.as<float>(), dis, p, r,
                        q, sc, partials);
   PZG_HIP(hipGetLastError());
   if (gather_grid) *gather_grid = grid;

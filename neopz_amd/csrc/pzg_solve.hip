@@ -5,8 +5,9 @@
 //  (z = r / diag, Matrix/pzmatrix.cpp:642-667), x0 = 0, stop on the recursive
 //  ||r||/||b|| <= tol.  The reference's Jacobi step also runs a residual SpMV
 //  whose result it discards; it has no effect on the iterates and is not run.
-//  Matrix: full CSR, FP64 values, int32 columns (expanded once per assembly
-//  from the reference upper CSR).
+//  Matrix: full (both triangles) block CSR of nstate x nstate blocks, FP64
+//  values, int32 block columns, expanded once per assembly from the
+//  reference upper CSR.
 //
 //  MIXED.  FP32 flexible CG (Notay's FCG(1)) on S = D^-1/2 K D^-1/2 stored as
 //  FP32 block CSR, preconditioned by the inverse diagonal blocks, with FP64
@@ -43,82 +44,140 @@ static int grid_for(int64_t n, int per_block) {
 }
 
 // ----------------------------------------------------------------------------
-#ifndef PZG_NT_LOADS
-#define PZG_NT_LOADS 0
+// Block CSR SpMV, shared by the FP64 path (unscaled K) and the FP32 path
+// (scaled S).  Every block row is handled by BSR_L lanes; each lane takes
+// BSR_U blocks per pass and issues all their loads before the products, so a
+// wavefront keeps 64/BSR_L rows x BSR_U blocks of loads in flight.  Values
+// are block-major (block t at [t*BS*BS, (t+1)*BS*BS), row-major inside).
+#ifndef PZG_BSR_L
+#define PZG_BSR_L 16
 #endif
-template <class T> __device__ inline T ld_stream0(const T *p) {
-#if PZG_NT_LOADS
-  return __builtin_nontemporal_load(p);
-#else
-  return *p;
+#ifndef PZG_BSR_U
+#define PZG_BSR_U 4
 #endif
-}
-// FP64 full CSR SpMV; VL lanes per row, U entries per lane loaded before use
-// so that a whole row (up to VL*U entries) is in flight at once.
-template <int VL>
-__device__ inline double row_dot64(int64_t r, const int64_t *__restrict__ ia,
-                                   const int32_t *__restrict__ ja, const double *__restrict__ a,
-                                   const double *__restrict__ x, int sub) {
-  constexpr int U = 4;
-  double s = 0.;
-  const int64_t k1 = ia[r + 1];
-  for (int64_t base = ia[r] + sub; base < k1; base += VL * U) {
-    double av[U];
-    int32_t cv[U];
+constexpr int BSR_L = PZG_BSR_L;
+constexpr int BSR_U = PZG_BSR_U;
+
+template <class T, int BS>
+__device__ inline void bsr_row(int64_t A, const int64_t *__restrict__ bia,
+                               const int32_t *__restrict__ bja, const T *__restrict__ v,
+                               const T *__restrict__ x, int sub, T (&out)[BS]) {
+  constexpr int BB = BS * BS;
+  T acc[BS];
+  for (int k = 0; k < BS; k++) acc[k] = T(0);
+  const int64_t b0 = bia[A], b1 = bia[A + 1];
+  for (int64_t base = b0 + sub; base < b1; base += BSR_L * BSR_U) {
+    T m[BSR_U][BB];
+    int64_t col[BSR_U];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int64_t k = base + u * VL;
-      const bool ok = k < k1;
-      av[u] = ok ? ld_stream0(&a[k]) : 0.;
-      cv[u] = ok ? ld_stream0(&ja[k]) : 0;
+    for (int u = 0; u < BSR_U; u++) {
+      const int64_t t = base + BSR_L * u;
+      if (t < b1) {
+        col[u] = bja[t];
+#pragma unroll
+        for (int c = 0; c < BB; c++) m[u][c] = v[t * BB + c];
+      } else {
+        col[u] = 0;
+#pragma unroll
+        for (int c = 0; c < BB; c++) m[u][c] = T(0);
+      }
     }
-    double xv[U];
+    T xv[BSR_U][BS];
 #pragma unroll
-    for (int u = 0; u < U; u++) xv[u] = x[cv[u]];
+    for (int u = 0; u < BSR_U; u++)
 #pragma unroll
-    for (int u = 0; u < U; u++) s += av[u] * xv[u];
+      for (int l = 0; l < BS; l++) xv[u][l] = x[col[u] * BS + l];
+#pragma unroll
+    for (int u = 0; u < BSR_U; u++)
+#pragma unroll
+      for (int k = 0; k < BS; k++)
+#pragma unroll
+        for (int l = 0; l < BS; l++) acc[k] += m[u][k * BS + l] * xv[u][l];
   }
-  for (int o = VL / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, VL);
-  return s;
-}
-
-template <int VL>
-__global__ __launch_bounds__(TPB) void k_spmv64(int64_t n, const int64_t *__restrict__ ia,
-                                                const int32_t *__restrict__ ja,
-                                                const double *__restrict__ a,
-                                                const double *__restrict__ x,
-                                                double *__restrict__ y) {
-  const int sub = threadIdx.x % VL;
-  int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / VL;
-  const int64_t ng = (int64_t)gridDim.x * blockDim.x / VL;
-  for (int64_t r = g; r < n; r += ng) {
-    double s = row_dot64<VL>(r, ia, ja, a, x, sub);
-    if (sub == 0) y[r] = s;
+#pragma unroll
+  for (int k = 0; k < BS; k++) {
+    T s = acc[k];
+    for (int o = BSR_L / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, BSR_L);
+    out[k] = s;
   }
 }
 
-// q = A p, then alpha = rho / (p.q)
-template <int VL>
-__global__ __launch_bounds__(TPB) void k_spmv64_pq(int64_t n, const int64_t *__restrict__ ia,
-                                                   const int32_t *__restrict__ ja,
-                                                   const double *__restrict__ a,
+// lane 'sub' (< BS) of a row group picks component sub of out
+template <class T, int BS>
+__device__ inline T pick(const T (&out)[BS], int sub) {
+  T o = out[0];
+#pragma unroll
+  for (int k = 1; k < BS; k++)
+    if (sub == k) o = out[k];
+  return o;
+}
+
+// y = A x over block rows; the rows of a workgroup are a contiguous slice
+// (xcd_block_range) so that gathers of x stay in one XCD's L2.
+template <class T, int BS>
+__global__ __launch_bounds__(TPB) void k_spmv_bsr(int64_t nb, const int64_t *__restrict__ bia,
+                                                  const int32_t *__restrict__ bja,
+                                                  const T *__restrict__ v,
+                                                  const T *__restrict__ x, T *__restrict__ y) {
+  const int sub = threadIdx.x % BSR_L;
+  int64_t lo, hi;
+  int xi, xg;
+  constexpr int RPB = TPB / BSR_L;  // rows per workgroup tile
+#if PZG_XCD_INTERLEAVE
+  xcd_slab(nb, lo, hi, xi, xg);
+  const int64_t first = lo + (int64_t)xi * RPB, step = (int64_t)xg * RPB;
+#else
+  xcd_block_range(nb, lo, hi);
+  xi = xg = 0;
+  const int64_t first = lo, step = RPB;
+#endif
+  for (int64_t A = first + threadIdx.x / BSR_L; A < hi; A += step) {
+    T out[BS];
+    bsr_row<T, BS>(A, bia, bja, v, x, sub, out);
+    if (sub < BS) y[A * BS + sub] = pick<T, BS>(out, sub);
+  }
+}
+
+// PCG64: q = K p and per-block partial sums of p.q (slot 0)
+template <int BS>
+__global__ __launch_bounds__(TPB) void k_spmv64_pq(int64_t nb, const int64_t *__restrict__ bia,
+                                                   const int32_t *__restrict__ bja,
+                                                   const double *__restrict__ v,
                                                    const double *__restrict__ p,
                                                    double *__restrict__ q, SolveScalars *sc,
-                                                   double *partials, unsigned *counter) {
+                                                   double *partials) {
   if (sc->done) return;
-  const int sub = threadIdx.x % VL;
-  int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / VL;
-  const int64_t ng = (int64_t)gridDim.x * blockDim.x / VL;
+  const int sub = threadIdx.x % BSR_L;
+  int64_t lo, hi;
+  int xi, xg;
+  constexpr int RPB = TPB / BSR_L;  // rows per workgroup tile
+#if PZG_XCD_INTERLEAVE
+  xcd_slab(nb, lo, hi, xi, xg);
+  const int64_t first = lo + (int64_t)xi * RPB, step = (int64_t)xg * RPB;
+#else
+  xcd_block_range(nb, lo, hi);
+  xi = xg = 0;
+  const int64_t first = lo, step = RPB;
+#endif
   double acc = 0.;
-  for (int64_t r = g; r < n; r += ng) {
-    double s = row_dot64<VL>(r, ia, ja, a, p, sub);
-    if (sub == 0) {
-      q[r] = s;
-      acc += s * p[r];
+  for (int64_t A = first + threadIdx.x / BSR_L; A < hi; A += step) {
+    double out[BS];
+    bsr_row<double, BS>(A, bia, bja, v, p, sub, out);
+    if (sub < BS) {
+      const int64_t i = A * BS + sub;
+      const double qi = pick<double, BS>(out, sub);
+      q[i] = qi;
+      acc += qi * p[i];
     }
   }
-  double v[1] = {acc};
-  block_partials<1>(v, partials);
+  double vv[1] = {acc};
+  block_partials<1>(vv, partials);
+}
+
+static int bsr_grid(int64_t nb) {
+  int64_t g = (nb + TPB / BSR_L - 1) / (TPB / BSR_L);
+  if (g > MAXBLK) g = MAXBLK;
+  return (int)std::max<int64_t>(g, 1);
 }
 
 // alpha = rho / (p.q)   (one block)
@@ -224,30 +283,41 @@ __global__ __launch_bounds__(TPB) void k_cg_init64(int64_t n, const double *__re
   }
 }
 
-// r = f - A u (FP64), |r|^2 and |f|^2
-template <int VL>
-__global__ __launch_bounds__(TPB) void k_residual64(int64_t n, const int64_t *__restrict__ ia,
-                                                    const int32_t *__restrict__ ja,
-                                                    const double *__restrict__ a,
+// r = f - K u (FP64 block CSR), |r|^2 and |f|^2
+template <int BS>
+__global__ __launch_bounds__(TPB) void k_residual64(int64_t nb, const int64_t *__restrict__ bia,
+                                                    const int32_t *__restrict__ bja,
+                                                    const double *__restrict__ v,
                                                     const double *__restrict__ u,
                                                     const double *__restrict__ f,
                                                     double *__restrict__ r, double *out,
                                                     double *partials, unsigned *counter) {
-  const int sub = threadIdx.x % VL;
-  int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / VL;
-  const int64_t ng = (int64_t)gridDim.x * blockDim.x / VL;
+  const int sub = threadIdx.x % BSR_L;
+  int64_t lo, hi;
+  int xi, xg;
+  constexpr int RPB = TPB / BSR_L;  // rows per workgroup tile
+#if PZG_XCD_INTERLEAVE
+  xcd_slab(nb, lo, hi, xi, xg);
+  const int64_t first = lo + (int64_t)xi * RPB, step = (int64_t)xg * RPB;
+#else
+  xcd_block_range(nb, lo, hi);
+  xi = xg = 0;
+  const int64_t first = lo, step = RPB;
+#endif
   double rr = 0., ff = 0.;
-  for (int64_t i = g; i < n; i += ng) {
-    double s = row_dot64<VL>(i, ia, ja, a, u, sub);
-    if (sub == 0) {
-      double ri = f[i] - s;
+  for (int64_t A = first + threadIdx.x / BSR_L; A < hi; A += step) {
+    double o[BS];
+    bsr_row<double, BS>(A, bia, bja, v, u, sub, o);
+    if (sub < BS) {
+      const int64_t i = A * BS + sub;
+      const double ri = f[i] - pick<double, BS>(o, sub);
       if (r) r[i] = ri;
       rr += ri * ri;
       ff += f[i] * f[i];
     }
   }
-  double v[2] = {rr, ff}, tot[2];
-  if (reduce_last<2>(v, partials, counter, tot) && threadIdx.x == 0) {
+  double vv[2] = {rr, ff}, tot[2];
+  if (reduce_last<2>(vv, partials, counter, tot) && threadIdx.x == 0) {
     out[0] = tot[0];
     out[1] = tot[1];
   }
@@ -260,38 +330,16 @@ __global__ void k_axpby64(int64_t n, double alpha, const double *__restrict__ x,
   for (; i < n; i += st) z[i] = alpha * x[i] + (y ? beta * y[i] : 0.);
 }
 
-static int pick_vl(int64_t nnz, int64_t n) {
-  double avg = n ? (double)nnz / n : 1;
-  if (avg > 96) return 64;
-  if (avg > 40) return 32;
-  if (avg > 20) return 16;
-  if (avg > 10) return 8;
-  return 4;
-}
-
-#define DISPATCH_VL(VLV, CALL)   \
-  switch (VLV) {                 \
-    case 64: { constexpr int VL = 64; CALL; } break; \
-    case 32: { constexpr int VL = 32; CALL; } break; \
-    case 16: { constexpr int VL = 16; CALL; } break; \
-    case 8:  { constexpr int VL = 8;  CALL; } break; \
-    default: { constexpr int VL = 4;  CALL; } break; \
+#define DISPATCH_BS(BSV, CALL)                         \
+  switch (BSV) {                                       \
+    case 3: { constexpr int BS = 3; CALL; } break;     \
+    default: { constexpr int BS = 1; CALL; } break;    \
   }
 
-static int spmv_grid(int64_t n, int vl) {
-  int rows_per_block = TPB / vl;
-  int64_t g = (n + rows_per_block - 1) / rows_per_block;
-  int64_t cap = 8192;
-  if (g > cap) g = cap;
-  return (int)std::max<int64_t>(g, 1);
-}
-
 int launch_spmv64(Ctx &c, const double *x, double *y) {
-  int vl = pick_vl(c.nnz_full, c.neq);
-  int g = spmv_grid(c.neq, vl);
-  DISPATCH_VL(vl, hipLaunchKernelGGL(k_spmv64<VL>, dim3(g), dim3(TPB), 0, c.stream, c.neq,
-                                     c.d_fia.as<int64_t>(), c.d_fja.as<int32_t>(),
-                                     c.d_fa.as<double>(), x, y));
+  DISPATCH_BS(c.bs, hipLaunchKernelGGL((k_spmv_bsr<double, BS>), dim3(bsr_grid(c.nb)), dim3(TPB), 0,
+                                       c.stream, c.nb, c.d_fbia.as<int64_t>(),
+                                       c.d_fbja.as<int32_t>(), c.d_fa.as<double>(), x, y));
   PZG_HIP(hipGetLastError());
   return PZG_OK;
 }
@@ -322,13 +370,11 @@ int true_residual(Ctx &c, const double *u, double *rel) {
   PZG_TRY(ensure_reduce(c));
   DevBuf out;
   PZG_TRY(dev_alloc(out, 2 * sizeof(double)));
-  int vl = pick_vl(c.nnz_full, c.neq);
-  int g = grid_for(c.neq, TPB / vl);
-  DISPATCH_VL(vl, hipLaunchKernelGGL(k_residual64<VL>, dim3(g), dim3(TPB), 0, c.stream, c.neq,
-                                     c.d_fia.as<int64_t>(), c.d_fja.as<int32_t>(),
-                                     c.d_fa.as<double>(), u, c.d_f.as<double>(), nullptr,
-                                     out.as<double>(), c.d_partials.as<double>(),
-                                     c.d_counter.as<unsigned>()));
+  DISPATCH_BS(c.bs, hipLaunchKernelGGL(k_residual64<BS>, dim3(bsr_grid(c.nb)), dim3(TPB), 0,
+                                       c.stream, c.nb, c.d_fbia.as<int64_t>(),
+                                       c.d_fbja.as<int32_t>(), c.d_fa.as<double>(), u,
+                                       c.d_f.as<double>(), nullptr, out.as<double>(),
+                                       c.d_partials.as<double>(), c.d_counter.as<unsigned>()));
   PZG_HIP(hipGetLastError());
   double h[2];
   PZG_HIP(hipMemcpyAsync(h, out.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
@@ -382,15 +428,14 @@ int solve_pcg64(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info) {
                      c.d_u.as<double>(), c.d_r.as<double>(), c.d_z.as<double>(),
                      c.d_p.as<double>(), c.d_diag.as<double>(), sc, part, cnt);
   PZG_HIP(hipGetLastError());
-  const int vl = pick_vl(c.nnz_full, n);
-  const int gs = grid_for(n, TPB / vl);
+  const int gs = bsr_grid(c.nb);
   auto iter = [&]() {
     hipLaunchKernelGGL(k_p_update64, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_z.as<double>(),
                        c.d_p.as<double>(), sc);
-    DISPATCH_VL(vl, hipLaunchKernelGGL(k_spmv64_pq<VL>, dim3(gs), dim3(TPB), 0, c.stream, n,
-                                       c.d_fia.as<int64_t>(), c.d_fja.as<int32_t>(),
-                                       c.d_fa.as<double>(), c.d_p.as<double>(),
-                                       c.d_q.as<double>(), sc, part, cnt));
+    DISPATCH_BS(c.bs, hipLaunchKernelGGL(k_spmv64_pq<BS>, dim3(gs), dim3(TPB), 0, c.stream, c.nb,
+                                         c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(),
+                                         c.d_fa.as<double>(), c.d_p.as<double>(),
+                                         c.d_q.as<double>(), sc, part));
     hipLaunchKernelGGL(k_fin_alpha64, dim3(1), dim3(TPB), 0, c.stream, sc, (const double *)part, gs);
     hipLaunchKernelGGL(k_cg_update64, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_u.as<double>(),
                        c.d_r.as<double>(), c.d_z.as<double>(), c.d_p.as<double>(),
@@ -417,113 +462,37 @@ int solve_pcg64(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info) {
 }
 
 // ----------------------------------------------------------------------------
-// Mixed path: FP32 block CSR SpMV (block size BS, one wave per block row).
-#ifndef PZG_BSR_VEC
-#define PZG_BSR_VEC 0
-#endif
-template <class T> __device__ inline T ld_stream(const T *p) {
-#if PZG_NT_LOADS
-  return __builtin_nontemporal_load(p);
-#else
-  return *p;
-#endif
-}
-
-template <int BS>
-__device__ inline void bsr_load(int64_t t, const float *__restrict__ v, int64_t nbf,
-                                float (&m)[BS * BS]) {
-#if PZG_BSR_SOA
-  for (int c = 0; c < BS * BS; c++) m[c] = ld_stream(&v[c * nbf + t]);
-#else
-  (void)nbf;
-  for (int c = 0; c < BS * BS; c++) m[c] = ld_stream(&v[t * (BS * BS) + c]);
-#endif
-}
-
-// One wavefront per block row; each lane takes U blocks per pass, all loads
-// of a pass issued before the products.
-template <int BS>
-__device__ inline void bsr_row(int64_t A, const int64_t *__restrict__ bia,
-                               const int32_t *__restrict__ bja, const float *__restrict__ v,
-                               int64_t nbf, const float *__restrict__ x, int lane,
-                               float (&out)[BS]) {
-  constexpr int U = 2;
-  float acc[BS];
-  for (int k = 0; k < BS; k++) acc[k] = 0.f;
-  const int64_t b0 = bia[A], b1 = bia[A + 1];
-  for (int64_t base = b0 + lane; base < b1; base += 64 * U) {
-    float m[U][BS * BS];
-    int64_t col[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int64_t t = base + 64 * u;
-      if (t < b1) {
-        col[u] = ld_stream(&bja[t]);
-        bsr_load<BS>(t, v, nbf, m[u]);
-      } else {
-        col[u] = 0;
-        for (int c = 0; c < BS * BS; c++) m[u][c] = 0.f;
-      }
-    }
-    float xv[U][BS];
-#pragma unroll
-    for (int u = 0; u < U; u++)
-      for (int l = 0; l < BS; l++) xv[u][l] = x[col[u] * BS + l];
-#pragma unroll
-    for (int u = 0; u < U; u++)
-      for (int k = 0; k < BS; k++)
-        for (int l = 0; l < BS; l++) acc[k] += m[u][k * BS + l] * xv[u][l];
-  }
-  for (int k = 0; k < BS; k++) {
-    float s = acc[k];
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    out[k] = s;
-  }
-}
-
-template <int BS>
-__global__ __launch_bounds__(TPB) void k_spmv_bsr(int64_t nb, const int64_t *__restrict__ bia,
-                                                  const int32_t *__restrict__ bja,
-                                                  const float *__restrict__ v, int64_t nbf,
-                                                  const float *__restrict__ x,
-                                                  float *__restrict__ y) {
-  const int lane = threadIdx.x & 63;
-  int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t A = w; A < nb; A += nw) {
-    float out[BS];
-    bsr_row<BS>(A, bia, bja, v, nbf, x, lane, out);
-    if (lane < BS) {
-      float o = out[0];
-      for (int k = 1; k < BS; k++)
-        if (lane == k) o = out[k];
-      y[A * BS + lane] = o;
-    }
-  }
-}
+// Mixed path.
 
 // q = S p; pq = p.q; pr = p.r; alpha = pr/pq
 template <int BS>
 __global__ __launch_bounds__(TPB) void k_fcg_spmv(int64_t nb, const int64_t *__restrict__ bia,
                                                   const int32_t *__restrict__ bja,
-                                                  const float *__restrict__ v, int64_t nbf,
+                                                  const float *__restrict__ v,
                                                   const float *__restrict__ p,
                                                   const float *__restrict__ r,
                                                   float *__restrict__ q, SolveScalars *sc,
-                                                  double *partials, unsigned *counter) {
+                                                  double *partials) {
   if (sc->done) return;
-  const int lane = threadIdx.x & 63;
-  int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const int sub = threadIdx.x % BSR_L;
+  int64_t lo, hi;
+  int xi, xg;
+  constexpr int RPB = TPB / BSR_L;  // rows per workgroup tile
+#if PZG_XCD_INTERLEAVE
+  xcd_slab(nb, lo, hi, xi, xg);
+  const int64_t first = lo + (int64_t)xi * RPB, step = (int64_t)xg * RPB;
+#else
+  xcd_block_range(nb, lo, hi);
+  xi = xg = 0;
+  const int64_t first = lo, step = RPB;
+#endif
   double pq = 0., pr = 0.;
-  for (int64_t A = w; A < nb; A += nw) {
+  for (int64_t A = first + threadIdx.x / BSR_L; A < hi; A += step) {
     float out[BS];
-    bsr_row<BS>(A, bia, bja, v, nbf, p, lane, out);
-    if (lane < BS) {
-      float o = out[0];
-      for (int k = 1; k < BS; k++)
-        if (lane == k) o = out[k];
-      const int64_t i = A * BS + lane;
+    bsr_row<float, BS>(A, bia, bja, v, p, sub, out);
+    if (sub < BS) {
+      const float o = pick<float, BS>(out, sub);
+      const int64_t i = A * BS + sub;
       q[i] = o;
       pq += (double)o * p[i];
       pr += (double)p[i] * r[i];
@@ -609,13 +578,23 @@ __global__ void k_fin_fcg_beta(SolveScalars *sc, const double *partials, int nbl
 
 // runs when the iteration completed normally (done 0) or completed and asked
 // for a reliable update (done 4)
+// p = z + beta p; with pt: also pt = D^-1/2 p (input of the matrix-free operator)
 __global__ void k_p_update32(int64_t n, const float *__restrict__ z, float *__restrict__ p,
-                             const SolveScalars *sc) {
+                             const SolveScalars *sc, const float *__restrict__ dis = nullptr,
+                             float *__restrict__ pt = nullptr) {
   if (sc->done != 0 && sc->done != 4) return;
   const float beta = (float)sc->beta;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t st = (int64_t)gridDim.x * blockDim.x;
-  for (; i < n; i += st) p[i] = z[i] + beta * p[i];
+  if (pt) {
+    for (; i < n; i += st) {
+      const float v = z[i] + beta * p[i];
+      p[i] = v;
+      pt[i] = v * dis[i];
+    }
+  } else {
+    for (; i < n; i += st) p[i] = z[i] + beta * p[i];
+  }
 }
 
 // b = D^-1/2 r64 (fp32); x = 0; r = b; z = Binv r; p = z; normb = |b|
@@ -739,14 +718,9 @@ __global__ void k_sub3(int64_t n, const float *__restrict__ a, const float *__re
 }
 
 int launch_spmv_bsr(Ctx &c, const float *x, float *y) {
-  int g = grid_for(c.nb, TPB / 64);
-  if (g < 1) g = 1;
-  if (c.bs == 3)
-    hipLaunchKernelGGL(k_spmv_bsr<3>, dim3(g), dim3(TPB), 0, c.stream, c.nb,
-                       c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(), c.d_bsr.as<float>(), c.nbf, x, y);
-  else
-    hipLaunchKernelGGL(k_spmv_bsr<1>, dim3(g), dim3(TPB), 0, c.stream, c.nb,
-                       c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(), c.d_bsr.as<float>(), c.nbf, x, y);
+  DISPATCH_BS(c.bs, hipLaunchKernelGGL((k_spmv_bsr<float, BS>), dim3(bsr_grid(c.nb)), dim3(TPB), 0,
+                                       c.stream, c.nb, c.d_fbia.as<int64_t>(),
+                                       c.d_fbja.as<int32_t>(), c.d_bsr.as<float>(), x, y));
   PZG_HIP(hipGetLastError());
   return PZG_OK;
 }
@@ -779,8 +753,7 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
   SolveScalars *sc = c.d_scal.as<SolveScalars>();
   double *part = c.d_partials.as<double>();
   unsigned *cnt = c.d_counter.as<unsigned>();
-  const int gv = grid_for(n, TPB), gb = grid_for(nb, TPB), gs = grid_for(nb, TPB / 64);
-  const int vl = pick_vl(c.nnz_full, n), gr = grid_for(n, TPB / vl);
+  const int gv = grid_for(n, TPB), gb = grid_for(nb, TPB), gs = bsr_grid(nb);
   PZG_HIP(hipEventRecord(c.ev0, c.stream));
   hipLaunchKernelGGL(k_axpby64, dim3(gv), dim3(TPB), 0, c.stream, n, 0.0, c.d_f.as<double>(), 0.0,
                      (const double *)nullptr, c.d_u.as<double>());
@@ -790,11 +763,10 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
   bool conv = false;
   const int batch = prm.check_every > 0 ? prm.check_every : 50;
   auto true_res = [&]() -> int {
-    DISPATCH_VL(vl, hipLaunchKernelGGL(k_residual64<VL>, dim3(gr), dim3(TPB), 0, c.stream, n,
-                                       c.d_fia.as<int64_t>(), c.d_fja.as<int32_t>(),
-                                       c.d_fa.as<double>(), c.d_u.as<double>(),
-                                       c.d_f.as<double>(), c.d_r.as<double>(), out.as<double>(),
-                                       part, cnt));
+    hipLaunchKernelGGL(k_residual64<BS>, dim3(bsr_grid(nb)), dim3(TPB), 0, c.stream, nb,
+                       c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(), c.d_fa.as<double>(),
+                       c.d_u.as<double>(), c.d_f.as<double>(), c.d_r.as<double>(), out.as<double>(),
+                       part, cnt);
     double h[2];
     PZG_HIP(hipMemcpyAsync(h, out.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
     PZG_HIP(hipStreamSynchronize(c.stream));
@@ -831,17 +803,18 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
       if (learned)
         PZG_TRY(nn_apply(c, c.d_rs.as<float>(), c.d_zs.as<float>(), c.d_ps.as<float>(), nullptr, sc,
                          part, cnt));
+      if (use_mf) PZG_TRY(mf_scale(c, c.d_ps.as<float>(), sc));
       restart = false;
     }
     auto iter = [&]() {
       int ga = gs;
       if (use_mf) {
-        (void)mf_apply(c, c.d_ps.as<float>(), c.d_rs.as<float>(), c.d_qs.as<float>(), sc, part, &ga);
+        (void)mf_apply(c, c.d_ps.as<float>(), c.d_rs.as<float>(), c.d_qs.as<float>(), sc, part, &ga,
+                       true);
       } else {
         hipLaunchKernelGGL(k_fcg_spmv<BS>, dim3(gs), dim3(TPB), 0, c.stream, nb,
                            c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(), c.d_bsr.as<float>(),
-                           c.nbf, c.d_ps.as<float>(), c.d_rs.as<float>(), c.d_qs.as<float>(), sc,
-                           part, cnt);
+                           c.d_ps.as<float>(), c.d_rs.as<float>(), c.d_qs.as<float>(), sc, part);
       }
       hipLaunchKernelGGL(k_fin_fcg_alpha, dim3(1), dim3(TPB), 0, c.stream, sc, (const double *)part, ga);
       hipLaunchKernelGGL(k_fcg_update<BS>, dim3(gb), dim3(TPB), 0, c.stream, nb,
@@ -853,7 +826,8 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
                        part, cnt);
       hipLaunchKernelGGL(k_fin_fcg_beta, dim3(1), dim3(TPB), 0, c.stream, sc, (const double *)part, gb);
       hipLaunchKernelGGL(k_p_update32, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_zs.as<float>(),
-                         c.d_ps.as<float>(), sc);
+                         c.d_ps.as<float>(), sc, use_mf ? c.d_mf_dis.as<float>() : nullptr,
+                         use_mf ? c.d_mf_pt.as<float>() : nullptr);
     };
     PZG_TRY(run_batches(c, batch, prm.use_graph != 0, iter, hs));
     total_it = hs.iter;
@@ -913,7 +887,7 @@ static int collect_impl(Ctx &c, int nrhs, const int32_t *rec, int nrec, uint64_t
   SolveScalars *sc = c.d_scal.as<SolveScalars>();
   double *part = c.d_partials.as<double>();
   unsigned *cnt = c.d_counter.as<unsigned>();
-  const int gv = grid_for(n, TPB), gb = grid_for(nb, TPB), gs = grid_for(nb, TPB / 64);
+  const int gv = grid_for(n, TPB), gb = grid_for(nb, TPB), gs = bsr_grid(nb);
   for (int j = 0; j < nrhs; j++) {
     for (int64_t i = 0; i < n; i++) {  // Box-Muller on a 64-bit LCG
       st = st * 6364136223846793005ULL + 1442695040888963407ULL;
@@ -934,8 +908,8 @@ static int collect_impl(Ctx &c, int nrhs, const int32_t *rec, int nrec, uint64_t
                        c.d_bjinv.as<float>(), sc, part, cnt);
     for (int k = 1; k <= kmax; k++) {
       hipLaunchKernelGGL(k_fcg_spmv<BS>, dim3(gs), dim3(TPB), 0, c.stream, nb, c.d_fbia.as<int64_t>(),
-                         c.d_fbja.as<int32_t>(), c.d_bsr.as<float>(), c.nbf, c.d_ps.as<float>(),
-                         c.d_rs.as<float>(), c.d_qs.as<float>(), sc, part, cnt);
+                         c.d_fbja.as<int32_t>(), c.d_bsr.as<float>(), c.d_ps.as<float>(),
+                         c.d_rs.as<float>(), c.d_qs.as<float>(), sc, part);
       hipLaunchKernelGGL(k_fin_fcg_alpha, dim3(1), dim3(TPB), 0, c.stream, sc, (const double *)part, gs);
       hipLaunchKernelGGL(k_fcg_update<BS>, dim3(gb), dim3(TPB), 0, c.stream, nb, c.d_xs.as<float>(),
                          c.d_rs.as<float>(), c.d_zs.as<float>(), c.d_ps.as<float>(), c.d_qs.as<float>(),
