@@ -99,6 +99,13 @@ typedef struct {
   const double *bumps;    /* 6 * nbumps                                       */
   double bump_width;
   double big_number;      /* Dirichlet penalty (TPZMaterial::fBigNumber or 1e12) */
+  /* hp meshes: integration rule order of every element (the material's
+   * IntegrationRuleOrder of the element's highest connect order,
+   * TPZInterpolationSpace::AdjustIntegrationRule, Mesh/pzinterpolationspace.
+   * cpp:69-83); NULL = rule_order_vol / rule_order_bc for all.  Connects of
+   * order 1 on higher sides carry no shapes: give them blk_size 0.          */
+  const int32_t *vol_rule;  /* nvol or NULL */
+  const int32_t *bc_rule;   /* nbc or NULL  */
 } pzg_mesh_desc;
 
 /* ---- solver selection -------------------------------------------------- */

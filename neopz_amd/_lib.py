@@ -44,6 +44,7 @@ class MeshDesc(C.Structure):
         ("force_const", C.c_double * 3),
         ("nbumps", C.c_int32), ("bumps", dp), ("bump_width", C.c_double),
         ("big_number", C.c_double),
+        ("vol_rule", i32p), ("bc_rule", i32p),
     ]
 
 
@@ -178,4 +179,6 @@ def make_mesh_desc(m):
     d.bumps = ptr(b, dp)
     d.bump_width = m["bump_width"]
     d.big_number = m["big_number"]
+    d.vol_rule = ptr(arr(m.get("vol_rule"), np.int32), i32p)
+    d.bc_rule = ptr(arr(m.get("bc_rule"), np.int32), i32p)
     return d, keep

@@ -447,9 +447,11 @@ static ElemArgs make_args(Ctx &c, bool vol) {
 template <int DIM, int NC, int NSD, int NSH, int NS>
 static int run_vol(Ctx &c) {
   ElemArgs g = make_args(c, true);
+  const size_t nbk = c.vol_rules.size();
   for (size_t k = 0; k + 1 < c.vcolor_ptr.size(); k++) {
     int64_t n = c.vcolor_ptr[k + 1] - c.vcolor_ptr[k];
     if (!n) continue;
+    g.tab = c.d_tvol.as<ElemTables>() + k % nbk;
     hipLaunchKernelGGL((k_vol<DIM, NC, NSD, NSH, NS>), dim3(n), dim3(TPB), 0, c.stream, g,
                        c.d_vcolor_list.as<int32_t>() + c.vcolor_ptr[k]);
     PZG_HIP(hipGetLastError());
@@ -461,9 +463,11 @@ template <int DIMB, int NC, int NSD, int NSH, int NS>
 static int run_bc(Ctx &c) {
   if (c.nbc == 0) return PZG_OK;
   ElemArgs g = make_args(c, false);
+  const size_t nbk = c.bc_rules.size();
   for (size_t k = 0; k + 1 < c.bcolor_ptr.size(); k++) {
     int64_t n = c.bcolor_ptr[k + 1] - c.bcolor_ptr[k];
     if (!n) continue;
+    g.tab = c.d_tbc.as<ElemTables>() + k % nbk;
     hipLaunchKernelGGL((k_bc<DIMB, NC, NSD, NSH, NS>), dim3(n), dim3(TPB), 0, c.stream, g,
                        c.d_bcolor_list.as<int32_t>() + c.bcolor_ptr[k]);
     PZG_HIP(hipGetLastError());
@@ -474,7 +478,10 @@ static int run_bc(Ctx &c) {
 int launch_assemble(Ctx &c) {
   fill<double>(c, c.d_a.as<double>(), c.nnz_u, 0.);
   fill<double>(c, c.d_rhs.as<double>(), c.neq, 0.);
-  if (c.tvol.nq > MAXQ || c.tbc.nq > MAXQ) return PZG_EUNSUP;
+  for (const ElemTables &t : c.tvol_all)
+    if (t.nq > MAXQ) return PZG_EUNSUP;
+  for (const ElemTables &t : c.tbc_all)
+    if (t.nq > MAXQ) return PZG_EUNSUP;
   int p = c.porder;
   if (c.dim == 3 && c.nstate == 3) {
     if (p == 2) { PZG_TRY((run_vol<3, 8, 27, 27, 3>(c))); PZG_TRY((run_bc<2, 4, 9, 9, 3>(c))); }

@@ -105,6 +105,26 @@ static int color_elements(int64_t nel, int nsides, const std::vector<int64_t> &c
   return PZG_OK;
 }
 
+// Splits every colour list into one list per rule bucket (stable): list k of
+// the result is colour k / nbk, bucket k % nbk.
+static void group_by_bucket(std::vector<int64_t> &ptr, std::vector<int32_t> &list,
+                            const std::vector<int32_t> &bucket, int nbk) {
+  if (nbk <= 1) return;
+  const size_t ncol = ptr.size() - 1;
+  std::vector<int64_t> nptr(ncol * nbk + 1, 0);
+  std::vector<int32_t> nlist(list.size());
+  int64_t o = 0;
+  for (size_t k = 0; k < ncol; k++)
+    for (int b = 0; b < nbk; b++) {
+      nptr[k * nbk + b] = o;
+      for (int64_t i = ptr[k]; i < ptr[k + 1]; i++)
+        if (bucket[list[i]] == b) nlist[o++] = list[i];
+    }
+  nptr[ncol * nbk] = o;
+  ptr.swap(nptr);
+  list.swap(nlist);
+}
+
 static int etype_nc(int t) { return t == PZG_LINE ? 2 : t == PZG_QUAD ? 4 : 8; }
 static int etype_ns(int t) { return t == PZG_LINE ? 3 : t == PZG_QUAD ? 9 : 27; }
 
@@ -200,8 +220,33 @@ static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
   for (int32_t v : c.bc_id)
     if (v < 0 || v >= c.nbcid) { set_error("pzg_mesh_set: bc id out of range"); return PZG_EINVAL; }
 
-  PZG_TRY(build_tables(vt, c.porder, c.rule_vol, c.tvol));
-  PZG_TRY(build_tables(bt, c.porder, c.rule_bc, c.tbc));
+  // integration-rule buckets (hp meshes) and their tables
+  auto bucket = [&](const int32_t *rules, int64_t n, int dflt, int etype, std::vector<int> &uniq,
+                    std::vector<int32_t> &bk, std::vector<ElemTables> &tabs) -> int {
+    uniq.clear();
+    bk.assign(n, 0);
+    if (!rules) uniq.push_back(dflt);
+    for (int64_t e = 0; rules && e < n; e++) {
+      const int r = rules[e];
+      if (r < 0 || r > 64) {
+        set_error("pzg_mesh_set: integration rule order out of range");
+        return PZG_EINVAL;
+      }
+      size_t k = 0;
+      while (k < uniq.size() && uniq[k] != r) k++;
+      if (k == uniq.size()) uniq.push_back(r);
+      bk[e] = (int32_t)k;
+    }
+    if (uniq.empty()) uniq.push_back(dflt);
+    tabs.assign(uniq.size(), ElemTables());
+    for (size_t k = 0; k < uniq.size(); k++) PZG_TRY(build_tables(etype, c.porder, uniq[k], tabs[k]));
+    return PZG_OK;
+  };
+  PZG_TRY(bucket(m->vol_rule, c.nvol, c.rule_vol, vt, c.vol_rules, c.vol_bucket, c.tvol_all));
+  PZG_TRY(bucket(c.nbc > 0 ? m->bc_rule : nullptr, c.nbc, c.rule_bc, bt, c.bc_rules, c.bc_bucket,
+                 c.tbc_all));
+  c.tvol = c.tvol_all[0];
+  c.tbc = c.tbc_all[0];
 
   hipStream_t s = c.stream;
   PZG_TRY(dev_upload(c.d_xyz, c.xyz.data(), c.xyz.size(), s));
@@ -220,8 +265,10 @@ static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
   }
   if (c.nbumps > 0) PZG_TRY(dev_upload(c.d_bumps, c.bumps.data(), c.bumps.size(), s));
   PZG_TRY(dev_upload(c.d_aidx, c.aidx.data(), c.aidx.size(), s));
-  PZG_TRY(dev_upload(c.d_tvol, reinterpret_cast<const char *>(&c.tvol), sizeof(ElemTables), s));
-  PZG_TRY(dev_upload(c.d_tbc, reinterpret_cast<const char *>(&c.tbc), sizeof(ElemTables), s));
+  PZG_TRY(dev_upload(c.d_tvol, reinterpret_cast<const char *>(c.tvol_all.data()),
+                     sizeof(ElemTables) * c.tvol_all.size(), s));
+  PZG_TRY(dev_upload(c.d_tbc, reinterpret_cast<const char *>(c.tbc_all.data()),
+                     sizeof(ElemTables) * c.tbc_all.size(), s));
   PZG_HIP(hipStreamSynchronize(s));
   c.has_mesh = true;
   c.has_struct = false;
@@ -360,10 +407,12 @@ static int struct_create(Ctx &c) {
   std::vector<int64_t> vptr, bptr2;
   std::vector<int32_t> vlist, blist;
   PZG_TRY(color_elements(c.nvol, nsv, c.vol_conn, c.aidx, nb, vptr, vlist));
+  group_by_bucket(vptr, vlist, c.vol_bucket, (int)c.vol_rules.size());
   c.vcolor_ptr = vptr;
   PZG_TRY(dev_upload(c.d_vcolor_list, vlist.data(), vlist.size(), s));
   if (c.nbc > 0) {
     PZG_TRY(color_elements(c.nbc, nsb, c.bc_conn, c.aidx, nb, bptr2, blist));
+    group_by_bucket(bptr2, blist, c.bc_bucket, (int)c.bc_rules.size());
     c.bcolor_ptr = bptr2;
     PZG_TRY(dev_upload(c.d_bcolor_list, blist.data(), blist.size(), s));
   } else {
@@ -717,7 +766,8 @@ int pzg_get_sizes(pzg_ctx *ctx, pzg_sizes *s) {
   s->nblocks = c.nb;
   s->nnz_blocks = c.nbf;
   s->nvol = c.nvol;
-  s->ncolors = c.vcolor_ptr.empty() ? 0 : (int64_t)c.vcolor_ptr.size() - 1;
+  s->ncolors = c.vcolor_ptr.empty() ? 0
+                                   : ((int64_t)c.vcolor_ptr.size() - 1) / (int64_t)std::max<size_t>(c.vol_rules.size(), 1);
   s->bs = c.bs;
   return PZG_OK;
 }

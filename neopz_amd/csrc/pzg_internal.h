@@ -90,7 +90,13 @@ struct Ctx {
   std::vector<int32_t> blk_size, bc_id, bc_type;
   double scale = 1, E = 1, nu = 0.3, big = 1e12, bump_width = 0.1, force_const[3] = {0, 0, 0};
   int force_kind = 0, nbumps = 0, rule_vol = 0, rule_bc = 0, nbcid = 0;
-  ElemTables tvol, tbc;
+  ElemTables tvol, tbc;        // tables of the first rule bucket
+  // hp: elements bucketed by integration rule (one table per bucket); the
+  // colour lists are grouped by bucket: list k covers colour k / nbk, bucket
+  // k % nbk
+  std::vector<int> vol_rules, bc_rules;
+  std::vector<int32_t> vol_bucket, bc_bucket;
+  std::vector<ElemTables> tvol_all, tbc_all;
   int vol_ncorner = 0, vol_nsides = 0, bc_ncorner = 0, bc_nsides = 0;
 
   // connect -> active block index (-1 for size-0 connects)
@@ -187,7 +193,8 @@ int mf_apply(Ctx &c, const float *p, const float *r, float *q, const SolveScalar
              double *partials, int *gather_grid, bool scaled_ready = false);
 // the matrix-free operator reproduces the assembled K (mesh-built, no AddKel extras)
 inline bool mf_available(const Ctx &c) {
-  return c.has_mesh && c.has_struct && !c.external_csr && !c.kel_added && c.assembled;
+  return c.has_mesh && c.has_struct && !c.external_csr && !c.kel_added && c.assembled &&
+         c.vol_rules.size() == 1 && c.bc_rules.size() <= 1;
 }
 int solve_pcg64(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info);
 int solve_mixed(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info);

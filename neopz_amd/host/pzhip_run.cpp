@@ -68,7 +68,7 @@ int main(int argc, char **argv) {
     return 2;
   }
   char magic[8];
-  if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, "PZGMESH1", 8) != 0) Reader::fail();
+  if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, "PZGMESH2", 8) != 0) Reader::fail();
   Reader r{f};
   pzg_mesh_desc m{};
   m.dim = r.scalar<int32_t>();
@@ -97,6 +97,8 @@ int main(int argc, char **argv) {
   auto bumps = r.array<double>();
   m.bump_width = r.scalar<double>();
   m.big_number = r.scalar<double>();
+  auto vol_rule = r.array<int32_t>();
+  auto bc_rule = r.array<int32_t>();
   std::fclose(f);
 
   const int ncorner = m.dim == 2 ? 4 : 8, bcorner = m.dim == 2 ? 2 : 4;
@@ -122,6 +124,11 @@ int main(int argc, char **argv) {
   for (int k = 0; k < 3; k++) m.force_const[k] = force_const[k];
   m.nbumps = int32_t(bumps.size() / 6);
   m.bumps = bumps.empty() ? nullptr : bumps.data();
+  if ((!vol_rule.empty() && int64_t(vol_rule.size()) != m.nvol) ||
+      (!bc_rule.empty() && int64_t(bc_rule.size()) != m.nbc))
+    Reader::fail();
+  m.vol_rule = vol_rule.empty() ? nullptr : vol_rule.data();
+  m.bc_rule = bc_rule.empty() ? nullptr : bc_rule.data();
 
   using namespace pzhip;
   TPZLinearAnalysis an(MakeContext(0), m, /*mustOptimizeBandwidth=*/false);

@@ -40,11 +40,37 @@ def build(pb):
     raise ValueError("structured builder supports dim 2 and 3")
 
 
+def _vol_rule(pb, p):
+    """TPZMatSingleSpaceT::IntegrationRuleOrder (Material/TPZMatSingleSpace.cpp:
+    60-72) of an element whose highest connect order is p; TPZElasticity3D
+    uses 2p (Material/Elasticity/TPZElasticity3D.h:82-85)."""
+    ff = pb.force_porder if pb.force_kind != P.FORCE_NONE else 0
+    if pb.material == P.ELAST3D:
+        return 2 * p
+    return np.where(p < ff, ff + p, 2 * p)
+
+
 def _common(pb, econn_vol, econn_bc, nconn, corner_flags, xyz, vol_nodes, bc_nodes, bc_id):
     ns = pb.nstate
-    # block size: corners carry nstate, higher sides nstate at p=2, 0 at p=1
-    nshape = np.where(corner_flags, 1, 1 if pb.p == 2 else 0)
+    # connect orders: corners 1; other sides the minimum preferred order of the
+    # elements sharing them (TPZInterpolatedElement::IdentifySideOrder /
+    # ComputeSideOrder, Mesh/pzintel.cpp:214-290, 1370-1390).  Boundary
+    # elements prefer the default order p, which never lowers the minimum.
+    pe = (np.full(econn_vol.shape[0], pb.p, dtype=np.int64) if pb.el_porder is None
+          else np.asarray(pb.el_porder, dtype=np.int64))
+    if pe.shape[0] != econn_vol.shape[0] or pe.min() < 1 or pe.max() > pb.p:
+        raise ValueError("el_porder must hold one order in 1..p per volume element")
+    order = np.full(nconn, pb.p, dtype=np.int64)
+    np.minimum.at(order, econn_vol.ravel(), np.repeat(pe, econn_vol.shape[1]))
+    order[corner_flags] = 1
+    # shapes per connect (p <= 2): corners 1, other sides 1 from order 2 on
+    nshape = np.where(corner_flags, 1, (order >= 2).astype(np.int64))
     blk_size = (nshape * ns).astype(np.int32)
+    # element integration rules from the highest connect order (MaxOrder,
+    # Mesh/pzinterpolationspace.cpp:53-83)
+    vol_rule = _vol_rule(pb, order[econn_vol].max(axis=1)).astype(np.int32)
+    bc_rule = (2 * order[econn_bc].max(axis=1)).astype(np.int32) if len(econn_bc) else \
+        np.zeros(0, np.int32)
     blk_pos = np.zeros(nconn, dtype=np.int64)
     if nconn > 1:
         blk_pos[1:] = np.cumsum(blk_size[:-1].astype(np.int64))
@@ -56,7 +82,9 @@ def _common(pb, econn_vol, econn_bc, nconn, corner_flags, xyz, vol_nodes, bc_nod
         rule_vol = ff + pb.p if pb.p < ff else 2 * pb.p
     big = 1.0e12 if pb.material == P.ELAST3D else 10.0 ** 17 * 2 / 3
     bc_ids_used = [f for f in range(6) if pb.bc_type[f] >= 0]
+    hp = pb.el_porder is not None
     return dict(
+        vol_rule=vol_rule if hp else None, bc_rule=bc_rule if hp else None,
         dim=pb.dim, material=pb.material, nstate=ns, porder=pb.p, xyz=xyz,
         vol_nodes=vol_nodes, vol_conn=econn_vol,
         vol_coef=None if pb.el_coef is None else np.ascontiguousarray(pb.el_coef, dtype=np.float64),
@@ -209,7 +237,7 @@ def _build3d(pb):
 # Flat little-endian file of a mesh description, read by the C++ host mirror
 # (neopz_amd/host/pzhip_run.cpp).  Fields in this fixed order; arrays are
 # preceded by their element count (int64).
-_FILE_MAGIC = b"PZGMESH1"
+_FILE_MAGIC = b"PZGMESH2"
 
 
 def save(m, path):
@@ -251,3 +279,5 @@ def save(m, path):
         arr(f, m["bumps"], np.float64)
         for k in ("bump_width", "big_number"):
             f64(f, m[k])
+        arr(f, m.get("vol_rule"), np.int32)
+        arr(f, m.get("bc_rule"), np.int32)

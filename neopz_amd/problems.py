@@ -43,6 +43,9 @@ class Problem:
     bc_type: list = field(default_factory=lambda: [-1] * 6)
     bc_val: list = field(default_factory=lambda: [[0.0, 0.0, 0.0] for _ in range(6)])
     name: str = ""
+    # hp: preferred order per volume element (1..p); None = p everywhere.
+    # Boundary elements keep the default order p.
+    el_porder: Optional[np.ndarray] = None
 
     @property
     def nstate(self):

@@ -33,6 +33,7 @@ class _Problem(C.Structure):
         ("nbumps", C.c_int), ("bumps", C.POINTER(C.c_double)),
         ("bump_width", C.c_double),
         ("bc_type", C.c_int * 6), ("bc_val", (C.c_double * 3) * 6),
+        ("el_porder", C.POINTER(C.c_int)),
     ]
 
 
@@ -43,7 +44,8 @@ class _Mesh(C.Structure):
         ("nel", C.c_int64), ("nvol", C.c_int64),
         ("etype", C.POINTER(C.c_int)), ("bcface", C.POINTER(C.c_int)),
         ("enodes", C.POINTER(C.c_int64)), ("econn", C.POINTER(C.c_int64)),
-        ("nconn", C.c_int64), ("conn_nshape", C.POINTER(C.c_int)),
+        ("nconn", C.c_int64), ("conn_order", C.POINTER(C.c_int)),
+        ("conn_nshape", C.POINTER(C.c_int)),
         ("conn_seq", C.POINTER(C.c_int64)), ("blk_pos", C.POINTER(C.c_int64)),
         ("blk_size", C.POINTER(C.c_int)), ("neq", C.c_int64),
     ]
@@ -157,6 +159,10 @@ class Oracle:
             pb.bc_type[f] = spec.bc_type[f]
             for k in range(3):
                 pb.bc_val[f][k] = spec.bc_val[f][k]
+        if getattr(spec, "el_porder", None) is not None:
+            po = np.ascontiguousarray(spec.el_porder, dtype=np.int32)
+            self._keep.append(po)
+            pb.el_porder = po.ctypes.data_as(C.POINTER(C.c_int))
         self.pb = pb
         self.m = _Mesh()
         if lib().orc_mesh_build(C.byref(pb), C.byref(self.m)) != 0:

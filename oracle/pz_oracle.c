@@ -351,11 +351,20 @@ int orc_mesh_build(const orc_problem *pb, orc_mesh *m) {
   sm.keys = (int64_t *)malloc(4 * sm.cap * sizeof(int64_t));
   memset(sm.keys, 0xff, 4 * sm.cap * sizeof(int64_t));
   sm.val = (int64_t *)malloc(sm.cap * sizeof(int64_t));
+  /* Connects are created in element order, side order, the first time a side
+   * is met (Mesh/pzintel.cpp:652-730: corners with order 1, other sides with
+   * the default order).  Orders of higher-dimensional sides follow the hp rule
+   * of TPZInterpolatedElement::IdentifySideOrder / ComputeSideOrder
+   * (Mesh/pzintel.cpp:214-290, 1370-1390): the minimum preferred order of the
+   * elements sharing the side.  Volume elements prefer el_porder[e] (or p);
+   * boundary elements keep the default order p. */
   int64_t ccap = 1024, nconn = 0;
-  int *cns = (int *)malloc(ccap * sizeof(int));
+  int *cord = (int *)malloc(ccap * sizeof(int));
   for (e = 0; e < m->nel; e++) {
     int t = m->etype[e];
+    int pe = (e < nvol && pb->el_porder) ? pb->el_porder[e] : p;
     for (int s = 0; s < etype_nsides(t); s++) {
+      int ord = side_dim(t, s) == 0 ? 1 : pe;
       int lc[8], nlc = side_corners(t, s, lc);
       int64_t key[8];
       for (int k = 0; k < nlc; k++) key[k] = m->enodes[8 * e + lc[k]];
@@ -367,16 +376,30 @@ int orc_mesh_build(const orc_problem *pb, orc_mesh *m) {
         for (int k = nlc; k < 4; k++) key[k] = -2;
         int ins;
         idx = sidemap_find_or_insert(&sm, key, nconn, &ins);
-        if (!ins) { m->econn[27 * e + s] = idx; continue; }
+        if (!ins) {
+          m->econn[27 * e + s] = idx;
+          if (ord < cord[idx]) cord[idx] = ord;
+          continue;
+        }
       }
-      if (nconn == ccap) { ccap *= 2; cns = (int *)realloc(cns, ccap * sizeof(int)); }
-      cns[nconn] = nconnect_shape(t, s, p);
+      if (nconn == ccap) { ccap *= 2; cord = (int *)realloc(cord, ccap * sizeof(int)); }
+      cord[nconn] = ord;
       m->econn[27 * e + s] = nconn++;
     }
   }
   free(sm.keys);
   free(sm.val);
+  int *cns = (int *)malloc((nconn > 0 ? nconn : 1) * sizeof(int));
+  for (int64_t c = 0; c < nconn; c++) cns[c] = -1;
+  for (e = 0; e < m->nel; e++) {
+    int t = m->etype[e];
+    for (int s = 0; s < etype_nsides(t); s++) {
+      int64_t c = m->econn[27 * e + s];
+      if (cns[c] < 0) cns[c] = nconnect_shape(t, s, cord[c]);
+    }
+  }
   m->nconn = nconn;
+  m->conn_order = cord;
   m->conn_nshape = cns;
   m->conn_seq = (int64_t *)malloc(nconn * sizeof(int64_t));
   m->blk_pos = (int64_t *)malloc(nconn * sizeof(int64_t));
@@ -394,7 +417,7 @@ int orc_mesh_build(const orc_problem *pb, orc_mesh *m) {
 
 void orc_mesh_free(orc_mesh *m) {
   free(m->xyz); free(m->etype); free(m->bcface); free(m->enodes); free(m->econn);
-  free(m->conn_nshape); free(m->conn_seq); free(m->blk_pos); free(m->blk_size);
+  free(m->conn_order); free(m->conn_nshape); free(m->conn_seq); free(m->blk_pos); free(m->blk_size);
   memset(m, 0, sizeof(*m));
 }
 
@@ -570,8 +593,17 @@ int orc_calcstiff(const orc_problem *pb, const orc_mesh *m, int64_t iel, double 
   double coord[8][3];
   for (int n = 0; n < nc; n++)
     for (int d = 0; d < 3; d++) coord[n][d] = m->xyz[3 * m->enodes[8 * iel + n] + d];
+  /* TPZInterpolationSpace::MaxOrder (Mesh/pzinterpolationspace.cpp:53-66)
+   * drives the rule (AdjustIntegrationRule, :69-83); shapes exist only for
+   * connects with shape functions, in side order */
+  int maxorder = -1, keep[27], nkeep = 0;
+  for (int s = 0; s < etype_nsides(t); s++) {
+    int64_t c = m->econn[27 * iel + s];
+    if (m->conn_order[c] > maxorder) maxorder = m->conn_order[c];
+    if (m->conn_nshape[c] > 0) keep[nkeep++] = s;
+  }
   double gl[64], gw[64];
-  int ng = orc_gauss_legendre(rule_order(pb, isbc, m->p), gl, gw);
+  int ng = orc_gauss_legendre(rule_order(pb, isbc, maxorder), gl, gw);
   int nq = 1;
   for (int d = 0; d < dim; d++) nq *= ng;
   double coef = 1.;
@@ -587,7 +619,16 @@ int orc_calcstiff(const orc_problem *pb, const orc_mesh *m, int64_t iel, double 
       int ik = q % ng, ie = (q % (ng * ng)) / ng, iz = q / (ng * ng);
       xi[0] = gl[ik]; xi[1] = gl[ie]; xi[2] = gl[iz]; w = gw[ik] * gw[ie] * gw[iz];
     }
-    int nsh = orc_shape(t, m->p, xi, phi, dphi);
+    double phiall[27], dphiall[81];
+    int nall = orc_shape(t, m->p, xi, phiall, dphiall);
+    /* p <= 2: shape i belongs to side i; keep those of sides with a shape */
+    int sel[27], nsh = 0;
+    for (int k = 0; k < nkeep; k++)
+      if (keep[k] < nall) sel[nsh++] = keep[k];
+    for (int i = 0; i < nsh; i++) {
+      phi[i] = phiall[sel[i]];
+      for (int a = 0; a < dim; a++) dphi[a * nsh + i] = dphiall[a * nall + sel[i]];
+    }
     /* corner derivatives for GradX (Geom/TPZGeoCube.h:124, pzgeoquad.h:144) */
     double cphi[27], cdphi[81];
     orc_shape(t, 1, xi, cphi, cdphi);

@@ -48,6 +48,8 @@ typedef struct {
   double bump_width; /* f = sum a exp(-|x-c|^2 / (2 w^2))                    */
   int bc_type[6];    /* per boundary face: -1 none, 0 Dirichlet, 1 Neumann   */
   double bc_val[6][3];
+  const int *el_porder; /* per volume element preferred order (1..p), NULL = p
+                           everywhere; boundary elements keep the default p   */
 } orc_problem;
 
 typedef struct {
@@ -60,6 +62,7 @@ typedef struct {
   int64_t *enodes;        /* stride 8 */
   int64_t *econn;         /* stride 27: connect index per side */
   int64_t nconn;
+  int *conn_order;        /* TPZConnect::Order (corners 1) */
   int *conn_nshape;
   int64_t *conn_seq;      /* first-touch: seq == creation index */
   int64_t *blk_pos;       /* by seq */

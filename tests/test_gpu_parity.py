@@ -49,7 +49,20 @@ CASES = {
     "poisson3d_p2": lambda: P.Problem(material=P.POISSON, dim=3, p=2, nx=4, ny=3, nz=5,
                                       force_kind=P.FORCE_CONST, force_const=(1.0, 0, 0),
                                       bc_type=[0, 0, 1, 1, 0, 0]),
+    # hp: per-element orders 1/2, shared sides at the minimum order
+    "poisson2d_hp": lambda: _hp(P.poisson2d(n=12, p=2, seed=8), 8),
+    "darcy2d_hp": lambda: _hp(P.darcy2d(n=14, p=2, seed=9), 9),
+    "elast3d_hp": lambda: _hp(P.elasticity3d(n=4, p=2, seed=10, field_kind="grf", load="body"), 10),
+    "poisson3d_hp": lambda: _hp(P.Problem(material=P.POISSON, dim=3, p=2, nx=4, ny=3, nz=5,
+                                          force_kind=P.FORCE_CONST, force_const=(1.0, 0, 0),
+                                          bc_type=[0, 0, 1, 1, 0, 0]), 11),
 }
+
+def _hp(pb, seed):
+    pb.el_porder = np.random.default_rng(seed).integers(1, 3, pb.nvol).astype(np.int32)
+    pb.name = (pb.name or "") + "_hp"
+    return pb
+
 
 _CTX = None
 
@@ -106,7 +119,7 @@ def test_assembly_is_deterministic():
     assert np.array_equal(a1, a2) and np.array_equal(r1, r2)
 
 
-@pytest.mark.parametrize("name", ["poisson2d_p2", "darcy2d_p2", "elast3d_p2_traction"])
+@pytest.mark.parametrize("name", ["poisson2d_p2", "darcy2d_p2", "elast3d_p2_traction", "elast3d_hp"])
 def test_pcg64_matches_oracle(name):
     pb = CASES[name]()
     an = TPZLinearAnalysis(pb, False, ctx=ctx())
@@ -189,6 +202,28 @@ def test_matfree_needs_mesh_built_system():
     st.SetMixedFCG(10000, 1e-8, inner_op=L.INNER_AUTO)   # falls back to the stored operator
     st.Solve(rhs)
     assert st.info["converged"]
+
+
+@pytest.mark.parametrize("name", ["poisson2d_hp", "elast3d_hp"])
+def test_mixed_refinement_hp(name):
+    """hp meshes (several integration rules): the mixed solver uses the stored
+    FP32 operator; the matrix-free one is refused."""
+    pb = CASES[name]()
+    an = TPZLinearAnalysis(pb, False, ctx=ctx())
+    st = TPZStepSolver()
+    st.SetMixedFCG(50000, 1e-8, inner_tol=1e-4, inner_op=L.INNER_AUTO)
+    an.SetSolver(st)
+    an.Assemble()
+    u = an.Solve()
+    assert st.info["converged"] and st.info["true_residual"] <= 1e-8
+    o, ia, ja, a, rhs = _oracle(pb)
+    rows = np.repeat(np.arange(o.neq), np.diff(ia))
+    U = sp.csr_matrix((a, (rows, ja)), shape=(o.neq, o.neq))
+    Kf = (U + sp.triu(U, 1).T).tocsc()
+    ud = spla.spsolve(Kf, rhs)
+    assert np.linalg.norm(u - ud) <= 1e-5 * np.linalg.norm(ud)
+    with pytest.raises(L.PzgError):
+        _scaled_apply(ctx(), L.INNER_MATFREE, np.ones(o.neq))
 
 
 def test_known_matrix_1d_addkel():

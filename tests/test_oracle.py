@@ -383,3 +383,95 @@ def test_pcg_matches_direct_and_quadratic_exactness():
     Kf = U + sp.triu(U, 1).T
     xd = spla.spsolve(Kf.tocsc(), rhs)
     assert np.linalg.norm(res["x"] - xd) <= 1e-8 * np.linalg.norm(xd)
+
+
+# --------------------------------------------------------------------------- hp (min rule)
+def _hp_problem(kind, seed):
+    rng = np.random.default_rng(seed)
+    if kind == "quad":
+        pb = P.Problem(material=P.POISSON, dim=2, p=2, nx=5, ny=4, bc_type=[0, 1, 1, 1, -1, -1],
+                       bc_val=[[0, 0, 0], [1.0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0]])
+    else:
+        pb = P.Problem(material=P.POISSON, dim=3, p=2, nx=3, ny=3, nz=2, bc_type=[0, 1, 1, 1, 1, 1],
+                       bc_val=[[0, 0, 0], [1.0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0]])
+    pb.el_porder = rng.integers(1, 3, pb.nvol).astype(np.int32)
+    return pb
+
+
+@pytest.mark.parametrize("kind", ["quad", "hex"])
+def test_hp_min_rule_pattern_and_sizes(kind):
+    """hp orders: a side's connect has shapes iff every element sharing it
+    prefers order 2 (pzintel.cpp:1370-1390); element dof sets follow; the CSR
+    pattern equals the brute-force union of element dof pairs."""
+    pb = _hp_problem(kind, 3)
+    o = O.Oracle(pb)
+    econn, pos, size = o.connects()
+    nvol = pb.nvol
+    ns = 9 if pb.dim == 2 else 27
+    nc = 4 if pb.dim == 2 else 8
+    # expected connect sizes from the rule, recomputed here from element lists
+    mins = {}
+    for e in range(nvol):
+        for s in range(nc, ns):
+            c = econn[e, s]
+            mins[c] = min(mins.get(c, 2), int(pb.el_porder[e]))
+    for c, pmin in mins.items():
+        assert size[c] == (1 if pmin >= 2 else 0)
+    assert o.neq == int(size.sum())
+    # independent of the formula: an element of order 1 spans exactly the
+    # (bi/tri)linear functions, and an order-2 element whose neighbours are all
+    # of order 2 keeps its full quadratic set
+    nx, ny, nz = pb.nx, pb.ny, (pb.nz if pb.dim == 3 else 1)
+    po = np.asarray(pb.el_porder).reshape(nz, ny, nx)
+    for e in range(nvol):
+        nd = sum(size[c] for c in econn[e, :ns])
+        iz, iy, ix = e // (nx * ny), (e // nx) % ny, e % nx
+        nb = po[max(iz - 1, 0):iz + 2, max(iy - 1, 0):iy + 2, max(ix - 1, 0):ix + 2]
+        if pb.el_porder[e] == 1:
+            assert nd == nc
+        elif nb.min() == 2:
+            assert nd == ns
+        else:
+            assert nc < nd <= ns  # the interior shape is always there
+    ia, ja = o.pattern()
+    rows = [set() for _ in range(o.neq)]
+    for e in range(o.m.nel):
+        _, _, dest = o.calcstiff(e)
+        for i in dest:
+            for j in dest:
+                if j >= i:
+                    rows[i].add(j)
+    exp = [sorted(r) for r in rows]
+    assert np.array_equal(np.diff(ia), [len(r) for r in exp])
+    assert np.array_equal(ja, np.concatenate([np.array(r, dtype=np.int64) for r in exp]))
+
+
+@pytest.mark.parametrize("kind", ["quad", "hex"])
+def test_hp_patch_test_linear_solution(kind):
+    """u = x solves -lap u = 0 with u = 0 on x=0, du/dn = 1 on x=1 and 0 on the
+    other faces.  A conforming hp space reproduces it exactly (corner values =
+    x, hierarchical coefficients = 0); a wrong side order (a shape kept on one
+    side of an interface only) breaks conformity and the patch test."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    pb = _hp_problem(kind, 5)
+    o = O.Oracle(pb)
+    ia, ja = o.pattern()
+    a, rhs = o.assemble()
+    rows = np.repeat(np.arange(o.neq), np.diff(ia))
+    U = sp.csr_matrix((a, (rows, ja)), shape=(o.neq, o.neq))
+    K = (U + sp.triu(U, 1).T).tocsc()
+    u = spla.spsolve(K, rhs)
+    econn, pos, size = o.connects()
+    enodes = np.ctypeslib.as_array(o.m.enodes, shape=(8 * o.m.nel,)).reshape(o.m.nel, 8)
+    xyz = np.ctypeslib.as_array(o.m.xyz, shape=(3 * o.m.nnodes,)).reshape(-1, 3)
+    nc = 4 if pb.dim == 2 else 8
+    ns = 9 if pb.dim == 2 else 27
+    for e in range(pb.nvol):
+        for k in range(nc):
+            c = econn[e, k]
+            assert abs(u[pos[c]] - xyz[enodes[e, k], 0]) < 1e-9
+        for s in range(nc, ns):
+            c = econn[e, s]
+            if size[c]:
+                assert abs(u[pos[c]]) < 1e-9
