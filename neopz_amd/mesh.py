@@ -234,6 +234,27 @@ def _build3d(pb):
                    vol_nodes.astype(np.int64), bc_nodes, bc_id)
 
 
+def permute(m, perm):
+    """The mesh description after a connect renumbering (new sequence number
+    perm[old], as TPZCompMesh::Permute leaves it after a bandwidth
+    optimisation): element connect lists relabelled, Block() sizes and
+    positions in the new sequence order."""
+    perm = np.asarray(perm, dtype=np.int64)
+    n = m["nconn"]
+    if perm.shape != (n,) or not np.array_equal(np.sort(perm), np.arange(n)):
+        raise ValueError("not a permutation of the connects")
+    out = dict(m)
+    out["vol_conn"] = perm[m["vol_conn"]]
+    out["bc_conn"] = perm[m["bc_conn"]]
+    size = np.empty(n, dtype=np.int32)
+    size[perm] = m["blk_size"]
+    out["blk_size"] = size
+    pos = np.zeros(n, dtype=np.int64)
+    pos[1:] = np.cumsum(size[:-1].astype(np.int64))
+    out["blk_pos"] = pos
+    return out
+
+
 # Flat little-endian file of a mesh description, read by the C++ host mirror
 # (neopz_amd/host/pzhip_run.cpp).  Fields in this fixed order; arrays are
 # preceded by their element count (int64).

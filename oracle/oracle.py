@@ -69,6 +69,7 @@ def lib():
         L.orc_gauss_legendre.argtypes = [C.c_int, dp, dp]
         L.orc_mesh_build.argtypes = [P, M]
         L.orc_mesh_free.argtypes = [M]
+        L.orc_mesh_permute.argtypes = [M, i64p]
         L.orc_calcstiff.argtypes = [P, M, C.c_int64, dp, dp, i64p]
         L.orc_shape.argtypes = [C.c_int, C.c_int, dp, dp, dp]
         L.orc_contribute.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, dp, dp,
@@ -177,6 +178,14 @@ class Oracle:
             pass
 
     # --- numbering ---------------------------------------------------------
+    def permute(self, perm):
+        """Renumbers the connects: new sequence number perm[old] (as after
+        TPZCompMesh::Permute); block positions follow the new order."""
+        pr = np.ascontiguousarray(perm, dtype=np.int64)
+        if len(pr) != self.m.nconn or lib().orc_mesh_permute(C.byref(self.m), _ip(pr)) != 0:
+            raise ValueError("not a permutation of the connects")
+        self.ia = self.ja = None
+
     def connects(self):
         m = self.m
         econn = np.ctypeslib.as_array(m.econn, shape=(27 * m.nel,)).reshape(m.nel, 27).copy()

@@ -415,6 +415,30 @@ int orc_mesh_build(const orc_problem *pb, orc_mesh *m) {
   return 0;
 }
 
+/* Renumbers the connects: new sequence number perm[old seq] (the effect of
+ * TPZCompMesh::Permute after a renumbering, Mesh/pzcmesh.cpp:1399-1459);
+ * Block() positions follow the new sequence order. */
+int orc_mesh_permute(orc_mesh *m, const int64_t *perm) {
+  int64_t n = m->nconn;
+  int *seen = (int *)calloc(n > 0 ? n : 1, sizeof(int));
+  for (int64_t i = 0; i < n; i++) {
+    if (perm[i] < 0 || perm[i] >= n || seen[perm[i]]) { free(seen); return -1; }
+    seen[perm[i]] = 1;
+  }
+  free(seen);
+  int *size = (int *)malloc((n > 0 ? n : 1) * sizeof(int));
+  for (int64_t sq = 0; sq < n; sq++) size[perm[sq]] = m->blk_size[sq];
+  int64_t pos = 0;
+  for (int64_t sq = 0; sq < n; sq++) {
+    m->blk_size[sq] = size[sq];
+    m->blk_pos[sq] = pos;
+    pos += size[sq];
+  }
+  free(size);
+  for (int64_t c = 0; c < n; c++) m->conn_seq[c] = perm[m->conn_seq[c]];
+  return 0;
+}
+
 void orc_mesh_free(orc_mesh *m) {
   free(m->xyz); free(m->etype); free(m->bcface); free(m->enodes); free(m->econn);
   free(m->conn_order); free(m->conn_nshape); free(m->conn_seq); free(m->blk_pos); free(m->blk_size);

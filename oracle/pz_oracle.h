@@ -77,6 +77,8 @@ int orc_gauss_legendre(int order, double *loc, double *w);
 /* Builds the structured mesh + first-touch numbering. Returns 0 on success. */
 int orc_mesh_build(const orc_problem *pb, orc_mesh *m);
 void orc_mesh_free(orc_mesh *m);
+/* new sequence number perm[old] for every connect (a bandwidth renumbering) */
+int orc_mesh_permute(orc_mesh *m, const int64_t *perm);
 
 /* number of element dofs and quadrature points of element iel */
 int orc_elem_ndof(const orc_mesh *m, int64_t iel);

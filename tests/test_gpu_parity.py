@@ -277,3 +277,33 @@ def test_errors_are_reported():
     with pytest.raises(L.PzgError):
         K.SetData(np.array([0, 1, 2, 3]), np.array([0, 2, 1]), np.ones(3))  # row 1 lacks diagonal
     c.close()
+
+
+@pytest.mark.parametrize("name", ["elast3d_p2_traction", "poisson2d_hp"])
+def test_renumbered_connects(name):
+    """Any connect numbering (e.g. after TPZAnalysis::OptimizeBandwidth and
+    TPZCompMesh::Permute) goes through the boundary unchanged: pattern and
+    entries equal the oracle's under the same renumbering; the mixed solve
+    converges."""
+    pb = CASES[name]()
+    md = M.build(pb)
+    perm = np.random.default_rng(21).permutation(md["nconn"])
+    md = M.permute(md, perm)
+    print(f"[gpu] renumbered {name}", flush=True)
+    c = ctx()
+    sm = TPZSSpStructMatrixHIP(c, md)
+    K = sm.Create()
+    sm.Assemble(K)
+    rhs = np.zeros(K.neq)
+    L.check(c.lib.pzg_get_system(c.h, None, L.ptr(rhs, L.dp)))
+    o = O.Oracle(pb)
+    o.permute(perm)
+    ia_r, ja_r = o.pattern()
+    a_r, rhs_r = o.assemble()
+    assert np.array_equal(K.IA(), ia_r) and np.array_equal(K.JA(), ja_r)
+    _check_values(ia_r, K.A(), a_r)
+    assert np.max(np.abs(rhs - rhs_r)) <= 1e-12 * np.max(np.abs(rhs_r))
+    st = TPZStepSolver(K)
+    st.SetMixedFCG(50000, 1e-8)
+    st.Solve(None, want_solution=False)
+    assert st.info["converged"] and st.info["true_residual"] <= 1e-8

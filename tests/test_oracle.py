@@ -475,3 +475,30 @@ def test_hp_patch_test_linear_solution(kind):
             c = econn[e, s]
             if size[c]:
                 assert abs(u[pos[c]]) < 1e-9
+
+
+def test_permuted_numbering_is_a_similarity():
+    """Renumbering the connects (what TPZCompMesh::Permute does after a
+    bandwidth optimisation) permutes K and f symmetrically: K' = P K P^T."""
+    pb = P.Problem(material=P.POISSON, dim=2, p=2, nx=4, ny=3, bc_type=[0, 1, 0, 1, -1, -1],
+                   force_kind=P.FORCE_CONST, force_const=(1.0, 0, 0))
+    o = O.Oracle(pb)
+    ia, ja = o.pattern()
+    a, rhs = o.assemble()
+    K = O.upper_to_dense(ia, ja, a)
+    econn, pos, size = o.connects()
+    perm = np.random.default_rng(0).permutation(o.m.nconn)
+    o2 = O.Oracle(pb)
+    o2.permute(perm)
+    ia2, ja2 = o2.pattern()
+    a2, rhs2 = o2.assemble()
+    K2 = O.upper_to_dense(ia2, ja2, a2)
+    _, pos2, size2 = o2.connects()
+    # equation map old -> new (one equation per connect here)
+    emap = np.empty(o.neq, dtype=np.int64)
+    for s_old in range(o.m.nconn):
+        for k in range(size[s_old]):
+            emap[pos[s_old] + k] = pos2[perm[s_old]] + k
+    assert np.array_equal(size2[perm], size)
+    assert np.allclose(K2[np.ix_(emap, emap)], K, rtol=0, atol=1e-9 * np.abs(K).max())
+    assert np.allclose(rhs2[emap], rhs, rtol=0, atol=1e-12 * np.abs(rhs).max())
