@@ -43,6 +43,8 @@ extern "C" {
 #define PZG_LINE 1
 #define PZG_QUAD 2
 #define PZG_HEX 3
+#define PZG_TRI 4  /* TPZTriangle: boundary elements of tetrahedral meshes */
+#define PZG_TET 5  /* TPZTetrahedron, H1 p <= 2                           */
 
 #define PZG_MAT_POISSON 0  /* TPZMatPoisson   (Material/Poisson/TPZMatPoisson.cpp) */
 #define PZG_MAT_DARCY 1    /* TPZDarcyFlow    (Material/DarcyFlow/TPZDarcyFlow.cpp) */
@@ -70,7 +72,7 @@ typedef struct {
   int32_t porder;     /* uniform H1 order, 1 or 2                             */
   int64_t nnodes;
   const double *xyz;  /* 3 * nnodes                                           */
-  /* volume elements: PZG_QUAD (dim 2) or PZG_HEX (dim 3)                     */
+  /* volume elements: PZG_QUAD (dim 2), PZG_HEX or PZG_TET (dim 3)           */
   int64_t nvol;
   const int64_t *vol_nodes; /* nvol * ncorner (4 or 8), NeoPZ corner order  */
   const int64_t *vol_conn;  /* nvol * nsides (9 or 27): connect seq per side */
@@ -106,6 +108,10 @@ typedef struct {
    * order 1 on higher sides carry no shapes: give them blk_size 0.          */
   const int32_t *vol_rule;  /* nvol or NULL */
   const int32_t *bc_rule;   /* nbc or NULL  */
+  /* volume element type: 0 = by dim (PZG_QUAD / PZG_HEX), or PZG_TET (then
+   * vol_nodes has 4 corners and vol_conn 15 sides per element, boundary
+   * elements are PZG_TRI with 3 corners and 7 sides)                        */
+  int32_t vol_etype;
 } pzg_mesh_desc;
 
 /* ---- solver selection -------------------------------------------------- */

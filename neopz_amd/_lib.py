@@ -44,7 +44,7 @@ class MeshDesc(C.Structure):
         ("force_const", C.c_double * 3),
         ("nbumps", C.c_int32), ("bumps", dp), ("bump_width", C.c_double),
         ("big_number", C.c_double),
-        ("vol_rule", i32p), ("bc_rule", i32p),
+        ("vol_rule", i32p), ("bc_rule", i32p), ("vol_etype", C.c_int32),
     ]
 
 
@@ -181,4 +181,5 @@ def make_mesh_desc(m):
     d.big_number = m["big_number"]
     d.vol_rule = ptr(arr(m.get("vol_rule"), np.int32), i32p)
     d.bc_rule = ptr(arr(m.get("bc_rule"), np.int32), i32p)
+    d.vol_etype = int(m.get("vol_etype", 0))
     return d, keep

@@ -483,6 +483,18 @@ int launch_assemble(Ctx &c) {
   for (const ElemTables &t : c.tbc_all)
     if (t.nq > MAXQ) return PZG_EUNSUP;
   int p = c.porder;
+  if (c.vol_etype == PZG_TET) {
+    // tetrahedra (4 corners, 15 sides; p <= 2: 4 or 10 shapes) with boundary
+    // triangles (3 corners, 7 sides; 3 or 6 shapes)
+    if (c.nstate == 3) {
+      if (p == 2) { PZG_TRY((run_vol<3, 4, 15, 10, 3>(c))); PZG_TRY((run_bc<2, 3, 7, 6, 3>(c))); }
+      else { PZG_TRY((run_vol<3, 4, 15, 4, 3>(c))); PZG_TRY((run_bc<2, 3, 7, 3, 3>(c))); }
+    } else {
+      if (p == 2) { PZG_TRY((run_vol<3, 4, 15, 10, 1>(c))); PZG_TRY((run_bc<2, 3, 7, 6, 1>(c))); }
+      else { PZG_TRY((run_vol<3, 4, 15, 4, 1>(c))); PZG_TRY((run_bc<2, 3, 7, 3, 1>(c))); }
+    }
+    return PZG_OK;
+  }
   if (c.dim == 3 && c.nstate == 3) {
     if (p == 2) { PZG_TRY((run_vol<3, 8, 27, 27, 3>(c))); PZG_TRY((run_bc<2, 4, 9, 9, 3>(c))); }
     else { PZG_TRY((run_vol<3, 8, 27, 8, 3>(c))); PZG_TRY((run_bc<2, 4, 9, 4, 3>(c))); }

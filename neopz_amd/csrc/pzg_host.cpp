@@ -125,8 +125,8 @@ static void group_by_bucket(std::vector<int64_t> &ptr, std::vector<int32_t> &lis
   list.swap(nlist);
 }
 
-static int etype_nc(int t) { return t == PZG_LINE ? 2 : t == PZG_QUAD ? 4 : 8; }
-static int etype_ns(int t) { return t == PZG_LINE ? 3 : t == PZG_QUAD ? 9 : 27; }
+static int etype_nc(int t) { return etype_ncorner(t); }
+static int etype_ns(int t) { return etype_nsides(t); }
 
 static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
   if (!m || (m->dim != 2 && m->dim != 3) || m->porder < 1 || m->porder > 2 ||
@@ -150,7 +150,17 @@ static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
   c.nbc = m->nbc;
   c.nconn = m->nconn;
   c.neq = m->neq;
+  if (m->vol_etype != 0 && !(m->vol_etype == PZG_TET && m->dim == 3)) {
+    set_error("pzg_mesh_set: unsupported volume element type");
+    return PZG_EUNSUP;
+  }
   int vt = m->dim == 2 ? PZG_QUAD : PZG_HEX, bt = m->dim == 2 ? PZG_LINE : PZG_QUAD;
+  if (m->vol_etype == PZG_TET) {
+    vt = PZG_TET;
+    bt = PZG_TRI;
+  }
+  c.vol_etype = vt;
+  c.bc_etype = bt;
   c.vol_ncorner = etype_nc(vt);
   c.vol_nsides = etype_ns(vt);
   c.bc_ncorner = etype_nc(bt);

@@ -98,6 +98,7 @@ struct Ctx {
   std::vector<int32_t> vol_bucket, bc_bucket;
   std::vector<ElemTables> tvol_all, tbc_all;
   int vol_ncorner = 0, vol_nsides = 0, bc_ncorner = 0, bc_nsides = 0;
+  int vol_etype = 0, bc_etype = 0;  // PZG_QUAD/PZG_HEX/PZG_TET, PZG_LINE/PZG_QUAD/PZG_TRI
 
   // connect -> active block index (-1 for size-0 connects)
   std::vector<int64_t> aidx;
@@ -158,6 +159,8 @@ struct Ctx {
 
 // ---- element tables (pzg_shape.cpp) ----------------------------------------
 int build_tables(int etype, int p, int rule_order, ElemTables &t);
+int etype_ncorner(int t);
+int etype_nsides(int t);
 int gauss_legendre(int order, double *loc, double *w);
 
 // ---- kernels launchers (pzg_assemble.hip / pzg_solve.hip) --------------------
@@ -194,7 +197,7 @@ int mf_apply(Ctx &c, const float *p, const float *r, float *q, const SolveScalar
 // the matrix-free operator reproduces the assembled K (mesh-built, no AddKel extras)
 inline bool mf_available(const Ctx &c) {
   return c.has_mesh && c.has_struct && !c.external_csr && !c.kel_added && c.assembled &&
-         c.vol_rules.size() == 1 && c.bc_rules.size() <= 1;
+         c.vol_rules.size() == 1 && c.bc_rules.size() <= 1 && c.vol_etype != PZG_TET;
 }
 int solve_pcg64(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info);
 int solve_mixed(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info);

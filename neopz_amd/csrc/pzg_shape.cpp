@@ -28,9 +28,34 @@ static const int HEX_FACE[6][4] = {{0, 1, 2, 3}, {0, 1, 5, 4}, {1, 2, 6, 5},
 static const int CORNER_BITS[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
                                       {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
 
+// Triangle / tetrahedron sides (Topology/tpztriangle.h:290-291,
+// Topology/tpztetrahedron.h:280-283).
+static const int TRI_EDGE[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+static const int TET_EDGE[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+static const int TET_FACE[4][3] = {{0, 1, 2}, {0, 1, 3}, {1, 2, 3}, {0, 2, 3}};
+
+int etype_ncorner(int t) {
+  return t == PZG_LINE ? 2 : t == PZG_QUAD ? 4 : t == PZG_HEX ? 8 : t == PZG_TRI ? 3 : 4;
+}
+int etype_nsides(int t) {
+  return t == PZG_LINE ? 3 : t == PZG_QUAD ? 9 : t == PZG_HEX ? 27 : t == PZG_TRI ? 7 : 15;
+}
+static int etype_dim(int t) { return t == PZG_LINE ? 1 : (t == PZG_QUAD || t == PZG_TRI) ? 2 : 3; }
+
 static int side_corner_list(int etype, int s, int *loc) {
-  int nc = etype == PZG_LINE ? 2 : etype == PZG_QUAD ? 4 : 8;
+  int nc = etype_ncorner(etype);
   if (s < nc) { loc[0] = s; return 1; }
+  if (etype == PZG_TRI) {
+    if (s < 6) { loc[0] = TRI_EDGE[s - 3][0]; loc[1] = TRI_EDGE[s - 3][1]; return 2; }
+    for (int i = 0; i < 3; i++) loc[i] = i;
+    return 3;
+  }
+  if (etype == PZG_TET) {
+    if (s < 10) { loc[0] = TET_EDGE[s - 4][0]; loc[1] = TET_EDGE[s - 4][1]; return 2; }
+    if (s < 14) { for (int i = 0; i < 3; i++) loc[i] = TET_FACE[s - 10][i]; return 3; }
+    for (int i = 0; i < 4; i++) loc[i] = i;
+    return 4;
+  }
   if (etype == PZG_LINE) { loc[0] = 0; loc[1] = 1; return 2; }
   if (etype == PZG_QUAD) {
     if (s < 8) { loc[0] = QUAD_EDGE[s - 4][0]; loc[1] = QUAD_EDGE[s - 4][1]; return 2; }
@@ -77,8 +102,83 @@ int gauss_legendre(int order, double *loc, double *w) {
   return n;
 }
 
+// Symmetric rules on the reference triangle (area 1/2) and tetrahedron
+// (volume 1/6): degree 2 with 3 / 4 points, degree 4 (triangle, 6 points) and
+// 5 (tetrahedron, 14 points); orbit parameters from the moment equations.
+// Affine simplices with p <= 2 and element-wise constant data have polynomial
+// integrands of degree <= 2p, which these rules integrate exactly.
+static int simplex_rule(int etype, int order, double (*pts)[3], double *w) {
+  int n = 0;
+  if (etype == PZG_TRI) {
+    if (order <= 2) {
+      const double a = 1. / 6., b = 2. / 3.;
+      const double P[3][2] = {{a, a}, {b, a}, {a, b}};
+      for (int i = 0; i < 3; i++) { pts[i][0] = P[i][0]; pts[i][1] = P[i][1]; pts[i][2] = 0; w[i] = 1. / 6.; }
+      return 3;
+    }
+    if (order > 4) return -1;
+    const double A[2] = {0.4459484909159655, 0.0915762135097718};
+    const double W[2] = {0.1116907948390057, 0.05497587182766096};
+    for (int o = 0; o < 2; o++) {
+      const double a = A[o], b = 1. - 2. * a;
+      const double L[3][3] = {{b, a, a}, {a, b, a}, {a, a, b}};
+      for (int k = 0; k < 3; k++, n++) { pts[n][0] = L[k][1]; pts[n][1] = L[k][2]; pts[n][2] = 0; w[n] = W[o]; }
+    }
+    return n;
+  }
+  if (order <= 2) {
+    const double a = 0.1381966011250105, b = 1. - 3. * a;
+    for (int k = 0; k < 4; k++, n++) {
+      double L[4] = {a, a, a, a};
+      L[k] = b;
+      pts[n][0] = L[1]; pts[n][1] = L[2]; pts[n][2] = L[3]; w[n] = 1. / 24.;
+    }
+    return 4;
+  }
+  if (order > 5) return -1;
+  const double A[2] = {0.09273525031089165, 0.31088591926330106};
+  const double W[2] = {0.0122488405193938, 0.018781320953003007};
+  for (int o = 0; o < 2; o++)
+    for (int k = 0; k < 4; k++, n++) {
+      double L[4] = {A[o], A[o], A[o], A[o]};
+      L[k] = 1. - 3. * A[o];
+      pts[n][0] = L[1]; pts[n][1] = L[2]; pts[n][2] = L[3]; w[n] = W[o];
+    }
+  const double a3 = 0.045503704125647394, w3 = 0.007091003462846565, b3 = 0.5 - a3;
+  for (int i = 0; i < 4; i++)
+    for (int j = i + 1; j < 4; j++, n++) {
+      double L[4] = {b3, b3, b3, b3};
+      L[i] = a3;
+      L[j] = a3;
+      pts[n][0] = L[1]; pts[n][1] = L[2]; pts[n][2] = L[3]; w[n] = w3;
+    }
+  return n;
+}
+
 // Values and reference gradients of all p<=2 shapes at xi.
 static int eval_shapes(int etype, int p, const double *xi, double *phi, double dphi[3][MAXSH]) {
+  if (etype == PZG_TRI || etype == PZG_TET) {
+    // barycentric corners; p = 2 adds 4 * la * lb per edge
+    // (Shape/pzshapetriang.cpp:34-81, Shape/pzshapetetra.cpp:53-150)
+    const int dim = etype_dim(etype), nc = dim + 1, ne = etype == PZG_TRI ? 3 : 6;
+    double l[4], dl[3][4];
+    l[0] = 1.;
+    for (int d = 0; d < dim; d++) { l[0] -= xi[d]; l[d + 1] = xi[d]; }
+    for (int d = 0; d < dim; d++)
+      for (int k = 0; k < nc; k++) dl[d][k] = k == 0 ? -1. : (k == d + 1 ? 1. : 0.);
+    for (int k = 0; k < nc; k++) {
+      phi[k] = l[k];
+      for (int d = 0; d < dim; d++) dphi[d][k] = dl[d][k];
+    }
+    if (p < 2) return nc;
+    for (int e = 0; e < ne; e++) {
+      const int a = etype == PZG_TRI ? TRI_EDGE[e][0] : TET_EDGE[e][0];
+      const int b = etype == PZG_TRI ? TRI_EDGE[e][1] : TET_EDGE[e][1];
+      phi[nc + e] = 4. * l[a] * l[b];
+      for (int d = 0; d < dim; d++) dphi[d][nc + e] = 4. * (dl[d][a] * l[b] + l[a] * dl[d][b]);
+    }
+    return nc + ne;
+  }
   int dim = etype == PZG_LINE ? 1 : etype == PZG_QUAD ? 2 : 3;
   int nc = etype == PZG_LINE ? 2 : etype == PZG_QUAD ? 4 : 8;
   int nsides = etype == PZG_LINE ? 3 : etype == PZG_QUAD ? 9 : 27;
@@ -115,13 +215,22 @@ static int eval_shapes(int etype, int p, const double *xi, double *phi, double d
 int build_tables(int etype, int p, int rule_order, ElemTables &t) {
   std::memset(&t, 0, sizeof(t));
   t.etype = etype;
-  t.dim = etype == PZG_LINE ? 1 : etype == PZG_QUAD ? 2 : 3;
-  t.ncorner = etype == PZG_LINE ? 2 : etype == PZG_QUAD ? 4 : 8;
-  t.nsides = etype == PZG_LINE ? 3 : etype == PZG_QUAD ? 9 : 27;
-  double gl[64], gw[64];
-  int ng = gauss_legendre(rule_order, gl, gw);
-  int nq = 1;
-  for (int d = 0; d < t.dim; d++) nq *= ng;
+  t.dim = etype_dim(etype);
+  t.ncorner = etype_ncorner(etype);
+  t.nsides = etype_nsides(etype);
+  const bool simplex = etype == PZG_TRI || etype == PZG_TET;
+  double gl[64], gw[64], spts[64][3], sw[64];
+  int ng = 0, nq = 1;
+  if (simplex) {
+    nq = simplex_rule(etype, rule_order, spts, sw);
+    if (nq < 0) {
+      set_error("integration rule order not available on simplices");
+      return PZG_EUNSUP;
+    }
+  } else {
+    ng = gauss_legendre(rule_order, gl, gw);
+    for (int d = 0; d < t.dim; d++) nq *= ng;
+  }
   if (nq > MAXQ) {
     set_error("integration rule too large for the element tables");
     return PZG_EUNSUP;
@@ -129,7 +238,10 @@ int build_tables(int etype, int p, int rule_order, ElemTables &t) {
   t.nq = nq;
   for (int q = 0; q < nq; q++) {
     double xi[3] = {0, 0, 0};
-    if (t.dim == 1) {
+    if (simplex) {
+      xi[0] = spts[q][0]; xi[1] = spts[q][1]; xi[2] = spts[q][2];
+      t.w[q] = sw[q];
+    } else if (t.dim == 1) {
       xi[0] = gl[q];
       t.w[q] = gw[q];
     } else if (t.dim == 2) {  // Integral/pzquad.cpp:159-169: ksi slow, eta fast

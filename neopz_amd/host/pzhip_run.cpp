@@ -68,7 +68,7 @@ int main(int argc, char **argv) {
     return 2;
   }
   char magic[8];
-  if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, "PZGMESH2", 8) != 0) Reader::fail();
+  if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, "PZGMESH3", 8) != 0) Reader::fail();
   Reader r{f};
   pzg_mesh_desc m{};
   m.dim = r.scalar<int32_t>();
@@ -99,9 +99,11 @@ int main(int argc, char **argv) {
   m.big_number = r.scalar<double>();
   auto vol_rule = r.array<int32_t>();
   auto bc_rule = r.array<int32_t>();
+  m.vol_etype = r.scalar<int32_t>();
   std::fclose(f);
 
-  const int ncorner = m.dim == 2 ? 4 : 8, bcorner = m.dim == 2 ? 2 : 4;
+  const bool tet = m.vol_etype == PZG_TET;
+  const int ncorner = m.dim == 2 ? 4 : tet ? 4 : 8, bcorner = m.dim == 2 ? 2 : tet ? 3 : 4;
   if (force_const.size() != 3 || vol_nodes.size() % ncorner || bc_nodes.size() % bcorner ||
       bumps.size() % 6 || blk_pos.size() != blk_size.size())
     Reader::fail();

@@ -35,6 +35,8 @@ def build(pb):
     """Flattened mesh description of a :class:`neopz_amd.problems.Problem`."""
     if pb.dim == 2:
         return _build2d(pb)
+    if pb.dim == 3 and getattr(pb, "simplex", 0):
+        return _build3d_tet(pb)
     if pb.dim == 3:
         return _build3d(pb)
     raise ValueError("structured builder supports dim 2 and 3")
@@ -50,7 +52,8 @@ def _vol_rule(pb, p):
     return np.where(p < ff, ff + p, 2 * p)
 
 
-def _common(pb, econn_vol, econn_bc, nconn, corner_flags, xyz, vol_nodes, bc_nodes, bc_id):
+def _common(pb, econn_vol, econn_bc, nconn, corner_flags, xyz, vol_nodes, bc_nodes, bc_id,
+            no_shape=None, vol_etype=0):
     ns = pb.nstate
     # connect orders: corners 1; other sides the minimum preferred order of the
     # elements sharing them (TPZInterpolatedElement::IdentifySideOrder /
@@ -65,6 +68,8 @@ def _common(pb, econn_vol, econn_bc, nconn, corner_flags, xyz, vol_nodes, bc_nod
     order[corner_flags] = 1
     # shapes per connect (p <= 2): corners 1, other sides 1 from order 2 on
     nshape = np.where(corner_flags, 1, (order >= 2).astype(np.int64))
+    if no_shape is not None:  # simplex faces and interiors: shapes from p = 3 / 4 on
+        nshape[no_shape] = 0
     blk_size = (nshape * ns).astype(np.int32)
     # element integration rules from the highest connect order (MaxOrder,
     # Mesh/pzinterpolationspace.cpp:53-83)
@@ -95,7 +100,7 @@ def _common(pb, econn_vol, econn_bc, nconn, corner_flags, xyz, vol_nodes, bc_nod
         scale=pb.scale, E=pb.E, nu=pb.nu, rule_order_vol=rule_vol, rule_order_bc=2 * pb.p,
         force_kind=pb.force_kind, force_const=np.array(pb.force_const, dtype=np.float64),
         bumps=None if pb.bumps is None else np.ascontiguousarray(pb.bumps, dtype=np.float64).reshape(-1, 6),
-        bump_width=pb.bump_width, big_number=big, bc_faces=bc_ids_used,
+        bump_width=pb.bump_width, big_number=big, bc_faces=bc_ids_used, vol_etype=vol_etype,
     )
 
 
@@ -255,10 +260,85 @@ def permute(m, perm):
     return out
 
 
+# Kuhn split of every cube into 6 tetrahedra around its diagonal 0 -> 6, one
+# per ordering of the axes (xyz, xzy, yxz, yzx, zxy, zyx), cube corners in the
+# hexahedron order; sides of a tetrahedron: 4 corners, 6 edges, 4 faces, the
+# interior (Topology/tpztetrahedron.h:280-283).
+_KUHN = np.array([[0, 1, 2, 6], [0, 1, 5, 6], [0, 3, 2, 6], [0, 3, 7, 6], [0, 4, 5, 6], [0, 4, 7, 6]])
+_TET_SIDES = [(0,), (1,), (2,), (3,), (0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3),
+              (0, 1, 2), (0, 1, 3), (1, 2, 3), (0, 2, 3), (0, 1, 2, 3)]
+_TRI_SIDES = [(0,), (1,), (2,), (0, 1), (1, 2), (2, 0), (0, 1, 2)]
+
+
+def _side_keys(nodes, sides, interior_base=None):
+    """(n_elem, n_sides, 4) keys: sorted corner nodes of every side, padded
+    with -1; element interiors get (-2, element, -1, -1)."""
+    ne = nodes.shape[0]
+    keys = np.full((ne, len(sides), 4), -1, dtype=np.int64)
+    for s, loc in enumerate(sides):
+        if interior_base is not None and len(loc) == 4:
+            keys[:, s, 0] = -2
+            keys[:, s, 1] = interior_base + np.arange(ne)
+            continue
+        keys[:, s, :len(loc)] = np.sort(nodes[:, list(loc)], axis=1)
+    return keys
+
+
+def _build3d_tet(pb):
+    nx, ny, nz = pb.nx, pb.ny, pb.nz
+    NX, NY, NZ = nx + 1, ny + 1, nz + 1
+    node = lambda i, j, k: i + NX * (j + NY * k)
+    iz, iy, ix = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    ix, iy, iz = (a.ravel().astype(np.int64) for a in (ix, iy, iz))
+    cube = np.stack([node(ix, iy, iz), node(ix + 1, iy, iz), node(ix + 1, iy + 1, iz),
+                     node(ix, iy + 1, iz), node(ix, iy, iz + 1), node(ix + 1, iy, iz + 1),
+                     node(ix + 1, iy + 1, iz + 1), node(ix, iy + 1, iz + 1)], 1)
+    tets = cube[:, _KUHN].reshape(-1, 4)                      # cubes x fastest, 6 per cube
+    gz, gy, gx = np.meshgrid(np.arange(NZ) / nz, np.arange(NY) / ny, np.arange(NX) / nx, indexing="ij")
+    xyz = np.stack([gx.ravel(), gy.ravel(), gz.ravel()], 1)
+    # boundary triangles: faces 0..5, tetrahedra in order, their faces in side order
+    nid = np.stack([np.arange(NX * NY * NZ) % NX, (np.arange(NX * NY * NZ) // NX) % NY,
+                    np.arange(NX * NY * NZ) // (NX * NY)], 1)
+    bn, bid = [], []
+    faces = np.array(_TET_SIDES[10:14])
+    for f in range(6):
+        if pb.bc_type[f] < 0:
+            continue
+        ax, side = f // 2, f % 2
+        plane = (nx, ny, nz)[ax] if side else 0
+        fn = tets[:, faces]                                     # (ntet, 4, 3)
+        on = (nid[fn][..., ax] == plane).all(axis=2)            # (ntet, 4)
+        t_idx, f_idx = np.nonzero(on)                           # row-major: tet order, face order
+        bn.append(fn[t_idx, f_idx])
+        bid.append(np.full(len(t_idx), f, dtype=np.int32))
+    bc_nodes = np.concatenate(bn).astype(np.int64) if bn else np.zeros((0, 3), np.int64)
+    bc_id = np.concatenate(bid) if bid else np.zeros(0, np.int32)
+    # first-touch numbering over the side keys (volume elements, then boundary)
+    kv = _side_keys(tets, _TET_SIDES, interior_base=0)
+    kb = _side_keys(bc_nodes, _TRI_SIDES)
+    allk = np.concatenate([kv.reshape(-1, 4), kb.reshape(-1, 4)])
+    uniq, first, inv = np.unique(allk, axis=0, return_index=True, return_inverse=True)
+    order = np.argsort(first, kind="stable")
+    rank = np.empty_like(order)
+    rank[order] = np.arange(len(order))
+    seq = rank[inv.ravel()]
+    nconn = len(uniq)
+    nv = kv.shape[0] * kv.shape[1]
+    econn = seq[:nv].reshape(kv.shape[0], kv.shape[1])
+    bc_conn = seq[nv:].reshape(kb.shape[0], kb.shape[1]) if len(kb) else np.zeros((0, 7), np.int64)
+    corner = np.zeros(nconn, dtype=bool)
+    corner[econn[:, :4].ravel()] = True
+    no_shape = np.zeros(nconn, dtype=bool)
+    no_shape[econn[:, 10:].ravel()] = True                     # faces and interior
+    return _common(pb, econn.astype(np.int64), bc_conn.astype(np.int64), nconn, corner,
+                   xyz.ravel(), tets.astype(np.int64), bc_nodes, bc_id, no_shape=no_shape,
+                   vol_etype=5)
+
+
 # Flat little-endian file of a mesh description, read by the C++ host mirror
 # (neopz_amd/host/pzhip_run.cpp).  Fields in this fixed order; arrays are
 # preceded by their element count (int64).
-_FILE_MAGIC = b"PZGMESH2"
+_FILE_MAGIC = b"PZGMESH3"
 
 
 def save(m, path):
@@ -302,3 +382,4 @@ def save(m, path):
             f64(f, m[k])
         arr(f, m.get("vol_rule"), np.int32)
         arr(f, m.get("bc_rule"), np.int32)
+        i32(f, m.get("vol_etype", 0))

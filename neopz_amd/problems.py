@@ -46,6 +46,8 @@ class Problem:
     # hp: preferred order per volume element (1..p); None = p everywhere.
     # Boundary elements keep the default order p.
     el_porder: Optional[np.ndarray] = None
+    # dim 3: 1 = six tetrahedra per cube (Kuhn split), boundary triangles
+    simplex: int = 0
 
     @property
     def nstate(self):
@@ -53,7 +55,8 @@ class Problem:
 
     @property
     def nvol(self):
-        return self.nx * (self.ny if self.dim >= 2 else 1) * (self.nz if self.dim >= 3 else 1)
+        n = self.nx * (self.ny if self.dim >= 2 else 1) * (self.nz if self.dim >= 3 else 1)
+        return 6 * n if self.simplex else n
 
     def n_equations(self):
         n = 1
@@ -137,6 +140,20 @@ def darcy2d(n=256, p=1, seed=0, sigma=1.5, corr_len=0.05):
                    name=f"darcy2d_q{n}_p{p}_s{seed}")
 
 
+def darcy3d_tet(n=32, pmax=2, seed=0, sigma=1.0, hp=True):
+    """hp Darcy on the Kuhn tetrahedral mesh of the unit cube (6 tetrahedra per
+    cube of an n^3 grid): kappa_e = exp(N(0, sigma^2)) per element, f = 1,
+    u = 0 on the boundary; per-element order drawn uniformly from 1..pmax by
+    seed (orders above 2 are outside this build's shape functions)."""
+    rng = np.random.default_rng(seed)
+    nvol = 6 * n ** 3
+    k = np.exp(sigma * rng.standard_normal(nvol))
+    po = rng.integers(1, pmax + 1, nvol).astype(np.int32) if hp else None
+    return Problem(material=DARCY, dim=3, p=pmax, nx=n, ny=n, nz=n, el_coef=k,
+                   force_kind=FORCE_CONST, force_const=(1.0, 0.0, 0.0), force_porder=0,
+                   bc_type=[0] * 6, simplex=1, el_porder=po, name=f"darcy_tet{n}_p{pmax}_s{seed}")
+
+
 def mock1d(n=20):
     """1D mesh with the constant mock element matrix [[2,1],[1,2]]
     (UnitTest_PZ/TestStruct/StructMatrixUnitTest.cpp:333-362)."""
@@ -147,6 +164,7 @@ CONFIGS = {
     # name: (factory, kwargs) -- the workloads quoted in the benchmark table
     "C1": (poisson2d, dict(n=64, p=2)),
     "C2": (elasticity3d, dict(n=48, p=2, field_kind="iid", load="traction")),
+    "C3": (darcy3d_tet, dict(n=32, pmax=2)),
     "C4p1": (darcy2d, dict(n=256, p=1)),
     "C4p2": (darcy2d, dict(n=256, p=2)),
     "C5": (elasticity3d, dict(n=64, p=2, field_kind="grf", corr_len=0.1, load="body")),

@@ -18,7 +18,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
 
 POISSON, DARCY, ELAST3D, MOCK1D = 0, 1, 2, 3
-LINE, QUAD, HEX = 1, 2, 3
+LINE, QUAD, HEX, TRI, TET = 1, 2, 3, 4, 5
 FORCE_NONE, FORCE_CONST, FORCE_BUMPS = 0, 1, 2
 
 
@@ -33,7 +33,7 @@ class _Problem(C.Structure):
         ("nbumps", C.c_int), ("bumps", C.POINTER(C.c_double)),
         ("bump_width", C.c_double),
         ("bc_type", C.c_int * 6), ("bc_val", (C.c_double * 3) * 6),
-        ("el_porder", C.POINTER(C.c_int)),
+        ("el_porder", C.POINTER(C.c_int)), ("simplex", C.c_int),
     ]
 
 
@@ -81,6 +81,7 @@ def lib():
         L.orc_addkel.argtypes = [i64p, i64p, dp, C.c_int, dp, i64p]
         L.orc_multadd.argtypes = [C.c_int64, i64p, i64p, dp, dp, dp, dp, C.c_double, C.c_double]
         L.orc_pcg.argtypes = [C.c_int64, i64p, i64p, dp, dp, dp, i64p, dp, C.c_int, dp]
+        L.orc_simplex_rule.argtypes = [C.c_int, C.c_int, dp, dp]
         L.orc_nnz_closed_form.argtypes = [C.c_int] * 6
         L.orc_nnz_closed_form.restype = C.c_int64
         L.free = C.CDLL(None).free
@@ -104,12 +105,21 @@ def gauss_legendre(order):
     return loc[:n].copy(), w[:n].copy()
 
 
+def simplex_rule(etype, order):
+    pts = np.zeros(3 * 64)
+    w = np.zeros(64)
+    n = lib().orc_simplex_rule(etype, order, _dp(pts), _dp(w))
+    if n < 0:
+        raise ValueError("rule order not available")
+    return pts[:3 * n].reshape(n, 3).copy(), w[:n].copy()
+
+
 def shape(etype, p, xi):
     xi = np.ascontiguousarray(xi, dtype=np.float64)
     phi = np.zeros(27)
     dphi = np.zeros(81)
     ns = lib().orc_shape(etype, p, _dp(np.r_[xi, np.zeros(3 - len(xi))]), _dp(phi), _dp(dphi))
-    dim = {LINE: 1, QUAD: 2, HEX: 3}[etype]
+    dim = {LINE: 1, QUAD: 2, HEX: 3, TRI: 2, TET: 3}[etype]
     return phi[:ns].copy(), dphi[: dim * ns].reshape(dim, ns).copy()
 
 
@@ -160,6 +170,7 @@ class Oracle:
             pb.bc_type[f] = spec.bc_type[f]
             for k in range(3):
                 pb.bc_val[f][k] = spec.bc_val[f][k]
+        pb.simplex = int(getattr(spec, "simplex", 0))
         if getattr(spec, "el_porder", None) is not None:
             po = np.ascontiguousarray(spec.el_porder, dtype=np.int32)
             self._keep.append(po)

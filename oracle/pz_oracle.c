@@ -25,14 +25,27 @@ static const int HEX_EDGE[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {0, 4}, {1, 
 static const int HEX_FACE[6][4] = {{0, 1, 2, 3}, {0, 1, 5, 4}, {1, 2, 6, 5},
                                    {3, 2, 6, 7}, {0, 3, 7, 4}, {4, 5, 6, 7}};
 
-static int etype_ncorner(int t) { return t == ORC_LINE ? 2 : t == ORC_QUAD ? 4 : 8; }
-static int etype_nsides(int t) { return t == ORC_LINE ? 3 : t == ORC_QUAD ? 9 : 27; }
-static int etype_dim(int t) { return t == ORC_LINE ? 1 : t == ORC_QUAD ? 2 : 3; }
+/* Triangle and tetrahedron (Topology/tpztriangle.h:290-291,
+ * Topology/tpztetrahedron.h:280-283). */
+static const int TRI_EDGE[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+static const int TET_EDGE[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+static const int TET_FACE[4][3] = {{0, 1, 2}, {0, 1, 3}, {1, 2, 3}, {0, 2, 3}};
+
+static int etype_ncorner(int t) {
+  return t == ORC_LINE ? 2 : t == ORC_QUAD ? 4 : t == ORC_HEX ? 8 : t == ORC_TRI ? 3 : 4;
+}
+static int etype_nsides(int t) {
+  return t == ORC_LINE ? 3 : t == ORC_QUAD ? 9 : t == ORC_HEX ? 27 : t == ORC_TRI ? 7 : 15;
+}
+static int etype_dim(int t) { return t == ORC_LINE ? 1 : (t == ORC_QUAD || t == ORC_TRI) ? 2 : 3; }
+static int is_simplex(int t) { return t == ORC_TRI || t == ORC_TET; }
 
 /* dimension of side s of element type t */
 static int side_dim(int t, int s) {
   int nc = etype_ncorner(t);
   if (s < nc) return 0;
+  if (t == ORC_TRI) return s < 6 ? 1 : 2;
+  if (t == ORC_TET) return s < 10 ? 1 : s < 14 ? 2 : 3;
   if (t == ORC_LINE) return 1;
   if (t == ORC_QUAD) return s < 8 ? 1 : 2;
   if (s < 20) return 1;
@@ -46,6 +59,13 @@ static int side_dim(int t, int s) {
 static int nconnect_shape(int t, int s, int p) {
   int d = side_dim(t, s);
   if (d == 0) return 1;
+  if (is_simplex(t)) {
+    /* TPZShapeTriang/TPZShapeTetra::NConnectShapeF: edge p-1, triangle
+     * (p-1)(p-2)/2, tetrahedron (p-1)(p-2)(p-3)/6 */
+    if (d == 1) return p - 1 > 0 ? p - 1 : 0;
+    if (d == 2) return p > 2 ? (p - 1) * (p - 2) / 2 : 0;
+    return p > 3 ? (p - 1) * (p - 2) * (p - 3) / 6 : 0;
+  }
   int q = p - 1;
   if (q <= 0) return 0;
   if (d == 1) return q;
@@ -57,6 +77,17 @@ static int nconnect_shape(int t, int s, int p) {
 static int side_corners(int t, int s, int *loc) {
   int nc = etype_ncorner(t);
   if (s < nc) { loc[0] = s; return 1; }
+  if (t == ORC_TRI) {
+    if (s < 6) { loc[0] = TRI_EDGE[s - 3][0]; loc[1] = TRI_EDGE[s - 3][1]; return 2; }
+    for (int i = 0; i < 3; i++) loc[i] = i;
+    return 3;
+  }
+  if (t == ORC_TET) {
+    if (s < 10) { loc[0] = TET_EDGE[s - 4][0]; loc[1] = TET_EDGE[s - 4][1]; return 2; }
+    if (s < 14) { for (int i = 0; i < 3; i++) loc[i] = TET_FACE[s - 10][i]; return 3; }
+    for (int i = 0; i < 4; i++) loc[i] = i;
+    return 4;
+  }
   if (t == ORC_LINE) { loc[0] = 0; loc[1] = 1; return 2; }
   if (t == ORC_QUAD) {
     if (s < 8) { loc[0] = QUAD_EDGE[s - 4][0]; loc[1] = QUAD_EDGE[s - 4][1]; return 2; }
@@ -115,6 +146,29 @@ int orc_gauss_legendre(int order, double *loc, double *w) {
 /* ------------------------------------------------------------------------ */
 int orc_shape(int t, int p, const double *xi, double *phi, double *dphi) {
   int dim = etype_dim(t), nc = etype_ncorner(t), nsides = etype_nsides(t);
+  if (is_simplex(t)) {
+    /* TPZShapeTriang/TPZShapeTetra::ShapeCorner (barycentric coordinates) and
+     * ShapeGenerating: edge = 4 * lambda_a * lambda_b (Shape/pzshapetriang.cpp:
+     * 34-81, Shape/pzshapetetra.cpp:53-150).  p <= 2: faces and interior carry
+     * no shapes, so the set is corners then edges. */
+    double l[4], dl[3][4];
+    l[0] = 1.;
+    for (int d = 0; d < dim; d++) { l[0] -= xi[d]; l[d + 1] = xi[d]; }
+    for (int d = 0; d < dim; d++)
+      for (int k = 0; k < nc; k++) dl[d][k] = k == 0 ? -1. : (k == d + 1 ? 1. : 0.);
+    int ne = t == ORC_TRI ? 3 : 6, ns = nc + (p >= 2 ? ne : 0);
+    for (int k = 0; k < nc; k++) {
+      phi[k] = l[k];
+      for (int d = 0; d < dim; d++) dphi[d * ns + k] = dl[d][k];
+    }
+    for (int e = 0; p >= 2 && e < ne; e++) {
+      int a = t == ORC_TRI ? TRI_EDGE[e][0] : TET_EDGE[e][0];
+      int b = t == ORC_TRI ? TRI_EDGE[e][1] : TET_EDGE[e][1];
+      phi[nc + e] = 4. * l[a] * l[b];
+      for (int d = 0; d < dim; d++) dphi[d * ns + nc + e] = 4. * (dl[d][a] * l[b] + l[a] * dl[d][b]);
+    }
+    return ns;
+  }
   /* raw (unscaled) generating functions on all sides, dims: [side], [d][side] */
   double ph[27], dph[3][27];
   if (t == ORC_LINE) {
@@ -271,6 +325,59 @@ int orc_mesh_build(const orc_problem *pb, orc_mesh *m) {
         m->xyz[3 * n + 2] = dim >= 3 ? (double)k / nz : 0.;
       }
 #define NODE(i, j, k) ((int64_t)(i) + NX * ((int64_t)(j) + NY * (int64_t)(k)))
+  if (pb->simplex && dim == 3) {
+    /* Kuhn split: 6 tetrahedra per cube around the diagonal corner 0 -> 6, one
+     * per ordering of the axes (xyz, xzy, yxz, yzx, zxy, zyx); cubes x fastest.
+     * Boundary triangles: faces 0..5 in turn, tetrahedra in order, tetrahedron
+     * faces in side order, those lying on the face. */
+    static const int KUHN[6][4] = {{0, 1, 2, 6}, {0, 1, 5, 6}, {0, 3, 2, 6},
+                                   {0, 3, 7, 6}, {0, 4, 5, 6}, {0, 4, 7, 6}};
+    int64_t ncube = (int64_t)nx * ny * nz, nvol = 6 * ncube, nbc = 0;
+    for (int f = 0; f < 6; f++) {
+      if (pb->bc_type[f] < 0) continue;
+      int ax = f / 2;
+      nbc += 2 * (int64_t)(ax == 0 ? ny * nz : ax == 1 ? nx * nz : nx * ny);
+    }
+    m->nvol = nvol;
+    m->nel = nvol + nbc;
+    m->etype = (int *)calloc(m->nel, sizeof(int));
+    m->bcface = (int *)calloc(m->nel, sizeof(int));
+    m->enodes = (int64_t *)calloc(8 * m->nel, sizeof(int64_t));
+    m->econn = (int64_t *)calloc(27 * m->nel, sizeof(int64_t));
+    int64_t e = 0;
+    for (int iz = 0; iz < nz; iz++)
+      for (int iy = 0; iy < ny; iy++)
+        for (int ix = 0; ix < nx; ix++) {
+          int64_t cn[8] = {NODE(ix, iy, iz), NODE(ix + 1, iy, iz), NODE(ix + 1, iy + 1, iz),
+                           NODE(ix, iy + 1, iz), NODE(ix, iy, iz + 1), NODE(ix + 1, iy, iz + 1),
+                           NODE(ix + 1, iy + 1, iz + 1), NODE(ix, iy + 1, iz + 1)};
+          for (int k = 0; k < 6; k++, e++) {
+            m->etype[e] = ORC_TET;
+            m->bcface[e] = -1;
+            for (int v = 0; v < 4; v++) m->enodes[8 * e + v] = cn[KUHN[k][v]];
+          }
+        }
+    for (int f = 0; f < 6; f++) {
+      if (pb->bc_type[f] < 0) continue;
+      int ax = f / 2, side = f % 2, lim = ax == 0 ? nx : ax == 1 ? ny : nz;
+      int64_t plane = side ? lim : 0;
+      for (int64_t t = 0; t < nvol; t++)
+        for (int fc = 0; fc < 4; fc++) {
+          int on = 1;
+          for (int v = 0; v < 3 && on; v++) {
+            int64_t nd = m->enodes[8 * t + TET_FACE[fc][v]];
+            int64_t ci = ax == 0 ? nd % NX : ax == 1 ? (nd / NX) % NY : nd / (NX * NY);
+            on = ci == plane;
+          }
+          if (!on) continue;
+          m->etype[e] = ORC_TRI;
+          m->bcface[e] = f;
+          for (int v = 0; v < 3; v++) m->enodes[8 * e + v] = m->enodes[8 * t + TET_FACE[fc][v]];
+          e++;
+        }
+    }
+    goto numbering;
+  }
   int64_t nvol = (int64_t)nx * ny * nz;
   int64_t nbc = 0;
   if (dim >= 2)
@@ -342,6 +449,7 @@ int orc_mesh_build(const orc_problem *pb, orc_mesh *m) {
           }
       }
     }
+numbering:;
 #undef NODE
   /* first-touch numbering */
   sidemap sm;
@@ -362,7 +470,7 @@ int orc_mesh_build(const orc_problem *pb, orc_mesh *m) {
   int *cord = (int *)malloc(ccap * sizeof(int));
   for (e = 0; e < m->nel; e++) {
     int t = m->etype[e];
-    int pe = (e < nvol && pb->el_porder) ? pb->el_porder[e] : p;
+    int pe = (e < m->nvol && pb->el_porder) ? pb->el_porder[e] : p;
     for (int s = 0; s < etype_nsides(t); s++) {
       int ord = side_dim(t, s) == 0 ? 1 : pe;
       int lc[8], nlc = side_corners(t, s, lc);
@@ -526,6 +634,64 @@ static void forcing(const orc_problem *pb, const double *x, double *f) {
   }
 }
 
+/* Symmetric rules on the reference triangle {x,y >= 0, x+y <= 1} and
+ * tetrahedron {x,y,z >= 0, x+y+z <= 1} (weights sum to the area 1/2 and the
+ * volume 1/6).  Degree 2: 3 and 4 points; degree 4 (triangle, 6 points) and 5
+ * (tetrahedron, 14 points), orbit parameters from the moment equations.  For
+ * the polynomial integrands of affine simplices (p <= 2, element-wise constant
+ * coefficients and loads) any rule of degree >= 2p integrates exactly, so the
+ * element matrices equal the reference's (TPZIntTriang / TPZIntTetra3D,
+ * Integral/pzquad.cpp:308-345) up to rounding. */
+int orc_simplex_rule(int t, int order, double *pts, double *w) {
+  int n = 0;
+  if (t == ORC_TRI) {
+    if (order <= 2) {
+      const double a = 1. / 6., b = 2. / 3.;
+      const double P[3][2] = {{a, a}, {b, a}, {a, b}};
+      for (int i = 0; i < 3; i++) { pts[3 * i] = P[i][0]; pts[3 * i + 1] = P[i][1]; pts[3 * i + 2] = 0; w[i] = 1. / 6.; }
+      return 3;
+    }
+    if (order > 4) return -1;
+    const double A[2] = {0.4459484909159655, 0.0915762135097718};
+    const double W[2] = {0.1116907948390057, 0.05497587182766096};
+    for (int o = 0; o < 2; o++) {
+      const double a = A[o], b = 1. - 2. * a;
+      const double L[3][3] = {{b, a, a}, {a, b, a}, {a, a, b}};
+      for (int k = 0; k < 3; k++, n++) {
+        pts[3 * n] = L[k][1]; pts[3 * n + 1] = L[k][2]; pts[3 * n + 2] = 0; w[n] = W[o];
+      }
+    }
+    return n;
+  }
+  if (order <= 2) {
+    const double a = 0.1381966011250105, b = 1. - 3. * a;
+    for (int k = 0; k < 4; k++, n++) {
+      double L[4] = {a, a, a, a};
+      L[k] = b;
+      pts[3 * n] = L[1]; pts[3 * n + 1] = L[2]; pts[3 * n + 2] = L[3]; w[n] = 1. / 24.;
+    }
+    return 4;
+  }
+  if (order > 5) return -1;
+  const double A[2] = {0.09273525031089165, 0.31088591926330106};
+  const double W[2] = {0.0122488405193938, 0.018781320953003007};
+  for (int o = 0; o < 2; o++)
+    for (int k = 0; k < 4; k++, n++) {
+      double L[4] = {A[o], A[o], A[o], A[o]};
+      L[k] = 1. - 3. * A[o];
+      pts[3 * n] = L[1]; pts[3 * n + 1] = L[2]; pts[3 * n + 2] = L[3]; w[n] = W[o];
+    }
+  const double a3 = 0.045503704125647394, w3 = 0.007091003462846565, b3 = 0.5 - a3;
+  for (int i = 0; i < 4; i++)
+    for (int j = i + 1; j < 4; j++, n++) {
+      double L[4] = {b3, b3, b3, b3};
+      L[i] = a3;
+      L[j] = a3;
+      pts[3 * n] = L[1]; pts[3 * n + 1] = L[2]; pts[3 * n + 2] = L[3]; w[n] = w3;
+    }
+  return n;
+}
+
 /* TPZMatSingleSpaceT::IntegrationRuleOrder (Material/TPZMatSingleSpace.cpp:
  * 60-72) and the TPZElasticity3D override 2p (TPZElasticity3D.h:82-85). */
 static int rule_order(const orc_problem *pb, int isbc, int p) {
@@ -626,16 +792,23 @@ int orc_calcstiff(const orc_problem *pb, const orc_mesh *m, int64_t iel, double 
     if (m->conn_order[c] > maxorder) maxorder = m->conn_order[c];
     if (m->conn_nshape[c] > 0) keep[nkeep++] = s;
   }
-  double gl[64], gw[64];
-  int ng = orc_gauss_legendre(rule_order(pb, isbc, maxorder), gl, gw);
-  int nq = 1;
-  for (int d = 0; d < dim; d++) nq *= ng;
+  double gl[64], gw[64], spts[3 * 64], sw[64];
+  int ng = 0, nq = 1;
+  if (is_simplex(t)) {
+    nq = orc_simplex_rule(t, rule_order(pb, isbc, maxorder), spts, sw);
+    if (nq < 0) return -1;
+  } else {
+    ng = orc_gauss_legendre(rule_order(pb, isbc, maxorder), gl, gw);
+    for (int d = 0; d < dim; d++) nq *= ng;
+  }
   double coef = 1.;
   if (!isbc && pb->el_coef) coef = pb->el_coef[iel];
   double phi[27], dphi[81], dphix[81];
   for (int q = 0; q < nq; q++) {
     double xi[3] = {0, 0, 0}, w;
-    if (dim == 1) { xi[0] = gl[q]; w = gw[q]; }
+    if (is_simplex(t)) {
+      xi[0] = spts[3 * q]; xi[1] = spts[3 * q + 1]; xi[2] = spts[3 * q + 2]; w = sw[q];
+    } else if (dim == 1) { xi[0] = gl[q]; w = gw[q]; }
     else if (dim == 2) { /* Integral/pzquad.cpp:159-169: ksi slow, eta fast */
       int ik = q / ng, ie = q % ng;
       xi[0] = gl[ik]; xi[1] = gl[ie]; w = gw[ik] * gw[ie];

@@ -28,7 +28,7 @@ extern "C" {
 #endif
 
 enum { ORC_POISSON = 0, ORC_DARCY = 1, ORC_ELAST3D = 2, ORC_MOCK1D = 3 };
-enum { ORC_LINE = 1, ORC_QUAD = 2, ORC_HEX = 3 };
+enum { ORC_LINE = 1, ORC_QUAD = 2, ORC_HEX = 3, ORC_TRI = 4, ORC_TET = 5 };
 enum { ORC_FORCE_NONE = 0, ORC_FORCE_CONST = 1, ORC_FORCE_BUMPS = 2 };
 
 /* Problem description shared with the product's structured mesh builder. */
@@ -50,6 +50,7 @@ typedef struct {
   double bc_val[6][3];
   const int *el_porder; /* per volume element preferred order (1..p), NULL = p
                            everywhere; boundary elements keep the default p   */
+  int simplex;       /* dim 3 only: 1 = 6 tetrahedra per cube (Kuhn split)     */
 } orc_problem;
 
 typedef struct {

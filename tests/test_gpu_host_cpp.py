@@ -49,7 +49,7 @@ def _read_result(path):
 def test_runner_rejects_malformed_mesh(tmp_path):
     assert RUN.exists(), "build the library first (neopz_amd/csrc make)"
     bad = tmp_path / "bad.bin"
-    bad.write_bytes(b"PZGMESH2" + b"\0" * 6)
+    bad.write_bytes(b"PZGMESH3" + b"\0" * 6)
     r = subprocess.run([str(RUN), str(bad), str(tmp_path / "o.bin")], capture_output=True, text=True,
                        timeout=60)
     assert r.returncode == 2 and "malformed" in r.stderr
@@ -60,6 +60,7 @@ def test_runner_rejects_malformed_mesh(tmp_path):
     ("poisson2d_p2", lambda: P.poisson2d(n=10, p=2, seed=1)),
     ("elast3d_p2_body", lambda: P.elasticity3d(n=4, p=2, seed=6, field_kind="grf", load="body")),
     ("poisson2d_hp", lambda: _hp(P.poisson2d(n=10, p=2, seed=2), 3)),
+    ("darcy_tet_hp", lambda: P.darcy3d_tet(n=3, pmax=2, seed=4, hp=True)),
 ])
 def test_cpp_analysis_matches_oracle(tmp_path, name, pb):
     pb = pb()

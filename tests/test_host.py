@@ -68,3 +68,22 @@ def test_library_exports_header_symbols():
     for n in sorted(names):
         assert hasattr(lib, n), n
     assert names == set(L.EXPORTS)
+
+
+@pytest.mark.parametrize("hp", [False, True])
+def test_tet_numbering_matches_oracle(hp):
+    pb = P.Problem(material=P.DARCY, dim=3, p=2, nx=3, ny=2, nz=2, simplex=1,
+                   force_kind=P.FORCE_CONST, force_const=(1.0, 0, 0), bc_type=[0, 1, 0, -1, 0, 0])
+    if hp:
+        pb.el_porder = np.random.default_rng(4).integers(1, 3, pb.nvol).astype(np.int32)
+    m = M.build(pb)
+    o = O.Oracle(pb)
+    econn, pos, size = o.connects()
+    assert m["nconn"] == o.m.nconn and m["neq"] == o.neq
+    assert np.array_equal(m["vol_conn"], econn[:pb.nvol, :15])
+    assert np.array_equal(m["bc_conn"], econn[pb.nvol:, :7])
+    assert np.array_equal(m["blk_pos"], pos) and np.array_equal(m["blk_size"], size)
+    enodes = np.ctypeslib.as_array(o.m.enodes, shape=(8 * o.m.nel,)).reshape(o.m.nel, 8)
+    assert np.array_equal(m["vol_nodes"], enodes[:pb.nvol, :4])
+    assert np.array_equal(m["bc_nodes"], enodes[pb.nvol:, :3])
+    assert m["vol_etype"] == 5

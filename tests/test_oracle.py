@@ -502,3 +502,109 @@ def test_permuted_numbering_is_a_similarity():
     assert np.array_equal(size2[perm], size)
     assert np.allclose(K2[np.ix_(emap, emap)], K, rtol=0, atol=1e-9 * np.abs(K).max())
     assert np.allclose(rhs2[emap], rhs, rtol=0, atol=1e-12 * np.abs(rhs).max())
+
+
+# --------------------------------------------------------------------------- simplices
+@pytest.mark.parametrize("etype,order,deg", [(O.TRI, 2, 2), (O.TRI, 4, 4), (O.TET, 2, 2), (O.TET, 4, 5)])
+def test_simplex_rules_integrate_monomials(etype, order, deg):
+    """Exact moments: int x^a y^b (z^c) over the unit simplex = a! b! c! / (a+b+c+dim)!"""
+    from math import factorial
+    pts, w = O.simplex_rule(etype, order)
+    assert (w > 0).all()
+    dim = 2 if etype == O.TRI else 3
+    for a in range(deg + 1):
+        for b in range(deg + 1 - a):
+            for c in range(deg + 1 - a - b if dim == 3 else 1):
+                v = np.sum(w * pts[:, 0] ** a * pts[:, 1] ** b * pts[:, 2] ** c)
+                ex = factorial(a) * factorial(b) * factorial(c) / factorial(a + b + c + dim)
+                assert abs(v - ex) < 1e-15
+
+
+@pytest.mark.parametrize("etype", [O.TRI, O.TET])
+def test_simplex_shapes(etype):
+    """Corners are barycentric coordinates; edge functions are 4*la*lb (value 1 at
+    the edge midpoint, 0 at the corners and on the other edges); derivatives
+    agree with finite differences."""
+    dim = 2 if etype == O.TRI else 3
+    nc = dim + 1
+    edges = [(0, 1), (1, 2), (2, 0)] if dim == 2 else [(0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3)]
+    corner = np.vstack([np.zeros(dim), np.eye(dim)])
+    for k in range(nc):
+        phi, _ = O.shape(etype, 2, corner[k])
+        assert np.allclose(phi[:nc], np.eye(nc)[k]) and np.allclose(phi[nc:], 0)
+    for e, (a, b) in enumerate(edges):
+        phi, _ = O.shape(etype, 2, 0.5 * (corner[a] + corner[b]))
+        assert abs(phi[nc + e] - 1.0) < 1e-15
+        assert np.allclose(np.delete(phi[nc:], e), 0)
+    rng = np.random.default_rng(0)
+    x = rng.random(dim) / dim
+    phi, dphi = O.shape(etype, 2, x)
+    assert abs(phi[:nc].sum() - 1) < 1e-15
+    h = 1e-6
+    for d in range(dim):
+        xp = x.copy(); xp[d] += h
+        xm = x.copy(); xm[d] -= h
+        fd = (O.shape(etype, 2, xp)[0] - O.shape(etype, 2, xm)[0]) / (2 * h)
+        assert np.allclose(dphi[d], fd, atol=1e-8)
+
+
+def _tet_problem(p, bc, bcval, force=0.0, n=(2, 2, 2)):
+    return P.Problem(material=P.POISSON, dim=3, p=p, nx=n[0], ny=n[1], nz=n[2], simplex=1,
+                     force_kind=P.FORCE_CONST if force else P.FORCE_NONE,
+                     force_const=(force, 0, 0), bc_type=bc,
+                     bc_val=[[v, 0, 0] for v in bcval])
+
+
+def _solve_oracle(o):
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    ia, ja = o.pattern()
+    a, rhs = o.assemble()
+    rows = np.repeat(np.arange(o.neq), np.diff(ia))
+    U = sp.csr_matrix((a, (rows, ja)), shape=(o.neq, o.neq))
+    return spla.spsolve((U + sp.triu(U, 1).T).tocsc(), rhs)
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_tet_patch_tests(p):
+    """u = x (Dirichlet 0 at x=0, flux 1 at x=1) for p = 1, 2, and for p = 2
+    u = x(1-x)/2 (-lap u = 1, u = 0 at x = 0 and x = 1): corner dofs equal u
+    at the nodes; edge dofs equal u(mid) - (u_a + u_b)/2, NeoPZ's edge
+    function being 1 at the midpoint."""
+    cases = [(lambda X: X, [0, 1, 1, 1, 1, 1], [0, 1, 0, 0, 0, 0], 0.0)]
+    if p == 2:
+        cases.append((lambda X: X * (1 - X) / 2, [0, 0, 1, 1, 1, 1], [0] * 6, 1.0))
+    for u_ex, bc, bcv, f in cases:
+        o = O.Oracle(_tet_problem(p, bc, bcv, f))
+        u = _solve_oracle(o)
+        econn, pos, size = o.connects()
+        enodes = np.ctypeslib.as_array(o.m.enodes, shape=(8 * o.m.nel,)).reshape(o.m.nel, 8)
+        xyz = np.ctypeslib.as_array(o.m.xyz, shape=(3 * o.m.nnodes,)).reshape(-1, 3)
+        edges = [(0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3)]
+        for e in range(o.m.nvol):
+            for k in range(4):
+                assert abs(u[pos[econn[e, k]]] - u_ex(xyz[enodes[e, k], 0])) < 1e-9
+            for j, (a, b) in enumerate(edges):
+                c = econn[e, 4 + j]
+                if size[c]:
+                    xa, xb = xyz[enodes[e, a], 0], xyz[enodes[e, b], 0]
+                    exp = u_ex(0.5 * (xa + xb)) - 0.5 * (u_ex(xa) + u_ex(xb))
+                    assert abs(u[pos[c]] - exp) < 1e-9
+
+
+def test_tet_pattern_brute_force_and_count():
+    """P2 on the Kuhn mesh of n^3 cubes has (2n+1)^3 equations (vertices + edge
+    midpoints); the pattern is the union of element dof pairs."""
+    n = (2, 3, 2)
+    o = O.Oracle(_tet_problem(2, [0] * 6, [0] * 6, 1.0, n))
+    assert o.neq == (2 * n[0] + 1) * (2 * n[1] + 1) * (2 * n[2] + 1)
+    ia, ja = o.pattern()
+    rows = [set() for _ in range(o.neq)]
+    for e in range(o.m.nel):
+        _, _, dest = o.calcstiff(e)
+        for i in dest:
+            for j in dest:
+                if j >= i:
+                    rows[i].add(j)
+    assert np.array_equal(np.diff(ia), [len(r) for r in rows])
+    assert np.array_equal(ja, np.concatenate([np.array(sorted(r), dtype=np.int64) for r in rows]))
