@@ -248,7 +248,7 @@ __global__ __launch_bounds__(MF_TPB) void k_mf_hex27_mfma(MfArgs a, const MfMfma
   __shared__ float4 s_cd[27][8];
   __shared__ float s_w[27];
   __shared__ float s_g[NW][EPT][27][NG];
-  __shared__ float s_x[NW][EPT][24 + 10];  // corners, then J^-1 and |det J|
+  __shared__ float s_x[NW][EPT][10];  // J^-1 (row-major) and |det J|; [9] < 0: not affine
   __shared__ __align__(16) MfMfmaTab s_mt;
   const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, lr = l >> 4;
   for (int t = tid; t < 27 * 8; t += MF_TPB) s_cd[t / 8][t % 8] = a.tab->cd[t / 8][t % 8];
@@ -267,33 +267,51 @@ __global__ __launch_bounds__(MF_TPB) void k_mf_hex27_mfma(MfArgs a, const MfMfma
   int64_t elo, ehi;
   int xi, xg;
   xcd_slab(a.nvol, elo, ehi, xi, xg);
-  for (int64_t g0 = elo + (int64_t)xi * (NW * EPT); g0 < ehi; g0 += (int64_t)xg * (NW * EPT)) {
-    const int64_t eend = ehi;
+  const int64_t gfirst = elo + (int64_t)xi * (NW * EPT), gstep = (int64_t)xg * (NW * EPT);
+  const int64_t eend = ehi;
 #else
-  for (int64_t g0 = (int64_t)blockIdx.x * (NW * EPT); g0 < a.nvol; g0 += (int64_t)gridDim.x * (NW * EPT)) {
-    const int64_t eend = a.nvol;
+  const int64_t gfirst = (int64_t)blockIdx.x * (NW * EPT), gstep = (int64_t)gridDim.x * (NW * EPT);
+  const int64_t eend = a.nvol;
 #endif
-    const int64_t eb = g0 + (int64_t)wv * EPT;
-    for (int t = l; t < EPT * 34; t += 64) {
-      const int el = t / 34, k = t % 34;
-      const int64_t e = eb + el;
+  // B operands of phase 1 for the group starting at g: u[s][(e,k)], s = 4t + lr
+  auto gather = [&](int64_t g, float (&u)[7]) {
+    const int64_t e = g + (int64_t)wv * EPT + ee;
+    const bool ok = colv && e < eend;
+#pragma unroll
+    for (int t = 0; t < 7; t++) {
+      const int s = 4 * t + lr;
       float v = 0.f;
-      if (e < eend) v = k < 24 ? a.xe[e * 24 + k] : a.geo[e * 10 + (k - 24)];
-      s_x[wv][el][k] = v;
+      if (ok && s < 27) {
+        const int32_t b = a.blk[e * 27 + s];
+        if (b >= 0) v = a.pt[(int64_t)b * NS + kc];
+      }
+      u[t] = v;
+    }
+  };
+  float ubn[7];
+  if (gfirst < eend) gather(gfirst, ubn);
+  for (int64_t g0 = gfirst; g0 < eend; g0 += gstep) {
+    const int64_t eb = g0 + (int64_t)wv * EPT;
+    for (int t = l; t < EPT * 10; t += 64) {
+      const int el = t / 10, k = t % 10;
+      const int64_t e = eb + el;
+      s_x[wv][el][k] = e < eend ? a.geo[e * 10 + k] : 0.f;
     }
     const int64_t e = eb + ee;
     const bool ev = colv && e < eend;
     float ub[7];
 #pragma unroll
-    for (int t = 0; t < 7; t++) {
-      const int s = 4 * t + lr;
-      float v = 0.f;
-      if (ev && s < 27) {
-        const int32_t b = a.blk[e * 27 + s];
-        if (b >= 0) v = a.pt[(int64_t)b * NS + kc];
+    for (int t = 0; t < 7; t++) ub[t] = ubn[t];
+    // the next group's gathers are in flight while this group computes
+    if (g0 + gstep < eend) gather(g0 + gstep, ubn);
+    int32_t opos[2][4];
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int sh = 16 * r + 4 * lr + i;
+        opos[r][i] = (ev && sh < 27) ? a.pos[e * 27 + sh] : -1;
       }
-      ub[t] = v;
-    }
     mf_f4 acc[3][2];
 #pragma unroll
     for (int m = 0; m < 3; m++)
@@ -322,20 +340,21 @@ __global__ __launch_bounds__(MF_TPB) void k_mf_hex27_mfma(MfArgs a, const MfMfma
       if (eb + el >= eend) continue;
       const float *xg = s_x[wv][el];
       float inv[3][3], adet;
-      if (xg[24 + 9] >= 0.f) {  // affine element: constant J^-1 and |det J|
+      if (xg[9] >= 0.f) {  // affine element: constant J^-1 and |det J|
 #pragma unroll
         for (int m = 0; m < 3; m++)
 #pragma unroll
-          for (int d = 0; d < 3; d++) inv[m][d] = xg[24 + 3 * m + d];
-        adet = xg[24 + 9];
+          for (int d = 0; d < 3; d++) inv[m][d] = xg[3 * m + d];
+        adet = xg[9];
       } else {
+        const float *xc = a.xe + (eb + el) * 24;
         float J[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
 #pragma unroll
         for (int c = 0; c < 8; c++) {
           const float4 d = s_cd[q][c];
 #pragma unroll
           for (int i = 0; i < 3; i++) {
-            const float x = xg[3 * c + i];
+            const float x = xc[3 * c + i];
             J[i][0] += x * d.x;
             J[i][1] += x * d.y;
             J[i][2] += x * d.z;
@@ -405,10 +424,7 @@ __global__ __launch_bounds__(MF_TPB) void k_mf_hex27_mfma(MfArgs a, const MfMfma
 #pragma unroll
         for (int i = 0; i < 4; i++) {
           const int s = 16 * r + 4 * lr + i;
-          if (s < 27) {
-            const int32_t t = a.pos[e * 27 + s];
-            if (t >= 0) a.ye[(int64_t)t * NS + kc] = y[r][i];
-          }
+          if (s < 27 && opos[r][i] >= 0) a.ye[(int64_t)opos[r][i] * NS + kc] = y[r][i];
         }
     }
     __syncthreads();
