@@ -415,6 +415,245 @@ __global__ __launch_bounds__(TPB) void k_bc(ElemArgs g, const int32_t *__restric
   }
 }
 
+// ---------------------------------------------------------------------------
+// High-order triangles / tetrahedra (scalar materials).  The element's shapes
+// are the virtual sides of its connect list; shape i reads table column
+// col_off[i] + (transform variant of its real side, from the corner ids).
+// The Gram matrix sum_q w|J| grad phi_i . grad phi_j is accumulated over
+// chunks of QC quadrature points, each thread keeping its own (i, j) entries.
+
+// transform id of a triangle from its corner ids (TPZShapeTriang::
+// GetTransformId2dT, Shape/pzshapetriang.cpp:410-431)
+__device__ inline int tri_tid(int64_t i0, int64_t i1, int64_t i2) {
+  const int a = i0 < i1 ? 0 : 1;
+  const int64_t ia = a ? i1 : i0;
+  const int mn = i2 < ia ? 2 : a;
+  const int64_t n1 = mn == 0 ? i1 : mn == 1 ? i2 : i0;
+  const int64_t n2 = mn == 0 ? i2 : mn == 1 ? i0 : i1;
+  return 2 * mn + (n1 < n2 ? 0 : 1);
+}
+
+// variant of side s (TPZTetrahedron / TPZTriangle::GetTransformId,
+// Topology/tpztetrahedron.cpp:1128-1168, Topology/tpztriangle.cpp:629-650)
+__device__ inline int hp_variant(bool tet, int s, const int64_t *id) {
+  if (tet) {
+    if (s < 4 || s == 14) return 0;
+    if (s < 10) {
+      const int a = s < 7 ? s - 4 : s - 7, b = s < 7 ? (s - 3) % 3 : 3;
+      return id[a] < id[b] ? 0 : 1;
+    }
+    const int f = s - 10;
+    const int f0 = f == 2 ? 1 : 0, f1 = f == 0 || f == 1 ? 1 : 2, f2 = f == 0 ? 2 : 3;
+    return tri_tid(id[f0], id[f1], id[f2]);
+  }
+  if (s < 3) return 0;
+  if (s < 6) return id[s - 3] < id[(s - 2) % 3] ? 0 : 1;
+  return tri_tid(id[0], id[1], id[2]);
+}
+
+template <int NSV>
+__global__ __launch_bounds__(TPB) void k_vol_hp(ElemArgs g, const HpTables *__restrict__ tab,
+                                                const int32_t *__restrict__ list) {
+  constexpr int QC = 16;
+  constexpr int NPAIR = NSV * NSV;
+  constexpr int PER = (NPAIR + TPB - 1) / TPB;
+  __shared__ double s_coord[12];
+  __shared__ double s_jinv[3][3];
+  __shared__ double s_det;
+  __shared__ double s_G[QC][3][NSV];
+  __shared__ double s_Gw[QC][3][NSV];
+  __shared__ double s_pw[QC][NSV];
+  __shared__ double s_f[QC];
+  __shared__ int s_col[NSV];
+  __shared__ int64_t s_blk[NSV], s_row[NSV];
+  __shared__ int32_t s_map[NSV * NSV];
+  const int tid = threadIdx.x;
+  const int64_t e = list[blockIdx.x];
+  const HpTables &T = *tab;
+  if (tid < 12) s_coord[tid] = g.xyz[3 * g.nodes[e * 4 + tid / 3] + tid % 3];
+  for (int k = tid; k < NPAIR; k += TPB) s_map[k] = g.map[e * NPAIR + k];
+  if (tid < NSV) {
+    int64_t id[4];
+    for (int k = 0; k < 4; k++) id[k] = g.nodes[e * 4 + k];
+    s_col[tid] = T.col_off[tid] + hp_variant(true, T.shape_side[tid], id);
+    const int64_t b = g.aidx[g.conn[e * NSV + tid]];
+    s_blk[tid] = b;
+    s_row[tid] = b >= 0 ? g.ia[b] : -1;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    // GradX of the straight tetrahedron: column k = x_{k+1} - x_0
+    double j[3][3];
+    for (int i = 0; i < 3; i++)
+      for (int k = 0; k < 3; k++) j[i][k] = s_coord[3 * (k + 1) + i] - s_coord[i];
+    const double det = -j[0][2] * j[1][1] * j[2][0] + j[0][1] * j[1][2] * j[2][0] +
+                       j[0][2] * j[1][0] * j[2][1] - j[0][0] * j[1][2] * j[2][1] -
+                       j[0][1] * j[1][0] * j[2][2] + j[0][0] * j[1][1] * j[2][2];
+    s_jinv[0][0] = (-j[1][2] * j[2][1] + j[1][1] * j[2][2]) / det;
+    s_jinv[0][1] = (j[0][2] * j[2][1] - j[0][1] * j[2][2]) / det;
+    s_jinv[0][2] = (-j[0][2] * j[1][1] + j[0][1] * j[1][2]) / det;
+    s_jinv[1][0] = (j[1][2] * j[2][0] - j[1][0] * j[2][2]) / det;
+    s_jinv[1][1] = (-j[0][2] * j[2][0] + j[0][0] * j[2][2]) / det;
+    s_jinv[1][2] = (j[0][2] * j[1][0] - j[0][0] * j[1][2]) / det;
+    s_jinv[2][0] = (-j[1][1] * j[2][0] + j[1][0] * j[2][1]) / det;
+    s_jinv[2][1] = (j[0][1] * j[2][0] - j[0][0] * j[2][1]) / det;
+    s_jinv[2][2] = (-j[0][1] * j[1][0] + j[0][0] * j[1][1]) / det;
+    s_det = fabs(det);
+  }
+  __syncthreads();
+  double acc[PER];
+#pragma unroll
+  for (int k = 0; k < PER; k++) acc[k] = 0.;
+  double accf = 0.;
+  const int nq = T.nq;
+  for (int q0 = 0; q0 < nq; q0 += QC) {
+    const int nqc = nq - q0 < QC ? nq - q0 : QC;
+    if (tid < nqc) {
+      const double *xi = T.pt[q0 + tid];
+      const double l[4] = {1. - xi[0] - xi[1] - xi[2], xi[0], xi[1], xi[2]};
+      double x[3] = {0., 0., 0.}, f[3];
+      for (int n = 0; n < 4; n++)
+        for (int i = 0; i < 3; i++) x[i] += s_coord[3 * n + i] * l[n];
+      forcing(g, x, f);
+      s_f[tid] = f[0];
+    }
+    for (int k = tid; k < nqc * NSV; k += TPB) {
+      const int qq = k / NSV, i = k % NSV, q = q0 + qq, col = s_col[i];
+      const double wd = T.w[q] * s_det;
+      for (int a = 0; a < 3; a++) {
+        double sum = 0.;
+        for (int b = 0; b < 3; b++) sum += s_jinv[b][a] * T.dphi[q][b][col];
+        s_G[qq][a][i] = sum;
+        s_Gw[qq][a][i] = sum * wd;
+      }
+      s_pw[qq][i] = wd * T.phi[q][col];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int idx = tid + k * TPB;
+      if (idx < NPAIR) {
+        const int i = idx / NSV, j = idx % NSV;
+        double sum = 0.;
+        for (int qq = 0; qq < nqc; qq++)
+          for (int a = 0; a < 3; a++) sum += s_Gw[qq][a][i] * s_G[qq][a][j];
+        acc[k] += sum;
+      }
+    }
+    if (tid < NSV)
+      for (int qq = 0; qq < nqc; qq++) accf += s_pw[qq][tid] * s_f[qq];
+    __syncthreads();
+  }
+  const double coef = g.coef ? g.coef[e] : g.scale;
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int idx = tid + k * TPB;
+    if (idx >= NPAIR) continue;
+    const int i = idx / NSV, j = idx % NSV;
+    const int64_t bi = s_blk[i], bj = s_blk[j];
+    if (bi < 0 || bj < 0 || bi > bj) continue;
+    const double v = coef * acc[k];
+    if (fabs(v) < 1.e-12) continue;  // IsZero drop rule of AddKel
+    g.a[bi == bj ? s_row[i] : s_row[i] + 1 + s_map[i * NSV + j]] += v;
+  }
+  if (tid < NSV && s_blk[tid] >= 0)
+    g.rhs[s_blk[tid]] += g.material == PZG_MAT_POISSON ? accf * g.scale : accf;
+}
+
+template <int NSB>
+__global__ __launch_bounds__(TPB) void k_bc_hp(ElemArgs g, const HpTables *__restrict__ tab,
+                                               const int32_t *__restrict__ list) {
+  constexpr int QC = 32;
+  constexpr int NPAIR = NSB * NSB;
+  constexpr int PER = (NPAIR + TPB - 1) / TPB;
+  __shared__ double s_coord[9];
+  __shared__ double s_det;
+  __shared__ double s_p[QC][NSB];
+  __shared__ double s_pw[QC][NSB];
+  __shared__ int s_col[NSB];
+  __shared__ int64_t s_blk[NSB], s_row[NSB];
+  __shared__ int32_t s_map[NSB * NSB];
+  const int tid = threadIdx.x;
+  const int64_t e = list[blockIdx.x];
+  const HpTables &T = *tab;
+  const int id = g.bcid[e];
+  const int type = g.bctype[id];
+  const double val = g.bcval[3 * id];
+  if (tid < 9) s_coord[tid] = g.xyz[3 * g.nodes[e * 3 + tid / 3] + tid % 3];
+  for (int k = tid; k < NPAIR; k += TPB) s_map[k] = g.map[e * NPAIR + k];
+  if (tid < NSB) {
+    int64_t ids[3];
+    for (int k = 0; k < 3; k++) ids[k] = g.nodes[e * 3 + k];
+    s_col[tid] = T.col_off[tid] + hp_variant(false, T.shape_side[tid], ids);
+    const int64_t b = g.aidx[g.conn[e * NSB + tid]];
+    s_blk[tid] = b;
+    s_row[tid] = b >= 0 ? g.ia[b] : -1;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    // 2D element in 3D space (TPZGeoEl::Jacobian): |det| = |g0| * |g1 - (g0.g1/|g0|^2) g0|
+    double gx[3][2];
+    for (int i = 0; i < 3; i++)
+      for (int k = 0; k < 2; k++) gx[i][k] = s_coord[3 * (k + 1) + i] - s_coord[i];
+    double n1 = 0, d12 = 0, n2 = 0;
+    for (int i = 0; i < 3; i++) { n1 += gx[i][0] * gx[i][0]; d12 += gx[i][0] * gx[i][1]; }
+    n1 = sqrt(n1);
+    for (int i = 0; i < 3; i++) {
+      const double v2 = gx[i][1] - d12 * (gx[i][0] / n1) / n1;
+      n2 += v2 * v2;
+    }
+    s_det = fabs(n1 * sqrt(n2));
+  }
+  __syncthreads();
+  double acc[PER];
+#pragma unroll
+  for (int k = 0; k < PER; k++) acc[k] = 0.;
+  double accf = 0.;
+  const int nq = T.nq;
+  for (int q0 = 0; q0 < nq; q0 += QC) {
+    const int nqc = nq - q0 < QC ? nq - q0 : QC;
+    for (int k = tid; k < nqc * NSB; k += TPB) {
+      const int qq = k / NSB, i = k % NSB, q = q0 + qq;
+      const double ph = T.phi[q][s_col[i]];
+      s_p[qq][i] = ph;
+      s_pw[qq][i] = ph * T.w[q] * s_det;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int idx = tid + k * TPB;
+      if (idx < NPAIR) {
+        const int i = idx / NSB, j = idx % NSB;
+        double sum = 0.;
+        for (int qq = 0; qq < nqc; qq++) sum += s_pw[qq][i] * s_p[qq][j];
+        acc[k] += sum;
+      }
+    }
+    if (tid < NSB)
+      for (int qq = 0; qq < nqc; qq++) accf += s_pw[qq][tid];
+    __syncthreads();
+  }
+  if (type == PZG_BC_DIRICHLET) {
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int idx = tid + k * TPB;
+      if (idx >= NPAIR) continue;
+      const int i = idx / NSB, j = idx % NSB;
+      const int64_t bi = s_blk[i], bj = s_blk[j];
+      if (bi < 0 || bj < 0 || bi > bj) continue;
+      const double v = g.big * acc[k];
+      if (fabs(v) < 1.e-12) continue;
+      g.a[bi == bj ? s_row[i] : s_row[i] + 1 + s_map[i * NSB + j]] += v;
+    }
+  }
+  if (tid < NSB && s_blk[tid] >= 0) {
+    double f;
+    if (type == PZG_BC_DIRICHLET) f = g.big * val * accf;
+    else f = val * accf * (g.material == PZG_MAT_POISSON ? g.scale : 1.0);
+    g.rhs[s_blk[tid]] += f;
+  }
+}
+
 static ElemArgs make_args(Ctx &c, bool vol) {
   ElemArgs g;
   g.xyz = c.d_xyz.as<double>();
@@ -475,9 +714,55 @@ static int run_bc(Ctx &c) {
   return PZG_OK;
 }
 
+template <int NSV, int NSB>
+static int run_hp(Ctx &c) {
+  if (c.bs != 1) {
+    set_error("pzg_assemble: high-order tetrahedra support scalar materials only");
+    return PZG_EUNSUP;
+  }
+  if (c.vol_nsides != NSV || (c.nbc > 0 && c.bc_nsides != NSB) || c.thp_vol.size() != c.vol_rules.size() ||
+      (c.nbc > 0 && c.thp_bc.size() != c.bc_rules.size())) {
+    set_error("pzg_assemble: high-order tables do not match the mesh");
+    return PZG_EINVAL;
+  }
+  ElemArgs g = make_args(c, true);
+  size_t nbk = c.vol_rules.size();
+  for (size_t k = 0; k + 1 < c.vcolor_ptr.size(); k++) {
+    const int64_t n = c.vcolor_ptr[k + 1] - c.vcolor_ptr[k];
+    if (!n) continue;
+    hipLaunchKernelGGL((k_vol_hp<NSV>), dim3(n), dim3(TPB), 0, c.stream, g,
+                       c.d_thp_vol.as<HpTables>() + k % nbk,
+                       c.d_vcolor_list.as<int32_t>() + c.vcolor_ptr[k]);
+    PZG_HIP(hipGetLastError());
+  }
+  if (c.nbc == 0) return PZG_OK;
+  g = make_args(c, false);
+  nbk = c.bc_rules.size();
+  for (size_t k = 0; k + 1 < c.bcolor_ptr.size(); k++) {
+    const int64_t n = c.bcolor_ptr[k + 1] - c.bcolor_ptr[k];
+    if (!n) continue;
+    hipLaunchKernelGGL((k_bc_hp<NSB>), dim3(n), dim3(TPB), 0, c.stream, g,
+                       c.d_thp_bc.as<HpTables>() + k % nbk,
+                       c.d_bcolor_list.as<int32_t>() + c.bcolor_ptr[k]);
+    PZG_HIP(hipGetLastError());
+  }
+  return PZG_OK;
+}
+
 int launch_assemble(Ctx &c) {
   fill<double>(c, c.d_a.as<double>(), c.nnz_u, 0.);
   fill<double>(c, c.d_rhs.as<double>(), c.neq, 0.);
+  if (c.hp_simplex) {
+    // shapes of the order-P set: tetrahedron 4/10/20/35/56, triangle 3/6/10/15/21
+    switch (c.hp_order) {
+      case 1: return run_hp<4, 3>(c);
+      case 2: return run_hp<10, 6>(c);
+      case 3: return run_hp<20, 10>(c);
+      case 4: return run_hp<35, 15>(c);
+      case 5: return run_hp<56, 21>(c);
+      default: set_error("pzg_assemble: unsupported order"); return PZG_EUNSUP;
+    }
+  }
   for (const ElemTables &t : c.tvol_all)
     if (t.nq > MAXQ) return PZG_EUNSUP;
   for (const ElemTables &t : c.tbc_all)

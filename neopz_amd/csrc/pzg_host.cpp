@@ -128,8 +128,71 @@ static void group_by_bucket(std::vector<int64_t> &ptr, std::vector<int32_t> &lis
 static int etype_nc(int t) { return etype_ncorner(t); }
 static int etype_ns(int t) { return etype_nsides(t); }
 
+// High-order tetrahedra: replace every connect of m shapes by m virtual
+// connects of nstate equations (positions pos + k*nstate, so the equation
+// numbering is unchanged) and every element's side list by its list of
+// shapes of the order-P set, the k-th shape of a side mapped to the side's
+// k-th virtual connect, or to an empty connect where the side's order
+// carries fewer shapes.  SetupMatrixData on the virtual connects gives the
+// same IA/JA: the connects of one side are consecutive, coupled exactly when
+// the side is, and listed in ascending order.
+static int expand_hp(Ctx &c) {
+  const int P = c.porder, ns = c.nstate;
+  int vside[HP_MAXSH], vloc[HP_MAXSH], bside[HP_MAXSH], bloc[HP_MAXSH];
+  const int nsv = hp_shape_list(PZG_TET, P, vside, vloc);
+  const int nsb = hp_shape_list(PZG_TRI, P, bside, bloc);
+  std::vector<int64_t> vbase(c.nconn + 1, 0);
+  for (int64_t s = 0; s < c.nconn; s++) {
+    if (c.blk_size[s] < 0 || c.blk_size[s] % ns) {
+      set_error("pzg_mesh_set: block size not a multiple of nstate");
+      return PZG_EINVAL;
+    }
+    vbase[s + 1] = vbase[s] + c.blk_size[s] / ns;
+  }
+  const int64_t nvirt = vbase[c.nconn], empty = nvirt;
+  auto expand = [&](const std::vector<int64_t> &conn, int nsr, int etype, int nsx, const int *side,
+                    const int *loc, int64_t nel, std::vector<int64_t> &out) -> int {
+    for (int64_t e = 0; e < nel; e++)
+      for (int r = 0; r < nsr; r++) {
+        const int64_t cn = conn[e * nsr + r];
+        if (vbase[cn + 1] - vbase[cn] > simplex_nshape(etype, r, P)) {
+          set_error("pzg_mesh_set: a connect carries more shapes than its side allows at order p");
+          return PZG_EINVAL;
+        }
+      }
+    out.assign(nel * nsx, empty);
+    for (int64_t e = 0; e < nel; e++)
+      for (int i = 0; i < nsx; i++) {
+        const int64_t cn = conn[e * nsr + side[i]];
+        if (loc[i] < vbase[cn + 1] - vbase[cn]) out[e * nsx + i] = vbase[cn] + loc[i];
+      }
+    return PZG_OK;
+  };
+  std::vector<int64_t> vconn, bconn;
+  PZG_TRY(expand(c.vol_conn, c.vol_nsides, PZG_TET, nsv, vside, vloc, c.nvol, vconn));
+  if (c.nbc > 0) PZG_TRY(expand(c.bc_conn, c.bc_nsides, PZG_TRI, nsb, bside, bloc, c.nbc, bconn));
+  std::vector<int64_t> pos(nvirt + 1);
+  std::vector<int32_t> size(nvirt + 1, ns);
+  for (int64_t s = 0; s < c.nconn; s++)
+    for (int64_t k = 0; k < vbase[s + 1] - vbase[s]; k++) pos[vbase[s] + k] = c.blk_pos[s] + k * ns;
+  pos[empty] = c.neq;
+  size[empty] = 0;
+  c.vol_conn.swap(vconn);
+  c.bc_conn.swap(bconn);
+  c.blk_pos.swap(pos);
+  c.blk_size.swap(size);
+  c.nconn = nvirt + 1;
+  c.vol_nsides = nsv;
+  c.bc_nsides = nsb;
+  c.vol_side_real.assign(vside, vside + nsv);
+  c.hp_simplex = true;
+  c.hp_order = P;
+  return PZG_OK;
+}
+
 static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
-  if (!m || (m->dim != 2 && m->dim != 3) || m->porder < 1 || m->porder > 2 ||
+  const int pmax = (m && m->vol_etype == PZG_TET && m->nstate == 1) ? HP_MAXP : 2;
+  if (!m || (m->dim != 2 && m->dim != 3) || m->porder < 1 || m->porder > pmax ||
       (m->nstate != 1 && m->nstate != 3) || m->material < 0 || m->material > 2 ||
       (m->material == PZG_MAT_ELAST3D && (m->nstate != 3 || m->dim != 3)) ||
       (m->material != PZG_MAT_ELAST3D && m->nstate != 1) || m->nvol <= 0 || !m->xyz ||
@@ -196,8 +259,30 @@ static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
   if (c.nbumps > 0) c.bumps.assign(m->bumps, m->bumps + 6 * c.nbumps);
   c.bump_width = m->bump_width;
 
-  // Uniform active block size: every connect carries 0 or nstate equations
-  // (true for H1 p<=2), block positions must be consecutive in seq order.
+  for (int64_t v : c.vol_conn)
+    if (v < 0 || v >= c.nconn) { set_error("pzg_mesh_set: connect index out of range"); return PZG_EINVAL; }
+  for (int64_t v : c.bc_conn)
+    if (v < 0 || v >= c.nconn) { set_error("pzg_mesh_set: connect index out of range"); return PZG_EINVAL; }
+  c.hp_simplex = false;
+  c.hp_order = 0;
+  c.nconn_in = c.nconn;
+  c.vol_side_real.clear();
+  c.thp_vol.clear();
+  c.thp_bc.clear();
+  {
+    bool multi = false;
+    for (int64_t s = 0; s < c.nconn; s++) multi |= c.blk_size[s] > c.nstate;
+    if (multi || (vt == PZG_TET && c.porder > 2)) {
+      if (vt != PZG_TET) {
+        set_error("pzg_mesh_set: connects with several shapes are supported on tetrahedra only");
+        return PZG_EUNSUP;
+      }
+      PZG_TRY(expand_hp(c));
+    }
+  }
+
+  // Uniform active block size: every (virtual) connect carries 0 or nstate
+  // equations, block positions must be consecutive in seq order.
   c.bs = c.nstate;
   c.aidx.assign(c.nconn, -1);
   int64_t nb = 0, pos = 0;
@@ -249,6 +334,13 @@ static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
     }
     if (uniq.empty()) uniq.push_back(dflt);
     tabs.assign(uniq.size(), ElemTables());
+    if (c.hp_simplex) {
+      std::vector<HpTables> &ht = etype == PZG_TET ? c.thp_vol : c.thp_bc;
+      ht.resize(uniq.size());
+      for (size_t k = 0; k < uniq.size(); k++)
+        PZG_TRY(build_hp_tables(etype, c.hp_order, uniq[k], ht[k]));
+      return PZG_OK;
+    }
     for (size_t k = 0; k < uniq.size(); k++) PZG_TRY(build_tables(etype, c.porder, uniq[k], tabs[k]));
     return PZG_OK;
   };
@@ -279,6 +371,13 @@ static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
                      sizeof(ElemTables) * c.tvol_all.size(), s));
   PZG_TRY(dev_upload(c.d_tbc, reinterpret_cast<const char *>(c.tbc_all.data()),
                      sizeof(ElemTables) * c.tbc_all.size(), s));
+  if (c.hp_simplex) {
+    PZG_TRY(dev_upload(c.d_thp_vol, reinterpret_cast<const char *>(c.thp_vol.data()),
+                       sizeof(HpTables) * c.thp_vol.size(), s));
+    if (!c.thp_bc.empty())
+      PZG_TRY(dev_upload(c.d_thp_bc, reinterpret_cast<const char *>(c.thp_bc.data()),
+                         sizeof(HpTables) * c.thp_bc.size(), s));
+  }
   PZG_HIP(hipStreamSynchronize(s));
   c.has_mesh = true;
   c.has_struct = false;

@@ -64,6 +64,29 @@ struct ElemTables {
   double cdphi[MAXQ][3][8];    // corner reference derivatives
 };
 
+// High-order triangles / tetrahedra (orders 3..5).  Every shape of the
+// maximum-order set is a "virtual side" with one connect of nstate equations:
+// a side of order k < P carries the first NConnectShapeF(side, k) functions of
+// the order-P set (edge: Chebyshev degree; face: total degree; interior at
+// its own order, prefix of the order-P set for P <= 5).  A shape's value
+// depends on the id-based transform of its real side (edge: 2 variants,
+// face: 6), so every variant has its own table column.
+constexpr int HP_MAXP = 5;       // highest supported order
+constexpr int HP_MAXQ = 343;     // collapsed rule, 7 points per direction
+constexpr int HP_MAXSH = 56;     // tetrahedron of order 5
+constexpr int HP_MAXCOL = 200;   // tetrahedron of order 5, all variants
+
+struct HpTables {
+  int etype = 0, dim = 0, ncorner = 0, order = 0, nsh = 0, nq = 0, ncol = 0;
+  int shape_side[HP_MAXSH];      // real side of shape i
+  int shape_loc[HP_MAXSH];       // index of shape i among its side's functions
+  int col_off[HP_MAXSH];         // table column of variant 0 (variant v: col_off + v)
+  double w[HP_MAXQ];             // quadrature weights
+  double pt[HP_MAXQ][3];         // quadrature points
+  double phi[HP_MAXQ][HP_MAXCOL];
+  double dphi[HP_MAXQ][3][HP_MAXCOL];
+};
+
 // Device-side scalar state of an iterative solve.
 struct SolveScalars {
   double rho, rho_old, pq, alpha, beta, rr, normb, resid, tol, pr, zq;
@@ -99,6 +122,13 @@ struct Ctx {
   std::vector<ElemTables> tvol_all, tbc_all;
   int vol_ncorner = 0, vol_nsides = 0, bc_ncorner = 0, bc_nsides = 0;
   int vol_etype = 0, bc_etype = 0;  // PZG_QUAD/PZG_HEX/PZG_TET, PZG_LINE/PZG_QUAD/PZG_TRI
+  // high-order simplices: connects expanded into one virtual connect per
+  // shape (vol_conn/bc_conn/blk_* then hold the virtual ones); hp_order = P
+  bool hp_simplex = false;
+  int hp_order = 0;
+  int64_t nconn_in = 0;              // connects as given by the caller
+  std::vector<int> vol_side_real;    // virtual side -> real side (volume elements)
+  std::vector<HpTables> thp_vol, thp_bc;
 
   // connect -> active block index (-1 for size-0 connects)
   std::vector<int64_t> aidx;
@@ -120,7 +150,7 @@ struct Ctx {
 
   // ---- assembly (device) -------------------------------------------------
   DevBuf d_xyz, d_vnodes, d_vconn, d_vcoef, d_bnodes, d_bconn, d_bcid, d_bctype, d_bcval;
-  DevBuf d_aidx, d_bumps, d_vmap, d_bmap, d_tvol, d_tbc;
+  DevBuf d_aidx, d_bumps, d_vmap, d_bmap, d_tvol, d_tbc, d_thp_vol, d_thp_bc;
   DevBuf d_vcolor_list, d_bcolor_list;
   std::vector<int64_t> vcolor_ptr, bcolor_ptr;  // color c: list[ptr[c] .. ptr[c+1])
   bool assembled = false;
@@ -159,6 +189,12 @@ struct Ctx {
 
 // ---- element tables (pzg_shape.cpp) ----------------------------------------
 int build_tables(int etype, int p, int rule_order, ElemTables &t);
+// high-order simplex tables of order P at the rule of rule_order
+int build_hp_tables(int etype, int P, int rule_order, HpTables &t);
+// canonical shape list of a simplex of order P: real side and local index
+int hp_shape_list(int etype, int P, int *side, int *loc);
+// NConnectShapeF of side s of a triangle / tetrahedron at order p
+int simplex_nshape(int etype, int s, int p);
 int etype_ncorner(int t);
 int etype_nsides(int t);
 int gauss_legendre(int order, double *loc, double *w);

@@ -12,8 +12,10 @@
 // their closed tensor form: a side function is the product over coordinate
 // directions of 4*l0*l1 (directions the side spans) or l0 / l1 (directions in
 // which the side is fixed at -1 / +1), l0 = (1-xi)/2, l1 = (1+xi)/2.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "pzg_internal.h"
 
@@ -210,6 +212,317 @@ static int eval_shapes(int etype, int p, const double *xi, double *phi, double d
     ns++;
   }
   return ns;
+}
+
+// ---------------------------------------------------------------------------
+// High-order triangles and tetrahedra (TPZShapeH1::Shape, Shape/TPZShapeH1.cpp:
+// 40-116, with the side transforms of ComputeTransforms,
+// Shape/pzgenericshape.cpp:12-67).  In barycentric coordinates l_k:
+//   blend of a side: 4 l_a l_b (edge), 27 l_a l_b l_c (face), 54 l_0 l_1 l_2 l_3
+//   (tetrahedron interior)  (Shape/pzshapetetra.cpp:101-150,
+//   Shape/pzshapetriang.cpp:53-81);
+//   edge (a,b): s = l_b - l_a (TransformElementToSide), negated unless
+//   id_a < id_b; functions blend * T_k(s), k = 0..n-1 (Chebyshev);
+//   face (a,b,c): m_b, m_c with m_k = l_k + l_o/3 for the face opposite
+//   corner 0 (o = 0), m_k = l_k otherwise, m_a = 1 - m_b - m_c; with u, v the
+//   m of the face corners holding the middle and the largest id
+//   (GetTransformId2dT + gTrans2dT), functions blend * T_i(2u-1) T_j(2v-1) by
+//   i+j ascending, then j;
+//   tetrahedron interior: blend * T_i(2x-1) T_j(2y-1) T_k(2z-1), i, j, k
+//   ascending with i+j+k < P-3.
+// Side numbering: Topology/tpztriangle.h:290-291, Topology/tpztetrahedron.h:
+// 280-283.
+
+int simplex_nshape(int etype, int s, int p) {
+  const int nc = etype == PZG_TRI ? 3 : 4;
+  if (s < nc) return 1;
+  int nl = etype == PZG_TRI ? (s < 6 ? 2 : 3) : (s < 10 ? 2 : s < 14 ? 3 : 4);
+  if (nl == 2) return p > 1 ? p - 1 : 0;
+  if (nl == 3) return p > 2 ? (p - 1) * (p - 2) / 2 : 0;
+  return p > 3 ? (p - 1) * (p - 2) * (p - 3) / 6 : 0;
+}
+
+int hp_shape_list(int etype, int P, int *side, int *loc) {
+  int n = 0;
+  for (int s = 0; s < etype_nsides(etype); s++)
+    for (int k = 0; k < simplex_nshape(etype, s, P); k++, n++) {
+      if (side) side[n] = s;
+      if (loc) loc[n] = k;
+    }
+  return n;
+}
+
+static int side_nvariants(int etype, int s) {
+  int loc[8], nl = side_corner_list(etype, s, loc);
+  if (nl == 2) return 2;
+  if (nl == 3) return 6;
+  return 1;
+}
+
+// transform id of a triangle from its three corner ids
+// (TPZShapeTriang::GetTransformId2dT, Shape/pzshapetriang.cpp:410-431)
+static int tri_transform_id(int64_t i0, int64_t i1, int64_t i2) {
+  const int64_t id[3] = {i0, i1, i2};
+  int a = id[0] < id[1] ? 0 : 1;
+  int mn = id[2] < id[a] ? 2 : a;
+  int n1 = (mn + 1) % 3, n2 = (mn + 2) % 3;
+  return 2 * mn + (id[n1] < id[n2] ? 0 : 1);
+}
+
+static void cheb(double x, int n, double *t, double *dt) {
+  if (n <= 0) return;
+  t[0] = 1.;
+  dt[0] = 0.;
+  if (n > 1) { t[1] = x; dt[1] = 1.; }
+  for (int k = 2; k < n; k++) {
+    t[k] = 2.0 * x * t[k - 1] - t[k - 2];
+    dt[k] = 2.0 * x * dt[k - 1] + 2.0 * t[k - 1] - dt[k - 2];
+  }
+}
+
+// all shapes of order P (every side at P) for corner ids `ids` at xi;
+// returns the count, dphi[d][i]
+static int eval_simplex_hp(int etype, int P, const int64_t *ids, const double *xi, double *phi,
+                           double dphi[3][HP_MAXSH]) {
+  const int dim = etype == PZG_TRI ? 2 : 3, nc = dim + 1, nsides = etype_nsides(etype);
+  double l[4], g[4][3];
+  l[0] = 1.;
+  for (int d = 0; d < dim; d++) { l[0] -= xi[d]; l[d + 1] = xi[d]; }
+  for (int k = 0; k < 4; k++)
+    for (int d = 0; d < 3; d++) g[k][d] = (d >= dim) ? 0. : (k == 0 ? -1. : (k == d + 1 ? 1. : 0.));
+  int n = 0;
+  for (int s = 0; s < nsides; s++) {
+    const int ns = simplex_nshape(etype, s, P);
+    if (!ns) continue;
+    int c[8];
+    const int nl = side_corner_list(etype, s, c);
+    if (nl == 1) {
+      phi[n] = l[c[0]];
+      for (int d = 0; d < dim; d++) dphi[d][n] = g[c[0]][d];
+      n++;
+      continue;
+    }
+    const double cst = nl == 2 ? 4. : nl == 3 ? 27. : 54.;
+    double B = cst, dB[3];
+    for (int k = 0; k < nl; k++) B *= l[c[k]];
+    for (int d = 0; d < 3; d++) {
+      double acc = 0.;
+      for (int k = 0; k < nl; k++) {
+        double pr = g[c[k]][d];
+        for (int j = 0; j < nl; j++)
+          if (j != k) pr *= l[c[j]];
+        acc += pr;
+      }
+      dB[d] = cst * acc;
+    }
+    double f[HP_MAXSH], df[HP_MAXSH][3];
+    if (nl == 2) {
+      const double sg = ids[c[0]] < ids[c[1]] ? 1. : -1.;
+      double T[8], dT[8];
+      cheb(sg * (l[c[1]] - l[c[0]]), ns, T, dT);
+      for (int k = 0; k < ns; k++) {
+        f[k] = T[k];
+        for (int d = 0; d < 3; d++) df[k][d] = dT[k] * sg * (g[c[1]][d] - g[c[0]][d]);
+      }
+    } else if (nl == 3) {
+      const int opp = (etype == PZG_TET) ? 6 - c[0] - c[1] - c[2] : -1;
+      const double wo = opp == 0 ? 1. / 3. : 0.;
+      double m[3], gm[3][3];
+      for (int k = 1; k < 3; k++) {
+        m[k] = l[c[k]] + (opp >= 0 ? wo * l[opp] : 0.);
+        for (int d = 0; d < 3; d++) gm[k][d] = g[c[k]][d] + (opp >= 0 ? wo * g[opp][d] : 0.);
+      }
+      m[0] = 1. - m[1] - m[2];
+      for (int d = 0; d < 3; d++) gm[0][d] = -gm[1][d] - gm[2][d];
+      // face corners ranked by id: u <- middle, v <- largest
+      int r0 = 0, r1 = 1, r2 = 2;
+      if (ids[c[r1]] < ids[c[r0]]) std::swap(r0, r1);
+      if (ids[c[r2]] < ids[c[r1]]) std::swap(r1, r2);
+      if (ids[c[r1]] < ids[c[r0]]) std::swap(r0, r1);
+      const int deg = P - 2;  // degrees 0..P-3
+      double Tu[8], dTu[8], Tv[8], dTv[8];
+      cheb(2. * m[r1] - 1., deg, Tu, dTu);
+      cheb(2. * m[r2] - 1., deg, Tv, dTv);
+      int k = 0;
+      for (int tot = 0; k < ns; tot++)
+        for (int j = 0; j <= tot && k < ns; j++, k++) {
+          const int i = tot - j;
+          f[k] = Tu[i] * Tv[j];
+          for (int d = 0; d < 3; d++)
+            df[k][d] = 2. * (dTu[i] * Tv[j] * gm[r1][d] + Tu[i] * dTv[j] * gm[r2][d]);
+        }
+    } else {
+      const int ord = P - 3;
+      double T[3][8], dT[3][8];
+      for (int a = 0; a < 3; a++) cheb(2. * xi[a] - 1., ord, T[a], dT[a]);
+      int k = 0;
+      for (int i = 0; i < ord; i++)
+        for (int j = 0; j < ord; j++)
+          for (int q = 0; q < ord; q++) {
+            if (i + j + q >= ord) continue;
+            f[k] = T[0][i] * T[1][j] * T[2][q];
+            df[k][0] = 2. * dT[0][i] * T[1][j] * T[2][q];
+            df[k][1] = 2. * T[0][i] * dT[1][j] * T[2][q];
+            df[k][2] = 2. * T[0][i] * T[1][j] * dT[2][q];
+            k++;
+          }
+    }
+    for (int k = 0; k < ns; k++, n++) {
+      phi[n] = k == 0 ? B : B * f[k];
+      for (int d = 0; d < dim; d++) dphi[d][n] = k == 0 ? dB[d] : dB[d] * f[k] + B * df[k][d];
+    }
+  }
+  return n;
+}
+
+// Gauss-Jacobi points and weights, weight (1-t)^alpha on [-1,1]: eigenpairs of
+// the Jacobi matrix of the monic recurrence (Golub-Welsch) by cyclic Jacobi
+// rotations; points ascending.
+static void gauss_jacobi(int n, double alpha, double *t, double *w) {
+  double A[8][8] = {}, V[8][8] = {};
+  for (int i = 0; i < n; i++) V[i][i] = 1.;
+  for (int k = 0; k < n; k++) {
+    const double s = 2. * k + alpha;
+    A[k][k] = alpha == 0. ? 0. : -alpha * alpha / (s * (s + 2.));
+    if (k) A[k][k - 1] = A[k - 1][k] = std::sqrt(4. * k * k * (k + alpha) * (k + alpha) /
+                                                 (s * s * (s + 1.) * (s - 1.)));
+  }
+  for (int sweep = 0; sweep < 64; sweep++) {
+    double off = 0.;
+    for (int p = 0; p < n; p++)
+      for (int q = p + 1; q < n; q++) off += A[p][q] * A[p][q];
+    if (off < 1e-300) break;
+    for (int p = 0; p < n; p++)
+      for (int q = p + 1; q < n; q++) {
+        if (A[p][q] == 0.) continue;
+        const double th = (A[q][q] - A[p][p]) / (2. * A[p][q]);
+        const double tt = (th >= 0. ? 1. : -1.) / (std::fabs(th) + std::sqrt(th * th + 1.));
+        const double c = 1. / std::sqrt(tt * tt + 1.), sn = tt * c;
+        for (int k = 0; k < n; k++) {
+          const double a = A[k][p], b = A[k][q];
+          A[k][p] = c * a - sn * b;
+          A[k][q] = sn * a + c * b;
+        }
+        for (int k = 0; k < n; k++) {
+          const double a = A[p][k], b = A[q][k];
+          A[p][k] = c * a - sn * b;
+          A[q][k] = sn * a + c * b;
+        }
+        for (int k = 0; k < n; k++) {
+          const double a = V[k][p], b = V[k][q];
+          V[k][p] = c * a - sn * b;
+          V[k][q] = sn * a + c * b;
+        }
+      }
+  }
+  int idx[8];
+  for (int i = 0; i < n; i++) idx[i] = i;
+  std::sort(idx, idx + n, [&](int a, int b) { return A[a][a] < A[b][b]; });
+  const double mu0 = std::pow(2., alpha + 1.) / (alpha + 1.);
+  for (int i = 0; i < n; i++) {
+    t[i] = A[idx[i]][idx[i]];
+    w[i] = mu0 * V[0][idx[i]] * V[0][idx[i]];
+  }
+}
+
+// Rule for high orders on the reference simplex: the symmetric rules up to
+// degree 4 (triangle) / 5 (tetrahedron), else the collapsed product of
+// Gauss-Jacobi rules of weights 1, (1-b), (1-c)^2 with n = (order+2)/2 points
+// per direction (degree 2n-1).  Exact for the polynomial integrands of affine
+// simplices with element-wise constant data, so the element matrices equal
+// the reference's (Integral/tpzintrulet3d.cpp tables) up to rounding.
+static int simplex_rule_hp(int etype, int order, double (*pts)[3], double *w, int cap) {
+  if ((etype == PZG_TRI && order <= 4) || (etype == PZG_TET && order <= 5))
+    return simplex_rule(etype, order, pts, w);
+  const int n = (order + 2) / 2, dim = etype == PZG_TRI ? 2 : 3;
+  int tot = 1;
+  for (int d = 0; d < dim; d++) tot *= n;
+  if (n > 8 || tot > cap) return -1;
+  double ta[8], wa[8], tb[8], wb[8], tc[8], wc[8];
+  gauss_jacobi(n, 0., ta, wa);
+  gauss_jacobi(n, 1., tb, wb);
+  gauss_jacobi(n, 2., tc, wc);
+  int q = 0;
+  for (int k = 0; k < (dim == 3 ? n : 1); k++)
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++, q++) {
+        const double a = 0.5 * (1. + ta[i]), b = 0.5 * (1. + tb[j]);
+        if (dim == 2) {
+          pts[q][0] = a * (1. - b); pts[q][1] = b; pts[q][2] = 0.;
+          w[q] = 0.5 * wa[i] * 0.25 * wb[j];
+        } else {
+          const double cc = 0.5 * (1. + tc[k]);
+          pts[q][0] = a * (1. - b) * (1. - cc); pts[q][1] = b * (1. - cc); pts[q][2] = cc;
+          w[q] = 0.5 * wa[i] * 0.25 * wb[j] * 0.125 * wc[k];
+        }
+      }
+  return q;
+}
+
+int build_hp_tables(int etype, int P, int rule_order, HpTables &t) {
+  if ((etype != PZG_TRI && etype != PZG_TET) || P < 1 || P > HP_MAXP) {
+    set_error("high-order tables: triangles and tetrahedra of order 1..5 only");
+    return PZG_EUNSUP;
+  }
+  std::memset(&t, 0, sizeof(t));
+  t.etype = etype;
+  t.dim = etype == PZG_TRI ? 2 : 3;
+  t.ncorner = t.dim + 1;
+  t.order = P;
+  t.nsh = hp_shape_list(etype, P, t.shape_side, t.shape_loc);
+  for (int i = 0; i < t.nsh; i++) {
+    t.col_off[i] = t.ncol;
+    t.ncol += side_nvariants(etype, t.shape_side[i]);
+  }
+  if (t.nsh > HP_MAXSH || t.ncol > HP_MAXCOL) {
+    set_error("high-order tables: too many shapes");
+    return PZG_EUNSUP;
+  }
+  std::vector<double> ptbuf(3 * HP_MAXQ), w(HP_MAXQ);
+  double(*pts)[3] = reinterpret_cast<double(*)[3]>(ptbuf.data());
+  const int nq = simplex_rule_hp(etype, rule_order, pts, w.data(), HP_MAXQ);
+  if (nq < 0) {
+    set_error("integration rule order not available on simplices");
+    return PZG_EUNSUP;
+  }
+  t.nq = nq;
+  for (int q = 0; q < nq; q++) {
+    t.w[q] = w[q];
+    for (int d = 0; d < 3; d++) t.pt[q][d] = pts[q][d];
+  }
+  // corner ids that realise variant v of side s: edge v = 0 <-> id_a < id_b;
+  // face: the id ranking whose transform id is v
+  for (int s = 0; s < etype_nsides(etype); s++) {
+    const int nv = side_nvariants(etype, s);
+    int c[8];
+    const int nl = side_corner_list(etype, s, c);
+    for (int v = 0; v < nv; v++) {
+      int64_t ids[4] = {10, 11, 12, 13};
+      if (nl == 2) {
+        ids[c[0]] = v == 0 ? 0 : 1;
+        ids[c[1]] = v == 0 ? 1 : 0;
+      } else if (nl == 3) {
+        int perm[3] = {0, 1, 2};
+        bool found = false;
+        do {
+          if (tri_transform_id(perm[0], perm[1], perm[2]) == v) { found = true; break; }
+        } while (std::next_permutation(perm, perm + 3));
+        if (!found) return PZG_EINVAL;
+        for (int k = 0; k < 3; k++) ids[c[k]] = perm[k];
+      }
+      for (int q = 0; q < nq; q++) {
+        double phi[HP_MAXSH], dphi[3][HP_MAXSH];
+        eval_simplex_hp(etype, P, ids, pts[q], phi, dphi);
+        for (int i = 0; i < t.nsh; i++) {
+          if (t.shape_side[i] != s) continue;
+          const int col = t.col_off[i] + v;
+          t.phi[q][col] = phi[i];
+          for (int d = 0; d < t.dim; d++) t.dphi[q][d][col] = dphi[d][i];
+        }
+      }
+    }
+  }
+  return PZG_OK;
 }
 
 int build_tables(int etype, int p, int rule_order, ElemTables &t) {

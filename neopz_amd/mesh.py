@@ -52,8 +52,24 @@ def _vol_rule(pb, p):
     return np.where(p < ff, ff + p, 2 * p)
 
 
+# side kinds of a connect: corner, edge, quadrilateral face, hexahedron
+# interior, triangle face, tetrahedron interior
+K_CORNER, K_EDGE, K_QFACE, K_HEX, K_TFACE, K_TET = range(6)
+
+
+def _nshape(kind, order):
+    """TPZShape*::NConnectShapeF (Shape/pzshapequad.cpp:468-482,
+    Shape/pzshapecube.cpp:573-584, Shape/pzshapetriang.cpp:452-470,
+    Shape/pzshapetetra.cpp:451-470) per connect."""
+    q = np.maximum(order - 1, 0)
+    return np.select(
+        [kind == K_CORNER, kind == K_EDGE, kind == K_QFACE, kind == K_HEX, kind == K_TFACE],
+        [np.ones_like(q), q, q * q, q * q * q, np.maximum((order - 1) * (order - 2) // 2, 0)],
+        np.maximum((order - 1) * (order - 2) * (order - 3) // 6, 0)).astype(np.int64)
+
+
 def _common(pb, econn_vol, econn_bc, nconn, corner_flags, xyz, vol_nodes, bc_nodes, bc_id,
-            no_shape=None, vol_etype=0):
+            side_kind=None, vol_etype=0):
     ns = pb.nstate
     # connect orders: corners 1; other sides the minimum preferred order of the
     # elements sharing them (TPZInterpolatedElement::IdentifySideOrder /
@@ -66,10 +82,13 @@ def _common(pb, econn_vol, econn_bc, nconn, corner_flags, xyz, vol_nodes, bc_nod
     order = np.full(nconn, pb.p, dtype=np.int64)
     np.minimum.at(order, econn_vol.ravel(), np.repeat(pe, econn_vol.shape[1]))
     order[corner_flags] = 1
-    # shapes per connect (p <= 2): corners 1, other sides 1 from order 2 on
-    nshape = np.where(corner_flags, 1, (order >= 2).astype(np.int64))
-    if no_shape is not None:  # simplex faces and interiors: shapes from p = 3 / 4 on
-        nshape[no_shape] = 0
+    if side_kind is None:
+        # quadrilaterals / hexahedra, p <= 2: corners 1, other sides 1 from order 2 on
+        if pb.p > 2:
+            raise ValueError("quadrilateral/hexahedral meshes support p <= 2")
+        nshape = np.where(corner_flags, 1, (order >= 2).astype(np.int64))
+    else:
+        nshape = _nshape(side_kind, order)
     blk_size = (nshape * ns).astype(np.int32)
     # element integration rules from the highest connect order (MaxOrder,
     # Mesh/pzinterpolationspace.cpp:53-83)
@@ -328,10 +347,13 @@ def _build3d_tet(pb):
     bc_conn = seq[nv:].reshape(kb.shape[0], kb.shape[1]) if len(kb) else np.zeros((0, 7), np.int64)
     corner = np.zeros(nconn, dtype=bool)
     corner[econn[:, :4].ravel()] = True
-    no_shape = np.zeros(nconn, dtype=bool)
-    no_shape[econn[:, 10:].ravel()] = True                     # faces and interior
+    kind = np.empty(nconn, dtype=np.int64)
+    kind[econn[:, :4].ravel()] = K_CORNER
+    kind[econn[:, 4:10].ravel()] = K_EDGE
+    kind[econn[:, 10:14].ravel()] = K_TFACE
+    kind[econn[:, 14].ravel()] = K_TET
     return _common(pb, econn.astype(np.int64), bc_conn.astype(np.int64), nconn, corner,
-                   xyz.ravel(), tets.astype(np.int64), bc_nodes, bc_id, no_shape=no_shape,
+                   xyz.ravel(), tets.astype(np.int64), bc_nodes, bc_id, side_kind=kind,
                    vol_etype=5)
 
 

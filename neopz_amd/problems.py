@@ -144,7 +144,7 @@ def darcy3d_tet(n=32, pmax=2, seed=0, sigma=1.0, hp=True):
     """hp Darcy on the Kuhn tetrahedral mesh of the unit cube (6 tetrahedra per
     cube of an n^3 grid): kappa_e = exp(N(0, sigma^2)) per element, f = 1,
     u = 0 on the boundary; per-element order drawn uniformly from 1..pmax by
-    seed (orders above 2 are outside this build's shape functions)."""
+    seed (pmax <= 5)."""
     rng = np.random.default_rng(seed)
     nvol = 6 * n ** 3
     k = np.exp(sigma * rng.standard_normal(nvol))

@@ -19,6 +19,7 @@ _LIB = os.path.join(_HERE, "liboracle.so")
 
 POISSON, DARCY, ELAST3D, MOCK1D = 0, 1, 2, 3
 LINE, QUAD, HEX, TRI, TET = 1, 2, 3, 4, 5
+MAXNE = 192  # largest element dof count (ORC_MAXNE)
 FORCE_NONE, FORCE_CONST, FORCE_BUMPS = 0, 1, 2
 
 
@@ -82,6 +83,8 @@ def lib():
         L.orc_multadd.argtypes = [C.c_int64, i64p, i64p, dp, dp, dp, dp, C.c_double, C.c_double]
         L.orc_pcg.argtypes = [C.c_int64, i64p, i64p, dp, dp, dp, i64p, dp, C.c_int, dp]
         L.orc_simplex_rule.argtypes = [C.c_int, C.c_int, dp, dp]
+        L.orc_collapsed_rule.argtypes = [C.c_int, C.c_int, dp, dp]
+        L.orc_shape_h1.argtypes = [C.c_int, i64p, C.POINTER(C.c_int), dp, dp, dp]
         L.orc_nnz_closed_form.argtypes = [C.c_int] * 6
         L.orc_nnz_closed_form.restype = C.c_int64
         L.free = C.CDLL(None).free
@@ -106,12 +109,37 @@ def gauss_legendre(order):
 
 
 def simplex_rule(etype, order):
-    pts = np.zeros(3 * 64)
-    w = np.zeros(64)
+    pts = np.zeros(3 * 4096)
+    w = np.zeros(4096)
     n = lib().orc_simplex_rule(etype, order, _dp(pts), _dp(w))
     if n < 0:
         raise ValueError("rule order not available")
     return pts[:3 * n].reshape(n, 3).copy(), w[:n].copy()
+
+
+def collapsed_rule(etype, n):
+    pts = np.zeros(3 * 4096)
+    w = np.zeros(4096)
+    q = lib().orc_collapsed_rule(etype, n, _dp(pts), _dp(w))
+    if q < 0:
+        raise ValueError("rule size not available")
+    return pts[:3 * q].reshape(q, 3).copy(), w[:q].copy()
+
+
+def shape_h1(etype, ids, orders, xi):
+    """Hierarchical shapes of a triangle/tetrahedron with corner ids and
+    connect orders (sides after the corners); returns (phi, dphi[dim, ns])."""
+    xi = np.ascontiguousarray(xi, dtype=np.float64)
+    ids = np.ascontiguousarray(ids, dtype=np.int64)
+    orders = np.ascontiguousarray(orders, dtype=np.int32)
+    phi = np.zeros(256)
+    dphi = np.zeros(3 * 256)
+    ns = lib().orc_shape_h1(etype, _ip(ids), orders.ctypes.data_as(C.POINTER(C.c_int)),
+                            _dp(np.r_[xi, np.zeros(3 - len(xi))]), _dp(phi), _dp(dphi))
+    if ns < 0:
+        raise ValueError("bad element type or order")
+    dim = {TRI: 2, TET: 3}[etype]
+    return phi[:ns].copy(), dphi[: dim * ns].reshape(dim, ns).copy()
 
 
 def shape(etype, p, xi):
@@ -204,6 +232,19 @@ class Oracle:
         blk_size = np.ctypeslib.as_array(m.blk_size, shape=(m.nconn,)).copy()
         return econn, blk_pos, blk_size
 
+    def element(self, iel):
+        """Element iel: type, corner node ids, corner coordinates and the
+        orders of its side connects (sides after the corners)."""
+        m = self.m
+        t = m.etype[iel]
+        nc = {LINE: 2, QUAD: 4, HEX: 8, TRI: 3, TET: 4}[t]
+        ns = {LINE: 3, QUAD: 9, HEX: 27, TRI: 7, TET: 15}[t]
+        ids = np.array([m.enodes[8 * iel + k] for k in range(nc)], dtype=np.int64)
+        xyz = np.array([[m.xyz[3 * i + d] for d in range(3)] for i in ids])
+        orders = np.array([m.conn_order[m.econn[27 * iel + s]] for s in range(nc, ns)],
+                          dtype=np.int32)
+        return {"etype": t, "ids": ids, "xyz": xyz, "orders": orders}
+
     def pattern(self):
         if self.ia is None:
             ia = C.POINTER(C.c_int64)()
@@ -216,11 +257,13 @@ class Oracle:
         return self.ia, self.ja
 
     def calcstiff(self, iel):
-        ek = np.zeros(81 * 81)
-        ef = np.zeros(81)
-        dest = np.zeros(81, dtype=np.int64)
+        ek = np.zeros(MAXNE * MAXNE)
+        ef = np.zeros(MAXNE)
+        dest = np.zeros(MAXNE, dtype=np.int64)
         ne = lib().orc_calcstiff(C.byref(self.pb), C.byref(self.m), iel, _dp(ek), _dp(ef),
                                  _ip(dest))
+        if ne < 0:
+            raise RuntimeError("oracle CalcStiff failed (element too large or order > 8)")
         return ek[: ne * ne].reshape(ne, ne).copy(), ef[:ne].copy(), dest[:ne].copy()
 
     def assemble(self):

@@ -252,6 +252,158 @@ int orc_shape(int t, int p, const double *xi, double *phi, double *dphi) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* Hierarchical H1 shapes of any order on triangles and tetrahedra           */
+/* (TPZShapeH1::Shape, Shape/TPZShapeH1.cpp:40-116).  Per side with n > 0    */
+/* functions (NConnectShapeF of the connect's order): the blend function,   */
+/* then blend * phin(i), i = 1..n-1, where phin are the side's internal      */
+/* functions (ShapeInternal) at the point mapped to the side and reoriented  */
+/* by the corner-node ids (ComputeTransforms, Shape/pzgenericshape.cpp:      */
+/* 12-67).  Written with barycentric coordinates l_k of the element:         */
+/*  * blends: edge 4 l_a l_b, face 27 l_a l_b l_c, tetrahedron interior 54   */
+/*    l_0 l_1 l_2 l_3 (ShapeGenerating, Shape/pzshapetetra.cpp:101-150,      */
+/*    Shape/pzshapetriang.cpp:53-81);                                        */
+/*  * edge (a,b): TransformElementToSide gives s = l_b - l_a                 */
+/*    (Topology/tpztetrahedron.cpp:460-545, tpztriangle.cpp:443-477), the    */
+/*    parametric transform flips it unless id_a < id_b                       */
+/*    (TPZShapeLinear::ParametricTransform, Shape/pzshapelinear.cpp:336-341);*/
+/*    phin(i) = T_i(s) (TPZShapeLinear::ShapeInternal + Chebyshev,           */
+/*    Shape/pzshapelinear.cpp:15-33,306-312);                                */
+/*  * face (a,b,c): element-to-face coordinates (xi, eta) = (m_b, m_c) with   */
+/*    m_k = l_k + w l_o (o the opposite corner, w = 1/3 for the face opposite */
+/*    corner 0, else 0) and m_a = 1 - xi - eta; the parametric transform of  */
+/*    GetTransformId2dT (Shape/pzshapetriang.cpp:18-27,410-431) turns them   */
+/*    into (u, v) = (m of the face corner with the middle id, m of the one   */
+/*    with the largest id); phin = T_i(2u-1) T_j(2v-1) enumerated by i+j     */
+/*    ascending, then j ascending (TPZShapeTriang::ShapeInternal,            */
+/*    Shape/pzshapetriang.cpp:279-311);                                      */
+/*  * tetrahedron interior: identity transform; phin = T_i(2x-1) T_j(2y-1)    */
+/*    T_k(2z-1) over i, j, k ascending with i+j+k < order-3                  */
+/*    (TPZShapeTetra::ShapeInternal, Shape/pzshapetetra.cpp:345-371).        */
+/* Gradients follow the reference's chain rule (dphiaux = mult^T dphin).     */
+/* ------------------------------------------------------------------------ */
+static void chebyshev(double x, int n, double *t, double *dt) {
+  if (n <= 0) return;
+  t[0] = 1.;
+  dt[0] = 0.;
+  if (n == 1) return;
+  t[1] = x;
+  dt[1] = 1.;
+  for (int k = 2; k < n; k++) {
+    t[k] = 2.0 * x * t[k - 1] - t[k - 2];
+    dt[k] = 2.0 * x * dt[k - 1] + 2.0 * t[k - 1] - dt[k - 2];
+  }
+}
+
+int orc_shape_h1(int t, const int64_t *ids, const int *orders, const double *xi, double *phi,
+                 double *dphi) {
+  if (!is_simplex(t)) return -1;
+  const int dim = etype_dim(t), nc = dim + 1, nsides = etype_nsides(t);
+  double l[4], dl[4][3];
+  l[0] = 1.;
+  for (int d = 0; d < dim; d++) { l[0] -= xi[d]; l[d + 1] = xi[d]; }
+  for (int k = 0; k < nc; k++)
+    for (int d = 0; d < 3; d++) dl[k][d] = d >= dim ? 0. : k == 0 ? -1. : (k == d + 1 ? 1. : 0.);
+  for (int s = nc; s < nsides; s++)
+    if (orders[s - nc] > 8) return -1; /* sizes of the local tables below */
+  /* count first: dphi is laid out [d][ns] */
+  int ns = nc;
+  for (int s = nc; s < nsides; s++) ns += nconnect_shape(t, s, orders[s - nc]);
+  int sh = 0;
+  for (int k = 0; k < nc; k++, sh++) {
+    phi[sh] = l[k];
+    for (int d = 0; d < dim; d++) dphi[d * ns + sh] = dl[k][d];
+  }
+  for (int s = nc; s < nsides; s++) {
+    int n = nconnect_shape(t, s, orders[s - nc]);
+    if (n == 0) continue;
+    int loc[4], nl = side_corners(t, s, loc);
+    double B = nl == 2 ? 4. : nl == 3 ? 27. : 54., dB[3] = {0, 0, 0};
+    for (int k = 0; k < nl; k++) B *= l[loc[k]];
+    for (int d = 0; d < dim; d++) {
+      double c = nl == 2 ? 4. : nl == 3 ? 27. : 54.;
+      double acc = 0.;
+      for (int k = 0; k < nl; k++) {
+        double pr = dl[loc[k]][d];
+        for (int j = 0; j < nl; j++)
+          if (j != k) pr *= l[loc[j]];
+        acc += pr;
+      }
+      dB[d] = c * acc;
+    }
+    double pin[64], dpin[64][3];
+    pin[0] = 1.;
+    dpin[0][0] = dpin[0][1] = dpin[0][2] = 0.;
+    if (nl == 2) {
+      int a = loc[0], b = loc[1];
+      double sg = ids[a] < ids[b] ? 1. : -1.;
+      double x = sg * (l[b] - l[a]), gx[3];
+      for (int d = 0; d < 3; d++) gx[d] = sg * (dl[b][d] - dl[a][d]);
+      double T[16], dT[16];
+      chebyshev(x, n, T, dT);
+      for (int i = 0; i < n; i++) {
+        pin[i] = T[i];
+        for (int d = 0; d < 3; d++) dpin[i][d] = dT[i] * gx[d];
+      }
+    } else if (nl == 3) {
+      /* face corners in side order and the face coordinates m_k */
+      double m[3], gm[3][3];
+      double w = 0.;
+      int o = -1;
+      if (t == ORC_TET) {
+        o = 0 + 1 + 2 + 3 - loc[0] - loc[1] - loc[2];
+        w = o == 0 ? 1. / 3. : 0.;
+      }
+      for (int k = 1; k < 3; k++) {
+        m[k] = l[loc[k]] + (o >= 0 ? w * l[o] : 0.);
+        for (int d = 0; d < 3; d++) gm[k][d] = dl[loc[k]][d] + (o >= 0 ? w * dl[o][d] : 0.);
+      }
+      m[0] = 1. - m[1] - m[2];
+      for (int d = 0; d < 3; d++) gm[0][d] = -gm[1][d] - gm[2][d];
+      /* order the face corners by id: middle -> u, largest -> v */
+      int r[3] = {0, 1, 2};
+      for (int i = 0; i < 3; i++)
+        for (int j = i + 1; j < 3; j++)
+          if (ids[loc[r[j]]] < ids[loc[r[i]]]) { int tmp = r[i]; r[i] = r[j]; r[j] = tmp; }
+      double u = 2. * m[r[1]] - 1., v = 2. * m[r[2]] - 1.;
+      double Tu[16], dTu[16], Tv[16], dTv[16];
+      chebyshev(u, orders[s - nc] - 2, Tu, dTu);
+      chebyshev(v, orders[s - nc] - 2, Tv, dTv);
+      int idx = 0;
+      for (int ipj = 0; idx < n; ipj++)
+        for (int j = 0; j <= ipj && idx < n; j++, idx++) {
+          int i = ipj - j;
+          pin[idx] = Tu[i] * Tv[j];
+          for (int d = 0; d < 3; d++)
+            dpin[idx][d] = 2. * dTu[i] * Tv[j] * gm[r[1]][d] + 2. * Tu[i] * dTv[j] * gm[r[2]][d];
+        }
+    } else {
+      int ord = orders[s - nc] - 3;
+      double T0[16], dT0[16], T1[16], dT1[16], T2[16], dT2[16];
+      chebyshev(2. * xi[0] - 1., ord, T0, dT0);
+      chebyshev(2. * xi[1] - 1., ord, T1, dT1);
+      chebyshev(2. * xi[2] - 1., ord, T2, dT2);
+      int idx = 0;
+      for (int i = 0; i < ord; i++)
+        for (int j = 0; j < ord; j++)
+          for (int k = 0; k < ord; k++)
+            if (i + j + k < ord) {
+              pin[idx] = T0[i] * T1[j] * T2[k];
+              dpin[idx][0] = 2. * dT0[i] * T1[j] * T2[k];
+              dpin[idx][1] = 2. * T0[i] * dT1[j] * T2[k];
+              dpin[idx][2] = 2. * T0[i] * T1[j] * dT2[k];
+              idx++;
+            }
+    }
+    for (int i = 0; i < n; i++, sh++) {
+      phi[sh] = i == 0 ? B : B * pin[i];
+      for (int d = 0; d < dim; d++)
+        dphi[d * ns + sh] = i == 0 ? dB[d] : dB[d] * pin[i] + B * dpin[i][d];
+    }
+  }
+  return ns;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Hash map: sorted corner-node tuple of a side -> connect index.            */
 /* ------------------------------------------------------------------------ */
 typedef struct {
@@ -642,8 +794,97 @@ static void forcing(const orc_problem *pb, const double *x, double *f) {
  * coefficients and loads) any rule of degree >= 2p integrates exactly, so the
  * element matrices equal the reference's (TPZIntTriang / TPZIntTetra3D,
  * Integral/pzquad.cpp:308-345) up to rounding. */
+/* Gauss-Jacobi rule with weight (1-t)^alpha on [-1,1]: Golub-Welsch on the
+ * symmetric tridiagonal matrix of the monic Jacobi recurrence (beta = 0),
+ * eigenpairs by cyclic Jacobi rotations; nodes ascending. */
+static void gauss_jacobi(int n, double alpha, double *t, double *w) {
+  double A[16][16], V[16][16];
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) { A[i][j] = 0.; V[i][j] = i == j ? 1. : 0.; }
+  for (int k = 0; k < n; k++) {
+    double s = 2. * k + alpha;
+    A[k][k] = (alpha == 0.) ? 0. : -alpha * alpha / (s * (s + 2.));
+    if (k > 0) {
+      double b = 4. * k * k * (k + alpha) * (k + alpha) / (s * s * (s + 1.) * (s - 1.));
+      A[k - 1][k] = A[k][k - 1] = sqrt(b);
+    }
+  }
+  for (int sweep = 0; sweep < 64; sweep++) {
+    double off = 0.;
+    for (int p = 0; p < n; p++)
+      for (int q = p + 1; q < n; q++) off += A[p][q] * A[p][q];
+    if (off < 1e-300) break;
+    for (int p = 0; p < n; p++)
+      for (int q = p + 1; q < n; q++) {
+        if (A[p][q] == 0.) continue;
+        double th = (A[q][q] - A[p][p]) / (2. * A[p][q]);
+        double tt = (th >= 0. ? 1. : -1.) / (fabs(th) + sqrt(th * th + 1.));
+        double c = 1. / sqrt(tt * tt + 1.), sn = tt * c;
+        for (int k = 0; k < n; k++) {
+          double akp = A[k][p], akq = A[k][q];
+          A[k][p] = c * akp - sn * akq;
+          A[k][q] = sn * akp + c * akq;
+        }
+        for (int k = 0; k < n; k++) {
+          double apk = A[p][k], aqk = A[q][k];
+          A[p][k] = c * apk - sn * aqk;
+          A[q][k] = sn * apk + c * aqk;
+        }
+        for (int k = 0; k < n; k++) {
+          double vkp = V[k][p], vkq = V[k][q];
+          V[k][p] = c * vkp - sn * vkq;
+          V[k][q] = sn * vkp + c * vkq;
+        }
+      }
+  }
+  double mu0 = pow(2., alpha + 1.) / (alpha + 1.);
+  int ord[16];
+  for (int i = 0; i < n; i++) ord[i] = i;
+  for (int i = 0; i < n; i++)
+    for (int j = i + 1; j < n; j++)
+      if (A[ord[j]][ord[j]] < A[ord[i]][ord[i]]) { int tmp = ord[i]; ord[i] = ord[j]; ord[j] = tmp; }
+  for (int i = 0; i < n; i++) {
+    t[i] = A[ord[i]][ord[i]];
+    w[i] = mu0 * V[0][ord[i]] * V[0][ord[i]];
+  }
+}
+
+/* Collapsed (conical product) rule of degree 2n-1 on the reference simplex:
+ * x = a(1-b)(1-c), y = b(1-c), z = c with Gauss-Jacobi rules of weights 1,
+ * (1-b), (1-c)^2 on [0,1].  Any rule exact for the polynomial integrands of
+ * affine simplices with element-wise constant data gives the reference's
+ * element matrices (TPZIntTriang / TPZIntTetra3D tables, Integral/pzquad.cpp:
+ * 308-345, Integral/tpzintrulet3d.cpp:447-505) up to rounding. */
+int orc_collapsed_rule(int t, int n, double *pts, double *w) {
+  if (n < 1 || n > 16) return -1;
+  double ta[16], wa[16], tb[16], wb[16], tc[16], wc[16];
+  gauss_jacobi(n, 0., ta, wa);
+  gauss_jacobi(n, 1., tb, wb);
+  gauss_jacobi(n, 2., tc, wc);
+  int q = 0;
+  if (t == ORC_TRI) {
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++, q++) {
+        double a = 0.5 * (1. + ta[i]), b = 0.5 * (1. + tb[j]);
+        pts[3 * q] = a * (1. - b); pts[3 * q + 1] = b; pts[3 * q + 2] = 0.;
+        w[q] = (wa[i] / 2.) * (wb[j] / 4.);
+      }
+    return q;
+  }
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++, q++) {
+        double a = 0.5 * (1. + ta[i]), b = 0.5 * (1. + tb[j]), c = 0.5 * (1. + tc[k]);
+        pts[3 * q] = a * (1. - b) * (1. - c); pts[3 * q + 1] = b * (1. - c); pts[3 * q + 2] = c;
+        w[q] = (wa[i] / 2.) * (wb[j] / 4.) * (wc[k] / 8.);
+      }
+  return q;
+}
+
 int orc_simplex_rule(int t, int order, double *pts, double *w) {
   int n = 0;
+  if ((t == ORC_TRI && order > 4) || (t == ORC_TET && order > 5))
+    return orc_collapsed_rule(t, (order + 2) / 2, pts, w);
   if (t == ORC_TRI) {
     if (order <= 2) {
       const double a = 1. / 6., b = 2. / 3.;
@@ -651,7 +892,6 @@ int orc_simplex_rule(int t, int order, double *pts, double *w) {
       for (int i = 0; i < 3; i++) { pts[3 * i] = P[i][0]; pts[3 * i + 1] = P[i][1]; pts[3 * i + 2] = 0; w[i] = 1. / 6.; }
       return 3;
     }
-    if (order > 4) return -1;
     const double A[2] = {0.4459484909159655, 0.0915762135097718};
     const double W[2] = {0.1116907948390057, 0.05497587182766096};
     for (int o = 0; o < 2; o++) {
@@ -672,7 +912,6 @@ int orc_simplex_rule(int t, int order, double *pts, double *w) {
     }
     return 4;
   }
-  if (order > 5) return -1;
   const double A[2] = {0.09273525031089165, 0.31088591926330106};
   const double W[2] = {0.0122488405193938, 0.018781320953003007};
   for (int o = 0; o < 2; o++)
@@ -768,6 +1007,7 @@ int orc_calcstiff(const orc_problem *pb, const orc_mesh *m, int64_t iel, double 
   int t = m->etype[iel], dim = etype_dim(t), nc = etype_ncorner(t), ns_ = m->nstate;
   int isbc = m->bcface[iel] >= 0;
   /* destination indices (Mesh/pzelmat.cpp:37-70) */
+  if (orc_elem_ndof(m, iel) > ORC_MAXNE) return -1;
   int ne = 0;
   for (int s = 0; s < etype_nsides(t); s++) {
     int64_t seq = m->conn_seq[m->econn[27 * iel + s]];
@@ -792,7 +1032,7 @@ int orc_calcstiff(const orc_problem *pb, const orc_mesh *m, int64_t iel, double 
     if (m->conn_order[c] > maxorder) maxorder = m->conn_order[c];
     if (m->conn_nshape[c] > 0) keep[nkeep++] = s;
   }
-  double gl[64], gw[64], spts[3 * 64], sw[64];
+  double gl[64], gw[64], spts[3 * 512], sw[512];
   int ng = 0, nq = 1;
   if (is_simplex(t)) {
     nq = orc_simplex_rule(t, rule_order(pb, isbc, maxorder), spts, sw);
@@ -803,7 +1043,12 @@ int orc_calcstiff(const orc_problem *pb, const orc_mesh *m, int64_t iel, double 
   }
   double coef = 1.;
   if (!isbc && pb->el_coef) coef = pb->el_coef[iel];
-  double phi[27], dphi[81], dphix[81];
+  double phi[64], dphi[192], dphix[192];
+  /* simplices: corner ids and connect orders drive the side transforms */
+  int64_t ids[4];
+  int sord[27];
+  for (int n = 0; n < nc; n++) ids[n] = m->enodes[8 * iel + n];
+  for (int s = nc; s < etype_nsides(t); s++) sord[s - nc] = m->conn_order[m->econn[27 * iel + s]];
   for (int q = 0; q < nq; q++) {
     double xi[3] = {0, 0, 0}, w;
     if (is_simplex(t)) {
@@ -816,15 +1061,21 @@ int orc_calcstiff(const orc_problem *pb, const orc_mesh *m, int64_t iel, double 
       int ik = q % ng, ie = (q % (ng * ng)) / ng, iz = q / (ng * ng);
       xi[0] = gl[ik]; xi[1] = gl[ie]; xi[2] = gl[iz]; w = gw[ik] * gw[ie] * gw[iz];
     }
-    double phiall[27], dphiall[81];
-    int nall = orc_shape(t, m->p, xi, phiall, dphiall);
-    /* p <= 2: shape i belongs to side i; keep those of sides with a shape */
-    int sel[27], nsh = 0;
-    for (int k = 0; k < nkeep; k++)
-      if (keep[k] < nall) sel[nsh++] = keep[k];
-    for (int i = 0; i < nsh; i++) {
-      phi[i] = phiall[sel[i]];
-      for (int a = 0; a < dim; a++) dphi[a * nsh + i] = dphiall[a * nall + sel[i]];
+    int nsh = 0;
+    if (is_simplex(t)) {
+      nsh = orc_shape_h1(t, ids, sord, xi, phi, dphi);
+      if (nsh < 0) return -1;
+    } else {
+      double phiall[27], dphiall[81];
+      int nall = orc_shape(t, m->p, xi, phiall, dphiall);
+      /* p <= 2: shape i belongs to side i; keep those of sides with a shape */
+      int sel[27];
+      for (int k = 0; k < nkeep; k++)
+        if (keep[k] < nall) sel[nsh++] = keep[k];
+      for (int i = 0; i < nsh; i++) {
+        phi[i] = phiall[sel[i]];
+        for (int a = 0; a < dim; a++) dphi[a * nsh + i] = dphiall[a * nall + sel[i]];
+      }
     }
     /* corner derivatives for GradX (Geom/TPZGeoCube.h:124, pzgeoquad.h:144) */
     double cphi[27], cdphi[81];
@@ -1011,11 +1262,12 @@ void orc_addkel(const int64_t *ia, const int64_t *ja, double *a, int ne, const d
 
 int orc_assemble_range(const orc_problem *pb, const orc_mesh *m, const int64_t *ia,
                        const int64_t *ja, double *a, double *rhs, int64_t e0, int64_t e1) {
-  double *ek = (double *)malloc(sizeof(double) * 81 * 81);
-  double ef[81];
-  int64_t dest[81];
+  double *ek = (double *)malloc(sizeof(double) * ORC_MAXNE * ORC_MAXNE);
+  double ef[ORC_MAXNE];
+  int64_t dest[ORC_MAXNE];
   for (int64_t e = e0; e < e1; e++) {
     int ne = orc_calcstiff(pb, m, e, ek, ef, dest);
+    if (ne < 0) { free(ek); return -1; }
     orc_addkel(ia, ja, a, ne, ek, dest);
     for (int i = 0; i < ne; i++) rhs[dest[i]] += ef[i]; /* AddFel, Matrix/pzfmatrix.cpp:284-298 */
   }

@@ -92,6 +92,22 @@ int orc_calcstiff(const orc_problem *pb, const orc_mesh *m, int64_t iel,
 /* Reference-element shape functions (p <= 2): returns nshape; dphi[d*ns+i]. */
 int orc_shape(int etype, int p, const double *xi, double *phi, double *dphi);
 
+/* Hierarchical H1 shapes of a triangle / tetrahedron with corner-node ids
+ * ids[ncorner] and connect orders orders[nsides - ncorner] (sides after the
+ * corners, each order <= 8): returns nshape (-1 on bad input); dphi[d*ns+i]. */
+int orc_shape_h1(int etype, const int64_t *ids, const int *orders, const double *xi,
+                 double *phi, double *dphi);
+
+/* Simplex rules: orc_simplex_rule picks the rule for an integration order
+ * (symmetric 3/6-point triangle, 4/14-point tetrahedron rules, collapsed
+ * Gauss-Jacobi beyond); orc_collapsed_rule gives the n^dim-point collapsed
+ * rule of degree 2n-1.  pts[3*q], returns the number of points. */
+int orc_simplex_rule(int etype, int order, double *pts, double *w);
+int orc_collapsed_rule(int etype, int n, double *pts, double *w);
+
+/* largest element dof count the oracle handles */
+#define ORC_MAXNE 192
+
 /* One integration point of Contribute (volume materials), ek row-major. */
 void orc_contribute(int material, int dim, int nsh, int ne, double weight, const double *phi,
                     const double *dphix, const double *f, double kappa, double E, double nu,

@@ -61,6 +61,7 @@ def test_runner_rejects_malformed_mesh(tmp_path):
     ("elast3d_p2_body", lambda: P.elasticity3d(n=4, p=2, seed=6, field_kind="grf", load="body")),
     ("poisson2d_hp", lambda: _hp(P.poisson2d(n=10, p=2, seed=2), 3)),
     ("darcy_tet_hp", lambda: P.darcy3d_tet(n=3, pmax=2, seed=4, hp=True)),
+    ("darcy_tet_hp5", lambda: P.darcy3d_tet(n=2, pmax=5, seed=5, hp=True)),
 ])
 def test_cpp_analysis_matches_oracle(tmp_path, name, pb):
     pb = pb()

@@ -53,6 +53,14 @@ CASES = {
     # tetrahedra (Kuhn split) with boundary triangles
     "darcy_tet_p2": lambda: P.darcy3d_tet(n=3, pmax=2, seed=12, hp=False),
     "darcy_tet_hp": lambda: P.darcy3d_tet(n=4, pmax=2, seed=13, hp=True),
+    # high-order tetrahedra: uniform p = 3, Poisson p = 4 with flux data, hp p_e in 1..5
+    "darcy_tet_p3": lambda: P.darcy3d_tet(n=2, pmax=3, seed=14, hp=False),
+    "poisson_tet_p4_flux": lambda: P.Problem(material=P.POISSON, dim=3, p=4, nx=2, ny=1, nz=2,
+                                             simplex=1, scale=1.5, force_kind=P.FORCE_CONST,
+                                             force_const=(1.0, 0, 0), bc_type=[0, 1, -1, 1, 0, -1],
+                                             bc_val=[[0, 0, 0], [0.7, 0, 0], [0, 0, 0], [-0.2, 0, 0],
+                                                     [0, 0, 0], [0, 0, 0]]),
+    "darcy_tet_hp5": lambda: P.darcy3d_tet(n=3, pmax=5, seed=15, hp=True),
     "elast_tet_p2": lambda: P.Problem(material=P.ELAST3D, dim=3, p=2, nx=2, ny=3, nz=2, simplex=1,
                                       E=2.0, force_kind=P.FORCE_CONST, force_const=(0.0, 0.0, -1.0),
                                       bc_type=[0, 1, -1, -1, 0, -1],
@@ -134,7 +142,7 @@ def test_assembly_is_deterministic():
 
 
 @pytest.mark.parametrize("name", ["poisson2d_p2", "darcy2d_p2", "elast3d_p2_traction", "elast3d_hp",
-                                  "darcy_tet_hp"])
+                                  "darcy_tet_hp", "darcy_tet_hp5"])
 def test_pcg64_matches_oracle(name):
     pb = CASES[name]()
     an = TPZLinearAnalysis(pb, False, ctx=ctx())
@@ -219,7 +227,7 @@ def test_matfree_needs_mesh_built_system():
     assert st.info["converged"]
 
 
-@pytest.mark.parametrize("name", ["poisson2d_hp", "elast3d_hp", "darcy_tet_hp"])
+@pytest.mark.parametrize("name", ["poisson2d_hp", "elast3d_hp", "darcy_tet_hp", "darcy_tet_hp5"])
 def test_mixed_refinement_hp(name):
     """hp meshes (several integration rules): the mixed solver uses the stored
     FP32 operator; the matrix-free one is refused."""
@@ -294,7 +302,7 @@ def test_errors_are_reported():
     c.close()
 
 
-@pytest.mark.parametrize("name", ["elast3d_p2_traction", "poisson2d_hp"])
+@pytest.mark.parametrize("name", ["elast3d_p2_traction", "poisson2d_hp", "darcy_tet_hp5"])
 def test_renumbered_connects(name):
     """Any connect numbering (e.g. after TPZAnalysis::OptimizeBandwidth and
     TPZCompMesh::Permute) goes through the boundary unchanged: pattern and

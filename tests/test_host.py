@@ -70,12 +70,12 @@ def test_library_exports_header_symbols():
     assert names == set(L.EXPORTS)
 
 
-@pytest.mark.parametrize("hp", [False, True])
-def test_tet_numbering_matches_oracle(hp):
-    pb = P.Problem(material=P.DARCY, dim=3, p=2, nx=3, ny=2, nz=2, simplex=1,
+@pytest.mark.parametrize("hp,pmax", [(False, 2), (True, 2), (False, 4), (True, 5)])
+def test_tet_numbering_matches_oracle(hp, pmax):
+    pb = P.Problem(material=P.DARCY, dim=3, p=pmax, nx=3, ny=2, nz=2, simplex=1,
                    force_kind=P.FORCE_CONST, force_const=(1.0, 0, 0), bc_type=[0, 1, 0, -1, 0, 0])
     if hp:
-        pb.el_porder = np.random.default_rng(4).integers(1, 3, pb.nvol).astype(np.int32)
+        pb.el_porder = np.random.default_rng(4).integers(1, pmax + 1, pb.nvol).astype(np.int32)
     m = M.build(pb)
     o = O.Oracle(pb)
     econn, pos, size = o.connects()

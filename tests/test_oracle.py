@@ -608,3 +608,268 @@ def test_tet_pattern_brute_force_and_count():
                     rows[i].add(j)
     assert np.array_equal(np.diff(ia), [len(r) for r in rows])
     assert np.array_equal(ja, np.concatenate([np.array(sorted(r), dtype=np.int64) for r in rows]))
+
+
+# --------------------------------------------------------------------------- simplices, p > 2
+def test_collapsed_rules_exact_and_jacobi_points():
+    """Collapsed rules of n^dim points integrate every monomial of degree
+    <= 2n-1 over the unit simplex; their 1-D factors are the Gauss-Jacobi
+    points of weights 1 and (1-b) (scipy's roots_jacobi on [-1, 1])."""
+    from math import factorial
+    from scipy.special import roots_jacobi
+    for n in range(1, 8):
+        for etype, dim in ((O.TRI, 2), (O.TET, 3)):
+            pts, w = O.collapsed_rule(etype, n)
+            assert len(w) == n ** dim and (w > 0).all()
+            deg = 2 * n - 1
+            for a in range(deg + 1):
+                for b in range(deg + 1 - a):
+                    for c in range(deg + 1 - a - b if dim == 3 else 1):
+                        v = np.sum(w * pts[:, 0] ** a * pts[:, 1] ** b * pts[:, 2] ** c)
+                        ex = factorial(a) * factorial(b) * factorial(c) / factorial(a + b + c + dim)
+                        assert abs(v - ex) < 1e-14 * max(ex, 1e-3)
+        pts, w = O.collapsed_rule(O.TRI, n)
+        b = np.unique(np.round(pts[:, 1], 14))
+        tb, _ = roots_jacobi(n, 1.0, 0.0)
+        assert np.allclose(np.sort(b), np.sort((1 + tb) / 2), atol=1e-13)
+        ta, _ = roots_jacobi(n, 0.0, 0.0)
+        a = pts[:, 0] / (1 - pts[:, 1])
+        assert np.allclose(np.sort(np.unique(np.round(a, 13))), np.sort((1 + ta) / 2), atol=1e-12)
+    for etype, top in ((O.TRI, 12), (O.TET, 12)):
+        for order in range(1, top + 1):
+            pts, w = O.simplex_rule(etype, order)
+            dim = 2 if etype == O.TRI else 3
+            for a in range(order + 1):
+                v = np.sum(w * pts[:, 0] ** a)
+                ex = factorial(a) / factorial(a + dim)
+                assert abs(v - ex) < 1e-14
+
+
+def _h1(etype, ids, orders, x):
+    return O.shape_h1(etype, ids, orders, x)
+
+
+def _monomials(x, p):
+    dim = x.shape[1]
+    cols = []
+    for a in range(p + 1):
+        for b in range(p + 1 - a):
+            if dim == 2:
+                cols.append(x[:, 0] ** a * x[:, 1] ** b)
+            else:
+                for c in range(p + 1 - a - b):
+                    cols.append(x[:, 0] ** a * x[:, 1] ** b * x[:, 2] ** c)
+    return np.array(cols).T
+
+
+@pytest.mark.parametrize("etype", [O.TRI, O.TET])
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5, 6])
+def test_h1_simplex_span_count_and_gradients(etype, p):
+    """With every side at order p the hierarchical set has dim P_p functions
+    and spans P_p exactly (a dropped or duplicated function, or a wrong blend
+    or Chebyshev degree, breaks the span); gradients match finite
+    differences for random corner ids."""
+    dim = 2 if etype == O.TRI else 3
+    nc, ns = dim + 1, (7 if dim == 2 else 15)
+    rng = np.random.default_rng(p)
+    ids = rng.permutation(50)[:nc]
+    orders = np.full(ns - nc, p)
+    nP = (p + 1) * (p + 2) // 2 if dim == 2 else (p + 1) * (p + 2) * (p + 3) // 6
+    x = rng.dirichlet(np.ones(nc), 3 * nP)[:, 1:]
+    Phi = np.array([_h1(etype, ids, orders, xi)[0] for xi in x])
+    assert Phi.shape[1] == nP
+    assert np.linalg.matrix_rank(Phi, tol=1e-10) == nP
+    M = _monomials(x, p)
+    coef, *_ = np.linalg.lstsq(Phi, M, rcond=None)
+    assert np.abs(Phi @ coef - M).max() < 1e-10
+    xi = x[3]
+    phi, dphi = _h1(etype, ids, orders, xi)
+    h = 1e-6
+    for d in range(dim):
+        xp = xi.copy(); xp[d] += h
+        xm = xi.copy(); xm[d] -= h
+        fd = (_h1(etype, ids, orders, xp)[0] - _h1(etype, ids, orders, xm)[0]) / (2 * h)
+        assert np.allclose(dphi[d], fd, atol=1e-7 * max(1, np.abs(fd).max()))
+
+
+_TET_SIDES = [(0,), (1,), (2,), (3,), (0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3),
+              (0, 1, 2), (0, 1, 3), (1, 2, 3), (0, 2, 3), (0, 1, 2, 3)]
+_TRI_SIDES = [(0,), (1,), (2,), (0, 1), (1, 2), (2, 0), (0, 1, 2)]
+
+
+def _side_blocks(sides, nc, orders, etype):
+    """offset and count of each side's functions in the element's list"""
+    def nshape(s, o):
+        k = len(sides[s])
+        if k == 1:
+            return 1
+        if k == 2:
+            return max(o - 1, 0)
+        if k == 3:
+            return max((o - 1) * (o - 2) // 2, 0)
+        return max((o - 1) * (o - 2) * (o - 3) // 6, 0)
+    off, out = 0, []
+    for s in range(len(sides)):
+        n = nshape(s, 1 if s < nc else orders[s - nc])
+        out.append((off, n))
+        off += n
+    return out
+
+
+def _ref_coords(X, P):
+    """reference coordinates of physical point(s) P in the simplex with corners X"""
+    J = (X[1:] - X[0]).T
+    return np.linalg.lstsq(J, (P - X[0]).T, rcond=None)[0].T
+
+
+@pytest.mark.parametrize("p", [3, 4, 5])
+def test_h1_conformity_across_faces_random_ids(p):
+    """Two tetrahedra sharing a face, and the boundary triangle on it, in
+    random local orders with random global ids: the functions of every shared
+    connect (corners, edges, face) agree on the face.  This pins the id-based
+    side transforms: a wrong edge flip or face permutation makes the traces
+    of the two elements differ."""
+    rng = np.random.default_rng(10 + p)
+    for trial in range(12):
+        G = rng.random((5, 3))
+        gid = rng.permutation(1000)[:5]
+        A = rng.permutation([0, 1, 2, 3])             # global node 4 is B's apex
+        B = rng.permutation([1, 2, 3, 4])
+        T = rng.permutation([1, 2, 3])
+        oA = np.full(11, p); oB = np.full(11, p); oT = np.full(4, p)
+        blkA = _side_blocks(_TET_SIDES, 4, oA, O.TET)
+        blkB = _side_blocks(_TET_SIDES, 4, oB, O.TET)
+        blkT = _side_blocks(_TRI_SIDES, 3, oT, O.TRI)
+        w = rng.dirichlet(np.ones(3), 6)
+        P = w @ G[[1, 2, 3]]
+        for Pk in P:
+            xa = _ref_coords(G[A], Pk[None])[0]
+            xb = _ref_coords(G[B], Pk[None])[0]
+            xt = _ref_coords(G[T], Pk[None])[0]
+            pa = _h1(O.TET, gid[A], oA, xa)[0]
+            pb = _h1(O.TET, gid[B], oB, xb)[0]
+            pt = _h1(O.TRI, gid[T], oT, xt)[0]
+            for gs in [(1,), (2,), (3,), (1, 2), (2, 3), (1, 3), (1, 2, 3)]:
+                def loc(order, sides):
+                    key = sorted(int(np.where(order == g)[0][0]) for g in gs)
+                    return next(s for s, c in enumerate(sides) if sorted(c) == key)
+                sa, sb, st = loc(A, _TET_SIDES), loc(B, _TET_SIDES), loc(T, _TRI_SIDES)
+                (a0, n), (b0, nb_), (t0, nt) = blkA[sa], blkB[sb], blkT[st]
+                assert n == nb_ == nt and n > 0
+                assert np.allclose(pa[a0:a0 + n], pb[b0:b0 + n], atol=1e-12)
+                assert np.allclose(pa[a0:a0 + n], pt[t0:t0 + n], atol=1e-12)
+
+
+def _tet_hp_problem(seed, pmax=5, n=(2, 2, 2), bc=None, bcv=None, force=0.0, material=None):
+    rng = np.random.default_rng(seed)
+    pb = P.Problem(material=P.DARCY if material is None else material, dim=3, p=pmax,
+                   nx=n[0], ny=n[1], nz=n[2], simplex=1,
+                   force_kind=P.FORCE_CONST if force else P.FORCE_NONE,
+                   force_const=(force, 0, 0),
+                   bc_type=bc if bc is not None else [0] * 6,
+                   bc_val=[[v, 0, 0] for v in (bcv if bcv is not None else [0] * 6)])
+    pb.el_porder = rng.integers(1, pmax + 1, pb.nvol).astype(np.int32)
+    return pb
+
+
+def test_tet_hp_sizes_and_pattern():
+    """p_e ~ U{1..5} on a Kuhn mesh: each connect carries NConnectShapeF of the
+    minimum preferred order of the tetrahedra sharing its side (boundary
+    triangles keep the default order), recomputed here from the element lists;
+    the CSR pattern is the union of element dof pairs."""
+    pb = _tet_hp_problem(3)
+    o = O.Oracle(pb)
+    econn, pos, size = o.connects()
+    mins = {}
+    for e in range(pb.nvol):
+        for s in range(4, 15):
+            c = econn[e, s]
+            mins[c] = min(mins.get(c, pb.p), int(pb.el_porder[e]))
+    kinds = {}
+    for e in range(pb.nvol):
+        for s in range(4, 15):
+            kinds[econn[e, s]] = len(_TET_SIDES[s])
+    for c, pmin in mins.items():
+        k = kinds[c]
+        exp = {2: pmin - 1, 3: (pmin - 1) * (pmin - 2) // 2,
+               4: (pmin - 1) * (pmin - 2) * (pmin - 3) // 6}[k]
+        assert size[c] == max(exp, 0), (c, k, pmin)
+    assert o.neq == int(size.sum())
+    ia, ja = o.pattern()
+    rows = [set() for _ in range(o.neq)]
+    for e in range(o.m.nel):
+        _, _, dest = o.calcstiff(e)
+        for i in dest:
+            for j in dest:
+                if j >= i:
+                    rows[i].add(j)
+    assert np.array_equal(np.diff(ia), [len(r) for r in rows])
+    assert np.array_equal(ja, np.concatenate([np.array(sorted(r), dtype=np.int64) for r in rows]))
+
+
+def test_tet_high_order_element_energy():
+    """For tetrahedra of order 2..5 in an hp mesh: the coefficients c of a
+    quadratic q in the element's own basis (interpolated by least squares)
+    give c^T K_e c = kappa_e * int_e |grad q|^2, the integral taken with an
+    independent collapsed rule; a wrong Jacobian, rule or gradient
+    transformation changes the energy."""
+    pb = _tet_hp_problem(5)
+    pb.el_porder = np.random.default_rng(5).integers(2, 6, pb.nvol).astype(np.int32)
+    pb.el_coef = np.exp(np.random.default_rng(1).standard_normal(pb.nvol))
+    o = O.Oracle(pb)
+    rng = np.random.default_rng(2)
+    q = lambda X: X[:, 0] ** 2 + 0.5 * X[:, 1] * X[:, 2] - 0.3 * X[:, 2]
+    gq = lambda X: np.stack([2 * X[:, 0], 0.5 * X[:, 2], 0.5 * X[:, 1] - 0.3], 1)
+    checked = 0
+    for e in range(pb.nvol):
+        el = o.element(e)
+        ek, _, _ = o.calcstiff(e)
+        assert el["orders"].min() >= 2 and ek.shape[0] >= 10
+        X = el["xyz"]
+        xi = rng.dirichlet(np.ones(4), 200)[:, 1:]
+        Phi = np.array([_h1(O.TET, el["ids"], el["orders"], x)[0] for x in xi])
+        phys = X[0] + xi @ (X[1:] - X[0])
+        c, *_ = np.linalg.lstsq(Phi, q(phys), rcond=None)
+        assert np.abs(Phi @ c - q(phys)).max() < 1e-10
+        pts, w = O.collapsed_rule(O.TET, 4)
+        J = (X[1:] - X[0]).T
+        P_ = X[0] + pts @ (X[1:] - X[0])
+        ex = pb.el_coef[e] * abs(np.linalg.det(J)) * np.sum(w * (gq(P_) ** 2).sum(1))
+        assert abs(c @ ek @ c - ex) < 1e-11 * ex
+        checked += 1
+    assert checked >= 5
+
+
+@pytest.mark.parametrize("pmax", [3, 5])
+def test_tet_hp_patch_tests(pmax):
+    """Random p_e in 1..pmax: u = x (Dirichlet 0 at x = 0, flux 1 at x = 1) is
+    reproduced exactly (corner dofs = x, all others 0); with every p_e >= 2,
+    u = x(1-x)/2 (-lap u = 1, u = 0 at x = 0, 1) has corner dofs u(node),
+    first edge dofs u(mid) - (u_a + u_b)/2 and every other dof 0."""
+    cases = [(lambda X: X, [0, 1, 1, 1, 1, 1], [0, 1, 0, 0, 0, 0], 0.0, 1),
+             (lambda X: X * (1 - X) / 2, [0, 0, 1, 1, 1, 1], [0] * 6, 1.0, 2)]
+    for u_ex, bc, bcv, f, pmin in cases:
+        pb = _tet_hp_problem(7, pmax, bc=bc, bcv=bcv, force=f, material=P.POISSON)
+        rng = np.random.default_rng(7)
+        pb.el_porder = rng.integers(pmin, pmax + 1, pb.nvol).astype(np.int32)
+        o = O.Oracle(pb)
+        u = _solve_oracle(o)
+        econn, pos, size = o.connects()
+        enodes = np.ctypeslib.as_array(o.m.enodes, shape=(8 * o.m.nel,)).reshape(o.m.nel, 8)
+        xyz = np.ctypeslib.as_array(o.m.xyz, shape=(3 * o.m.nnodes,)).reshape(-1, 3)
+        for e in range(pb.nvol):
+            for s in range(15):
+                c = econn[e, s]
+                if not size[c]:
+                    continue
+                blk = u[pos[c]:pos[c] + size[c]]
+                if s < 4:
+                    assert abs(blk[0] - u_ex(xyz[enodes[e, s], 0])) < 1e-9
+                elif s < 10:
+                    a, b = _TET_SIDES[s]
+                    xa, xb = xyz[enodes[e, a], 0], xyz[enodes[e, b], 0]
+                    exp = u_ex(0.5 * (xa + xb)) - 0.5 * (u_ex(xa) + u_ex(xb))
+                    assert abs(blk[0] - exp) < 1e-9
+                    assert np.abs(blk[1:]).max(initial=0) < 1e-9
+                else:
+                    assert np.abs(blk).max() < 1e-9
