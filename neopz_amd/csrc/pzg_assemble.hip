@@ -451,41 +451,56 @@ __device__ inline int hp_variant(bool tet, int s, const int64_t *id) {
   return tri_tid(id[0], id[1], id[2]);
 }
 
-template <int NSV>
-__global__ __launch_bounds__(TPB) void k_vol_hp(ElemArgs g, const HpTables *__restrict__ tab,
+// Shapes of one rule bucket: the sub-list sub[0..NSUB) of the element's NSV
+// virtual sides (the shapes an element of the bucket's highest order can
+// carry).  The Gram matrix is symmetric: thread t owns the 4x4 tile (ti, tj),
+// ti <= tj, of sub-list pairs and scatters every pair once.
+template <int NSUB>
+__global__ __launch_bounds__(TPB) void k_vol_hp(ElemArgs g, int nsv, const HpTables *__restrict__ tab,
+                                                const int32_t *__restrict__ sub,
                                                 const int32_t *__restrict__ list) {
   constexpr int QC = 16;
-  constexpr int NPAIR = NSV * NSV;
-  constexpr int PER = (NPAIR + TPB - 1) / TPB;
+  constexpr int NT = (NSUB + 3) / 4, NSP = 4 * NT;
+  constexpr int NTILE = NT * (NT + 1) / 2;
+  static_assert(NTILE <= TPB, "tiles per element exceed the workgroup");
   __shared__ double s_coord[12];
   __shared__ double s_jinv[3][3];
   __shared__ double s_det;
-  __shared__ double s_G[QC][3][NSV];
-  __shared__ double s_Gw[QC][3][NSV];
-  __shared__ double s_pw[QC][NSV];
+  __shared__ double s_G[QC][3][NSP];
+  __shared__ double s_Gw[QC][3][NSP];
+  __shared__ double s_pw[QC][NSP];
   __shared__ double s_f[QC];
-  __shared__ int s_col[NSV];
-  __shared__ int64_t s_blk[NSV], s_row[NSV];
-  __shared__ int32_t s_map[NSV * NSV];
+  __shared__ int s_col[NSUB];
+  __shared__ int64_t s_blk[NSUB], s_row[NSUB];
+  __shared__ int32_t s_map[NSUB * NSUB];
   const int tid = threadIdx.x;
   const int64_t e = list[blockIdx.x];
   const HpTables &T = *tab;
   if (tid < 12) s_coord[tid] = g.xyz[3 * g.nodes[e * 4 + tid / 3] + tid % 3];
-  for (int k = tid; k < NPAIR; k += TPB) s_map[k] = g.map[e * NPAIR + k];
-  if (tid < NSV) {
+  for (int k = tid; k < NSUB * NSUB; k += TPB) {
+    const int i = k / NSUB, j = k % NSUB;
+    s_map[k] = g.map[((int64_t)e * nsv + sub[i]) * nsv + sub[j]];
+  }
+  if (tid < NSUB) {
     int64_t id[4];
     for (int k = 0; k < 4; k++) id[k] = g.nodes[e * 4 + k];
-    s_col[tid] = T.col_off[tid] + hp_variant(true, T.shape_side[tid], id);
-    const int64_t b = g.aidx[g.conn[e * NSV + tid]];
+    const int gi = sub[tid];
+    s_col[tid] = T.col_off[gi] + hp_variant(true, T.shape_side[gi], id);
+    const int64_t b = g.aidx[g.conn[e * nsv + gi]];
     s_blk[tid] = b;
     s_row[tid] = b >= 0 ? g.ia[b] : -1;
   }
-  __syncthreads();
+  // padding columns of the tiles stay zero
+  for (int k = tid; k < QC * 3 * (NSP - NSUB); k += TPB) {
+    const int qa = k / (NSP - NSUB), i = NSUB + k % (NSP - NSUB);
+    s_G[qa / 3][qa % 3][i] = 0.;
+    s_Gw[qa / 3][qa % 3][i] = 0.;
+  }
   if (tid == 0) {
     // GradX of the straight tetrahedron: column k = x_{k+1} - x_0
     double j[3][3];
     for (int i = 0; i < 3; i++)
-      for (int k = 0; k < 3; k++) j[i][k] = s_coord[3 * (k + 1) + i] - s_coord[i];
+      for (int k = 0; k < 3; k++) j[i][k] = g.xyz[3 * g.nodes[e * 4 + k + 1] + i] - g.xyz[3 * g.nodes[e * 4] + i];
     const double det = -j[0][2] * j[1][1] * j[2][0] + j[0][1] * j[1][2] * j[2][0] +
                        j[0][2] * j[1][0] * j[2][1] - j[0][0] * j[1][2] * j[2][1] -
                        j[0][1] * j[1][0] * j[2][2] + j[0][0] * j[1][1] * j[2][2];
@@ -500,10 +515,19 @@ __global__ __launch_bounds__(TPB) void k_vol_hp(ElemArgs g, const HpTables *__re
     s_jinv[2][2] = (-j[0][1] * j[1][0] + j[0][0] * j[1][1]) / det;
     s_det = fabs(det);
   }
+  // tile of this thread
+  int ti = 0, tj = 0;
+  if (tid < NTILE) {
+    int t = tid;
+    while (t >= NT - ti) { t -= NT - ti; ti++; }
+    tj = ti + t;
+  }
   __syncthreads();
-  double acc[PER];
+  double acc[4][4];
 #pragma unroll
-  for (int k = 0; k < PER; k++) acc[k] = 0.;
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc[r][q] = 0.;
   double accf = 0.;
   const int nq = T.nq;
   for (int q0 = 0; q0 < nq; q0 += QC) {
@@ -517,8 +541,8 @@ __global__ __launch_bounds__(TPB) void k_vol_hp(ElemArgs g, const HpTables *__re
       forcing(g, x, f);
       s_f[tid] = f[0];
     }
-    for (int k = tid; k < nqc * NSV; k += TPB) {
-      const int qq = k / NSV, i = k % NSV, q = q0 + qq, col = s_col[i];
+    for (int k = tid; k < nqc * NSUB; k += TPB) {
+      const int qq = k / NSUB, i = k % NSUB, q = q0 + qq, col = s_col[i];
       const double wd = T.w[q] * s_det;
       for (int a = 0; a < 3; a++) {
         double sum = 0.;
@@ -529,72 +553,86 @@ __global__ __launch_bounds__(TPB) void k_vol_hp(ElemArgs g, const HpTables *__re
       s_pw[qq][i] = wd * T.phi[q][col];
     }
     __syncthreads();
+    if (tid < NTILE) {
+      for (int qq = 0; qq < nqc; qq++)
 #pragma unroll
-    for (int k = 0; k < PER; k++) {
-      const int idx = tid + k * TPB;
-      if (idx < NPAIR) {
-        const int i = idx / NSV, j = idx % NSV;
-        double sum = 0.;
-        for (int qq = 0; qq < nqc; qq++)
-          for (int a = 0; a < 3; a++) sum += s_Gw[qq][a][i] * s_G[qq][a][j];
-        acc[k] += sum;
-      }
+        for (int a = 0; a < 3; a++) {
+          double u[4], v[4];
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            u[r] = s_Gw[qq][a][4 * ti + r];
+            v[r] = s_G[qq][a][4 * tj + r];
+          }
+#pragma unroll
+          for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) acc[r][q] += u[r] * v[q];
+        }
     }
-    if (tid < NSV)
+    if (tid < NSUB)
       for (int qq = 0; qq < nqc; qq++) accf += s_pw[qq][tid] * s_f[qq];
     __syncthreads();
   }
   const double coef = g.coef ? g.coef[e] : g.scale;
+  if (tid < NTILE) {
 #pragma unroll
-  for (int k = 0; k < PER; k++) {
-    const int idx = tid + k * TPB;
-    if (idx >= NPAIR) continue;
-    const int i = idx / NSV, j = idx % NSV;
-    const int64_t bi = s_blk[i], bj = s_blk[j];
-    if (bi < 0 || bj < 0 || bi > bj) continue;
-    const double v = coef * acc[k];
-    if (fabs(v) < 1.e-12) continue;  // IsZero drop rule of AddKel
-    g.a[bi == bj ? s_row[i] : s_row[i] + 1 + s_map[i * NSV + j]] += v;
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int i = 4 * ti + r, j = 4 * tj + q;
+        if (i >= NSUB || j >= NSUB || (ti == tj && q < r)) continue;
+        const int64_t bi = s_blk[i], bj = s_blk[j];
+        if (bi < 0 || bj < 0) continue;
+        const double v = coef * acc[r][q];
+        if (fabs(v) < 1.e-12) continue;  // IsZero drop rule of AddKel
+        int64_t slot;
+        if (bi == bj) slot = s_row[i];
+        else if (bi < bj) slot = s_row[i] + 1 + s_map[i * NSUB + j];
+        else slot = s_row[j] + 1 + s_map[j * NSUB + i];
+        g.a[slot] += v;
+      }
   }
-  if (tid < NSV && s_blk[tid] >= 0)
+  if (tid < NSUB && s_blk[tid] >= 0)
     g.rhs[s_blk[tid]] += g.material == PZG_MAT_POISSON ? accf * g.scale : accf;
 }
 
-template <int NSB>
-__global__ __launch_bounds__(TPB) void k_bc_hp(ElemArgs g, const HpTables *__restrict__ tab,
+template <int NSUB>
+__global__ __launch_bounds__(TPB) void k_bc_hp(ElemArgs g, int nsv, const HpTables *__restrict__ tab,
+                                               const int32_t *__restrict__ sub,
                                                const int32_t *__restrict__ list) {
   constexpr int QC = 32;
-  constexpr int NPAIR = NSB * NSB;
+  constexpr int NPAIR = NSUB * NSUB;
   constexpr int PER = (NPAIR + TPB - 1) / TPB;
-  __shared__ double s_coord[9];
   __shared__ double s_det;
-  __shared__ double s_p[QC][NSB];
-  __shared__ double s_pw[QC][NSB];
-  __shared__ int s_col[NSB];
-  __shared__ int64_t s_blk[NSB], s_row[NSB];
-  __shared__ int32_t s_map[NSB * NSB];
+  __shared__ double s_p[QC][NSUB];
+  __shared__ double s_pw[QC][NSUB];
+  __shared__ int s_col[NSUB];
+  __shared__ int64_t s_blk[NSUB], s_row[NSUB];
+  __shared__ int32_t s_map[NPAIR];
   const int tid = threadIdx.x;
   const int64_t e = list[blockIdx.x];
   const HpTables &T = *tab;
   const int id = g.bcid[e];
   const int type = g.bctype[id];
   const double val = g.bcval[3 * id];
-  if (tid < 9) s_coord[tid] = g.xyz[3 * g.nodes[e * 3 + tid / 3] + tid % 3];
-  for (int k = tid; k < NPAIR; k += TPB) s_map[k] = g.map[e * NPAIR + k];
-  if (tid < NSB) {
+  for (int k = tid; k < NPAIR; k += TPB) {
+    const int i = k / NSUB, j = k % NSUB;
+    s_map[k] = g.map[((int64_t)e * nsv + sub[i]) * nsv + sub[j]];
+  }
+  if (tid < NSUB) {
     int64_t ids[3];
     for (int k = 0; k < 3; k++) ids[k] = g.nodes[e * 3 + k];
-    s_col[tid] = T.col_off[tid] + hp_variant(false, T.shape_side[tid], ids);
-    const int64_t b = g.aidx[g.conn[e * NSB + tid]];
+    const int gi = sub[tid];
+    s_col[tid] = T.col_off[gi] + hp_variant(false, T.shape_side[gi], ids);
+    const int64_t b = g.aidx[g.conn[e * nsv + gi]];
     s_blk[tid] = b;
     s_row[tid] = b >= 0 ? g.ia[b] : -1;
   }
-  __syncthreads();
   if (tid == 0) {
     // 2D element in 3D space (TPZGeoEl::Jacobian): |det| = |g0| * |g1 - (g0.g1/|g0|^2) g0|
     double gx[3][2];
     for (int i = 0; i < 3; i++)
-      for (int k = 0; k < 2; k++) gx[i][k] = s_coord[3 * (k + 1) + i] - s_coord[i];
+      for (int k = 0; k < 2; k++) gx[i][k] = g.xyz[3 * g.nodes[e * 3 + k + 1] + i] - g.xyz[3 * g.nodes[e * 3] + i];
     double n1 = 0, d12 = 0, n2 = 0;
     for (int i = 0; i < 3; i++) { n1 += gx[i][0] * gx[i][0]; d12 += gx[i][0] * gx[i][1]; }
     n1 = sqrt(n1);
@@ -612,8 +650,8 @@ __global__ __launch_bounds__(TPB) void k_bc_hp(ElemArgs g, const HpTables *__res
   const int nq = T.nq;
   for (int q0 = 0; q0 < nq; q0 += QC) {
     const int nqc = nq - q0 < QC ? nq - q0 : QC;
-    for (int k = tid; k < nqc * NSB; k += TPB) {
-      const int qq = k / NSB, i = k % NSB, q = q0 + qq;
+    for (int k = tid; k < nqc * NSUB; k += TPB) {
+      const int qq = k / NSUB, i = k % NSUB, q = q0 + qq;
       const double ph = T.phi[q][s_col[i]];
       s_p[qq][i] = ph;
       s_pw[qq][i] = ph * T.w[q] * s_det;
@@ -623,13 +661,13 @@ __global__ __launch_bounds__(TPB) void k_bc_hp(ElemArgs g, const HpTables *__res
     for (int k = 0; k < PER; k++) {
       const int idx = tid + k * TPB;
       if (idx < NPAIR) {
-        const int i = idx / NSB, j = idx % NSB;
+        const int i = idx / NSUB, j = idx % NSUB;
         double sum = 0.;
         for (int qq = 0; qq < nqc; qq++) sum += s_pw[qq][i] * s_p[qq][j];
         acc[k] += sum;
       }
     }
-    if (tid < NSB)
+    if (tid < NSUB)
       for (int qq = 0; qq < nqc; qq++) accf += s_pw[qq][tid];
     __syncthreads();
   }
@@ -638,15 +676,15 @@ __global__ __launch_bounds__(TPB) void k_bc_hp(ElemArgs g, const HpTables *__res
     for (int k = 0; k < PER; k++) {
       const int idx = tid + k * TPB;
       if (idx >= NPAIR) continue;
-      const int i = idx / NSB, j = idx % NSB;
+      const int i = idx / NSUB, j = idx % NSUB;
       const int64_t bi = s_blk[i], bj = s_blk[j];
       if (bi < 0 || bj < 0 || bi > bj) continue;
       const double v = g.big * acc[k];
       if (fabs(v) < 1.e-12) continue;
-      g.a[bi == bj ? s_row[i] : s_row[i] + 1 + s_map[i * NSB + j]] += v;
+      g.a[bi == bj ? s_row[i] : s_row[i] + 1 + s_map[i * NSUB + j]] += v;
     }
   }
-  if (tid < NSB && s_blk[tid] >= 0) {
+  if (tid < NSUB && s_blk[tid] >= 0) {
     double f;
     if (type == PZG_BC_DIRICHLET) f = g.big * val * accf;
     else f = val * accf * (g.material == PZG_MAT_POISSON ? g.scale : 1.0);
@@ -714,14 +752,36 @@ static int run_bc(Ctx &c) {
   return PZG_OK;
 }
 
-template <int NSV, int NSB>
+// shape counts of the order-P sets: tetrahedron 4/10/20/35/56, triangle 3/6/10/15/21
+template <bool VOL>
+static int launch_hp_bucket(Ctx &c, const ElemArgs &g, int P, const HpTables *tab, const int32_t *sub,
+                            const int32_t *list, int64_t n) {
+  const int nsv = VOL ? c.vol_nsides : c.bc_nsides;
+#define PZG_HP_CASE(PP, NV, NB)                                                                  \
+  case PP:                                                                                       \
+    if (VOL) hipLaunchKernelGGL((k_vol_hp<NV>), dim3(n), dim3(TPB), 0, c.stream, g, nsv, tab, sub, list); \
+    else hipLaunchKernelGGL((k_bc_hp<NB>), dim3(n), dim3(TPB), 0, c.stream, g, nsv, tab, sub, list); \
+    break;
+  switch (P) {
+    PZG_HP_CASE(1, 4, 3)
+    PZG_HP_CASE(2, 10, 6)
+    PZG_HP_CASE(3, 20, 10)
+    PZG_HP_CASE(4, 35, 15)
+    PZG_HP_CASE(5, 56, 21)
+    default: set_error("pzg_assemble: unsupported order"); return PZG_EUNSUP;
+  }
+#undef PZG_HP_CASE
+  PZG_HIP(hipGetLastError());
+  return PZG_OK;
+}
+
 static int run_hp(Ctx &c) {
   if (c.bs != 1) {
     set_error("pzg_assemble: high-order tetrahedra support scalar materials only");
     return PZG_EUNSUP;
   }
-  if (c.vol_nsides != NSV || (c.nbc > 0 && c.bc_nsides != NSB) || c.thp_vol.size() != c.vol_rules.size() ||
-      (c.nbc > 0 && c.thp_bc.size() != c.bc_rules.size())) {
+  if (c.thp_vol.size() != c.vol_rules.size() || c.hp_vol_P.size() != c.vol_rules.size() ||
+      (c.nbc > 0 && (c.thp_bc.size() != c.bc_rules.size() || c.hp_bc_P.size() != c.bc_rules.size()))) {
     set_error("pzg_assemble: high-order tables do not match the mesh");
     return PZG_EINVAL;
   }
@@ -730,10 +790,10 @@ static int run_hp(Ctx &c) {
   for (size_t k = 0; k + 1 < c.vcolor_ptr.size(); k++) {
     const int64_t n = c.vcolor_ptr[k + 1] - c.vcolor_ptr[k];
     if (!n) continue;
-    hipLaunchKernelGGL((k_vol_hp<NSV>), dim3(n), dim3(TPB), 0, c.stream, g,
-                       c.d_thp_vol.as<HpTables>() + k % nbk,
-                       c.d_vcolor_list.as<int32_t>() + c.vcolor_ptr[k]);
-    PZG_HIP(hipGetLastError());
+    const size_t b = k % nbk;
+    PZG_TRY(launch_hp_bucket<true>(c, g, c.hp_vol_P[b], c.d_thp_vol.as<HpTables>() + b,
+                                   c.d_hp_vsub.as<int32_t>() + b * HP_MAXSH,
+                                   c.d_vcolor_list.as<int32_t>() + c.vcolor_ptr[k], n));
   }
   if (c.nbc == 0) return PZG_OK;
   g = make_args(c, false);
@@ -741,10 +801,10 @@ static int run_hp(Ctx &c) {
   for (size_t k = 0; k + 1 < c.bcolor_ptr.size(); k++) {
     const int64_t n = c.bcolor_ptr[k + 1] - c.bcolor_ptr[k];
     if (!n) continue;
-    hipLaunchKernelGGL((k_bc_hp<NSB>), dim3(n), dim3(TPB), 0, c.stream, g,
-                       c.d_thp_bc.as<HpTables>() + k % nbk,
-                       c.d_bcolor_list.as<int32_t>() + c.bcolor_ptr[k]);
-    PZG_HIP(hipGetLastError());
+    const size_t b = k % nbk;
+    PZG_TRY(launch_hp_bucket<false>(c, g, c.hp_bc_P[b], c.d_thp_bc.as<HpTables>() + b,
+                                    c.d_hp_bsub.as<int32_t>() + b * HP_MAXSH,
+                                    c.d_bcolor_list.as<int32_t>() + c.bcolor_ptr[k], n));
   }
   return PZG_OK;
 }
@@ -752,17 +812,7 @@ static int run_hp(Ctx &c) {
 int launch_assemble(Ctx &c) {
   fill<double>(c, c.d_a.as<double>(), c.nnz_u, 0.);
   fill<double>(c, c.d_rhs.as<double>(), c.neq, 0.);
-  if (c.hp_simplex) {
-    // shapes of the order-P set: tetrahedron 4/10/20/35/56, triangle 3/6/10/15/21
-    switch (c.hp_order) {
-      case 1: return run_hp<4, 3>(c);
-      case 2: return run_hp<10, 6>(c);
-      case 3: return run_hp<20, 10>(c);
-      case 4: return run_hp<35, 15>(c);
-      case 5: return run_hp<56, 21>(c);
-      default: set_error("pzg_assemble: unsupported order"); return PZG_EUNSUP;
-    }
-  }
+  if (c.hp_simplex) return run_hp(c);
   for (const ElemTables &t : c.tvol_all)
     if (t.nq > MAXQ) return PZG_EUNSUP;
   for (const ElemTables &t : c.tbc_all)

@@ -349,6 +349,35 @@ static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
                  c.tbc_all));
   c.tvol = c.tvol_all[0];
   c.tbc = c.tbc_all[0];
+  // high-order buckets: the highest order any element of the bucket uses
+  // (from the shapes its connects carry) and the matching sub-list
+  std::vector<int32_t> vsub, bsub;
+  if (c.hp_simplex) {
+    auto subs = [&](int etype, int nsx, const std::vector<int64_t> &conn, int64_t nel,
+                    const std::vector<int32_t> &bk, size_t nbk, std::vector<int> &Pb,
+                    std::vector<int32_t> &flat) {
+      int side[HP_MAXSH], loc[HP_MAXSH];
+      hp_shape_list(etype, c.hp_order, side, loc);
+      int need[HP_MAXSH];
+      for (int i = 0; i < nsx; i++) {
+        int o = 1;
+        while (o < c.hp_order && simplex_nshape(etype, side[i], o) <= loc[i]) o++;
+        need[i] = o;
+      }
+      Pb.assign(nbk, 1);
+      for (int64_t e = 0; e < nel; e++)
+        for (int i = 0; i < nsx; i++)
+          if (c.blk_size[conn[e * nsx + i]] > 0) Pb[bk[e]] = std::max(Pb[bk[e]], need[i]);
+      flat.assign(nbk * HP_MAXSH, 0);
+      for (size_t b = 0; b < nbk; b++) {
+        int n = 0;
+        for (int i = 0; i < nsx; i++)
+          if (loc[i] < simplex_nshape(etype, side[i], Pb[b])) flat[b * HP_MAXSH + n++] = i;
+      }
+    };
+    subs(PZG_TET, c.vol_nsides, c.vol_conn, c.nvol, c.vol_bucket, c.vol_rules.size(), c.hp_vol_P, vsub);
+    subs(PZG_TRI, c.bc_nsides, c.bc_conn, c.nbc, c.bc_bucket, c.bc_rules.size(), c.hp_bc_P, bsub);
+  }
 
   hipStream_t s = c.stream;
   PZG_TRY(dev_upload(c.d_xyz, c.xyz.data(), c.xyz.size(), s));
@@ -372,6 +401,8 @@ static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
   PZG_TRY(dev_upload(c.d_tbc, reinterpret_cast<const char *>(c.tbc_all.data()),
                      sizeof(ElemTables) * c.tbc_all.size(), s));
   if (c.hp_simplex) {
+    PZG_TRY(dev_upload(c.d_hp_vsub, vsub.data(), vsub.size(), s));
+    PZG_TRY(dev_upload(c.d_hp_bsub, bsub.data(), bsub.size(), s));
     PZG_TRY(dev_upload(c.d_thp_vol, reinterpret_cast<const char *>(c.thp_vol.data()),
                        sizeof(HpTables) * c.thp_vol.size(), s));
     if (!c.thp_bc.empty())

@@ -129,6 +129,10 @@ struct Ctx {
   int64_t nconn_in = 0;              // connects as given by the caller
   std::vector<int> vol_side_real;    // virtual side -> real side (volume elements)
   std::vector<HpTables> thp_vol, thp_bc;
+  // per rule bucket: highest order of its elements and the sub-list of the
+  // virtual sides an element of that order can use (HP_MAXSH slots a bucket)
+  std::vector<int> hp_vol_P, hp_bc_P;
+  DevBuf d_hp_vsub, d_hp_bsub;
 
   // connect -> active block index (-1 for size-0 connects)
   std::vector<int64_t> aidx;

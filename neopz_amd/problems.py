@@ -164,7 +164,7 @@ CONFIGS = {
     # name: (factory, kwargs) -- the workloads quoted in the benchmark table
     "C1": (poisson2d, dict(n=64, p=2)),
     "C2": (elasticity3d, dict(n=48, p=2, field_kind="iid", load="traction")),
-    "C3": (darcy3d_tet, dict(n=32, pmax=2)),
+    "C3": (darcy3d_tet, dict(n=32, pmax=5)),
     "C4p1": (darcy2d, dict(n=256, p=1)),
     "C4p2": (darcy2d, dict(n=256, p=2)),
     "C5": (elasticity3d, dict(n=64, p=2, field_kind="grf", corr_len=0.1, load="body")),
