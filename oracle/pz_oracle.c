@@ -7,6 +7,12 @@
  * Every function cites the reference file:line whose algorithm it restates.
  * Pinning of each function (golden vectors / closed forms / invariants) is in
  * tests/test_oracle.py; see the header comment there.
+ *
+ * Partially pinned: orc_shape_h1 at orders >= 3.  Span, conformity under
+ * random ids, gradients and element energies are pinned; the choice of basis
+ * within the space (Chebyshev degrees, which face corner maps to u and v, the
+ * 1/3 weight of the face opposite corner 0) is restated from the reference's
+ * source and parity of these choices is unpinned by a reference run.
  */
 #include "pz_oracle.h"
 
