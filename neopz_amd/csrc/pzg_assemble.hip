@@ -864,7 +864,8 @@ int launch_diag(Ctx &c) {
 // --------------------------------------------------------------------------
 // Full (both triangles) FP64 block CSR of K, from the reference's upper CSR:
 // diagonal blocks (k_bsr64_diag) and coupled blocks with their transposes
-// (k_bsr64_off).  Block values are block-major, row-major inside a block.
+// (k_bsr64_off).  Values at bsr_pos (chunks of BSR_CHUNK blocks per row,
+// component-major inside a chunk).
 __global__ void k_nlow(int64_t nb, const int64_t *__restrict__ fbia, const int64_t *__restrict__ ubia,
                        int32_t *__restrict__ nlow) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -878,12 +879,13 @@ __global__ void k_bsr64_diag(int64_t nb, const int64_t *__restrict__ ia, const d
                              double *__restrict__ v) {
   int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (A >= nb) return;
-  double *blk = v + (fbia[A] + nlow[A]) * (BS * BS);
+  constexpr int BB = BS * BS;
+  const int64_t b0 = fbia[A], n = fbia[A + 1] - b0, i = nlow[A];
   for (int k = 0; k < BS; k++)
     for (int l = k; l < BS; l++) {
       const double x = a[ia[A * BS + k] + (l - k)];
-      blk[k * BS + l] = x;
-      blk[l * BS + k] = x;
+      v[bsr_pos<BB>(b0, n, i, k * BS + l)] = x;
+      v[bsr_pos<BB>(b0, n, i, l * BS + k)] = x;
     }
 }
 
@@ -902,8 +904,8 @@ __global__ void k_bsr64_off(int64_t nb, const int64_t *__restrict__ ia, const do
     int k = (int)((e % BB) / BS), l = (int)(e % BS);
     int64_t B = ubja[u0 + t];
     double x = a[ia[A * BS + k] + BS + t * BS - k + l];
-    v[(fbia[A] + rfwd[u0 + t]) * BB + k * BS + l] = x;
-    v[(fbia[B] + rbwd[u0 + t]) * BB + l * BS + k] = x;
+    v[bsr_pos<BB>(fbia[A], fbia[A + 1] - fbia[A], rfwd[u0 + t], k * BS + l)] = x;
+    v[bsr_pos<BB>(fbia[B], fbia[B + 1] - fbia[B], rbwd[u0 + t], l * BS + k)] = x;
   }
 }
 
@@ -951,7 +953,7 @@ __global__ void k_dinvsqrt(int64_t n, const double *__restrict__ d, double *__re
   if (i < n) s[i] = 1.0 / sqrt(fabs(d[i]));
 }
 
-// Block values are block-major, row-major inside a block.
+// Values at bsr_pos, as the FP64 block CSR.
 template <int BS>
 __global__ void k_bsr_diag(int64_t nb, int64_t nbf, const int64_t *__restrict__ ia,
                            const double *__restrict__ a, const int32_t *__restrict__ nlow,
@@ -966,9 +968,9 @@ __global__ void k_bsr_diag(int64_t nb, int64_t nbf, const int64_t *__restrict__ 
       m[k][l] = v;
       m[l][k] = v;
     }
-  const int64_t t = fbia[A] + nlow[A];
+  const int64_t b0 = fbia[A], n = fbia[A + 1] - b0;
   for (int k = 0; k < BS; k++)
-    for (int l = 0; l < BS; l++) bsr[t * (BS * BS) + k * BS + l] = (float)m[k][l];
+    for (int l = 0; l < BS; l++) bsr[bsr_pos<BS * BS>(b0, n, nlow[A], k * BS + l)] = (float)m[k][l];
   float *inv = binv + A * BS * BS;
   if (BS == 1) {
     inv[0] = (float)(1.0 / m[0][0]);
@@ -1011,8 +1013,8 @@ __global__ void k_bsr_off(int64_t nb, int64_t nbf, const int64_t *__restrict__ i
     int k = (int)((e % BB) / BS), l = (int)(e % BS);
     int64_t B = ubja[u0 + t];
     double v = a[ia[A * BS + k] + BS + t * BS - k + l] * dis[A * BS + k] * dis[B * BS + l];
-    bsr[(fbia[A] + rfwd[u0 + t]) * BB + k * BS + l] = (float)v;
-    bsr[(fbia[B] + rbwd[u0 + t]) * BB + l * BS + k] = (float)v;
+    bsr[bsr_pos<BB>(fbia[A], fbia[A + 1] - fbia[A], rfwd[u0 + t], k * BS + l)] = (float)v;
+    bsr[bsr_pos<BB>(fbia[B], fbia[B + 1] - fbia[B], rbwd[u0 + t], l * BS + k)] = (float)v;
   }
 }
 

@@ -87,6 +87,19 @@ struct HpTables {
   double dphi[HP_MAXQ][3][HP_MAXCOL];
 };
 
+// Full block CSR values (FP64 K, FP32 S): the blocks of a block row come in
+// chunks of BSR_CHUNK, component-major inside a chunk, so that the BSR_CHUNK
+// lanes that share a block row read consecutive addresses with every load.
+// Position of value k (row-major inside the block) of the i-th block of a
+// row whose blocks are [b0, b0 + n).
+constexpr int BSR_CHUNK = 16;
+template <int BB>
+__host__ __device__ inline int64_t bsr_pos(int64_t b0, int64_t n, int64_t i, int k) {
+  const int64_t j = i - i % BSR_CHUNK;
+  const int64_t m = n - j < BSR_CHUNK ? n - j : BSR_CHUNK;
+  return (b0 + j) * BB + (int64_t)k * m + (i - j);
+}
+
 // Device-side scalar state of an iterative solve.
 struct SolveScalars {
   double rho, rho_old, pq, alpha, beta, rr, normb, resid, tol, pr, zq;

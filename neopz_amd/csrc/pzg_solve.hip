@@ -47,8 +47,10 @@ static int grid_for(int64_t n, int per_block) {
 // Block CSR SpMV, shared by the FP64 path (unscaled K) and the FP32 path
 // (scaled S).  Every block row is handled by BSR_L lanes; each lane takes
 // BSR_U blocks per pass and issues all their loads before the products, so a
-// wavefront keeps 64/BSR_L rows x BSR_U blocks of loads in flight.  Values
-// are block-major (block t at [t*BS*BS, (t+1)*BS*BS), row-major inside).
+// wavefront keeps 64/BSR_L rows x BSR_U blocks of loads in flight.  Values at
+// bsr_pos: lane l of a row reads block l of each chunk of BSR_L blocks, the
+// chunk component-major, so every value load of the row's lanes is one
+// contiguous segment.
 #ifndef PZG_BSR_L
 #define PZG_BSR_L 16
 #endif
@@ -57,6 +59,7 @@ static int grid_for(int64_t n, int per_block) {
 #endif
 constexpr int BSR_L = PZG_BSR_L;
 constexpr int BSR_U = PZG_BSR_U;
+static_assert(BSR_L == BSR_CHUNK, "lanes per block row must match the value chunks");
 
 template <class T, int BS>
 __device__ inline void bsr_row(int64_t A, const int64_t *__restrict__ bia,
@@ -65,17 +68,19 @@ __device__ inline void bsr_row(int64_t A, const int64_t *__restrict__ bia,
   constexpr int BB = BS * BS;
   T acc[BS];
   for (int k = 0; k < BS; k++) acc[k] = T(0);
-  const int64_t b0 = bia[A], b1 = bia[A + 1];
-  for (int64_t base = b0 + sub; base < b1; base += BSR_L * BSR_U) {
+  const int64_t b0 = bia[A], n = bia[A + 1] - b0;
+  for (int64_t base = sub; base < n; base += BSR_L * BSR_U) {
     T m[BSR_U][BB];
     int64_t col[BSR_U];
 #pragma unroll
     for (int u = 0; u < BSR_U; u++) {
-      const int64_t t = base + BSR_L * u;
-      if (t < b1) {
-        col[u] = bja[t];
+      const int64_t i = base + BSR_L * u;  // block of the row; i % BSR_L == sub
+      if (i < n) {
+        col[u] = bja[b0 + i];
+        const int64_t j = i - sub, mc = n - j < BSR_L ? n - j : BSR_L;
+        const T *vp = v + (b0 + j) * BB + sub;
 #pragma unroll
-        for (int c = 0; c < BB; c++) m[u][c] = v[t * BB + c];
+        for (int c = 0; c < BB; c++) m[u][c] = vp[c * mc];
       } else {
         col[u] = 0;
 #pragma unroll
