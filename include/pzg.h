@@ -191,6 +191,15 @@ int pzg_assemble(pzg_ctx *ctx);
  * NULL).  TPZSYsmpMatrix::A (Matrix/pzsysmp.h:104-127) and the TPZFMatrix rhs. */
 int pzg_get_system(pzg_ctx *ctx, double *a, double *rhs);
 
+/* The assembled system in the non-symmetric sparse format of
+ * TPZSpStructMatrix / TPZFYsmpMatrix: every equation row holds the equations
+ * of its own and of every adjacent connect, sorted (both triangles;
+ * StrMatrix/TPZSpStructMatrix.cpp:53-190), values as TPZFYsmpMatrix::AddKel
+ * accumulates them (Matrix/pzysmp.cpp:135-176, |v| < 1e-12 dropped).
+ * ia[neq+1], ja[nnz_full], a[nnz_full] (nnz_full from pzg_get_sizes); any of
+ * them may be NULL.  Needs an assembled system. */
+int pzg_full_csr(pzg_ctx *ctx, int64_t *ia, int64_t *ja, double *a);
+
 /* Installs a caller-built matrix (and optional rhs): TPZSYsmpMatrix::SetData
  * (Matrix/pzsysmp.h:232-240). */
 int pzg_csr_set(pzg_ctx *ctx, int64_t neq, const int64_t *ia, const int64_t *ja,

@@ -22,7 +22,7 @@ EXPORTS = [
     "pzg_add_kel_batch", "pzg_multadd", "pzg_solve", "pzg_true_residual", "pzg_time_kernel",
     "pzg_get_sizes", "pzg_sync", "pzg_mesh_set_coef", "pzg_learned_setup", "pzg_learned_collect",
     "pzg_learned_train", "pzg_learned_weights", "pzg_learned_graph", "pzg_learned_samples",
-    "pzg_learned_loss_grad", "pzg_scaled_apply",
+    "pzg_learned_loss_grad", "pzg_scaled_apply", "pzg_full_csr",
 ]
 
 dp = C.POINTER(C.c_double)

@@ -132,6 +132,39 @@ class TPZSYsmpMatrixHIP:
         return st.Solve(F)
 
 
+class TPZFYsmpMatrixHIP(TPZSYsmpMatrixHIP):
+    """Non-symmetric sparse rows (both triangles, sorted columns) of the same
+    device system: TPZFYsmpMatrix (Matrix/pzysmp.h), as TPZSpStructMatrix
+    creates it.  MultAdd and the solvers are those of the symmetric matrix."""
+
+    def SetData(self, ia, ja, a, rhs=None):
+        raise L.PzgError(L.PZG_EUNSUP, "TPZFYsmpMatrixHIP: caller matrices go through TPZSYsmpMatrixHIP")
+
+    def AddKel(self, ek_batch, dest_batch):
+        raise L.PzgError(L.PZG_EUNSUP, "TPZFYsmpMatrixHIP: AddKel goes through TPZSYsmpMatrixHIP")
+
+    def _pattern(self):
+        s = self.ctx.sizes()
+        ia = np.zeros(self.neq + 1, np.int64)
+        ja = np.zeros(max(s["nnz_full"], 1), np.int64)
+        L.check(self.ctx.lib.pzg_full_csr(self.ctx.h, L.ptr(ia, L.i64p), L.ptr(ja, L.i64p), None))
+        self._ia, self._ja = ia, ja[: s["nnz_full"]]
+
+    def A(self):
+        a = np.zeros(max(len(self.JA()), 1))
+        L.check(self.ctx.lib.pzg_full_csr(self.ctx.h, None, None, L.ptr(a, L.dp)))
+        return a[: len(self.JA())]
+
+    def GetVal(self, r, c):
+        """TPZFYsmpMatrix::GetVal: the row holds both triangles."""
+        ia, ja = self.IA(), self.JA()
+        row = ja[ia[r]:ia[r + 1]]
+        k = np.searchsorted(row, c)
+        if k < len(row) and row[k] == c:
+            return float(self.A()[ia[r] + k])
+        return 0.0
+
+
 class TPZStepSolver:
     """TPZStepSolver subset used on the path."""
 
@@ -264,6 +297,18 @@ class TPZSSpStructMatrixHIP:
         K = self.Create()
         self.Assemble(K)
         return K
+
+
+class TPZSpStructMatrixHIP(TPZSSpStructMatrixHIP):
+    """GPU TPZSpStructMatrix (StrMatrix/TPZSpStructMatrix.cpp:33-190): the same
+    assembly, the matrix handed out in the non-symmetric TPZFYsmpMatrix format."""
+
+    def Clone(self):
+        return TPZSpStructMatrixHIP(self.ctx, self.mesh)
+
+    def Create(self):
+        K = super().Create()
+        return TPZFYsmpMatrixHIP(self.ctx, K.neq)
 
 
 class TPZLinearAnalysis:

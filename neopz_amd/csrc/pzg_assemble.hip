@@ -946,6 +946,49 @@ int launch_expand_full(Ctx &c) {
 }
 
 // --------------------------------------------------------------------------
+// Equation-level full CSR (TPZFYsmpMatrix layout) from the FP64 block CSR: one
+// wavefront per block row A; row A*BS+k holds, for every block (A, B) in
+// ascending B, the columns B*BS .. B*BS+BS-1.
+template <int BS>
+__global__ void k_full_csr(int64_t nb, const int64_t *__restrict__ fbia,
+                           const int32_t *__restrict__ fbja, const double *__restrict__ fa,
+                           int64_t *__restrict__ ia, int64_t *__restrict__ ja,
+                           double *__restrict__ a) {
+  const int64_t A = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (A >= nb) return;
+  constexpr int BB = BS * BS;
+  const int64_t b0 = fbia[A], n = fbia[A + 1] - b0;
+  if (ia && lane < BS) ia[A * BS + lane] = b0 * BB + lane * n * BS;
+  for (int64_t e = lane; e < n * BB; e += 64) {
+    const int k = (int)(e / (n * BS));
+    const int64_t r = e % (n * BS), i = r / BS;
+    const int l = (int)(r % BS);
+    const int64_t out = b0 * BB + e;
+    if (ja) ja[out] = (int64_t)fbja[b0 + i] * BS + l;
+    if (a) a[out] = fa[bsr_pos<BB>(b0, n, i, k * BS + l)];
+  }
+}
+
+int launch_full_csr(Ctx &c, int64_t *d_ia, int64_t *d_ja, double *d_a) {
+  if (!c.full_valid) PZG_TRY(launch_expand_full(c));
+  const int64_t threads = c.nb * 64;
+  const dim3 g((unsigned)((threads + TPB - 1) / TPB));
+  if (c.bs == 3)
+    hipLaunchKernelGGL(k_full_csr<3>, g, dim3(TPB), 0, c.stream, c.nb, c.d_fbia.as<int64_t>(),
+                       c.d_fbja.as<int32_t>(), c.d_fa.as<double>(), d_ia, d_ja, d_a);
+  else if (c.bs == 1)
+    hipLaunchKernelGGL(k_full_csr<1>, g, dim3(TPB), 0, c.stream, c.nb, c.d_fbia.as<int64_t>(),
+                       c.d_fbja.as<int32_t>(), c.d_fa.as<double>(), d_ia, d_ja, d_a);
+  else {
+    set_error("full CSR: unsupported block size");
+    return PZG_EUNSUP;
+  }
+  PZG_HIP(hipGetLastError());
+  return PZG_OK;
+}
+
+// --------------------------------------------------------------------------
 // Scaled FP32 block CSR S = D^-1/2 K D^-1/2 (unit diagonal) and the inverses
 // of its diagonal blocks.
 __global__ void k_dinvsqrt(int64_t n, const double *__restrict__ d, double *__restrict__ s) {

@@ -684,6 +684,38 @@ int pzg_get_system(pzg_ctx *ctx, double *a, double *rhs) {
   return PZG_OK;
 }
 
+int pzg_full_csr(pzg_ctx *ctx, int64_t *ia, int64_t *ja, double *a) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (!c.assembled) {
+    set_error("pzg_full_csr: needs an assembled system");
+    return PZG_ESTATE;
+  }
+  const int64_t nnz = c.nnz_full;
+  DevBuf dia, dja, da;
+  if (ia) PZG_TRY(dev_alloc(dia, sizeof(int64_t) * (c.neq + 1)));
+  if (ja) PZG_TRY(dev_alloc(dja, sizeof(int64_t) * std::max<int64_t>(nnz, 1)));
+  if (a) PZG_TRY(dev_alloc(da, sizeof(double) * std::max<int64_t>(nnz, 1)));
+  int rc = launch_full_csr(c, ia ? dia.as<int64_t>() : nullptr, ja ? dja.as<int64_t>() : nullptr,
+                           a ? da.as<double>() : nullptr);
+  if (rc == PZG_OK) {
+    hipError_t e = hipSuccess;
+    if (ia && e == hipSuccess) e = hipMemcpyAsync(ia, dia.p, sizeof(int64_t) * c.neq, hipMemcpyDeviceToHost, c.stream);
+    if (ja && e == hipSuccess) e = hipMemcpyAsync(ja, dja.p, sizeof(int64_t) * nnz, hipMemcpyDeviceToHost, c.stream);
+    if (a && e == hipSuccess) e = hipMemcpyAsync(a, da.p, sizeof(double) * nnz, hipMemcpyDeviceToHost, c.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+    if (e != hipSuccess) {
+      set_error(std::string("pzg_full_csr: ") + hipGetErrorString(e));
+      rc = PZG_EHIP;
+    }
+  }
+  if (ia && rc == PZG_OK) ia[c.neq] = nnz;
+  dev_free(dia);
+  dev_free(dja);
+  dev_free(da);
+  return rc;
+}
+
 int pzg_csr_set(pzg_ctx *ctx, int64_t neq, const int64_t *ia, const int64_t *ja, const double *a,
                 const double *rhs) {
   CTX_CHECK();

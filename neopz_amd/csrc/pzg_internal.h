@@ -222,6 +222,8 @@ int launch_elem_maps(Ctx &c);
 int launch_assemble(Ctx &c);
 int launch_diag(Ctx &c);
 int launch_expand_full(Ctx &c);
+// equation-level full CSR (device arrays, any may be null; ia without ia[neq])
+int launch_full_csr(Ctx &c, int64_t *d_ia, int64_t *d_ja, double *d_a);
 int launch_build_bsr(Ctx &c);
 int launch_spmv64(Ctx &c, const double *x, double *y);
 int launch_spmv_bsr(Ctx &c, const float *x, float *y);

@@ -5,7 +5,9 @@
 // behaviour (a failing call prints to stderr and stops, as PZError +
 // DebugStop() do), so that code written against
 //   TPZSSpStructMatrix (StrMatrix/TPZSSpStructMatrix.h:17-38),
+//   TPZSpStructMatrix  (StrMatrix/TPZSpStructMatrix.h),
 //   TPZSYsmpMatrix     (Matrix/pzsysmp.h:20-230),
+//   TPZFYsmpMatrix     (Matrix/pzysmp.h),
 //   TPZStepSolver      (Solvers/pzstepsolver.h:34-89),
 //   TPZLinearAnalysis  (Analysis/TPZLinearAnalysis.h:20-45)
 // reads the same.  NeoPZ's own containers are replaced by std::vector; the
@@ -44,6 +46,7 @@ inline Ctx MakeContext(int device = 0) {
 class TPZSYsmpMatrixHIP {
  public:
   TPZSYsmpMatrixHIP(Ctx ctx, int64_t neq) : fCtx(std::move(ctx)), fNeq(neq) {}
+  virtual ~TPZSYsmpMatrixHIP() = default;
   int64_t Rows() const { return fNeq; }
   // TPZSYsmpMatrix::SetData (Matrix/pzsysmp.h:232-240)
   void SetData(std::vector<int64_t> IA, std::vector<int64_t> JA, std::vector<double> A) {
@@ -60,13 +63,13 @@ class TPZSYsmpMatrixHIP {
     Pattern();
     return fJA;
   }
-  std::vector<double> A() {
+  virtual std::vector<double> A() {
     std::vector<double> a(JA().size());
     Check(pzg_get_system(fCtx.get(), a.data(), nullptr), "A");
     return a;
   }
   // TPZSYsmpMatrix::GetVal (Matrix/pzsysmp.cpp:59-77): lower entries read the transpose
-  double GetVal(int64_t r, int64_t c) {
+  virtual double GetVal(int64_t r, int64_t c) {
     if (r > c) std::swap(r, c);
     const auto &ia = IA();
     const auto &ja = JA();
@@ -88,8 +91,8 @@ class TPZSYsmpMatrixHIP {
   }
   pzg_ctx *Context() const { return fCtx.get(); }
 
- private:
-  void Pattern() {
+ protected:
+  virtual void Pattern() {
     if (!fIA.empty()) return;
     pzg_sizes s;
     Check(pzg_get_sizes(fCtx.get(), &s), "sizes");
@@ -100,6 +103,36 @@ class TPZSYsmpMatrixHIP {
   Ctx fCtx;
   int64_t fNeq;
   std::vector<int64_t> fIA, fJA;
+};
+
+// TPZFYsmpMatrix: non-symmetric sparse rows (both triangles, sorted columns)
+// over the same device system; MultAdd and the solvers are the symmetric ones.
+class TPZFYsmpMatrixHIP : public TPZSYsmpMatrixHIP {
+ public:
+  using TPZSYsmpMatrixHIP::TPZSYsmpMatrixHIP;
+  std::vector<double> A() override {
+    std::vector<double> a(JA().size());
+    Check(pzg_full_csr(fCtx.get(), nullptr, nullptr, a.data()), "A");
+    return a;
+  }
+  // TPZFYsmpMatrix::GetVal (Matrix/pzysmp.cpp): the row holds both triangles
+  double GetVal(int64_t r, int64_t c) override {
+    const auto &ia = IA();
+    const auto &ja = JA();
+    for (int64_t k = ia[r]; k < ia[r + 1]; k++)
+      if (ja[k] == c) return A()[k];
+    return 0.;
+  }
+
+ protected:
+  void Pattern() override {
+    if (!fIA.empty()) return;
+    pzg_sizes s;
+    Check(pzg_get_sizes(fCtx.get(), &s), "sizes");
+    fIA.resize(fNeq + 1);
+    fJA.resize(s.nnz_full);
+    Check(pzg_full_csr(fCtx.get(), fIA.data(), fJA.data(), nullptr), "IA/JA");
+  }
 };
 
 // TPZStepSolver: ECG with an EJacobi(1) preconditioner, or the mixed solver.
@@ -162,15 +195,10 @@ class TPZStepSolver {
 class TPZSSpStructMatrixHIP {
  public:
   TPZSSpStructMatrixHIP(Ctx ctx, const pzg_mesh_desc &mesh) : fCtx(std::move(ctx)), fMesh(mesh) {}
+  virtual ~TPZSSpStructMatrixHIP() = default;
   // TPZSSpStructMatrix::Create (StrMatrix/TPZSSpStructMatrix.cpp:30-47)
-  std::unique_ptr<TPZSYsmpMatrixHIP> Create() {
-    if (!fFlattened) {
-      Check(pzg_mesh_set(fCtx.get(), &fMesh), "mesh");
-      fFlattened = true;
-    }
-    int64_t neq = 0, nnz = 0;
-    Check(pzg_struct_create(fCtx.get(), &neq, &nnz), "Create");
-    return std::make_unique<TPZSYsmpMatrixHIP>(fCtx, neq);
+  virtual std::unique_ptr<TPZSYsmpMatrixHIP> Create() {
+    return std::make_unique<TPZSYsmpMatrixHIP>(fCtx, Setup());
   }
   // TPZStructMatrixOR::Assemble(K, rhs) (StrMatrix/pzstrmatrixor.cpp:41-71)
   void Assemble(TPZSYsmpMatrixHIP & /*K*/, std::vector<double> &rhs) {
@@ -179,24 +207,48 @@ class TPZSSpStructMatrixHIP {
     Check(pzg_get_system(fCtx.get(), nullptr, rhs.data()), "rhs");
   }
 
- private:
+ protected:
+  int64_t Setup() {
+    if (!fFlattened) {
+      Check(pzg_mesh_set(fCtx.get(), &fMesh), "mesh");
+      fFlattened = true;
+    }
+    int64_t neq = 0, nnz = 0;
+    Check(pzg_struct_create(fCtx.get(), &neq, &nnz), "Create");
+    return neq;
+  }
   Ctx fCtx;
   pzg_mesh_desc fMesh;
   bool fFlattened = false;
+};
+
+// TPZSpStructMatrix over the GPU: the same assembly, the matrix in the
+// non-symmetric format (StrMatrix/TPZSpStructMatrix.cpp:33-190).
+class TPZSpStructMatrixHIP : public TPZSSpStructMatrixHIP {
+ public:
+  using TPZSSpStructMatrixHIP::TPZSSpStructMatrixHIP;
+  std::unique_ptr<TPZSYsmpMatrixHIP> Create() override {
+    return std::make_unique<TPZFYsmpMatrixHIP>(fCtx, Setup());
+  }
 };
 
 // TPZLinearAnalysis(cmesh, mustOptimizeBandwidth=false): Assemble()/Solve().
 class TPZLinearAnalysis {
  public:
   TPZLinearAnalysis(Ctx ctx, const pzg_mesh_desc &mesh, bool mustOptimizeBandwidth = false)
-      : fCtx(std::move(ctx)), fStr(fCtx, mesh) {
+      : fCtx(std::move(ctx)), fStr(std::make_unique<TPZSSpStructMatrixHIP>(fCtx, mesh)) {
     if (mustOptimizeBandwidth) DebugStop();  // first-touch numbering only
+  }
+  // TPZAnalysis::SetStructuralMatrix (Analysis/TPZAnalysis.cpp:83-104)
+  void SetStructuralMatrix(std::unique_ptr<TPZSSpStructMatrixHIP> str) {
+    fStr = std::move(str);
+    fK.reset();
   }
   void SetSolver(const TPZStepSolver &s) { fSolver = s; }
   // Analysis/TPZLinearAnalysis.cpp:42-94
   void Assemble() {
-    if (!fK) fK = fStr.Create();
-    fStr.Assemble(*fK, fRhs);
+    if (!fK) fK = fStr->Create();
+    fStr->Assemble(*fK, fRhs);
     fSolver.SetMatrix(fK.get());
   }
   // Analysis/TPZLinearAnalysis.cpp:130-227
@@ -208,7 +260,7 @@ class TPZLinearAnalysis {
 
  private:
   Ctx fCtx;
-  TPZSSpStructMatrixHIP fStr;
+  std::unique_ptr<TPZSSpStructMatrixHIP> fStr;
   std::unique_ptr<TPZSYsmpMatrixHIP> fK;
   TPZStepSolver fSolver;
   std::vector<double> fRhs, fSolution;

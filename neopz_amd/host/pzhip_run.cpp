@@ -14,7 +14,8 @@
 // (int64/double arrays, each with its count), then iterations and the
 // achieved tolerance.
 //
-// usage: pzhip_run MESH OUT [cg|mixed] [tol] [maxit]
+// usage: pzhip_run MESH OUT [cg|mixed] [tol] [maxit] [sym|full]
+// (full: TPZSpStructMatrix, IA/JA/A of the non-symmetric TPZFYsmpMatrix)
 #include <cstring>
 #include <string>
 
@@ -55,12 +56,17 @@ void put(FILE *f, const std::vector<T> &v) {
 
 int main(int argc, char **argv) {
   if (argc < 3) {
-    std::fprintf(stderr, "usage: %s MESH OUT [cg|mixed] [tol] [maxit]\n", argv[0]);
+    std::fprintf(stderr, "usage: %s MESH OUT [cg|mixed] [tol] [maxit] [sym|full]\n", argv[0]);
     return 2;
   }
   const std::string method = argc > 3 ? argv[3] : "cg";
   const double tol = argc > 4 ? std::atof(argv[4]) : 1e-10;
   const int64_t maxit = argc > 5 ? std::atoll(argv[5]) : 100000;
+  const std::string format = argc > 6 ? argv[6] : "sym";
+  if (format != "sym" && format != "full") {
+    std::fprintf(stderr, "pzhip_run: unknown matrix format %s\n", format.c_str());
+    return 2;
+  }
 
   FILE *f = std::fopen(argv[1], "rb");
   if (!f) {
@@ -133,7 +139,9 @@ int main(int argc, char **argv) {
   m.bc_rule = bc_rule.empty() ? nullptr : bc_rule.data();
 
   using namespace pzhip;
-  TPZLinearAnalysis an(MakeContext(0), m, /*mustOptimizeBandwidth=*/false);
+  Ctx ctx = MakeContext(0);
+  TPZLinearAnalysis an(ctx, m, /*mustOptimizeBandwidth=*/false);
+  if (format == "full") an.SetStructuralMatrix(std::make_unique<TPZSpStructMatrixHIP>(ctx, m));
   TPZStepSolver step;
   if (method == "cg") {
     TPZStepSolver jac;

@@ -77,6 +77,9 @@ def lib():
                                      dp, C.c_double, C.c_double, C.c_double, dp, dp]
         L.orc_setup_pattern.argtypes = [M, C.POINTER(i64p), C.POINTER(i64p)]
         L.orc_setup_pattern.restype = C.c_int64
+        L.orc_setup_pattern_full.argtypes = [M, C.POINTER(i64p), C.POINTER(i64p)]
+        L.orc_setup_pattern_full.restype = C.c_int64
+        L.orc_assemble_full.argtypes = [P, M, i64p, i64p, dp, dp]
         L.orc_assemble.argtypes = [P, M, i64p, i64p, dp, dp]
         L.orc_assemble_range.argtypes = [P, M, i64p, i64p, dp, dp, C.c_int64, C.c_int64]
         L.orc_addkel.argtypes = [i64p, i64p, dp, C.c_int, dp, i64p]
@@ -265,6 +268,27 @@ class Oracle:
         if ne < 0:
             raise RuntimeError("oracle CalcStiff failed (element too large or order > 8)")
         return ek[: ne * ne].reshape(ne, ne).copy(), ef[:ne].copy(), dest[:ne].copy()
+
+    def pattern_full(self):
+        """TPZSpStructMatrix / TPZFYsmpMatrix pattern (both triangles)."""
+        ia = C.POINTER(C.c_int64)()
+        ja = C.POINTER(C.c_int64)()
+        nnz = lib().orc_setup_pattern_full(C.byref(self.m), C.byref(ia), C.byref(ja))
+        out_ia = np.ctypeslib.as_array(ia, shape=(self.neq + 1,)).copy()
+        out_ja = np.ctypeslib.as_array(ja, shape=(max(nnz, 1),))[:nnz].copy()
+        lib().free(ia)
+        lib().free(ja)
+        return out_ia, out_ja
+
+    def assemble_full(self):
+        """Assembly into the TPZFYsmpMatrix pattern (TPZFYsmpMatrix::AddKel)."""
+        ia, ja = self.pattern_full()
+        a = np.zeros(len(ja))
+        rhs = np.zeros(self.neq)
+        if lib().orc_assemble_full(C.byref(self.pb), C.byref(self.m), _ip(ia), _ip(ja), _dp(a),
+                                   _dp(rhs)) != 0:
+            raise RuntimeError("oracle full assembly failed")
+        return ia, ja, a, rhs
 
     def assemble(self):
         ia, ja = self.pattern()

@@ -1160,7 +1160,9 @@ static int cmp_i64(const void *a, const void *b) {
   return x < y ? -1 : x > y;
 }
 
-int64_t orc_setup_pattern(const orc_mesh *m, int64_t **ia_out, int64_t **ja_out) {
+/* Node graph of TPZRenumbering::ConvertGraph: per connect (sequence number)
+ * the sorted set of the other connects of its elements. */
+static void node_graph(const orc_mesh *m, int64_t **ngp_out, int64_t **ng_out) {
   int64_t nb = m->nconn;
   /* node -> element graph */
   int64_t *cnt = (int64_t *)calloc(nb + 1, sizeof(int64_t));
@@ -1201,6 +1203,14 @@ int64_t orc_setup_pattern(const orc_mesh *m, int64_t **ia_out, int64_t **ja_out)
   free(tmp);
   free(n2e);
   free(cnt);
+  *ngp_out = ngp;
+  *ng_out = ng;
+}
+
+int64_t orc_setup_pattern(const orc_mesh *m, int64_t **ia_out, int64_t **ja_out) {
+  const int64_t nb = m->nconn;
+  int64_t *ngp, *ng;
+  node_graph(m, &ngp, &ng);
   /* count pass */
   int64_t total = 0;
   for (int64_t i = 0; i < nb; i++) {
@@ -1247,6 +1257,78 @@ int64_t orc_setup_pattern(const orc_mesh *m, int64_t **ia_out, int64_t **ja_out)
 /* IsZero(v) = |v| < 1e-12 (Common/pzreal.h:736-741), sequential guess then  */
 /* row search.                                                               */
 /* ------------------------------------------------------------------------ */
+/* TPZSpStructMatrix::SetupMatrixData (StrMatrix/TPZSpStructMatrix.cpp:53-190):
+ * every equation of block i gets the equations of block i and of every
+ * adjacent block (both triangles), then the row is stable-sorted. */
+int64_t orc_setup_pattern_full(const orc_mesh *m, int64_t **ia_out, int64_t **ja_out) {
+  const int64_t nb = m->nconn;
+  int64_t *ngp, *ng;
+  node_graph(m, &ngp, &ng);
+  int64_t total = 0;
+  for (int64_t i = 0; i < nb; i++) {
+    int64_t bs = m->blk_size[i];
+    if (!bs) continue;
+    total += bs * bs;
+    for (int64_t j = ngp[i]; j < ngp[i + 1]; j++) total += bs * m->blk_size[ng[j]];
+  }
+  int64_t *ia = (int64_t *)malloc((m->neq + 1) * sizeof(int64_t));
+  int64_t *ja = (int64_t *)malloc((total ? total : 1) * sizeof(int64_t));
+  int64_t pos = 0, ieq = 0;
+  for (int64_t i = 0; i < nb; i++) {
+    int64_t bs = m->blk_size[i], bp = m->blk_pos[i];
+    for (int64_t k = 0; k < bs; k++, ieq++) {
+      ia[ieq] = pos;
+      for (int64_t l = 0; l < bs; l++) ja[pos++] = bp + l;
+      for (int64_t j = ngp[i]; j < ngp[i + 1]; j++)
+        for (int64_t l = 0; l < m->blk_size[ng[j]]; l++) ja[pos++] = m->blk_pos[ng[j]] + l;
+      qsort(ja + ia[ieq], pos - ia[ieq], sizeof(int64_t), cmp_i64); /* distinct keys */
+    }
+  }
+  ia[ieq] = pos;
+  free(ng);
+  free(ngp);
+  *ia_out = ia;
+  *ja_out = ja;
+  return pos;
+}
+
+/* TPZFYsmpMatrix::AddKel (Matrix/pzysmp.cpp:135-176): every entry of ek with
+ * |v| >= 1e-12, both triangles, sequential guess then row search. */
+void orc_addkel_full(const int64_t *ia, const int64_t *ja, double *a, int ne, const double *ek,
+                     const int64_t *dest) {
+  int64_t k = 0;
+  for (int i = 0; i < ne; i++)
+    for (int j = 0; j < ne; j++) {
+      int64_t ip = dest[i], jp = dest[j];
+      double v = ek[i * ne + j];
+      if (fabs(v) < 1.e-12) continue;
+      k++;
+      if (k >= ia[ip] && k < ia[ip + 1] && ja[k] == jp) {
+        a[k] += v;
+      } else {
+        for (k = ia[ip]; k < ia[ip + 1]; k++)
+          if (ja[k] == jp) { a[k] += v; break; }
+      }
+    }
+}
+
+int orc_assemble_full(const orc_problem *pb, const orc_mesh *m, const int64_t *ia,
+                      const int64_t *ja, double *a, double *rhs) {
+  memset(a, 0, sizeof(double) * ia[m->neq]);
+  memset(rhs, 0, sizeof(double) * m->neq);
+  double *ek = (double *)malloc(sizeof(double) * ORC_MAXNE * ORC_MAXNE);
+  double ef[ORC_MAXNE];
+  int64_t dest[ORC_MAXNE];
+  for (int64_t e = 0; e < m->nel; e++) {
+    int ne = orc_calcstiff(pb, m, e, ek, ef, dest);
+    if (ne < 0) { free(ek); return -1; }
+    orc_addkel_full(ia, ja, a, ne, ek, dest);
+    for (int i = 0; i < ne; i++) rhs[dest[i]] += ef[i];
+  }
+  free(ek);
+  return 0;
+}
+
 void orc_addkel(const int64_t *ia, const int64_t *ja, double *a, int ne, const double *ek,
                 const int64_t *dest) {
   int64_t k = 0;

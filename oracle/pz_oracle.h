@@ -116,6 +116,15 @@ void orc_contribute(int material, int dim, int nsh, int ne, double weight, const
 /* SetupMatrixData: returns nnz; ia (neq+1) and ja (nnz) are malloc'ed. */
 int64_t orc_setup_pattern(const orc_mesh *m, int64_t **ia, int64_t **ja);
 
+/* TPZSpStructMatrix::SetupMatrixData: full (both triangles) sorted CSR of the
+ * non-symmetric TPZFYsmpMatrix; returns nnz, ia/ja malloc'ed. */
+int64_t orc_setup_pattern_full(const orc_mesh *m, int64_t **ia, int64_t **ja);
+/* TPZFYsmpMatrix::AddKel and the full assembly on that pattern */
+void orc_addkel_full(const int64_t *ia, const int64_t *ja, double *a, int ne,
+                     const double *ek, const int64_t *dest);
+int orc_assemble_full(const orc_problem *pb, const orc_mesh *m, const int64_t *ia,
+                      const int64_t *ja, double *a, double *rhs);
+
 /* Full serial assembly (Serial_Assemble + AddKel + AddFel). a/rhs zeroed here. */
 int orc_assemble(const orc_problem *pb, const orc_mesh *m, const int64_t *ia,
                  const int64_t *ja, double *a, double *rhs);

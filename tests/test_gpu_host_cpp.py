@@ -82,3 +82,31 @@ def test_cpp_analysis_matches_oracle(tmp_path, name, pb):
     ref = O.pcg(ia, ja, a, rhs, maxit=20000, tol=1e-13, wasted_spmv=False)
     assert ref["converged"] and res["achieved"] <= 1e-13
     assert np.linalg.norm(res["u"] - ref["x"]) <= 1e-8 * np.linalg.norm(ref["x"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,pb", [
+    ("elast3d_p2_body", lambda: P.elasticity3d(n=3, p=2, seed=6, field_kind="grf", load="body")),
+    ("darcy_tet_hp5", lambda: P.darcy3d_tet(n=2, pmax=5, seed=5, hp=True)),
+])
+def test_cpp_full_format_matches_oracle(tmp_path, name, pb):
+    """The C++ mirror with TPZSpStructMatrixHIP: IA/JA/A of the non-symmetric
+    TPZFYsmpMatrix equal the oracle's, and the CG solution its solve."""
+    pb = pb()
+    mfile, ofile = tmp_path / "mesh.bin", tmp_path / "out.bin"
+    M.save(M.build(pb), mfile)
+    print(f"[gpu] pzhip_run full {name}", flush=True)
+    r = subprocess.run([str(RUN), str(mfile), str(ofile), "cg", "1e-13", "20000", "full"],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    res = _read_result(ofile)
+    o = O.Oracle(pb)
+    fia, fja, fa, frhs = o.assemble_full()
+    assert np.array_equal(res["ia"], fia) and np.array_equal(res["ja"], fja)
+    assert np.max(np.abs(res["a"] - fa) - 1e-12 * np.abs(fa)) <= 1e-15 * np.max(np.abs(fa))
+    assert np.max(np.abs(res["rhs"] - frhs)) <= 1e-12 * np.max(np.abs(frhs))
+    ia, ja = o.pattern()
+    a, rhs = o.assemble()
+    ref = O.pcg(ia, ja, a, rhs, maxit=20000, tol=1e-13, wasted_spmv=False)
+    assert np.linalg.norm(res["u"] - ref["x"]) <= 1e-8 * np.linalg.norm(ref["x"])

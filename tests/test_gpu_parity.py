@@ -24,7 +24,7 @@ from neopz_amd import _lib as L
 from neopz_amd import mesh as M
 from neopz_amd import problems as P
 from neopz_amd.analysis import Context, TPZLinearAnalysis, TPZSSpStructMatrixHIP, TPZStepSolver, \
-    TPZSYsmpMatrixHIP
+    TPZSYsmpMatrixHIP, TPZSpStructMatrixHIP
 
 pytestmark = pytest.mark.gpu
 
@@ -132,6 +132,29 @@ def test_pattern_and_values(name):
     assert np.array_equal(ja, ja_r)
     _check_values(ia, a, a_r)
     assert np.max(np.abs(rhs - rhs_r)) <= 1e-12 * max(np.max(np.abs(rhs_r)), 1e-300)
+
+
+@pytest.mark.parametrize("name", ["poisson2d_p2", "elast3d_p2_traction", "darcy_tet_hp5"])
+def test_full_csr_matches_oracle(name):
+    """TPZSpStructMatrix format: pattern bit-exact with the oracle's
+    TPZFYsmpMatrix, values within 1e-12, and the operator of the non-symmetric
+    matrix object solves like the symmetric one."""
+    pb = CASES[name]()
+    c = ctx()
+    sm = TPZSpStructMatrixHIP(c, M.build(pb))
+    K = sm.Create()
+    sm.Assemble(K)
+    o = O.Oracle(pb)
+    fia, fja, fa, frhs = o.assemble_full()
+    assert np.array_equal(K.IA(), fia) and np.array_equal(K.JA(), fja)
+    _check_values(fia, K.A(), fa)
+    r0 = int(fia[1] - 1)
+    assert K.GetVal(0, int(fja[r0])) == pytest.approx(fa[r0], rel=1e-12)
+    x = np.random.default_rng(3).standard_normal(o.neq)
+    rows = np.repeat(np.arange(o.neq), np.diff(fia))
+    Kf = sp.csr_matrix((fa, (rows, fja)), shape=(o.neq, o.neq))
+    y = K.Multiply(x)
+    assert np.linalg.norm(y - Kf @ x) <= 1e-12 * np.linalg.norm(Kf @ x)
 
 
 def test_assembly_is_deterministic():

@@ -873,3 +873,37 @@ def test_tet_hp_patch_tests(pmax):
                     assert np.abs(blk[1:]).max(initial=0) < 1e-9
                 else:
                     assert np.abs(blk).max() < 1e-9
+
+
+# --------------------------------------------------------------------------- non-symmetric format
+@pytest.mark.parametrize("kind", ["poisson2d", "elast3d", "tet_hp5"])
+def test_full_pattern_and_values(kind):
+    """TPZSpStructMatrix / TPZFYsmpMatrix: every row lists all coupled
+    equations (both triangles) sorted; the pattern equals the brute-force
+    union of element dof pairs and the symmetric closure of the upper pattern;
+    TPZFYsmpMatrix::AddKel values agree with the symmetric matrix (lower
+    entries with their transposes up to the rounding of ek's asymmetry)."""
+    if kind == "poisson2d":
+        pb = P.poisson2d(n=5, p=2, seed=1)
+    elif kind == "elast3d":
+        pb = P.elasticity3d(n=2, p=2, seed=2, load="traction")
+    else:
+        pb = _tet_hp_problem(9, force=1.0)
+    o = O.Oracle(pb)
+    fia, fja, fa, frhs = o.assemble_full()
+    rows = [set() for _ in range(o.neq)]
+    for e in range(o.m.nel):
+        _, _, dest = o.calcstiff(e)
+        for i in dest:
+            rows[i].update(dest)
+    assert np.array_equal(np.diff(fia), [len(r) for r in rows])
+    assert np.array_equal(fja, np.concatenate([np.array(sorted(r), dtype=np.int64) for r in rows]))
+    ia, ja = o.pattern()
+    a, rhs = o.assemble()
+    K = O.upper_to_dense(ia, ja, a)
+    F = np.zeros_like(K)
+    for r in range(o.neq):
+        F[r, fja[fia[r]:fia[r + 1]]] = fa[fia[r]:fia[r + 1]]
+    scale = np.abs(K).max(axis=1, keepdims=True)
+    assert np.all(np.abs(F - K) <= 1e-13 * scale)
+    assert np.array_equal(frhs, rhs)
