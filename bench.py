@@ -305,6 +305,9 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    tuning = L.hp_tuning(ctx)
+    if tuning:
+        out["hp_assembly_tuning"] = tuning
     print(json.dumps(out))
     ctx.close()
 

@@ -237,6 +237,16 @@ int pzg_true_residual(pzg_ctx *ctx, double *rel);
  * algorithmic flops of one launch instead of bytes. */
 int pzg_time_kernel(pzg_ctx *ctx, int32_t which, int32_t nrep, double *avg_ms, double *bytes);
 
+/* No counterpart in the reference (measurement): the choices of the
+ * assembly-kernel autotuner of high-order tetrahedral meshes.  On entry *n is
+ * the capacity of the arrays; on return it is the number of rule buckets,
+ * order[b] the bucket's highest element order, variant[b] the kernel variant
+ * kept and ms[b*5 + v] the measured milliseconds per launch of variant v
+ * (0: 16 points/pass, 4x4 tiles; 1: 8, 4x4; 2: 32, 4x4; 3: 16, 2x2;
+ * 4: 32, 2x2).  *n = 0 when the mesh has no high-order tetrahedra or has not
+ * been assembled. */
+int pzg_hp_tuning(pzg_ctx *ctx, int32_t *n, int32_t *order, int32_t *variant, float *ms);
+
 /* y = S x with S = D^-1/2 K D^-1/2 applied by the FP32 inner operator op
  * (PZG_INNER_BSR or PZG_INNER_MATFREE) on host vectors of length neq; D is
  * the diagonal of the assembled K.  Exposes the operator of the mixed

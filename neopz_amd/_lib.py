@@ -22,7 +22,7 @@ EXPORTS = [
     "pzg_add_kel_batch", "pzg_multadd", "pzg_solve", "pzg_true_residual", "pzg_time_kernel",
     "pzg_get_sizes", "pzg_sync", "pzg_mesh_set_coef", "pzg_learned_setup", "pzg_learned_collect",
     "pzg_learned_train", "pzg_learned_weights", "pzg_learned_graph", "pzg_learned_samples",
-    "pzg_learned_loss_grad", "pzg_scaled_apply", "pzg_full_csr",
+    "pzg_learned_loss_grad", "pzg_scaled_apply", "pzg_full_csr", "pzg_hp_tuning",
 ]
 
 dp = C.POINTER(C.c_double)
@@ -118,6 +118,8 @@ def load():
     L.pzg_learned_samples.argtypes = [vp, fp, fp]
     L.pzg_learned_loss_grad.argtypes = [vp, C.c_int64, fp, fp, dp, fp]
     L.pzg_scaled_apply.argtypes = [vp, C.c_int32, dp, dp]
+    L.pzg_full_csr.argtypes = [vp, i64p, i64p, dp]
+    L.pzg_hp_tuning.argtypes = [vp, i32p, i32p, i32p, fp]
     _lib = L
     return L
 
@@ -183,3 +185,16 @@ def make_mesh_desc(m):
     d.bc_rule = ptr(arr(m.get("bc_rule"), np.int32), i32p)
     d.vol_etype = int(m.get("vol_etype", 0))
     return d, keep
+
+
+def hp_tuning(ctx):
+    """Choices of the high-order assembly autotuner: per rule bucket the
+    highest order, the kept variant and the milliseconds of every variant."""
+    n = C.c_int32(64)
+    order = np.zeros(64, np.int32)
+    var = np.zeros(64, np.int32)
+    ms = np.zeros(64 * 5, np.float32)
+    check(ctx.lib.pzg_hp_tuning(ctx.h, C.byref(n), ptr(order, i32p), ptr(var, i32p), ptr(ms, fp)))
+    k = n.value
+    return [{"order": int(order[b]), "variant": int(var[b]),
+             "ms_per_variant": [round(float(x), 4) for x in ms[5 * b:5 * b + 5]]} for b in range(k)]

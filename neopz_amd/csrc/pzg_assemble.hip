@@ -453,16 +453,17 @@ __device__ inline int hp_variant(bool tet, int s, const int64_t *id) {
 
 // Shapes of one rule bucket: the sub-list sub[0..NSUB) of the element's NSV
 // virtual sides (the shapes an element of the bucket's highest order can
-// carry).  The Gram matrix is symmetric: thread t owns the 4x4 tile (ti, tj),
-// ti <= tj, of sub-list pairs and scatters every pair once.
-template <int NSUB>
+// carry).  The Gram matrix is symmetric: the TSxTS tiles (ti, tj), ti <= tj,
+// of sub-list pairs are dealt to the threads, which scatter every pair once.
+// QC quadrature points are staged per pass.  (QC, TS) is picked per bucket by
+// timing (hp_tune).
+template <int NSUB, int QC, int TS>
 __global__ __launch_bounds__(TPB) void k_vol_hp(ElemArgs g, int nsv, const HpTables *__restrict__ tab,
                                                 const int32_t *__restrict__ sub,
                                                 const int32_t *__restrict__ list) {
-  constexpr int QC = 16;
-  constexpr int NT = (NSUB + 3) / 4, NSP = 4 * NT;
+  constexpr int NT = (NSUB + TS - 1) / TS, NSP = TS * NT;
   constexpr int NTILE = NT * (NT + 1) / 2;
-  static_assert(NTILE <= TPB, "tiles per element exceed the workgroup");
+  constexpr int MT = (NTILE + TPB - 1) / TPB;  // tiles per thread
   __shared__ double s_coord[12];
   __shared__ double s_jinv[3][3];
   __shared__ double s_det;
@@ -515,19 +516,27 @@ __global__ __launch_bounds__(TPB) void k_vol_hp(ElemArgs g, int nsv, const HpTab
     s_jinv[2][2] = (-j[0][1] * j[1][0] + j[0][0] * j[1][1]) / det;
     s_det = fabs(det);
   }
-  // tile of this thread
-  int ti = 0, tj = 0;
-  if (tid < NTILE) {
-    int t = tid;
-    while (t >= NT - ti) { t -= NT - ti; ti++; }
-    tj = ti + t;
+  // tiles of this thread: t = tid + m*TPB
+  int ti[MT], tj[MT];
+#pragma unroll
+  for (int m = 0; m < MT; m++) {
+    int t = tid + m * TPB, a = 0;
+    if (t < NTILE) {
+      while (t >= NT - a) { t -= NT - a; a++; }
+      ti[m] = a;
+      tj[m] = a + t;
+    } else {
+      ti[m] = tj[m] = -1;
+    }
   }
   __syncthreads();
-  double acc[4][4];
+  double acc[MT][TS][TS];
 #pragma unroll
-  for (int r = 0; r < 4; r++)
+  for (int m = 0; m < MT; m++)
 #pragma unroll
-    for (int q = 0; q < 4; q++) acc[r][q] = 0.;
+    for (int r = 0; r < TS; r++)
+#pragma unroll
+      for (int q = 0; q < TS; q++) acc[m][r][q] = 0.;
   double accf = 0.;
   const int nq = T.nq;
   for (int q0 = 0; q0 < nq; q0 += QC) {
@@ -553,20 +562,22 @@ __global__ __launch_bounds__(TPB) void k_vol_hp(ElemArgs g, int nsv, const HpTab
       s_pw[qq][i] = wd * T.phi[q][col];
     }
     __syncthreads();
-    if (tid < NTILE) {
+#pragma unroll
+    for (int m = 0; m < MT; m++) {
+      if (ti[m] < 0) continue;
       for (int qq = 0; qq < nqc; qq++)
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-          double u[4], v[4];
+          double u[TS], v[TS];
 #pragma unroll
-          for (int r = 0; r < 4; r++) {
-            u[r] = s_Gw[qq][a][4 * ti + r];
-            v[r] = s_G[qq][a][4 * tj + r];
+          for (int r = 0; r < TS; r++) {
+            u[r] = s_Gw[qq][a][TS * ti[m] + r];
+            v[r] = s_G[qq][a][TS * tj[m] + r];
           }
 #pragma unroll
-          for (int r = 0; r < 4; r++)
+          for (int r = 0; r < TS; r++)
 #pragma unroll
-            for (int q = 0; q < 4; q++) acc[r][q] += u[r] * v[q];
+            for (int q = 0; q < TS; q++) acc[m][r][q] += u[r] * v[q];
         }
     }
     if (tid < NSUB)
@@ -574,16 +585,18 @@ __global__ __launch_bounds__(TPB) void k_vol_hp(ElemArgs g, int nsv, const HpTab
     __syncthreads();
   }
   const double coef = g.coef ? g.coef[e] : g.scale;
-  if (tid < NTILE) {
 #pragma unroll
-    for (int r = 0; r < 4; r++)
+  for (int m = 0; m < MT; m++) {
+    if (ti[m] < 0) continue;
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int i = 4 * ti + r, j = 4 * tj + q;
-        if (i >= NSUB || j >= NSUB || (ti == tj && q < r)) continue;
+    for (int r = 0; r < TS; r++)
+#pragma unroll
+      for (int q = 0; q < TS; q++) {
+        const int i = TS * ti[m] + r, j = TS * tj[m] + q;
+        if (i >= NSUB || j >= NSUB || (ti[m] == tj[m] && q < r)) continue;
         const int64_t bi = s_blk[i], bj = s_blk[j];
         if (bi < 0 || bj < 0) continue;
-        const double v = coef * acc[r][q];
+        const double v = coef * acc[m][r][q];
         if (fabs(v) < 1.e-12) continue;  // IsZero drop rule of AddKel
         int64_t slot;
         if (bi == bj) slot = s_row[i];
@@ -752,14 +765,29 @@ static int run_bc(Ctx &c) {
   return PZG_OK;
 }
 
-// shape counts of the order-P sets: tetrahedron 4/10/20/35/56, triangle 3/6/10/15/21
+// shape counts of the order-P sets: tetrahedron 4/10/20/35/56, triangle 3/6/10/15/21.
+// Volume kernel variants (QC quadrature points per pass, TSxTS tiles):
+// 0: (16,4)  1: (8,4)  2: (32,4)  3: (16,2)  4: (32,2)
+constexpr int HP_NVARIANT = 5;
+template <int NV>
+static void launch_vol_variant(Ctx &c, const ElemArgs &g, int var, int nsv, const HpTables *tab,
+                               const int32_t *sub, const int32_t *list, int64_t n) {
+  switch (var) {
+    case 1: hipLaunchKernelGGL((k_vol_hp<NV, 8, 4>), dim3(n), dim3(TPB), 0, c.stream, g, nsv, tab, sub, list); break;
+    case 2: hipLaunchKernelGGL((k_vol_hp<NV, 32, 4>), dim3(n), dim3(TPB), 0, c.stream, g, nsv, tab, sub, list); break;
+    case 3: hipLaunchKernelGGL((k_vol_hp<NV, 16, 2>), dim3(n), dim3(TPB), 0, c.stream, g, nsv, tab, sub, list); break;
+    case 4: hipLaunchKernelGGL((k_vol_hp<NV, 32, 2>), dim3(n), dim3(TPB), 0, c.stream, g, nsv, tab, sub, list); break;
+    default: hipLaunchKernelGGL((k_vol_hp<NV, 16, 4>), dim3(n), dim3(TPB), 0, c.stream, g, nsv, tab, sub, list); break;
+  }
+}
+
 template <bool VOL>
-static int launch_hp_bucket(Ctx &c, const ElemArgs &g, int P, const HpTables *tab, const int32_t *sub,
-                            const int32_t *list, int64_t n) {
+static int launch_hp_bucket(Ctx &c, const ElemArgs &g, int P, int var, const HpTables *tab,
+                            const int32_t *sub, const int32_t *list, int64_t n) {
   const int nsv = VOL ? c.vol_nsides : c.bc_nsides;
-#define PZG_HP_CASE(PP, NV, NB)                                                                  \
-  case PP:                                                                                       \
-    if (VOL) hipLaunchKernelGGL((k_vol_hp<NV>), dim3(n), dim3(TPB), 0, c.stream, g, nsv, tab, sub, list); \
+#define PZG_HP_CASE(PP, NV, NB)                                                            \
+  case PP:                                                                                 \
+    if (VOL) launch_vol_variant<NV>(c, g, var, nsv, tab, sub, list, n);                    \
     else hipLaunchKernelGGL((k_bc_hp<NB>), dim3(n), dim3(TPB), 0, c.stream, g, nsv, tab, sub, list); \
     break;
   switch (P) {
@@ -775,6 +803,49 @@ static int launch_hp_bucket(Ctx &c, const ElemArgs &g, int P, const HpTables *ta
   return PZG_OK;
 }
 
+// Autotuner of the volume kernels: per rule bucket, every variant runs on the
+// bucket's largest colour list (once to warm up, then timed over three
+// launches with HIP events) and the fastest is kept for the context.  The
+// launches scatter into the system, which launch_assemble zeroes afterwards.
+static int hp_tune(Ctx &c, const ElemArgs &g) {
+  const size_t nbk = c.vol_rules.size();
+  c.hp_vol_var.assign(nbk, 0);
+  c.hp_vol_ms.assign(nbk * HP_NVARIANT, 0.f);
+  hipEvent_t e0, e1;
+  PZG_HIP(hipEventCreate(&e0));
+  PZG_HIP(hipEventCreate(&e1));
+  int rc = PZG_OK;
+  for (size_t b = 0; b < nbk && rc == PZG_OK; b++) {
+    int64_t best = -1, nbest = 0;
+    for (size_t k = b; k + 1 < c.vcolor_ptr.size(); k += nbk) {
+      const int64_t n = c.vcolor_ptr[k + 1] - c.vcolor_ptr[k];
+      if (n > nbest) { nbest = n; best = (int64_t)k; }
+    }
+    if (best < 0) continue;
+    const int32_t *list = c.d_vcolor_list.as<int32_t>() + c.vcolor_ptr[best];
+    const HpTables *tab = c.d_thp_vol.as<HpTables>() + b;
+    const int32_t *sub = c.d_hp_vsub.as<int32_t>() + b * HP_MAXSH;
+    float tbest = 0.f;
+    for (int v = 0; v < HP_NVARIANT && rc == PZG_OK; v++) {
+      rc = launch_hp_bucket<true>(c, g, c.hp_vol_P[b], v, tab, sub, list, nbest);
+      if (rc != PZG_OK) break;
+      hipEventRecord(e0, c.stream);
+      for (int r = 0; r < 3 && rc == PZG_OK; r++)
+        rc = launch_hp_bucket<true>(c, g, c.hp_vol_P[b], v, tab, sub, list, nbest);
+      hipEventRecord(e1, c.stream);
+      if (hipEventSynchronize(e1) != hipSuccess) { rc = PZG_EHIP; break; }
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, e0, e1);
+      c.hp_vol_ms[b * HP_NVARIANT + v] = ms / 3.f;
+      if (v == 0 || ms < tbest) { tbest = ms; c.hp_vol_var[b] = v; }
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc == PZG_OK) c.hp_tuned = true;
+  return rc;
+}
+
 static int run_hp(Ctx &c) {
   if (c.bs != 1) {
     set_error("pzg_assemble: high-order tetrahedra support scalar materials only");
@@ -787,11 +858,16 @@ static int run_hp(Ctx &c) {
   }
   ElemArgs g = make_args(c, true);
   size_t nbk = c.vol_rules.size();
+  if (!c.hp_tuned) {
+    PZG_TRY(hp_tune(c, g));
+    fill<double>(c, c.d_a.as<double>(), c.nnz_u, 0.);
+    fill<double>(c, c.d_rhs.as<double>(), c.neq, 0.);
+  }
   for (size_t k = 0; k + 1 < c.vcolor_ptr.size(); k++) {
     const int64_t n = c.vcolor_ptr[k + 1] - c.vcolor_ptr[k];
     if (!n) continue;
     const size_t b = k % nbk;
-    PZG_TRY(launch_hp_bucket<true>(c, g, c.hp_vol_P[b], c.d_thp_vol.as<HpTables>() + b,
+    PZG_TRY(launch_hp_bucket<true>(c, g, c.hp_vol_P[b], c.hp_vol_var[b], c.d_thp_vol.as<HpTables>() + b,
                                    c.d_hp_vsub.as<int32_t>() + b * HP_MAXSH,
                                    c.d_vcolor_list.as<int32_t>() + c.vcolor_ptr[k], n));
   }
@@ -802,7 +878,7 @@ static int run_hp(Ctx &c) {
     const int64_t n = c.bcolor_ptr[k + 1] - c.bcolor_ptr[k];
     if (!n) continue;
     const size_t b = k % nbk;
-    PZG_TRY(launch_hp_bucket<false>(c, g, c.hp_bc_P[b], c.d_thp_bc.as<HpTables>() + b,
+    PZG_TRY(launch_hp_bucket<false>(c, g, c.hp_bc_P[b], 0, c.d_thp_bc.as<HpTables>() + b,
                                     c.d_hp_bsub.as<int32_t>() + b * HP_MAXSH,
                                     c.d_bcolor_list.as<int32_t>() + c.bcolor_ptr[k], n));
   }

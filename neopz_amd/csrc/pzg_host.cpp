@@ -264,6 +264,7 @@ static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
   for (int64_t v : c.bc_conn)
     if (v < 0 || v >= c.nconn) { set_error("pzg_mesh_set: connect index out of range"); return PZG_EINVAL; }
   c.hp_simplex = false;
+  c.hp_tuned = false;
   c.hp_order = 0;
   c.nconn_in = c.nconn;
   c.vol_side_real.clear();
@@ -681,6 +682,26 @@ int pzg_get_system(pzg_ctx *ctx, double *a, double *rhs) {
   if (a && c.nnz_u) PZG_HIP(hipMemcpyAsync(a, c.d_a.p, sizeof(double) * c.nnz_u, hipMemcpyDeviceToHost, c.stream));
   if (rhs) PZG_HIP(hipMemcpyAsync(rhs, c.d_rhs.p, sizeof(double) * c.neq, hipMemcpyDeviceToHost, c.stream));
   PZG_HIP(hipStreamSynchronize(c.stream));
+  return PZG_OK;
+}
+
+int pzg_hp_tuning(pzg_ctx *ctx, int32_t *n, int32_t *order, int32_t *variant, float *ms) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (!n) return PZG_EINVAL;
+  const int cap = *n;
+  if (!c.hp_simplex || !c.hp_tuned) {
+    *n = 0;
+    return PZG_OK;
+  }
+  const int nbk = (int)c.hp_vol_var.size();
+  *n = nbk;
+  for (int b = 0; b < nbk && b < cap; b++) {
+    if (order) order[b] = c.hp_vol_P[b];
+    if (variant) variant[b] = c.hp_vol_var[b];
+    if (ms)
+      for (int v = 0; v < 5; v++) ms[b * 5 + v] = c.hp_vol_ms[b * 5 + v];
+  }
   return PZG_OK;
 }
 

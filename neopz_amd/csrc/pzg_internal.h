@@ -146,6 +146,11 @@ struct Ctx {
   // virtual sides an element of that order can use (HP_MAXSH slots a bucket)
   std::vector<int> hp_vol_P, hp_bc_P;
   DevBuf d_hp_vsub, d_hp_bsub;
+  // volume kernel variant per bucket, picked by timing at the first assembly
+  // (hp_tune), and the measured milliseconds per launch of every variant
+  bool hp_tuned = false;
+  std::vector<int> hp_vol_var;
+  std::vector<float> hp_vol_ms;
 
   // connect -> active block index (-1 for size-0 connects)
   std::vector<int64_t> aidx;

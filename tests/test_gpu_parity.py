@@ -157,6 +157,27 @@ def test_full_csr_matches_oracle(name):
     assert np.linalg.norm(y - Kf @ x) <= 1e-12 * np.linalg.norm(Kf @ x)
 
 
+def test_hp_assembly_autotuner():
+    """The high-order assembly picks one kernel variant per rule bucket by
+    timing (the fastest of the five), the tuning launches leave no trace in the
+    system, and later assemblies reproduce the first bit for bit."""
+    pb = CASES["darcy_tet_hp5"]()
+    K, ia, ja, a1, r1 = _assemble(pb)
+    tun = L.hp_tuning(ctx())
+    assert len(tun) == len(set(M.build(pb)["vol_rule"]))
+    for t in tun:
+        ms = t["ms_per_variant"]
+        assert 1 <= t["order"] <= 5 and 0 <= t["variant"] < 5
+        assert all(m > 0 for m in ms) and ms[t["variant"]] == min(ms)
+    o, ia_r, ja_r, a_r, rhs_r = _oracle(pb)
+    _check_values(ia, a1, a_r)
+    L.check(ctx().lib.pzg_assemble(ctx().h))
+    a2 = np.zeros(len(ja))
+    r2 = np.zeros(len(ia) - 1)
+    L.check(ctx().lib.pzg_get_system(ctx().h, L.ptr(a2, L.dp), L.ptr(r2, L.dp)))
+    assert np.array_equal(a1, a2) and np.array_equal(r1, r2)
+
+
 def test_assembly_is_deterministic():
     pb = CASES["elast3d_p2_body"]()
     _, _, _, a1, r1 = _assemble(pb)
