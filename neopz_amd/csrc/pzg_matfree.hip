@@ -47,6 +47,9 @@ struct MfArgs {
   int64_t nvol;
   int nq;
   float nu;
+  // elasticity: C1 = E*lam1, C2 = E*lam2, C3 = E*lam3 (TPZElasticity3D.h:183-188)
+  float lam1, lam2, lam3;
+  int zblk;             // block whose pt entries are zero (pt holds one extra block)
   const MfTab *tab;
   const int32_t *blk;   // [nvol][NSH] active block of each shape (-1: none)
   const float *xe;      // [nvol][NC*3] corner coordinates
@@ -248,7 +251,8 @@ __global__ __launch_bounds__(MF_TPB) void k_mf_hex27_mfma(MfArgs a, const MfMfma
   __shared__ float4 s_cd[27][8];
   __shared__ float s_w[27];
   __shared__ float s_g[NW][EPT][27][NG];
-  __shared__ float s_x[NW][EPT][10];  // J^-1 (row-major) and |det J|; [9] < 0: not affine
+  // J^-1 (row-major), |det J| ([9] < 0: not affine), coefficient [10]
+  __shared__ float s_x[NW][EPT][11];
   __shared__ __align__(16) MfMfmaTab s_mt;
   const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, lr = l >> 4;
   for (int t = tid; t < 27 * 8; t += MF_TPB) s_cd[t / 8][t % 8] = a.tab->cd[t / 8][t % 8];
@@ -274,28 +278,37 @@ __global__ __launch_bounds__(MF_TPB) void k_mf_hex27_mfma(MfArgs a, const MfMfma
   const int64_t eend = a.nvol;
 #endif
   // B operands of phase 1 for the group starting at g: u[s][(e,k)], s = 4t + lr
+  // element, shape and output offsets fit in 32 bits (checked on the host).
+  // Branch-free: the seven block indices are loaded together (clamped
+  // addresses), then the seven inputs, absent ones from the zero block, so
+  // that each batch is one round trip to memory.
   auto gather = [&](int64_t g, float (&u)[7]) {
     const int64_t e = g + (int64_t)wv * EPT + ee;
     const bool ok = colv && e < eend;
+    const int eo = ok ? (int)e * 27 : 0;
+    int32_t bb[7];
 #pragma unroll
     for (int t = 0; t < 7; t++) {
       const int s = 4 * t + lr;
-      float v = 0.f;
-      if (ok && s < 27) {
-        const int32_t b = a.blk[e * 27 + s];
-        if (b >= 0) v = a.pt[(int64_t)b * NS + kc];
-      }
-      u[t] = v;
+      bb[t] = a.blk[eo + (s < 27 ? s : 26)];
+    }
+#pragma unroll
+    for (int t = 0; t < 7; t++) {
+      const int s = 4 * t + lr;
+      const int b = (ok && s < 27 && bb[t] >= 0) ? bb[t] : a.zblk;
+      u[t] = a.pt[b * NS + kc];
     }
   };
   float ubn[7];
   if (gfirst < eend) gather(gfirst, ubn);
   for (int64_t g0 = gfirst; g0 < eend; g0 += gstep) {
     const int64_t eb = g0 + (int64_t)wv * EPT;
-    for (int t = l; t < EPT * 10; t += 64) {
-      const int el = t / 10, k = t % 10;
+    for (int t = l; t < EPT * 11; t += 64) {
+      const int el = t / 11, k = t % 11;
       const int64_t e = eb + el;
-      s_x[wv][el][k] = e < eend ? a.geo[e * 10 + k] : 0.f;
+      const int ec = e < eend ? (int)e : 0;
+      const float v = k < 10 ? a.geo[ec * 10 + k] : a.coef[ec];
+      s_x[wv][el][k] = e < eend ? v : 0.f;
     }
     const int64_t e = eb + ee;
     const bool ev = colv && e < eend;
@@ -304,13 +317,15 @@ __global__ __launch_bounds__(MF_TPB) void k_mf_hex27_mfma(MfArgs a, const MfMfma
     for (int t = 0; t < 7; t++) ub[t] = ubn[t];
     // the next group's gathers are in flight while this group computes
     if (g0 + gstep < eend) gather(g0 + gstep, ubn);
+    // output positions (raw; validity applied at the stores)
     int32_t opos[2][4];
+    const int epo = ev ? (int)e * 27 : 0;
 #pragma unroll
     for (int r = 0; r < 2; r++)
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int sh = 16 * r + 4 * lr + i;
-        opos[r][i] = (ev && sh < 27) ? a.pos[e * 27 + sh] : -1;
+        opos[r][i] = a.pos[epo + (sh < 27 ? sh : 26)];
       }
     mf_f4 acc[3][2];
 #pragma unroll
@@ -376,7 +391,7 @@ __global__ __launch_bounds__(MF_TPB) void k_mf_hex27_mfma(MfArgs a, const MfMfma
         adet = fabsf(det);
       }
       const float wdet = s_w[q] * adet;
-      const float coef = a.coef[eb + el];
+      const float coef = xg[10];
       float *gp = s_g[wv][el][q];
       float g[NS][3];
 #pragma unroll
@@ -386,11 +401,7 @@ __global__ __launch_bounds__(MF_TPB) void k_mf_hex27_mfma(MfArgs a, const MfMfma
           g[k][d] = gp[k * 3 + 0] * inv[0][d] + gp[k * 3 + 1] * inv[1][d] + gp[k * 3 + 2] * inv[2][d];
       float S[NS][3];
       if constexpr (NS == 3) {
-        const float nu = a.nu;
-        const float C1 = coef / (2.f + 2.f * nu);
-        const float den = -1.f + nu + 2.f * nu * nu;
-        const float C2 = coef * nu / den;
-        const float C3 = coef * (nu - 1.f) / den;
+        const float C1 = coef * a.lam1, C2 = coef * a.lam2, C3 = coef * a.lam3;
         const float tr = g[0][0] + g[1][1] + g[2][2];
 #pragma unroll
         for (int k = 0; k < 3; k++)
@@ -424,7 +435,7 @@ __global__ __launch_bounds__(MF_TPB) void k_mf_hex27_mfma(MfArgs a, const MfMfma
 #pragma unroll
         for (int i = 0; i < 4; i++) {
           const int s = 16 * r + 4 * lr + i;
-          if (s < 27 && opos[r][i] >= 0) a.ye[(int64_t)opos[r][i] * NS + kc] = y[r][i];
+          if (s < 27 && opos[r][i] >= 0) a.ye[opos[r][i] * NS + kc] = y[r][i];  // ev: enclosing if
         }
     }
     __syncthreads();
@@ -698,7 +709,9 @@ int mf_setup(Ctx &c) {
   PZG_TRY(dev_upload(c.d_mf_blk, blk.data(), blk.size(), s));
   PZG_TRY(dev_upload(c.d_mf_xe, xe.data(), xe.size(), s));
   PZG_TRY(dev_upload(c.d_mf_geo, geo.data(), geo.size(), s));
-  PZG_TRY(dev_alloc(c.d_mf_pt, sizeof(float) * std::max<int64_t>(c.neq, 1)));
+  // pt: nb blocks + one zero block read for absent shapes
+  PZG_TRY(dev_alloc(c.d_mf_pt, sizeof(float) * (c.nb + 1) * ns));
+  PZG_HIP(hipMemsetAsync(c.d_mf_pt.p, 0, sizeof(float) * (c.nb + 1) * ns, s));
   PZG_TRY(dev_upload(c.d_mf_bblk, bblk.data(), bblk.size(), s));
   PZG_TRY(dev_upload(c.d_mf_bm, bm.data(), bm.size(), s));
   PZG_TRY(dev_upload(c.d_mf_ptr, ptr.data(), ptr.size(), s));
@@ -754,6 +767,13 @@ int mf_launch_vol(Ctx &c, const SolveScalars *sc) {
   a.nvol = c.nvol;
   a.nq = c.tvol.nq;
   a.nu = (float)c.nu;
+  {
+    const double nu = c.nu, den = -1. + nu + 2. * nu * nu;
+    a.lam1 = (float)(1. / (2. + 2. * nu));
+    a.lam2 = (float)(nu / den);
+    a.lam3 = (float)((nu - 1.) / den);
+  }
+  a.zblk = (int)c.nb;
   a.tab = c.d_mf_tab.as<MfTab>();
   a.blk = c.d_mf_blk.as<int32_t>();
   a.xe = c.d_mf_xe.as<float>();
@@ -765,6 +785,11 @@ int mf_launch_vol(Ctx &c, const SolveScalars *sc) {
   const int nsh = c.tvol.nsh;
   if (c.nvol > 0) {
     const bool mfma = PZG_MF_MFMA && c.dim == 3 && nsh == 27 && c.tvol.nq == 27;
+    if (mfma && (c.nvol * 27 >= INT32_MAX || c.mf_nout * c.nstate >= INT32_MAX ||
+                 c.nb * c.nstate >= INT32_MAX)) {
+      set_error("matrix-free operator: mesh too large for 32-bit element offsets");
+      return PZG_EUNSUP;
+    }
     if (mfma) {
       // persistent grid: as many workgroups as fit on the device at once
       int64_t groups = (c.nvol + 4 * (16 / c.nstate) - 1) / (4 * (16 / c.nstate));
