@@ -299,16 +299,30 @@ __global__ __launch_bounds__(MF_TPB) void k_mf_hex27_mfma(MfArgs a, const MfMfma
       u[t] = a.pt[b * NS + kc];
     }
   };
-  float ubn[7];
-  if (gfirst < eend) gather(gfirst, ubn);
+  // per-element geometry and coefficient of a group: entry t = l + 64*h
+  // (t < EPT*11) is entry t % 11 of element t / 11 (absent elements read
+  // element 0)
+  constexpr int XG = (EPT * 11 + 63) / 64;
+  auto geo_load = [&](int64_t g, float (&xv)[XG]) {
+#pragma unroll
+    for (int h = 0; h < XG; h++) {
+      const int t = l + 64 * h, el = t / 11, k = t % 11;
+      const int64_t e = g + (int64_t)wv * EPT + el;
+      const int ec = (t < EPT * 11 && e < eend) ? (int)e : 0;
+      xv[h] = k < 10 ? a.geo[ec * 10 + k] : a.coef[ec];
+    }
+  };
+  float ubn[7], xgn[XG];
+  if (gfirst < eend) {
+    gather(gfirst, ubn);
+    geo_load(gfirst, xgn);
+  }
   for (int64_t g0 = gfirst; g0 < eend; g0 += gstep) {
     const int64_t eb = g0 + (int64_t)wv * EPT;
-    for (int t = l; t < EPT * 11; t += 64) {
-      const int el = t / 11, k = t % 11;
-      const int64_t e = eb + el;
-      const int ec = e < eend ? (int)e : 0;
-      const float v = k < 10 ? a.geo[ec * 10 + k] : a.coef[ec];
-      s_x[wv][el][k] = e < eend ? v : 0.f;
+#pragma unroll
+    for (int h = 0; h < XG; h++) {
+      const int t = l + 64 * h, el = t / 11;
+      if (t < EPT * 11) s_x[wv][el][t % 11] = eb + el < eend ? xgn[h] : 0.f;
     }
     const int64_t e = eb + ee;
     const bool ev = colv && e < eend;
@@ -316,7 +330,10 @@ __global__ __launch_bounds__(MF_TPB) void k_mf_hex27_mfma(MfArgs a, const MfMfma
 #pragma unroll
     for (int t = 0; t < 7; t++) ub[t] = ubn[t];
     // the next group's gathers are in flight while this group computes
-    if (g0 + gstep < eend) gather(g0 + gstep, ubn);
+    if (g0 + gstep < eend) {
+      gather(g0 + gstep, ubn);
+      geo_load(g0 + gstep, xgn);
+    }
     // output positions (raw; validity applied at the stores)
     int32_t opos[2][4];
     const int epo = ev ? (int)e * 27 : 0;
