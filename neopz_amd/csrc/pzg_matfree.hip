@@ -506,19 +506,47 @@ __global__ __launch_bounds__(MF_TPB) void k_mf_gather(int64_t nb, const int64_t 
   for (int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; A < nb;
        A += (int64_t)gridDim.x * blockDim.x) {
 #endif
-    float acc[NS];
-    for (int k = 0; k < NS; k++) acc[k] = 0.f;
-    const int64_t t1 = ptr[A + 1];
-    for (int64_t t = ptr[A]; t < t1; t++)
-      for (int k = 0; k < NS; k++) acc[k] += ye[t * NS + k];
+    // every load of the row is issued before the first use: the vectors, then
+    // the row's first MF_GE entries (clamped addresses, masked sums in entry
+    // order), entries beyond MF_GE in a loop
+    constexpr int MF_GE = 8;
+    const int64_t t0 = ptr[A], n = ptr[A + 1] - t0;
+    // without partials r may be absent: read p in its place (unused)
+    const float *rs = partials ? r : p;
+    float dv[NS], pv[NS], rv[NS];
+#pragma unroll
     for (int k = 0; k < NS; k++) {
       const int64_t i = A * NS + k;
-      const float qi = acc[k] * dis[i];
-      q[i] = qi;
+      dv[k] = dis[i];
+      pv[k] = p[i];
+      rv[k] = rs[i];
+    }
+    float v[MF_GE][NS];
+#pragma unroll
+    for (int j = 0; j < MF_GE; j++) {
+      const int64_t t = t0 + (j < n ? j : 0);
+#pragma unroll
+      for (int k = 0; k < NS; k++) v[j][k] = ye[t * NS + k];
+    }
+    // keep every use below the loads (one wait for the whole batch)
+    __builtin_amdgcn_sched_barrier(0);
+    float acc[NS];
+#pragma unroll
+    for (int k = 0; k < NS; k++) acc[k] = 0.f;
+#pragma unroll
+    for (int j = 0; j < MF_GE; j++)
+      if (j < n)
+#pragma unroll
+        for (int k = 0; k < NS; k++) acc[k] += v[j][k];
+    for (int64_t t = t0 + MF_GE; t < t0 + n; t++)
+      for (int k = 0; k < NS; k++) acc[k] += ye[t * NS + k];
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+      const float qi = acc[k] * dv[k];
+      q[A * NS + k] = qi;
       if (partials) {
-        const float pi = p[i];
-        pq += (double)qi * pi;
-        pr += (double)pi * r[i];
+        pq += (double)qi * pv[k];
+        pr += (double)pv[k] * rv[k];
       }
     }
   }
