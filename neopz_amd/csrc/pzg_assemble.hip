@@ -283,8 +283,20 @@ __global__ __launch_bounds__(TPB) void k_vol(ElemArgs g, const int32_t *__restri
     C2 = E * nu / (-1. + nu + 2. * nu * nu);
     C3 = E * (nu - 1.) / (-1. + nu + 2. * nu * nu);
   }
-  // element matrix entries, upper part by global equation, scatter
-  for (int k = tid; k < NE * NE; k += TPB) {
+  // element matrix entries, upper part by global equation, scatter; SU
+  // entries per thread at a time, their read-modify-writes issued together
+  constexpr int SU = 8;
+  for (int k0 = tid; k0 < NE * NE; k0 += TPB * SU) {
+   int64_t slot[SU];
+   double val[SU];
+   bool on[SU];
+#pragma unroll
+   for (int u = 0; u < SU; u++) {
+    on[u] = false;
+    slot[u] = 0;
+    val[u] = 0.;
+    const int k = k0 + u * TPB;
+    if (k >= NE * NE) continue;
     int I = k / NE, J = k % NE;
     int i = I / NS, ci = I % NS, j = J / NS, cj = J % NS;
     int64_t bi = s_blk[i], bj = s_blk[j];
@@ -309,13 +321,20 @@ __global__ __launch_bounds__(TPB) void k_vol(ElemArgs g, const int32_t *__restri
       v = coef * s_P[0][i][j];
     }
     if (fabs(v) < 1.e-12) continue;  // IsZero drop rule of AddKel
-    int64_t slot;
-    if (bi == bj) slot = s_row[I] + (cj - ci);
+    if (bi == bj) slot[u] = s_row[I] + (cj - ci);
     else {
       int t = s_map[T.shape_side[i] * NSD + T.shape_side[j]];
-      slot = s_row[I] + g.bs + (int64_t)t * g.bs - ci + cj;
+      slot[u] = s_row[I] + g.bs + (int64_t)t * g.bs - ci + cj;
     }
-    g.a[slot] += v;
+    val[u] = v;
+    on[u] = true;
+   }
+   double old[SU];
+#pragma unroll
+   for (int u = 0; u < SU; u++) old[u] = g.a[slot[u]];  // slot 0 for inactive entries
+#pragma unroll
+   for (int u = 0; u < SU; u++)
+     if (on[u]) g.a[slot[u]] = old[u] + val[u];
   }
   // load vector
   for (int I = tid; I < NE; I += TPB) {
