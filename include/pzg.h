@@ -253,6 +253,72 @@ int pzg_hp_tuning(pzg_ctx *ctx, int32_t *n, int32_t *order, int32_t *variant, fl
  * solver's inner iterations, which has no reference counterpart. */
 int pzg_scaled_apply(pzg_ctx *ctx, int32_t op, const double *x, double *y);
 
+/* ---- post-processing: solution and error norms -------------------------- */
+
+/* Exact solutions for pzg_evaluate_error.  PZG_EXACT_POINTS: the caller
+ * evaluates its own exact-solution function (the std::function of
+ * TPZMatError::SetExactSol, Material/TPZMatError.h:35-48) at the points that
+ * pzg_error_points returns and hands over the values.  The other kinds are
+ * evaluated on the device: scalar solutions of TLaplaceExample1::uxy
+ * (Pre/TPZAnalyticSolution.cpp:1254-1310) with fCenter = center over the
+ * mesh dimension d, and their exact gradients:
+ *   CONST  u = constant (TLaplaceExample1::gC)
+ *   X      u = x_0 - c_0
+ *   SINSIN u = prod_{i<d} sin(pi (x_i - c_i))
+ *   COSCOS u = prod_{i<d} cos(pi (x_i - c_i) / 2) */
+#define PZG_EXACT_POINTS 0
+#define PZG_EXACT_CONST 1
+#define PZG_EXACT_X 2
+#define PZG_EXACT_SINSIN 3
+#define PZG_EXACT_COSCOS 4
+
+typedef struct {
+  int32_t kind;       /* PZG_EXACT_*                                          */
+  int32_t porder;     /* polynomial order given with the exact solution
+                         (TPZMatError::fExactPOrder): raises the element rules */
+  double center[3];   /* device kinds: TLaplaceExample1::fCenter              */
+  double constant;    /* PZG_EXACT_CONST: the value                           */
+  int64_t npts;       /* PZG_EXACT_POINTS: number of points (pzg_error_points) */
+  const double *u;    /* PZG_EXACT_POINTS: [npts][nstate] exact values        */
+  const double *du;   /* PZG_EXACT_POINTS: [npts][dim][nstate] derivatives,
+                         du[q][i][j] = d u_j / d x_i (the layout of dsol and of
+                         the exact function's deriv matrix)                  */
+} pzg_exact;
+
+/* Replaces the device solution used by post-processing with u[neq] (host).
+ * TPZCompMesh::LoadSolution (Mesh/pzcmesh.cpp:524-540); pzg_solve leaves its
+ * solution loaded already (TPZLinearAnalysis::Solve calls LoadSolution). */
+int pzg_load_solution(pzg_ctx *ctx, const double *u);
+
+/* Integration points of the error norms of every volume element for an exact
+ * solution of order porder: each element's rule is raised to
+ * max(its rule order, porder), or to the rule family's highest order when
+ * porder reaches it (TPZInterpolationSpace::EvaluateError,
+ * Mesh/pzinterpolationspace.cpp:1140-1160; highest orders 36 line/quad/hex,
+ * 21 triangle, 14 tetrahedron, Integral/pzquad.h, tpzintrulet.h:72,
+ * tpzintrulet3d.h:72).  *npts receives the number of points; xq (NULL or
+ * [npts][3]) their physical coordinates, element by element in element
+ * order, points in rule order. */
+int pzg_error_points(pzg_ctx *ctx, int32_t porder, int64_t *npts, double *xq);
+
+/* Error norms of the loaded solution against an exact solution.
+ * TPZAnalysis::PostProcessError -> TPZCompMesh::EvaluateError
+ * (Analysis/TPZAnalysis.cpp:625-648, Mesh/pzcmesh.cpp:1497-1552) ->
+ * TPZInterpolationSpace::EvaluateError (Mesh/pzinterpolationspace.cpp:
+ * 1108-1207) with the material's Errors at every point:
+ *   TPZMatPoisson::Errors (Material/Poisson/TPZMatPoisson.cpp:173-218),
+ *     3 values: H1, L2, H1 semi-norm;
+ *   TPZDarcyFlow::Errors (Material/DarcyFlow/TPZDarcyFlow.cpp:268-326),
+ *     6 values: energy, L2, semi-norm, flux components;
+ *   TPZElasticity3D::Errors (Material/Elasticity/TPZElasticity3D.cpp:
+ *     1081-1133), 3 values: energy, L2, H1 semi-norm.
+ * el_errors (NULL or [nvol][nerr]): per volume element the square roots of
+ * the integrated values (TPZCompMesh::ElementSolution with store_error);
+ * errors (NULL or [nerr]): the mesh totals sqrt(sum_e el^2); *nerr (may be
+ * NULL) receives the number of values. */
+int pzg_evaluate_error(pzg_ctx *ctx, const pzg_exact *exact, double *el_errors, double *errors,
+                       int32_t *nerr);
+
 /* Sizes of the device formats (for roofline accounting). */
 typedef struct {
   int64_t neq, nnz_upper, nnz_full, nblocks, nnz_blocks, nvol, ncolors;

@@ -15,6 +15,7 @@ PZG_OK, PZG_EINVAL, PZG_EHIP, PZG_ENOCONV, PZG_ESTATE, PZG_ENOMEM, PZG_EUNSUP = 
 SOLVER_PCG64, SOLVER_MIXED = 0, 1
 PRECOND_JACOBI, PRECOND_BLOCK_JACOBI, PRECOND_LEARNED = 0, 1, 2
 INNER_AUTO, INNER_BSR, INNER_MATFREE = 0, 1, 2
+EXACT_POINTS, EXACT_CONST, EXACT_X, EXACT_SINSIN, EXACT_COSCOS = range(5)
 
 EXPORTS = [
     "pzg_last_error", "pzg_create", "pzg_destroy", "pzg_mesh_set", "pzg_struct_create",
@@ -23,6 +24,7 @@ EXPORTS = [
     "pzg_get_sizes", "pzg_sync", "pzg_mesh_set_coef", "pzg_learned_setup", "pzg_learned_collect",
     "pzg_learned_train", "pzg_learned_weights", "pzg_learned_graph", "pzg_learned_samples",
     "pzg_learned_loss_grad", "pzg_scaled_apply", "pzg_full_csr", "pzg_hp_tuning",
+    "pzg_load_solution", "pzg_error_points", "pzg_evaluate_error",
 ]
 
 dp = C.POINTER(C.c_double)
@@ -73,6 +75,13 @@ class Sizes(C.Structure):
     ]
 
 
+class Exact(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("porder", C.c_int32), ("center", C.c_double * 3),
+        ("constant", C.c_double), ("npts", C.c_int64), ("u", dp), ("du", dp),
+    ]
+
+
 class PzgError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"pzg error {code}: {msg}")
@@ -120,6 +129,9 @@ def load():
     L.pzg_scaled_apply.argtypes = [vp, C.c_int32, dp, dp]
     L.pzg_full_csr.argtypes = [vp, i64p, i64p, dp]
     L.pzg_hp_tuning.argtypes = [vp, i32p, i32p, i32p, fp]
+    L.pzg_load_solution.argtypes = [vp, dp]
+    L.pzg_error_points.argtypes = [vp, C.c_int32, i64p, dp]
+    L.pzg_evaluate_error.argtypes = [vp, C.POINTER(Exact), dp, dp, i32p]
     _lib = L
     return L
 

@@ -311,6 +311,22 @@ class TPZSpStructMatrixHIP(TPZSSpStructMatrixHIP):
         return TPZFYsmpMatrixHIP(self.ctx, K.neq)
 
 
+class TLaplaceExample1:
+    """Scalar exact solutions evaluated on the device: the EConst, EX, ESinSin
+    and ECosCos cases of TLaplaceExample1::uxy (Pre/TPZAnalyticSolution.cpp:
+    1254-1310) with fCenter = center (gC = constant for EConst)."""
+
+    EConst, EX, ESinSin, ECosCos = L.EXACT_CONST, L.EXACT_X, L.EXACT_SINSIN, L.EXACT_COSCOS
+
+    def __init__(self, kind, center=(0.0, 0.0, 0.0), constant=1.0):
+        if kind not in (self.EConst, self.EX, self.ESinSin, self.ECosCos):
+            raise ValueError("TLaplaceExample1: EConst, EX, ESinSin or ECosCos")
+        self.fExact, self.fCenter, self.gC = kind, tuple(center), float(constant)
+
+    def ExactSolution(self):
+        return self
+
+
 class TPZLinearAnalysis:
     """TPZLinearAnalysis(cmesh, mustOptimizeBandwidth=false) over the GPU path."""
 
@@ -355,3 +371,82 @@ class TPZLinearAnalysis:
 
     def Solution(self):
         return self.fSolution
+
+    # --- post-processing ------------------------------------------------------
+    def _mesh_on_device(self):
+        # the mesh is flattened to the device by the struct matrix (Assemble);
+        # post-processing a loaded solution needs it without an assembly
+        if self.fStructMatrix is None:
+            self.fStructMatrix = TPZSSpStructMatrixHIP(self.ctx, self.mesh)
+        self.fStructMatrix._flatten()
+
+    def LoadSolution(self, sol=None):
+        """TPZLinearAnalysis::LoadSolution -> TPZCompMesh::LoadSolution
+        (Mesh/pzcmesh.cpp:524-540): the solution the error norms use.  Solve
+        loads its solution already; sol replaces it."""
+        if sol is not None:
+            self._mesh_on_device()
+            sol = np.ascontiguousarray(sol, dtype=np.float64)
+            if sol.shape != (self.mesh["neq"],):
+                raise ValueError("LoadSolution: solution of the wrong length")
+            L.check(self.ctx.lib.pzg_load_solution(self.ctx.h, L.ptr(sol, L.dp)))
+            self.fSolution = sol
+
+    def SetExact(self, f, p):
+        """TPZAnalysis::SetExact (Analysis/TPZAnalysis.cpp:655-676).  f is a
+        TLaplaceExample1 (evaluated on the device) or a vectorised exact
+        solution f(x[npts, 3]) -> (u[npts, nstate], du[npts, dim, nstate]),
+        du[q, i, j] = d u_j / d x_i, evaluated on the host at the error rule's
+        points."""
+        self.fExact, self.fExactOrder = f, int(p)
+
+    def ErrorPoints(self):
+        """Physical coordinates [npts, 3] of the error rule's points."""
+        self._mesh_on_device()
+        n = C.c_int64(0)
+        L.check(self.ctx.lib.pzg_error_points(self.ctx.h, self.fExactOrder, C.byref(n), None))
+        xq = np.zeros((n.value, 3))
+        L.check(self.ctx.lib.pzg_error_points(self.ctx.h, self.fExactOrder, C.byref(n),
+                                              L.ptr(xq, L.dp)))
+        return xq
+
+    def PostProcessError(self, store_error=False):
+        """TPZAnalysis::PostProcessError (Analysis/TPZAnalysis.cpp:480-487,
+        625-648): the mesh totals of the material's error norms; with
+        store_error the per-element values go to ElementSolution()."""
+        f = getattr(self, "fExact", None)
+        if f is None:
+            raise ValueError("PostProcessError: no exact solution (SetExact)")
+        self._mesh_on_device()
+        ex = L.Exact()
+        ex.porder = self.fExactOrder
+        keep = []
+        if isinstance(f, TLaplaceExample1):
+            ex.kind = f.fExact
+            for d in range(3):
+                ex.center[d] = f.fCenter[d]
+            ex.constant = f.gC
+        else:
+            xq = self.ErrorPoints()
+            u, du = f(xq)
+            ns, dim = self.mesh["nstate"], self.mesh["dim"]
+            u = np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(len(xq), ns))
+            du = np.ascontiguousarray(np.asarray(du, dtype=np.float64).reshape(len(xq), dim, ns))
+            keep += [u, du]
+            ex.kind, ex.npts = L.EXACT_POINTS, len(xq)
+            ex.u, ex.du = L.ptr(u, L.dp), L.ptr(du, L.dp)
+        nvol = len(self.mesh["vol_nodes"])
+        el = np.zeros((nvol, 6))
+        tot = np.zeros(6)
+        nerr = C.c_int32(0)
+        L.check(self.ctx.lib.pzg_evaluate_error(self.ctx.h, C.byref(ex), L.ptr(el, L.dp),
+                                                L.ptr(tot, L.dp), C.byref(nerr)))
+        k = nerr.value
+        if store_error:
+            self.fElementSolution = el.reshape(-1)[: nvol * k].reshape(nvol, k).copy()
+        return tot[:k].copy()
+
+    def ElementSolution(self):
+        """TPZCompMesh::ElementSolution: per volume element the error norms of
+        the last PostProcessError(store_error=True)."""
+        return self.fElementSolution

@@ -440,35 +440,7 @@ __global__ __launch_bounds__(TPB) void k_bc(ElemArgs g, const int32_t *__restric
 // col_off[i] + (transform variant of its real side, from the corner ids).
 // The Gram matrix sum_q w|J| grad phi_i . grad phi_j is accumulated over
 // chunks of QC quadrature points, each thread keeping its own (i, j) entries.
-
-// transform id of a triangle from its corner ids (TPZShapeTriang::
-// GetTransformId2dT, Shape/pzshapetriang.cpp:410-431)
-__device__ inline int tri_tid(int64_t i0, int64_t i1, int64_t i2) {
-  const int a = i0 < i1 ? 0 : 1;
-  const int64_t ia = a ? i1 : i0;
-  const int mn = i2 < ia ? 2 : a;
-  const int64_t n1 = mn == 0 ? i1 : mn == 1 ? i2 : i0;
-  const int64_t n2 = mn == 0 ? i2 : mn == 1 ? i0 : i1;
-  return 2 * mn + (n1 < n2 ? 0 : 1);
-}
-
-// variant of side s (TPZTetrahedron / TPZTriangle::GetTransformId,
-// Topology/tpztetrahedron.cpp:1128-1168, Topology/tpztriangle.cpp:629-650)
-__device__ inline int hp_variant(bool tet, int s, const int64_t *id) {
-  if (tet) {
-    if (s < 4 || s == 14) return 0;
-    if (s < 10) {
-      const int a = s < 7 ? s - 4 : s - 7, b = s < 7 ? (s - 3) % 3 : 3;
-      return id[a] < id[b] ? 0 : 1;
-    }
-    const int f = s - 10;
-    const int f0 = f == 2 ? 1 : 0, f1 = f == 0 || f == 1 ? 1 : 2, f2 = f == 0 ? 2 : 3;
-    return tri_tid(id[f0], id[f1], id[f2]);
-  }
-  if (s < 3) return 0;
-  if (s < 6) return id[s - 3] < id[(s - 2) % 3] ? 0 : 1;
-  return tri_tid(id[0], id[1], id[2]);
-}
+// (tri_tid / hp_variant: pzg_internal.h)
 
 // Shapes of one rule bucket: the sub-list sub[0..NSUB) of the element's NSV
 // virtual sides (the shapes an element of the bucket's highest order can

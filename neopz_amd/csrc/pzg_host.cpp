@@ -65,7 +65,12 @@ static void free_all(Ctx &c) {
                     &c.d_nn_X, &c.d_nn_T, &c.d_nn_xi, &c.d_nn_zi, &c.d_nn_h0, &c.d_nn_h1,
                     &c.d_nn_mi, &c.d_nn_trH, &c.d_nn_trM, &c.d_nn_trP, &c.d_nn_dH, &c.d_nn_G,
                     &c.d_nn_tmp, &c.d_nn_Z, &c.d_nn_dZ, &c.d_nn_den, &c.d_nn_part,
-                    &c.d_nn_loss, &c.d_nn_ystar, &c.d_nn_e};
+                    &c.d_nn_loss, &c.d_nn_ystar, &c.d_nn_e, &c.d_post_tab, &c.d_post_list, &c.d_post_slot,
+                    &c.d_post_off, &c.d_post_out, &c.d_post_uq, &c.d_post_duq, &c.d_post_xq,
+                    &c.d_hp_vsub, &c.d_hp_bsub, &c.d_thp_vol, &c.d_thp_bc, &c.d_mf_blk,
+                    &c.d_mf_xe, &c.d_mf_bblk, &c.d_mf_bm, &c.d_mf_ptr, &c.d_mf_pos, &c.d_mf_bpos,
+                    &c.d_mf_tab, &c.d_mf_ye, &c.d_mf_coef, &c.d_mf_dis, &c.d_mf_mfma, &c.d_mf_geo,
+                    &c.d_mf_pt};
   for (DevBuf *b : bufs) dev_free(*b);
 }
 
@@ -417,6 +422,7 @@ static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
   c.assembled = false;
   c.external_csr = false;
   c.has_solution = false;
+  c.post_porder = -1;
   return PZG_OK;
 }
 
@@ -947,6 +953,38 @@ int pzg_true_residual(pzg_ctx *ctx, double *rel) {
     return PZG_ESTATE;
   }
   return true_residual(c, c.d_u.as<double>(), rel);
+}
+
+int pzg_load_solution(pzg_ctx *ctx, const double *u) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (!c.has_mesh || !u) {
+    set_error(!u ? "pzg_load_solution: null solution" : "pzg_load_solution: no mesh");
+    return !u ? PZG_EINVAL : PZG_ESTATE;
+  }
+  PZG_TRY(dev_alloc(c.d_u, c.neq * sizeof(double)));
+  PZG_HIP(hipMemcpyAsync(c.d_u.p, u, c.neq * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  PZG_HIP(hipStreamSynchronize(c.stream));
+  c.has_solution = true;
+  return PZG_OK;
+}
+
+int pzg_error_points(pzg_ctx *ctx, int32_t porder, int64_t *npts, double *xq) {
+  CTX_CHECK();
+  return post_points(ctx->c, porder, npts, xq);
+}
+
+int pzg_evaluate_error(pzg_ctx *ctx, const pzg_exact *exact, double *el_errors, double *errors,
+                       int32_t *nerr) {
+  CTX_CHECK();
+  if (!exact) {
+    set_error("pzg_evaluate_error: null exact solution");
+    return PZG_EINVAL;
+  }
+  int n = 0;
+  PZG_TRY(post_errors(ctx->c, *exact, el_errors, errors, &n));
+  if (nerr) *nerr = n;
+  return PZG_OK;
 }
 
 int pzg_get_sizes(pzg_ctx *ctx, pzg_sizes *s) {

@@ -87,6 +87,48 @@ struct HpTables {
   double dphi[HP_MAXQ][3][HP_MAXCOL];
 };
 
+// Tables of an error integration rule (post-processing), for kernels with
+// lanes over quadrature points: every array has the point index fastest.
+// d = w[nq] | phi[ncol][nq] | dphi[dim][ncol][nq] | cphi[nc][nq] |
+// cdphi[dim][nc][nq] (cphi: corner functions of the geometric map).  Columns
+// are the shapes of order p (p <= 2, column = side) or, for high-order
+// simplices, one column per shape and side variant (col_off[i] + variant).
+struct PostTables {
+  int etype = 0, dim = 0, nc = 0, nq = 0, ncol = 0, nsh = 0;
+  int col_off[HP_MAXSH] = {}, shape_side[HP_MAXSH] = {};
+  std::vector<double> d;
+};
+
+// Transform id of a triangle from its three corner ids
+// (TPZShapeTriang::GetTransformId2dT, Shape/pzshapetriang.cpp:410-431).
+__host__ __device__ inline int tri_tid(int64_t i0, int64_t i1, int64_t i2) {
+  const int a = i0 < i1 ? 0 : 1;
+  const int64_t ia = a ? i1 : i0;
+  const int mn = i2 < ia ? 2 : a;
+  const int64_t n1 = mn == 0 ? i1 : mn == 1 ? i2 : i0;
+  const int64_t n2 = mn == 0 ? i2 : mn == 1 ? i0 : i1;
+  return 2 * mn + (n1 < n2 ? 0 : 1);
+}
+
+// Variant of side s of a tetrahedron / triangle with corner ids id[]
+// (TPZTetrahedron / TPZTriangle::GetTransformId, Topology/tpztetrahedron.cpp:
+// 1128-1168, Topology/tpztriangle.cpp:629-650).
+__host__ __device__ inline int hp_variant(bool tet, int s, const int64_t *id) {
+  if (tet) {
+    if (s < 4 || s == 14) return 0;
+    if (s < 10) {
+      const int a = s < 7 ? s - 4 : s - 7, b = s < 7 ? (s - 3) % 3 : 3;
+      return id[a] < id[b] ? 0 : 1;
+    }
+    const int f = s - 10;
+    const int f0 = f == 2 ? 1 : 0, f1 = f == 0 || f == 1 ? 1 : 2, f2 = f == 0 ? 2 : 3;
+    return tri_tid(id[f0], id[f1], id[f2]);
+  }
+  if (s < 3) return 0;
+  if (s < 6) return id[s - 3] < id[(s - 2) % 3] ? 0 : 1;
+  return tri_tid(id[0], id[1], id[2]);
+}
+
 // Full block CSR values (FP64 K, FP32 S): the blocks of a block row come in
 // chunks of BSR_CHUNK, component-major inside a chunk, so that the BSR_CHUNK
 // lanes that share a block row read consecutive addresses with every load.
@@ -207,6 +249,16 @@ struct Ctx {
   DevBuf d_nn_trH, d_nn_trM, d_nn_trP, d_nn_dH, d_nn_G, d_nn_tmp, d_nn_Z, d_nn_dZ;
   DevBuf d_nn_den, d_nn_part, d_nn_loss;
   DevBuf d_nn_ystar, d_nn_e;                      // pair generation
+
+  // post-processing (pzg_post.hip): error-rule tables per rule bucket for the
+  // exact-solution order post_porder (-1: not built), the elements of every
+  // bucket, and per element the offset of its first point
+  int post_porder = -1;
+  int64_t post_npts = 0;
+  std::vector<PostTables> post_tab;
+  std::vector<int64_t> post_tab_off, post_lptr;
+  DevBuf d_post_tab, d_post_list, d_post_slot, d_post_off, d_post_out, d_post_uq, d_post_duq,
+      d_post_xq;
 };
 
 // ---- element tables (pzg_shape.cpp) ----------------------------------------
@@ -220,6 +272,9 @@ int simplex_nshape(int etype, int s, int p);
 int etype_ncorner(int t);
 int etype_nsides(int t);
 int gauss_legendre(int order, double *loc, double *w);
+// error integration rule order and its tables (post-processing)
+int post_rule_order(int etype, int rule, int porder);
+int build_post_tables(int etype, int p, bool hp, int rule_order, PostTables &t);
 
 // ---- kernels launchers (pzg_assemble.hip / pzg_solve.hip) --------------------
 int launch_expand_upper_ja(Ctx &c);
@@ -262,5 +317,8 @@ inline bool mf_available(const Ctx &c) {
 int solve_pcg64(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info);
 int solve_mixed(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info);
 int true_residual(Ctx &c, const double *u, double *rel);
+// post-processing of the device solution (pzg_post.hip)
+int post_points(Ctx &c, int porder, int64_t *npts, double *xq);
+int post_errors(Ctx &c, const pzg_exact &ex, double *el_errors, double *errors, int *nerr);
 
 }  // namespace pzg

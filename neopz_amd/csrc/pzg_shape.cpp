@@ -13,6 +13,7 @@
 // directions of 4*l0*l1 (directions the side spans) or l0 / l1 (directions in
 // which the side is fixed at -1 / +1), l0 = (1-xi)/2, l1 = (1+xi)/2.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -379,7 +380,7 @@ static int eval_simplex_hp(int etype, int P, const int64_t *ids, const double *x
 // the Jacobi matrix of the monic recurrence (Golub-Welsch) by cyclic Jacobi
 // rotations; points ascending.
 static void gauss_jacobi(int n, double alpha, double *t, double *w) {
-  double A[8][8] = {}, V[8][8] = {};
+  double A[16][16] = {}, V[16][16] = {};
   for (int i = 0; i < n; i++) V[i][i] = 1.;
   for (int k = 0; k < n; k++) {
     const double s = 2. * k + alpha;
@@ -415,7 +416,7 @@ static void gauss_jacobi(int n, double alpha, double *t, double *w) {
         }
       }
   }
-  int idx[8];
+  int idx[16];
   for (int i = 0; i < n; i++) idx[i] = i;
   std::sort(idx, idx + n, [&](int a, int b) { return A[a][a] < A[b][b]; });
   const double mu0 = std::pow(2., alpha + 1.) / (alpha + 1.);
@@ -437,8 +438,8 @@ static int simplex_rule_hp(int etype, int order, double (*pts)[3], double *w, in
   const int n = (order + 2) / 2, dim = etype == PZG_TRI ? 2 : 3;
   int tot = 1;
   for (int d = 0; d < dim; d++) tot *= n;
-  if (n > 8 || tot > cap) return -1;
-  double ta[8], wa[8], tb[8], wb[8], tc[8], wc[8];
+  if (n > 16 || tot > cap) return -1;
+  double ta[16], wa[16], tb[16], wb[16], tc[16], wc[16];
   gauss_jacobi(n, 0., ta, wa);
   gauss_jacobi(n, 1., tb, wb);
   gauss_jacobi(n, 2., tc, wc);
@@ -457,6 +458,27 @@ static int simplex_rule_hp(int etype, int order, double (*pts)[3], double *w, in
         }
       }
   return q;
+}
+
+// Corner ids that realise variant v of side s: edge v = 0 <-> id_a < id_b;
+// face: the id ranking whose transform id is v.
+static int variant_ids(int etype, int s, int v, int64_t ids[4]) {
+  int c[8];
+  const int nl = side_corner_list(etype, s, c);
+  for (int k = 0; k < 4; k++) ids[k] = 10 + k;
+  if (nl == 2) {
+    ids[c[0]] = v == 0 ? 0 : 1;
+    ids[c[1]] = v == 0 ? 1 : 0;
+  } else if (nl == 3) {
+    int perm[3] = {0, 1, 2};
+    bool found = false;
+    do {
+      if (tri_transform_id(perm[0], perm[1], perm[2]) == v) { found = true; break; }
+    } while (std::next_permutation(perm, perm + 3));
+    if (!found) return PZG_EINVAL;
+    for (int k = 0; k < 3; k++) ids[c[k]] = perm[k];
+  }
+  return PZG_OK;
 }
 
 int build_hp_tables(int etype, int P, int rule_order, HpTables &t) {
@@ -490,26 +512,11 @@ int build_hp_tables(int etype, int P, int rule_order, HpTables &t) {
     t.w[q] = w[q];
     for (int d = 0; d < 3; d++) t.pt[q][d] = pts[q][d];
   }
-  // corner ids that realise variant v of side s: edge v = 0 <-> id_a < id_b;
-  // face: the id ranking whose transform id is v
   for (int s = 0; s < etype_nsides(etype); s++) {
     const int nv = side_nvariants(etype, s);
-    int c[8];
-    const int nl = side_corner_list(etype, s, c);
     for (int v = 0; v < nv; v++) {
-      int64_t ids[4] = {10, 11, 12, 13};
-      if (nl == 2) {
-        ids[c[0]] = v == 0 ? 0 : 1;
-        ids[c[1]] = v == 0 ? 1 : 0;
-      } else if (nl == 3) {
-        int perm[3] = {0, 1, 2};
-        bool found = false;
-        do {
-          if (tri_transform_id(perm[0], perm[1], perm[2]) == v) { found = true; break; }
-        } while (std::next_permutation(perm, perm + 3));
-        if (!found) return PZG_EINVAL;
-        for (int k = 0; k < 3; k++) ids[c[k]] = perm[k];
-      }
+      int64_t ids[4];
+      PZG_TRY(variant_ids(etype, s, v, ids));
       for (int q = 0; q < nq; q++) {
         double phi[HP_MAXSH], dphi[3][HP_MAXSH];
         eval_simplex_hp(etype, P, ids, pts[q], phi, dphi);
@@ -577,6 +584,123 @@ int build_tables(int etype, int p, int rule_order, ElemTables &t) {
     }
   }
   for (int i = 0; i < t.nsh; i++) t.shape_side[i] = i;  // p<=2: shape i lives on side i
+  return PZG_OK;
+}
+
+// Highest rule order of the element's rule family (TPZIntPoints::GetMaxOrder:
+// Integral/pzquad.h 36 for line/quadrilateral/hexahedron, TPZIntTriang 21
+// (tpzintrulet.h:72), TPZIntTetra3D 14 (tpzintrulet3d.h:72)) and the order
+// TPZInterpolationSpace::EvaluateError sets for an exact solution of order
+// porder (Mesh/pzinterpolationspace.cpp:1140-1160).
+int post_rule_order(int etype, int rule, int porder) {
+  const int mx = etype == PZG_TRI ? 21 : etype == PZG_TET ? 14 : 36;
+  if (mx > porder) return rule > porder ? rule : porder;
+  return mx;
+}
+
+int build_post_tables(int etype, int p, bool hp, int rule_order, PostTables &t) {
+  t = PostTables();
+  t.etype = etype;
+  t.dim = etype_dim(etype);
+  t.nc = etype_ncorner(etype);
+  const bool simplex = etype == PZG_TRI || etype == PZG_TET;
+  if (hp && !simplex) return PZG_EINVAL;
+  // points and weights
+  std::vector<double> w;
+  std::vector<std::array<double, 3>> xi;
+  if (simplex) {
+    const int cap = 4096;
+    std::vector<double> pb(3 * cap), wb(cap);
+    const int nq = simplex_rule_hp(etype, rule_order, reinterpret_cast<double(*)[3]>(pb.data()),
+                                   wb.data(), cap);
+    if (nq < 0) {
+      set_error("error integration rule order not available on simplices");
+      return PZG_EUNSUP;
+    }
+    for (int q = 0; q < nq; q++) {
+      w.push_back(wb[q]);
+      xi.push_back({pb[3 * q], pb[3 * q + 1], pb[3 * q + 2]});
+    }
+  } else {
+    double gl[64], gw[64];
+    const int ng = gauss_legendre(rule_order, gl, gw);
+    int nq = 1;
+    for (int d = 0; d < t.dim; d++) nq *= ng;
+    for (int q = 0; q < nq; q++) {  // point order as in build_tables
+      if (t.dim == 1) {
+        w.push_back(gw[q]);
+        xi.push_back({gl[q], 0., 0.});
+      } else if (t.dim == 2) {
+        const int ik = q / ng, ie = q % ng;
+        w.push_back(gw[ik] * gw[ie]);
+        xi.push_back({gl[ik], gl[ie], 0.});
+      } else {
+        const int ik = q % ng, ie = (q % (ng * ng)) / ng, iz = q / (ng * ng);
+        w.push_back(gw[ik] * gw[ie] * gw[iz]);
+        xi.push_back({gl[ik], gl[ie], gl[iz]});
+      }
+    }
+  }
+  const int nq = (int)w.size();
+  t.nq = nq;
+  // columns
+  if (hp) {
+    int loc[HP_MAXSH];
+    t.nsh = hp_shape_list(etype, p, t.shape_side, loc);
+    for (int i = 0; i < t.nsh; i++) {
+      t.col_off[i] = t.ncol;
+      t.ncol += side_nvariants(etype, t.shape_side[i]);
+    }
+  } else {
+    double phi[MAXSH], dphi[3][MAXSH];
+    const double x0[3] = {0., 0., 0.};
+    t.nsh = eval_shapes(etype, p, x0, phi, dphi);
+    for (int i = 0; i < t.nsh; i++) {
+      t.col_off[i] = i;
+      t.shape_side[i] = i;
+    }
+    t.ncol = t.nsh;
+  }
+  const int dim = t.dim, nc = t.nc, ncol = t.ncol;
+  t.d.assign((size_t)nq * (1 + ncol * (1 + dim) + nc * (1 + dim)), 0.);
+  double *W = t.d.data(), *PH = W + nq, *DPH = PH + (size_t)ncol * nq;
+  double *CP = DPH + (size_t)dim * ncol * nq, *CD = CP + (size_t)nc * nq;
+  for (int q = 0; q < nq; q++) {
+    W[q] = w[q];
+    double cphi[MAXSH], cd[3][MAXSH];
+    eval_shapes(etype, 1, xi[q].data(), cphi, cd);
+    for (int n = 0; n < nc; n++) {
+      CP[(size_t)n * nq + q] = cphi[n];
+      for (int d = 0; d < dim; d++) CD[((size_t)d * nc + n) * nq + q] = cd[d][n];
+    }
+    if (!hp) {
+      double phi[MAXSH], dphi[3][MAXSH];
+      eval_shapes(etype, p, xi[q].data(), phi, dphi);
+      for (int i = 0; i < ncol; i++) {
+        PH[(size_t)i * nq + q] = phi[i];
+        for (int d = 0; d < dim; d++) DPH[((size_t)d * ncol + i) * nq + q] = dphi[d][i];
+      }
+    }
+  }
+  if (hp) {
+    for (int s = 0; s < etype_nsides(etype); s++) {
+      const int nv = side_nvariants(etype, s);
+      for (int v = 0; v < nv; v++) {
+        int64_t ids[4];
+        PZG_TRY(variant_ids(etype, s, v, ids));
+        for (int q = 0; q < nq; q++) {
+          double phi[HP_MAXSH], dphi[3][HP_MAXSH];
+          eval_simplex_hp(etype, p, ids, xi[q].data(), phi, dphi);
+          for (int i = 0; i < t.nsh; i++) {
+            if (t.shape_side[i] != s) continue;
+            const int col = t.col_off[i] + v;
+            PH[(size_t)col * nq + q] = phi[i];
+            for (int d = 0; d < dim; d++) DPH[((size_t)d * ncol + col) * nq + q] = dphi[d][i];
+          }
+        }
+      }
+    }
+  }
   return PZG_OK;
 }
 

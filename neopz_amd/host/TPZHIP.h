@@ -15,9 +15,11 @@
 // subclasses that plug into NeoPZ itself are given in INTEGRATION.md.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <functional>
 #include <memory>
 #include <vector>
 
@@ -233,10 +235,17 @@ class TPZSpStructMatrixHIP : public TPZSSpStructMatrixHIP {
 };
 
 // TPZLinearAnalysis(cmesh, mustOptimizeBandwidth=false): Assemble()/Solve().
+// The exact solution of TPZAnalysis::SetExact: the reference's
+// std::function(loc, result, deriv) with result[nstate] and deriv the
+// dim x nstate matrix stored row-major (deriv[i * nstate + j] = d u_j / d x_i).
+using ExactSolType = std::function<void(const std::vector<double> &loc, std::vector<double> &result,
+                                        std::vector<double> &deriv)>;
+
 class TPZLinearAnalysis {
  public:
   TPZLinearAnalysis(Ctx ctx, const pzg_mesh_desc &mesh, bool mustOptimizeBandwidth = false)
-      : fCtx(std::move(ctx)), fStr(std::make_unique<TPZSSpStructMatrixHIP>(fCtx, mesh)) {
+      : fCtx(std::move(ctx)), fStr(std::make_unique<TPZSSpStructMatrixHIP>(fCtx, mesh)),
+        fDim(mesh.dim), fNState(mesh.nstate), fNVol(mesh.nvol) {
     if (mustOptimizeBandwidth) DebugStop();  // first-touch numbering only
   }
   // TPZAnalysis::SetStructuralMatrix (Analysis/TPZAnalysis.cpp:83-104)
@@ -258,12 +267,79 @@ class TPZLinearAnalysis {
   TPZSYsmpMatrixHIP &Matrix() { return *fK; }
   TPZStepSolver &Solver() { return fSolver; }
 
+  // TPZLinearAnalysis::LoadSolution / TPZCompMesh::LoadSolution
+  // (Mesh/pzcmesh.cpp:524-540); Solve loads its own solution already.
+  void LoadSolution(const std::vector<double> &sol) {
+    fSolution = sol;
+    Check(pzg_load_solution(fCtx.get(), fSolution.data()), "LoadSolution");
+  }
+  // TPZAnalysis::SetExact (Analysis/TPZAnalysis.cpp:655-676): a host
+  // function, evaluated at the error rule's points ...
+  void SetExact(ExactSolType f, int p) {
+    fExact = std::move(f);
+    fExactDev = pzg_exact{};
+    fExactDev.kind = PZG_EXACT_POINTS;
+    fExactDev.porder = p;
+  }
+  // ... or a TLaplaceExample1 case evaluated on the device (PZG_EXACT_CONST,
+  // _X, _SINSIN, _COSCOS; Pre/TPZAnalyticSolution.cpp:1254-1310)
+  void SetExactLaplace(int kind, int p, const double center[3] = nullptr, double constant = 1.) {
+    fExact = nullptr;
+    fExactDev = pzg_exact{};
+    fExactDev.kind = kind;
+    fExactDev.porder = p;
+    fExactDev.constant = constant;
+    for (int d = 0; d < 3; d++) fExactDev.center[d] = center ? center[d] : 0.;
+  }
+  // TPZAnalysis::PostProcessError (Analysis/TPZAnalysis.cpp:480-487,
+  // 625-648): ervec receives the mesh totals; with store_error the element
+  // values go to ElementSolution() (nvol x nerr, row-major).
+  void PostProcessError(std::vector<double> &ervec, bool store_error = false) {
+    pzg_exact ex = fExactDev;
+    std::vector<double> u, du;
+    if (ex.kind == PZG_EXACT_POINTS) {
+      if (!fExact) DebugStop();  // no exact solution set
+      int64_t n = 0;
+      Check(pzg_error_points(fCtx.get(), ex.porder, &n, nullptr), "PostProcessError");
+      std::vector<double> xq(3 * n);
+      Check(pzg_error_points(fCtx.get(), ex.porder, &n, xq.data()), "PostProcessError");
+      u.assign(n * fNState, 0.);
+      du.assign(n * fDim * fNState, 0.);
+      std::vector<double> loc(3), res(fNState), der(fDim * fNState);
+      for (int64_t q = 0; q < n; q++) {
+        for (int d = 0; d < 3; d++) loc[d] = xq[3 * q + d];
+        std::fill(res.begin(), res.end(), 0.);
+        std::fill(der.begin(), der.end(), 0.);
+        fExact(loc, res, der);
+        for (int k = 0; k < fNState; k++) u[q * fNState + k] = res[k];
+        for (int k = 0; k < fDim * fNState; k++) du[q * fDim * fNState + k] = der[k];
+      }
+      ex.npts = n;
+      ex.u = u.data();
+      ex.du = du.data();
+    }
+    std::vector<double> el(fNVol * 6);
+    ervec.assign(6, 0.);
+    int32_t nerr = 0;
+    Check(pzg_evaluate_error(fCtx.get(), &ex, el.data(), ervec.data(), &nerr), "PostProcessError");
+    ervec.resize(nerr);
+    if (store_error) {
+      el.resize(fNVol * nerr);
+      fElementSolution = std::move(el);
+    }
+  }
+  const std::vector<double> &ElementSolution() const { return fElementSolution; }
+
  private:
   Ctx fCtx;
   std::unique_ptr<TPZSSpStructMatrixHIP> fStr;
   std::unique_ptr<TPZSYsmpMatrixHIP> fK;
   TPZStepSolver fSolver;
-  std::vector<double> fRhs, fSolution;
+  std::vector<double> fRhs, fSolution, fElementSolution;
+  int fDim = 0, fNState = 1;
+  int64_t fNVol = 0;
+  ExactSolType fExact;
+  pzg_exact fExactDev{};
 };
 
 }  // namespace pzhip

@@ -14,8 +14,12 @@
 // (int64/double arrays, each with its count), then iterations and the
 // achieved tolerance.
 //
-// usage: pzhip_run MESH OUT [cg|mixed] [tol] [maxit] [sym|full]
-// (full: TPZSpStructMatrix, IA/JA/A of the non-symmetric TPZFYsmpMatrix)
+// usage: pzhip_run MESH OUT [cg|mixed] [tol] [maxit] [sym|full] [errors]
+// (full: TPZSpStructMatrix, IA/JA/A of the non-symmetric TPZFYsmpMatrix;
+// errors: then an.SetExact(f, 6) with the host function of exact_solution
+// below and an.PostProcessError(ervec, true); ervec and the element errors
+// are appended to OUT as two more arrays)
+#include <cmath>
 #include <cstring>
 #include <string>
 
@@ -52,11 +56,42 @@ void put(FILE *f, const std::vector<T> &v) {
   std::fwrite(v.data(), sizeof(T), v.size(), f);
 }
 
+// The exact solution handed to SetExact: scalar problems
+// u = sin(pi x) sin(pi y) [sin(pi z)]; elasticity u_j = (j+1) sin(pi x_j) +
+// x_0 x_1 x_2.  deriv[i * nstate + j] = d u_j / d x_i.
+void exact_solution(int dim, int nstate, const std::vector<double> &x, std::vector<double> &u,
+                    std::vector<double> &du) {
+  const double pi = 3.14159265358979323846;
+  if (nstate == 1) {
+    double s[3], c[3];
+    for (int i = 0; i < dim; i++) {
+      s[i] = std::sin(pi * x[i]);
+      c[i] = pi * std::cos(pi * x[i]);
+    }
+    u[0] = 1.;
+    for (int i = 0; i < dim; i++) u[0] *= s[i];
+    for (int i = 0; i < dim; i++) {
+      du[i] = c[i];
+      for (int k = 0; k < dim; k++)
+        if (k != i) du[i] *= s[k];
+    }
+    return;
+  }
+  for (int j = 0; j < 3; j++) {
+    u[j] = (j + 1) * std::sin(pi * x[j]) + x[0] * x[1] * x[2];
+    for (int i = 0; i < 3; i++) {
+      double d = (i == j) ? (j + 1) * pi * std::cos(pi * x[j]) : 0.;
+      d += (i == 0 ? x[1] * x[2] : i == 1 ? x[0] * x[2] : x[0] * x[1]);
+      du[i * 3 + j] = d;
+    }
+  }
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
   if (argc < 3) {
-    std::fprintf(stderr, "usage: %s MESH OUT [cg|mixed] [tol] [maxit] [sym|full]\n", argv[0]);
+    std::fprintf(stderr, "usage: %s MESH OUT [cg|mixed] [tol] [maxit] [sym|full] [errors]\n", argv[0]);
     return 2;
   }
   const std::string method = argc > 3 ? argv[3] : "cg";
@@ -67,6 +102,7 @@ int main(int argc, char **argv) {
     std::fprintf(stderr, "pzhip_run: unknown matrix format %s\n", format.c_str());
     return 2;
   }
+  const bool errors = argc > 7 && std::string(argv[7]) == "errors";
 
   FILE *f = std::fopen(argv[1], "rb");
   if (!f) {
@@ -175,6 +211,18 @@ int main(int argc, char **argv) {
   const double achieved = an.Solver().GetTolerance();
   std::fwrite(&its, sizeof its, 1, o);
   std::fwrite(&achieved, sizeof achieved, 1, o);
+  if (errors) {
+    const int dim = m.dim, ns = m.nstate;
+    an.SetExact(
+        [dim, ns](const std::vector<double> &x, std::vector<double> &u, std::vector<double> &du) {
+          exact_solution(dim, ns, x, u, du);
+        },
+        6);
+    std::vector<double> ervec;
+    an.PostProcessError(ervec, true);
+    put(o, ervec);
+    put(o, an.ElementSolution());
+  }
   std::fclose(o);
   std::printf("pzhip_run: neq %lld nnz %lld iterations %lld achieved %.3e\n", (long long)neq,
               (long long)nnz, (long long)its, achieved);

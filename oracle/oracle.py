@@ -38,6 +38,16 @@ class _Problem(C.Structure):
     ]
 
 
+EXACT_POINTS, EXACT_CONST, EXACT_X, EXACT_SINSIN, EXACT_COSCOS = range(5)
+
+
+class _Exact(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int), ("porder", C.c_int), ("center", C.c_double * 3),
+        ("constant", C.c_double), ("u", C.POINTER(C.c_double)), ("du", C.POINTER(C.c_double)),
+    ]
+
+
 class _Mesh(C.Structure):
     _fields_ = [
         ("dim", C.c_int), ("p", C.c_int), ("nstate", C.c_int),
@@ -88,6 +98,9 @@ def lib():
         L.orc_simplex_rule.argtypes = [C.c_int, C.c_int, dp, dp]
         L.orc_collapsed_rule.argtypes = [C.c_int, C.c_int, dp, dp]
         L.orc_shape_h1.argtypes = [C.c_int, i64p, C.POINTER(C.c_int), dp, dp, dp]
+        L.orc_error_points.argtypes = [P, M, C.c_int, dp]
+        L.orc_error_points.restype = C.c_int64
+        L.orc_evaluate_error.argtypes = [P, M, dp, C.POINTER(_Exact), dp, dp]
         L.orc_nnz_closed_form.argtypes = [C.c_int] * 6
         L.orc_nnz_closed_form.restype = C.c_int64
         L.free = C.CDLL(None).free
@@ -296,6 +309,44 @@ class Oracle:
         rhs = np.zeros(self.neq)
         lib().orc_assemble(C.byref(self.pb), C.byref(self.m), _ip(ia), _ip(ja), _dp(a), _dp(rhs))
         return a, rhs
+
+    # --- post-processing ---------------------------------------------------
+    def error_points(self, porder):
+        """Physical coordinates [npts, 3] of the error rule's points of every
+        volume element (element order) for an exact solution of order porder."""
+        n = lib().orc_error_points(C.byref(self.pb), C.byref(self.m), porder, None)
+        if n < 0:
+            raise RuntimeError("oracle error rule failed")
+        xq = np.zeros((n, 3))
+        lib().orc_error_points(C.byref(self.pb), C.byref(self.m), porder, _dp(xq))
+        return xq
+
+    def evaluate_error(self, u, kind=EXACT_POINTS, porder=0, center=(0.0, 0.0, 0.0), constant=0.0,
+                       uq=None, duq=None):
+        """TPZCompMesh::EvaluateError of solution u: (per-element errors
+        [nvol, nerr], mesh totals [nerr]).  kind EXACT_POINTS takes the exact
+        values uq [npts, nstate] and derivatives duq [npts, dim, nstate] at the
+        points of error_points(porder)."""
+        ex = _Exact()
+        ex.kind, ex.porder = kind, porder
+        for d in range(3):
+            ex.center[d] = center[d]
+        ex.constant = constant
+        keep = []
+        if kind == EXACT_POINTS:
+            uq = np.ascontiguousarray(uq, dtype=np.float64)
+            duq = np.ascontiguousarray(duq, dtype=np.float64)
+            keep += [uq, duq]
+            ex.u, ex.du = _dp(uq), _dp(duq)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        el = np.zeros((self.m.nvol, 6))
+        tot = np.zeros(6)
+        nerr = lib().orc_evaluate_error(C.byref(self.pb), C.byref(self.m), _dp(u), C.byref(ex),
+                                        _dp(el), _dp(tot))
+        if nerr < 0:
+            raise RuntimeError("oracle EvaluateError failed")
+        el = el.reshape(-1)[: self.m.nvol * nerr].reshape(self.m.nvol, nerr).copy()
+        return el, tot[:nerr].copy()
 
     def assemble_range(self, e0, e1, a=None, rhs=None):
         ia, ja = self.pattern()

@@ -1460,3 +1460,280 @@ int64_t orc_nnz_closed_form(int dim, int p, int nstate, int nx, int ny, int nz) 
   N *= nstate;
   return (full + N) / 2;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Post-processing: TPZCompMesh::EvaluateError (Mesh/pzcmesh.cpp:1497-1552) */
+/* over TPZInterpolationSpace::EvaluateError (Mesh/pzinterpolationspace.cpp: */
+/* 1108-1207) with the materials' Errors.                                    */
+/* ------------------------------------------------------------------------ */
+
+/* Order of the error rule (EvaluateError, :1140-1160): the element rule's
+ * order raised to the exact solution's order, unless that order reaches the
+ * rule family's GetMaxOrder (36 for TPZInt1d / TPZIntQuad / TPZIntCube3D,
+ * Integral/pzquad.h; 21 TPZIntTriang, Integral/tpzintrulet.h:72; 14
+ * TPZIntTetra3D, Integral/tpzintrulet3d.h:72), which is then used. */
+static int error_order(int t, int rule, int porder) {
+  int mx = t == ORC_TRI ? 21 : t == ORC_TET ? 14 : 36;
+  if (mx > porder) return rule > porder ? rule : porder;
+  return mx;
+}
+
+/* Points (xi[3q]) and weights of element iel's error rule; malloc'ed. */
+static int error_rule(const orc_problem *pb, const orc_mesh *m, int64_t iel, int porder,
+                      double **xi_out, double **w_out) {
+  int t = m->etype[iel], dim = etype_dim(t), maxorder = -1;
+  for (int s = 0; s < etype_nsides(t); s++) {
+    int64_t c = m->econn[27 * iel + s];
+    if (m->conn_order[c] > maxorder) maxorder = m->conn_order[c];
+  }
+  int order = error_order(t, rule_order(pb, 0, maxorder), porder);
+  double *xi = NULL, *w = NULL;
+  int nq;
+  if (is_simplex(t)) {
+    xi = malloc(sizeof(double) * 3 * 4096);
+    w = malloc(sizeof(double) * 4096);
+    nq = orc_simplex_rule(t, order, xi, w);
+    if (nq < 0) { free(xi); free(w); return -1; }
+  } else {
+    double gl[64], gw[64];
+    int ng = orc_gauss_legendre(order, gl, gw);
+    nq = 1;
+    for (int d = 0; d < dim; d++) nq *= ng;
+    xi = malloc(sizeof(double) * 3 * nq);
+    w = malloc(sizeof(double) * nq);
+    for (int q = 0; q < nq; q++) {
+      double *x = xi + 3 * q;
+      x[0] = x[1] = x[2] = 0.;
+      if (dim == 1) { x[0] = gl[q]; w[q] = gw[q]; }
+      else if (dim == 2) { /* Integral/pzquad.cpp:159-169 */
+        int ik = q / ng, ie = q % ng;
+        x[0] = gl[ik]; x[1] = gl[ie]; w[q] = gw[ik] * gw[ie];
+      } else { /* Integral/pzquad.cpp:274-287 */
+        int ik = q % ng, ie = (q % (ng * ng)) / ng, iz = q / (ng * ng);
+        x[0] = gl[ik]; x[1] = gl[ie]; x[2] = gl[iz]; w[q] = gw[ik] * gw[ie] * gw[iz];
+      }
+    }
+  }
+  *xi_out = xi;
+  *w_out = w;
+  return nq;
+}
+
+/* Geometry at xi: x (TPZGeoEl::X), detjac, jacinv and axes (jacobian()). */
+static double elem_geometry(const orc_mesh *m, int64_t iel, const double *xi, double *x,
+                            double jinv[3][3], double axes[3][3]) {
+  int t = m->etype[iel], dim = etype_dim(t), nc = etype_ncorner(t);
+  double cphi[27], cdphi[81], gradx[3][3] = {{0}};
+  orc_shape(t, 1, xi, cphi, cdphi);
+  x[0] = x[1] = x[2] = 0.;
+  for (int n = 0; n < nc; n++)
+    for (int i = 0; i < 3; i++) {
+      double c = m->xyz[3 * m->enodes[8 * iel + n] + i];
+      x[i] += c * cphi[n];
+      for (int j = 0; j < dim; j++) gradx[i][j] += c * cdphi[j * nc + n];
+    }
+  memset(jinv, 0, 9 * sizeof(double));
+  return jacobian(dim, gradx, jinv, axes);
+}
+
+/* Shapes of element iel at xi as in orc_calcstiff; returns nshape. */
+static int elem_shapes(const orc_mesh *m, int64_t iel, const double *xi, double *phi, double *dphi) {
+  int t = m->etype[iel], dim = etype_dim(t), nc = etype_ncorner(t), nsh = 0;
+  if (is_simplex(t)) {
+    int64_t ids[4];
+    int sord[27];
+    for (int n = 0; n < nc; n++) ids[n] = m->enodes[8 * iel + n];
+    for (int s = nc; s < etype_nsides(t); s++) sord[s - nc] = m->conn_order[m->econn[27 * iel + s]];
+    return orc_shape_h1(t, ids, sord, xi, phi, dphi);
+  }
+  int keep[27], nkeep = 0;
+  for (int s = 0; s < etype_nsides(t); s++)
+    if (m->conn_nshape[m->econn[27 * iel + s]] > 0) keep[nkeep++] = s;
+  double phiall[27], dphiall[81];
+  int nall = orc_shape(t, m->p, xi, phiall, dphiall), sel[27];
+  for (int k = 0; k < nkeep; k++)
+    if (keep[k] < nall) sel[nsh++] = keep[k];
+  for (int i = 0; i < nsh; i++) {
+    phi[i] = phiall[sel[i]];
+    for (int a = 0; a < dim; a++) dphi[a * nsh + i] = dphiall[a * nall + sel[i]];
+  }
+  return nsh;
+}
+
+/* Scalar exact solutions of TLaplaceExample1::uxy (Pre/TPZAnalyticSolution.
+ * cpp:1254-1310: EConst, EX, ESinSin, ECosCos) over the first dim
+ * coordinates, with their gradients (graduxy, :1927-1943). */
+static void exact_laplace(const orc_exact *ex, int dim, const double *x, double *u, double *du) {
+  const double pi = 3.14159265358979323846;
+  double xl[3];
+  for (int i = 0; i < 3; i++) { xl[i] = x[i] - ex->center[i]; du[i] = 0.; }
+  if (ex->kind == ORC_EXACT_CONST) { *u = ex->constant; return; }
+  if (ex->kind == ORC_EXACT_X) { *u = xl[0]; du[0] = 1.; return; }
+  double f[3], df[3];
+  for (int i = 0; i < dim; i++) {
+    if (ex->kind == ORC_EXACT_SINSIN) { f[i] = sin(pi * xl[i]); df[i] = pi * cos(pi * xl[i]); }
+    else { f[i] = cos(pi * xl[i] / 2); df[i] = -pi / 2 * sin(pi * xl[i] / 2); }
+  }
+  *u = 1.;
+  for (int i = 0; i < dim; i++) *u *= f[i];
+  for (int j = 0; j < dim; j++) {
+    du[j] = df[j];
+    for (int i = 0; i < dim; i++)
+      if (i != j) du[j] *= f[i];
+  }
+}
+
+/* TPZMatPoisson::Errors (Material/Poisson/TPZMatPoisson.cpp:173-218) */
+static void errors_poisson(int dim, double u, const double *gradu, double ue, const double *due,
+                           double *v) {
+  double diff = u - ue;
+  v[1] = diff * diff;
+  v[2] = 0.;
+  for (int i = 0; i < dim; i++) {
+    diff = gradu[i] - due[i];
+    v[2] += diff * diff;
+  }
+  v[0] = v[1] + v[2];
+}
+
+/* TPZDarcyFlow::Errors (Material/DarcyFlow/TPZDarcyFlow.cpp:268-326) */
+static void errors_darcy(int dim, double perm, double p, const double *gradu, double pe,
+                         const double *flux, double *v) {
+  for (int k = 0; k < 6; k++) v[k] = 0.;
+  double diff = fabs(p - pe);
+  v[1] = diff * diff;
+  double gd[3] = {0, 0, 0};
+  for (int i = 0; i < dim; i++) gd[i] += fabs(gradu[i] - flux[i]);
+  diff = 0;
+  for (int i = 0; i < dim; i++) diff += perm * gd[i];
+  v[2] += fabs(diff * diff);
+  v[0] = v[1] + v[2];
+  for (int i = 0; i < dim; i++) {
+    double fs = -perm * gradu[i];
+    diff = fabs(flux[i] - fs);
+    v[3 + i] = diff * diff;
+  }
+}
+
+/* TPZElasticity3D::Errors (Material/Elasticity/TPZElasticity3D.cpp:1081-1133)
+ * with ComputeStressTensor / ComputeStressVector (:1152-1192), fPreStress 0.
+ * dudx[i][j] = d u_j / d x_i. */
+static void errors_elast(double E, double nu, const double *u, double dudx[3][3], const double *ue,
+                         double due[3][3], double *v) {
+  double L2 = 0., semi = 0., d[3][3];
+  for (int i = 0; i < 3; i++) L2 += (u[i] - ue[i]) * (u[i] - ue[i]);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      d[i][j] = dudx[i][j] - due[i][j];
+      semi += d[i][j] * d[i][j];
+    }
+  double const1 = -1. + nu, const2 = -1. + nu + 2. * nu * nu, const3 = 2. * (1. + nu), vec[6];
+  vec[0] = E * (d[0][0] * const1 - (d[1][1] + d[2][2]) * nu) / const2;
+  vec[1] = E * (d[1][1] * const1 - (d[0][0] + d[2][2]) * nu) / const2;
+  vec[2] = E * (d[2][2] * const1 - (d[0][0] + d[1][1]) * nu) / const2;
+  vec[3] = E * (d[1][0] + d[0][1]) / const3;
+  vec[4] = E * (d[2][0] + d[0][2]) / const3;
+  vec[5] = E * (d[2][1] + d[1][2]) / const3;
+  double st[3][3] = {{vec[0], vec[3], vec[4]}, {vec[3], vec[1], vec[5]}, {vec[4], vec[5], vec[2]}};
+  double energy = 0.;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) energy += d[i][j] * st[i][j];
+  v[1] = L2;
+  v[2] = semi;
+  v[0] = energy;
+}
+
+int64_t orc_error_points(const orc_problem *pb, const orc_mesh *m, int porder, double *xq) {
+  int64_t np = 0;
+  for (int64_t iel = 0; iel < m->nvol; iel++) {
+    double *xi, *w;
+    int nq = error_rule(pb, m, iel, porder, &xi, &w);
+    if (nq < 0) return -1;
+    for (int q = 0; q < nq && xq; q++) {
+      double jinv[3][3], axes[3][3];
+      elem_geometry(m, iel, xi + 3 * q, xq + 3 * (np + q), jinv, axes);
+    }
+    np += nq;
+    free(xi);
+    free(w);
+  }
+  return np;
+}
+
+int orc_evaluate_error(const orc_problem *pb, const orc_mesh *m, const double *sol,
+                       const orc_exact *ex, double *el_err, double *errors) {
+  if (pb->material == ORC_MOCK1D) return -1;
+  int ns = m->nstate, dim = m->dim, nerr = pb->material == ORC_DARCY ? 6 : 3;
+  if (ex->kind != ORC_EXACT_POINTS && ns != 1) return -1;
+  double sum[6] = {0, 0, 0, 0, 0, 0};
+  int64_t np = 0;
+  for (int64_t iel = 0; iel < m->nvol; iel++) {
+    /* element solution (TPZInterpolationSpace::ComputeSolution): dofs in
+     * connect order, shape-major (Mesh/pzelmat.cpp:37-70) */
+    double uel[ORC_MAXNE];
+    int t = m->etype[iel], ne = 0;
+    for (int s = 0; s < etype_nsides(t); s++) {
+      int64_t seq = m->conn_seq[m->econn[27 * iel + s]];
+      for (int k = 0; k < m->blk_size[seq]; k++) uel[ne++] = sol[m->blk_pos[seq] + k];
+    }
+    double coef = pb->el_coef ? pb->el_coef[iel] : (pb->material == ORC_ELAST3D ? pb->E : pb->scale);
+    double *xi, *w;
+    int nq = error_rule(pb, m, iel, ex->porder, &xi, &w);
+    if (nq < 0) return -1;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int q = 0; q < nq; q++) {
+      double phi[64], dphi[192], x[3], jinv[3][3], axes[3][3];
+      int nsh = elem_shapes(m, iel, xi + 3 * q, phi, dphi);
+      double detjac = elem_geometry(m, iel, xi + 3 * q, x, jinv, axes);
+      /* sol and dsol in axes coordinates, then Axes2XYZ (Util/pzaxestools.h:
+       * 61-86): dudx(i,k) = sum_a axes(a,i) dudaxes(a,k) */
+      double u[3] = {0, 0, 0}, dax[3][3] = {{0}}, dudx[3][3] = {{0}};
+      for (int i = 0; i < nsh; i++) {
+        double dx[3] = {0, 0, 0};
+        for (int a = 0; a < dim; a++)
+          for (int b = 0; b < dim; b++) dx[a] += jinv[b][a] * dphi[b * nsh + i];
+        for (int k = 0; k < ns; k++) {
+          u[k] += phi[i] * uel[i * ns + k];
+          for (int a = 0; a < dim; a++) dax[a][k] += dx[a] * uel[i * ns + k];
+        }
+      }
+      for (int i = 0; i < 3; i++)
+        for (int k = 0; k < ns; k++)
+          for (int a = 0; a < dim; a++) dudx[i][k] += axes[a][i] * dax[a][k];
+      double ue[3] = {0, 0, 0}, due[3][3] = {{0}};
+      if (ex->kind == ORC_EXACT_POINTS) {
+        int64_t p = np + q;
+        for (int k = 0; k < ns; k++) {
+          ue[k] = ex->u[p * ns + k];
+          for (int d = 0; d < dim; d++) due[d][k] = ex->du[(p * dim + d) * ns + k];
+        }
+      } else {
+        double g[3];
+        exact_laplace(ex, dim, x, &ue[0], g);
+        for (int d = 0; d < 3; d++) due[d][0] = g[d];
+      }
+      double v[6] = {0, 0, 0, 0, 0, 0};
+      if (pb->material == ORC_POISSON) {
+        double gu[3] = {dudx[0][0], dudx[1][0], dudx[2][0]}, ge[3] = {due[0][0], due[1][0], due[2][0]};
+        errors_poisson(dim, u[0], gu, ue[0], ge, v);
+      } else if (pb->material == ORC_DARCY) {
+        double gu[3] = {dudx[0][0], dudx[1][0], dudx[2][0]}, ge[3] = {due[0][0], due[1][0], due[2][0]};
+        errors_darcy(dim, coef, u[0], gu, ue[0], ge, v);
+      } else {
+        errors_elast(coef, pb->nu, u, dudx, ue, due, v);
+      }
+      double weight = w[q] * fabs(detjac);
+      for (int k = 0; k < nerr; k++) acc[k] += weight * v[k];
+    }
+    free(xi);
+    free(w);
+    np += nq;
+    for (int k = 0; k < nerr; k++) {
+      double e = sqrt(acc[k]);
+      if (el_err) el_err[iel * nerr + k] = e;
+      sum[k] += e * e;
+    }
+  }
+  for (int k = 0; k < nerr && errors; k++) errors[k] = sqrt(sum[k]);
+  return nerr;
+}

@@ -18,6 +18,7 @@
  *   - TPZSSpStructMatrix::SetupMatrixData (upper CSR pattern)
  *   - TPZSYsmpMatrix::AddKel (with the |v|<1e-12 drop rule), AddFel
  *   - TPZSYsmpMatrix::MultAdd, CG<> with the EJacobi TPZStepSolver
+ *   - TPZCompMesh::EvaluateError with the Errors of the three materials
  */
 #ifndef PZ_ORACLE_H
 #define PZ_ORACLE_H
@@ -149,6 +150,29 @@ void orc_multadd(int64_t n, const int64_t *ia, const int64_t *ja,
 int orc_pcg(int64_t n, const int64_t *ia, const int64_t *ja, const double *a,
             const double *b, double *x, int64_t *iters, double *tol,
             int wasted_spmv, double *res_history);
+
+/* ---- post-processing: error norms (TPZAnalysis::PostProcessError) ------- */
+enum { ORC_EXACT_POINTS = 0, ORC_EXACT_CONST = 1, ORC_EXACT_X = 2, ORC_EXACT_SINSIN = 3,
+       ORC_EXACT_COSCOS = 4 };
+typedef struct {
+  int kind;             /* ORC_EXACT_*                                         */
+  int porder;           /* exact solution's order (TPZMatError::fExactPOrder)  */
+  double center[3];     /* TLaplaceExample1::fCenter                           */
+  double constant;      /* ORC_EXACT_CONST value                               */
+  const double *u;      /* ORC_EXACT_POINTS: [npts][nstate]                    */
+  const double *du;     /* ORC_EXACT_POINTS: [npts][dim][nstate], d u_j / d x_i */
+} orc_exact;
+
+/* Physical coordinates xq[3*npts] (NULL: count only) of the error rule's
+ * points of every volume element in element order; returns npts (-1 on
+ * failure). */
+int64_t orc_error_points(const orc_problem *pb, const orc_mesh *m, int porder, double *xq);
+
+/* Error norms of the solution sol[neq]: el_err[nvol][nerr] (may be NULL) and
+ * the mesh totals errors[nerr] (may be NULL); returns nerr (3 Poisson and
+ * Elasticity3D, 6 Darcy) or -1. */
+int orc_evaluate_error(const orc_problem *pb, const orc_mesh *m, const double *sol,
+                       const orc_exact *ex, double *el_err, double *errors);
 
 /* Number of CSR entries for structured meshes, closed form (for checks). */
 int64_t orc_nnz_closed_form(int dim, int p, int nstate, int nx, int ny, int nz);

@@ -42,6 +42,10 @@ def _read_result(path):
         out[name] = take(dt, n)
     out["iterations"] = int(take(np.int64, 1)[0])
     out["achieved"] = float(take(np.float64, 1)[0])
+    if off < len(buf):  # errors mode: ervec and the element errors
+        for name in ("ervec", "el_errors"):
+            n = int(take(np.int64, 1)[0])
+            out[name] = take(np.float64, n)
     assert off == len(buf)
     return out
 
@@ -110,3 +114,46 @@ def test_cpp_full_format_matches_oracle(tmp_path, name, pb):
     a, rhs = o.assemble()
     ref = O.pcg(ia, ja, a, rhs, maxit=20000, tol=1e-13, wasted_spmv=False)
     assert np.linalg.norm(res["u"] - ref["x"]) <= 1e-8 * np.linalg.norm(ref["x"])
+
+
+def _exact_np(dim, ns, x):
+    """numpy statement of pzhip_run's exact_solution."""
+    if ns == 1:
+        s, c = np.sin(np.pi * x[:, :dim]), np.pi * np.cos(np.pi * x[:, :dim])
+        u = np.prod(s, 1)[:, None]
+        du = np.stack([c[:, i] * np.prod(np.delete(s, i, 1), 1) for i in range(dim)], 1)[:, :, None]
+        return u, du
+    u = np.stack([(j + 1) * np.sin(np.pi * x[:, j]) + x[:, 0] * x[:, 1] * x[:, 2] for j in range(3)], 1)
+    prod = np.stack([x[:, 1] * x[:, 2], x[:, 0] * x[:, 2], x[:, 0] * x[:, 1]], 1)
+    du = np.repeat(prod[:, :, None], 3, axis=2)
+    for j in range(3):
+        du[:, j, j] += (j + 1) * np.pi * np.cos(np.pi * x[:, j])
+    return u, du
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,pb", [
+    ("poisson2d_hp", lambda: _hp(P.poisson2d(n=10, p=2, seed=2), 3)),
+    ("elast3d_p2_body", lambda: P.elasticity3d(n=3, p=2, seed=6, field_kind="grf", load="body")),
+    ("darcy_tet_hp5", lambda: P.darcy3d_tet(n=2, pmax=5, seed=5, hp=True)),
+])
+def test_cpp_post_process_error_matches_oracle(tmp_path, name, pb):
+    """TPZLinearAnalysis::SetExact(std::function, 6) + PostProcessError(ervec,
+    true) in the C++ mirror: totals and element errors of its CG solution
+    within 1e-12 of the oracle's EvaluateError on the same solution."""
+    pb = pb()
+    mfile, ofile = tmp_path / "mesh.bin", tmp_path / "out.bin"
+    M.save(M.build(pb), mfile)
+    print(f"[gpu] pzhip_run errors {name}", flush=True)
+    r = subprocess.run([str(RUN), str(mfile), str(ofile), "cg", "1e-12", "20000", "sym", "errors"],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    res = _read_result(ofile)
+    o = O.Oracle(pb)
+    xq = o.error_points(6)
+    uq, duq = _exact_np(pb.dim, pb.nstate, xq)
+    el, tot = o.evaluate_error(res["u"], kind=O.EXACT_POINTS, porder=6, uq=uq, duq=duq)
+    assert np.allclose(res["ervec"], tot, rtol=1e-12, atol=0)
+    el_g = res["el_errors"].reshape(el.shape)
+    assert np.max(np.abs(el_g - el) - 1e-12 * np.abs(el)) <= 1e-13 * np.max(np.abs(el))
