@@ -253,6 +253,22 @@ int pzg_hp_tuning(pzg_ctx *ctx, int32_t *n, int32_t *order, int32_t *variant, fl
  * solver's inner iterations, which has no reference counterpart. */
 int pzg_scaled_apply(pzg_ctx *ctx, int32_t op, const double *x, double *y);
 
+/* ---- bandwidth renumbering ------------------------------------------------ */
+
+/* TPZAnalysis::OptimizeBandwidth (Analysis/TPZAnalysis.cpp:225-300) with the
+ * default TPZSloanRenumbering (RENUMBER, :74-78): the permutation of
+ * TPZSloanRenumbering::Resequence (External/TPZSloanRenumbering.cpp:23-161)
+ * of the element graph of TPZCompMesh::ComputeElGraph (Mesh/pzcmesh.cpp:
+ * 1223-1266): elgraph[elgraphindex[e] .. elgraphindex[e+1]) are the sequence
+ * numbers of element e's independent connects, nnodes the number of
+ * independent connects.  perm[seq] receives the new sequence number (the
+ * permGather handed to TPZCompMesh::Permute); apply it to blk_pos/blk_size
+ * and the element connect lists before pzg_mesh_set.  Host computation, no
+ * context and no GPU.  PZG_EINVAL if the graph is not connected (the
+ * reference stops there too). */
+int pzg_sloan_renumber(int64_t nnodes, int64_t nel, const int64_t *elgraphindex,
+                       const int64_t *elgraph, int64_t *perm);
+
 /* ---- post-processing: solution and error norms -------------------------- */
 
 /* Exact solutions for pzg_evaluate_error.  PZG_EXACT_POINTS: the caller

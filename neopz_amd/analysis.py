@@ -327,15 +327,43 @@ class TLaplaceExample1:
         return self
 
 
+def element_graph(mesh):
+    """TPZCompMesh::ComputeElGraph (Mesh/pzcmesh.cpp:1223-1266) of a flattened
+    mesh: (elgraphindex, elgraph) over the volume then the boundary elements,
+    each element's connect sequence numbers in side order."""
+    vc = np.asarray(mesh["vol_conn"], dtype=np.int64)
+    vc = vc.reshape(len(mesh["vol_nodes"]), -1)
+    bc = np.asarray(mesh["bc_conn"], dtype=np.int64)
+    nbc = len(mesh["bc_nodes"])
+    bc = bc.reshape(nbc, -1) if nbc else np.zeros((0, 1), dtype=np.int64)
+    eptr = np.concatenate([[0], np.cumsum([vc.shape[1]] * vc.shape[0] + [bc.shape[1]] * nbc)])
+    return eptr.astype(np.int64), np.concatenate([vc.ravel(), bc.ravel()]).astype(np.int64)
+
+
+def OptimizeBandwidth(mesh):
+    """TPZAnalysis::OptimizeBandwidth (Analysis/TPZAnalysis.cpp:225-300) with
+    the default TPZSloanRenumbering, computed by the library on the host:
+    returns the renumbered mesh description (TPZCompMesh::Permute) and the
+    permutation perm[old sequence number] = new one."""
+    eptr, elist = element_graph(mesh)
+    n = int(mesh["nconn"])
+    perm = np.zeros(n, dtype=np.int64)
+    L.check(L.load().pzg_sloan_renumber(n, len(eptr) - 1, L.ptr(eptr, L.i64p), L.ptr(elist, L.i64p),
+                                        L.ptr(perm, L.i64p)))
+    return M.permute(mesh, perm), perm
+
+
 class TPZLinearAnalysis:
-    """TPZLinearAnalysis(cmesh, mustOptimizeBandwidth=false) over the GPU path."""
+    """TPZLinearAnalysis(cmesh, mustOptimizeBandwidth) over the GPU path; with
+    mustOptimizeBandwidth the connects are renumbered by TPZSloanRenumbering
+    first (TPZAnalysis::SetCompMeshInit, Analysis/TPZAnalysis.cpp:150-194)."""
 
     def __init__(self, problem_or_mesh, mustOptimizeBandwidth=False, ctx=None):
-        if mustOptimizeBandwidth:
-            raise ValueError("Sloan renumbering is not part of the GPU path: use "
-                             "mustOptimizeBandwidth=False (first-touch numbering)")
         self.ctx = ctx or Context(0)
         self.mesh = problem_or_mesh if isinstance(problem_or_mesh, dict) else M.build(problem_or_mesh)
+        self.fPermutation = None
+        if mustOptimizeBandwidth:
+            self.mesh, self.fPermutation = OptimizeBandwidth(self.mesh)
         self.fStructMatrix = None
         self.fSolver = None
         self.K = None

@@ -987,6 +987,11 @@ int pzg_evaluate_error(pzg_ctx *ctx, const pzg_exact *exact, double *el_errors, 
   return PZG_OK;
 }
 
+int pzg_sloan_renumber(int64_t nnodes, int64_t nel, const int64_t *elgraphindex,
+                       const int64_t *elgraph, int64_t *perm) {
+  return sloan_renumber(nnodes, nel, elgraphindex, elgraph, perm);
+}
+
 int pzg_get_sizes(pzg_ctx *ctx, pzg_sizes *s) {
   CTX_CHECK();
   Ctx &c = ctx->c;

@@ -320,5 +320,8 @@ int true_residual(Ctx &c, const double *u, double *rel);
 // post-processing of the device solution (pzg_post.hip)
 int post_points(Ctx &c, int porder, int64_t *npts, double *xq);
 int post_errors(Ctx &c, const pzg_exact &ex, double *el_errors, double *errors, int *nerr);
+// Sloan renumbering of the connect graph (pzg_renumber.cpp)
+int sloan_renumber(int64_t nnodes, int64_t nel, const int64_t *eptr, const int64_t *elist,
+                   int64_t *perm);
 
 }  // namespace pzg

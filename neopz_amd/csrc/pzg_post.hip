@@ -53,12 +53,15 @@ __device__ inline void exact_dev(const PostArgs &a, const double x[3], double &u
     const bool sn = a.kind == PZG_EXACT_SINSIN;
     double f[3], df[3];
     for (int i = 0; i < a.dim; i++) {
+      double s, c;
       if (sn) {
-        f[i] = sin(PI * xl[i]);
-        df[i] = PI * cos(PI * xl[i]);
+        sincos(PI * xl[i], &s, &c);
+        f[i] = s;
+        df[i] = PI * c;
       } else {
-        f[i] = cos(PI * xl[i] / 2);
-        df[i] = -(PI / 2) * sin(PI * xl[i] / 2);
+        sincos(PI * xl[i] / 2, &s, &c);
+        f[i] = c;
+        df[i] = -(PI / 2) * s;
       }
     }
     ue = 1.;

@@ -202,6 +202,11 @@ class TPZSSpStructMatrixHIP {
   virtual std::unique_ptr<TPZSYsmpMatrixHIP> Create() {
     return std::make_unique<TPZSYsmpMatrixHIP>(fCtx, Setup());
   }
+  // the analysis' mesh (after a renumbering) replaces the one given here
+  void SetMesh(const pzg_mesh_desc &mesh) {
+    fMesh = mesh;
+    fFlattened = false;
+  }
   // TPZStructMatrixOR::Assemble(K, rhs) (StrMatrix/pzstrmatrixor.cpp:41-71)
   void Assemble(TPZSYsmpMatrixHIP & /*K*/, std::vector<double> &rhs) {
     Check(pzg_assemble(fCtx.get()), "Assemble");
@@ -243,14 +248,54 @@ using ExactSolType = std::function<void(const std::vector<double> &loc, std::vec
 
 class TPZLinearAnalysis {
  public:
+  // TPZLinearAnalysis(cmesh, mustOptimizeBandwidth) -> TPZAnalysis::
+  // SetCompMeshInit (Analysis/TPZAnalysis.cpp:150-194): with
+  // mustOptimizeBandwidth the connects are renumbered (OptimizeBandwidth)
   TPZLinearAnalysis(Ctx ctx, const pzg_mesh_desc &mesh, bool mustOptimizeBandwidth = false)
-      : fCtx(std::move(ctx)), fStr(std::make_unique<TPZSSpStructMatrixHIP>(fCtx, mesh)),
-        fDim(mesh.dim), fNState(mesh.nstate), fNVol(mesh.nvol) {
-    if (mustOptimizeBandwidth) DebugStop();  // first-touch numbering only
+      : fCtx(std::move(ctx)), fMesh(mesh), fDim(mesh.dim), fNState(mesh.nstate), fNVol(mesh.nvol) {
+    if (mustOptimizeBandwidth) OptimizeBandwidth();
+    fStr = std::make_unique<TPZSSpStructMatrixHIP>(fCtx, fMesh);
   }
-  // TPZAnalysis::SetStructuralMatrix (Analysis/TPZAnalysis.cpp:83-104)
+  // TPZAnalysis::OptimizeBandwidth (Analysis/TPZAnalysis.cpp:225-300) with the
+  // default TPZSloanRenumbering; the analysis keeps the renumbered connect
+  // lists and blocks (TPZCompMesh::Permute)
+  void OptimizeBandwidth() {
+    const bool tet = fMesh.vol_etype == PZG_TET;
+    const int nsv = fMesh.dim == 2 ? 9 : tet ? 15 : 27, nsb = fMesh.dim == 2 ? 3 : tet ? 7 : 9;
+    std::vector<int64_t> eptr(1, 0), elist;
+    for (int64_t e = 0; e < fMesh.nvol; e++) {
+      elist.insert(elist.end(), fMesh.vol_conn + e * nsv, fMesh.vol_conn + (e + 1) * nsv);
+      eptr.push_back(int64_t(elist.size()));
+    }
+    for (int64_t e = 0; e < fMesh.nbc; e++) {
+      elist.insert(elist.end(), fMesh.bc_conn + e * nsb, fMesh.bc_conn + (e + 1) * nsb);
+      eptr.push_back(int64_t(elist.size()));
+    }
+    const int64_t n = fMesh.nconn;
+    fPerm.assign(n, 0);
+    Check(pzg_sloan_renumber(n, int64_t(eptr.size()) - 1, eptr.data(), elist.data(), fPerm.data()),
+          "OptimizeBandwidth");
+    fVolConn.resize(fMesh.nvol * nsv);
+    for (size_t k = 0; k < fVolConn.size(); k++) fVolConn[k] = fPerm[fMesh.vol_conn[k]];
+    fBcConn.resize(fMesh.nbc * nsb);
+    for (size_t k = 0; k < fBcConn.size(); k++) fBcConn[k] = fPerm[fMesh.bc_conn[k]];
+    fBlkSize.assign(n, 0);
+    for (int64_t c = 0; c < n; c++) fBlkSize[fPerm[c]] = fMesh.blk_size[c];
+    fBlkPos.assign(n, 0);
+    for (int64_t c = 1; c < n; c++) fBlkPos[c] = fBlkPos[c - 1] + fBlkSize[c - 1];
+    fMesh.vol_conn = fVolConn.data();
+    fMesh.bc_conn = fBcConn.data();
+    fMesh.blk_size = fBlkSize.data();
+    fMesh.blk_pos = fBlkPos.data();
+    if (fStr) fStr->SetMesh(fMesh);
+  }
+  // perm[old sequence number] = new one (empty without OptimizeBandwidth)
+  const std::vector<int64_t> &Permutation() const { return fPerm; }
+  // TPZAnalysis::SetStructuralMatrix (Analysis/TPZAnalysis.cpp:83-104): the
+  // struct matrix works on the analysis' mesh
   void SetStructuralMatrix(std::unique_ptr<TPZSSpStructMatrixHIP> str) {
     fStr = std::move(str);
+    fStr->SetMesh(fMesh);
     fK.reset();
   }
   void SetSolver(const TPZStepSolver &s) { fSolver = s; }
@@ -336,6 +381,9 @@ class TPZLinearAnalysis {
   std::unique_ptr<TPZSYsmpMatrixHIP> fK;
   TPZStepSolver fSolver;
   std::vector<double> fRhs, fSolution, fElementSolution;
+  pzg_mesh_desc fMesh;
+  std::vector<int64_t> fPerm, fVolConn, fBcConn, fBlkPos;
+  std::vector<int32_t> fBlkSize;
   int fDim = 0, fNState = 1;
   int64_t fNVol = 0;
   ExactSolType fExact;

@@ -14,11 +14,13 @@
 // (int64/double arrays, each with its count), then iterations and the
 // achieved tolerance.
 //
-// usage: pzhip_run MESH OUT [cg|mixed] [tol] [maxit] [sym|full] [errors]
-// (full: TPZSpStructMatrix, IA/JA/A of the non-symmetric TPZFYsmpMatrix;
-// errors: then an.SetExact(f, 6) with the host function of exact_solution
-// below and an.PostProcessError(ervec, true); ervec and the element errors
-// are appended to OUT as two more arrays)
+// usage: pzhip_run MESH OUT [cg|mixed] [tol] [maxit] [flags...]
+// flags: sym (default) or full: TPZSpStructMatrix, IA/JA/A of the
+// non-symmetric TPZFYsmpMatrix; sloan: TPZLinearAnalysis(cmesh, true), the
+// connects renumbered by TPZSloanRenumbering first (the permutation is
+// appended to OUT); errors: then an.SetExact(f, 6) with the host function of
+// exact_solution below and an.PostProcessError(ervec, true) (ervec and the
+// element errors are appended to OUT).
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -91,18 +93,25 @@ void exact_solution(int dim, int nstate, const std::vector<double> &x, std::vect
 
 int main(int argc, char **argv) {
   if (argc < 3) {
-    std::fprintf(stderr, "usage: %s MESH OUT [cg|mixed] [tol] [maxit] [sym|full] [errors]\n", argv[0]);
+    std::fprintf(stderr, "usage: %s MESH OUT [cg|mixed] [tol] [maxit] [sym|full] [sloan] [errors]\n",
+                 argv[0]);
     return 2;
   }
   const std::string method = argc > 3 ? argv[3] : "cg";
   const double tol = argc > 4 ? std::atof(argv[4]) : 1e-10;
   const int64_t maxit = argc > 5 ? std::atoll(argv[5]) : 100000;
-  const std::string format = argc > 6 ? argv[6] : "sym";
-  if (format != "sym" && format != "full") {
-    std::fprintf(stderr, "pzhip_run: unknown matrix format %s\n", format.c_str());
-    return 2;
+  std::string format = "sym";
+  bool errors = false, sloan = false;
+  for (int k = 6; k < argc; k++) {
+    const std::string f = argv[k];
+    if (f == "sym" || f == "full") format = f;
+    else if (f == "errors") errors = true;
+    else if (f == "sloan") sloan = true;
+    else {
+      std::fprintf(stderr, "pzhip_run: unknown flag %s\n", f.c_str());
+      return 2;
+    }
   }
-  const bool errors = argc > 7 && std::string(argv[7]) == "errors";
 
   FILE *f = std::fopen(argv[1], "rb");
   if (!f) {
@@ -176,7 +185,7 @@ int main(int argc, char **argv) {
 
   using namespace pzhip;
   Ctx ctx = MakeContext(0);
-  TPZLinearAnalysis an(ctx, m, /*mustOptimizeBandwidth=*/false);
+  TPZLinearAnalysis an(ctx, m, /*mustOptimizeBandwidth=*/sloan);
   if (format == "full") an.SetStructuralMatrix(std::make_unique<TPZSpStructMatrixHIP>(ctx, m));
   TPZStepSolver step;
   if (method == "cg") {
@@ -223,6 +232,7 @@ int main(int argc, char **argv) {
     put(o, ervec);
     put(o, an.ElementSolution());
   }
+  if (sloan) put(o, an.Permutation());
   std::fclose(o);
   std::printf("pzhip_run: neq %lld nnz %lld iterations %lld achieved %.3e\n", (long long)neq,
               (long long)nnz, (long long)its, achieved);

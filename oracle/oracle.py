@@ -101,6 +101,8 @@ def lib():
         L.orc_error_points.argtypes = [P, M, C.c_int, dp]
         L.orc_error_points.restype = C.c_int64
         L.orc_evaluate_error.argtypes = [P, M, dp, C.POINTER(_Exact), dp, dp]
+        L.orc_sloan.argtypes = [C.c_int64, i64p, i64p, i64p]
+        L.orc_mesh_sloan.argtypes = [M, i64p]
         L.orc_nnz_closed_form.argtypes = [C.c_int] * 6
         L.orc_nnz_closed_form.restype = C.c_int64
         L.free = C.CDLL(None).free
@@ -310,6 +312,14 @@ class Oracle:
         lib().orc_assemble(C.byref(self.pb), C.byref(self.m), _ip(ia), _ip(ja), _dp(a), _dp(rhs))
         return a, rhs
 
+    def sloan(self):
+        """TPZAnalysis::OptimizeBandwidth's permutation (TPZSloanRenumbering) of
+        the current connect numbering: perm[seq] = new sequence number."""
+        perm = np.zeros(self.m.nconn, dtype=np.int64)
+        if lib().orc_mesh_sloan(C.byref(self.m), _ip(perm)) != 0:
+            raise RuntimeError("oracle Sloan renumbering failed (graph not connected)")
+        return perm
+
     # --- post-processing ---------------------------------------------------
     def error_points(self, porder):
         """Physical coordinates [npts, 3] of the error rule's points of every
@@ -387,6 +397,18 @@ def pcg(ia, ja, a, b, maxit=100000, tol=1e-10, wasted_spmv=True, history=False):
     if history:
         out["history"] = hist[: it.value].copy()
     return out
+
+
+def sloan(xadj, adj):
+    """TPZSloanRenumbering::Resequence on a node graph (sorted adjacency, no
+    self loops): perm[node] = new number."""
+    xadj = np.ascontiguousarray(xadj, dtype=np.int64)
+    adj = np.ascontiguousarray(adj, dtype=np.int64)
+    n = len(xadj) - 1
+    perm = np.zeros(max(n, 1), dtype=np.int64)
+    if lib().orc_sloan(n, _ip(xadj), _ip(adj), _ip(perm)) != 0:
+        raise RuntimeError("graph not connected")
+    return perm[:n]
 
 
 def nnz_closed_form(dim, p, nstate, nx, ny, nz):

@@ -1737,3 +1737,159 @@ int orc_evaluate_error(const orc_problem *pb, const orc_mesh *m, const double *s
   for (int k = 0; k < nerr && errors; k++) errors[k] = sqrt(sum[k]);
   return nerr;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Bandwidth renumbering of TPZAnalysis::OptimizeBandwidth (Analysis/        */
+/* TPZAnalysis.cpp:225-300) with the default TPZSloanRenumbering (RENUMBER,  */
+/* :74-78): TPZSloanRenumbering::Resequence (External/                       */
+/* TPZSloanRenumbering.cpp:23-161) on the ConvertGraph node graph, with the  */
+/* SGraph helpers of External/TPZCutHillMcKee.cpp:16-240.                    */
+/* ------------------------------------------------------------------------ */
+
+static int64_t sg_degree(const int64_t *xadj, int64_t v) { return xadj[v + 1] - xadj[v]; }
+
+/* RootedLevelStructure (TPZCutHillMcKee.cpp:93-122): breadth-first levels
+ * from root; a node joins the first level that reaches it, in the order of
+ * its parents and of their (sorted) adjacency.  lv[] holds the nodes level
+ * after level, lptr[k] the start of level k; returns the number of levels. */
+static int64_t sg_levels(int64_t n, const int64_t *xadj, const int64_t *adj, int64_t root,
+                         int64_t *lv, int64_t *lptr, char *seen) {
+  memset(seen, 0, n);
+  int64_t nl = 0, head = 0, tail = 0;
+  lv[tail++] = root;
+  seen[root] = 1;
+  while (head < tail) {
+    int64_t end = tail;
+    lptr[nl++] = head;
+    for (; head < end; head++) {
+      int64_t v = lv[head];
+      for (int64_t k = xadj[v]; k < xadj[v + 1]; k++)
+        if (!seen[adj[k]]) { seen[adj[k]] = 1; lv[tail++] = adj[k]; }
+    }
+  }
+  lptr[nl] = tail;
+  return nl;
+}
+
+/* PseudoPeripheralNodes (TPZCutHillMcKee.cpp:165-240) with SmallestDegree
+ * (:16-31), SortNodes (:124-137, stable by degree) and ShrinkLastLevel
+ * (:139-163, the last node of each degree, at most (size+2)/2 of them). */
+static void sg_peripheral(int64_t n, const int64_t *xadj, const int64_t *adj, int64_t *start,
+                          int64_t *endn) {
+  int64_t *lv = malloc(sizeof(int64_t) * n), *lptr = malloc(sizeof(int64_t) * (n + 1));
+  int64_t *lv2 = malloc(sizeof(int64_t) * n), *lptr2 = malloc(sizeof(int64_t) * (n + 1));
+  int64_t *last = malloc(sizeof(int64_t) * n);
+  char *seen = malloc(n);
+  int64_t mindeg = 1000000, s = -1;
+  for (int64_t i = 0; i < n; i++)
+    if (sg_degree(xadj, i) < mindeg) { mindeg = sg_degree(xadj, i); s = i; }
+  int64_t e = -1;
+  int64_t nlev = sg_levels(n, xadj, adj, s, lv, lptr, seen);
+  int count = 0;
+  while (count < 10 || e == -1) {
+    count++;
+    int64_t cps = s, cpe = e;
+    if (!nlev) break;
+    int64_t nlast = lptr[nlev] - lptr[nlev - 1];
+    memcpy(last, lv + lptr[nlev - 1], sizeof(int64_t) * nlast);
+    /* stable sort by degree (insertion sort keeps equal keys in order) */
+    for (int64_t i = 1; i < nlast; i++) {
+      int64_t x = last[i], dx = sg_degree(xadj, x), j = i - 1;
+      while (j >= 0 && sg_degree(xadj, last[j]) > dx) { last[j + 1] = last[j]; j--; }
+      last[j + 1] = x;
+    }
+    /* one node per degree, the last of each run; then at most (n+2)/2 */
+    int64_t nk = 0;
+    for (int64_t i = 0; i < nlast; i++) {
+      if (i + 1 < nlast && sg_degree(xadj, last[i + 1]) == sg_degree(xadj, last[i])) continue;
+      last[nk++] = last[i];
+    }
+    if ((nlast + 2) / 2 < nk) nk = (nlast + 2) / 2;
+    int64_t we = 1000000000L, hs = nlev;
+    for (int64_t iq = 0; iq < nk; iq++) {
+      int64_t q = last[iq];
+      int64_t h = sg_levels(n, xadj, adj, q, lv2, lptr2, seen), w = 0;
+      for (int64_t j = 0; j < h; j++)
+        if (lptr2[j + 1] - lptr2[j] > w) w = lptr2[j + 1] - lptr2[j];
+      if (h > hs && w < we) {
+        s = q;
+        memcpy(lv, lv2, sizeof(int64_t) * n);
+        memcpy(lptr, lptr2, sizeof(int64_t) * (h + 1));
+        nlev = h;
+        break;
+      } else if (w < we) {
+        e = q;
+        we = w;
+      }
+    }
+    if (cps == s && cpe == e) break;
+  }
+  *start = s;
+  *endn = e;
+  free(lv); free(lptr); free(lv2); free(lptr2); free(last); free(seen);
+}
+
+int orc_sloan(int64_t n, const int64_t *xadj, const int64_t *adj, int64_t *perm) {
+  enum { INACTIVE = 0, PREACTIVE = 1, ACTIVE = 2, POSTACTIVE = 3 };
+  const int64_t W1 = 1, W2 = 2;
+  if (n <= 0) return 0;
+  int64_t s, e;
+  sg_peripheral(n, xadj, adj, &s, &e);
+  int64_t *lv = malloc(sizeof(int64_t) * n), *lptr = malloc(sizeof(int64_t) * (n + 1));
+  char *seen = malloc(n);
+  int64_t nlev = sg_levels(n, xadj, adj, e, lv, lptr, seen);
+  int64_t *prio = malloc(sizeof(int64_t) * n), *dist = malloc(sizeof(int64_t) * n);
+  for (int64_t i = 0; i < n; i++) dist[i] = -1;
+  for (int64_t l = 0; l < nlev; l++)
+    for (int64_t k = lptr[l]; k < lptr[l + 1]; k++) dist[lv[k]] = l;
+  int ok = 1;
+  for (int64_t i = 0; i < n; i++) {
+    if (dist[i] < 0) ok = 0;
+    prio[i] = W1 * dist[i] - W2 * (sg_degree(xadj, i) + 1);
+  }
+  /* SList: list without duplicates, a node enters at most once (history);
+   * remove(i) moves the last entry into slot i */
+  int64_t *Q = malloc(sizeof(int64_t) * n), qn = 0, nr = 0;
+  char *hist = calloc(n, 1), *st = calloc(n, 1);
+#define QPUSH(v) (hist[v] ? 0 : (hist[v] = 1, Q[qn++] = (v), 1))
+  st[s] = PREACTIVE;
+  QPUSH(s);
+  while (ok && qn) {
+    int64_t qi = 0, cur = Q[0];
+    for (int64_t i = 1; i < qn; i++)
+      if (prio[Q[i]] > prio[cur]) { cur = Q[i]; qi = i; }
+    Q[qi] = Q[qn - 1];
+    qn--;
+    if (st[cur] == PREACTIVE)
+      for (int64_t k = xadj[cur]; k < xadj[cur + 1]; k++) {
+        int64_t a = adj[k];
+        prio[a] += W2;
+        if (QPUSH(a) && st[a] == INACTIVE) st[a] = PREACTIVE;
+      }
+    perm[cur] = nr++;
+    st[cur] = POSTACTIVE;
+    for (int64_t k = xadj[cur]; k < xadj[cur + 1]; k++) {
+      int64_t a = adj[k];
+      if (st[a] != PREACTIVE) continue;
+      st[a] = ACTIVE;
+      prio[a] += W2;
+      for (int64_t j = xadj[a]; j < xadj[a + 1]; j++) {
+        int64_t b = adj[j];
+        if (st[b] != POSTACTIVE) prio[b] += W2;
+        if (QPUSH(b) && st[b] == INACTIVE) st[b] = PREACTIVE;
+      }
+    }
+  }
+#undef QPUSH
+  free(lv); free(lptr); free(seen); free(prio); free(dist); free(Q); free(hist); free(st);
+  return (ok && nr == n) ? 0 : -1;
+}
+
+int orc_mesh_sloan(const orc_mesh *m, int64_t *perm) {
+  int64_t *ngp, *ng;
+  node_graph(m, &ngp, &ng);
+  int rc = orc_sloan(m->nconn, ngp, ng, perm);
+  free(ngp);
+  free(ng);
+  return rc;
+}

@@ -19,6 +19,7 @@
  *   - TPZSYsmpMatrix::AddKel (with the |v|<1e-12 drop rule), AddFel
  *   - TPZSYsmpMatrix::MultAdd, CG<> with the EJacobi TPZStepSolver
  *   - TPZCompMesh::EvaluateError with the Errors of the three materials
+ *   - TPZSloanRenumbering::Resequence (TPZAnalysis::OptimizeBandwidth)
  */
 #ifndef PZ_ORACLE_H
 #define PZ_ORACLE_H
@@ -173,6 +174,15 @@ int64_t orc_error_points(const orc_problem *pb, const orc_mesh *m, int porder, d
  * Elasticity3D, 6 Darcy) or -1. */
 int orc_evaluate_error(const orc_problem *pb, const orc_mesh *m, const double *sol,
                        const orc_exact *ex, double *el_err, double *errors);
+
+/* ---- bandwidth renumbering (TPZSloanRenumbering) ------------------------ */
+/* Sloan numbering of a node graph (xadj[n+1], adj sorted per node, no self
+ * loops): perm[node] = new number (permGather).  Returns 0, or -1 if the
+ * graph is not connected. */
+int orc_sloan(int64_t n, const int64_t *xadj, const int64_t *adj, int64_t *perm);
+/* the same on the connect graph of the mesh (TPZAnalysis::OptimizeBandwidth):
+ * perm[seq] = new sequence number, to be applied with orc_mesh_permute */
+int orc_mesh_sloan(const orc_mesh *m, int64_t *perm);
 
 /* Number of CSR entries for structured meshes, closed form (for checks). */
 int64_t orc_nnz_closed_form(int dim, int p, int nstate, int nx, int ny, int nz);

@@ -24,7 +24,7 @@ def _hp(pb, seed):
 RUN = Path(__file__).resolve().parents[1] / "neopz_amd" / "host" / "pzhip_run"
 
 
-def _read_result(path):
+def _read_result(path, extra=()):
     buf = path.read_bytes()
     off = 0
 
@@ -42,10 +42,9 @@ def _read_result(path):
         out[name] = take(dt, n)
     out["iterations"] = int(take(np.int64, 1)[0])
     out["achieved"] = float(take(np.float64, 1)[0])
-    if off < len(buf):  # errors mode: ervec and the element errors
-        for name in ("ervec", "el_errors"):
-            n = int(take(np.int64, 1)[0])
-            out[name] = take(np.float64, n)
+    for name in extra:  # errors: ervec, el_errors; sloan: perm
+        n = int(take(np.int64, 1)[0])
+        out[name] = take(np.int64 if name == "perm" else np.float64, n)
     assert off == len(buf)
     return out
 
@@ -149,7 +148,7 @@ def test_cpp_post_process_error_matches_oracle(tmp_path, name, pb):
                        capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0
-    res = _read_result(ofile)
+    res = _read_result(ofile, ("ervec", "el_errors"))
     o = O.Oracle(pb)
     xq = o.error_points(6)
     uq, duq = _exact_np(pb.dim, pb.nstate, xq)
@@ -157,3 +156,36 @@ def test_cpp_post_process_error_matches_oracle(tmp_path, name, pb):
     assert np.allclose(res["ervec"], tot, rtol=1e-12, atol=0)
     el_g = res["el_errors"].reshape(el.shape)
     assert np.max(np.abs(el_g - el) - 1e-12 * np.abs(el)) <= 1e-13 * np.max(np.abs(el))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,pb", [
+    ("poisson2d_hp", lambda: _hp(P.poisson2d(n=10, p=2, seed=2), 3)),
+    ("elast3d_p2_body", lambda: P.elasticity3d(n=3, p=2, seed=6, field_kind="grf", load="body")),
+    ("darcy_tet_hp5", lambda: P.darcy3d_tet(n=2, pmax=5, seed=5, hp=True)),
+])
+def test_cpp_sloan_renumbered_analysis_matches_oracle(tmp_path, name, pb):
+    """TPZLinearAnalysis(cmesh, mustOptimizeBandwidth = true) in the C++ mirror:
+    the permutation equals the oracle's TPZSloanRenumbering, and IA/JA/A/rhs of
+    the renumbered system and the CG solution equal the oracle's on the
+    renumbered mesh."""
+    pb = pb()
+    mfile, ofile = tmp_path / "mesh.bin", tmp_path / "out.bin"
+    M.save(M.build(pb), mfile)
+    print(f"[gpu] pzhip_run sloan {name}", flush=True)
+    r = subprocess.run([str(RUN), str(mfile), str(ofile), "cg", "1e-13", "20000", "sloan"],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    res = _read_result(ofile, ("perm",))
+    o = O.Oracle(pb)
+    perm = o.sloan()
+    assert np.array_equal(res["perm"], perm)
+    o.permute(perm)
+    ia, ja = o.pattern()
+    a, rhs = o.assemble()
+    assert np.array_equal(res["ia"], ia) and np.array_equal(res["ja"], ja)
+    assert np.max(np.abs(res["a"] - a) - 1e-12 * np.abs(a)) <= 1e-15 * np.max(np.abs(a))
+    assert np.max(np.abs(res["rhs"] - rhs)) <= 1e-12 * np.max(np.abs(rhs))
+    ref = O.pcg(ia, ja, a, rhs, maxit=20000, tol=1e-13, wasted_spmv=False)
+    assert np.linalg.norm(res["u"] - ref["x"]) <= 1e-8 * np.linalg.norm(ref["x"])

@@ -374,3 +374,30 @@ def test_renumbered_connects(name):
     st.SetMixedFCG(50000, 1e-8)
     st.Solve(None, want_solution=False)
     assert st.info["converged"] and st.info["true_residual"] <= 1e-8
+
+
+@pytest.mark.parametrize("name", ["elast3d_p2_body", "poisson2d_hp", "darcy_tet_hp5"])
+def test_sloan_optimized_analysis(name):
+    """TPZLinearAnalysis(cmesh, mustOptimizeBandwidth=True): the connects are
+    renumbered by the library's TPZSloanRenumbering (the oracle's permutation),
+    then assembly and the FP64 PCG solution equal the oracle's on the
+    renumbered mesh."""
+    pb = CASES[name]()
+    print(f"[gpu] Sloan-renumbered analysis {name}", flush=True)
+    an = TPZLinearAnalysis(pb, True, ctx=ctx())
+    o = O.Oracle(pb)
+    perm = o.sloan()
+    assert np.array_equal(an.fPermutation, perm)
+    o.permute(perm)
+    st = TPZStepSolver()
+    st.SetCG(20000, TPZStepSolver.jacobi(), 1e-13, 0)
+    an.SetSolver(st)
+    an.Assemble()
+    ia_r, ja_r = o.pattern()
+    a_r, rhs_r = o.assemble()
+    assert np.array_equal(an.K.IA(), ia_r) and np.array_equal(an.K.JA(), ja_r)
+    _check_values(ia_r, an.K.A(), a_r)
+    assert np.max(np.abs(an.Rhs() - rhs_r)) <= 1e-12 * np.max(np.abs(rhs_r))
+    u = an.Solve()
+    ref = O.pcg(ia_r, ja_r, a_r, rhs_r, maxit=20000, tol=1e-13, wasted_spmv=False)
+    assert np.linalg.norm(u - ref["x"]) <= 1e-8 * np.linalg.norm(ref["x"])
