@@ -112,6 +112,18 @@ typedef struct {
    * vol_nodes has 4 corners and vol_conn 15 sides per element, boundary
    * elements are PZG_TRI with 3 corners and 7 sides)                        */
   int32_t vol_etype;
+  /* constrained connects of hanging sides (TPZConnect::TPZDepend,
+   * Mesh/pzconnect.h; TPZInterpolatedElement::RestrainSide,
+   * Mesh/pzintel.cpp:847-1010): connect c depends on the connects
+   * dep_conn[dep_ptr[c] .. dep_ptr[c+1]) with the coefficients dep_coef
+   * (TPZDepend::fDepMatrix, one shape per connect).  Constrained connects
+   * carry no equations: their blocks come after all the others in sequence
+   * order.  The connects they depend on are unconstrained (one level).
+   * Quadrilateral meshes (dim 2, p <= 2), volume elements only.  NULL = no
+   * constraints. */
+  const int64_t *dep_ptr;   /* nconn + 1, or NULL */
+  const int64_t *dep_conn;
+  const double *dep_coef;
 } pzg_mesh_desc;
 
 /* ---- solver selection -------------------------------------------------- */

@@ -47,6 +47,7 @@ class MeshDesc(C.Structure):
         ("nbumps", C.c_int32), ("bumps", dp), ("bump_width", C.c_double),
         ("big_number", C.c_double),
         ("vol_rule", i32p), ("bc_rule", i32p), ("vol_etype", C.c_int32),
+        ("dep_ptr", i64p), ("dep_conn", i64p), ("dep_coef", dp),
     ]
 
 
@@ -197,6 +198,10 @@ def make_mesh_desc(m):
     d.vol_rule = ptr(arr(m.get("vol_rule"), np.int32), i32p)
     d.bc_rule = ptr(arr(m.get("bc_rule"), np.int32), i32p)
     d.vol_etype = int(m.get("vol_etype", 0))
+    if m.get("dep_ptr") is not None:
+        d.dep_ptr = ptr(arr(m["dep_ptr"], np.int64), i64p)
+        d.dep_conn = ptr(arr(m["dep_conn"], np.int64), i64p)
+        d.dep_coef = ptr(arr(m["dep_coef"], np.float64), dp)
     return d, keep
 
 

@@ -143,6 +143,11 @@ struct ElemArgs {
   double scale, E, nu, big;
   int material;
   int bs;
+  // element-matrix output (constrained elements): shapes are active when the
+  // side's connect has shapes (aidx_ek >= 0 unconstrained, -2 constrained);
+  // the element matrix / load vector of list entry k go to ek_out / ef_out
+  const int64_t *aidx_ek;
+  double *ek_out, *ef_out;
 };
 
 __device__ inline void forcing(const ElemArgs &g, const double *x, double *f) {
@@ -167,7 +172,7 @@ __device__ inline void forcing(const ElemArgs &g, const double *x, double *f) {
 
 // Volume elements.  DIM: 2 (quad) / 3 (hex); NC corners; NSD sides; NSH shapes;
 // NS states.  One workgroup per element of the current colour.
-template <int DIM, int NC, int NSD, int NSH, int NS>
+template <int DIM, int NC, int NSD, int NSH, int NS, bool EK = false>
 __global__ __launch_bounds__(TPB) void k_vol(ElemArgs g, const int32_t *__restrict__ list) {
   constexpr int NE = NSH * NS;
   constexpr int NP = (NS == 3) ? 6 : 1;  // Gram blocks P^{ab}, a <= b
@@ -187,8 +192,9 @@ __global__ __launch_bounds__(TPB) void k_vol(ElemArgs g, const int32_t *__restri
   const int nq = T.nq;
 
   if (tid < NC * 3) s_coord[tid] = g.xyz[3 * g.nodes[e * NC + tid / 3] + tid % 3];
-  for (int k = tid; k < NSD * NSD; k += TPB) s_map[k] = g.map[e * NSD * NSD + k];
-  if (tid < NSH) s_blk[tid] = g.aidx[g.conn[e * NSD + T.shape_side[tid]]];
+  if (!EK)
+    for (int k = tid; k < NSD * NSD; k += TPB) s_map[k] = g.map[e * NSD * NSD + k];
+  if (tid < NSH) s_blk[tid] = (EK ? g.aidx_ek : g.aidx)[g.conn[e * NSD + T.shape_side[tid]]];
   __syncthreads();
   // geometry per quadrature point (TPZGeoEl::Jacobian, Mesh/pzgeoel.cpp:1168-1355)
   if (tid < nq) {
@@ -254,10 +260,11 @@ __global__ __launch_bounds__(TPB) void k_vol(ElemArgs g, const int32_t *__restri
     }
   }
   // row starts of the element's equations in the upper CSR
-  for (int k = tid; k < NE; k += TPB) {
-    int64_t b = s_blk[k / NS];
-    s_row[k] = b >= 0 ? g.ia[b * g.bs + k % NS] : -1;
-  }
+  if (!EK)
+    for (int k = tid; k < NE; k += TPB) {
+      int64_t b = s_blk[k / NS];
+      s_row[k] = b >= 0 ? g.ia[b * g.bs + k % NS] : -1;
+    }
   __syncthreads();
   // Gram blocks
   for (int k = tid; k < NP * NSH * NSH; k += TPB) {
@@ -283,6 +290,42 @@ __global__ __launch_bounds__(TPB) void k_vol(ElemArgs g, const int32_t *__restri
     C2 = E * nu / (-1. + nu + 2. * nu * nu);
     C3 = E * (nu - 1.) / (-1. + nu + 2. * nu * nu);
   }
+  auto entry = [&](int i, int ci, int j, int cj) -> double {
+    if (NS == 3) {
+      // P^{xy}[i][j] = sum_q w dphi_x(i) dphi_y(j)
+      auto P = [&](int x, int y, int ii, int jj) -> double {
+        const int idx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+        return x <= y ? s_P[idx[x][y]][ii][jj] : s_P[idx[y][x]][jj][ii];
+      };
+      if (ci == cj) {
+        double tr = 0.;
+        for (int cc = 0; cc < 3; cc++)
+          if (cc != ci) tr += P(cc, cc, i, j);
+        return tr * C1 + P(ci, ci, i, j) * C3;
+      }
+      return P(cj, ci, i, j) * C1 - P(ci, cj, i, j) * C2;
+    }
+    return coef * s_P[0][i][j];
+  };
+  if (EK) {
+    // the whole element matrix and load vector of list entry blockIdx.x
+    // (shapes of every side with shapes, constrained or not)
+    double *ek = g.ek_out + (int64_t)blockIdx.x * NE * NE;
+    for (int k = tid; k < NE * NE; k += TPB) {
+      const int I = k / NE, J = k % NE, i = I / NS, ci = I % NS, j = J / NS, cj = J % NS;
+      ek[k] = (s_blk[i] == -1 || s_blk[j] == -1) ? 0. : entry(i, ci, j, cj);
+    }
+    for (int I = tid; I < NE; I += TPB) {
+      const int i = I / NS, ci = I % NS;
+      double sf = 0.;
+      if (s_blk[i] != -1) {
+        for (int q = 0; q < nq; q++) sf += s_wdet[q] * T.phi[q][i] * s_f[q][ci];
+        if (g.material == PZG_MAT_POISSON) sf *= g.scale;
+      }
+      g.ef_out[(int64_t)blockIdx.x * NE + I] = sf;
+    }
+    return;
+  }
   // element matrix entries, upper part by global equation, scatter; SU
   // entries per thread at a time, their read-modify-writes issued together
   constexpr int SU = 8;
@@ -302,24 +345,7 @@ __global__ __launch_bounds__(TPB) void k_vol(ElemArgs g, const int32_t *__restri
     int64_t bi = s_blk[i], bj = s_blk[j];
     if (bi < 0 || bj < 0) continue;
     if (bi > bj || (bi == bj && (ci > cj || (ci == cj && i > j)))) continue;
-    double v;
-    if (NS == 3) {
-      // P^{xy}[i][j] = sum_q w dphi_x(i) dphi_y(j)
-      auto P = [&](int x, int y, int ii, int jj) -> double {
-        const int idx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
-        return x <= y ? s_P[idx[x][y]][ii][jj] : s_P[idx[y][x]][jj][ii];
-      };
-      if (ci == cj) {
-        double tr = 0.;
-        for (int cc = 0; cc < 3; cc++)
-          if (cc != ci) tr += P(cc, cc, i, j);
-        v = tr * C1 + P(ci, ci, i, j) * C3;
-      } else {
-        v = P(cj, ci, i, j) * C1 - P(ci, cj, i, j) * C2;
-      }
-    } else {
-      v = coef * s_P[0][i][j];
-    }
+    const double v = entry(i, ci, j, cj);
     if (fabs(v) < 1.e-12) continue;  // IsZero drop rule of AddKel
     if (bi == bj) slot[u] = s_row[I] + (cj - ci);
     else {
@@ -740,6 +766,100 @@ static int run_vol(Ctx &c) {
   return PZG_OK;
 }
 
+// Constrained elements (TPZElementMatrixT::ApplyConstraints, Mesh/
+// TPZElementMatrixT.cpp:134-322, and the assembly of StrMatrix/
+// pzstrmatrixor.cpp:273-280): with T[i][a] the weight of list entry a in
+// shape i (1 for the shape's own unconstrained connect, the dependency
+// coefficients for a constrained one), K_c = T^T K T and F_c = T^T F over the
+// element's unconstrained blocks, added with AddKel's upper-part rule and
+// drop rule |v| < 1e-12, and AddFel.  One workgroup per element of a colour
+// (elements of a colour share no block).
+template <int NSH, int NS>
+__global__ __launch_bounds__(TPB) void k_constrain(const int32_t *__restrict__ slots,
+                                                   const double *__restrict__ ek,
+                                                   const double *__restrict__ ef,
+                                                   const double *__restrict__ Tm, int nsv,
+                                                   const int64_t *__restrict__ blk,
+                                                   const int32_t *__restrict__ ncl, const int64_t *__restrict__ ia,
+                                                   const int64_t *__restrict__ ja, double *__restrict__ a,
+                                                   double *__restrict__ rhs, int bs) {
+  constexpr int NE = NSH * NS, NCS = CEL_NC * NS;
+  __shared__ double s_T[NSH][CEL_NC];
+  __shared__ double s_W[NE][NCS];  // W = K T
+  __shared__ double s_K[NE][NE];
+  __shared__ int64_t s_b[CEL_NC];
+  const int tid = threadIdx.x;
+  const int64_t k = slots[blockIdx.x];
+  const int n = ncl[k];
+  for (int x = tid; x < NSH * CEL_NC; x += TPB) s_T[x / CEL_NC][x % CEL_NC] = Tm[(k * nsv + x / CEL_NC) * CEL_NC + x % CEL_NC];
+  for (int x = tid; x < NE * NE; x += TPB) s_K[x / NE][x % NE] = ek[k * NE * NE + x];
+  if (tid < CEL_NC) s_b[tid] = blk[k * CEL_NC + tid];
+  __syncthreads();
+  for (int x = tid; x < NE * NCS; x += TPB) {
+    const int I = x / NCS, B = x % NCS, b = B / NS, cb = B % NS;
+    double w = 0.;
+    for (int j = 0; j < NSH; j++) w += s_K[I][j * NS + cb] * s_T[j][b];
+    s_W[I][B] = w;
+  }
+  __syncthreads();
+  for (int x = tid; x < n * NS * n * NS; x += TPB) {
+    const int A = x / (n * NS), B = x % (n * NS), aa = A / NS, ca = A % NS;
+    const int64_t ra = s_b[aa] * bs + ca, rb = s_b[B / NS] * bs + B % NS;
+    if (rb < ra) continue;
+    double v = 0.;
+    for (int i = 0; i < NSH; i++) v += s_T[i][aa] * s_W[i * NS + ca][(B / NS) * NS + B % NS];
+    if (fabs(v) < 1.e-12) continue;
+    int64_t l = ia[ra], h = ia[ra + 1] - 1;
+    while (l <= h) {
+      const int64_t m = (l + h) >> 1;
+      if (ja[m] == rb) {
+        a[m] += v;
+        break;
+      }
+      if (ja[m] < rb) l = m + 1;
+      else h = m - 1;
+    }
+  }
+  for (int A = tid; A < n * NS; A += TPB) {
+    const int aa = A / NS, ca = A % NS;
+    double f = 0.;
+    for (int i = 0; i < NSH; i++) f += s_T[i][aa] * ef[k * NE + i * NS + ca];
+    rhs[s_b[aa] * bs + ca] += f;
+  }
+}
+
+template <int DIM, int NC, int NSD, int NSH, int NS>
+static int run_constrained(Ctx &c) {
+  if (!c.has_dep || c.cel.empty()) return PZG_OK;
+  constexpr int NE = NSH * NS;
+  const int64_t ncel = (int64_t)c.cel.size();
+  PZG_TRY(dev_alloc(c.d_cel_ek, sizeof(double) * ncel * NE * NE));
+  PZG_TRY(dev_alloc(c.d_cel_ef, sizeof(double) * ncel * NE));
+  ElemArgs g = make_args(c, true);
+  g.aidx_ek = c.d_aidx_ek.as<int64_t>();
+  for (size_t b = 0; b + 1 < c.cel_bptr.size(); b++) {
+    const int64_t n = c.cel_bptr[b + 1] - c.cel_bptr[b];
+    if (!n) continue;
+    g.tab = c.d_tvol.as<ElemTables>() + b;
+    g.ek_out = c.d_cel_ek.as<double>() + c.cel_bptr[b] * NE * NE;
+    g.ef_out = c.d_cel_ef.as<double>() + c.cel_bptr[b] * NE;
+    hipLaunchKernelGGL((k_vol<DIM, NC, NSD, NSH, NS, true>), dim3(n), dim3(TPB), 0, c.stream, g,
+                       c.d_cel.as<int32_t>() + c.cel_bptr[b]);
+    PZG_HIP(hipGetLastError());
+  }
+  for (size_t k = 0; k + 1 < c.cel_cptr.size(); k++) {
+    const int64_t n = c.cel_cptr[k + 1] - c.cel_cptr[k];
+    if (!n) continue;
+    hipLaunchKernelGGL((k_constrain<NSH, NS>), dim3(n), dim3(TPB), 0, c.stream,
+                       c.d_cel_cslot.as<int32_t>() + c.cel_cptr[k], c.d_cel_ek.as<double>(),
+                       c.d_cel_ef.as<double>(), c.d_cel_T.as<double>(), c.vol_nsides,
+                       c.d_cel_blk.as<int64_t>(), c.d_cel_n.as<int32_t>(), c.d_ia.as<int64_t>(),
+                       c.d_ja.as<int64_t>(), c.d_a.as<double>(), c.d_rhs.as<double>(), c.bs);
+    PZG_HIP(hipGetLastError());
+  }
+  return PZG_OK;
+}
+
 template <int DIMB, int NC, int NSD, int NSH, int NS>
 static int run_bc(Ctx &c) {
   if (c.nbc == 0) return PZG_OK;
@@ -904,8 +1024,15 @@ int launch_assemble(Ctx &c) {
     if (p == 2) { PZG_TRY((run_vol<3, 8, 27, 27, 1>(c))); PZG_TRY((run_bc<2, 4, 9, 9, 1>(c))); }
     else { PZG_TRY((run_vol<3, 8, 27, 8, 1>(c))); PZG_TRY((run_bc<2, 4, 9, 4, 1>(c))); }
   } else if (c.dim == 2 && c.nstate == 1) {
-    if (p == 2) { PZG_TRY((run_vol<2, 4, 9, 9, 1>(c))); PZG_TRY((run_bc<1, 2, 3, 3, 1>(c))); }
-    else { PZG_TRY((run_vol<2, 4, 9, 4, 1>(c))); PZG_TRY((run_bc<1, 2, 3, 2, 1>(c))); }
+    if (p == 2) {
+      PZG_TRY((run_vol<2, 4, 9, 9, 1>(c)));
+      PZG_TRY((run_constrained<2, 4, 9, 9, 1>(c)));
+      PZG_TRY((run_bc<1, 2, 3, 3, 1>(c)));
+    } else {
+      PZG_TRY((run_vol<2, 4, 9, 4, 1>(c)));
+      PZG_TRY((run_constrained<2, 4, 9, 4, 1>(c)));
+      PZG_TRY((run_bc<1, 2, 3, 2, 1>(c)));
+    }
   } else {
     set_error("pzg_assemble: unsupported dimension/state combination");
     return PZG_EUNSUP;

@@ -70,7 +70,9 @@ static void free_all(Ctx &c) {
                     &c.d_hp_vsub, &c.d_hp_bsub, &c.d_thp_vol, &c.d_thp_bc, &c.d_mf_blk,
                     &c.d_mf_xe, &c.d_mf_bblk, &c.d_mf_bm, &c.d_mf_ptr, &c.d_mf_pos, &c.d_mf_bpos,
                     &c.d_mf_tab, &c.d_mf_ye, &c.d_mf_coef, &c.d_mf_dis, &c.d_mf_mfma, &c.d_mf_geo,
-                    &c.d_mf_pt};
+                    &c.d_mf_pt, &c.d_cel, &c.d_cel_cslot, &c.d_cel_T, &c.d_cel_blk, &c.d_cel_n,
+                    &c.d_cel_ek, &c.d_cel_ef, &c.d_aidx_ek, &c.d_dep_ptr, &c.d_dep_blk,
+                    &c.d_dep_coef};
   for (DevBuf *b : bufs) dev_free(*b);
 }
 
@@ -287,16 +289,59 @@ static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
     }
   }
 
+  // constrained connects (hanging sides)
+  c.has_dep = false;
+  c.dep_ptr.clear();
+  c.dep_conn.clear();
+  c.dep_coef.clear();
+  c.is_dep.assign(c.nconn, 0);
+  if (m->dep_ptr) {
+    if (!m->dep_conn || !m->dep_coef || m->dep_ptr[0] != 0) {
+      set_error("pzg_mesh_set: incomplete constraint arrays");
+      return PZG_EINVAL;
+    }
+    c.dep_ptr.assign(m->dep_ptr, m->dep_ptr + c.nconn + 1);
+    for (int64_t s = 0; s < c.nconn; s++)
+      if (c.dep_ptr[s + 1] < c.dep_ptr[s]) {
+        set_error("pzg_mesh_set: dep_ptr is not monotone");
+        return PZG_EINVAL;
+      }
+    c.dep_conn.assign(m->dep_conn, m->dep_conn + c.dep_ptr[c.nconn]);
+    c.dep_coef.assign(m->dep_coef, m->dep_coef + c.dep_ptr[c.nconn]);
+    for (int64_t s = 0; s < c.nconn; s++) c.is_dep[s] = c.dep_ptr[s + 1] > c.dep_ptr[s];
+    for (int64_t s = 0; s < c.nconn; s++)
+      for (int64_t k = c.dep_ptr[s]; k < c.dep_ptr[s + 1]; k++)
+        if (c.dep_conn[k] < 0 || c.dep_conn[k] >= c.nconn || c.is_dep[c.dep_conn[k]]) {
+          set_error("pzg_mesh_set: a connect depends on a missing or constrained connect");
+          return PZG_EINVAL;
+        }
+    c.has_dep = c.dep_ptr[c.nconn] > 0;
+    if (c.has_dep && (c.dim != 2 || c.hp_simplex)) {
+      set_error("pzg_mesh_set: constrained connects on quadrilateral meshes only");
+      return PZG_EUNSUP;
+    }
+  }
+
   // Uniform active block size: every (virtual) connect carries 0 or nstate
-  // equations, block positions must be consecutive in seq order.
+  // equations, block positions must be consecutive in seq order; the blocks
+  // of constrained connects carry no equations and come last.
   c.bs = c.nstate;
   c.aidx.assign(c.nconn, -1);
   int64_t nb = 0, pos = 0;
+  bool seen_dep = false;
   for (int64_t s = 0; s < c.nconn; s++) {
     int sz = c.blk_size[s];
     if (sz != 0 && sz != c.bs) {
       set_error("pzg_mesh_set: block sizes other than 0 or nstate are not supported");
       return PZG_EUNSUP;
+    }
+    if (c.is_dep[s]) {
+      seen_dep |= sz > 0;
+      continue;
+    }
+    if (sz && seen_dep) {
+      set_error("pzg_mesh_set: constrained connects must come after the others in sequence order");
+      return PZG_EINVAL;
     }
     if (c.blk_pos[s] != pos) {
       set_error("pzg_mesh_set: Block() positions are not cumulative in sequence order");
@@ -385,8 +430,97 @@ static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
     subs(PZG_TRI, c.bc_nsides, c.bc_conn, c.nbc, c.bc_bucket, c.bc_rules.size(), c.hp_bc_P, bsub);
   }
 
+  // constrained elements (hanging sides): the volume elements with a
+  // constrained connect that has shapes, grouped by rule bucket; per element
+  // the unconstrained blocks of its connects and of the connects they depend
+  // on (TPZConnect::BuildConnectList, sorted by connect) and T
+  c.cel.clear();
+  c.is_cel.assign(c.nvol, 0);
+  c.cel_xptr.assign(c.nvol + 1, 0);
+  c.cel_xconn.clear();
+  c.cel_bptr.assign(c.vol_rules.size() + 1, 0);
+  std::vector<double> celT;
+  std::vector<int64_t> celB;
+  std::vector<int32_t> celN;
+  std::vector<int64_t> aidx_ek(c.nconn), dep_blk;
+  for (int64_t s = 0; s < c.nconn; s++)
+    aidx_ek[s] = c.aidx[s] >= 0 ? c.aidx[s] : (c.is_dep[s] && c.blk_size[s] > 0 ? -2 : -1);
+  if (c.has_dep) {
+    const int nsv = c.vol_nsides, nsb = c.bc_nsides;
+    for (int64_t e = 0; e < c.nbc; e++)
+      for (int k = 0; k < nsb; k++) {
+        const int64_t cc = c.bc_conn[e * nsb + k];
+        if (c.is_dep[cc] && c.blk_size[cc] > 0) {
+          set_error("pzg_mesh_set: boundary elements on constrained connects are not supported");
+          return PZG_EUNSUP;
+        }
+      }
+    for (int64_t e = 0; e < c.nvol; e++) {
+      for (int k = 0; k < nsv; k++) {
+        const int64_t cc = c.vol_conn[e * nsv + k];
+        if (!c.is_dep[cc] || c.blk_size[cc] == 0) continue;
+        c.is_cel[e] = 1;
+        for (int64_t d = c.dep_ptr[cc]; d < c.dep_ptr[cc + 1]; d++) c.cel_xconn.push_back(c.dep_conn[d]);
+      }
+      c.cel_xptr[e + 1] = (int64_t)c.cel_xconn.size();
+    }
+    for (size_t b = 0; b < c.vol_rules.size(); b++) {
+      for (int64_t e = 0; e < c.nvol; e++)
+        if (c.is_cel[e] && c.vol_bucket[e] == (int32_t)b) c.cel.push_back((int32_t)e);
+      c.cel_bptr[b + 1] = (int64_t)c.cel.size();
+    }
+    const int nsh = c.tvol_all[0].nsh;
+    for (int32_t e : c.cel) {
+      std::vector<int64_t> L;
+      for (int k = 0; k < nsv; k++) L.push_back(c.vol_conn[(int64_t)e * nsv + k]);
+      for (int64_t x = c.cel_xptr[e]; x < c.cel_xptr[e + 1]; x++) L.push_back(c.cel_xconn[x]);
+      std::sort(L.begin(), L.end());
+      L.erase(std::unique(L.begin(), L.end()), L.end());
+      std::vector<int64_t> B, Lc;
+      for (int64_t cc : L)
+        if (c.aidx[cc] >= 0) {
+          B.push_back(c.aidx[cc]);
+          Lc.push_back(cc);
+        }
+      if ((int)B.size() > CEL_NC) {
+        set_error("pzg_mesh_set: a constrained element couples too many connects");
+        return PZG_EUNSUP;
+      }
+      auto slot = [&](int64_t cc) {
+        return (int)(std::lower_bound(Lc.begin(), Lc.end(), cc) - Lc.begin());
+      };
+      const size_t t0 = celT.size();
+      celT.resize(t0 + (size_t)nsv * CEL_NC, 0.);
+      for (int i = 0; i < nsh; i++) {  // p <= 2: shape i lives on side i
+        const int64_t cc = c.vol_conn[(int64_t)e * nsv + c.tvol_all[0].shape_side[i]];
+        if (c.blk_size[cc] == 0) continue;
+        if (c.aidx[cc] >= 0) {
+          celT[t0 + (size_t)i * CEL_NC + slot(cc)] = 1.;
+          continue;
+        }
+        for (int64_t d = c.dep_ptr[cc]; d < c.dep_ptr[cc + 1]; d++)
+          if (c.aidx[c.dep_conn[d]] >= 0)
+            celT[t0 + (size_t)i * CEL_NC + slot(c.dep_conn[d])] += c.dep_coef[d];
+      }
+      for (int a = 0; a < CEL_NC; a++) celB.push_back(a < (int)B.size() ? B[a] : -1);
+      celN.push_back((int32_t)B.size());
+    }
+    dep_blk.resize(c.dep_conn.size());
+    for (size_t k = 0; k < c.dep_conn.size(); k++) dep_blk[k] = c.aidx[c.dep_conn[k]];
+  }
+
   hipStream_t s = c.stream;
   PZG_TRY(dev_upload(c.d_xyz, c.xyz.data(), c.xyz.size(), s));
+  PZG_TRY(dev_upload(c.d_aidx_ek, aidx_ek.data(), aidx_ek.size(), s));
+  if (c.has_dep) {
+    PZG_TRY(dev_upload(c.d_cel, c.cel.data(), c.cel.size(), s));
+    PZG_TRY(dev_upload(c.d_cel_T, celT.data(), celT.size(), s));
+    PZG_TRY(dev_upload(c.d_cel_blk, celB.data(), celB.size(), s));
+    PZG_TRY(dev_upload(c.d_cel_n, celN.data(), celN.size(), s));
+    PZG_TRY(dev_upload(c.d_dep_ptr, c.dep_ptr.data(), c.dep_ptr.size(), s));
+    PZG_TRY(dev_upload(c.d_dep_blk, dep_blk.data(), dep_blk.size(), s));
+    PZG_TRY(dev_upload(c.d_dep_coef, c.dep_coef.data(), c.dep_coef.size(), s));
+  }
   PZG_TRY(dev_upload(c.d_vnodes, c.vol_nodes.data(), c.vol_nodes.size(), s));
   PZG_TRY(dev_upload(c.d_vconn, c.vol_conn.data(), c.vol_conn.size(), s));
   if (!c.vol_coef.empty()) PZG_TRY(dev_upload(c.d_vcoef, c.vol_coef.data(), c.vol_coef.size(), s));
@@ -438,10 +572,18 @@ static int struct_create(Ctx &c) {
   const int64_t nb = c.nb, bs = c.bs;
   const int nsv = c.vol_nsides, nsb = c.bc_nsides;
   const int64_t nel = c.nvol + c.nbc;
+  // a constrained element also couples the connects its constrained ones
+  // depend on (TPZCompMesh::ComputeElGraph over BuildConnectList)
+  const bool dep = c.has_dep;
   auto elem_conn = [&](int64_t e, int s) -> int64_t {
-    return e < c.nvol ? c.aidx[c.vol_conn[e * nsv + s]] : c.aidx[c.bc_conn[(e - c.nvol) * nsb + s]];
+    if (e >= c.nvol) return c.aidx[c.bc_conn[(e - c.nvol) * nsb + s]];
+    if (s < nsv) return c.aidx[c.vol_conn[e * nsv + s]];
+    return c.aidx[c.cel_xconn[c.cel_xptr[e] + s - nsv]];
   };
-  auto elem_ns = [&](int64_t e) { return e < c.nvol ? nsv : nsb; };
+  auto elem_ns = [&](int64_t e) -> int {
+    if (e >= c.nvol) return nsb;
+    return dep ? nsv + (int)(c.cel_xptr[e + 1] - c.cel_xptr[e]) : nsv;
+  };
   // block -> element incidence
   std::vector<int64_t> bptr(nb + 1, 0);
   for (int64_t e = 0; e < nel; e++)
@@ -555,6 +697,46 @@ static int struct_create(Ctx &c) {
   std::vector<int32_t> vlist, blist;
   PZG_TRY(color_elements(c.nvol, nsv, c.vol_conn, c.aidx, nb, vptr, vlist));
   group_by_bucket(vptr, vlist, c.vol_bucket, (int)c.vol_rules.size());
+  if (c.has_dep) {
+    std::vector<int64_t> p2(vptr.size(), 0);
+    std::vector<int32_t> l2;
+    for (size_t k = 0; k + 1 < vptr.size(); k++) {
+      for (int64_t i = vptr[k]; i < vptr[k + 1]; i++)
+        if (!c.is_cel[vlist[i]]) l2.push_back(vlist[i]);
+      p2[k + 1] = (int64_t)l2.size();
+    }
+    vptr.swap(p2);
+    vlist.swap(l2);
+    // constrained elements: first-fit colours over their unconstrained blocks
+    const int64_t ncel = (int64_t)c.cel.size();
+    std::vector<uint64_t> used(nb, 0);
+    std::vector<int> col(ncel);
+    int ncol = 0;
+    for (int64_t k = 0; k < ncel; k++) {
+      const int32_t e = c.cel[k];
+      std::vector<int64_t> bl;
+      for (int q = 0; q < elem_ns(e); q++)
+        if (elem_conn(e, q) >= 0) bl.push_back(elem_conn(e, q));
+      uint64_t msk = 0;
+      for (int64_t b : bl) msk |= used[b];
+      if (msk == ~0ULL) {
+        set_error("constrained element colouring needs more than 64 colours");
+        return PZG_EUNSUP;
+      }
+      col[k] = __builtin_ctzll(~msk);
+      ncol = std::max(ncol, col[k] + 1);
+      for (int64_t b : bl) used[b] |= 1ULL << col[k];
+    }
+    c.cel_cptr.assign(ncol + 1, 0);
+    for (int64_t k = 0; k < ncel; k++) c.cel_cptr[col[k] + 1]++;
+    for (int k = 0; k < ncol; k++) c.cel_cptr[k + 1] += c.cel_cptr[k];
+    std::vector<int32_t> cslot(ncel);
+    std::vector<int64_t> fill(c.cel_cptr.begin(), c.cel_cptr.end() - 1);
+    for (int64_t k = 0; k < ncel; k++) cslot[fill[col[k]]++] = (int32_t)k;
+    PZG_TRY(dev_upload(c.d_cel_cslot, cslot.data(), cslot.size(), s));
+  } else {
+    c.cel_cptr.assign(1, 0);
+  }
   c.vcolor_ptr = vptr;
   PZG_TRY(dev_upload(c.d_vcolor_list, vlist.data(), vlist.size(), s));
   if (c.nbc > 0) {

@@ -194,9 +194,27 @@ struct Ctx {
   std::vector<int> hp_vol_var;
   std::vector<float> hp_vol_ms;
 
-  // connect -> active block index (-1 for size-0 connects)
+  // connect -> active block index (-1 for size-0 and constrained connects)
   std::vector<int64_t> aidx;
   int64_t nb = 0;  // active blocks
+
+  // constrained connects (hanging sides): dependencies per connect, and the
+  // volume elements that hold a constrained connect with shapes.  Such an
+  // element is assembled apart: its element matrix (shapes of every side)
+  // goes to a buffer, then K_c = T^T K T over its constrained list (the
+  // unconstrained blocks of its connects and of those they depend on), and
+  // K_c is added with AddKel's drop rule.  T[i][a]: shape i -> list entry a.
+  bool has_dep = false;
+  std::vector<int64_t> dep_ptr, dep_conn;
+  std::vector<double> dep_coef;
+  std::vector<char> is_dep;
+  std::vector<int32_t> cel;               // constrained elements, by rule bucket
+  std::vector<char> is_cel;               // per volume element
+  std::vector<int64_t> cel_xptr, cel_xconn;  // per volume element: connects it depends on
+  std::vector<int64_t> cel_bptr;          // bucket b: cel[cel_bptr[b] .. cel_bptr[b+1])
+  std::vector<int64_t> cel_cptr;          // colour k: d_cel_cslot[cel_cptr[k] .. ]
+  DevBuf d_cel, d_cel_cslot, d_cel_T, d_cel_blk, d_cel_n, d_cel_ek, d_cel_ef, d_aidx_ek;
+  DevBuf d_dep_ptr, d_dep_blk, d_dep_coef;
 
   // ---- pattern (device) -------------------------------------------------
   bool has_struct = false, external_csr = false;
@@ -261,6 +279,9 @@ struct Ctx {
       d_post_xq;
 };
 
+// constrained elements: at most CEL_NC unconstrained connects in the list
+constexpr int CEL_NC = 16;
+
 // ---- element tables (pzg_shape.cpp) ----------------------------------------
 int build_tables(int etype, int p, int rule_order, ElemTables &t);
 // high-order simplex tables of order P at the rule of rule_order
@@ -311,7 +332,7 @@ int mf_apply(Ctx &c, const float *p, const float *r, float *q, const SolveScalar
              double *partials, int *gather_grid, bool scaled_ready = false);
 // the matrix-free operator reproduces the assembled K (mesh-built, no AddKel extras)
 inline bool mf_available(const Ctx &c) {
-  return c.has_mesh && c.has_struct && !c.external_csr && !c.kel_added && c.assembled &&
+  return c.has_mesh && c.has_struct && !c.external_csr && !c.kel_added && c.assembled && !c.has_dep &&
          c.vol_rules.size() == 1 && c.bc_rules.size() <= 1 && c.vol_etype != PZG_TET;
 }
 int solve_pcg64(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info);

@@ -30,6 +30,8 @@ struct PostArgs {
   const int64_t *nodes, *conn, *aidx, *off;
   const int32_t *list, *slot;  // slot: col_off[HP_MAXSH] | side[HP_MAXSH]
   const double *coef, *u, *tab, *uq, *duq;
+  const int64_t *dep_ptr, *dep_blk;  // constrained connects (may be null)
+  const double *dep_coef;
   double *out, *xq;
   int64_t nlist;
   int nq, ncol, nslot, nsv, nc, dim, bs, G, hp, tet, material, kind, nerr;
@@ -153,8 +155,17 @@ __global__ __launch_bounds__(PTPB) void k_post(const PostArgs a) {
     for (int i = lq; i < a.nslot; i += G) {
       s_col[i] = a.hp ? a.slot[i] + hp_variant(a.tet, a.slot[HP_MAXSH + i], id) : i;
       if (!POINTS) {
-        const int64_t b = a.aidx[a.conn[e * a.nsv + i]];
-        for (int k = 0; k < NS; k++) s_u[i * NS + k] = b >= 0 ? a.u[b * a.bs + k] : 0.;
+        // a constrained connect's value follows from the connects it depends
+        // on (TPZCompMesh::LoadSolution expands the solution)
+        const int64_t cc = a.conn[e * a.nsv + i], b = a.aidx[cc];
+        for (int k = 0; k < NS; k++) {
+          double v = 0.;
+          if (b >= 0) v = a.u[b * a.bs + k];
+          else if (a.dep_ptr)
+            for (int64_t d = a.dep_ptr[cc]; d < a.dep_ptr[cc + 1]; d++)
+              if (a.dep_blk[d] >= 0) v += a.dep_coef[d] * a.u[a.dep_blk[d] * a.bs + k];
+          s_u[i * NS + k] = v;
+        }
       }
     }
     for (int k = lq; k < a.nc * 3; k += G) s_x[k] = a.xyz[3 * a.nodes[e * a.nc + k / 3] + k % 3];
@@ -378,6 +389,11 @@ static PostArgs post_args(const Ctx &c) {
   a.E = c.E;
   a.nu = c.nu;
   a.kconst = c.scale;
+  if (c.has_dep) {
+    a.dep_ptr = c.d_dep_ptr.as<int64_t>();
+    a.dep_blk = c.d_dep_blk.as<int64_t>();
+    a.dep_coef = c.d_dep_coef.as<double>();
+  }
   return a;
 }
 

@@ -405,3 +405,145 @@ def save(m, path):
         arr(f, m.get("vol_rule"), np.int32)
         arr(f, m.get("bc_rule"), np.int32)
         i32(f, m.get("vol_etype", 0))
+
+
+def build_refined2d(pb, refine):
+    """A quadrilateral mesh of the unit square with hanging sides: the coarse
+    nx x ny elements flagged in ``refine`` are divided into four (children in
+    corner order, TPZRefQuad), the others stay.  Returns the flattened mesh
+    (volume elements in coarse order, children in place of their parent) with
+    the constraints of the hanging sides:
+
+    * the middle node M of a coarse edge (A, B) next to a divided element
+      depends on the coarse edge's connects A, B and, for p = 2, its edge
+      connect E with coefficients 1/2, 1/2, 1 (the coarse side's functions at
+      M);
+    * for p = 2 each half edge's connect depends on A, B, E with 0, 0, 1/4
+      (the coarse edge function restricted to the half edge is 1/4 of the
+      half edge's function plus a linear part).
+
+    These are the restraints TPZInterpolatedElement::RestrainSide computes by
+    projection (Mesh/pzintel.cpp:847-1010): only the side's own connect is
+    restrained, against every connect of the large side that has shapes, the
+    block of a negligible coefficient zeroed but kept.  Connects are numbered
+    by first touch over the elements (sides in order); the constrained ones
+    then move behind the others, keeping their order, so that their blocks
+    hold no equations.  The element coefficients of ``pb.el_coef`` (one per
+    coarse element) pass to the children."""
+    if pb.dim != 2 or pb.p > 2 or pb.el_porder is not None:
+        raise ValueError("build_refined2d: quadrilaterals, uniform p <= 2")
+    nx, ny = pb.nx, pb.ny
+    refine = np.asarray(refine, dtype=bool).reshape(ny, nx)
+    NX = 2 * nx + 1
+    lat = lambda I, J: I + NX * J
+    els, parent = [], []
+    for j in range(ny):
+        for i in range(nx):
+            I, J = 2 * i, 2 * j
+            if refine[j, i]:
+                for (a, b) in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                    I0, J0 = I + a, J + b
+                    els.append([lat(I0, J0), lat(I0 + 1, J0), lat(I0 + 1, J0 + 1), lat(I0, J0 + 1)])
+                    parent.append(i + nx * j)
+            else:
+                els.append([lat(I, J), lat(I + 2, J), lat(I + 2, J + 2), lat(I, J + 2)])
+                parent.append(i + nx * j)
+    vol_nodes = np.array(els, dtype=np.int64)
+    nel = len(els)
+    # side keys: corner = lattice node; edge = ('e', min, max); interior = ('i', e)
+    keymap, seq_of = {}, []
+    econn = np.zeros((nel, 9), dtype=np.int64)
+    for e, cn in enumerate(els):
+        keys = [("n", c) for c in cn]
+        keys += [("e", min(cn[k], cn[(k + 1) % 4]), max(cn[k], cn[(k + 1) % 4])) for k in range(4)]
+        keys.append(("i", e))
+        for s, k in enumerate(keys):
+            if k not in keymap:
+                keymap[k] = len(keymap)
+                seq_of.append(k)
+            econn[e, s] = keymap[k]
+    nconn = len(keymap)
+    # hanging sides: a coarse edge (A, B) whose middle node is a corner of
+    # some element; the coarse element holding the edge is the large side
+    dep = {}
+    for e, cn in enumerate(els):
+        for k in range(4):
+            A, B = cn[k], cn[(k + 1) % 4]
+            mid = (A + B) // 2
+            if (A + B) % 2 or ("n", mid) not in keymap:
+                continue
+            if abs(A - B) not in (2, 2 * NX):
+                continue  # not a coarse edge
+            cA, cB, cE = keymap[("n", A)], keymap[("n", B)], econn[e, 4 + k]
+            large = [cA, cB] + ([cE] if pb.p == 2 else [])
+            dep[keymap[("n", mid)]] = (large, [0.5, 0.5] + ([1.0] if pb.p == 2 else []))
+            if pb.p == 2:
+                for h in ((min(A, mid), max(A, mid)), (min(mid, B), max(mid, B))):
+                    dep[keymap[("e",) + h]] = (large, [0.0, 0.0, 0.25])
+    # boundary elements: boundary edges of the elements, face by face
+    xs = lambda I: I / (2.0 * nx)
+    ys = lambda J: J / (2.0 * ny)
+    bn, bk, bid = [], [], []
+    for f in range(4):
+        if pb.bc_type[f] < 0:
+            continue
+        for e, cn in enumerate(els):
+            for k in range(4):
+                A, B = cn[k], cn[(k + 1) % 4]
+                IA, JA, IB, JB = A % NX, A // NX, B % NX, B // NX
+                on = {0: IA == 0 and IB == 0, 1: IA == 2 * nx and IB == 2 * nx,
+                      2: JA == 0 and JB == 0, 3: JA == 2 * ny and JB == 2 * ny}[f]
+                if on:
+                    lo, hi = (A, B) if (IA, JA) < (IB, JB) else (B, A)
+                    bn.append((lo, hi))
+                    bk.append((keymap[("n", lo)], keymap[("n", hi)], econn[e, 4 + k]))
+                    bid.append(f)
+    bc_nodes = np.array(bn, dtype=np.int64).reshape(-1, 2)
+    bc_conn = np.array(bk, dtype=np.int64).reshape(-1, 3)
+    bc_id = np.array(bid, dtype=np.int32)
+    # constrained connects behind the others (relative order kept)
+    is_dep = np.zeros(nconn, dtype=bool)
+    is_dep[list(dep.keys())] = True
+    order = np.concatenate([np.flatnonzero(~is_dep), np.flatnonzero(is_dep)])
+    perm = np.empty(nconn, dtype=np.int64)
+    perm[order] = np.arange(nconn)
+    econn = perm[econn]
+    bc_conn = perm[bc_conn] if len(bc_conn) else bc_conn
+    kinds = np.array([{"n": K_CORNER, "e": K_EDGE, "i": K_QFACE}[k[0]] for k in seq_of])[order]
+    ns = pb.nstate
+    nshape = np.where(kinds == K_CORNER, 1, 1 if pb.p == 2 else 0)
+    blk_size = (nshape * ns).astype(np.int32)
+    blk_pos = np.zeros(nconn, dtype=np.int64)
+    blk_pos[1:] = np.cumsum(blk_size[:-1].astype(np.int64))
+    ndep = int(is_dep.sum())
+    neq = int(blk_size[: nconn - ndep].astype(np.int64).sum())
+    dep_ptr = np.zeros(nconn + 1, dtype=np.int64)
+    dc, dv = [], []
+    for newc in range(nconn):
+        oldc = order[newc]
+        if oldc in dep:
+            large, coef = dep[oldc]
+            dc += [int(perm[c]) for c in large]
+            dv += coef
+        dep_ptr[newc + 1] = len(dc)
+    xyz = np.zeros((NX * (2 * ny + 1), 3))
+    xyz[:, 0] = xs(np.arange(NX * (2 * ny + 1)) % NX)
+    xyz[:, 1] = ys(np.arange(NX * (2 * ny + 1)) // NX)
+    coef = None if pb.el_coef is None else np.asarray(pb.el_coef, dtype=np.float64)[parent]
+    rule = 2 * pb.p
+    big = 1.0e12 if pb.material == P.ELAST3D else 10.0 ** 17 * 2 / 3
+    return dict(
+        vol_rule=None, bc_rule=None, dim=2, material=pb.material, nstate=ns, porder=pb.p,
+        xyz=xyz.ravel(), vol_nodes=vol_nodes, vol_conn=econn, vol_coef=coef,
+        bc_nodes=bc_nodes, bc_conn=bc_conn, bc_id=bc_id,
+        bc_type=np.array([pb.bc_type[f] for f in range(6)], dtype=np.int32),
+        bc_val=np.array([pb.bc_val[f] for f in range(6)], dtype=np.float64).ravel(),
+        nconn=nconn, blk_pos=blk_pos, blk_size=blk_size, neq=neq,
+        scale=pb.scale, E=pb.E, nu=pb.nu, rule_order_vol=rule, rule_order_bc=2 * pb.p,
+        force_kind=pb.force_kind, force_const=np.array(pb.force_const, dtype=np.float64),
+        bumps=None if pb.bumps is None else np.ascontiguousarray(pb.bumps, dtype=np.float64).reshape(-1, 6),
+        bump_width=pb.bump_width, big_number=big, bc_faces=[f for f in range(6) if pb.bc_type[f] >= 0],
+        vol_etype=0, dep_ptr=dep_ptr, dep_conn=np.array(dc, dtype=np.int64),
+        dep_coef=np.array(dv, dtype=np.float64), conn_order=np.where(kinds == K_CORNER, 1, pb.p),
+        leaf_parent=np.array(parent, dtype=np.int64),
+    )

@@ -101,6 +101,13 @@ def lib():
         L.orc_error_points.argtypes = [P, M, C.c_int, dp]
         L.orc_error_points.restype = C.c_int64
         L.orc_evaluate_error.argtypes = [P, M, dp, C.POINTER(_Exact), dp, dp]
+        ip = C.POINTER(C.c_int)
+        L.orc_mesh_from_arrays.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64, dp,
+                                           C.c_int64, i64p, i64p, C.c_int64, i64p, i64p, ip,
+                                           C.c_int64, i64p, ip, ip, C.c_int64, M]
+        L.orc_setup_pattern_constrained.argtypes = [M, i64p, i64p, C.POINTER(i64p), C.POINTER(i64p)]
+        L.orc_setup_pattern_constrained.restype = C.c_int64
+        L.orc_assemble_constrained.argtypes = [P, M, i64p, i64p, dp, i64p, i64p, dp, dp]
         L.orc_sloan.argtypes = [C.c_int64, i64p, i64p, i64p]
         L.orc_mesh_sloan.argtypes = [M, i64p]
         L.orc_nnz_closed_form.argtypes = [C.c_int] * 6
@@ -192,15 +199,24 @@ class Oracle:
     access) describing material, mesh sizes, coefficients, loads and BCs.
     """
 
-    def __init__(self, spec):
+    def __init__(self, spec, mesh=None):
+        """With ``mesh`` (a flattened description, e.g. of
+        ``neopz_amd.mesh.build_refined2d``) the oracle takes that mesh's
+        elements, connects, numbering and constraints instead of building the
+        structured mesh of ``spec``; ``spec`` still gives the material, loads
+        and boundary conditions (``el_coef`` per element of ``mesh``)."""
         self.spec = spec
         self._keep = []
+        self.dep = None
         pb = _Problem()
         pb.material, pb.dim, pb.p = spec.material, spec.dim, spec.p
         pb.nx, pb.ny, pb.nz = spec.nx, spec.ny, spec.nz
         pb.scale, pb.E, pb.nu = spec.scale, spec.E, spec.nu
-        if spec.el_coef is not None:
-            c = np.ascontiguousarray(spec.el_coef, dtype=np.float64)
+        coef = spec.el_coef
+        if mesh is not None:  # coefficients per element of the given mesh
+            coef = mesh.get("vol_coef")
+        if coef is not None:
+            c = np.ascontiguousarray(coef, dtype=np.float64)
             self._keep.append(c)
             pb.el_coef = _dp(c)
         pb.force_kind, pb.force_porder = spec.force_kind, spec.force_porder
@@ -223,10 +239,37 @@ class Oracle:
             pb.el_porder = po.ctypes.data_as(C.POINTER(C.c_int))
         self.pb = pb
         self.m = _Mesh()
-        if lib().orc_mesh_build(C.byref(pb), C.byref(self.m)) != 0:
+        if mesh is not None:
+            self._from_arrays(mesh)
+        elif lib().orc_mesh_build(C.byref(pb), C.byref(self.m)) != 0:
             raise RuntimeError("oracle mesh build failed")
         self.neq = self.m.neq
         self.ia = self.ja = None
+
+    def _from_arrays(self, md):
+        def a(x, dt):
+            y = np.ascontiguousarray(x, dtype=dt).ravel()
+            self._keep.append(y)
+            return y
+        ip = C.POINTER(C.c_int)
+        dim, ns = md["dim"], md["nstate"]
+        vt = {2: QUAD, 3: HEX}[dim] if not md.get("vol_etype") else TET
+        bt = {2: LINE, 3: QUAD}[dim] if vt != TET else TRI
+        xyz = a(md["xyz"], np.float64)
+        vn, vc = a(md["vol_nodes"], np.int64), a(md["vol_conn"], np.int64)
+        bn, bc = a(md["bc_nodes"], np.int64), a(md["bc_conn"], np.int64)
+        bf = a(md["bc_id"], np.int32)
+        pos, size = a(md["blk_pos"], np.int64), a(md["blk_size"], np.int32)
+        order = a(md["conn_order"], np.int32)
+        nvol, nbc = len(md["vol_nodes"]), len(md["bc_nodes"])
+        if lib().orc_mesh_from_arrays(dim, md["porder"], ns, vt, bt, len(xyz) // 3, _dp(xyz), nvol,
+                                      _ip(vn), _ip(vc), nbc, _ip(bn), _ip(bc), bf.ctypes.data_as(ip),
+                                      md["nconn"], _ip(pos), size.ctypes.data_as(ip),
+                                      order.ctypes.data_as(ip), md["neq"], C.byref(self.m)) != 0:
+            raise RuntimeError("oracle mesh from arrays failed")
+        if md.get("dep_ptr") is not None:
+            self.dep = (a(md["dep_ptr"], np.int64), a(md["dep_conn"], np.int64),
+                        a(md["dep_coef"], np.float64))
 
     def __del__(self):
         try:
@@ -267,7 +310,13 @@ class Oracle:
         if self.ia is None:
             ia = C.POINTER(C.c_int64)()
             ja = C.POINTER(C.c_int64)()
-            nnz = lib().orc_setup_pattern(C.byref(self.m), C.byref(ia), C.byref(ja))
+            if self.dep is not None:
+                nnz = lib().orc_setup_pattern_constrained(C.byref(self.m), _ip(self.dep[0]),
+                                                          _ip(self.dep[1]), C.byref(ia), C.byref(ja))
+                if nnz < 0:
+                    raise RuntimeError("constrained connects must come after the others")
+            else:
+                nnz = lib().orc_setup_pattern(C.byref(self.m), C.byref(ia), C.byref(ja))
             self.ia = np.ctypeslib.as_array(ia, shape=(self.neq + 1,)).copy()
             self.ja = np.ctypeslib.as_array(ja, shape=(max(nnz, 1),))[:nnz].copy()
             lib().free(ia)
@@ -309,7 +358,13 @@ class Oracle:
         ia, ja = self.pattern()
         a = np.zeros(len(ja))
         rhs = np.zeros(self.neq)
-        lib().orc_assemble(C.byref(self.pb), C.byref(self.m), _ip(ia), _ip(ja), _dp(a), _dp(rhs))
+        if self.dep is not None:
+            if lib().orc_assemble_constrained(C.byref(self.pb), C.byref(self.m), _ip(self.dep[0]),
+                                              _ip(self.dep[1]), _dp(self.dep[2]), _ip(ia), _ip(ja),
+                                              _dp(a), _dp(rhs)) != 0:
+                raise RuntimeError("oracle constrained assembly failed")
+        else:
+            lib().orc_assemble(C.byref(self.pb), C.byref(self.m), _ip(ia), _ip(ja), _dp(a), _dp(rhs))
         return a, rhs
 
     def sloan(self):

@@ -1160,20 +1160,21 @@ static int cmp_i64(const void *a, const void *b) {
   return x < y ? -1 : x > y;
 }
 
-/* Node graph of TPZRenumbering::ConvertGraph: per connect (sequence number)
- * the sorted set of the other connects of its elements. */
-static void node_graph(const orc_mesh *m, int64_t **ngp_out, int64_t **ng_out) {
-  int64_t nb = m->nconn;
+/* Node graph of TPZRenumbering::ConvertGraph over explicit element lists
+ * (element e holds the sequence numbers el[ep[e] .. ep[e+1])): per node the
+ * sorted set of the other nodes of its elements. */
+static void node_graph_lists(int64_t nb, int64_t nel, const int64_t *ep, const int64_t *el,
+                             int64_t **ngp_out, int64_t **ng_out) {
   /* node -> element graph */
   int64_t *cnt = (int64_t *)calloc(nb + 1, sizeof(int64_t));
-  for (int64_t e = 0; e < m->nel; e++)
-    for (int s = 0; s < etype_nsides(m->etype[e]); s++) cnt[m->conn_seq[m->econn[27 * e + s]] + 1]++;
+  for (int64_t e = 0; e < nel; e++)
+    for (int64_t k = ep[e]; k < ep[e + 1]; k++) cnt[el[k] + 1]++;
   for (int64_t i = 0; i < nb; i++) cnt[i + 1] += cnt[i];
-  int64_t *n2e = (int64_t *)malloc(cnt[nb] * sizeof(int64_t));
-  int64_t *fill = (int64_t *)calloc(nb, sizeof(int64_t));
-  for (int64_t e = 0; e < m->nel; e++)
-    for (int s = 0; s < etype_nsides(m->etype[e]); s++) {
-      int64_t c = m->conn_seq[m->econn[27 * e + s]];
+  int64_t *n2e = (int64_t *)malloc((cnt[nb] ? cnt[nb] : 1) * sizeof(int64_t));
+  int64_t *fill = (int64_t *)calloc(nb ? nb : 1, sizeof(int64_t));
+  for (int64_t e = 0; e < nel; e++)
+    for (int64_t k = ep[e]; k < ep[e + 1]; k++) {
+      int64_t c = el[k];
       n2e[cnt[c] + fill[c]++] = e;
     }
   free(fill);
@@ -1181,13 +1182,13 @@ static void node_graph(const orc_mesh *m, int64_t **ngp_out, int64_t **ng_out) {
   int64_t *ngp = (int64_t *)calloc(nb + 1, sizeof(int64_t));
   int64_t ngcap = 1 << 20, ngn = 0;
   int64_t *ng = (int64_t *)malloc(ngcap * sizeof(int64_t));
-  int64_t *tmp = (int64_t *)malloc(4096 * sizeof(int64_t));
+  int64_t *tmp = (int64_t *)malloc(8192 * sizeof(int64_t));
   for (int64_t i = 0; i < nb; i++) {
     int nt = 0;
     for (int64_t k = cnt[i]; k < cnt[i + 1]; k++) {
       int64_t e = n2e[k];
-      for (int s = 0; s < etype_nsides(m->etype[e]); s++) {
-        int64_t c = m->conn_seq[m->econn[27 * e + s]];
+      for (int64_t j = ep[e]; j < ep[e + 1]; j++) {
+        int64_t c = el[j];
         if (c != i) tmp[nt++] = c;
       }
     }
@@ -1207,24 +1208,46 @@ static void node_graph(const orc_mesh *m, int64_t **ngp_out, int64_t **ng_out) {
   *ng_out = ng;
 }
 
-int64_t orc_setup_pattern(const orc_mesh *m, int64_t **ia_out, int64_t **ja_out) {
+/* element lists of the mesh: every side's connect (sequence number) */
+static void mesh_lists(const orc_mesh *m, int64_t **ep_out, int64_t **el_out) {
+  int64_t *ep = (int64_t *)malloc((m->nel + 1) * sizeof(int64_t)), n = 0;
+  ep[0] = 0;
+  for (int64_t e = 0; e < m->nel; e++) n += etype_nsides(m->etype[e]), ep[e + 1] = n;
+  int64_t *el = (int64_t *)malloc((n ? n : 1) * sizeof(int64_t));
+  for (int64_t e = 0; e < m->nel; e++)
+    for (int s = 0; s < etype_nsides(m->etype[e]); s++)
+      el[ep[e] + s] = m->conn_seq[m->econn[27 * e + s]];
+  *ep_out = ep;
+  *el_out = el;
+}
+
+static void node_graph(const orc_mesh *m, int64_t **ngp_out, int64_t **ng_out) {
+  int64_t *ep, *el;
+  mesh_lists(m, &ep, &el);
+  node_graph_lists(m->nconn, m->nel, ep, el, ngp_out, ng_out);
+  free(ep);
+  free(el);
+}
+
+/* The upper CSR of SetupMatrixData from a node graph; connects with skip[c]
+ * set carry no equations (constrained connects, numbered after the others). */
+static int64_t pattern_csr(const orc_mesh *m, const int64_t *ngp, const int64_t *ng, const char *skip,
+                           int64_t **ia_out, int64_t **ja_out) {
   const int64_t nb = m->nconn;
-  int64_t *ngp, *ng;
-  node_graph(m, &ngp, &ng);
-  /* count pass */
+#define BSZ(c) ((skip && skip[c]) ? 0 : (int64_t)m->blk_size[c])
   int64_t total = 0;
   for (int64_t i = 0; i < nb; i++) {
-    int64_t bs = m->blk_size[i];
+    int64_t bs = BSZ(i);
     if (!bs) continue;
     total += bs * (bs + 1) / 2;
     for (int64_t j = ngp[i]; j < ngp[i + 1]; j++)
-      if (ng[j] > i) total += bs * m->blk_size[ng[j]];
+      if (ng[j] > i) total += bs * BSZ(ng[j]);
   }
   int64_t *ia = (int64_t *)malloc((m->neq + 1) * sizeof(int64_t));
   int64_t *ja = (int64_t *)malloc((total ? total : 1) * sizeof(int64_t));
   int64_t pos = 0, ieq = 0;
   for (int64_t i = 0; i < nb; i++) {
-    int64_t bs = m->blk_size[i], bp = m->blk_pos[i];
+    int64_t bs = BSZ(i), bp = m->blk_pos[i];
     for (int64_t k = 0; k < bs; k++, ieq++) {
       ia[ieq] = pos;
       int diag = 0;
@@ -1236,7 +1259,7 @@ int64_t orc_setup_pattern(const orc_mesh *m, int64_t **ia_out, int64_t **ja_out)
           for (int64_t l = 0; l < bs; l++)
             if (bp + l >= ieq) ja[pos++] = bp + l;
         }
-        for (int64_t l = 0; l < m->blk_size[col]; l++)
+        for (int64_t l = 0; l < BSZ(col); l++)
           if (m->blk_pos[col] + l >= ieq) ja[pos++] = m->blk_pos[col] + l;
       }
       if (!diag)
@@ -1244,12 +1267,20 @@ int64_t orc_setup_pattern(const orc_mesh *m, int64_t **ia_out, int64_t **ja_out)
           if (bp + l >= ieq) ja[pos++] = bp + l;
     }
   }
+#undef BSZ
   ia[ieq] = pos;
-  free(ng);
-  free(ngp);
   *ia_out = ia;
   *ja_out = ja;
   return pos;
+}
+
+int64_t orc_setup_pattern(const orc_mesh *m, int64_t **ia_out, int64_t **ja_out) {
+  int64_t *ngp, *ng;
+  node_graph(m, &ngp, &ng);
+  int64_t nnz = pattern_csr(m, ngp, ng, NULL, ia_out, ja_out);
+  free(ng);
+  free(ngp);
+  return nnz;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1892,4 +1923,244 @@ int orc_mesh_sloan(const orc_mesh *m, int64_t *perm) {
   free(ngp);
   free(ng);
   return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Meshes handed over as arrays, and constrained connects (hanging sides):   */
+/* TPZConnect dependencies (TPZConnect::TPZDepend, Mesh/pzconnect.h), the    */
+/* element's constrained matrix of TPZElementMatrixT::ApplyConstraints       */
+/* (Mesh/TPZElementMatrixT.cpp:134-322) and its assembly                     */
+/* (StrMatrix/pzstrmatrixor.cpp:273-280).                                    */
+/* ------------------------------------------------------------------------ */
+
+int orc_mesh_from_arrays(int dim, int p, int nstate, int vt, int bt, int64_t nnodes,
+                         const double *xyz, int64_t nvol, const int64_t *vol_nodes,
+                         const int64_t *vol_conn, int64_t nbc, const int64_t *bc_nodes,
+                         const int64_t *bc_conn, const int *bc_face, int64_t nconn,
+                         const int64_t *blk_pos, const int *blk_size, const int *conn_order,
+                         int64_t neq, orc_mesh *m) {
+  memset(m, 0, sizeof(*m));
+  int ncv = etype_ncorner(vt), nsv = etype_nsides(vt), ncb = etype_ncorner(bt), nsb = etype_nsides(bt);
+  m->dim = dim; m->p = p; m->nstate = nstate;
+  m->nnodes = nnodes;
+  m->xyz = malloc(sizeof(double) * 3 * (nnodes ? nnodes : 1));
+  memcpy(m->xyz, xyz, sizeof(double) * 3 * nnodes);
+  m->nvol = nvol;
+  m->nel = nvol + nbc;
+  int64_t ne = m->nel ? m->nel : 1;
+  m->etype = malloc(sizeof(int) * ne);
+  m->bcface = malloc(sizeof(int) * ne);
+  m->enodes = calloc(8 * ne, sizeof(int64_t));
+  m->econn = calloc(27 * ne, sizeof(int64_t));
+  for (int64_t e = 0; e < m->nel; e++) {
+    int isbc = e >= nvol;
+    int64_t k = isbc ? e - nvol : e;
+    m->etype[e] = isbc ? bt : vt;
+    m->bcface[e] = isbc ? bc_face[k] : -1;
+    for (int n = 0; n < (isbc ? ncb : ncv); n++)
+      m->enodes[8 * e + n] = isbc ? bc_nodes[k * ncb + n] : vol_nodes[k * ncv + n];
+    for (int sd = 0; sd < (isbc ? nsb : nsv); sd++) {
+      int64_t c = isbc ? bc_conn[k * nsb + sd] : vol_conn[k * nsv + sd];
+      if (c < 0 || c >= nconn) return -1;
+      m->econn[27 * e + sd] = c;
+    }
+  }
+  m->nconn = nconn;
+  int64_t nc = nconn ? nconn : 1;
+  m->conn_order = malloc(sizeof(int) * nc);
+  m->conn_nshape = malloc(sizeof(int) * nc);
+  m->conn_seq = malloc(sizeof(int64_t) * nc);
+  m->blk_pos = malloc(sizeof(int64_t) * nc);
+  m->blk_size = malloc(sizeof(int) * nc);
+  for (int64_t c = 0; c < nconn; c++) {
+    m->conn_order[c] = conn_order[c];
+    m->conn_nshape[c] = blk_size[c] / nstate;
+    m->conn_seq[c] = c;
+    m->blk_pos[c] = blk_pos[c];
+    m->blk_size[c] = blk_size[c];
+  }
+  m->neq = neq;
+  return 0;
+}
+
+static int cmp_i64b(const void *a, const void *b) {
+  int64_t x = *(const int64_t *)a, y = *(const int64_t *)b;
+  return x < y ? -1 : x > y;
+}
+
+/* TPZConnect::BuildConnectList: the element's connects and, recursively, the
+ * connects they depend on; sorted, unique (std::set<int64_t>). */
+static int constr_list(const orc_mesh *m, int64_t e, const int64_t *dep_ptr, const int64_t *dep_conn,
+                       int64_t *out) {
+  int n = 0;
+  for (int sd = 0; sd < etype_nsides(m->etype[e]); sd++) out[n++] = m->conn_seq[m->econn[27 * e + sd]];
+  for (int i = 0; i < n; i++) {  /* the list grows while it is scanned: closure */
+    int64_t c = out[i];
+    for (int64_t k = dep_ptr[c]; k < dep_ptr[c + 1]; k++) {
+      int have = 0;
+      for (int j = 0; j < n; j++) have |= out[j] == dep_conn[k];
+      if (!have) out[n++] = dep_conn[k];
+    }
+  }
+  qsort(out, n, sizeof(int64_t), cmp_i64b);
+  int u = 0;
+  for (int i = 0; i < n; i++)
+    if (!u || out[i] != out[u - 1]) out[u++] = out[i];
+  return u;
+}
+
+/* SetupMatrixData over TPZCompMesh::ComputeElGraph's lists (Mesh/pzcmesh.cpp:
+ * 1223-1266): every element's BuildConnectList without the constrained
+ * connects. */
+int64_t orc_setup_pattern_constrained(const orc_mesh *m, const int64_t *dep_ptr,
+                                      const int64_t *dep_conn, int64_t **ia_out, int64_t **ja_out) {
+  int64_t *ep = malloc((m->nel + 1) * sizeof(int64_t)), n = 0, cap = 1024;
+  int64_t *el = malloc(cap * sizeof(int64_t)), buf[512];
+  char *skip = calloc(m->nconn ? m->nconn : 1, 1);
+  for (int64_t c = 0; c < m->nconn; c++) {
+    skip[c] = dep_ptr[c + 1] > dep_ptr[c];
+    if (skip[c] && m->blk_size[c] && m->blk_pos[c] < m->neq) {  /* constrained blocks come last */
+      free(ep); free(el); free(skip);
+      return -1;
+    }
+  }
+  ep[0] = 0;
+  for (int64_t e = 0; e < m->nel; e++) {
+    int k = constr_list(m, e, dep_ptr, dep_conn, buf);
+    for (int i = 0; i < k; i++) {
+      if (skip[buf[i]]) continue;
+      if (n == cap) { cap *= 2; el = realloc(el, cap * sizeof(int64_t)); }
+      el[n++] = buf[i];
+    }
+    ep[e + 1] = n;
+  }
+  int64_t *ngp, *ng;
+  node_graph_lists(m->nconn, m->nel, ep, el, &ngp, &ng);
+  int64_t nnz = pattern_csr(m, ngp, ng, skip, ia_out, ja_out);
+  free(ngp); free(ng); free(ep); free(el); free(skip);
+  return nnz;
+}
+
+/* TPZConnect::SetDependenceOrder (Mesh/pzconnect.cpp:558-600): the connects
+ * a connect of order `order` depends on get at least order + 1. */
+static void set_dep_order(int64_t c, int order, const int64_t *dep_ptr, const int64_t *dep_conn,
+                          const int64_t *list, int n, int *dord) {
+  for (int64_t k = dep_ptr[c]; k < dep_ptr[c + 1]; k++) {
+    int64_t t = dep_conn[k];
+    for (int i = 0; i < n; i++)
+      if (list[i] == t) {
+        if (dord[i] < order + 1) dord[i] = order + 1;
+        set_dep_order(t, order + 1, dep_ptr, dep_conn, list, n, dord);
+      }
+  }
+}
+
+int orc_assemble_constrained(const orc_problem *pb, const orc_mesh *m, const int64_t *dep_ptr,
+                             const int64_t *dep_conn, const double *dep_coef, const int64_t *ia,
+                             const int64_t *ja, double *a, double *rhs) {
+  memset(a, 0, sizeof(double) * ia[m->neq]);
+  memset(rhs, 0, sizeof(double) * m->neq);
+  const int ns = m->nstate;
+  double *ek = malloc(sizeof(double) * ORC_MAXNE * ORC_MAXNE);
+  double ef[ORC_MAXNE];
+  int64_t dest[ORC_MAXNE];
+  int64_t list[512], pos[513], kdest[ORC_MAXNE];
+  int dord[512];
+  double *kc = malloc(sizeof(double) * 4 * ORC_MAXNE * ORC_MAXNE);
+  double *kr = malloc(sizeof(double) * 4 * ORC_MAXNE * ORC_MAXNE);
+  double fc[4 * ORC_MAXNE], fr[4 * ORC_MAXNE];
+  for (int64_t e = 0; e < m->nel; e++) {
+    int ne = orc_calcstiff(pb, m, e, ek, ef, dest);
+    if (ne < 0) { free(ek); free(kc); free(kr); return -1; }
+    int t = m->etype[e], nsd = etype_nsides(t), hasdep = 0;
+    for (int sd = 0; sd < nsd; sd++) {
+      int64_t c = m->conn_seq[m->econn[27 * e + sd]];
+      hasdep |= dep_ptr[c + 1] > dep_ptr[c] && m->blk_size[c] > 0;
+    }
+    if (!hasdep) {
+      orc_addkel(ia, ja, a, ne, ek, dest);
+      for (int i = 0; i < ne; i++) rhs[dest[i]] += ef[i];
+      continue;
+    }
+    /* constrained block structure over the sorted connect list */
+    int n = constr_list(m, e, dep_ptr, dep_conn, list);
+    pos[0] = 0;
+    for (int i = 0; i < n; i++) pos[i + 1] = pos[i] + m->blk_size[list[i]];
+    int64_t tot = pos[n];
+    if (tot > 4 * ORC_MAXNE) { free(ek); free(kc); free(kr); return -1; }
+    memset(kc, 0, sizeof(double) * tot * tot);
+    memset(fc, 0, sizeof(double) * tot);
+    /* copy: element block of side sd -> block of its connect in the list */
+    int epos[27], eidx[27], off = 0;
+    for (int sd = 0; sd < nsd; sd++) {
+      int64_t c = m->conn_seq[m->econn[27 * e + sd]];
+      int i = 0;
+      while (i < n && list[i] != c) i++;
+      epos[sd] = off;
+      eidx[sd] = i;
+      off += m->blk_size[c];
+    }
+    for (int si = 0; si < nsd; si++) {
+      int bi = m->blk_size[m->conn_seq[m->econn[27 * e + si]]];
+      for (int r = 0; r < bi; r++) {
+        fc[pos[eidx[si]] + r] = ef[epos[si] + r];
+        for (int sj = 0; sj < nsd; sj++) {
+          int bj = m->blk_size[m->conn_seq[m->econn[27 * e + sj]]];
+          for (int q = 0; q < bj; q++)
+            kc[(pos[eidx[si]] + r) * tot + pos[eidx[sj]] + q] = ek[(epos[si] + r) * ne + epos[sj] + q];
+        }
+      }
+    }
+    /* BuildDependencyOrder (Mesh/pzelmat.cpp:117-160) */
+    for (int i = 0; i < n; i++) dord[i] = 0;
+    for (int cur = 0, done = 1; done; cur++) {
+      done = 0;
+      for (int i = 0; i < n; i++)
+        if (dep_ptr[list[i] + 1] > dep_ptr[list[i]] && dord[i] == cur) {
+          set_dep_order(list[i], cur, dep_ptr, dep_conn, list, n, dord);
+          done++;
+        }
+    }
+    /* fold every constrained connect into the connects it depends on, in
+     * dependency order: rows, then columns (one shape per connect) */
+    int processed = 0;
+    for (int cur = 0; processed < n; cur++)
+      for (int i = 0; i < n; i++) {
+        if (dord[i] != cur) continue;
+        processed++;
+        int64_t c = list[i];
+        for (int64_t k = dep_ptr[c]; k < dep_ptr[c + 1]; k++) {
+          int j = 0;
+          while (j < n && list[j] != dep_conn[k]) j++;
+          double coef = dep_coef[k];
+          int64_t send = pos[i], recv = pos[j];
+          if (m->blk_size[c] == 0 || m->blk_size[list[j]] == 0) continue;
+          for (int64_t q = 0; q < tot; q++)
+            for (int d = 0; d < ns; d++) kc[(recv + d) * tot + q] += coef * kc[(send + d) * tot + q];
+          for (int d = 0; d < ns; d++) fc[recv + d] += coef * fc[send + d];
+          for (int64_t q = 0; q < tot; q++)
+            for (int d = 0; d < ns; d++) kc[q * tot + recv + d] += coef * kc[q * tot + send + d];
+        }
+      }
+    /* AddKel / AddFel over the unconstrained connects of the list */
+    int nr = 0;
+    int64_t src[4 * ORC_MAXNE];
+    for (int i = 0; i < n; i++) {
+      int64_t c = list[i];
+      if (dep_ptr[c + 1] > dep_ptr[c]) continue;
+      for (int d = 0; d < m->blk_size[c]; d++) {
+        src[nr] = pos[i] + d;
+        kdest[nr] = m->blk_pos[c] + d;
+        nr++;
+      }
+    }
+    for (int i = 0; i < nr; i++) {
+      fr[i] = fc[src[i]];
+      for (int j = 0; j < nr; j++) kr[i * nr + j] = kc[src[i] * tot + src[j]];
+    }
+    orc_addkel(ia, ja, a, nr, kr, kdest);
+    for (int i = 0; i < nr; i++) rhs[kdest[i]] += fr[i];
+  }
+  free(ek); free(kc); free(kr);
+  return 0;
 }

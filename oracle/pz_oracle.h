@@ -20,6 +20,8 @@
  *   - TPZSYsmpMatrix::MultAdd, CG<> with the EJacobi TPZStepSolver
  *   - TPZCompMesh::EvaluateError with the Errors of the three materials
  *   - TPZSloanRenumbering::Resequence (TPZAnalysis::OptimizeBandwidth)
+ *   - TPZElementMatrixT::ApplyConstraints and the assembly of constrained
+ *     elements (hanging sides)
  */
 #ifndef PZ_ORACLE_H
 #define PZ_ORACLE_H
@@ -183,6 +185,25 @@ int orc_sloan(int64_t n, const int64_t *xadj, const int64_t *adj, int64_t *perm)
 /* the same on the connect graph of the mesh (TPZAnalysis::OptimizeBandwidth):
  * perm[seq] = new sequence number, to be applied with orc_mesh_permute */
 int orc_mesh_sloan(const orc_mesh *m, int64_t *perm);
+
+/* ---- meshes as arrays, constrained (hanging) connects ---------------------
+ * orc_mesh_from_arrays: volume elements of type vt, boundary elements of type
+ * bt (bc_face = index into the problem's bc tables), connect index =
+ * sequence number; conn_order per connect (corners 1).
+ * Dependencies: connect c depends on dep_conn[dep_ptr[c] .. dep_ptr[c+1])
+ * with the coefficients dep_coef (one shape per connect); constrained
+ * connects carry no equations and come after the others in sequence order. */
+int orc_mesh_from_arrays(int dim, int p, int nstate, int vt, int bt, int64_t nnodes,
+                         const double *xyz, int64_t nvol, const int64_t *vol_nodes,
+                         const int64_t *vol_conn, int64_t nbc, const int64_t *bc_nodes,
+                         const int64_t *bc_conn, const int *bc_face, int64_t nconn,
+                         const int64_t *blk_pos, const int *blk_size, const int *conn_order,
+                         int64_t neq, orc_mesh *m);
+int64_t orc_setup_pattern_constrained(const orc_mesh *m, const int64_t *dep_ptr,
+                                      const int64_t *dep_conn, int64_t **ia, int64_t **ja);
+int orc_assemble_constrained(const orc_problem *pb, const orc_mesh *m, const int64_t *dep_ptr,
+                             const int64_t *dep_conn, const double *dep_coef, const int64_t *ia,
+                             const int64_t *ja, double *a, double *rhs);
 
 /* Number of CSR entries for structured meshes, closed form (for checks). */
 int64_t orc_nnz_closed_form(int dim, int p, int nstate, int nx, int ny, int nz);

@@ -1,0 +1,119 @@
+"""GPU parity of constrained (hanging-side) assembly against the oracle:
+quadrilateral meshes with divided elements (neopz_amd.mesh.build_refined2d),
+p = 1 and 2, TPZMatPoisson and TPZDarcyFlow with element coefficients,
+Dirichlet and Neumann faces.
+
+Bars (as for unconstrained meshes): pattern bit-exact; values
+|a - a_ref| <= 1e-12 |a_ref| + 1e-15 max|row|; rhs <= 1e-12 max|rhs|;
+FP64 PCG solution within 1e-8 of a direct solve of the oracle's system; the
+mixed solver (stored FP32 operator: the matrix-free one is not built for
+constrained meshes) to a true residual 1e-8; error norms of the solution
+within 1e-12 of the oracle's on the expanded solution T u.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import oracle as O
+from neopz_amd import _lib as L
+from neopz_amd import mesh as M
+from neopz_amd import problems as P
+from neopz_amd.analysis import Context, TLaplaceExample1, TPZLinearAnalysis, TPZStepSolver
+
+pytestmark = pytest.mark.gpu
+
+_CTX = None
+
+
+def ctx():
+    global _CTX
+    if _CTX is None:
+        _CTX = Context(0)
+    return _CTX
+
+
+def _case(name):
+    rng = np.random.default_rng(len(name))
+    if name == "poisson_p1":
+        pb = P.Problem(material=P.POISSON, dim=2, p=1, nx=6, ny=5, force_kind=P.FORCE_CONST,
+                       force_const=(1.0, 0, 0), bc_type=[0, 1, 0, -1, -1, -1],
+                       bc_val=[[0, 0, 0], [0.5, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0]])
+    elif name == "poisson_p2":
+        pb = P.Problem(material=P.POISSON, dim=2, p=2, nx=6, ny=6, scale=1.7, force_kind=P.FORCE_CONST,
+                       force_const=(1.0, 0, 0), bc_type=[0, 1, 0, 1, -1, -1],
+                       bc_val=[[0, 0, 0], [0.5, 0, 0], [0, 0, 0], [-0.3, 0, 0], [0, 0, 0], [0, 0, 0]])
+    else:  # darcy_p2
+        pb = P.darcy2d(n=8, p=2, seed=4)
+    ref = rng.random(pb.nx * pb.ny) < 0.4
+    return pb, M.build_refined2d(pb, ref)
+
+
+def _T(md, neq):
+    nfree = int(np.asarray(md["blk_size"], dtype=np.int64).sum())
+    T = sp.lil_matrix((nfree, neq))
+    for i in range(neq):
+        T[i, i] = 1.0
+    dp, dc, dv = md["dep_ptr"], md["dep_conn"], md["dep_coef"]
+    for c in range(md["nconn"]):
+        for k in range(dp[c], dp[c + 1]):
+            if md["blk_size"][c] and md["blk_size"][dc[k]]:
+                T[md["blk_pos"][c], md["blk_pos"][dc[k]]] += dv[k]
+    return T.tocsr()
+
+
+def _csc(ia, ja, a, n):
+    rows = np.repeat(np.arange(n), np.diff(ia))
+    U = sp.csr_matrix((a, (rows, ja)), shape=(n, n))
+    return (U + sp.triu(U, 1).T).tocsc()
+
+
+@pytest.mark.parametrize("name", ["poisson_p1", "poisson_p2", "darcy_p2"])
+def test_constrained_assembly_and_solve(name):
+    pb, md = _case(name)
+    print(f"[gpu] constrained {name}: {len(md['vol_nodes'])} elements, "
+          f"{int(np.count_nonzero(np.diff(md['dep_ptr'])))} constrained connects", flush=True)
+    an = TPZLinearAnalysis(md, False, ctx=ctx())
+    st = TPZStepSolver()
+    st.SetCG(20000, TPZStepSolver.jacobi(), 1e-13, 0)
+    an.SetSolver(st)
+    an.Assemble()
+    o = O.Oracle(pb, mesh=md)
+    ia, ja = o.pattern()
+    a, rhs = o.assemble()
+    assert np.array_equal(an.K.IA(), ia) and np.array_equal(an.K.JA(), ja)
+    ag = an.K.A()
+    rowmax = np.maximum.reduceat(np.abs(a), ia[:-1])
+    rows = np.repeat(np.arange(len(ia) - 1), np.diff(ia))
+    assert np.all(np.abs(ag - a) <= 1e-12 * np.abs(a) + 1e-15 * rowmax[rows])
+    assert np.max(np.abs(an.Rhs() - rhs)) <= 1e-12 * np.max(np.abs(rhs))
+    u = an.Solve()
+    x = spla.spsolve(_csc(ia, ja, a, o.neq), rhs)
+    assert np.linalg.norm(u - x) <= 1e-8 * np.linalg.norm(x)
+    # error norms: the device expands the constrained connects itself
+    an.SetExact(TLaplaceExample1(TLaplaceExample1.ESinSin, center=(0.1, 0.2, 0.0)), 4)
+    tot = an.PostProcessError(store_error=True)
+    el_r, tot_r = o.evaluate_error(_T(md, o.neq) @ u, kind=O.EXACT_SINSIN, porder=4,
+                                   center=(0.1, 0.2, 0.0))
+    assert np.all(np.abs(an.ElementSolution() - el_r) <= 1e-12 * np.abs(el_r) + 1e-13 * np.abs(el_r).max())
+    assert np.allclose(tot, tot_r, rtol=1e-12, atol=0)
+    # mixed FP32 FCG + FP64 refinement on the stored operator
+    st2 = TPZStepSolver(an.K)
+    st2.SetMixedFCG(50000, 1e-8)
+    st2.Solve(None, want_solution=False)
+    assert st2.info["converged"] and st2.info["true_residual"] <= 1e-8
+
+
+def test_constrained_boundary_elements_refused():
+    """Boundary elements on constrained connects are not supported: the call
+    fails with PZG_EUNSUP instead of assembling something else."""
+    pb, md = _case("poisson_p2")
+    md = dict(md)
+    bc = np.array(md["bc_conn"]).copy()
+    dep_c = int(np.flatnonzero(np.diff(md["dep_ptr"]))[0])
+    bc[0, 0] = dep_c
+    md["bc_conn"] = bc
+    d, keep = L.make_mesh_desc(md)
+    import ctypes
+    rc = ctx().lib.pzg_mesh_set(ctx().h, ctypes.byref(d))
+    assert rc == L.PZG_EUNSUP
