@@ -509,10 +509,12 @@ __global__ __launch_bounds__(TPB) void k_vol_hp(ElemArgs g, int nsv, const HpTab
     s_row[tid] = b >= 0 ? g.ia[b] : -1;
   }
   // padding columns of the tiles stay zero
-  for (int k = tid; k < QC * 3 * (NSP - NSUB); k += TPB) {
-    const int qa = k / (NSP - NSUB), i = NSUB + k % (NSP - NSUB);
-    s_G[qa / 3][qa % 3][i] = 0.;
-    s_Gw[qa / 3][qa % 3][i] = 0.;
+  if constexpr (NSP > NSUB) {
+    for (int k = tid; k < QC * 3 * (NSP - NSUB); k += TPB) {
+      const int qa = k / (NSP - NSUB), i = NSUB + k % (NSP - NSUB);
+      s_G[qa / 3][qa % 3][i] = 0.;
+      s_Gw[qa / 3][qa % 3][i] = 0.;
+    }
   }
   if (tid == 0) {
     // GradX of the straight tetrahedron: column k = x_{k+1} - x_0
@@ -940,19 +942,23 @@ static int hp_tune(Ctx &c, const ElemArgs &g) {
     for (int v = 0; v < HP_NVARIANT && rc == PZG_OK; v++) {
       rc = launch_hp_bucket<true>(c, g, c.hp_vol_P[b], v, tab, sub, list, nbest);
       if (rc != PZG_OK) break;
-      hipEventRecord(e0, c.stream);
+      if (hipEventRecord(e0, c.stream) != hipSuccess) { rc = PZG_EHIP; break; }
       for (int r = 0; r < 3 && rc == PZG_OK; r++)
         rc = launch_hp_bucket<true>(c, g, c.hp_vol_P[b], v, tab, sub, list, nbest);
-      hipEventRecord(e1, c.stream);
-      if (hipEventSynchronize(e1) != hipSuccess) { rc = PZG_EHIP; break; }
+      if (rc != PZG_OK) break;
       float ms = 0.f;
-      hipEventElapsedTime(&ms, e0, e1);
+      if (hipEventRecord(e1, c.stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        set_error("hp_tune: event timing failed");
+        rc = PZG_EHIP;
+        break;
+      }
       c.hp_vol_ms[b * HP_NVARIANT + v] = ms / 3.f;
       if (v == 0 || ms < tbest) { tbest = ms; c.hp_vol_var[b] = v; }
     }
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   if (rc == PZG_OK) c.hp_tuned = true;
   return rc;
 }
