@@ -345,6 +345,8 @@ def OptimizeBandwidth(mesh):
     the default TPZSloanRenumbering, computed by the library on the host:
     returns the renumbered mesh description (TPZCompMesh::Permute) and the
     permutation perm[old sequence number] = new one."""
+    if mesh.get("dep_ptr") is not None:
+        raise ValueError("OptimizeBandwidth: meshes with constrained connects are not renumbered here")
     eptr, elist = element_graph(mesh)
     n = int(mesh["nconn"])
     perm = np.zeros(n, dtype=np.int64)

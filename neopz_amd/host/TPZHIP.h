@@ -260,6 +260,7 @@ class TPZLinearAnalysis {
   // default TPZSloanRenumbering; the analysis keeps the renumbered connect
   // lists and blocks (TPZCompMesh::Permute)
   void OptimizeBandwidth() {
+    if (fMesh.dep_ptr) DebugStop();  // constrained connects: not renumbered here
     const bool tet = fMesh.vol_etype == PZG_TET;
     const int nsv = fMesh.dim == 2 ? 9 : tet ? 15 : 27, nsb = fMesh.dim == 2 ? 3 : tet ? 7 : 9;
     std::vector<int64_t> eptr(1, 0), elist;

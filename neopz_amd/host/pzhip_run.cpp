@@ -119,7 +119,7 @@ int main(int argc, char **argv) {
     return 2;
   }
   char magic[8];
-  if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, "PZGMESH3", 8) != 0) Reader::fail();
+  if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, "PZGMESH4", 8) != 0) Reader::fail();
   Reader r{f};
   pzg_mesh_desc m{};
   m.dim = r.scalar<int32_t>();
@@ -151,6 +151,9 @@ int main(int argc, char **argv) {
   auto vol_rule = r.array<int32_t>();
   auto bc_rule = r.array<int32_t>();
   m.vol_etype = r.scalar<int32_t>();
+  auto dep_ptr = r.array<int64_t>();
+  auto dep_conn = r.array<int64_t>();
+  auto dep_coef = r.array<double>();
   std::fclose(f);
 
   const bool tet = m.vol_etype == PZG_TET;
@@ -182,6 +185,14 @@ int main(int argc, char **argv) {
     Reader::fail();
   m.vol_rule = vol_rule.empty() ? nullptr : vol_rule.data();
   m.bc_rule = bc_rule.empty() ? nullptr : bc_rule.data();
+  if (!dep_ptr.empty()) {  // constrained connects (hanging sides)
+    if (int64_t(dep_ptr.size()) != m.nconn + 1 || dep_conn.size() != dep_coef.size() ||
+        dep_ptr.back() != int64_t(dep_conn.size()))
+      Reader::fail();
+    m.dep_ptr = dep_ptr.data();
+    m.dep_conn = dep_conn.data();
+    m.dep_coef = dep_coef.data();
+  }
 
   using namespace pzhip;
   Ctx ctx = MakeContext(0);

@@ -360,7 +360,7 @@ def _build3d_tet(pb):
 # Flat little-endian file of a mesh description, read by the C++ host mirror
 # (neopz_amd/host/pzhip_run.cpp).  Fields in this fixed order; arrays are
 # preceded by their element count (int64).
-_FILE_MAGIC = b"PZGMESH3"
+_FILE_MAGIC = b"PZGMESH4"
 
 
 def save(m, path):
@@ -405,6 +405,9 @@ def save(m, path):
         arr(f, m.get("vol_rule"), np.int32)
         arr(f, m.get("bc_rule"), np.int32)
         i32(f, m.get("vol_etype", 0))
+        arr(f, m.get("dep_ptr"), np.int64)
+        arr(f, m.get("dep_conn"), np.int64)
+        arr(f, m.get("dep_coef"), np.float64)
 
 
 def build_refined2d(pb, refine):

@@ -10,6 +10,8 @@ mixed solver (stored FP32 operator: the matrix-free one is not built for
 constrained meshes) to a true residual 1e-8; error norms of the solution
 within 1e-12 of the oracle's on the expanded solution T u.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -117,3 +119,29 @@ def test_constrained_boundary_elements_refused():
     import ctypes
     rc = ctx().lib.pzg_mesh_set(ctx().h, ctypes.byref(d))
     assert rc == L.PZG_EUNSUP
+
+
+def test_constrained_large_mesh_patch_and_time():
+    """160 x 160 coarse quadrilaterals, p = 2, about 30% divided: the GPU solve
+    reproduces u = x at every unconstrained corner (a size-independent
+    property of a conforming constrained space); assembly time printed."""
+    pb = P.Problem(material=P.POISSON, dim=2, p=2, nx=160, ny=160, bc_type=[0, 1, -1, -1, -1, -1],
+                   bc_val=[[0, 0, 0], [1.0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0]])
+    md = M.build_refined2d(pb, np.random.default_rng(9).random(160 * 160) < 0.3)
+    ncel = int(np.count_nonzero(np.diff(md["dep_ptr"])))
+    print(f"[gpu] constrained large: {len(md['vol_nodes'])} elements, {md['neq']} equations, "
+          f"{ncel} constrained connects", flush=True)
+    an = TPZLinearAnalysis(md, False, ctx=ctx())
+    st = TPZStepSolver()
+    st.SetCG(100000, TPZStepSolver.jacobi(), 1e-13, 0)
+    an.SetSolver(st)
+    an.Assemble()
+    ms = C.c_double(0.)
+    L.check(ctx().lib.pzg_time_kernel(ctx().h, 2, 5, C.byref(ms), None))
+    print(f"[gpu] constrained large: assembly {ms.value:.3f} ms per pass", flush=True)
+    u = an.Solve()
+    xyz = md["xyz"].reshape(-1, 3)
+    vn, vc = md["vol_nodes"], md["vol_conn"]
+    free = np.diff(md["dep_ptr"])[vc[:, :4]] == 0
+    err = np.abs(u[md["blk_pos"][vc[:, :4][free]]] - xyz[vn[:, :4][free], 0])
+    assert err.max() < 1e-8

@@ -52,7 +52,7 @@ def _read_result(path, extra=()):
 def test_runner_rejects_malformed_mesh(tmp_path):
     assert RUN.exists(), "build the library first (neopz_amd/csrc make)"
     bad = tmp_path / "bad.bin"
-    bad.write_bytes(b"PZGMESH3" + b"\0" * 6)
+    bad.write_bytes(b"PZGMESH4" + b"\0" * 6)
     r = subprocess.run([str(RUN), str(bad), str(tmp_path / "o.bin")], capture_output=True, text=True,
                        timeout=60)
     assert r.returncode == 2 and "malformed" in r.stderr
@@ -182,6 +182,34 @@ def test_cpp_sloan_renumbered_analysis_matches_oracle(tmp_path, name, pb):
     perm = o.sloan()
     assert np.array_equal(res["perm"], perm)
     o.permute(perm)
+    ia, ja = o.pattern()
+    a, rhs = o.assemble()
+    assert np.array_equal(res["ia"], ia) and np.array_equal(res["ja"], ja)
+    assert np.max(np.abs(res["a"] - a) - 1e-12 * np.abs(a)) <= 1e-15 * np.max(np.abs(a))
+    assert np.max(np.abs(res["rhs"] - rhs)) <= 1e-12 * np.max(np.abs(rhs))
+    ref = O.pcg(ia, ja, a, rhs, maxit=20000, tol=1e-13, wasted_spmv=False)
+    assert np.linalg.norm(res["u"] - ref["x"]) <= 1e-8 * np.linalg.norm(ref["x"])
+
+
+@pytest.mark.gpu
+def test_cpp_constrained_mesh_matches_oracle(tmp_path):
+    """A quadrilateral mesh with hanging sides through the C++ mirror: the
+    dependencies travel in the mesh file; pattern, entries, rhs and the CG
+    solution equal the oracle's constrained assembly."""
+    pb = P.Problem(material=P.DARCY, dim=2, p=2, nx=6, ny=6,
+                   el_coef=np.exp(np.random.default_rng(7).standard_normal(36)),
+                   force_kind=P.FORCE_CONST, force_const=(1.0, 0, 0), bc_type=[0, 1, 0, -1, -1, -1],
+                   bc_val=[[0, 0, 0], [0.4, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0]])
+    md = M.build_refined2d(pb, np.random.default_rng(8).random(36) < 0.4)
+    mfile, ofile = tmp_path / "mesh.bin", tmp_path / "out.bin"
+    M.save(md, mfile)
+    print("[gpu] pzhip_run constrained", flush=True)
+    r = subprocess.run([str(RUN), str(mfile), str(ofile), "cg", "1e-13", "20000"],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    res = _read_result(ofile)
+    o = O.Oracle(pb, mesh=md)
     ia, ja = o.pattern()
     a, rhs = o.assemble()
     assert np.array_equal(res["ia"], ia) and np.array_equal(res["ja"], ja)
