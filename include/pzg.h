@@ -318,6 +318,11 @@ typedef struct {
  * solution loaded already (TPZLinearAnalysis::Solve calls LoadSolution). */
 int pzg_load_solution(pzg_ctx *ctx, const double *u);
 
+/* Copies the loaded solution (the last solve's or pzg_load_solution's) to
+ * u[neq]: TPZLinearAnalysis::Solution / TPZCompMesh::Solution
+ * (Analysis/TPZAnalysis.h, Mesh/pzcmesh.h). */
+int pzg_get_solution(pzg_ctx *ctx, double *u);
+
 /* Integration points of the error norms of every volume element for an exact
  * solution of order porder: each element's rule is raised to
  * max(its rule order, porder), or to the rule family's highest order when

@@ -25,6 +25,7 @@ EXPORTS = [
     "pzg_learned_train", "pzg_learned_weights", "pzg_learned_graph", "pzg_learned_samples",
     "pzg_learned_loss_grad", "pzg_scaled_apply", "pzg_full_csr", "pzg_hp_tuning",
     "pzg_load_solution", "pzg_error_points", "pzg_evaluate_error", "pzg_sloan_renumber",
+    "pzg_get_solution",
 ]
 
 dp = C.POINTER(C.c_double)
@@ -131,6 +132,7 @@ def load():
     L.pzg_full_csr.argtypes = [vp, i64p, i64p, dp]
     L.pzg_hp_tuning.argtypes = [vp, i32p, i32p, i32p, fp]
     L.pzg_load_solution.argtypes = [vp, dp]
+    L.pzg_get_solution.argtypes = [vp, dp]
     L.pzg_error_points.argtypes = [vp, C.c_int32, i64p, dp]
     L.pzg_evaluate_error.argtypes = [vp, C.POINTER(Exact), dp, dp, i32p]
     L.pzg_sloan_renumber.argtypes = [C.c_int64, C.c_int64, i64p, i64p, i64p]

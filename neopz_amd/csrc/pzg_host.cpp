@@ -1151,6 +1151,18 @@ int pzg_load_solution(pzg_ctx *ctx, const double *u) {
   return PZG_OK;
 }
 
+int pzg_get_solution(pzg_ctx *ctx, double *u) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (!c.has_solution || !u) {
+    set_error(!u ? "pzg_get_solution: null output" : "pzg_get_solution: no solution loaded");
+    return !u ? PZG_EINVAL : PZG_ESTATE;
+  }
+  PZG_HIP(hipMemcpyAsync(u, c.d_u.p, c.neq * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  PZG_HIP(hipStreamSynchronize(c.stream));
+  return PZG_OK;
+}
+
 int pzg_error_points(pzg_ctx *ctx, int32_t porder, int64_t *npts, double *xq) {
   CTX_CHECK();
   return post_points(ctx->c, porder, npts, xq);

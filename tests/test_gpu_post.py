@@ -123,6 +123,9 @@ def test_errors_of_solution_match_oracle(name):
     # a caller-loaded solution (TPZCompMesh::LoadSolution)
     v = np.random.default_rng(3).standard_normal(len(u))
     an.LoadSolution(v)
+    back = np.zeros(len(v))
+    L.check(ctx().lib.pzg_get_solution(ctx().h, L.ptr(back, L.dp)))
+    assert np.array_equal(back, v)
     an.SetExact(lambda x: _exact_np(dim, ns, x), 0)
     tot = an.PostProcessError(store_error=True)
     xr = o.error_points(0)
