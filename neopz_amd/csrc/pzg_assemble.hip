@@ -774,7 +774,9 @@ static int run_vol(Ctx &c) {
 // shape i (1 for the shape's own unconstrained connect, the dependency
 // coefficients for a constrained one), K_c = T^T K T and F_c = T^T F over the
 // element's unconstrained blocks, added with AddKel's upper-part rule and
-// drop rule |v| < 1e-12, and AddFel.  One workgroup per element of a colour
+// drop rule |v| < 1e-12, and AddFel.  T is sparse (a list entry collects a
+// few shapes), so each entry of K_c is a short double sum over the shapes of
+// its row and column entries.  One workgroup per element of a colour
 // (elements of a colour share no block).
 template <int NSH, int NS>
 __global__ __launch_bounds__(TPB) void k_constrain(const int32_t *__restrict__ slots,
@@ -785,31 +787,41 @@ __global__ __launch_bounds__(TPB) void k_constrain(const int32_t *__restrict__ s
                                                    const int32_t *__restrict__ ncl, const int64_t *__restrict__ ia,
                                                    const int64_t *__restrict__ ja, double *__restrict__ a,
                                                    double *__restrict__ rhs, int bs) {
-  constexpr int NE = NSH * NS, NCS = CEL_NC * NS;
-  __shared__ double s_T[NSH][CEL_NC];
-  __shared__ double s_W[NE][NCS];  // W = K T
-  __shared__ double s_K[NE][NE];
+  constexpr int NE = NSH * NS;
+  __shared__ int s_cnt[CEL_NC];
+  __shared__ int s_i[CEL_NC][NSH];
+  __shared__ double s_c[CEL_NC][NSH];
   __shared__ int64_t s_b[CEL_NC];
   const int tid = threadIdx.x;
   const int64_t k = slots[blockIdx.x];
   const int n = ncl[k];
-  for (int x = tid; x < NSH * CEL_NC; x += TPB) s_T[x / CEL_NC][x % CEL_NC] = Tm[(k * nsv + x / CEL_NC) * CEL_NC + x % CEL_NC];
-  for (int x = tid; x < NE * NE; x += TPB) s_K[x / NE][x % NE] = ek[k * NE * NE + x];
-  if (tid < CEL_NC) s_b[tid] = blk[k * CEL_NC + tid];
-  __syncthreads();
-  for (int x = tid; x < NE * NCS; x += TPB) {
-    const int I = x / NCS, B = x % NCS, b = B / NS, cb = B % NS;
-    double w = 0.;
-    for (int j = 0; j < NSH; j++) w += s_K[I][j * NS + cb] * s_T[j][b];
-    s_W[I][B] = w;
+  const double *K = ek + k * NE * NE;
+  for (int e = tid; e < n; e += TPB) {
+    int cnt = 0;
+    for (int i = 0; i < NSH; i++) {
+      const double t = Tm[(k * nsv + i) * CEL_NC + e];
+      if (t != 0.) {
+        s_i[e][cnt] = i;
+        s_c[e][cnt] = t;
+        cnt++;
+      }
+    }
+    s_cnt[e] = cnt;
+    s_b[e] = blk[k * CEL_NC + e];
   }
   __syncthreads();
-  for (int x = tid; x < n * NS * n * NS; x += TPB) {
-    const int A = x / (n * NS), B = x % (n * NS), aa = A / NS, ca = A % NS;
-    const int64_t ra = s_b[aa] * bs + ca, rb = s_b[B / NS] * bs + B % NS;
+  const int nn = n * NS;
+  for (int x = tid; x < nn * nn; x += TPB) {
+    const int A = x / nn, B = x % nn, ea = A / NS, ca = A % NS, eb = B / NS, cb = B % NS;
+    const int64_t ra = s_b[ea] * bs + ca, rb = s_b[eb] * bs + cb;
     if (rb < ra) continue;
     double v = 0.;
-    for (int i = 0; i < NSH; i++) v += s_T[i][aa] * s_W[i * NS + ca][(B / NS) * NS + B % NS];
+    for (int p = 0; p < s_cnt[ea]; p++) {
+      const double *Kr = K + (s_i[ea][p] * NS + ca) * NE + cb;
+      double w = 0.;
+      for (int q = 0; q < s_cnt[eb]; q++) w += s_c[eb][q] * Kr[s_i[eb][q] * NS];
+      v += s_c[ea][p] * w;
+    }
     if (fabs(v) < 1.e-12) continue;
     int64_t l = ia[ra], h = ia[ra + 1] - 1;
     while (l <= h) {
@@ -822,11 +834,11 @@ __global__ __launch_bounds__(TPB) void k_constrain(const int32_t *__restrict__ s
       else h = m - 1;
     }
   }
-  for (int A = tid; A < n * NS; A += TPB) {
-    const int aa = A / NS, ca = A % NS;
+  for (int A = tid; A < nn; A += TPB) {
+    const int ea = A / NS, ca = A % NS;
     double f = 0.;
-    for (int i = 0; i < NSH; i++) f += s_T[i][aa] * ef[k * NE + i * NS + ca];
-    rhs[s_b[aa] * bs + ca] += f;
+    for (int p = 0; p < s_cnt[ea]; p++) f += s_c[ea][p] * ef[k * NE + s_i[ea][p] * NS + ca];
+    rhs[s_b[ea] * bs + ca] += f;
   }
 }
 
@@ -1024,11 +1036,25 @@ int launch_assemble(Ctx &c) {
     return PZG_OK;
   }
   if (c.dim == 3 && c.nstate == 3) {
-    if (p == 2) { PZG_TRY((run_vol<3, 8, 27, 27, 3>(c))); PZG_TRY((run_bc<2, 4, 9, 9, 3>(c))); }
-    else { PZG_TRY((run_vol<3, 8, 27, 8, 3>(c))); PZG_TRY((run_bc<2, 4, 9, 4, 3>(c))); }
+    if (p == 2) {
+      PZG_TRY((run_vol<3, 8, 27, 27, 3>(c)));
+      PZG_TRY((run_constrained<3, 8, 27, 27, 3>(c)));
+      PZG_TRY((run_bc<2, 4, 9, 9, 3>(c)));
+    } else {
+      PZG_TRY((run_vol<3, 8, 27, 8, 3>(c)));
+      PZG_TRY((run_constrained<3, 8, 27, 8, 3>(c)));
+      PZG_TRY((run_bc<2, 4, 9, 4, 3>(c)));
+    }
   } else if (c.dim == 3 && c.nstate == 1) {
-    if (p == 2) { PZG_TRY((run_vol<3, 8, 27, 27, 1>(c))); PZG_TRY((run_bc<2, 4, 9, 9, 1>(c))); }
-    else { PZG_TRY((run_vol<3, 8, 27, 8, 1>(c))); PZG_TRY((run_bc<2, 4, 9, 4, 1>(c))); }
+    if (p == 2) {
+      PZG_TRY((run_vol<3, 8, 27, 27, 1>(c)));
+      PZG_TRY((run_constrained<3, 8, 27, 27, 1>(c)));
+      PZG_TRY((run_bc<2, 4, 9, 9, 1>(c)));
+    } else {
+      PZG_TRY((run_vol<3, 8, 27, 8, 1>(c)));
+      PZG_TRY((run_constrained<3, 8, 27, 8, 1>(c)));
+      PZG_TRY((run_bc<2, 4, 9, 4, 1>(c)));
+    }
   } else if (c.dim == 2 && c.nstate == 1) {
     if (p == 2) {
       PZG_TRY((run_vol<2, 4, 9, 9, 1>(c)));

@@ -316,8 +316,8 @@ static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
           return PZG_EINVAL;
         }
     c.has_dep = c.dep_ptr[c.nconn] > 0;
-    if (c.has_dep && (c.dim != 2 || c.hp_simplex)) {
-      set_error("pzg_mesh_set: constrained connects on quadrilateral meshes only");
+    if (c.has_dep && (c.vol_etype == PZG_TET || c.hp_simplex)) {
+      set_error("pzg_mesh_set: constrained connects on quadrilateral / hexahedral meshes only");
       return PZG_EUNSUP;
     }
   }

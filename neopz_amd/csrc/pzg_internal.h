@@ -280,7 +280,7 @@ struct Ctx {
 };
 
 // constrained elements: at most CEL_NC unconstrained connects in the list
-constexpr int CEL_NC = 16;
+constexpr int CEL_NC = 64;
 
 // ---- element tables (pzg_shape.cpp) ----------------------------------------
 int build_tables(int etype, int p, int rule_order, ElemTables &t);

@@ -550,3 +550,226 @@ def build_refined2d(pb, refine):
         dep_coef=np.array(dv, dtype=np.float64), conn_order=np.where(kinds == K_CORNER, 1, pb.p),
         leaf_parent=np.array(parent, dtype=np.int64),
     )
+
+
+_HEX_EDGE = [(0, 1), (1, 2), (2, 3), (3, 0), (0, 4), (1, 5), (2, 6), (3, 7), (4, 5), (5, 6), (6, 7), (7, 4)]
+_HEX_FACE = [(0, 1, 2, 3), (0, 1, 5, 4), (1, 2, 6, 5), (3, 2, 6, 7), (0, 3, 7, 4), (4, 5, 6, 7)]
+_HEX_BITS = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+_HEX_FACE_BC = {4: 0, 2: 1, 1: 2, 3: 3, 0: 4, 5: 5}  # hexahedron face -> boundary face id
+
+
+def build_refined3d(pb, refine):
+    """A hexahedral mesh of the unit cube with hanging sides: the coarse
+    elements flagged in ``refine`` (which must not touch the boundary) are
+    divided into eight (children in corner order, TPZRefCube).  As in
+    :func:`build_refined2d`, every side of a divided element that lies inside a
+    side of an undivided one (the smallest such large side: an edge, else a
+    face) has its own connect restrained against the large side's connects
+    with shapes, with the coefficients of the large side's functions in the
+    small side's basis (TPZInterpolatedElement::RestrainSide,
+    Mesh/pzintel.cpp:847-1010).  For p <= 2 the large side's functions are
+    products of 1-t, t and 4t(1-t) along its directions and the coefficients
+    follow from values at points: a node takes phi(P); a half edge
+    phi(mid) - (phi(P0) + phi(P1))/2; a quarter face phi(centre) -
+    sum(phi(edge middles))/2 + sum(phi(corners))/4."""
+    if pb.dim != 3 or pb.p > 2 or pb.el_porder is not None or getattr(pb, "simplex", 0):
+        raise ValueError("build_refined3d: hexahedra, uniform p <= 2")
+    nx, ny, nz = pb.nx, pb.ny, pb.nz
+    refine = np.asarray(refine, dtype=bool).reshape(nz, ny, nx)
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                if refine[k, j, i] and (i in (0, nx - 1) or j in (0, ny - 1) or k in (0, nz - 1)):
+                    raise ValueError("build_refined3d: divided elements must not touch the boundary")
+    NX, NY, NZ = 2 * nx + 1, 2 * ny + 1, 2 * nz + 1
+    lat = lambda I, J, K: I + NX * (J + NY * K)
+    coords = lambda n: np.array([n % NX, (n // NX) % NY, n // (NX * NY)], dtype=np.int64)
+    els, parent, size = [], [], []
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                e = i + nx * (j + ny * k)
+                b = (2 * i, 2 * j, 2 * k)
+                if refine[k, j, i]:
+                    for c in range(8):
+                        o = [b[d] + _HEX_BITS[c][d] for d in range(3)]
+                        els.append([lat(o[0] + q[0], o[1] + q[1], o[2] + q[2]) for q in _HEX_BITS])
+                        parent.append(e)
+                        size.append(1)
+                else:
+                    els.append([lat(b[0] + 2 * q[0], b[1] + 2 * q[1], b[2] + 2 * q[2]) for q in _HEX_BITS])
+                    parent.append(e)
+                    size.append(2)
+    nel = len(els)
+
+    def side_keys(cn, e):
+        keys = [("n", c) for c in cn]
+        keys += [("e",) + tuple(sorted((cn[a], cn[b]))) for a, b in _HEX_EDGE]
+        keys += [("f",) + tuple(sorted(cn[q] for q in f)) for f in _HEX_FACE]
+        keys.append(("i", e))
+        return keys
+
+    keymap, seq_of = {}, []
+    econn = np.zeros((nel, 27), dtype=np.int64)
+    for e, cn in enumerate(els):
+        for s, key in enumerate(side_keys(cn, e)):
+            if key not in keymap:
+                keymap[key] = len(keymap)
+                seq_of.append(key)
+            econn[e, s] = keymap[key]
+    nconn = len(keymap)
+    used = {k[1] for k in keymap if k[0] == "n"}
+    has = lambda key: key in keymap
+    ekey = lambda a, b: ("e",) + tuple(sorted((a, b)))
+    fkey = lambda ns: ("f",) + tuple(sorted(ns))
+
+    def mid(a, b):  # lattice node halfway between two lattice nodes (coarse sides)
+        return int(lat(*((coords(a) + coords(b)) // 2)))
+
+    X = lambda n: coords(n).astype(np.float64)  # lattice coordinates of a node
+    half = lambda a, b: (X(a) + X(b)) / 2.0      # a point halfway (may be off the lattice)
+
+    dep = {}
+
+    def restrain(small_keys, large_conn, phi):
+        """small_keys: (key, kind, points) of small sides; phi(P) -> values of
+        the large side's functions at lattice point P."""
+        for key, kind, pts in small_keys:
+            if not has(key):
+                continue
+            if kind == "n":
+                v = phi(pts[0])
+            elif kind == "e":
+                v = phi(pts[2]) - (phi(pts[0]) + phi(pts[1])) / 2
+            else:  # quarter face: corners pts[0:4], edge middles pts[4:8], centre pts[8]
+                v = phi(pts[8]) - sum(phi(q) for q in pts[4:8]) / 2 + sum(phi(q) for q in pts[0:4]) / 4
+            v = np.where(np.abs(v) < 1e-14, 0.0, v)
+            dep[keymap[key]] = (list(large_conn), [float(x) for x in v])
+
+    for e, cn in enumerate(els):
+        if size[e] != 2:
+            continue
+        # hanging edges
+        for s, (a, b) in enumerate(_HEX_EDGE):
+            A, B = cn[a], cn[b]
+            M = mid(A, B)
+            if M not in used:
+                continue
+            cA, cB, cE = keymap[("n", A)], keymap[("n", B)], econn[e, 8 + s]
+            large = [cA, cB] + ([cE] if pb.p == 2 else [])
+            pa, pb_ = coords(A), coords(B)
+
+            def phi_e(P, pa=pa, pb_=pb_):
+                t = float(np.dot(P - pa, pb_ - pa)) / float(np.dot(pb_ - pa, pb_ - pa))
+                f = [1 - t, t] + ([4 * t * (1 - t)] if pb.p == 2 else [])
+                return np.array(f)
+
+            small = [(("n", M), "n", [X(M)])]
+            if pb.p == 2:
+                small += [(ekey(A, M), "e", [X(A), X(M), half(A, M)]),
+                          (ekey(M, B), "e", [X(M), X(B), half(M, B)])]
+            restrain(small, large, phi_e)
+        # hanging faces
+        for s, f in enumerate(_HEX_FACE):
+            C = [cn[q] for q in f]
+            O = int(lat(*((coords(C[0]) + coords(C[2])) // 2)))
+            if O not in used:
+                continue
+            large = [keymap[("n", c)] for c in C]
+            if pb.p == 2:
+                large += [keymap[ekey(C[q], C[(q + 1) % 4])] for q in range(4)] + [econn[e, 20 + s]]
+            c0, c1, c3 = coords(C[0]), coords(C[1]), coords(C[3])
+
+            def phi_f(P, c0=c0, c1=c1, c3=c3):
+                r = P - c0
+                u = float(np.dot(r, c1 - c0)) / float(np.dot(c1 - c0, c1 - c0))
+                v = float(np.dot(r, c3 - c0)) / float(np.dot(c3 - c0, c3 - c0))
+                vals = [(1 - u) * (1 - v), u * (1 - v), u * v, (1 - u) * v]
+                if pb.p == 2:
+                    vals += [4 * u * (1 - u) * (1 - v), 4 * v * (1 - v) * u, 4 * u * (1 - u) * v,
+                             4 * v * (1 - v) * (1 - u), 16 * u * (1 - u) * v * (1 - v)]
+                return np.array(vals)
+
+            Ms = [mid(C[q], C[(q + 1) % 4]) for q in range(4)]
+            small = [(("n", O), "n", [X(O)])]
+            if pb.p == 2:
+                for q in range(4):
+                    small.append((ekey(O, Ms[q]), "e", [X(O), X(Ms[q]), half(O, Ms[q])]))
+                for q in range(4):  # quarter face at corner C[q]
+                    qc = [C[q], Ms[q], O, Ms[(q + 3) % 4]]
+                    qm = [half(qc[r], qc[(r + 1) % 4]) for r in range(4)]
+                    small.append((fkey(qc), "f", [X(n) for n in qc] + qm + [half(qc[0], qc[2])]))
+            restrain(small, large, phi_f)
+
+    # boundary elements: boundary faces of the (undivided) elements
+    bn, bk, bid = [], [], []
+    for f_id in range(6):
+        if pb.bc_type[f_id] < 0:
+            continue
+        for e, cn in enumerate(els):
+            for s, f in enumerate(_HEX_FACE):
+                if _HEX_FACE_BC[s] != f_id:
+                    continue
+                C = [cn[q] for q in f]
+                cc = np.array([coords(c) for c in C])
+                ax, lo = {0: (0, 0), 1: (0, 2 * nx), 2: (1, 0), 3: (1, 2 * ny), 4: (2, 0), 5: (2, 2 * nz)}[f_id]
+                if not np.all(cc[:, ax] == lo):
+                    continue
+                bn.append(C)
+                conn = [keymap[("n", c)] for c in C] + \
+                       [keymap[ekey(C[q], C[(q + 1) % 4])] for q in range(4)] + [econn[e, 20 + s]]
+                bk.append(conn)
+                bid.append(f_id)
+    bc_nodes = np.array(bn, dtype=np.int64).reshape(-1, 4)
+    bc_conn = np.array(bk, dtype=np.int64).reshape(-1, 9)
+    bc_id = np.array(bid, dtype=np.int32)
+    # constrained connects behind the others (relative order kept)
+    is_dep = np.zeros(nconn, dtype=bool)
+    is_dep[list(dep.keys())] = True
+    order = np.concatenate([np.flatnonzero(~is_dep), np.flatnonzero(is_dep)])
+    perm = np.empty(nconn, dtype=np.int64)
+    perm[order] = np.arange(nconn)
+    econn = perm[econn]
+    bc_conn = perm[bc_conn] if len(bc_conn) else bc_conn
+    kinds = np.array([{"n": K_CORNER, "e": K_EDGE, "f": K_QFACE, "i": K_HEX}[k[0]] for k in seq_of])[order]
+    ns = pb.nstate
+    nshape = np.where(kinds == K_CORNER, 1, 1 if pb.p == 2 else 0)
+    blk_size = (nshape * ns).astype(np.int32)
+    blk_pos = np.zeros(nconn, dtype=np.int64)
+    blk_pos[1:] = np.cumsum(blk_size[:-1].astype(np.int64))
+    ndep = int(is_dep.sum())
+    neq = int(blk_size[: nconn - ndep].astype(np.int64).sum())
+    dep_ptr = np.zeros(nconn + 1, dtype=np.int64)
+    dc, dv = [], []
+    for newc in range(nconn):
+        oldc = order[newc]
+        if oldc in dep:
+            large, coef = dep[oldc]
+            dc += [int(perm[c]) for c in large]
+            dv += coef
+        dep_ptr[newc + 1] = len(dc)
+    nn = NX * NY * NZ
+    xyz = np.zeros((nn, 3))
+    idx = np.arange(nn)
+    xyz[:, 0] = (idx % NX) / (2.0 * nx)
+    xyz[:, 1] = ((idx // NX) % NY) / (2.0 * ny)
+    xyz[:, 2] = (idx // (NX * NY)) / (2.0 * nz)
+    coef = None if pb.el_coef is None else np.asarray(pb.el_coef, dtype=np.float64)[parent]
+    ff = pb.force_porder if pb.force_kind != P.FORCE_NONE else 0
+    rule = 2 * pb.p if (pb.material == P.ELAST3D or pb.p >= ff) else ff + pb.p
+    big = 1.0e12 if pb.material == P.ELAST3D else 10.0 ** 17 * 2 / 3
+    return dict(
+        vol_rule=None, bc_rule=None, dim=3, material=pb.material, nstate=ns, porder=pb.p,
+        xyz=xyz.ravel(), vol_nodes=np.array(els, dtype=np.int64), vol_conn=econn, vol_coef=coef,
+        bc_nodes=bc_nodes, bc_conn=bc_conn, bc_id=bc_id,
+        bc_type=np.array([pb.bc_type[f] for f in range(6)], dtype=np.int32),
+        bc_val=np.array([pb.bc_val[f] for f in range(6)], dtype=np.float64).ravel(),
+        nconn=nconn, blk_pos=blk_pos, blk_size=blk_size, neq=neq,
+        scale=pb.scale, E=pb.E, nu=pb.nu, rule_order_vol=rule, rule_order_bc=2 * pb.p,
+        force_kind=pb.force_kind, force_const=np.array(pb.force_const, dtype=np.float64),
+        bumps=None if pb.bumps is None else np.ascontiguousarray(pb.bumps, dtype=np.float64).reshape(-1, 6),
+        bump_width=pb.bump_width, big_number=big, bc_faces=[f for f in range(6) if pb.bc_type[f] >= 0],
+        vol_etype=0, dep_ptr=dep_ptr, dep_conn=np.array(dc, dtype=np.int64),
+        dep_coef=np.array(dv, dtype=np.float64), conn_order=np.where(kinds == K_CORNER, 1, pb.p),
+        leaf_parent=np.array(parent, dtype=np.int64),
+    )

@@ -1,14 +1,17 @@
 """Pins of the oracle's constrained assembly (hanging sides:
 orc_setup_pattern_constrained / orc_assemble_constrained, the restatement of
 TPZElementMatrixT::ApplyConstraints and of the assembly of constrained
-elements) on quadrilateral meshes with divided elements
-(neopz_amd.mesh.build_refined2d) -- CPU only.
+elements) on quadrilateral and hexahedral meshes with divided elements
+(neopz_amd.mesh.build_refined2d / build_refined3d) -- CPU only.
 
 None of these reruns the oracle's formulas:
   * patch tests: with the hanging sides constrained, u = x (p = 1, 2) and the
     quadratic u = x - x^2/2 (p = 2, f = 1) are reproduced exactly; a wrong
     dependency coefficient (1/2 at the middle node, 1 and 1/4 for the coarse
-    edge function) breaks conformity and the reproduction;
+    edge function; on hexahedra also the face centre, the half edges inside a
+    face and the quarter faces) breaks conformity and the reproduction;
+  * conformity: on hexahedra, a random unconstrained vector expanded by T
+    takes the same value from both sides of every face of a divided element;
   * energy: for any unconstrained vector u, u^T K_c u equals (T u)^T K (T u)
     with K the matrix of the same mesh assembled with every connect free and
     T the dependency map (K_c = T^T K T), and f_c . u = f . (T u);
@@ -55,14 +58,27 @@ def _dense(ia, ja, a, n):
     return U + np.triu(U, 1).T
 
 
-def _patch_problem(p, force=0.0, flux=1.0, coarse=4):
-    return P.Problem(material=P.POISSON, dim=2, p=p, nx=coarse, ny=coarse,
+def _patch_problem(p, force=0.0, flux=1.0, coarse=4, dim=2):
+    return P.Problem(material=P.POISSON, dim=dim, p=p, nx=coarse, ny=coarse,
+                     nz=coarse if dim == 3 else 1,
                      force_kind=P.FORCE_CONST if force else P.FORCE_NONE, force_const=(force, 0, 0),
                      bc_type=[0, 1, -1, -1, -1, -1],
                      bc_val=[[0, 0, 0], [flux, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0]])
 
 
 REFINE = [5, 6, 9, 0, 15]
+# interior coarse hexahedra of a 4^3 mesh (i, j, k in 1..2)
+REFINE3 = [1 + 4 * (1 + 4 * 1), 2 + 4 * (2 + 4 * 1), 2 + 4 * (1 + 4 * 2)]
+
+
+def _mesh(pb, dim):
+    if dim == 2:
+        ref = np.zeros(16, bool)
+        ref[REFINE] = True
+        return M.build_refined2d(pb, ref)
+    ref = np.zeros(64, bool)
+    ref[REFINE3] = True
+    return M.build_refined3d(pb, ref)
 
 
 def _solve(pb, md):
@@ -73,62 +89,108 @@ def _solve(pb, md):
     return o, spla.spsolve(K, rhs)
 
 
-@pytest.mark.parametrize("p", [1, 2])
-def test_linear_patch_with_hanging_sides(p):
-    pb = _patch_problem(p)
-    ref = np.zeros(16, bool)
-    ref[REFINE] = True
-    md = M.build_refined2d(pb, ref)
+_EDGES = {2: [(0, 1), (1, 2), (2, 3), (3, 0)], 3: M._HEX_EDGE}
+
+
+@pytest.mark.parametrize("dim,p", [(2, 1), (2, 2), (3, 1), (3, 2)])
+def test_linear_patch_with_hanging_sides(dim, p):
+    pb = _patch_problem(p, dim=dim)
+    md = _mesh(pb, dim)
     assert md["dep_ptr"][-1] > 0
     o, u = _solve(pb, md)
     T = _T(md, o.neq)
     ufull = T @ u
     xyz = md["xyz"].reshape(-1, 3)
+    nc, ns = (4, 9) if dim == 2 else (8, 27)
     for e in range(len(md["vol_nodes"])):
-        for k in range(4):
+        for k in range(nc):
             c = md["vol_conn"][e, k]
             assert abs(ufull[md["blk_pos"][c]] - xyz[md["vol_nodes"][e, k], 0]) < 1e-12
-        for k in range(4, 9):
+        for k in range(nc, ns):
             c = md["vol_conn"][e, k]
             if md["blk_size"][c]:
                 assert abs(ufull[md["blk_pos"][c]]) < 1e-12
 
 
-def test_quadratic_patch_with_hanging_sides():
+@pytest.mark.parametrize("dim", [2, 3])
+def test_quadratic_patch_with_hanging_sides(dim):
     """u = x - x^2/2 solves -u'' = 1, u(0) = 0, u'(1) = 0: corner values
-    u(node), horizontal edge coefficients u(mid) - average = h^2/8, vertical
-    ones 0 -- also at the constrained middle nodes and half edges, through
-    the coefficients 1 and 1/4 of the coarse edge function."""
-    pb = _patch_problem(2, force=1.0, flux=0.0)
-    ref = np.zeros(16, bool)
-    ref[REFINE] = True
-    md = M.build_refined2d(pb, ref)
+    u(node), edge coefficients u(mid) - average (h^2/8 along x, 0 across),
+    face and interior coefficients 0 -- also on the constrained sides, through
+    the dependency coefficients."""
+    pb = _patch_problem(2, force=1.0, flux=0.0, dim=dim)
+    md = _mesh(pb, dim)
     o, u = _solve(pb, md)
     ufull = _T(md, o.neq) @ u
     xyz = md["xyz"].reshape(-1, 3)
     f = lambda x: x - x * x / 2
+    nc, ns = (4, 9) if dim == 2 else (8, 27)
     for e in range(len(md["vol_nodes"])):
         nd = md["vol_nodes"][e]
-        for k in range(4):
+        for k in range(nc):
             c = md["vol_conn"][e, k]
             assert abs(ufull[md["blk_pos"][c]] - f(xyz[nd[k], 0])) < 1e-12
-        for k in range(4):
-            c = md["vol_conn"][e, 4 + k]
-            x0, x1 = xyz[nd[k], 0], xyz[nd[(k + 1) % 4], 0]
+        for k, (a, b) in enumerate(_EDGES[dim]):
+            c = md["vol_conn"][e, nc + k]
+            x0, x1 = xyz[nd[a], 0], xyz[nd[b], 0]
             expect = f((x0 + x1) / 2) - (f(x0) + f(x1)) / 2
             assert abs(ufull[md["blk_pos"][c]] - expect) < 1e-12
+        for k in range(nc + len(_EDGES[dim]), ns):
+            assert abs(ufull[md["blk_pos"][md["vol_conn"][e, k]]]) < 1e-12
 
 
-@pytest.mark.parametrize("p,material", [(1, P.POISSON), (2, P.POISSON), (2, P.DARCY)])
-def test_constrained_energy_identity(p, material):
-    rng = np.random.default_rng(p + 3 * material)
-    pb = P.Problem(material=material, dim=2, p=p, nx=5, ny=4, scale=1.3,
-                   el_coef=np.exp(rng.standard_normal(20)) if material == P.DARCY else None,
-                   force_kind=P.FORCE_CONST, force_const=(0.7, 0, 0),
-                   bc_type=[0, 1, 0, -1, -1, -1],
-                   bc_val=[[0.2, 0, 0], [1.0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0]])
-    ref = rng.random(20) < 0.4
-    md = M.build_refined2d(pb, ref)
+def test_hexahedral_conformity():
+    """A random unconstrained vector expanded by T is continuous across the
+    faces of the divided hexahedra (evaluated with the oracle's shapes from
+    the elements on both sides)."""
+    pb = _patch_problem(2, dim=3, coarse=3)
+    ref = np.zeros(27, bool)
+    ref[13] = True
+    md = M.build_refined3d(pb, ref)
+    xyz = md["xyz"].reshape(-1, 3)
+    vn, vc, pos = md["vol_nodes"], md["vol_conn"], md["blk_pos"]
+    rng = np.random.default_rng(0)
+    ufull = _T(md, md["neq"]) @ rng.standard_normal(md["neq"])
+
+    def value(e, x):
+        lo, hi = xyz[vn[e, 0]], xyz[vn[e, 6]]
+        xi = 2 * (x - lo) / (hi - lo) - 1
+        if np.any(np.abs(xi) > 1 + 1e-12):
+            return None
+        phi, _ = O.shape(O.HEX, 2, xi)
+        return sum(phi[s] * ufull[pos[vc[e, s]]] for s in range(27))
+
+    worst = 0.0
+    for _ in range(200):
+        fc = rng.integers(6)
+        x = rng.random(3) / 3 + 1 / 3
+        x[fc // 2] = 1 / 3 if fc % 2 == 0 else 2 / 3
+        vals = [v for v in (value(e, x) for e in range(len(vn))) if v is not None]
+        assert len(vals) >= 2
+        worst = max(worst, max(vals) - min(vals))
+    assert worst < 1e-12
+
+
+@pytest.mark.parametrize("dim,p,material", [(2, 1, P.POISSON), (2, 2, P.POISSON), (2, 2, P.DARCY),
+                                            (3, 1, P.POISSON), (3, 2, P.DARCY), (3, 2, P.ELAST3D)])
+def test_constrained_energy_identity(dim, p, material):
+    rng = np.random.default_rng(p + 3 * material + 7 * dim)
+    if dim == 2:
+        pb = P.Problem(material=material, dim=2, p=p, nx=5, ny=4, scale=1.3,
+                       el_coef=np.exp(rng.standard_normal(20)) if material == P.DARCY else None,
+                       force_kind=P.FORCE_CONST, force_const=(0.7, 0, 0),
+                       bc_type=[0, 1, 0, -1, -1, -1],
+                       bc_val=[[0.2, 0, 0], [1.0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0]])
+        md = M.build_refined2d(pb, rng.random(20) < 0.4)
+    else:
+        pb = P.Problem(material=material, dim=3, p=p, nx=4, ny=4, nz=3, scale=1.3, nu=0.3,
+                       el_coef=np.exp(rng.standard_normal(48)) if material != P.POISSON else None,
+                       force_kind=P.FORCE_CONST, force_const=(0.7, -0.2, 0.4),
+                       bc_type=[0, 1, 0, -1, -1, -1],
+                       bc_val=[[0.2, 0, 0], [1.0, 0.5, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0]])
+        ref = np.zeros(48, bool)
+        ref[[1 + 4 * (1 + 4 * 1), 2 + 4 * (2 + 4 * 1), 1 + 4 * 2 + 16]] = True
+        md = M.build_refined3d(pb, ref)
     o = O.Oracle(pb, mesh=md)
     ia, ja = o.pattern()
     a, rhs = o.assemble()
