@@ -119,8 +119,8 @@ typedef struct {
    * (TPZDepend::fDepMatrix, one shape per connect).  Constrained connects
    * carry no equations: their blocks come after all the others in sequence
    * order.  The connects they depend on are unconstrained (one level).
-   * Quadrilateral meshes (dim 2, p <= 2), volume elements only.  NULL = no
-   * constraints. */
+   * Quadrilateral and hexahedral meshes (p <= 2), volume elements only.
+   * NULL = no constraints. */
   const int64_t *dep_ptr;   /* nconn + 1, or NULL */
   const int64_t *dep_conn;
   const double *dep_coef;
