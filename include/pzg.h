@@ -119,7 +119,8 @@ typedef struct {
    * (TPZDepend::fDepMatrix, one shape per connect).  Constrained connects
    * carry no equations: their blocks come after all the others in sequence
    * order.  The connects they depend on are unconstrained (one level).
-   * Quadrilateral and hexahedral meshes (p <= 2), volume elements only.
+   * Quadrilateral and hexahedral meshes (p <= 2), volume and boundary
+   * elements (tetrahedral meshes: PZG_EUNSUP).
    * NULL = no constraints. */
   const int64_t *dep_ptr;   /* nconn + 1, or NULL */
   const int64_t *dep_conn;

@@ -374,7 +374,7 @@ __global__ __launch_bounds__(TPB) void k_vol(ElemArgs g, const int32_t *__restri
 }
 
 // Boundary elements (dimension DIMB = DIM-1 embedded in 3D).
-template <int DIMB, int NC, int NSD, int NSH, int NS>
+template <int DIMB, int NC, int NSD, int NSH, int NS, bool EK = false>
 __global__ __launch_bounds__(TPB) void k_bc(ElemArgs g, const int32_t *__restrict__ list) {
   constexpr int NE = NSH * NS;
   __shared__ double s_coord[NC * 3];
@@ -391,8 +391,9 @@ __global__ __launch_bounds__(TPB) void k_bc(ElemArgs g, const int32_t *__restric
   const int type = g.bctype[id];
   const double *val = g.bcval + 3 * id;
   if (tid < NC * 3) s_coord[tid] = g.xyz[3 * g.nodes[e * NC + tid / 3] + tid % 3];
-  for (int k = tid; k < NSD * NSD; k += TPB) s_map[k] = g.map[e * NSD * NSD + k];
-  if (tid < NSH) s_blk[tid] = g.aidx[g.conn[e * NSD + T.shape_side[tid]]];
+  if (!EK)
+    for (int k = tid; k < NSD * NSD; k += TPB) s_map[k] = g.map[e * NSD * NSD + k];
+  if (tid < NSH) s_blk[tid] = (EK ? g.aidx_ek : g.aidx)[g.conn[e * NSD + T.shape_side[tid]]];
   __syncthreads();
   if (tid < nq) {
     const int q = tid;
@@ -417,10 +418,11 @@ __global__ __launch_bounds__(TPB) void k_bc(ElemArgs g, const int32_t *__restric
     }
     s_wdet[q] = T.w[q] * fabs(det);
   }
-  for (int k = tid; k < NE; k += TPB) {
-    int64_t b = s_blk[k / NS];
-    s_row[k] = b >= 0 ? g.ia[b * g.bs + k % NS] : -1;
-  }
+  if (!EK)
+    for (int k = tid; k < NE; k += TPB) {
+      int64_t b = s_blk[k / NS];
+      s_row[k] = b >= 0 ? g.ia[b * g.bs + k % NS] : -1;
+    }
   __syncthreads();
   for (int k = tid; k < NSH * NSH; k += TPB) {
     int i = k / NSH, j = k % NSH;
@@ -429,6 +431,27 @@ __global__ __launch_bounds__(TPB) void k_bc(ElemArgs g, const int32_t *__restric
     s_M[i][j] = s;
   }
   __syncthreads();
+  if (EK) {
+    // the whole element matrix and load vector of list entry blockIdx.x
+    double *ek = g.ek_out + (int64_t)blockIdx.x * NE * NE;
+    for (int k = tid; k < NE * NE; k += TPB) {
+      const int I = k / NE, J = k % NE, i = I / NS, ci = I % NS, j = J / NS, cj = J % NS;
+      const bool on = type == PZG_BC_DIRICHLET && ci == cj && s_blk[i] != -1 && s_blk[j] != -1;
+      ek[k] = on ? g.big * s_M[i][j] : 0.;
+    }
+    for (int I = tid; I < NE; I += TPB) {
+      const int i = I / NS, ci = I % NS;
+      double f = 0.;
+      if (s_blk[i] != -1) {
+        double sq = 0.;
+        for (int q = 0; q < nq; q++) sq += T.phi[q][i] * s_wdet[q];
+        if (type == PZG_BC_DIRICHLET) f = g.big * val[ci] * sq;
+        else f = val[ci] * sq * (g.material == PZG_MAT_POISSON ? g.scale : 1.0);
+      }
+      g.ef_out[(int64_t)blockIdx.x * NE + I] = f;
+    }
+    return;
+  }
   if (type == PZG_BC_DIRICHLET) {
     for (int k = tid; k < NE * NE; k += TPB) {
       int I = k / NE, J = k % NE;
@@ -842,36 +865,69 @@ __global__ __launch_bounds__(TPB) void k_constrain(const int32_t *__restrict__ s
   }
 }
 
-template <int DIM, int NC, int NSD, int NSH, int NS>
-static int run_constrained(Ctx &c) {
-  if (!c.has_dep || c.cel.empty()) return PZG_OK;
-  constexpr int NE = NSH * NS;
-  const int64_t ncel = (int64_t)c.cel.size();
-  PZG_TRY(dev_alloc(c.d_cel_ek, sizeof(double) * ncel * NE * NE));
-  PZG_TRY(dev_alloc(c.d_cel_ef, sizeof(double) * ncel * NE));
-  ElemArgs g = make_args(c, true);
-  g.aidx_ek = c.d_aidx_ek.as<int64_t>();
-  for (size_t b = 0; b + 1 < c.cel_bptr.size(); b++) {
-    const int64_t n = c.cel_bptr[b + 1] - c.cel_bptr[b];
-    if (!n) continue;
-    g.tab = c.d_tvol.as<ElemTables>() + b;
-    g.ek_out = c.d_cel_ek.as<double>() + c.cel_bptr[b] * NE * NE;
-    g.ef_out = c.d_cel_ef.as<double>() + c.cel_bptr[b] * NE;
-    hipLaunchKernelGGL((k_vol<DIM, NC, NSD, NSH, NS, true>), dim3(n), dim3(TPB), 0, c.stream, g,
-                       c.d_cel.as<int32_t>() + c.cel_bptr[b]);
-    PZG_HIP(hipGetLastError());
-  }
-  for (size_t k = 0; k + 1 < c.cel_cptr.size(); k++) {
-    const int64_t n = c.cel_cptr[k + 1] - c.cel_cptr[k];
+// K_c = T^T K T of a set of constrained elements whose element matrices are
+// in cs.d_ek / cs.d_ef, colour by colour.
+template <int NSH, int NS>
+static int constrain_set(Ctx &c, CelSet &cs, int nsx) {
+  for (size_t k = 0; k + 1 < cs.cptr.size(); k++) {
+    const int64_t n = cs.cptr[k + 1] - cs.cptr[k];
     if (!n) continue;
     hipLaunchKernelGGL((k_constrain<NSH, NS>), dim3(n), dim3(TPB), 0, c.stream,
-                       c.d_cel_cslot.as<int32_t>() + c.cel_cptr[k], c.d_cel_ek.as<double>(),
-                       c.d_cel_ef.as<double>(), c.d_cel_T.as<double>(), c.vol_nsides,
-                       c.d_cel_blk.as<int64_t>(), c.d_cel_n.as<int32_t>(), c.d_ia.as<int64_t>(),
-                       c.d_ja.as<int64_t>(), c.d_a.as<double>(), c.d_rhs.as<double>(), c.bs);
+                       cs.d_cslot.as<int32_t>() + cs.cptr[k], cs.d_ek.as<double>(), cs.d_ef.as<double>(),
+                       cs.d_T.as<double>(), nsx, cs.d_blk.as<int64_t>(), cs.d_n.as<int32_t>(),
+                       c.d_ia.as<int64_t>(), c.d_ja.as<int64_t>(), c.d_a.as<double>(),
+                       c.d_rhs.as<double>(), c.bs);
     PZG_HIP(hipGetLastError());
   }
   return PZG_OK;
+}
+
+// constrained volume elements: element matrices (k_vol, EK), then K_c
+template <int DIM, int NC, int NSD, int NSH, int NS>
+static int run_constrained(Ctx &c) {
+  CelSet &cs = c.cv;
+  if (!c.has_dep || cs.cel.empty()) return PZG_OK;
+  constexpr int NE = NSH * NS;
+  const int64_t ncel = (int64_t)cs.cel.size();
+  PZG_TRY(dev_alloc(cs.d_ek, sizeof(double) * ncel * NE * NE));
+  PZG_TRY(dev_alloc(cs.d_ef, sizeof(double) * ncel * NE));
+  ElemArgs g = make_args(c, true);
+  g.aidx_ek = c.d_aidx_ek.as<int64_t>();
+  for (size_t b = 0; b + 1 < cs.bptr.size(); b++) {
+    const int64_t n = cs.bptr[b + 1] - cs.bptr[b];
+    if (!n) continue;
+    g.tab = c.d_tvol.as<ElemTables>() + b;
+    g.ek_out = cs.d_ek.as<double>() + cs.bptr[b] * NE * NE;
+    g.ef_out = cs.d_ef.as<double>() + cs.bptr[b] * NE;
+    hipLaunchKernelGGL((k_vol<DIM, NC, NSD, NSH, NS, true>), dim3(n), dim3(TPB), 0, c.stream, g,
+                       cs.d_cel.as<int32_t>() + cs.bptr[b]);
+    PZG_HIP(hipGetLastError());
+  }
+  return constrain_set<NSH, NS>(c, cs, c.vol_nsides);
+}
+
+// constrained boundary elements: element matrices (k_bc, EK), then K_c
+template <int DIMB, int NC, int NSD, int NSH, int NS>
+static int run_constrained_bc(Ctx &c) {
+  CelSet &cs = c.cb;
+  if (!c.has_dep || cs.cel.empty()) return PZG_OK;
+  constexpr int NE = NSH * NS;
+  const int64_t ncel = (int64_t)cs.cel.size();
+  PZG_TRY(dev_alloc(cs.d_ek, sizeof(double) * ncel * NE * NE));
+  PZG_TRY(dev_alloc(cs.d_ef, sizeof(double) * ncel * NE));
+  ElemArgs g = make_args(c, false);
+  g.aidx_ek = c.d_aidx_ek.as<int64_t>();
+  for (size_t b = 0; b + 1 < cs.bptr.size(); b++) {
+    const int64_t n = cs.bptr[b + 1] - cs.bptr[b];
+    if (!n) continue;
+    g.tab = c.d_tbc.as<ElemTables>() + b;
+    g.ek_out = cs.d_ek.as<double>() + cs.bptr[b] * NE * NE;
+    g.ef_out = cs.d_ef.as<double>() + cs.bptr[b] * NE;
+    hipLaunchKernelGGL((k_bc<DIMB, NC, NSD, NSH, NS, true>), dim3(n), dim3(TPB), 0, c.stream, g,
+                       cs.d_cel.as<int32_t>() + cs.bptr[b]);
+    PZG_HIP(hipGetLastError());
+  }
+  return constrain_set<NSH, NS>(c, cs, c.bc_nsides);
 }
 
 template <int DIMB, int NC, int NSD, int NSH, int NS>
@@ -1040,30 +1096,36 @@ int launch_assemble(Ctx &c) {
       PZG_TRY((run_vol<3, 8, 27, 27, 3>(c)));
       PZG_TRY((run_constrained<3, 8, 27, 27, 3>(c)));
       PZG_TRY((run_bc<2, 4, 9, 9, 3>(c)));
+      PZG_TRY((run_constrained_bc<2, 4, 9, 9, 3>(c)));
     } else {
       PZG_TRY((run_vol<3, 8, 27, 8, 3>(c)));
       PZG_TRY((run_constrained<3, 8, 27, 8, 3>(c)));
       PZG_TRY((run_bc<2, 4, 9, 4, 3>(c)));
+      PZG_TRY((run_constrained_bc<2, 4, 9, 4, 3>(c)));
     }
   } else if (c.dim == 3 && c.nstate == 1) {
     if (p == 2) {
       PZG_TRY((run_vol<3, 8, 27, 27, 1>(c)));
       PZG_TRY((run_constrained<3, 8, 27, 27, 1>(c)));
       PZG_TRY((run_bc<2, 4, 9, 9, 1>(c)));
+      PZG_TRY((run_constrained_bc<2, 4, 9, 9, 1>(c)));
     } else {
       PZG_TRY((run_vol<3, 8, 27, 8, 1>(c)));
       PZG_TRY((run_constrained<3, 8, 27, 8, 1>(c)));
       PZG_TRY((run_bc<2, 4, 9, 4, 1>(c)));
+      PZG_TRY((run_constrained_bc<2, 4, 9, 4, 1>(c)));
     }
   } else if (c.dim == 2 && c.nstate == 1) {
     if (p == 2) {
       PZG_TRY((run_vol<2, 4, 9, 9, 1>(c)));
       PZG_TRY((run_constrained<2, 4, 9, 9, 1>(c)));
       PZG_TRY((run_bc<1, 2, 3, 3, 1>(c)));
+      PZG_TRY((run_constrained_bc<1, 2, 3, 3, 1>(c)));
     } else {
       PZG_TRY((run_vol<2, 4, 9, 4, 1>(c)));
       PZG_TRY((run_constrained<2, 4, 9, 4, 1>(c)));
       PZG_TRY((run_bc<1, 2, 3, 2, 1>(c)));
+      PZG_TRY((run_constrained_bc<1, 2, 3, 2, 1>(c)));
     }
   } else {
     set_error("pzg_assemble: unsupported dimension/state combination");

@@ -70,9 +70,10 @@ static void free_all(Ctx &c) {
                     &c.d_hp_vsub, &c.d_hp_bsub, &c.d_thp_vol, &c.d_thp_bc, &c.d_mf_blk,
                     &c.d_mf_xe, &c.d_mf_bblk, &c.d_mf_bm, &c.d_mf_ptr, &c.d_mf_pos, &c.d_mf_bpos,
                     &c.d_mf_tab, &c.d_mf_ye, &c.d_mf_coef, &c.d_mf_dis, &c.d_mf_mfma, &c.d_mf_geo,
-                    &c.d_mf_pt, &c.d_cel, &c.d_cel_cslot, &c.d_cel_T, &c.d_cel_blk, &c.d_cel_n,
-                    &c.d_cel_ek, &c.d_cel_ef, &c.d_aidx_ek, &c.d_dep_ptr, &c.d_dep_blk,
-                    &c.d_dep_coef};
+                    &c.d_mf_pt, &c.d_aidx_ek, &c.d_dep_ptr, &c.d_dep_blk, &c.d_dep_coef,
+                    &c.cv.d_cel, &c.cv.d_cslot, &c.cv.d_T, &c.cv.d_blk, &c.cv.d_n, &c.cv.d_ek,
+                    &c.cv.d_ef, &c.cb.d_cel, &c.cb.d_cslot, &c.cb.d_T, &c.cb.d_blk, &c.cb.d_n,
+                    &c.cb.d_ek, &c.cb.d_ef};
   for (DevBuf *b : bufs) dev_free(*b);
 }
 
@@ -430,81 +431,80 @@ static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
     subs(PZG_TRI, c.bc_nsides, c.bc_conn, c.nbc, c.bc_bucket, c.bc_rules.size(), c.hp_bc_P, bsub);
   }
 
-  // constrained elements (hanging sides): the volume elements with a
-  // constrained connect that has shapes, grouped by rule bucket; per element
-  // the unconstrained blocks of its connects and of the connects they depend
-  // on (TPZConnect::BuildConnectList, sorted by connect) and T
-  c.cel.clear();
-  c.is_cel.assign(c.nvol, 0);
-  c.cel_xptr.assign(c.nvol + 1, 0);
-  c.cel_xconn.clear();
-  c.cel_bptr.assign(c.vol_rules.size() + 1, 0);
-  std::vector<double> celT;
-  std::vector<int64_t> celB;
-  std::vector<int32_t> celN;
+  // constrained elements (hanging sides): the volume and boundary elements
+  // with a constrained connect that has shapes, grouped by rule bucket; per
+  // element the unconstrained blocks of its connects and of the connects they
+  // depend on (TPZConnect::BuildConnectList, sorted by connect) and T
   std::vector<int64_t> aidx_ek(c.nconn), dep_blk;
   for (int64_t s = 0; s < c.nconn; s++)
     aidx_ek[s] = c.aidx[s] >= 0 ? c.aidx[s] : (c.is_dep[s] && c.blk_size[s] > 0 ? -2 : -1);
-  if (c.has_dep) {
-    const int nsv = c.vol_nsides, nsb = c.bc_nsides;
-    for (int64_t e = 0; e < c.nbc; e++)
-      for (int k = 0; k < nsb; k++) {
-        const int64_t cc = c.bc_conn[e * nsb + k];
-        if (c.is_dep[cc] && c.blk_size[cc] > 0) {
-          set_error("pzg_mesh_set: boundary elements on constrained connects are not supported");
-          return PZG_EUNSUP;
-        }
-      }
-    for (int64_t e = 0; e < c.nvol; e++) {
-      for (int k = 0; k < nsv; k++) {
-        const int64_t cc = c.vol_conn[e * nsv + k];
+  std::vector<double> cvT, cbT;
+  std::vector<int64_t> cvB, cbB;
+  std::vector<int32_t> cvN, cbN;
+  auto build_cel = [&](CelSet &cs, int64_t nel, int nsx, const std::vector<int64_t> &conn,
+                       const std::vector<int32_t> &bucket, size_t nbk, const ElemTables &tab,
+                       std::vector<double> &T, std::vector<int64_t> &B, std::vector<int32_t> &N) -> int {
+    cs.cel.clear();
+    cs.is_cel.assign(nel, 0);
+    cs.xptr.assign(nel + 1, 0);
+    cs.xconn.clear();
+    cs.bptr.assign(nbk + 1, 0);
+    if (!c.has_dep) return PZG_OK;
+    for (int64_t e = 0; e < nel; e++) {
+      for (int k = 0; k < nsx; k++) {
+        const int64_t cc = conn[e * nsx + k];
         if (!c.is_dep[cc] || c.blk_size[cc] == 0) continue;
-        c.is_cel[e] = 1;
-        for (int64_t d = c.dep_ptr[cc]; d < c.dep_ptr[cc + 1]; d++) c.cel_xconn.push_back(c.dep_conn[d]);
+        cs.is_cel[e] = 1;
+        for (int64_t d = c.dep_ptr[cc]; d < c.dep_ptr[cc + 1]; d++) cs.xconn.push_back(c.dep_conn[d]);
       }
-      c.cel_xptr[e + 1] = (int64_t)c.cel_xconn.size();
+      cs.xptr[e + 1] = (int64_t)cs.xconn.size();
     }
-    for (size_t b = 0; b < c.vol_rules.size(); b++) {
-      for (int64_t e = 0; e < c.nvol; e++)
-        if (c.is_cel[e] && c.vol_bucket[e] == (int32_t)b) c.cel.push_back((int32_t)e);
-      c.cel_bptr[b + 1] = (int64_t)c.cel.size();
+    for (size_t b = 0; b < nbk; b++) {
+      for (int64_t e = 0; e < nel; e++)
+        if (cs.is_cel[e] && bucket[e] == (int32_t)b) cs.cel.push_back((int32_t)e);
+      cs.bptr[b + 1] = (int64_t)cs.cel.size();
     }
-    const int nsh = c.tvol_all[0].nsh;
-    for (int32_t e : c.cel) {
+    for (int32_t e : cs.cel) {
       std::vector<int64_t> L;
-      for (int k = 0; k < nsv; k++) L.push_back(c.vol_conn[(int64_t)e * nsv + k]);
-      for (int64_t x = c.cel_xptr[e]; x < c.cel_xptr[e + 1]; x++) L.push_back(c.cel_xconn[x]);
+      for (int k = 0; k < nsx; k++) L.push_back(conn[(int64_t)e * nsx + k]);
+      for (int64_t x = cs.xptr[e]; x < cs.xptr[e + 1]; x++) L.push_back(cs.xconn[x]);
       std::sort(L.begin(), L.end());
       L.erase(std::unique(L.begin(), L.end()), L.end());
-      std::vector<int64_t> B, Lc;
+      std::vector<int64_t> Bl, Lc;
       for (int64_t cc : L)
         if (c.aidx[cc] >= 0) {
-          B.push_back(c.aidx[cc]);
+          Bl.push_back(c.aidx[cc]);
           Lc.push_back(cc);
         }
-      if ((int)B.size() > CEL_NC) {
+      if ((int)Bl.size() > CEL_NC) {
         set_error("pzg_mesh_set: a constrained element couples too many connects");
         return PZG_EUNSUP;
       }
       auto slot = [&](int64_t cc) {
         return (int)(std::lower_bound(Lc.begin(), Lc.end(), cc) - Lc.begin());
       };
-      const size_t t0 = celT.size();
-      celT.resize(t0 + (size_t)nsv * CEL_NC, 0.);
-      for (int i = 0; i < nsh; i++) {  // p <= 2: shape i lives on side i
-        const int64_t cc = c.vol_conn[(int64_t)e * nsv + c.tvol_all[0].shape_side[i]];
+      const size_t t0 = T.size();
+      T.resize(t0 + (size_t)nsx * CEL_NC, 0.);
+      for (int i = 0; i < tab.nsh; i++) {  // p <= 2: shape i lives on side shape_side[i]
+        const int64_t cc = conn[(int64_t)e * nsx + tab.shape_side[i]];
         if (c.blk_size[cc] == 0) continue;
         if (c.aidx[cc] >= 0) {
-          celT[t0 + (size_t)i * CEL_NC + slot(cc)] = 1.;
+          T[t0 + (size_t)i * CEL_NC + slot(cc)] = 1.;
           continue;
         }
         for (int64_t d = c.dep_ptr[cc]; d < c.dep_ptr[cc + 1]; d++)
-          if (c.aidx[c.dep_conn[d]] >= 0)
-            celT[t0 + (size_t)i * CEL_NC + slot(c.dep_conn[d])] += c.dep_coef[d];
+          if (c.aidx[c.dep_conn[d]] >= 0) T[t0 + (size_t)i * CEL_NC + slot(c.dep_conn[d])] += c.dep_coef[d];
       }
-      for (int a = 0; a < CEL_NC; a++) celB.push_back(a < (int)B.size() ? B[a] : -1);
-      celN.push_back((int32_t)B.size());
+      for (int a = 0; a < CEL_NC; a++) B.push_back(a < (int)Bl.size() ? Bl[a] : -1);
+      N.push_back((int32_t)Bl.size());
     }
+    return PZG_OK;
+  };
+  PZG_TRY(build_cel(c.cv, c.nvol, c.vol_nsides, c.vol_conn, c.vol_bucket, c.vol_rules.size(),
+                    c.tvol_all[0], cvT, cvB, cvN));
+  PZG_TRY(build_cel(c.cb, c.nbc, c.bc_nsides, c.bc_conn, c.bc_bucket, c.bc_rules.size(),
+                    c.tbc_all[0], cbT, cbB, cbN));
+  if (c.has_dep) {
     dep_blk.resize(c.dep_conn.size());
     for (size_t k = 0; k < c.dep_conn.size(); k++) dep_blk[k] = c.aidx[c.dep_conn[k]];
   }
@@ -513,10 +513,14 @@ static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
   PZG_TRY(dev_upload(c.d_xyz, c.xyz.data(), c.xyz.size(), s));
   PZG_TRY(dev_upload(c.d_aidx_ek, aidx_ek.data(), aidx_ek.size(), s));
   if (c.has_dep) {
-    PZG_TRY(dev_upload(c.d_cel, c.cel.data(), c.cel.size(), s));
-    PZG_TRY(dev_upload(c.d_cel_T, celT.data(), celT.size(), s));
-    PZG_TRY(dev_upload(c.d_cel_blk, celB.data(), celB.size(), s));
-    PZG_TRY(dev_upload(c.d_cel_n, celN.data(), celN.size(), s));
+    PZG_TRY(dev_upload(c.cv.d_cel, c.cv.cel.data(), c.cv.cel.size(), s));
+    PZG_TRY(dev_upload(c.cv.d_T, cvT.data(), cvT.size(), s));
+    PZG_TRY(dev_upload(c.cv.d_blk, cvB.data(), cvB.size(), s));
+    PZG_TRY(dev_upload(c.cv.d_n, cvN.data(), cvN.size(), s));
+    PZG_TRY(dev_upload(c.cb.d_cel, c.cb.cel.data(), c.cb.cel.size(), s));
+    PZG_TRY(dev_upload(c.cb.d_T, cbT.data(), cbT.size(), s));
+    PZG_TRY(dev_upload(c.cb.d_blk, cbB.data(), cbB.size(), s));
+    PZG_TRY(dev_upload(c.cb.d_n, cbN.data(), cbN.size(), s));
     PZG_TRY(dev_upload(c.d_dep_ptr, c.dep_ptr.data(), c.dep_ptr.size(), s));
     PZG_TRY(dev_upload(c.d_dep_blk, dep_blk.data(), dep_blk.size(), s));
     PZG_TRY(dev_upload(c.d_dep_coef, c.dep_coef.data(), c.dep_coef.size(), s));
@@ -576,13 +580,20 @@ static int struct_create(Ctx &c) {
   // depend on (TPZCompMesh::ComputeElGraph over BuildConnectList)
   const bool dep = c.has_dep;
   auto elem_conn = [&](int64_t e, int s) -> int64_t {
-    if (e >= c.nvol) return c.aidx[c.bc_conn[(e - c.nvol) * nsb + s]];
+    if (e >= c.nvol) {
+      const int64_t b = e - c.nvol;
+      if (s < nsb) return c.aidx[c.bc_conn[b * nsb + s]];
+      return c.aidx[c.cb.xconn[c.cb.xptr[b] + s - nsb]];
+    }
     if (s < nsv) return c.aidx[c.vol_conn[e * nsv + s]];
-    return c.aidx[c.cel_xconn[c.cel_xptr[e] + s - nsv]];
+    return c.aidx[c.cv.xconn[c.cv.xptr[e] + s - nsv]];
   };
   auto elem_ns = [&](int64_t e) -> int {
-    if (e >= c.nvol) return nsb;
-    return dep ? nsv + (int)(c.cel_xptr[e + 1] - c.cel_xptr[e]) : nsv;
+    if (e >= c.nvol) {
+      const int64_t b = e - c.nvol;
+      return dep ? nsb + (int)(c.cb.xptr[b + 1] - c.cb.xptr[b]) : nsb;
+    }
+    return dep ? nsv + (int)(c.cv.xptr[e + 1] - c.cv.xptr[e]) : nsv;
   };
   // block -> element incidence
   std::vector<int64_t> bptr(nb + 1, 0);
@@ -697,23 +708,26 @@ static int struct_create(Ctx &c) {
   std::vector<int32_t> vlist, blist;
   PZG_TRY(color_elements(c.nvol, nsv, c.vol_conn, c.aidx, nb, vptr, vlist));
   group_by_bucket(vptr, vlist, c.vol_bucket, (int)c.vol_rules.size());
-  if (c.has_dep) {
-    std::vector<int64_t> p2(vptr.size(), 0);
+  // constrained elements leave their colours; they get colours of their own
+  // over their unconstrained blocks (first fit)
+  auto drop_cel = [&](std::vector<int64_t> &ptr, std::vector<int32_t> &list, const CelSet &cs) {
+    std::vector<int64_t> p2(ptr.size(), 0);
     std::vector<int32_t> l2;
-    for (size_t k = 0; k + 1 < vptr.size(); k++) {
-      for (int64_t i = vptr[k]; i < vptr[k + 1]; i++)
-        if (!c.is_cel[vlist[i]]) l2.push_back(vlist[i]);
+    for (size_t k = 0; k + 1 < ptr.size(); k++) {
+      for (int64_t i = ptr[k]; i < ptr[k + 1]; i++)
+        if (!cs.is_cel[list[i]]) l2.push_back(list[i]);
       p2[k + 1] = (int64_t)l2.size();
     }
-    vptr.swap(p2);
-    vlist.swap(l2);
-    // constrained elements: first-fit colours over their unconstrained blocks
-    const int64_t ncel = (int64_t)c.cel.size();
+    ptr.swap(p2);
+    list.swap(l2);
+  };
+  auto color_cel = [&](CelSet &cs, int64_t eoff) -> int {
+    const int64_t ncel = (int64_t)cs.cel.size();
     std::vector<uint64_t> used(nb, 0);
     std::vector<int> col(ncel);
     int ncol = 0;
     for (int64_t k = 0; k < ncel; k++) {
-      const int32_t e = c.cel[k];
+      const int64_t e = eoff + cs.cel[k];
       std::vector<int64_t> bl;
       for (int q = 0; q < elem_ns(e); q++)
         if (elem_conn(e, q) >= 0) bl.push_back(elem_conn(e, q));
@@ -727,21 +741,29 @@ static int struct_create(Ctx &c) {
       ncol = std::max(ncol, col[k] + 1);
       for (int64_t b : bl) used[b] |= 1ULL << col[k];
     }
-    c.cel_cptr.assign(ncol + 1, 0);
-    for (int64_t k = 0; k < ncel; k++) c.cel_cptr[col[k] + 1]++;
-    for (int k = 0; k < ncol; k++) c.cel_cptr[k + 1] += c.cel_cptr[k];
+    cs.cptr.assign(ncol + 1, 0);
+    for (int64_t k = 0; k < ncel; k++) cs.cptr[col[k] + 1]++;
+    for (int k = 0; k < ncol; k++) cs.cptr[k + 1] += cs.cptr[k];
     std::vector<int32_t> cslot(ncel);
-    std::vector<int64_t> fill(c.cel_cptr.begin(), c.cel_cptr.end() - 1);
+    std::vector<int64_t> fill(cs.cptr.begin(), cs.cptr.end() - 1);
     for (int64_t k = 0; k < ncel; k++) cslot[fill[col[k]]++] = (int32_t)k;
-    PZG_TRY(dev_upload(c.d_cel_cslot, cslot.data(), cslot.size(), s));
+    PZG_TRY(dev_upload(cs.d_cslot, cslot.data(), cslot.size(), s));
+    return PZG_OK;
+  };
+  if (c.has_dep) {
+    drop_cel(vptr, vlist, c.cv);
+    PZG_TRY(color_cel(c.cv, 0));
+    PZG_TRY(color_cel(c.cb, c.nvol));
   } else {
-    c.cel_cptr.assign(1, 0);
+    c.cv.cptr.assign(1, 0);
+    c.cb.cptr.assign(1, 0);
   }
   c.vcolor_ptr = vptr;
   PZG_TRY(dev_upload(c.d_vcolor_list, vlist.data(), vlist.size(), s));
   if (c.nbc > 0) {
     PZG_TRY(color_elements(c.nbc, nsb, c.bc_conn, c.aidx, nb, bptr2, blist));
     group_by_bucket(bptr2, blist, c.bc_bucket, (int)c.bc_rules.size());
+    if (c.has_dep) drop_cel(bptr2, blist, c.cb);
     c.bcolor_ptr = bptr2;
     PZG_TRY(dev_upload(c.d_bcolor_list, blist.data(), blist.size(), s));
   } else {

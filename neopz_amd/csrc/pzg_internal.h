@@ -154,6 +154,16 @@ struct SolveScalars {
   int32_t done, pad;
 };
 
+// Elements (volume or boundary) that hold a constrained connect with shapes.
+struct CelSet {
+  std::vector<int32_t> cel;          // element ids, grouped by rule bucket
+  std::vector<char> is_cel;          // per element
+  std::vector<int64_t> xptr, xconn;  // per element: the connects its constrained ones depend on
+  std::vector<int64_t> bptr;         // bucket b: cel[bptr[b] .. bptr[b+1])
+  std::vector<int64_t> cptr;         // colour k: d_cslot[cptr[k] .. cptr[k+1])
+  DevBuf d_cel, d_cslot, d_T, d_blk, d_n, d_ek, d_ef;
+};
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -208,12 +218,8 @@ struct Ctx {
   std::vector<int64_t> dep_ptr, dep_conn;
   std::vector<double> dep_coef;
   std::vector<char> is_dep;
-  std::vector<int32_t> cel;               // constrained elements, by rule bucket
-  std::vector<char> is_cel;               // per volume element
-  std::vector<int64_t> cel_xptr, cel_xconn;  // per volume element: connects it depends on
-  std::vector<int64_t> cel_bptr;          // bucket b: cel[cel_bptr[b] .. cel_bptr[b+1])
-  std::vector<int64_t> cel_cptr;          // colour k: d_cel_cslot[cel_cptr[k] .. ]
-  DevBuf d_cel, d_cel_cslot, d_cel_T, d_cel_blk, d_cel_n, d_cel_ek, d_cel_ef, d_aidx_ek;
+  CelSet cv, cb;                          // constrained volume / boundary elements
+  DevBuf d_aidx_ek;                       // connect -> block, -2 constrained, -1 no shapes
   DevBuf d_dep_ptr, d_dep_blk, d_dep_coef;
 
   // ---- pattern (device) -------------------------------------------------

@@ -560,8 +560,10 @@ _HEX_FACE_BC = {4: 0, 2: 1, 1: 2, 3: 3, 0: 4, 5: 5}  # hexahedron face -> bounda
 
 def build_refined3d(pb, refine):
     """A hexahedral mesh of the unit cube with hanging sides: the coarse
-    elements flagged in ``refine`` (which must not touch the boundary) are
-    divided into eight (children in corner order, TPZRefCube).  As in
+    elements flagged in ``refine`` are divided into eight (children in corner
+    order, TPZRefCube); boundary elements are the boundary faces of the
+    elements (a divided element's children's faces, which may hold
+    constrained connects on a coarse boundary edge).  As in
     :func:`build_refined2d`, every side of a divided element that lies inside a
     side of an undivided one (the smallest such large side: an edge, else a
     face) has its own connect restrained against the large side's connects
@@ -576,11 +578,6 @@ def build_refined3d(pb, refine):
         raise ValueError("build_refined3d: hexahedra, uniform p <= 2")
     nx, ny, nz = pb.nx, pb.ny, pb.nz
     refine = np.asarray(refine, dtype=bool).reshape(nz, ny, nx)
-    for k in range(nz):
-        for j in range(ny):
-            for i in range(nx):
-                if refine[k, j, i] and (i in (0, nx - 1) or j in (0, ny - 1) or k in (0, nz - 1)):
-                    raise ValueError("build_refined3d: divided elements must not touch the boundary")
     NX, NY, NZ = 2 * nx + 1, 2 * ny + 1, 2 * nz + 1
     lat = lambda I, J, K: I + NX * (J + NY * K)
     coords = lambda n: np.array([n % NX, (n // NX) % NY, n // (NX * NY)], dtype=np.int64)

@@ -4,8 +4,12 @@ quadrilateral and hexahedral meshes with divided elements
 TPZMatPoisson, TPZDarcyFlow and TPZElasticity3D with element coefficients,
 Dirichlet and Neumann faces.
 
-Bars (as for unconstrained meshes): pattern bit-exact; values
-|a - a_ref| <= 1e-12 |a_ref| + 1e-15 max|row|; rhs <= 1e-12 max|rhs|;
+Bars: pattern bit-exact; values |a - a_ref| <= 1e-12 |a_ref| + 1e-14 max|row|
+(the floor is ten times the unconstrained one: an entry of T^T K T that is
+zero in exact arithmetic is a sum of products of several elements' entries
+and the coefficients, and its residue, +-5e-17 for a row maximum of 0.085 on
+the 6^3 elasticity mesh, depends on the summation order); rhs <= 1e-12
+max|rhs|;
 FP64 PCG solution within 1e-8 of a direct solve of the oracle's system; the
 mixed solver (stored FP32 operator: the matrix-free one is not built for
 constrained meshes) to a true residual 1e-8; error norms of the solution
@@ -48,7 +52,7 @@ def _case(name):
                        bc_val=[[0, 0, 0], [0.5, 0, 0], [0, 0, 0], [-0.3, 0, 0], [0, 0, 0], [0, 0, 0]])
     elif name == "darcy_p2":
         pb = P.darcy2d(n=8, p=2, seed=4)
-    else:  # hexahedra: divided elements away from the boundary
+    else:  # hexahedra: divided elements anywhere, boundary faces included
         n = 6
         if name == "poisson_hex_p2":
             pb = P.Problem(material=P.POISSON, dim=3, p=2, nx=n, ny=n, nz=n, force_kind=P.FORCE_CONST,
@@ -60,9 +64,7 @@ def _case(name):
                            force_const=(1.0, 0, 0), bc_type=[0] * 6)
         else:  # elast_hex_p2
             pb = P.elasticity3d(n=n, p=2, seed=3, field_kind="grf", load="body")
-        inner = np.zeros((n, n, n), bool)
-        inner[1:-1, 1:-1, 1:-1] = True
-        ref = inner.ravel() & (rng.random(n ** 3) < 0.35)
+        ref = rng.random(n ** 3) < 0.35  # also on the boundary: constrained boundary elements
         return pb, M.build_refined3d(pb, ref)
     ref = rng.random(pb.nx * pb.ny) < 0.4
     return pb, M.build_refined2d(pb, ref)
@@ -106,7 +108,7 @@ def test_constrained_assembly_and_solve(name):
     ag = an.K.A()
     rowmax = np.maximum.reduceat(np.abs(a), ia[:-1])
     rows = np.repeat(np.arange(len(ia) - 1), np.diff(ia))
-    assert np.all(np.abs(ag - a) <= 1e-12 * np.abs(a) + 1e-15 * rowmax[rows])
+    assert np.all(np.abs(ag - a) <= 1e-12 * np.abs(a) + 1e-14 * rowmax[rows])
     assert np.max(np.abs(an.Rhs() - rhs)) <= 1e-12 * np.max(np.abs(rhs))
     u = an.Solve()
     x = spla.spsolve(_csc(ia, ja, a, o.neq), rhs)
@@ -132,18 +134,17 @@ def test_constrained_assembly_and_solve(name):
     assert st2.info["converged"] and st2.info["true_residual"] <= 1e-8
 
 
-def test_constrained_boundary_elements_refused():
-    """Boundary elements on constrained connects are not supported: the call
+def test_constrained_tetrahedra_refused():
+    """Constrained connects on tetrahedral meshes are not supported: the call
     fails with PZG_EUNSUP instead of assembling something else."""
-    pb, md = _case("poisson_p2")
-    md = dict(md)
-    bc = np.array(md["bc_conn"]).copy()
-    dep_c = int(np.flatnonzero(np.diff(md["dep_ptr"]))[0])
-    bc[0, 0] = dep_c
-    md["bc_conn"] = bc
+    pb = P.darcy3d_tet(n=2, pmax=2, seed=1, hp=False)
+    md = dict(M.build(pb))
+    n = md["nconn"]
+    ptr = np.zeros(n + 1, dtype=np.int64)
+    ptr[n] = 1  # the last connect depends on connect 0
+    md.update(dep_ptr=ptr, dep_conn=np.array([0], dtype=np.int64), dep_coef=np.array([0.5]))
     d, keep = L.make_mesh_desc(md)
-    import ctypes
-    rc = ctx().lib.pzg_mesh_set(ctx().h, ctypes.byref(d))
+    rc = ctx().lib.pzg_mesh_set(ctx().h, C.byref(d))
     assert rc == L.PZG_EUNSUP
 
 
@@ -174,15 +175,14 @@ def test_constrained_large_mesh_patch_and_time():
 
 
 def test_constrained_large_hex_mesh_patch_and_time():
-    """20^3 coarse hexahedra, p = 2, about 30% of the interior ones divided:
-    the GPU solve reproduces u = x at every unconstrained corner; assembly
-    time printed."""
+    """20^3 coarse hexahedra, p = 2, about 30% of them divided (boundary ones
+    too, so the Dirichlet and Neumann faces hold constrained connects): the
+    GPU solve reproduces u = x at every unconstrained corner; assembly time
+    printed."""
     n = 20
     pb = P.Problem(material=P.POISSON, dim=3, p=2, nx=n, ny=n, nz=n, bc_type=[0, 1, -1, -1, -1, -1],
                    bc_val=[[0, 0, 0], [1.0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0]])
-    inner = np.zeros((n, n, n), bool)
-    inner[1:-1, 1:-1, 1:-1] = True
-    md = M.build_refined3d(pb, inner.ravel() & (np.random.default_rng(10).random(n ** 3) < 0.3))
+    md = M.build_refined3d(pb, np.random.default_rng(10).random(n ** 3) < 0.3)
     ncel = int(np.count_nonzero(np.diff(md["dep_ptr"])))
     print(f"[gpu] constrained large hex: {len(md['vol_nodes'])} elements, {md['neq']} equations, "
           f"{ncel} constrained connects", flush=True)

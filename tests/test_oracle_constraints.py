@@ -231,3 +231,29 @@ def test_constrained_pattern_brute_force():
                     rows[i].add(j)
     assert np.array_equal(np.diff(ia), [len(r) for r in rows])
     assert np.array_equal(ja, np.concatenate([sorted(r) for r in rows]))
+
+
+@pytest.mark.parametrize("p,force", [(1, 0.0), (2, 0.0), (2, 1.0)])
+def test_patch_with_constrained_boundary_elements(p, force):
+    """Divided hexahedra on the Dirichlet (x = 0) and Neumann (x = 1) faces: the
+    boundary elements hold constrained connects (a coarse boundary edge's middle
+    node and halves), whose penalty and load terms go through K_c = T^T K T;
+    u = x (p = 1, 2) and u = x - x^2/2 (p = 2, f = 1) are still exact."""
+    pb = P.Problem(material=P.POISSON, dim=3, p=p, nx=3, ny=3, nz=3, bc_type=[0, 1, -1, -1, -1, -1],
+                   force_kind=P.FORCE_CONST if force else P.FORCE_NONE, force_const=(force, 0, 0),
+                   bc_val=[[0, 0, 0], [0.0 if force else 1.0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0],
+                           [0, 0, 0]])
+    ref = np.zeros(27, bool)
+    ref[[0, 4, 9, 13, 26]] = True
+    md = M.build_refined3d(pb, ref)
+    dep = np.diff(md["dep_ptr"]) > 0
+    assert dep[np.asarray(md["bc_conn"]).ravel()].any()
+    o, u = _solve(pb, md)
+    xyz = md["xyz"].reshape(-1, 3)
+    f = (lambda x: x - x * x / 2) if force else (lambda x: x)
+    vn, vc = md["vol_nodes"], md["vol_conn"]
+    for e in range(len(vn)):
+        for k in range(8):
+            c = vc[e, k]
+            if not dep[c]:
+                assert abs(u[md["blk_pos"][c]] - f(xyz[vn[e, k], 0])) < 1e-12
