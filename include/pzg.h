@@ -139,6 +139,9 @@ typedef struct {
 #define PZG_PRECOND_JACOBI 0       /* point Jacobi                              */
 #define PZG_PRECOND_BLOCK_JACOBI 1 /* node-block (nstate x nstate) Jacobi       */
 #define PZG_PRECOND_LEARNED 2      /* learned message-passing preconditioner    */
+#define PZG_PRECOND_MULTIGRID 3    /* V-cycle: p = 2 -> trilinear -> halved
+                                      lattices, Chebyshev / block-Jacobi
+                                      smoothing (hexahedral lattice meshes)     */
 
 typedef struct {
   int32_t solver;        /* PZG_SOLVER_*                                       */
@@ -152,6 +155,9 @@ typedef struct {
   int32_t refine;        /* mixed: 0 = restart FCG from each true residual,
                                    1 = reliable update (keep the direction)    */
   int32_t inner_op;      /* mixed: PZG_INNER_* operator of the FP32 iterations */
+  int32_t mg_degree;     /* multigrid: Chebyshev degree per smoothing (0 = 2)  */
+  double mg_range;       /* multigrid: lmax / lmin of the smoothed interval
+                            (0 = 15)                                           */
 } pzg_solve_params;
 
 /* Operator of the mixed solver's FP32 iterations.  Both approximate the
@@ -265,6 +271,19 @@ int pzg_hp_tuning(pzg_ctx *ctx, int32_t *n, int32_t *order, int32_t *variant, fl
  * the diagonal of the assembled K.  Exposes the operator of the mixed
  * solver's inner iterations, which has no reference counterpart. */
 int pzg_scaled_apply(pzg_ctx *ctx, int32_t op, const double *x, double *y);
+
+/* z = M r with M the V-cycle of PZG_PRECOND_MULTIGRID for the scaled system
+ * S (level 0 applied by op: PZG_INNER_BSR or PZG_INNER_MATFREE), on host
+ * vectors of length neq; degree and range as mg_degree and mg_range of
+ * pzg_solve_params, 0 for the defaults.  *nlev receives the number of lattice levels (the first is the
+ * trilinear subspace of the mesh's vertices, the last the dense one),
+ * dims[3l .. 3l+2] their node counts per direction, lmax[0] the estimated
+ * largest eigenvalue of the block-Jacobi scaled S and lmax[1+l] that of
+ * lattice level l (arrays of 48 and 16 entries; any may be NULL).  The
+ * levels are built on the first call after an assembly.  Exposes the mixed
+ * solver's multigrid preconditioner, which has no reference counterpart. */
+int pzg_mg_apply(pzg_ctx *ctx, int32_t op, int32_t degree, double range, const double *r, double *z,
+                 int32_t *nlev, int32_t *dims, double *lmax);
 
 /* ---- bandwidth renumbering ------------------------------------------------ */
 

@@ -13,7 +13,7 @@ LIBPATH = os.path.join(_HERE, "libpzg.so")
 
 PZG_OK, PZG_EINVAL, PZG_EHIP, PZG_ENOCONV, PZG_ESTATE, PZG_ENOMEM, PZG_EUNSUP = range(7)
 SOLVER_PCG64, SOLVER_MIXED = 0, 1
-PRECOND_JACOBI, PRECOND_BLOCK_JACOBI, PRECOND_LEARNED = 0, 1, 2
+PRECOND_JACOBI, PRECOND_BLOCK_JACOBI, PRECOND_LEARNED, PRECOND_MULTIGRID = 0, 1, 2, 3
 INNER_AUTO, INNER_BSR, INNER_MATFREE = 0, 1, 2
 EXACT_POINTS, EXACT_CONST, EXACT_X, EXACT_SINSIN, EXACT_COSCOS = range(5)
 
@@ -23,7 +23,7 @@ EXPORTS = [
     "pzg_add_kel_batch", "pzg_multadd", "pzg_solve", "pzg_true_residual", "pzg_time_kernel",
     "pzg_get_sizes", "pzg_sync", "pzg_mesh_set_coef", "pzg_learned_setup", "pzg_learned_collect",
     "pzg_learned_train", "pzg_learned_weights", "pzg_learned_graph", "pzg_learned_samples",
-    "pzg_learned_loss_grad", "pzg_scaled_apply", "pzg_full_csr", "pzg_hp_tuning",
+    "pzg_learned_loss_grad", "pzg_scaled_apply", "pzg_mg_apply", "pzg_full_csr", "pzg_hp_tuning",
     "pzg_load_solution", "pzg_error_points", "pzg_evaluate_error", "pzg_sloan_renumber",
     "pzg_get_solution",
 ]
@@ -57,7 +57,7 @@ class SolveParams(C.Structure):
         ("solver", C.c_int32), ("precond", C.c_int32), ("max_iter", C.c_int64),
         ("tol", C.c_double), ("inner_tol", C.c_double), ("max_outer", C.c_int32),
         ("check_every", C.c_int32), ("use_graph", C.c_int32), ("refine", C.c_int32),
-        ("inner_op", C.c_int32),
+        ("inner_op", C.c_int32), ("mg_degree", C.c_int32), ("mg_range", C.c_double),
     ]
 
 
@@ -129,6 +129,7 @@ def load():
     L.pzg_learned_samples.argtypes = [vp, fp, fp]
     L.pzg_learned_loss_grad.argtypes = [vp, C.c_int64, fp, fp, dp, fp]
     L.pzg_scaled_apply.argtypes = [vp, C.c_int32, dp, dp]
+    L.pzg_mg_apply.argtypes = [vp, C.c_int32, C.c_int32, C.c_double, dp, dp, i32p, i32p, dp]
     L.pzg_full_csr.argtypes = [vp, i64p, i64p, dp]
     L.pzg_hp_tuning.argtypes = [vp, i32p, i32p, i32p, fp]
     L.pzg_load_solution.argtypes = [vp, dp]

@@ -213,15 +213,19 @@ class TPZStepSolver:
         self.pre = pre
 
     def SetMixedFCG(self, numiterations, tol, precond=L.PRECOND_BLOCK_JACOBI, inner_tol=1e-4,
-                    max_outer=200, refine=0, inner_op=L.INNER_AUTO):
+                    max_outer=200, refine=0, inner_op=L.INNER_AUTO, mg_degree=0, mg_range=0.0):
         """FP32 flexible CG inside FP64 iterative refinement; stops on the true
         residual.  inner_tol is the reduction of the FP32 recursive residual
         between two true-residual evaluations; refine 0 restarts FCG from each
         true residual, 1 keeps the search direction (reliable update);
         inner_op selects the FP32 operator (INNER_BSR: stored block CSR,
         INNER_MATFREE: element-by-element, INNER_AUTO: matrix-free for
-        mesh-built systems)."""
+        mesh-built systems).  precond PRECOND_MULTIGRID preconditions each FCG
+        step with a V-cycle (hexahedral lattice meshes; mg_degree and mg_range:
+        Chebyshev degree and interval ratio of the smoothing, 0 = 2 and 15)."""
         self.kind = self.EMixedFCG
+        self.mg_degree = int(mg_degree)
+        self.mg_range = float(mg_range)
         self.refine = refine
         self.inner_op = inner_op
         self.maxit = int(numiterations)
@@ -242,6 +246,10 @@ class TPZStepSolver:
         p.use_graph = 1 if self.use_graph else 0
         p.refine = self.refine
         p.inner_op = self.inner_op
+        p.mg_degree = getattr(self, "mg_degree", 0)
+        p.mg_range = getattr(self, "mg_range", 0.0)
+        if self.precond == L.PRECOND_MULTIGRID and self.check_every == 50:
+            p.check_every = 0  # the solver's short batches
         return p
 
     def Solve(self, F=None, want_solution=True):

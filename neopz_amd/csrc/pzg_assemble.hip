@@ -1391,6 +1391,7 @@ int launch_build_bsr(Ctx &c) {
   dev_free(nl);
   c.bsr_valid = true;
   c.mf_vals_valid = false;  // D^-1/2 recomputed
+  c.mg_ready = false;       // multigrid levels are built from K and D
   return PZG_OK;
 }
 

@@ -75,6 +75,7 @@ static void free_all(Ctx &c) {
                     &c.cv.d_ef, &c.cb.d_cel, &c.cb.d_cslot, &c.cb.d_T, &c.cb.d_blk, &c.cb.d_n,
                     &c.cb.d_ek, &c.cb.d_ef};
   for (DevBuf *b : bufs) dev_free(*b);
+  mg_free(c);
 }
 
 // Greedy first-fit colouring over connects (the idea of
@@ -1346,6 +1347,52 @@ int pzg_scaled_apply(pzg_ctx *ctx, int32_t op, const double *x, double *y) {
   PZG_HIP(hipMemcpyAsync(h.data(), c.d_qs.p, n * sizeof(float), hipMemcpyDeviceToHost, c.stream));
   PZG_HIP(hipStreamSynchronize(c.stream));
   for (int64_t i = 0; i < n; i++) y[i] = h[i];
+  return PZG_OK;
+}
+
+int pzg_mg_apply(pzg_ctx *ctx, int32_t op, int32_t degree, double range, const double *r, double *z,
+                 int32_t *nlev, int32_t *dims, double *lmax) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (!c.assembled || !r || !z) {
+    set_error("pzg_mg_apply: needs an assembled system and host vectors");
+    return PZG_ESTATE;
+  }
+  if (op != PZG_INNER_BSR && op != PZG_INNER_MATFREE) {
+    set_error("pzg_mg_apply: op must be PZG_INNER_BSR or PZG_INNER_MATFREE");
+    return PZG_EINVAL;
+  }
+  if (op == PZG_INNER_MATFREE && !mf_available(c)) {
+    set_error("pzg_mg_apply: matrix-free operator needs a mesh-built system");
+    return PZG_ESTATE;
+  }
+  const int64_t n = c.neq;
+  if (!c.full_valid) PZG_TRY(launch_expand_full(c));
+  if (!c.bsr_valid) PZG_TRY(launch_build_bsr(c));
+  if (op == PZG_INNER_MATFREE) PZG_TRY(mf_prepare(c));
+  PZG_TRY(ensure_reduce_pub(c));
+  c.mg_degree = degree > 0 ? degree : 2;
+  c.mg_range = range > 0 ? range : 15.;
+  const bool use_mf = op == PZG_INNER_MATFREE;
+  if (!c.mg_ready || c.mg_use_mf != use_mf) PZG_TRY(mg_setup(c, use_mf));
+  std::vector<float> h(n);
+  for (int64_t i = 0; i < n; i++) h[i] = (float)r[i];
+  PZG_TRY(dev_alloc(c.d_rs, n * sizeof(float)));
+  PZG_TRY(dev_alloc(c.d_zs, n * sizeof(float)));
+  PZG_HIP(hipMemcpyAsync(c.d_rs.p, h.data(), n * sizeof(float), hipMemcpyHostToDevice, c.stream));
+  PZG_TRY(mg_apply(c, c.d_rs.as<float>(), c.d_zs.as<float>(), nullptr, nullptr, nullptr, nullptr));
+  PZG_HIP(hipMemcpyAsync(h.data(), c.d_zs.p, n * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+  PZG_HIP(hipStreamSynchronize(c.stream));
+  for (int64_t i = 0; i < n; i++) z[i] = h[i];
+  const int nl = (int)std::min<size_t>(c.mg.size(), 15);
+  if (nlev) *nlev = (int32_t)c.mg.size();
+  if (dims)
+    for (int l = 0; l < nl; l++)
+      for (int d = 0; d < 3; d++) dims[3 * l + d] = c.mg[l].n[d];
+  if (lmax) {
+    lmax[0] = c.mg_lmax0;
+    for (int l = 0; l < nl; l++) lmax[1 + l] = c.mg[l].lmax;
+  }
   return PZG_OK;
 }
 

@@ -154,6 +154,16 @@ struct SolveScalars {
   int32_t done, pad;
 };
 
+// A lattice level of the multigrid preconditioner (pzg_mg.hip): n[0] x n[1] x
+// n[2] nodes, a 27-point block stencil (entry (s, e) of node v at
+// A[(s*bs*bs + e)*nn + v]) and the inverse diagonal blocks ([e*nn + v]).
+struct MgLevel {
+  int n[3] = {0, 0, 0};
+  int64_t nn = 0;
+  double lmax = 0.;  // largest eigenvalue of M^-1 A (CG-Lanczos estimate)
+  DevBuf A, minv, x, b, r, d, t;
+};
+
 // Elements (volume or boundary) that hold a constrained connect with shapes.
 struct CelSet {
   std::vector<int32_t> cel;          // element ids, grouped by rule bucket
@@ -274,6 +284,17 @@ struct Ctx {
   DevBuf d_nn_den, d_nn_part, d_nn_loss;
   DevBuf d_nn_ystar, d_nn_e;                      // pair generation
 
+  // multigrid preconditioner (pzg_mg.hip): lattice levels 1.. (level 0 is the
+  // FCG system), vertex block of each lattice node of level 1, D^1/2 at the
+  // vertex equations, the dense inverse of the coarsest level
+  bool mg_ready = false, mg_use_mf = false;
+  int mg_degree = 2;       // Chebyshev degree of the pre- and post-smoothing
+  double mg_range = 15.;   // ratio of the smoothed interval's ends
+  double mg_lmax0 = 0.;    // largest eigenvalue of the block-Jacobi scaled S
+  int64_t mg_ncoarse = 0;  // unknowns of the coarsest level
+  std::vector<MgLevel> mg;
+  DevBuf d_mg_vmap, d_mg_dsq, d_mg_cinv, d_mg_r0, d_mg_d0, d_mg_t0, d_mg_w0;
+
   // post-processing (pzg_post.hip): error-rule tables per rule bucket for the
   // exact-solution order post_porder (-1: not built), the elements of every
   // bucket, and per element the offset of its first point
@@ -328,6 +349,15 @@ int nn_adam_step(Ctx &c, double lr);
 int nn_store_pair(Ctx &c, const float *r, const float *e_minus_bjr, int64_t slot);
 int nn_collect(Ctx &c, int nrhs, const int32_t *rec, int nrec, uint64_t seed);
 int mf_setup(Ctx &c);
+// multigrid preconditioner: levels, operators and eigenvalue estimates of the
+// current system (use_mf: level 0 applied matrix-free); z = V-cycle(r), then
+// p = z and the z.q partials of the FCG update (when p / q are given)
+int mg_setup(Ctx &c, bool use_mf);
+int mg_apply(Ctx &c, const float *r, float *z, float *p, const float *q, SolveScalars *sc,
+             double *partials);
+void mg_free(Ctx &c);
+// reduction partials and device scalars of the solvers
+int ensure_reduce_pub(Ctx &c);
 int mf_prepare(Ctx &c);
 // volume element kernel alone (for timing)
 int mf_launch_vol(Ctx &c, const SolveScalars *sc);

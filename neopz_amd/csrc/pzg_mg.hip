@@ -1,0 +1,942 @@
+// pzg_mg.hip -- multigrid preconditioner of the mixed FCG path on hexahedral
+// meshes whose vertices form a lattice (p <= 2).
+//
+// Levels.  Level 0 is the FCG system S = D^-1/2 K D^-1/2 (matrix-free or
+// stored, as the FCG iteration uses it).  Level 1 is the trilinear subspace:
+// the hierarchical shape functions of H1 (TPZShapeCube: vertex functions are
+// the trilinear ones, edge/face/interior functions vanish at the vertices)
+// make it the span of the vertex functions, so its prolongation is the
+// injection of vertex values and its Galerkin operator P^T K P is the
+// vertex-vertex part K_vv of the assembled K, read from the FP64 block CSR.
+// In level 0's scaled unknowns the transfer carries D^1/2 at the vertex
+// equations.  Level 1 is stored in lattice order as a 27-point block
+// stencil.  Levels 2.. halve the lattice with trilinear prolongation and the
+// Galerkin product A_c = P^T A P formed on the device, while every element
+// count is even and the level is too large for the dense solve; the coarsest
+// level (at most MG_MAXDENSE unknowns) is solved with a dense inverse.
+//
+// Smoother: Chebyshev iteration (Saad, Iterative Methods for Sparse Linear
+// Systems, 2nd ed., Alg. 12.1) of degree m on M^-1 A, M = the diagonal
+// nstate x nstate blocks, on the interval [lhi/range, lhi], lhi = 1.1 x the
+// largest eigenvalue estimated by CG-Lanczos.  The V-cycle takes m
+// pre- and m post-smoothing steps with the same polynomial and an exact
+// coarsest solve, so the preconditioner is a fixed symmetric positive
+// definite operator.  Every kernel of the cycle leaves at once when the FCG
+// iteration has stopped (graph batches run past the stop).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "pzg_device.h"
+#include "pzg_internal.h"
+
+namespace pzg {
+
+namespace {
+
+constexpr int TPB = RED_TPB;
+constexpr int64_t MG_DENSE = 512;      // coarsening stops at this many unknowns
+constexpr int64_t MG_MAXDENSE = 1536;  // largest dense coarsest solve
+constexpr int MG_LANCZOS_IT = 20;      // CG-Lanczos steps per eigenvalue estimate
+
+int grid1(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + TPB - 1) / TPB, 1 << 20)); }
+// one thread per item (kernels without a grid-stride loop)
+int gridx(int64_t n) { return (int)std::max<int64_t>(1, (n + TPB - 1) / TPB); }
+int gridr(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + TPB - 1) / TPB, MAXBLK)); }
+
+__device__ inline bool stopped(const SolveScalars *sc) { return sc && sc->done; }
+
+// y = M^-1 x at block A: entry (A, k) of the inverse blocks at m[A*sa + k*sk]
+template <int BS>
+__device__ inline void minv_mul(const float *__restrict__ m, int64_t sa, int64_t sk, int64_t A,
+                                const float (&x)[BS], float (&y)[BS]) {
+#pragma unroll
+  for (int i = 0; i < BS; i++) {
+    float s = 0.f;
+#pragma unroll
+    for (int j = 0; j < BS; j++) s += m[A * sa + (i * BS + j) * sk] * x[j];
+    y[i] = s;
+  }
+}
+
+// ---- Chebyshev steps (vectors [nb][BS]) --------------------------------------
+// r = b; d = c0 M^-1 b; x = d
+template <int BS>
+__global__ void k_cheb_init(int64_t nb, const float *__restrict__ m, int64_t sa, int64_t sk,
+                            const float *__restrict__ b, float *__restrict__ x, float *__restrict__ r,
+                            float *__restrict__ d, float c0, const SolveScalars *sc) {
+  if (stopped(sc)) return;
+  for (int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; A < nb;
+       A += (int64_t)gridDim.x * blockDim.x) {
+    float bv[BS], y[BS];
+#pragma unroll
+    for (int k = 0; k < BS; k++) bv[k] = b[A * BS + k];
+    minv_mul<BS>(m, sa, sk, A, bv, y);
+#pragma unroll
+    for (int k = 0; k < BS; k++) {
+      r[A * BS + k] = bv[k];
+      d[A * BS + k] = c0 * y[k];
+      x[A * BS + k] = c0 * y[k];
+    }
+  }
+}
+
+// r = b - t (t = A x); d = c0 M^-1 r; x += d
+template <int BS>
+__global__ void k_cheb_restart(int64_t nb, const float *__restrict__ m, int64_t sa, int64_t sk,
+                               const float *__restrict__ b, const float *__restrict__ t,
+                               float *__restrict__ x, float *__restrict__ r, float *__restrict__ d,
+                               float c0, const SolveScalars *sc) {
+  if (stopped(sc)) return;
+  for (int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; A < nb;
+       A += (int64_t)gridDim.x * blockDim.x) {
+    float rv[BS], y[BS];
+#pragma unroll
+    for (int k = 0; k < BS; k++) {
+      rv[k] = b[A * BS + k] - t[A * BS + k];
+      r[A * BS + k] = rv[k];
+    }
+    minv_mul<BS>(m, sa, sk, A, rv, y);
+#pragma unroll
+    for (int k = 0; k < BS; k++) {
+      d[A * BS + k] = c0 * y[k];
+      x[A * BS + k] += c0 * y[k];
+    }
+  }
+}
+
+// r -= t; d = c1 d + c2 M^-1 r; x += d
+template <int BS>
+__global__ void k_cheb_step(int64_t nb, const float *__restrict__ m, int64_t sa, int64_t sk,
+                            const float *__restrict__ t, float *__restrict__ r, float *__restrict__ d,
+                            float *__restrict__ x, float c1, float c2, const SolveScalars *sc) {
+  if (stopped(sc)) return;
+  for (int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; A < nb;
+       A += (int64_t)gridDim.x * blockDim.x) {
+    float rv[BS], y[BS];
+#pragma unroll
+    for (int k = 0; k < BS; k++) {
+      rv[k] = r[A * BS + k] - t[A * BS + k];
+      r[A * BS + k] = rv[k];
+    }
+    minv_mul<BS>(m, sa, sk, A, rv, y);
+#pragma unroll
+    for (int k = 0; k < BS; k++) {
+      const float dv = c1 * d[A * BS + k] + c2 * y[k];
+      d[A * BS + k] = dv;
+      x[A * BS + k] += dv;
+    }
+  }
+}
+
+__global__ void k_sub(int64_t n, float *__restrict__ r, const float *__restrict__ t,
+                      const SolveScalars *sc) {
+  if (stopped(sc)) return;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    r[i] -= t[i];
+}
+
+// ---- lattice operators ---------------------------------------------------------
+struct Dims {
+  int n[3];
+};
+__device__ inline void node_ijk(const Dims &D, int64_t v, int &i, int &j, int &k) {
+  i = (int)(v % D.n[0]);
+  const int64_t q = v / D.n[0];
+  j = (int)(q % D.n[1]);
+  k = (int)(q / D.n[1]);
+}
+__device__ inline int64_t node_id(const Dims &D, int i, int j, int k) {
+  return i + (int64_t)D.n[0] * (j + (int64_t)D.n[1] * k);
+}
+
+// y = A x over the 27-point block stencil; stencil entry s = (dz+1)*9 +
+// (dy+1)*3 + (dx+1), value (s, e) of node v at A[(s*BS*BS + e)*nn + v]
+template <int BS>
+__global__ void k_stencil(Dims D, int64_t nn, const float *__restrict__ A, const float *__restrict__ x,
+                          float *__restrict__ y, const SolveScalars *sc) {
+  if (stopped(sc)) return;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nn;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    int i, j, k;
+    node_ijk(D, v, i, j, k);
+    float acc[BS];
+#pragma unroll
+    for (int e = 0; e < BS; e++) acc[e] = 0.f;
+    for (int s = 0; s < 27; s++) {
+      const int ii = i + s % 3 - 1, jj = j + (s / 3) % 3 - 1, kk = k + s / 9 - 1;
+      if (ii < 0 || jj < 0 || kk < 0 || ii >= D.n[0] || jj >= D.n[1] || kk >= D.n[2]) continue;
+      const int64_t w = node_id(D, ii, jj, kk);
+      float xv[BS];
+#pragma unroll
+      for (int e = 0; e < BS; e++) xv[e] = x[w * BS + e];
+#pragma unroll
+      for (int a = 0; a < BS; a++)
+#pragma unroll
+        for (int b = 0; b < BS; b++) acc[a] += A[((int64_t)s * BS * BS + a * BS + b) * nn + v] * xv[b];
+    }
+#pragma unroll
+    for (int e = 0; e < BS; e++) y[v * BS + e] = acc[e];
+  }
+}
+
+// trilinear restriction: b_c(N) = sum over the fine nodes 2N + d, d in
+// {-1,0,1}^3, of prod_i (1 - |d_i|/2) r_f
+template <int BS>
+__global__ void k_restrict(Dims C, Dims F, const float *__restrict__ rf, float *__restrict__ bc,
+                           const SolveScalars *sc) {
+  if (stopped(sc)) return;
+  const int64_t nc = (int64_t)C.n[0] * C.n[1] * C.n[2];
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nc;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    int I, J, K;
+    node_ijk(C, v, I, J, K);
+    float acc[BS];
+#pragma unroll
+    for (int e = 0; e < BS; e++) acc[e] = 0.f;
+    for (int dz = -1; dz <= 1; dz++) {
+      const int k = 2 * K + dz;
+      if (k < 0 || k >= F.n[2]) continue;
+      for (int dy = -1; dy <= 1; dy++) {
+        const int j = 2 * J + dy;
+        if (j < 0 || j >= F.n[1]) continue;
+        for (int dx = -1; dx <= 1; dx++) {
+          const int i = 2 * I + dx;
+          if (i < 0 || i >= F.n[0]) continue;
+          const float w = (dx ? 0.5f : 1.f) * (dy ? 0.5f : 1.f) * (dz ? 0.5f : 1.f);
+          const int64_t f = node_id(F, i, j, k);
+#pragma unroll
+          for (int e = 0; e < BS; e++) acc[e] += w * rf[f * BS + e];
+        }
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < BS; e++) bc[v * BS + e] = acc[e];
+  }
+}
+
+// trilinear prolongation, added: x_f += P x_c
+template <int BS>
+__global__ void k_prolong(Dims C, Dims F, const float *__restrict__ xc, float *__restrict__ xf,
+                          const SolveScalars *sc) {
+  if (stopped(sc)) return;
+  const int64_t nf = (int64_t)F.n[0] * F.n[1] * F.n[2];
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nf;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    int i, j, k;
+    node_ijk(F, v, i, j, k);
+    const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
+    const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (k & 1) ? 2 : 1;
+    const float wi = (i & 1) ? 0.5f : 1.f, wj = (j & 1) ? 0.5f : 1.f, wk = (k & 1) ? 0.5f : 1.f;
+    float acc[BS];
+#pragma unroll
+    for (int e = 0; e < BS; e++) acc[e] = 0.f;
+    for (int c = 0; c < nk; c++)
+      for (int b = 0; b < nj; b++)
+        for (int a = 0; a < ni; a++) {
+          const int64_t w = node_id(C, i0 + a, j0 + b, k0 + c);
+#pragma unroll
+          for (int e = 0; e < BS; e++) acc[e] += xc[w * BS + e];
+        }
+    const float s = wi * wj * wk;
+#pragma unroll
+    for (int e = 0; e < BS; e++) xf[v * BS + e] += s * acc[e];
+  }
+}
+
+// level 0 <-> level 1: b1(v) = D^1/2 r0 at the vertex block vmap[v];
+// x0(vmap[v]) += D^1/2 x1(v)
+template <int BS>
+__global__ void k_restrict0(int64_t nn, const int32_t *__restrict__ vmap, const float *__restrict__ dsq,
+                            const float *__restrict__ r0, float *__restrict__ b1, const SolveScalars *sc) {
+  if (stopped(sc)) return;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nn;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t A = vmap[v];
+#pragma unroll
+    for (int e = 0; e < BS; e++) b1[v * BS + e] = dsq[v * BS + e] * r0[A * BS + e];
+  }
+}
+template <int BS>
+__global__ void k_prolong0(int64_t nn, const int32_t *__restrict__ vmap, const float *__restrict__ dsq,
+                           const float *__restrict__ x1, float *__restrict__ x0, const SolveScalars *sc) {
+  if (stopped(sc)) return;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nn;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t A = vmap[v];
+#pragma unroll
+    for (int e = 0; e < BS; e++) x0[A * BS + e] += dsq[v * BS + e] * x1[v * BS + e];
+  }
+}
+
+// x = Ainv b (dense, symmetric, nd unknowns)
+__global__ void k_dense(int nd, const float *__restrict__ ainv, const float *__restrict__ b,
+                        float *__restrict__ x, const SolveScalars *sc) {
+  if (stopped(sc)) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nd) return;
+  float s = 0.f;
+  for (int j = 0; j < nd; j++) s += ainv[(int64_t)j * nd + i] * b[j];
+  x[i] = s;
+}
+
+// z stays the preconditioned residual; p = z (restart); z.q replaces slot 1
+// of the FCG update kernel's partials (same grid)
+template <int BS>
+__global__ __launch_bounds__(TPB) void k_mg_finish(int64_t nb, const float *__restrict__ z,
+                                                   float *__restrict__ p, const float *__restrict__ q,
+                                                   const SolveScalars *sc, double *partials) {
+  if (stopped(sc)) return;
+  double zq = 0.;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nb * BS;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float v = z[i];
+    if (p) p[i] = v;
+    if (q) zq += (double)v * q[i];
+  }
+  if (!q) return;
+  __shared__ double sm[TPB / 64];
+  const double t = block_sum(zq, sm);
+  if (threadIdx.x == 0) partials[1 * MAXBLK + blockIdx.x] = t;
+}
+
+// ---- set-up kernels --------------------------------------------------------------
+// level 1 stencil from the FP64 full block CSR (values at bsr_pos): block
+// (vmap[v], vmap[w]) for the lattice neighbour w of v in direction s (zero
+// off the lattice or when the row does not hold the column)
+template <int BS>
+__global__ void k_mg_l1(Dims D, int64_t nn, const int32_t *__restrict__ vmap,
+                        const int64_t *__restrict__ fbia, const int32_t *__restrict__ fbja,
+                        const double *__restrict__ fa, float *__restrict__ A) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= nn * 27) return;
+  const int64_t v = x % nn;
+  const int s = (int)(x / nn);
+  int i, j, k;
+  node_ijk(D, v, i, j, k);
+  const int ii = i + s % 3 - 1, jj = j + (s / 3) % 3 - 1, kk = k + s / 9 - 1;
+  int64_t pos = -1, b0 = 0, nr = 0;
+  if (ii >= 0 && jj >= 0 && kk >= 0 && ii < D.n[0] && jj < D.n[1] && kk < D.n[2]) {
+    const int32_t col = vmap[node_id(D, ii, jj, kk)];
+    const int64_t row = vmap[v];
+    b0 = fbia[row];
+    nr = fbia[row + 1] - b0;
+    int64_t lo = b0, hi = b0 + nr - 1;
+    while (lo <= hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      const int32_t cm = fbja[mid];
+      if (cm == col) {
+        pos = mid;
+        break;
+      }
+      if (cm < col) lo = mid + 1;
+      else hi = mid - 1;
+    }
+  }
+  for (int e = 0; e < BS * BS; e++)
+    A[((int64_t)s * BS * BS + e) * nn + v] = pos >= 0 ? (float)fa[bsr_pos<BS * BS>(b0, nr, pos - b0, e)] : 0.f;
+}
+
+// Galerkin product A_c = P^T A_f P for trilinear P, one thread per coarse
+// (node, stencil entry).  Along each direction the fine node f = 2N + a
+// (a in {-1,0,1}, weight 1 - |a|/2) couples to the fine node g = 2M + b of
+// the neighbour M = N + o when |g - f| <= 1; the pairs (a, b) are enumerated
+// per direction.
+template <int BS>
+__global__ void k_mg_rap(Dims C, Dims F, const float *__restrict__ Af, float *__restrict__ Ac) {
+  const int64_t nc = (int64_t)C.n[0] * C.n[1] * C.n[2];
+  const int64_t nf = (int64_t)F.n[0] * F.n[1] * F.n[2];
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= nc * 27) return;
+  const int64_t v = x % nc;
+  const int s = (int)(x / nc);
+  int N[3];
+  node_ijk(C, v, N[0], N[1], N[2]);
+  const int o[3] = {s % 3 - 1, (s / 3) % 3 - 1, s / 9 - 1};
+  double acc[BS * BS];
+  for (int e = 0; e < BS * BS; e++) acc[e] = 0.;
+  bool inside = true;
+  for (int d = 0; d < 3; d++) inside &= N[d] + o[d] >= 0 && N[d] + o[d] < C.n[d];
+  if (inside) {
+    // per direction: fine pairs (f, g) with weights and the stencil offset g - f
+    int pf[3][9], pg[3][9], np[3];
+    float pw[3][9];
+    for (int d = 0; d < 3; d++) {
+      np[d] = 0;
+      const int M = N[d] + o[d];
+      for (int a = -1; a <= 1; a++) {
+        const int f = 2 * N[d] + a;
+        if (f < 0 || f >= F.n[d]) continue;
+        for (int b = -1; b <= 1; b++) {
+          const int g = 2 * M + b;
+          if (g < 0 || g >= F.n[d] || g - f < -1 || g - f > 1) continue;
+          pf[d][np[d]] = f;
+          pg[d][np[d]] = g - f;
+          pw[d][np[d]] = (a ? 0.5f : 1.f) * (b ? 0.5f : 1.f);
+          np[d]++;
+        }
+      }
+    }
+    for (int c = 0; c < np[2]; c++)
+      for (int b = 0; b < np[1]; b++)
+        for (int a = 0; a < np[0]; a++) {
+          const int64_t f = pf[0][a] + (int64_t)F.n[0] * (pf[1][b] + (int64_t)F.n[1] * pf[2][c]);
+          const int sf = (pg[2][c] + 1) * 9 + (pg[1][b] + 1) * 3 + (pg[0][a] + 1);
+          const double w = (double)pw[0][a] * pw[1][b] * pw[2][c];
+          for (int e = 0; e < BS * BS; e++) acc[e] += w * Af[((int64_t)sf * BS * BS + e) * nf + f];
+        }
+  }
+  for (int e = 0; e < BS * BS; e++) Ac[((int64_t)s * BS * BS + e) * nc + v] = (float)acc[e];
+}
+
+// inverse of the diagonal block (stencil entry 13), stored [e*nn + v]
+template <int BS>
+__global__ void k_mg_minv(int64_t nn, const float *__restrict__ A, float *__restrict__ minv) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nn) return;
+  double a[BS * BS];
+  for (int e = 0; e < BS * BS; e++) a[e] = A[((int64_t)13 * BS * BS + e) * nn + v];
+  double r[BS * BS];
+  if (BS == 1) {
+    r[0] = a[0] != 0. ? 1. / a[0] : 0.;
+  } else {
+    const double c00 = a[4] * a[8] - a[5] * a[7], c01 = a[5] * a[6] - a[3] * a[8],
+                 c02 = a[3] * a[7] - a[4] * a[6];
+    const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    const double id = det != 0. ? 1. / det : 0.;
+    r[0] = c00 * id;
+    r[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+    r[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+    r[3] = c01 * id;
+    r[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+    r[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+    r[6] = c02 * id;
+    r[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+    r[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+  }
+  for (int e = 0; e < BS * BS; e++) minv[(int64_t)e * nn + v] = (float)r[e];
+}
+
+// dsq(v) = 1 / dinvsqrt at the vertex block vmap[v]
+template <int BS>
+__global__ void k_mg_dsq(int64_t nn, const int32_t *__restrict__ vmap, const double *__restrict__ dis,
+                         float *__restrict__ dsq) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nn) return;
+  for (int e = 0; e < BS; e++) dsq[v * BS + e] = (float)(1.0 / dis[(int64_t)vmap[v] * BS + e]);
+}
+
+__global__ void k_hash_fill(int64_t n, float *__restrict__ v, uint64_t seed) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t h = (uint64_t)i * 0x9E3779B97F4A7C15ULL + seed;
+    h ^= h >> 31;
+    h *= 0xBF58476D1CE4E5B9ULL;
+    h ^= h >> 29;
+    v[i] = (float)((double)(h >> 11) / 9007199254740992.0) - 0.5f;
+  }
+}
+
+// Chebyshev coefficients of one level
+struct Cheb {
+  float c0;
+  std::vector<float> c1, c2;  // steps 1 .. m-1
+};
+Cheb cheb_coeffs(double lmax, double range, int m) {
+  const double hi = 1.1 * lmax, lo = hi / range;
+  const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+  Cheb ch;
+  ch.c0 = (float)(1.0 / theta);
+  double rho = 1.0 / sigma;
+  for (int j = 1; j < m; j++) {
+    const double rn = 1.0 / (2.0 * sigma - rho);
+    ch.c1.push_back((float)(rn * rho));
+    ch.c2.push_back((float)(2.0 * rn / delta));
+    rho = rn;
+  }
+  return ch;
+}
+
+// A level for the smoother: operator, inverse diagonal blocks, vectors
+struct LevelView {
+  int64_t nb;              // blocks
+  const float *m;          // inverse diagonal blocks
+  int64_t sa, sk;          // their layout
+  float *r, *d, *t;        // work vectors
+  Cheb ch;
+};
+
+template <int BS, class Op>
+int smooth_pre(Ctx &c, const LevelView &L, const float *b, float *x, Op op, const SolveScalars *sc) {
+  const int g = grid1(L.nb);
+  hipLaunchKernelGGL(k_cheb_init<BS>, dim3(g), dim3(TPB), 0, c.stream, L.nb, L.m, L.sa, L.sk, b, x, L.r,
+                     L.d, L.ch.c0, sc);
+  PZG_HIP(hipGetLastError());
+  for (size_t j = 0; j < L.ch.c1.size(); j++) {
+    PZG_TRY(op(L.d, L.t));
+    hipLaunchKernelGGL(k_cheb_step<BS>, dim3(g), dim3(TPB), 0, c.stream, L.nb, L.m, L.sa, L.sk, L.t, L.r,
+                       L.d, x, L.ch.c1[j], L.ch.c2[j], sc);
+    PZG_HIP(hipGetLastError());
+  }
+  PZG_TRY(op(L.d, L.t));  // r = b - A x for the restriction
+  hipLaunchKernelGGL(k_sub, dim3(grid1(L.nb * BS)), dim3(TPB), 0, c.stream, L.nb * BS, L.r, L.t, sc);
+  PZG_HIP(hipGetLastError());
+  return PZG_OK;
+}
+
+template <int BS, class Op>
+int smooth_post(Ctx &c, const LevelView &L, const float *b, float *x, Op op, const SolveScalars *sc) {
+  const int g = grid1(L.nb);
+  PZG_TRY(op(x, L.t));
+  hipLaunchKernelGGL(k_cheb_restart<BS>, dim3(g), dim3(TPB), 0, c.stream, L.nb, L.m, L.sa, L.sk, b, L.t, x,
+                     L.r, L.d, L.ch.c0, sc);
+  PZG_HIP(hipGetLastError());
+  for (size_t j = 0; j < L.ch.c1.size(); j++) {
+    PZG_TRY(op(L.d, L.t));
+    hipLaunchKernelGGL(k_cheb_step<BS>, dim3(g), dim3(TPB), 0, c.stream, L.nb, L.m, L.sa, L.sk, L.t, L.r,
+                       L.d, x, L.ch.c1[j], L.ch.c2[j], sc);
+    PZG_HIP(hipGetLastError());
+  }
+  return PZG_OK;
+}
+
+Dims dims_of(const MgLevel &l) { return Dims{{l.n[0], l.n[1], l.n[2]}}; }
+
+// level-0 operator y = S x (matrix-free or stored FP32)
+int op0(Ctx &c, const float *x, float *y, const SolveScalars *sc) {
+  if (c.mg_use_mf) return mf_apply(c, x, nullptr, y, sc, nullptr, nullptr, false);
+  return launch_spmv_bsr(c, x, y);
+}
+
+template <int BS>
+int op_lattice(Ctx &c, const MgLevel &l, const float *x, float *y, const SolveScalars *sc) {
+  hipLaunchKernelGGL(k_stencil<BS>, dim3(grid1(l.nn)), dim3(TPB), 0, c.stream, dims_of(l), l.nn,
+                     l.A.as<float>(), x, y, sc);
+  PZG_HIP(hipGetLastError());
+  return PZG_OK;
+}
+
+LevelView view0(Ctx &c) {
+  LevelView v;
+  v.nb = c.nb;
+  v.m = c.d_bjinv.as<float>();
+  v.sa = (int64_t)c.bs * c.bs;
+  v.sk = 1;
+  v.r = c.d_mg_r0.as<float>();
+  v.d = c.d_mg_d0.as<float>();
+  v.t = c.d_mg_t0.as<float>();
+  v.ch = cheb_coeffs(c.mg_lmax0, c.mg_range, c.mg_degree);
+  return v;
+}
+LevelView view_l(Ctx &c, MgLevel &l) {
+  LevelView v;
+  v.nb = l.nn;
+  v.m = l.minv.as<float>();
+  v.sa = 1;
+  v.sk = l.nn;
+  v.r = l.r.as<float>();
+  v.d = l.d.as<float>();
+  v.t = l.t.as<float>();
+  v.ch = cheb_coeffs(l.lmax, c.mg_range, c.mg_degree);
+  return v;
+}
+
+// largest eigenvalue of the symmetric tridiagonal (a, b): bisection on the
+// Sturm count (eigenvalues below x = negative pivots of T - x I)
+double tridiag_max(const std::vector<double> &a, const std::vector<double> &b) {
+  const int k = (int)a.size();
+  double hi = 0.;
+  for (int i = 0; i < k; i++)
+    hi = std::max(hi, a[i] + (i > 0 ? std::fabs(b[i - 1]) : 0.) + (i + 1 < k ? std::fabs(b[i]) : 0.));
+  double lo = 0.;
+  auto below = [&](double x) {
+    int cnt = 0;
+    double d = 1.;
+    for (int i = 0; i < k; i++) {
+      d = a[i] - x - (i > 0 ? b[i - 1] * b[i - 1] / d : 0.);
+      if (d == 0.) d = -1e-300;
+      if (d < 0.) cnt++;
+    }
+    return cnt;
+  };
+  for (int it = 0; it < 100; it++) {
+    const double mid = 0.5 * (lo + hi);
+    if (below(mid) == k) hi = mid;
+    else lo = mid;
+  }
+  return hi;
+}
+
+// r -= alpha q; z = M^-1 r; partial sums of r.z
+template <int BS>
+__global__ __launch_bounds__(TPB) void k_lz_update(int64_t nb, const float *__restrict__ m, int64_t sa,
+                                                   int64_t sk, float alpha, const float *__restrict__ q,
+                                                   float *__restrict__ r, float *__restrict__ z,
+                                                   double *partials) {
+  double rz = 0.;
+  for (int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; A < nb;
+       A += (int64_t)gridDim.x * blockDim.x) {
+    float rv[BS], y[BS];
+    for (int k = 0; k < BS; k++) {
+      rv[k] = r[A * BS + k] - alpha * q[A * BS + k];
+      r[A * BS + k] = rv[k];
+    }
+    minv_mul<BS>(m, sa, sk, A, rv, y);
+    for (int k = 0; k < BS; k++) {
+      z[A * BS + k] = y[k];
+      rz += (double)rv[k] * y[k];
+    }
+  }
+  double vv[1] = {rz};
+  block_partials<1>(vv, partials);
+}
+__global__ __launch_bounds__(TPB) void k_dot(int64_t n, const float *__restrict__ a,
+                                             const float *__restrict__ b, double *partials) {
+  double s = 0.;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    s += (double)a[i] * b[i];
+  double vv[1] = {s};
+  block_partials<1>(vv, partials);
+}
+// p = z + beta p
+__global__ void k_xpby(int64_t n, const float *__restrict__ z, float beta, float *__restrict__ p) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = z[i] + beta * p[i];
+}
+
+// largest eigenvalue of M^-1 A: MG_LANCZOS_IT steps of preconditioned CG
+// from a pseudo-random right-hand side; the CG coefficients give the
+// Lanczos tridiagonal (diagonal 1/alpha_k + beta_{k-1}/alpha_{k-1},
+// off-diagonal sqrt(beta_k)/alpha_k), whose largest eigenvalue
+// approximates lmax from below.  Work vectors: r, z, p, q.
+template <int BS, class Op>
+int lanczos_lmax(Ctx &c, int64_t nb, const float *m, int64_t sa, int64_t sk, float *r, float *z, float *p,
+                 float *q, Op op, double &lmax) {
+  const int64_t n = nb * BS;
+  double *part = c.d_partials.as<double>();
+  const int gr = gridr(n), gb = gridr(nb);
+  std::vector<double> hp(MAXBLK);
+  auto sum = [&](int g, double &out) -> int {
+    PZG_HIP(hipMemcpyAsync(hp.data(), part, sizeof(double) * g, hipMemcpyDeviceToHost, c.stream));
+    PZG_HIP(hipStreamSynchronize(c.stream));
+    out = 0.;
+    for (int b = 0; b < g; b++) out += hp[b];
+    return PZG_OK;
+  };
+  hipLaunchKernelGGL(k_hash_fill, dim3(grid1(n)), dim3(TPB), 0, c.stream, n, r, (uint64_t)0x5DEECE66DULL);
+  // z = M^-1 r, r.z (the update with alpha = 0 and q = r)
+  hipLaunchKernelGGL(k_lz_update<BS>, dim3(gb), dim3(TPB), 0, c.stream, nb, m, sa, sk, 0.f, r, r, z, part);
+  PZG_HIP(hipGetLastError());
+  double rz;
+  PZG_TRY(sum(gb, rz));
+  PZG_HIP(hipMemcpyAsync(p, z, sizeof(float) * n, hipMemcpyDeviceToDevice, c.stream));
+  std::vector<double> al, be, ta, tb;
+  for (int it = 0; it < MG_LANCZOS_IT; it++) {
+    PZG_TRY(op(p, q));
+    hipLaunchKernelGGL(k_dot, dim3(gr), dim3(TPB), 0, c.stream, n, p, q, part);
+    PZG_HIP(hipGetLastError());
+    double pq;
+    PZG_TRY(sum(gr, pq));
+    if (!(pq > 0.) || !(rz > 0.)) break;
+    const double alpha = rz / pq;
+    hipLaunchKernelGGL(k_lz_update<BS>, dim3(gb), dim3(TPB), 0, c.stream, nb, m, sa, sk, (float)alpha, q, r, z,
+                       part);
+    PZG_HIP(hipGetLastError());
+    double rzn;
+    PZG_TRY(sum(gb, rzn));
+    const double beta = rzn / rz;
+    ta.push_back(1.0 / alpha + (al.empty() ? 0. : be.back() / al.back()));
+    al.push_back(alpha);
+    be.push_back(beta);
+    if (!(rzn > 0.)) break;
+    tb.push_back(std::sqrt(beta) / alpha);
+    rz = rzn;
+    hipLaunchKernelGGL(k_xpby, dim3(grid1(n)), dim3(TPB), 0, c.stream, n, z, (float)beta, p);
+    PZG_HIP(hipGetLastError());
+  }
+  if (ta.empty()) {
+    set_error("multigrid: operator not positive definite on a level");
+    return PZG_EUNSUP;
+  }
+  tb.resize(ta.size());
+  lmax = tridiag_max(ta, tb);
+  return PZG_OK;
+}
+
+// inverse of a symmetric positive definite matrix (Cholesky, in place)
+bool spd_inverse(std::vector<double> &a, int n) {
+  std::vector<double> L(a);
+  for (int j = 0; j < n; j++) {
+    double s = L[(size_t)j * n + j];
+    for (int k = 0; k < j; k++) s -= L[(size_t)j * n + k] * L[(size_t)j * n + k];
+    if (!(s > 0.)) return false;
+    const double d = std::sqrt(s);
+    L[(size_t)j * n + j] = d;
+    for (int i = j + 1; i < n; i++) {
+      double t = L[(size_t)i * n + j];
+      for (int k = 0; k < j; k++) t -= L[(size_t)i * n + k] * L[(size_t)j * n + k];
+      L[(size_t)i * n + j] = t / d;
+    }
+  }
+  // columns of A^-1 = L^-T L^-1 e_c
+  std::vector<double> y(n);
+  for (int col = 0; col < n; col++) {
+    for (int i = 0; i < n; i++) {
+      double t = i == col ? 1. : 0.;
+      for (int k = 0; k < i; k++) t -= L[(size_t)i * n + k] * y[k];
+      y[i] = t / L[(size_t)i * n + i];
+    }
+    for (int i = n - 1; i >= 0; i--) {
+      double t = y[i];
+      for (int k = i + 1; k < n; k++) t -= L[(size_t)k * n + i] * y[k];
+      y[i] = t / L[(size_t)i * n + i];
+    }
+    for (int i = 0; i < n; i++) a[(size_t)i * n + col] = y[i];
+  }
+  return true;
+}
+
+template <int BS>
+int setup_impl(Ctx &c) {
+  // vertex lattice: the corner nodes of the volume elements and their blocks
+  if (!c.has_mesh || c.dim != 3 || c.vol_etype != PZG_HEX || c.hp_simplex || c.has_dep || c.external_csr) {
+    set_error("multigrid preconditioner: needs a hexahedral mesh-built system without constrained connects");
+    return PZG_EUNSUP;
+  }
+  const int nc = c.vol_ncorner, nsd = c.vol_nsides;
+  std::vector<int64_t> nblk(c.nnodes, -1);
+  int64_t nvert = 0;
+  for (int64_t e = 0; e < c.nvol; e++)
+    for (int k = 0; k < 8; k++) {
+      const int64_t node = c.vol_nodes[e * nc + k];
+      const int64_t b = c.aidx[c.vol_conn[e * nsd + k]];
+      if (b < 0 || c.blk_size[c.vol_conn[e * nsd + k]] != BS) {
+        set_error("multigrid preconditioner: a vertex connect without its own equations");
+        return PZG_EUNSUP;
+      }
+      if (nblk[node] == -1) {
+        nblk[node] = b;
+        nvert++;
+      } else if (nblk[node] != b) {
+        set_error("multigrid preconditioner: a mesh node with two vertex connects");
+        return PZG_EUNSUP;
+      }
+    }
+  // distinct coordinates per direction (to 1e-9 of the extent)
+  int n[3];
+  std::vector<std::vector<double>> u(3);
+  for (int d = 0; d < 3; d++) {
+    std::vector<double> x;
+    for (int64_t p = 0; p < c.nnodes; p++)
+      if (nblk[p] >= 0) x.push_back(c.xyz[3 * p + d]);
+    std::sort(x.begin(), x.end());
+    const double tol = 1e-9 * std::max(1.0, std::fabs(x.back() - x.front()));
+    for (double v : x)
+      if (u[d].empty() || v - u[d].back() > tol) u[d].push_back(v);
+    n[d] = (int)u[d].size();
+  }
+  const int64_t nn1 = (int64_t)n[0] * n[1] * n[2];
+  if (nn1 != nvert || nn1 > INT32_MAX) {
+    set_error("multigrid preconditioner: the mesh vertices do not form a lattice");
+    return PZG_EUNSUP;
+  }
+  std::vector<int32_t> vmap(nn1, -1);
+  for (int64_t p = 0; p < c.nnodes; p++) {
+    if (nblk[p] < 0) continue;
+    int id[3];
+    for (int d = 0; d < 3; d++) {
+      const double v = c.xyz[3 * p + d];
+      auto it = std::lower_bound(u[d].begin(), u[d].end(), v - 1e-9 * std::max(1.0, std::fabs(u[d].back() - u[d].front())));
+      id[d] = (int)(it - u[d].begin());
+      if (id[d] >= n[d]) id[d] = n[d] - 1;
+    }
+    const int64_t l = id[0] + (int64_t)n[0] * (id[1] + (int64_t)n[1] * id[2]);
+    if (vmap[l] != -1) {
+      set_error("multigrid preconditioner: the mesh vertices do not form a lattice");
+      return PZG_EUNSUP;
+    }
+    vmap[l] = (int32_t)nblk[p];
+  }
+  // levels: halve while the element counts are even and the level is too
+  // large for the dense solve
+  {
+    MgLevel l;
+    for (int d = 0; d < 3; d++) l.n[d] = n[d];
+    l.nn = nn1;
+    c.mg.push_back(std::move(l));
+  }
+  for (;;) {
+    const MgLevel &f = c.mg.back();
+    if (f.nn * BS <= MG_DENSE) break;
+    bool even = true;
+    for (int d = 0; d < 3; d++) even &= (f.n[d] - 1) % 2 == 0 && f.n[d] >= 3;
+    if (!even) break;
+    MgLevel l;
+    for (int d = 0; d < 3; d++) l.n[d] = (f.n[d] - 1) / 2 + 1;
+    l.nn = (int64_t)l.n[0] * l.n[1] * l.n[2];
+    c.mg.push_back(std::move(l));
+  }
+  if (c.mg.back().nn * BS > MG_MAXDENSE) {
+    set_error("multigrid preconditioner: the lattice does not coarsen to a dense level (element counts must halve)");
+    return PZG_EUNSUP;
+  }
+  // device arrays
+  PZG_TRY(dev_upload(c.d_mg_vmap, vmap.data(), vmap.size(), c.stream));
+  PZG_TRY(dev_alloc(c.d_mg_dsq, sizeof(float) * nn1 * BS));
+  for (DevBuf *b : {&c.d_mg_r0, &c.d_mg_d0, &c.d_mg_t0, &c.d_mg_w0}) PZG_TRY(dev_alloc(*b, sizeof(float) * c.neq));
+  for (MgLevel &l : c.mg) {
+    PZG_TRY(dev_alloc(l.A, sizeof(float) * 27 * BS * BS * l.nn));
+    PZG_TRY(dev_alloc(l.minv, sizeof(float) * BS * BS * l.nn));
+    for (DevBuf *b : {&l.x, &l.b, &l.r, &l.d, &l.t}) PZG_TRY(dev_alloc(*b, sizeof(float) * BS * l.nn));
+  }
+  hipLaunchKernelGGL(k_mg_dsq<BS>, dim3(gridx(nn1)), dim3(TPB), 0, c.stream, nn1, c.d_mg_vmap.as<int32_t>(),
+                     c.d_dinvsqrt.as<double>(), c.d_mg_dsq.as<float>());
+  hipLaunchKernelGGL(k_mg_l1<BS>, dim3(gridx(27 * nn1)), dim3(TPB), 0, c.stream, dims_of(c.mg[0]), nn1,
+                     c.d_mg_vmap.as<int32_t>(), c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(),
+                     c.d_fa.as<double>(), c.mg[0].A.as<float>());
+  PZG_HIP(hipGetLastError());
+  for (size_t k = 1; k < c.mg.size(); k++) {
+    hipLaunchKernelGGL(k_mg_rap<BS>, dim3(gridx(27 * c.mg[k].nn)), dim3(TPB), 0, c.stream, dims_of(c.mg[k]),
+                       dims_of(c.mg[k - 1]), c.mg[k - 1].A.as<float>(), c.mg[k].A.as<float>());
+    PZG_HIP(hipGetLastError());
+  }
+  for (MgLevel &l : c.mg) {
+    hipLaunchKernelGGL(k_mg_minv<BS>, dim3(gridx(l.nn)), dim3(TPB), 0, c.stream, l.nn, l.A.as<float>(),
+                       l.minv.as<float>());
+    PZG_HIP(hipGetLastError());
+  }
+  // dense inverse of the coarsest level
+  {
+    const MgLevel &l = c.mg.back();
+    const int nd = (int)(l.nn * BS);
+    std::vector<float> h((size_t)27 * BS * BS * l.nn);
+    PZG_HIP(hipMemcpyAsync(h.data(), l.A.p, h.size() * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+    PZG_HIP(hipStreamSynchronize(c.stream));
+    std::vector<double> a((size_t)nd * nd, 0.);
+    for (int64_t v = 0; v < l.nn; v++) {
+      const int i = (int)(v % l.n[0]), j = (int)((v / l.n[0]) % l.n[1]), k = (int)(v / ((int64_t)l.n[0] * l.n[1]));
+      for (int s = 0; s < 27; s++) {
+        const int ii = i + s % 3 - 1, jj = j + (s / 3) % 3 - 1, kk = k + s / 9 - 1;
+        if (ii < 0 || jj < 0 || kk < 0 || ii >= l.n[0] || jj >= l.n[1] || kk >= l.n[2]) continue;
+        const int64_t w = ii + (int64_t)l.n[0] * (jj + (int64_t)l.n[1] * kk);
+        for (int e = 0; e < BS * BS; e++)
+          a[(size_t)(v * BS + e / BS) * nd + w * BS + e % BS] = h[((size_t)s * BS * BS + e) * l.nn + v];
+      }
+    }
+    for (int i = 0; i < nd; i++)  // symmetric part (the stencil holds both triangles)
+      for (int j = i + 1; j < nd; j++) {
+        const double m = 0.5 * (a[(size_t)i * nd + j] + a[(size_t)j * nd + i]);
+        a[(size_t)i * nd + j] = a[(size_t)j * nd + i] = m;
+      }
+    if (!spd_inverse(a, nd)) {
+      set_error("multigrid preconditioner: the coarsest operator is not positive definite");
+      return PZG_EUNSUP;
+    }
+    std::vector<float> af(a.begin(), a.end());
+    PZG_TRY(dev_upload(c.d_mg_cinv, af.data(), af.size(), c.stream));
+    c.mg_ncoarse = nd;
+  }
+  // largest eigenvalues of M^-1 A: level 0, then every smoothed lattice level
+  {
+    auto op = [&](const float *x, float *y) { return op0(c, x, y, nullptr); };
+    PZG_TRY(lanczos_lmax<BS>(c, c.nb, c.d_bjinv.as<float>(), (int64_t)BS * BS, 1, c.d_mg_r0.as<float>(),
+                             c.d_mg_d0.as<float>(), c.d_mg_t0.as<float>(), c.d_mg_w0.as<float>(), op,
+                             c.mg_lmax0));
+  }
+  for (size_t k = 0; k + 1 < c.mg.size(); k++) {
+    MgLevel &l = c.mg[k];
+    auto op = [&](const float *x, float *y) { return op_lattice<BS>(c, l, x, y, nullptr); };
+    PZG_TRY(lanczos_lmax<BS>(c, l.nn, l.minv.as<float>(), 1, l.nn, l.x.as<float>(), l.b.as<float>(),
+                             l.d.as<float>(), l.t.as<float>(), op, l.lmax));
+  }
+  PZG_HIP(hipStreamSynchronize(c.stream));
+  c.mg_ready = true;
+  return PZG_OK;
+}
+
+// z = V-cycle applied to r; then p = z (when given) and the z.q partials
+// (when q is given), as the block-Jacobi update leaves them
+template <int BS>
+int apply_impl(Ctx &c, const float *r, float *z, float *p, const float *q, SolveScalars *sc,
+               double *partials) {
+  const LevelView L0 = view0(c);
+  auto op_0 = [&](const float *x, float *y) { return op0(c, x, y, sc); };
+  const int nl = (int)c.mg.size();
+  PZG_TRY(smooth_pre<BS>(c, L0, r, z, op_0, sc));
+  hipLaunchKernelGGL(k_restrict0<BS>, dim3(grid1(c.mg[0].nn)), dim3(TPB), 0, c.stream, c.mg[0].nn,
+                     c.d_mg_vmap.as<int32_t>(), c.d_mg_dsq.as<float>(), L0.r, c.mg[0].b.as<float>(), sc);
+  PZG_HIP(hipGetLastError());
+  for (int k = 0; k + 1 < nl; k++) {
+    MgLevel &l = c.mg[k];
+    const LevelView V = view_l(c, l);
+    auto op = [&](const float *x, float *y) { return op_lattice<BS>(c, l, x, y, sc); };
+    PZG_TRY(smooth_pre<BS>(c, V, l.b.as<float>(), l.x.as<float>(), op, sc));
+    hipLaunchKernelGGL(k_restrict<BS>, dim3(grid1(c.mg[k + 1].nn)), dim3(TPB), 0, c.stream,
+                       dims_of(c.mg[k + 1]), dims_of(l), V.r, c.mg[k + 1].b.as<float>(), sc);
+    PZG_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_dense, dim3((c.mg_ncoarse + TPB - 1) / TPB), dim3(TPB), 0, c.stream, (int)c.mg_ncoarse,
+                     c.d_mg_cinv.as<float>(), c.mg.back().b.as<float>(), c.mg.back().x.as<float>(), sc);
+  PZG_HIP(hipGetLastError());
+  for (int k = nl - 2; k >= 0; k--) {
+    MgLevel &l = c.mg[k];
+    hipLaunchKernelGGL(k_prolong<BS>, dim3(grid1(l.nn)), dim3(TPB), 0, c.stream, dims_of(c.mg[k + 1]),
+                       dims_of(l), c.mg[k + 1].x.as<float>(), l.x.as<float>(), sc);
+    PZG_HIP(hipGetLastError());
+    const LevelView V = view_l(c, l);
+    auto op = [&](const float *x, float *y) { return op_lattice<BS>(c, l, x, y, sc); };
+    PZG_TRY(smooth_post<BS>(c, V, l.b.as<float>(), l.x.as<float>(), op, sc));
+  }
+  hipLaunchKernelGGL(k_prolong0<BS>, dim3(grid1(c.mg[0].nn)), dim3(TPB), 0, c.stream, c.mg[0].nn,
+                     c.d_mg_vmap.as<int32_t>(), c.d_mg_dsq.as<float>(), c.mg[0].x.as<float>(), z, sc);
+  PZG_HIP(hipGetLastError());
+  PZG_TRY(smooth_post<BS>(c, L0, r, z, op_0, sc));
+  if (p || q) {
+    // same grid as the FCG update kernel, whose partials slot 1 this replaces
+    hipLaunchKernelGGL(k_mg_finish<BS>, dim3(gridr(c.nb)), dim3(TPB), 0, c.stream, c.nb, z, p, q, sc, partials);
+    PZG_HIP(hipGetLastError());
+  }
+  return PZG_OK;
+}
+
+}  // namespace
+
+void mg_free(Ctx &c) {
+  for (MgLevel &l : c.mg)
+    for (DevBuf *b : {&l.A, &l.minv, &l.x, &l.b, &l.r, &l.d, &l.t}) dev_free(*b);
+  c.mg.clear();
+  for (DevBuf *b : {&c.d_mg_vmap, &c.d_mg_dsq, &c.d_mg_cinv, &c.d_mg_r0, &c.d_mg_d0, &c.d_mg_t0, &c.d_mg_w0})
+    dev_free(*b);
+  c.mg_ready = false;
+}
+
+int mg_setup(Ctx &c, bool use_mf) {
+  mg_free(c);
+  c.mg_use_mf = use_mf;
+  if (c.mg_degree < 1 || c.mg_degree > 8 || !(c.mg_range > 1.)) {
+    set_error("multigrid preconditioner: degree must be 1..8 and the range above 1");
+    return PZG_EINVAL;
+  }
+  if (c.bs == 3) return setup_impl<3>(c);
+  if (c.bs == 1) return setup_impl<1>(c);
+  set_error("multigrid preconditioner: nstate must be 1 or 3");
+  return PZG_EUNSUP;
+}
+
+int mg_apply(Ctx &c, const float *r, float *z, float *p, const float *q, SolveScalars *sc, double *partials) {
+  if (!c.mg_ready) {
+    set_error("multigrid preconditioner: not set up");
+    return PZG_ESTATE;
+  }
+  if (c.bs == 3) return apply_impl<3>(c, r, z, p, q, sc, partials);
+  return apply_impl<1>(c, r, z, p, q, sc, partials);
+}
+
+}  // namespace pzg

@@ -358,6 +358,7 @@ static int ensure_reduce(Ctx &c) {
   PZG_TRY(dev_alloc(c.d_scal, sizeof(SolveScalars) * 2));
   return PZG_OK;
 }
+int ensure_reduce_pub(Ctx &c) { return ensure_reduce(c); }
 
 int launch_multadd_upper(Ctx &c, const double *x, const double *y, double *z, double alpha,
                          double beta) {
@@ -751,6 +752,7 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
   }
   if (use_mf) PZG_TRY(mf_prepare(c));
   PZG_TRY(ensure_reduce(c));
+  const bool multigrid = prm.precond == PZG_PRECOND_MULTIGRID;
   for (DevBuf *b : {&c.d_u, &c.d_r}) PZG_TRY(dev_alloc(*b, n * sizeof(double)));
   for (DevBuf *b : {&c.d_xs, &c.d_rs, &c.d_zs, &c.d_ps, &c.d_qs}) PZG_TRY(dev_alloc(*b, n * sizeof(float)));
   DevBuf out;
@@ -766,7 +768,8 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
   int outer = 0;
   double rel = 1.0;
   bool conv = false;
-  const int batch = prm.check_every > 0 ? prm.check_every : 50;
+  // a multigrid iteration costs several operator applications: short batches
+  const int batch = prm.check_every > 0 ? prm.check_every : (multigrid ? 4 : 50);
   auto true_res = [&]() -> int {
     hipLaunchKernelGGL(k_residual64<BS>, dim3(bsr_grid(nb)), dim3(TPB), 0, c.stream, nb,
                        c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(), c.d_fa.as<double>(),
@@ -787,6 +790,12 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
   };
   PZG_TRY(true_res());
   if (rel <= prm.tol) conv = true;
+  if (multigrid && !conv) {
+    // levels and eigenvalue estimates (timed with the solve)
+    c.mg_degree = prm.mg_degree > 0 ? prm.mg_degree : 2;
+    c.mg_range = prm.mg_range > 0 ? prm.mg_range : 15.;
+    if (!c.mg_ready || c.mg_use_mf != use_mf) PZG_TRY(mg_setup(c, use_mf));
+  }
   bool restart = true;
   bool learned = prm.precond == PZG_PRECOND_LEARNED;
   int fallback = 0;
@@ -808,6 +817,8 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
       if (learned)
         PZG_TRY(nn_apply(c, c.d_rs.as<float>(), c.d_zs.as<float>(), c.d_ps.as<float>(), nullptr, sc,
                          part, cnt));
+      if (multigrid)
+        PZG_TRY(mg_apply(c, c.d_rs.as<float>(), c.d_zs.as<float>(), c.d_ps.as<float>(), nullptr, sc, part));
       if (use_mf) PZG_TRY(mf_scale(c, c.d_ps.as<float>(), sc));
       restart = false;
     }
@@ -829,6 +840,8 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
       if (learned)
         (void)nn_apply(c, c.d_rs.as<float>(), c.d_zs.as<float>(), nullptr, c.d_qs.as<float>(), sc,
                        part, cnt);
+      if (multigrid)
+        (void)mg_apply(c, c.d_rs.as<float>(), c.d_zs.as<float>(), nullptr, c.d_qs.as<float>(), sc, part);
       hipLaunchKernelGGL(k_fin_fcg_beta, dim3(1), dim3(TPB), 0, c.stream, sc, (const double *)part, gb);
       hipLaunchKernelGGL(k_p_update32, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_zs.as<float>(),
                          c.d_ps.as<float>(), sc, use_mf ? c.d_mf_dis.as<float>() : nullptr,

@@ -170,7 +170,7 @@ class TPZStepSolver {
     p.tol = fTol;
     p.inner_tol = 1e-4;
     p.max_outer = 200;
-    p.check_every = 50;
+    p.check_every = fPrecond == PZG_PRECOND_MULTIGRID ? 0 : 50;  // 0: the solver's batch
     p.use_graph = 1;
     p.refine = 0;
     p.inner_op = PZG_INNER_AUTO;

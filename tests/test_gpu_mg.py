@@ -1,0 +1,135 @@
+"""GPU checks of the multigrid preconditioner of the mixed solver
+(PZG_PRECOND_MULTIGRID) against the float64 model of tests/mg_model.py
+built from the oracle's matrix.
+
+Bars (written here, checked below):
+  * levels: the lattice node counts equal the model's;
+  * eigenvalue estimates (20 CG-Lanczos steps, from below): 0.9 lmax <=
+    est <= (1 + 1e-3) lmax, lmax from scipy on the model's operators;
+  * one V-cycle z = M r (with the device's estimates): |z - z_model| <=
+    2e-4 |z_model| (FP32 operators and vectors through every level);
+  * the mixed solve with the V-cycle: true residual <= 1e-8, the oracle's
+    residual of the solution <= 2e-8 and the solution within 1e-5 of a
+    direct solve of the oracle's system (the bars of the block-Jacobi mixed
+    solves), in fewer iterations than block Jacobi;
+  * refusals: tetrahedra, a lattice that does not halve to a dense level.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import mg_model as G
+import oracle as O
+from neopz_amd import _lib as L
+from neopz_amd import problems as P
+from neopz_amd.analysis import Context, TPZLinearAnalysis, TPZStepSolver
+
+pytestmark = pytest.mark.gpu
+
+_CTX = None
+
+
+def ctx():
+    global _CTX
+    if _CTX is None:
+        _CTX = Context(0)
+    return _CTX
+
+
+CASES = {
+    "elast8_p2": lambda: P.elasticity3d(n=8, p=2, seed=3, field_kind="grf", load="body"),
+    "poisson16_p2": lambda: P.Problem(material=P.POISSON, dim=3, p=2, nx=16, ny=16, nz=16,
+                                      force_kind=P.FORCE_CONST, force_const=(1.0, 0, 0),
+                                      bc_type=[0, 1, 0, -1, -1, -1]),
+    "darcy8_p1": lambda: P.Problem(material=P.DARCY, dim=3, p=1, nx=8, ny=8, nz=8,
+                                   el_coef=np.exp(np.random.default_rng(5).standard_normal(512)),
+                                   force_kind=P.FORCE_CONST, force_const=(1.0, 0, 0), bc_type=[0] * 6),
+    "elast4_p2": lambda: P.elasticity3d(n=4, p=2, seed=1, field_kind="iid", load="traction"),
+}
+
+_MODELS = {}
+
+
+def _model(name):
+    if name not in _MODELS:
+        pb = CASES[name]()
+        o = O.Oracle(pb)
+        ia, ja = o.pattern()
+        a, rhs = o.assemble()
+        K = G.full_from_upper(ia, ja, a, o.neq)
+        an = TPZLinearAnalysis(pb, False, ctx=ctx())
+        m = G.Model(K, an.mesh, pb.nstate)
+        _MODELS[name] = (pb, o, K, rhs, m, m.lmax_exact())
+    return _MODELS[name]
+
+
+def _mg_apply(op, r, degree=0, rng=0.0):
+    z = np.zeros_like(r)
+    nlev = C.c_int32(0)
+    dims = np.zeros(48, dtype=np.int32)
+    lmax = np.zeros(16)
+    L.check(ctx().lib.pzg_mg_apply(ctx().h, op, degree, rng, L.ptr(r, L.dp), L.ptr(z, L.dp),
+                                   C.byref(nlev), L.ptr(dims, L.i32p), L.ptr(lmax, L.dp)))
+    nl = nlev.value
+    return z, [list(dims[3 * k:3 * k + 3]) for k in range(nl)], lmax[:nl + 1]
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("op", [L.INNER_BSR, L.INNER_MATFREE])
+def test_vcycle_matches_model(name, op):
+    pb, o, K, rhs, m, lex = _model(name)
+    print(f"[gpu] multigrid V-cycle {name} op {op}", flush=True)
+    an = TPZLinearAnalysis(pb, False, ctx=ctx())
+    an.Assemble()
+    r = np.random.default_rng(7).standard_normal(o.neq)
+    for degree, rng in ((0, 0.0), (3, 25.0)):
+        z, dims, lmax = _mg_apply(op, r, degree, rng)
+        assert dims == m.dims()
+        lex_ = np.asarray(lex)
+        assert np.all(lmax[:-1] >= 0.9 * lex_[:-1]) and np.all(lmax[:-1] <= (1 + 1e-3) * lex_[:-1]), \
+            (lmax, lex_)
+        zm = m.apply(r, lmax, degree or 2, rng or 15.0)
+        err = np.linalg.norm(z - zm) / np.linalg.norm(zm)
+        print(f"[gpu]   degree {degree} range {rng}: levels {dims}, lmax {np.round(lmax, 4)}, "
+              f"rel diff {err:.2e}", flush=True)
+        assert err <= 2e-4
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_mixed_solve_with_multigrid(name):
+    pb, o, K, rhs, m, lex = _model(name)
+    print(f"[gpu] multigrid mixed solve {name}", flush=True)
+    x = spla.spsolve(K.tocsc(), rhs)
+    its = {}
+    for pre in (L.PRECOND_BLOCK_JACOBI, L.PRECOND_MULTIGRID):
+        an = TPZLinearAnalysis(pb, False, ctx=ctx())
+        st = TPZStepSolver()
+        st.SetMixedFCG(200000, 1e-8, precond=pre)
+        an.SetSolver(st)
+        an.Assemble()
+        u = an.Solve()
+        info = st.info
+        assert info["converged"] and info["true_residual"] <= 1e-8
+        # the residual bar of the mixed solves on unconstrained meshes: the
+        # oracle's residual of u, and u against the direct solve as far as
+        # the condition number lets a 1e-8 residual fix it
+        assert np.linalg.norm(rhs - K @ u) <= 2e-8 * np.linalg.norm(rhs)
+        assert np.linalg.norm(u - x) <= 1e-5 * np.linalg.norm(x)
+        its[pre] = info["iterations"]
+    print(f"[gpu]   iterations: block Jacobi {its[L.PRECOND_BLOCK_JACOBI]}, "
+          f"multigrid {its[L.PRECOND_MULTIGRID]}", flush=True)
+    assert its[L.PRECOND_MULTIGRID] < its[L.PRECOND_BLOCK_JACOBI]
+
+
+def test_multigrid_refusals():
+    for pb in (P.darcy3d_tet(n=2, pmax=2, seed=1, hp=False),
+               P.Problem(material=P.POISSON, dim=3, p=2, nx=11, ny=11, nz=11, bc_type=[0] * 6)):
+        an = TPZLinearAnalysis(pb, False, ctx=ctx())
+        an.Assemble()
+        r = np.ones(an.mesh["neq"])
+        z = np.zeros_like(r)
+        rc = ctx().lib.pzg_mg_apply(ctx().h, L.INNER_BSR, 0, 0.0, L.ptr(r, L.dp), L.ptr(z, L.dp),
+                                    None, None, None)
+        assert rc == L.PZG_EUNSUP
