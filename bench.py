@@ -37,7 +37,10 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-baseline", action="store_true", help="skip the FP64 Jacobi-PCG run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle sample")
-    ap.add_argument("--precond", default="block_jacobi", choices=["block_jacobi", "learned"])
+    ap.add_argument("--precond", default="multigrid", choices=["multigrid", "block_jacobi", "learned"])
+    ap.add_argument("--mg-degree", type=int, default=0, help="multigrid Chebyshev degree (0 = 2)")
+    ap.add_argument("--mg-range", type=float, default=0.0, help="multigrid smoothing interval ratio (0 = 15)")
+    ap.add_argument("--no-bj", action="store_true", help="skip the block-Jacobi mixed solve beside multigrid")
     ap.add_argument("--refine", type=int, default=0, help="0 restart FCG per refinement, 1 reliable update")
     ap.add_argument("--inner-tol", type=float, default=1e-4)
     ap.add_argument("--no-graph", action="store_true", help="plain launches instead of hipGraph replay")
@@ -192,16 +195,20 @@ def main():
         kernels_only(ctx, L)
         return
 
-    def solver(kind):
+    def solver(kind, precond=None):
         st = TPZStepSolver(K)
         if kind == "pcg64":
             st.SetCG(200000, TPZStepSolver.jacobi(), args.tol, 0)
+            st.check_every = 100
         else:
-            pre = L.PRECOND_LEARNED if args.precond == "learned" else L.PRECOND_BLOCK_JACOBI
+            pre = {"learned": L.PRECOND_LEARNED, "block_jacobi": L.PRECOND_BLOCK_JACOBI,
+                   "multigrid": L.PRECOND_MULTIGRID}[precond or args.precond]
             op = {"auto": L.INNER_AUTO, "bsr": L.INNER_BSR, "matfree": L.INNER_MATFREE}[args.inner_op]
             st.SetMixedFCG(200000, args.tol, precond=pre, inner_tol=args.inner_tol,
-                           refine=args.refine, inner_op=op)
-        st.check_every = 100
+                           refine=args.refine, inner_op=op, mg_degree=args.mg_degree,
+                           mg_range=args.mg_range)
+            # multigrid iterations are few and costly: the solver's short batches
+            st.check_every = 0 if pre == L.PRECOND_MULTIGRID else 100
         st.use_graph = not args.no_graph
         return st
 
@@ -230,6 +237,16 @@ def main():
     # solutions leave the device outside the timed regions
     u_mixed = np.zeros(md["neq"])
     L.check(ctx.lib.pzg_get_solution(ctx.h, L.ptr(u_mixed, L.dp)))
+    # the block-Jacobi mixed solve on the same system, for comparison
+    bj = None
+    if args.precond == "multigrid" and not args.no_bj:
+        stj = solver("mixed", "block_jacobi")
+        t0 = time.perf_counter()
+        ij = step(stj)
+        L.check(ctx.lib.pzg_sync(ctx.h))
+        bj = {"seconds": time.perf_counter() - t0, "iterations": ij["iterations"],
+              "true_residual": ij["true_residual"]}
+        log(f"mixed block Jacobi: {bj}")
     # plain FP64 Jacobi-PCG GPU path (denominator of the speed-up)
     base = None
     if not args.no_baseline:
@@ -302,6 +319,7 @@ def main():
                    "nnz_full": sizes["nnz_full"], "elements": sizes["nvol"], "seed": args.seed},
         "solve": infos[-1],
         "jacobi_pcg64_gpu": base,
+        "mixed_block_jacobi": bj,
         "speedup_vs_pcg64": (base["seconds"] / value) if base else None,
         "assembly_ms": asm_ms,
         "elements_per_s": sizes["nvol"] / (asm_ms * 1e-3),

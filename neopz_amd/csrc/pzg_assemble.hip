@@ -1300,8 +1300,9 @@ __global__ void k_bsr_diag(int64_t nb, int64_t nbf, const int64_t *__restrict__ 
       m[l][k] = v;
     }
   const int64_t b0 = fbia[A], n = fbia[A + 1] - b0;
-  for (int k = 0; k < BS; k++)
-    for (int l = 0; l < BS; l++) bsr[bsr_pos<BS * BS>(b0, n, nlow[A], k * BS + l)] = (float)m[k][l];
+  if (bsr)
+    for (int k = 0; k < BS; k++)
+      for (int l = 0; l < BS; l++) bsr[bsr_pos<BS * BS>(b0, n, nlow[A], k * BS + l)] = (float)m[k][l];
   float *inv = binv + A * BS * BS;
   if (BS == 1) {
     inv[0] = (float)(1.0 / m[0][0]);
@@ -1349,9 +1350,10 @@ __global__ void k_bsr_off(int64_t nb, int64_t nbf, const int64_t *__restrict__ i
   }
 }
 
-int launch_build_bsr(Ctx &c) {
+int launch_build_bsr(Ctx &c, bool values) {
   const int bs = c.bs;
-  PZG_TRY(dev_alloc(c.d_bsr, sizeof(float) * c.nbf * bs * bs));
+  if (values) PZG_TRY(dev_alloc(c.d_bsr, sizeof(float) * c.nbf * bs * bs));
+  float *bsr = values ? c.d_bsr.as<float>() : nullptr;
   PZG_TRY(dev_alloc(c.d_bjinv, sizeof(float) * c.nb * bs * bs));
   PZG_TRY(dev_alloc(c.d_dinvsqrt, sizeof(double) * c.neq));
   DevBuf nl;
@@ -1364,9 +1366,9 @@ int launch_build_bsr(Ctx &c) {
   if (bs == 3) {
     hipLaunchKernelGGL(k_bsr_diag<3>, dim3((c.nb + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
                        c.nbf, c.d_ia.as<int64_t>(), c.d_a.as<double>(), nl.as<int32_t>(),
-                       c.d_fbia.as<int64_t>(), c.d_dinvsqrt.as<double>(), c.d_bsr.as<float>(),
-                       c.d_bjinv.as<float>());
-    hipLaunchKernelGGL(k_bsr_off<3>, dim3((threads + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
+                       c.d_fbia.as<int64_t>(), c.d_dinvsqrt.as<double>(), bsr, c.d_bjinv.as<float>());
+    if (values)
+      hipLaunchKernelGGL(k_bsr_off<3>, dim3((threads + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
                        c.nbf, c.d_ia.as<int64_t>(), c.d_a.as<double>(), c.d_ubia.as<int64_t>(),
                        c.d_ubja.as<int32_t>(), c.d_rank_fwd.as<int32_t>(),
                        c.d_rank_bwd.as<int32_t>(), c.d_fbia.as<int64_t>(),
@@ -1374,9 +1376,9 @@ int launch_build_bsr(Ctx &c) {
   } else if (bs == 1) {
     hipLaunchKernelGGL(k_bsr_diag<1>, dim3((c.nb + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
                        c.nbf, c.d_ia.as<int64_t>(), c.d_a.as<double>(), nl.as<int32_t>(),
-                       c.d_fbia.as<int64_t>(), c.d_dinvsqrt.as<double>(), c.d_bsr.as<float>(),
-                       c.d_bjinv.as<float>());
-    hipLaunchKernelGGL(k_bsr_off<1>, dim3((threads + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
+                       c.d_fbia.as<int64_t>(), c.d_dinvsqrt.as<double>(), bsr, c.d_bjinv.as<float>());
+    if (values)
+      hipLaunchKernelGGL(k_bsr_off<1>, dim3((threads + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
                        c.nbf, c.d_ia.as<int64_t>(), c.d_a.as<double>(), c.d_ubia.as<int64_t>(),
                        c.d_ubja.as<int32_t>(), c.d_rank_fwd.as<int32_t>(),
                        c.d_rank_bwd.as<int32_t>(), c.d_fbia.as<int64_t>(),
@@ -1389,7 +1391,8 @@ int launch_build_bsr(Ctx &c) {
   PZG_HIP(hipGetLastError());
   PZG_HIP(hipStreamSynchronize(c.stream));
   dev_free(nl);
-  c.bsr_valid = true;
+  c.scal_valid = true;
+  if (values) c.bsr_valid = true;
   c.mf_vals_valid = false;  // D^-1/2 recomputed
   c.mg_ready = false;       // multigrid levels are built from K and D
   return PZG_OK;

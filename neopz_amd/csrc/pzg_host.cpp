@@ -777,7 +777,7 @@ static int struct_create(Ctx &c) {
   c.assembled = false;
   c.kel_added = false;
   c.mf_ready = c.mf_vals_valid = false;
-  c.full_valid = c.bsr_valid = false;
+  c.full_valid = c.bsr_valid = c.scal_valid = false;
   c.has_solution = false;
   return PZG_OK;
 }
@@ -878,7 +878,7 @@ int pzg_assemble(pzg_ctx *ctx) {
   PZG_TRY(launch_diag(c));
   c.assembled = true;
   c.kel_added = false;
-  c.full_valid = c.bsr_valid = false;
+  c.full_valid = c.bsr_valid = c.scal_valid = false;
   c.has_solution = false;
   return PZG_OK;
 }
@@ -1028,7 +1028,7 @@ int pzg_csr_set(pzg_ctx *ctx, int64_t neq, const int64_t *ia, const int64_t *ja,
   c.external_csr = true;
   c.mf_ready = c.mf_vals_valid = false;
   c.assembled = true;
-  c.full_valid = c.bsr_valid = false;
+  c.full_valid = c.bsr_valid = c.scal_valid = false;
   c.has_solution = false;
   return PZG_OK;
 }
@@ -1086,7 +1086,7 @@ int pzg_add_kel_batch(pzg_ctx *ctx, int64_t nel, int32_t ne, const int64_t *dest
   dev_free(d_list);
   c.assembled = true;
   c.kel_added = true;
-  c.full_valid = c.bsr_valid = false;
+  c.full_valid = c.bsr_valid = c.scal_valid = false;
   return rc;
 }
 
@@ -1407,7 +1407,7 @@ int pzg_mesh_set_coef(pzg_ctx *ctx, const double *vol_coef) {
   PZG_TRY(dev_upload(c.d_vcoef, c.vol_coef.data(), c.vol_coef.size(), c.stream));
   PZG_HIP(hipStreamSynchronize(c.stream));
   c.assembled = false;
-  c.full_valid = c.bsr_valid = false;
+  c.full_valid = c.bsr_valid = c.scal_valid = false;
   c.has_solution = false;
   return PZG_OK;
 }

@@ -245,6 +245,7 @@ struct Ctx {
   DevBuf d_dinvsqrt;          // fp64 D^-1/2
   DevBuf d_bjinv;             // fp32 inverse diagonal blocks of the scaled matrix
   bool full_valid = false, bsr_valid = false;
+  bool scal_valid = false;    // d_dinvsqrt and d_bjinv (also without the d_bsr values)
 
   // ---- assembly (device) -------------------------------------------------
   DevBuf d_xyz, d_vnodes, d_vconn, d_vcoef, d_bnodes, d_bconn, d_bcid, d_bctype, d_bcval;
@@ -332,7 +333,8 @@ int launch_diag(Ctx &c);
 int launch_expand_full(Ctx &c);
 // equation-level full CSR (device arrays, any may be null; ia without ia[neq])
 int launch_full_csr(Ctx &c, int64_t *d_ia, int64_t *d_ja, double *d_a);
-int launch_build_bsr(Ctx &c);
+// D^-1/2 and the block-Jacobi inverses of S; values: also the FP32 block CSR of S
+int launch_build_bsr(Ctx &c, bool values = true);
 int launch_spmv64(Ctx &c, const double *x, double *y);
 int launch_spmv_bsr(Ctx &c, const float *x, float *y);
 int launch_multadd_upper(Ctx &c, const double *x, const double *y, double *z, double alpha,

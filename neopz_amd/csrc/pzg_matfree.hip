@@ -778,7 +778,7 @@ int mf_setup(Ctx &c) {
 int mf_prepare(Ctx &c) {
   PZG_TRY(mf_setup(c));
   if (c.mf_vals_valid) return PZG_OK;
-  if (!c.bsr_valid) PZG_TRY(launch_build_bsr(c));  // computes D^-1/2
+  if (!c.scal_valid) PZG_TRY(launch_build_bsr(c, false));  // computes D^-1/2
   std::vector<float> coef(c.nvol);
   const double dflt = c.material == PZG_MAT_ELAST3D ? c.E : c.scale;
   for (int64_t e = 0; e < c.nvol; e++) coef[e] = (float)(c.vol_coef.empty() ? dflt : c.vol_coef[e]);

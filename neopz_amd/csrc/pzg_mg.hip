@@ -274,15 +274,17 @@ __global__ void k_prolong0(int64_t nn, const int32_t *__restrict__ vmap, const f
   }
 }
 
-// x = Ainv b (dense, symmetric, nd unknowns)
+// x = Ainv b (dense, symmetric, nd unknowns): one wavefront per row
 __global__ void k_dense(int nd, const float *__restrict__ ainv, const float *__restrict__ b,
                         float *__restrict__ x, const SolveScalars *sc) {
   if (stopped(sc)) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
   if (i >= nd) return;
   float s = 0.f;
-  for (int j = 0; j < nd; j++) s += ainv[(int64_t)j * nd + i] * b[j];
-  x[i] = s;
+  for (int j = lane; j < nd; j += 64) s += ainv[(int64_t)i * nd + j] * b[j];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) x[i] = s;
 }
 
 // z stays the preconditioned residual; p = z (restart); z.q replaces slot 1
@@ -882,7 +884,7 @@ int apply_impl(Ctx &c, const float *r, float *z, float *p, const float *q, Solve
                        dims_of(c.mg[k + 1]), dims_of(l), V.r, c.mg[k + 1].b.as<float>(), sc);
     PZG_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_dense, dim3((c.mg_ncoarse + TPB - 1) / TPB), dim3(TPB), 0, c.stream, (int)c.mg_ncoarse,
+  hipLaunchKernelGGL(k_dense, dim3(gridx(c.mg_ncoarse * 64)), dim3(TPB), 0, c.stream, (int)c.mg_ncoarse,
                      c.d_mg_cinv.as<float>(), c.mg.back().b.as<float>(), c.mg.back().x.as<float>(), sc);
   PZG_HIP(hipGetLastError());
   for (int k = nl - 2; k >= 0; k--) {

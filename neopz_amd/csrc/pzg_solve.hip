@@ -735,7 +735,6 @@ template <int BS>
 static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info) {
   const int64_t n = c.neq, nb = c.nb;
   if (!c.full_valid) PZG_TRY(launch_expand_full(c));
-  if (!c.bsr_valid) PZG_TRY(launch_build_bsr(c));
   // operator of the FP32 iterations
   bool use_mf = false;
   if (prm.inner_op == PZG_INNER_MATFREE) {
@@ -750,7 +749,10 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
     set_error("pzg_solve: unknown inner operator");
     return PZG_EINVAL;
   }
+  // the stored FP32 values of S only for the stored operator (the
+  // matrix-free one needs D^-1/2 and the block-Jacobi inverses alone)
   if (use_mf) PZG_TRY(mf_prepare(c));
+  else if (!c.bsr_valid) PZG_TRY(launch_build_bsr(c, true));
   PZG_TRY(ensure_reduce(c));
   const bool multigrid = prm.precond == PZG_PRECOND_MULTIGRID;
   for (DevBuf *b : {&c.d_u, &c.d_r}) PZG_TRY(dev_alloc(*b, n * sizeof(double)));
