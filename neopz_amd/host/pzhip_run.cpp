@@ -14,7 +14,7 @@
 // (int64/double arrays, each with its count), then iterations and the
 // achieved tolerance.
 //
-// usage: pzhip_run MESH OUT [cg|mixed] [tol] [maxit] [flags...]
+// usage: pzhip_run MESH OUT [cg|mixed|mixed-mg] [tol] [maxit] [flags...]
 // flags: sym (default) or full: TPZSpStructMatrix, IA/JA/A of the
 // non-symmetric TPZFYsmpMatrix; sloan: TPZLinearAnalysis(cmesh, true), the
 // connects renumbered by TPZSloanRenumbering first (the permutation is
@@ -93,7 +93,7 @@ void exact_solution(int dim, int nstate, const std::vector<double> &x, std::vect
 
 int main(int argc, char **argv) {
   if (argc < 3) {
-    std::fprintf(stderr, "usage: %s MESH OUT [cg|mixed] [tol] [maxit] [sym|full] [sloan] [errors]\n",
+    std::fprintf(stderr, "usage: %s MESH OUT [cg|mixed|mixed-mg] [tol] [maxit] [sym|full] [sloan] [errors]\n",
                  argv[0]);
     return 2;
   }
@@ -205,6 +205,8 @@ int main(int argc, char **argv) {
     step.SetCG(maxit, jac, tol, 0);
   } else if (method == "mixed") {
     step.SetMixedFCG(maxit, tol);
+  } else if (method == "mixed-mg") {  // multigrid-preconditioned FP32 iterations
+    step.SetMixedFCG(maxit, tol, PZG_PRECOND_MULTIGRID);
   } else {
     std::fprintf(stderr, "pzhip_run: unknown method %s\n", method.c_str());
     return 2;
