@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle sample")
     ap.add_argument("--precond", default="multigrid", choices=["multigrid", "block_jacobi", "learned"])
     ap.add_argument("--mg-degree", type=int, default=0, help="multigrid Chebyshev degree (0 = 2)")
-    ap.add_argument("--mg-range", type=float, default=0.0, help="multigrid smoothing interval ratio (0 = 15)")
+    ap.add_argument("--mg-range", type=float, default=0.0, help="multigrid smoothing interval ratio (0 = 30)")
     ap.add_argument("--no-bj", action="store_true", help="skip the block-Jacobi mixed solve beside multigrid")
     ap.add_argument("--refine", type=int, default=0, help="0 restart FCG per refinement, 1 reliable update")
     ap.add_argument("--inner-tol", type=float, default=1e-4)

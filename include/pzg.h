@@ -155,9 +155,10 @@ typedef struct {
   int32_t refine;        /* mixed: 0 = restart FCG from each true residual,
                                    1 = reliable update (keep the direction)    */
   int32_t inner_op;      /* mixed: PZG_INNER_* operator of the FP32 iterations */
-  int32_t mg_degree;     /* multigrid: Chebyshev degree per smoothing (0 = 2)  */
+  int32_t mg_degree;     /* multigrid: Chebyshev degree of the smoothing on the
+                            p level (0 = 2; the lattice levels take 1)         */
   double mg_range;       /* multigrid: lmax / lmin of the smoothed interval
-                            (0 = 15)                                           */
+                            (0 = 30)                                           */
 } pzg_solve_params;
 
 /* Operator of the mixed solver's FP32 iterations.  Both approximate the

@@ -558,6 +558,7 @@ static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
   c.has_mesh = true;
   c.has_struct = false;
   c.mf_ready = c.mf_vals_valid = false;
+  mg_free(c);  // the multigrid lattice belongs to the mesh
   c.assembled = false;
   c.external_csr = false;
   c.has_solution = false;
@@ -1372,7 +1373,7 @@ int pzg_mg_apply(pzg_ctx *ctx, int32_t op, int32_t degree, double range, const d
   if (op == PZG_INNER_MATFREE) PZG_TRY(mf_prepare(c));
   PZG_TRY(ensure_reduce_pub(c));
   c.mg_degree = degree > 0 ? degree : 2;
-  c.mg_range = range > 0 ? range : 15.;
+  c.mg_range = range > 0 ? range : 30.;
   const bool use_mf = op == PZG_INNER_MATFREE;
   if (!c.mg_ready || c.mg_use_mf != use_mf) PZG_TRY(mg_setup(c, use_mf));
   std::vector<float> h(n);

@@ -289,12 +289,13 @@ struct Ctx {
   // FCG system), vertex block of each lattice node of level 1, D^1/2 at the
   // vertex equations, the dense inverse of the coarsest level
   bool mg_ready = false, mg_use_mf = false;
+  bool mg_struct = false;  // lattice, levels and buffers of the current mesh
   int mg_degree = 2;       // Chebyshev degree of the pre- and post-smoothing
-  double mg_range = 15.;   // ratio of the smoothed interval's ends
+  double mg_range = 30.;   // ratio of the smoothed interval's ends
   double mg_lmax0 = 0.;    // largest eigenvalue of the block-Jacobi scaled S
   int64_t mg_ncoarse = 0;  // unknowns of the coarsest level
   std::vector<MgLevel> mg;
-  DevBuf d_mg_vmap, d_mg_dsq, d_mg_cinv, d_mg_r0, d_mg_d0, d_mg_t0, d_mg_w0;
+  DevBuf d_mg_vmap, d_mg_dsq, d_mg_cinv, d_mg_r0, d_mg_d0, d_mg_t0, d_mg_w0, d_mg_lz;
 
   // post-processing (pzg_post.hip): error-rule tables per rule bucket for the
   // exact-solution order post_porder (-1: not built), the elements of every

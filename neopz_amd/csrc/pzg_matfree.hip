@@ -777,8 +777,10 @@ int mf_setup(Ctx &c) {
 // Value-dependent set-up (after each assembly): coefficients and D^-1/2 in FP32.
 int mf_prepare(Ctx &c) {
   PZG_TRY(mf_setup(c));
+  // D^-1/2 and the block-Jacobi inverses of the current assembly (a rebuild
+  // clears mf_vals_valid, so the coefficients and D^-1/2 below follow it)
+  if (!c.scal_valid) PZG_TRY(launch_build_bsr(c, false));
   if (c.mf_vals_valid) return PZG_OK;
-  if (!c.scal_valid) PZG_TRY(launch_build_bsr(c, false));  // computes D^-1/2
   std::vector<float> coef(c.nvol);
   const double dflt = c.material == PZG_MAT_ELAST3D ? c.E : c.scale;
   for (int64_t e = 0; e < c.nvol; e++) coef[e] = (float)(c.vol_coef.empty() ? dflt : c.vol_coef[e]);

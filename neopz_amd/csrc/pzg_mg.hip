@@ -18,10 +18,11 @@
 // Smoother: Chebyshev iteration (Saad, Iterative Methods for Sparse Linear
 // Systems, 2nd ed., Alg. 12.1) of degree m on M^-1 A, M = the diagonal
 // nstate x nstate blocks, on the interval [lhi/range, lhi], lhi = 1.1 x the
-// largest eigenvalue estimated by CG-Lanczos.  The V-cycle takes m
-// pre- and m post-smoothing steps with the same polynomial and an exact
-// coarsest solve, so the preconditioner is a fixed symmetric positive
-// definite operator.  Every kernel of the cycle leaves at once when the FCG
+// largest eigenvalue estimated by CG-Lanczos; m = the requested degree on
+// level 0 and 1 on the lattice levels (a second step there does not lower
+// the iteration count).  Each level takes m pre- and m post-smoothing steps
+// with the same polynomial and the coarsest solve is exact, so the
+// preconditioner is a fixed symmetric positive definite operator.  Every kernel of the cycle leaves at once when the FCG
 // iteration has stopped (graph batches run past the stop).
 #include <hip/hip_runtime.h>
 
@@ -39,9 +40,10 @@ namespace pzg {
 namespace {
 
 constexpr int TPB = RED_TPB;
-constexpr int64_t MG_DENSE = 512;      // coarsening stops at this many unknowns
+constexpr int64_t MG_DENSE = 128;      // coarsening stops at this many unknowns
 constexpr int64_t MG_MAXDENSE = 1536;  // largest dense coarsest solve
 constexpr int MG_LANCZOS_IT = 20;      // CG-Lanczos steps per eigenvalue estimate
+constexpr int MG_LATTICE_DEGREE = 1;   // Chebyshev degree on the lattice levels
 
 int grid1(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + TPB - 1) / TPB, 1 << 20)); }
 // one thread per item (kernels without a grid-stride loop)
@@ -64,11 +66,14 @@ __device__ inline void minv_mul(const float *__restrict__ m, int64_t sa, int64_t
 }
 
 // ---- Chebyshev steps (vectors [nb][BS]) --------------------------------------
+// Level-0 variants also write pt = D^-1/2 d (pt != null), the input of the
+// matrix-free operator's next application.
 // r = b; d = c0 M^-1 b; x = d
 template <int BS>
 __global__ void k_cheb_init(int64_t nb, const float *__restrict__ m, int64_t sa, int64_t sk,
                             const float *__restrict__ b, float *__restrict__ x, float *__restrict__ r,
-                            float *__restrict__ d, float c0, const SolveScalars *sc) {
+                            float *__restrict__ d, float c0, const float *__restrict__ dis,
+                            float *__restrict__ pt, const SolveScalars *sc) {
   if (stopped(sc)) return;
   for (int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; A < nb;
        A += (int64_t)gridDim.x * blockDim.x) {
@@ -81,6 +86,7 @@ __global__ void k_cheb_init(int64_t nb, const float *__restrict__ m, int64_t sa,
       r[A * BS + k] = bv[k];
       d[A * BS + k] = c0 * y[k];
       x[A * BS + k] = c0 * y[k];
+      if (pt) pt[A * BS + k] = c0 * y[k] * dis[A * BS + k];
     }
   }
 }
@@ -90,7 +96,8 @@ template <int BS>
 __global__ void k_cheb_restart(int64_t nb, const float *__restrict__ m, int64_t sa, int64_t sk,
                                const float *__restrict__ b, const float *__restrict__ t,
                                float *__restrict__ x, float *__restrict__ r, float *__restrict__ d,
-                               float c0, const SolveScalars *sc) {
+                               float c0, const float *__restrict__ dis, float *__restrict__ pt,
+                               const SolveScalars *sc) {
   if (stopped(sc)) return;
   for (int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; A < nb;
        A += (int64_t)gridDim.x * blockDim.x) {
@@ -105,6 +112,7 @@ __global__ void k_cheb_restart(int64_t nb, const float *__restrict__ m, int64_t 
     for (int k = 0; k < BS; k++) {
       d[A * BS + k] = c0 * y[k];
       x[A * BS + k] += c0 * y[k];
+      if (pt) pt[A * BS + k] = c0 * y[k] * dis[A * BS + k];
     }
   }
 }
@@ -113,7 +121,8 @@ __global__ void k_cheb_restart(int64_t nb, const float *__restrict__ m, int64_t 
 template <int BS>
 __global__ void k_cheb_step(int64_t nb, const float *__restrict__ m, int64_t sa, int64_t sk,
                             const float *__restrict__ t, float *__restrict__ r, float *__restrict__ d,
-                            float *__restrict__ x, float c1, float c2, const SolveScalars *sc) {
+                            float *__restrict__ x, float c1, float c2, const float *__restrict__ dis,
+                            float *__restrict__ pt, const SolveScalars *sc) {
   if (stopped(sc)) return;
   for (int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; A < nb;
        A += (int64_t)gridDim.x * blockDim.x) {
@@ -129,16 +138,9 @@ __global__ void k_cheb_step(int64_t nb, const float *__restrict__ m, int64_t sa,
       const float dv = c1 * d[A * BS + k] + c2 * y[k];
       d[A * BS + k] = dv;
       x[A * BS + k] += dv;
+      if (pt) pt[A * BS + k] = dv * dis[A * BS + k];
     }
   }
-}
-
-__global__ void k_sub(int64_t n, float *__restrict__ r, const float *__restrict__ t,
-                      const SolveScalars *sc) {
-  if (stopped(sc)) return;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    r[i] -= t[i];
 }
 
 // ---- lattice operators ---------------------------------------------------------
@@ -185,11 +187,12 @@ __global__ void k_stencil(Dims D, int64_t nn, const float *__restrict__ A, const
   }
 }
 
-// trilinear restriction: b_c(N) = sum over the fine nodes 2N + d, d in
-// {-1,0,1}^3, of prod_i (1 - |d_i|/2) r_f
+// trilinear restriction of the residual r_f - t_f (the pre-smoothing leaves
+// its last product t_f unsubtracted): b_c(N) = sum over the fine nodes
+// 2N + d, d in {-1,0,1}^3, of prod_i (1 - |d_i|/2) (r_f - t_f)
 template <int BS>
-__global__ void k_restrict(Dims C, Dims F, const float *__restrict__ rf, float *__restrict__ bc,
-                           const SolveScalars *sc) {
+__global__ void k_restrict(Dims C, Dims F, const float *__restrict__ rf, const float *__restrict__ tf,
+                           float *__restrict__ bc, const SolveScalars *sc) {
   if (stopped(sc)) return;
   const int64_t nc = (int64_t)C.n[0] * C.n[1] * C.n[2];
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nc;
@@ -211,7 +214,7 @@ __global__ void k_restrict(Dims C, Dims F, const float *__restrict__ rf, float *
           const float w = (dx ? 0.5f : 1.f) * (dy ? 0.5f : 1.f) * (dz ? 0.5f : 1.f);
           const int64_t f = node_id(F, i, j, k);
 #pragma unroll
-          for (int e = 0; e < BS; e++) acc[e] += w * rf[f * BS + e];
+          for (int e = 0; e < BS; e++) acc[e] += w * (rf[f * BS + e] - tf[f * BS + e]);
         }
       }
     }
@@ -249,17 +252,18 @@ __global__ void k_prolong(Dims C, Dims F, const float *__restrict__ xc, float *_
   }
 }
 
-// level 0 <-> level 1: b1(v) = D^1/2 r0 at the vertex block vmap[v];
+// level 0 <-> level 1: b1(v) = D^1/2 (r0 - t0) at the vertex block vmap[v];
 // x0(vmap[v]) += D^1/2 x1(v)
 template <int BS>
 __global__ void k_restrict0(int64_t nn, const int32_t *__restrict__ vmap, const float *__restrict__ dsq,
-                            const float *__restrict__ r0, float *__restrict__ b1, const SolveScalars *sc) {
+                            const float *__restrict__ r0, const float *__restrict__ t0,
+                            float *__restrict__ b1, const SolveScalars *sc) {
   if (stopped(sc)) return;
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nn;
        v += (int64_t)gridDim.x * blockDim.x) {
     const int64_t A = vmap[v];
 #pragma unroll
-    for (int e = 0; e < BS; e++) b1[v * BS + e] = dsq[v * BS + e] * r0[A * BS + e];
+    for (int e = 0; e < BS; e++) b1[v * BS + e] = dsq[v * BS + e] * (r0[A * BS + e] - t0[A * BS + e]);
   }
 }
 template <int BS>
@@ -470,38 +474,41 @@ struct LevelView {
   const float *m;          // inverse diagonal blocks
   int64_t sa, sk;          // their layout
   float *r, *d, *t;        // work vectors
+  const float *dis = nullptr;  // level 0, matrix-free: D^-1/2 and the operator's
+  float *pt = nullptr;         // pre-scaled input, written with each new d
   Cheb ch;
 };
 
+// op(x, y, prescaled): y = A x; prescaled: L.pt already holds D^-1/2 x.
+// Leaves x and, as r - t, the residual b - A x (the restriction subtracts).
 template <int BS, class Op>
 int smooth_pre(Ctx &c, const LevelView &L, const float *b, float *x, Op op, const SolveScalars *sc) {
   const int g = grid1(L.nb);
+  const bool ps = L.pt != nullptr;
   hipLaunchKernelGGL(k_cheb_init<BS>, dim3(g), dim3(TPB), 0, c.stream, L.nb, L.m, L.sa, L.sk, b, x, L.r,
-                     L.d, L.ch.c0, sc);
+                     L.d, L.ch.c0, L.dis, L.pt, sc);
   PZG_HIP(hipGetLastError());
   for (size_t j = 0; j < L.ch.c1.size(); j++) {
-    PZG_TRY(op(L.d, L.t));
+    PZG_TRY(op(L.d, L.t, ps));
     hipLaunchKernelGGL(k_cheb_step<BS>, dim3(g), dim3(TPB), 0, c.stream, L.nb, L.m, L.sa, L.sk, L.t, L.r,
-                       L.d, x, L.ch.c1[j], L.ch.c2[j], sc);
+                       L.d, x, L.ch.c1[j], L.ch.c2[j], L.dis, L.pt, sc);
     PZG_HIP(hipGetLastError());
   }
-  PZG_TRY(op(L.d, L.t));  // r = b - A x for the restriction
-  hipLaunchKernelGGL(k_sub, dim3(grid1(L.nb * BS)), dim3(TPB), 0, c.stream, L.nb * BS, L.r, L.t, sc);
-  PZG_HIP(hipGetLastError());
-  return PZG_OK;
+  return op(L.d, L.t, ps);
 }
 
 template <int BS, class Op>
 int smooth_post(Ctx &c, const LevelView &L, const float *b, float *x, Op op, const SolveScalars *sc) {
   const int g = grid1(L.nb);
-  PZG_TRY(op(x, L.t));
+  const bool ps = L.pt != nullptr;
+  PZG_TRY(op(x, L.t, false));
   hipLaunchKernelGGL(k_cheb_restart<BS>, dim3(g), dim3(TPB), 0, c.stream, L.nb, L.m, L.sa, L.sk, b, L.t, x,
-                     L.r, L.d, L.ch.c0, sc);
+                     L.r, L.d, L.ch.c0, L.dis, L.pt, sc);
   PZG_HIP(hipGetLastError());
   for (size_t j = 0; j < L.ch.c1.size(); j++) {
-    PZG_TRY(op(L.d, L.t));
+    PZG_TRY(op(L.d, L.t, ps));
     hipLaunchKernelGGL(k_cheb_step<BS>, dim3(g), dim3(TPB), 0, c.stream, L.nb, L.m, L.sa, L.sk, L.t, L.r,
-                       L.d, x, L.ch.c1[j], L.ch.c2[j], sc);
+                       L.d, x, L.ch.c1[j], L.ch.c2[j], L.dis, L.pt, sc);
     PZG_HIP(hipGetLastError());
   }
   return PZG_OK;
@@ -509,9 +516,10 @@ int smooth_post(Ctx &c, const LevelView &L, const float *b, float *x, Op op, con
 
 Dims dims_of(const MgLevel &l) { return Dims{{l.n[0], l.n[1], l.n[2]}}; }
 
-// level-0 operator y = S x (matrix-free or stored FP32)
-int op0(Ctx &c, const float *x, float *y, const SolveScalars *sc) {
-  if (c.mg_use_mf) return mf_apply(c, x, nullptr, y, sc, nullptr, nullptr, false);
+// level-0 operator y = S x (matrix-free or stored FP32); prescaled: the
+// matrix-free input D^-1/2 x is already in place
+int op0(Ctx &c, const float *x, float *y, const SolveScalars *sc, bool prescaled) {
+  if (c.mg_use_mf) return mf_apply(c, x, nullptr, y, sc, nullptr, nullptr, prescaled);
   return launch_spmv_bsr(c, x, y);
 }
 
@@ -532,6 +540,10 @@ LevelView view0(Ctx &c) {
   v.r = c.d_mg_r0.as<float>();
   v.d = c.d_mg_d0.as<float>();
   v.t = c.d_mg_t0.as<float>();
+  if (c.mg_use_mf) {
+    v.dis = c.d_mf_dis.as<float>();
+    v.pt = c.d_mf_pt.as<float>();
+  }
   v.ch = cheb_coeffs(c.mg_lmax0, c.mg_range, c.mg_degree);
   return v;
 }
@@ -544,7 +556,7 @@ LevelView view_l(Ctx &c, MgLevel &l) {
   v.r = l.r.as<float>();
   v.d = l.d.as<float>();
   v.t = l.t.as<float>();
-  v.ch = cheb_coeffs(l.lmax, c.mg_range, c.mg_degree);
+  v.ch = cheb_coeffs(l.lmax, c.mg_range, MG_LATTICE_DEGREE);
   return v;
 }
 
@@ -574,12 +586,18 @@ double tridiag_max(const std::vector<double> &a, const std::vector<double> &b) {
   return hi;
 }
 
-// r -= alpha q; z = M^-1 r; partial sums of r.z
+// CG-Lanczos on the device.  lz[0] = r.z of the current step, lz[1 + 2k] =
+// alpha_k, lz[2 + 2k] = beta_k, lz[LZ_ZERO] = 0; a step whose p.q or r.z is
+// not positive leaves alpha = 0 (and every later step with it).
+constexpr int LZ_ZERO = 2 * MG_LANCZOS_IT + 1;
+
+// r -= alpha q; z = M^-1 r; partial sums of r.z (alpha = *al)
 template <int BS>
 __global__ __launch_bounds__(TPB) void k_lz_update(int64_t nb, const float *__restrict__ m, int64_t sa,
-                                                   int64_t sk, float alpha, const float *__restrict__ q,
-                                                   float *__restrict__ r, float *__restrict__ z,
-                                                   double *partials) {
+                                                   int64_t sk, const double *__restrict__ al,
+                                                   const float *__restrict__ q, float *__restrict__ r,
+                                                   float *__restrict__ z, double *partials) {
+  const float alpha = (float)*al;
   double rz = 0.;
   for (int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; A < nb;
        A += (int64_t)gridDim.x * blockDim.x) {
@@ -606,62 +624,71 @@ __global__ __launch_bounds__(TPB) void k_dot(int64_t n, const float *__restrict_
   double vv[1] = {s};
   block_partials<1>(vv, partials);
 }
-// p = z + beta p
-__global__ void k_xpby(int64_t n, const float *__restrict__ z, float beta, float *__restrict__ p) {
+// one block: what = 0: lz[0] = r.z; 1: alpha_k = lz[0] / p.q; 2: beta_k =
+// r.z / lz[0], lz[0] = r.z
+__global__ void k_lz_scalar(const double *partials, int nblk, double *lz, int k, int what) {
+  const double s = sum_partials(partials, 0, nblk);
+  if (threadIdx.x != 0) return;
+  if (what == 0) {
+    lz[0] = s;
+  } else if (what == 1) {
+    const bool live = k == 0 || lz[1 + 2 * (k - 1)] > 0.;
+    lz[1 + 2 * k] = live && s > 0. && lz[0] > 0. ? lz[0] / s : 0.;
+  } else {
+    const bool live = lz[1 + 2 * k] > 0. && s > 0.;
+    lz[2 + 2 * k] = live ? s / lz[0] : 0.;
+    if (live) lz[0] = s;
+  }
+}
+// p = z + beta p (beta = *be)
+__global__ void k_xpby(int64_t n, const float *__restrict__ z, const double *__restrict__ be,
+                       float *__restrict__ p) {
+  const float beta = (float)*be;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     p[i] = z[i] + beta * p[i];
 }
 
 // largest eigenvalue of M^-1 A: MG_LANCZOS_IT steps of preconditioned CG
-// from a pseudo-random right-hand side; the CG coefficients give the
-// Lanczos tridiagonal (diagonal 1/alpha_k + beta_{k-1}/alpha_{k-1},
-// off-diagonal sqrt(beta_k)/alpha_k), whose largest eigenvalue
-// approximates lmax from below.  Work vectors: r, z, p, q.
+// from a pseudo-random right-hand side, all on the device; the CG
+// coefficients give the Lanczos tridiagonal (diagonal 1/alpha_k +
+// beta_{k-1}/alpha_{k-1}, off-diagonal sqrt(beta_k)/alpha_k), whose largest
+// eigenvalue approximates lmax from below.  Work vectors: r, z, p, q.
 template <int BS, class Op>
 int lanczos_lmax(Ctx &c, int64_t nb, const float *m, int64_t sa, int64_t sk, float *r, float *z, float *p,
                  float *q, Op op, double &lmax) {
   const int64_t n = nb * BS;
   double *part = c.d_partials.as<double>();
+  double *lz = c.d_mg_lz.as<double>();
   const int gr = gridr(n), gb = gridr(nb);
-  std::vector<double> hp(MAXBLK);
-  auto sum = [&](int g, double &out) -> int {
-    PZG_HIP(hipMemcpyAsync(hp.data(), part, sizeof(double) * g, hipMemcpyDeviceToHost, c.stream));
-    PZG_HIP(hipStreamSynchronize(c.stream));
-    out = 0.;
-    for (int b = 0; b < g; b++) out += hp[b];
-    return PZG_OK;
-  };
+  PZG_HIP(hipMemsetAsync(lz, 0, sizeof(double) * (LZ_ZERO + 1), c.stream));
   hipLaunchKernelGGL(k_hash_fill, dim3(grid1(n)), dim3(TPB), 0, c.stream, n, r, (uint64_t)0x5DEECE66DULL);
-  // z = M^-1 r, r.z (the update with alpha = 0 and q = r)
-  hipLaunchKernelGGL(k_lz_update<BS>, dim3(gb), dim3(TPB), 0, c.stream, nb, m, sa, sk, 0.f, r, r, z, part);
+  // z = M^-1 r and r.z (the update with alpha = 0)
+  hipLaunchKernelGGL(k_lz_update<BS>, dim3(gb), dim3(TPB), 0, c.stream, nb, m, sa, sk, lz + LZ_ZERO, r, r, z,
+                     part);
+  hipLaunchKernelGGL(k_lz_scalar, dim3(1), dim3(TPB), 0, c.stream, (const double *)part, gb, lz, 0, 0);
   PZG_HIP(hipGetLastError());
-  double rz;
-  PZG_TRY(sum(gb, rz));
   PZG_HIP(hipMemcpyAsync(p, z, sizeof(float) * n, hipMemcpyDeviceToDevice, c.stream));
-  std::vector<double> al, be, ta, tb;
-  for (int it = 0; it < MG_LANCZOS_IT; it++) {
+  for (int k = 0; k < MG_LANCZOS_IT; k++) {
     PZG_TRY(op(p, q));
     hipLaunchKernelGGL(k_dot, dim3(gr), dim3(TPB), 0, c.stream, n, p, q, part);
+    hipLaunchKernelGGL(k_lz_scalar, dim3(1), dim3(TPB), 0, c.stream, (const double *)part, gr, lz, k, 1);
+    hipLaunchKernelGGL(k_lz_update<BS>, dim3(gb), dim3(TPB), 0, c.stream, nb, m, sa, sk, lz + 1 + 2 * k, q,
+                       r, z, part);
+    hipLaunchKernelGGL(k_lz_scalar, dim3(1), dim3(TPB), 0, c.stream, (const double *)part, gb, lz, k, 2);
+    hipLaunchKernelGGL(k_xpby, dim3(grid1(n)), dim3(TPB), 0, c.stream, n, z, lz + 2 + 2 * k, p);
     PZG_HIP(hipGetLastError());
-    double pq;
-    PZG_TRY(sum(gr, pq));
-    if (!(pq > 0.) || !(rz > 0.)) break;
-    const double alpha = rz / pq;
-    hipLaunchKernelGGL(k_lz_update<BS>, dim3(gb), dim3(TPB), 0, c.stream, nb, m, sa, sk, (float)alpha, q, r, z,
-                       part);
-    PZG_HIP(hipGetLastError());
-    double rzn;
-    PZG_TRY(sum(gb, rzn));
-    const double beta = rzn / rz;
-    ta.push_back(1.0 / alpha + (al.empty() ? 0. : be.back() / al.back()));
-    al.push_back(alpha);
-    be.push_back(beta);
-    if (!(rzn > 0.)) break;
+  }
+  std::vector<double> h(LZ_ZERO + 1);
+  PZG_HIP(hipMemcpyAsync(h.data(), lz, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c.stream));
+  PZG_HIP(hipStreamSynchronize(c.stream));
+  std::vector<double> ta, tb;
+  for (int k = 0; k < MG_LANCZOS_IT; k++) {
+    const double alpha = h[1 + 2 * k], beta = h[2 + 2 * k];
+    if (!(alpha > 0.)) break;
+    ta.push_back(1.0 / alpha + (k > 0 ? h[2 * k] / h[2 * k - 1] : 0.));
+    if (!(beta > 0.)) break;
     tb.push_back(std::sqrt(beta) / alpha);
-    rz = rzn;
-    hipLaunchKernelGGL(k_xpby, dim3(grid1(n)), dim3(TPB), 0, c.stream, n, z, (float)beta, p);
-    PZG_HIP(hipGetLastError());
   }
   if (ta.empty()) {
     set_error("multigrid: operator not positive definite on a level");
@@ -705,8 +732,11 @@ bool spd_inverse(std::vector<double> &a, int n) {
   return true;
 }
 
+// Mesh-level part: the vertex lattice, the levels and their buffers (kept
+// until the mesh changes).
 template <int BS>
-int setup_impl(Ctx &c) {
+int structure_impl(Ctx &c) {
+  mg_free(c);
   // vertex lattice: the corner nodes of the volume elements and their blocks
   if (!c.has_mesh || c.dim != 3 || c.vol_etype != PZG_HEX || c.hp_simplex || c.has_dep || c.external_csr) {
     set_error("multigrid preconditioner: needs a hexahedral mesh-built system without constrained connects");
@@ -792,12 +822,23 @@ int setup_impl(Ctx &c) {
   // device arrays
   PZG_TRY(dev_upload(c.d_mg_vmap, vmap.data(), vmap.size(), c.stream));
   PZG_TRY(dev_alloc(c.d_mg_dsq, sizeof(float) * nn1 * BS));
+  PZG_TRY(dev_alloc(c.d_mg_lz, sizeof(double) * (LZ_ZERO + 1)));
   for (DevBuf *b : {&c.d_mg_r0, &c.d_mg_d0, &c.d_mg_t0, &c.d_mg_w0}) PZG_TRY(dev_alloc(*b, sizeof(float) * c.neq));
   for (MgLevel &l : c.mg) {
     PZG_TRY(dev_alloc(l.A, sizeof(float) * 27 * BS * BS * l.nn));
     PZG_TRY(dev_alloc(l.minv, sizeof(float) * BS * BS * l.nn));
     for (DevBuf *b : {&l.x, &l.b, &l.r, &l.d, &l.t}) PZG_TRY(dev_alloc(*b, sizeof(float) * BS * l.nn));
   }
+  c.mg_struct = true;
+  return PZG_OK;
+}
+
+// Matrix-level part (after every assembly): D^1/2 at the vertices, the
+// level operators, the coarsest inverse and the eigenvalue estimates.
+template <int BS>
+int setup_impl(Ctx &c) {
+  if (!c.mg_struct) PZG_TRY(structure_impl<BS>(c));
+  const int64_t nn1 = c.mg[0].nn;
   hipLaunchKernelGGL(k_mg_dsq<BS>, dim3(gridx(nn1)), dim3(TPB), 0, c.stream, nn1, c.d_mg_vmap.as<int32_t>(),
                      c.d_dinvsqrt.as<double>(), c.d_mg_dsq.as<float>());
   hipLaunchKernelGGL(k_mg_l1<BS>, dim3(gridx(27 * nn1)), dim3(TPB), 0, c.stream, dims_of(c.mg[0]), nn1,
@@ -847,7 +888,7 @@ int setup_impl(Ctx &c) {
   }
   // largest eigenvalues of M^-1 A: level 0, then every smoothed lattice level
   {
-    auto op = [&](const float *x, float *y) { return op0(c, x, y, nullptr); };
+    auto op = [&](const float *x, float *y) { return op0(c, x, y, nullptr, false); };
     PZG_TRY(lanczos_lmax<BS>(c, c.nb, c.d_bjinv.as<float>(), (int64_t)BS * BS, 1, c.d_mg_r0.as<float>(),
                              c.d_mg_d0.as<float>(), c.d_mg_t0.as<float>(), c.d_mg_w0.as<float>(), op,
                              c.mg_lmax0));
@@ -869,19 +910,19 @@ template <int BS>
 int apply_impl(Ctx &c, const float *r, float *z, float *p, const float *q, SolveScalars *sc,
                double *partials) {
   const LevelView L0 = view0(c);
-  auto op_0 = [&](const float *x, float *y) { return op0(c, x, y, sc); };
+  auto op_0 = [&](const float *x, float *y, bool ps) { return op0(c, x, y, sc, ps); };
   const int nl = (int)c.mg.size();
   PZG_TRY(smooth_pre<BS>(c, L0, r, z, op_0, sc));
   hipLaunchKernelGGL(k_restrict0<BS>, dim3(grid1(c.mg[0].nn)), dim3(TPB), 0, c.stream, c.mg[0].nn,
-                     c.d_mg_vmap.as<int32_t>(), c.d_mg_dsq.as<float>(), L0.r, c.mg[0].b.as<float>(), sc);
+                     c.d_mg_vmap.as<int32_t>(), c.d_mg_dsq.as<float>(), L0.r, L0.t, c.mg[0].b.as<float>(), sc);
   PZG_HIP(hipGetLastError());
   for (int k = 0; k + 1 < nl; k++) {
     MgLevel &l = c.mg[k];
     const LevelView V = view_l(c, l);
-    auto op = [&](const float *x, float *y) { return op_lattice<BS>(c, l, x, y, sc); };
+    auto op = [&](const float *x, float *y, bool) { return op_lattice<BS>(c, l, x, y, sc); };
     PZG_TRY(smooth_pre<BS>(c, V, l.b.as<float>(), l.x.as<float>(), op, sc));
     hipLaunchKernelGGL(k_restrict<BS>, dim3(grid1(c.mg[k + 1].nn)), dim3(TPB), 0, c.stream,
-                       dims_of(c.mg[k + 1]), dims_of(l), V.r, c.mg[k + 1].b.as<float>(), sc);
+                       dims_of(c.mg[k + 1]), dims_of(l), V.r, V.t, c.mg[k + 1].b.as<float>(), sc);
     PZG_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(k_dense, dim3(gridx(c.mg_ncoarse * 64)), dim3(TPB), 0, c.stream, (int)c.mg_ncoarse,
@@ -893,7 +934,7 @@ int apply_impl(Ctx &c, const float *r, float *z, float *p, const float *q, Solve
                        dims_of(l), c.mg[k + 1].x.as<float>(), l.x.as<float>(), sc);
     PZG_HIP(hipGetLastError());
     const LevelView V = view_l(c, l);
-    auto op = [&](const float *x, float *y) { return op_lattice<BS>(c, l, x, y, sc); };
+    auto op = [&](const float *x, float *y, bool) { return op_lattice<BS>(c, l, x, y, sc); };
     PZG_TRY(smooth_post<BS>(c, V, l.b.as<float>(), l.x.as<float>(), op, sc));
   }
   hipLaunchKernelGGL(k_prolong0<BS>, dim3(grid1(c.mg[0].nn)), dim3(TPB), 0, c.stream, c.mg[0].nn,
@@ -914,13 +955,15 @@ void mg_free(Ctx &c) {
   for (MgLevel &l : c.mg)
     for (DevBuf *b : {&l.A, &l.minv, &l.x, &l.b, &l.r, &l.d, &l.t}) dev_free(*b);
   c.mg.clear();
-  for (DevBuf *b : {&c.d_mg_vmap, &c.d_mg_dsq, &c.d_mg_cinv, &c.d_mg_r0, &c.d_mg_d0, &c.d_mg_t0, &c.d_mg_w0})
+  c.mg_struct = false;
+  for (DevBuf *b : {&c.d_mg_vmap, &c.d_mg_dsq, &c.d_mg_cinv, &c.d_mg_r0, &c.d_mg_d0, &c.d_mg_t0, &c.d_mg_w0,
+                    &c.d_mg_lz})
     dev_free(*b);
   c.mg_ready = false;
 }
 
 int mg_setup(Ctx &c, bool use_mf) {
-  mg_free(c);
+  c.mg_ready = false;
   c.mg_use_mf = use_mf;
   if (c.mg_degree < 1 || c.mg_degree > 8 || !(c.mg_range > 1.)) {
     set_error("multigrid preconditioner: degree must be 1..8 and the range above 1");

@@ -795,7 +795,7 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
   if (multigrid && !conv) {
     // levels and eigenvalue estimates (timed with the solve)
     c.mg_degree = prm.mg_degree > 0 ? prm.mg_degree : 2;
-    c.mg_range = prm.mg_range > 0 ? prm.mg_range : 15.;
+    c.mg_range = prm.mg_range > 0 ? prm.mg_range : 30.;
     if (!c.mg_ready || c.mg_use_mf != use_mf) PZG_TRY(mg_setup(c, use_mf));
   }
   bool restart = true;

@@ -6,16 +6,17 @@ Level 0: S = D^-1/2 K D^-1/2.  Level 1: the vertex-vertex part K_vv of K
 (x fastest); level 0 -> 1 restriction r_1 = D^1/2 r at the vertex
 equations, prolongation x_0 += D^1/2 x_1 there.  Levels 2..: trilinear
 P (kron of the 1D prolongations), A_c = P^T A P, while the level has more
-than 512 unknowns and every element count is even.  Smoother: Chebyshev
+than 128 unknowns and every element count is even.  Smoother: Chebyshev
 iteration (Saad, Alg. 12.1) on M^-1 A with M the nstate x nstate diagonal
-blocks, interval [1.1 lmax / range, 1.1 lmax]; m pre- and m post-steps;
+blocks, interval [1.1 lmax / range, 1.1 lmax]; m pre- and m post-steps
+(m = degree on level 0, coarse_degree = 1 on the lattice levels);
 dense solve on the coarsest level.  The largest eigenvalues are inputs
 (the device estimates them by power iteration; tests bound them apart).
 """
 import numpy as np
 import scipy.sparse as sp
 
-DENSE = 512
+DENSE = 128
 
 
 def full_from_upper(ia, ja, a, n):
@@ -138,9 +139,9 @@ class Model:
             x = x + d
         return x
 
-    def apply(self, r, lmax, degree=2, rng=15.0):
+    def apply(self, r, lmax, degree=2, rng=30.0, coarse_degree=1):
         co0 = self.cheb(lmax[0], rng, degree)
-        cos = [self.cheb(lmax[1 + k], rng, degree) for k in range(len(self.levels))]
+        cos = [self.cheb(lmax[1 + k], rng, coarse_degree) for k in range(len(self.levels))]
         x0, r0 = self.pre(self.S, self.M0, r, co0)
         b = [None] * len(self.levels)
         x = [None] * len(self.levels)
