@@ -158,32 +158,28 @@ __device__ inline int64_t node_id(const Dims &D, int i, int j, int k) {
 }
 
 // y = A x over the 27-point block stencil; stencil entry s = (dz+1)*9 +
-// (dy+1)*3 + (dx+1), value (s, e) of node v at A[(s*BS*BS + e)*nn + v]
+// (dy+1)*3 + (dx+1), value (s, e) of node v at A[(s*BS*BS + e)*nn + v].
+// One thread per (node, row component a), node fastest: the small levels
+// are latency-bound and take BS times the threads.
 template <int BS>
 __global__ void k_stencil(Dims D, int64_t nn, const float *__restrict__ A, const float *__restrict__ x,
                           float *__restrict__ y, const SolveScalars *sc) {
   if (stopped(sc)) return;
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nn;
-       v += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nn * BS;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = t % nn;
+    const int a = (int)(t / nn);
     int i, j, k;
     node_ijk(D, v, i, j, k);
-    float acc[BS];
-#pragma unroll
-    for (int e = 0; e < BS; e++) acc[e] = 0.f;
+    float acc = 0.f;
     for (int s = 0; s < 27; s++) {
       const int ii = i + s % 3 - 1, jj = j + (s / 3) % 3 - 1, kk = k + s / 9 - 1;
       if (ii < 0 || jj < 0 || kk < 0 || ii >= D.n[0] || jj >= D.n[1] || kk >= D.n[2]) continue;
       const int64_t w = node_id(D, ii, jj, kk);
-      float xv[BS];
 #pragma unroll
-      for (int e = 0; e < BS; e++) xv[e] = x[w * BS + e];
-#pragma unroll
-      for (int a = 0; a < BS; a++)
-#pragma unroll
-        for (int b = 0; b < BS; b++) acc[a] += A[((int64_t)s * BS * BS + a * BS + b) * nn + v] * xv[b];
+      for (int b = 0; b < BS; b++) acc += A[((int64_t)s * BS * BS + a * BS + b) * nn + v] * x[w * BS + b];
     }
-#pragma unroll
-    for (int e = 0; e < BS; e++) y[v * BS + e] = acc[e];
+    y[v * BS + a] = acc;
   }
 }
 
@@ -525,7 +521,7 @@ int op0(Ctx &c, const float *x, float *y, const SolveScalars *sc, bool prescaled
 
 template <int BS>
 int op_lattice(Ctx &c, const MgLevel &l, const float *x, float *y, const SolveScalars *sc) {
-  hipLaunchKernelGGL(k_stencil<BS>, dim3(grid1(l.nn)), dim3(TPB), 0, c.stream, dims_of(l), l.nn,
+  hipLaunchKernelGGL(k_stencil<BS>, dim3(grid1(l.nn * BS)), dim3(TPB), 0, c.stream, dims_of(l), l.nn,
                      l.A.as<float>(), x, y, sc);
   PZG_HIP(hipGetLastError());
   return PZG_OK;

@@ -328,6 +328,24 @@ __global__ __launch_bounds__(TPB) void k_residual64(int64_t nb, const int64_t *_
   }
 }
 
+// the residual of u = 0: r = f, out = (|f|^2, |f|^2)
+__global__ __launch_bounds__(TPB) void k_residual_zero(int64_t n, const double *__restrict__ f,
+                                                       double *__restrict__ r, double *out,
+                                                       double *partials, unsigned *counter) {
+  double ff = 0.;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double v = f[i];
+    r[i] = v;
+    ff += v * v;
+  }
+  double vv[1] = {ff}, tot[1];
+  if (reduce_last<1>(vv, partials, counter, tot) && threadIdx.x == 0) {
+    out[0] = tot[0];
+    out[1] = tot[0];
+  }
+}
+
 __global__ void k_axpby64(int64_t n, double alpha, const double *__restrict__ x, double beta,
                           const double *__restrict__ y, double *__restrict__ z) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -772,11 +790,15 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
   bool conv = false;
   // a multigrid iteration costs several operator applications: short batches
   const int batch = prm.check_every > 0 ? prm.check_every : (multigrid ? 4 : 50);
-  auto true_res = [&]() -> int {
-    hipLaunchKernelGGL(k_residual64<BS>, dim3(bsr_grid(nb)), dim3(TPB), 0, c.stream, nb,
-                       c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(), c.d_fa.as<double>(),
-                       c.d_u.as<double>(), c.d_f.as<double>(), c.d_r.as<double>(), out.as<double>(),
-                       part, cnt);
+  auto true_res = [&](bool u_zero = false) -> int {
+    if (u_zero)  // the starting guess: no product needed
+      hipLaunchKernelGGL(k_residual_zero, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_f.as<double>(),
+                         c.d_r.as<double>(), out.as<double>(), part, cnt);
+    else
+      hipLaunchKernelGGL(k_residual64<BS>, dim3(bsr_grid(nb)), dim3(TPB), 0, c.stream, nb,
+                         c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(), c.d_fa.as<double>(),
+                         c.d_u.as<double>(), c.d_f.as<double>(), c.d_r.as<double>(), out.as<double>(),
+                         part, cnt);
     double h[2];
     PZG_HIP(hipMemcpyAsync(h, out.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
     PZG_HIP(hipStreamSynchronize(c.stream));
@@ -790,7 +812,7 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
     if (need > d) d = need;
     return d > 0.5 ? 0.5 : d;
   };
-  PZG_TRY(true_res());
+  PZG_TRY(true_res(true));
   if (rel <= prm.tol) conv = true;
   if (multigrid && !conv) {
     // levels and eigenvalue estimates (timed with the solve)
