@@ -1,5 +1,7 @@
 // pzg_mg.hip -- multigrid preconditioner of the mixed FCG path on hexahedral
-// meshes whose vertices form a lattice (p <= 2).
+// and quadrilateral meshes whose vertices form a lattice (p <= 2; a plane
+// lattice has one node in z, and every kernel treats that direction as the
+// identity).
 //
 // Levels.  Level 0 is the FCG system S = D^-1/2 K D^-1/2 (matrix-free or
 // stored, as the FCG iteration uses it).  Level 1 is the trilinear subspace:
@@ -734,15 +736,17 @@ template <int BS>
 int structure_impl(Ctx &c) {
   mg_free(c);
   // vertex lattice: the corner nodes of the volume elements and their blocks
-  if (!c.has_mesh || c.dim != 3 || c.vol_etype != PZG_HEX || c.hp_simplex || c.has_dep || c.external_csr) {
-    set_error("multigrid preconditioner: needs a hexahedral mesh-built system without constrained connects");
+  const bool hex = c.dim == 3 && c.vol_etype == PZG_HEX, quad = c.dim == 2 && c.vol_etype == PZG_QUAD;
+  if (!c.has_mesh || !(hex || quad) || c.hp_simplex || c.has_dep || c.external_csr) {
+    set_error("multigrid preconditioner: needs a hexahedral or quadrilateral mesh-built system without "
+              "constrained connects");
     return PZG_EUNSUP;
   }
   const int nc = c.vol_ncorner, nsd = c.vol_nsides;
   std::vector<int64_t> nblk(c.nnodes, -1);
   int64_t nvert = 0;
   for (int64_t e = 0; e < c.nvol; e++)
-    for (int k = 0; k < 8; k++) {
+    for (int k = 0; k < nc; k++) {  // sides 0 .. nc-1: the corners
       const int64_t node = c.vol_nodes[e * nc + k];
       const int64_t b = c.aidx[c.vol_conn[e * nsd + k]];
       if (b < 0 || c.blk_size[c.vol_conn[e * nsd + k]] != BS) {
@@ -803,11 +807,12 @@ int structure_impl(Ctx &c) {
   for (;;) {
     const MgLevel &f = c.mg.back();
     if (f.nn * BS <= MG_DENSE) break;
+    // a direction of one node (the z of a plane lattice) stays as it is
     bool even = true;
-    for (int d = 0; d < 3; d++) even &= (f.n[d] - 1) % 2 == 0 && f.n[d] >= 3;
+    for (int d = 0; d < 3; d++) even &= f.n[d] == 1 || ((f.n[d] - 1) % 2 == 0 && f.n[d] >= 3);
     if (!even) break;
     MgLevel l;
-    for (int d = 0; d < 3; d++) l.n[d] = (f.n[d] - 1) / 2 + 1;
+    for (int d = 0; d < 3; d++) l.n[d] = f.n[d] == 1 ? 1 : (f.n[d] - 1) / 2 + 1;
     l.nn = (int64_t)l.n[0] * l.n[1] * l.n[2];
     c.mg.push_back(std::move(l));
   }

@@ -28,8 +28,9 @@ def full_from_upper(ia, ja, a, n):
 def vertex_lattice(md, bs):
     """Equation of each lattice node's first component (x fastest) and the
     node counts per direction."""
-    vn = np.asarray(md["vol_nodes"]).reshape(len(md["vol_nodes"]), -1)[:, :8]
-    vc = np.asarray(md["vol_conn"]).reshape(len(md["vol_conn"]), -1)[:, :8]
+    vn = np.asarray(md["vol_nodes"]).reshape(len(md["vol_nodes"]), -1)
+    nc = vn.shape[1]  # corners (sides 0 .. nc-1)
+    vc = np.asarray(md["vol_conn"]).reshape(len(md["vol_conn"]), -1)[:, :nc]
     xyz = np.asarray(md["xyz"]).reshape(-1, 3)
     nodes, first = np.unique(vn.ravel(), return_index=True)
     eq = np.asarray(md["blk_pos"])[vc.ravel()[first]]
@@ -84,12 +85,12 @@ class Model:
         self.levels = [(A1, list(dims))]
         while True:
             A, dm = self.levels[-1]
-            if A.shape[0] <= DENSE or any((x - 1) % 2 or x < 3 for x in dm):
+            if A.shape[0] <= DENSE or any(x != 1 and ((x - 1) % 2 or x < 3) for x in dm):
                 break
             Px, Py, Pz = (prolong_1d(x) for x in dm)
             P = sp.kron(sp.kron(Pz, Py), Px)
             P = sp.kron(P, sp.identity(bs)).tocsr()
-            self.levels.append(((P.T @ A @ P).tocsr(), [(x - 1) // 2 + 1 for x in dm]))
+            self.levels.append(((P.T @ A @ P).tocsr(), [1 if x == 1 else (x - 1) // 2 + 1 for x in dm]))
             self.levels[-2] = self.levels[-2] + (P,)
         self.M0 = block_diag_inv(self.S, bs)
         self.Ml = [block_diag_inv(A, bs) for A, *_ in self.levels]

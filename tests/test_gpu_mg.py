@@ -13,6 +13,8 @@ Bars (written here, checked below):
     direct solve of the oracle's system (the bars of the block-Jacobi mixed
     solves), in fewer iterations than block Jacobi;
   * refusals: tetrahedra, a lattice that does not halve to a dense level.
+Cases: hexahedra (p = 1, 2; Poisson, Darcy, elasticity) and quadrilaterals
+(a plane lattice: one node in z).
 """
 import ctypes as C
 
@@ -47,6 +49,8 @@ CASES = {
                                    el_coef=np.exp(np.random.default_rng(5).standard_normal(512)),
                                    force_kind=P.FORCE_CONST, force_const=(1.0, 0, 0), bc_type=[0] * 6),
     "elast4_p2": lambda: P.elasticity3d(n=4, p=2, seed=1, field_kind="iid", load="traction"),
+    "poisson2d16_p2": lambda: P.poisson2d(n=16, p=2, seed=1),
+    "darcy2d32_p1": lambda: P.darcy2d(n=32, p=1, seed=2),
 }
 
 _MODELS = {}
