@@ -139,9 +139,10 @@ typedef struct {
 #define PZG_PRECOND_JACOBI 0       /* point Jacobi                              */
 #define PZG_PRECOND_BLOCK_JACOBI 1 /* node-block (nstate x nstate) Jacobi       */
 #define PZG_PRECOND_LEARNED 2      /* learned message-passing preconditioner    */
-#define PZG_PRECOND_MULTIGRID 3    /* V-cycle: p = 2 -> trilinear -> halved
-                                      lattices, Chebyshev / block-Jacobi
-                                      smoothing (hexahedral lattice meshes)     */
+#define PZG_PRECOND_MULTIGRID 3    /* V-cycle: p = 2 -> vertex functions ->
+                                      halved lattices, Chebyshev / block-Jacobi
+                                      smoothing (hexahedral or quadrilateral
+                                      meshes on a lattice)                      */
 
 typedef struct {
   int32_t solver;        /* PZG_SOLVER_*                                       */
