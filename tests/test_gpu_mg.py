@@ -167,3 +167,33 @@ def test_coefficient_change_rebuilds_the_preconditioner(pre):
     x = spla.spsolve(K.tocsc(), rhs)
     assert np.linalg.norm(rhs - K @ u) <= 2e-8 * np.linalg.norm(rhs)
     assert np.linalg.norm(u - x) <= 1e-5 * np.linalg.norm(x)
+
+
+def test_hp_hexahedra_stored_operator():
+    """p_e in {1, 2} on hexahedra (several integration rules: the stored FP32
+    operator, no matrix-free one): the vertex functions are still the
+    trilinear subspace; the V-cycle matches the model and the mixed solve
+    meets the bars."""
+    pb = P.elasticity3d(n=4, p=2, seed=8, field_kind="grf", load="body")
+    pb.el_porder = np.random.default_rng(9).integers(1, 3, pb.nvol).astype(np.int32)
+    print("[gpu] multigrid hp hexahedra", flush=True)
+    o = O.Oracle(pb)
+    ia, ja = o.pattern()
+    a, rhs = o.assemble()
+    K = G.full_from_upper(ia, ja, a, o.neq)
+    an = TPZLinearAnalysis(pb, False, ctx=ctx())
+    m = G.Model(K, an.mesh, pb.nstate)
+    an.Assemble()
+    r = np.random.default_rng(7).standard_normal(o.neq)
+    z, dims, lmax = _mg_apply(L.INNER_BSR, r)
+    assert dims == m.dims()
+    zm = m.apply(r, lmax)
+    assert np.linalg.norm(z - zm) <= 2e-4 * np.linalg.norm(zm)
+    st = TPZStepSolver()
+    st.SetMixedFCG(200000, 1e-8, precond=L.PRECOND_MULTIGRID)
+    an.SetSolver(st)
+    u = an.Solve()
+    assert st.info["converged"] and st.info["true_residual"] <= 1e-8
+    x = spla.spsolve(K.tocsc(), rhs)
+    assert np.linalg.norm(rhs - K @ u) <= 2e-8 * np.linalg.norm(rhs)
+    assert np.linalg.norm(u - x) <= 1e-5 * np.linalg.norm(x)
