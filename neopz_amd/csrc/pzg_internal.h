@@ -290,6 +290,7 @@ struct Ctx {
   // vertex equations, the dense inverse of the coarsest level
   bool mg_ready = false, mg_use_mf = false;
   bool mg_struct = false;  // lattice, levels and buffers of the current mesh
+  bool mg_dense = true;    // the coarsest level has a dense inverse (else a Chebyshev solve)
   int mg_degree = 2;       // Chebyshev degree of the pre- and post-smoothing
   double mg_range = 30.;   // ratio of the smoothed interval's ends
   double mg_lmax0 = 0.;    // largest eigenvalue of the block-Jacobi scaled S

@@ -15,7 +15,10 @@
 // stencil.  Levels 2.. halve the lattice with trilinear prolongation and the
 // Galerkin product A_c = P^T A P formed on the device, while every element
 // count is even and the level is too large for the dense solve; the coarsest
-// level (at most MG_MAXDENSE unknowns) is solved with a dense inverse.
+// level is solved with a dense inverse when it has at most MG_MAXDENSE
+// unknowns; a larger one (an odd element count stopped the halving) gets a
+// Chebyshev iteration of degree MG_COARSE_DEGREE from zero instead, a fixed
+// SPD polynomial in M^-1 A, so the cycle stays a fixed SPD operator.
 //
 // Smoother: Chebyshev iteration (Saad, Iterative Methods for Sparse Linear
 // Systems, 2nd ed., Alg. 12.1) of degree m on M^-1 A, M = the diagonal
@@ -44,6 +47,8 @@ namespace {
 constexpr int TPB = RED_TPB;
 constexpr int64_t MG_DENSE = 128;      // coarsening stops at this many unknowns
 constexpr int64_t MG_MAXDENSE = 1536;  // largest dense coarsest solve
+constexpr int MG_COARSE_DEGREE = 8;    // Chebyshev degree of a coarsest level beyond it
+constexpr double MG_COARSE_RANGE = 100.;
 constexpr int MG_LANCZOS_IT = 20;      // CG-Lanczos steps per eigenvalue estimate
 constexpr int MG_LATTICE_DEGREE = 1;   // Chebyshev degree on the lattice levels
 
@@ -816,10 +821,7 @@ int structure_impl(Ctx &c) {
     l.nn = (int64_t)l.n[0] * l.n[1] * l.n[2];
     c.mg.push_back(std::move(l));
   }
-  if (c.mg.back().nn * BS > MG_MAXDENSE) {
-    set_error("multigrid preconditioner: the lattice does not coarsen to a dense level (element counts must halve)");
-    return PZG_EUNSUP;
-  }
+  c.mg_dense = c.mg.back().nn * BS <= MG_MAXDENSE;
   // device arrays
   PZG_TRY(dev_upload(c.d_mg_vmap, vmap.data(), vmap.size(), c.stream));
   PZG_TRY(dev_alloc(c.d_mg_dsq, sizeof(float) * nn1 * BS));
@@ -857,7 +859,7 @@ int setup_impl(Ctx &c) {
     PZG_HIP(hipGetLastError());
   }
   // dense inverse of the coarsest level
-  {
+  if (c.mg_dense) {
     const MgLevel &l = c.mg.back();
     const int nd = (int)(l.nn * BS);
     std::vector<float> h((size_t)27 * BS * BS * l.nn);
@@ -894,7 +896,7 @@ int setup_impl(Ctx &c) {
                              c.d_mg_d0.as<float>(), c.d_mg_t0.as<float>(), c.d_mg_w0.as<float>(), op,
                              c.mg_lmax0));
   }
-  for (size_t k = 0; k + 1 < c.mg.size(); k++) {
+  for (size_t k = 0; k + (c.mg_dense ? 1 : 0) < c.mg.size(); k++) {
     MgLevel &l = c.mg[k];
     auto op = [&](const float *x, float *y) { return op_lattice<BS>(c, l, x, y, nullptr); };
     PZG_TRY(lanczos_lmax<BS>(c, l.nn, l.minv.as<float>(), 1, l.nn, l.x.as<float>(), l.b.as<float>(),
@@ -926,9 +928,17 @@ int apply_impl(Ctx &c, const float *r, float *z, float *p, const float *q, Solve
                        dims_of(c.mg[k + 1]), dims_of(l), V.r, V.t, c.mg[k + 1].b.as<float>(), sc);
     PZG_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_dense, dim3(gridx(c.mg_ncoarse * 64)), dim3(TPB), 0, c.stream, (int)c.mg_ncoarse,
-                     c.d_mg_cinv.as<float>(), c.mg.back().b.as<float>(), c.mg.back().x.as<float>(), sc);
-  PZG_HIP(hipGetLastError());
+  if (c.mg_dense) {
+    hipLaunchKernelGGL(k_dense, dim3(gridx(c.mg_ncoarse * 64)), dim3(TPB), 0, c.stream, (int)c.mg_ncoarse,
+                       c.d_mg_cinv.as<float>(), c.mg.back().b.as<float>(), c.mg.back().x.as<float>(), sc);
+    PZG_HIP(hipGetLastError());
+  } else {  // x = q(M^-1 A) M^-1 b, a Chebyshev iteration from zero
+    MgLevel &l = c.mg.back();
+    LevelView V = view_l(c, l);
+    V.ch = cheb_coeffs(l.lmax, MG_COARSE_RANGE, MG_COARSE_DEGREE);
+    auto op = [&](const float *x, float *y, bool) { return op_lattice<BS>(c, l, x, y, sc); };
+    PZG_TRY(smooth_pre<BS>(c, V, l.b.as<float>(), l.x.as<float>(), op, sc));
+  }
   for (int k = nl - 2; k >= 0; k--) {
     MgLevel &l = c.mg[k];
     hipLaunchKernelGGL(k_prolong<BS>, dim3(grid1(l.nn)), dim3(TPB), 0, c.stream, dims_of(c.mg[k + 1]),

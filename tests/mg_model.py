@@ -17,6 +17,7 @@ import numpy as np
 import scipy.sparse as sp
 
 DENSE = 128
+MAXDENSE = 1536  # beyond: the coarsest level gets a degree-8 Chebyshev iteration (range 100)
 
 
 def full_from_upper(ia, ja, a, n):
@@ -94,7 +95,8 @@ class Model:
             self.levels[-2] = self.levels[-2] + (P,)
         self.M0 = block_diag_inv(self.S, bs)
         self.Ml = [block_diag_inv(A, bs) for A, *_ in self.levels]
-        self.coarse_inv = np.linalg.inv(self.levels[-1][0].toarray())
+        Ac = self.levels[-1][0]
+        self.coarse_inv = np.linalg.inv(Ac.toarray()) if Ac.shape[0] <= MAXDENSE else None
         self.n = n
 
     def dims(self):
@@ -152,7 +154,11 @@ class Model:
             A, _, P = self.levels[k]
             x[k], rk = self.pre(A, self.Ml[k], b[k], cos[k])
             b[k + 1] = P.T @ rk
-        x[L - 1] = self.coarse_inv @ b[L - 1]
+        if self.coarse_inv is not None:
+            x[L - 1] = self.coarse_inv @ b[L - 1]
+        else:
+            x[L - 1] = self.pre(self.levels[-1][0], self.Ml[-1], b[L - 1],
+                                self.cheb(lmax[L], 100.0, 8))[0]
         for k in range(L - 2, -1, -1):
             A, _, P = self.levels[k]
             x[k] = x[k] + P @ x[k + 1]

@@ -12,7 +12,9 @@ Bars (written here, checked below):
     residual of the solution <= 2e-8 and the solution within 1e-5 of a
     direct solve of the oracle's system (the bars of the block-Jacobi mixed
     solves), in fewer iterations than block Jacobi;
-  * refusals: tetrahedra, a lattice that does not halve to a dense level.
+  * refusals: tetrahedra.  Lattices whose halving stops above the dense
+    limit (odd element counts) get a Chebyshev coarsest solve, checked the
+    same way.
 Cases: hexahedra (p = 1, 2; Poisson, Darcy, elasticity) and quadrilaterals
 (a plane lattice: one node in z).
 """
@@ -50,6 +52,13 @@ CASES = {
                                    force_kind=P.FORCE_CONST, force_const=(1.0, 0, 0), bc_type=[0] * 6),
     "elast4_p2": lambda: P.elasticity3d(n=4, p=2, seed=1, field_kind="iid", load="traction"),
     "poisson2d16_p2": lambda: P.poisson2d(n=16, p=2, seed=1),
+    # odd element counts stop the halving above the dense limit: the coarsest
+    # level is a degree-8 Chebyshev iteration
+    "poisson11_p2": lambda: P.Problem(material=P.POISSON, dim=3, p=2, nx=11, ny=11, nz=11,
+                                      force_kind=P.FORCE_CONST, force_const=(1.0, 0, 0), bc_type=[0] * 6),
+    "darcy30_p1": lambda: P.Problem(material=P.DARCY, dim=3, p=1, nx=30, ny=30, nz=30,
+                                    el_coef=np.exp(np.random.default_rng(6).standard_normal(27000)),
+                                    force_kind=P.FORCE_CONST, force_const=(1.0, 0, 0), bc_type=[0] * 6),
     "darcy2d32_p1": lambda: P.darcy2d(n=32, p=1, seed=2),
 }
 
@@ -128,8 +137,7 @@ def test_mixed_solve_with_multigrid(name):
 
 
 def test_multigrid_refusals():
-    for pb in (P.darcy3d_tet(n=2, pmax=2, seed=1, hp=False),
-               P.Problem(material=P.POISSON, dim=3, p=2, nx=11, ny=11, nz=11, bc_type=[0] * 6)):
+    for pb in (P.darcy3d_tet(n=2, pmax=2, seed=1, hp=False),):
         an = TPZLinearAnalysis(pb, False, ctx=ctx())
         an.Assemble()
         r = np.ones(an.mesh["neq"])
