@@ -54,8 +54,30 @@ extern "C" {
 #define PZG_FORCE_CONST 1
 #define PZG_FORCE_BUMPS 2  /* f(x) = sum_k a_k exp(-|x-c_k|^2/(2 w^2)), rows (cx,cy,cz,a0,a1,a2) */
 
+/* TPZBndCondT types (TPZBndCondT::Type), per material:
+ *   TPZMatPoisson::ContributeBC   (Material/Poisson/TPZMatPoisson.cpp:45-121):
+ *     0 Dirichlet (penalty fBigNumber), 1 Neumann (Val2 * fScale),
+ *     2 Robin as written there: penalty * Val2 load, penalty * Val1(0,0) times
+ *       the first axis derivative (dphix row 0) of the boundary element;
+ *   TPZDarcyFlow::ContributeBC    (Material/DarcyFlow/TPZDarcyFlow.cpp:49-91):
+ *     0 Dirichlet, 1 Neumann, 2 Robin (Val1(0,0) mass term + Val2 load);
+ *   TPZElasticity3D::ContributeBC (Material/Elasticity/TPZElasticity3D.cpp:
+ *     616-773, penalty 1e12): 0 Dirichlet, 1 Neumann, 2 mixed (Val1 3x3 mass
+ *     coupling + Val2 load), 3 directional null Dirichlet (penalty * Val2[k]
+ *     on component k), 4 stress field (load -Val1 . n, n the outward normal of
+ *     TPZInterpolationSpace::ComputeNormal), 5/6/7 Dirichlet on x/y/z only,
+ *     8 Dirichlet on x and z.
+ * Any other type is refused by pzg_mesh_set with PZG_EUNSUP (the reference
+ * prints an error there and adds nothing, or stops). */
 #define PZG_BC_DIRICHLET 0
 #define PZG_BC_NEUMANN 1
+#define PZG_BC_MIXED 2
+#define PZG_BC_DIR_NULL 3
+#define PZG_BC_STRESS 4
+#define PZG_BC_DIR_X 5
+#define PZG_BC_DIR_Y 6
+#define PZG_BC_DIR_Z 7
+#define PZG_BC_DIR_XZ 8
 
 typedef struct pzg_ctx pzg_ctx;
 
@@ -85,6 +107,7 @@ typedef struct {
   int32_t nbcid;
   const int32_t *bc_type;   /* nbcid: PZG_BC_*                                */
   const double *bc_val;     /* 3 * nbcid: TPZBndCondT::Val2                   */
+  /* (TPZBndCondT::Val1: bc_val1 at the end of this struct)                   */
   /* connects by sequence number: TPZCompMesh::Block() (Matrix/pzblock.h)     */
   int64_t nconn;
   const int64_t *blk_pos;
@@ -125,6 +148,12 @@ typedef struct {
   const int64_t *dep_ptr;   /* nconn + 1, or NULL */
   const int64_t *dep_conn;
   const double *dep_coef;
+  /* TPZBndCondT::Val1 per bc id, 9 * nbcid row-major (nstate x nstate used:
+   * Val1(0,0) for Poisson/Darcy, the 3x3 matrix for Elasticity3D); read by
+   * types 2 (all materials) and 4 (Elasticity3D).  NULL = zero.  Type 4 also
+   * needs each such boundary element on exactly one volume element (the
+   * neighbour that orients the normal; an interior face is PZG_EUNSUP). */
+  const double *bc_val1;
 } pzg_mesh_desc;
 
 /* ---- solver selection -------------------------------------------------- */
@@ -195,6 +224,24 @@ int pzg_mesh_set(pzg_ctx *ctx, const pzg_mesh_desc *mesh);
  * (StrMatrix/TPZSSpStructMatrix.cpp:30-193) with
  * TPZRenumbering::ConvertGraph (External/TPZRenumbering.cpp:76-110). */
 int pzg_struct_create(pzg_ctx *ctx, int64_t *neq, int64_t *nnz);
+
+/* Equation filter of the struct matrix: TPZEquationFilter::SetActiveEquations
+ * (StrMatrix/TPZEquationFilter.h; active sorted, duplicates removed) on
+ * TPZStructMatrix::EquationFilter(), or Reset() when active is NULL or
+ * nactive < 0.  Call after pzg_mesh_set (a new mesh resets the filter) and
+ * before pzg_struct_create, which then builds the condensed pattern of
+ * SetupMatrixData with the filter (StrMatrix/TPZSSpStructMatrix.cpp:49-193;
+ * neq = the number of active equations) and pzg_assemble the condensed
+ * system (TPZStructMatrixOR::Assemble filtering every element's destinations,
+ * StrMatrix/pzstrmatrixor.cpp:41-71,226-233).  With a filter, every vector of
+ * the matrix-level entry points (pzg_get_system's rhs, pzg_multadd,
+ * pzg_solve, pzg_add_kel_batch destinations) is in the condensed numbering;
+ * the post-processing entry points (pzg_load_solution, pzg_get_solution,
+ * pzg_evaluate_error) keep the mesh numbering, inactive equations 0
+ * (TPZEquationFilter::Scatter, as TPZLinearAnalysis::SolveT loads it).
+ * The mixed solver then uses the stored FP32 operator, and pzg_full_csr
+ * returns PZG_EUNSUP. */
+int pzg_set_equation_filter(pzg_ctx *ctx, int64_t nactive, const int64_t *active);
 
 /* Copies the pattern to host arrays ia[neq+1], ja[nnz] (TPZSYsmpMatrix::IA/JA,
  * Matrix/pzsysmp.h:104-127). */

@@ -25,7 +25,7 @@ EXPORTS = [
     "pzg_learned_train", "pzg_learned_weights", "pzg_learned_graph", "pzg_learned_samples",
     "pzg_learned_loss_grad", "pzg_scaled_apply", "pzg_mg_apply", "pzg_full_csr", "pzg_hp_tuning",
     "pzg_load_solution", "pzg_error_points", "pzg_evaluate_error", "pzg_sloan_renumber",
-    "pzg_get_solution",
+    "pzg_get_solution", "pzg_set_equation_filter",
 ]
 
 dp = C.POINTER(C.c_double)
@@ -49,6 +49,7 @@ class MeshDesc(C.Structure):
         ("big_number", C.c_double),
         ("vol_rule", i32p), ("bc_rule", i32p), ("vol_etype", C.c_int32),
         ("dep_ptr", i64p), ("dep_conn", i64p), ("dep_coef", dp),
+        ("bc_val1", dp),
     ]
 
 
@@ -137,6 +138,7 @@ def load():
     L.pzg_error_points.argtypes = [vp, C.c_int32, i64p, dp]
     L.pzg_evaluate_error.argtypes = [vp, C.POINTER(Exact), dp, dp, i32p]
     L.pzg_sloan_renumber.argtypes = [C.c_int64, C.c_int64, i64p, i64p, i64p]
+    L.pzg_set_equation_filter.argtypes = [vp, C.c_int64, i64p]
     _lib = L
     return L
 
@@ -184,6 +186,8 @@ def make_mesh_desc(m):
     d.nbcid = len(bt)
     d.bc_type = ptr(bt, i32p)
     d.bc_val = ptr(arr(m["bc_val"], np.float64), dp)
+    if m.get("bc_val1") is not None:
+        d.bc_val1 = ptr(arr(m["bc_val1"], np.float64), dp)
     d.nconn = m["nconn"]
     d.blk_pos = ptr(arr(m["blk_pos"], np.int64), i64p)
     d.blk_size = ptr(arr(m["blk_size"], np.int32), i32p)

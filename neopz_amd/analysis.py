@@ -87,7 +87,9 @@ class TPZSYsmpMatrixHIP:
         ia = np.zeros(self.neq + 1, np.int64)
         ja = np.zeros(max(s["nnz_upper"], 1), np.int64)
         L.check(self.ctx.lib.pzg_struct_get_pattern(self.ctx.h, L.ptr(ia, L.i64p), L.ptr(ja, L.i64p)))
-        self._ia, self._ja = ia, ja[: s["nnz_upper"]]
+        # with an equation filter the condensed pattern is shorter than the
+        # device's mesh-numbered one
+        self._ia, self._ja = ia, ja[: ia[-1]]
 
     def A(self):
         a = np.zeros(max(len(self.JA()), 1))
@@ -272,6 +274,48 @@ class TPZStepSolver:
         return u
 
 
+class TPZEquationFilter:
+    """TPZEquationFilter (StrMatrix/TPZEquationFilter.h) as held by a struct
+    matrix: SetActiveEquations / SetMinMaxEq / Reset, applied by the GPU
+    struct matrix at Create()."""
+
+    def __init__(self, neq=0):
+        self.fNumEq = neq
+        self.fActiveEqs = None
+
+    def SetNumEq(self, neq):
+        self.fNumEq = neq
+
+    def SetActiveEquations(self, active):
+        if self.fActiveEqs is not None:
+            raise RuntimeError("TPZEquationFilter::SetActiveEquations: call Reset first")
+        self.fActiveEqs = np.unique(np.asarray(active, dtype=np.int64))
+
+    def SetMinMaxEq(self, mineq, maxeq):
+        if not (0 <= mineq <= maxeq <= self.fNumEq):
+            raise ValueError("TPZEquationFilter::SetMinMaxEq: bad range")
+        self.SetActiveEquations(np.arange(mineq, maxeq))
+
+    def Reset(self):
+        self.fActiveEqs = None
+
+    def IsActive(self):
+        return self.fActiveEqs is not None
+
+    def NActiveEquations(self):
+        return len(self.fActiveEqs) if self.IsActive() else self.fNumEq
+
+    def Scatter(self, small):
+        if not self.IsActive():
+            return np.array(small, dtype=np.float64)
+        big = np.zeros(self.fNumEq)
+        big[self.fActiveEqs] = small
+        return big
+
+    def Gather(self, big):
+        return np.asarray(big)[self.fActiveEqs] if self.IsActive() else np.asarray(big)
+
+
 class TPZSSpStructMatrixHIP:
     """GPU TPZStructMatrix: Create() -> pattern, Assemble() -> values + rhs."""
 
@@ -280,14 +324,31 @@ class TPZSSpStructMatrixHIP:
         self.mesh = mesh_dict
         self._desc, self._keep = L.make_mesh_desc(mesh_dict)
         self._flattened = False
+        self._filter = TPZEquationFilter(mesh_dict["neq"])
 
     def Clone(self):
-        return TPZSSpStructMatrixHIP(self.ctx, self.mesh)
+        c = type(self)(self.ctx, self.mesh)
+        if self._filter.IsActive():
+            c._filter.SetActiveEquations(self._filter.fActiveEqs)
+        return c
+
+    def EquationFilter(self):
+        """TPZStructMatrix::EquationFilter (StrMatrix/TPZStructMatrix.h)."""
+        return self._filter
+
+    def NReducedEquations(self):
+        return self._filter.NActiveEquations()
 
     def _flatten(self):
         if not self._flattened:
             L.check(self.ctx.lib.pzg_mesh_set(self.ctx.h, C.byref(self._desc)))
             self._flattened = True
+        f = self._filter
+        if f.IsActive():
+            act = np.ascontiguousarray(f.fActiveEqs, dtype=np.int64)
+            L.check(self.ctx.lib.pzg_set_equation_filter(self.ctx.h, len(act), L.ptr(act, L.i64p)))
+        else:
+            L.check(self.ctx.lib.pzg_set_equation_filter(self.ctx.h, -1, None))
 
     def Create(self):
         """TPZSSpStructMatrix::Create (StrMatrix/TPZSSpStructMatrix.cpp:30-47)."""
@@ -310,9 +371,6 @@ class TPZSSpStructMatrixHIP:
 class TPZSpStructMatrixHIP(TPZSSpStructMatrixHIP):
     """GPU TPZSpStructMatrix (StrMatrix/TPZSpStructMatrix.cpp:33-190): the same
     assembly, the matrix handed out in the non-symmetric TPZFYsmpMatrix format."""
-
-    def Clone(self):
-        return TPZSpStructMatrixHIP(self.ctx, self.mesh)
 
     def Create(self):
         K = super().Create()
@@ -396,15 +454,21 @@ class TPZLinearAnalysis:
             self.fSolver.SetMatrix(self.K)
 
     def Rhs(self):
+        """fRhs: the load in the mesh numbering (with an equation filter the
+        condensed load scattered back, StrMatrix/pzstrmatrixor.cpp:63)."""
         r = np.zeros(self.K.neq)
         L.check(self.ctx.lib.pzg_get_system(self.ctx.h, None, L.ptr(r, L.dp)))
-        return r
+        return self.fStructMatrix.EquationFilter().Scatter(r)
 
     def Solve(self, want_solution=True):
+        """TPZLinearAnalysis::SolveT (Analysis/TPZLinearAnalysis.cpp:130-227):
+        with an equation filter the condensed solution is scattered into the
+        mesh numbering (inactive equations 0) before LoadSolution."""
         if self.fSolver is None:
             raise ValueError("TPZLinearAnalysis::Solve: no solver")
         self.fSolver.SetMatrix(self.K)
-        self.fSolution = self.fSolver.Solve(None, want_solution=want_solution)
+        u = self.fSolver.Solve(None, want_solution=want_solution)
+        self.fSolution = None if u is None else self.fStructMatrix.EquationFilter().Scatter(u)
         return self.fSolution
 
     def Solution(self):

@@ -136,6 +136,8 @@ struct ElemArgs {
   const int32_t *bcid;
   const int32_t *bctype;
   const double *bcval;
+  const double *bccoef;  // BCC coefficients per bc id
+  const double *bcvec;   // per boundary element: neighbour-to-element centre vector
   int nbumps;
   double bw;
   double fconst[3];
@@ -373,13 +375,21 @@ __global__ __launch_bounds__(TPB) void k_vol(ElemArgs g, const int32_t *__restri
   }
 }
 
-// Boundary elements (dimension DIMB = DIM-1 embedded in 3D).
+// Boundary elements (dimension DIMB = DIM-1 embedded in 3D).  The material's
+// ContributeBC (Material/Poisson/TPZMatPoisson.cpp:45-121, Material/DarcyFlow/
+// TPZDarcyFlow.cpp:49-91, Material/Elasticity/TPZElasticity3D.cpp:616-773)
+// enters as the coefficients of the bc id (BCC layout): ek(I,J) =
+// ck[ci][cj] M_ij + cd D_ij, ef(I) = cf[ci] m_i (+ the stress-field load).
 template <int DIMB, int NC, int NSD, int NSH, int NS, bool EK = false>
 __global__ __launch_bounds__(TPB) void k_bc(ElemArgs g, const int32_t *__restrict__ list) {
   constexpr int NE = NSH * NS;
   __shared__ double s_coord[NC * 3];
   __shared__ double s_wdet[MAXQ];
+  __shared__ double s_ia0[MAXQ];   // 1 / |dx/dxi_0|: dphix row 0 = dphi row 0 * s_ia0
+  __shared__ double s_g[MAXQ][3];  // stress-field load -val1 . n at the points
   __shared__ double s_M[NSH][NSH];
+  __shared__ double s_D[NSH][NSH];
+  __shared__ double s_cf[BCC];
   __shared__ int64_t s_row[NE];
   __shared__ int64_t s_blk[NSH];
   __shared__ int32_t s_map[NSD * NSD];
@@ -388,35 +398,65 @@ __global__ __launch_bounds__(TPB) void k_bc(ElemArgs g, const int32_t *__restric
   const ElemTables &T = *g.tab;
   const int nq = T.nq;
   const int id = g.bcid[e];
-  const int type = g.bctype[id];
-  const double *val = g.bcval + 3 * id;
   if (tid < NC * 3) s_coord[tid] = g.xyz[3 * g.nodes[e * NC + tid / 3] + tid % 3];
+  if (tid < BCC) s_cf[tid] = g.bccoef[(int64_t)id * BCC + tid];
   if (!EK)
     for (int k = tid; k < NSD * NSD; k += TPB) s_map[k] = g.map[e * NSD * NSD + k];
   if (tid < NSH) s_blk[tid] = (EK ? g.aidx_ek : g.aidx)[g.conn[e * NSD + T.shape_side[tid]]];
   __syncthreads();
+  const bool stress = s_cf[BCC_STRESS] != 0.;
+  const double cd = s_cf[BCC_CD];
   if (tid < nq) {
     const int q = tid;
     double gx[3][2] = {{0, 0}, {0, 0}, {0, 0}};
     for (int n = 0; n < NC; n++)
       for (int i = 0; i < 3; i++)
         for (int j = 0; j < DIMB; j++) gx[i][j] += s_coord[3 * n + i] * T.cdphi[q][j][n];
-    double det;
+    // TPZGeoEl::Jacobian of a line / face in 3D space (Gram-Schmidt axes)
+    double det, n1 = 0, a0[3], a1[3] = {0, 0, 0};
+    for (int i = 0; i < 3; i++) n1 += gx[i][0] * gx[i][0];
+    n1 = sqrt(n1);
+    for (int i = 0; i < 3; i++) a0[i] = gx[i][0] / n1;
     if (DIMB == 1) {
-      double n1 = 0;
-      for (int i = 0; i < 3; i++) n1 += gx[i][0] * gx[i][0];
-      det = sqrt(n1);
+      det = n1;
     } else {
-      double n1 = 0, d12 = 0, n2 = 0;
-      for (int i = 0; i < 3; i++) { n1 += gx[i][0] * gx[i][0]; d12 += gx[i][0] * gx[i][1]; }
-      n1 = sqrt(n1);
+      double d12 = 0, n2 = 0;
+      for (int i = 0; i < 3; i++) d12 += gx[i][0] * gx[i][1];
       for (int i = 0; i < 3; i++) {
-        double v2 = gx[i][1] - d12 * (gx[i][0] / n1) / n1;
-        n2 += v2 * v2;
+        a1[i] = gx[i][1] - d12 * a0[i] / n1;
+        n2 += a1[i] * a1[i];
       }
-      det = n1 * sqrt(n2);
+      n2 = sqrt(n2);
+      for (int i = 0; i < 3; i++) a1[i] /= n2;
+      det = n1 * n2;
     }
     s_wdet[q] = T.w[q] * fabs(det);
+    s_ia0[q] = 1.0 / n1;
+    if (stress) {
+      // TPZInterpolationSpace::ComputeNormal: unit normal from the axes,
+      // turned to point away from the volume neighbour
+      const double *v = g.bcvec + 3 * e;
+      double t[3];
+      if (DIMB == 1) {
+        const double b[3] = {a0[1] * v[2] - a0[2] * v[1], -a0[0] * v[2] + a0[2] * v[0],
+                             a0[0] * v[1] - a0[1] * v[0]};
+        t[0] = b[1] * a0[2] - b[2] * a0[1];
+        t[1] = -b[0] * a0[2] + b[2] * a0[0];
+        t[2] = b[0] * a0[1] - b[1] * a0[0];
+      } else {
+        t[0] = a0[1] * a1[2] - a0[2] * a1[1];
+        t[1] = -a0[0] * a1[2] + a0[2] * a1[0];
+        t[2] = a0[0] * a1[1] - a0[1] * a1[0];
+      }
+      const double sz = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+      double nrm[3], dot = 0.;
+      for (int i = 0; i < 3; i++) { nrm[i] = t[i] / sz; dot += nrm[i] * v[i]; }
+      if (dot < 0.)
+        for (int i = 0; i < 3; i++) nrm[i] = -nrm[i];
+      for (int k = 0; k < 3; k++)
+        s_g[q][k] = -(s_cf[BCC_VAL1 + 3 * k] * nrm[0] + s_cf[BCC_VAL1 + 3 * k + 1] * nrm[1] +
+                      s_cf[BCC_VAL1 + 3 * k + 2] * nrm[2]);
+    }
   }
   if (!EK)
     for (int k = tid; k < NE; k += TPB) {
@@ -426,60 +466,59 @@ __global__ __launch_bounds__(TPB) void k_bc(ElemArgs g, const int32_t *__restric
   __syncthreads();
   for (int k = tid; k < NSH * NSH; k += TPB) {
     int i = k / NSH, j = k % NSH;
-    double s = 0.;
+    double s = 0., sd = 0.;
     for (int q = 0; q < nq; q++) s += T.phi[q][i] * T.phi[q][j] * s_wdet[q];
+    if (cd != 0.)
+      for (int q = 0; q < nq; q++)
+        sd += T.dphi[q][0][i] * s_ia0[q] * T.dphi[q][0][j] * s_ia0[q] * s_wdet[q];
     s_M[i][j] = s;
+    s_D[i][j] = sd;
   }
   __syncthreads();
+  auto ek_entry = [&](int i, int ci, int j, int cj) {
+    return s_cf[BCC_CK + NS * ci + cj] * s_M[i][j] + (ci == cj ? cd * s_D[i][j] : 0.);
+  };
+  auto ef_entry = [&](int i, int ci) {
+    double sq = 0., st = 0.;
+    for (int q = 0; q < nq; q++) sq += T.phi[q][i] * s_wdet[q];
+    if (stress)
+      for (int q = 0; q < nq; q++) st += T.phi[q][i] * s_wdet[q] * s_g[q][ci];
+    return s_cf[BCC_CF + ci] * sq + st;
+  };
   if (EK) {
     // the whole element matrix and load vector of list entry blockIdx.x
     double *ek = g.ek_out + (int64_t)blockIdx.x * NE * NE;
     for (int k = tid; k < NE * NE; k += TPB) {
       const int I = k / NE, J = k % NE, i = I / NS, ci = I % NS, j = J / NS, cj = J % NS;
-      const bool on = type == PZG_BC_DIRICHLET && ci == cj && s_blk[i] != -1 && s_blk[j] != -1;
-      ek[k] = on ? g.big * s_M[i][j] : 0.;
+      ek[k] = (s_blk[i] != -1 && s_blk[j] != -1) ? ek_entry(i, ci, j, cj) : 0.;
     }
     for (int I = tid; I < NE; I += TPB) {
       const int i = I / NS, ci = I % NS;
-      double f = 0.;
-      if (s_blk[i] != -1) {
-        double sq = 0.;
-        for (int q = 0; q < nq; q++) sq += T.phi[q][i] * s_wdet[q];
-        if (type == PZG_BC_DIRICHLET) f = g.big * val[ci] * sq;
-        else f = val[ci] * sq * (g.material == PZG_MAT_POISSON ? g.scale : 1.0);
-      }
-      g.ef_out[(int64_t)blockIdx.x * NE + I] = f;
+      g.ef_out[(int64_t)blockIdx.x * NE + I] = s_blk[i] != -1 ? ef_entry(i, ci) : 0.;
     }
     return;
   }
-  if (type == PZG_BC_DIRICHLET) {
-    for (int k = tid; k < NE * NE; k += TPB) {
-      int I = k / NE, J = k % NE;
-      int i = I / NS, ci = I % NS, j = J / NS, cj = J % NS;
-      if (ci != cj) continue;  // the penalty couples equal components only
-      int64_t bi = s_blk[i], bj = s_blk[j];
-      if (bi < 0 || bj < 0) continue;
-      if (bi > bj || (bi == bj && i > j)) continue;
-      double v = g.big * s_M[i][j];
-      if (fabs(v) < 1.e-12) continue;
-      int64_t slot;
-      if (bi == bj) slot = s_row[I];
-      else {
-        int t = s_map[T.shape_side[i] * NSD + T.shape_side[j]];
-        slot = s_row[I] + g.bs + (int64_t)t * g.bs;
-      }
-      g.a[slot] += v;
+  for (int k = tid; k < NE * NE; k += TPB) {
+    int I = k / NE, J = k % NE;
+    int i = I / NS, ci = I % NS, j = J / NS, cj = J % NS;
+    int64_t bi = s_blk[i], bj = s_blk[j];
+    if (bi < 0 || bj < 0) continue;
+    // upper part of the equation numbering: block order, then component
+    if (bi > bj || (bi == bj && (i != j || cj < ci))) continue;
+    double v = ek_entry(i, ci, j, cj);
+    if (fabs(v) < 1.e-12) continue;  // AddKel's drop rule
+    int64_t slot;
+    if (bi == bj) slot = s_row[I] + (cj - ci);
+    else {
+      int t = s_map[T.shape_side[i] * NSD + T.shape_side[j]];
+      slot = s_row[I] + (g.bs - ci) + (int64_t)t * g.bs + cj;
     }
+    g.a[slot] += v;
   }
   for (int I = tid; I < NE; I += TPB) {
     int i = I / NS, ci = I % NS;
     if (s_blk[i] < 0) continue;
-    double s = 0.;
-    for (int q = 0; q < nq; q++) s += T.phi[q][i] * s_wdet[q];
-    double f;
-    if (type == PZG_BC_DIRICHLET) f = g.big * val[ci] * s;
-    else f = val[ci] * s * (g.material == PZG_MAT_POISSON ? g.scale : 1.0);
-    g.rhs[s_blk[i] * g.bs + ci] += f;
+    g.rhs[s_blk[i] * g.bs + ci] += ef_entry(i, ci);
   }
 }
 
@@ -668,8 +707,9 @@ __global__ __launch_bounds__(TPB) void k_bc_hp(ElemArgs g, int nsv, const HpTabl
   const int64_t e = list[blockIdx.x];
   const HpTables &T = *tab;
   const int id = g.bcid[e];
-  const int type = g.bctype[id];
-  const double val = g.bcval[3 * id];
+  // scalar materials: ek = ck M, ef = cf m (TPZMatPoisson type 2 is refused
+  // on these meshes by pzg_mesh_set)
+  const double ck = g.bccoef[(int64_t)id * BCC + BCC_CK], cfl = g.bccoef[(int64_t)id * BCC + BCC_CF];
   for (int k = tid; k < NPAIR; k += TPB) {
     const int i = k / NSUB, j = k % NSUB;
     s_map[k] = g.map[((int64_t)e * nsv + sub[i]) * nsv + sub[j]];
@@ -726,7 +766,7 @@ __global__ __launch_bounds__(TPB) void k_bc_hp(ElemArgs g, int nsv, const HpTabl
       for (int qq = 0; qq < nqc; qq++) accf += s_pw[qq][tid];
     __syncthreads();
   }
-  if (type == PZG_BC_DIRICHLET) {
+  if (ck != 0.) {
 #pragma unroll
     for (int k = 0; k < PER; k++) {
       const int idx = tid + k * TPB;
@@ -734,17 +774,12 @@ __global__ __launch_bounds__(TPB) void k_bc_hp(ElemArgs g, int nsv, const HpTabl
       const int i = idx / NSUB, j = idx % NSUB;
       const int64_t bi = s_blk[i], bj = s_blk[j];
       if (bi < 0 || bj < 0 || bi > bj) continue;
-      const double v = g.big * acc[k];
+      const double v = ck * acc[k];
       if (fabs(v) < 1.e-12) continue;
       g.a[bi == bj ? s_row[i] : s_row[i] + 1 + s_map[i * NSUB + j]] += v;
     }
   }
-  if (tid < NSUB && s_blk[tid] >= 0) {
-    double f;
-    if (type == PZG_BC_DIRICHLET) f = g.big * val * accf;
-    else f = val * accf * (g.material == PZG_MAT_POISSON ? g.scale : 1.0);
-    g.rhs[s_blk[tid]] += f;
-  }
+  if (tid < NSUB && s_blk[tid] >= 0) g.rhs[s_blk[tid]] += cfl * accf;
 }
 
 static ElemArgs make_args(Ctx &c, bool vol) {
@@ -763,6 +798,8 @@ static ElemArgs make_args(Ctx &c, bool vol) {
   g.bcid = c.d_bcid.as<int32_t>();
   g.bctype = c.d_bctype.as<int32_t>();
   g.bcval = c.d_bcval.as<double>();
+  g.bccoef = c.d_bccoef.as<double>();
+  g.bcvec = c.d_bcvec.as<double>();
   g.nbumps = c.nbumps;
   g.bw = c.bump_width;
   for (int k = 0; k < 3; k++) g.fconst[k] = c.force_const[k];
@@ -1139,6 +1176,35 @@ __global__ void k_diag(int64_t n, const int64_t *__restrict__ ia, const double *
                        double *__restrict__ d) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) d[i] = a[ia[i]];  // diagonal is the first entry of every row
+}
+
+// Equation filter: the inactive equations of the mesh numbering become
+// decoupled unit rows (diagonal 1, the rest of the row, the column entries
+// held in other rows and the load 0), so the device system equals the
+// condensed TPZSYsmpMatrix on the active equations.  One thread per row.
+__global__ void k_filter_mask(int64_t n, const int64_t *__restrict__ ia, const int64_t *__restrict__ ja,
+                              const int64_t *__restrict__ dest, double *__restrict__ a,
+                              double *__restrict__ rhs) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int64_t k0 = ia[r], k1 = ia[r + 1];
+  if (dest[r] < 0) {
+    a[k0] = 1.0;
+    for (int64_t k = k0 + 1; k < k1; k++) a[k] = 0.0;
+    rhs[r] = 0.0;
+  } else {
+    for (int64_t k = k0 + 1; k < k1; k++)
+      if (dest[ja[k]] < 0) a[k] = 0.0;
+  }
+}
+
+int launch_filter_mask(Ctx &c) {
+  if (!c.filt || !c.neq) return PZG_OK;
+  hipLaunchKernelGGL(k_filter_mask, dim3((c.neq + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.neq,
+                     c.d_ia.as<int64_t>(), c.d_ja.as<int64_t>(), c.d_filt_dest.as<int64_t>(),
+                     c.d_a.as<double>(), c.d_rhs.as<double>());
+  PZG_HIP(hipGetLastError());
+  return PZG_OK;
 }
 
 int launch_diag(Ctx &c) {

@@ -2,6 +2,8 @@
 // symbolic phase (pattern + maps + colouring) and the entry points that
 // dispatch to the device kernels in pzg_assemble.hip / pzg_solve.hip.
 #include <algorithm>
+#include <map>
+#include <set>
 #include <cmath>
 #include <cstring>
 #include <thread>
@@ -56,7 +58,7 @@ static void free_all(Ctx &c) {
                     &c.d_fbia, &c.d_fbja, &c.d_rank_fwd, &c.d_rank_bwd,
                     &c.d_fa, &c.d_bsr, &c.d_dinvsqrt, &c.d_bjinv, &c.d_xyz, &c.d_vnodes,
                     &c.d_vconn, &c.d_vcoef, &c.d_bnodes, &c.d_bconn, &c.d_bcid, &c.d_bctype,
-                    &c.d_bcval, &c.d_aidx, &c.d_bumps, &c.d_vmap, &c.d_bmap, &c.d_tvol,
+                    &c.d_bcval, &c.d_bccoef, &c.d_bcvec, &c.d_aidx, &c.d_bumps, &c.d_vmap, &c.d_bmap, &c.d_tvol,
                     &c.d_tbc, &c.d_vcolor_list, &c.d_bcolor_list, &c.d_x, &c.d_r, &c.d_z,
                     &c.d_p, &c.d_p2, &c.d_q, &c.d_u, &c.d_f, &c.d_w, &c.d_xs, &c.d_rs,
                     &c.d_zs, &c.d_ps, &c.d_ps2, &c.d_qs, &c.d_bs, &c.d_scal, &c.d_partials,
@@ -199,6 +201,130 @@ static int expand_hp(Ctx &c) {
   return PZG_OK;
 }
 
+// Boundary conditions of every bc id as coefficients of the boundary
+// element's integrals (BCC layout, pzg_internal.h), from the ContributeBC
+// switches of TPZMatPoisson (Material/Poisson/TPZMatPoisson.cpp:45-121),
+// TPZDarcyFlow (Material/DarcyFlow/TPZDarcyFlow.cpp:49-91) and
+// TPZElasticity3D (Material/Elasticity/TPZElasticity3D.cpp:616-773); big is
+// the material's penalty.  Types outside a switch are refused.  For the
+// stress-field type the vector from the volume neighbour's centre to the
+// boundary element's centre (TPZInterpolationSpace::ComputeNormal,
+// Mesh/pzinterpolationspace.cpp:301-383) is kept per boundary element.
+static int bc_setup(Ctx &c, double big) {
+  c.bc_coef.assign((size_t)BCC * std::max(0, c.nbcid), 0.);
+  c.bc_mf_ok = true;
+  bool any_stress = false;
+  std::vector<char> used(std::max(0, c.nbcid), 0);
+  for (int64_t e = 0; e < c.nbc; e++) used[c.bc_id[e]] = 1;
+  for (int id = 0; id < c.nbcid; id++) {
+    const int type = c.bc_type[id];
+    const double *v2 = &c.bc_val[3 * id], *v1 = &c.bc_val1[9 * id];
+    double *cf = &c.bc_coef[(size_t)BCC * id];
+    double *ck = cf + BCC_CK, *cl = cf + BCC_CF;
+    bool ok = true, penalty_only = true;
+    if (c.material == PZG_MAT_ELAST3D) {
+      int pen[3] = {0, 0, 0};
+      switch (type) {
+        case PZG_BC_DIRICHLET: pen[0] = pen[1] = pen[2] = 1; break;
+        case PZG_BC_DIR_X: pen[0] = 1; break;
+        case PZG_BC_DIR_Y: pen[1] = 1; break;
+        case PZG_BC_DIR_Z: pen[2] = 1; break;
+        case PZG_BC_DIR_XZ: pen[0] = pen[2] = 1; break;
+        case PZG_BC_NEUMANN: case PZG_BC_MIXED: case PZG_BC_DIR_NULL: case PZG_BC_STRESS: break;
+        default: ok = false;
+      }
+      for (int k = 0; k < 3; k++) {
+        if (pen[k]) { ck[4 * k] = big; cl[k] = big * v2[k]; }
+        else if (type == PZG_BC_NEUMANN || type == PZG_BC_MIXED) cl[k] = v2[k];
+        if (type == PZG_BC_DIR_NULL) ck[4 * k] = big * v2[k];
+      }
+      if (type == PZG_BC_MIXED)
+        for (int k = 0; k < 9; k++) ck[k] = v1[k];
+      if (type == PZG_BC_STRESS) {
+        cf[BCC_STRESS] = 1.;
+        for (int k = 0; k < 9; k++) cf[BCC_VAL1 + k] = v1[k];
+        any_stress |= used[id] != 0;
+      }
+      penalty_only = type == PZG_BC_DIRICHLET || type == PZG_BC_NEUMANN || type == PZG_BC_STRESS;
+    } else {
+      switch (type) {
+        case PZG_BC_DIRICHLET: ck[0] = big; cl[0] = big * v2[0]; break;
+        case PZG_BC_NEUMANN:
+          cl[0] = v2[0] * (c.material == PZG_MAT_POISSON ? c.scale : 1.0);
+          break;
+        case PZG_BC_MIXED:
+          if (c.material == PZG_MAT_POISSON) { cf[BCC_CD] = big * v1[0]; cl[0] = big * v2[0]; }
+          else { ck[0] = v1[0]; cl[0] = v2[0]; }
+          break;
+        default: ok = false;
+      }
+      penalty_only = type == PZG_BC_DIRICHLET || type == PZG_BC_NEUMANN;
+    }
+    if (!ok) {
+      set_error("pzg_mesh_set: boundary condition type " + std::to_string(type) + " of bc id " +
+                std::to_string(id) + " is not one of the material's ContributeBC types");
+      return PZG_EUNSUP;
+    }
+    if (used[id] && !penalty_only) c.bc_mf_ok = false;
+    if (used[id] && c.material == PZG_MAT_POISSON && type == PZG_BC_MIXED && c.hp_simplex) {
+      set_error("pzg_mesh_set: TPZMatPoisson type 2 on high-order tetrahedral meshes is not "
+                "implemented");
+      return PZG_EUNSUP;
+    }
+  }
+  c.bc_vec.clear();
+  if (!any_stress) return PZG_OK;
+  if (c.has_dep || c.hp_simplex) {
+    set_error("pzg_mesh_set: stress-field boundary conditions on constrained or high-order "
+              "simplex meshes are not implemented");
+    return PZG_EUNSUP;
+  }
+  // volume neighbour of every stress-field boundary element: the volume
+  // element with a side whose corners are the boundary element's corners
+  const int vt = c.vol_etype, ncb = c.bc_ncorner, ncv = c.vol_ncorner;
+  std::vector<std::vector<int>> faces;
+  for (int sd = 0; sd < c.vol_nsides; sd++) {
+    int loc[8];
+    const int n = side_corner_list_pub(vt, sd, loc);
+    if (n == ncb) faces.emplace_back(loc, loc + n);
+  }
+  auto sorted = [](std::vector<int64_t> v) {
+    std::sort(v.begin(), v.end());
+    return v;
+  };
+  std::vector<std::vector<int64_t>> want(c.nbc);
+  std::map<std::vector<int64_t>, std::pair<int64_t, int>> owner;  // corners -> (volume, count)
+  for (int64_t e = 0; e < c.nbc; e++) {
+    if (c.bc_coef[(size_t)BCC * c.bc_id[e] + BCC_STRESS] == 0.) continue;
+    want[e] = sorted(std::vector<int64_t>(c.bc_nodes.begin() + e * ncb, c.bc_nodes.begin() + (e + 1) * ncb));
+    owner[want[e]] = {-1, 0};
+  }
+  for (int64_t v = 0; v < c.nvol; v++)
+    for (const auto &f : faces) {
+      std::vector<int64_t> k(f.size());
+      for (size_t i = 0; i < f.size(); i++) k[i] = c.vol_nodes[v * ncv + f[i]];
+      auto it = owner.find(sorted(k));
+      if (it == owner.end()) continue;
+      if (it->second.second++ == 0) it->second.first = v;
+    }
+  c.bc_vec.assign(3 * c.nbc, 0.);
+  for (int64_t e = 0; e < c.nbc; e++) {
+    if (want[e].empty()) continue;
+    const auto &o = owner[want[e]];
+    if (o.second != 1) {
+      set_error("pzg_mesh_set: a stress-field boundary element needs exactly one volume neighbour");
+      return PZG_EUNSUP;
+    }
+    for (int d = 0; d < 3; d++) {
+      double xb = 0., xv = 0.;
+      for (int k = 0; k < ncb; k++) xb += c.xyz[3 * c.bc_nodes[e * ncb + k] + d] / ncb;
+      for (int k = 0; k < ncv; k++) xv += c.xyz[3 * c.vol_nodes[o.first * ncv + k] + d] / ncv;
+      c.bc_vec[3 * e + d] = -xv + xb;
+    }
+  }
+  return PZG_OK;
+}
+
 static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
   const int pmax = (m && m->vol_etype == PZG_TET && m->nstate == 1) ? HP_MAXP : 2;
   if (!m || (m->dim != 2 && m->dim != 3) || m->porder < 1 || m->porder > pmax ||
@@ -253,6 +379,8 @@ static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
   c.nbcid = m->nbcid;
   c.bc_type.assign(m->bc_type, m->bc_type + std::max(0, m->nbcid));
   c.bc_val.assign(m->bc_val, m->bc_val + 3 * std::max(0, m->nbcid));
+  c.bc_val1.assign(9 * std::max(0, m->nbcid), 0.);
+  if (m->bc_val1) c.bc_val1.assign(m->bc_val1, m->bc_val1 + 9 * std::max(0, m->nbcid));
   c.blk_pos.assign(m->blk_pos, m->blk_pos + m->nconn);
   c.blk_size.assign(m->blk_size, m->blk_size + m->nconn);
   c.scale = m->scale;
@@ -367,6 +495,7 @@ static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
     if (v < 0 || v >= c.nnodes) { set_error("pzg_mesh_set: node index out of range"); return PZG_EINVAL; }
   for (int32_t v : c.bc_id)
     if (v < 0 || v >= c.nbcid) { set_error("pzg_mesh_set: bc id out of range"); return PZG_EINVAL; }
+  PZG_TRY(bc_setup(c, m->big_number));
 
   // integration-rule buckets (hp meshes) and their tables
   auto bucket = [&](const int32_t *rules, int64_t n, int dflt, int etype, std::vector<int> &uniq,
@@ -538,7 +667,10 @@ static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
   if (c.nbcid > 0) {
     PZG_TRY(dev_upload(c.d_bctype, c.bc_type.data(), c.bc_type.size(), s));
     PZG_TRY(dev_upload(c.d_bcval, c.bc_val.data(), c.bc_val.size(), s));
+    PZG_TRY(dev_upload(c.d_bccoef, c.bc_coef.data(), c.bc_coef.size(), s));
   }
+  if (!c.bc_vec.empty()) PZG_TRY(dev_upload(c.d_bcvec, c.bc_vec.data(), c.bc_vec.size(), s));
+  else dev_free(c.d_bcvec);
   if (c.nbumps > 0) PZG_TRY(dev_upload(c.d_bumps, c.bumps.data(), c.bumps.size(), s));
   PZG_TRY(dev_upload(c.d_aidx, c.aidx.data(), c.aidx.size(), s));
   PZG_TRY(dev_upload(c.d_tvol, reinterpret_cast<const char *>(c.tvol_all.data()),
@@ -555,6 +687,12 @@ static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
                          sizeof(HpTables) * c.thp_bc.size(), s));
   }
   PZG_HIP(hipStreamSynchronize(s));
+  // a new mesh has its own numbering: no equation filter
+  c.filt = false;
+  c.filt_active.clear();
+  c.filt_dest.clear();
+  c.filt_keep.clear();
+  c.filt_ia.clear();
   c.has_mesh = true;
   c.has_struct = false;
   c.mf_ready = c.mf_vals_valid = false;
@@ -705,6 +843,24 @@ static int struct_create(Ctx &c) {
   PZG_TRY(dev_alloc(c.d_rhs, sizeof(double) * std::max<int64_t>(N, 1)));
   PZG_TRY(dev_alloc(c.d_diag, sizeof(double) * std::max<int64_t>(N, 1)));
   PZG_TRY(launch_expand_upper_ja(c));
+  if (c.filt) {
+    // condensed pattern: the mesh-numbered upper CSR restricted to the active
+    // rows and columns (what SetupMatrixData builds with the filter,
+    // StrMatrix/TPZSSpStructMatrix.cpp:49-193)
+    std::vector<int64_t> ja(std::max<int64_t>(c.nnz_u, 1));
+    PZG_HIP(hipMemcpyAsync(ja.data(), c.d_ja.p, sizeof(int64_t) * c.nnz_u, hipMemcpyDeviceToHost, s));
+    PZG_HIP(hipStreamSynchronize(s));
+    const int64_t na = (int64_t)c.filt_active.size();
+    c.filt_ia.assign(na + 1, 0);
+    c.filt_keep.clear();
+    for (int64_t i = 0; i < na; i++) {
+      const int64_t r = c.filt_active[i];
+      for (int64_t k = ia[r]; k < ia[r + 1]; k++)
+        if (c.filt_dest[ja[k]] >= 0) c.filt_keep.push_back(k);
+      c.filt_ia[i + 1] = (int64_t)c.filt_keep.size();
+    }
+    PZG_TRY(dev_upload(c.d_filt_dest, c.filt_dest.data(), c.filt_dest.size(), s));
+  }
   // colourings and element -> upper-block maps
   std::vector<int64_t> vptr, bptr2;
   std::vector<int32_t> vlist, blist;
@@ -850,9 +1006,58 @@ int pzg_mesh_set(pzg_ctx *ctx, const pzg_mesh_desc *mesh) {
 int pzg_struct_create(pzg_ctx *ctx, int64_t *neq, int64_t *nnz) {
   CTX_CHECK();
   PZG_TRY(struct_create(ctx->c));
-  if (neq) *neq = ctx->c.neq;
-  if (nnz) *nnz = ctx->c.nnz_u;
+  const Ctx &c = ctx->c;
+  if (neq) *neq = c.filt ? (int64_t)c.filt_active.size() : c.neq;
+  if (nnz) *nnz = c.filt ? (int64_t)c.filt_keep.size() : c.nnz_u;
   return PZG_OK;
+}
+
+int pzg_set_equation_filter(pzg_ctx *ctx, int64_t nactive, const int64_t *active) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (!c.has_mesh) {
+    set_error("pzg_set_equation_filter: call pzg_mesh_set first");
+    return PZG_ESTATE;
+  }
+  if (nactive < 0 || !active) {
+    c.filt = false;
+    c.filt_active.clear();
+    c.filt_dest.clear();
+  } else {
+    std::vector<int64_t> v(active, active + nactive);
+    for (int64_t e : v)
+      if (e < 0 || e >= c.neq) {
+        set_error("pzg_set_equation_filter: active equation out of range");
+        return PZG_EINVAL;
+      }
+    // TPZEquationFilter::SetActiveEquations: sorted, duplicates removed
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    c.filt_active.swap(v);
+    c.filt_dest.assign(c.neq, -1);
+    for (size_t i = 0; i < c.filt_active.size(); i++) c.filt_dest[c.filt_active[i]] = (int64_t)i;
+    c.filt = true;
+  }
+  c.filt_keep.clear();
+  c.filt_ia.clear();
+  // the pattern belongs to the filter: pzg_struct_create builds it again
+  c.has_struct = false;
+  c.assembled = false;
+  c.has_solution = false;
+  c.mf_ready = c.mf_vals_valid = false;
+  c.full_valid = c.bsr_valid = c.scal_valid = false;
+  mg_free(c);
+  return PZG_OK;
+}
+
+// TPZEquationFilter::Scatter / Gather of host vectors between the condensed
+// numbering (the matrix's rows) and the mesh numbering of the device system.
+static void filt_scatter(const Ctx &c, const double *small, std::vector<double> &big) {
+  big.assign(c.neq, 0.);
+  for (size_t i = 0; i < c.filt_active.size(); i++) big[c.filt_active[i]] = small[i];
+}
+static void filt_gather(const Ctx &c, const double *big, double *small) {
+  for (size_t i = 0; i < c.filt_active.size(); i++) small[i] = big[c.filt_active[i]];
 }
 
 int pzg_struct_get_pattern(pzg_ctx *ctx, int64_t *ia, int64_t *ja) {
@@ -861,6 +1066,16 @@ int pzg_struct_get_pattern(pzg_ctx *ctx, int64_t *ia, int64_t *ja) {
   if (!c.has_struct) {
     set_error("pzg_struct_get_pattern: no pattern");
     return PZG_ESTATE;
+  }
+  if (c.filt) {
+    if (ia) std::copy(c.filt_ia.begin(), c.filt_ia.end(), ia);
+    if (ja) {
+      std::vector<int64_t> full(std::max<int64_t>(c.nnz_u, 1));
+      PZG_HIP(hipMemcpyAsync(full.data(), c.d_ja.p, sizeof(int64_t) * c.nnz_u, hipMemcpyDeviceToHost, c.stream));
+      PZG_HIP(hipStreamSynchronize(c.stream));
+      for (size_t k = 0; k < c.filt_keep.size(); k++) ja[k] = c.filt_dest[full[c.filt_keep[k]]];
+    }
+    return PZG_OK;
   }
   if (ia) PZG_HIP(hipMemcpyAsync(ia, c.d_ia.p, sizeof(int64_t) * (c.neq + 1), hipMemcpyDeviceToHost, c.stream));
   if (ja && c.nnz_u) PZG_HIP(hipMemcpyAsync(ja, c.d_ja.p, sizeof(int64_t) * c.nnz_u, hipMemcpyDeviceToHost, c.stream));
@@ -876,6 +1091,7 @@ int pzg_assemble(pzg_ctx *ctx) {
     return PZG_ESTATE;
   }
   PZG_TRY(launch_assemble(c));
+  PZG_TRY(launch_filter_mask(c));
   PZG_TRY(launch_diag(c));
   c.assembled = true;
   c.kel_added = false;
@@ -890,6 +1106,20 @@ int pzg_get_system(pzg_ctx *ctx, double *a, double *rhs) {
   if (!c.assembled) {
     set_error("pzg_get_system: nothing assembled");
     return PZG_ESTATE;
+  }
+  if (c.filt) {  // the condensed values and load (matrix numbering)
+    std::vector<double> full(std::max<int64_t>(std::max(c.nnz_u, c.neq), 1));
+    if (a) {
+      PZG_HIP(hipMemcpyAsync(full.data(), c.d_a.p, sizeof(double) * c.nnz_u, hipMemcpyDeviceToHost, c.stream));
+      PZG_HIP(hipStreamSynchronize(c.stream));
+      for (size_t k = 0; k < c.filt_keep.size(); k++) a[k] = full[c.filt_keep[k]];
+    }
+    if (rhs) {
+      PZG_HIP(hipMemcpyAsync(full.data(), c.d_rhs.p, sizeof(double) * c.neq, hipMemcpyDeviceToHost, c.stream));
+      PZG_HIP(hipStreamSynchronize(c.stream));
+      filt_gather(c, full.data(), rhs);
+    }
+    return PZG_OK;
   }
   if (a && c.nnz_u) PZG_HIP(hipMemcpyAsync(a, c.d_a.p, sizeof(double) * c.nnz_u, hipMemcpyDeviceToHost, c.stream));
   if (rhs) PZG_HIP(hipMemcpyAsync(rhs, c.d_rhs.p, sizeof(double) * c.neq, hipMemcpyDeviceToHost, c.stream));
@@ -923,6 +1153,10 @@ int pzg_full_csr(pzg_ctx *ctx, int64_t *ia, int64_t *ja, double *a) {
   if (!c.assembled) {
     set_error("pzg_full_csr: needs an assembled system");
     return PZG_ESTATE;
+  }
+  if (c.filt) {
+    set_error("pzg_full_csr: the non-symmetric format with an equation filter is not implemented");
+    return PZG_EUNSUP;
   }
   const int64_t nnz = c.nnz_full;
   DevBuf dia, dja, da;
@@ -969,8 +1203,12 @@ int pzg_csr_set(pzg_ctx *ctx, int64_t neq, const int64_t *ia, const int64_t *ja,
         return PZG_EINVAL;
       }
   }
-  // A caller matrix is handled as a scalar problem (block size 1).
+  // A caller matrix is handled as a scalar problem (block size 1), in its
+  // own numbering (no equation filter).
   c.has_mesh = false;
+  c.filt = false;
+  c.filt_active.clear();
+  c.filt_dest.clear();
   c.neq = neq;
   c.bs = 1;
   c.nb = neq;
@@ -1046,11 +1284,18 @@ int pzg_add_kel_batch(pzg_ctx *ctx, int64_t nel, int32_t ne, const int64_t *dest
     return PZG_EINVAL;
   }
   if (nel == 0) return PZG_OK;
+  const int64_t nrow = c.filt ? (int64_t)c.filt_active.size() : c.neq;
   for (int64_t i = 0; i < nel * ne; i++)
-    if (dest[i] < 0 || dest[i] >= c.neq) {
+    if (dest[i] < 0 || dest[i] >= nrow) {
       set_error("pzg_add_kel_batch: destination equation out of range");
       return PZG_EINVAL;
     }
+  std::vector<int64_t> dmesh;
+  if (c.filt) {  // condensed destinations -> mesh numbering of the device system
+    dmesh.resize(nel * ne);
+    for (int64_t i = 0; i < nel * ne; i++) dmesh[i] = c.filt_active[dest[i]];
+    dest = dmesh.data();
+  }
   // colour by equations so that one launch never writes a row twice
   std::vector<uint64_t> used(c.neq, 0);
   std::vector<int> col(nel);
@@ -1103,6 +1348,18 @@ int pzg_multadd(pzg_ctx *ctx, const double *x, const double *y, double *z, doubl
     return PZG_EINVAL;
   }
   int64_t n = c.neq;
+  std::vector<double> xb, yb, zb;
+  double *zout = z;
+  if (c.filt) {  // condensed vectors
+    filt_scatter(c, x, xb);
+    x = xb.data();
+    if (beta != 0.) {
+      filt_scatter(c, y, yb);
+      y = yb.data();
+    }
+    zb.resize(n);
+    zout = zb.data();
+  }
   PZG_TRY(dev_alloc(c.d_x, n * sizeof(double)));
   PZG_TRY(dev_alloc(c.d_w, n * sizeof(double)));
   PZG_TRY(dev_alloc(c.d_z, n * sizeof(double)));
@@ -1110,8 +1367,9 @@ int pzg_multadd(pzg_ctx *ctx, const double *x, const double *y, double *z, doubl
   if (beta != 0.) PZG_HIP(hipMemcpyAsync(c.d_w.p, y, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
   PZG_TRY(launch_multadd_upper(c, c.d_x.as<double>(), beta != 0. ? c.d_w.as<double>() : nullptr,
                                c.d_z.as<double>(), alpha, beta));
-  PZG_HIP(hipMemcpyAsync(z, c.d_z.p, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  PZG_HIP(hipMemcpyAsync(zout, c.d_z.p, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   PZG_HIP(hipStreamSynchronize(c.stream));
+  if (c.filt) filt_gather(c, zout, z);
   return PZG_OK;
 }
 
@@ -1131,6 +1389,11 @@ int pzg_solve(pzg_ctx *ctx, const pzg_solve_params *params, const double *rhs, d
   pzg_solve_info &inf = info ? *info : tmp;
   std::memset(&inf, 0, sizeof(inf));
   int64_t n = c.neq;
+  std::vector<double> rb;
+  if (rhs && c.filt) {  // condensed load -> mesh numbering (inactive 0)
+    filt_scatter(c, rhs, rb);
+    rhs = rb.data();
+  }
   PZG_TRY(dev_alloc(c.d_f, n * sizeof(double)));
   if (rhs) PZG_HIP(hipMemcpyAsync(c.d_f.p, rhs, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
   else PZG_HIP(hipMemcpyAsync(c.d_f.p, c.d_rhs.p, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
@@ -1144,8 +1407,11 @@ int pzg_solve(pzg_ctx *ctx, const pzg_solve_params *params, const double *rhs, d
   if (rc != PZG_OK && rc != PZG_ENOCONV) return rc;
   c.has_solution = true;
   if (u) {
-    PZG_HIP(hipMemcpyAsync(u, c.d_u.p, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    std::vector<double> ub(c.filt ? n : 0);
+    double *dst = c.filt ? ub.data() : u;
+    PZG_HIP(hipMemcpyAsync(dst, c.d_u.p, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     PZG_HIP(hipStreamSynchronize(c.stream));
+    if (c.filt) filt_gather(c, dst, u);
   }
   if (rc == PZG_ENOCONV) set_error("pzg_solve: no convergence within max_iter");
   return rc;

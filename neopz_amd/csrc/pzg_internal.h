@@ -48,6 +48,15 @@ constexpr int NNF = 32;
 constexpr int NNL = 3;
 constexpr float NN_SLOPE = 0.1f;
 
+// Boundary condition of one bc id as coefficients of the boundary element's
+// integrals (filled by mesh_set from the material's ContributeBC):
+//   ek(I,J) += ck[ci][cj] M_ij + cd D_ij,   ef(I) += cf[ci] m_i
+//            + stress ? sum_q w_q phi_i(q) (-sum_m val1[ci][m] n_m(q)) : 0
+// with M_ij = sum_q w|J| phi_i phi_j, m_i = sum_q w|J| phi_i and
+// D_ij = sum_q w|J| dphix0_i dphix0_j (derivative along the first axis).
+constexpr int BCC = 24;
+enum { BCC_CK = 0, BCC_CF = 9, BCC_CD = 12, BCC_STRESS = 13, BCC_VAL1 = 14 };
+
 // Maximum sizes of the element tables (hexahedron p=2).
 constexpr int MAXQ = 27;      // quadrature points per element
 constexpr int MAXSH = 27;     // shape functions per element
@@ -183,7 +192,12 @@ struct Ctx {
   bool has_mesh = false;
   int dim = 0, material = 0, nstate = 1, porder = 1, bs = 1;
   int64_t nnodes = 0, nvol = 0, nbc = 0, nconn = 0, neq = 0;
-  std::vector<double> xyz, vol_coef, bc_val, bumps;
+  std::vector<double> xyz, vol_coef, bc_val, bc_val1, bumps;
+  // boundary conditions as coefficients of the boundary element's matrices
+  // (bc_coef_of): per bc id BCC doubles; per boundary element the vector from
+  // its volume neighbour's centre to its own (stress-field type only)
+  std::vector<double> bc_coef, bc_vec;
+  bool bc_mf_ok = true;  // every boundary matrix is the Dirichlet penalty or zero
   std::vector<int64_t> vol_nodes, vol_conn, bc_nodes, bc_conn, blk_pos;
   std::vector<int32_t> blk_size, bc_id, bc_type;
   double scale = 1, E = 1, nu = 0.3, big = 1e12, bump_width = 0.1, force_const[3] = {0, 0, 0};
@@ -232,6 +246,18 @@ struct Ctx {
   DevBuf d_aidx_ek;                       // connect -> block, -2 constrained, -1 no shapes
   DevBuf d_dep_ptr, d_dep_blk, d_dep_coef;
 
+  // ---- equation filter (TPZEquationFilter held by the struct matrix) -------
+  // filt_active: active equations of the mesh numbering, ascending;
+  // filt_dest[eq]: condensed index or -1.  The device keeps the system in the
+  // mesh numbering with every inactive equation decoupled (unit diagonal, zero
+  // row, column and load), which has the condensed system's iterates on the
+  // active equations; the condensed pattern and values (the reference's
+  // TPZSYsmpMatrix) are gathered through filt_keep, the positions of the
+  // condensed entries in the mesh-numbered upper CSR.
+  bool filt = false;
+  std::vector<int64_t> filt_active, filt_dest, filt_keep, filt_ia;
+  DevBuf d_filt_dest;
+
   // ---- pattern (device) -------------------------------------------------
   bool has_struct = false, external_csr = false;
   int64_t nnz_u = 0, nnz_full = 0, nbu = 0, nbf = 0;
@@ -249,6 +275,7 @@ struct Ctx {
 
   // ---- assembly (device) -------------------------------------------------
   DevBuf d_xyz, d_vnodes, d_vconn, d_vcoef, d_bnodes, d_bconn, d_bcid, d_bctype, d_bcval;
+  DevBuf d_bccoef, d_bcvec;
   DevBuf d_aidx, d_bumps, d_vmap, d_bmap, d_tvol, d_tbc, d_thp_vol, d_thp_bc;
   DevBuf d_vcolor_list, d_bcolor_list;
   std::vector<int64_t> vcolor_ptr, bcolor_ptr;  // color c: list[ptr[c] .. ptr[c+1])
@@ -332,6 +359,8 @@ int launch_expand_upper_ja(Ctx &c);
 int launch_elem_maps(Ctx &c);
 int launch_assemble(Ctx &c);
 int launch_diag(Ctx &c);
+// decouples the inactive equations of the equation filter (mesh numbering)
+int launch_filter_mask(Ctx &c);
 int launch_expand_full(Ctx &c);
 // equation-level full CSR (device arrays, any may be null; ia without ia[neq])
 int launch_full_csr(Ctx &c, int64_t *d_ia, int64_t *d_ja, double *d_a);
@@ -373,7 +402,8 @@ int mf_apply(Ctx &c, const float *p, const float *r, float *q, const SolveScalar
 // the matrix-free operator reproduces the assembled K (mesh-built, no AddKel extras)
 inline bool mf_available(const Ctx &c) {
   return c.has_mesh && c.has_struct && !c.external_csr && !c.kel_added && c.assembled && !c.has_dep &&
-         c.vol_rules.size() == 1 && c.bc_rules.size() <= 1 && c.vol_etype != PZG_TET;
+         !c.filt &&
+         c.vol_rules.size() == 1 && c.bc_rules.size() <= 1 && c.vol_etype != PZG_TET && c.bc_mf_ok;
 }
 int solve_pcg64(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info);
 int solve_mixed(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info);

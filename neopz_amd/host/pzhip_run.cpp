@@ -119,7 +119,7 @@ int main(int argc, char **argv) {
     return 2;
   }
   char magic[8];
-  if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, "PZGMESH4", 8) != 0) Reader::fail();
+  if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, "PZGMESH5", 8) != 0) Reader::fail();
   Reader r{f};
   pzg_mesh_desc m{};
   m.dim = r.scalar<int32_t>();
@@ -154,6 +154,7 @@ int main(int argc, char **argv) {
   auto dep_ptr = r.array<int64_t>();
   auto dep_conn = r.array<int64_t>();
   auto dep_coef = r.array<double>();
+  auto bc_val1 = r.array<double>();
   std::fclose(f);
 
   const bool tet = m.vol_etype == PZG_TET;
@@ -174,6 +175,8 @@ int main(int argc, char **argv) {
   m.nbcid = int32_t(bc_type.size());
   m.bc_type = bc_type.data();
   m.bc_val = bc_val.data();
+  if (!bc_val1.empty() && bc_val1.size() != 9 * bc_type.size()) Reader::fail();
+  m.bc_val1 = bc_val1.empty() ? nullptr : bc_val1.data();
   m.nconn = int64_t(blk_pos.size());
   m.blk_pos = blk_pos.data();
   m.blk_size = blk_size.data();

@@ -115,6 +115,8 @@ def _common(pb, econn_vol, econn_bc, nconn, corner_flags, xyz, vol_nodes, bc_nod
         bc_nodes=bc_nodes, bc_conn=econn_bc, bc_id=bc_id,
         bc_type=np.array([pb.bc_type[f] for f in range(6)], dtype=np.int32),
         bc_val=np.array([pb.bc_val[f] for f in range(6)], dtype=np.float64).ravel(),
+        bc_val1=np.array([getattr(pb, "bc_val1", [[0.0] * 9] * 6)[f] for f in range(6)],
+                         dtype=np.float64).ravel(),
         nconn=nconn, blk_pos=blk_pos, blk_size=blk_size, neq=neq,
         scale=pb.scale, E=pb.E, nu=pb.nu, rule_order_vol=rule_vol, rule_order_bc=2 * pb.p,
         force_kind=pb.force_kind, force_const=np.array(pb.force_const, dtype=np.float64),
@@ -360,7 +362,7 @@ def _build3d_tet(pb):
 # Flat little-endian file of a mesh description, read by the C++ host mirror
 # (neopz_amd/host/pzhip_run.cpp).  Fields in this fixed order; arrays are
 # preceded by their element count (int64).
-_FILE_MAGIC = b"PZGMESH4"
+_FILE_MAGIC = b"PZGMESH5"
 
 
 def save(m, path):
@@ -408,6 +410,7 @@ def save(m, path):
         arr(f, m.get("dep_ptr"), np.int64)
         arr(f, m.get("dep_conn"), np.int64)
         arr(f, m.get("dep_coef"), np.float64)
+        arr(f, m.get("bc_val1"), np.float64)
 
 
 def build_refined2d(pb, refine):
@@ -541,6 +544,8 @@ def build_refined2d(pb, refine):
         bc_nodes=bc_nodes, bc_conn=bc_conn, bc_id=bc_id,
         bc_type=np.array([pb.bc_type[f] for f in range(6)], dtype=np.int32),
         bc_val=np.array([pb.bc_val[f] for f in range(6)], dtype=np.float64).ravel(),
+        bc_val1=np.array([getattr(pb, "bc_val1", [[0.0] * 9] * 6)[f] for f in range(6)],
+                         dtype=np.float64).ravel(),
         nconn=nconn, blk_pos=blk_pos, blk_size=blk_size, neq=neq,
         scale=pb.scale, E=pb.E, nu=pb.nu, rule_order_vol=rule, rule_order_bc=2 * pb.p,
         force_kind=pb.force_kind, force_const=np.array(pb.force_const, dtype=np.float64),
@@ -761,6 +766,8 @@ def build_refined3d(pb, refine):
         bc_nodes=bc_nodes, bc_conn=bc_conn, bc_id=bc_id,
         bc_type=np.array([pb.bc_type[f] for f in range(6)], dtype=np.int32),
         bc_val=np.array([pb.bc_val[f] for f in range(6)], dtype=np.float64).ravel(),
+        bc_val1=np.array([getattr(pb, "bc_val1", [[0.0] * 9] * 6)[f] for f in range(6)],
+                         dtype=np.float64).ravel(),
         nconn=nconn, blk_pos=blk_pos, blk_size=blk_size, neq=neq,
         scale=pb.scale, E=pb.E, nu=pb.nu, rule_order_vol=rule, rule_order_bc=2 * pb.p,
         force_kind=pb.force_kind, force_const=np.array(pb.force_const, dtype=np.float64),

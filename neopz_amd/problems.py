@@ -12,7 +12,9 @@ numpy's seeded ``default_rng`` and is handed to both the HIP path and the CPU
 oracle as data.
 
 Boundary face numbering: 0: x=0, 1: x=1, 2: y=0, 3: y=1, 4: z=0, 5: z=1.
-BC type -1 = no boundary element, 0 = Dirichlet (penalty), 1 = Neumann.
+BC type -1 = no boundary element, else the TPZBndCondT type of the material's
+ContributeBC (0 Dirichlet penalty, 1 Neumann, 2 Robin/mixed; TPZElasticity3D
+also 3..8), with Val2 = ``bc_val`` and Val1 = ``bc_val1``.
 """
 from dataclasses import dataclass, field
 from typing import Optional
@@ -42,6 +44,8 @@ class Problem:
     bump_width: float = 0.1
     bc_type: list = field(default_factory=lambda: [-1] * 6)
     bc_val: list = field(default_factory=lambda: [[0.0, 0.0, 0.0] for _ in range(6)])
+    # TPZBndCondT::Val1 per face, row-major 3x3 (Val1(0,0) for scalar materials)
+    bc_val1: list = field(default_factory=lambda: [[0.0] * 9 for _ in range(6)])
     name: str = ""
     # hp: preferred order per volume element (1..p); None = p everywhere.
     # Boundary elements keep the default order p.

@@ -35,6 +35,7 @@ class _Problem(C.Structure):
         ("bump_width", C.c_double),
         ("bc_type", C.c_int * 6), ("bc_val", (C.c_double * 3) * 6),
         ("el_porder", C.POINTER(C.c_int)), ("simplex", C.c_int),
+        ("bc_val1", (C.c_double * 9) * 6),
     ]
 
 
@@ -108,6 +109,9 @@ def lib():
         L.orc_setup_pattern_constrained.argtypes = [M, i64p, i64p, C.POINTER(i64p), C.POINTER(i64p)]
         L.orc_setup_pattern_constrained.restype = C.c_int64
         L.orc_assemble_constrained.argtypes = [P, M, i64p, i64p, dp, i64p, i64p, dp, dp]
+        L.orc_setup_pattern_filtered.argtypes = [M, i64p, C.POINTER(i64p), C.POINTER(i64p)]
+        L.orc_setup_pattern_filtered.restype = C.c_int64
+        L.orc_assemble_filtered.argtypes = [P, M, i64p, i64p, i64p, dp, dp]
         L.orc_sloan.argtypes = [C.c_int64, i64p, i64p, i64p]
         L.orc_mesh_sloan.argtypes = [M, i64p]
         L.orc_nnz_closed_form.argtypes = [C.c_int] * 6
@@ -232,6 +236,9 @@ class Oracle:
             pb.bc_type[f] = spec.bc_type[f]
             for k in range(3):
                 pb.bc_val[f][k] = spec.bc_val[f][k]
+            v1 = getattr(spec, "bc_val1", None)
+            for k in range(9):
+                pb.bc_val1[f][k] = v1[f][k] if v1 is not None else 0.0
         pb.simplex = int(getattr(spec, "simplex", 0))
         if getattr(spec, "el_porder", None) is not None:
             po = np.ascontiguousarray(spec.el_porder, dtype=np.int32)
@@ -363,9 +370,35 @@ class Oracle:
                                               _ip(self.dep[1]), _dp(self.dep[2]), _ip(ia), _ip(ja),
                                               _dp(a), _dp(rhs)) != 0:
                 raise RuntimeError("oracle constrained assembly failed")
-        else:
-            lib().orc_assemble(C.byref(self.pb), C.byref(self.m), _ip(ia), _ip(ja), _dp(a), _dp(rhs))
+        elif lib().orc_assemble(C.byref(self.pb), C.byref(self.m), _ip(ia), _ip(ja), _dp(a),
+                                _dp(rhs)) != 0:
+            raise RuntimeError("oracle assembly failed (boundary condition type outside the "
+                               "material's ContributeBC, or element too large)")
         return a, rhs
+
+    def filtered(self, active):
+        """Pattern and assembly with an active TPZEquationFilter over the
+        equations ``active`` (TPZEquationFilter::SetActiveEquations: sorted,
+        duplicates removed): condensed (ia, ja, a) and the load in the mesh
+        numbering (inactive entries 0)."""
+        act = np.unique(np.asarray(active, dtype=np.int64))
+        dest = np.full(self.neq, -1, dtype=np.int64)
+        dest[act] = np.arange(len(act))
+        ia = C.POINTER(C.c_int64)()
+        ja = C.POINTER(C.c_int64)()
+        nnz = lib().orc_setup_pattern_filtered(C.byref(self.m), _ip(dest), C.byref(ia), C.byref(ja))
+        if nnz < 0:
+            raise RuntimeError("filtered pattern failed")
+        out_ia = np.ctypeslib.as_array(ia, shape=(len(act) + 1,)).copy()
+        out_ja = np.ctypeslib.as_array(ja, shape=(max(nnz, 1),))[:nnz].copy()
+        lib().free(ia)
+        lib().free(ja)
+        a = np.zeros(max(nnz, 1))
+        rhs = np.zeros(self.neq)
+        if lib().orc_assemble_filtered(C.byref(self.pb), C.byref(self.m), _ip(dest), _ip(out_ia),
+                                       _ip(out_ja), _dp(a), _dp(rhs)) != 0:
+            raise RuntimeError("filtered assembly failed")
+        return out_ia, out_ja, a[:nnz].copy(), rhs
 
     def sloan(self):
         """TPZAnalysis::OptimizeBandwidth's permutation (TPZSloanRenumbering) of

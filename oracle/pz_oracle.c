@@ -1003,6 +1003,148 @@ void orc_contribute(int material, int dim, int nsh, int ne, double weight, const
 }
 
 /* ------------------------------------------------------------------------ */
+/* Boundary conditions: TPZMatPoisson::ContributeBC (Material/Poisson/        */
+/* TPZMatPoisson.cpp:45-121), TPZDarcyFlow::ContributeBC (Material/DarcyFlow/ */
+/* TPZDarcyFlow.cpp:49-91), TPZElasticity3D::ContributeBC (Material/          */
+/* Elasticity/TPZElasticity3D.cpp:616-773), one integration point.  Types     */
+/* outside each switch are refused (-1): the reference prints an error and    */
+/* adds nothing (Poisson, Elasticity3D) or stops (Darcy).                     */
+/* ------------------------------------------------------------------------ */
+static int bc_contribute(const orc_problem *pb, int face, int nsh, int ne, double weight,
+                         const double *phi, const double *dphix, const double *normal, double *ek,
+                         double *ef) {
+  const int type = pb->bc_type[face];
+  const double *v2 = pb->bc_val[face];
+  const double *v1 = pb->bc_val1[face];
+  if (pb->material == ORC_ELAST3D) {
+    const double BIG = 1.e12;
+    double v2l[3] = {v2[0], v2[1], v2[2]};
+    /* components held by the penalty: 0 all, 5 x, 6 y, 7 z, 8 x and z */
+    int pen[3] = {0, 0, 0};
+    switch (type) {
+      case 0: pen[0] = pen[1] = pen[2] = 1; break;
+      case 5: pen[0] = 1; break;
+      case 6: pen[1] = 1; break;
+      case 7: pen[2] = 1; break;
+      case 8: pen[0] = pen[2] = 1; break;
+      case 1: case 2: case 3: case 4: break;
+      default: return -1;
+    }
+    if (type == 4)
+      for (int k = 0; k < 3; k++)
+        v2l[k] = -(v1[3 * k + 0] * normal[0] + v1[3 * k + 1] * normal[1] + v1[3 * k + 2] * normal[2]);
+    for (int in = 0; in < nsh; in++) {
+      for (int k = 0; k < 3; k++) {
+        if (pen[k]) ef[3 * in + k] += BIG * v2l[k] * phi[in] * weight;
+        else if (type == 1 || type == 2 || type == 4) ef[3 * in + k] += v2l[k] * phi[in] * weight;
+      }
+      for (int jn = 0; jn < nsh; jn++) {
+        for (int k = 0; k < 3; k++) {
+          if (pen[k]) ek[(3 * in + k) * ne + 3 * jn + k] += BIG * phi[in] * phi[jn] * weight;
+          if (type == 3)
+            ek[(3 * in + k) * ne + 3 * jn + k] += BIG * phi[in] * phi[jn] * weight * v2l[k];
+        }
+        if (type == 2)
+          for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+              ek[(3 * in + a) * ne + 3 * jn + b] += v1[3 * a + b] * weight * phi[in] * phi[jn];
+      }
+    }
+    return 0;
+  }
+  const double big = big_number();
+  if (pb->material == ORC_POISSON) {
+    for (int in = 0; in < nsh; in++) {
+      if (type == 0) {
+        ef[in] += big * v2[0] * phi[in] * weight;
+        for (int jn = 0; jn < nsh; jn++) ek[in * ne + jn] += big * phi[in] * phi[jn] * weight;
+      } else if (type == 1) {
+        ef[in] += v2[0] * pb->scale * phi[in] * weight;
+      } else if (type == 2) {
+        /* Robin as written there: the penalty times Val1 on the first
+         * derivative (dphix row 0) of the boundary element */
+        ef[in] += big * v2[0] * phi[in] * weight;
+        for (int jn = 0; jn < nsh; jn++)
+          ek[in * ne + jn] += big * v1[0] * dphix[in] * dphix[jn] * weight;
+      } else {
+        return -1;
+      }
+    }
+    return 0;
+  }
+  for (int in = 0; in < nsh; in++) { /* Darcy */
+    if (type == 0) {
+      ef[in] += (big * phi[in] * weight) * v2[0];
+      for (int jn = 0; jn < nsh; jn++) ek[in * ne + jn] += big * phi[in] * phi[jn] * weight;
+    } else if (type == 1) {
+      ef[in] += v2[0] * (phi[in] * weight);
+    } else if (type == 2) {
+      ef[in] += v2[0] * (phi[in] * weight);
+      for (int jn = 0; jn < nsh; jn++) ek[in * ne + jn] += v1[0] * (phi[in] * phi[jn] * weight);
+    } else {
+      return -1;
+    }
+  }
+  return 0;
+}
+
+/* Centre of a geometric element: X at TPZGeoEl::CenterPoint of its own side,
+ * the mean of the corners for the linear / multilinear maps used here. */
+static void elem_centre(const orc_mesh *m, int64_t iel, double *x) {
+  int nc = etype_ncorner(m->etype[iel]);
+  x[0] = x[1] = x[2] = 0.;
+  for (int n = 0; n < nc; n++)
+    for (int d = 0; d < 3; d++) x[d] += m->xyz[3 * m->enodes[8 * iel + n] + d] / nc;
+}
+
+/* vec of TPZInterpolationSpace::ComputeNormal (Mesh/pzinterpolationspace.cpp:
+ * 301-345): centre of the boundary element minus the centre of its first
+ * neighbour of higher dimension, here the volume element holding all its
+ * corners (unique on the domain boundary). */
+static int bc_neighbour_vec(const orc_mesh *m, int64_t iel, double *vec) {
+  int nc = etype_ncorner(m->etype[iel]);
+  for (int64_t v = 0; v < m->nvol; v++) {
+    int nv = etype_ncorner(m->etype[v]), found = 0;
+    for (int a = 0; a < nc; a++)
+      for (int b = 0; b < nv; b++)
+        if (m->enodes[8 * iel + a] == m->enodes[8 * v + b]) { found++; break; }
+    if (found == nc) {
+      double xb[3], xv[3];
+      elem_centre(m, iel, xb);
+      elem_centre(m, v, xv);
+      for (int d = 0; d < 3; d++) vec[d] = -xv[d] + xb[d];
+      return 0;
+    }
+  }
+  return -1;
+}
+
+static void cross(const double *a, const double *b, double *c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = -a[0] * b[2] + a[2] * b[0];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* ComputeNormal (Mesh/pzinterpolationspace.cpp:347-383): the unit normal of
+ * the boundary element from its axes, turned to have a non-negative product
+ * with vec. */
+static void bc_normal(int dim, const double axes[3][3], const double *vec, double *n) {
+  double a1[3] = {axes[0][0], axes[0][1], axes[0][2]}, a2[3], t[3];
+  if (dim == 1) {
+    cross(a1, vec, a2);
+    cross(a2, a1, t);
+  } else {
+    for (int i = 0; i < 3; i++) a2[i] = axes[1][i];
+    cross(a1, a2, t);
+  }
+  double sz = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+  double dot = 0.;
+  for (int i = 0; i < 3; i++) { n[i] = t[i] / sz; dot += n[i] * vec[i]; }
+  if (dot < 0.)
+    for (int i = 0; i < 3; i++) n[i] = -n[i];
+}
+
+/* ------------------------------------------------------------------------ */
 /* CalcStiff: TPZInterpolationSpace::CalcStiffInternal                        */
 /* (Mesh/pzinterpolationspace.cpp:403-474) with ComputeRequiredDataT (:265-  */
 /* 298) and TPZCompElH1::ComputeShape (Mesh/TPZCompElH1.cpp:139-149).         */
@@ -1055,6 +1197,10 @@ int orc_calcstiff(const orc_problem *pb, const orc_mesh *m, int64_t iel, double 
   int sord[27];
   for (int n = 0; n < nc; n++) ids[n] = m->enodes[8 * iel + n];
   for (int s = nc; s < etype_nsides(t); s++) sord[s - nc] = m->conn_order[m->econn[27 * iel + s]];
+  double vec[3] = {0, 0, 0};
+  if (isbc && pb->material == ORC_ELAST3D && pb->bc_type[m->bcface[iel]] == 4 &&
+      bc_neighbour_vec(m, iel, vec) != 0)
+    return -1;
   for (int q = 0; q < nq; q++) {
     double xi[3] = {0, 0, 0}, w;
     if (is_simplex(t)) {
@@ -1105,42 +1251,10 @@ int orc_calcstiff(const orc_problem *pb, const orc_mesh *m, int64_t iel, double 
     double f[3];
     forcing(pb, x, f);
     if (isbc) {
-      int face = m->bcface[iel], type = pb->bc_type[face];
-      const double *v2 = pb->bc_val[face];
-      double big = pb->material == ORC_ELAST3D ? 1.e12 : big_number();
-      if (pb->material == ORC_ELAST3D) {
-        /* Material/Elasticity/TPZElasticity3D.cpp:671-692 */
-        for (int in = 0; in < nsh; in++) {
-          if (type == 0) {
-            for (int k = 0; k < 3; k++) ef[3 * in + k] += big * v2[k] * phi[in] * weight;
-            for (int jn = 0; jn < nsh; jn++)
-              for (int k = 0; k < 3; k++)
-                ek[(3 * in + k) * ne + 3 * jn + k] += big * phi[in] * phi[jn] * weight;
-          } else {
-            for (int k = 0; k < 3; k++) ef[3 * in + k] += v2[k] * phi[in] * weight;
-          }
-        }
-      } else if (pb->material == ORC_POISSON) {
-        /* Material/Poisson/TPZMatPoisson.cpp:76-100 */
-        for (int in = 0; in < nsh; in++) {
-          if (type == 0) {
-            ef[in] += big * v2[0] * phi[in] * weight;
-            for (int jn = 0; jn < nsh; jn++) ek[in * ne + jn] += big * phi[in] * phi[jn] * weight;
-          } else {
-            ef[in] += v2[0] * pb->scale * phi[in] * weight;
-          }
-        }
-      } else {
-        /* Material/DarcyFlow/TPZDarcyFlow.cpp:65-78 */
-        for (int in = 0; in < nsh; in++) {
-          if (type == 0) {
-            ef[in] += (big * phi[in] * weight) * v2[0];
-            for (int jn = 0; jn < nsh; jn++) ek[in * ne + jn] += big * phi[in] * phi[jn] * weight;
-          } else {
-            ef[in] += v2[0] * (phi[in] * weight);
-          }
-        }
-      }
+      int face = m->bcface[iel];
+      double nrm[3] = {0, 0, 0};
+      if (pb->material == ORC_ELAST3D && pb->bc_type[face] == 4) bc_normal(dim, axes, vec, nrm);
+      if (bc_contribute(pb, face, nsh, ne, weight, phi, dphix, nrm, ek, ef) != 0) return -1;
       continue;
     }
     orc_contribute(pb->material, dim, nsh, ne, weight, phi, dphix, f,
@@ -1281,6 +1395,110 @@ int64_t orc_setup_pattern(const orc_mesh *m, int64_t **ia_out, int64_t **ja_out)
   free(ng);
   free(ngp);
   return nnz;
+}
+
+/* SetupMatrixData with an active TPZEquationFilter (StrMatrix/
+ * TPZSSpStructMatrix.cpp:49-193 with TPZEquationFilter::NumActive/Filter,
+ * StrMatrix/TPZEquationFilter.h): blocks without an active equation are
+ * skipped; every active equation of a block is a row of the condensed
+ * numbering dest[] (original equation -> condensed index, -1 inactive;
+ * increasing over the active equations); AddColEqs keeps the active
+ * equations of each column block, renumbered, with index >= the row.  The
+ * reference sizes Eq by whole blocks (Eq[totaleq] of the partially active
+ * blocks is never written) and EqCol by whole block pairs; the used part,
+ * returned here, is IA[0..nactive] and JA[0..IA[nactive]). */
+int64_t orc_setup_pattern_filtered(const orc_mesh *m, const int64_t *dest, int64_t **ia_out,
+                                   int64_t **ja_out) {
+  int64_t *ngp, *ng;
+  node_graph(m, &ngp, &ng);
+  const int64_t nb = m->nconn;
+  int64_t nact = 0, total = 0;
+  for (int64_t e = 0; e < m->neq; e++) nact += dest[e] >= 0;
+  for (int64_t i = 0; i < nb; i++) {
+    int64_t bs = m->blk_size[i];
+    total += bs * (bs + 1) / 2;
+    for (int64_t j = ngp[i]; j < ngp[i + 1]; j++)
+      if (ng[j] > i) total += bs * m->blk_size[ng[j]];
+  }
+  int64_t *ia = (int64_t *)malloc((nact + 1) * sizeof(int64_t));
+  int64_t *ja = (int64_t *)malloc((total ? total : 1) * sizeof(int64_t));
+  int64_t pos = 0, ieq = 0;
+#define ADDCOLEQS(csize, cpos)                                   \
+  for (int64_t l_ = 0; l_ < (csize); l_++) {                     \
+    int64_t d_ = dest[(cpos) + l_];                              \
+    if (d_ < 0 || d_ < ieq) continue;                            \
+    ja[pos++] = d_;                                              \
+  }
+  for (int64_t i = 0; i < nb; i++) {
+    int64_t bs = m->blk_size[i], bp = m->blk_pos[i], na = 0;
+    for (int64_t l = 0; l < bs; l++) na += dest[bp + l] >= 0;
+    if (!na) continue;
+    for (int64_t k = 0; k < bs; k++) {
+      if (dest[bp + k] < 0) continue;
+      if (dest[bp + k] != ieq) { free(ia); free(ja); free(ng); free(ngp); return -1; }
+      ia[ieq] = pos;
+      int diag = 0;
+      for (int64_t j = ngp[i]; j < ngp[i + 1]; j++) {
+        int64_t col = ng[j];
+        if (col < i) continue;
+        if (!diag && col > i) {
+          diag = 1;
+          ADDCOLEQS(bs, bp);
+        }
+        int64_t cs = m->blk_size[col], cp = m->blk_pos[col], ca = 0;
+        for (int64_t l = 0; l < cs; l++) ca += dest[cp + l] >= 0;
+        if (!ca) continue;
+        ADDCOLEQS(cs, cp);
+      }
+      if (!diag) { ADDCOLEQS(bs, bp); }
+      ieq++;
+    }
+  }
+#undef ADDCOLEQS
+  ia[ieq] = pos;
+  free(ng);
+  free(ngp);
+  *ia_out = ia;
+  *ja_out = ja;
+  return pos;
+}
+
+/* TPZStructMatrixOR::Assemble with an active filter (StrMatrix/
+ * pzstrmatrixor.cpp:41-71, 226-233): every element matrix passes through
+ * TPZEquationFilter::Filter(source, dest) -- inactive equations dropped, the
+ * others renumbered -- before AddKel / AddFel into the condensed system;
+ * the condensed load is then scattered into rhs_full[neq] (inactive 0). */
+int orc_assemble_filtered(const orc_problem *pb, const orc_mesh *m, const int64_t *dest,
+                          const int64_t *ia, const int64_t *ja, double *a, double *rhs_full) {
+  int64_t nact = 0;
+  for (int64_t e = 0; e < m->neq; e++) nact += dest[e] >= 0;
+  memset(a, 0, sizeof(double) * ia[nact]);
+  memset(rhs_full, 0, sizeof(double) * m->neq);
+  double *ek = (double *)malloc(sizeof(double) * ORC_MAXNE * ORC_MAXNE);
+  double *ekf = (double *)malloc(sizeof(double) * ORC_MAXNE * ORC_MAXNE);
+  double *rhsloc = (double *)calloc(nact ? nact : 1, sizeof(double));
+  double ef[ORC_MAXNE];
+  int64_t dst[ORC_MAXNE], src[ORC_MAXNE], dstf[ORC_MAXNE];
+  int rc = 0;
+  for (int64_t e = 0; e < m->nel && !rc; e++) {
+    int ne = orc_calcstiff(pb, m, e, ek, ef, dst);
+    if (ne < 0) { rc = -1; break; }
+    int nf = 0;
+    for (int i = 0; i < ne; i++)
+      if (dest[dst[i]] >= 0) { src[nf] = i; dstf[nf] = dest[dst[i]]; nf++; }
+    /* AddKel(elmat, source, destination): the filtered sub-matrix */
+    for (int i = 0; i < nf; i++)
+      for (int j = 0; j < nf; j++) ekf[i * nf + j] = ek[src[i] * ne + src[j]];
+    orc_addkel(ia, ja, a, nf, ekf, dstf);
+    for (int i = 0; i < nf; i++) rhsloc[dstf[i]] += ef[src[i]];
+  }
+  /* TPZEquationFilter::Scatter */
+  for (int64_t e = 0; e < m->neq; e++)
+    if (dest[e] >= 0) rhs_full[e] = rhsloc[dest[e]];
+  free(ek);
+  free(ekf);
+  free(rhsloc);
+  return rc;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -50,11 +50,14 @@ typedef struct {
   int nbumps;        /* Gaussian bumps: rows (cx,cy,cz,a0,a1,a2)             */
   const double *bumps;
   double bump_width; /* f = sum a exp(-|x-c|^2 / (2 w^2))                    */
-  int bc_type[6];    /* per boundary face: -1 none, 0 Dirichlet, 1 Neumann   */
-  double bc_val[6][3];
+  int bc_type[6];    /* per boundary face: -1 none, else the TPZBndCondT type
+                        (Poisson/Darcy 0..2, Elasticity3D 0..8)              */
+  double bc_val[6][3];  /* TPZBndCondT::Val2                                 */
   const int *el_porder; /* per volume element preferred order (1..p), NULL = p
                            everywhere; boundary elements keep the default p   */
   int simplex;       /* dim 3 only: 1 = 6 tetrahedra per cube (Kuhn split)     */
+  double bc_val1[6][9]; /* TPZBndCondT::Val1 per face, row-major (nstate x nstate
+                           used: 1x1 Poisson/Darcy, 3x3 Elasticity3D)          */
 } orc_problem;
 
 typedef struct {
@@ -116,6 +119,15 @@ int orc_collapsed_rule(int etype, int n, double *pts, double *w);
 void orc_contribute(int material, int dim, int nsh, int ne, double weight, const double *phi,
                     const double *dphix, const double *f, double kappa, double E, double nu,
                     double *ek, double *ef);
+
+/* SetupMatrixData / assembly with an active TPZEquationFilter: dest[neq] is
+ * the condensed index of each equation (-1 inactive, increasing over the
+ * active ones).  The pattern has nactive rows; the load is returned scattered
+ * back to the mesh numbering (rhs_full[neq], inactive 0). */
+int64_t orc_setup_pattern_filtered(const orc_mesh *m, const int64_t *dest, int64_t **ia_out,
+                                   int64_t **ja_out);
+int orc_assemble_filtered(const orc_problem *pb, const orc_mesh *m, const int64_t *dest,
+                          const int64_t *ia, const int64_t *ja, double *a, double *rhs_full);
 
 /* SetupMatrixData: returns nnz; ia (neq+1) and ja (nnz) are malloc'ed. */
 int64_t orc_setup_pattern(const orc_mesh *m, int64_t **ia, int64_t **ja);
