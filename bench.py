@@ -8,10 +8,18 @@ nu = 0.3, clamped base, seeded smooth body load.  Synthetic seeded data.
 A step = one numeric assembly (element integration + scatter into the
 reference's upper CSR, rhs) + one solve to ||f - K u|| / ||f|| <= 1e-8 in
 FP64, with the mesh, pattern and all inputs already resident in HBM.
-The measured path is the mixed-precision FCG (FP32 inner, FP64 refinement);
-the plain FP64 Jacobi-PCG GPU path is timed once beside it (same assembly,
-its own solve) for the speed-up, and the CPU oracle is timed on a bounded
-sample for the CPU baseline.
+The measured path is the learned-preconditioned mixed-precision FCG: FP32
+flexible CG inside FP64 iterative refinement, preconditioned by the multigrid
+V-cycle whose p-level smoothing is the learned element smoother (trained
+before the timed region, with the project's kernels, on seeded 12^3 training
+systems of the same workload family).  Timed once beside it on the same
+assembly: the same FCG with the plain (block-Jacobi) multigrid cycle, the
+block-Jacobi mixed FCG, and the plain FP64 Jacobi-PCG GPU path (the
+denominator of the speed-up).  Every solve reports whether it met the FP64
+true-residual target.  The CPU baseline is the oracle (a C restatement of the
+reference path) on the full workload: coloured multi-threaded assembly, and
+Jacobi-PCG iterations timed on the full system and scaled to the FP64 PCG
+iteration count.
 """
 import argparse
 import json
@@ -37,7 +45,10 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-baseline", action="store_true", help="skip the FP64 Jacobi-PCG run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle sample")
-    ap.add_argument("--precond", default="multigrid", choices=["multigrid", "block_jacobi", "learned"])
+    ap.add_argument("--precond", default="learned", choices=["multigrid", "block_jacobi", "learned"])
+    ap.add_argument("--train-n", type=int, default=12, help="elements per direction of the training systems")
+    ap.add_argument("--train-seeds", type=int, default=4)
+    ap.add_argument("--train-steps", type=int, default=150)
     ap.add_argument("--mg-degree", type=int, default=0, help="multigrid Chebyshev degree (0 = 2)")
     ap.add_argument("--mg-range", type=float, default=0.0, help="multigrid smoothing interval ratio (0 = 30)")
     ap.add_argument("--no-bj", action="store_true", help="skip the block-Jacobi mixed solve beside multigrid")
@@ -69,54 +80,42 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(pb, iters_target, seed, tol):
-    """Times the CPU oracle (a scalar C port of the reference path, 1 thread).
-    Small problems (<= 300k equations) run whole: assembly + Jacobi-PCG to the
-    same tolerance.  Larger ones run a bounded sample: full assembly + 10
-    Jacobi-PCG iterations of a 20^3 instance of the same elasticity workload,
-    scaled per element and per stored entry to the target size and to the
-    GPU FP64 PCG iteration count."""
+def cpu_baseline(pb, iters_target, tol):
+    """Times the CPU oracle (a C restatement of the reference path) on the
+    full workload.  Small problems (<= 300k equations): pattern, assembly and
+    Jacobi-PCG to the tolerance, all measured.  Larger ones: the assembly of
+    the whole system measured (coloured element loop of TPZStructMatrixOT,
+    one thread per host core up to 16), then 3 Jacobi-PCG iterations of the
+    whole system (one thread, the reference's EJacobi step with its discarded
+    residual product) measured and scaled to the GPU FP64 PCG iteration
+    count."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    from neopz_amd import problems as P
+    nth = max(1, min(16, os.cpu_count() or 1))
+    o = O.Oracle(pb)
+    ia, ja = o.pattern()
+    t0 = time.perf_counter()
+    a, rhs = o.assemble_colored(nth)
+    t_asm = time.perf_counter() - t0
     if pb.n_equations() <= 300000:
-        o = O.Oracle(pb)
-        t0 = time.perf_counter()
-        ia, ja = o.pattern()
-        a, rhs = o.assemble()
-        t_asm = time.perf_counter() - t0
         t0 = time.perf_counter()
         r = O.pcg(ia, ja, a, rhs, maxit=1000000, tol=tol, wasted_spmv=True)
         t_sol = time.perf_counter() - t0
-        return {"value": t_asm + t_sol, "unit": "s", "cores": 1, "kind": "port",
-                "sample": f"oracle/pz_oracle.c, whole problem: pattern+assembly {t_asm:.2f}s, "
-                          f"Jacobi-PCG {r['iters']} iterations {t_sol:.2f}s (reference semantics incl. "
-                          f"the discarded residual SpMV)",
-                "assembly_s": t_asm, "iterations": r["iters"]}
-    ns = 20
-    spb = P.elasticity3d(n=ns, p=2, seed=seed, field_kind="grf", corr_len=0.1,
-                         load="body" if pb.bc_type[4] >= 0 else "traction")
-    o = O.Oracle(spb)
-    ia, ja = o.pattern()
-    t0 = time.perf_counter()
-    a, rhs = o.assemble()
-    t_asm = time.perf_counter() - t0
-    k = 10
+        return {"value": t_asm + t_sol, "unit": "s", "cores": nth, "kind": "port",
+                "sample": f"oracle/pz_oracle.c on the whole problem: coloured assembly on {nth} threads "
+                          f"{t_asm:.2f}s, Jacobi-PCG (1 thread) {r['iters']} iterations {t_sol:.2f}s",
+                "assembly_s": t_asm, "iterations": r["iters"], "extrapolated": False}
+    k = 3
     t0 = time.perf_counter()
     O.pcg(ia, ja, a, rhs, maxit=k, tol=0.0, wasted_spmv=True)
     t_it = (time.perf_counter() - t0) / k
-    nel_s, nnz_s = ns ** 3, len(ja)
-    nel_t = pb.nvol
-    nnz_t = O.nnz_closed_form(3, pb.p, pb.nstate, pb.nx, pb.ny, pb.nz)
-    asm_t = t_asm / nel_s * nel_t
-    it_t = t_it / nnz_s * nnz_t
-    total = asm_t + it_t * iters_target
-    return {"value": total, "unit": "s", "cores": 1, "kind": "port",
-            "sample": f"oracle/pz_oracle.c on a {ns}^3 instance ({nel_s} hexes, {nnz_s} upper "
-                      f"entries): assembly {t_asm:.2f}s, Jacobi-PCG {t_it * 1e3:.1f} ms/iteration "
-                      f"(reference semantics incl. the discarded residual SpMV); scaled by elements "
-                      f"and stored entries to {pb.nx}^3 and x {iters_target} iterations",
-            "assembly_s": asm_t, "s_per_iteration": it_t}
+    total = t_asm + t_it * iters_target
+    return {"value": total, "unit": "s", "cores": nth, "kind": "port",
+            "sample": f"oracle/pz_oracle.c on the whole {pb.nvol}-element system ({len(ja)} stored entries): "
+                      f"coloured assembly on {nth} threads {t_asm:.2f}s measured; Jacobi-PCG (1 thread, "
+                      f"reference semantics incl. the discarded residual product) {t_it:.2f}s/iteration "
+                      f"measured over {k} iterations, scaled to the GPU FP64 PCG's {iters_target} iterations",
+            "assembly_s": t_asm, "s_per_iteration": t_it, "extrapolated": True}
 
 
 def traffic_of(kernel):
@@ -169,6 +168,40 @@ def kernels_only(ctx, L):
     print(json.dumps({"kernels": res, "copy_bytes": nb}))
 
 
+def train_smoother(args, L):
+    """Trains the learned element smoother in its own context on seeded
+    training systems of the same workload family (args.train_n elements per
+    direction, coefficient correlation scaled to the same number of
+    elements), outside the timed region.  Returns the weights and the loss."""
+    from neopz_amd import mesh as M
+    from neopz_amd import problems as P
+    from neopz_amd.analysis import Context, TPZSSpStructMatrixHIP
+    from neopz_amd.learned import train_on_seeds
+    t0 = time.perf_counter()
+    big = P.config(args.config, seed=args.seed)
+    nt = args.train_n
+
+    def make(seed):
+        kw = {"n": nt}
+        if args.config in ("C5",):
+            kw["corr_len"] = 0.1 * big.nx / nt
+        if args.config in ("C4p1", "C4p2"):
+            kw["corr_len"] = 0.05 * big.nx / nt
+        return P.config(args.config, seed=seed, **kw)
+    tctx = Context(0)
+    sm = TPZSSpStructMatrixHIP(tctx, M.build(make(1000)))
+    K = sm.Create()
+    sm.Assemble(K)
+    seeds = [1001 + k for k in range(args.train_seeds)]
+    ls, l0, l1 = train_on_seeds(tctx, sm, K, lambda s: make(s).el_coef, seeds, nrhs=4,
+                                steps=args.train_steps, lr=3e-3)
+    w = ls.get_weights()
+    tctx.close()
+    return {"weights": w, "loss_first": l0, "loss_last": l1, "seconds": time.perf_counter() - t0,
+            "systems": f"{len(seeds)} seeded {nt}-per-direction systems of the same family, "
+                       f"{args.train_steps} Adam steps each"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -208,7 +241,7 @@ def main():
                            refine=args.refine, inner_op=op, mg_degree=args.mg_degree,
                            mg_range=args.mg_range)
             # multigrid iterations are few and costly: the solver's short batches
-            st.check_every = 0 if pre == L.PRECOND_MULTIGRID else 100
+            st.check_every = 0 if pre in (L.PRECOND_MULTIGRID, L.PRECOND_LEARNED) else 100
         st.use_graph = not args.no_graph
         return st
 
@@ -216,6 +249,19 @@ def main():
         L.check(ctx.lib.pzg_assemble(ctx.h))
         st.Solve(None, want_solution=False)
         return st.info
+
+    trained = None
+    if args.precond == "learned":
+        trained = train_smoother(args, L)
+        from neopz_amd.learned import LearnedSmoother
+        ls = LearnedSmoother(ctx)
+        ls.setup(seed=0)
+        ls.set_weights(trained["weights"])
+        log(f"trained smoother: loss {trained['loss_first']:.4f} -> {trained['loss_last']:.4f} "
+            f"({trained['seconds']:.1f}s)")
+
+    def meets(info):
+        return bool(info["converged"] == 1 and info["true_residual"] <= args.tol)
 
     st = solver("mixed")
     for w in range(args.warmup):
@@ -237,15 +283,24 @@ def main():
     # solutions leave the device outside the timed regions
     u_mixed = np.zeros(md["neq"])
     L.check(ctx.lib.pzg_get_solution(ctx.h, L.ptr(u_mixed, L.dp)))
-    # the block-Jacobi mixed solve on the same system, for comparison
-    bj = None
-    if args.precond == "multigrid" and not args.no_bj:
-        stj = solver("mixed", "block_jacobi")
-        t0 = time.perf_counter()
-        ij = step(stj)
+    # the same FCG with the plain multigrid cycle and the block-Jacobi mixed
+    # solve on the same system, for comparison
+    def beside(pre):
+        stx = solver("mixed", pre)
+        step(stx)  # warm-up (graph instantiation, buffers)
         L.check(ctx.lib.pzg_sync(ctx.h))
-        bj = {"seconds": time.perf_counter() - t0, "iterations": ij["iterations"],
-              "true_residual": ij["true_residual"]}
+        t0 = time.perf_counter()
+        ix = step(stx)
+        L.check(ctx.lib.pzg_sync(ctx.h))
+        return {"seconds": time.perf_counter() - t0, "iterations": ix["iterations"],
+                "true_residual": ix["true_residual"], "fallback": ix["fallback"], "meets_tol": meets(ix)}
+    mg = None
+    if args.precond == "learned":
+        mg = beside("multigrid")
+        log(f"mixed multigrid (block-Jacobi smoothing): {mg}")
+    bj = None
+    if args.precond != "block_jacobi" and not args.no_bj:
+        bj = beside("block_jacobi")
         log(f"mixed block Jacobi: {bj}")
     # plain FP64 Jacobi-PCG GPU path (denominator of the speed-up)
     base = None
@@ -261,7 +316,8 @@ def main():
         # against the CPU oracle at small sizes), both at residual 1e-8
         rel = float(np.linalg.norm(u_mixed - u64) / np.linalg.norm(u64))
         base = {"seconds": tb, "iterations": ib["iterations"], "true_residual": ib["true_residual"],
-                "achieved_recursive": ib["achieved"], "mixed_solution_rel_diff": rel}
+                "achieved_recursive": ib["achieved"], "mixed_solution_rel_diff": rel,
+                "meets_tol": meets(ib)}
         log(f"pcg64: {tb:.3f}s {ib}, |u_mixed - u_pcg64|/|u_pcg64| = {rel:.2e}")
     # roofline of the dominant kernel (FP32 block CSR SpMV of the inner solve)
     import ctypes as C
@@ -305,22 +361,36 @@ def main():
     cpu = None
     if not args.no_cpu:
         its = base["iterations"] if base else infos[-1]["iterations"]
-        cpu = cpu_baseline(pb, its, args.seed, args.tol)
+        cpu = cpu_baseline(pb, its, args.tol)
+    measured_ok = all(meets(i) for i in infos)
+    speedup = None
+    if base and base["meets_tol"] and measured_ok:
+        speedup = base["seconds"] / value
+    metric_name = {"learned": "assembly+solve time to rel. residual 1e-8, learned-FCG vs Jacobi-PCG, 1 GPU",
+                   "multigrid": "assembly+solve time to rel. residual 1e-8, multigrid-FCG (not the learned "
+                                "metric), 1 GPU",
+                   "block_jacobi": "assembly+solve time to rel. residual 1e-8, block-Jacobi FCG (not the "
+                                   "learned metric), 1 GPU"}[args.precond]
     out = {
-        "metric": "assembly+solve time to rel. residual 1e-8, learned-FCG vs Jacobi-PCG, 1 GPU",
+        "metric": metric_name,
         "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 inner / f64 refinement+assembly", "data": "synthetic",
         "config": {"workload": WORKLOADS[args.config].format(n=n) +
-                               f"; mixed FCG ({args.precond}, {args.inner_op} inner operator) + "
-                               f"FP64 refinement",
+                               f"; mixed FCG ({args.precond}"
+                               f"{' element smoother in the multigrid cycle' if args.precond == 'learned' else ''}"
+                               f", {args.inner_op} inner operator) + FP64 refinement",
                    "name": args.config,
                    "neq": sizes["neq"], "nnz_upper": sizes["nnz_upper"],
                    "nnz_full": sizes["nnz_full"], "elements": sizes["nvol"], "seed": args.seed},
-        "solve": infos[-1],
+        "solve": dict(infos[-1], meets_tol=measured_ok),
+        "learned_training": ({k: v for k, v in trained.items() if k != "weights"} if trained else None),
+        "multigrid_fcg": mg,
         "jacobi_pcg64_gpu": base,
         "mixed_block_jacobi": bj,
-        "speedup_vs_pcg64": (base["seconds"] / value) if base else None,
+        "speedup_vs_pcg64": speedup,
+        "speedup_note": None if speedup is not None else
+        "not computed: a solve missed the FP64 true-residual target or the baseline was skipped",
         "assembly_ms": asm_ms,
         "elements_per_s": sizes["nvol"] / (asm_ms * 1e-3),
         "spmv": {"fp64_block_csr": {"ms": spmv64_ms, "GBps": spmv64_gbs,

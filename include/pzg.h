@@ -205,7 +205,12 @@ typedef struct {
   double achieved;        /* PCG64: recursive ||r||/||b|| (TPZStepSolver fTol)  */
   double true_residual;   /* ||f - K64 u|| / ||f|| in FP64                      */
   double solve_ms;        /* device time of the solve                           */
-  int32_t fallback;       /* 1 if the learned preconditioner was replaced       */
+  int32_t fallback;       /* mixed: 1 if the FP32 iterations stalled (the true
+                             residual did not halve over three refinement
+                             steps, the refinement budget ran out, or with the
+                             learned preconditioner one step gained < 10%)
+                             and plain FP64 Jacobi-PCG (CG<> + EJacobi, from
+                             the current solution) finished the solve        */
 } pzg_solve_info;
 
 /* ---- lifecycle ---------------------------------------------------------- */
@@ -439,32 +444,54 @@ int pzg_sync(pzg_ctx *ctx);
 int pzg_mesh_set_coef(pzg_ctx *ctx, const double *vol_coef);
 
 /* ---- learned preconditioner (no counterpart in the reference) -------------
- * A message-passing network on the aggregated block graph that adds a coarse
- * correction to the block-Jacobi step of the mixed solver
- * (PZG_PRECOND_LEARNED).  NeoPZ has no learned component; these entry points
+ * PZG_PRECOND_LEARNED: the multigrid V-cycle of PZG_PRECOND_MULTIGRID whose
+ * smoothing on the p level uses, in place of the block-Jacobi M^-1, the
+ * learned element smoother
+ *     z = sum_e s_e P_e^T G P_e r  (z_i = r_i on Dirichlet-penalty equations),
+ * G one learned symmetric ne x ne matrix (ne = shapes x states of the volume
+ * element) and s_e the output of a message-passing network (3 layers, 32
+ * features) on the element graph (elements adjacent through a face) from
+ * four features of the element's coefficient field and boundary.  Trained
+ * with the project's kernels on (residual, error) pairs of the solver's local
+ * problems: per element of a training system, random residuals r and the
+ * errors e = S_e^-1 r of its block S_e of the assembled scaled matrix.
+ * Quadrilateral / hexahedral lattice meshes with one rule, no constraints,
+ * no equation filter.  NeoPZ has no learned component; these entry points
  * serve the mixed solver only. */
-/* Builds aggregates (coarsen x coarsen [x coarsen] elements), the coarse graph
- * and random initial weights (output layer zero = block Jacobi); reserves
- * room for max_samples training pairs. */
-int pzg_learned_setup(pzg_ctx *ctx, int32_t coarsen, uint64_t seed, int32_t max_samples);
-/* Generates (residual, error) pairs with the block-Jacobi FCG on the assembled
- * system: nrhs random solutions, pairs stored at the given iterations. */
-int pzg_learned_collect(pzg_ctx *ctx, int32_t nrhs, const int32_t *record_iters, int32_t nrec,
-                        uint64_t seed);
-/* Full-batch Adam on the stored pairs; mean loss per pair before/after. */
-int pzg_learned_train(pzg_ctx *ctx, int32_t steps, double lr, double *loss_first,
-                      double *loss_last);
-/* Reads (get) and/or writes (set) the nparams FP32 weights; *n = nparams. */
+/* Parameters for the context's mesh: G = diag(1 / elements sharing the side
+ * of each dof's shape), network weights from seed, output layer 0 (s = 1). */
+int pzg_learned_setup(pzg_ctx *ctx, uint64_t seed);
+/* Training pairs of the assembled system (replacing earlier ones): nrhs
+ * (1..8) Gaussian residuals per element (zero on penalized dofs, seeded) and
+ * their local errors. */
+int pzg_learned_collect(pzg_ctx *ctx, int32_t nrhs, uint64_t seed);
+/* Training: G = the least-squares fit B A^-1 (A = sum r r^T, B = sum e r^T
+ * over the pairs of every collect since pzg_learned_setup, elements without
+ * penalized dofs), symmetrised and projected onto the positive definite
+ * matrices; then steps Adam steps (learning rate lr) of the network on the
+ * stored pairs.  The loss sum |mask (s_e G r - e)|^2 / sum |e|^2 after the
+ * fit and after the last step. */
+int pzg_learned_train(pzg_ctx *ctx, int32_t steps, double lr, double *loss_first, double *loss_last);
+/* Reads (get) and/or writes (set) the FP32 parameters; *n = their number.
+ * Layout: G (ne x ne, row-major), W0 (4 x 32), A_l, B_l (32 x 32, l = 1..3,
+ * interleaved A_1 B_1 A_2 ...), w (32); weights [in][out] row-major. */
 int pzg_learned_weights(pzg_ctx *ctx, float *get, const float *set, int32_t *n);
-/* Coarse graph (cia[nc+1], cja[cia[nc]]), aggregate of every block (agg[nb])
- * and the number of stored pairs; arrays may be NULL. */
-int pzg_learned_graph(pzg_ctx *ctx, int64_t *nc, int64_t *cia, int32_t *cja, int32_t *agg,
-                      int64_t *nsamples);
-/* Copies the stored pairs X, T ([sample][aggregate][bs] FP32). */
-int pzg_learned_samples(pzg_ctx *ctx, float *X, float *T);
-/* Loss and gradient of the network on caller data X, T (ns samples). */
-int pzg_learned_loss_grad(pzg_ctx *ctx, int64_t ns, const float *X, const float *T, double *loss,
-                          float *grad);
+/* The element graph (cia[nel+1], cja[cia[nel]]), the features of the
+ * current assembly ([nel][4]) and the penalized-equation mask ([neq], 1 =
+ * penalized); any pointer may be NULL. */
+int pzg_learned_graph(pzg_ctx *ctx, int64_t *nel, int64_t *cia, int32_t *cja, float *features,
+                      float *pen);
+/* The stored pairs R, E ([element][pair][ne]) and the dof mask ([element][ne],
+ * 0 on penalized or absent dofs); *ne and *nr receive their sizes. */
+int pzg_learned_samples(pzg_ctx *ctx, int32_t *ne, int32_t *nr, float *R, float *E, float *mask);
+/* Loss and gradient (nparams floats) of the current parameters on the stored
+ * pairs. */
+int pzg_learned_loss_grad(pzg_ctx *ctx, double *loss, float *grad);
+/* z = M r of the learned smoother of the current assembly on host vectors
+ * of length neq (scaled unknowns, FP32 on the device). */
+int pzg_learned_apply(pzg_ctx *ctx, const double *r, double *z);
+/* The scales s_e ([nel]) the network gives for the current assembly. */
+int pzg_learned_scales(pzg_ctx *ctx, float *s);
 
 #ifdef __cplusplus
 }

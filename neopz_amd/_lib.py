@@ -25,7 +25,8 @@ EXPORTS = [
     "pzg_learned_train", "pzg_learned_weights", "pzg_learned_graph", "pzg_learned_samples",
     "pzg_learned_loss_grad", "pzg_scaled_apply", "pzg_mg_apply", "pzg_full_csr", "pzg_hp_tuning",
     "pzg_load_solution", "pzg_error_points", "pzg_evaluate_error", "pzg_sloan_renumber",
-    "pzg_get_solution", "pzg_set_equation_filter",
+    "pzg_get_solution", "pzg_set_equation_filter", "pzg_learned_scales",
+    "pzg_learned_apply",
 ]
 
 dp = C.POINTER(C.c_double)
@@ -122,13 +123,15 @@ def load():
     L.pzg_get_sizes.argtypes = [vp, C.POINTER(Sizes)]
     L.pzg_sync.argtypes = [vp]
     L.pzg_mesh_set_coef.argtypes = [vp, dp]
-    L.pzg_learned_setup.argtypes = [vp, C.c_int32, C.c_uint64, C.c_int32]
-    L.pzg_learned_collect.argtypes = [vp, C.c_int32, i32p, C.c_int32, C.c_uint64]
+    L.pzg_learned_setup.argtypes = [vp, C.c_uint64]
+    L.pzg_learned_collect.argtypes = [vp, C.c_int32, C.c_uint64]
     L.pzg_learned_train.argtypes = [vp, C.c_int32, C.c_double, dp, dp]
     L.pzg_learned_weights.argtypes = [vp, fp, fp, i32p]
-    L.pzg_learned_graph.argtypes = [vp, i64p, i64p, i32p, i32p, i64p]
-    L.pzg_learned_samples.argtypes = [vp, fp, fp]
-    L.pzg_learned_loss_grad.argtypes = [vp, C.c_int64, fp, fp, dp, fp]
+    L.pzg_learned_graph.argtypes = [vp, i64p, i64p, i32p, fp, fp]
+    L.pzg_learned_samples.argtypes = [vp, i32p, i32p, fp, fp, fp]
+    L.pzg_learned_loss_grad.argtypes = [vp, dp, fp]
+    L.pzg_learned_scales.argtypes = [vp, fp]
+    L.pzg_learned_apply.argtypes = [vp, dp, dp]
     L.pzg_scaled_apply.argtypes = [vp, C.c_int32, dp, dp]
     L.pzg_mg_apply.argtypes = [vp, C.c_int32, C.c_int32, C.c_double, dp, dp, i32p, i32p, dp]
     L.pzg_full_csr.argtypes = [vp, i64p, i64p, dp]

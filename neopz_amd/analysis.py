@@ -343,6 +343,8 @@ class TPZSSpStructMatrixHIP:
         if not self._flattened:
             L.check(self.ctx.lib.pzg_mesh_set(self.ctx.h, C.byref(self._desc)))
             self._flattened = True
+
+    def _apply_filter(self):
         f = self._filter
         if f.IsActive():
             act = np.ascontiguousarray(f.fActiveEqs, dtype=np.int64)
@@ -353,6 +355,7 @@ class TPZSSpStructMatrixHIP:
     def Create(self):
         """TPZSSpStructMatrix::Create (StrMatrix/TPZSSpStructMatrix.cpp:30-47)."""
         self._flatten()
+        self._apply_filter()
         neq = C.c_int64()
         nnz = C.c_int64()
         L.check(self.ctx.lib.pzg_struct_create(self.ctx.h, C.byref(neq), C.byref(nnz)))

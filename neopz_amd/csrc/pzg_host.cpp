@@ -62,12 +62,12 @@ static void free_all(Ctx &c) {
                     &c.d_tbc, &c.d_vcolor_list, &c.d_bcolor_list, &c.d_x, &c.d_r, &c.d_z,
                     &c.d_p, &c.d_p2, &c.d_q, &c.d_u, &c.d_f, &c.d_w, &c.d_xs, &c.d_rs,
                     &c.d_zs, &c.d_ps, &c.d_ps2, &c.d_qs, &c.d_bs, &c.d_scal, &c.d_partials,
-                    &c.d_counter, &c.d_nn_agg, &c.d_nn_captr, &c.d_nn_calist, &c.d_nn_cia,
-                    &c.d_nn_cja, &c.d_nn_w8, &c.d_nn_w, &c.d_nn_m, &c.d_nn_v, &c.d_nn_g,
-                    &c.d_nn_X, &c.d_nn_T, &c.d_nn_xi, &c.d_nn_zi, &c.d_nn_h0, &c.d_nn_h1,
-                    &c.d_nn_mi, &c.d_nn_trH, &c.d_nn_trM, &c.d_nn_trP, &c.d_nn_dH, &c.d_nn_G,
-                    &c.d_nn_tmp, &c.d_nn_Z, &c.d_nn_dZ, &c.d_nn_den, &c.d_nn_part,
-                    &c.d_nn_loss, &c.d_nn_ystar, &c.d_nn_e, &c.d_post_tab, &c.d_post_list, &c.d_post_slot,
+                    &c.d_counter, &c.d_res2, &c.d_les_th, &c.d_les_m, &c.d_les_v, &c.d_les_g,
+                    &c.d_les_cia, &c.d_les_cja, &c.d_les_pen, &c.d_les_X, &c.d_les_z, &c.d_les_s,
+                    &c.d_les_ye, &c.d_les_Se, &c.d_les_M, &c.d_les_R, &c.d_les_E, &c.d_les_D,
+                    &c.d_les_le, &c.d_les_ds, &c.d_les_loss, &c.d_les_fail, &c.d_nn_trH, &c.d_nn_trM,
+                    &c.d_nn_trP, &c.d_nn_dH, &c.d_nn_G, &c.d_nn_tmp, &c.d_nn_dZ, &c.d_nn_part,
+                    &c.d_filt_dest, &c.d_post_tab, &c.d_post_list, &c.d_post_slot,
                     &c.d_post_off, &c.d_post_out, &c.d_post_uq, &c.d_post_duq, &c.d_post_xq,
                     &c.d_hp_vsub, &c.d_hp_bsub, &c.d_thp_vol, &c.d_thp_bc, &c.d_mf_blk,
                     &c.d_mf_xe, &c.d_mf_bblk, &c.d_mf_bm, &c.d_mf_ptr, &c.d_mf_pos, &c.d_mf_bpos,
@@ -260,7 +260,7 @@ static int bc_setup(Ctx &c, double big) {
       }
       penalty_only = type == PZG_BC_DIRICHLET || type == PZG_BC_NEUMANN;
     }
-    if (!ok) {
+    if (!ok && used[id]) {  // ids without boundary elements are never read
       set_error("pzg_mesh_set: boundary condition type " + std::to_string(type) + " of bc id " +
                 std::to_string(id) + " is not one of the material's ContributeBC types");
       return PZG_EUNSUP;
@@ -687,6 +687,7 @@ static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
                          sizeof(HpTables) * c.thp_bc.size(), s));
   }
   PZG_HIP(hipStreamSynchronize(s));
+  c.les_graph = false;
   // a new mesh has its own numbering: no equation filter
   c.filt = false;
   c.filt_active.clear();
@@ -1020,6 +1021,7 @@ int pzg_set_equation_filter(pzg_ctx *ctx, int64_t nactive, const int64_t *active
     return PZG_ESTATE;
   }
   if (nactive < 0 || !active) {
+    if (!c.filt) return PZG_OK;  // no filter before either: nothing changes
     c.filt = false;
     c.filt_active.clear();
     c.filt_dest.clear();
@@ -1641,7 +1643,10 @@ int pzg_mg_apply(pzg_ctx *ctx, int32_t op, int32_t degree, double range, const d
   c.mg_degree = degree > 0 ? degree : 2;
   c.mg_range = range > 0 ? range : 30.;
   const bool use_mf = op == PZG_INNER_MATFREE;
-  if (!c.mg_ready || c.mg_use_mf != use_mf) PZG_TRY(mg_setup(c, use_mf));
+  if (!c.mg_ready || c.mg_use_mf != use_mf || c.mg_les) {
+    c.mg_les = false;  // the plain cycle (block-Jacobi smoothing)
+    PZG_TRY(mg_setup(c, use_mf));
+  }
   std::vector<float> h(n);
   for (int64_t i = 0; i < n; i++) h[i] = (float)r[i];
   PZG_TRY(dev_alloc(c.d_rs, n * sizeof(float)));
@@ -1679,114 +1684,146 @@ int pzg_mesh_set_coef(pzg_ctx *ctx, const double *vol_coef) {
   return PZG_OK;
 }
 
-int pzg_learned_setup(pzg_ctx *ctx, int32_t coarsen, uint64_t seed, int32_t max_samples) {
+int pzg_learned_setup(pzg_ctx *ctx, uint64_t seed) {
   CTX_CHECK();
-  return nn_setup(ctx->c, coarsen, seed, max_samples);
+  return les_setup(ctx->c, seed);
 }
 
-int pzg_learned_collect(pzg_ctx *ctx, int32_t nrhs, const int32_t *record_iters, int32_t nrec,
-                        uint64_t seed) {
+int pzg_learned_collect(pzg_ctx *ctx, int32_t nrhs, uint64_t seed) {
   CTX_CHECK();
-  Ctx &c = ctx->c;
-  if (!c.assembled || nrhs < 0 || nrec <= 0 || !record_iters) {
-    set_error("pzg_learned_collect: needs an assembled system and record iterations");
-    return PZG_ESTATE;
-  }
-  for (int k = 0; k < nrec; k++)
-    if (record_iters[k] < 1 || record_iters[k] > 100000) {
-      set_error("pzg_learned_collect: record iterations must be in [1, 100000]");
-      return PZG_EINVAL;
-    }
-  return nn_collect(c, nrhs, record_iters, nrec, seed);
+  return les_collect(ctx->c, nrhs, seed);
 }
 
-int pzg_learned_train(pzg_ctx *ctx, int32_t steps, double lr, double *loss_first,
-                      double *loss_last) {
+int pzg_learned_train(pzg_ctx *ctx, int32_t steps, double lr, double *loss_first, double *loss_last) {
   CTX_CHECK();
   Ctx &c = ctx->c;
-  if (!c.nn_ready || c.nn_nsamp == 0) {
-    set_error("pzg_learned_train: no training pairs (pzg_learned_collect)");
-    return PZG_ESTATE;
+  if (steps < 0 || !(lr > 0.)) {
+    set_error("pzg_learned_train: invalid arguments");
+    return PZG_EINVAL;
   }
-  double l = 0;
+  double l = 0.;
+  PZG_TRY(les_fit_G(c));  // G: least squares on every pair collected so far
   for (int t = 0; t < steps; t++) {
-    PZG_TRY(nn_loss_grad(c, c.nn_nsamp, c.d_nn_X.as<float>(), c.d_nn_T.as<float>(),
-                         (t == 0 || t == steps - 1) ? &l : nullptr));
-    if (t == 0 && loss_first) *loss_first = l / c.nn_nsamp;
-    PZG_TRY(nn_adam_step(c, lr));
+    PZG_TRY(les_loss_grad(c, t == 0 ? &l : nullptr));
+    if (t == 0 && loss_first) *loss_first = l;
+    PZG_TRY(les_adam_step(c, lr));
   }
-  PZG_TRY(nn_loss_grad(c, c.nn_nsamp, c.d_nn_X.as<float>(), c.d_nn_T.as<float>(), &l));
-  if (loss_last) *loss_last = l / c.nn_nsamp;
+  PZG_TRY(les_loss_grad(c, &l));
+  if (loss_last) *loss_last = l;
+  if (steps == 0 && loss_first) *loss_first = l;
   return PZG_OK;
 }
 
 int pzg_learned_weights(pzg_ctx *ctx, float *get, const float *set, int32_t *n) {
   CTX_CHECK();
   Ctx &c = ctx->c;
-  int np = nn_nparams(c.bs);
+  if (!c.les_ready) {
+    set_error("pzg_learned_weights: call pzg_learned_setup first");
+    return PZG_ESTATE;
+  }
+  const int np = les_nparams(c.les_ne);
   if (n) *n = np;
-  if (!c.nn_ready) {
-    if (get || set) {
-      set_error("pzg_learned_weights: call pzg_learned_setup first");
+  if (set) {
+    PZG_HIP(hipMemcpyAsync(c.d_les_th.p, set, sizeof(float) * np, hipMemcpyHostToDevice, c.stream));
+    c.mg_ready = false;
+  }
+  if (get) PZG_HIP(hipMemcpyAsync(get, c.d_les_th.p, sizeof(float) * np, hipMemcpyDeviceToHost, c.stream));
+  PZG_HIP(hipStreamSynchronize(c.stream));
+  return PZG_OK;
+}
+
+int pzg_learned_graph(pzg_ctx *ctx, int64_t *nel, int64_t *cia, int32_t *cja, float *features,
+                      float *pen) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (!c.les_ready || !c.assembled) {
+    set_error("pzg_learned_graph: needs pzg_learned_setup and an assembled system");
+    return PZG_ESTATE;
+  }
+  PZG_TRY(les_prepare(c));
+  if (nel) *nel = c.nvol;
+  if (cia) std::copy(c.les_cia.begin(), c.les_cia.end(), cia);
+  if (cja) std::copy(c.les_cja.begin(), c.les_cja.end(), cja);
+  if (pen) std::copy(c.les_pen.begin(), c.les_pen.end(), pen);
+  if (features) {
+    PZG_HIP(hipMemcpyAsync(features, c.d_les_X.p, sizeof(float) * 4 * c.nvol, hipMemcpyDeviceToHost, c.stream));
+    PZG_HIP(hipStreamSynchronize(c.stream));
+  }
+  return PZG_OK;
+}
+
+int pzg_learned_samples(pzg_ctx *ctx, int32_t *ne, int32_t *nr, float *R, float *E, float *mask) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (ne) *ne = c.les_ne;
+  if (nr) *nr = (int32_t)c.les_nr;
+  if (!c.les_ready || c.les_nr == 0) {
+    if (R || E || mask) {
+      set_error("pzg_learned_samples: no training pairs");
       return PZG_ESTATE;
     }
     return PZG_OK;
   }
-  if (get) PZG_HIP(hipMemcpy(get, c.d_nn_w.p, sizeof(float) * np, hipMemcpyDeviceToHost));
-  if (set) PZG_HIP(hipMemcpy(c.d_nn_w.p, set, sizeof(float) * np, hipMemcpyHostToDevice));
+  const size_t n = (size_t)c.nvol * c.les_nr * c.les_ne;
+  if (R) PZG_HIP(hipMemcpyAsync(R, c.d_les_R.p, sizeof(float) * n, hipMemcpyDeviceToHost, c.stream));
+  if (E) PZG_HIP(hipMemcpyAsync(E, c.d_les_E.p, sizeof(float) * n, hipMemcpyDeviceToHost, c.stream));
+  if (mask)
+    PZG_HIP(hipMemcpyAsync(mask, c.d_les_M.p, sizeof(float) * c.nvol * c.les_ne, hipMemcpyDeviceToHost,
+                           c.stream));
+  PZG_HIP(hipStreamSynchronize(c.stream));
   return PZG_OK;
 }
 
-int pzg_learned_graph(pzg_ctx *ctx, int64_t *nc, int64_t *cia, int32_t *cja, int32_t *agg,
-                      int64_t *nsamples) {
+int pzg_learned_loss_grad(pzg_ctx *ctx, double *loss, float *grad) {
   CTX_CHECK();
   Ctx &c = ctx->c;
-  if (!c.nn_ready) {
-    set_error("pzg_learned_graph: call pzg_learned_setup first");
-    return PZG_ESTATE;
+  PZG_TRY(les_loss_grad(c, loss));
+  if (grad) {
+    PZG_HIP(hipMemcpyAsync(grad, c.d_les_g.p, sizeof(float) * les_nparams(c.les_ne), hipMemcpyDeviceToHost,
+                           c.stream));
+    PZG_HIP(hipStreamSynchronize(c.stream));
   }
-  if (nc) *nc = c.nn_nc;
-  if (nsamples) *nsamples = c.nn_nsamp;
-  if (cia) std::memcpy(cia, c.nn_cia_host.data(), sizeof(int64_t) * c.nn_cia_host.size());
-  if (cja) std::memcpy(cja, c.nn_cja_host.data(), sizeof(int32_t) * c.nn_cja_host.size());
-  if (agg) std::memcpy(agg, c.nn_agg_host.data(), sizeof(int32_t) * c.nn_agg_host.size());
   return PZG_OK;
 }
 
-int pzg_learned_samples(pzg_ctx *ctx, float *X, float *T) {
+int pzg_learned_apply(pzg_ctx *ctx, const double *r, double *z) {
   CTX_CHECK();
   Ctx &c = ctx->c;
-  if (!c.nn_ready) {
-    set_error("pzg_learned_samples: call pzg_learned_setup first");
+  if (!c.les_ready || !c.assembled) {
+    set_error("pzg_learned_apply: needs pzg_learned_setup and an assembled system");
     return PZG_ESTATE;
   }
-  size_t bytes = sizeof(float) * c.nn_nsamp * c.nn_nc * c.bs;
-  if (X && bytes) PZG_HIP(hipMemcpy(X, c.d_nn_X.p, bytes, hipMemcpyDeviceToHost));
-  if (T && bytes) PZG_HIP(hipMemcpy(T, c.d_nn_T.p, bytes, hipMemcpyDeviceToHost));
+  if (!r || !z) {
+    set_error("pzg_learned_apply: null vector");
+    return PZG_EINVAL;
+  }
+  PZG_TRY(les_prepare(c));
+  const int64_t n = c.neq;
+  std::vector<float> h(n);
+  for (int64_t i = 0; i < n; i++) h[i] = (float)r[i];
+  PZG_TRY(dev_alloc(c.d_xs, n * sizeof(float)));
+  PZG_TRY(dev_alloc(c.d_zs, n * sizeof(float)));
+  PZG_HIP(hipMemcpyAsync(c.d_xs.p, h.data(), n * sizeof(float), hipMemcpyHostToDevice, c.stream));
+  PZG_TRY(les_apply(c, c.d_xs.as<float>(), c.d_zs.as<float>(), nullptr));
+  PZG_HIP(hipMemcpyAsync(h.data(), c.d_zs.p, n * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+  PZG_HIP(hipStreamSynchronize(c.stream));
+  for (int64_t i = 0; i < n; i++) z[i] = h[i];
   return PZG_OK;
 }
 
-int pzg_learned_loss_grad(pzg_ctx *ctx, int64_t ns, const float *X, const float *T, double *loss,
-                          float *grad) {
+int pzg_learned_scales(pzg_ctx *ctx, float *s) {
   CTX_CHECK();
   Ctx &c = ctx->c;
-  if (!c.nn_ready || ns <= 0 || !X || !T) {
-    set_error("pzg_learned_loss_grad: needs pzg_learned_setup and data");
+  if (!c.les_ready || !c.assembled) {
+    set_error("pzg_learned_scales: needs pzg_learned_setup and an assembled system");
     return PZG_ESTATE;
   }
-  DevBuf dx, dt;
-  size_t n = (size_t)ns * c.nn_nc * c.bs;
-  int rc = dev_upload(dx, X, n, c.stream);
-  if (rc == PZG_OK) rc = dev_upload(dt, T, n, c.stream);
-  double l = 0;
-  if (rc == PZG_OK) rc = nn_loss_grad(c, ns, dx.as<float>(), dt.as<float>(), &l);
-  if (rc == PZG_OK && grad &&
-      hipMemcpy(grad, c.d_nn_g.p, sizeof(float) * nn_nparams(c.bs), hipMemcpyDeviceToHost) != hipSuccess)
-    rc = PZG_EHIP;
-  if (loss) *loss = l;
-  dev_free(dx);
-  dev_free(dt);
-  return rc;
+  PZG_TRY(les_prepare(c));
+  if (s) {
+    PZG_HIP(hipMemcpyAsync(s, c.d_les_s.p, sizeof(float) * c.nvol, hipMemcpyDeviceToHost, c.stream));
+    PZG_HIP(hipStreamSynchronize(c.stream));
+  }
+  return PZG_OK;
 }
 
 }  // extern "C"

@@ -285,6 +285,7 @@ struct Ctx {
   DevBuf d_x, d_r, d_z, d_p, d_p2, d_q, d_u, d_f, d_w;        // fp64
   DevBuf d_xs, d_rs, d_zs, d_ps, d_ps2, d_qs, d_bs;          // fp32
   DevBuf d_scal, d_partials, d_counter;
+  DevBuf d_res2;  // |r|^2, |f|^2 of the FP64 true residual
   int reduce_blocks = 1024;
   bool has_solution = false;
 
@@ -297,25 +298,29 @@ struct Ctx {
   int mf_resident = 0;     // workgroups of the element kernel resident at once
   DevBuf d_mf_coef, d_mf_dis, d_mf_mfma, d_mf_geo, d_mf_pt;
 
-  // learned preconditioner (pzg_learned.hip)
-  bool nn_ready = false;
-  int64_t nn_nc = 0;                   // aggregates (coarse nodes)
-  std::vector<int32_t> nn_agg_host, nn_cja_host;
-  std::vector<int64_t> nn_cia_host;
-  DevBuf d_nn_agg, d_nn_captr, d_nn_calist, d_nn_cia, d_nn_cja, d_nn_w8;
-  DevBuf d_nn_w, d_nn_m, d_nn_v, d_nn_g;          // weights, Adam moments, gradient
-  int nn_adam_t = 0;
-  int64_t nn_nsamp = 0, nn_maxsamp = 0;
-  DevBuf d_nn_X, d_nn_T;                          // training pairs [sample][aggregate][bs]
-  DevBuf d_nn_xi, d_nn_zi, d_nn_h0, d_nn_h1, d_nn_mi;  // inference workspace
-  DevBuf d_nn_trH, d_nn_trM, d_nn_trP, d_nn_dH, d_nn_G, d_nn_tmp, d_nn_Z, d_nn_dZ;
-  DevBuf d_nn_den, d_nn_part, d_nn_loss;
-  DevBuf d_nn_ystar, d_nn_e;                      // pair generation
+  // learned element smoother of the multigrid cycle (pzg_learned.hip):
+  // parameters (G, network), Adam moments and gradient; element graph,
+  // penalized-equation mask and features; training pairs of the current set
+  bool les_ready = false, les_graph = false;
+  int les_ne = 0, les_nfaces = 0, les_adam_t = 0;
+  int64_t les_nr = 0;
+  double les_den = 1.;
+  std::vector<int64_t> les_cia;
+  std::vector<int32_t> les_cja;
+  std::vector<float> les_pen, les_penfrac;
+  std::vector<double> les_A, les_B;  // least-squares sums of the G fit
+  int64_t les_fit_rows = 0;
+  bool les_fit_stale = false;
+  DevBuf d_les_th, d_les_m, d_les_v, d_les_g;
+  DevBuf d_les_cia, d_les_cja, d_les_pen, d_les_X, d_les_z, d_les_s, d_les_ye;
+  DevBuf d_les_Se, d_les_M, d_les_R, d_les_E, d_les_D, d_les_le, d_les_ds, d_les_loss, d_les_fail;
+  DevBuf d_nn_trH, d_nn_trM, d_nn_trP, d_nn_dH, d_nn_G, d_nn_tmp, d_nn_dZ, d_nn_part;
 
   // multigrid preconditioner (pzg_mg.hip): lattice levels 1.. (level 0 is the
   // FCG system), vertex block of each lattice node of level 1, D^1/2 at the
   // vertex equations, the dense inverse of the coarsest level
   bool mg_ready = false, mg_use_mf = false;
+  bool mg_les = false;     // level 0 smoothed with the learned element smoother
   bool mg_struct = false;  // lattice, levels and buffers of the current mesh
   bool mg_dense = true;    // the coarsest level has a dense inverse (else a Chebyshev solve)
   int mg_degree = 2;       // Chebyshev degree of the pre- and post-smoothing
@@ -323,7 +328,7 @@ struct Ctx {
   double mg_lmax0 = 0.;    // largest eigenvalue of the block-Jacobi scaled S
   int64_t mg_ncoarse = 0;  // unknowns of the coarsest level
   std::vector<MgLevel> mg;
-  DevBuf d_mg_vmap, d_mg_dsq, d_mg_cinv, d_mg_r0, d_mg_d0, d_mg_t0, d_mg_w0, d_mg_lz;
+  DevBuf d_mg_vmap, d_mg_dsq, d_mg_cinv, d_mg_r0, d_mg_d0, d_mg_t0, d_mg_w0, d_mg_lz, d_mg_y0;
 
   // post-processing (pzg_post.hip): error-rule tables per rule bucket for the
   // exact-solution order post_porder (-1: not built), the elements of every
@@ -373,14 +378,15 @@ int launch_multadd_upper(Ctx &c, const double *x, const double *y, double *z, do
 int launch_addkel_batch(Ctx &c, int64_t nel, int ne, const int64_t *d_dest, const double *d_ek,
                         const int32_t *d_list, int64_t nlist);
 int side_corner_list_pub(int etype, int s, int *loc);
-int nn_nparams(int bs);
-int nn_setup(Ctx &c, int coarsen, uint64_t seed, int max_samples);
-int nn_apply(Ctx &c, const float *r, float *z, float *p, const float *q, SolveScalars *sc,
-             double *partials, unsigned *counter);
-int nn_loss_grad(Ctx &c, int64_t ns, const float *X, const float *T, double *loss_out);
-int nn_adam_step(Ctx &c, double lr);
-int nn_store_pair(Ctx &c, const float *r, const float *e_minus_bjr, int64_t slot);
-int nn_collect(Ctx &c, int nrhs, const int32_t *rec, int nrec, uint64_t seed);
+// learned element smoother (pzg_learned.hip)
+int les_nparams(int ne);
+int les_setup(Ctx &c, uint64_t seed);
+int les_prepare(Ctx &c);  // per-element scales of the current assembly, apply buffers
+int les_apply(Ctx &c, const float *r, float *z, const SolveScalars *sc);
+int les_collect(Ctx &c, int nrhs, uint64_t seed);
+int les_loss_grad(Ctx &c, double *loss_out);
+int les_adam_step(Ctx &c, double lr);
+int les_fit_G(Ctx &c);  // G from the accumulated least-squares sums
 int mf_setup(Ctx &c);
 // multigrid preconditioner: levels, operators and eigenvalue estimates of the
 // current system (use_mf: level 0 applied matrix-free); z = V-cycle(r), then

@@ -1,30 +1,46 @@
-// pzg_learned.hip -- learned coarse-correction preconditioner for the mixed
-// FCG path: message passing on a coarse graph derived from the CSR block
-// graph, trained on the GPU with these kernels.
+// pzg_learned.hip -- the learned element smoother of the mixed solver's
+// multigrid V-cycle (PZG_PRECOND_LEARNED).  NeoPZ has no learned component;
+// this serves the FP32 iterations only (the FP64 true residual still decides
+// convergence).
 //
-// Structure.  Active blocks are grouped into aggregates (cells of a coarse
-// grid over the block coordinates); two aggregates are adjacent when a block
-// graph edge joins them.  For a scaled residual r the preconditioner returns
-//     z = B^-1 r + P Z,   Z = N(X),  X_I = mean_{i in I} r_i
-// where B^-1 r is the block-Jacobi step, P copies Z_I to every block of
-// aggregate I, and N is the network
-//     H0 = X W0;  H_l = lrelu(H_{l-1} A_l + (Ahat H_{l-1}) B_l), l = 1..3;  Z = H3 Wout
-// (Ahat = mean over coarse neighbours, 32 hidden features, no biases, so
-// N(a X) = a N(X) for a > 0).  With Wout = 0 it is block Jacobi.
+// Operator.  For a scaled residual r (S = D^-1/2 K D^-1/2) the smoother returns
+//     z = sum_e s_e P_e^T G P_e r   on the unpenalized equations,
+//     z_i = r_i                     on the Dirichlet-penalty equations,
+// where P_e gathers the element's ne = nshape x nstate dofs (shape-major, the
+// order of the element matrix), G is one learned ne x ne symmetric matrix and
+// s_e > 0 is the output of a message-passing network on the element graph
+// (elements adjacent through a face), 3 layers of 32 features:
+//     H0 = X W0;  H_l = lrelu(H_{l-1} A_l + (Ahat H_{l-1}) B_l);  s = exp(H3 w)
+// (Ahat = mean over the face neighbours).  X holds four features of the
+// element: log c_e - mean of the neighbours' log c, the largest |log c_n -
+// log c_e| over the neighbours (c: the element's E or permeability), the
+// fraction of penalized dofs, and 1 - neighbours / faces.  It replaces the
+// block-Jacobi M^-1 of the Chebyshev smoothing on the p level; the cycle
+// stays a fixed SPD operator for a fixed field.
 //
-// Training data are (residual, error) pairs generated by the solver itself:
-// for a random scaled solution y*, b = S y*, block-Jacobi FCG is run from
-// y = 0 and at chosen iterations k the pair (X = R r_k, T = R(e_k - B^-1 r_k))
-// is stored, R = aggregate mean, e_k = y* - y_k.  The loss is
-//     sum_s sum_I |I| |Z_sI - T_sI|^2 / sum_I |I| |T_sI|^2,
-// minimised with Adam.  Since the fine-level error of z is piecewise
-// constant-plus-fixed per aggregate, this is the least-squares fit of the
-// coarse correction.
+// Training data are (residual, error) pairs of the local problems the solver
+// sees: for every element of a training system, the element's block S_e of
+// the assembled scaled matrix (penalized dofs decoupled), random residuals r
+// (zero on penalized dofs) and the errors e = S_e^-1 r from a batched
+// Cholesky solve.  The loss
+//     L = sum_e sum_k |mask (s_e G r_ek - e_ek)|^2 / sum_e sum_k |e_ek|^2
+// is minimised in two parts.  G: the least-squares fit G = B A^-1 with
+// A = sum r r^T, B = sum e r^T over the pairs of the elements without
+// penalized dofs (sums formed on the device, accumulated over every
+// training system; for isotropic r this is the mean of the local inverses
+// S_e^-1), symmetrised and projected onto the positive definite matrices
+// (eigenvalues below 1e-3 of the largest raised), so the smoother stays SPD.
+// The network: Adam on the loss with G fixed.  G starts as diag(1 / number of
+// elements sharing the dof's side) (a point-Jacobi-like smoother), the output
+// layer at zero (s = 1).  Forward, backward and Adam are the kernels below;
+// the gradient (of every parameter, G included) is checked against a numpy
+// statement.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <random>
 
 #include "pzg_device.h"
@@ -34,50 +50,11 @@ namespace pzg {
 
 constexpr int TPB = RED_TPB;
 constexpr int NBLK_RED = 512;  // blocks of the weight-gradient reductions
-
-int nn_nparams(int bs) { return bs * NNF + 2 * NNL * NNF * NNF + NNF * bs; }
-
-struct NNOff {
-  int w0, a[NNL], b[NNL], wout, n;
-};
-static NNOff nn_offsets(int bs) {
-  NNOff o;
-  int k = 0;
-  o.w0 = k; k += bs * NNF;
-  for (int l = 0; l < NNL; l++) {
-    o.a[l] = k; k += NNF * NNF;
-    o.b[l] = k; k += NNF * NNF;
-  }
-  o.wout = k; k += NNF * bs;
-  o.n = k;
-  return o;
-}
+constexpr int LES_FIN = 4;     // element features
+typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 __device__ inline float lrelu(float x) { return x > 0.f ? x : NN_SLOPE * x; }
 __device__ inline float dlrelu(float x) { return x > 0.f ? 1.f : NN_SLOPE; }
-
-// out_I = mean_{i in I} (a_i - b_i), one wave per aggregate
-template <int BS>
-__global__ void k_nn_restrict(int64_t nc, const int64_t *__restrict__ captr,
-                              const int32_t *__restrict__ calist, const float *__restrict__ a,
-                              const float *__restrict__ b, float *__restrict__ out) {
-  int64_t I = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  int lane = threadIdx.x & 63;
-  if (I >= nc) return;
-  float s[BS];
-  for (int k = 0; k < BS; k++) s[k] = 0.f;
-  const int64_t c0 = captr[I], c1 = captr[I + 1];
-  for (int64_t t = c0 + lane; t < c1; t += 64) {
-    const int64_t i = calist[t];
-    for (int k = 0; k < BS; k++) s[k] += a[i * BS + k] - (b ? b[i * BS + k] : 0.f);
-  }
-  for (int k = 0; k < BS; k++)
-    for (int o = 32; o > 0; o >>= 1) s[k] += __shfl_xor(s[k], o, 64);
-  if (lane == 0) {
-    float inv = c1 > c0 ? 1.f / (float)(c1 - c0) : 0.f;
-    for (int k = 0; k < BS; k++) out[I * BS + k] = s[k] * inv;
-  }
-}
 
 // H0 = X W0   (rows = samples * nc)
 template <int BS>
@@ -160,71 +137,7 @@ __global__ void k_nn_out(int64_t rows, const float *__restrict__ H, const float 
   Z[g] = s;
 }
 
-// z_i += Z_agg(i) (and p_i += Z_agg(i)); optionally beta = -(z.q)/(p.q)
-template <int BS>
-__global__ __launch_bounds__(TPB) void k_nn_prolong(int64_t nb, const int32_t *__restrict__ agg,
-                                                    const float *__restrict__ Z,
-                                                    float *__restrict__ z, float *__restrict__ p,
-                                                    const float *__restrict__ q, SolveScalars *sc,
-                                                    double *partials, unsigned *counter) {
-  if (sc->done != 0 && sc->done != 4) return;  // completes the current iteration
-  double zq = 0.;
-  int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t st = (int64_t)gridDim.x * blockDim.x;
-  for (; A < nb; A += st) {
-    const int64_t I = agg[A];
-    for (int k = 0; k < BS; k++) {
-      const int64_t i = A * BS + k;
-      float v = z[i] + Z[I * BS + k];
-      z[i] = v;
-      if (p) p[i] = v;
-      if (q) zq += (double)v * q[i];
-    }
-  }
-  if (!q) return;
-  // z.q replaces slot 1 of the update kernel's partials (same grid size)
-  __shared__ double sm[TPB / 64];
-  double t = block_sum(zq, sm);
-  if (threadIdx.x == 0) partials[1 * MAXBLK + blockIdx.x] = t;
-}
-
 // ---- training kernels -------------------------------------------------------
-template <int BS>
-__global__ void k_nn_den(int64_t nc, const float *__restrict__ T, const float *__restrict__ w,
-                         double *__restrict__ den) {
-  __shared__ double sm[TPB / 64];
-  const int64_t s = blockIdx.x;
-  double acc = 0.;
-  for (int64_t I = threadIdx.x; I < nc; I += blockDim.x)
-    for (int k = 0; k < BS; k++) {
-      double t = T[(s * nc + I) * BS + k];
-      acc += (double)w[I] * t * t;
-    }
-  acc = block_sum(acc, sm);
-  if (threadIdx.x == 0) den[s] = acc > 1e-300 ? acc : 1e-300;
-}
-
-template <int BS>
-__global__ __launch_bounds__(TPB) void k_nn_loss(int64_t rows, int64_t nc, const float *__restrict__ Z,
-                                                 const float *__restrict__ T,
-                                                 const float *__restrict__ w,
-                                                 const double *__restrict__ den,
-                                                 float *__restrict__ dZ, double *loss,
-                                                 double *partials, unsigned *counter) {
-  double acc = 0.;
-  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t st = (int64_t)gridDim.x * blockDim.x;
-  for (; g < rows * BS; g += st) {
-    int64_t r = g / BS, s = r / nc, I = r % nc;
-    double d = (double)Z[g] - (double)T[g];
-    double wd = (double)w[I] / den[s];
-    acc += wd * d * d;
-    dZ[g] = (float)(2.0 * wd * d);
-  }
-  double vv[1] = {acc}, tot[1];
-  if (reduce_last<1>(vv, partials, counter, tot) && threadIdx.x == 0) *loss = tot[0];
-}
-
 // dH = dZ Wout^T
 template <int BS>
 __global__ void k_nn_back_out(int64_t rows, const float *__restrict__ dZ,
@@ -333,311 +246,865 @@ __global__ void k_adam(int n, float *__restrict__ th, const float *__restrict__ 
   th[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
+
+// ---- parameters ---------------------------------------------------------------
+struct LesOff {
+  int g, w0, a[NNL], b[NNL], wout, n;
+};
+static LesOff les_offsets(int ne) {
+  LesOff o;
+  int k = 0;
+  o.g = k; k += ne * ne;
+  o.w0 = k; k += LES_FIN * NNF;
+  for (int l = 0; l < NNL; l++) {
+    o.a[l] = k; k += NNF * NNF;
+    o.b[l] = k; k += NNF * NNF;
+  }
+  o.wout = k; k += NNF;
+  o.n = k;
+  return o;
+}
+int les_nparams(int ne) { return les_offsets(ne).n; }
+
+// s = exp(z), the exponent clamped to [-4, 4]
+__global__ void k_les_exp(int64_t n, const float *__restrict__ z, float *__restrict__ s) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) s[i] = expf(fminf(4.f, fmaxf(-4.f, z[i])));
+}
+
+// ---- application ---------------------------------------------------------------
+// Hexahedra of order 2 (27 shapes): z_e = s_e G r_e for 16 elements per
+// wavefront as v_mfma_f32_16x16x4f32 tiles: rows = the element's dofs padded
+// to NT*16, columns = elements, k = dofs.  G is staged in LDS in the A
+// operand order (tile ti, k-step kk, lane l: G[16 ti + l%16][4 kk + l/16]),
+// so every LDS read of a wavefront is one contiguous segment.  The element's
+// inputs r (zero on penalized dofs) are gathered once into registers; the
+// outputs go to the element's slots in the row-ordered output lists.
+template <int NSH, int NS>
+__global__ __launch_bounds__(256) void k_les_mfma(int64_t nvol, const float *__restrict__ G,
+                                                  const float *__restrict__ scale,
+                                                  const int32_t *__restrict__ blk,
+                                                  const int32_t *__restrict__ pos,
+                                                  const float *__restrict__ pen,
+                                                  const float *__restrict__ r, float *__restrict__ ye,
+                                                  const SolveScalars *sc) {
+  if (sc && sc->done) return;
+  constexpr int NE = NSH * NS, NT = (NE + 15) / 16, NK = 4 * NT;
+  __shared__ float sA[NT * NK * 64];
+  const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, col = l & 15, kq = l >> 4;
+  for (int t = tid; t < NT * NK * 64; t += 256) {
+    const int ll = t & 63, kk = (t >> 6) % NK, ti = (t >> 6) / NK;
+    const int row = 16 * ti + (ll & 15), k = 4 * kk + (ll >> 4);
+    sA[t] = (row < NE && k < NE) ? G[row * NE + k] : 0.f;
+  }
+  __syncthreads();
+  for (int64_t g0 = (int64_t)blockIdx.x * 64; g0 < nvol; g0 += (int64_t)gridDim.x * 64) {
+    const int64_t e = g0 + wv * 16 + col;
+    const bool ev = e < nvol;
+    const int64_t ec = ev ? e : 0;
+    float u[NK];
+#pragma unroll
+    for (int kk = 0; kk < NK; kk++) {
+      const int k = 4 * kk + kq;
+      const int sh = k < NE ? k / NS : 0, cp = k < NE ? k % NS : 0;
+      const int32_t b = blk[ec * NSH + sh];
+      const int64_t i = (int64_t)(b >= 0 ? b : 0) * NS + cp;
+      const float v = r[i] * (1.f - pen[i]);
+      u[kk] = (ev && k < NE && b >= 0) ? v : 0.f;
+    }
+    const float s = ev ? scale[ec] : 0.f;
+#pragma unroll
+    for (int ti = 0; ti < NT; ti++) {
+      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kk = 0; kk < NK; kk++)
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(sA[(ti * NK + kk) * 64 + l], u[kk], acc, 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int o = 16 * ti + 4 * kq + i;
+        if (!ev || o >= NE) continue;
+        const int32_t p = pos[e * NSH + o / NS];
+        if (p >= 0) ye[(int64_t)p * NS + o % NS] = s * acc[i];
+      }
+    }
+  }
+}
+
+// Any other element: one thread per (element, output dof).
+template <int NSH, int NS>
+__global__ __launch_bounds__(256) void k_les_gen(int64_t nvol, const float *__restrict__ G,
+                                                 const float *__restrict__ scale,
+                                                 const int32_t *__restrict__ blk,
+                                                 const int32_t *__restrict__ pos,
+                                                 const float *__restrict__ pen,
+                                                 const float *__restrict__ r, float *__restrict__ ye,
+                                                 const SolveScalars *sc) {
+  if (sc && sc->done) return;
+  constexpr int NE = NSH * NS;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nvol * NE;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t e = t / NE;
+    const int o = (int)(t % NE);
+    const int32_t p = pos[e * NSH + o / NS];
+    if (p < 0) continue;
+    float acc = 0.f;
+    for (int k = 0; k < NE; k++) {
+      const int32_t b = blk[e * NSH + k / NS];
+      if (b < 0) continue;
+      const int64_t i = (int64_t)b * NS + k % NS;
+      acc += G[o * NE + k] * r[i] * (1.f - pen[i]);
+    }
+    ye[(int64_t)p * NS + o % NS] = scale[e] * acc;
+  }
+}
+
+// z = the sum of a block row's element outputs (fixed order); r on the
+// penalized equations
+template <int NS>
+__global__ __launch_bounds__(256) void k_les_gather(int64_t nb, const int64_t *__restrict__ ptr,
+                                                    const float *__restrict__ ye,
+                                                    const float *__restrict__ pen,
+                                                    const float *__restrict__ r, float *__restrict__ z,
+                                                    const SolveScalars *sc) {
+  if (sc && sc->done) return;
+  for (int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; A < nb;
+       A += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t0 = ptr[A], t1 = ptr[A + 1];
+    float acc[NS];
+#pragma unroll
+    for (int k = 0; k < NS; k++) acc[k] = 0.f;
+    for (int64_t t = t0; t < t1; t++)
+#pragma unroll
+      for (int k = 0; k < NS; k++) acc[k] += ye[t * NS + k];
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+      const int64_t i = A * NS + k;
+      z[i] = pen[i] != 0.f ? r[i] : acc[k];
+    }
+  }
+}
+
+// ---- training kernels ------------------------------------------------------------
+// S_e (FP64, ne x ne, row-major) of every element from the FP64 full block
+// CSR of K scaled by D^-1/2; rows / columns of penalized dofs and of absent
+// shapes become unit rows.  One thread per (element, shape a, shape b).
+template <int NS>
+__global__ void k_les_local(int64_t nvol, int nsh, const int32_t *__restrict__ blk,
+                            const int64_t *__restrict__ fbia, const int32_t *__restrict__ fbja,
+                            const double *__restrict__ fa, const double *__restrict__ dis,
+                            const float *__restrict__ pen, double *__restrict__ Se) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nvol * nsh * nsh) return;
+  const int64_t e = t / (nsh * nsh);
+  const int a = (int)(t % (nsh * nsh)) / nsh, b = (int)(t % nsh);
+  const int ne = nsh * NS;
+  double *out = Se + e * ne * ne;
+  const int32_t A = blk[e * nsh + a], B = blk[e * nsh + b];
+  int64_t idx = -1, n = 0, r0 = 0;
+  if (A >= 0 && B >= 0) {
+    r0 = fbia[A];
+    n = fbia[A + 1] - r0;
+    int64_t lo = 0, hi = n;  // columns of a full block row are ascending
+    while (lo < hi) {
+      const int64_t m = (lo + hi) / 2;
+      if (fbja[r0 + m] < B) lo = m + 1;
+      else hi = m;
+    }
+    if (lo < n && fbja[r0 + lo] == B) idx = lo;
+  }
+  for (int ka = 0; ka < NS; ka++)
+    for (int kb = 0; kb < NS; kb++) {
+      const int i = a * NS + ka, j = b * NS + kb;
+      double v = 0.;
+      bool unit = A < 0 || B < 0;
+      if (!unit) {
+        const int64_t ia = (int64_t)A * NS + ka, ib = (int64_t)B * NS + kb;
+        unit = pen[ia] != 0.f || pen[ib] != 0.f;
+        if (!unit && idx >= 0) v = dis[ia] * fa[bsr_pos<NS * NS>(r0, n, idx, ka * NS + kb)] * dis[ib];
+      }
+      if (unit) v = i == j ? 1. : 0.;
+      out[(int64_t)i * ne + j] = v;
+    }
+}
+
+// Cholesky of S_e in LDS and E = S_e^-1 R for the element's nr residuals
+// (one workgroup per element; ne <= LES_MAXNE).
+constexpr int LES_MAXNE = 81;
+__global__ __launch_bounds__(256) void k_les_solve(int ne, int nr, const double *__restrict__ Se,
+                                                   const float *__restrict__ R, float *__restrict__ E,
+                                                   int *__restrict__ fail) {
+  __shared__ double L[LES_MAXNE * LES_MAXNE];
+  __shared__ double y[8][LES_MAXNE];
+  const int64_t e = blockIdx.x;
+  const int tid = threadIdx.x;
+  for (int t = tid; t < ne * ne; t += blockDim.x) L[t] = Se[e * ne * ne + t];
+  __syncthreads();
+  for (int j = 0; j < ne; j++) {
+    if (tid == 0) {
+      const double d = L[j * ne + j];
+      if (!(d > 0.)) atomicExch(fail, 1);
+      L[j * ne + j] = d > 0. ? sqrt(d) : 1.;
+    }
+    __syncthreads();
+    const double pj = L[j * ne + j];
+    for (int i = j + 1 + tid; i < ne; i += blockDim.x) L[i * ne + j] /= pj;
+    __syncthreads();
+    const int m = ne - j - 1;
+    for (int t = tid; t < m * m; t += blockDim.x) {
+      const int i = j + 1 + t / m, k = j + 1 + t % m;
+      if (k <= i) L[i * ne + k] -= L[i * ne + j] * L[k * ne + j];
+    }
+    __syncthreads();
+  }
+  // forward / back substitution, one thread per right-hand side
+  if (tid < nr && tid < 8) {
+    const float *rr = R + (e * nr + tid) * ne;
+    double *yy = y[tid];
+    for (int i = 0; i < ne; i++) {
+      double s = rr[i];
+      for (int k = 0; k < i; k++) s -= L[i * ne + k] * yy[k];
+      yy[i] = s / L[i * ne + i];
+    }
+    for (int i = ne - 1; i >= 0; i--) {
+      double s = yy[i];
+      for (int k = i + 1; k < ne; k++) s -= L[k * ne + i] * yy[k];
+      yy[i] = s / L[i * ne + i];
+    }
+    float *ee = E + (e * nr + tid) * ne;
+    for (int i = 0; i < ne; i++) ee[i] = (float)yy[i];
+  }
+}
+
+// per element: y_k = G r_k, d_k = mask (s y_k - e_k); loss_e = sum |d|^2,
+// ds_e = sum d.y; Dsc = 2 s d / den (the G gradient rows).  One workgroup
+// per element.
+__global__ __launch_bounds__(128) void k_les_fwd(int ne, int nr, const float *__restrict__ G,
+                                                 const float *__restrict__ scale,
+                                                 const float *__restrict__ M, const float *__restrict__ R,
+                                                 const float *__restrict__ E, double den,
+                                                 float *__restrict__ Dsc, double *__restrict__ loss_e,
+                                                 double *__restrict__ ds_e) {
+  __shared__ float sr[LES_MAXNE];
+  __shared__ double red[2][128];
+  const int64_t e = blockIdx.x;
+  const int tid = threadIdx.x;
+  const float s = scale[e];
+  double lacc = 0., dacc = 0.;
+  for (int k = 0; k < nr; k++) {
+    __syncthreads();
+    for (int t = tid; t < ne; t += blockDim.x) sr[t] = R[(e * nr + k) * ne + t];
+    __syncthreads();
+    for (int i = tid; i < ne; i += blockDim.x) {
+      float y = 0.f;
+      for (int j = 0; j < ne; j++) y += G[i * ne + j] * sr[j];
+      const float m = M[e * ne + i];
+      const double d = m * ((double)s * y - (double)E[(e * nr + k) * ne + i]);
+      lacc += d * d;
+      dacc += d * y;
+      Dsc[(e * nr + k) * ne + i] = (float)(2.0 * s * d / den);
+    }
+  }
+  red[0][tid] = lacc;
+  red[1][tid] = dacc;
+  __syncthreads();
+  for (int w = 64; w > 0; w >>= 1) {
+    if (tid < w) {
+      red[0][tid] += red[0][tid + w];
+      red[1][tid] += red[1][tid + w];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    loss_e[e] = red[0][0];
+    ds_e[e] = red[1][0];
+  }
+}
+
+// dZ_e = s_e * 2 ds_e / den (the scale is exp(z) inside the clamp)
+__global__ void k_les_dz(int64_t n, const float *__restrict__ z, const float *__restrict__ s,
+                         const double *__restrict__ ds, double den, float *__restrict__ dz) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const bool inside = z[i] > -4.f && z[i] < 4.f;
+  dz[i] = inside ? (float)(2.0 * (double)s[i] * ds[i] / den) : 0.f;
+}
+
+// symmetric part of the G gradient (G stays symmetric under Adam)
+__global__ void k_les_sym(int ne, float *__restrict__ g) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ne * ne) return;
+  const int i = t / ne, j = t % ne;
+  if (j <= i) return;
+  const float m = 0.5f * (g[i * ne + j] + g[j * ne + i]);
+  g[i * ne + j] = m;
+  g[j * ne + i] = m;
+}
+
+// fixed-order sum of per-element doubles (one block)
+__global__ void k_les_sum(int64_t n, const double *__restrict__ v, double *__restrict__ out) {
+  __shared__ double red[256];
+  double a = 0.;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) a += v[i];
+  red[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
 // ============================================================================
 // host side
-static int grid1(int64_t n) { return (int)std::max<int64_t>(1, (n + TPB - 1) / TPB); }
+static int grid1(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + TPB - 1) / TPB, 1 << 20)); }
 
-int nn_setup(Ctx &c, int coarsen, uint64_t seed, int max_samples) {
-  if (!c.has_struct || c.external_csr || c.nb == 0) {
+static bool les_supported(const Ctx &c) {
+  return c.vol_etype != PZG_TET && !c.hp_simplex && c.vol_rules.size() == 1 && !c.has_dep && !c.filt &&
+         c.tvol.nsh * c.bs <= LES_MAXNE;
+}
+
+// Element graph (elements sharing a face: a side with as many corners as a
+// boundary element), the penalized-equation mask and the per-element
+// penalized fraction; kept per mesh.
+static int les_graph(Ctx &c) {
+  if (c.les_graph) return PZG_OK;
+  const int nsd = c.vol_nsides, ncv = c.vol_ncorner, nfc = c.bc_ncorner;
+  std::vector<int> faces;
+  for (int sd = 0; sd < nsd; sd++) {
+    int loc[8];
+    if (side_corner_list_pub(c.vol_etype, sd, loc) == nfc) faces.push_back(sd);
+  }
+  // face connect -> the elements holding it
+  std::map<int64_t, std::vector<int64_t>> byface;
+  for (int64_t e = 0; e < c.nvol; e++)
+    for (int sd : faces) byface[c.vol_conn[e * nsd + sd]].push_back(e);
+  std::vector<std::vector<int32_t>> nb(c.nvol);
+  for (auto &kv : byface)
+    for (size_t i = 0; i < kv.second.size(); i++)
+      for (size_t j = 0; j < kv.second.size(); j++)
+        if (i != j) nb[kv.second[i]].push_back((int32_t)kv.second[j]);
+  c.les_cia.assign(c.nvol + 1, 0);
+  c.les_cja.clear();
+  for (int64_t e = 0; e < c.nvol; e++) {
+    auto &v = nb[e];
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    c.les_cja.insert(c.les_cja.end(), v.begin(), v.end());
+    c.les_cia[e + 1] = (int64_t)c.les_cja.size();
+  }
+  c.les_nfaces = (int)faces.size();
+  // penalized equations: the components a boundary element holds with a
+  // penalty (a diagonal coefficient of the penalty size)
+  const int ns = c.bs;
+  c.les_pen.assign(c.neq, 0.f);
+  const ElemTables &B = c.tbc;
+  for (int64_t e = 0; e < c.nbc; e++) {
+    const double *ck = &c.bc_coef[(size_t)BCC * c.bc_id[e] + BCC_CK];
+    for (int k = 0; k < ns; k++) {
+      if (!(std::fabs(ck[(ns + 1) * k]) >= 1e6)) continue;
+      for (int s = 0; s < B.nsh; s++) {
+        const int64_t a = c.aidx[c.bc_conn[e * c.bc_nsides + B.shape_side[s]]];
+        if (a >= 0) c.les_pen[a * ns + k] = 1.f;
+      }
+    }
+  }
+  c.les_penfrac.assign(c.nvol, 0.f);
+  const ElemTables &V = c.tvol;
+  for (int64_t e = 0; e < c.nvol; e++) {
+    int np = 0, nt = 0;
+    for (int s = 0; s < V.nsh; s++) {
+      const int64_t a = c.aidx[c.vol_conn[e * nsd + V.shape_side[s]]];
+      if (a < 0) continue;
+      for (int k = 0; k < ns; k++, nt++) np += c.les_pen[a * ns + k] != 0.f;
+    }
+    c.les_penfrac[e] = nt ? (float)np / nt : 0.f;
+  }
+  hipStream_t s = c.stream;
+  PZG_TRY(dev_upload(c.d_les_cia, c.les_cia.data(), c.les_cia.size(), s));
+  std::vector<int32_t> cja(c.les_cja);
+  if (cja.empty()) cja.push_back(0);
+  PZG_TRY(dev_upload(c.d_les_cja, cja.data(), cja.size(), s));
+  PZG_TRY(dev_upload(c.d_les_pen, c.les_pen.data(), c.les_pen.size(), s));
+  PZG_HIP(hipStreamSynchronize(s));
+  c.les_graph = true;
+  return PZG_OK;
+}
+
+// features of the current coefficient field (host) -> d_les_X
+static int les_features(Ctx &c) {
+  std::vector<float> X((size_t)c.nvol * LES_FIN, 0.f);
+  const double dflt = c.material == PZG_MAT_ELAST3D ? c.E : c.scale;
+  auto lc = [&](int64_t e) { return std::log(c.vol_coef.empty() ? dflt : c.vol_coef[e]); };
+  for (int64_t e = 0; e < c.nvol; e++) {
+    const int64_t j0 = c.les_cia[e], j1 = c.les_cia[e + 1];
+    const double le = lc(e);
+    double mean = 0., mx = 0.;
+    for (int64_t j = j0; j < j1; j++) {
+      const double ln = lc(c.les_cja[j]);
+      mean += ln;
+      mx = std::max(mx, std::fabs(ln - le));
+    }
+    const int64_t nn = j1 - j0;
+    X[e * LES_FIN + 0] = nn ? (float)(le - mean / nn) : 0.f;
+    X[e * LES_FIN + 1] = (float)mx;
+    X[e * LES_FIN + 2] = c.les_penfrac[e];
+    X[e * LES_FIN + 3] = c.les_nfaces ? 1.f - (float)nn / c.les_nfaces : 0.f;
+  }
+  PZG_TRY(dev_upload(c.d_les_X, X.data(), X.size(), c.stream));
+  return PZG_OK;
+}
+
+// GNN forward over the element graph; keep: the intermediate tensors for
+// the backward pass (training)
+static int les_forward(Ctx &c, bool keep) {
+  const int64_t rows = c.nvol;
+  const float *th = c.d_les_th.as<float>();
+  LesOff o = les_offsets(c.les_ne);
+  const size_t need = (size_t)rows * NNF;
+  PZG_TRY(dev_alloc(c.d_nn_trH, sizeof(float) * need * (NNL + 1)));
+  PZG_TRY(dev_alloc(c.d_nn_trM, sizeof(float) * need * NNL));
+  PZG_TRY(dev_alloc(c.d_nn_trP, sizeof(float) * need * NNL));
+  PZG_TRY(dev_alloc(c.d_les_z, sizeof(float) * std::max<int64_t>(rows, 1)));
+  PZG_TRY(dev_alloc(c.d_les_s, sizeof(float) * std::max<int64_t>(rows, 1)));
+  float *H = c.d_nn_trH.as<float>(), *M = c.d_nn_trM.as<float>(), *P = c.d_nn_trP.as<float>();
+  const int g = grid1(rows * NNF);
+  hipLaunchKernelGGL(k_nn_in<LES_FIN>, dim3(g), dim3(TPB), 0, c.stream, rows, c.d_les_X.as<float>(),
+                     th + o.w0, H);
+  for (int l = 0; l < NNL; l++) {
+    hipLaunchKernelGGL(k_nn_agg, dim3(g), dim3(TPB), 0, c.stream, rows, rows, c.d_les_cia.as<int64_t>(),
+                       c.d_les_cja.as<int32_t>(), H + (int64_t)l * need, M + (int64_t)l * need);
+    hipLaunchKernelGGL(k_nn_layer, dim3(g), dim3(TPB), 0, c.stream, rows, H + (int64_t)l * need,
+                       M + (int64_t)l * need, th + o.a[l], th + o.b[l], keep ? P + (int64_t)l * need : nullptr,
+                       H + (int64_t)(l + 1) * need);
+  }
+  hipLaunchKernelGGL(k_nn_out<1>, dim3(grid1(rows)), dim3(TPB), 0, c.stream, rows, H + (int64_t)NNL * need,
+                     th + o.wout, c.d_les_z.as<float>());
+  hipLaunchKernelGGL(k_les_exp, dim3(grid1(rows)), dim3(TPB), 0, c.stream, rows, c.d_les_z.as<float>(),
+                     c.d_les_s.as<float>());
+  PZG_HIP(hipGetLastError());
+  return PZG_OK;
+}
+
+int les_setup(Ctx &c, uint64_t seed) {
+  if (!c.has_struct || c.external_csr || !c.has_mesh) {
     set_error("learned preconditioner: needs a mesh pattern (pzg_struct_create)");
     return PZG_ESTATE;
   }
-  if (coarsen < 1) coarsen = 4;
-  // block coordinates: centroid of the corners of the side that owns the block
-  std::vector<double> bx(3 * c.nb, 0.0);
-  std::vector<char> seen(c.nb, 0);
-  const int nsd = c.vol_nsides, nc_ = c.vol_ncorner, vt = c.vol_etype;
-  for (int64_t e = 0; e < c.nvol; e++)
-    for (int s = 0; s < nsd; s++) {
-      int64_t a = c.aidx[c.vol_conn[e * nsd + s]];
-      if (a < 0 || seen[a]) continue;
-      seen[a] = 1;
-      // high-order tetrahedra: the virtual side of a shape lies on its real side
-      const int rs = c.hp_simplex ? c.vol_side_real[s] : s;
-      int loc[8], nl = side_corner_list_pub(vt, rs, loc);
-      for (int k = 0; k < nl; k++)
-        for (int d = 0; d < 3; d++) bx[3 * a + d] += c.xyz[3 * c.vol_nodes[e * nc_ + loc[k]] + d] / nl;
-    }
-  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-  for (int64_t a = 0; a < c.nb; a++)
-    for (int d = 0; d < 3; d++) {
-      lo[d] = std::min(lo[d], bx[3 * a + d]);
-      hi[d] = std::max(hi[d], bx[3 * a + d]);
-    }
-  double vol = 1.0;
-  for (int d = 0; d < c.dim; d++) vol *= std::max(hi[d] - lo[d], 1e-300);
-  double h = std::pow(vol / (double)c.nvol, 1.0 / c.dim);
-  double H = coarsen * h;
-  int64_t ncell[3] = {1, 1, 1};
-  for (int d = 0; d < c.dim; d++) ncell[d] = std::max<int64_t>(1, (int64_t)std::ceil((hi[d] - lo[d]) / H - 1e-9));
-  std::vector<int64_t> cell(c.nb);
-  for (int64_t a = 0; a < c.nb; a++) {
-    int64_t id = 0, mul = 1;
-    for (int d = 0; d < c.dim; d++) {
-      int64_t k = (int64_t)((bx[3 * a + d] - lo[d]) / H);
-      k = std::min(std::max<int64_t>(k, 0), ncell[d] - 1);
-      id += k * mul;
-      mul *= ncell[d];
-    }
-    cell[a] = id;
+  if (!les_supported(c)) {
+    set_error("learned preconditioner: quadrilateral / hexahedral meshes with one rule, no constraints "
+              "and no equation filter");
+    return PZG_EUNSUP;
   }
-  std::vector<int64_t> sorted(cell);
-  std::sort(sorted.begin(), sorted.end());
-  sorted.erase(std::unique(sorted.begin(), sorted.end()), sorted.end());
-  const int64_t nc = (int64_t)sorted.size();
-  std::vector<int32_t> agg(c.nb);
-  for (int64_t a = 0; a < c.nb; a++)
-    agg[a] = (int32_t)(std::lower_bound(sorted.begin(), sorted.end(), cell[a]) - sorted.begin());
-  std::vector<int64_t> captr(nc + 1, 0);
-  for (int64_t a = 0; a < c.nb; a++) captr[agg[a] + 1]++;
-  for (int64_t I = 0; I < nc; I++) captr[I + 1] += captr[I];
-  std::vector<int32_t> calist(c.nb);
-  {
-    std::vector<int64_t> fill(captr.begin(), captr.end() - 1);
-    for (int64_t a = 0; a < c.nb; a++) calist[fill[agg[a]]++] = (int32_t)a;
-  }
-  // coarse graph from the block graph edges
-  std::vector<int64_t> ubia((size_t)c.nb + 1);
-  std::vector<int32_t> ubja(std::max<int64_t>(c.nbu, 1));
-  PZG_HIP(hipMemcpy(ubia.data(), c.d_ubia.p, sizeof(int64_t) * (c.nb + 1), hipMemcpyDeviceToHost));
-  if (c.nbu) PZG_HIP(hipMemcpy(ubja.data(), c.d_ubja.p, sizeof(int32_t) * c.nbu, hipMemcpyDeviceToHost));
-  std::vector<std::vector<int32_t>> nbr(nc);
-  for (int64_t a = 0; a < c.nb; a++)
-    for (int64_t k = ubia[a]; k < ubia[a + 1]; k++) {
-      int32_t I = agg[a], J = agg[ubja[k]];
-      if (I != J) {
-        nbr[I].push_back(J);
-        nbr[J].push_back(I);
-      }
-    }
-  std::vector<int64_t> cia(nc + 1, 0);
-  std::vector<int32_t> cja;
-  for (int64_t I = 0; I < nc; I++) {
-    auto &v = nbr[I];
-    std::sort(v.begin(), v.end());
-    v.erase(std::unique(v.begin(), v.end()), v.end());
-    cja.insert(cja.end(), v.begin(), v.end());
-    cia[I + 1] = (int64_t)cja.size();
-  }
-  if (cja.empty()) cja.push_back(0);
-  std::vector<float> wI(nc);
-  for (int64_t I = 0; I < nc; I++) wI[I] = (float)(captr[I + 1] - captr[I]);
-  hipStream_t s = c.stream;
-  c.nn_nc = nc;
-  c.nn_agg_host = agg;
-  c.nn_cia_host = cia;
-  c.nn_cja_host.assign(cja.begin(), cja.begin() + cia[nc]);
-  PZG_TRY(dev_upload(c.d_nn_agg, agg.data(), agg.size(), s));
-  PZG_TRY(dev_upload(c.d_nn_captr, captr.data(), captr.size(), s));
-  PZG_TRY(dev_upload(c.d_nn_calist, calist.data(), calist.size(), s));
-  PZG_TRY(dev_upload(c.d_nn_cia, cia.data(), cia.size(), s));
-  PZG_TRY(dev_upload(c.d_nn_cja, cja.data(), cja.size(), s));
-  PZG_TRY(dev_upload(c.d_nn_w8, wI.data(), wI.size(), s));
-  // weights: W0, A, B ~ N(0, 1/fan_in), Wout = 0 (starts as block Jacobi)
-  const int bs = c.bs;
-  NNOff o = nn_offsets(bs);
+  const int ne = c.tvol.nsh * c.bs;
+  c.les_ne = ne;
+  LesOff o = les_offsets(ne);
   std::vector<float> th(o.n, 0.f);
+  // G = diag(1 / elements sharing the side of the dof's shape), the sharing
+  // of an interior element of a structured mesh: 2^(dim - side dimension)
+  for (int s = 0; s < c.tvol.nsh; s++) {
+    int loc[8];
+    const int nc = side_corner_list_pub(c.vol_etype, c.tvol.shape_side[s], loc);
+    const int sdim = nc == 1 ? 0 : nc == 2 ? 1 : nc == 4 ? 2 : 3;
+    const float mult = (float)(1 << std::max(0, c.dim - sdim));
+    for (int k = 0; k < c.bs; k++) th[o.g + (s * c.bs + k) * ne + s * c.bs + k] = 1.f / mult;
+  }
   std::mt19937_64 rng(seed);
   std::normal_distribution<float> nd(0.f, 1.f);
-  for (int k = 0; k < bs * NNF; k++) th[o.w0 + k] = nd(rng) / std::sqrt((float)bs);
+  for (int k = 0; k < LES_FIN * NNF; k++) th[o.w0 + k] = nd(rng) / std::sqrt((float)LES_FIN);
   for (int l = 0; l < NNL; l++)
     for (int k = 0; k < NNF * NNF; k++) {
       th[o.a[l] + k] = nd(rng) / std::sqrt(2.f * NNF);
       th[o.b[l] + k] = nd(rng) / std::sqrt(2.f * NNF);
     }
-  PZG_TRY(dev_upload(c.d_nn_w, th.data(), th.size(), s));
-  PZG_TRY(dev_alloc(c.d_nn_m, sizeof(float) * o.n));
-  PZG_TRY(dev_alloc(c.d_nn_v, sizeof(float) * o.n));
-  PZG_TRY(dev_alloc(c.d_nn_g, sizeof(float) * o.n));
-  PZG_HIP(hipMemsetAsync(c.d_nn_m.p, 0, sizeof(float) * o.n, s));
-  PZG_HIP(hipMemsetAsync(c.d_nn_v.p, 0, sizeof(float) * o.n, s));
-  c.nn_adam_t = 0;
-  c.nn_nsamp = 0;
-  c.nn_maxsamp = std::max(1, max_samples);
-  PZG_TRY(dev_alloc(c.d_nn_X, sizeof(float) * c.nn_maxsamp * nc * bs));
-  PZG_TRY(dev_alloc(c.d_nn_T, sizeof(float) * c.nn_maxsamp * nc * bs));
-  // inference workspace (one sample)
-  PZG_TRY(dev_alloc(c.d_nn_xi, sizeof(float) * nc * bs));
-  PZG_TRY(dev_alloc(c.d_nn_zi, sizeof(float) * nc * bs));
-  PZG_TRY(dev_alloc(c.d_nn_h0, sizeof(float) * nc * NNF));
-  PZG_TRY(dev_alloc(c.d_nn_h1, sizeof(float) * nc * NNF));
-  PZG_TRY(dev_alloc(c.d_nn_mi, sizeof(float) * nc * NNF));
+  hipStream_t s = c.stream;
+  PZG_TRY(dev_upload(c.d_les_th, th.data(), th.size(), s));
+  std::vector<float> zero(o.n, 0.f);
+  PZG_TRY(dev_upload(c.d_les_m, zero.data(), zero.size(), s));
+  PZG_TRY(dev_upload(c.d_les_v, zero.data(), zero.size(), s));
+  PZG_TRY(dev_alloc(c.d_les_g, sizeof(float) * o.n));
   PZG_HIP(hipStreamSynchronize(s));
-  c.nn_ready = true;
+  c.les_adam_t = 0;
+  c.les_nr = 0;
+  c.les_A.clear();
+  c.les_B.clear();
+  c.les_fit_rows = 0;
+  c.les_fit_stale = false;
+  c.les_ready = true;
+  c.mg_ready = false;  // a smoother of the cycle changed
   return PZG_OK;
 }
 
-// Z = N(X) for 'ns' samples laid out [s][I]; the intermediate tensors are
-// kept when 'keep' (training).
-static int nn_forward(Ctx &c, int64_t ns, const float *X, float *Z, bool keep) {
-  const int64_t nc = c.nn_nc, rows = ns * nc;
-  const float *th = c.d_nn_w.as<float>();
-  NNOff o = nn_offsets(c.bs);
-  float *Hbuf[NNL + 1];
-  float *Mbuf[NNL], *Pbuf[NNL];
-  if (keep) {
-    for (int l = 0; l <= NNL; l++) Hbuf[l] = c.d_nn_trH.as<float>() + (int64_t)l * rows * NNF;
-    for (int l = 0; l < NNL; l++) {
-      Mbuf[l] = c.d_nn_trM.as<float>() + (int64_t)l * rows * NNF;
-      Pbuf[l] = c.d_nn_trP.as<float>() + (int64_t)l * rows * NNF;
-    }
-  } else {
-    for (int l = 0; l <= NNL; l++) Hbuf[l] = (l % 2 == 0) ? c.d_nn_h0.as<float>() : c.d_nn_h1.as<float>();
-    for (int l = 0; l < NNL; l++) {
-      Mbuf[l] = c.d_nn_mi.as<float>();
-      Pbuf[l] = nullptr;
-    }
+// per-element scales of the current assembly and the apply buffers
+int les_prepare(Ctx &c) {
+  if (!c.les_ready) {
+    set_error("learned preconditioner: call pzg_learned_setup first");
+    return PZG_ESTATE;
   }
-  const int g = grid1(rows * NNF);
-  if (c.bs == 3) hipLaunchKernelGGL(k_nn_in<3>, dim3(g), dim3(TPB), 0, c.stream, rows, X, th + o.w0, Hbuf[0]);
-  else hipLaunchKernelGGL(k_nn_in<1>, dim3(g), dim3(TPB), 0, c.stream, rows, X, th + o.w0, Hbuf[0]);
-  for (int l = 0; l < NNL; l++) {
-    hipLaunchKernelGGL(k_nn_agg, dim3(g), dim3(TPB), 0, c.stream, rows, nc, c.d_nn_cia.as<int64_t>(),
-                       c.d_nn_cja.as<int32_t>(), Hbuf[l], Mbuf[l]);
-    hipLaunchKernelGGL(k_nn_layer, dim3(g), dim3(TPB), 0, c.stream, rows, Hbuf[l], Mbuf[l],
-                       th + o.a[l], th + o.b[l], Pbuf[l], Hbuf[l + 1]);
+  if (c.tvol.nsh * c.bs != c.les_ne || !les_supported(c)) {
+    set_error("learned preconditioner: the mesh's element differs from the trained one");
+    return PZG_EUNSUP;
   }
-  const int gz = grid1(rows * c.bs);
-  if (c.bs == 3) hipLaunchKernelGGL(k_nn_out<3>, dim3(gz), dim3(TPB), 0, c.stream, rows, Hbuf[NNL], th + o.wout, Z);
-  else hipLaunchKernelGGL(k_nn_out<1>, dim3(gz), dim3(TPB), 0, c.stream, rows, Hbuf[NNL], th + o.wout, Z);
+  PZG_TRY(mf_prepare(c));  // element -> block map and row-ordered output slots
+  PZG_TRY(les_graph(c));
+  PZG_TRY(les_features(c));
+  PZG_TRY(les_forward(c, false));
+  const size_t nye = (size_t)std::max<int64_t>(c.mf_nout, 1) * c.bs;
+  if (c.d_les_ye.bytes < nye * sizeof(float)) {
+    PZG_TRY(dev_alloc(c.d_les_ye, nye * sizeof(float)));
+    // slots of boundary elements are never written: zero once
+    PZG_HIP(hipMemsetAsync(c.d_les_ye.p, 0, nye * sizeof(float), c.stream));
+  }
+  PZG_HIP(hipStreamSynchronize(c.stream));
+  return PZG_OK;
+}
+
+// z = M r (learned smoother), both FP32 device vectors of neq entries
+int les_apply(Ctx &c, const float *r, float *z, const SolveScalars *sc) {
+  const float *G = c.d_les_th.as<float>() + les_offsets(c.les_ne).g;
+  const int nsh = c.tvol.nsh;
+  const int32_t *blk = c.d_mf_blk.as<int32_t>(), *pos = c.d_mf_pos.as<int32_t>();
+  const float *pen = c.d_les_pen.as<float>(), *s = c.d_les_s.as<float>();
+  float *ye = c.d_les_ye.as<float>();
+  const int64_t nv = c.nvol;
+  const int gm = (int)std::max<int64_t>(1, std::min<int64_t>((nv + 63) / 64, 4096));
+  if (nsh == 27 && c.bs == 3)
+    hipLaunchKernelGGL((k_les_mfma<27, 3>), dim3(gm), dim3(256), 0, c.stream, nv, G, s, blk, pos, pen, r, ye, sc);
+  else if (nsh == 27 && c.bs == 1)
+    hipLaunchKernelGGL((k_les_mfma<27, 1>), dim3(gm), dim3(256), 0, c.stream, nv, G, s, blk, pos, pen, r, ye, sc);
+  else if (nsh == 8 && c.bs == 3)
+    hipLaunchKernelGGL((k_les_mfma<8, 3>), dim3(gm), dim3(256), 0, c.stream, nv, G, s, blk, pos, pen, r, ye, sc);
+  else if (nsh == 8 && c.bs == 1)
+    hipLaunchKernelGGL((k_les_gen<8, 1>), dim3(grid1(nv * 8)), dim3(256), 0, c.stream, nv, G, s, blk, pos, pen, r, ye, sc);
+  else if (nsh == 9 && c.bs == 1)
+    hipLaunchKernelGGL((k_les_gen<9, 1>), dim3(grid1(nv * 9)), dim3(256), 0, c.stream, nv, G, s, blk, pos, pen, r, ye, sc);
+  else if (nsh == 4 && c.bs == 1)
+    hipLaunchKernelGGL((k_les_gen<4, 1>), dim3(grid1(nv * 4)), dim3(256), 0, c.stream, nv, G, s, blk, pos, pen, r, ye, sc);
+  else {
+    set_error("learned preconditioner: element not supported");
+    return PZG_EUNSUP;
+  }
+  const int gg = grid1(c.nb);
+  if (c.bs == 3)
+    hipLaunchKernelGGL(k_les_gather<3>, dim3(gg), dim3(256), 0, c.stream, c.nb, c.d_mf_ptr.as<int64_t>(), ye, pen, r, z, sc);
+  else
+    hipLaunchKernelGGL(k_les_gather<1>, dim3(gg), dim3(256), 0, c.stream, c.nb, c.d_mf_ptr.as<int64_t>(), ye, pen, r, z, sc);
   PZG_HIP(hipGetLastError());
   return PZG_OK;
 }
 
-// Preconditioner application inside FCG: z already holds B^-1 r.
-int nn_apply(Ctx &c, const float *r, float *z, float *p, const float *q, SolveScalars *sc,
-             double *partials, unsigned *counter) {
-  const int64_t nc = c.nn_nc;
-  const int gr = grid1(nc * 64);
+// (residual, error) pairs of every element's local problem of the assembled
+// system: nrhs Gaussian residuals per element (zero on penalized dofs) and
+// their local solutions
+int les_collect(Ctx &c, int nrhs, uint64_t seed) {
+  if (!c.les_ready || !c.assembled) {
+    set_error("learned preconditioner: call pzg_learned_setup on an assembled system first");
+    return PZG_ESTATE;
+  }
+  if (nrhs < 1 || nrhs > 8) {
+    set_error("learned preconditioner: 1..8 residuals per element");
+    return PZG_EINVAL;
+  }
+  PZG_TRY(les_prepare(c));
+  if (!c.full_valid) PZG_TRY(launch_expand_full(c));
+  if (!c.scal_valid) PZG_TRY(launch_build_bsr(c, false));
+  const int ne = c.les_ne, nsh = c.tvol.nsh;
+  const int64_t nv = c.nvol;
+  PZG_TRY(dev_alloc(c.d_les_Se, sizeof(double) * nv * ne * ne));
+  const int64_t nt = nv * nsh * nsh;
   if (c.bs == 3)
-    hipLaunchKernelGGL(k_nn_restrict<3>, dim3(gr), dim3(TPB), 0, c.stream, nc, c.d_nn_captr.as<int64_t>(),
-                       c.d_nn_calist.as<int32_t>(), r, (const float *)nullptr, c.d_nn_xi.as<float>());
+    hipLaunchKernelGGL(k_les_local<3>, dim3(grid1(nt)), dim3(TPB), 0, c.stream, nv, nsh, c.d_mf_blk.as<int32_t>(),
+                       c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(), c.d_fa.as<double>(),
+                       c.d_dinvsqrt.as<double>(), c.d_les_pen.as<float>(), c.d_les_Se.as<double>());
   else
-    hipLaunchKernelGGL(k_nn_restrict<1>, dim3(gr), dim3(TPB), 0, c.stream, nc, c.d_nn_captr.as<int64_t>(),
-                       c.d_nn_calist.as<int32_t>(), r, (const float *)nullptr, c.d_nn_xi.as<float>());
-  PZG_TRY(nn_forward(c, 1, c.d_nn_xi.as<float>(), c.d_nn_zi.as<float>(), false));
-  // same grid as k_fcg_update, whose partials slot 1 this kernel replaces
-  int64_t g = std::min<int64_t>(MAXBLK, std::max<int64_t>(1, (c.nb + TPB - 1) / TPB));
-  if (c.bs == 3)
-    hipLaunchKernelGGL(k_nn_prolong<3>, dim3(g), dim3(TPB), 0, c.stream, c.nb, c.d_nn_agg.as<int32_t>(),
-                       c.d_nn_zi.as<float>(), z, p, q, sc, partials, counter);
-  else
-    hipLaunchKernelGGL(k_nn_prolong<1>, dim3(g), dim3(TPB), 0, c.stream, c.nb, c.d_nn_agg.as<int32_t>(),
-                       c.d_nn_zi.as<float>(), z, p, q, sc, partials, counter);
+    hipLaunchKernelGGL(k_les_local<1>, dim3(grid1(nt)), dim3(TPB), 0, c.stream, nv, nsh, c.d_mf_blk.as<int32_t>(),
+                       c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(), c.d_fa.as<double>(),
+                       c.d_dinvsqrt.as<double>(), c.d_les_pen.as<float>(), c.d_les_Se.as<double>());
   PZG_HIP(hipGetLastError());
+  // residuals (host, seeded) and the per-element dof mask
+  std::vector<int32_t> blk((size_t)nv * nsh);
+  PZG_HIP(hipMemcpyAsync(blk.data(), c.d_mf_blk.p, blk.size() * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+  PZG_HIP(hipStreamSynchronize(c.stream));
+  std::vector<float> mask((size_t)nv * ne), R((size_t)nv * nrhs * ne);
+  std::mt19937_64 rng(seed);
+  std::normal_distribution<float> nd(0.f, 1.f);
+  for (int64_t e = 0; e < nv; e++)
+    for (int i = 0; i < ne; i++) {
+      const int32_t b = blk[e * nsh + i / c.bs];
+      mask[e * ne + i] = (b >= 0 && c.les_pen[(int64_t)b * c.bs + i % c.bs] == 0.f) ? 1.f : 0.f;
+    }
+  for (int64_t e = 0; e < nv; e++)
+    for (int k = 0; k < nrhs; k++)
+      for (int i = 0; i < ne; i++) R[(e * nrhs + k) * ne + i] = nd(rng) * mask[e * ne + i];
+  hipStream_t s = c.stream;
+  PZG_TRY(dev_upload(c.d_les_M, mask.data(), mask.size(), s));
+  PZG_TRY(dev_upload(c.d_les_R, R.data(), R.size(), s));
+  PZG_TRY(dev_alloc(c.d_les_E, sizeof(float) * R.size()));
+  PZG_TRY(dev_alloc(c.d_les_fail, sizeof(int)));
+  PZG_HIP(hipMemsetAsync(c.d_les_fail.p, 0, sizeof(int), s));
+  hipLaunchKernelGGL(k_les_solve, dim3(nv), dim3(256), 0, s, ne, nrhs, c.d_les_Se.as<double>(),
+                     c.d_les_R.as<float>(), c.d_les_E.as<float>(), c.d_les_fail.as<int>());
+  PZG_HIP(hipGetLastError());
+  int fail = 0;
+  std::vector<float> E(R.size());
+  PZG_HIP(hipMemcpyAsync(&fail, c.d_les_fail.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  PZG_HIP(hipMemcpyAsync(E.data(), c.d_les_E.p, E.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+  PZG_HIP(hipStreamSynchronize(s));
+  if (fail) {
+    set_error("learned preconditioner: an element block is not positive definite");
+    return PZG_EUNSUP;
+  }
+  double den = 0.;
+  for (size_t i = 0; i < E.size(); i++) den += (double)E[i] * E[i];
+  c.les_den = den > 0. ? den : 1.;
+  c.les_nr = nrhs;
+  dev_free(c.d_les_Se);
+  // least-squares sums of the G fit over the elements without penalized dofs
+  std::vector<float> Rf, Ef;
+  for (int64_t e = 0; e < nv; e++) {
+    bool full = true;
+    for (int i = 0; i < ne && full; i++) full = mask[e * ne + i] != 0.f;
+    if (!full) continue;
+    Rf.insert(Rf.end(), R.begin() + e * nrhs * ne, R.begin() + (e + 1) * nrhs * ne);
+    Ef.insert(Ef.end(), E.begin() + e * nrhs * ne, E.begin() + (e + 1) * nrhs * ne);
+  }
+  const int64_t rows = (int64_t)Rf.size() / ne;
+  if (c.les_A.size() != (size_t)ne * ne) {
+    c.les_A.assign((size_t)ne * ne, 0.);
+    c.les_B.assign((size_t)ne * ne, 0.);
+  }
+  if (rows > 0) {
+    DevBuf dR, dE;
+    PZG_TRY(dev_upload(dR, Rf.data(), Rf.size(), s));
+    PZG_TRY(dev_upload(dE, Ef.data(), Ef.size(), s));
+    PZG_TRY(dev_alloc(c.d_nn_part, sizeof(float) * NBLK_RED * std::max(ne * ne, NNF * NNF)));
+    PZG_TRY(dev_alloc(c.d_les_g, sizeof(float) * les_offsets(ne).n));
+    const int nred = (int)std::min<int64_t>(NBLK_RED, std::max<int64_t>(1, rows / 64));
+    std::vector<float> h((size_t)ne * ne);
+    for (int which = 0; which < 2; which++) {
+      const float *Lm = which ? dE.as<float>() : dR.as<float>();
+      float *part = c.d_nn_part.as<float>();
+      switch (ne) {
+        case 81: hipLaunchKernelGGL((k_nn_outer<81, 81>), dim3(nred), dim3(TPB), 0, s, rows, Lm, dR.as<float>(), part); break;
+        case 27: hipLaunchKernelGGL((k_nn_outer<27, 27>), dim3(nred), dim3(TPB), 0, s, rows, Lm, dR.as<float>(), part); break;
+        case 24: hipLaunchKernelGGL((k_nn_outer<24, 24>), dim3(nred), dim3(TPB), 0, s, rows, Lm, dR.as<float>(), part); break;
+        case 9: hipLaunchKernelGGL((k_nn_outer<9, 9>), dim3(nred), dim3(TPB), 0, s, rows, Lm, dR.as<float>(), part); break;
+        case 8: hipLaunchKernelGGL((k_nn_outer<8, 8>), dim3(nred), dim3(TPB), 0, s, rows, Lm, dR.as<float>(), part); break;
+        case 4: hipLaunchKernelGGL((k_nn_outer<4, 4>), dim3(nred), dim3(TPB), 0, s, rows, Lm, dR.as<float>(), part); break;
+        default:
+          set_error("learned preconditioner: element not supported");
+          return PZG_EUNSUP;
+      }
+      hipLaunchKernelGGL(k_nn_sum_partials, dim3(grid1(ne * ne)), dim3(TPB), 0, s, nred, ne * ne, part,
+                         c.d_les_g.as<float>());
+      PZG_HIP(hipGetLastError());
+      PZG_HIP(hipMemcpyAsync(h.data(), c.d_les_g.p, sizeof(float) * ne * ne, hipMemcpyDeviceToHost, s));
+      PZG_HIP(hipStreamSynchronize(s));
+      std::vector<double> &acc = which ? c.les_B : c.les_A;
+      for (int k = 0; k < ne * ne; k++) acc[k] += h[k];
+    }
+    dev_free(dR);
+    dev_free(dE);
+    c.les_fit_rows += rows;
+    c.les_fit_stale = true;
+  }
   return PZG_OK;
 }
 
-// loss and gradient (into d_nn_g) over ns samples X, T (device pointers)
-int nn_loss_grad(Ctx &c, int64_t ns, const float *X, const float *T, double *loss_out) {
-  const int64_t nc = c.nn_nc, rows = ns * nc;
-  const int bs = c.bs;
-  NNOff o = nn_offsets(bs);
-  size_t need = (size_t)rows * NNF;
-  PZG_TRY(dev_alloc(c.d_nn_trH, sizeof(float) * need * (NNL + 1)));
-  PZG_TRY(dev_alloc(c.d_nn_trM, sizeof(float) * need * NNL));
-  PZG_TRY(dev_alloc(c.d_nn_trP, sizeof(float) * need * NNL));
+// eigen-decomposition of a symmetric n x n matrix (cyclic Jacobi rotations):
+// a is destroyed, w receives the eigenvalues, v the eigenvectors (columns)
+static void sym_eig(std::vector<double> &a, int n, std::vector<double> &w, std::vector<double> &v) {
+  v.assign((size_t)n * n, 0.);
+  for (int i = 0; i < n; i++) v[(size_t)i * n + i] = 1.;
+  for (int sweep = 0; sweep < 60; sweep++) {
+    double off = 0.;
+    for (int i = 0; i < n; i++)
+      for (int j = i + 1; j < n; j++) off += a[(size_t)i * n + j] * a[(size_t)i * n + j];
+    if (off < 1e-30) break;
+    for (int p = 0; p < n; p++)
+      for (int q = p + 1; q < n; q++) {
+        const double apq = a[(size_t)p * n + q];
+        if (std::fabs(apq) < 1e-300) continue;
+        const double app = a[(size_t)p * n + p], aqq = a[(size_t)q * n + q];
+        const double th = 0.5 * (aqq - app) / apq;
+        const double t = (th >= 0 ? 1. : -1.) / (std::fabs(th) + std::sqrt(th * th + 1.));
+        const double cs = 1. / std::sqrt(t * t + 1.), sn = t * cs;
+        for (int k = 0; k < n; k++) {
+          const double akp = a[(size_t)k * n + p], akq = a[(size_t)k * n + q];
+          a[(size_t)k * n + p] = cs * akp - sn * akq;
+          a[(size_t)k * n + q] = sn * akp + cs * akq;
+        }
+        for (int k = 0; k < n; k++) {
+          const double apk = a[(size_t)p * n + k], aqk = a[(size_t)q * n + k];
+          a[(size_t)p * n + k] = cs * apk - sn * aqk;
+          a[(size_t)q * n + k] = sn * apk + cs * aqk;
+        }
+        for (int k = 0; k < n; k++) {
+          const double vkp = v[(size_t)k * n + p], vkq = v[(size_t)k * n + q];
+          v[(size_t)k * n + p] = cs * vkp - sn * vkq;
+          v[(size_t)k * n + q] = sn * vkp + cs * vkq;
+        }
+      }
+  }
+  w.resize(n);
+  for (int i = 0; i < n; i++) w[i] = a[(size_t)i * n + i];
+}
+
+// G = B A^-1 from the accumulated sums, symmetrised and made positive definite
+int les_fit_G(Ctx &c) {
+  if (!c.les_fit_stale) return PZG_OK;
+  const int ne = c.les_ne;
+  if (c.les_fit_rows < ne) {
+    set_error("learned preconditioner: too few interior pairs for the element fit");
+    return PZG_EINVAL;
+  }
+  // A = V diag(w) V^T  ->  G = B V diag(1/w) V^T
+  std::vector<double> a(c.les_A), w, V;
+  sym_eig(a, ne, w, V);
+  double wmax = 0.;
+  for (double x : w) wmax = std::max(wmax, x);
+  std::vector<double> BV((size_t)ne * ne, 0.), G((size_t)ne * ne, 0.);
+  for (int i = 0; i < ne; i++)
+    for (int k = 0; k < ne; k++) {
+      double t = 0.;
+      for (int j = 0; j < ne; j++) t += c.les_B[(size_t)i * ne + j] * V[(size_t)j * ne + k];
+      BV[(size_t)i * ne + k] = t / std::max(w[k], 1e-12 * wmax);
+    }
+  for (int i = 0; i < ne; i++)
+    for (int j = 0; j < ne; j++) {
+      double t = 0.;
+      for (int k = 0; k < ne; k++) t += BV[(size_t)i * ne + k] * V[(size_t)j * ne + k];
+      G[(size_t)i * ne + j] = t;
+    }
+  for (int i = 0; i < ne; i++)
+    for (int j = i + 1; j < ne; j++) {
+      const double m = 0.5 * (G[(size_t)i * ne + j] + G[(size_t)j * ne + i]);
+      G[(size_t)i * ne + j] = G[(size_t)j * ne + i] = m;
+    }
+  // positive definite projection
+  std::vector<double> g(G), u, U;
+  sym_eig(g, ne, u, U);
+  double umax = 0.;
+  for (double x : u) umax = std::max(umax, x);
+  if (!(umax > 0.)) {
+    set_error("learned preconditioner: the element fit has no positive eigenvalue");
+    return PZG_EUNSUP;
+  }
+  for (double &x : u) x = std::max(x, 1e-3 * umax);
+  std::vector<float> Gf((size_t)ne * ne);
+  for (int i = 0; i < ne; i++)
+    for (int j = 0; j < ne; j++) {
+      double t = 0.;
+      for (int k = 0; k < ne; k++) t += U[(size_t)i * ne + k] * u[k] * U[(size_t)j * ne + k];
+      Gf[(size_t)i * ne + j] = (float)t;
+    }
+  PZG_HIP(hipMemcpyAsync(c.d_les_th.as<float>() + les_offsets(ne).g, Gf.data(), sizeof(float) * ne * ne,
+                         hipMemcpyHostToDevice, c.stream));
+  PZG_HIP(hipStreamSynchronize(c.stream));
+  c.les_fit_stale = false;
+  c.mg_ready = false;
+  return PZG_OK;
+}
+
+// loss and gradient (into d_les_g) on the stored pairs
+int les_loss_grad(Ctx &c, double *loss_out) {
+  if (!c.les_ready || c.les_nr == 0) {
+    set_error("learned preconditioner: no training pairs (pzg_learned_collect)");
+    return PZG_ESTATE;
+  }
+  const int ne = c.les_ne, nr = (int)c.les_nr;
+  const int64_t nv = c.nvol, rows = nv, need = rows * NNF;
+  LesOff o = les_offsets(ne);
+  PZG_TRY(les_features(c));
+  PZG_TRY(les_forward(c, true));
+  PZG_TRY(dev_alloc(c.d_les_D, sizeof(float) * nv * nr * ne));
+  PZG_TRY(dev_alloc(c.d_les_le, sizeof(double) * nv));
+  PZG_TRY(dev_alloc(c.d_les_ds, sizeof(double) * nv));
+  PZG_TRY(dev_alloc(c.d_les_loss, sizeof(double)));
+  PZG_TRY(dev_alloc(c.d_nn_dZ, sizeof(float) * rows));
   PZG_TRY(dev_alloc(c.d_nn_dH, sizeof(float) * need));
   PZG_TRY(dev_alloc(c.d_nn_G, sizeof(float) * need));
   PZG_TRY(dev_alloc(c.d_nn_tmp, sizeof(float) * need));
-  PZG_TRY(dev_alloc(c.d_nn_Z, sizeof(float) * rows * bs));
-  PZG_TRY(dev_alloc(c.d_nn_dZ, sizeof(float) * rows * bs));
-  PZG_TRY(dev_alloc(c.d_nn_den, sizeof(double) * std::max<int64_t>(ns, 1)));
-  PZG_TRY(dev_alloc(c.d_nn_part, sizeof(float) * NBLK_RED * NNF * NNF));
-  PZG_TRY(dev_alloc(c.d_nn_loss, sizeof(double)));
-  if (!c.d_partials.p) {
-    PZG_TRY(dev_alloc(c.d_partials, sizeof(double) * 4 * 2048));
-    PZG_TRY(dev_alloc(c.d_counter, sizeof(unsigned) * 16));
-    PZG_HIP(hipMemsetAsync(c.d_counter.p, 0, sizeof(unsigned) * 16, c.stream));
-  }
-  PZG_TRY(nn_forward(c, ns, X, c.d_nn_Z.as<float>(), true));
-  float *H = c.d_nn_trH.as<float>(), *M = c.d_nn_trM.as<float>(), *P = c.d_nn_trP.as<float>();
-  const float *th = c.d_nn_w.as<float>();
-  float *gr = c.d_nn_g.as<float>();
-  const int gnode = grid1(rows * NNF), gz = grid1(rows * bs);
-  const int glos = (int)std::min<int64_t>(2048, gz);
-  if (bs == 3) {
-    hipLaunchKernelGGL(k_nn_den<3>, dim3(ns), dim3(TPB), 0, c.stream, nc, T, c.d_nn_w8.as<float>(), c.d_nn_den.as<double>());
-    hipLaunchKernelGGL(k_nn_loss<3>, dim3(glos), dim3(TPB), 0, c.stream, rows, nc, c.d_nn_Z.as<float>(), T,
-                       c.d_nn_w8.as<float>(), c.d_nn_den.as<double>(), c.d_nn_dZ.as<float>(),
-                       c.d_nn_loss.as<double>(), c.d_partials.as<double>(), c.d_counter.as<unsigned>());
-    hipLaunchKernelGGL(k_nn_back_out<3>, dim3(gnode), dim3(TPB), 0, c.stream, rows, c.d_nn_dZ.as<float>(),
-                       th + o.wout, c.d_nn_dH.as<float>());
-  } else {
-    hipLaunchKernelGGL(k_nn_den<1>, dim3(ns), dim3(TPB), 0, c.stream, nc, T, c.d_nn_w8.as<float>(), c.d_nn_den.as<double>());
-    hipLaunchKernelGGL(k_nn_loss<1>, dim3(glos), dim3(TPB), 0, c.stream, rows, nc, c.d_nn_Z.as<float>(), T,
-                       c.d_nn_w8.as<float>(), c.d_nn_den.as<double>(), c.d_nn_dZ.as<float>(),
-                       c.d_nn_loss.as<double>(), c.d_partials.as<double>(), c.d_counter.as<unsigned>());
-    hipLaunchKernelGGL(k_nn_back_out<1>, dim3(gnode), dim3(TPB), 0, c.stream, rows, c.d_nn_dZ.as<float>(),
-                       th + o.wout, c.d_nn_dH.as<float>());
-  }
-  const int nred = (int)std::min<int64_t>(NBLK_RED, std::max<int64_t>(1, rows / 64));
-  auto outer = [&](const float *Lm, const float *Rm, int fa, int fb, int off) {
-    if (fa == NNF && fb == NNF)
-      hipLaunchKernelGGL((k_nn_outer<NNF, NNF>), dim3(nred), dim3(TPB), 0, c.stream, rows, Lm, Rm, c.d_nn_part.as<float>());
-    else if (fa == NNF && fb == 3)
-      hipLaunchKernelGGL((k_nn_outer<NNF, 3>), dim3(nred), dim3(TPB), 0, c.stream, rows, Lm, Rm, c.d_nn_part.as<float>());
-    else if (fa == NNF && fb == 1)
-      hipLaunchKernelGGL((k_nn_outer<NNF, 1>), dim3(nred), dim3(TPB), 0, c.stream, rows, Lm, Rm, c.d_nn_part.as<float>());
-    else if (fa == 3)
-      hipLaunchKernelGGL((k_nn_outer<3, NNF>), dim3(nred), dim3(TPB), 0, c.stream, rows, Lm, Rm, c.d_nn_part.as<float>());
-    else
-      hipLaunchKernelGGL((k_nn_outer<1, NNF>), dim3(nred), dim3(TPB), 0, c.stream, rows, Lm, Rm, c.d_nn_part.as<float>());
-    hipLaunchKernelGGL(k_nn_sum_partials, dim3(grid1(fa * fb)), dim3(TPB), 0, c.stream, nred, fa * fb,
-                       c.d_nn_part.as<float>(), gr + off);
+  PZG_TRY(dev_alloc(c.d_nn_part, sizeof(float) * NBLK_RED * std::max(ne * ne, NNF * NNF)));
+  const float *th = c.d_les_th.as<float>();
+  float *gr = c.d_les_g.as<float>();
+  hipStream_t s = c.stream;
+  hipLaunchKernelGGL(k_les_fwd, dim3(nv), dim3(128), 0, s, ne, nr, th + o.g, c.d_les_s.as<float>(),
+                     c.d_les_M.as<float>(), c.d_les_R.as<float>(), c.d_les_E.as<float>(), c.les_den,
+                     c.d_les_D.as<float>(), c.d_les_le.as<double>(), c.d_les_ds.as<double>());
+  hipLaunchKernelGGL(k_les_sum, dim3(1), dim3(256), 0, s, nv, c.d_les_le.as<double>(), c.d_les_loss.as<double>());
+  hipLaunchKernelGGL(k_les_dz, dim3(grid1(nv)), dim3(TPB), 0, s, nv, c.d_les_z.as<float>(),
+                     c.d_les_s.as<float>(), c.d_les_ds.as<double>(), c.les_den, c.d_nn_dZ.as<float>());
+  PZG_HIP(hipGetLastError());
+  // gradient of G: sum over (element, pair) rows of Dsc (x) r, symmetrised
+  const int64_t grow = nv * nr;
+  const int nred = (int)std::min<int64_t>(NBLK_RED, std::max<int64_t>(1, grow / 64));
+  auto outerG = [&]() -> int {
+    float *part = c.d_nn_part.as<float>();
+    const float *Lm = c.d_les_D.as<float>(), *Rm = c.d_les_R.as<float>();
+    switch (ne) {
+      case 81: hipLaunchKernelGGL((k_nn_outer<81, 81>), dim3(nred), dim3(TPB), 0, s, grow, Lm, Rm, part); break;
+      case 27: hipLaunchKernelGGL((k_nn_outer<27, 27>), dim3(nred), dim3(TPB), 0, s, grow, Lm, Rm, part); break;
+      case 24: hipLaunchKernelGGL((k_nn_outer<24, 24>), dim3(nred), dim3(TPB), 0, s, grow, Lm, Rm, part); break;
+      case 9: hipLaunchKernelGGL((k_nn_outer<9, 9>), dim3(nred), dim3(TPB), 0, s, grow, Lm, Rm, part); break;
+      case 8: hipLaunchKernelGGL((k_nn_outer<8, 8>), dim3(nred), dim3(TPB), 0, s, grow, Lm, Rm, part); break;
+      case 4: hipLaunchKernelGGL((k_nn_outer<4, 4>), dim3(nred), dim3(TPB), 0, s, grow, Lm, Rm, part); break;
+      default:
+        set_error("learned preconditioner: element not supported");
+        return PZG_EUNSUP;
+    }
+    hipLaunchKernelGGL(k_nn_sum_partials, dim3(grid1(ne * ne)), dim3(TPB), 0, s, nred, ne * ne, part, gr + o.g);
+    hipLaunchKernelGGL(k_les_sym, dim3(grid1(ne * ne)), dim3(TPB), 0, s, ne, gr + o.g);
+    return PZG_OK;
   };
-  // dWout = H_L^T dZ
-  outer(H + (int64_t)NNL * rows * NNF, c.d_nn_dZ.as<float>(), NNF, bs, o.wout);
+  PZG_TRY(outerG());
+  // network backward (rows = elements)
+  float *H = c.d_nn_trH.as<float>(), *M = c.d_nn_trM.as<float>(), *P = c.d_nn_trP.as<float>();
+  const int gnode = grid1(need);
+  const int nrn = (int)std::min<int64_t>(NBLK_RED, std::max<int64_t>(1, rows / 64));
+  auto outer = [&](const float *Lm, const float *Rm, int fa, int fb, int off) {
+    float *part = c.d_nn_part.as<float>();
+    if (fa == NNF && fb == NNF)
+      hipLaunchKernelGGL((k_nn_outer<NNF, NNF>), dim3(nrn), dim3(TPB), 0, s, rows, Lm, Rm, part);
+    else if (fa == NNF && fb == 1)
+      hipLaunchKernelGGL((k_nn_outer<NNF, 1>), dim3(nrn), dim3(TPB), 0, s, rows, Lm, Rm, part);
+    else
+      hipLaunchKernelGGL((k_nn_outer<LES_FIN, NNF>), dim3(nrn), dim3(TPB), 0, s, rows, Lm, Rm, part);
+    hipLaunchKernelGGL(k_nn_sum_partials, dim3(grid1(fa * fb)), dim3(TPB), 0, s, nrn, fa * fb, part, gr + off);
+  };
+  hipLaunchKernelGGL(k_nn_back_out<1>, dim3(gnode), dim3(TPB), 0, s, rows, c.d_nn_dZ.as<float>(), th + o.wout,
+                     c.d_nn_dH.as<float>());
+  outer(H + (int64_t)NNL * need, c.d_nn_dZ.as<float>(), NNF, 1, o.wout);
   for (int l = NNL - 1; l >= 0; l--) {
-    hipLaunchKernelGGL(k_nn_back_layer, dim3(gnode), dim3(TPB), 0, c.stream, rows, c.d_nn_dH.as<float>(),
-                       P + (int64_t)l * rows * NNF, th + o.a[l], th + o.b[l], c.d_nn_G.as<float>(),
-                       c.d_nn_dH.as<float>() /* overwritten below */, c.d_nn_tmp.as<float>());
-    // dA_l = H_{l-1}^T G ; dB_l = M_l^T G
-    outer(H + (int64_t)l * rows * NNF, c.d_nn_G.as<float>(), NNF, NNF, o.a[l]);
-    outer(M + (int64_t)l * rows * NNF, c.d_nn_G.as<float>(), NNF, NNF, o.b[l]);
-    // dH_{l-1} = G A^T (already in d_nn_dH) + Ahat^T (G B^T)
-    hipLaunchKernelGGL(k_nn_agg_t, dim3(gnode), dim3(TPB), 0, c.stream, rows, nc, c.d_nn_cia.as<int64_t>(),
-                       c.d_nn_cja.as<int32_t>(), c.d_nn_tmp.as<float>(), c.d_nn_dH.as<float>());
+    hipLaunchKernelGGL(k_nn_back_layer, dim3(gnode), dim3(TPB), 0, s, rows, c.d_nn_dH.as<float>(),
+                       P + (int64_t)l * need, th + o.a[l], th + o.b[l], c.d_nn_G.as<float>(),
+                       c.d_nn_dH.as<float>(), c.d_nn_tmp.as<float>());
+    outer(H + (int64_t)l * need, c.d_nn_G.as<float>(), NNF, NNF, o.a[l]);
+    outer(M + (int64_t)l * need, c.d_nn_G.as<float>(), NNF, NNF, o.b[l]);
+    hipLaunchKernelGGL(k_nn_agg_t, dim3(gnode), dim3(TPB), 0, s, rows, rows, c.d_les_cia.as<int64_t>(),
+                       c.d_les_cja.as<int32_t>(), c.d_nn_tmp.as<float>(), c.d_nn_dH.as<float>());
   }
-  // dW0 = X^T dH_0
-  outer(X, c.d_nn_dH.as<float>(), bs, NNF, o.w0);
+  outer(c.d_les_X.as<float>(), c.d_nn_dH.as<float>(), LES_FIN, NNF, o.w0);
   PZG_HIP(hipGetLastError());
   if (loss_out) {
-    PZG_HIP(hipMemcpyAsync(loss_out, c.d_nn_loss.p, sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    PZG_HIP(hipStreamSynchronize(c.stream));
+    PZG_HIP(hipMemcpyAsync(loss_out, c.d_les_loss.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    PZG_HIP(hipStreamSynchronize(s));
+    *loss_out /= c.les_den;
   }
   return PZG_OK;
 }
 
-int nn_adam_step(Ctx &c, double lr) {
-  NNOff o = nn_offsets(c.bs);
-  c.nn_adam_t++;
+// Adam on the network weights (G is the least-squares fit)
+int les_adam_step(Ctx &c, double lr) {
+  LesOff o = les_offsets(c.les_ne);
+  c.les_adam_t++;
   const float b1 = 0.9f, b2 = 0.999f;
-  float c1 = 1.f - std::pow(b1, (float)c.nn_adam_t), c2 = 1.f - std::pow(b2, (float)c.nn_adam_t);
-  hipLaunchKernelGGL(k_adam, dim3(grid1(o.n)), dim3(TPB), 0, c.stream, o.n, c.d_nn_w.as<float>(),
-                     c.d_nn_g.as<float>(), c.d_nn_m.as<float>(), c.d_nn_v.as<float>(), (float)lr, b1,
-                     b2, c1, c2, 1e-8f);
+  const float c1 = 1.f - std::pow(b1, (float)c.les_adam_t), c2 = 1.f - std::pow(b2, (float)c.les_adam_t);
+  const int n = o.n - o.w0;
+  hipLaunchKernelGGL(k_adam, dim3(grid1(n)), dim3(TPB), 0, c.stream, n, c.d_les_th.as<float>() + o.w0,
+                     c.d_les_g.as<float>() + o.w0, c.d_les_m.as<float>() + o.w0,
+                     c.d_les_v.as<float>() + o.w0, (float)lr, b1, b2, c1, c2, 1e-8f);
   PZG_HIP(hipGetLastError());
-  return PZG_OK;
-}
-
-// restrict one (r, e) pair into dataset slot 'slot': X = R r, T = R(e - B^-1 r)
-int nn_store_pair(Ctx &c, const float *r, const float *e_minus_bjr, int64_t slot) {
-  const int64_t nc = c.nn_nc;
-  const int gr = grid1(nc * 64);
-  float *X = c.d_nn_X.as<float>() + slot * nc * c.bs, *T = c.d_nn_T.as<float>() + slot * nc * c.bs;
-  if (c.bs == 3) {
-    hipLaunchKernelGGL(k_nn_restrict<3>, dim3(gr), dim3(TPB), 0, c.stream, nc, c.d_nn_captr.as<int64_t>(),
-                       c.d_nn_calist.as<int32_t>(), r, (const float *)nullptr, X);
-    hipLaunchKernelGGL(k_nn_restrict<3>, dim3(gr), dim3(TPB), 0, c.stream, nc, c.d_nn_captr.as<int64_t>(),
-                       c.d_nn_calist.as<int32_t>(), e_minus_bjr, (const float *)nullptr, T);
-  } else {
-    hipLaunchKernelGGL(k_nn_restrict<1>, dim3(gr), dim3(TPB), 0, c.stream, nc, c.d_nn_captr.as<int64_t>(),
-                       c.d_nn_calist.as<int32_t>(), r, (const float *)nullptr, X);
-    hipLaunchKernelGGL(k_nn_restrict<1>, dim3(gr), dim3(TPB), 0, c.stream, nc, c.d_nn_captr.as<int64_t>(),
-                       c.d_nn_calist.as<int32_t>(), e_minus_bjr, (const float *)nullptr, T);
-  }
-  PZG_HIP(hipGetLastError());
+  c.mg_ready = false;  // the smoother changed: the cycle's eigenvalue estimate too
   return PZG_OK;
 }
 

@@ -150,6 +150,30 @@ __global__ void k_cheb_step(int64_t nb, const float *__restrict__ m, int64_t sa,
   }
 }
 
+// Learned smoother on level 0 (PZG_PRECOND_LEARNED): M r is an operator of
+// its own (les_apply), so a Chebyshev step splits into the residual update,
+// y = M r, and the direction / iterate update.
+// mode 0: r = b;  1: r -= t;  2: r = b - t
+__global__ void k_les_res(int64_t n, int mode, const float *__restrict__ b, const float *__restrict__ t,
+                          float *__restrict__ r, const SolveScalars *sc) {
+  if (stopped(sc)) return;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    r[i] = mode == 0 ? b[i] : mode == 1 ? r[i] - t[i] : b[i] - t[i];
+}
+// mode 0: d = c2 y, x = d;  1: d = c2 y, x += d;  2: d = c1 d + c2 y, x += d;
+// pt = D^-1/2 d (matrix-free operator input) when given
+__global__ void k_les_cheb(int64_t n, int mode, const float *__restrict__ y, float *__restrict__ d,
+                           float *__restrict__ x, float c1, float c2, const float *__restrict__ dis,
+                           float *__restrict__ pt, const SolveScalars *sc) {
+  if (stopped(sc)) return;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float dv = mode == 2 ? c1 * d[i] + c2 * y[i] : c2 * y[i];
+    d[i] = dv;
+    x[i] = mode == 0 ? dv : x[i] + dv;
+    if (pt) pt[i] = dv * dis[i];
+  }
+}
+
 // ---- lattice operators ---------------------------------------------------------
 struct Dims {
   int n[3];
@@ -479,13 +503,43 @@ struct LevelView {
   float *r, *d, *t;        // work vectors
   const float *dis = nullptr;  // level 0, matrix-free: D^-1/2 and the operator's
   float *pt = nullptr;         // pre-scaled input, written with each new d
+  float *y = nullptr;          // level 0, learned smoother: M r (else the blocks m)
   Cheb ch;
 };
 
 // op(x, y, prescaled): y = A x; prescaled: L.pt already holds D^-1/2 x.
 // Leaves x and, as r - t, the residual b - A x (the restriction subtracts).
+// the learned smoother's steps (LevelView::y set)
+template <int BS, class Op>
+int smooth_les(Ctx &c, const LevelView &L, const float *b, float *x, Op op, const SolveScalars *sc,
+               bool pre) {
+  const int64_t n = L.nb * BS;
+  const int g = grid1(n);
+  const bool ps = L.pt != nullptr;
+  if (pre) {
+    hipLaunchKernelGGL(k_les_res, dim3(g), dim3(TPB), 0, c.stream, n, 0, b, (const float *)nullptr, L.r, sc);
+  } else {
+    PZG_TRY(op(x, L.t, false));
+    hipLaunchKernelGGL(k_les_res, dim3(g), dim3(TPB), 0, c.stream, n, 2, b, L.t, L.r, sc);
+  }
+  PZG_TRY(les_apply(c, L.r, L.y, sc));
+  hipLaunchKernelGGL(k_les_cheb, dim3(g), dim3(TPB), 0, c.stream, n, pre ? 0 : 1, L.y, L.d, x, 0.f, L.ch.c0,
+                     L.dis, L.pt, sc);
+  PZG_HIP(hipGetLastError());
+  for (size_t j = 0; j < L.ch.c1.size(); j++) {
+    PZG_TRY(op(L.d, L.t, ps));
+    hipLaunchKernelGGL(k_les_res, dim3(g), dim3(TPB), 0, c.stream, n, 1, b, L.t, L.r, sc);
+    PZG_TRY(les_apply(c, L.r, L.y, sc));
+    hipLaunchKernelGGL(k_les_cheb, dim3(g), dim3(TPB), 0, c.stream, n, 2, L.y, L.d, x, L.ch.c1[j], L.ch.c2[j],
+                       L.dis, L.pt, sc);
+    PZG_HIP(hipGetLastError());
+  }
+  return pre ? op(L.d, L.t, ps) : PZG_OK;
+}
+
 template <int BS, class Op>
 int smooth_pre(Ctx &c, const LevelView &L, const float *b, float *x, Op op, const SolveScalars *sc) {
+  if (L.y) return smooth_les<BS>(c, L, b, x, op, sc, true);
   const int g = grid1(L.nb);
   const bool ps = L.pt != nullptr;
   hipLaunchKernelGGL(k_cheb_init<BS>, dim3(g), dim3(TPB), 0, c.stream, L.nb, L.m, L.sa, L.sk, b, x, L.r,
@@ -502,6 +556,7 @@ int smooth_pre(Ctx &c, const LevelView &L, const float *b, float *x, Op op, cons
 
 template <int BS, class Op>
 int smooth_post(Ctx &c, const LevelView &L, const float *b, float *x, Op op, const SolveScalars *sc) {
+  if (L.y) return smooth_les<BS>(c, L, b, x, op, sc, false);
   const int g = grid1(L.nb);
   const bool ps = L.pt != nullptr;
   PZG_TRY(op(x, L.t, false));
@@ -547,6 +602,7 @@ LevelView view0(Ctx &c) {
     v.dis = c.d_mf_dis.as<float>();
     v.pt = c.d_mf_pt.as<float>();
   }
+  if (c.mg_les) v.y = c.d_mg_y0.as<float>();
   v.ch = cheb_coeffs(c.mg_lmax0, c.mg_range, c.mg_degree);
   return v;
 }
@@ -657,28 +713,47 @@ __global__ void k_xpby(int64_t n, const float *__restrict__ z, const double *__r
 // coefficients give the Lanczos tridiagonal (diagonal 1/alpha_k +
 // beta_{k-1}/alpha_{k-1}, off-diagonal sqrt(beta_k)/alpha_k), whose largest
 // eigenvalue approximates lmax from below.  Work vectors: r, z, p, q.
+// r -= alpha q (alpha = *al)
+__global__ void k_lz_rsub(int64_t n, const double *__restrict__ al, const float *__restrict__ q,
+                          float *__restrict__ r) {
+  const float alpha = (float)*al;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    r[i] -= alpha * q[i];
+}
+
 template <int BS, class Op>
 int lanczos_lmax(Ctx &c, int64_t nb, const float *m, int64_t sa, int64_t sk, float *r, float *z, float *p,
-                 float *q, Op op, double &lmax) {
+                 float *q, Op op, double &lmax, bool les = false) {
   const int64_t n = nb * BS;
   double *part = c.d_partials.as<double>();
   double *lz = c.d_mg_lz.as<double>();
   const int gr = gridr(n), gb = gridr(nb);
+  // r -= alpha q; z = M r; partials of r.z; returns the number of partial blocks
+  auto mstep = [&](const double *al, const float *qq) -> int {
+    if (!les) {
+      hipLaunchKernelGGL(k_lz_update<BS>, dim3(gb), dim3(TPB), 0, c.stream, nb, m, sa, sk, al, qq, r, z, part);
+      return gb;
+    }
+    hipLaunchKernelGGL(k_lz_rsub, dim3(grid1(n)), dim3(TPB), 0, c.stream, n, al, qq, r);
+    if (les_apply(c, r, z, nullptr) != PZG_OK) return -1;
+    hipLaunchKernelGGL(k_dot, dim3(gr), dim3(TPB), 0, c.stream, n, r, z, part);
+    return gr;
+  };
   PZG_HIP(hipMemsetAsync(lz, 0, sizeof(double) * (LZ_ZERO + 1), c.stream));
   hipLaunchKernelGGL(k_hash_fill, dim3(grid1(n)), dim3(TPB), 0, c.stream, n, r, (uint64_t)0x5DEECE66DULL);
   // z = M^-1 r and r.z (the update with alpha = 0)
-  hipLaunchKernelGGL(k_lz_update<BS>, dim3(gb), dim3(TPB), 0, c.stream, nb, m, sa, sk, lz + LZ_ZERO, r, r, z,
-                     part);
-  hipLaunchKernelGGL(k_lz_scalar, dim3(1), dim3(TPB), 0, c.stream, (const double *)part, gb, lz, 0, 0);
+  int nblk = mstep(lz + LZ_ZERO, r);
+  if (nblk < 0) return PZG_EHIP;
+  hipLaunchKernelGGL(k_lz_scalar, dim3(1), dim3(TPB), 0, c.stream, (const double *)part, nblk, lz, 0, 0);
   PZG_HIP(hipGetLastError());
   PZG_HIP(hipMemcpyAsync(p, z, sizeof(float) * n, hipMemcpyDeviceToDevice, c.stream));
   for (int k = 0; k < MG_LANCZOS_IT; k++) {
     PZG_TRY(op(p, q));
     hipLaunchKernelGGL(k_dot, dim3(gr), dim3(TPB), 0, c.stream, n, p, q, part);
     hipLaunchKernelGGL(k_lz_scalar, dim3(1), dim3(TPB), 0, c.stream, (const double *)part, gr, lz, k, 1);
-    hipLaunchKernelGGL(k_lz_update<BS>, dim3(gb), dim3(TPB), 0, c.stream, nb, m, sa, sk, lz + 1 + 2 * k, q,
-                       r, z, part);
-    hipLaunchKernelGGL(k_lz_scalar, dim3(1), dim3(TPB), 0, c.stream, (const double *)part, gb, lz, k, 2);
+    nblk = mstep(lz + 1 + 2 * k, q);
+    if (nblk < 0) return PZG_EHIP;
+    hipLaunchKernelGGL(k_lz_scalar, dim3(1), dim3(TPB), 0, c.stream, (const double *)part, nblk, lz, k, 2);
     hipLaunchKernelGGL(k_xpby, dim3(grid1(n)), dim3(TPB), 0, c.stream, n, z, lz + 2 + 2 * k, p);
     PZG_HIP(hipGetLastError());
   }
@@ -891,10 +966,14 @@ int setup_impl(Ctx &c) {
   }
   // largest eigenvalues of M^-1 A: level 0, then every smoothed lattice level
   {
+    if (c.mg_les) {  // the learned smoother of the current assembly
+      PZG_TRY(les_prepare(c));
+      PZG_TRY(dev_alloc(c.d_mg_y0, sizeof(float) * std::max<int64_t>(c.neq, 1)));
+    }
     auto op = [&](const float *x, float *y) { return op0(c, x, y, nullptr, false); };
     PZG_TRY(lanczos_lmax<BS>(c, c.nb, c.d_bjinv.as<float>(), (int64_t)BS * BS, 1, c.d_mg_r0.as<float>(),
                              c.d_mg_d0.as<float>(), c.d_mg_t0.as<float>(), c.d_mg_w0.as<float>(), op,
-                             c.mg_lmax0));
+                             c.mg_lmax0, c.mg_les));
   }
   for (size_t k = 0; k + (c.mg_dense ? 1 : 0) < c.mg.size(); k++) {
     MgLevel &l = c.mg[k];
@@ -968,7 +1047,7 @@ void mg_free(Ctx &c) {
   c.mg.clear();
   c.mg_struct = false;
   for (DevBuf *b : {&c.d_mg_vmap, &c.d_mg_dsq, &c.d_mg_cinv, &c.d_mg_r0, &c.d_mg_d0, &c.d_mg_t0, &c.d_mg_w0,
-                    &c.d_mg_lz})
+                    &c.d_mg_lz, &c.d_mg_y0})
     dev_free(*b);
   c.mg_ready = false;
 }

@@ -288,6 +288,39 @@ __global__ __launch_bounds__(TPB) void k_cg_init64(int64_t n, const double *__re
   }
 }
 
+// Continuation from the current x with its residual r = f - K x already in
+// r (CG<> with fromCurrent, Solvers/LinearSolvers/cg.h:29-132): z = r/diag,
+// p = 0, rho = r.z, normb = |f|, resid = |r|/|f|.
+__global__ __launch_bounds__(TPB) void k_cg_restart64(int64_t n, const double *__restrict__ f,
+                                                      const double *__restrict__ r,
+                                                      double *__restrict__ z, double *__restrict__ p,
+                                                      const double *__restrict__ diag,
+                                                      SolveScalars *sc, double *partials,
+                                                      unsigned *counter) {
+  double ff = 0., rr = 0., rz = 0.;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) {
+    const double fi = f[i], ri = r[i], zi = ri / diag[i];
+    p[i] = 0.;
+    z[i] = zi;
+    ff += fi * fi;
+    rr += ri * ri;
+    rz += ri * zi;
+  }
+  double v[3] = {ff, rr, rz}, tot[3];
+  if (reduce_last<3>(v, partials, counter, tot) && threadIdx.x == 0) {
+    double normb = sqrt(tot[0]);
+    if (normb == 0.) normb = 1.;
+    const double resid = sqrt(tot[1]) / normb;
+    sc->normb = normb;
+    sc->resid = resid;
+    sc->rho = tot[2];
+    sc->beta = 0.;
+    sc->done = resid <= sc->tol ? 1 : 0;
+  }
+}
+
 // r = f - K u (FP64 block CSR), |r|^2 and |f|^2
 template <int BS>
 __global__ __launch_bounds__(TPB) void k_residual64(int64_t nb, const int64_t *__restrict__ bia,
@@ -374,6 +407,7 @@ static int ensure_reduce(Ctx &c) {
     PZG_HIP(hipMemsetAsync(c.d_counter.p, 0, sizeof(unsigned) * 16, c.stream));
   }
   PZG_TRY(dev_alloc(c.d_scal, sizeof(SolveScalars) * 2));
+  PZG_TRY(dev_alloc(c.d_res2, 2 * sizeof(double)));
   return PZG_OK;
 }
 int ensure_reduce_pub(Ctx &c) { return ensure_reduce(c); }
@@ -392,18 +426,15 @@ int launch_multadd_upper(Ctx &c, const double *x, const double *y, double *z, do
 int true_residual(Ctx &c, const double *u, double *rel) {
   if (!c.full_valid) PZG_TRY(launch_expand_full(c));
   PZG_TRY(ensure_reduce(c));
-  DevBuf out;
-  PZG_TRY(dev_alloc(out, 2 * sizeof(double)));
   DISPATCH_BS(c.bs, hipLaunchKernelGGL(k_residual64<BS>, dim3(bsr_grid(c.nb)), dim3(TPB), 0,
                                        c.stream, c.nb, c.d_fbia.as<int64_t>(),
                                        c.d_fbja.as<int32_t>(), c.d_fa.as<double>(), u,
-                                       c.d_f.as<double>(), nullptr, out.as<double>(),
+                                       c.d_f.as<double>(), nullptr, c.d_res2.as<double>(),
                                        c.d_partials.as<double>(), c.d_counter.as<unsigned>()));
   PZG_HIP(hipGetLastError());
   double h[2];
-  PZG_HIP(hipMemcpyAsync(h, out.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+  PZG_HIP(hipMemcpyAsync(h, c.d_res2.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
   PZG_HIP(hipStreamSynchronize(c.stream));
-  dev_free(out);
   *rel = h[1] > 0 ? std::sqrt(h[0] / h[1]) : std::sqrt(h[0]);
   return PZG_OK;
 }
@@ -430,6 +461,64 @@ static int run_batches(Ctx &c, int batch, bool use_graph, F enqueue_iter, SolveS
   }
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
+  return PZG_OK;
+}
+
+// FP64 Jacobi-PCG iterations (the EJacobi CG<> of solve_pcg64) continued from
+// the current solution u, whose true residual f - K u is in d_r: the mixed
+// solver's fallback when the FP32 iterations stall.  Runs until the
+// recursive residual reaches 0.9 tol, then checks the true residual; at most
+// four such restarts.  iters accumulates the iterations; rel receives the
+// true relative residual.
+static int pcg64_continue(Ctx &c, const pzg_solve_params &prm, int64_t &iters, double &rel) {
+  const int64_t n = c.neq;
+  for (DevBuf *b : {&c.d_z, &c.d_p, &c.d_q}) PZG_TRY(dev_alloc(*b, n * sizeof(double)));
+  SolveScalars *sc = c.d_scal.as<SolveScalars>();
+  double *part = c.d_partials.as<double>();
+  unsigned *cnt = c.d_counter.as<unsigned>();
+  const int gv = grid_for(n, TPB), gs = bsr_grid(c.nb);
+  for (int round = 0; round < 4 && rel > prm.tol && iters < prm.max_iter; round++) {
+    SolveScalars hs;
+    std::memset(&hs, 0, sizeof(hs));
+    hs.tol = 0.9 * prm.tol;
+    hs.maxit = prm.max_iter;
+    hs.iter = iters;
+    PZG_HIP(hipMemcpyAsync(c.d_scal.p, &hs, sizeof(hs), hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(k_cg_restart64, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_f.as<double>(),
+                       c.d_r.as<double>(), c.d_z.as<double>(), c.d_p.as<double>(),
+                       c.d_diag.as<double>(), sc, part, cnt);
+    PZG_HIP(hipGetLastError());
+    auto iter = [&]() {
+      hipLaunchKernelGGL(k_p_update64, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_z.as<double>(),
+                         c.d_p.as<double>(), sc);
+      DISPATCH_BS(c.bs, hipLaunchKernelGGL(k_spmv64_pq<BS>, dim3(gs), dim3(TPB), 0, c.stream, c.nb,
+                                           c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(),
+                                           c.d_fa.as<double>(), c.d_p.as<double>(),
+                                           c.d_q.as<double>(), sc, part));
+      hipLaunchKernelGGL(k_fin_alpha64, dim3(1), dim3(TPB), 0, c.stream, sc, (const double *)part, gs);
+      hipLaunchKernelGGL(k_cg_update64, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_u.as<double>(),
+                         c.d_r.as<double>(), c.d_z.as<double>(), c.d_p.as<double>(),
+                         c.d_q.as<double>(), c.d_diag.as<double>(), sc, (double *)nullptr, part,
+                         cnt);
+      hipLaunchKernelGGL(k_fin_cg64, dim3(1), dim3(TPB), 0, c.stream, sc, (const double *)part, gv,
+                         (double *)nullptr);
+    };
+    PZG_HIP(hipMemcpyAsync(&hs, c.d_scal.p, sizeof(hs), hipMemcpyDeviceToHost, c.stream));
+    PZG_HIP(hipStreamSynchronize(c.stream));
+    if (!hs.done) PZG_TRY(run_batches(c, prm.check_every > 0 ? prm.check_every : 50, prm.use_graph != 0, iter, hs));
+    iters = hs.iter;
+    // true residual into d_r (the next round restarts from it)
+    DISPATCH_BS(c.bs, hipLaunchKernelGGL(k_residual64<BS>, dim3(gs), dim3(TPB), 0, c.stream, c.nb,
+                                         c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(),
+                                         c.d_fa.as<double>(), c.d_u.as<double>(), c.d_f.as<double>(),
+                                         c.d_r.as<double>(), c.d_res2.as<double>(), part, cnt));
+    PZG_HIP(hipGetLastError());
+    double h[2];
+    PZG_HIP(hipMemcpyAsync(h, c.d_res2.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+    PZG_HIP(hipStreamSynchronize(c.stream));
+    rel = h[1] > 0 ? std::sqrt(h[0] / h[1]) : std::sqrt(h[0]);
+    if (hs.done == 2) break;
+  }
   return PZG_OK;
 }
 
@@ -695,51 +784,7 @@ __global__ __launch_bounds__(TPB) void k_fcg_reset(int64_t n, const double *__re
   }
 }
 
-// x = 0; r = b (fp32); z = Binv r; p = z; normb = |b|
-template <int BS>
-__global__ __launch_bounds__(TPB) void k_fcg_start(int64_t nb, const float *__restrict__ b,
-                                                   float *__restrict__ x, float *__restrict__ r,
-                                                   float *__restrict__ z, float *__restrict__ p,
-                                                   const float *__restrict__ binv,
-                                                   SolveScalars *sc, double *partials,
-                                                   unsigned *counter) {
-  double bb = 0.;
-  int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t st = (int64_t)gridDim.x * blockDim.x;
-  for (; A < nb; A += st) {
-    float rv[BS];
-    for (int k = 0; k < BS; k++) {
-      const int64_t i = A * BS + k;
-      rv[k] = b[i];
-      r[i] = rv[k];
-      x[i] = 0.f;
-      bb += (double)rv[k] * rv[k];
-    }
-    const float *bi = binv + A * BS * BS;
-    for (int k = 0; k < BS; k++) {
-      float s = 0.f;
-      for (int l = 0; l < BS; l++) s += bi[k * BS + l] * rv[l];
-      z[A * BS + k] = s;
-      p[A * BS + k] = s;
-    }
-  }
-  double vv[1] = {bb}, tot[1];
-  if (reduce_last<1>(vv, partials, counter, tot) && threadIdx.x == 0) {
-    double nbv = sqrt(tot[0]);
-    sc->normb = nbv > 0. ? nbv : 1.;
-    sc->rlast = nbv;
-    sc->resid = 1.;
-    sc->done = 0;
-  }
-}
 
-// out = a - b - c
-__global__ void k_sub3(int64_t n, const float *__restrict__ a, const float *__restrict__ b,
-                       const float *__restrict__ c, float *__restrict__ out) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t st = (int64_t)gridDim.x * blockDim.x;
-  for (; i < n; i += st) out[i] = a[i] - b[i] - c[i];
-}
 
 int launch_spmv_bsr(Ctx &c, const float *x, float *y) {
   DISPATCH_BS(c.bs, hipLaunchKernelGGL((k_spmv_bsr<float, BS>), dim3(bsr_grid(c.nb)), dim3(TPB), 0,
@@ -772,11 +817,12 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
   if (use_mf) PZG_TRY(mf_prepare(c));
   else if (!c.bsr_valid) PZG_TRY(launch_build_bsr(c, true));
   PZG_TRY(ensure_reduce(c));
-  const bool multigrid = prm.precond == PZG_PRECOND_MULTIGRID;
+  // the learned preconditioner is the multigrid cycle with the learned
+  // element smoother on the p level
+  const bool multigrid = prm.precond == PZG_PRECOND_MULTIGRID || prm.precond == PZG_PRECOND_LEARNED;
   for (DevBuf *b : {&c.d_u, &c.d_r}) PZG_TRY(dev_alloc(*b, n * sizeof(double)));
   for (DevBuf *b : {&c.d_xs, &c.d_rs, &c.d_zs, &c.d_ps, &c.d_qs}) PZG_TRY(dev_alloc(*b, n * sizeof(float)));
-  DevBuf out;
-  PZG_TRY(dev_alloc(out, 2 * sizeof(double)));
+  double *out = c.d_res2.as<double>();
   SolveScalars *sc = c.d_scal.as<SolveScalars>();
   double *part = c.d_partials.as<double>();
   unsigned *cnt = c.d_counter.as<unsigned>();
@@ -793,14 +839,14 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
   auto true_res = [&](bool u_zero = false) -> int {
     if (u_zero)  // the starting guess: no product needed
       hipLaunchKernelGGL(k_residual_zero, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_f.as<double>(),
-                         c.d_r.as<double>(), out.as<double>(), part, cnt);
+                         c.d_r.as<double>(), out, part, cnt);
     else
       hipLaunchKernelGGL(k_residual64<BS>, dim3(bsr_grid(nb)), dim3(TPB), 0, c.stream, nb,
                          c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(), c.d_fa.as<double>(),
-                         c.d_u.as<double>(), c.d_f.as<double>(), c.d_r.as<double>(), out.as<double>(),
+                         c.d_u.as<double>(), c.d_f.as<double>(), c.d_r.as<double>(), out,
                          part, cnt);
     double h[2];
-    PZG_HIP(hipMemcpyAsync(h, out.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+    PZG_HIP(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, c.stream));
     PZG_HIP(hipStreamSynchronize(c.stream));
     rel = h[1] > 0 ? std::sqrt(h[0] / h[1]) : std::sqrt(h[0]);
     return PZG_OK;
@@ -818,11 +864,15 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
     // levels and eigenvalue estimates (timed with the solve)
     c.mg_degree = prm.mg_degree > 0 ? prm.mg_degree : 2;
     c.mg_range = prm.mg_range > 0 ? prm.mg_range : 30.;
-    if (!c.mg_ready || c.mg_use_mf != use_mf) PZG_TRY(mg_setup(c, use_mf));
+    const bool les = prm.precond == PZG_PRECOND_LEARNED;
+    if (!c.mg_ready || c.mg_use_mf != use_mf || c.mg_les != les) {
+      c.mg_les = les;
+      PZG_TRY(mg_setup(c, use_mf));
+    }
   }
   bool restart = true;
   bool learned = prm.precond == PZG_PRECOND_LEARNED;
-  int fallback = 0;
+  int fallback = 0, stall = 0;
   double prev_rel = rel;
   SolveScalars hs;
   std::memset(&hs, 0, sizeof(hs));
@@ -838,9 +888,6 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
                          c.d_zs.as<float>(), c.d_ps.as<float>(), c.d_bjinv.as<float>(), sc, part,
                          cnt);
       PZG_HIP(hipGetLastError());
-      if (learned)
-        PZG_TRY(nn_apply(c, c.d_rs.as<float>(), c.d_zs.as<float>(), c.d_ps.as<float>(), nullptr, sc,
-                         part, cnt));
       if (multigrid)
         PZG_TRY(mg_apply(c, c.d_rs.as<float>(), c.d_zs.as<float>(), c.d_ps.as<float>(), nullptr, sc, part));
       if (use_mf) PZG_TRY(mf_scale(c, c.d_ps.as<float>(), sc));
@@ -861,9 +908,6 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
                          c.d_xs.as<float>(), c.d_rs.as<float>(), c.d_zs.as<float>(),
                          c.d_ps.as<float>(), c.d_qs.as<float>(), c.d_bjinv.as<float>(), sc, part,
                          cnt);
-      if (learned)
-        (void)nn_apply(c, c.d_rs.as<float>(), c.d_zs.as<float>(), nullptr, c.d_qs.as<float>(), sc,
-                       part, cnt);
       if (multigrid)
         (void)mg_apply(c, c.d_rs.as<float>(), c.d_zs.as<float>(), nullptr, c.d_qs.as<float>(), sc, part);
       hipLaunchKernelGGL(k_fin_fcg_beta, dim3(1), dim3(TPB), 0, c.stream, sc, (const double *)part, gb);
@@ -880,15 +924,19 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
     PZG_TRY(true_res());
     outer++;
     if (rel <= prm.tol) { conv = true; break; }
-    if (hs.done == 2 || outer >= prm.max_outer) break;
-    // stall of the learned preconditioner: fall back to block Jacobi and
-    // restart from the true residual
-    if (learned && (hs.done == 3 || rel > 0.9 * prev_rel)) {
-      learned = false;
+    if (hs.done == 2) break;
+    // Stall of the FP32 iterations (the true residual no longer halves over
+    // three refinement steps, or the refinement budget is spent; with the
+    // learned preconditioner already when one step gains less than 10%):
+    // continue with plain FP64 Jacobi-PCG from the current solution, judged
+    // by the true FP64 residual.
+    stall = rel > 0.5 * prev_rel ? stall + 1 : 0;
+    if (stall >= 3 || outer >= prm.max_outer ||
+        (learned && rel > 0.9 * prev_rel)) {
       fallback = 1;
-      restart = true;
-      prev_rel = rel;
-      continue;
+      PZG_TRY(pcg64_continue(c, prm, total_it, rel));
+      conv = rel <= prm.tol;
+      break;
     }
     prev_rel = rel;
     if (hs.done == 3 || prm.refine == 0) { restart = true; continue; }
@@ -900,7 +948,6 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
   PZG_HIP(hipEventSynchronize(c.ev1));
   float ms = 0;
   PZG_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
-  dev_free(out);
   info.solve_ms = ms;
   info.iterations = total_it;
   info.outer = outer;
@@ -911,89 +958,8 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
   return conv ? PZG_OK : PZG_ENOCONV;
 }
 
-// (residual, error) pairs from the solver itself: random scaled solutions y*,
-// b = S y*, block-Jacobi FCG from zero; at the iterations in rec the pair
-// (R r_k, R(y* - y_k - B^-1 r_k)) is stored.
-template <int BS>
-static int collect_impl(Ctx &c, int nrhs, const int32_t *rec, int nrec, uint64_t seed) {
-  const int64_t n = c.neq, nb = c.nb;
-  if (!c.full_valid) PZG_TRY(launch_expand_full(c));
-  if (!c.bsr_valid) PZG_TRY(launch_build_bsr(c));
-  PZG_TRY(ensure_reduce(c));
-  for (DevBuf *b : {&c.d_xs, &c.d_rs, &c.d_zs, &c.d_ps, &c.d_qs, &c.d_bs, &c.d_nn_ystar, &c.d_nn_e})
-    PZG_TRY(dev_alloc(*b, n * sizeof(float)));
-  int kmax = 0;
-  for (int k = 0; k < nrec; k++) kmax = rec[k] > kmax ? rec[k] : kmax;
-  std::vector<float> ys(n);
-  uint64_t st = seed * 6364136223846793005ULL + 1442695040888963407ULL;
-  SolveScalars *sc = c.d_scal.as<SolveScalars>();
-  double *part = c.d_partials.as<double>();
-  unsigned *cnt = c.d_counter.as<unsigned>();
-  const int gv = grid_for(n, TPB), gb = grid_for(nb, TPB), gs = bsr_grid(nb);
-  for (int j = 0; j < nrhs; j++) {
-    for (int64_t i = 0; i < n; i++) {  // Box-Muller on a 64-bit LCG
-      st = st * 6364136223846793005ULL + 1442695040888963407ULL;
-      double u1 = ((st >> 11) + 1.0) / 9007199254740993.0;
-      st = st * 6364136223846793005ULL + 1442695040888963407ULL;
-      double u2 = (st >> 11) / 9007199254740992.0;
-      ys[i] = (float)(std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2));
-    }
-    PZG_HIP(hipMemcpyAsync(c.d_nn_ystar.p, ys.data(), n * sizeof(float), hipMemcpyHostToDevice, c.stream));
-    PZG_TRY(launch_spmv_bsr(c, c.d_nn_ystar.as<float>(), c.d_bs.as<float>()));
-    SolveScalars hs;
-    std::memset(&hs, 0, sizeof(hs));
-    hs.maxit = (int64_t)1 << 40;
-    hs.delta = 0.;
-    PZG_HIP(hipMemcpyAsync(c.d_scal.p, &hs, sizeof(hs), hipMemcpyHostToDevice, c.stream));
-    hipLaunchKernelGGL(k_fcg_start<BS>, dim3(gb), dim3(TPB), 0, c.stream, nb, c.d_bs.as<float>(),
-                       c.d_xs.as<float>(), c.d_rs.as<float>(), c.d_zs.as<float>(), c.d_ps.as<float>(),
-                       c.d_bjinv.as<float>(), sc, part, cnt);
-    for (int k = 1; k <= kmax; k++) {
-      hipLaunchKernelGGL(k_fcg_spmv<BS>, dim3(gs), dim3(TPB), 0, c.stream, nb, c.d_fbia.as<int64_t>(),
-                         c.d_fbja.as<int32_t>(), c.d_bsr.as<float>(), c.d_ps.as<float>(),
-                         c.d_rs.as<float>(), c.d_qs.as<float>(), sc, part);
-      hipLaunchKernelGGL(k_fin_fcg_alpha, dim3(1), dim3(TPB), 0, c.stream, sc, (const double *)part, gs);
-      hipLaunchKernelGGL(k_fcg_update<BS>, dim3(gb), dim3(TPB), 0, c.stream, nb, c.d_xs.as<float>(),
-                         c.d_rs.as<float>(), c.d_zs.as<float>(), c.d_ps.as<float>(), c.d_qs.as<float>(),
-                         c.d_bjinv.as<float>(), sc, part, cnt);
-      hipLaunchKernelGGL(k_fin_fcg_beta, dim3(1), dim3(TPB), 0, c.stream, sc, (const double *)part, gb);
-      hipLaunchKernelGGL(k_p_update32, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_zs.as<float>(),
-                         c.d_ps.as<float>(), sc);
-      for (int t = 0; t < nrec; t++)
-        if (rec[t] == k) {
-          if (c.nn_nsamp >= c.nn_maxsamp) {
-            set_error("learned preconditioner: sample capacity exceeded");
-            return PZG_EINVAL;
-          }
-          hipLaunchKernelGGL(k_sub3, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_nn_ystar.as<float>(),
-                             c.d_xs.as<float>(), c.d_zs.as<float>(), c.d_nn_e.as<float>());
-          PZG_TRY(nn_store_pair(c, c.d_rs.as<float>(), c.d_nn_e.as<float>(), c.nn_nsamp));
-          c.nn_nsamp++;
-        }
-    }
-    PZG_HIP(hipGetLastError());
-    PZG_HIP(hipStreamSynchronize(c.stream));
-    SolveScalars chk;
-    PZG_HIP(hipMemcpy(&chk, c.d_scal.p, sizeof(chk), hipMemcpyDeviceToHost));
-    if (chk.done == 3) {
-      set_error("learned preconditioner: pair generation diverged");
-      return PZG_EINVAL;
-    }
-  }
-  return PZG_OK;
-}
-
-int nn_collect(Ctx &c, int nrhs, const int32_t *rec, int nrec, uint64_t seed) {
-  if (!c.nn_ready) {
-    set_error("learned preconditioner: call pzg_learned_setup first");
-    return PZG_ESTATE;
-  }
-  if (c.bs == 3) return collect_impl<3>(c, nrhs, rec, nrec, seed);
-  return collect_impl<1>(c, nrhs, rec, nrec, seed);
-}
-
 int solve_mixed(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info) {
-  if (prm.precond == PZG_PRECOND_LEARNED && !c.nn_ready) {
+  if (prm.precond == PZG_PRECOND_LEARNED && !c.les_ready) {
     set_error("pzg_solve: learned preconditioner not set up in this context");
     return PZG_ESTATE;
   }

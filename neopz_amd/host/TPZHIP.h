@@ -7,6 +7,7 @@
 //   TPZSSpStructMatrix (StrMatrix/TPZSSpStructMatrix.h:17-38),
 //   TPZSpStructMatrix  (StrMatrix/TPZSpStructMatrix.h),
 //   TPZSYsmpMatrix     (Matrix/pzsysmp.h:20-230),
+//   TPZEquationFilter  (StrMatrix/TPZEquationFilter.h),
 //   TPZFYsmpMatrix     (Matrix/pzysmp.h),
 //   TPZStepSolver      (Solvers/pzstepsolver.h:34-89),
 //   TPZLinearAnalysis  (Analysis/TPZLinearAnalysis.h:20-45)
@@ -44,10 +45,22 @@ inline Ctx MakeContext(int device = 0) {
   return Ctx(c, pzg_destroy);
 }
 
+// Solver that TPZSYsmpMatrixHIP::SolveCG runs for a TPZStepSolver set up with
+// SetCG(n, EJacobi step, tol, 0): the reference's FP64 Jacobi-PCG (default),
+// or, when the struct matrix was asked for it (SetMixedPrecision), the FP32
+// flexible CG inside FP64 refinement with the given preconditioner.  Either
+// way the iteration count and the achieved tolerance are written back.
+struct SolveMode {
+  int solver = PZG_SOLVER_PCG64;
+  int precond = PZG_PRECOND_JACOBI;
+  double inner_tol = 1e-4;
+};
+
 // TPZSYsmpMatrix: symmetric CSR, upper triangle with the diagonal first.
 class TPZSYsmpMatrixHIP {
  public:
-  TPZSYsmpMatrixHIP(Ctx ctx, int64_t neq) : fCtx(std::move(ctx)), fNeq(neq) {}
+  TPZSYsmpMatrixHIP(Ctx ctx, int64_t neq, SolveMode mode = SolveMode())
+      : fCtx(std::move(ctx)), fNeq(neq), fMode(mode) {}
   virtual ~TPZSYsmpMatrixHIP() = default;
   int64_t Rows() const { return fNeq; }
   // TPZSYsmpMatrix::SetData (Matrix/pzsysmp.h:232-240)
@@ -92,6 +105,31 @@ class TPZSYsmpMatrixHIP {
           "MultAdd");
   }
   pzg_ctx *Context() const { return fCtx.get(); }
+  const SolveMode &Mode() const { return fMode; }
+  // TPZMatrix::SolveCG (Matrix/pzmatrix.cpp:750-765, Matrix/pzmatrix.h:460-463)
+  // as TPZStepSolver::Solve calls it for ECG with an EJacobi preconditioner
+  // (Solvers/pzstepsolver.cpp:107-112): F and result in the matrix numbering;
+  // numiterations and tol receive the iterations and the achieved residual.
+  int SolveCG(int64_t &numiterations, const std::vector<double> *F, std::vector<double> &result,
+              double &tol, pzg_solve_info &info) {
+    pzg_solve_params p{};
+    p.solver = fMode.solver;
+    p.precond = fMode.solver == PZG_SOLVER_PCG64 ? PZG_PRECOND_JACOBI : fMode.precond;
+    p.max_iter = numiterations;
+    p.tol = tol;
+    p.inner_tol = fMode.inner_tol;
+    p.max_outer = 200;
+    p.check_every = p.precond == PZG_PRECOND_MULTIGRID ? 0 : 50;  // 0: the solver's batch
+    p.use_graph = 1;
+    p.refine = 0;
+    p.inner_op = PZG_INNER_AUTO;
+    result.resize(fNeq);
+    int rc = pzg_solve(fCtx.get(), &p, F ? F->data() : nullptr, result.data(), &info);
+    if (rc != PZG_OK && rc != PZG_ENOCONV) Check(rc, "SolveCG");
+    numiterations = info.iterations;
+    tol = info.achieved;
+    return rc;
+  }
 
  protected:
   virtual void Pattern() {
@@ -101,10 +139,12 @@ class TPZSYsmpMatrixHIP {
     fIA.resize(fNeq + 1);
     fJA.resize(s.nnz_upper);
     Check(pzg_struct_get_pattern(fCtx.get(), fIA.data(), fJA.data()), "IA/JA");
+    fJA.resize(fIA[fNeq]);  // condensed (equation filter) patterns are shorter
   }
   Ctx fCtx;
   int64_t fNeq;
   std::vector<int64_t> fIA, fJA;
+  SolveMode fMode;
 };
 
 // TPZFYsmpMatrix: non-symmetric sparse rows (both triangles, sorted columns)
@@ -163,6 +203,14 @@ class TPZStepSolver {
   // iteration count and overwrites the tolerance with the achieved one.
   void Solve(const std::vector<double> *F, std::vector<double> &result) {
     if (!fMatrix || (fSolver != ECG && fSolver != EMixedFCG)) DebugStop();
+    if (fSolver == ECG) {  // mat->SolveCG(...), the matrix picks the algorithm
+      int64_t its = fMaxIterations;
+      double tol = fTol;
+      fMatrix->SolveCG(its, F, result, tol, fInfo);
+      fNumIterations = its;
+      fTol = tol;
+      return;
+    }
     pzg_solve_params p{};
     p.solver = fSolver == ECG ? PZG_SOLVER_PCG64 : PZG_SOLVER_MIXED;
     p.precond = fPrecond;
@@ -193,25 +241,86 @@ class TPZStepSolver {
   pzg_solve_info fInfo{};
 };
 
+// TPZEquationFilter (StrMatrix/TPZEquationFilter.h): the active equations of
+// the mesh numbering, sorted without duplicates; Scatter / Gather between the
+// condensed and the mesh numbering.
+class TPZEquationFilter {
+ public:
+  explicit TPZEquationFilter(int64_t numeq = 0) : fNumEq(numeq) {}
+  void SetNumEq(int64_t n) { fNumEq = n; }
+  void SetActiveEquations(std::vector<int64_t> active) {
+    if (fIsActive) DebugStop();  // call Reset first
+    std::sort(active.begin(), active.end());
+    active.erase(std::unique(active.begin(), active.end()), active.end());
+    if (!active.empty() && (active.front() < 0 || active.back() >= fNumEq)) DebugStop();
+    fActiveEqs = std::move(active);
+    fIsActive = true;
+  }
+  void SetMinMaxEq(int64_t mineq, int64_t maxeq) {
+    if (mineq < 0 || maxeq > fNumEq || mineq > maxeq) DebugStop();
+    std::vector<int64_t> a(maxeq - mineq);
+    for (int64_t i = 0; i < maxeq - mineq; i++) a[i] = mineq + i;
+    SetActiveEquations(std::move(a));
+  }
+  void Reset() {
+    fIsActive = false;
+    fActiveEqs.clear();
+  }
+  bool IsActive() const { return fIsActive; }
+  int64_t NActiveEquations() const { return fIsActive ? (int64_t)fActiveEqs.size() : fNumEq; }
+  int64_t NEqExpand() const { return fNumEq; }
+  const std::vector<int64_t> &ActiveEquations() const { return fActiveEqs; }
+  void Scatter(const std::vector<double> &small, std::vector<double> &big) const {
+    if (!fIsActive) { big = small; return; }
+    big.assign(fNumEq, 0.);
+    for (size_t i = 0; i < fActiveEqs.size(); i++) big[fActiveEqs[i]] = small[i];
+  }
+  void Gather(const std::vector<double> &big, std::vector<double> &small) const {
+    if (!fIsActive) { small = big; return; }
+    small.resize(fActiveEqs.size());
+    for (size_t i = 0; i < fActiveEqs.size(); i++) small[i] = big[fActiveEqs[i]];
+  }
+
+ private:
+  int64_t fNumEq = 0;
+  bool fIsActive = false;
+  std::vector<int64_t> fActiveEqs;
+};
+
 // TPZSSpStructMatrix over the GPU: Create() -> pattern, Assemble() -> values + rhs.
 class TPZSSpStructMatrixHIP {
  public:
-  TPZSSpStructMatrixHIP(Ctx ctx, const pzg_mesh_desc &mesh) : fCtx(std::move(ctx)), fMesh(mesh) {}
+  TPZSSpStructMatrixHIP(Ctx ctx, const pzg_mesh_desc &mesh)
+      : fCtx(std::move(ctx)), fMesh(mesh), fFilter(mesh.neq) {}
   virtual ~TPZSSpStructMatrixHIP() = default;
+  // TPZStructMatrix::EquationFilter (StrMatrix/TPZStructMatrix.h)
+  TPZEquationFilter &EquationFilter() { return fFilter; }
+  int64_t NReducedEquations() const { return fFilter.NActiveEquations(); }
+  // the matrices this struct matrix creates solve SetCG step solvers with the
+  // mixed-precision FCG (FP32 iterations with the given preconditioner, FP64
+  // refinement to the FP64 true residual) instead of FP64 Jacobi-PCG
+  void SetMixedPrecision(int precond, double inner_tol = 1e-4) {
+    fMode.solver = PZG_SOLVER_MIXED;
+    fMode.precond = precond;
+    fMode.inner_tol = inner_tol;
+  }
   // TPZSSpStructMatrix::Create (StrMatrix/TPZSSpStructMatrix.cpp:30-47)
   virtual std::unique_ptr<TPZSYsmpMatrixHIP> Create() {
-    return std::make_unique<TPZSYsmpMatrixHIP>(fCtx, Setup());
+    return std::make_unique<TPZSYsmpMatrixHIP>(fCtx, Setup(), fMode);
   }
   // the analysis' mesh (after a renumbering) replaces the one given here
   void SetMesh(const pzg_mesh_desc &mesh) {
     fMesh = mesh;
     fFlattened = false;
   }
-  // TPZStructMatrixOR::Assemble(K, rhs) (StrMatrix/pzstrmatrixor.cpp:41-71)
+  // TPZStructMatrixOR::Assemble(K, rhs) (StrMatrix/pzstrmatrixor.cpp:41-71):
+  // rhs in the mesh numbering (with a filter the condensed load scattered,
+  // inactive equations 0); the values stay on the device
   void Assemble(TPZSYsmpMatrixHIP & /*K*/, std::vector<double> &rhs) {
     Check(pzg_assemble(fCtx.get()), "Assemble");
-    rhs.resize(fMesh.neq);
-    Check(pzg_get_system(fCtx.get(), nullptr, rhs.data()), "rhs");
+    std::vector<double> small(fFilter.NActiveEquations());
+    Check(pzg_get_system(fCtx.get(), nullptr, small.data()), "rhs");
+    fFilter.Scatter(small, rhs);
   }
 
  protected:
@@ -220,6 +329,10 @@ class TPZSSpStructMatrixHIP {
       Check(pzg_mesh_set(fCtx.get(), &fMesh), "mesh");
       fFlattened = true;
     }
+    const auto &act = fFilter.ActiveEquations();
+    Check(pzg_set_equation_filter(fCtx.get(), fFilter.IsActive() ? (int64_t)act.size() : -1,
+                                  fFilter.IsActive() ? act.data() : nullptr),
+          "EquationFilter");
     int64_t neq = 0, nnz = 0;
     Check(pzg_struct_create(fCtx.get(), &neq, &nnz), "Create");
     return neq;
@@ -227,6 +340,8 @@ class TPZSSpStructMatrixHIP {
   Ctx fCtx;
   pzg_mesh_desc fMesh;
   bool fFlattened = false;
+  TPZEquationFilter fFilter;
+  SolveMode fMode;
 };
 
 // TPZSpStructMatrix over the GPU: the same assembly, the matrix in the
@@ -235,7 +350,7 @@ class TPZSpStructMatrixHIP : public TPZSSpStructMatrixHIP {
  public:
   using TPZSSpStructMatrixHIP::TPZSSpStructMatrixHIP;
   std::unique_ptr<TPZSYsmpMatrixHIP> Create() override {
-    return std::make_unique<TPZFYsmpMatrixHIP>(fCtx, Setup());
+    return std::make_unique<TPZFYsmpMatrixHIP>(fCtx, Setup(), fMode);
   }
 };
 
@@ -306,8 +421,20 @@ class TPZLinearAnalysis {
     fStr->Assemble(*fK, fRhs);
     fSolver.SetMatrix(fK.get());
   }
-  // Analysis/TPZLinearAnalysis.cpp:130-227
-  void Solve() { fSolver.Solve(&fRhs, fSolution); }
+  // Analysis/TPZLinearAnalysis.cpp:130-227: with an equation filter the rhs
+  // is gathered to the condensed numbering and the solution scattered back
+  void Solve() {
+    const TPZEquationFilter &f = fStr->EquationFilter();
+    if (!f.IsActive()) {
+      fSolver.Solve(&fRhs, fSolution);
+      return;
+    }
+    std::vector<double> r, delu;
+    f.Gather(fRhs, r);
+    fSolver.Solve(&r, delu);
+    f.Scatter(delu, fSolution);
+  }
+  TPZSSpStructMatrixHIP &StructMatrix() { return *fStr; }
   const std::vector<double> &Solution() const { return fSolution; }
   const std::vector<double> &Rhs() const { return fRhs; }
   TPZSYsmpMatrixHIP &Matrix() { return *fK; }

@@ -14,13 +14,18 @@
 // (int64/double arrays, each with its count), then iterations and the
 // achieved tolerance.
 //
-// usage: pzhip_run MESH OUT [cg|mixed|mixed-mg] [tol] [maxit] [flags...]
+// usage: pzhip_run MESH OUT [cg|cg-mixed|cg-mixed-mg|mixed|mixed-mg] [tol] [maxit] [flags...]
+// cg-mixed / cg-mixed-mg: the same SetCG(maxit, jac, tol, 0) step solver, the
+// struct matrix set to the mixed-precision FCG (block Jacobi / multigrid)
+// before Assemble, so that TPZStepSolver::Solve -> SolveCG runs it.
 // flags: sym (default) or full: TPZSpStructMatrix, IA/JA/A of the
 // non-symmetric TPZFYsmpMatrix; sloan: TPZLinearAnalysis(cmesh, true), the
 // connects renumbered by TPZSloanRenumbering first (the permutation is
 // appended to OUT); errors: then an.SetExact(f, 6) with the host function of
 // exact_solution below and an.PostProcessError(ervec, true) (ervec and the
-// element errors are appended to OUT).
+// element errors are appended to OUT); minmax:A:B: the struct matrix's
+// equation filter SetMinMaxEq(A, B) (IA/JA/A/solution then condensed, the
+// rhs in the mesh numbering).
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -93,7 +98,8 @@ void exact_solution(int dim, int nstate, const std::vector<double> &x, std::vect
 
 int main(int argc, char **argv) {
   if (argc < 3) {
-    std::fprintf(stderr, "usage: %s MESH OUT [cg|mixed|mixed-mg] [tol] [maxit] [sym|full] [sloan] [errors]\n",
+    std::fprintf(stderr, "usage: %s MESH OUT [cg|cg-mixed|cg-mixed-mg|mixed|mixed-mg] [tol] [maxit] "
+                         "[sym|full] [sloan] [errors] [minmax:A:B]\n",
                  argv[0]);
     return 2;
   }
@@ -102,11 +108,18 @@ int main(int argc, char **argv) {
   const int64_t maxit = argc > 5 ? std::atoll(argv[5]) : 100000;
   std::string format = "sym";
   bool errors = false, sloan = false;
+  int64_t fmin = -1, fmax = -1;
   for (int k = 6; k < argc; k++) {
     const std::string f = argv[k];
     if (f == "sym" || f == "full") format = f;
     else if (f == "errors") errors = true;
     else if (f == "sloan") sloan = true;
+    else if (f.rfind("minmax:", 0) == 0) {
+      const size_t c = f.find(':', 7);
+      if (c == std::string::npos) return 2;
+      fmin = std::atoll(f.substr(7, c - 7).c_str());
+      fmax = std::atoll(f.substr(c + 1).c_str());
+    }
     else {
       std::fprintf(stderr, "pzhip_run: unknown flag %s\n", f.c_str());
       return 2;
@@ -200,9 +213,15 @@ int main(int argc, char **argv) {
   using namespace pzhip;
   Ctx ctx = MakeContext(0);
   TPZLinearAnalysis an(ctx, m, /*mustOptimizeBandwidth=*/sloan);
-  if (format == "full") an.SetStructuralMatrix(std::make_unique<TPZSpStructMatrixHIP>(ctx, m));
+  std::unique_ptr<TPZSSpStructMatrixHIP> strmat =
+      format == "full" ? std::make_unique<TPZSpStructMatrixHIP>(ctx, m)
+                       : std::make_unique<TPZSSpStructMatrixHIP>(ctx, m);
+  if (fmin >= 0) strmat->EquationFilter().SetMinMaxEq(fmin, fmax);
+  if (method == "cg-mixed") strmat->SetMixedPrecision(PZG_PRECOND_BLOCK_JACOBI);
+  if (method == "cg-mixed-mg") strmat->SetMixedPrecision(PZG_PRECOND_MULTIGRID);
+  an.SetStructuralMatrix(std::move(strmat));
   TPZStepSolver step;
-  if (method == "cg") {
+  if (method == "cg" || method == "cg-mixed" || method == "cg-mixed-mg") {
     TPZStepSolver jac;
     jac.SetJacobi(1, 0., 0);
     step.SetCG(maxit, jac, tol, 0);

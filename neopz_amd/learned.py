@@ -1,11 +1,14 @@
-"""Learned coarse-correction preconditioner of the mixed solver (host driver).
+"""Learned element smoother of the mixed solver's multigrid cycle (host
+driver over the C ABI).
 
-The network, its training kernels and its use inside FCG live in
-``neopz_amd/csrc/pzg_learned.hip``; this module drives the C ABI:
-aggregation set-up, pair generation on seeded problem instances, training,
-and weight transfer.  NeoPZ has no learned component; the preconditioner is
-an option of the mixed solver (``PZG_PRECOND_LEARNED``) whose fallback is
-block Jacobi.
+The operator, the network and its training kernels live in
+``neopz_amd/csrc/pzg_learned.hip``: the smoothing of the p level uses
+``z = sum_e s_e P_e^T G P_e r`` with one learned symmetric element matrix G
+and per-element scales s_e from a message-passing network on the element
+graph.  Training pairs are the (residual, error) pairs of every element's
+local problem in seeded training systems.  NeoPZ has no learned component;
+the preconditioner is an option of the mixed solver (``PZG_PRECOND_LEARNED``)
+whose fallback is plain FP64 Jacobi-PCG.
 """
 import ctypes as C
 
@@ -13,20 +16,23 @@ import numpy as np
 
 from . import _lib as L
 
+NFEAT = 4    # element features
+NHID = 32    # hidden features
+NLAYER = 3   # message-passing layers
 
-class LearnedPreconditioner:
+
+class LearnedSmoother:
     def __init__(self, ctx):
         self.ctx = ctx
         self.lib = ctx.lib
 
-    def setup(self, coarsen=4, seed=0, max_samples=512):
-        L.check(self.lib.pzg_learned_setup(self.ctx.h, coarsen, seed, max_samples))
+    def setup(self, seed=0):
+        L.check(self.lib.pzg_learned_setup(self.ctx.h, seed))
 
-    def collect(self, nrhs=2, record=(2, 5, 10, 20, 40, 80), seed=1):
-        rec = np.ascontiguousarray(record, dtype=np.int32)
-        L.check(self.lib.pzg_learned_collect(self.ctx.h, nrhs, L.ptr(rec, L.i32p), len(rec), seed))
+    def collect(self, nrhs=4, seed=1):
+        L.check(self.lib.pzg_learned_collect(self.ctx.h, nrhs, seed))
 
-    def train(self, steps=300, lr=3e-3):
+    def train(self, steps=200, lr=2e-3):
         l0, l1 = C.c_double(), C.c_double()
         L.check(self.lib.pzg_learned_train(self.ctx.h, steps, lr, C.byref(l0), C.byref(l1)))
         return l0.value, l1.value
@@ -46,45 +52,66 @@ class LearnedPreconditioner:
         assert len(w) == self.nparams()
         L.check(self.lib.pzg_learned_weights(self.ctx.h, None, L.ptr(w, L.fp), None))
 
-    def graph(self, nb):
-        nc = C.c_int64()
-        ns = C.c_int64()
-        L.check(self.lib.pzg_learned_graph(self.ctx.h, C.byref(nc), None, None, None, C.byref(ns)))
-        cia = np.zeros(nc.value + 1, np.int64)
+    def graph(self, neq):
+        """(cia, cja, features [nel, 4], penalized mask [neq]) of the current
+        assembly."""
+        nel = C.c_int64()
+        L.check(self.lib.pzg_learned_graph(self.ctx.h, C.byref(nel), None, None, None, None))
+        cia = np.zeros(nel.value + 1, np.int64)
         L.check(self.lib.pzg_learned_graph(self.ctx.h, None, L.ptr(cia, L.i64p), None, None, None))
         cja = np.zeros(max(int(cia[-1]), 1), np.int32)
-        agg = np.zeros(nb, np.int32)
-        L.check(self.lib.pzg_learned_graph(self.ctx.h, None, None, L.ptr(cja, L.i32p),
-                                           L.ptr(agg, L.i32p), None))
-        return cia, cja[: cia[-1]].astype(np.int64), agg, ns.value
+        X = np.zeros((nel.value, NFEAT), np.float32)
+        pen = np.zeros(neq, np.float32)
+        L.check(self.lib.pzg_learned_graph(self.ctx.h, None, None, L.ptr(cja, L.i32p), L.ptr(X, L.fp),
+                                           L.ptr(pen, L.fp)))
+        return cia, cja[: cia[-1]].astype(np.int64), X, pen
 
-    def samples(self, nc, bs, ns):
-        X = np.zeros(ns * nc * bs, np.float32)
-        T = np.zeros(ns * nc * bs, np.float32)
-        L.check(self.lib.pzg_learned_samples(self.ctx.h, L.ptr(X, L.fp), L.ptr(T, L.fp)))
-        return X.reshape(ns, nc, bs), T.reshape(ns, nc, bs)
+    def samples(self, nel):
+        ne, nr = C.c_int32(), C.c_int32()
+        L.check(self.lib.pzg_learned_samples(self.ctx.h, C.byref(ne), C.byref(nr), None, None, None))
+        R = np.zeros((nel, nr.value, ne.value), np.float32)
+        E = np.zeros_like(R)
+        Mk = np.zeros((nel, ne.value), np.float32)
+        L.check(self.lib.pzg_learned_samples(self.ctx.h, None, None, L.ptr(R, L.fp), L.ptr(E, L.fp),
+                                             L.ptr(Mk, L.fp)))
+        return R, E, Mk
 
-    def loss_grad(self, X, T):
-        X = np.ascontiguousarray(X, np.float32)
-        T = np.ascontiguousarray(T, np.float32)
+    def loss_grad(self):
         g = np.zeros(self.nparams(), np.float32)
         loss = C.c_double()
-        L.check(self.lib.pzg_learned_loss_grad(self.ctx.h, X.shape[0], L.ptr(X, L.fp), L.ptr(T, L.fp),
-                                               C.byref(loss), L.ptr(g, L.fp)))
+        L.check(self.lib.pzg_learned_loss_grad(self.ctx.h, C.byref(loss), L.ptr(g, L.fp)))
         return loss.value, g
 
+    def apply(self, r):
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.zeros_like(r)
+        L.check(self.lib.pzg_learned_apply(self.ctx.h, L.ptr(r, L.dp), L.ptr(z, L.dp)))
+        return z
 
-def train_on_seeds(ctx, strmat, K, make_coef, seeds, coarsen=4, nrhs=2,
-                   record=(2, 5, 10, 20, 40, 80), steps=400, lr=3e-3, setup_seed=0):
-    """Generates pairs on every seeded coefficient field of the same mesh
-    (pzg_mesh_set_coef + pzg_assemble + pzg_learned_collect), then trains.
-    Returns (LearnedPreconditioner, loss_first, loss_last)."""
-    lp = LearnedPreconditioner(ctx)
-    lp.setup(coarsen=coarsen, seed=setup_seed, max_samples=len(seeds) * nrhs * len(record))
-    for s in seeds:
-        coef = np.ascontiguousarray(make_coef(s), np.float64)
-        L.check(ctx.lib.pzg_mesh_set_coef(ctx.h, L.ptr(coef, L.dp)))
-        strmat.Assemble(K)
-        lp.collect(nrhs=nrhs, record=record, seed=1000 + s)
-    l0, l1 = lp.train(steps=steps, lr=lr)
-    return lp, l0, l1
+    def scales(self, nel):
+        s = np.zeros(nel, np.float32)
+        L.check(self.lib.pzg_learned_scales(self.ctx.h, L.ptr(s, L.fp)))
+        return s
+
+
+def train_on_seeds(ctx, strmat, K, make_coef, seeds, nrhs=4, steps=100, lr=2e-3, setup_seed=0,
+                   rounds=1):
+    """Trains the learned smoother on seeded coefficient fields of one mesh:
+    for each seed, pzg_mesh_set_coef + assembly + pairs of the local problems,
+    then the least-squares fit of G on every pair so far and ``steps`` Adam
+    steps of the network on the seed's pairs (``rounds`` passes over the
+    seeds).  Returns (LearnedSmoother, loss of the initial parameters on the
+    first seed's pairs, loss after the last step)."""
+    ls = LearnedSmoother(ctx)
+    ls.setup(seed=setup_seed)
+    first = last = None
+    for _ in range(rounds):
+        for s in seeds:
+            coef = np.ascontiguousarray(make_coef(s), np.float64)
+            L.check(ctx.lib.pzg_mesh_set_coef(ctx.h, L.ptr(coef, L.dp)))
+            strmat.Assemble(K)
+            ls.collect(nrhs=nrhs, seed=1000 + s)
+            if first is None:  # the loss of the initial parameters
+                first = ls.loss_grad()[0]
+            _, last = ls.train(steps=steps, lr=lr)
+    return ls, first, last

@@ -777,3 +777,24 @@ def build_refined3d(pb, refine):
         dep_coef=np.array(dv, dtype=np.float64), conn_order=np.where(kinds == K_CORNER, 1, pb.p),
         leaf_parent=np.array(parent, dtype=np.int64),
     )
+
+
+def volume_dofs(m):
+    """Equation of every dof of every volume element, shape-major
+    (index = nstate * shape + state), the order of the element matrix: the
+    shapes are the sides whose connect carries equations (uniform order
+    p <= 2: all sides for p = 2, the corners for p = 1).  -1 where a shape's
+    connect has none in this element."""
+    vc = np.asarray(m["vol_conn"]).reshape(len(m["vol_nodes"]), -1)
+    pos = np.asarray(m["blk_pos"])
+    size = np.asarray(m["blk_size"])
+    ns = m["nstate"]
+    has = (size[vc] > 0).any(axis=0)  # sides that carry shapes in some element
+    sides = np.flatnonzero(has)
+    out = np.full((vc.shape[0], len(sides) * ns), -1, dtype=np.int64)
+    for j, sd in enumerate(sides):
+        c = vc[:, sd]
+        ok = size[c] > 0
+        for k in range(ns):
+            out[ok, j * ns + k] = pos[c[ok]] + k
+    return out

@@ -92,6 +92,7 @@ def lib():
         L.orc_setup_pattern_full.restype = C.c_int64
         L.orc_assemble_full.argtypes = [P, M, i64p, i64p, dp, dp]
         L.orc_assemble.argtypes = [P, M, i64p, i64p, dp, dp]
+        L.orc_assemble_colored.argtypes = [P, M, i64p, i64p, dp, dp, C.c_int]
         L.orc_assemble_range.argtypes = [P, M, i64p, i64p, dp, dp, C.c_int64, C.c_int64]
         L.orc_addkel.argtypes = [i64p, i64p, dp, C.c_int, dp, i64p]
         L.orc_multadd.argtypes = [C.c_int64, i64p, i64p, dp, dp, dp, dp, C.c_double, C.c_double]
@@ -445,6 +446,17 @@ class Oracle:
             raise RuntimeError("oracle EvaluateError failed")
         el = el.reshape(-1)[: self.m.nvol * nerr].reshape(self.m.nvol, nerr).copy()
         return el, tot[:nerr].copy()
+
+    def assemble_colored(self, nthreads):
+        """Coloured multi-threaded assembly (TPZStructMatrixOT's element
+        loop): the same system up to the summation order."""
+        ia, ja = self.pattern()
+        a = np.zeros(len(ja))
+        rhs = np.zeros(self.neq)
+        if lib().orc_assemble_colored(C.byref(self.pb), C.byref(self.m), _ip(ia), _ip(ja), _dp(a),
+                                      _dp(rhs), nthreads) != 0:
+            raise RuntimeError("oracle coloured assembly failed")
+        return a, rhs
 
     def assemble_range(self, e0, e1, a=None, rhs=None):
         ia, ja = self.pattern()

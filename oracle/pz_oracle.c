@@ -1613,6 +1613,67 @@ int orc_assemble_range(const orc_problem *pb, const orc_mesh *m, const int64_t *
 }
 
 /* Serial_Assemble (StrMatrix/pzstrmatrixor.cpp:104-365), element order. */
+/* The element loop of TPZStructMatrixOT (StrMatrix/pzstrmatrixot.cpp:
+ * 92-553): elements are coloured so that no two of a colour share a connect
+ * (first fit over the connects, as TPZStructMatrixOT::ElementColoring), and
+ * the elements of one colour are integrated and added by nthreads threads
+ * at once; colours run in order.  The sums per entry then follow the colour
+ * order instead of the element order (a rounding difference).  CPU baseline
+ * of the assembly only. */
+int orc_assemble_colored(const orc_problem *pb, const orc_mesh *m, const int64_t *ia,
+                         const int64_t *ja, double *a, double *rhs, int nthreads) {
+  memset(a, 0, sizeof(double) * ia[m->neq]);
+  memset(rhs, 0, sizeof(double) * m->neq);
+  const int64_t nel = m->nel;
+  int *color = (int *)malloc(sizeof(int) * (nel ? nel : 1));
+  uint64_t *used = (uint64_t *)calloc(m->nconn ? m->nconn : 1, sizeof(uint64_t));
+  int ncolor = 0;
+  for (int64_t e = 0; e < nel; e++) {
+    const int ns = etype_nsides(m->etype[e]);
+    uint64_t msk = 0;
+    for (int s = 0; s < ns; s++) msk |= used[m->econn[27 * e + s]];
+    if (msk == ~0ULL) { free(color); free(used); return -1; }
+    int c = __builtin_ctzll(~msk);
+    color[e] = c;
+    if (c + 1 > ncolor) ncolor = c + 1;
+    for (int s = 0; s < ns; s++) used[m->econn[27 * e + s]] |= 1ULL << c;
+  }
+  free(used);
+  int64_t *start = (int64_t *)calloc(ncolor + 1, sizeof(int64_t));
+  int64_t *list = (int64_t *)malloc(sizeof(int64_t) * (nel ? nel : 1));
+  for (int64_t e = 0; e < nel; e++) start[color[e] + 1]++;
+  for (int c = 0; c < ncolor; c++) start[c + 1] += start[c];
+  int64_t *fill = (int64_t *)malloc(sizeof(int64_t) * (ncolor ? ncolor : 1));
+  for (int c = 0; c < ncolor; c++) fill[c] = start[c];
+  for (int64_t e = 0; e < nel; e++) list[fill[color[e]]++] = e;
+  free(fill);
+  free(color);
+  int rc = 0;
+  for (int c = 0; c < ncolor; c++) {
+#pragma omp parallel num_threads(nthreads)
+    {
+      double *ek = (double *)malloc(sizeof(double) * ORC_MAXNE * ORC_MAXNE);
+      double ef[ORC_MAXNE];
+      int64_t dest[ORC_MAXNE];
+#pragma omp for schedule(dynamic, 64)
+      for (int64_t k = start[c]; k < start[c + 1]; k++) {
+        int ne = orc_calcstiff(pb, m, list[k], ek, ef, dest);
+        if (ne < 0) {
+#pragma omp atomic write
+          rc = -1;
+          continue;
+        }
+        orc_addkel(ia, ja, a, ne, ek, dest);
+        for (int i = 0; i < ne; i++) rhs[dest[i]] += ef[i];
+      }
+      free(ek);
+    }
+  }
+  free(start);
+  free(list);
+  return rc;
+}
+
 int orc_assemble(const orc_problem *pb, const orc_mesh *m, const int64_t *ia, const int64_t *ja,
                  double *a, double *rhs) {
   memset(a, 0, sizeof(double) * ia[m->neq]);

@@ -120,6 +120,11 @@ void orc_contribute(int material, int dim, int nsh, int ne, double weight, const
                     const double *dphix, const double *f, double kappa, double E, double nu,
                     double *ek, double *ef);
 
+/* Coloured multi-threaded assembly (the element loop of TPZStructMatrixOT):
+ * the CPU baseline's assembly. */
+int orc_assemble_colored(const orc_problem *pb, const orc_mesh *m, const int64_t *ia,
+                         const int64_t *ja, double *a, double *rhs, int nthreads);
+
 /* SetupMatrixData / assembly with an active TPZEquationFilter: dest[neq] is
  * the condensed index of each equation (-1 inactive, increasing over the
  * active ones).  The pattern has nactive rows; the load is returned scattered

@@ -52,7 +52,7 @@ def _read_result(path, extra=()):
 def test_runner_rejects_malformed_mesh(tmp_path):
     assert RUN.exists(), "build the library first (neopz_amd/csrc make)"
     bad = tmp_path / "bad.bin"
-    bad.write_bytes(b"PZGMESH4" + b"\0" * 6)
+    bad.write_bytes(b"PZGMESH5" + b"\0" * 6)
     r = subprocess.run([str(RUN), str(bad), str(tmp_path / "o.bin")], capture_output=True, text=True,
                        timeout=60)
     assert r.returncode == 2 and "malformed" in r.stderr
@@ -245,3 +245,56 @@ def test_cpp_multigrid_mixed_solve(tmp_path):
     assert np.linalg.norm(rhs - K @ res["u"]) <= 2e-8 * np.linalg.norm(rhs)
     assert np.linalg.norm(res["u"] - x) <= 1e-5 * np.linalg.norm(x)
     assert res["iterations"] < 200  # block Jacobi needs about 520 here
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", ["cg-mixed", "cg-mixed-mg"])
+def test_cpp_mixed_solver_through_setcg(tmp_path, method):
+    """TPZStepSolver::SetCG(maxit, jac, tol, 0) with the struct matrix set to
+    the mixed-precision FCG: TPZStepSolver::Solve -> SolveCG runs it, writes
+    back the iterations and the achieved (true FP64) residual; the solution
+    agrees with the oracle's CG solution within 1e-6."""
+    pb = P.elasticity3d(n=4, p=2, seed=6, field_kind="grf", load="body")
+    mfile, ofile = tmp_path / "mesh.bin", tmp_path / "out.bin"
+    M.save(M.build(pb), mfile)
+    print(f"[gpu] pzhip_run {method}", flush=True)
+    r = subprocess.run([str(RUN), str(mfile), str(ofile), method, "1e-8", "20000"],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    res = _read_result(ofile)
+    assert 0 < res["iterations"] < 20000 and res["achieved"] <= 1e-8
+    o = O.Oracle(pb)
+    ia, ja = o.pattern()
+    a, rhs = o.assemble()
+    ref = O.pcg(ia, ja, a, rhs, maxit=20000, tol=1e-12, wasted_spmv=False)
+    assert np.linalg.norm(res["u"] - ref["x"]) <= 1e-6 * np.linalg.norm(ref["x"])
+
+
+@pytest.mark.gpu
+def test_cpp_equation_filter(tmp_path):
+    """The struct matrix's TPZEquationFilter::SetMinMaxEq through the C++
+    mirror: condensed IA/JA/A equal the oracle's filtered SetupMatrixData and
+    assembly, the rhs comes back in the mesh numbering and the CG solution
+    of the condensed system matches the oracle's."""
+    pb = P.darcy2d(n=8, p=2, seed=3)
+    mfile, ofile = tmp_path / "mesh.bin", tmp_path / "out.bin"
+    M.save(M.build(pb), mfile)
+    o = O.Oracle(pb)
+    lo, hi = 7, o.neq - 5
+    print("[gpu] pzhip_run minmax", flush=True)
+    r = subprocess.run([str(RUN), str(mfile), str(ofile), "cg", "1e-13", "20000", f"minmax:{lo}:{hi}"],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    res = _read_result(ofile)
+    act = np.arange(lo, hi)
+    fia, fja, fa, frhs = o.filtered(act)
+    assert np.array_equal(res["ia"], fia) and np.array_equal(res["ja"], fja)
+    assert np.max(np.abs(res["a"] - fa) - 1e-12 * np.abs(fa)) <= 1e-15 * np.max(np.abs(fa))
+    assert len(res["rhs"]) == o.neq
+    assert np.max(np.abs(res["rhs"] - frhs)) <= 1e-12 * np.max(np.abs(frhs))
+    ref = O.pcg(fia, fja, fa, frhs[act], maxit=20000, tol=1e-13, wasted_spmv=False)
+    u = res["u"]
+    assert len(u) == o.neq and np.all(u[:lo] == 0) and np.all(u[hi:] == 0)
+    assert np.linalg.norm(u[act] - ref["x"]) <= 1e-8 * np.linalg.norm(ref["x"])
