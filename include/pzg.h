@@ -327,7 +327,9 @@ int pzg_hp_tuning(pzg_ctx *ctx, int32_t *n, int32_t *order, int32_t *variant, fl
 int pzg_scaled_apply(pzg_ctx *ctx, int32_t op, const double *x, double *y);
 
 /* z = M r with M the V-cycle of PZG_PRECOND_MULTIGRID for the scaled system
- * S (level 0 applied by op: PZG_INNER_BSR or PZG_INNER_MATFREE), on host
+ * S (level 0 applied by op: PZG_INNER_BSR or PZG_INNER_MATFREE; op |
+ * PZG_MG_LEARNED smooths level 0 with the learned element smoother, the
+ * cycle of PZG_PRECOND_LEARNED, and lmax[0] is then that of M_learned S), on host
  * vectors of length neq; degree and range as mg_degree and mg_range of
  * pzg_solve_params, 0 for the defaults.  *nlev receives the number of lattice levels (the first is the
  * trilinear subspace of the mesh's vertices, the last the dense one),
@@ -336,6 +338,7 @@ int pzg_scaled_apply(pzg_ctx *ctx, int32_t op, const double *x, double *y);
  * lattice level l (arrays of 48 and 16 entries; any may be NULL).  The
  * levels are built on the first call after an assembly.  Exposes the mixed
  * solver's multigrid preconditioner, which has no reference counterpart. */
+#define PZG_MG_LEARNED 4
 int pzg_mg_apply(pzg_ctx *ctx, int32_t op, int32_t degree, double range, const double *r, double *z,
                  int32_t *nlev, int32_t *dims, double *lmax);
 

@@ -15,6 +15,7 @@ PZG_OK, PZG_EINVAL, PZG_EHIP, PZG_ENOCONV, PZG_ESTATE, PZG_ENOMEM, PZG_EUNSUP = 
 SOLVER_PCG64, SOLVER_MIXED = 0, 1
 PRECOND_JACOBI, PRECOND_BLOCK_JACOBI, PRECOND_LEARNED, PRECOND_MULTIGRID = 0, 1, 2, 3
 INNER_AUTO, INNER_BSR, INNER_MATFREE = 0, 1, 2
+MG_LEARNED = 4  # pzg_mg_apply: level-0 smoothing by the learned element smoother
 EXACT_POINTS, EXACT_CONST, EXACT_X, EXACT_SINSIN, EXACT_COSCOS = range(5)
 
 EXPORTS = [

@@ -1461,6 +1461,7 @@ int launch_build_bsr(Ctx &c, bool values) {
   if (values) c.bsr_valid = true;
   c.mf_vals_valid = false;  // D^-1/2 recomputed
   c.mg_ready = false;       // multigrid levels are built from K and D
+  c.pen_ready = false;      // so is the penalty block of the learned path
   return PZG_OK;
 }
 

@@ -64,7 +64,7 @@ static void free_all(Ctx &c) {
                     &c.d_zs, &c.d_ps, &c.d_ps2, &c.d_qs, &c.d_bs, &c.d_scal, &c.d_partials,
                     &c.d_counter, &c.d_res2, &c.d_les_th, &c.d_les_m, &c.d_les_v, &c.d_les_g,
                     &c.d_les_cia, &c.d_les_cja, &c.d_les_pen, &c.d_les_X, &c.d_les_z, &c.d_les_s,
-                    &c.d_les_ye, &c.d_les_Se, &c.d_les_M, &c.d_les_R, &c.d_les_E, &c.d_les_D,
+                    &c.d_les_ye, &c.d_les_blkm, &c.d_les_eoff, &c.d_les_Se, &c.d_les_M, &c.d_les_R, &c.d_les_E, &c.d_les_D,
                     &c.d_les_le, &c.d_les_ds, &c.d_les_loss, &c.d_les_fail, &c.d_nn_trH, &c.d_nn_trM,
                     &c.d_nn_trP, &c.d_nn_dH, &c.d_nn_G, &c.d_nn_tmp, &c.d_nn_dZ, &c.d_nn_part,
                     &c.d_filt_dest, &c.d_post_tab, &c.d_post_list, &c.d_post_slot,
@@ -700,7 +700,7 @@ static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
   mg_free(c);  // the multigrid lattice belongs to the mesh
   c.assembled = false;
   c.external_csr = false;
-  c.has_solution = false;
+  c.has_solution = c.f_valid = false;
   c.post_porder = -1;
   return PZG_OK;
 }
@@ -936,7 +936,7 @@ static int struct_create(Ctx &c) {
   c.kel_added = false;
   c.mf_ready = c.mf_vals_valid = false;
   c.full_valid = c.bsr_valid = c.scal_valid = false;
-  c.has_solution = false;
+  c.has_solution = c.f_valid = false;
   return PZG_OK;
 }
 
@@ -1045,7 +1045,7 @@ int pzg_set_equation_filter(pzg_ctx *ctx, int64_t nactive, const int64_t *active
   // the pattern belongs to the filter: pzg_struct_create builds it again
   c.has_struct = false;
   c.assembled = false;
-  c.has_solution = false;
+  c.has_solution = c.f_valid = false;
   c.mf_ready = c.mf_vals_valid = false;
   c.full_valid = c.bsr_valid = c.scal_valid = false;
   mg_free(c);
@@ -1098,7 +1098,7 @@ int pzg_assemble(pzg_ctx *ctx) {
   c.assembled = true;
   c.kel_added = false;
   c.full_valid = c.bsr_valid = c.scal_valid = false;
-  c.has_solution = false;
+  c.has_solution = c.f_valid = false;
   return PZG_OK;
 }
 
@@ -1270,7 +1270,7 @@ int pzg_csr_set(pzg_ctx *ctx, int64_t neq, const int64_t *ia, const int64_t *ja,
   c.mf_ready = c.mf_vals_valid = false;
   c.assembled = true;
   c.full_valid = c.bsr_valid = c.scal_valid = false;
-  c.has_solution = false;
+  c.has_solution = c.f_valid = false;
   return PZG_OK;
 }
 
@@ -1335,6 +1335,7 @@ int pzg_add_kel_batch(pzg_ctx *ctx, int64_t nel, int32_t ne, const int64_t *dest
   c.assembled = true;
   c.kel_added = true;
   c.full_valid = c.bsr_valid = c.scal_valid = false;
+  c.mg_ready = c.pen_ready = false;  // both are built from K
   return rc;
 }
 
@@ -1399,6 +1400,7 @@ int pzg_solve(pzg_ctx *ctx, const pzg_solve_params *params, const double *rhs, d
   PZG_TRY(dev_alloc(c.d_f, n * sizeof(double)));
   if (rhs) PZG_HIP(hipMemcpyAsync(c.d_f.p, rhs, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
   else PZG_HIP(hipMemcpyAsync(c.d_f.p, c.d_rhs.p, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  c.f_valid = true;
   int rc;
   if (params->solver == PZG_SOLVER_PCG64) rc = solve_pcg64(c, *params, inf);
   else if (params->solver == PZG_SOLVER_MIXED) rc = solve_mixed(c, *params, inf);
@@ -1425,6 +1427,11 @@ int pzg_true_residual(pzg_ctx *ctx, double *rel) {
   if (!c.has_solution || !rel) {
     set_error("pzg_true_residual: no solution on the device");
     return PZG_ESTATE;
+  }
+  if (!c.f_valid) {  // no solve of this system yet: the assembled load
+    PZG_TRY(dev_alloc(c.d_f, c.neq * sizeof(double)));
+    PZG_HIP(hipMemcpyAsync(c.d_f.p, c.d_rhs.p, c.neq * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    c.f_valid = true;
   }
   return true_residual(c, c.d_u.as<double>(), rel);
 }
@@ -1627,9 +1634,15 @@ int pzg_mg_apply(pzg_ctx *ctx, int32_t op, int32_t degree, double range, const d
     set_error("pzg_mg_apply: needs an assembled system and host vectors");
     return PZG_ESTATE;
   }
+  const bool learned = (op & PZG_MG_LEARNED) != 0;
+  op &= ~PZG_MG_LEARNED;
   if (op != PZG_INNER_BSR && op != PZG_INNER_MATFREE) {
-    set_error("pzg_mg_apply: op must be PZG_INNER_BSR or PZG_INNER_MATFREE");
+    set_error("pzg_mg_apply: op must be PZG_INNER_BSR or PZG_INNER_MATFREE (| PZG_MG_LEARNED)");
     return PZG_EINVAL;
+  }
+  if (learned && !c.les_ready) {
+    set_error("pzg_mg_apply: learned smoother not set up in this context");
+    return PZG_ESTATE;
   }
   if (op == PZG_INNER_MATFREE && !mf_available(c)) {
     set_error("pzg_mg_apply: matrix-free operator needs a mesh-built system");
@@ -1643,8 +1656,8 @@ int pzg_mg_apply(pzg_ctx *ctx, int32_t op, int32_t degree, double range, const d
   c.mg_degree = degree > 0 ? degree : 2;
   c.mg_range = range > 0 ? range : 30.;
   const bool use_mf = op == PZG_INNER_MATFREE;
-  if (!c.mg_ready || c.mg_use_mf != use_mf || c.mg_les) {
-    c.mg_les = false;  // the plain cycle (block-Jacobi smoothing)
+  if (!c.mg_ready || c.mg_use_mf != use_mf || c.mg_les != learned) {
+    c.mg_les = learned;  // level-0 smoothing: learned element smoother or block Jacobi
     PZG_TRY(mg_setup(c, use_mf));
   }
   std::vector<float> h(n);
@@ -1680,7 +1693,7 @@ int pzg_mesh_set_coef(pzg_ctx *ctx, const double *vol_coef) {
   PZG_HIP(hipStreamSynchronize(c.stream));
   c.assembled = false;
   c.full_valid = c.bsr_valid = c.scal_valid = false;
-  c.has_solution = false;
+  c.has_solution = c.f_valid = false;
   return PZG_OK;
 }
 

@@ -288,6 +288,7 @@ struct Ctx {
   DevBuf d_res2;  // |r|^2, |f|^2 of the FP64 true residual
   int reduce_blocks = 1024;
   bool has_solution = false;
+  bool f_valid = false;  // d_f holds the load of the current system (set by a solve)
 
   // matrix-free FP32 inner operator (pzg_matfree.hip)
   bool mf_ready = false, mf_vals_valid = false;
@@ -321,6 +322,16 @@ struct Ctx {
   // vertex equations, the dense inverse of the coarsest level
   bool mg_ready = false, mg_use_mf = false;
   bool mg_les = false;     // level 0 smoothed with the learned element smoother
+  DevBuf d_les_blkm;       // element blocks with penalized-component bits (learned apply)
+  DevBuf d_les_eoff;       // row-ordered slot -> element-major output offset (learned apply)
+  // penalty rows of the learned path, solved in FP64 by the outer refinement
+  // (pzg_penalty.hip): K_pp CSR over the penalized equations, the Schwarz
+  // sets of the penalized boundary elements with their inverses, the row ->
+  // slot gather, CG work vectors and scalars
+  bool pen_ready = false, pen_struct = false;  // values of this assembly; structure of this mesh
+  int64_t pen_n = 0, pen_nnz = 0, pen_nset = 0, pen_nslot = 0;
+  DevBuf d_pen_idx, d_pen_pmap, d_pen_ia, d_pen_ja, d_pen_a, d_pen_sptr, d_pen_sidx, d_pen_sset, d_pen_soff,
+      d_pen_sval, d_pen_sinv, d_pen_gptr, d_pen_gslot, d_pen_w, d_pen_scal;
   bool mg_struct = false;  // lattice, levels and buffers of the current mesh
   bool mg_dense = true;    // the coarsest level has a dense inverse (else a Chebyshev solve)
   int mg_degree = 2;       // Chebyshev degree of the pre- and post-smoothing
@@ -387,6 +398,11 @@ int les_collect(Ctx &c, int nrhs, uint64_t seed);
 int les_loss_grad(Ctx &c, double *loss_out);
 int les_adam_step(Ctx &c, double lr);
 int les_fit_G(Ctx &c);  // G from the accumulated least-squares sums
+// penalty rows of the learned path (pzg_penalty.hip)
+int pen_setup(Ctx &c);
+int pen_solve(Ctx &c, const double *r64, double *u64, double stop_abs, int64_t *its);
+int pen_zero32(Ctx &c, float *r32);
+void pen_free(Ctx &c);
 int mf_setup(Ctx &c);
 // multigrid preconditioner: levels, operators and eigenvalue estimates of the
 // current system (use_mf: level 0 applied matrix-free); z = V-cycle(r), then

@@ -273,23 +273,27 @@ __global__ void k_les_exp(int64_t n, const float *__restrict__ z, float *__restr
 }
 
 // ---- application ---------------------------------------------------------------
-// Hexahedra of order 2 (27 shapes): z_e = s_e G r_e for 16 elements per
-// wavefront as v_mfma_f32_16x16x4f32 tiles: rows = the element's dofs padded
-// to NT*16, columns = elements, k = dofs.  G is staged in LDS in the A
-// operand order (tile ti, k-step kk, lane l: G[16 ti + l%16][4 kk + l/16]),
-// so every LDS read of a wavefront is one contiguous segment.  The element's
-// inputs r (zero on penalized dofs) are gathered once into registers; the
-// outputs go to the element's slots in the row-ordered output lists.
+// Elements with NE = NSH*NS <= 96 dofs: z_e = s_e G r_e for 16 elements per
+// wavefront as v_mfma_f32_16x16x4f32 tiles: rows = the element's output dofs
+// padded to NT*16, columns = elements, k = input dofs padded to NK*4.  G is
+// staged in LDS in the A operand order (tile ti, k-step kk, lane l:
+// G[16 ti + l%16][4 kk + l/16]), so every LDS read of a wavefront is one
+// contiguous segment.  Lane (element l%16, quarter l/16) gathers its NK
+// inputs: the element's block of each input shape (blkm: block index, bit
+// 28 + component set on penalized dofs, -1 when the side has no shapes),
+// then the values, both as batches so each is one round trip to memory; the
+// next group's batches are in flight while the current group multiplies.
+// The outputs are stored element by element (NE floats each); the gather
+// below sums them per block row.
 template <int NSH, int NS>
 __global__ __launch_bounds__(256) void k_les_mfma(int64_t nvol, const float *__restrict__ G,
                                                   const float *__restrict__ scale,
-                                                  const int32_t *__restrict__ blk,
-                                                  const int32_t *__restrict__ pos,
-                                                  const float *__restrict__ pen,
+                                                  const int32_t *__restrict__ blkm,
                                                   const float *__restrict__ r, float *__restrict__ ye,
                                                   const SolveScalars *sc) {
   if (sc && sc->done) return;
-  constexpr int NE = NSH * NS, NT = (NE + 15) / 16, NK = 4 * NT;
+  constexpr int NE = NSH * NS, NT = (NE + 15) / 16, NK = (NE + 3) / 4;
+  static_assert(NT * 16 <= 96, "element too large for the tile set");
   __shared__ float sA[NT * NK * 64];
   const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, col = l & 15, kq = l >> 4;
   for (int t = tid; t < NT * NK * 64; t += 256) {
@@ -298,22 +302,43 @@ __global__ __launch_bounds__(256) void k_les_mfma(int64_t nvol, const float *__r
     sA[t] = (row < NE && k < NE) ? G[row * NE + k] : 0.f;
   }
   __syncthreads();
-  for (int64_t g0 = (int64_t)blockIdx.x * 64; g0 < nvol; g0 += (int64_t)gridDim.x * 64) {
-    const int64_t e = g0 + wv * 16 + col;
-    const bool ev = e < nvol;
-    const int64_t ec = ev ? e : 0;
-    float u[NK];
+  int64_t lo, hi;
+  int xi, xg;
+  xcd_slab(nvol, lo, hi, xi, xg);
+  const int64_t gfirst = lo + (int64_t)xi * 64, gstep = (int64_t)xg * 64;
+  auto gather = [&](int64_t g, float (&u)[NK]) {
+    const int64_t e = g + wv * 16 + col;
+    const bool ev = e < hi;
+    const int64_t eo = ev ? e * NSH : 0;
+    int32_t bk[NK];
 #pragma unroll
     for (int kk = 0; kk < NK; kk++) {
       const int k = 4 * kk + kq;
-      const int sh = k < NE ? k / NS : 0, cp = k < NE ? k % NS : 0;
-      const int32_t b = blk[ec * NSH + sh];
-      const int64_t i = (int64_t)(b >= 0 ? b : 0) * NS + cp;
-      const float v = r[i] * (1.f - pen[i]);
-      u[kk] = (ev && k < NE && b >= 0) ? v : 0.f;
+      bk[kk] = blkm[eo + (k < NE ? k / NS : NSH - 1)];
     }
-    const float s = ev ? scale[ec] : 0.f;
 #pragma unroll
+    for (int kk = 0; kk < NK; kk++) {
+      const int k = 4 * kk + kq, cp = k % NS;
+      const int32_t bm = bk[kk];
+      const bool ok = ev && k < NE && bm >= 0 && !((bm >> (28 + cp)) & 1);
+      const float v = r[ok ? (int64_t)(bm & 0x0FFFFFFF) * NS + cp : 0];
+      u[kk] = ok ? v : 0.f;
+    }
+  };
+  float un[NK];
+  if (gfirst < hi) gather(gfirst, un);
+  for (int64_t g0 = gfirst; g0 < hi; g0 += gstep) {
+    float u[NK];
+#pragma unroll
+    for (int kk = 0; kk < NK; kk++) u[kk] = un[kk];
+    const int64_t e = g0 + wv * 16 + col;
+    const bool ev = e < hi;
+    const float s = scale[ev ? e : 0];
+    float *yo = ye + (ev ? e : 0) * NE;
+    if (g0 + gstep < hi) gather(g0 + gstep, un);
+    // one row tile at a time (registers: the A operands of a tile come from
+    // LDS as the products issue); the element's NE outputs are contiguous
+#pragma unroll 1
     for (int ti = 0; ti < NT; ti++) {
       floatx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -322,12 +347,24 @@ __global__ __launch_bounds__(256) void k_les_mfma(int64_t nvol, const float *__r
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int o = 16 * ti + 4 * kq + i;
-        if (!ev || o >= NE) continue;
-        const int32_t p = pos[e * NSH + o / NS];
-        if (p >= 0) ye[(int64_t)p * NS + o % NS] = s * acc[i];
+        if (ev && o < NE) yo[o] = s * acc[i];
       }
     }
   }
+}
+
+// blkm = the element's block per shape with the penalized components as bits
+// 28 + k (-1 where the side has no shapes)
+__global__ void k_les_blkm(int64_t n, int ns, const int32_t *__restrict__ blk, const float *__restrict__ pen,
+                           int32_t *__restrict__ blkm) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int32_t b = blk[t];
+  int32_t m = b;
+  if (b >= 0)
+    for (int k = 0; k < ns; k++)
+      if (pen[(int64_t)b * ns + k] != 0.f) m |= 1 << (28 + k);
+  blkm[t] = m;
 }
 
 // Any other element: one thread per (element, output dof).
@@ -335,7 +372,6 @@ template <int NSH, int NS>
 __global__ __launch_bounds__(256) void k_les_gen(int64_t nvol, const float *__restrict__ G,
                                                  const float *__restrict__ scale,
                                                  const int32_t *__restrict__ blk,
-                                                 const int32_t *__restrict__ pos,
                                                  const float *__restrict__ pen,
                                                  const float *__restrict__ r, float *__restrict__ ye,
                                                  const SolveScalars *sc) {
@@ -345,8 +381,6 @@ __global__ __launch_bounds__(256) void k_les_gen(int64_t nvol, const float *__re
        t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t e = t / NE;
     const int o = (int)(t % NE);
-    const int32_t p = pos[e * NSH + o / NS];
-    if (p < 0) continue;
     float acc = 0.f;
     for (int k = 0; k < NE; k++) {
       const int32_t b = blk[e * NSH + k / NS];
@@ -354,14 +388,29 @@ __global__ __launch_bounds__(256) void k_les_gen(int64_t nvol, const float *__re
       const int64_t i = (int64_t)b * NS + k % NS;
       acc += G[o * NE + k] * r[i] * (1.f - pen[i]);
     }
-    ye[(int64_t)p * NS + o % NS] = scale[e] * acc;
+    ye[t] = scale[e] * acc;
   }
 }
 
-// z = the sum of a block row's element outputs (fixed order); r on the
-// penalized equations
+// eoff[slot] = e*NE + sh*NS for the volume element shape (e, sh) that a
+// row-ordered output slot belongs to (-1: a boundary element's slot)
+__global__ void k_les_eoff(int64_t n, int nsh, int ns, const int32_t *__restrict__ pos,
+                           int32_t *__restrict__ eoff) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int32_t p = pos[t];
+  if (p >= 0) eoff[p] = (int32_t)((t / nsh) * nsh * ns + (t % nsh) * ns);
+}
+__global__ void k_les_fill_i32(int64_t n, int32_t v, int32_t *__restrict__ a) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    a[t] = v;
+}
+
+// z = the sum of a block row's element outputs (fixed order of the row's
+// slots); r on the penalized equations
 template <int NS>
 __global__ __launch_bounds__(256) void k_les_gather(int64_t nb, const int64_t *__restrict__ ptr,
+                                                    const int32_t *__restrict__ eoff,
                                                     const float *__restrict__ ye,
                                                     const float *__restrict__ pen,
                                                     const float *__restrict__ r, float *__restrict__ z,
@@ -373,9 +422,12 @@ __global__ __launch_bounds__(256) void k_les_gather(int64_t nb, const int64_t *_
     float acc[NS];
 #pragma unroll
     for (int k = 0; k < NS; k++) acc[k] = 0.f;
-    for (int64_t t = t0; t < t1; t++)
+    for (int64_t t = t0; t < t1; t++) {
+      const int32_t o = eoff[t];
+      if (o < 0) continue;
 #pragma unroll
-      for (int k = 0; k < NS; k++) acc[k] += ye[t * NS + k];
+      for (int k = 0; k < NS; k++) acc[k] += ye[(int64_t)o + k];
+    }
 #pragma unroll
     for (int k = 0; k < NS; k++) {
       const int64_t i = A * NS + k;
@@ -746,14 +798,33 @@ int les_prepare(Ctx &c) {
   }
   PZG_TRY(mf_prepare(c));  // element -> block map and row-ordered output slots
   PZG_TRY(les_graph(c));
+  if (c.nb >= (1 << 28)) {
+    set_error("learned preconditioner: more than 2^28 blocks");
+    return PZG_EUNSUP;
+  }
+  {
+    const int64_t nt = c.nvol * (int64_t)c.tvol.nsh;
+    PZG_TRY(dev_alloc(c.d_les_blkm, sizeof(int32_t) * std::max<int64_t>(nt, 1)));
+    hipLaunchKernelGGL(k_les_blkm, dim3((unsigned)std::max<int64_t>(1, (nt + TPB - 1) / TPB)), dim3(TPB), 0,
+                       c.stream, nt, c.bs, c.d_mf_blk.as<int32_t>(), c.d_les_pen.as<float>(),
+                       c.d_les_blkm.as<int32_t>());
+    PZG_HIP(hipGetLastError());
+  }
   PZG_TRY(les_features(c));
   PZG_TRY(les_forward(c, false));
-  const size_t nye = (size_t)std::max<int64_t>(c.mf_nout, 1) * c.bs;
-  if (c.d_les_ye.bytes < nye * sizeof(float)) {
-    PZG_TRY(dev_alloc(c.d_les_ye, nye * sizeof(float)));
-    // slots of boundary elements are never written: zero once
-    PZG_HIP(hipMemsetAsync(c.d_les_ye.p, 0, nye * sizeof(float), c.stream));
+  // element outputs, element-major, and the slot -> output offset map
+  const int64_t nte = c.nvol * (int64_t)c.tvol.nsh;
+  if (nte * c.bs >= ((int64_t)1 << 31)) {
+    set_error("learned preconditioner: more than 2^31 element outputs");
+    return PZG_EUNSUP;
   }
+  PZG_TRY(dev_alloc(c.d_les_ye, sizeof(float) * std::max<int64_t>(nte * c.bs, 1)));
+  PZG_TRY(dev_alloc(c.d_les_eoff, sizeof(int32_t) * std::max<int64_t>(c.mf_nout, 1)));
+  hipLaunchKernelGGL(k_les_fill_i32, dim3(grid1(c.mf_nout)), dim3(TPB), 0, c.stream, c.mf_nout, -1,
+                     c.d_les_eoff.as<int32_t>());
+  hipLaunchKernelGGL(k_les_eoff, dim3(grid1(nte)), dim3(TPB), 0, c.stream, nte, c.tvol.nsh, c.bs,
+                     c.d_mf_pos.as<int32_t>(), c.d_les_eoff.as<int32_t>());
+  PZG_HIP(hipGetLastError());
   PZG_HIP(hipStreamSynchronize(c.stream));
   return PZG_OK;
 }
@@ -763,31 +834,32 @@ int les_apply(Ctx &c, const float *r, float *z, const SolveScalars *sc) {
   const float *G = c.d_les_th.as<float>() + les_offsets(c.les_ne).g;
   const int nsh = c.tvol.nsh;
   const int32_t *blk = c.d_mf_blk.as<int32_t>(), *pos = c.d_mf_pos.as<int32_t>();
+  const int32_t *blkm = c.d_les_blkm.as<int32_t>(), *eoff = c.d_les_eoff.as<int32_t>();
   const float *pen = c.d_les_pen.as<float>(), *s = c.d_les_s.as<float>();
   float *ye = c.d_les_ye.as<float>();
   const int64_t nv = c.nvol;
   const int gm = (int)std::max<int64_t>(1, std::min<int64_t>((nv + 63) / 64, 4096));
   if (nsh == 27 && c.bs == 3)
-    hipLaunchKernelGGL((k_les_mfma<27, 3>), dim3(gm), dim3(256), 0, c.stream, nv, G, s, blk, pos, pen, r, ye, sc);
+    hipLaunchKernelGGL((k_les_mfma<27, 3>), dim3(gm), dim3(256), 0, c.stream, nv, G, s, blkm, r, ye, sc);
   else if (nsh == 27 && c.bs == 1)
-    hipLaunchKernelGGL((k_les_mfma<27, 1>), dim3(gm), dim3(256), 0, c.stream, nv, G, s, blk, pos, pen, r, ye, sc);
+    hipLaunchKernelGGL((k_les_mfma<27, 1>), dim3(gm), dim3(256), 0, c.stream, nv, G, s, blkm, r, ye, sc);
   else if (nsh == 8 && c.bs == 3)
-    hipLaunchKernelGGL((k_les_mfma<8, 3>), dim3(gm), dim3(256), 0, c.stream, nv, G, s, blk, pos, pen, r, ye, sc);
+    hipLaunchKernelGGL((k_les_mfma<8, 3>), dim3(gm), dim3(256), 0, c.stream, nv, G, s, blkm, r, ye, sc);
   else if (nsh == 8 && c.bs == 1)
-    hipLaunchKernelGGL((k_les_gen<8, 1>), dim3(grid1(nv * 8)), dim3(256), 0, c.stream, nv, G, s, blk, pos, pen, r, ye, sc);
+    hipLaunchKernelGGL((k_les_gen<8, 1>), dim3(grid1(nv * 8)), dim3(256), 0, c.stream, nv, G, s, blk, pen, r, ye, sc);
   else if (nsh == 9 && c.bs == 1)
-    hipLaunchKernelGGL((k_les_gen<9, 1>), dim3(grid1(nv * 9)), dim3(256), 0, c.stream, nv, G, s, blk, pos, pen, r, ye, sc);
+    hipLaunchKernelGGL((k_les_gen<9, 1>), dim3(grid1(nv * 9)), dim3(256), 0, c.stream, nv, G, s, blk, pen, r, ye, sc);
   else if (nsh == 4 && c.bs == 1)
-    hipLaunchKernelGGL((k_les_gen<4, 1>), dim3(grid1(nv * 4)), dim3(256), 0, c.stream, nv, G, s, blk, pos, pen, r, ye, sc);
+    hipLaunchKernelGGL((k_les_gen<4, 1>), dim3(grid1(nv * 4)), dim3(256), 0, c.stream, nv, G, s, blk, pen, r, ye, sc);
   else {
     set_error("learned preconditioner: element not supported");
     return PZG_EUNSUP;
   }
   const int gg = grid1(c.nb);
   if (c.bs == 3)
-    hipLaunchKernelGGL(k_les_gather<3>, dim3(gg), dim3(256), 0, c.stream, c.nb, c.d_mf_ptr.as<int64_t>(), ye, pen, r, z, sc);
+    hipLaunchKernelGGL(k_les_gather<3>, dim3(gg), dim3(256), 0, c.stream, c.nb, c.d_mf_ptr.as<int64_t>(), eoff, ye, pen, r, z, sc);
   else
-    hipLaunchKernelGGL(k_les_gather<1>, dim3(gg), dim3(256), 0, c.stream, c.nb, c.d_mf_ptr.as<int64_t>(), ye, pen, r, z, sc);
+    hipLaunchKernelGGL(k_les_gather<1>, dim3(gg), dim3(256), 0, c.stream, c.nb, c.d_mf_ptr.as<int64_t>(), eoff, ye, pen, r, z, sc);
   PZG_HIP(hipGetLastError());
   return PZG_OK;
 }

@@ -1031,6 +1031,8 @@ int apply_impl(Ctx &c, const float *r, float *z, float *p, const float *q, Solve
                      c.d_mg_vmap.as<int32_t>(), c.d_mg_dsq.as<float>(), c.mg[0].x.as<float>(), z, sc);
   PZG_HIP(hipGetLastError());
   PZG_TRY(smooth_post<BS>(c, L0, r, z, op_0, sc));
+  // the learned path leaves the penalty rows to the FP64 refinement
+  if ((p || q) && c.mg_les) PZG_TRY(pen_zero32(c, z));
   if (p || q) {
     // same grid as the FCG update kernel, whose partials slot 1 this replaces
     hipLaunchKernelGGL(k_mg_finish<BS>, dim3(gridr(c.nb)), dim3(TPB), 0, c.stream, c.nb, z, p, q, sc, partials);
@@ -1042,6 +1044,7 @@ int apply_impl(Ctx &c, const float *r, float *z, float *p, const float *q, Solve
 }  // namespace
 
 void mg_free(Ctx &c) {
+  pen_free(c);
   for (MgLevel &l : c.mg)
     for (DevBuf *b : {&l.A, &l.minv, &l.x, &l.b, &l.r, &l.d, &l.t}) dev_free(*b);
   c.mg.clear();

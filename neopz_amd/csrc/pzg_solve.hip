@@ -27,7 +27,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 
 #include "pzg_device.h"
 #include "pzg_internal.h"
@@ -424,6 +427,10 @@ int launch_multadd_upper(Ctx &c, const double *x, const double *y, double *z, do
 }
 
 int true_residual(Ctx &c, const double *u, double *rel) {
+  if (!c.f_valid || !c.d_f.p || !u) {  // never launch on a missing load or solution
+    set_error("true residual: no load or solution of the current system on the device");
+    return PZG_ESTATE;
+  }
   if (!c.full_valid) PZG_TRY(launch_expand_full(c));
   PZG_TRY(ensure_reduce(c));
   DISPATCH_BS(c.bs, hipLaunchKernelGGL(k_residual64<BS>, dim3(bsr_grid(c.nb)), dim3(TPB), 0,
@@ -794,6 +801,8 @@ int launch_spmv_bsr(Ctx &c, const float *x, float *y) {
   return PZG_OK;
 }
 
+constexpr int64_t MG_INNER_CAP = 2000;  // FP32 iterations per refinement step, multigrid paths
+
 template <int BS>
 static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info) {
   const int64_t n = c.neq, nb = c.nb;
@@ -832,7 +841,7 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
                      (const double *)nullptr, c.d_u.as<double>());
   int64_t total_it = 0;
   int outer = 0;
-  double rel = 1.0;
+  double rel = 1.0, fnorm = 0.;
   bool conv = false;
   // a multigrid iteration costs several operator applications: short batches
   const int batch = prm.check_every > 0 ? prm.check_every : (multigrid ? 4 : 50);
@@ -849,6 +858,7 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
     PZG_HIP(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, c.stream));
     PZG_HIP(hipStreamSynchronize(c.stream));
     rel = h[1] > 0 ? std::sqrt(h[0] / h[1]) : std::sqrt(h[0]);
+    fnorm = std::sqrt(h[1]);
     return PZG_OK;
   };
   auto next_delta = [&]() {
@@ -869,9 +879,25 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
       c.mg_les = les;
       PZG_TRY(mg_setup(c, use_mf));
     }
+    if (les && !c.pen_ready) PZG_TRY(pen_setup(c));
   }
   bool restart = true;
   bool learned = prm.precond == PZG_PRECOND_LEARNED;
+  // PZG_TRACE=1: one line per refinement step on stderr
+  const char *tr = std::getenv("PZG_TRACE");
+  const bool trace = tr && tr[0] == '1';
+  // learned path: the penalty rows are solved in FP64 here, after every fold,
+  // to a tenth of the target residual; the FP32 iterations leave them alone
+  const bool pen_rows = learned && !conv && c.pen_ready && c.pen_n > 0;
+  int64_t pen_its = 0;
+  auto pen_fix = [&]() -> int {
+    pen_its = 0;
+    if (!pen_rows || rel <= prm.tol) return PZG_OK;
+    PZG_TRY(pen_solve(c, c.d_r.as<double>(), c.d_u.as<double>(), 0.1 * prm.tol * fnorm, &pen_its));
+    return true_res();
+  };
+  PZG_TRY(pen_fix());
+  if (rel <= prm.tol) conv = true;
   int fallback = 0, stall = 0;
   double prev_rel = rel;
   SolveScalars hs;
@@ -880,7 +906,10 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
     if (restart) {
       hs.done = 0;
       hs.iter = total_it;
-      hs.maxit = prm.max_iter;
+      // a multigrid-preconditioned inner solve takes tens of iterations: one
+      // that runs past MG_INNER_CAP has stalled (the refinement step below
+      // then sees no progress and falls back)
+      hs.maxit = multigrid ? std::min<int64_t>(prm.max_iter, total_it + MG_INNER_CAP) : prm.max_iter;
       hs.delta = next_delta();
       PZG_HIP(hipMemcpyAsync(c.d_scal.p, &hs, sizeof(hs), hipMemcpyHostToDevice, c.stream));
       hipLaunchKernelGGL(k_fcg_init<BS>, dim3(gb), dim3(TPB), 0, c.stream, nb, c.d_r.as<double>(),
@@ -888,6 +917,7 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
                          c.d_zs.as<float>(), c.d_ps.as<float>(), c.d_bjinv.as<float>(), sc, part,
                          cnt);
       PZG_HIP(hipGetLastError());
+      if (pen_rows) PZG_TRY(pen_zero32(c, c.d_rs.as<float>()));
       if (multigrid)
         PZG_TRY(mg_apply(c, c.d_rs.as<float>(), c.d_zs.as<float>(), c.d_ps.as<float>(), nullptr, sc, part));
       if (use_mf) PZG_TRY(mf_scale(c, c.d_ps.as<float>(), sc));
@@ -908,6 +938,9 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
                          c.d_xs.as<float>(), c.d_rs.as<float>(), c.d_zs.as<float>(),
                          c.d_ps.as<float>(), c.d_qs.as<float>(), c.d_bjinv.as<float>(), sc, part,
                          cnt);
+      // S couples the penalty rows to the others (at the 1/sqrt(big) level):
+      // keep their residual at zero rather than let it drift
+      if (pen_rows) (void)pen_zero32(c, c.d_rs.as<float>());
       if (multigrid)
         (void)mg_apply(c, c.d_rs.as<float>(), c.d_zs.as<float>(), nullptr, c.d_qs.as<float>(), sc, part);
       hipLaunchKernelGGL(k_fin_fcg_beta, dim3(1), dim3(TPB), 0, c.stream, sc, (const double *)part, gb);
@@ -922,9 +955,15 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
                        c.d_dinvsqrt.as<double>(), c.d_u.as<double>());
     PZG_HIP(hipGetLastError());
     PZG_TRY(true_res());
+    const double rel_fold = rel;
+    PZG_TRY(pen_fix());
     outer++;
+    if (trace)
+      std::fprintf(stderr, "[pzg] refinement %d: FP32 iterations %lld (stop %d), true residual %.3e"
+                   " after the fold, %.3e after the penalty rows (%lld CG iterations)\n",
+                   outer, (long long)total_it, hs.done, rel_fold, rel, (long long)pen_its);
     if (rel <= prm.tol) { conv = true; break; }
-    if (hs.done == 2) break;
+    if (hs.done == 2 && total_it >= prm.max_iter) break;
     // Stall of the FP32 iterations (the true residual no longer halves over
     // three refinement steps, or the refinement budget is spent; with the
     // learned preconditioner already when one step gains less than 10%):
@@ -934,6 +973,7 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
     if (stall >= 3 || outer >= prm.max_outer ||
         (learned && rel > 0.9 * prev_rel)) {
       fallback = 1;
+      if (trace) std::fprintf(stderr, "[pzg] stalled: FP64 Jacobi-PCG from residual %.3e\n", rel);
       PZG_TRY(pcg64_continue(c, prm, total_it, rel));
       conv = rel <= prm.tol;
       break;
@@ -943,6 +983,7 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
     hipLaunchKernelGGL(k_fcg_reset, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_r.as<double>(),
                        c.d_dinvsqrt.as<double>(), c.d_rs.as<float>(), next_delta(), sc, part, cnt);
     PZG_HIP(hipGetLastError());
+    if (pen_rows) PZG_TRY(pen_zero32(c, c.d_rs.as<float>()));
   }
   PZG_HIP(hipEventRecord(c.ev1, c.stream));
   PZG_HIP(hipEventSynchronize(c.ev1));
