@@ -302,6 +302,18 @@ int pzg_solve(pzg_ctx *ctx, const pzg_solve_params *params, const double *rhs, d
 /* FP64 true relative residual ||rhs - K u|| / ||rhs|| of the device solution. */
 int pzg_true_residual(pzg_ctx *ctx, double *rel);
 
+/* pzg_true_residual without any allocation: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  The FP64 block
+ * CSR and the solution must already be on the device (after a solve,
+ * pzg_multadd or pzg_true_residual of the current assembly), else
+ * PZG_ESTATE.  Reduces in a fixed per-GPU static scratch of 32.8 KB (2 x 2048
+ * partial sums, the two norms, a counter), set up by pzg_create and held
+ * until the process ends; any system size fits (the residual grid is capped
+ * at 2048 workgroups).  Every context on one GPU shares that scratch: calls
+ * that overlap on one GPU (different streams or threads) race. */
+int pzg_true_residual_noalloc(pzg_ctx *ctx, double *rel);
+
 /* Average device duration (HIP events on the context stream) of nrep launches
  * of the SpMV kernel of the given format: 0 = FP64 full CSR (plain PCG path),
  * 1 = FP32 block CSR of the mixed path, 2 = element assembly (whole pass),

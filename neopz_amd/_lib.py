@@ -27,7 +27,7 @@ EXPORTS = [
     "pzg_learned_loss_grad", "pzg_scaled_apply", "pzg_mg_apply", "pzg_full_csr", "pzg_hp_tuning",
     "pzg_load_solution", "pzg_error_points", "pzg_evaluate_error", "pzg_sloan_renumber",
     "pzg_get_solution", "pzg_set_equation_filter", "pzg_learned_scales",
-    "pzg_learned_apply",
+    "pzg_learned_apply", "pzg_true_residual_noalloc",
 ]
 
 dp = C.POINTER(C.c_double)
@@ -120,6 +120,7 @@ def load():
     L.pzg_multadd.argtypes = [vp, dp, dp, dp, C.c_double, C.c_double]
     L.pzg_solve.argtypes = [vp, C.POINTER(SolveParams), dp, dp, C.POINTER(SolveInfo)]
     L.pzg_true_residual.argtypes = [vp, dp]
+    L.pzg_true_residual_noalloc.argtypes = [vp, dp]
     L.pzg_time_kernel.argtypes = [vp, C.c_int32, C.c_int32, dp, dp]
     L.pzg_get_sizes.argtypes = [vp, C.POINTER(Sizes)]
     L.pzg_sync.argtypes = [vp]

@@ -965,6 +965,7 @@ int pzg_create(int device, pzg_ctx **out) {
     return PZG_EINVAL;
   }
   PZG_HIP(hipSetDevice(device));
+  PZG_TRY(noalloc_scratch_setup(device));
   pzg_ctx *p = new pzg_ctx();
   p->c.device = device;
   hipError_t e = hipStreamCreateWithFlags(&p->c.stream, hipStreamNonBlocking);
@@ -1434,6 +1435,17 @@ int pzg_true_residual(pzg_ctx *ctx, double *rel) {
     c.f_valid = true;
   }
   return true_residual(c, c.d_u.as<double>(), rel);
+}
+
+int pzg_true_residual_noalloc(pzg_ctx *ctx, double *rel) {
+  CTX_CHECK();
+  Ctx &c = ctx->c;
+  if (!c.has_solution || !rel) {
+    set_error(!rel ? "pzg_true_residual_noalloc: null output"
+                   : "pzg_true_residual_noalloc: no solution on the device");
+    return !rel ? PZG_EINVAL : PZG_ESTATE;
+  }
+  return true_residual_noalloc(c, rel);
 }
 
 int pzg_load_solution(pzg_ctx *ctx, const double *u) {

@@ -430,6 +430,9 @@ inline bool mf_available(const Ctx &c) {
 int solve_pcg64(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info);
 int solve_mixed(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info);
 int true_residual(Ctx &c, const double *u, double *rel);
+// allocation-free variant on the per-device static scratch (pzg_solve.hip)
+int noalloc_scratch_setup(int device);
+int true_residual_noalloc(Ctx &c, double *rel);
 // post-processing of the device solution (pzg_post.hip)
 int post_points(Ctx &c, int porder, int64_t *npts, double *xq);
 int post_errors(Ctx &c, const pzg_exact &ex, double *el_errors, double *errors, int *nerr);

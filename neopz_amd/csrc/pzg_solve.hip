@@ -30,6 +30,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "pzg_device.h"
@@ -441,6 +442,79 @@ int true_residual(Ctx &c, const double *u, double *rel) {
   PZG_HIP(hipGetLastError());
   double h[2];
   PZG_HIP(hipMemcpyAsync(h, c.d_res2.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+  PZG_HIP(hipStreamSynchronize(c.stream));
+  *rel = h[1] > 0 ? std::sqrt(h[0] / h[1]) : std::sqrt(h[0]);
+  return PZG_OK;
+}
+
+// ---- allocation-free true residual ---------------------------------------
+// Fixed device scratch of pzg_true_residual_noalloc, one copy per GPU, held
+// until the process ends: the reduction partials (2 x MAXBLK doubles, one
+// slot per workgroup of the capped residual grid), the two sums and the
+// arrival counter.  32.8 KB whatever the system size, because the residual
+// kernel strides over block rows with at most MAXBLK workgroups.
+constexpr int NOALLOC_PART = 2 * MAXBLK;           // partials, doubles
+constexpr int NOALLOC_OUT = NOALLOC_PART;          // |r|^2, |f|^2
+constexpr int NOALLOC_CNT = NOALLOC_PART + 4;      // counter (as unsigned)
+constexpr int NOALLOC_WORDS = NOALLOC_PART + 8;
+static __device__ __align__(256) double scratch[NOALLOC_WORDS];
+
+constexpr int NOALLOC_MAXDEV = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[NOALLOC_MAXDEV] = {};
+
+// Looks up this device's address of 'scratch' (the host-side name is only a
+// stand-in).  Called by pzg_create with the context's device current.
+int noalloc_scratch_setup(int device) {
+  if (device < 0 || device >= NOALLOC_MAXDEV) {
+    set_error("pzg_create: device index beyond the allocation-free scratch table");
+    return PZG_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return PZG_OK;
+  void *p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    return PZG_EHIP;
+  }
+  g_scratch_ptr[device] = static_cast<double *>(p);
+  return PZG_OK;
+}
+
+__global__ void k_noalloc_clear(double *s) {
+  if (threadIdx.x < 2) s[NOALLOC_OUT + threadIdx.x] = 0.;
+  if (threadIdx.x == 0) *reinterpret_cast<unsigned *>(s + NOALLOC_CNT) = 0u;
+}
+
+// The residual of true_residual on the full block CSR already on the device,
+// reducing in the per-device scratch: no allocation of any kind.
+int true_residual_noalloc(Ctx &c, double *rel) {
+  if (!c.full_valid || !c.d_fbia.p || !c.f_valid || !c.d_f.p) {
+    set_error("pzg_true_residual_noalloc: no full block CSR or load of the current system on the "
+              "device (call a solve or pzg_true_residual after assembling first)");
+    return PZG_ESTATE;
+  }
+  double *s = (c.device >= 0 && c.device < NOALLOC_MAXDEV) ? g_scratch_ptr[c.device] : nullptr;
+  if (!s) {
+    set_error("pzg_true_residual_noalloc: scratch of this device not set up");
+    return PZG_ESTATE;
+  }
+  const int grid = bsr_grid(c.nb);
+  if (grid > NOALLOC_PART / 2) {
+    set_error("pzg_true_residual_noalloc: residual grid of " + std::to_string(grid) +
+              " workgroups exceeds the scratch's " + std::to_string(NOALLOC_PART / 2));
+    return PZG_EINVAL;
+  }
+  hipLaunchKernelGGL(k_noalloc_clear, dim3(1), dim3(64), 0, c.stream, s);
+  DISPATCH_BS(c.bs, hipLaunchKernelGGL(k_residual64<BS>, dim3(grid), dim3(TPB), 0, c.stream, c.nb,
+                                       c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(),
+                                       c.d_fa.as<double>(), c.d_u.as<double>(),
+                                       c.d_f.as<double>(), nullptr, s + NOALLOC_OUT, s,
+                                       reinterpret_cast<unsigned *>(s + NOALLOC_CNT)));
+  PZG_HIP(hipGetLastError());
+  double h[2];
+  PZG_HIP(hipMemcpyAsync(h, s + NOALLOC_OUT, sizeof(h), hipMemcpyDeviceToHost, c.stream));
   PZG_HIP(hipStreamSynchronize(c.stream));
   *rel = h[1] > 0 ? std::sqrt(h[0] / h[1]) : std::sqrt(h[0]);
   return PZG_OK;
