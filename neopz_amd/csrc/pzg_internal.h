@@ -332,6 +332,7 @@ struct Ctx {
   int64_t pen_n = 0, pen_nnz = 0, pen_nset = 0, pen_nslot = 0;
   DevBuf d_pen_idx, d_pen_pmap, d_pen_ia, d_pen_ja, d_pen_a, d_pen_sptr, d_pen_sidx, d_pen_sset, d_pen_soff,
       d_pen_sval, d_pen_sinv, d_pen_gptr, d_pen_gslot, d_pen_w, d_pen_scal;
+  hipGraphExec_t pen_graph = nullptr;  // one batch of its CG iterations
   bool mg_struct = false;  // lattice, levels and buffers of the current mesh
   bool mg_dense = true;    // the coarsest level has a dense inverse (else a Chebyshev solve)
   int mg_degree = 2;       // Chebyshev degree of the pre- and post-smoothing
@@ -393,14 +394,24 @@ int side_corner_list_pub(int etype, int s, int *loc);
 int les_nparams(int ne);
 int les_setup(Ctx &c, uint64_t seed);
 int les_prepare(Ctx &c);  // per-element scales of the current assembly, apply buffers
-int les_apply(Ctx &c, const float *r, float *z, const SolveScalars *sc);
+// Chebyshev step fused into the learned smoother's gather (d, x, pt, dis:
+// the smoothing's direction, iterate and pre-scaled direction)
+struct LesCheb {
+  int mode;
+  float c1, c2;
+  float *d, *x, *pt;
+  const float *dis;
+};
+// z = M r (learned smoother); with ch, the Chebyshev step with y = M r instead
+int les_apply(Ctx &c, const float *r, float *z, const SolveScalars *sc, const LesCheb *ch = nullptr);
 int les_collect(Ctx &c, int nrhs, uint64_t seed);
 int les_loss_grad(Ctx &c, double *loss_out);
 int les_adam_step(Ctx &c, double lr);
 int les_fit_G(Ctx &c);  // G from the accumulated least-squares sums
 // penalty rows of the learned path (pzg_penalty.hip)
 int pen_setup(Ctx &c);
-int pen_solve(Ctx &c, const double *r64, double *u64, double stop_abs, int64_t *its);
+int pen_solve(Ctx &c, double *r64, double *u64, double stop_abs, bool use_graph, int64_t *its, double *rr0,
+              double *rr1);
 int pen_zero32(Ctx &c, float *r32);
 void pen_free(Ctx &c);
 int mf_setup(Ctx &c);

@@ -436,6 +436,41 @@ __global__ __launch_bounds__(256) void k_les_gather(int64_t nb, const int64_t *_
   }
 }
 
+// The same gather fused with the Chebyshev step of the smoothing that uses it
+// (y = M r never stored): mode 0: d = c2 y, x = d;  1: d = c2 y, x += d;
+// 2: d = c1 d + c2 y, x += d;  pt = D^-1/2 d when given.
+template <int NS>
+__global__ __launch_bounds__(256) void k_les_gather_cheb(int64_t nb, const int64_t *__restrict__ ptr,
+                                                         const int32_t *__restrict__ eoff,
+                                                         const float *__restrict__ ye,
+                                                         const float *__restrict__ pen,
+                                                         const float *__restrict__ r, LesCheb ch,
+                                                         const SolveScalars *sc) {
+  if (sc && sc->done) return;
+  for (int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; A < nb;
+       A += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t0 = ptr[A], t1 = ptr[A + 1];
+    float acc[NS];
+#pragma unroll
+    for (int k = 0; k < NS; k++) acc[k] = 0.f;
+    for (int64_t t = t0; t < t1; t++) {
+      const int32_t o = eoff[t];
+      if (o < 0) continue;
+#pragma unroll
+      for (int k = 0; k < NS; k++) acc[k] += ye[(int64_t)o + k];
+    }
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+      const int64_t i = A * NS + k;
+      const float y = pen[i] != 0.f ? r[i] : acc[k];
+      const float dv = ch.mode == 2 ? ch.c1 * ch.d[i] + ch.c2 * y : ch.c2 * y;
+      ch.d[i] = dv;
+      ch.x[i] = ch.mode == 0 ? dv : ch.x[i] + dv;
+      if (ch.pt) ch.pt[i] = dv * ch.dis[i];
+    }
+  }
+}
+
 // ---- training kernels ------------------------------------------------------------
 // S_e (FP64, ne x ne, row-major) of every element from the FP64 full block
 // CSR of K scaled by D^-1/2; rows / columns of penalized dofs and of absent
@@ -830,7 +865,7 @@ int les_prepare(Ctx &c) {
 }
 
 // z = M r (learned smoother), both FP32 device vectors of neq entries
-int les_apply(Ctx &c, const float *r, float *z, const SolveScalars *sc) {
+int les_apply(Ctx &c, const float *r, float *z, const SolveScalars *sc, const LesCheb *ch) {
   const float *G = c.d_les_th.as<float>() + les_offsets(c.les_ne).g;
   const int nsh = c.tvol.nsh;
   const int32_t *blk = c.d_mf_blk.as<int32_t>(), *pos = c.d_mf_pos.as<int32_t>();
@@ -856,7 +891,14 @@ int les_apply(Ctx &c, const float *r, float *z, const SolveScalars *sc) {
     return PZG_EUNSUP;
   }
   const int gg = grid1(c.nb);
-  if (c.bs == 3)
+  if (ch) {
+    if (c.bs == 3)
+      hipLaunchKernelGGL(k_les_gather_cheb<3>, dim3(gg), dim3(256), 0, c.stream, c.nb, c.d_mf_ptr.as<int64_t>(), eoff,
+                         ye, pen, r, *ch, sc);
+    else
+      hipLaunchKernelGGL(k_les_gather_cheb<1>, dim3(gg), dim3(256), 0, c.stream, c.nb, c.d_mf_ptr.as<int64_t>(), eoff,
+                         ye, pen, r, *ch, sc);
+  } else if (c.bs == 3)
     hipLaunchKernelGGL(k_les_gather<3>, dim3(gg), dim3(256), 0, c.stream, c.nb, c.d_mf_ptr.as<int64_t>(), eoff, ye, pen, r, z, sc);
   else
     hipLaunchKernelGGL(k_les_gather<1>, dim3(gg), dim3(256), 0, c.stream, c.nb, c.d_mf_ptr.as<int64_t>(), eoff, ye, pen, r, z, sc);

@@ -151,8 +151,8 @@ __global__ void k_cheb_step(int64_t nb, const float *__restrict__ m, int64_t sa,
 }
 
 // Learned smoother on level 0 (PZG_PRECOND_LEARNED): M r is an operator of
-// its own (les_apply), so a Chebyshev step splits into the residual update,
-// y = M r, and the direction / iterate update.
+// its own (les_apply), so a Chebyshev step splits into the residual update and
+// y = M r, whose gather also applies the direction / iterate update.
 // mode 0: r = b;  1: r -= t;  2: r = b - t
 __global__ void k_les_res(int64_t n, int mode, const float *__restrict__ b, const float *__restrict__ t,
                           float *__restrict__ r, const SolveScalars *sc) {
@@ -160,20 +160,6 @@ __global__ void k_les_res(int64_t n, int mode, const float *__restrict__ b, cons
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     r[i] = mode == 0 ? b[i] : mode == 1 ? r[i] - t[i] : b[i] - t[i];
 }
-// mode 0: d = c2 y, x = d;  1: d = c2 y, x += d;  2: d = c1 d + c2 y, x += d;
-// pt = D^-1/2 d (matrix-free operator input) when given
-__global__ void k_les_cheb(int64_t n, int mode, const float *__restrict__ y, float *__restrict__ d,
-                           float *__restrict__ x, float c1, float c2, const float *__restrict__ dis,
-                           float *__restrict__ pt, const SolveScalars *sc) {
-  if (stopped(sc)) return;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const float dv = mode == 2 ? c1 * d[i] + c2 * y[i] : c2 * y[i];
-    d[i] = dv;
-    x[i] = mode == 0 ? dv : x[i] + dv;
-    if (pt) pt[i] = dv * dis[i];
-  }
-}
-
 // ---- lattice operators ---------------------------------------------------------
 struct Dims {
   int n[3];
@@ -522,17 +508,13 @@ int smooth_les(Ctx &c, const LevelView &L, const float *b, float *x, Op op, cons
     PZG_TRY(op(x, L.t, false));
     hipLaunchKernelGGL(k_les_res, dim3(g), dim3(TPB), 0, c.stream, n, 2, b, L.t, L.r, sc);
   }
-  PZG_TRY(les_apply(c, L.r, L.y, sc));
-  hipLaunchKernelGGL(k_les_cheb, dim3(g), dim3(TPB), 0, c.stream, n, pre ? 0 : 1, L.y, L.d, x, 0.f, L.ch.c0,
-                     L.dis, L.pt, sc);
-  PZG_HIP(hipGetLastError());
+  LesCheb ch{pre ? 0 : 1, 0.f, (float)L.ch.c0, L.d, x, L.pt, L.dis};
+  PZG_TRY(les_apply(c, L.r, nullptr, sc, &ch));
   for (size_t j = 0; j < L.ch.c1.size(); j++) {
     PZG_TRY(op(L.d, L.t, ps));
     hipLaunchKernelGGL(k_les_res, dim3(g), dim3(TPB), 0, c.stream, n, 1, b, L.t, L.r, sc);
-    PZG_TRY(les_apply(c, L.r, L.y, sc));
-    hipLaunchKernelGGL(k_les_cheb, dim3(g), dim3(TPB), 0, c.stream, n, 2, L.y, L.d, x, L.ch.c1[j], L.ch.c2[j],
-                       L.dis, L.pt, sc);
-    PZG_HIP(hipGetLastError());
+    LesCheb chj{2, (float)L.ch.c1[j], (float)L.ch.c2[j], L.d, x, L.pt, L.dis};
+    PZG_TRY(les_apply(c, L.r, nullptr, sc, &chj));
   }
   return pre ? op(L.d, L.t, ps) : PZG_OK;
 }

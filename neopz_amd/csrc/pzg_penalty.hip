@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <string>
 #include <vector>
 
 #include "pzg_device.h"
@@ -35,7 +36,7 @@ constexpr int PEN_BATCH = 16;    // CG iterations between host checks
 constexpr int PEN_MAXIT = 1000;  // CG iterations per solve at most
 
 struct PenScal {
-  double rz, pq, alpha, beta, rr, stop2;
+  double rz, pq, alpha, beta, rr, stop2, rr0;
   int64_t it;
   int done;
 };
@@ -150,6 +151,7 @@ __global__ void k_pen_start(PenScal *sc, const double *partials, int nblk, doubl
   const double rr = sum_partials(partials, 0, nblk);
   if (threadIdx.x == 0) {
     sc->rr = rr;
+    sc->rr0 = rr;
     sc->stop2 = std::max(stop_abs * stop_abs, 1e-30 * rr);
     sc->it = 0;
     sc->rz = 0.;
@@ -271,11 +273,13 @@ __global__ void k_pen_check(PenScal *sc, const double *partials, int nblk, int64
   }
 }
 
-// u[idx] += x
+// u[idx] += x; r64[idx] = r (the penalty rows' residual after the solve)
 __global__ void k_pen_scatter(int64_t np, const int64_t *__restrict__ idx, const double *__restrict__ x,
-                              double *__restrict__ u) {
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < np; k += (int64_t)gridDim.x * blockDim.x)
+                              const double *__restrict__ r, double *__restrict__ u, double *__restrict__ r64) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < np; k += (int64_t)gridDim.x * blockDim.x) {
     u[idx[k]] += x[k];
+    r64[idx[k]] = r[k];
+  }
 }
 
 // r (float, scaled residual of the inner iterations) = 0 on the penalty rows
@@ -463,11 +467,16 @@ int pen_setup(Ctx &c) {
   return PZG_OK;
 }
 
-// u_p += K_pp^-1 r_p (r: the FP64 residual f - K u of the whole system), by
-// Schwarz-preconditioned CG until ||r_p - K_pp d|| <= stop_abs.  *its: the CG
-// iterations taken.
-int pen_solve(Ctx &c, const double *r64, double *u64, double stop_abs, int64_t *its) {
+// u_p += d = K_pp^-1 r_p (r64: the FP64 residual f - K u of the whole system),
+// by Schwarz-preconditioned CG until ||r_p - K_pp d|| <= stop_abs.  r64's
+// penalty rows receive r_p - K_pp d; its other rows change by -K_ip d, which
+// is left to the next true residual (|K_ip d| ~ |r_p| E/(big h)).  *its: the
+// CG iterations taken; *rr0, *rr1: |r_p|^2 before and after.
+int pen_solve(Ctx &c, double *r64, double *u64, double stop_abs, bool use_graph, int64_t *its, double *rr0,
+              double *rr1) {
   if (its) *its = 0;
+  if (rr0) *rr0 = 0.;
+  if (rr1) *rr1 = 0.;
   if (!c.pen_ready) {
     set_error("penalty rows: not set up");
     return PZG_ESTATE;
@@ -497,7 +506,7 @@ int pen_solve(Ctx &c, const double *r64, double *u64, double stop_abs, int64_t *
   precond(1);
   PZG_HIP(hipGetLastError());
   PenScal hs{};
-  for (int64_t done_it = 0; done_it < PEN_MAXIT; done_it += PEN_BATCH) {
+  auto batch = [&]() {
     for (int k = 0; k < PEN_BATCH; k++) {
       hipLaunchKernelGGL(k_pen_spmv, dim3(g), dim3(TPB), 0, s, np, c.d_pen_ia.as<int64_t>(),
                          c.d_pen_ja.as<int32_t>(), c.d_pen_a.as<double>(), (const double *)p, q,
@@ -508,6 +517,25 @@ int pen_solve(Ctx &c, const double *r64, double *u64, double stop_abs, int64_t *
       hipLaunchKernelGGL(k_pen_check, dim3(1), dim3(TPB), 0, s, sc, (const double *)part, g, (int64_t)PEN_MAXIT);
       precond(0);
     }
+  };
+  // a batch is one graph, captured once per mesh (its buffers live as long)
+  if (use_graph && !c.pen_graph) {
+    hipGraph_t gr = nullptr;
+    PZG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    batch();
+    PZG_HIP(hipStreamEndCapture(s, &gr));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) {
+      set_error(std::string("penalty rows: hipGraphInstantiate: ") + hipGetErrorString(e));
+      return PZG_EHIP;
+    }
+    c.pen_graph = ex;
+  }
+  for (int64_t done_it = 0; done_it < PEN_MAXIT; done_it += PEN_BATCH) {
+    if (use_graph) PZG_HIP(hipGraphLaunch(c.pen_graph, s));
+    else batch();
     PZG_HIP(hipGetLastError());
     PZG_HIP(hipMemcpyAsync(&hs, sc, sizeof(hs), hipMemcpyDeviceToHost, s));
     PZG_HIP(hipStreamSynchronize(s));
@@ -517,9 +545,14 @@ int pen_solve(Ctx &c, const double *r64, double *u64, double stop_abs, int64_t *
     set_error("penalty rows: K_pp not positive definite");
     return PZG_EUNSUP;
   }
-  hipLaunchKernelGGL(k_pen_scatter, dim3(g), dim3(TPB), 0, s, np, idx, (const double *)x, u64);
-  PZG_HIP(hipGetLastError());
+  if (hs.it > 0) {
+    hipLaunchKernelGGL(k_pen_scatter, dim3(g), dim3(TPB), 0, s, np, idx, (const double *)x, (const double *)r, u64,
+                       r64);
+    PZG_HIP(hipGetLastError());
+  }
   if (its) *its = hs.it;
+  if (rr0) *rr0 = hs.rr0;
+  if (rr1) *rr1 = hs.rr;
   return PZG_OK;
 }
 
@@ -537,6 +570,8 @@ void pen_free(Ctx &c) {
                     &c.d_pen_sidx, &c.d_pen_sset, &c.d_pen_soff, &c.d_pen_sval, &c.d_pen_sinv, &c.d_pen_gptr,
                     &c.d_pen_gslot, &c.d_pen_w, &c.d_pen_scal})
     dev_free(*b);
+  if (c.pen_graph) (void)hipGraphExecDestroy(c.pen_graph);
+  c.pen_graph = nullptr;
   c.pen_ready = c.pen_struct = false;
   c.pen_n = c.pen_nnz = c.pen_nset = c.pen_nslot = 0;
 }

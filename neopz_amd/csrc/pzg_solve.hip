@@ -963,15 +963,36 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
   // learned path: the penalty rows are solved in FP64 here, after every fold,
   // to a tenth of the target residual; the FP32 iterations leave them alone
   const bool pen_rows = learned && !conv && c.pen_ready && c.pen_n > 0;
+  // After the penalty solve the residual's norm is known up to the -K_ip d
+  // change of the other rows (see pen_solve); rel is then that estimate, and a
+  // true residual confirms it before the solve is called converged.
   int64_t pen_its = 0;
+  bool rel_exact = true;
   auto pen_fix = [&]() -> int {
     pen_its = 0;
     if (!pen_rows || rel <= prm.tol) return PZG_OK;
-    PZG_TRY(pen_solve(c, c.d_r.as<double>(), c.d_u.as<double>(), 0.1 * prm.tol * fnorm, &pen_its));
-    return true_res();
+    double p0 = 0., p1 = 0.;
+    PZG_TRY(pen_solve(c, c.d_r.as<double>(), c.d_u.as<double>(), 0.1 * prm.tol * fnorm, prm.use_graph != 0,
+                      &pen_its, &p0, &p1));
+    if (pen_its == 0) return PZG_OK;  // u unchanged
+    const double rr = rel * rel * fnorm * fnorm - p0 + p1;
+    rel = std::sqrt(std::max(rr, p1)) / (fnorm > 0 ? fnorm : 1.);
+    rel_exact = false;
+    return PZG_OK;
+  };
+  // the test on the estimate, confirmed by a true residual
+  auto converged = [&](bool &cv) -> int {
+    cv = false;
+    if (rel > prm.tol) return PZG_OK;
+    if (!rel_exact) {
+      PZG_TRY(true_res());
+      rel_exact = true;
+    }
+    cv = rel <= prm.tol;
+    return PZG_OK;
   };
   PZG_TRY(pen_fix());
-  if (rel <= prm.tol) conv = true;
+  PZG_TRY(converged(conv));
   int fallback = 0, stall = 0;
   double prev_rel = rel;
   SolveScalars hs;
@@ -1029,6 +1050,7 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
                        c.d_dinvsqrt.as<double>(), c.d_u.as<double>());
     PZG_HIP(hipGetLastError());
     PZG_TRY(true_res());
+    rel_exact = true;
     const double rel_fold = rel;
     PZG_TRY(pen_fix());
     outer++;
@@ -1036,7 +1058,8 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
       std::fprintf(stderr, "[pzg] refinement %d: FP32 iterations %lld (stop %d), true residual %.3e"
                    " after the fold, %.3e after the penalty rows (%lld CG iterations)\n",
                    outer, (long long)total_it, hs.done, rel_fold, rel, (long long)pen_its);
-    if (rel <= prm.tol) { conv = true; break; }
+    PZG_TRY(converged(conv));
+    if (conv) break;
     if (hs.done == 2 && total_it >= prm.max_iter) break;
     // Stall of the FP32 iterations (the true residual no longer halves over
     // three refinement steps, or the refinement budget is spent; with the
@@ -1047,6 +1070,10 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
     if (stall >= 3 || outer >= prm.max_outer ||
         (learned && rel > 0.9 * prev_rel)) {
       fallback = 1;
+      if (!rel_exact) {  // the PCG restarts from d_r: make it the true residual
+        PZG_TRY(true_res());
+        rel_exact = true;
+      }
       if (trace) std::fprintf(stderr, "[pzg] stalled: FP64 Jacobi-PCG from residual %.3e\n", rel);
       PZG_TRY(pcg64_continue(c, prm, total_it, rel));
       conv = rel <= prm.tol;
@@ -1059,6 +1086,7 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
     PZG_HIP(hipGetLastError());
     if (pen_rows) PZG_TRY(pen_zero32(c, c.d_rs.as<float>()));
   }
+  if (!rel_exact) PZG_TRY(true_res());  // the reported residual is a true one
   PZG_HIP(hipEventRecord(c.ev1, c.stream));
   PZG_HIP(hipEventSynchronize(c.ev1));
   float ms = 0;
