@@ -115,6 +115,9 @@ def lib():
         L.orc_assemble_filtered.argtypes = [P, M, i64p, i64p, i64p, dp, dp]
         L.orc_sloan.argtypes = [C.c_int64, i64p, i64p, i64p]
         L.orc_mesh_sloan.argtypes = [M, i64p]
+        L.orc_elems_touching.argtypes = [M, C.c_void_p, i64p, C.c_int64]
+        L.orc_elems_touching.restype = C.c_int64
+        L.orc_assemble_rows.argtypes = [P, M, i64p, i64p, i64p, dp, dp, i64p, C.c_int64, C.c_int]
         L.orc_nnz_closed_form.argtypes = [C.c_int] * 6
         L.orc_nnz_closed_form.restype = C.c_int64
         L.free = C.CDLL(None).free
@@ -467,6 +470,34 @@ class Oracle:
         lib().orc_assemble_range(C.byref(self.pb), C.byref(self.m), _ip(ia), _ip(ja), _dp(a),
                                  _dp(rhs), e0, e1)
         return a, rhs
+
+
+def assemble_rows(o, rows, ia_s, ja_s, nthreads=8):
+    """The oracle's assembled upper-CSR entries and load of the given rows
+    (ascending, unique) over the pattern ia_s/ja_s of those rows: only the
+    elements that touch them are integrated, and every entry receives the
+    same terms in the same (element) order as in :meth:`Oracle.assemble`."""
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    mask = np.zeros(o.neq, dtype=np.uint8)
+    mask[rows] = 1
+    cap = o.m.nel
+    elems = np.zeros(max(cap, 1), dtype=np.int64)
+    ne = lib().orc_elems_touching(C.byref(o.m), mask.ctypes.data_as(C.c_void_p), _ip(elems), cap)
+    if ne < 0:
+        raise RuntimeError("element list overflow")
+    rowmap = np.full(o.neq, -1, dtype=np.int64)
+    rowmap[rows] = np.arange(len(rows))
+    ia_s = np.ascontiguousarray(ia_s, dtype=np.int64)
+    ja_s = np.ascontiguousarray(ja_s, dtype=np.int64)
+    a_s = np.zeros(max(len(ja_s), 1))
+    rhs_s = np.zeros(len(rows))
+    rc = lib().orc_assemble_rows(C.byref(o.pb), C.byref(o.m), _ip(rowmap), _ip(ia_s), _ip(ja_s),
+                                 _dp(a_s), _dp(rhs_s), _ip(elems), ne, nthreads)
+    if rc == -1:
+        raise ValueError("an element entry's column is missing from the given row pattern")
+    if rc != 0:
+        raise RuntimeError("oracle CalcStiff failed")
+    return a_s[: len(ja_s)], rhs_s, int(ne)
 
 
 def addkel(ia, ja, a, ek, dest):

@@ -1612,6 +1612,78 @@ int orc_assemble_range(const orc_problem *pb, const orc_mesh *m, const int64_t *
   return 0;
 }
 
+/* Elements (volume and boundary, ascending) whose destination indices hold a
+ * row with rowmask[row] != 0 (ComputeDestinationIndices, Mesh/pzelmat.cpp:37-70,
+ * without integration).  Returns the count written to elems (capacity cap). */
+int64_t orc_elems_touching(const orc_mesh *m, const unsigned char *rowmask, int64_t *elems, int64_t cap) {
+  int64_t n = 0;
+  for (int64_t e = 0; e < m->nel; e++) {
+    int t = m->etype[e], hit = 0;
+    for (int s = 0; s < etype_nsides(t) && !hit; s++) {
+      int64_t seq = m->conn_seq[m->econn[27 * e + s]];
+      for (int k = 0; k < m->blk_size[seq]; k++)
+        if (rowmask[m->blk_pos[seq] + k]) { hit = 1; break; }
+    }
+    if (hit) {
+      if (n >= cap) return -1;
+      elems[n++] = e;
+    }
+  }
+  return n;
+}
+
+/* AddKel / AddFel (the drop rule of orc_addkel) restricted to the rows with
+ * rowmap[row] >= 0, whose upper-CSR rows are given as a sub-CSR (ia_s over the
+ * local rows, ja_s the global columns, ascending), over the elements elems[]
+ * in ascending order: every entry of these rows gets the same terms in the same
+ * order as in orc_assemble.  Element matrices are integrated in parallel in
+ * chunks and added in element order.  Returns -1 when a column is missing from
+ * its row. */
+int orc_assemble_rows(const orc_problem *pb, const orc_mesh *m, const int64_t *rowmap,
+                      const int64_t *ia_s, const int64_t *ja_s, double *a_s, double *rhs_s,
+                      const int64_t *elems, int64_t nel, int nthreads) {
+  enum { CH = 256 };
+  double *ek = (double *)malloc(sizeof(double) * (size_t)CH * ORC_MAXNE * ORC_MAXNE);
+  double *ef = (double *)malloc(sizeof(double) * (size_t)CH * ORC_MAXNE);
+  int64_t *dest = (int64_t *)malloc(sizeof(int64_t) * (size_t)CH * ORC_MAXNE);
+  int ne[CH];
+  int rc = 0;
+  for (int64_t c0 = 0; c0 < nel && rc == 0; c0 += CH) {
+    const int nc = (int)(nel - c0 < CH ? nel - c0 : CH);
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 4)
+    for (int q = 0; q < nc; q++)
+      ne[q] = orc_calcstiff(pb, m, elems[c0 + q], ek + (size_t)q * ORC_MAXNE * ORC_MAXNE,
+                            ef + (size_t)q * ORC_MAXNE, dest + (size_t)q * ORC_MAXNE);
+    for (int q = 0; q < nc && rc == 0; q++) {
+      const int n = ne[q];
+      if (n < 0) { rc = -2; break; }
+      const double *K = ek + (size_t)q * ORC_MAXNE * ORC_MAXNE;
+      const int64_t *d = dest + (size_t)q * ORC_MAXNE;
+      for (int i = 0; i < n && rc == 0; i++) {
+        const int64_t lr = rowmap[d[i]];
+        if (lr < 0) continue;
+        rhs_s[lr] += ef[(size_t)q * ORC_MAXNE + i];
+        for (int j = 0; j < n; j++) {
+          if (d[j] < d[i]) continue;
+          const double v = K[i * n + j];
+          if (fabs(v) < 1.e-12) continue;
+          int64_t lo = ia_s[lr], hi = ia_s[lr + 1];
+          while (lo < hi) {
+            const int64_t mid = (lo + hi) / 2;
+            if (ja_s[mid] < d[j]) lo = mid + 1; else hi = mid;
+          }
+          if (lo == ia_s[lr + 1] || ja_s[lo] != d[j]) { rc = -1; break; }
+          a_s[lo] += v;
+        }
+      }
+    }
+  }
+  free(ek);
+  free(ef);
+  free(dest);
+  return rc;
+}
+
 /* Serial_Assemble (StrMatrix/pzstrmatrixor.cpp:104-365), element order. */
 /* The element loop of TPZStructMatrixOT (StrMatrix/pzstrmatrixot.cpp:
  * 92-553): elements are coloured so that no two of a colour share a connect

@@ -155,6 +155,16 @@ int orc_assemble_range(const orc_problem *pb, const orc_mesh *m,
                        const int64_t *ia, const int64_t *ja, double *a,
                        double *rhs, int64_t e0, int64_t e1);
 
+/* Elements whose destination indices hold a row of rowmask (ascending);
+ * returns the count or -1 beyond cap. */
+int64_t orc_elems_touching(const orc_mesh *m, const unsigned char *rowmask, int64_t *elems, int64_t cap);
+
+/* orc_assemble restricted to the rows with rowmap[row] >= 0 (sub-CSR ia_s /
+ * ja_s of those rows), over the given ascending elements; -1: missing column. */
+int orc_assemble_rows(const orc_problem *pb, const orc_mesh *m, const int64_t *rowmap,
+                      const int64_t *ia_s, const int64_t *ja_s, double *a_s, double *rhs_s,
+                      const int64_t *elems, int64_t nel, int nthreads);
+
 /* AddKel of a caller-given element matrix (row-major ne x ne). */
 void orc_addkel(const int64_t *ia, const int64_t *ja, double *a, int ne,
                 const double *ek, const int64_t *dest);

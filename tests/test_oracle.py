@@ -907,3 +907,30 @@ def test_full_pattern_and_values(kind):
     scale = np.abs(K).max(axis=1, keepdims=True)
     assert np.all(np.abs(F - K) <= 1e-13 * scale)
     assert np.array_equal(frhs, rhs)
+
+
+def test_assemble_rows_equals_whole_assembly():
+    """oracle.assemble_rows (the elements touching the given rows only) gives
+    those rows' entries and loads bit for bit as the whole assembly does."""
+    for pb in (P.elasticity3d(n=4, p=2, seed=1, field_kind="grf", load="body"),
+               P.darcy3d_tet(n=2, pmax=4, seed=3), P.poisson2d(n=6, p=2, seed=2)):
+        o = O.Oracle(pb)
+        ia, ja = o.pattern()
+        a, rhs = o.assemble()
+        rows = np.sort(np.random.default_rng(0).choice(o.neq, min(150, o.neq // 2), replace=False))
+        lens = ia[rows + 1] - ia[rows]
+        ia_s = np.concatenate([[0], np.cumsum(lens)])
+        idx = np.repeat(ia[rows] - ia_s[:-1], lens) + np.arange(ia_s[-1])
+        a_s, rhs_s, nel = O.assemble_rows(o, rows, ia_s, ja[idx])
+        assert 0 < nel <= o.m.nel
+        assert np.array_equal(a_s, a[idx]) and np.array_equal(rhs_s, rhs[rows])
+        # the rows of one element touch only its neighbours
+        r1 = np.unique(o.calcstiff(0)[2])
+        l1 = ia[r1 + 1] - ia[r1]
+        i1 = np.concatenate([[0], np.cumsum(l1)])
+        x1 = np.repeat(ia[r1] - i1[:-1], l1) + np.arange(i1[-1])
+        a1, f1, n1 = O.assemble_rows(o, r1, i1, ja[x1])
+        assert n1 < o.m.nel and np.array_equal(a1, a[x1]) and np.array_equal(f1, rhs[r1])
+        # a column outside the given row pattern is reported
+        with pytest.raises(ValueError):
+            O.assemble_rows(o, rows, ia_s, ja[idx] + 10 ** 9)
