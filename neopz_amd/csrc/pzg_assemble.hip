@@ -375,6 +375,199 @@ __global__ __launch_bounds__(TPB) void k_vol(ElemArgs g, const int32_t *__restri
   }
 }
 
+// Affine volume elements (parallelepipeds / parallelograms: constant
+// Jacobian), several per workgroup.  With R_mn(i,j) = sum_q w_q dphi_m(i)
+// dphi_n(j) the reference Gram matrices of the rule (the same for every
+// element, staged in LDS) the gradient Gram blocks are
+//     P^{ab}(i,j) = |det J| sum_{m,n} Jinv[m][a] Jinv[n][b] R_mn(i,j),
+// exactly the quadrature sum of k_vol for a constant Jacobian (rounding
+// aside), so an element costs 81 FMA per shape pair instead of a pass over
+// its quadrature points.  One thread per (element, shape pair) of the upper
+// part builds the pair's nstate x nstate block and adds it with AddKel's drop
+// rule; the load vector keeps the quadrature (the body load varies).
+constexpr int AFF_EPB = 4;  // elements per workgroup
+template <int DIM, int NC, int NSD, int NSH, int NS>
+__global__ __launch_bounds__(TPB) void k_vol_aff(ElemArgs g, const double *__restrict__ R,
+                                                 const int32_t *__restrict__ list, int64_t n) {
+  constexpr int NR = DIM * (DIM + 1) / 2;  // R_mn, m <= n
+  constexpr int NE = NSH * NS;
+  __shared__ double s_R[NR][NSH][NSH];
+  __shared__ double s_jinv[AFF_EPB][DIM][DIM];
+  __shared__ double s_det[AFF_EPB];
+  __shared__ double s_coef[AFF_EPB];
+  __shared__ double s_coord[AFF_EPB][NC * 3];
+  __shared__ double s_f[AFF_EPB][MAXQ][3];
+  __shared__ int64_t s_blk[AFF_EPB][NSH];
+  __shared__ int64_t s_row[AFF_EPB][NE];
+  __shared__ int32_t s_map[AFF_EPB][NSD * NSD];
+  const int tid = threadIdx.x;
+  const ElemTables &T = *g.tab;
+  const int nq = T.nq;
+  const int64_t e0 = (int64_t)blockIdx.x * AFF_EPB;
+  const int nel = (int)(n - e0 < AFF_EPB ? n - e0 : AFF_EPB);
+  for (int t = tid; t < NR * NSH * NSH; t += TPB) (&s_R[0][0][0])[t] = R[t];
+  for (int t = tid; t < nel * NC * 3; t += TPB) {
+    const int el = t / (NC * 3), k = t % (NC * 3);
+    const int64_t e = list[e0 + el];
+    s_coord[el][k] = g.xyz[3 * g.nodes[e * NC + k / 3] + k % 3];
+  }
+  for (int t = tid; t < nel * NSD * NSD; t += TPB) {
+    const int el = t / (NSD * NSD), k = t % (NSD * NSD);
+    s_map[el][k] = g.map[(int64_t)list[e0 + el] * NSD * NSD + k];
+  }
+  for (int t = tid; t < nel * NSH; t += TPB) {
+    const int el = t / NSH, k = t % NSH;
+    s_blk[el][k] = g.aidx[g.conn[(int64_t)list[e0 + el] * NSD + T.shape_side[k]]];
+  }
+  __syncthreads();
+  // the constant Jacobian (TPZGeoEl::Jacobian at the first point) and the
+  // row starts of the element's equations in the upper CSR
+  if (tid < nel) {
+    const int el = tid;
+    const int64_t e = list[e0 + el];
+    double gx[3][DIM];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < DIM; j++) {
+        double v = 0.;
+        for (int c = 0; c < NC; c++) v += s_coord[el][3 * c + i] * T.cdphi[0][j][c];
+        gx[i][j] = v;
+      }
+    double det;
+    if (DIM == 3) {
+      const double(*j)[DIM] = gx;
+      det = -j[0][2] * j[1][1] * j[2][0] + j[0][1] * j[1][2] * j[2][0] + j[0][2] * j[1][0] * j[2][1] -
+            j[0][0] * j[1][2] * j[2][1] - j[0][1] * j[1][0] * j[2][2] + j[0][0] * j[1][1] * j[2][2];
+      double inv[3][3];
+      inv[0][0] = (-j[1][2] * j[2][1] + j[1][1] * j[2][2]) / det;
+      inv[0][1] = (j[0][2] * j[2][1] - j[0][1] * j[2][2]) / det;
+      inv[0][2] = (-j[0][2] * j[1][1] + j[0][1] * j[1][2]) / det;
+      inv[1][0] = (j[1][2] * j[2][0] - j[1][0] * j[2][2]) / det;
+      inv[1][1] = (-j[0][2] * j[2][0] + j[0][0] * j[2][2]) / det;
+      inv[1][2] = (j[0][2] * j[1][0] - j[0][0] * j[1][2]) / det;
+      inv[2][0] = (-j[1][1] * j[2][0] + j[1][0] * j[2][1]) / det;
+      inv[2][1] = (j[0][1] * j[2][0] - j[0][0] * j[2][1]) / det;
+      inv[2][2] = (-j[0][1] * j[1][0] + j[0][0] * j[1][1]) / det;
+      for (int a = 0; a < DIM; a++)
+        for (int b = 0; b < DIM; b++) s_jinv[el][a][b] = inv[a][b];
+    } else {
+      double n1 = 0, d12 = 0, n2 = 0;
+      for (int i = 0; i < 3; i++) { n1 += gx[i][0] * gx[i][0]; d12 += gx[i][0] * gx[i][1]; }
+      n1 = sqrt(n1);
+      for (int i = 0; i < 3; i++) {
+        const double v2 = gx[i][1] - d12 * (gx[i][0] / n1) / n1;
+        n2 += v2 * v2;
+      }
+      n2 = sqrt(n2);
+      const double j01 = d12 / n1;
+      det = n1 * n2;
+      s_jinv[el][0][0] = n2 / det;
+      s_jinv[el][1][1] = n1 / det;
+      s_jinv[el][0][1] = -j01 / det;
+      s_jinv[el][1][0] = 0.;
+    }
+    s_det[el] = fabs(det);
+    s_coef[el] = g.coef ? g.coef[e] : (g.material == PZG_MAT_ELAST3D ? g.E : g.scale);
+  }
+  for (int t = tid; t < nel * NE; t += TPB) {
+    const int el = t / NE, k = t % NE;
+    const int64_t b = s_blk[el][k / NS];
+    s_row[el][k] = b >= 0 ? g.ia[b * g.bs + k % NS] : -1;
+  }
+  // the load at the quadrature points
+  for (int t = tid; t < nel * nq; t += TPB) {
+    const int el = t / nq, q = t % nq;
+    double x[3] = {0., 0., 0.}, f[3];
+    for (int c = 0; c < NC; c++)
+      for (int i = 0; i < 3; i++) x[i] += s_coord[el][3 * c + i] * T.cphi[q][c];
+    forcing(g, x, f);
+    for (int i = 0; i < 3; i++) s_f[el][q][i] = f[i];
+  }
+  __syncthreads();
+  for (int t = tid; t < nel * NSH * NSH; t += TPB) {
+    const int el = t / (NSH * NSH), ij = t % (NSH * NSH), i = ij / NSH, j = ij % NSH;
+    const int64_t bi = s_blk[el][i], bj = s_blk[el][j];
+    if (bi < 0 || bj < 0 || bi > bj || (bi == bj && i > j)) continue;
+    // P[a][b] = |det| Jinv^T R(i,j) Jinv
+    double Q[DIM][DIM], P[DIM][DIM];
+    for (int m = 0; m < DIM; m++)
+      for (int b = 0; b < DIM; b++) {
+        double v = 0.;
+        for (int nn = 0; nn < DIM; nn++) {
+          const int lo = m < nn ? m : nn, hi = m < nn ? nn : m;
+          const int r = lo * DIM - lo * (lo - 1) / 2 + (hi - lo);
+          const double rv = m <= nn ? s_R[r][i][j] : s_R[r][j][i];
+          v += rv * s_jinv[el][nn][b];
+        }
+        Q[m][b] = v;
+      }
+    for (int a = 0; a < DIM; a++)
+      for (int b = 0; b < DIM; b++) {
+        double v = 0.;
+        for (int m = 0; m < DIM; m++) v += s_jinv[el][m][a] * Q[m][b];
+        P[a][b] = v * s_det[el];
+      }
+    const double coef = s_coef[el];
+    if (NS == 3) {
+      const double E = coef, nu = g.nu;
+      const double C1 = E / (2. + 2. * nu);
+      const double C2 = E * nu / (-1. + nu + 2. * nu * nu);
+      const double C3 = E * (nu - 1.) / (-1. + nu + 2. * nu * nu);
+      const int tb = bi == bj ? 0 : s_map[el][T.shape_side[i] * NSD + T.shape_side[j]];
+      // the block's nine entries and slots first, then all read-modify-writes
+      // issued together (entries below the diagonal and dropped ones read their
+      // row's first slot and write nothing)
+      int64_t slot[9];
+      double val[9];
+      bool on[9];
+#pragma unroll
+      for (int ci = 0; ci < 3; ci++)
+#pragma unroll
+        for (int cj = 0; cj < 3; cj++) {
+          const int u = 3 * ci + cj;
+          double v;
+          if (ci == cj) {
+            double tr = 0.;
+#pragma unroll
+            for (int cc = 0; cc < 3; cc++)
+              if (cc != ci) tr += P[cc][cc];
+            v = tr * C1 + P[ci][ci] * C3;
+          } else {
+            v = P[cj][ci] * C1 - P[ci][cj] * C2;
+          }
+          const int64_t row = s_row[el][i * NS + ci];
+          on[u] = !(bi == bj && cj < ci) && !(fabs(v) < 1.e-12);  // upper part; IsZero drop rule
+          slot[u] = on[u] ? (bi == bj ? row + (cj - ci) : row + g.bs + (int64_t)tb * g.bs - ci + cj) : row;
+          val[u] = v;
+        }
+      double old[9];
+#pragma unroll
+      for (int u = 0; u < 9; u++) old[u] = g.a[slot[u]];
+#pragma unroll
+      for (int u = 0; u < 9; u++)
+        if (on[u]) g.a[slot[u]] = old[u] + val[u];
+    } else {
+      double tr = 0.;
+      for (int a = 0; a < DIM; a++) tr += P[a][a];
+      const double v = coef * tr;
+      if (fabs(v) >= 1.e-12) {
+        int64_t slot;
+        if (bi == bj) slot = s_row[el][i];
+        else slot = s_row[el][i] + g.bs + (int64_t)s_map[el][T.shape_side[i] * NSD + T.shape_side[j]] * g.bs;
+        g.a[slot] += v;
+      }
+    }
+  }
+  // load vector
+  for (int t = tid; t < nel * NE; t += TPB) {
+    const int el = t / NE, I = t % NE, i = I / NS, ci = I % NS;
+    if (s_blk[el][i] < 0) continue;
+    double sf = 0.;
+    for (int q = 0; q < nq; q++) sf += T.w[q] * s_det[el] * T.phi[q][i] * s_f[el][q][ci];
+    if (g.material == PZG_MAT_POISSON) sf *= g.scale;
+    g.rhs[s_blk[el][i] * g.bs + ci] += sf;
+  }
+}
+
 // Boundary elements (dimension DIMB = DIM-1 embedded in 3D).  The material's
 // ContributeBC (Material/Poisson/TPZMatPoisson.cpp:45-121, Material/DarcyFlow/
 // TPZDarcyFlow.cpp:49-91, Material/Elasticity/TPZElasticity3D.cpp:616-773)
@@ -817,13 +1010,25 @@ template <int DIM, int NC, int NSD, int NSH, int NS>
 static int run_vol(Ctx &c) {
   ElemArgs g = make_args(c, true);
   const size_t nbk = c.vol_rules.size();
+  // affine elements first in each colour (c.vcolor_aff of them), when the
+  // reference Gram matrices of the single rule are on the device
+  const bool aff = !c.vcolor_aff.empty() && c.d_vol_R.p && c.aff_on;
   for (size_t k = 0; k + 1 < c.vcolor_ptr.size(); k++) {
     int64_t n = c.vcolor_ptr[k + 1] - c.vcolor_ptr[k];
     if (!n) continue;
     g.tab = c.d_tvol.as<ElemTables>() + k % nbk;
-    hipLaunchKernelGGL((k_vol<DIM, NC, NSD, NSH, NS>), dim3(n), dim3(TPB), 0, c.stream, g,
-                       c.d_vcolor_list.as<int32_t>() + c.vcolor_ptr[k]);
-    PZG_HIP(hipGetLastError());
+    const int32_t *lst = c.d_vcolor_list.as<int32_t>() + c.vcolor_ptr[k];
+    int64_t na = aff ? c.vcolor_aff[k] : 0;
+    if (na > 0) {
+      hipLaunchKernelGGL((k_vol_aff<DIM, NC, NSD, NSH, NS>), dim3((unsigned)((na + AFF_EPB - 1) / AFF_EPB)),
+                         dim3(TPB), 0, c.stream, g, c.d_vol_R.as<double>(), lst, na);
+      PZG_HIP(hipGetLastError());
+    }
+    if (n > na) {
+      hipLaunchKernelGGL((k_vol<DIM, NC, NSD, NSH, NS>), dim3((unsigned)(n - na)), dim3(TPB), 0, c.stream, g,
+                         lst + na);
+      PZG_HIP(hipGetLastError());
+    }
   }
   return PZG_OK;
 }

@@ -5,6 +5,7 @@
 #include <map>
 #include <set>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 
@@ -59,7 +60,7 @@ static void free_all(Ctx &c) {
                     &c.d_fa, &c.d_bsr, &c.d_dinvsqrt, &c.d_bjinv, &c.d_xyz, &c.d_vnodes,
                     &c.d_vconn, &c.d_vcoef, &c.d_bnodes, &c.d_bconn, &c.d_bcid, &c.d_bctype,
                     &c.d_bcval, &c.d_bccoef, &c.d_bcvec, &c.d_aidx, &c.d_bumps, &c.d_vmap, &c.d_bmap, &c.d_tvol,
-                    &c.d_tbc, &c.d_vcolor_list, &c.d_bcolor_list, &c.d_x, &c.d_r, &c.d_z,
+                    &c.d_tbc, &c.d_vcolor_list, &c.d_vol_R, &c.d_bcolor_list, &c.d_x, &c.d_r, &c.d_z,
                     &c.d_p, &c.d_p2, &c.d_q, &c.d_u, &c.d_f, &c.d_w, &c.d_xs, &c.d_rs,
                     &c.d_zs, &c.d_ps, &c.d_ps2, &c.d_qs, &c.d_bs, &c.d_scal, &c.d_partials,
                     &c.d_counter, &c.d_res2, &c.d_les_th, &c.d_les_m, &c.d_les_v, &c.d_les_g,
@@ -916,6 +917,54 @@ static int struct_create(Ctx &c) {
   } else {
     c.cv.cptr.assign(1, 0);
     c.cb.cptr.assign(1, 0);
+  }
+  // affine volume elements (parallelepipeds / parallelograms) first in each
+  // colour: the assembly integrates them from the rule's reference Gram
+  // matrices (k_vol_aff); single-rule quad/hex meshes only
+  c.vcolor_aff.clear();
+  c.aff_on = !(std::getenv("PZG_NO_AFFINE") && std::getenv("PZG_NO_AFFINE")[0] == '1');
+  if (c.vol_rules.size() == 1 && !c.hp_simplex && c.vol_etype != PZG_TET &&
+      ((c.dim == 3 && c.vol_ncorner == 8) || (c.dim == 2 && c.vol_ncorner == 4))) {
+    const int NC = c.vol_ncorner;
+    auto affine = [&](int64_t e) {
+      const double *x[8];
+      for (int k = 0; k < NC; k++) x[k] = &c.xyz[3 * c.vol_nodes[e * NC + k]];
+      double scale = 0., dev = 0.;
+      for (int i = 0; i < 3; i++) {
+        const double a1 = x[1][i] - x[0][i], a3 = x[3][i] - x[0][i];
+        scale = std::max(scale, std::max(std::fabs(a1), std::fabs(a3)));
+        dev = std::max(dev, std::fabs(x[2][i] - (x[1][i] + a3)));
+        if (NC == 8) {
+          const double a4 = x[4][i] - x[0][i];
+          scale = std::max(scale, std::fabs(a4));
+          dev = std::max(dev, std::fabs(x[5][i] - (x[1][i] + a4)));
+          dev = std::max(dev, std::fabs(x[7][i] - (x[3][i] + a4)));
+          dev = std::max(dev, std::fabs(x[6][i] - (x[1][i] + a3 + a4)));
+        }
+      }
+      return scale > 0. && dev <= 1e-12 * scale;
+    };
+    const size_t ncol = vptr.size() - 1;
+    c.vcolor_aff.assign(ncol, 0);
+    for (size_t k = 0; k < ncol; k++) {
+      auto b = vlist.begin() + vptr[k], en = vlist.begin() + vptr[k + 1];
+      auto mid = std::stable_partition(b, en, [&](int32_t e) { return affine(e); });
+      c.vcolor_aff[k] = mid - b;
+    }
+    // reference Gram matrices R_mn(i,j) = sum_q w_q dphi_m(i) dphi_n(j), m <= n
+    const ElemTables &T = c.tvol;
+    const int D = c.dim, ns = T.nsh;
+    std::vector<double> R((size_t)(D * (D + 1) / 2) * ns * ns, 0.);
+    int r = 0;
+    for (int m = 0; m < D; m++)
+      for (int n = m; n < D; n++, r++)
+        for (int i = 0; i < ns; i++)
+          for (int j = 0; j < ns; j++) {
+            double v = 0.;
+            for (int q = 0; q < T.nq; q++) v += T.w[q] * T.dphi[q][m][i] * T.dphi[q][n][j];
+            R[((size_t)r * ns + i) * ns + j] = v;
+          }
+    PZG_TRY(dev_upload(c.d_vol_R, R.data(), R.size(), s));
   }
   c.vcolor_ptr = vptr;
   PZG_TRY(dev_upload(c.d_vcolor_list, vlist.data(), vlist.size(), s));

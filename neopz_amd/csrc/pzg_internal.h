@@ -230,6 +230,11 @@ struct Ctx {
 
   // connect -> active block index (-1 for size-0 and constrained connects)
   std::vector<int64_t> aidx;
+  // affine volume elements first in each colour (count per colour list) and
+  // the reference Gram matrices of the rule (k_vol_aff)
+  std::vector<int64_t> vcolor_aff;
+  DevBuf d_vol_R;
+  bool aff_on = true;
   int64_t nb = 0;  // active blocks
 
   // constrained connects (hanging sides): dependencies per connect, and the
