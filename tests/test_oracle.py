@@ -57,6 +57,21 @@ def test_gauss_line_known_integrals():
         assert abs(got - ref) < tol, (order, got, ref)
 
 
+def test_triangle_rules_known_integrals():
+    """The simplex rule of each order p integrates the reference test's
+    polynomial of total degree p over the triangle to the values of
+    UnitTest_PZ/TestIntegNum/Triangle.txt, with that test's tolerance."""
+    vals = _load("triangle_integrals.json")["values"]
+    for p in range(0, 32):
+        pts, w = O.simplex_rule(O.TRI, p)
+        x, y = pts[:, 0], pts[:, 1]
+        f = sum((i + 1) * (x + 0.1) ** i * (j + 1) * (y + 0.1) ** j
+                for i in range(p + 1) for j in range(p + 1 - i))
+        got, ref = float(np.dot(w, f)), float(vals[p])
+        tol = 6e-13 * (10 if ref > 10 else 1) * (10 if ref > 100 else 1) * (10 if ref > 1000 else 1)
+        assert abs(got - ref) < tol, (p, got, ref)
+
+
 def test_gauss_rule_structure():
     for order in range(0, 20):
         x, w = O.gauss_legendre(order)

@@ -9,6 +9,9 @@ External/TPZCutHillMcKee.cpp:16-240), none of them a rerun of its code:
   * a star with centre 2 and leaves 0, 1, 3, 4, 5, traced by hand through
     PseudoPeripheralNodes (start 0, end 5) and the queue of Resequence (leftmost
     highest priority, a removed slot takes the last entry): perm [0 1 4 2 3 5];
+  * a broom whose last level holds two nodes of equal degree, traced by hand
+    through SortNodes and ShrinkLastLevel (one node per degree, the last one
+    inserted) and the queue: perm [0 1 2 4 5 6 3];
   * grids with shuffled labels: Sloan minimises the profile, so its profile is
     at most that of the natural ordering along the short side and at most half
     a random one's.
@@ -92,6 +95,30 @@ def test_star_hand_trace():
     perm = O.sloan(*_graph(6, edges))
     assert perm.tolist() == [0, 1, 4, 2, 3, 5]
     assert np.array_equal(_lib_sloan_edges(6, edges), perm)
+
+
+def test_broom_hand_trace():
+    """Path 0-1-2-3 ending in a hub 3 with leaves 4, 5, 6 and the edge 4-5,
+    traced by hand through External/TPZCutHillMcKee.cpp and
+    External/TPZSloanRenumbering.cpp:
+      * start: the first node of least degree, 0; its level structure ends in
+        {4, 5, 6};
+      * SortNodes (multimap on degree, equal degrees in insertion order) gives
+        [6, 4, 5]; ShrinkLastLevel keeps one node per degree, the last one
+        inserted (std::map assignment), so 4 is dropped: [6, 5], then at most
+        (3 + 2) / 2 = 2 nodes;
+      * from 6 the structure has 5 levels of width up to 3, from 5 5 levels of
+        width 2, so the end node is 5 (keeping 4 instead would end at 4);
+      * distances to 5: 0:4, 1:3, 2:2, 6:2, 3:1, 4:1, 5:0; priorities
+        dist - 2 (degree + 1): 0, -3, -4, -9, -5, -6, -2; the queue of
+        Resequence (leftmost highest priority, a removed slot takes the last
+        entry, W2 = 2 added on activation) takes 0, 1, 2, 6 (priority 0 against
+        3: -5, 4: -3, 5: -4), then 3 (tied with 4 at -3, leftmost), 4, 5.
+    permGather: [0 1 2 4 5 6 3]."""
+    edges = [(0, 1), (1, 2), (2, 3), (3, 4), (3, 5), (3, 6), (4, 5)]
+    perm = O.sloan(*_graph(7, edges))
+    assert perm.tolist() == [0, 1, 2, 4, 5, 6, 3]
+    assert np.array_equal(_lib_sloan_edges(7, edges), perm)
 
 
 @pytest.mark.parametrize("k,m", [(5, 12), (8, 8), (10, 30)])
