@@ -154,6 +154,10 @@ typedef struct {
    * needs each such boundary element on exactly one volume element (the
    * neighbour that orients the normal; an interior face is PZG_EUNSUP). */
   const double *bc_val1;
+  /* TPZElasticity3D::fPreStress (XX, YY, ZZ): the load term
+   * ef(3j+k) -= weight * prestress[k] * dphix(k, j) of Contribute
+   * (Material/Elasticity/TPZElasticity3D.cpp:271-278); zeros = none. */
+  double prestress[3];
 } pzg_mesh_desc;
 
 /* ---- solver selection -------------------------------------------------- */
@@ -166,7 +170,13 @@ typedef struct {
                                 true FP64 ||f - K u|| / ||f|| <= tol          */
 
 #define PZG_PRECOND_JACOBI 0       /* point Jacobi                              */
-#define PZG_PRECOND_BLOCK_JACOBI 1 /* node-block (nstate x nstate) Jacobi       */
+#define PZG_PRECOND_BLOCK_JACOBI 1 /* connect-block Jacobi: the nstate x nstate
+                                      diagonal block of each connect (one
+                                      shape per connect on these meshes)
+                                      inverted; with PZG_SOLVER_PCG64 the
+                                      preconditioner of TPZAnalysis::
+                                      BuildPreconditioner(EBlockJacobi, true),
+                                      Analysis/TPZAnalysis.cpp:1062-1070      */
 #define PZG_PRECOND_LEARNED 2      /* learned message-passing preconditioner    */
 #define PZG_PRECOND_MULTIGRID 3    /* V-cycle: p = 2 -> vertex functions ->
                                       halved lattices, Chebyshev / block-Jacobi
@@ -175,7 +185,9 @@ typedef struct {
 
 typedef struct {
   int32_t solver;        /* PZG_SOLVER_*                                       */
-  int32_t precond;       /* PZG_PRECOND_* (mixed solver only)                  */
+  int32_t precond;       /* PZG_PRECOND_*; PZG_SOLVER_PCG64 takes JACOBI
+                            (EJacobi) or BLOCK_JACOBI (EBlockJacobi; not with
+                            high-order simplex connects, PZG_EUNSUP)           */
   int64_t max_iter;      /* total inner iterations allowed                     */
   double tol;            /* relative residual target                           */
   double inner_tol;      /* mixed: relative reduction per refinement step      */

@@ -16,6 +16,7 @@ SOLVER_PCG64, SOLVER_MIXED = 0, 1
 PRECOND_JACOBI, PRECOND_BLOCK_JACOBI, PRECOND_LEARNED, PRECOND_MULTIGRID = 0, 1, 2, 3
 INNER_AUTO, INNER_BSR, INNER_MATFREE = 0, 1, 2
 MG_LEARNED = 4  # pzg_mg_apply: level-0 smoothing by the learned element smoother
+LES_NH = 32  # hidden features of the learned smoother's network; its output layer is the last LES_NH weights
 EXACT_POINTS, EXACT_CONST, EXACT_X, EXACT_SINSIN, EXACT_COSCOS = range(5)
 
 EXPORTS = [
@@ -52,6 +53,7 @@ class MeshDesc(C.Structure):
         ("vol_rule", i32p), ("bc_rule", i32p), ("vol_etype", C.c_int32),
         ("dep_ptr", i64p), ("dep_conn", i64p), ("dep_coef", dp),
         ("bc_val1", dp),
+        ("prestress", C.c_double * 3),
     ]
 
 
@@ -193,6 +195,8 @@ def make_mesh_desc(m):
     d.bc_val = ptr(arr(m["bc_val"], np.float64), dp)
     if m.get("bc_val1") is not None:
         d.bc_val1 = ptr(arr(m["bc_val1"], np.float64), dp)
+    for k in range(3):
+        d.prestress[k] = float(m.get("prestress", (0.0, 0.0, 0.0))[k])
     d.nconn = m["nconn"]
     d.blk_pos = ptr(arr(m["blk_pos"], np.int64), i64p)
     d.blk_size = ptr(arr(m["blk_size"], np.int32), i32p)

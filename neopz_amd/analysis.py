@@ -170,7 +170,7 @@ class TPZFYsmpMatrixHIP(TPZSYsmpMatrixHIP):
 class TPZStepSolver:
     """TPZStepSolver subset used on the path."""
 
-    ECG, EJacobi, EMixedFCG = 0, 1, 2
+    ECG, EJacobi, EMixedFCG, EBlockJacobi = 0, 1, 2, 3
 
     def __init__(self, matrix=None):
         self.matrix = matrix
@@ -205,14 +205,19 @@ class TPZStepSolver:
         self.kind = self.EJacobi
 
     def SetCG(self, numiterations, pre, tol, FromCurrent):
+        """CG preconditioned by pre: an EJacobi step solver (point Jacobi) or
+        the connect-block Jacobi that TPZLinearAnalysis.BuildPreconditioner
+        (EBlockJacobi, overlap=True) returns."""
         if FromCurrent != 0:
             raise ValueError("SetCG: FromCurrent=1 is not supported (x starts at zero)")
-        if pre is None or pre.kind != self.EJacobi:
-            raise ValueError("SetCG: the GPU path needs an EJacobi TPZStepSolver preconditioner")
+        if pre is None or pre.kind not in (self.EJacobi, self.EBlockJacobi):
+            raise ValueError("SetCG: the GPU path needs an EJacobi step solver or the "
+                             "BuildPreconditioner(EBlockJacobi, True) preconditioner")
         self.kind = self.ECG
         self.maxit = int(numiterations)
         self.tol = float(tol)
         self.pre = pre
+        self.precond = L.PRECOND_BLOCK_JACOBI if pre.kind == self.EBlockJacobi else L.PRECOND_JACOBI
 
     def SetMixedFCG(self, numiterations, tol, precond=L.PRECOND_BLOCK_JACOBI, inner_tol=1e-4,
                     max_outer=200, refine=0, inner_op=L.INNER_AUTO, mg_degree=0, mg_range=0.0):
@@ -439,6 +444,28 @@ class TPZLinearAnalysis:
         self.fSolver = None
         self.K = None
         self.fSolution = None
+
+    EJacobi, EBlockJacobi, EElement, ENodeCentered = 0, 1, 2, 3  # TPZAnalysis::EPrecond
+
+    def BuildPreconditioner(self, preconditioner, overlap):
+        """TPZAnalysis::BuildPreconditioner (Analysis/TPZAnalysis.cpp:995-1086).
+        EJacobi returns None as there.  EBlockJacobi with overlap returns the
+        block diagonal of the connects (TPZBlockDiagonalStructMatrix::
+        AssembleBlockDiagonal, StrMatrix/pzbdstrmatrix.cpp:13-56: one block per
+        connect, its nshape x nstate equations, the stiffness restricted to
+        them) solved by LU; the solver inverts the blocks of the assembled
+        matrix on the device when CG starts.  The other forms (overlapping
+        element and vertex blocks, the coloured sequence solver of
+        overlap=False) are not built."""
+        if preconditioner == self.EJacobi:
+            return None
+        if preconditioner != self.EBlockJacobi or not overlap:
+            raise NotImplementedError("BuildPreconditioner: only EJacobi and EBlockJacobi with overlap")
+        if self.fSolver is None:
+            raise ValueError("BuildPreconditioner called with uninitialized stiffness matrix")
+        step = TPZStepSolver(self.fSolver.matrix)
+        step.kind = TPZStepSolver.EBlockJacobi
+        return step
 
     def SetStructuralMatrix(self, strmat):
         self.fStructMatrix = strmat.Clone()  # TPZAnalysis clones (Analysis/TPZAnalysis.cpp:83-104)

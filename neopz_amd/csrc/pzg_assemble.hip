@@ -143,6 +143,7 @@ struct ElemArgs {
   double fconst[3];
   int force_kind;
   double scale, E, nu, big;
+  double prestress[3];  // TPZElasticity3D::fPreStress
   int material;
   int bs;
   // element-matrix output (constrained elements): shapes are active when the
@@ -323,6 +324,11 @@ __global__ __launch_bounds__(TPB) void k_vol(ElemArgs g, const int32_t *__restri
       if (s_blk[i] != -1) {
         for (int q = 0; q < nq; q++) sf += s_wdet[q] * T.phi[q][i] * s_f[q][ci];
         if (g.material == PZG_MAT_POISSON) sf *= g.scale;
+        if (NS == 3 && g.prestress[ci] != 0.) {  // - w prestress_k dphix_k (fPreStress)
+          double sg = 0.;
+          for (int q = 0; q < nq; q++) sg += s_Gw[q][ci][i];
+          sf -= g.prestress[ci] * sg;
+        }
       }
       g.ef_out[(int64_t)blockIdx.x * NE + I] = sf;
     }
@@ -371,6 +377,11 @@ __global__ __launch_bounds__(TPB) void k_vol(ElemArgs g, const int32_t *__restri
     double s = 0.;
     for (int q = 0; q < nq; q++) s += s_wdet[q] * T.phi[q][i] * s_f[q][ci];
     if (g.material == PZG_MAT_POISSON) s *= g.scale;
+    if (NS == 3 && g.prestress[ci] != 0.) {  // - w prestress_k dphix_k (fPreStress)
+      double sg = 0.;
+      for (int q = 0; q < nq; q++) sg += s_Gw[q][ci][i];
+      s -= g.prestress[ci] * sg;
+    }
     g.rhs[s_blk[i] * g.bs + ci] += s;
   }
 }
@@ -564,6 +575,15 @@ __global__ __launch_bounds__(TPB) void k_vol_aff(ElemArgs g, const double *__res
     double sf = 0.;
     for (int q = 0; q < nq; q++) sf += T.w[q] * s_det[el] * T.phi[q][i] * s_f[el][q][ci];
     if (g.material == PZG_MAT_POISSON) sf *= g.scale;
+    if (NS == 3 && g.prestress[ci] != 0.) {  // - w prestress_k dphix_k (fPreStress)
+      double sg = 0.;
+      for (int q = 0; q < nq; q++) {
+        double d = 0.;
+        for (int m = 0; m < DIM; m++) d += s_jinv[el][m][ci] * T.dphi[q][m][i];
+        sg += T.w[q] * s_det[el] * d;
+      }
+      sf -= g.prestress[ci] * sg;
+    }
     g.rhs[s_blk[el][i] * g.bs + ci] += sf;
   }
 }
@@ -1000,6 +1020,7 @@ static ElemArgs make_args(Ctx &c, bool vol) {
   g.scale = c.scale;
   g.E = c.E;
   g.nu = c.nu;
+  for (int k = 0; k < 3; k++) g.prestress[k] = c.prestress[k];
   g.big = c.big;
   g.material = c.material;
   g.bs = c.bs;

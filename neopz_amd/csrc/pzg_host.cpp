@@ -65,7 +65,7 @@ static void free_all(Ctx &c) {
                     &c.d_zs, &c.d_ps, &c.d_ps2, &c.d_qs, &c.d_bs, &c.d_scal, &c.d_partials,
                     &c.d_counter, &c.d_res2, &c.d_les_th, &c.d_les_m, &c.d_les_v, &c.d_les_g,
                     &c.d_les_cia, &c.d_les_cja, &c.d_les_pen, &c.d_les_X, &c.d_les_z, &c.d_les_s,
-                    &c.d_les_ye, &c.d_les_blkm, &c.d_les_eoff, &c.d_les_Se, &c.d_les_M, &c.d_les_R, &c.d_les_E, &c.d_les_D,
+                    &c.d_les_ye, &c.d_les_blkm, &c.d_les_eoff, &c.d_bjinv64, &c.d_les_Se, &c.d_les_M, &c.d_les_R, &c.d_les_E, &c.d_les_D,
                     &c.d_les_le, &c.d_les_ds, &c.d_les_loss, &c.d_les_fail, &c.d_nn_trH, &c.d_nn_trM,
                     &c.d_nn_trP, &c.d_nn_dH, &c.d_nn_G, &c.d_nn_tmp, &c.d_nn_dZ, &c.d_nn_part,
                     &c.d_filt_dest, &c.d_post_tab, &c.d_post_list, &c.d_post_slot,
@@ -387,6 +387,7 @@ static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
   c.scale = m->scale;
   c.E = m->E;
   c.nu = m->nu;
+  for (int k = 0; k < 3; k++) c.prestress[k] = m->material == PZG_MAT_ELAST3D ? m->prestress[k] : 0.;
   c.big = m->big_number;
   c.rule_vol = m->rule_order_vol;
   c.rule_bc = m->rule_order_bc;

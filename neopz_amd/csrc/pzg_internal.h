@@ -201,6 +201,7 @@ struct Ctx {
   std::vector<int64_t> vol_nodes, vol_conn, bc_nodes, bc_conn, blk_pos;
   std::vector<int32_t> blk_size, bc_id, bc_type;
   double scale = 1, E = 1, nu = 0.3, big = 1e12, bump_width = 0.1, force_const[3] = {0, 0, 0};
+  double prestress[3] = {0, 0, 0};  // TPZElasticity3D::fPreStress
   int force_kind = 0, nbumps = 0, rule_vol = 0, rule_bc = 0, nbcid = 0;
   ElemTables tvol, tbc;        // tables of the first rule bucket
   // hp: elements bucketed by integration rule (one table per bucket); the
@@ -327,6 +328,7 @@ struct Ctx {
   // vertex equations, the dense inverse of the coarsest level
   bool mg_ready = false, mg_use_mf = false;
   bool mg_les = false;     // level 0 smoothed with the learned element smoother
+  DevBuf d_bjinv64;        // FP64 inverses of the connect blocks (PCG64 with block Jacobi)
   DevBuf d_les_blkm;       // element blocks with penalized-component bits (learned apply)
   DevBuf d_les_eoff;       // row-ordered slot -> element-major output offset (learned apply)
   // penalty rows of the learned path, solved in FP64 by the outer refinement

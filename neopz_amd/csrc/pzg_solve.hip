@@ -230,13 +230,89 @@ __global__ __launch_bounds__(TPB) void k_cg_update64(int64_t n, double *__restri
     x[i] += alpha * p[i];
     double ri = r[i] + (-alpha) * q[i];
     r[i] = ri;
-    double zi = ri / diag[i];
-    z[i] = zi;
     rr += ri * ri;
-    rz += ri * zi;
+    if (diag) {  // point Jacobi; the block form follows in k_bj_apply64
+      double zi = ri / diag[i];
+      z[i] = zi;
+      rz += ri * zi;
+    }
   }
   double v[2] = {rr, rz};
   block_partials<2>(v, partials);
+}
+
+// Connect-block Jacobi (TPZBlockDiagonal with ELU blocks, the preconditioner
+// TPZAnalysis::BuildPreconditioner(EBlockJacobi, true) returns,
+// Analysis/TPZAnalysis.cpp:1062-1070): z = Binv r per block row, r.z into
+// partials slot 1 (same grid as k_cg_update64).
+template <int BS>
+__global__ __launch_bounds__(TPB) void k_bj_apply64(int64_t nb, const double *__restrict__ binv,
+                                                    const double *__restrict__ r, double *__restrict__ z,
+                                                    const SolveScalars *sc, double *partials) {
+  if (sc && sc->done) return;
+  double rz = 0.;
+  for (int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; A < nb; A += (int64_t)gridDim.x * blockDim.x) {
+    double rv[BS];
+#pragma unroll
+    for (int k = 0; k < BS; k++) rv[k] = r[A * BS + k];
+    const double *bi = binv + A * BS * BS;
+#pragma unroll
+    for (int k = 0; k < BS; k++) {
+      double s = 0.;
+#pragma unroll
+      for (int l = 0; l < BS; l++) s += bi[k * BS + l] * rv[l];
+      z[A * BS + k] = s;
+      rz += rv[k] * s;
+    }
+  }
+  __shared__ double sm[TPB / 64];
+  const double t = block_sum(rz, sm);
+  if (threadIdx.x == 0) partials[1 * MAXBLK + blockIdx.x] = t;
+}
+
+// rho = r.z from slot 1 after the block apply of the initial residual
+__global__ void k_bj_rho64(SolveScalars *sc, const double *partials, int nblk) {
+  const double rz = sum_partials(partials, 1, nblk);
+  if (threadIdx.x == 0) sc->rho = rz;
+}
+
+// inverse of each diagonal block of the FP64 full block CSR (cofactors for
+// 3 x 3, the reciprocal for 1 x 1); a singular block sets *fail
+template <int BS>
+__global__ void k_bj_inv64(int64_t nb, const int64_t *__restrict__ fbia, const int32_t *__restrict__ fbja,
+                           const double *__restrict__ fa, double *__restrict__ binv, int *fail) {
+  const int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (A >= nb) return;
+  const int64_t b0 = fbia[A], n = fbia[A + 1] - b0;
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t m = (lo + hi) / 2;
+    if (fbja[b0 + m] < A) lo = m + 1;
+    else hi = m;
+  }
+  double a[BS * BS];
+#pragma unroll
+  for (int k = 0; k < BS * BS; k++) a[k] = fa[bsr_pos<BS * BS>(b0, n, lo, k)];
+  static_assert(BS == 1 || BS == 3, "connect blocks of 1 or 3 equations");
+  double *o = binv + A * BS * BS;
+  if (BS == 1) {
+    if (a[0] == 0.) atomicExch(fail, 1);
+    o[0] = 1. / a[0];
+  } else {
+    const double c00 = a[4] * a[8] - a[5] * a[7], c01 = a[5] * a[6] - a[3] * a[8], c02 = a[3] * a[7] - a[4] * a[6];
+    const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    if (det == 0.) atomicExch(fail, 1);
+    const double id = 1. / det;
+    o[0] = c00 * id;
+    o[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+    o[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+    o[3] = c01 * id;
+    o[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+    o[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+    o[6] = c02 * id;
+    o[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+    o[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+  }
 }
 
 // resid = |r|/|b|; stop tests; beta = (r.z)/rho   (one block)
@@ -273,10 +349,12 @@ __global__ __launch_bounds__(TPB) void k_cg_init64(int64_t n, const double *__re
     x[i] = 0.;
     r[i] = bi;
     p[i] = 0.;
-    double zi = bi / diag[i];
-    z[i] = zi;
     bb += bi * bi;
-    rz += bi * zi;
+    if (diag) {
+      double zi = bi / diag[i];
+      z[i] = zi;
+      rz += bi * zi;
+    }
   }
   double v[2] = {bb, rz}, tot[2];
   if (reduce_last<2>(v, partials, counter, tot) && threadIdx.x == 0) {
@@ -618,9 +696,44 @@ int solve_pcg64(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info) {
   double *part = c.d_partials.as<double>();
   unsigned *cnt = c.d_counter.as<unsigned>();
   const int gv = grid_for(n, TPB);
+  // EJacobi (reference default) or the connect-block Jacobi of
+  // BuildPreconditioner(EBlockJacobi): blocks of one connect's nstate
+  // equations (the connects of order <= 2 carry one shape)
+  const bool block = prm.precond == PZG_PRECOND_BLOCK_JACOBI;
+  if (!block && prm.precond != PZG_PRECOND_JACOBI) {
+    set_error("pzg_solve: PCG64 takes the point or the connect-block Jacobi preconditioner");
+    return PZG_EINVAL;
+  }
+  if (block) {
+    if (c.hp_simplex || c.external_csr) {
+      set_error(c.hp_simplex ? "pzg_solve: connect-block Jacobi with high-order simplex connects not built"
+                             : "pzg_solve: connect-block Jacobi needs the mesh's connects");
+      return PZG_EUNSUP;
+    }
+    PZG_TRY(dev_alloc(c.d_bjinv64, sizeof(double) * c.nb * c.bs * c.bs));
+    int *fail = reinterpret_cast<int *>(c.d_counter.as<unsigned>() + 8);
+    PZG_HIP(hipMemsetAsync(fail, 0, sizeof(int), c.stream));
+    DISPATCH_BS(c.bs, hipLaunchKernelGGL(k_bj_inv64<BS>, dim3((unsigned)((c.nb + TPB - 1) / TPB)), dim3(TPB), 0,
+                                         c.stream, c.nb, c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(),
+                                         c.d_fa.as<double>(), c.d_bjinv64.as<double>(), fail));
+    int hf = 0;
+    PZG_HIP(hipMemcpyAsync(&hf, fail, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    PZG_HIP(hipStreamSynchronize(c.stream));
+    if (hf) {
+      set_error("pzg_solve: a connect block of K is singular");
+      return PZG_EUNSUP;
+    }
+  }
+  const double *dg = block ? nullptr : c.d_diag.as<double>();
   hipLaunchKernelGGL(k_cg_init64, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_f.as<double>(),
                      c.d_u.as<double>(), c.d_r.as<double>(), c.d_z.as<double>(),
-                     c.d_p.as<double>(), c.d_diag.as<double>(), sc, part, cnt);
+                     c.d_p.as<double>(), dg, sc, part, cnt);
+  if (block) {
+    DISPATCH_BS(c.bs, hipLaunchKernelGGL(k_bj_apply64<BS>, dim3(gv), dim3(TPB), 0, c.stream, c.nb,
+                                         c.d_bjinv64.as<double>(), c.d_r.as<double>(), c.d_z.as<double>(),
+                                         (const SolveScalars *)nullptr, part));
+    hipLaunchKernelGGL(k_bj_rho64, dim3(1), dim3(TPB), 0, c.stream, sc, (const double *)part, gv);
+  }
   PZG_HIP(hipGetLastError());
   const int gs = bsr_grid(c.nb);
   auto iter = [&]() {
@@ -633,8 +746,12 @@ int solve_pcg64(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info) {
     hipLaunchKernelGGL(k_fin_alpha64, dim3(1), dim3(TPB), 0, c.stream, sc, (const double *)part, gs);
     hipLaunchKernelGGL(k_cg_update64, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_u.as<double>(),
                        c.d_r.as<double>(), c.d_z.as<double>(), c.d_p.as<double>(),
-                       c.d_q.as<double>(), c.d_diag.as<double>(), sc, (double *)nullptr, part,
+                       c.d_q.as<double>(), dg, sc, (double *)nullptr, part,
                        cnt);
+    if (block)
+      DISPATCH_BS(c.bs, hipLaunchKernelGGL(k_bj_apply64<BS>, dim3(gv), dim3(TPB), 0, c.stream, c.nb,
+                                           c.d_bjinv64.as<double>(), c.d_r.as<double>(), c.d_z.as<double>(),
+                                           (const SolveScalars *)sc, part));
     hipLaunchKernelGGL(k_fin_cg64, dim3(1), dim3(TPB), 0, c.stream, sc, (const double *)part, gv,
                        (double *)nullptr);
   };

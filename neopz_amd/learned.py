@@ -95,16 +95,18 @@ class LearnedSmoother:
 
 
 def train_on_seeds(ctx, strmat, K, make_coef, seeds, nrhs=4, steps=100, lr=2e-3, setup_seed=0,
-                   rounds=1):
+                   rounds=1, progress=None):
     """Trains the learned smoother on seeded coefficient fields of one mesh:
     for each seed, pzg_mesh_set_coef + assembly + pairs of the local problems,
     then the least-squares fit of G on every pair so far and ``steps`` Adam
     steps of the network on the seed's pairs (``rounds`` passes over the
-    seeds).  Returns (LearnedSmoother, loss of the initial parameters on the
-    first seed's pairs, loss after the last step)."""
+    seeds; ``progress(k, loss)``, if given, after every 100th seed).  Returns
+    (LearnedSmoother, loss of the initial parameters on the first seed's
+    pairs, loss after the last step)."""
     ls = LearnedSmoother(ctx)
     ls.setup(seed=setup_seed)
     first = last = None
+    k = 0
     for _ in range(rounds):
         for s in seeds:
             coef = np.ascontiguousarray(make_coef(s), np.float64)
@@ -114,4 +116,7 @@ def train_on_seeds(ctx, strmat, K, make_coef, seeds, nrhs=4, steps=100, lr=2e-3,
             if first is None:  # the loss of the initial parameters
                 first = ls.loss_grad()[0]
             _, last = ls.train(steps=steps, lr=lr)
+            k += 1
+            if progress is not None and k % 100 == 0:
+                progress(k, last)
     return ls, first, last

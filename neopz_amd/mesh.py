@@ -115,6 +115,7 @@ def _common(pb, econn_vol, econn_bc, nconn, corner_flags, xyz, vol_nodes, bc_nod
         bc_nodes=bc_nodes, bc_conn=econn_bc, bc_id=bc_id,
         bc_type=np.array([pb.bc_type[f] for f in range(6)], dtype=np.int32),
         bc_val=np.array([pb.bc_val[f] for f in range(6)], dtype=np.float64).ravel(),
+        prestress=np.array(getattr(pb, "prestress", (0.0, 0.0, 0.0)), dtype=np.float64),
         bc_val1=np.array([getattr(pb, "bc_val1", [[0.0] * 9] * 6)[f] for f in range(6)],
                          dtype=np.float64).ravel(),
         nconn=nconn, blk_pos=blk_pos, blk_size=blk_size, neq=neq,
@@ -544,6 +545,7 @@ def build_refined2d(pb, refine):
         bc_nodes=bc_nodes, bc_conn=bc_conn, bc_id=bc_id,
         bc_type=np.array([pb.bc_type[f] for f in range(6)], dtype=np.int32),
         bc_val=np.array([pb.bc_val[f] for f in range(6)], dtype=np.float64).ravel(),
+        prestress=np.array(getattr(pb, "prestress", (0.0, 0.0, 0.0)), dtype=np.float64),
         bc_val1=np.array([getattr(pb, "bc_val1", [[0.0] * 9] * 6)[f] for f in range(6)],
                          dtype=np.float64).ravel(),
         nconn=nconn, blk_pos=blk_pos, blk_size=blk_size, neq=neq,
@@ -766,6 +768,7 @@ def build_refined3d(pb, refine):
         bc_nodes=bc_nodes, bc_conn=bc_conn, bc_id=bc_id,
         bc_type=np.array([pb.bc_type[f] for f in range(6)], dtype=np.int32),
         bc_val=np.array([pb.bc_val[f] for f in range(6)], dtype=np.float64).ravel(),
+        prestress=np.array(getattr(pb, "prestress", (0.0, 0.0, 0.0)), dtype=np.float64),
         bc_val1=np.array([getattr(pb, "bc_val1", [[0.0] * 9] * 6)[f] for f in range(6)],
                          dtype=np.float64).ravel(),
         nconn=nconn, blk_pos=blk_pos, blk_size=blk_size, neq=neq,

@@ -46,6 +46,8 @@ class Problem:
     bc_val: list = field(default_factory=lambda: [[0.0, 0.0, 0.0] for _ in range(6)])
     # TPZBndCondT::Val1 per face, row-major 3x3 (Val1(0,0) for scalar materials)
     bc_val1: list = field(default_factory=lambda: [[0.0] * 9 for _ in range(6)])
+    # TPZElasticity3D::fPreStress (XX, YY, ZZ)
+    prestress: tuple = (0.0, 0.0, 0.0)
     name: str = ""
     # hp: preferred order per volume element (1..p); None = p everywhere.
     # Boundary elements keep the default order p.

@@ -36,6 +36,7 @@ class _Problem(C.Structure):
         ("bc_type", C.c_int * 6), ("bc_val", (C.c_double * 3) * 6),
         ("el_porder", C.POINTER(C.c_int)), ("simplex", C.c_int),
         ("bc_val1", (C.c_double * 9) * 6),
+        ("prestress", C.c_double * 3),
     ]
 
 
@@ -97,6 +98,8 @@ def lib():
         L.orc_addkel.argtypes = [i64p, i64p, dp, C.c_int, dp, i64p]
         L.orc_multadd.argtypes = [C.c_int64, i64p, i64p, dp, dp, dp, dp, C.c_double, C.c_double]
         L.orc_pcg.argtypes = [C.c_int64, i64p, i64p, dp, dp, dp, i64p, dp, C.c_int, dp]
+        L.orc_pcg_block.argtypes = [C.c_int64, i64p, i64p, dp, dp, C.c_int64, C.POINTER(C.c_int32), dp,
+                                    i64p, dp, dp]
         L.orc_simplex_rule.argtypes = [C.c_int, C.c_int, dp, dp]
         L.orc_collapsed_rule.argtypes = [C.c_int, C.c_int, dp, dp]
         L.orc_shape_h1.argtypes = [C.c_int, i64p, C.POINTER(C.c_int), dp, dp, dp]
@@ -244,6 +247,8 @@ class Oracle:
             for k in range(9):
                 pb.bc_val1[f][k] = v1[f][k] if v1 is not None else 0.0
         pb.simplex = int(getattr(spec, "simplex", 0))
+        for k in range(3):
+            pb.prestress[k] = float(getattr(spec, "prestress", (0.0, 0.0, 0.0))[k])
         if getattr(spec, "el_porder", None) is not None:
             po = np.ascontiguousarray(spec.el_porder, dtype=np.int32)
             self._keep.append(po)
@@ -524,6 +529,27 @@ def pcg(ia, ja, a, b, maxit=100000, tol=1e-10, wasted_spmv=True, history=False):
     b = np.ascontiguousarray(b, dtype=np.float64)
     ret = lib().orc_pcg(n, _ip(ia), _ip(ja), _dp(a), _dp(b), _dp(x), C.byref(it), C.byref(t),
                         1 if wasted_spmv else 0, _dp(hist) if history else None)
+    out = {"x": x, "iters": it.value, "tol": t.value, "converged": ret == 0}
+    if history:
+        out["history"] = hist[: it.value].copy()
+    return out
+
+
+def pcg_block(ia, ja, a, b, bsize, maxit=100000, tol=1e-10, history=False):
+    """CG with the EBlockJacobi(overlap) preconditioner: LU of the consecutive
+    diagonal blocks of bsize[k] equations (orc_pcg_block)."""
+    n = len(ia) - 1
+    x = np.zeros(n)
+    it = C.c_int64(maxit)
+    t = C.c_double(tol)
+    hist = np.zeros(maxit) if history else None
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    bs = np.ascontiguousarray(bsize, dtype=np.int32)
+    ret = lib().orc_pcg_block(n, _ip(ia), _ip(ja), _dp(a), _dp(b), len(bs),
+                              bs.ctypes.data_as(C.POINTER(C.c_int32)), _dp(x), C.byref(it), C.byref(t),
+                              _dp(hist) if history else None)
+    if ret < 0:
+        raise ValueError("pcg_block: block sizes do not sum to the number of equations")
     out = {"x": x, "iters": it.value, "tol": t.value, "converged": ret == 0}
     if history:
         out["history"] = hist[: it.value].copy()

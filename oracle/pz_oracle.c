@@ -1260,6 +1260,10 @@ int orc_calcstiff(const orc_problem *pb, const orc_mesh *m, int64_t iel, double 
     orc_contribute(pb->material, dim, nsh, ne, weight, phi, dphix, f,
                    pb->material == ORC_DARCY ? (pb->el_coef ? coef : pb->scale) : pb->scale,
                    pb->el_coef ? coef : pb->E, pb->nu, ek, ef);
+    /* CODE3 load term of the prestress (Material/Elasticity/TPZElasticity3D.cpp:271-278) */
+    if (pb->material == ORC_ELAST3D)
+      for (int jn = 0; jn < nsh; jn++)
+        for (int kd = 0; kd < 3; kd++) ef[3 * jn + kd] -= weight * pb->prestress[kd] * dphix[kd * nsh + jn];
   }
   return ne;
 }
@@ -1778,15 +1782,83 @@ static double getval_diag(const int64_t *ia, const int64_t *ja, const double *a,
   return 0.;
 }
 
-/* CG<> (Solvers/LinearSolvers/cg.h:29-132) preconditioned by a TPZStepSolver
- * EJacobi(1 iteration, tol 0) = TPZMatrix::SolveJacobi (Matrix/pzmatrix.cpp:
+/* Preconditioner application z = M^-1 r of the CG below. */
+typedef struct {
+  int64_t n;
+  const int64_t *ia, *ja;
+  const double *a;
+  int wasted_spmv;
+  double *scr;
+  int64_t nblk;          /* block form: consecutive blocks of bsize[] eqs */
+  const int32_t *bsize;
+  const int64_t *bpos;   /* first equation of each block                  */
+  const int64_t *lpos;   /* offset of each block's LU factors in lu       */
+  double *lu;
+} pcg_pre;
+
+/* EJacobi(1 iteration, tol 0) = TPZMatrix::SolveJacobi (Matrix/pzmatrix.cpp:
  * 642-667): z = r / GetVal(i,i), followed by a Residual whose result is
  * discarded (reproduced when wasted_spmv != 0). */
-int orc_pcg(int64_t n, const int64_t *ia, const int64_t *ja, const double *a, const double *b,
-            double *x, int64_t *iters, double *tol, int wasted_spmv, double *res_history) {
+static void pre_jacobi(const pcg_pre *P, const double *r, double *z) {
+  for (int64_t k = 0; k < P->n; k++) z[k] = r[k] / getval_diag(P->ia, P->ja, P->a, k);
+  if (P->wasted_spmv) orc_multadd(P->n, P->ia, P->ja, P->a, z, r, P->scr, -1., 1.);
+}
+
+/* TPZBlockDiagonal after UpdateFrom (Matrix/pzblockdiag.cpp:567-590: the
+ * consecutive diagonal blocks of the matrix) and Decompose_LU (each block by
+ * TPZFMatrix::Decompose_LU, Matrix/pzfmatrix.cpp:1285-1318: no pivoting, a
+ * zero pivot replaced by 1), applied by forward/back substitution. */
+static void pre_block(const pcg_pre *P, const double *r, double *z) {
+  for (int64_t b = 0; b < P->nblk; b++) {
+    const int m = P->bsize[b];
+    const int64_t e0 = P->bpos[b];
+    const double *f = P->lu + P->lpos[b];  /* column-major m x m */
+    double *y = z + e0;
+    for (int i = 0; i < m; i++) {          /* L y = r, unit lower */
+      double s = r[e0 + i];
+      for (int k = 0; k < i; k++) s -= f[i + k * m] * y[k];
+      y[i] = s;
+    }
+    for (int i = m - 1; i >= 0; i--) {     /* U z = y */
+      double s = y[i];
+      for (int k = i + 1; k < m; k++) s -= f[i + k * m] * y[k];
+      y[i] = s / f[i + i * m];
+    }
+  }
+}
+
+static double getval_sym(const int64_t *ia, const int64_t *ja, const double *a, int64_t r, int64_t c) {
+  if (r > c) { int64_t t = r; r = c; c = t; }
+  for (int64_t k = ia[r]; k < ia[r + 1]; k++)
+    if (ja[k] == c) return a[k];
+  return 0.;
+}
+
+static void block_factor(pcg_pre *P) {
+  for (int64_t b = 0; b < P->nblk; b++) {
+    const int m = P->bsize[b];
+    const int64_t e0 = P->bpos[b];
+    double *f = P->lu + P->lpos[b];
+    for (int i = 0; i < m; i++)
+      for (int j = 0; j < m; j++) f[i + j * m] = getval_sym(P->ia, P->ja, P->a, e0 + i, e0 + j);
+    for (int k = 0; k < m; k++) {
+      double piv = f[k + k * m];
+      if (piv == 0.) { f[k + k * m] = 1.; piv = 1.; }
+      for (int i = k + 1; i < m; i++) f[i + k * m] /= piv;
+      for (int j = k + 1; j < m; j++) {
+        const double kj = f[k + j * m];
+        for (int i = k + 1; i < m; i++) f[i + j * m] -= f[i + k * m] * kj;
+      }
+    }
+  }
+}
+
+/* CG<> (Solvers/LinearSolvers/cg.h:29-132) with the preconditioner pre. */
+static int pcg_core(int64_t n, const int64_t *ia, const int64_t *ja, const double *a, const double *b,
+                    double *x, int64_t *iters, double *tol, double *res_history,
+                    void (*pre)(const pcg_pre *, const double *, double *), const pcg_pre *P) {
   double *r = (double *)malloc(n * sizeof(double)), *z = (double *)malloc(n * sizeof(double));
   double *p = (double *)malloc(n * sizeof(double)), *q = (double *)malloc(n * sizeof(double));
-  double *scr = (double *)malloc(n * sizeof(double));
   int64_t maxit = *iters;
   double normb = sqrt(dot(n, b, b)), resid, rho, rho1 = 0., alpha, beta;
   for (int64_t i = 0; i < n; i++) { x[i] = 0.; r[i] = b[i]; }
@@ -1800,8 +1872,7 @@ int orc_pcg(int64_t n, const int64_t *ia, const int64_t *ja, const double *a, co
   }
   int64_t i;
   for (i = 1; i <= maxit; i++) {
-    for (int64_t k = 0; k < n; k++) z[k] = r[k] / getval_diag(ia, ja, a, k);
-    if (wasted_spmv) orc_multadd(n, ia, ja, a, z, r, scr, -1., 1.);
+    pre(P, r, z);
     rho = dot(n, r, z);
     if (i == 1) memcpy(p, z, n * sizeof(double));
     else {
@@ -1825,7 +1896,51 @@ int orc_pcg(int64_t n, const int64_t *ia, const int64_t *ja, const double *a, co
   *tol = resid;
   *iters = maxit;
 done:
-  free(r); free(z); free(p); free(q); free(scr);
+  free(r); free(z); free(p); free(q);
+  return ret;
+}
+
+/* CG<> preconditioned by a TPZStepSolver EJacobi(1, 0, 0) (pre_jacobi). */
+int orc_pcg(int64_t n, const int64_t *ia, const int64_t *ja, const double *a, const double *b,
+            double *x, int64_t *iters, double *tol, int wasted_spmv, double *res_history) {
+  pcg_pre P;
+  memset(&P, 0, sizeof(P));
+  P.n = n; P.ia = ia; P.ja = ja; P.a = a; P.wasted_spmv = wasted_spmv;
+  P.scr = (double *)malloc((n > 0 ? n : 1) * sizeof(double));
+  int ret = pcg_core(n, ia, ja, a, b, x, iters, tol, res_history, pre_jacobi, &P);
+  free(P.scr);
+  return ret;
+}
+
+/* CG<> preconditioned by the TPZStepSolver(TPZBlockDiagonal) with ELU that
+ * TPZAnalysis::BuildPreconditioner(EBlockJacobi, true) returns
+ * (Analysis/TPZAnalysis.cpp:1062-1070); the block sizes are those of the
+ * connects in sequence order (TPZBlockDiagonalStructMatrix::BlockSizes,
+ * StrMatrix/pzbdstrmatrix.cpp:59-93) and the blocks are taken from the
+ * matrix when the solve starts (TPZStepSolver::Solve -> UpdateFrom,
+ * Solvers/pzstepsolver.cpp:72-76). */
+int orc_pcg_block(int64_t n, const int64_t *ia, const int64_t *ja, const double *a, const double *b,
+                  int64_t nblk, const int32_t *bsize, double *x, int64_t *iters, double *tol,
+                  double *res_history) {
+  pcg_pre P;
+  memset(&P, 0, sizeof(P));
+  P.n = n; P.ia = ia; P.ja = ja; P.a = a; P.nblk = nblk; P.bsize = bsize;
+  int64_t *bpos = (int64_t *)malloc((nblk + 1) * sizeof(int64_t));
+  int64_t *lpos = (int64_t *)malloc((nblk + 1) * sizeof(int64_t));
+  bpos[0] = lpos[0] = 0;
+  for (int64_t k = 0; k < nblk; k++) {
+    bpos[k + 1] = bpos[k] + bsize[k];
+    lpos[k + 1] = lpos[k] + (int64_t)bsize[k] * bsize[k];
+  }
+  if (bpos[nblk] != n) {
+    free(bpos); free(lpos);
+    return -1;
+  }
+  P.bpos = bpos; P.lpos = lpos;
+  P.lu = (double *)malloc((lpos[nblk] > 0 ? lpos[nblk] : 1) * sizeof(double));
+  block_factor(&P);
+  int ret = pcg_core(n, ia, ja, a, b, x, iters, tol, res_history, pre_block, &P);
+  free(P.lu); free(bpos); free(lpos);
   return ret;
 }
 

@@ -58,6 +58,7 @@ typedef struct {
   int simplex;       /* dim 3 only: 1 = 6 tetrahedra per cube (Kuhn split)     */
   double bc_val1[6][9]; /* TPZBndCondT::Val1 per face, row-major (nstate x nstate
                            used: 1x1 Poisson/Darcy, 3x3 Elasticity3D)          */
+  double prestress[3];  /* TPZElasticity3D::fPreStress (XX, YY, ZZ)            */
 } orc_problem;
 
 typedef struct {
@@ -180,6 +181,13 @@ void orc_multadd(int64_t n, const int64_t *ia, const int64_t *ja,
 int orc_pcg(int64_t n, const int64_t *ia, const int64_t *ja, const double *a,
             const double *b, double *x, int64_t *iters, double *tol,
             int wasted_spmv, double *res_history);
+
+/* CG preconditioned by the LU-solved consecutive diagonal blocks of bsize[]
+ * equations (EBlockJacobi with overlap); returns -1 if the sizes do not sum
+ * to n, else as orc_pcg. */
+int orc_pcg_block(int64_t n, const int64_t *ia, const int64_t *ja, const double *a,
+                  const double *b, int64_t nblk, const int32_t *bsize, double *x,
+                  int64_t *iters, double *tol, double *res_history);
 
 /* ---- post-processing: error norms (TPZAnalysis::PostProcessError) ------- */
 enum { ORC_EXACT_POINTS = 0, ORC_EXACT_CONST = 1, ORC_EXACT_X = 2, ORC_EXACT_SINSIN = 3,

@@ -1,6 +1,6 @@
 """GPU parity of the boundary-condition types beyond Dirichlet/Neumann
-(TPZMatPoisson type 2, TPZDarcyFlow type 2, TPZElasticity3D types 2-8)
-against the oracle, with the bars of test_gpu_parity (pattern bit-exact,
+(TPZMatPoisson type 2, TPZDarcyFlow type 2, TPZElasticity3D types 2-8) and of
+TPZElasticity3D's prestress load against the oracle, with the bars of test_gpu_parity (pattern bit-exact,
 entries 1e-12 relative, rhs 1e-12 of max|rhs|), and the refusal of types
 outside each material's ContributeBC switch.  The mixed solver's FP32
 operator falls back from matrix-free to the stored block CSR when a
@@ -45,6 +45,12 @@ def _scalar(material, types, vals, val1, p=2, n=6, simplex=0):
     return pb
 
 
+def _prestress(pb):
+    pb.prestress = (0.7, -0.3, 0.2)  # TPZElasticity3D::fPreStress
+    pb.name += "_prestress"
+    return pb
+
+
 Z3 = [0, 0, 0]
 Z9 = [0.0] * 9
 CASES = {
@@ -61,6 +67,9 @@ CASES = {
                                      [Z9, A[0], S[0], Z9, Z9, Z9], n=4, p=1),
     "elast_types_tet": lambda: _elast([0, 2, 4, 5, -1, 8], [Z3, [0, 1, 0], Z3, [1.0, 0, 0], Z3, [0.5, 0, 0.5]],
                                       [Z9, A[0], S[0], Z9, Z9, Z9], n=2, p=2, simplex=1),
+    "elast_prestress_hex": lambda: _prestress(_elast([0, 1, -1, 2, -1, 4], [Z3, [0.1, 0.2, 0.3], Z3, Z3, Z3, Z3],
+                                                     [Z9, Z9, Z9, A[0], Z9, S[0]], n=3, p=2)),
+    "elast_prestress_tet": lambda: _prestress(_elast([0, 1, -1, -1, -1, -1], n=2, p=2, simplex=1)),
     "darcy_robin_q2": lambda: _scalar(P.DARCY, [0, 2, 1, 2, -1, -1],
                                       [Z3, [0.5, 0, 0], [0.25, 0, 0], [-1.0, 0, 0], Z3, Z3],
                                       [Z9, [2.0] + [0.0] * 8, Z9, [0.5] + [0.0] * 8, Z9, Z9]),

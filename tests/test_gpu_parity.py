@@ -203,6 +203,59 @@ def test_pcg64_matches_oracle(name):
     assert st.fTol <= 1e-13
 
 
+@pytest.mark.parametrize("name", ["poisson2d_p2", "elast3d_p2_traction", "elast3d_p1", "elast3d_hp",
+                                  "elast_tet_p2", "darcy_tet_hp"])
+def test_pcg64_block_jacobi_matches_oracle(name):
+    """CG with BuildPreconditioner(EBlockJacobi, true) against the oracle's
+    CG with the LU-solved connect blocks (nstate equations each): solution
+    within 1e-8, iterations within max(3, 5%)."""
+    pb = CASES[name]()
+    print(f"[gpu] PCG64 block Jacobi {name}", flush=True)
+    an = TPZLinearAnalysis(pb, False, ctx=ctx())
+    st = TPZStepSolver()
+    an.SetSolver(st)
+    an.Assemble()
+    assert an.BuildPreconditioner(an.EJacobi, True) is None
+    pre = an.BuildPreconditioner(an.EBlockJacobi, True)
+    st.SetCG(20000, pre, 1e-13, 0)
+    u = an.Solve()
+    o, ia, ja, a, rhs = _oracle(pb)
+    ref = O.pcg_block(ia, ja, a, rhs, np.full(o.neq // pb.nstate, pb.nstate, np.int32),
+                      maxit=20000, tol=1e-13)
+    pj = O.pcg(ia, ja, a, rhs, maxit=20000, tol=1e-13, wasted_spmv=False)
+    print(f"[gpu]   iterations: device {st.fNumIterations}, oracle {ref['iters']}, "
+          f"point Jacobi {pj['iters']}", flush=True)
+    assert ref["converged"] and st.info["converged"]
+    assert np.linalg.norm(u - ref["x"]) <= 1e-8 * np.linalg.norm(ref["x"])
+    assert abs(st.fNumIterations - ref["iters"]) <= max(3, 0.05 * ref["iters"])
+    assert st.fTol <= 1e-13
+    if pb.nstate == 1:  # one equation per connect: point Jacobi
+        assert abs(st.fNumIterations - pj["iters"]) <= max(3, 0.05 * pj["iters"])
+
+
+def test_pcg64_block_jacobi_refusals():
+    """High-order simplex connects carry several shapes (blocks the device
+    does not build): PZG_EUNSUP; PCG64 with a multigrid preconditioner:
+    PZG_EINVAL; the other BuildPreconditioner forms: NotImplementedError."""
+    an = TPZLinearAnalysis(CASES["darcy_tet_hp5"](), False, ctx=ctx())
+    st = TPZStepSolver()
+    an.SetSolver(st)
+    an.Assemble()
+    with pytest.raises(NotImplementedError):
+        an.BuildPreconditioner(an.EElement, True)
+    with pytest.raises(NotImplementedError):
+        an.BuildPreconditioner(an.EBlockJacobi, False)
+    st.SetCG(100, an.BuildPreconditioner(an.EBlockJacobi, True), 1e-10, 0)
+    with pytest.raises(L.PzgError) as e:
+        an.Solve()
+    assert e.value.code == L.PZG_EUNSUP
+    st.SetCG(100, TPZStepSolver.jacobi(), 1e-10, 0)
+    st.precond = L.PRECOND_MULTIGRID
+    with pytest.raises(L.PzgError) as e:
+        an.Solve()
+    assert e.value.code == L.PZG_EINVAL
+
+
 @pytest.mark.parametrize("inner_op", [L.INNER_MATFREE, L.INNER_BSR])
 @pytest.mark.parametrize("name", ["poisson2d_p2", "darcy2d_p1_neumann_h", "elast3d_p2_body",
                                   "elast3d_p1"])

@@ -400,6 +400,56 @@ def test_pcg_matches_direct_and_quadratic_exactness():
     assert np.linalg.norm(res["x"] - xd) <= 1e-8 * np.linalg.norm(xd)
 
 
+def test_pcg_block_jacobi_special_cases():
+    """orc_pcg_block (EBlockJacobi with overlap) pinned by what the algebra
+    fixes: blocks of one equation are point Jacobi (the same iterates as
+    orc_pcg, bit for bit); one block holding every equation is the exact
+    inverse (one iteration reaches the direct solution); on a matrix that is
+    block diagonal in blocks of mixed sizes each block is solved exactly (one
+    iteration), which fails for a wrong block offset, a transposed factor or a
+    dropped substitution term."""
+    pb = P.elasticity3d(n=3, p=2, seed=4, field_kind="grf", load="body")
+    o = O.Oracle(pb)
+    ia, ja = o.pattern()
+    a, rhs = o.assemble()
+    n = o.neq
+    r1 = O.pcg(ia, ja, a, rhs, maxit=5000, tol=1e-12, wasted_spmv=False, history=True)
+    r2 = O.pcg_block(ia, ja, a, rhs, np.ones(n, np.int32), maxit=5000, tol=1e-12, history=True)
+    assert r1["iters"] == r2["iters"] and np.array_equal(r1["x"], r2["x"])
+    assert np.array_equal(r1["history"], r2["history"])
+    # the connect blocks (nstate equations) converge, to the same solution
+    r3 = O.pcg_block(ia, ja, a, rhs, np.full(n // 3, 3, np.int32), maxit=5000, tol=1e-12)
+    assert r3["converged"] and np.linalg.norm(r3["x"] - r1["x"]) <= 1e-9 * np.linalg.norm(r1["x"])
+    # one block: the exact solve
+    pb = P.poisson2d(n=4, p=2, seed=2)
+    o = O.Oracle(pb)
+    ia, ja = o.pattern()
+    a, rhs = o.assemble()
+    K = O.upper_to_dense(ia, ja, a)
+    xd = np.linalg.solve(K, rhs)
+    r = O.pcg_block(ia, ja, a, rhs, [o.neq], maxit=10, tol=1e-12)
+    assert r["iters"] == 1 and np.allclose(r["x"], xd, rtol=1e-10, atol=1e-12 * np.abs(xd).max())
+    # block diagonal SPD matrix, blocks of sizes 1..4
+    rng = np.random.default_rng(3)
+    sizes = rng.integers(1, 5, 40).astype(np.int32)
+    nn = int(sizes.sum())
+    D = np.zeros((nn, nn))
+    e = 0
+    for m in sizes:
+        B = rng.standard_normal((m, m))
+        D[e:e + m, e:e + m] = B @ B.T + m * np.eye(m)
+        e += m
+    iu, ju = np.triu_indices(nn)
+    keep = D[iu, ju] != 0
+    iu, ju, av = iu[keep], ju[keep], D[iu, ju][keep]
+    ia_d = np.concatenate([[0], np.cumsum(np.bincount(iu, minlength=nn))]).astype(np.int64)
+    b = rng.standard_normal(nn)
+    r = O.pcg_block(ia_d, ju.astype(np.int64), av, b, sizes, maxit=10, tol=1e-13)
+    assert r["iters"] == 1 and np.allclose(r["x"], np.linalg.solve(D, b), rtol=1e-11, atol=1e-13)
+    with pytest.raises(ValueError):
+        O.pcg_block(ia_d, ju.astype(np.int64), av, b, sizes[:-1])
+
+
 # --------------------------------------------------------------------------- hp (min rule)
 def _hp_problem(kind, seed):
     rng = np.random.default_rng(seed)

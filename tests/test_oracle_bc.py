@@ -215,3 +215,32 @@ def test_types_outside_the_switch_are_refused(material, t):
         pb.bc_type = [0, t, -1, -1, -1, -1]
     with pytest.raises(RuntimeError):
         O.Oracle(pb).assemble()
+
+
+def test_elasticity_prestress_load():
+    """TPZElasticity3D's prestress load ef(3j+k) -= w P_k dphix_k(j)
+    (Material/Elasticity/TPZElasticity3D.cpp:271-278): for a field u with
+    component c equal to the coordinate x_d (interpolated exactly), f . u =
+    -P_c * (d == c) * |domain|; the matrix is unchanged."""
+    P3 = (0.7, -0.3, 0.2)
+
+    def pb(ps):
+        q = P.elasticity3d(n=2, p=2, seed=4, load="traction")
+        q.bc_type = [-1] * 6
+        q.force_kind = P.FORCE_NONE
+        q.prestress = ps
+        return q
+
+    o, K, f = _system(pb(P3))
+    _, K0, f0 = _system(pb((0.0, 0.0, 0.0)))
+    assert abs(K - K0).max() == 0.0 and np.abs(f0).max() == 0.0
+    xyz = np.ctypeslib.as_array(o.m.xyz, shape=(3 * o.m.nnodes,)).reshape(-1, 3)
+    econn, pos, size = o.connects()
+    for c in range(3):
+        for d in range(3):
+            u = np.zeros(o.neq)
+            for e in range(o.m.nvol):
+                for k in range(8):
+                    u[pos[o.m.conn_seq[econn[e, k]]] + c] = xyz[o.m.enodes[8 * e + k], d]
+            expect = -P3[c] if c == d else 0.0
+            assert abs(f @ u - expect) < 1e-13, (c, d, f @ u)
