@@ -57,6 +57,12 @@ def parse():
     ap.add_argument("--no-graph", action="store_true", help="plain launches instead of hipGraph replay")
     ap.add_argument("--inner-op", default="auto", choices=["auto", "bsr", "matfree"],
                     help="FP32 operator of the inner iterations")
+    ap.add_argument("--protocol", default="", choices=["", "c4"],
+                    help="c4: the learned-preconditioner protocol on config C4p1/C4p2 (train on "
+                         "--train-seeds seeded fields, then FCG with the learned cycle against FP64 "
+                         "Jacobi-PCG and the block-Jacobi cycle on --test-seeds held-out fields)")
+    ap.add_argument("--test-seeds", type=int, default=200)
+    ap.add_argument("--train-steps-per-seed", type=int, default=4)
     ap.add_argument("--kernels-only", action="store_true",
                     help="assemble once, then time each operator kernel (5 launches) and a 1 GiB "
                          "device copy (HBM counter calibration); no solve")
@@ -202,11 +208,103 @@ def train_smoother(args, L):
                        f"{args.train_steps} Adam steps each"}
 
 
+def protocol_c4(args):
+    """The C4 protocol: seeded Darcy fields (kappa = exp(GRF), l = 0.05,
+    sigma = 1.5) on one 256^2 quad mesh.  The learned element smoother is
+    trained on the (residual, error) pairs of --train-seeds fields (the
+    least-squares G over all of them, a few Adam steps of the scale network
+    per field); then every one of --test-seeds held-out fields is assembled
+    and solved to the FP64 true residual 1e-8 by FCG with the learned cycle,
+    by the same FCG with the network's scales switched off (G alone), by the
+    same FCG with the block-Jacobi cycle, and by FP64 Jacobi-PCG.  Prints one
+    JSON line with the per-method means, maxima and target checks."""
+    from neopz_amd import _lib as L
+    from neopz_amd import mesh as M
+    from neopz_amd import problems as P
+    from neopz_amd.analysis import Context, TPZSSpStructMatrixHIP, TPZStepSolver
+    from neopz_amd.learned import LearnedSmoother, train_on_seeds
+    name = args.config if args.config in ("C4p1", "C4p2") else "C4p1"
+    over = {"n": args.n} if args.n else {}
+
+    def make(seed):
+        return P.config(name, seed=seed, **over)
+    pb = make(0)
+    ctx = Context(0)
+    sm = TPZSSpStructMatrixHIP(ctx, M.build(pb))
+    K = sm.Create()
+    sm.Assemble(K)
+    t0 = time.perf_counter()
+    seeds = [10_000 + k for k in range(args.train_seeds)]
+    ls, l0, l1 = train_on_seeds(ctx, sm, K, lambda s: make(s).el_coef, seeds, nrhs=4,
+                                steps=args.train_steps_per_seed, lr=3e-3,
+                                progress=lambda k, l: log(f"c4 {name}: {k} fields trained, loss {l:.3e}"))
+    t_train = time.perf_counter() - t0
+    log(f"c4 {name}: trained on {len(seeds)} fields in {t_train:.1f}s, loss {l0:.4f} -> {l1:.4f}")
+    w_net = ls.get_weights()
+    w_g = w_net.copy()
+    w_g[-L.LES_NH:] = 0.0  # output layer at zero: every scale 1, G alone
+
+    def solver(kind, pre=None):
+        st = TPZStepSolver(K)
+        if kind == "pcg64":
+            st.SetCG(200000, TPZStepSolver.jacobi(), args.tol, 0)
+            st.check_every = 100
+        else:
+            st.SetMixedFCG(200000, args.tol, precond=pre)
+            st.check_every = 0
+        st.use_graph = not args.no_graph
+        return st
+
+    runs = {"learned": [], "learned_G_only": [], "multigrid_block_jacobi": [], "jacobi_pcg64": []}
+    for k in range(args.test_seeds):
+        seed = 20_000 + k
+        coef = np.ascontiguousarray(make(seed).el_coef, np.float64)
+        L.check(ctx.lib.pzg_mesh_set_coef(ctx.h, L.ptr(coef, L.dp)))
+        L.check(ctx.lib.pzg_assemble(ctx.h))
+        for key, kind, pre, w in (("learned", "mixed", L.PRECOND_LEARNED, w_net),
+                                  ("learned_G_only", "mixed", L.PRECOND_LEARNED, w_g),
+                                  ("multigrid_block_jacobi", "mixed", L.PRECOND_MULTIGRID, None),
+                                  ("jacobi_pcg64", "pcg64", None, None)):
+            if w is not None:
+                ls.set_weights(w)
+            st = solver(kind, pre)
+            if k == 0:
+                st.Solve(None, want_solution=False)  # warm-up (graphs, buffers)
+            st.Solve(None, want_solution=False)
+            i = st.info
+            runs[key].append((i["iterations"], i["solve_ms"], i["true_residual"], i["fallback"]))
+        if k < 3 or (k + 1) % 50 == 0:
+            log(f"c4 held-out {k + 1}/{args.test_seeds}: " +
+                ", ".join(f"{key} {v[-1][0]} it {v[-1][1]:.1f} ms" for key, v in runs.items()))
+
+    def summary(v):
+        a = np.array(v, dtype=np.float64)
+        return {"iterations_mean": float(a[:, 0].mean()), "iterations_max": int(a[:, 0].max()),
+                "solve_ms_mean": float(a[:, 1].mean()), "solve_ms_max": float(a[:, 1].max()),
+                "true_residual_max": float(a[:, 2].max()), "fallbacks": int(a[:, 3].sum()),
+                "all_meet_tol": bool((a[:, 2] <= args.tol).all())}
+    out = {"protocol": "C4 learned preconditioner", "config": name,
+           "mesh": f"{pb.nx}x{pb.ny} quads, p={pb.p}, neq={ctx.sizes()['neq']}",
+           "train_fields": len(seeds), "test_fields": args.test_seeds,
+           "training_s": t_train, "loss_first": l0, "loss_last": l1,
+           "adam_steps_per_field": args.train_steps_per_seed,
+           "methods": {k: summary(v) for k, v in runs.items()}}
+    m = out["methods"]
+    out["speedup_learned_vs_pcg64"] = m["jacobi_pcg64"]["solve_ms_mean"] / m["learned"]["solve_ms_mean"]
+    out["iteration_ratio_learned_vs_block_jacobi_cycle"] = (m["learned"]["iterations_mean"] /
+                                                           m["multigrid_block_jacobi"]["iterations_mean"])
+    print(json.dumps(out))
+    ctx.close()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.protocol == "c4":
+        protocol_c4(args)
+        return
     from neopz_amd import _lib as L
     from neopz_amd import mesh as M
     from neopz_amd import problems as P
