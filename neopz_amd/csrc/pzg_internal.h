@@ -329,6 +329,12 @@ struct Ctx {
   bool mg_ready = false, mg_use_mf = false;
   bool mg_les = false;     // level 0 smoothed with the learned element smoother
   DevBuf d_bjinv64;        // FP64 inverses of the connect blocks (PCG64 with block Jacobi)
+  // tiled learned smoother on hexahedral lattice meshes (pzg_learned.hip)
+  bool les_tile_ready = false, les_tile_ok = false;
+  int les_tP = 0, les_tn[3] = {0, 0, 0};
+  int16_t les_toff[27] = {}, les_tinv[27] = {};
+  int64_t les_nhalo = 0;
+  DevBuf d_les_cmap, d_les_tmapb, d_les_tmap, d_les_part, d_les_hmapb, d_les_hmap, d_les_hptr, d_les_hsrc;
   DevBuf d_les_blkm;       // element blocks with penalized-component bits (learned apply)
   DevBuf d_les_eoff;       // row-ordered slot -> element-major output offset (learned apply)
   // penalty rows of the learned path, solved in FP64 by the outer refinement

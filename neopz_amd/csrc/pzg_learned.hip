@@ -39,6 +39,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <random>
@@ -471,6 +472,262 @@ __global__ __launch_bounds__(256) void k_les_gather_cheb(int64_t nb, const int64
   }
 }
 
+// ---- tiled application on hexahedral lattice meshes -------------------------------
+// Elements of order P (1 or 2) on a lattice of cells, all with the same
+// shape -> node table: shape k of the element in cell (I, J, K) sits at the
+// fine-lattice node (P I, P J, P K) + off(k).  A workgroup takes a tile of
+// LT_TC^3 cells at a time: it stages r at the tile's (P LT_TC + 1)^3 fine
+// nodes in LDS (zero at penalized dofs), runs the 64 element products
+// s_e G r_e as 16 x 16 x 4 FP32 MFMA tiles with B operands from LDS (16
+// elements per wavefront), keeps the scaled outputs per element in LDS and
+// sums them per node in a fixed order.  Nodes inside the tile box are
+// complete and leave through the output step; nodes on the box faces leave
+// as per-tile partial sums, which k_les_halo adds up in tile order.  Global
+// loads are issued a phase ahead: the next tile's node map and r during this
+// tile's products, the output operands of this tile's inner nodes before them.
+constexpr int LT_TC = 4;
+
+struct LesTileArgs {
+  int n[3], nt[3];          // cells, tiles per direction
+  const int32_t *cmap;      // cell -> element (-1: none)
+  const int32_t *tmap;      // per tile and box node: block | penalized components as bits 28 + k (-1: none)
+  float *part;              // per tile and box node: partial sums of face nodes
+  int64_t nhalo;            // face nodes of the lattice (block order)
+  const int32_t *hmap;      // their block | penalty bits
+  const int32_t *hptr, *hsrc;  // their partial sums: part[(tile NB + q) NS ...] in tile order
+  int16_t off[27];          // shape -> box offset ox + BN (oy + BN oz)
+  int16_t inv[27];          // local node (lx + (P+1)(ly + (P+1) lz)) -> shape
+};
+
+// Output of a row: z = y (mode -1) or the Chebyshev step of LesCheb.
+struct LesOutArgs {
+  int mode;
+  float c1, c2;
+  float *z, *d, *x, *pt;
+  const float *dis;
+};
+
+__device__ inline void les_emit(const LesOutArgs &o, int64_t i, float y) {
+  if (o.mode < 0) {
+    o.z[i] = y;
+    return;
+  }
+  const float dv = o.mode == 2 ? o.c1 * o.d[i] + o.c2 * y : o.c2 * y;
+  o.d[i] = dv;
+  o.x[i] = o.mode == 0 ? dv : o.x[i] + dv;
+  if (o.pt) o.pt[i] = dv * o.dis[i];
+}
+
+template <int P, int NS>
+__global__ __launch_bounds__(256) void k_les_tile(LesTileArgs a, const float *__restrict__ G,
+                                                  const float *__restrict__ scale,
+                                                  const float *__restrict__ r, LesOutArgs out,
+                                                  const SolveScalars *sc) {
+  if (sc && sc->done) return;
+  constexpr int NSH = (P + 1) * (P + 1) * (P + 1), NE = NSH * NS, NT = (NE + 15) / 16, NK = (NE + 3) / 4;
+  constexpr int W = P * LT_TC, BN = W + 1, NB = BN * BN * BN, NEL = LT_TC * LT_TC * LT_TC;
+  constexpr int NQ = (NB + 255) / 256;  // box nodes per thread
+  static_assert(NEL == 64, "four wavefronts of 16 elements");
+  // r at the box nodes (sR) and the scaled element outputs (sY) share one
+  // array: the B operands are in registers before the outputs are written
+  constexpr int NRY = NB * NS > NEL * NE ? NB * NS : NEL * NE;
+  __shared__ float sA[NT * NK * 64];
+  __shared__ float sRY[NRY];
+  __shared__ int16_t sInv[NSH];
+  float *sR = sRY, *sY = sRY;
+  const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, col = l & 15, kq = l >> 4;
+  for (int t = tid; t < NT * NK * 64; t += 256) {
+    const int ll = t & 63, kk = (t >> 6) % NK, ti = (t >> 6) / NK;
+    const int row = 16 * ti + (ll & 15), k = 4 * kk + (ll >> 4);
+    sA[t] = (row < NE && k < NE) ? G[row * NE + k] : 0.f;
+  }
+  if (tid < NSH) sInv[tid] = a.inv[tid];
+  // this lane's B-operand offsets into sR relative to its element's cell
+  int uo[NK];
+#pragma unroll
+  for (int kk = 0; kk < NK; kk++) {
+    const int k = 4 * kk + kq;
+    uo[kk] = k < NE ? a.off[k / NS] * NS + k % NS : -1;
+  }
+  const int le = wv * 16 + col;  // this lane's element of the tile
+  const int cx = le % LT_TC, cy = (le / LT_TC) % LT_TC, cz = le / (LT_TC * LT_TC);
+  const int cb = (P * cx + BN * (P * cy + BN * P * cz)) * NS;
+  // this thread's box nodes, their cells and whether they are inner
+  bool inner[NQ];
+#pragma unroll
+  for (int j = 0; j < NQ; j++) {
+    const int q = tid + 256 * j;
+    const int bx = q % BN, by = (q / BN) % BN, bz = q / (BN * BN);
+    inner[j] = q < NB && bx > 0 && bx < W && by > 0 && by < W && bz > 0 && bz < W;
+  }
+  const int64_t ntile = (int64_t)a.nt[0] * a.nt[1] * a.nt[2];
+  int64_t lo, hi;
+  int xi, xg;
+  xcd_slab(ntile, lo, hi, xi, xg);
+  int32_t mq[NQ];
+  float rq[NQ][NS];
+  auto load_map = [&](int64_t tile) {
+#pragma unroll
+    for (int j = 0; j < NQ; j++) {
+      const int q = tid + 256 * j;
+      mq[j] = (tile < hi && q < NB) ? a.tmap[tile * NB + q] : -1;
+    }
+  };
+  auto load_r = [&]() {
+#pragma unroll
+    for (int j = 0; j < NQ; j++)
+#pragma unroll
+      for (int k = 0; k < NS; k++) rq[j][k] = mq[j] >= 0 ? r[(int64_t)(mq[j] & 0x0FFFFFFF) * NS + k] : 0.f;
+  };
+  load_map(lo + xi);
+  load_r();
+  for (int64_t tile = lo + xi; tile < hi; tile += xg) {
+    const int tx = (int)(tile % a.nt[0]), ty = (int)((tile / a.nt[0]) % a.nt[1]),
+              tz = (int)(tile / ((int64_t)a.nt[0] * a.nt[1]));
+    __syncthreads();  // the previous tile's LDS reads are done (and sA is staged)
+    int32_t m[NQ];
+    float rv[NQ][NS];
+#pragma unroll
+    for (int j = 0; j < NQ; j++) {
+      const int q = tid + 256 * j;
+      m[j] = mq[j];
+#pragma unroll
+      for (int k = 0; k < NS; k++) {
+        rv[j][k] = rq[j][k];
+        if (q < NB) sR[q * NS + k] = ((m[j] >> (28 + k)) & 1) ? 0.f : rq[j][k];
+      }
+    }
+    // output operands of the inner nodes
+    float od[NQ][NS], ox[NQ][NS], os[NQ][NS];
+#pragma unroll
+    for (int j = 0; j < NQ; j++)
+#pragma unroll
+      for (int k = 0; k < NS; k++) {
+        const bool ok = inner[j] && m[j] >= 0 && out.mode >= 0;
+        const int64_t i = (int64_t)(m[j] & 0x0FFFFFFF) * NS + k;
+        od[j][k] = ok && out.mode == 2 ? out.d[i] : 0.f;
+        ox[j][k] = ok && out.mode >= 1 ? out.x[i] : 0.f;
+        os[j][k] = ok && out.pt ? out.dis[i] : 0.f;
+      }
+    load_map(tile + xg);  // the next tile's node map, in flight during the products
+    __syncthreads();
+    {
+      const int ex = LT_TC * tx + cx, ey = LT_TC * ty + cy, ez = LT_TC * tz + cz;
+      int32_t e = -1;
+      if (ex < a.n[0] && ey < a.n[1] && ez < a.n[2]) e = a.cmap[ex + (int64_t)a.n[0] * (ey + (int64_t)a.n[1] * ez)];
+      const float s = e >= 0 ? scale[e] : 0.f;
+      float u[NK];
+#pragma unroll
+      for (int kk = 0; kk < NK; kk++) u[kk] = uo[kk] >= 0 ? sR[cb + uo[kk]] : 0.f;
+      __syncthreads();  // every B operand read before sY overwrites sR
+#pragma unroll 1
+      for (int ti = 0; ti < NT; ti++) {
+        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kk = 0; kk < NK; kk++)
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(sA[(ti * NK + kk) * 64 + l], u[kk], acc, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int o = 16 * ti + 4 * kq + i;
+          if (o < NE) sY[le * NE + o] = s * acc[i];
+        }
+      }
+    }
+    load_r();  // the next tile's r, in flight during the sums below
+    __syncthreads();
+    // per box node: the sum over the tile's elements holding it (cells in
+    // z, y, x order)
+#pragma unroll
+    for (int j = 0; j < NQ; j++) {
+      const int q = tid + 256 * j;
+      if (q >= NB) continue;
+      const int b[3] = {q % BN, (q / BN) % BN, q / (BN * BN)};
+      int c0[3], c1[3];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        c1[d] = min(b[d] / P, LT_TC - 1);
+        c0[d] = (b[d] % P == 0 && b[d] > 0) ? b[d] / P - 1 : c1[d];
+      }
+      float acc[NS];
+#pragma unroll
+      for (int k = 0; k < NS; k++) acc[k] = 0.f;
+      for (int z = c0[2]; z <= c1[2]; z++)
+        for (int y = c0[1]; y <= c1[1]; y++)
+          for (int x = c0[0]; x <= c1[0]; x++) {
+            const int lx = b[0] - P * x, ly = b[1] - P * y, lz = b[2] - P * z;
+            const int sh = sInv[lx + (P + 1) * (ly + (P + 1) * lz)];
+            const float *yy = sY + (x + LT_TC * (y + LT_TC * z)) * NE + sh * NS;
+#pragma unroll
+            for (int k = 0; k < NS; k++) acc[k] += yy[k];
+          }
+      if (inner[j]) {
+        if (m[j] >= 0) {
+          const int64_t B = m[j] & 0x0FFFFFFF;
+#pragma unroll
+          for (int k = 0; k < NS; k++) {
+            const int64_t i = B * NS + k;
+            const float y = ((m[j] >> (28 + k)) & 1) ? rv[j][k] : acc[k];
+            if (out.mode < 0) {
+              out.z[i] = y;
+            } else {
+              const float dv = out.mode == 2 ? out.c1 * od[j][k] + out.c2 * y : out.c2 * y;
+              out.d[i] = dv;
+              out.x[i] = out.mode == 0 ? dv : ox[j][k] + dv;
+              if (out.pt) out.pt[i] = dv * os[j][k];
+            }
+          }
+        }
+      } else {
+        float *pp = a.part + (tile * NB + q) * NS;
+#pragma unroll
+        for (int k = 0; k < NS; k++) pp[k] = acc[k];
+      }
+    }
+  }
+}
+
+// Fine nodes on tile faces, in block order: the partial sums of the tiles
+// holding them, in tile order, then the output step.
+template <int NS>
+__global__ __launch_bounds__(256) void k_les_halo(LesTileArgs a, const float *__restrict__ r, LesOutArgs out,
+                                                  const SolveScalars *sc) {
+  if (sc && sc->done) return;
+  for (int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; h < a.nhalo;
+       h += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t m = a.hmap[h];
+    const int32_t p0 = a.hptr[h], p1 = a.hptr[h + 1];
+    float acc[NS];
+#pragma unroll
+    for (int k = 0; k < NS; k++) acc[k] = 0.f;
+    for (int32_t p = p0; p < p1; p++) {
+      const float *pp = a.part + (int64_t)a.hsrc[p] * NS;
+#pragma unroll
+      for (int k = 0; k < NS; k++) acc[k] += pp[k];
+    }
+    const int64_t B = m & 0x0FFFFFFF;
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+      const int64_t i = B * NS + k;
+      les_emit(out, i, ((m >> (28 + k)) & 1) ? r[i] : acc[k]);
+    }
+  }
+}
+
+// tmap / hmap = the node maps with the penalized components of each block
+// as bits 28 + k
+__global__ void k_les_penmap(int64_t n, int ns, const int32_t *__restrict__ src, const float *__restrict__ pen,
+                             int32_t *__restrict__ dst) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int32_t b = src[t];
+  int32_t m = b;
+  if (b >= 0)
+    for (int k = 0; k < ns; k++)
+      if (pen[(int64_t)b * ns + k] != 0.f) m |= 1 << (28 + k);
+  dst[t] = m;
+}
+
+
 // ---- training kernels ------------------------------------------------------------
 // S_e (FP64, ne x ne, row-major) of every element from the FP64 full block
 // CSR of K scaled by D^-1/2; rows / columns of penalized dofs and of absent
@@ -643,6 +900,15 @@ __global__ void k_les_sum(int64_t n, const double *__restrict__ v, double *__res
 
 // ============================================================================
 // host side
+// PZG_LES_TILES=0 keeps the element-major kernels on lattice meshes too
+static bool les_tiles_on() {
+  static const bool on = [] {
+    const char *v = std::getenv("PZG_LES_TILES");
+    return !(v && v[0] == '0');
+  }();
+  return on;
+}
+
 static int grid1(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + TPB - 1) / TPB, 1 << 20)); }
 
 static bool les_supported(const Ctx &c) {
@@ -822,6 +1088,172 @@ int les_setup(Ctx &c, uint64_t seed) {
 }
 
 // per-element scales of the current assembly and the apply buffers
+// Lattice structure for k_les_tile (once per mesh): every volume element a
+// hexahedron of order 1 or 2 filling one cell of the lattice of its corner
+// coordinates, all with the same shape -> node table, connects shared
+// consistently.  Anything else keeps the element-major kernels.
+static int les_tile_structure(Ctx &c) {
+  c.les_tile_ready = true;
+  c.les_tile_ok = false;
+  const int nsh = c.tvol.nsh;
+  const int P = nsh == 27 ? 2 : nsh == 8 ? 1 : 0;
+  if (c.dim != 3 || c.vol_etype != PZG_HEX || P == 0 || c.vol_ncorner != 8 || (c.bs != 1 && c.bs != 3) ||
+      c.nvol == 0)
+    return PZG_OK;
+  std::vector<double> u[3];
+  for (int d = 0; d < 3; d++) {
+    std::vector<double> x;
+    x.reserve(c.nvol * 8);
+    for (int64_t e = 0; e < c.nvol; e++)
+      for (int k = 0; k < 8; k++) x.push_back(c.xyz[3 * c.vol_nodes[e * 8 + k] + d]);
+    std::sort(x.begin(), x.end());
+    const double tol = 1e-9 * std::max(1.0, std::fabs(x.back() - x.front()));
+    for (double v : x)
+      if (u[d].empty() || v - u[d].back() > tol) u[d].push_back(v);
+    if (u[d].size() < 2) return PZG_OK;
+  }
+  int n[3], fn[3];
+  for (int d = 0; d < 3; d++) {
+    n[d] = (int)u[d].size() - 1;
+    fn[d] = P * n[d] + 1;
+  }
+  const int64_t ncell = (int64_t)n[0] * n[1] * n[2], nfine = (int64_t)fn[0] * fn[1] * fn[2];
+  if (ncell != c.nvol || nfine >= ((int64_t)1 << 31)) return PZG_OK;
+  auto index = [&](int d, double v) {
+    const double tol = 1e-9 * std::max(1.0, std::fabs(u[d].back() - u[d].front()));
+    auto it = std::lower_bound(u[d].begin(), u[d].end(), v - tol);
+    return (int)std::min<int64_t>(it - u[d].begin(), (int64_t)u[d].size() - 1);
+  };
+  std::vector<int32_t> cmap(ncell, -1), fmapb(nfine, -2);
+  int16_t off[27], inv[27];
+  for (int k = 0; k < 27; k++) inv[k] = -1;
+  const int BN = P * LT_TC + 1, NSD = c.vol_nsides;
+  for (int64_t e = 0; e < c.nvol; e++) {
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; d++) {
+      lo[d] = INT32_MAX;
+      hi[d] = -1;
+    }
+    for (int k = 0; k < 8; k++)
+      for (int d = 0; d < 3; d++) {
+        const int i = index(d, c.xyz[3 * c.vol_nodes[e * 8 + k] + d]);
+        lo[d] = std::min(lo[d], i);
+        hi[d] = std::max(hi[d], i);
+      }
+    for (int d = 0; d < 3; d++)
+      if (hi[d] != lo[d] + 1) return PZG_OK;
+    const int64_t cell = lo[0] + (int64_t)n[0] * (lo[1] + (int64_t)n[1] * lo[2]);
+    if (cmap[cell] != -1) return PZG_OK;
+    cmap[cell] = (int32_t)e;
+    for (int s = 0; s < nsh; s++) {
+      const int side = c.tvol.shape_side[s];
+      int loc[8];
+      const int nl = side_corner_list_pub(PZG_HEX, side, loc);
+      int o[3];
+      for (int d = 0; d < 3; d++) {
+        double m = 0.;
+        for (int k = 0; k < nl; k++) m += c.xyz[3 * c.vol_nodes[e * 8 + loc[k]] + d];
+        m /= nl;
+        const double rel = (m - u[d][lo[d]]) / (u[d][lo[d] + 1] - u[d][lo[d]]) * P;
+        o[d] = (int)std::lround(rel);
+        if (std::fabs(rel - o[d]) > 1e-6 || o[d] < 0 || o[d] > P) return PZG_OK;
+      }
+      const int16_t bo = (int16_t)(o[0] + BN * (o[1] + BN * o[2]));
+      const int16_t lp = (int16_t)(o[0] + (P + 1) * (o[1] + (P + 1) * o[2]));
+      if (e == 0) {
+        off[s] = bo;
+        if (inv[lp] != -1) return PZG_OK;
+        inv[lp] = (int16_t)s;
+      } else if (off[s] != bo) {
+        return PZG_OK;
+      }
+      const int64_t f = (P * lo[0] + o[0]) + (int64_t)fn[0] * ((P * lo[1] + o[1]) + (int64_t)fn[1] * (P * lo[2] + o[2]));
+      const int64_t conn = c.vol_conn[e * NSD + side];
+      const int32_t b = c.aidx[conn] >= 0 ? (int32_t)c.aidx[conn] : -1;  // as the matrix-free block map
+      if (fmapb[f] == -2) fmapb[f] = b;
+      else if (fmapb[f] != b) return PZG_OK;
+    }
+  }
+  for (int k = 0; k < nsh; k++)
+    if (inv[k] < 0) return PZG_OK;
+  for (int64_t f = 0; f < nfine; f++)
+    if (fmapb[f] == -2) fmapb[f] = -1;
+  int nt[3];
+  for (int d = 0; d < 3; d++) nt[d] = (n[d] + LT_TC - 1) / LT_TC;
+  const int W = P * LT_TC, NB = BN * BN * BN;
+  const int64_t ntile = (int64_t)nt[0] * nt[1] * nt[2];
+  if (ntile * NB >= ((int64_t)1 << 31)) return PZG_OK;
+  // per tile and box node: the block (-1 outside the lattice)
+  std::vector<int32_t> tmapb(ntile * NB, -1);
+  for (int64_t t = 0; t < ntile; t++) {
+    const int tx = (int)(t % nt[0]), ty = (int)((t / nt[0]) % nt[1]), tz = (int)(t / ((int64_t)nt[0] * nt[1]));
+    for (int q = 0; q < NB; q++) {
+      const int fx = W * tx + q % BN, fy = W * ty + (q / BN) % BN, fz = W * tz + q / (BN * BN);
+      if (fx < fn[0] && fy < fn[1] && fz < fn[2])
+        tmapb[t * NB + q] = fmapb[fx + (int64_t)fn[0] * (fy + (int64_t)fn[1] * fz)];
+    }
+  }
+  // nodes on tile faces (a coordinate a multiple of W), in block order, with
+  // the (tile, box node) slots of their partial sums in tile order
+  std::vector<std::pair<int32_t, int64_t>> face;
+  for (int64_t f = 0; f < nfine; f++) {
+    const int fx = (int)(f % fn[0]), fy = (int)((f / fn[0]) % fn[1]), fz = (int)(f / ((int64_t)fn[0] * fn[1]));
+    if ((fx % W) && (fy % W) && (fz % W)) continue;
+    if (fmapb[f] >= 0) face.push_back({fmapb[f], f});
+  }
+  std::sort(face.begin(), face.end());
+  std::vector<int32_t> hmapb(face.size()), hptr(face.size() + 1, 0), hsrc;
+  hsrc.reserve(face.size() * 3);
+  for (size_t h = 0; h < face.size(); h++) {
+    const int64_t f = face[h].second;
+    const int fc[3] = {(int)(f % fn[0]), (int)((f / fn[0]) % fn[1]), (int)(f / ((int64_t)fn[0] * fn[1]))};
+    int t0[3], t1[3];
+    for (int d = 0; d < 3; d++) {
+      const int t = fc[d] / W;
+      if (fc[d] % W) {
+        t0[d] = t1[d] = t;
+      } else {
+        t0[d] = t > 0 ? t - 1 : t;
+        t1[d] = t < nt[d] ? t : t - 1;
+      }
+    }
+    for (int z = t0[2]; z <= t1[2]; z++)
+      for (int y = t0[1]; y <= t1[1]; y++)
+        for (int x = t0[0]; x <= t1[0]; x++) {
+          const int q = (fc[0] - W * x) + BN * ((fc[1] - W * y) + BN * (fc[2] - W * z));
+          hsrc.push_back((int32_t)((((int64_t)z * nt[1] + y) * nt[0] + x) * NB + q));
+        }
+    hmapb[h] = face[h].first;
+    hptr[h + 1] = (int32_t)hsrc.size();
+  }
+  const int64_t nh = (int64_t)face.size();
+  PZG_TRY(dev_alloc(c.d_les_cmap, sizeof(int32_t) * ncell));
+  PZG_TRY(dev_alloc(c.d_les_tmapb, sizeof(int32_t) * ntile * NB));
+  PZG_TRY(dev_alloc(c.d_les_tmap, sizeof(int32_t) * ntile * NB));
+  PZG_TRY(dev_alloc(c.d_les_part, sizeof(float) * ntile * NB * c.bs));
+  PZG_TRY(dev_alloc(c.d_les_hmapb, sizeof(int32_t) * std::max<int64_t>(nh, 1)));
+  PZG_TRY(dev_alloc(c.d_les_hmap, sizeof(int32_t) * std::max<int64_t>(nh, 1)));
+  PZG_TRY(dev_alloc(c.d_les_hptr, sizeof(int32_t) * (nh + 1)));
+  PZG_TRY(dev_alloc(c.d_les_hsrc, sizeof(int32_t) * std::max<size_t>(hsrc.size(), 1)));
+  PZG_HIP(hipMemcpyAsync(c.d_les_cmap.p, cmap.data(), sizeof(int32_t) * ncell, hipMemcpyHostToDevice, c.stream));
+  PZG_HIP(hipMemcpyAsync(c.d_les_tmapb.p, tmapb.data(), sizeof(int32_t) * ntile * NB, hipMemcpyHostToDevice,
+                         c.stream));
+  if (nh) {
+    PZG_HIP(hipMemcpyAsync(c.d_les_hmapb.p, hmapb.data(), sizeof(int32_t) * nh, hipMemcpyHostToDevice, c.stream));
+    PZG_HIP(hipMemcpyAsync(c.d_les_hsrc.p, hsrc.data(), sizeof(int32_t) * hsrc.size(), hipMemcpyHostToDevice,
+                           c.stream));
+  }
+  PZG_HIP(hipMemcpyAsync(c.d_les_hptr.p, hptr.data(), sizeof(int32_t) * (nh + 1), hipMemcpyHostToDevice, c.stream));
+  PZG_HIP(hipStreamSynchronize(c.stream));
+  c.les_nhalo = nh;
+  c.les_tP = P;
+  for (int d = 0; d < 3; d++) c.les_tn[d] = n[d];
+  std::memcpy(c.les_toff, off, sizeof(off));
+  std::memcpy(c.les_tinv, inv, sizeof(inv));
+  c.les_tile_ok = true;
+  return PZG_OK;
+}
+
 int les_prepare(Ctx &c) {
   if (!c.les_ready) {
     set_error("learned preconditioner: call pzg_learned_setup first");
@@ -843,6 +1275,20 @@ int les_prepare(Ctx &c) {
     hipLaunchKernelGGL(k_les_blkm, dim3((unsigned)std::max<int64_t>(1, (nt + TPB - 1) / TPB)), dim3(TPB), 0,
                        c.stream, nt, c.bs, c.d_mf_blk.as<int32_t>(), c.d_les_pen.as<float>(),
                        c.d_les_blkm.as<int32_t>());
+    PZG_HIP(hipGetLastError());
+  }
+  if (!c.les_tile_ready) PZG_TRY(les_tile_structure(c));
+  if (c.les_tile_ok) {  // the node maps with this system's penalized components
+    const int BN = c.les_tP * LT_TC + 1;
+    int64_t ntile = 1;
+    for (int d = 0; d < 3; d++) ntile *= (c.les_tn[d] + LT_TC - 1) / LT_TC;
+    const int64_t nq = ntile * BN * BN * BN;
+    hipLaunchKernelGGL(k_les_penmap, dim3((unsigned)((nq + TPB - 1) / TPB)), dim3(TPB), 0, c.stream, nq, c.bs,
+                       c.d_les_tmapb.as<int32_t>(), c.d_les_pen.as<float>(), c.d_les_tmap.as<int32_t>());
+    if (c.les_nhalo)
+      hipLaunchKernelGGL(k_les_penmap, dim3((unsigned)((c.les_nhalo + TPB - 1) / TPB)), dim3(TPB), 0, c.stream,
+                         c.les_nhalo, c.bs, c.d_les_hmapb.as<int32_t>(), c.d_les_pen.as<float>(),
+                         c.d_les_hmap.as<int32_t>());
     PZG_HIP(hipGetLastError());
   }
   PZG_TRY(les_features(c));
@@ -873,6 +1319,50 @@ int les_apply(Ctx &c, const float *r, float *z, const SolveScalars *sc, const Le
   const float *pen = c.d_les_pen.as<float>(), *s = c.d_les_s.as<float>();
   float *ye = c.d_les_ye.as<float>();
   const int64_t nv = c.nvol;
+  if (c.les_tile_ok && les_tiles_on()) {
+    LesTileArgs a;
+    for (int d = 0; d < 3; d++) {
+      a.n[d] = c.les_tn[d];
+      a.nt[d] = (c.les_tn[d] + LT_TC - 1) / LT_TC;
+    }
+    a.cmap = c.d_les_cmap.as<int32_t>();
+    a.tmap = c.d_les_tmap.as<int32_t>();
+    a.part = c.d_les_part.as<float>();
+    a.nhalo = c.les_nhalo;
+    a.hmap = c.d_les_hmap.as<int32_t>();
+    a.hptr = c.d_les_hptr.as<int32_t>();
+    a.hsrc = c.d_les_hsrc.as<int32_t>();
+    std::memcpy(a.off, c.les_toff, sizeof(a.off));
+    std::memcpy(a.inv, c.les_tinv, sizeof(a.inv));
+    LesOutArgs o;
+    std::memset(&o, 0, sizeof(o));
+    o.mode = -1;
+    o.z = z;
+    if (ch) {
+      o.mode = ch->mode;
+      o.c1 = ch->c1;
+      o.c2 = ch->c2;
+      o.d = ch->d;
+      o.x = ch->x;
+      o.pt = ch->pt;
+      o.dis = ch->dis;
+    }
+    const int64_t ntile = (int64_t)a.nt[0] * a.nt[1] * a.nt[2];
+    const int gt = (int)std::max<int64_t>(1, std::min<int64_t>(ntile, 768));
+    const int gh = grid1(std::max<int64_t>(a.nhalo, 1));
+#define LES_TILE(PP, NSV)                                                                                   \
+  do {                                                                                                    \
+    hipLaunchKernelGGL((k_les_tile<PP, NSV>), dim3(gt), dim3(256), 0, c.stream, a, G, s, r, o, sc);         \
+    hipLaunchKernelGGL((k_les_halo<NSV>), dim3(gh), dim3(256), 0, c.stream, a, r, o, sc);                   \
+  } while (0)
+    if (c.les_tP == 2 && c.bs == 3) LES_TILE(2, 3);
+    else if (c.les_tP == 2) LES_TILE(2, 1);
+    else if (c.bs == 3) LES_TILE(1, 3);
+    else LES_TILE(1, 1);
+#undef LES_TILE
+    PZG_HIP(hipGetLastError());
+    return PZG_OK;
+  }
   const int gm = (int)std::max<int64_t>(1, std::min<int64_t>((nv + 63) / 64, 4096));
   if (nsh == 27 && c.bs == 3)
     hipLaunchKernelGGL((k_les_mfma<27, 3>), dim3(gm), dim3(256), 0, c.stream, nv, G, s, blkm, r, ye, sc);

@@ -4,8 +4,10 @@
   (oracle/les_ref.py, pinned by finite differences): loss within 1e-4
   relative, every parameter block's gradient within 2e-3 relative in the
   2-norm (FP32 kernels).
-* The smoother application (MFMA element kernel + row gather) against the
-  numpy statement within 1e-5 relative.
+* The smoother application against the numpy statement within 1e-5
+  relative: on hexahedral lattice meshes the tiled kernel (4^3 cells per
+  workgroup, partial tiles and whole tiles, vector and scalar dofs), on the
+  quadrilateral meshes the element-major kernel and its row gather.
 * Training on the (residual, error) pairs of seeded training systems lowers
   the loss.
 * The learned cycle against the numpy V-cycle of tests/mg_model.py with the
@@ -54,6 +56,12 @@ CASES = {
     "darcy2d_p2": lambda: P.darcy2d(n=12, p=2, seed=1),
     "darcy2d_p1": lambda: P.darcy2d(n=16, p=1, seed=3),
     "elast3d_p1": lambda: P.elasticity3d(n=5, p=1, seed=4, field_kind="grf", load="body"),
+    # lattice tiles of 4^3 cells: whole tiles (8^3) and scalar dofs
+    "elast3d_p1_8": lambda: P.elasticity3d(n=8, p=1, seed=6, field_kind="grf", load="body"),
+    "poisson3d_p2": lambda: P.Problem(material=P.POISSON, dim=3, p=2, nx=6, ny=4, nz=5,
+                                      el_coef=np.exp(np.random.default_rng(8).standard_normal(120)),
+                                      force_kind=P.FORCE_CONST, force_const=(1.0, 0, 0),
+                                      bc_type=[0, 0, 1, 1, 0, -1]),
 }
 
 
