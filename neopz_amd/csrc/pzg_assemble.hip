@@ -1442,9 +1442,8 @@ int launch_diag(Ctx &c) {
 }
 
 // --------------------------------------------------------------------------
-// Full (both triangles) FP64 block CSR of K, from the reference's upper CSR:
-// diagonal blocks (k_bsr64_diag) and coupled blocks with their transposes
-// (k_bsr64_off).  Values at bsr_pos (chunks of BSR_CHUNK blocks per row,
+// Full (both triangles) FP64 block CSR of K, from the reference's upper CSR
+// (k_bsr_gather).  Values at bsr_pos (chunks of BSR_CHUNK blocks per row,
 // component-major inside a chunk).
 __global__ void k_nlow(int64_t nb, const int64_t *__restrict__ fbia, const int64_t *__restrict__ ubia,
                        int32_t *__restrict__ nlow) {
@@ -1453,74 +1452,101 @@ __global__ void k_nlow(int64_t nb, const int64_t *__restrict__ fbia, const int64
 }
 
 // nlow[A] = number of lower blocks of block row A (= rank of its diagonal block)
-template <int BS>
-__global__ void k_bsr64_diag(int64_t nb, const int64_t *__restrict__ ia, const double *__restrict__ a,
-                             const int32_t *__restrict__ nlow, const int64_t *__restrict__ fbia,
-                             double *__restrict__ v) {
-  int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+
+
+// Row-gather form of the expansion: one wavefront per block row A writes the
+// row's storage in order (coalesced), each lane one value.  Upper blocks
+// come from row A's scalar rows, lower blocks (A, B), B < A, from the upper
+// block (B, A) in row B (lowsrc: its index among the upper blocks).  SCALED:
+// the FP32 values of D^-1/2 K D^-1/2, diagonal blocks left to k_bsr_diag.
+template <int BS, typename T, bool SCALED>
+__global__ __launch_bounds__(256) void k_bsr_gather(int64_t nb, const int64_t *__restrict__ ia,
+                                                    const double *__restrict__ a,
+                                                    const int64_t *__restrict__ ubia,
+                                                    const int64_t *__restrict__ fbia,
+                                                    const int32_t *__restrict__ fbja,
+                                                    const int32_t *__restrict__ lowsrc,
+                                                    const double *__restrict__ dis, T *__restrict__ v) {
+  const int64_t A = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
   if (A >= nb) return;
   constexpr int BB = BS * BS;
-  const int64_t b0 = fbia[A], n = fbia[A + 1] - b0, i = nlow[A];
-  for (int k = 0; k < BS; k++)
-    for (int l = k; l < BS; l++) {
-      const double x = a[ia[A * BS + k] + (l - k)];
-      v[bsr_pos<BB>(b0, n, i, k * BS + l)] = x;
-      v[bsr_pos<BB>(b0, n, i, l * BS + k)] = x;
+  const int64_t b0 = fbia[A], u0 = ubia[A];
+  const int n = (int)(fbia[A + 1] - b0), nl = n - 1 - (int)(ubia[A + 1] - u0);
+  const int64_t lb = b0 - A - u0;  // first lower block of row A in lowsrc
+  for (int p = lane; p < n * BB; p += 64) {
+    const int j0 = p / (BB * BSR_CHUNK) * BSR_CHUNK, m = min(BSR_CHUNK, n - j0), w = p - j0 * BB;
+    const int kl = w / m, j = j0 + w % m, k = kl / BS, l = kl % BS;
+    double x;
+    if (j == nl) {
+      if (SCALED) continue;
+      x = k <= l ? a[ia[A * BS + k] + (l - k)] : a[ia[A * BS + l] + (k - l)];
+    } else if (j > nl) {
+      const int t = j - nl - 1;
+      x = a[ia[A * BS + k] + BS + (int64_t)t * BS - k + l];
+      if (SCALED) x = x * dis[A * BS + k] * dis[(int64_t)fbja[b0 + j] * BS + l];
+    } else {
+      const int64_t u = lowsrc[lb + j], B = fbja[b0 + j];
+      const int64_t t = u - ubia[B];
+      x = a[ia[B * BS + l] + BS + t * BS - l + k];
+      if (SCALED) x = x * dis[B * BS + l] * dis[A * BS + k];
     }
+    v[b0 * BB + p] = (T)x;
+  }
 }
 
-template <int BS>
-__global__ void k_bsr64_off(int64_t nb, const int64_t *__restrict__ ia, const double *__restrict__ a,
-                            const int64_t *__restrict__ ubia, const int32_t *__restrict__ ubja,
-                            const int32_t *__restrict__ rfwd, const int32_t *__restrict__ rbwd,
-                            const int64_t *__restrict__ fbia, double *__restrict__ v) {
-  int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  int lane = threadIdx.x & 63;
+// lowsrc[lower block of (A, B)] = index of the upper block (B, A)
+__global__ void k_lowsrc(int64_t nb, const int64_t *__restrict__ ubia, const int32_t *__restrict__ ubja,
+                         const int32_t *__restrict__ rbwd, const int64_t *__restrict__ fbia,
+                         int32_t *__restrict__ lowsrc) {
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
   if (wave >= nb) return;
-  const int64_t A = wave, u0 = ubia[A], nup = ubia[A + 1] - u0;
-  constexpr int BB = BS * BS;
-  for (int64_t e = lane; e < nup * BB; e += 64) {
-    int64_t t = e / BB;
-    int k = (int)((e % BB) / BS), l = (int)(e % BS);
-    int64_t B = ubja[u0 + t];
-    double x = a[ia[A * BS + k] + BS + t * BS - k + l];
-    v[bsr_pos<BB>(fbia[A], fbia[A + 1] - fbia[A], rfwd[u0 + t], k * BS + l)] = x;
-    v[bsr_pos<BB>(fbia[B], fbia[B + 1] - fbia[B], rbwd[u0 + t], l * BS + k)] = x;
+  const int64_t B = wave;
+  for (int64_t u = ubia[B] + lane; u < ubia[B + 1]; u += 64) {
+    const int64_t A = ubja[u];
+    lowsrc[fbia[A] - A - ubia[A] + rbwd[u]] = (int32_t)u;
   }
+}
+
+static int ensure_lowsrc(Ctx &c) {
+  if (c.lowsrc_ready) return PZG_OK;
+  if (c.nbu >= ((int64_t)1 << 31)) {
+    set_error("block CSR: more than 2^31 upper blocks");
+    return PZG_EUNSUP;
+  }
+  PZG_TRY(dev_alloc(c.d_lowsrc, sizeof(int32_t) * std::max<int64_t>(c.nbu, 1)));
+  const int64_t threads = c.nb * 64;
+  hipLaunchKernelGGL(k_lowsrc, dim3((unsigned)((threads + TPB - 1) / TPB)), dim3(TPB), 0, c.stream, c.nb,
+                     c.d_ubia.as<int64_t>(), c.d_ubja.as<int32_t>(), c.d_rank_bwd.as<int32_t>(),
+                     c.d_fbia.as<int64_t>(), c.d_lowsrc.as<int32_t>());
+  PZG_HIP(hipGetLastError());
+  c.lowsrc_ready = true;
+  return PZG_OK;
 }
 
 int launch_expand_full(Ctx &c) {
   const int bs = c.bs;
   PZG_TRY(dev_alloc(c.d_fa, sizeof(double) * c.nbf * bs * bs));
-  DevBuf nl;
-  PZG_TRY(dev_alloc(nl, sizeof(int32_t) * c.nb));
-  hipLaunchKernelGGL(k_nlow, dim3((c.nb + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
-                     c.d_fbia.as<int64_t>(), c.d_ubia.as<int64_t>(), nl.as<int32_t>());
-  int64_t threads = c.nb * 64;
-  if (bs == 3) {
-    hipLaunchKernelGGL(k_bsr64_diag<3>, dim3((c.nb + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
-                       c.d_ia.as<int64_t>(), c.d_a.as<double>(), nl.as<int32_t>(),
-                       c.d_fbia.as<int64_t>(), c.d_fa.as<double>());
-    hipLaunchKernelGGL(k_bsr64_off<3>, dim3((threads + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
-                       c.d_ia.as<int64_t>(), c.d_a.as<double>(), c.d_ubia.as<int64_t>(),
-                       c.d_ubja.as<int32_t>(), c.d_rank_fwd.as<int32_t>(),
-                       c.d_rank_bwd.as<int32_t>(), c.d_fbia.as<int64_t>(), c.d_fa.as<double>());
-  } else if (bs == 1) {
-    hipLaunchKernelGGL(k_bsr64_diag<1>, dim3((c.nb + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
-                       c.d_ia.as<int64_t>(), c.d_a.as<double>(), nl.as<int32_t>(),
-                       c.d_fbia.as<int64_t>(), c.d_fa.as<double>());
-    hipLaunchKernelGGL(k_bsr64_off<1>, dim3((threads + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
-                       c.d_ia.as<int64_t>(), c.d_a.as<double>(), c.d_ubia.as<int64_t>(),
-                       c.d_ubja.as<int32_t>(), c.d_rank_fwd.as<int32_t>(),
-                       c.d_rank_bwd.as<int32_t>(), c.d_fbia.as<int64_t>(), c.d_fa.as<double>());
-  } else {
-    dev_free(nl);
+  PZG_TRY(ensure_lowsrc(c));
+  const int64_t threads = c.nb * 64;
+  const dim3 g((unsigned)((threads + TPB - 1) / TPB));
+  if (bs == 3)
+    hipLaunchKernelGGL((k_bsr_gather<3, double, false>), g, dim3(TPB), 0, c.stream, c.nb, c.d_ia.as<int64_t>(),
+                       c.d_a.as<double>(), c.d_ubia.as<int64_t>(), c.d_fbia.as<int64_t>(),
+                       c.d_fbja.as<int32_t>(), c.d_lowsrc.as<int32_t>(), (const double *)nullptr,
+                       c.d_fa.as<double>());
+  else if (bs == 1)
+    hipLaunchKernelGGL((k_bsr_gather<1, double, false>), g, dim3(TPB), 0, c.stream, c.nb, c.d_ia.as<int64_t>(),
+                       c.d_a.as<double>(), c.d_ubia.as<int64_t>(), c.d_fbia.as<int64_t>(),
+                       c.d_fbja.as<int32_t>(), c.d_lowsrc.as<int32_t>(), (const double *)nullptr,
+                       c.d_fa.as<double>());
+  else {
     set_error("block CSR: unsupported block size");
     return PZG_EUNSUP;
   }
   PZG_HIP(hipGetLastError());
   PZG_HIP(hipStreamSynchronize(c.stream));
-  dev_free(nl);
   c.full_valid = true;
   return PZG_OK;
 }
@@ -1620,27 +1646,6 @@ __global__ void k_bsr_diag(int64_t nb, int64_t nbf, const int64_t *__restrict__ 
   }
 }
 
-template <int BS>
-__global__ void k_bsr_off(int64_t nb, int64_t nbf, const int64_t *__restrict__ ia,
-                          const double *__restrict__ a,
-                          const int64_t *__restrict__ ubia, const int32_t *__restrict__ ubja,
-                          const int32_t *__restrict__ rfwd, const int32_t *__restrict__ rbwd,
-                          const int64_t *__restrict__ fbia, const double *__restrict__ dis,
-                          float *__restrict__ bsr) {
-  int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  int lane = threadIdx.x & 63;
-  if (wave >= nb) return;
-  const int64_t A = wave, u0 = ubia[A], nup = ubia[A + 1] - u0;
-  constexpr int BB = BS * BS;
-  for (int64_t e = lane; e < nup * BB; e += 64) {
-    int64_t t = e / BB;
-    int k = (int)((e % BB) / BS), l = (int)(e % BS);
-    int64_t B = ubja[u0 + t];
-    double v = a[ia[A * BS + k] + BS + t * BS - k + l] * dis[A * BS + k] * dis[B * BS + l];
-    bsr[bsr_pos<BB>(fbia[A], fbia[A + 1] - fbia[A], rfwd[u0 + t], k * BS + l)] = (float)v;
-    bsr[bsr_pos<BB>(fbia[B], fbia[B + 1] - fbia[B], rbwd[u0 + t], l * BS + k)] = (float)v;
-  }
-}
 
 int launch_build_bsr(Ctx &c, bool values) {
   const int bs = c.bs;
@@ -1648,6 +1653,7 @@ int launch_build_bsr(Ctx &c, bool values) {
   float *bsr = values ? c.d_bsr.as<float>() : nullptr;
   PZG_TRY(dev_alloc(c.d_bjinv, sizeof(float) * c.nb * bs * bs));
   PZG_TRY(dev_alloc(c.d_dinvsqrt, sizeof(double) * c.neq));
+  if (values) PZG_TRY(ensure_lowsrc(c));
   DevBuf nl;
   PZG_TRY(dev_alloc(nl, sizeof(int32_t) * c.nb));
   hipLaunchKernelGGL(k_nlow, dim3((c.nb + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
@@ -1660,21 +1666,19 @@ int launch_build_bsr(Ctx &c, bool values) {
                        c.nbf, c.d_ia.as<int64_t>(), c.d_a.as<double>(), nl.as<int32_t>(),
                        c.d_fbia.as<int64_t>(), c.d_dinvsqrt.as<double>(), bsr, c.d_bjinv.as<float>());
     if (values)
-      hipLaunchKernelGGL(k_bsr_off<3>, dim3((threads + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
-                       c.nbf, c.d_ia.as<int64_t>(), c.d_a.as<double>(), c.d_ubia.as<int64_t>(),
-                       c.d_ubja.as<int32_t>(), c.d_rank_fwd.as<int32_t>(),
-                       c.d_rank_bwd.as<int32_t>(), c.d_fbia.as<int64_t>(),
-                       c.d_dinvsqrt.as<double>(), c.d_bsr.as<float>());
+      hipLaunchKernelGGL((k_bsr_gather<3, float, true>), dim3((threads + TPB - 1) / TPB), dim3(TPB), 0, c.stream,
+                         c.nb, c.d_ia.as<int64_t>(), c.d_a.as<double>(), c.d_ubia.as<int64_t>(),
+                         c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(), c.d_lowsrc.as<int32_t>(),
+                         c.d_dinvsqrt.as<double>(), c.d_bsr.as<float>());
   } else if (bs == 1) {
     hipLaunchKernelGGL(k_bsr_diag<1>, dim3((c.nb + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
                        c.nbf, c.d_ia.as<int64_t>(), c.d_a.as<double>(), nl.as<int32_t>(),
                        c.d_fbia.as<int64_t>(), c.d_dinvsqrt.as<double>(), bsr, c.d_bjinv.as<float>());
     if (values)
-      hipLaunchKernelGGL(k_bsr_off<1>, dim3((threads + TPB - 1) / TPB), dim3(TPB), 0, c.stream, c.nb,
-                       c.nbf, c.d_ia.as<int64_t>(), c.d_a.as<double>(), c.d_ubia.as<int64_t>(),
-                       c.d_ubja.as<int32_t>(), c.d_rank_fwd.as<int32_t>(),
-                       c.d_rank_bwd.as<int32_t>(), c.d_fbia.as<int64_t>(),
-                       c.d_dinvsqrt.as<double>(), c.d_bsr.as<float>());
+      hipLaunchKernelGGL((k_bsr_gather<1, float, true>), dim3((threads + TPB - 1) / TPB), dim3(TPB), 0, c.stream,
+                         c.nb, c.d_ia.as<int64_t>(), c.d_a.as<double>(), c.d_ubia.as<int64_t>(),
+                         c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(), c.d_lowsrc.as<int32_t>(),
+                         c.d_dinvsqrt.as<double>(), c.d_bsr.as<float>());
   } else {
     dev_free(nl);
     set_error("block CSR: unsupported block size");

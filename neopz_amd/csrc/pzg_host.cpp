@@ -56,7 +56,7 @@ template <class F> static void parallel_for(int64_t n, F f) {
 
 static void free_all(Ctx &c) {
   DevBuf *bufs[] = {&c.d_ia, &c.d_ja, &c.d_a, &c.d_rhs, &c.d_diag, &c.d_ubia, &c.d_ubja,
-                    &c.d_fbia, &c.d_fbja, &c.d_rank_fwd, &c.d_rank_bwd,
+                    &c.d_fbia, &c.d_fbja, &c.d_rank_fwd, &c.d_rank_bwd, &c.d_lowsrc,
                     &c.d_fa, &c.d_bsr, &c.d_dinvsqrt, &c.d_bjinv, &c.d_xyz, &c.d_vnodes,
                     &c.d_vconn, &c.d_vcoef, &c.d_bnodes, &c.d_bconn, &c.d_bcid, &c.d_bctype,
                     &c.d_bcval, &c.d_bccoef, &c.d_bcvec, &c.d_aidx, &c.d_bumps, &c.d_vmap, &c.d_bmap, &c.d_tvol,
@@ -841,6 +841,7 @@ static int struct_create(Ctx &c) {
   PZG_TRY(dev_upload(c.d_fbja, fbja.data(), fbja.size(), s));
   PZG_TRY(dev_upload(c.d_rank_fwd, rank_fwd.data(), rank_fwd.size(), s));
   PZG_TRY(dev_upload(c.d_rank_bwd, rank_bwd.data(), rank_bwd.size(), s));
+  c.lowsrc_ready = false;
   PZG_TRY(dev_upload(c.d_ia, ia.data(), ia.size(), s));
   PZG_TRY(dev_alloc(c.d_ja, sizeof(int64_t) * std::max<int64_t>(c.nnz_u, 1)));
   PZG_TRY(dev_alloc(c.d_a, sizeof(double) * std::max<int64_t>(c.nnz_u, 1)));
@@ -1310,6 +1311,7 @@ int pzg_csr_set(pzg_ctx *ctx, int64_t neq, const int64_t *ia, const int64_t *ja,
   PZG_TRY(dev_upload(c.d_fbja, fbja.data(), fbja.size(), s));
   PZG_TRY(dev_upload(c.d_rank_fwd, rank_fwd.data(), rank_fwd.size(), s));
   PZG_TRY(dev_upload(c.d_rank_bwd, rank_bwd.data(), rank_bwd.size(), s));
+  c.lowsrc_ready = false;
   PZG_TRY(dev_upload(c.d_ia, ia, neq + 1, s));
   PZG_TRY(dev_upload(c.d_ja, ja, c.nnz_u, s));
   PZG_TRY(dev_upload(c.d_a, a, c.nnz_u, s));

@@ -272,6 +272,8 @@ struct Ctx {
   DevBuf d_ubia, d_ubja;      // upper block graph: int64 [nb+1], int32 [nbu]
   DevBuf d_fbia, d_fbja;      // full block graph:  int64 [nb+1], int32 [nbf]
   DevBuf d_rank_fwd, d_rank_bwd; // per upper block entry: rank of (a,b) in full row a, (b,a) in row b
+  DevBuf d_lowsrc;               // per lower block (a,b) of the full rows: the upper block (b,a)
+  bool lowsrc_ready = false;
   DevBuf d_fa;                // full block CSR values fp64 (K, unscaled), block-major
   DevBuf d_bsr;               // full block CSR fp32 values, scaled D^-1/2 K D^-1/2
   DevBuf d_dinvsqrt;          // fp64 D^-1/2
