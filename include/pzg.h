@@ -200,7 +200,7 @@ typedef struct {
   int32_t mg_degree;     /* multigrid: Chebyshev degree of the smoothing on the
                             p level (0 = 2; the lattice levels take 1)         */
   double mg_range;       /* multigrid: lmax / lmin of the smoothed interval
-                            (0 = 30)                                           */
+                            (0 = 30; 60 with PZG_PRECOND_LEARNED)              */
 } pzg_solve_params;
 
 /* Operator of the mixed solver's FP32 iterations.  Both approximate the

@@ -229,7 +229,8 @@ class TPZStepSolver:
         INNER_MATFREE: element-by-element, INNER_AUTO: matrix-free for
         mesh-built systems).  precond PRECOND_MULTIGRID preconditions each FCG
         step with a V-cycle (hexahedral lattice meshes; mg_degree and mg_range:
-        Chebyshev degree and interval ratio of the smoothing, 0 = 2 and 30)."""
+        Chebyshev degree and interval ratio of the smoothing, 0 = 2 and 30, 60
+        with PRECOND_LEARNED)."""
         self.kind = self.EMixedFCG
         self.mg_degree = int(mg_degree)
         self.mg_range = float(mg_range)

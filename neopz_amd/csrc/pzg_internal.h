@@ -426,7 +426,7 @@ int les_fit_G(Ctx &c);  // G from the accumulated least-squares sums
 // penalty rows of the learned path (pzg_penalty.hip)
 int pen_setup(Ctx &c);
 int pen_solve(Ctx &c, double *r64, double *u64, double stop_abs, bool use_graph, int64_t *its, double *rr0,
-              double *rr1);
+              double *rr1, double frac = 0., double tot2 = 0.);
 int pen_zero32(Ctx &c, float *r32);
 void pen_free(Ctx &c);
 int mf_setup(Ctx &c);

@@ -147,12 +147,14 @@ __global__ __launch_bounds__(TPB) void k_pen_gather(int64_t np, const int64_t *_
   block_partials<1>(vv, partials);
 }
 
-__global__ void k_pen_start(PenScal *sc, const double *partials, int nblk, double stop_abs) {
+// stop: |r_p| <= max(stop_abs, frac |r_other|), |r_other|^2 = tot2 - |r_p|^2
+__global__ void k_pen_start(PenScal *sc, const double *partials, int nblk, double stop_abs, double frac,
+                            double tot2) {
   const double rr = sum_partials(partials, 0, nblk);
   if (threadIdx.x == 0) {
     sc->rr = rr;
     sc->rr0 = rr;
-    sc->stop2 = std::max(stop_abs * stop_abs, 1e-30 * rr);
+    sc->stop2 = std::max(std::max(stop_abs * stop_abs, frac * frac * (tot2 - rr)), 1e-30 * rr);
     sc->it = 0;
     sc->rz = 0.;
     sc->done = rr <= sc->stop2 ? 1 : 0;
@@ -468,12 +470,14 @@ int pen_setup(Ctx &c) {
 }
 
 // u_p += d = K_pp^-1 r_p (r64: the FP64 residual f - K u of the whole system),
-// by Schwarz-preconditioned CG until ||r_p - K_pp d|| <= stop_abs.  r64's
+// by Schwarz-preconditioned CG until ||r_p - K_pp d|| <= max(stop_abs, frac
+// ||r_other||) (tot2 = ||r||^2 of the whole residual: the other rows' part is
+// what the next FP32 iterations leave, and solving past it gains nothing).  r64's
 // penalty rows receive r_p - K_pp d; its other rows change by -K_ip d, which
 // is left to the next true residual (|K_ip d| ~ |r_p| E/(big h)).  *its: the
 // CG iterations taken; *rr0, *rr1: |r_p|^2 before and after.
 int pen_solve(Ctx &c, double *r64, double *u64, double stop_abs, bool use_graph, int64_t *its, double *rr0,
-              double *rr1) {
+              double *rr1, double frac, double tot2) {
   if (its) *its = 0;
   if (rr0) *rr0 = 0.;
   if (rr1) *rr1 = 0.;
@@ -489,6 +493,7 @@ int pen_solve(Ctx &c, double *r64, double *u64, double stop_abs, bool use_graph,
   PenScal *sc = c.d_pen_scal.as<PenScal>();
   double *part = c.d_partials.as<double>();
   const int g = pen_grid(np);
+  const int gq = pen_grid(np * PEN_L);  // the product: PEN_L lanes per row
   const int gs = (int)((c.pen_nslot + TPB - 1) / TPB);
   const int64_t *idx = c.d_pen_idx.as<int64_t>();
   auto precond = [&](int first) {
@@ -502,16 +507,16 @@ int pen_solve(Ctx &c, double *r64, double *u64, double stop_abs, bool use_graph,
     hipLaunchKernelGGL(k_pen_p, dim3(g), dim3(TPB), 0, s, np, (const double *)z, p, (const PenScal *)sc, first);
   };
   hipLaunchKernelGGL(k_pen_gather, dim3(g), dim3(TPB), 0, s, np, idx, r64, r, x, part);
-  hipLaunchKernelGGL(k_pen_start, dim3(1), dim3(TPB), 0, s, sc, (const double *)part, g, stop_abs);
+  hipLaunchKernelGGL(k_pen_start, dim3(1), dim3(TPB), 0, s, sc, (const double *)part, g, stop_abs, frac, tot2);
   precond(1);
   PZG_HIP(hipGetLastError());
   PenScal hs{};
   auto batch = [&]() {
     for (int k = 0; k < PEN_BATCH; k++) {
-      hipLaunchKernelGGL(k_pen_spmv, dim3(g), dim3(TPB), 0, s, np, c.d_pen_ia.as<int64_t>(),
+      hipLaunchKernelGGL(k_pen_spmv, dim3(gq), dim3(TPB), 0, s, np, c.d_pen_ia.as<int64_t>(),
                          c.d_pen_ja.as<int32_t>(), c.d_pen_a.as<double>(), (const double *)p, q,
                          (const PenScal *)sc, part);
-      hipLaunchKernelGGL(k_pen_alpha, dim3(1), dim3(TPB), 0, s, sc, (const double *)part, g);
+      hipLaunchKernelGGL(k_pen_alpha, dim3(1), dim3(TPB), 0, s, sc, (const double *)part, gq);
       hipLaunchKernelGGL(k_pen_update, dim3(g), dim3(TPB), 0, s, np, x, r, (const double *)p, (const double *)q,
                          (const PenScal *)sc, part);
       hipLaunchKernelGGL(k_pen_check, dim3(1), dim3(TPB), 0, s, sc, (const double *)part, g, (int64_t)PEN_MAXIT);
