@@ -235,6 +235,13 @@ __global__ __launch_bounds__(MF_TPB) void k_mf_vol(MfArgs a, const SolveScalars 
 
 typedef float mf_f4 __attribute__((ext_vector_type(4)));
 
+// s_g and s_x below belong to one wavefront each: between its phases a wave
+// only needs its own LDS stores ordered before its loads (no workgroup barrier)
+__device__ inline void mf_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 struct MfMfmaTab {
   float a1[3][2][7][64];  // phase 1 A operands per (m, row tile, k step, lane)
   float a3[2][21][64];    // phase 3 A operands per (row tile, k step, lane)
@@ -365,7 +372,7 @@ __global__ __launch_bounds__(MF_TPB) void k_mf_hex27_mfma(MfArgs a, const MfMfma
             if (q < 27) s_g[wv][ee][q][kc * 3 + m] = acc[m][r][i];
           }
     }
-    __syncthreads();
+    mf_wave_sync();
     // point-wise: Jacobian, physical gradient, flux/stress, weighted back-map
     for (int pi = l; pi < EPT * 27; pi += 64) {
       const int el = pi / 27, q = pi % 27;
@@ -435,7 +442,7 @@ __global__ __launch_bounds__(MF_TPB) void k_mf_hex27_mfma(MfArgs a, const MfMfma
         for (int m = 0; m < 3; m++)
           gp[k * 3 + m] = wdet * (S[k][0] * inv[m][0] + S[k][1] * inv[m][1] + S[k][2] * inv[m][2]);
     }
-    __syncthreads();
+    mf_wave_sync();
     mf_f4 y[2];
 #pragma unroll
     for (int r = 0; r < 2; r++) y[r] = mf_f4{0.f, 0.f, 0.f, 0.f};
@@ -455,7 +462,7 @@ __global__ __launch_bounds__(MF_TPB) void k_mf_hex27_mfma(MfArgs a, const MfMfma
           if (s < 27 && opos[r][i] >= 0) a.ye[opos[r][i] * NS + kc] = y[r][i];  // ev: enclosing if
         }
     }
-    __syncthreads();
+    mf_wave_sync();
   }
 }
 
