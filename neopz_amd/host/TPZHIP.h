@@ -110,11 +110,14 @@ class TPZSYsmpMatrixHIP {
   // as TPZStepSolver::Solve calls it for ECG with an EJacobi preconditioner
   // (Solvers/pzstepsolver.cpp:107-112): F and result in the matrix numbering;
   // numiterations and tol receive the iterations and the achieved residual.
+  // pcg_precond: the preconditioner step solver's kind for the plain CG
+  // (PZG_PRECOND_JACOBI for EJacobi, PZG_PRECOND_BLOCK_JACOBI for the
+  // EBlockJacobi block diagonal of BuildPreconditioner)
   int SolveCG(int64_t &numiterations, const std::vector<double> *F, std::vector<double> &result,
-              double &tol, pzg_solve_info &info) {
+              double &tol, pzg_solve_info &info, int pcg_precond = PZG_PRECOND_JACOBI) {
     pzg_solve_params p{};
     p.solver = fMode.solver;
-    p.precond = fMode.solver == PZG_SOLVER_PCG64 ? PZG_PRECOND_JACOBI : fMode.precond;
+    p.precond = fMode.solver == PZG_SOLVER_PCG64 ? pcg_precond : fMode.precond;
     p.max_iter = numiterations;
     p.tol = tol;
     p.inner_tol = fMode.inner_tol;
@@ -177,19 +180,28 @@ class TPZFYsmpMatrixHIP : public TPZSYsmpMatrixHIP {
   }
 };
 
-// TPZStepSolver: ECG with an EJacobi(1) preconditioner, or the mixed solver.
+// TPZStepSolver: ECG with an EJacobi(1) preconditioner or with the
+// EBlockJacobi block diagonal of TPZLinearAnalysis::BuildPreconditioner
+// (EDirectBlocks here: a TPZBlockDiagonal solved by ELU), or the mixed solver.
 class TPZStepSolver {
  public:
-  enum MSolver { ENoSolver, EJacobi, ECG, EMixedFCG };
+  enum MSolver { ENoSolver, EJacobi, ECG, EMixedFCG, EDirectBlocks };
   void SetJacobi(int64_t numiterations, double /*tol*/, int FromCurrent) {
     if (numiterations != 1 || FromCurrent) DebugStop();
     fSolver = EJacobi;
   }
   void SetCG(int64_t numiterations, const TPZStepSolver &pre, double tol, int FromCurrent) {
-    if (pre.fSolver != EJacobi || FromCurrent) DebugStop();
+    if ((pre.fSolver != EJacobi && pre.fSolver != EDirectBlocks) || FromCurrent) DebugStop();
     fSolver = ECG;
     fMaxIterations = numiterations;
     fTol = tol;
+    fPcgPrecond = pre.fSolver == EDirectBlocks ? PZG_PRECOND_BLOCK_JACOBI : PZG_PRECOND_JACOBI;
+  }
+  // the step solver BuildPreconditioner(EBlockJacobi, true) returns
+  static TPZStepSolver ConnectBlocks() {
+    TPZStepSolver s;
+    s.fSolver = EDirectBlocks;
+    return s;
   }
   void SetMixedFCG(int64_t numiterations, double tol, int precond = PZG_PRECOND_BLOCK_JACOBI) {
     fSolver = EMixedFCG;
@@ -206,7 +218,7 @@ class TPZStepSolver {
     if (fSolver == ECG) {  // mat->SolveCG(...), the matrix picks the algorithm
       int64_t its = fMaxIterations;
       double tol = fTol;
-      fMatrix->SolveCG(its, F, result, tol, fInfo);
+      fMatrix->SolveCG(its, F, result, tol, fInfo, fPcgPrecond);
       fNumIterations = its;
       fTol = tol;
       return;
@@ -237,6 +249,7 @@ class TPZStepSolver {
   int64_t fMaxIterations = 0, fNumIterations = 0;
   double fTol = 0.;
   int fPrecond = PZG_PRECOND_BLOCK_JACOBI;
+  int fPcgPrecond = PZG_PRECOND_JACOBI;
   TPZSYsmpMatrixHIP *fMatrix = nullptr;
   pzg_solve_info fInfo{};
 };
@@ -415,6 +428,21 @@ class TPZLinearAnalysis {
     fK.reset();
   }
   void SetSolver(const TPZStepSolver &s) { fSolver = s; }
+  // TPZAnalysis::BuildPreconditioner (Analysis/TPZAnalysis.cpp:995-1086):
+  // EBlockJacobi with overlap gives the block diagonal of the connects, the
+  // blocks of the assembled matrix solved directly (inverted on the device
+  // when CG starts); EJacobi has no preconditioner object there (an
+  // EJacobi step solver stands for it here); the other forms are not built.
+  enum EPrecond { EJacobi, EBlockJacobi, EElement, ENodeCentered };
+  TPZStepSolver BuildPreconditioner(EPrecond pre, bool overlap) const {
+    TPZStepSolver s;
+    if (pre == EJacobi) {
+      s.SetJacobi(1, 0., 0);
+      return s;
+    }
+    if (pre != EBlockJacobi || !overlap) DebugStop();
+    return TPZStepSolver::ConnectBlocks();
+  }
   // Analysis/TPZLinearAnalysis.cpp:42-94
   void Assemble() {
     if (!fK) fK = fStr->Create();

@@ -14,7 +14,7 @@
 // (int64/double arrays, each with its count), then iterations and the
 // achieved tolerance.
 //
-// usage: pzhip_run MESH OUT [cg|cg-mixed|cg-mixed-mg|mixed|mixed-mg] [tol] [maxit] [flags...]
+// usage: pzhip_run MESH OUT [cg|cg-bj|cg-mixed|cg-mixed-mg|mixed|mixed-mg] [tol] [maxit] [flags...]
 // cg-mixed / cg-mixed-mg: the same SetCG(maxit, jac, tol, 0) step solver, the
 // struct matrix set to the mixed-precision FCG (block Jacobi / multigrid)
 // before Assemble, so that TPZStepSolver::Solve -> SolveCG runs it.
@@ -98,7 +98,7 @@ void exact_solution(int dim, int nstate, const std::vector<double> &x, std::vect
 
 int main(int argc, char **argv) {
   if (argc < 3) {
-    std::fprintf(stderr, "usage: %s MESH OUT [cg|cg-mixed|cg-mixed-mg|mixed|mixed-mg] [tol] [maxit] "
+    std::fprintf(stderr, "usage: %s MESH OUT [cg|cg-bj|cg-mixed|cg-mixed-mg|mixed|mixed-mg] [tol] [maxit] "
                          "[sym|full] [sloan] [errors] [minmax:A:B]\n",
                  argv[0]);
     return 2;
@@ -225,6 +225,8 @@ int main(int argc, char **argv) {
     TPZStepSolver jac;
     jac.SetJacobi(1, 0., 0);
     step.SetCG(maxit, jac, tol, 0);
+  } else if (method == "cg-bj") {  // BuildPreconditioner(EBlockJacobi, true)
+    step.SetCG(maxit, an.BuildPreconditioner(TPZLinearAnalysis::EBlockJacobi, true), tol, 0);
   } else if (method == "mixed") {
     step.SetMixedFCG(maxit, tol);
   } else if (method == "mixed-mg") {  // multigrid-preconditioned FP32 iterations

@@ -89,6 +89,32 @@ def test_cpp_analysis_matches_oracle(tmp_path, name, pb):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name,pb", [
+    ("elast3d_p2_body", lambda: P.elasticity3d(n=4, p=2, seed=6, field_kind="grf", load="body")),
+    ("poisson2d_p2", lambda: P.poisson2d(n=10, p=2, seed=1)),
+])
+def test_cpp_block_jacobi_cg_matches_oracle(tmp_path, name, pb):
+    """SetCG with BuildPreconditioner(EBlockJacobi, true) in the C++ mirror:
+    within 1e-8 of the oracle's CG with the LU-solved connect blocks."""
+    pb = pb()
+    mfile, ofile = tmp_path / "mesh.bin", tmp_path / "out.bin"
+    M.save(M.build(pb), mfile)
+    print(f"[gpu] pzhip_run cg-bj {name}", flush=True)
+    r = subprocess.run([str(RUN), str(mfile), str(ofile), "cg-bj", "1e-13", "20000"],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    res = _read_result(ofile)
+    o = O.Oracle(pb)
+    ia, ja = o.pattern()
+    a, rhs = o.assemble()
+    ref = O.pcg_block(ia, ja, a, rhs, np.full(o.neq // pb.nstate, pb.nstate, np.int32), maxit=20000, tol=1e-13)
+    assert ref["converged"] and res["achieved"] <= 1e-13
+    assert abs(res["iterations"] - ref["iters"]) <= max(3, 0.05 * ref["iters"])
+    assert np.linalg.norm(res["u"] - ref["x"]) <= 1e-8 * np.linalg.norm(ref["x"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,pb", [
     ("elast3d_p2_body", lambda: P.elasticity3d(n=3, p=2, seed=6, field_kind="grf", load="body")),
     ("darcy_tet_hp5", lambda: P.darcy3d_tet(n=2, pmax=5, seed=5, hp=True)),
 ])
