@@ -356,10 +356,12 @@ int pzg_scaled_apply(pzg_ctx *ctx, int32_t op, const double *x, double *y);
  * cycle of PZG_PRECOND_LEARNED, and lmax[0] is then that of M_learned S), on host
  * vectors of length neq; degree and range as mg_degree and mg_range of
  * pzg_solve_params, 0 for the defaults.  *nlev receives the number of lattice levels (the first is the
- * trilinear subspace of the mesh's vertices, the last the dense one),
- * dims[3l .. 3l+2] their node counts per direction, lmax[0] the estimated
- * largest eigenvalue of the block-Jacobi scaled S and lmax[1+l] that of
- * lattice level l (arrays of 48 and 16 entries; any may be NULL).  The
+ * trilinear subspace of the mesh's vertices, the last the coarsest: solved
+ * densely when it has at most 1536 unknowns, else by a degree-8 Chebyshev
+ * polynomial), dims[3l .. 3l+2] their node counts per direction, lmax[0] the
+ * estimated largest eigenvalue of the block-Jacobi scaled S and lmax[1+l]
+ * that of lattice level l (0 for a densely solved coarsest level; arrays of
+ * 48 and 16 entries; any may be NULL).  The
  * levels are built on the first call after an assembly.  Exposes the mixed
  * solver's multigrid preconditioner, which has no reference counterpart. */
 #define PZG_MG_LEARNED 4

@@ -144,7 +144,9 @@ class Model:
 
     def apply(self, r, lmax, degree=2, rng=30.0, coarse_degree=1):
         co0 = self.cheb(lmax[0], rng, degree)
-        cos = [self.cheb(lmax[1 + k], rng, coarse_degree) for k in range(len(self.levels))]
+        # smoothing coefficients of the levels above the coarsest (the coarsest
+        # is solved directly or by its own polynomial below)
+        cos = [self.cheb(lmax[1 + k], rng, coarse_degree) for k in range(len(self.levels) - 1)]
         x0, r0 = self.pre(self.S, self.M0, r, co0)
         b = [None] * len(self.levels)
         x = [None] * len(self.levels)
