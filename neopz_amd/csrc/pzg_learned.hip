@@ -552,13 +552,38 @@ __global__ __launch_bounds__(256) void k_les_tile(LesTileArgs a, const float *__
   const int le = wv * 16 + col;  // this lane's element of the tile
   const int cx = le % LT_TC, cy = (le / LT_TC) % LT_TC, cz = le / (LT_TC * LT_TC);
   const int cb = (P * cx + BN * (P * cy + BN * P * cz)) * NS;
-  // this thread's box nodes, their cells and whether they are inner
+  // this thread's box nodes: whether they are inner, and the sY offsets of
+  // the (element, shape) outputs summed into each (cells in z, y, x order;
+  // the same for every tile), packed two per register
+  __syncthreads();  // sInv staged
   bool inner[NQ];
+  int ncnt[NQ];
+  uint32_t noff[NQ][4];
 #pragma unroll
   for (int j = 0; j < NQ; j++) {
     const int q = tid + 256 * j;
-    const int bx = q % BN, by = (q / BN) % BN, bz = q / (BN * BN);
-    inner[j] = q < NB && bx > 0 && bx < W && by > 0 && by < W && bz > 0 && bz < W;
+    const int b[3] = {q % BN, (q / BN) % BN, q / (BN * BN)};
+    inner[j] = q < NB && b[0] > 0 && b[0] < W && b[1] > 0 && b[1] < W && b[2] > 0 && b[2] < W;
+    int c0[3], c1[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      c1[d] = min(b[d] / P, LT_TC - 1);
+      c0[d] = (b[d] % P == 0 && b[d] > 0) ? b[d] / P - 1 : c1[d];
+    }
+    int cnt = 0;
+#pragma unroll
+    for (int h = 0; h < 4; h++) noff[j][h] = 0u;
+    if (q < NB)
+      for (int z = c0[2]; z <= c1[2]; z++)
+        for (int y = c0[1]; y <= c1[1]; y++)
+          for (int x = c0[0]; x <= c1[0]; x++) {
+            const int lx = b[0] - P * x, ly = b[1] - P * y, lz = b[2] - P * z;
+            const int sh = sInv[lx + (P + 1) * (ly + (P + 1) * lz)];
+            const uint32_t o = (uint32_t)((x + LT_TC * (y + LT_TC * z)) * NE + sh * NS);
+            noff[j][cnt >> 1] |= o << (16 * (cnt & 1));
+            cnt++;
+          }
+    ncnt[j] = cnt;
   }
   const int64_t ntile = (int64_t)a.nt[0] * a.nt[1] * a.nt[2];
   int64_t lo, hi;
@@ -641,25 +666,16 @@ __global__ __launch_bounds__(256) void k_les_tile(LesTileArgs a, const float *__
     for (int j = 0; j < NQ; j++) {
       const int q = tid + 256 * j;
       if (q >= NB) continue;
-      const int b[3] = {q % BN, (q / BN) % BN, q / (BN * BN)};
-      int c0[3], c1[3];
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        c1[d] = min(b[d] / P, LT_TC - 1);
-        c0[d] = (b[d] % P == 0 && b[d] > 0) ? b[d] / P - 1 : c1[d];
-      }
       float acc[NS];
 #pragma unroll
       for (int k = 0; k < NS; k++) acc[k] = 0.f;
-      for (int z = c0[2]; z <= c1[2]; z++)
-        for (int y = c0[1]; y <= c1[1]; y++)
-          for (int x = c0[0]; x <= c1[0]; x++) {
-            const int lx = b[0] - P * x, ly = b[1] - P * y, lz = b[2] - P * z;
-            const int sh = sInv[lx + (P + 1) * (ly + (P + 1) * lz)];
-            const float *yy = sY + (x + LT_TC * (y + LT_TC * z)) * NE + sh * NS;
 #pragma unroll
-            for (int k = 0; k < NS; k++) acc[k] += yy[k];
-          }
+      for (int h = 0; h < 8; h++) {
+        if (h >= ncnt[j]) break;
+        const float *yy = sY + ((noff[j][h >> 1] >> (16 * (h & 1))) & 0xFFFFu);
+#pragma unroll
+        for (int k = 0; k < NS; k++) acc[k] += yy[k];
+      }
       if (inner[j]) {
         if (m[j] >= 0) {
           const int64_t B = m[j] & 0x0FFFFFFF;
