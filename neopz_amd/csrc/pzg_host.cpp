@@ -65,7 +65,7 @@ static void free_all(Ctx &c) {
                     &c.d_zs, &c.d_ps, &c.d_ps2, &c.d_qs, &c.d_bs, &c.d_scal, &c.d_partials,
                     &c.d_counter, &c.d_res2, &c.d_les_th, &c.d_les_m, &c.d_les_v, &c.d_les_g,
                     &c.d_les_cia, &c.d_les_cja, &c.d_les_pen, &c.d_les_X, &c.d_les_z, &c.d_les_s,
-                    &c.d_les_ye, &c.d_les_blkm, &c.d_les_eoff, &c.d_bjinv64, &c.d_les_cmap, &c.d_les_tmapb, &c.d_les_tmap, &c.d_les_part, &c.d_les_hmapb, &c.d_les_hmap, &c.d_les_hptr, &c.d_les_hsrc, &c.d_les_Se, &c.d_les_M, &c.d_les_R, &c.d_les_E, &c.d_les_D,
+                    &c.d_les_ye, &c.d_les_blkm, &c.d_les_eoff, &c.d_bjinv64, &c.d_les_cmap, &c.d_les_tmapb, &c.d_les_tmap, &c.d_les_part, &c.d_les_hmapb, &c.d_les_hmap, &c.d_les_hptr, &c.d_les_hsrc, &c.d_les_tscale, &c.d_les_Se, &c.d_les_M, &c.d_les_R, &c.d_les_E, &c.d_les_D,
                     &c.d_les_le, &c.d_les_ds, &c.d_les_loss, &c.d_les_fail, &c.d_nn_trH, &c.d_nn_trM,
                     &c.d_nn_trP, &c.d_nn_dH, &c.d_nn_G, &c.d_nn_tmp, &c.d_nn_dZ, &c.d_nn_part,
                     &c.d_filt_dest, &c.d_post_tab, &c.d_post_list, &c.d_post_slot,
@@ -699,7 +699,7 @@ static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
   c.has_mesh = true;
   c.has_struct = false;
   c.mf_ready = c.mf_vals_valid = false;
-  c.les_tile_ready = false;
+  c.les_tile_ready = c.les_tile_ok = false;
   mg_free(c);  // the multigrid lattice belongs to the mesh
   c.assembled = false;
   c.external_csr = false;
@@ -987,7 +987,7 @@ static int struct_create(Ctx &c) {
   c.assembled = false;
   c.kel_added = false;
   c.mf_ready = c.mf_vals_valid = false;
-  c.les_tile_ready = false;
+  c.les_tile_ready = c.les_tile_ok = false;
   c.full_valid = c.bsr_valid = c.scal_valid = false;
   c.has_solution = c.f_valid = false;
   return PZG_OK;
@@ -1101,7 +1101,7 @@ int pzg_set_equation_filter(pzg_ctx *ctx, int64_t nactive, const int64_t *active
   c.assembled = false;
   c.has_solution = c.f_valid = false;
   c.mf_ready = c.mf_vals_valid = false;
-  c.les_tile_ready = false;
+  c.les_tile_ready = c.les_tile_ok = false;
   c.full_valid = c.bsr_valid = c.scal_valid = false;
   mg_free(c);
   return PZG_OK;
@@ -1324,7 +1324,7 @@ int pzg_csr_set(pzg_ctx *ctx, int64_t neq, const int64_t *ia, const int64_t *ja,
   c.has_struct = true;
   c.external_csr = true;
   c.mf_ready = c.mf_vals_valid = false;
-  c.les_tile_ready = false;
+  c.les_tile_ready = c.les_tile_ok = false;
   c.assembled = true;
   c.full_valid = c.bsr_valid = c.scal_valid = false;
   c.has_solution = c.f_valid = false;

@@ -336,7 +336,8 @@ struct Ctx {
   int les_tP = 0, les_tn[3] = {0, 0, 0};
   int16_t les_toff[27] = {}, les_tinv[27] = {};
   int64_t les_nhalo = 0;
-  DevBuf d_les_cmap, d_les_tmapb, d_les_tmap, d_les_part, d_les_hmapb, d_les_hmap, d_les_hptr, d_les_hsrc;
+  DevBuf d_les_cmap, d_les_tmapb, d_les_tmap, d_les_part, d_les_hmapb, d_les_hmap, d_les_hptr, d_les_hsrc,
+      d_les_tscale;
   DevBuf d_les_blkm;       // element blocks with penalized-component bits (learned apply)
   DevBuf d_les_eoff;       // row-ordered slot -> element-major output offset (learned apply)
   // penalty rows of the learned path, solved in FP64 by the outer refinement

@@ -286,6 +286,36 @@ __global__ void k_les_exp(int64_t n, const float *__restrict__ z, float *__restr
 // next group's batches are in flight while the current group multiplies.
 // The outputs are stored element by element (NE floats each); the gather
 // below sums them per block row.
+// G (row-major, ne x ne) into LDS in A-operand order of the 16 x 16 x 4
+// tiles: entry (row, k) at ((row/16) NK + k/4) 64 + row%16 + 16 (k%4), the
+// padding zero.  G is read in order (coalesced); the caller's next barrier
+// publishes sA.
+template <int NE, int NT, int NK>
+__device__ inline void les_stage_G(const float *__restrict__ G, float *sA, int tid) {
+  for (int t = tid; t < NT * NK * 64; t += 256) {
+    const int ll = t & 63, kk = (t >> 6) % NK, ti = (t >> 6) / NK;
+    const int row = 16 * ti + (ll & 15), k = 4 * kk + (ll >> 4);
+    if (row >= NE || k >= NE) sA[t] = 0.f;
+  }
+  // all of a thread's loads issued before its first store (a load-store
+  // pair per iteration would wait out the memory latency every time)
+  constexpr int NPT = (NE * NE + 255) / 256;
+  float v[NPT];
+#pragma unroll
+  for (int u = 0; u < NPT; u++) {
+    const int t = tid + 256 * u;
+    v[u] = G[t < NE * NE ? t : 0];
+  }
+#pragma unroll
+  for (int u = 0; u < NPT; u++) {
+    const int t = tid + 256 * u;
+    if (t < NE * NE) {
+      const int row = t / NE, k = t % NE;
+      sA[((row >> 4) * NK + (k >> 2)) * 64 + (row & 15) + 16 * (k & 3)] = v[u];
+    }
+  }
+}
+
 template <int NSH, int NS>
 __global__ __launch_bounds__(256) void k_les_mfma(int64_t nvol, const float *__restrict__ G,
                                                   const float *__restrict__ scale,
@@ -297,11 +327,7 @@ __global__ __launch_bounds__(256) void k_les_mfma(int64_t nvol, const float *__r
   static_assert(NT * 16 <= 96, "element too large for the tile set");
   __shared__ float sA[NT * NK * 64];
   const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, col = l & 15, kq = l >> 4;
-  for (int t = tid; t < NT * NK * 64; t += 256) {
-    const int ll = t & 63, kk = (t >> 6) % NK, ti = (t >> 6) / NK;
-    const int row = 16 * ti + (ll & 15), k = 4 * kk + (ll >> 4);
-    sA[t] = (row < NE && k < NE) ? G[row * NE + k] : 0.f;
-  }
+  les_stage_G<NE, NT, NK>(G, sA, tid);
   __syncthreads();
   int64_t lo, hi;
   int xi, xg;
@@ -490,6 +516,7 @@ constexpr int LT_TC = 4;
 struct LesTileArgs {
   int n[3], nt[3];          // cells, tiles per direction
   const int32_t *cmap;      // cell -> element (-1: none)
+  const float *tscale;      // per tile and element slot: the element's scale (0: no element)
   const int32_t *tmap;      // per tile and box node: block | penalized components as bits 28 + k (-1: none)
   float *part;              // per tile and box node: partial sums of face nodes
   int64_t nhalo;            // face nodes of the lattice (block order)
@@ -518,7 +545,10 @@ __device__ inline void les_emit(const LesOutArgs &o, int64_t i, float y) {
   if (o.pt) o.pt[i] = dv * o.dis[i];
 }
 
-template <int P, int NS>
+// MODE: out.mode (-1: z = y; 0, 1, 2: the Chebyshev steps), fixed per launch.
+// Loads are branch-free (clamped addresses, results selected afterwards), so
+// each batch of them is one round trip to memory.
+template <int P, int NS, int MODE>
 __global__ __launch_bounds__(256) void k_les_tile(LesTileArgs a, const float *__restrict__ G,
                                                   const float *__restrict__ scale,
                                                   const float *__restrict__ r, LesOutArgs out,
@@ -536,11 +566,7 @@ __global__ __launch_bounds__(256) void k_les_tile(LesTileArgs a, const float *__
   __shared__ int16_t sInv[NSH];
   float *sR = sRY, *sY = sRY;
   const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, col = l & 15, kq = l >> 4;
-  for (int t = tid; t < NT * NK * 64; t += 256) {
-    const int ll = t & 63, kk = (t >> 6) % NK, ti = (t >> 6) / NK;
-    const int row = 16 * ti + (ll & 15), k = 4 * kk + (ll >> 4);
-    sA[t] = (row < NE && k < NE) ? G[row * NE + k] : 0.f;
-  }
+  les_stage_G<NE, NT, NK>(G, sA, tid);
   if (tid < NSH) sInv[tid] = a.inv[tid];
   // this lane's B-operand offsets into sR relative to its element's cell
   int uo[NK];
@@ -589,58 +615,72 @@ __global__ __launch_bounds__(256) void k_les_tile(LesTileArgs a, const float *__
   int64_t lo, hi;
   int xi, xg;
   xcd_slab(ntile, lo, hi, xi, xg);
-  int32_t mq[NQ];
-  float rq[NQ][NS];
-  auto load_map = [&](int64_t tile) {
+  // loads run one tile ahead of their use: at the top of a tile, r of the
+  // next tile (its node map came one tile earlier), the node map of the tile
+  // after that and the next tile's element scales are issued.  Loaded values
+  // are selected only where they are used, so no load is waited for early.
+  auto load_map = [&](int64_t tile, int32_t (&mm)[NQ], bool (&vv)[NQ]) {
 #pragma unroll
     for (int j = 0; j < NQ; j++) {
       const int q = tid + 256 * j;
-      mq[j] = (tile < hi && q < NB) ? a.tmap[tile * NB + q] : -1;
+      vv[j] = tile < hi && q < NB;
+      mm[j] = a.tmap[vv[j] ? tile * NB + q : 0];
     }
   };
-  auto load_r = [&]() {
+  auto load_r = [&](const int32_t (&mm)[NQ], const bool (&vv)[NQ], float (&rr)[NQ][NS]) {
 #pragma unroll
     for (int j = 0; j < NQ; j++)
 #pragma unroll
-      for (int k = 0; k < NS; k++) rq[j][k] = mq[j] >= 0 ? r[(int64_t)(mq[j] & 0x0FFFFFFF) * NS + k] : 0.f;
+      for (int k = 0; k < NS; k++)
+        rr[j][k] = r[(int64_t)(vv[j] && mm[j] >= 0 ? mm[j] & 0x0FFFFFFF : 0) * NS + k];
   };
-  load_map(lo + xi);
-  load_r();
+  auto scale_of = [&](int64_t tile) {  // this lane's element's scale (0: no element)
+    return a.tscale[(tile < hi ? tile : 0) * NEL + le];
+  };
+  int32_t mq[NQ], mn[NQ];
+  bool vq[NQ], vn[NQ];
+  float rq[NQ][NS];
+  load_map(lo + xi, mq, vq);
+  load_r(mq, vq, rq);
+  load_map(lo + xi + xg, mn, vn);
+  float s_cur = scale_of(lo + xi);
   for (int64_t tile = lo + xi; tile < hi; tile += xg) {
-    const int tx = (int)(tile % a.nt[0]), ty = (int)((tile / a.nt[0]) % a.nt[1]),
-              tz = (int)(tile / ((int64_t)a.nt[0] * a.nt[1]));
     __syncthreads();  // the previous tile's LDS reads are done (and sA is staged)
     int32_t m[NQ];
     float rv[NQ][NS];
 #pragma unroll
     for (int j = 0; j < NQ; j++) {
       const int q = tid + 256 * j;
-      m[j] = mq[j];
+      m[j] = vq[j] ? mq[j] : -1;
 #pragma unroll
       for (int k = 0; k < NS; k++) {
-        rv[j][k] = rq[j][k];
-        if (q < NB) sR[q * NS + k] = ((m[j] >> (28 + k)) & 1) ? 0.f : rq[j][k];
+        rv[j][k] = m[j] >= 0 ? rq[j][k] : 0.f;
+        if (q < NB) sR[q * NS + k] = ((m[j] >> (28 + k)) & 1) ? 0.f : rv[j][k];
       }
     }
+    const float s_el = s_cur;
+    load_r(mn, vn, rq);  // next tile
+#pragma unroll
+    for (int j = 0; j < NQ; j++) {
+      mq[j] = mn[j];
+      vq[j] = vn[j];
+    }
+    load_map(tile + 2 * xg, mn, vn);
+    s_cur = scale_of(tile + xg);
     // output operands of the inner nodes
     float od[NQ][NS], ox[NQ][NS], os[NQ][NS];
 #pragma unroll
     for (int j = 0; j < NQ; j++)
 #pragma unroll
       for (int k = 0; k < NS; k++) {
-        const bool ok = inner[j] && m[j] >= 0 && out.mode >= 0;
-        const int64_t i = (int64_t)(m[j] & 0x0FFFFFFF) * NS + k;
-        od[j][k] = ok && out.mode == 2 ? out.d[i] : 0.f;
-        ox[j][k] = ok && out.mode >= 1 ? out.x[i] : 0.f;
-        os[j][k] = ok && out.pt ? out.dis[i] : 0.f;
+        const int64_t i = (int64_t)(inner[j] && m[j] >= 0 ? m[j] & 0x0FFFFFFF : 0) * NS + k;
+        od[j][k] = MODE == 2 ? out.d[i] : 0.f;
+        ox[j][k] = MODE >= 1 ? out.x[i] : 0.f;
+        os[j][k] = MODE >= 0 && out.pt ? out.dis[i] : 0.f;
       }
-    load_map(tile + xg);  // the next tile's node map, in flight during the products
     __syncthreads();
     {
-      const int ex = LT_TC * tx + cx, ey = LT_TC * ty + cy, ez = LT_TC * tz + cz;
-      int32_t e = -1;
-      if (ex < a.n[0] && ey < a.n[1] && ez < a.n[2]) e = a.cmap[ex + (int64_t)a.n[0] * (ey + (int64_t)a.n[1] * ez)];
-      const float s = e >= 0 ? scale[e] : 0.f;
+      const float s = s_el;
       float u[NK];
 #pragma unroll
       for (int kk = 0; kk < NK; kk++) u[kk] = uo[kk] >= 0 ? sR[cb + uo[kk]] : 0.f;
@@ -658,7 +698,6 @@ __global__ __launch_bounds__(256) void k_les_tile(LesTileArgs a, const float *__
         }
       }
     }
-    load_r();  // the next tile's r, in flight during the sums below
     __syncthreads();
     // per box node: the sum over the tile's elements holding it (cells in
     // z, y, x order)
@@ -683,12 +722,12 @@ __global__ __launch_bounds__(256) void k_les_tile(LesTileArgs a, const float *__
           for (int k = 0; k < NS; k++) {
             const int64_t i = B * NS + k;
             const float y = ((m[j] >> (28 + k)) & 1) ? rv[j][k] : acc[k];
-            if (out.mode < 0) {
+            if (MODE < 0) {
               out.z[i] = y;
             } else {
-              const float dv = out.mode == 2 ? out.c1 * od[j][k] + out.c2 * y : out.c2 * y;
+              const float dv = MODE == 2 ? out.c1 * od[j][k] + out.c2 * y : out.c2 * y;
               out.d[i] = dv;
-              out.x[i] = out.mode == 0 ? dv : ox[j][k] + dv;
+              out.x[i] = MODE == 0 ? dv : ox[j][k] + dv;
               if (out.pt) out.pt[i] = dv * os[j][k];
             }
           }
@@ -727,6 +766,21 @@ __global__ __launch_bounds__(256) void k_les_halo(LesTileArgs a, const float *__
       les_emit(out, i, ((m >> (28 + k)) & 1) ? r[i] : acc[k]);
     }
   }
+}
+
+// tscale[tile NEL + slot] = the scale of the element in that cell of the tile
+__global__ void k_les_tscale(int64_t ntile, int nt0, int nt1, int n0, int n1, int n2,
+                             const int32_t *__restrict__ cmap, const float *__restrict__ s,
+                             float *__restrict__ ts) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntile * 64) return;
+  const int64_t tile = t / 64;
+  const int le = (int)(t % 64);
+  const int tx = (int)(tile % nt0), ty = (int)((tile / nt0) % nt1), tz = (int)(tile / ((int64_t)nt0 * nt1));
+  const int ex = LT_TC * tx + le % LT_TC, ey = LT_TC * ty + (le / LT_TC) % LT_TC, ez = LT_TC * tz + le / (LT_TC * LT_TC);
+  int32_t e = -1;
+  if (ex < n0 && ey < n1 && ez < n2) e = cmap[ex + (int64_t)n0 * (ey + (int64_t)n1 * ez)];
+  ts[t] = e >= 0 ? s[e] : 0.f;
 }
 
 // tmap / hmap = the node maps with the penalized components of each block
@@ -1050,6 +1104,15 @@ static int les_forward(Ctx &c, bool keep) {
                      th + o.wout, c.d_les_z.as<float>());
   hipLaunchKernelGGL(k_les_exp, dim3(grid1(rows)), dim3(TPB), 0, c.stream, rows, c.d_les_z.as<float>(),
                      c.d_les_s.as<float>());
+  if (c.les_tile_ok) {  // the scales in tile order for k_les_tile
+    int nt[3];
+    for (int d = 0; d < 3; d++) nt[d] = (c.les_tn[d] + LT_TC - 1) / LT_TC;
+    const int64_t ntile = (int64_t)nt[0] * nt[1] * nt[2];
+    PZG_TRY(dev_alloc(c.d_les_tscale, sizeof(float) * ntile * 64));
+    hipLaunchKernelGGL(k_les_tscale, dim3(grid1(ntile * 64)), dim3(TPB), 0, c.stream, ntile, nt[0], nt[1],
+                       c.les_tn[0], c.les_tn[1], c.les_tn[2], c.d_les_cmap.as<int32_t>(), c.d_les_s.as<float>(),
+                       c.d_les_tscale.as<float>());
+  }
   PZG_HIP(hipGetLastError());
   return PZG_OK;
 }
@@ -1342,6 +1405,7 @@ int les_apply(Ctx &c, const float *r, float *z, const SolveScalars *sc, const Le
       a.nt[d] = (c.les_tn[d] + LT_TC - 1) / LT_TC;
     }
     a.cmap = c.d_les_cmap.as<int32_t>();
+    a.tscale = c.d_les_tscale.as<float>();
     a.tmap = c.d_les_tmap.as<int32_t>();
     a.part = c.d_les_part.as<float>();
     a.nhalo = c.les_nhalo;
@@ -1368,7 +1432,14 @@ int les_apply(Ctx &c, const float *r, float *z, const SolveScalars *sc, const Le
     const int gh = grid1(std::max<int64_t>(a.nhalo, 1));
 #define LES_TILE(PP, NSV)                                                                                   \
   do {                                                                                                    \
-    hipLaunchKernelGGL((k_les_tile<PP, NSV>), dim3(gt), dim3(256), 0, c.stream, a, G, s, r, o, sc);         \
+    if (o.mode < 0)                                                                                       \
+      hipLaunchKernelGGL((k_les_tile<PP, NSV, -1>), dim3(gt), dim3(256), 0, c.stream, a, G, s, r, o, sc);   \
+    else if (o.mode == 0)                                                                                 \
+      hipLaunchKernelGGL((k_les_tile<PP, NSV, 0>), dim3(gt), dim3(256), 0, c.stream, a, G, s, r, o, sc);    \
+    else if (o.mode == 1)                                                                                 \
+      hipLaunchKernelGGL((k_les_tile<PP, NSV, 1>), dim3(gt), dim3(256), 0, c.stream, a, G, s, r, o, sc);    \
+    else                                                                                                  \
+      hipLaunchKernelGGL((k_les_tile<PP, NSV, 2>), dim3(gt), dim3(256), 0, c.stream, a, G, s, r, o, sc);    \
     hipLaunchKernelGGL((k_les_halo<NSV>), dim3(gh), dim3(256), 0, c.stream, a, r, o, sc);                   \
   } while (0)
     if (c.les_tP == 2 && c.bs == 3) LES_TILE(2, 3);

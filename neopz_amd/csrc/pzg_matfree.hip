@@ -265,9 +265,21 @@ __global__ __launch_bounds__(MF_TPB) void k_mf_hex27_mfma(MfArgs a, const MfMfma
   for (int t = tid; t < 27 * 8; t += MF_TPB) s_cd[t / 8][t % 8] = a.tab->cd[t / 8][t % 8];
   if (tid < 27) s_w[tid] = a.tab->w[tid];
   {
+    // every load of the thread issued before its first store
     const float4 *src = reinterpret_cast<const float4 *>(mt);
     float4 *dst = reinterpret_cast<float4 *>(&s_mt);
-    for (int t = tid; t < (int)(sizeof(MfMfmaTab) / 16); t += MF_TPB) dst[t] = src[t];
+    constexpr int N4 = (int)(sizeof(MfMfmaTab) / 16), NPT = (N4 + MF_TPB - 1) / MF_TPB;
+    float4 v[NPT];
+#pragma unroll
+    for (int u = 0; u < NPT; u++) {
+      const int t = tid + MF_TPB * u;
+      v[u] = src[t < N4 ? t : 0];
+    }
+#pragma unroll
+    for (int u = 0; u < NPT; u++) {
+      const int t = tid + MF_TPB * u;
+      if (t < N4) dst[t] = v[u];
+    }
   }
   __syncthreads();
   const float(&A1)[3][2][7][64] = s_mt.a1;
