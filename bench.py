@@ -45,7 +45,9 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-baseline", action="store_true", help="skip the FP64 Jacobi-PCG run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle sample")
-    ap.add_argument("--precond", default="learned", choices=["multigrid", "block_jacobi", "learned"])
+    ap.add_argument("--precond", default="", choices=["", "multigrid", "block_jacobi", "learned"],
+                    help="preconditioner of the FP32 iterations (default: learned; block_jacobi on the "
+                         "tetrahedral C3, where neither the learned smoother nor the lattice cycle applies)")
     ap.add_argument("--train-n", type=int, default=12, help="elements per direction of the training systems")
     ap.add_argument("--train-seeds", type=int, default=4)
     ap.add_argument("--train-steps", type=int, default=150)
@@ -66,7 +68,10 @@ def parse():
     ap.add_argument("--kernels-only", action="store_true",
                     help="assemble once, then time each operator kernel (5 launches) and a 1 GiB "
                          "device copy (HBM counter calibration); no solve")
-    return ap.parse_args()
+    args = ap.parse_args()
+    if not args.precond:
+        args.precond = "block_jacobi" if args.config == "C3" else "learned"
+    return args
 
 
 WORKLOADS = {
