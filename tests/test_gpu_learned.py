@@ -124,6 +124,48 @@ def test_smoother_apply_matches_numpy(name):
     assert np.linalg.norm(z - zr) <= 1e-5 * np.linalg.norm(zr)
 
 
+def test_tiled_and_element_major_application_agree():
+    """On hexahedral lattice meshes the smoother runs in lattice tiles; the
+    element-major kernels (PZG_LES_TILES=0, a separate process) give the same
+    z within FP32 summation-order differences."""
+    import os
+    import subprocess
+    import sys
+    import tempfile
+    pb = CASES["elast3d_p2"]()
+    sm, K = _system(pb)
+    ls = LearnedSmoother(ctx())
+    ls.setup(seed=3)
+    w = ls.get_weights()
+    rng = np.random.default_rng(9)
+    ne = [k for k in range(1, 200) if R.offsets(k)["n"] == len(w)][0]
+    G = rng.standard_normal((ne, ne))
+    w[: ne * ne] = (0.5 * (G + G.T)).ravel()
+    ls.set_weights(w)
+    r = rng.standard_normal(K.neq)
+    z_tile = ls.apply(r)
+    with tempfile.TemporaryDirectory() as d:
+        np.save(os.path.join(d, "w.npy"), w)
+        np.save(os.path.join(d, "r.npy"), r)
+        code = (
+            "import sys, numpy as np\n"
+            f"sys.path.insert(0, {str(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))!r})\n"
+            "from neopz_amd import mesh as M, problems as P\n"
+            "from neopz_amd.analysis import Context, TPZSSpStructMatrixHIP\n"
+            "from neopz_amd.learned import LearnedSmoother\n"
+            "pb = P.elasticity3d(n=5, p=2, seed=2, field_kind='grf', corr_len=6.4 / 5, load='body')\n"
+            "c = Context(0)\n"
+            "sm = TPZSSpStructMatrixHIP(c, M.build(pb)); K = sm.Create(); sm.Assemble(K)\n"
+            "ls = LearnedSmoother(c); ls.setup(seed=3)\n"
+            f"ls.set_weights(np.load({os.path.join(d, 'w.npy')!r}))\n"
+            f"np.save({os.path.join(d, 'z.npy')!r}, ls.apply(np.load({os.path.join(d, 'r.npy')!r})))\n")
+        env = dict(os.environ, PZG_LES_TILES="0")
+        p = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=240)
+        assert p.returncode == 0, p.stderr[-2000:]
+        z_elem = np.load(os.path.join(d, "z.npy"))
+    assert np.linalg.norm(z_tile - z_elem) <= 1e-5 * np.linalg.norm(z_elem)
+
+
 @pytest.mark.parametrize("op", [L.INNER_BSR, L.INNER_MATFREE])
 def test_learned_vcycle_matches_model(op):
     """The cycle of PZG_PRECOND_LEARNED (pzg_mg_apply with PZG_MG_LEARNED)
