@@ -36,6 +36,9 @@ namespace pzg {
 namespace {
 
 constexpr int MF_TPB = 256;
+// k_mf_hex27_mfma: sixteen wavefronts per workgroup share one copy of the
+// tables in LDS (4 waves/SIMD; fewer copies staged per launch)
+constexpr int MFK_TPB = 1024;
 
 struct MfTab {
   float w[MAXQ];
@@ -248,12 +251,12 @@ struct MfMfmaTab {
 };
 
 template <int NS>
-__global__ __launch_bounds__(MF_TPB) void k_mf_hex27_mfma(MfArgs a, const MfMfmaTab *__restrict__ mt,
+__global__ __launch_bounds__(MFK_TPB) void k_mf_hex27_mfma(MfArgs a, const MfMfmaTab *__restrict__ mt,
                                                          const SolveScalars *sc) {
   if (sc && sc->done) return;
   constexpr int EPT = 16 / NS;  // elements per wavefront tile
   constexpr int NCOL = EPT * NS;
-  constexpr int NW = MF_TPB / 64;
+  constexpr int NW = MFK_TPB / 64;
   constexpr int NG = NS * 3;
   __shared__ float4 s_cd[27][8];
   __shared__ float s_w[27];
@@ -262,22 +265,22 @@ __global__ __launch_bounds__(MF_TPB) void k_mf_hex27_mfma(MfArgs a, const MfMfma
   __shared__ float s_x[NW][EPT][11];
   __shared__ __align__(16) MfMfmaTab s_mt;
   const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, lr = l >> 4;
-  for (int t = tid; t < 27 * 8; t += MF_TPB) s_cd[t / 8][t % 8] = a.tab->cd[t / 8][t % 8];
+  for (int t = tid; t < 27 * 8; t += MFK_TPB) s_cd[t / 8][t % 8] = a.tab->cd[t / 8][t % 8];
   if (tid < 27) s_w[tid] = a.tab->w[tid];
   {
     // every load of the thread issued before its first store
     const float4 *src = reinterpret_cast<const float4 *>(mt);
     float4 *dst = reinterpret_cast<float4 *>(&s_mt);
-    constexpr int N4 = (int)(sizeof(MfMfmaTab) / 16), NPT = (N4 + MF_TPB - 1) / MF_TPB;
+    constexpr int N4 = (int)(sizeof(MfMfmaTab) / 16), NPT = (N4 + MFK_TPB - 1) / MFK_TPB;
     float4 v[NPT];
 #pragma unroll
     for (int u = 0; u < NPT; u++) {
-      const int t = tid + MF_TPB * u;
+      const int t = tid + MFK_TPB * u;
       v[u] = src[t < N4 ? t : 0];
     }
 #pragma unroll
     for (int u = 0; u < NPT; u++) {
-      const int t = tid + MF_TPB * u;
+      const int t = tid + MFK_TPB * u;
       if (t < N4) dst[t] = v[u];
     }
   }
@@ -858,20 +861,21 @@ int mf_launch_vol(Ctx &c, const SolveScalars *sc) {
     }
     if (mfma) {
       // persistent grid: as many workgroups as fit on the device at once
-      int64_t groups = (c.nvol + 4 * (16 / c.nstate) - 1) / (4 * (16 / c.nstate));
+      const int64_t per_wg = (MFK_TPB / 64) * (16 / c.nstate);  // elements per workgroup pass
+      int64_t groups = (c.nvol + per_wg - 1) / per_wg;
       if (c.mf_resident <= 0) {
         int per_cu = 0, ncu = 0;
         PZG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device));
         if (c.nstate == 3)
-          PZG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mf_hex27_mfma<3>, MF_TPB, 0));
+          PZG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mf_hex27_mfma<3>, MFK_TPB, 0));
         else
-          PZG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mf_hex27_mfma<1>, MF_TPB, 0));
+          PZG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mf_hex27_mfma<1>, MFK_TPB, 0));
         c.mf_resident = std::max(1, per_cu) * std::max(1, ncu);
       }
       dim3 grid((unsigned)std::min<int64_t>(std::max<int64_t>(groups, 1), c.mf_resident));
       const MfMfmaTab *mt = c.d_mf_mfma.as<MfMfmaTab>();
-      if (c.nstate == 3) hipLaunchKernelGGL(k_mf_hex27_mfma<3>, grid, dim3(MF_TPB), 0, c.stream, a, mt, sc);
-      else hipLaunchKernelGGL(k_mf_hex27_mfma<1>, grid, dim3(MF_TPB), 0, c.stream, a, mt, sc);
+      if (c.nstate == 3) hipLaunchKernelGGL(k_mf_hex27_mfma<3>, grid, dim3(MFK_TPB), 0, c.stream, a, mt, sc);
+      else hipLaunchKernelGGL(k_mf_hex27_mfma<1>, grid, dim3(MFK_TPB), 0, c.stream, a, mt, sc);
       PZG_HIP(hipGetLastError());
     } else if (c.dim == 3 && c.nstate == 3) {
       if (nsh == 27) PZG_TRY((launch_vol_lpe<3, 8, 27, 3>(c, a, sc)));
