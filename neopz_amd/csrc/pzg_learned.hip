@@ -290,25 +290,25 @@ __global__ void k_les_exp(int64_t n, const float *__restrict__ z, float *__restr
 // tiles: entry (row, k) at ((row/16) NK + k/4) 64 + row%16 + 16 (k%4), the
 // padding zero.  G is read in order (coalesced); the caller's next barrier
 // publishes sA.
-template <int NE, int NT, int NK>
+template <int NE, int NT, int NK, int NTH = 256>
 __device__ inline void les_stage_G(const float *__restrict__ G, float *sA, int tid) {
-  for (int t = tid; t < NT * NK * 64; t += 256) {
+  for (int t = tid; t < NT * NK * 64; t += NTH) {
     const int ll = t & 63, kk = (t >> 6) % NK, ti = (t >> 6) / NK;
     const int row = 16 * ti + (ll & 15), k = 4 * kk + (ll >> 4);
     if (row >= NE || k >= NE) sA[t] = 0.f;
   }
   // all of a thread's loads issued before its first store (a load-store
   // pair per iteration would wait out the memory latency every time)
-  constexpr int NPT = (NE * NE + 255) / 256;
+  constexpr int NPT = (NE * NE + NTH - 1) / NTH;
   float v[NPT];
 #pragma unroll
   for (int u = 0; u < NPT; u++) {
-    const int t = tid + 256 * u;
+    const int t = tid + NTH * u;
     v[u] = G[t < NE * NE ? t : 0];
   }
 #pragma unroll
   for (int u = 0; u < NPT; u++) {
-    const int t = tid + 256 * u;
+    const int t = tid + NTH * u;
     if (t < NE * NE) {
       const int row = t / NE, k = t % NE;
       sA[((row >> 4) * NK + (k >> 2)) * 64 + (row & 15) + 16 * (k & 3)] = v[u];
@@ -512,6 +512,10 @@ __global__ __launch_bounds__(256) void k_les_gather_cheb(int64_t nb, const int64
 // loads are issued a phase ahead: the next tile's node map and r during this
 // tile's products, the output operands of this tile's inner nodes before them.
 constexpr int LT_TC = 4;
+// tiles in flight per workgroup (k_les_tile): three workgroups of one tile
+// per CU ran the 64^3 application in 72-83 us, one of three tiles (one G
+// copy per CU, barriers over twelve waves) in 100-121 us
+constexpr int LES_NGRP = 1;
 
 struct LesTileArgs {
   int n[3], nt[3];          // cells, tiles per direction
@@ -548,8 +552,11 @@ __device__ inline void les_emit(const LesOutArgs &o, int64_t i, float y) {
 // MODE: out.mode (-1: z = y; 0, 1, 2: the Chebyshev steps), fixed per launch.
 // Loads are branch-free (clamped addresses, results selected afterwards), so
 // each batch of them is one round trip to memory.
-template <int P, int NS, int MODE>
-__global__ __launch_bounds__(256) void k_les_tile(LesTileArgs a, const float *__restrict__ G,
+// NGRP tiles at a time per workgroup (NGRP groups of four wavefronts sharing
+// the staged G; every group runs the same number of rounds, the barriers are
+// the workgroup's).
+template <int P, int NS, int MODE, int NGRP>
+__global__ __launch_bounds__(256 * NGRP) void k_les_tile(LesTileArgs a, const float *__restrict__ G,
                                                   const float *__restrict__ scale,
                                                   const float *__restrict__ r, LesOutArgs out,
                                                   const SolveScalars *sc) {
@@ -562,12 +569,13 @@ __global__ __launch_bounds__(256) void k_les_tile(LesTileArgs a, const float *__
   // array: the B operands are in registers before the outputs are written
   constexpr int NRY = NB * NS > NEL * NE ? NB * NS : NEL * NE;
   __shared__ float sA[NT * NK * 64];
-  __shared__ float sRY[NRY];
+  __shared__ float sRY[NGRP][NRY];
   __shared__ int16_t sInv[NSH];
-  float *sR = sRY, *sY = sRY;
-  const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, col = l & 15, kq = l >> 4;
-  les_stage_G<NE, NT, NK>(G, sA, tid);
-  if (tid < NSH) sInv[tid] = a.inv[tid];
+  const int grp = threadIdx.x >> 8;  // this thread's tile group
+  float *sR = sRY[grp], *sY = sRY[grp];
+  const int tid = threadIdx.x & 255, wv = tid >> 6, l = tid & 63, col = l & 15, kq = l >> 4;
+  les_stage_G<NE, NT, NK, 256 * NGRP>(G, sA, threadIdx.x);
+  if (threadIdx.x < NSH) sInv[threadIdx.x] = a.inv[threadIdx.x];
   // this lane's B-operand offsets into sR relative to its element's cell
   int uo[NK];
 #pragma unroll
@@ -640,11 +648,15 @@ __global__ __launch_bounds__(256) void k_les_tile(LesTileArgs a, const float *__
   int32_t mq[NQ], mn[NQ];
   bool vq[NQ], vn[NQ];
   float rq[NQ][NS];
-  load_map(lo + xi, mq, vq);
+  const int64_t first = lo + (int64_t)xi * NGRP + grp, step = (int64_t)xg * NGRP;
+  load_map(first, mq, vq);
   load_r(mq, vq, rq);
-  load_map(lo + xi + xg, mn, vn);
-  float s_cur = scale_of(lo + xi);
-  for (int64_t tile = lo + xi; tile < hi; tile += xg) {
+  load_map(first + step, mn, vn);
+  float s_cur = scale_of(first);
+  // rounds: every group takes its tile of the round (or none past the slab)
+  for (int64_t t0 = lo + (int64_t)xi * NGRP; t0 < hi; t0 += step) {
+    const int64_t tile = t0 + grp;
+    const bool active = tile < hi;
     __syncthreads();  // the previous tile's LDS reads are done (and sA is staged)
     int32_t m[NQ];
     float rv[NQ][NS];
@@ -665,8 +677,8 @@ __global__ __launch_bounds__(256) void k_les_tile(LesTileArgs a, const float *__
       mq[j] = mn[j];
       vq[j] = vn[j];
     }
-    load_map(tile + 2 * xg, mn, vn);
-    s_cur = scale_of(tile + xg);
+    load_map(tile + 2 * step, mn, vn);
+    s_cur = scale_of(tile + step);
     // output operands of the inner nodes
     float od[NQ][NS], ox[NQ][NS], os[NQ][NS];
 #pragma unroll
@@ -732,7 +744,7 @@ __global__ __launch_bounds__(256) void k_les_tile(LesTileArgs a, const float *__
             }
           }
         }
-      } else {
+      } else if (active) {
         float *pp = a.part + (tile * NB + q) * NS;
 #pragma unroll
         for (int k = 0; k < NS; k++) pp[k] = acc[k];
@@ -1428,18 +1440,19 @@ int les_apply(Ctx &c, const float *r, float *z, const SolveScalars *sc, const Le
       o.dis = ch->dis;
     }
     const int64_t ntile = (int64_t)a.nt[0] * a.nt[1] * a.nt[2];
-    const int gt = (int)std::max<int64_t>(1, std::min<int64_t>(ntile, 768));
+    // 768 / LES_NGRP workgroups: three per CU at one tile each
+    const int gt = (int)std::max<int64_t>(1, std::min<int64_t>((ntile + LES_NGRP - 1) / LES_NGRP, 768 / LES_NGRP));
     const int gh = grid1(std::max<int64_t>(a.nhalo, 1));
 #define LES_TILE(PP, NSV)                                                                                   \
   do {                                                                                                    \
     if (o.mode < 0)                                                                                       \
-      hipLaunchKernelGGL((k_les_tile<PP, NSV, -1>), dim3(gt), dim3(256), 0, c.stream, a, G, s, r, o, sc);   \
+      hipLaunchKernelGGL((k_les_tile<PP, NSV, -1, LES_NGRP>), dim3(gt), dim3(256 * LES_NGRP), 0, c.stream, a, G, s, r, o, sc);   \
     else if (o.mode == 0)                                                                                 \
-      hipLaunchKernelGGL((k_les_tile<PP, NSV, 0>), dim3(gt), dim3(256), 0, c.stream, a, G, s, r, o, sc);    \
+      hipLaunchKernelGGL((k_les_tile<PP, NSV, 0, LES_NGRP>), dim3(gt), dim3(256 * LES_NGRP), 0, c.stream, a, G, s, r, o, sc);    \
     else if (o.mode == 1)                                                                                 \
-      hipLaunchKernelGGL((k_les_tile<PP, NSV, 1>), dim3(gt), dim3(256), 0, c.stream, a, G, s, r, o, sc);    \
+      hipLaunchKernelGGL((k_les_tile<PP, NSV, 1, LES_NGRP>), dim3(gt), dim3(256 * LES_NGRP), 0, c.stream, a, G, s, r, o, sc);    \
     else                                                                                                  \
-      hipLaunchKernelGGL((k_les_tile<PP, NSV, 2>), dim3(gt), dim3(256), 0, c.stream, a, G, s, r, o, sc);    \
+      hipLaunchKernelGGL((k_les_tile<PP, NSV, 2, LES_NGRP>), dim3(gt), dim3(256 * LES_NGRP), 0, c.stream, a, G, s, r, o, sc);    \
     hipLaunchKernelGGL((k_les_halo<NSV>), dim3(gh), dim3(256), 0, c.stream, a, r, o, sc);                   \
   } while (0)
     if (c.les_tP == 2 && c.bs == 3) LES_TILE(2, 3);
