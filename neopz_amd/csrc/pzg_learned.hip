@@ -538,17 +538,6 @@ struct LesOutArgs {
   const float *dis;
 };
 
-__device__ inline void les_emit(const LesOutArgs &o, int64_t i, float y) {
-  if (o.mode < 0) {
-    o.z[i] = y;
-    return;
-  }
-  const float dv = o.mode == 2 ? o.c1 * o.d[i] + o.c2 * y : o.c2 * y;
-  o.d[i] = dv;
-  o.x[i] = o.mode == 0 ? dv : o.x[i] + dv;
-  if (o.pt) o.pt[i] = dv * o.dis[i];
-}
-
 // MODE: out.mode (-1: z = y; 0, 1, 2: the Chebyshev steps), fixed per launch.
 // Loads are branch-free (clamped addresses, results selected afterwards), so
 // each batch of them is one round trip to memory.
@@ -754,8 +743,9 @@ __global__ __launch_bounds__(256 * NGRP) void k_les_tile(LesTileArgs a, const fl
 }
 
 // Fine nodes on tile faces, in block order: the partial sums of the tiles
-// holding them, in tile order, then the output step.
-template <int NS>
+// holding them, in tile order, then the output step (MODE as in k_les_tile;
+// the output operands are loaded before the partial sums, with them in flight).
+template <int NS, int MODE>
 __global__ __launch_bounds__(256) void k_les_halo(LesTileArgs a, const float *__restrict__ r, LesOutArgs out,
                                                   const SolveScalars *sc) {
   if (sc && sc->done) return;
@@ -763,6 +753,16 @@ __global__ __launch_bounds__(256) void k_les_halo(LesTileArgs a, const float *__
        h += (int64_t)gridDim.x * blockDim.x) {
     const int32_t m = a.hmap[h];
     const int32_t p0 = a.hptr[h], p1 = a.hptr[h + 1];
+    const int64_t B = m & 0x0FFFFFFF;
+    float od[NS], ox[NS], os[NS], orr[NS];
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+      const int64_t i = B * NS + k;
+      od[k] = MODE == 2 ? out.d[i] : 0.f;
+      ox[k] = MODE >= 1 ? out.x[i] : 0.f;
+      os[k] = MODE >= 0 && out.pt ? out.dis[i] : 0.f;
+      orr[k] = r[i];
+    }
     float acc[NS];
 #pragma unroll
     for (int k = 0; k < NS; k++) acc[k] = 0.f;
@@ -771,11 +771,18 @@ __global__ __launch_bounds__(256) void k_les_halo(LesTileArgs a, const float *__
 #pragma unroll
       for (int k = 0; k < NS; k++) acc[k] += pp[k];
     }
-    const int64_t B = m & 0x0FFFFFFF;
 #pragma unroll
     for (int k = 0; k < NS; k++) {
       const int64_t i = B * NS + k;
-      les_emit(out, i, ((m >> (28 + k)) & 1) ? r[i] : acc[k]);
+      const float y = ((m >> (28 + k)) & 1) ? orr[k] : acc[k];
+      if (MODE < 0) {
+        out.z[i] = y;
+      } else {
+        const float dv = MODE == 2 ? out.c1 * od[k] + out.c2 * y : out.c2 * y;
+        out.d[i] = dv;
+        out.x[i] = MODE == 0 ? dv : ox[k] + dv;
+        if (out.pt) out.pt[i] = dv * os[k];
+      }
     }
   }
 }
@@ -1443,23 +1450,25 @@ int les_apply(Ctx &c, const float *r, float *z, const SolveScalars *sc, const Le
     // 768 / LES_NGRP workgroups: three per CU at one tile each
     const int gt = (int)std::max<int64_t>(1, std::min<int64_t>((ntile + LES_NGRP - 1) / LES_NGRP, 768 / LES_NGRP));
     const int gh = grid1(std::max<int64_t>(a.nhalo, 1));
+#define LES_TILE_M(PP, NSV, MD)                                                                             \
+  do {                                                                                                    \
+    hipLaunchKernelGGL((k_les_tile<PP, NSV, MD, LES_NGRP>), dim3(gt), dim3(256 * LES_NGRP), 0, c.stream, a, G, s, \
+                       r, o, sc);                                                                         \
+    hipLaunchKernelGGL((k_les_halo<NSV, MD>), dim3(gh), dim3(256), 0, c.stream, a, r, o, sc);               \
+  } while (0)
 #define LES_TILE(PP, NSV)                                                                                   \
   do {                                                                                                    \
-    if (o.mode < 0)                                                                                       \
-      hipLaunchKernelGGL((k_les_tile<PP, NSV, -1, LES_NGRP>), dim3(gt), dim3(256 * LES_NGRP), 0, c.stream, a, G, s, r, o, sc);   \
-    else if (o.mode == 0)                                                                                 \
-      hipLaunchKernelGGL((k_les_tile<PP, NSV, 0, LES_NGRP>), dim3(gt), dim3(256 * LES_NGRP), 0, c.stream, a, G, s, r, o, sc);    \
-    else if (o.mode == 1)                                                                                 \
-      hipLaunchKernelGGL((k_les_tile<PP, NSV, 1, LES_NGRP>), dim3(gt), dim3(256 * LES_NGRP), 0, c.stream, a, G, s, r, o, sc);    \
-    else                                                                                                  \
-      hipLaunchKernelGGL((k_les_tile<PP, NSV, 2, LES_NGRP>), dim3(gt), dim3(256 * LES_NGRP), 0, c.stream, a, G, s, r, o, sc);    \
-    hipLaunchKernelGGL((k_les_halo<NSV>), dim3(gh), dim3(256), 0, c.stream, a, r, o, sc);                   \
+    if (o.mode < 0) LES_TILE_M(PP, NSV, -1);                                                              \
+    else if (o.mode == 0) LES_TILE_M(PP, NSV, 0);                                                         \
+    else if (o.mode == 1) LES_TILE_M(PP, NSV, 1);                                                         \
+    else LES_TILE_M(PP, NSV, 2);                                                                          \
   } while (0)
     if (c.les_tP == 2 && c.bs == 3) LES_TILE(2, 3);
     else if (c.les_tP == 2) LES_TILE(2, 1);
     else if (c.bs == 3) LES_TILE(1, 3);
     else LES_TILE(1, 1);
 #undef LES_TILE
+#undef LES_TILE_M
     PZG_HIP(hipGetLastError());
     return PZG_OK;
   }
