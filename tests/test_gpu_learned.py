@@ -127,7 +127,7 @@ def test_smoother_apply_matches_numpy(name):
 def test_tiled_and_element_major_application_agree():
     """On hexahedral lattice meshes the smoother runs in lattice tiles; the
     element-major kernels (PZG_LES_TILES=0, a separate process) give the same
-    z within FP32 summation-order differences."""
+    z within FP32 summation-order differences, and the same learned V-cycle."""
     import os
     import subprocess
     import sys
@@ -139,8 +139,10 @@ def test_tiled_and_element_major_application_agree():
     w = ls.get_weights()
     rng = np.random.default_rng(9)
     ne = [k for k in range(1, 200) if R.offsets(k)["n"] == len(w)][0]
-    G = rng.standard_normal((ne, ne))
-    w[: ne * ne] = (0.5 * (G + G.T)).ravel()
+    B = rng.standard_normal((ne, ne))  # G kept SPD: the cycle's eigenvalue estimate needs it
+    G = w[: ne * ne].reshape(ne, ne).astype(np.float64) + 0.02 * (B @ B.T) / ne
+    w[: ne * ne] = G.ravel().astype(np.float32)
+    w[R.offsets(ne)["w"]:R.offsets(ne)["w"] + R.NH] = 0.5 * rng.standard_normal(R.NH)
     ls.set_weights(w)
     r = rng.standard_normal(K.neq)
     z_tile = ls.apply(r)
@@ -158,12 +160,25 @@ def test_tiled_and_element_major_application_agree():
             "sm = TPZSSpStructMatrixHIP(c, M.build(pb)); K = sm.Create(); sm.Assemble(K)\n"
             "ls = LearnedSmoother(c); ls.setup(seed=3)\n"
             f"ls.set_weights(np.load({os.path.join(d, 'w.npy')!r}))\n"
-            f"np.save({os.path.join(d, 'z.npy')!r}, ls.apply(np.load({os.path.join(d, 'r.npy')!r})))\n")
+            f"r = np.load({os.path.join(d, 'r.npy')!r})\n"
+            f"np.save({os.path.join(d, 'z.npy')!r}, ls.apply(r))\n"
+            "import ctypes as C\n"
+            "from neopz_amd import _lib as L\n"
+            "zc = np.zeros_like(r)\n"
+            "L.check(c.lib.pzg_mg_apply(c.h, L.INNER_MATFREE | L.MG_LEARNED, 0, 0.0, L.ptr(r, L.dp), "
+            "L.ptr(zc, L.dp), None, None, None))\n"
+            f"np.save({os.path.join(d, 'zc.npy')!r}, zc)\n")
         env = dict(os.environ, PZG_LES_TILES="0")
         p = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=240)
         assert p.returncode == 0, p.stderr[-2000:]
         z_elem = np.load(os.path.join(d, "z.npy"))
+        zc_elem = np.load(os.path.join(d, "zc.npy"))
     assert np.linalg.norm(z_tile - z_elem) <= 1e-5 * np.linalg.norm(z_elem)
+    # the learned V-cycle on either path
+    zc = np.zeros_like(r)
+    L.check(ctx().lib.pzg_mg_apply(ctx().h, L.INNER_MATFREE | L.MG_LEARNED, 0, 0.0, L.ptr(r, L.dp),
+                                   L.ptr(zc, L.dp), None, None, None))
+    assert np.linalg.norm(zc - zc_elem) <= 1e-4 * np.linalg.norm(zc_elem)
 
 
 @pytest.mark.parametrize("op", [L.INNER_BSR, L.INNER_MATFREE])
