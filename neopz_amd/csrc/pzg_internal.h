@@ -345,7 +345,8 @@ struct Ctx {
   // sets of the penalized boundary elements with their inverses, the row ->
   // slot gather, CG work vectors and scalars
   bool pen_ready = false, pen_struct = false;  // values of this assembly; structure of this mesh
-  int64_t pen_n = 0, pen_nnz = 0, pen_nset = 0, pen_nslot = 0;
+  int64_t pen_n = 0, pen_nnz = 0, pen_nset = 0, pen_nslot = 0, pen_narow = 0;
+  DevBuf d_pen_arow, d_pen_delta;
   DevBuf d_pen_idx, d_pen_pmap, d_pen_ia, d_pen_ja, d_pen_a, d_pen_sptr, d_pen_sidx, d_pen_sset, d_pen_soff,
       d_pen_sval, d_pen_sinv, d_pen_gptr, d_pen_gslot, d_pen_w, d_pen_scal;
   hipGraphExec_t pen_graph = nullptr;  // one batch of its CG iterations
@@ -427,7 +428,7 @@ int les_fit_G(Ctx &c);  // G from the accumulated least-squares sums
 // penalty rows of the learned path (pzg_penalty.hip)
 int pen_setup(Ctx &c);
 int pen_solve(Ctx &c, double *r64, double *u64, double stop_abs, bool use_graph, int64_t *its, double *rr0,
-              double *rr1, double frac = 0., double tot2 = 0.);
+              double *rr1, double frac = 0., double tot2 = 0., double *rr_full = nullptr);
 int pen_zero32(Ctx &c, float *r32);
 void pen_free(Ctx &c);
 int mf_setup(Ctx &c);

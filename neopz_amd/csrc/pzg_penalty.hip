@@ -284,6 +284,58 @@ __global__ void k_pen_scatter(int64_t np, const int64_t *__restrict__ idx, const
   }
 }
 
+// u_p += x; delta_p = x (delta is zero off the penalty rows)
+__global__ void k_pen_scatter_d(int64_t np, const int64_t *__restrict__ idx, const double *__restrict__ x,
+                                double *__restrict__ u, double *__restrict__ delta) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < np; k += (int64_t)gridDim.x * blockDim.x) {
+    u[idx[k]] += x[k];
+    delta[idx[k]] = x[k];
+  }
+}
+
+// the rows whose block row holds a penalized block: flag[A] = 1
+__global__ void k_pen_mark(int64_t npb, const int32_t *__restrict__ pblk, const int64_t *__restrict__ fbia,
+                           const int32_t *__restrict__ fbja, unsigned char *__restrict__ flag) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= npb) return;
+  const int64_t B = pblk[k];
+  for (int64_t q = fbia[B]; q < fbia[B + 1]; q++) flag[fbja[q]] = 1;  // symmetric pattern
+}
+
+// r -= K delta on the listed block rows (one thread per row and component);
+// delta is zero off the penalty rows, so this is the exact change of the
+// residual f - K u from u_p += delta_p
+template <int BS>
+__global__ void k_pen_rupd(int64_t nr, const int32_t *__restrict__ rows, const int64_t *__restrict__ fbia,
+                           const int32_t *__restrict__ fbja, const double *__restrict__ fa,
+                           const double *__restrict__ delta, double *__restrict__ r) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nr * BS) return;
+  const int64_t A = rows[t / BS];
+  const int k = (int)(t % BS);
+  const int64_t b0 = fbia[A], n = fbia[A + 1] - b0;
+  double y = 0.;
+  for (int64_t i = 0; i < n; i++) {
+    const int64_t B = fbja[b0 + i];
+#pragma unroll
+    for (int l = 0; l < BS; l++) y += fa[bsr_pos<BS * BS>(b0, n, i, k * BS + l)] * delta[B * BS + l];
+  }
+  r[A * BS + k] -= y;
+}
+
+// ||r||^2 over n entries: per-block partials, then one block sums them
+__global__ __launch_bounds__(TPB) void k_pen_norm2(int64_t n, const double *__restrict__ r, double *partials) {
+  double s = 0.;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    s += r[i] * r[i];
+  double v[1] = {s};
+  block_partials<1>(v, partials);
+}
+__global__ void k_pen_norm2_fin(const double *partials, int nblk, double *out) {
+  const double s = sum_partials(partials, 0, nblk);
+  if (threadIdx.x == 0) *out = s;
+}
+
 // r (float, scaled residual of the inner iterations) = 0 on the penalty rows
 __global__ void k_pen_zero32(int64_t np, const int64_t *__restrict__ idx, float *__restrict__ r) {
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < np; k += (int64_t)gridDim.x * blockDim.x)
@@ -407,6 +459,34 @@ static int pen_structure(Ctx &c) {
   PZG_TRY(dev_upload(c.d_pen_gslot, gslot.data(), gslot.size(), s));
   PZG_TRY(dev_alloc(c.d_pen_w, sizeof(double) * (5 * np + nslot)));
   PZG_TRY(dev_alloc(c.d_pen_scal, sizeof(PenScal) + 16));
+  // the block rows that a change of the penalized unknowns reaches (the
+  // columns of the penalized block rows; the pattern is symmetric), and the
+  // FP64 change itself, zero off the penalized rows
+  {
+    std::vector<int32_t> pblk;
+    for (int64_t k = 0; k < np; k++)
+      if (pblk.empty() || pblk.back() != (int32_t)(idx[k] / bs)) pblk.push_back((int32_t)(idx[k] / bs));
+    const int64_t nbr = c.neq / bs;
+    DevBuf d_pblk, d_flag;
+    PZG_TRY(dev_upload(d_pblk, pblk.data(), pblk.size(), s));
+    PZG_TRY(dev_alloc(d_flag, (size_t)nbr));
+    PZG_HIP(hipMemsetAsync(d_flag.p, 0, (size_t)nbr, s));
+    const int64_t npb = (int64_t)pblk.size();
+    hipLaunchKernelGGL(k_pen_mark, dim3((unsigned)((npb + TPB - 1) / TPB)), dim3(TPB), 0, s, npb,
+                       d_pblk.as<int32_t>(), c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(),
+                       d_flag.as<unsigned char>());
+    PZG_HIP(hipGetLastError());
+    std::vector<unsigned char> flag(nbr);
+    PZG_HIP(hipMemcpyAsync(flag.data(), d_flag.p, (size_t)nbr, hipMemcpyDeviceToHost, s));
+    PZG_HIP(hipStreamSynchronize(s));
+    std::vector<int32_t> arow;
+    for (int64_t A = 0; A < nbr; A++)
+      if (flag[A]) arow.push_back((int32_t)A);
+    c.pen_narow = (int64_t)arow.size();
+    PZG_TRY(dev_upload(c.d_pen_arow, arow.data(), arow.size(), s));
+    PZG_TRY(dev_alloc(c.d_pen_delta, sizeof(double) * c.neq));
+    PZG_HIP(hipMemsetAsync(c.d_pen_delta.p, 0, sizeof(double) * c.neq, s));
+  }
   PZG_HIP(hipStreamSynchronize(s));
   c.pen_struct = true;
   return PZG_OK;
@@ -472,12 +552,15 @@ int pen_setup(Ctx &c) {
 // u_p += d = K_pp^-1 r_p (r64: the FP64 residual f - K u of the whole system),
 // by Schwarz-preconditioned CG until ||r_p - K_pp d|| <= max(stop_abs, frac
 // ||r_other||) (tot2 = ||r||^2 of the whole residual: the other rows' part is
-// what the next FP32 iterations leave, and solving past it gains nothing).  r64's
-// penalty rows receive r_p - K_pp d; its other rows change by -K_ip d, which
-// is left to the next true residual (|K_ip d| ~ |r_p| E/(big h)).  *its: the
-// CG iterations taken; *rr0, *rr1: |r_p|^2 before and after.
+// what the next FP32 iterations leave, and solving past it gains nothing).
+// With rr_full, r64 becomes the exact residual of the new u (r64 -= K delta on
+// the block rows that the penalized columns reach, delta = d on the penalized
+// rows and zero elsewhere) and *rr_full its squared norm (-1 when u did not
+// change); without it, r64's penalty rows receive r_p - K_pp d and its other
+// rows keep the -K_ip d change for the next true residual.  *its: the CG
+// iterations taken; *rr0, *rr1: |r_p|^2 before and after.
 int pen_solve(Ctx &c, double *r64, double *u64, double stop_abs, bool use_graph, int64_t *its, double *rr0,
-              double *rr1, double frac, double tot2) {
+              double *rr1, double frac, double tot2, double *rr_full) {
   if (its) *its = 0;
   if (rr0) *rr0 = 0.;
   if (rr1) *rr1 = 0.;
@@ -550,11 +633,33 @@ int pen_solve(Ctx &c, double *r64, double *u64, double stop_abs, bool use_graph,
     set_error("penalty rows: K_pp not positive definite");
     return PZG_EUNSUP;
   }
-  if (hs.it > 0) {
+  if (hs.it > 0 && rr_full && c.pen_narow > 0) {
+    // the exact residual: r -= K delta on the rows delta reaches, then its norm
+    hipLaunchKernelGGL(k_pen_scatter_d, dim3(g), dim3(TPB), 0, s, np, idx, (const double *)x, u64,
+                       c.d_pen_delta.as<double>());
+    const int64_t th = c.pen_narow * c.bs;
+    if (c.bs == 3)
+      hipLaunchKernelGGL(k_pen_rupd<3>, dim3((unsigned)((th + TPB - 1) / TPB)), dim3(TPB), 0, s, c.pen_narow,
+                         c.d_pen_arow.as<int32_t>(), c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(),
+                         c.d_fa.as<double>(), (const double *)c.d_pen_delta.as<double>(), r64);
+    else
+      hipLaunchKernelGGL(k_pen_rupd<1>, dim3((unsigned)((th + TPB - 1) / TPB)), dim3(TPB), 0, s, c.pen_narow,
+                         c.d_pen_arow.as<int32_t>(), c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(),
+                         c.d_fa.as<double>(), (const double *)c.d_pen_delta.as<double>(), r64);
+    const int gn = pen_grid(c.neq);
+    hipLaunchKernelGGL(k_pen_norm2, dim3(gn), dim3(TPB), 0, s, c.neq, (const double *)r64, part);
+    hipLaunchKernelGGL(k_pen_norm2_fin, dim3(1), dim3(TPB), 0, s, (const double *)part, gn, &sc->rr0);
+    PZG_HIP(hipGetLastError());
+    double h = 0.;
+    PZG_HIP(hipMemcpyAsync(&h, &sc->rr0, sizeof(double), hipMemcpyDeviceToHost, s));
+    PZG_HIP(hipStreamSynchronize(s));
+    *rr_full = h;
+  } else if (hs.it > 0) {
     hipLaunchKernelGGL(k_pen_scatter, dim3(g), dim3(TPB), 0, s, np, idx, (const double *)x, (const double *)r, u64,
                        r64);
     PZG_HIP(hipGetLastError());
   }
+  if (rr_full && hs.it == 0) *rr_full = -1.;  // nothing changed: the caller's norm stands
   if (its) *its = hs.it;
   if (rr0) *rr0 = hs.rr0;
   if (rr1) *rr1 = hs.rr;
@@ -573,12 +678,12 @@ int pen_zero32(Ctx &c, float *r32) {
 void pen_free(Ctx &c) {
   for (DevBuf *b : {&c.d_pen_idx, &c.d_pen_pmap, &c.d_pen_ia, &c.d_pen_ja, &c.d_pen_a, &c.d_pen_sptr,
                     &c.d_pen_sidx, &c.d_pen_sset, &c.d_pen_soff, &c.d_pen_sval, &c.d_pen_sinv, &c.d_pen_gptr,
-                    &c.d_pen_gslot, &c.d_pen_w, &c.d_pen_scal})
+                    &c.d_pen_gslot, &c.d_pen_w, &c.d_pen_scal, &c.d_pen_arow, &c.d_pen_delta})
     dev_free(*b);
   if (c.pen_graph) (void)hipGraphExecDestroy(c.pen_graph);
   c.pen_graph = nullptr;
   c.pen_ready = c.pen_struct = false;
-  c.pen_n = c.pen_nnz = c.pen_nset = c.pen_nslot = 0;
+  c.pen_n = c.pen_nnz = c.pen_nset = c.pen_nslot = c.pen_narow = 0;
 }
 
 }  // namespace pzg

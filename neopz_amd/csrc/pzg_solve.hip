@@ -1082,23 +1082,24 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
   // learned path: the penalty rows are solved in FP64 here, after every fold,
   // to a tenth of the target residual; the FP32 iterations leave them alone
   const bool pen_rows = learned && !conv && c.pen_ready && c.pen_n > 0;
-  // After the penalty solve the residual's norm is known up to the -K_ip d
-  // change of the other rows (see pen_solve); rel is then that estimate, and a
-  // true residual confirms it before the solve is called converged.
+  // The penalty solve keeps d_r the exact residual f - K u (it subtracts
+  // K delta on the rows that the penalized unknowns reach) and returns its
+  // norm; rel_exact turns false only where an estimate stands in for it.
   int64_t pen_its = 0;
   bool rel_exact = true;
   auto pen_fix = [&]() -> int {
     pen_its = 0;
     if (!pen_rows || rel <= prm.tol) return PZG_OK;
-    double p0 = 0., p1 = 0.;
+    double p0 = 0., p1 = 0., rrf = -1.;
     // to a tenth of the target, or of what the other rows still carry
     const double tot2 = rel * rel * fnorm * fnorm;
     PZG_TRY(pen_solve(c, c.d_r.as<double>(), c.d_u.as<double>(), 0.1 * prm.tol * fnorm, prm.use_graph != 0,
-                      &pen_its, &p0, &p1, 0.1, tot2));
+                      &pen_its, &p0, &p1, 0.1, tot2, &rrf));
     if (pen_its == 0) return PZG_OK;  // u unchanged
-    const double rr = rel * rel * fnorm * fnorm - p0 + p1;
-    rel = std::sqrt(std::max(rr, p1)) / (fnorm > 0 ? fnorm : 1.);
-    rel_exact = false;
+    // without the exact update: the estimate with the -K_ip d change left out
+    const double rr = rrf >= 0. ? rrf : std::max(tot2 - p0 + p1, p1);
+    rel = std::sqrt(rr) / (fnorm > 0 ? fnorm : 1.);
+    rel_exact = rrf >= 0.;
     return PZG_OK;
   };
   // the test on the estimate, confirmed by a true residual
