@@ -44,7 +44,7 @@ extern "C" {
 #define PZG_QUAD 2
 #define PZG_HEX 3
 #define PZG_TRI 4  /* TPZTriangle: boundary elements of tetrahedral meshes */
-#define PZG_TET 5  /* TPZTetrahedron, H1 p <= 2                           */
+#define PZG_TET 5  /* TPZTetrahedron, H1 p <= 5 (hp: per-element orders)  */
 
 #define PZG_MAT_POISSON 0  /* TPZMatPoisson   (Material/Poisson/TPZMatPoisson.cpp) */
 #define PZG_MAT_DARCY 1    /* TPZDarcyFlow    (Material/DarcyFlow/TPZDarcyFlow.cpp) */
@@ -91,7 +91,8 @@ typedef struct {
   int32_t dim;        /* 2 or 3                                               */
   int32_t material;   /* PZG_MAT_*                                            */
   int32_t nstate;     /* 1 (Poisson/Darcy) or 3 (Elasticity3D)                */
-  int32_t porder;     /* uniform H1 order, 1 or 2                             */
+  int32_t porder;     /* H1 order: 1 or 2; 1..5 on tetrahedral meshes (the
+                         highest order when vol_rule carries hp orders)      */
   int64_t nnodes;
   const double *xyz;  /* 3 * nnodes                                           */
   /* volume elements: PZG_QUAD (dim 2), PZG_HEX or PZG_TET (dim 3)           */
@@ -339,9 +340,10 @@ int pzg_time_kernel(pzg_ctx *ctx, int32_t which, int32_t nrep, double *avg_ms, d
  * the capacity of the arrays; on return it is the number of rule buckets,
  * order[b] the bucket's highest element order, variant[b] the kernel variant
  * kept and ms[b*5 + v] the measured milliseconds per launch of variant v
- * (0: 16 points/pass, 4x4 tiles; 1: 8, 4x4; 2: 32, 4x4; 3: 16, 2x2;
- * 4: 32, 2x2).  *n = 0 when the mesh has no high-order tetrahedra or has not
- * been assembled. */
+ * (scalar materials 0: 16 points/pass, 4x4 tiles; 1: 8, 4x4; 2: 32, 4x4;
+ * 3: 16, 2x2; 4: 32, 2x2; Elasticity3D 0: 16, 2x2; 1: 8, 2x2; 2: 32, 2x2;
+ * 3: 16, 1x1; 4: 8, 1x1).  *n = 0 when the mesh has no high-order
+ * tetrahedra or has not been assembled. */
 int pzg_hp_tuning(pzg_ctx *ctx, int32_t *n, int32_t *order, int32_t *variant, float *ms);
 
 /* y = S x with S = D^-1/2 K D^-1/2 applied by the FP32 inner operator op

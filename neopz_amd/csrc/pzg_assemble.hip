@@ -903,7 +903,197 @@ __global__ __launch_bounds__(TPB) void k_vol_hp(ElemArgs g, int nsv, const HpTab
     g.rhs[s_blk[tid]] += g.material == PZG_MAT_POISSON ? accf * g.scale : accf;
 }
 
-template <int NSUB>
+// TPZElasticity3D on high-order tetrahedra: the same tiles of shape pairs,
+// each keeping the nine directional sums Q_xy(i,j) = sum_q w|J| dphi_x(i)
+// dphi_y(j), from which the pair's 3x3 block is formed as in k_vol (C1, C2,
+// C3 of TPZElasticity3D.h:183-188).  A pair (i, j), i <= j in shape order, is
+// scattered to the upper triangle of the equation numbering: the block of
+// row i when block(i) < block(j), the transposed block of row j otherwise
+// (K is symmetric); the diagonal block keeps its upper part.
+template <int NSUB, int QC, int TS>
+__global__ __launch_bounds__(TPB) void k_vol_hp3(ElemArgs g, int nsv, const HpTables *__restrict__ tab,
+                                                 const int32_t *__restrict__ sub,
+                                                 const int32_t *__restrict__ list) {
+  constexpr int NT = (NSUB + TS - 1) / TS, NSP = TS * NT;
+  constexpr int NTILE = NT * (NT + 1) / 2;
+  constexpr int MT = (NTILE + TPB - 1) / TPB;
+  __shared__ double s_coord[12];
+  __shared__ double s_jinv[3][3];
+  __shared__ double s_det;
+  __shared__ double s_G[QC][3][NSP];
+  __shared__ double s_Gw[QC][3][NSP];
+  __shared__ double s_pw[QC][NSP];
+  __shared__ double s_f[QC][3];
+  __shared__ int s_col[NSUB];
+  __shared__ int64_t s_blk[NSUB], s_row[NSUB][3];
+  __shared__ int32_t s_map[NSUB * NSUB];
+  const int tid = threadIdx.x;
+  const int64_t e = list[blockIdx.x];
+  const HpTables &T = *tab;
+  if (tid < 12) s_coord[tid] = g.xyz[3 * g.nodes[e * 4 + tid / 3] + tid % 3];
+  for (int k = tid; k < NSUB * NSUB; k += TPB) {
+    const int i = k / NSUB, j = k % NSUB;
+    s_map[k] = g.map[((int64_t)e * nsv + sub[i]) * nsv + sub[j]];
+  }
+  if (tid < NSUB) {
+    int64_t id[4];
+    for (int k = 0; k < 4; k++) id[k] = g.nodes[e * 4 + k];
+    const int gi = sub[tid];
+    s_col[tid] = T.col_off[gi] + hp_variant(true, T.shape_side[gi], id);
+    const int64_t b = g.aidx[g.conn[e * nsv + gi]];
+    s_blk[tid] = b;
+    for (int c = 0; c < 3; c++) s_row[tid][c] = b >= 0 ? g.ia[b * 3 + c] : -1;
+  }
+  if constexpr (NSP > NSUB) {
+    for (int k = tid; k < QC * 3 * (NSP - NSUB); k += TPB) {
+      const int qa = k / (NSP - NSUB), i = NSUB + k % (NSP - NSUB);
+      s_G[qa / 3][qa % 3][i] = 0.;
+      s_Gw[qa / 3][qa % 3][i] = 0.;
+    }
+  }
+  if (tid == 0) {
+    double j[3][3];
+    for (int i = 0; i < 3; i++)
+      for (int k = 0; k < 3; k++) j[i][k] = g.xyz[3 * g.nodes[e * 4 + k + 1] + i] - g.xyz[3 * g.nodes[e * 4] + i];
+    const double det = -j[0][2] * j[1][1] * j[2][0] + j[0][1] * j[1][2] * j[2][0] +
+                       j[0][2] * j[1][0] * j[2][1] - j[0][0] * j[1][2] * j[2][1] -
+                       j[0][1] * j[1][0] * j[2][2] + j[0][0] * j[1][1] * j[2][2];
+    s_jinv[0][0] = (-j[1][2] * j[2][1] + j[1][1] * j[2][2]) / det;
+    s_jinv[0][1] = (j[0][2] * j[2][1] - j[0][1] * j[2][2]) / det;
+    s_jinv[0][2] = (-j[0][2] * j[1][1] + j[0][1] * j[1][2]) / det;
+    s_jinv[1][0] = (j[1][2] * j[2][0] - j[1][0] * j[2][2]) / det;
+    s_jinv[1][1] = (-j[0][2] * j[2][0] + j[0][0] * j[2][2]) / det;
+    s_jinv[1][2] = (j[0][2] * j[1][0] - j[0][0] * j[1][2]) / det;
+    s_jinv[2][0] = (-j[1][1] * j[2][0] + j[1][0] * j[2][1]) / det;
+    s_jinv[2][1] = (j[0][1] * j[2][0] - j[0][0] * j[2][1]) / det;
+    s_jinv[2][2] = (-j[0][1] * j[1][0] + j[0][0] * j[1][1]) / det;
+    s_det = fabs(det);
+  }
+  int ti[MT], tj[MT];
+#pragma unroll
+  for (int m = 0; m < MT; m++) {
+    int t = tid + m * TPB, a = 0;
+    if (t < NTILE) {
+      while (t >= NT - a) { t -= NT - a; a++; }
+      ti[m] = a;
+      tj[m] = a + t;
+    } else {
+      ti[m] = tj[m] = -1;
+    }
+  }
+  __syncthreads();
+  double acc[MT][TS][TS][9];
+#pragma unroll
+  for (int m = 0; m < MT; m++)
+#pragma unroll
+    for (int r = 0; r < TS; r++)
+#pragma unroll
+      for (int q = 0; q < TS; q++)
+#pragma unroll
+        for (int x = 0; x < 9; x++) acc[m][r][q][x] = 0.;
+  double accf[3] = {0., 0., 0.}, accp[3] = {0., 0., 0.};
+  const int nq = T.nq;
+  for (int q0 = 0; q0 < nq; q0 += QC) {
+    const int nqc = nq - q0 < QC ? nq - q0 : QC;
+    if (tid < nqc) {
+      const double *xi = T.pt[q0 + tid];
+      const double l[4] = {1. - xi[0] - xi[1] - xi[2], xi[0], xi[1], xi[2]};
+      double x[3] = {0., 0., 0.}, f[3];
+      for (int n = 0; n < 4; n++)
+        for (int i = 0; i < 3; i++) x[i] += s_coord[3 * n + i] * l[n];
+      forcing(g, x, f);
+      s_f[tid][0] = f[0];
+      s_f[tid][1] = f[1];
+      s_f[tid][2] = f[2];
+    }
+    for (int k = tid; k < nqc * NSUB; k += TPB) {
+      const int qq = k / NSUB, i = k % NSUB, q = q0 + qq, col = s_col[i];
+      const double wd = T.w[q] * s_det;
+      for (int a = 0; a < 3; a++) {
+        double sum = 0.;
+        for (int b = 0; b < 3; b++) sum += s_jinv[b][a] * T.dphi[q][b][col];
+        s_G[qq][a][i] = sum;
+        s_Gw[qq][a][i] = sum * wd;
+      }
+      s_pw[qq][i] = wd * T.phi[q][col];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < MT; m++) {
+      if (ti[m] < 0) continue;
+      for (int qq = 0; qq < nqc; qq++) {
+        double u[3][TS], v[3][TS];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+          for (int r = 0; r < TS; r++) {
+            u[a][r] = s_Gw[qq][a][TS * ti[m] + r];
+            v[a][r] = s_G[qq][a][TS * tj[m] + r];
+          }
+#pragma unroll
+        for (int r = 0; r < TS; r++)
+#pragma unroll
+          for (int q = 0; q < TS; q++)
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+              for (int b = 0; b < 3; b++) acc[m][r][q][3 * a + b] += u[a][r] * v[b][q];
+      }
+    }
+    if (tid < NSUB)
+      for (int qq = 0; qq < nqc; qq++) {
+        for (int c = 0; c < 3; c++) {
+          accf[c] += s_pw[qq][tid] * s_f[qq][c];
+          accp[c] += s_Gw[qq][c][tid];
+        }
+      }
+    __syncthreads();
+  }
+  const double E = g.coef ? g.coef[e] : g.E, nu = g.nu;
+  const double C1 = E / (2. + 2. * nu), C2 = E * nu / (-1. + nu + 2. * nu * nu),
+               C3 = E * (nu - 1.) / (-1. + nu + 2. * nu * nu);
+#pragma unroll
+  for (int m = 0; m < MT; m++) {
+    if (ti[m] < 0) continue;
+#pragma unroll
+    for (int r = 0; r < TS; r++)
+#pragma unroll
+      for (int q = 0; q < TS; q++) {
+        const int i = TS * ti[m] + r, j = TS * tj[m] + q;
+        if (i >= NSUB || j >= NSUB || (ti[m] == tj[m] && q < r)) continue;
+        const int64_t bi = s_blk[i], bj = s_blk[j];
+        if (bi < 0 || bj < 0) continue;
+        const double *Q = acc[m][r][q];  // Q[3x+y] = sum w dphi_x(i) dphi_y(j)
+        for (int ci = 0; ci < 3; ci++)
+          for (int cj = 0; cj < 3; cj++) {
+            if (bi == bj && cj < ci) continue;
+            double v;
+            if (ci == cj) {
+              double tr = 0.;
+              for (int cc = 0; cc < 3; cc++)
+                if (cc != ci) tr += Q[4 * cc];
+              v = tr * C1 + Q[4 * ci] * C3;
+            } else {
+              v = Q[3 * cj + ci] * C1 - Q[3 * ci + cj] * C2;
+            }
+            if (fabs(v) < 1.e-12) continue;  // IsZero drop rule of AddKel
+            int64_t slot;
+            if (bi == bj) slot = s_row[i][ci] + (cj - ci);
+            else if (bi < bj) slot = s_row[i][ci] + (3 - ci) + 3 * (int64_t)s_map[i * NSUB + j] + cj;
+            else slot = s_row[j][cj] + (3 - cj) + 3 * (int64_t)s_map[j * NSUB + i] + ci;
+            g.a[slot] += v;
+          }
+      }
+  }
+  if (tid < NSUB && s_blk[tid] >= 0)
+    for (int c = 0; c < 3; c++)  // body load, minus w prestress_c dphix_c (fPreStress)
+      g.rhs[s_blk[tid] * 3 + c] += accf[c] - g.prestress[c] * accp[c];
+}
+
+// NS states: ek block (i, j) = ck (NS x NS) M_ij, ef_i = cf m_i with the
+// boundary mass M and moment m (TPZMatPoisson type 2 and the TPZElasticity3D
+// stress field are refused on these meshes by pzg_mesh_set).
+template <int NSUB, int NS>
 __global__ __launch_bounds__(TPB) void k_bc_hp(ElemArgs g, int nsv, const HpTables *__restrict__ tab,
                                                const int32_t *__restrict__ sub,
                                                const int32_t *__restrict__ list) {
@@ -914,15 +1104,13 @@ __global__ __launch_bounds__(TPB) void k_bc_hp(ElemArgs g, int nsv, const HpTabl
   __shared__ double s_p[QC][NSUB];
   __shared__ double s_pw[QC][NSUB];
   __shared__ int s_col[NSUB];
-  __shared__ int64_t s_blk[NSUB], s_row[NSUB];
+  __shared__ int64_t s_blk[NSUB], s_row[NSUB][NS];
   __shared__ int32_t s_map[NPAIR];
   const int tid = threadIdx.x;
   const int64_t e = list[blockIdx.x];
   const HpTables &T = *tab;
   const int id = g.bcid[e];
-  // scalar materials: ek = ck M, ef = cf m (TPZMatPoisson type 2 is refused
-  // on these meshes by pzg_mesh_set)
-  const double ck = g.bccoef[(int64_t)id * BCC + BCC_CK], cfl = g.bccoef[(int64_t)id * BCC + BCC_CF];
+  const double *cfs = g.bccoef + (int64_t)id * BCC;
   for (int k = tid; k < NPAIR; k += TPB) {
     const int i = k / NSUB, j = k % NSUB;
     s_map[k] = g.map[((int64_t)e * nsv + sub[i]) * nsv + sub[j]];
@@ -934,7 +1122,7 @@ __global__ __launch_bounds__(TPB) void k_bc_hp(ElemArgs g, int nsv, const HpTabl
     s_col[tid] = T.col_off[gi] + hp_variant(false, T.shape_side[gi], ids);
     const int64_t b = g.aidx[g.conn[e * nsv + gi]];
     s_blk[tid] = b;
-    s_row[tid] = b >= 0 ? g.ia[b] : -1;
+    for (int c = 0; c < NS; c++) s_row[tid][c] = b >= 0 ? g.ia[b * NS + c] : -1;
   }
   if (tid == 0) {
     // 2D element in 3D space (TPZGeoEl::Jacobian): |det| = |g0| * |g1 - (g0.g1/|g0|^2) g0|
@@ -979,20 +1167,22 @@ __global__ __launch_bounds__(TPB) void k_bc_hp(ElemArgs g, int nsv, const HpTabl
       for (int qq = 0; qq < nqc; qq++) accf += s_pw[qq][tid];
     __syncthreads();
   }
-  if (ck != 0.) {
 #pragma unroll
-    for (int k = 0; k < PER; k++) {
-      const int idx = tid + k * TPB;
-      if (idx >= NPAIR) continue;
-      const int i = idx / NSUB, j = idx % NSUB;
-      const int64_t bi = s_blk[i], bj = s_blk[j];
-      if (bi < 0 || bj < 0 || bi > bj) continue;
-      const double v = ck * acc[k];
-      if (fabs(v) < 1.e-12) continue;
-      g.a[bi == bj ? s_row[i] : s_row[i] + 1 + s_map[i * NSUB + j]] += v;
-    }
+  for (int k = 0; k < PER; k++) {
+    const int idx = tid + k * TPB;
+    if (idx >= NPAIR) continue;
+    const int i = idx / NSUB, j = idx % NSUB;
+    const int64_t bi = s_blk[i], bj = s_blk[j];
+    if (bi < 0 || bj < 0 || bi > bj) continue;
+    for (int ci = 0; ci < NS; ci++)
+      for (int cj = bi == bj ? ci : 0; cj < NS; cj++) {
+        const double v = cfs[BCC_CK + NS * ci + cj] * acc[k];
+        if (fabs(v) < 1.e-12) continue;  // AddKel's drop rule
+        g.a[bi == bj ? s_row[i][ci] + (cj - ci) : s_row[i][ci] + (NS - ci) + NS * (int64_t)s_map[i * NSUB + j] + cj] += v;
+      }
   }
-  if (tid < NSUB && s_blk[tid] >= 0) g.rhs[s_blk[tid]] += cfl * accf;
+  if (tid < NSUB && s_blk[tid] >= 0)
+    for (int c = 0; c < NS; c++) g.rhs[s_blk[tid] * NS + c] += cfs[BCC_CF + c] * accf;
 }
 
 static ElemArgs make_args(Ctx &c, bool vol) {
@@ -1225,14 +1415,30 @@ static void launch_vol_variant(Ctx &c, const ElemArgs &g, int var, int nsv, cons
   }
 }
 
+// elasticity (nine sums per shape pair): 0: (16,2)  1: (8,2)  2: (32,2)  3: (16,1)  4: (8,1)
+template <int NV>
+static void launch_vol_variant3(Ctx &c, const ElemArgs &g, int var, int nsv, const HpTables *tab,
+                                const int32_t *sub, const int32_t *list, int64_t n) {
+  switch (var) {
+    case 1: hipLaunchKernelGGL((k_vol_hp3<NV, 8, 2>), dim3(n), dim3(TPB), 0, c.stream, g, nsv, tab, sub, list); break;
+    case 2: hipLaunchKernelGGL((k_vol_hp3<NV, 32, 2>), dim3(n), dim3(TPB), 0, c.stream, g, nsv, tab, sub, list); break;
+    case 3: hipLaunchKernelGGL((k_vol_hp3<NV, 16, 1>), dim3(n), dim3(TPB), 0, c.stream, g, nsv, tab, sub, list); break;
+    case 4: hipLaunchKernelGGL((k_vol_hp3<NV, 8, 1>), dim3(n), dim3(TPB), 0, c.stream, g, nsv, tab, sub, list); break;
+    default: hipLaunchKernelGGL((k_vol_hp3<NV, 16, 2>), dim3(n), dim3(TPB), 0, c.stream, g, nsv, tab, sub, list); break;
+  }
+}
+
 template <bool VOL>
 static int launch_hp_bucket(Ctx &c, const ElemArgs &g, int P, int var, const HpTables *tab,
                             const int32_t *sub, const int32_t *list, int64_t n) {
   const int nsv = VOL ? c.vol_nsides : c.bc_nsides;
-#define PZG_HP_CASE(PP, NV, NB)                                                            \
-  case PP:                                                                                 \
-    if (VOL) launch_vol_variant<NV>(c, g, var, nsv, tab, sub, list, n);                    \
-    else hipLaunchKernelGGL((k_bc_hp<NB>), dim3(n), dim3(TPB), 0, c.stream, g, nsv, tab, sub, list); \
+  const bool el = c.bs == 3;
+#define PZG_HP_CASE(PP, NV, NB)                                                                        \
+  case PP:                                                                                             \
+    if (VOL && el) launch_vol_variant3<NV>(c, g, var, nsv, tab, sub, list, n);                         \
+    else if (VOL) launch_vol_variant<NV>(c, g, var, nsv, tab, sub, list, n);                           \
+    else if (el) hipLaunchKernelGGL((k_bc_hp<NB, 3>), dim3(n), dim3(TPB), 0, c.stream, g, nsv, tab, sub, list); \
+    else hipLaunchKernelGGL((k_bc_hp<NB, 1>), dim3(n), dim3(TPB), 0, c.stream, g, nsv, tab, sub, list); \
     break;
   switch (P) {
     PZG_HP_CASE(1, 4, 3)
@@ -1295,8 +1501,8 @@ static int hp_tune(Ctx &c, const ElemArgs &g) {
 }
 
 static int run_hp(Ctx &c) {
-  if (c.bs != 1) {
-    set_error("pzg_assemble: high-order tetrahedra support scalar materials only");
+  if (c.bs != 1 && !(c.bs == 3 && c.material == PZG_MAT_ELAST3D)) {
+    set_error("pzg_assemble: high-order tetrahedra: block size other than the material's states");
     return PZG_EUNSUP;
   }
   if (c.thp_vol.size() != c.vol_rules.size() || c.hp_vol_P.size() != c.vol_rules.size() ||

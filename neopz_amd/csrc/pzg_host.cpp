@@ -327,7 +327,7 @@ static int bc_setup(Ctx &c, double big) {
 }
 
 static int mesh_set(Ctx &c, const pzg_mesh_desc *m) {
-  const int pmax = (m && m->vol_etype == PZG_TET && m->nstate == 1) ? HP_MAXP : 2;
+  const int pmax = (m && m->vol_etype == PZG_TET) ? HP_MAXP : 2;
   if (!m || (m->dim != 2 && m->dim != 3) || m->porder < 1 || m->porder > pmax ||
       (m->nstate != 1 && m->nstate != 3) || m->material < 0 || m->material > 2 ||
       (m->material == PZG_MAT_ELAST3D && (m->nstate != 3 || m->dim != 3)) ||

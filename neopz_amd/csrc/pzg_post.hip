@@ -349,11 +349,17 @@ static int post_launch(Ctx &c, PostArgs a) {
     a.nq = t.nq;
     a.ncol = t.ncol;
     a.nslot = hp ? t.nsh : (t.ncol < c.vol_nsides ? t.ncol : c.vol_nsides);
+    // G lanes per element (a power of two up to a wavefront), enough for the
+    // element's points, and enough for the workgroup's elements to fit in LDS
+    const int ns = POINTS ? 0 : c.nstate;
+    auto lds_of = [&](int g) {
+      return (size_t)(PTPB / g) * ((size_t)a.nslot * (ns ? ns : 1) * 8 + 24 * 8 + a.nslot * 4);
+    };
     int G = 1;
-    while (G < t.nq && G < 64) G <<= 1;
+    while (G < 64 && (G < t.nq || lds_of(G) > 65536)) G <<= 1;
     a.G = G;
-    const int ngrp = PTPB / G, ns = POINTS ? 0 : c.nstate;
-    const size_t lds = (size_t)ngrp * ((size_t)a.nslot * (ns ? ns : 1) * 8 + 24 * 8 + a.nslot * 4);
+    const int ngrp = PTPB / G;
+    const size_t lds = lds_of(G);
     if (lds > 65536) {
       set_error("post-processing: element too large for the error kernel");
       return PZG_EUNSUP;

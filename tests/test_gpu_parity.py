@@ -66,6 +66,14 @@ CASES = {
                                       bc_type=[0, 1, -1, -1, 0, -1],
                                       bc_val=[[0, 0, 0], [0.5, 0, 0], [0, 0, 0], [0, 0, 0],
                                               [0, 0, 0], [0, 0, 0]]),
+    # TPZElasticity3D on high-order tetrahedra: uniform p = 3 with a body load
+    # and a traction, hp p_e in 1..4 with heterogeneous E and prestress
+    "elast_tet_p3": lambda: P.Problem(material=P.ELAST3D, dim=3, p=3, nx=2, ny=1, nz=2, simplex=1,
+                                      E=2.0, force_kind=P.FORCE_CONST, force_const=(0.3, 0.0, -1.0),
+                                      bc_type=[0, 1, -1, 1, 0, -1],
+                                      bc_val=[[0, 0, 0], [0.5, 0, 0.1], [0, 0, 0], [0, -0.2, 0],
+                                              [0, 0, 0], [0, 0, 0]]),
+    "elast_tet_hp4": lambda: _elast_tet_hp(),
     # hp: per-element orders 1/2, shared sides at the minimum order
     "poisson2d_hp": lambda: _hp(P.poisson2d(n=12, p=2, seed=8), 8),
     "darcy2d_hp": lambda: _hp(P.darcy2d(n=14, p=2, seed=9), 9),
@@ -74,6 +82,19 @@ CASES = {
                                           force_kind=P.FORCE_CONST, force_const=(1.0, 0, 0),
                                           bc_type=[0, 0, 1, 1, 0, 0]), 11),
 }
+
+def _elast_tet_hp():
+    rng = np.random.default_rng(16)
+    pb = P.Problem(material=P.ELAST3D, dim=3, p=4, nx=2, ny=2, nz=2, simplex=1,
+                   el_coef=np.exp(0.5 * rng.standard_normal(48)),
+                   el_porder=rng.integers(1, 5, 48).astype(np.int32),
+                   force_kind=P.FORCE_CONST, force_const=(0.0, 1.0, -1.0),
+                   bc_type=[0, -1, -1, -1, 1, -1],
+                   bc_val=[[0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0], [0.2, 0, 0.4], [0, 0, 0]])
+    pb.prestress = (0.1, -0.2, 0.05)
+    pb.name = "elast_tet_hp4"
+    return pb
+
 
 def _hp(pb, seed):
     pb.el_porder = np.random.default_rng(seed).integers(1, 3, pb.nvol).astype(np.int32)
@@ -186,7 +207,7 @@ def test_assembly_is_deterministic():
 
 
 @pytest.mark.parametrize("name", ["poisson2d_p2", "darcy2d_p2", "elast3d_p2_traction", "elast3d_hp",
-                                  "darcy_tet_hp", "darcy_tet_hp5"])
+                                  "darcy_tet_hp", "darcy_tet_hp5", "elast_tet_hp4"])
 def test_pcg64_matches_oracle(name):
     pb = CASES[name]()
     an = TPZLinearAnalysis(pb, False, ctx=ctx())
@@ -324,7 +345,7 @@ def test_matfree_needs_mesh_built_system():
     assert st.info["converged"]
 
 
-@pytest.mark.parametrize("name", ["poisson2d_hp", "elast3d_hp", "darcy_tet_hp", "darcy_tet_hp5"])
+@pytest.mark.parametrize("name", ["poisson2d_hp", "elast3d_hp", "darcy_tet_hp", "darcy_tet_hp5", "elast_tet_hp4"])
 def test_mixed_refinement_hp(name):
     """hp meshes (several integration rules): the mixed solver uses the stored
     FP32 operator; the matrix-free one is refused."""

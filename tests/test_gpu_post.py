@@ -41,6 +41,10 @@ CASES = {
     "elast_tet_p2": lambda: P.Problem(material=P.ELAST3D, dim=3, p=2, nx=2, ny=2, nz=2, simplex=1,
                                       E=2.0, nu=0.27, force_kind=P.FORCE_CONST,
                                       force_const=(0.0, 0.0, -1.0), bc_type=[0] * 6),
+    "elast_tet_hp4": lambda: P.Problem(material=P.ELAST3D, dim=3, p=4, nx=2, ny=1, nz=2, simplex=1,
+                                       E=2.0, nu=0.27, force_kind=P.FORCE_CONST,
+                                       force_const=(0.0, 0.0, -1.0), bc_type=[0] * 6,
+                                       el_porder=np.random.default_rng(9).integers(1, 5, 24).astype(np.int32)),
 }
 
 _CTX = None
