@@ -1090,9 +1090,11 @@ __global__ __launch_bounds__(TPB) void k_vol_hp3(ElemArgs g, int nsv, const HpTa
       g.rhs[s_blk[tid] * 3 + c] += accf[c] - g.prestress[c] * accp[c];
 }
 
-// NS states: ek block (i, j) = ck (NS x NS) M_ij, ef_i = cf m_i with the
-// boundary mass M and moment m (TPZMatPoisson type 2 and the TPZElasticity3D
-// stress field are refused on these meshes by pzg_mesh_set).
+// NS states: ek block (i, j) = ck (NS x NS) M_ij, ef_i = (cf + g) m_i with the
+// boundary mass M and moment m; g = -Val1 . n for the TPZElasticity3D stress
+// field (constant on the flat triangle), n the unit normal turned away from
+// the volume neighbour as in k_bc (TPZMatPoisson type 2 is refused on these
+// meshes by pzg_mesh_set).
 template <int NSUB, int NS>
 __global__ __launch_bounds__(TPB) void k_bc_hp(ElemArgs g, int nsv, const HpTables *__restrict__ tab,
                                                const int32_t *__restrict__ sub,
@@ -1100,7 +1102,7 @@ __global__ __launch_bounds__(TPB) void k_bc_hp(ElemArgs g, int nsv, const HpTabl
   constexpr int QC = 32;
   constexpr int NPAIR = NSUB * NSUB;
   constexpr int PER = (NPAIR + TPB - 1) / TPB;
-  __shared__ double s_det;
+  __shared__ double s_det, s_g[3];
   __shared__ double s_p[QC][NSUB];
   __shared__ double s_pw[QC][NSUB];
   __shared__ int s_col[NSUB];
@@ -1129,14 +1131,34 @@ __global__ __launch_bounds__(TPB) void k_bc_hp(ElemArgs g, int nsv, const HpTabl
     double gx[3][2];
     for (int i = 0; i < 3; i++)
       for (int k = 0; k < 2; k++) gx[i][k] = g.xyz[3 * g.nodes[e * 3 + k + 1] + i] - g.xyz[3 * g.nodes[e * 3] + i];
-    double n1 = 0, d12 = 0, n2 = 0;
+    double n1 = 0, d12 = 0, n2 = 0, a1[3];
     for (int i = 0; i < 3; i++) { n1 += gx[i][0] * gx[i][0]; d12 += gx[i][0] * gx[i][1]; }
     n1 = sqrt(n1);
     for (int i = 0; i < 3; i++) {
-      const double v2 = gx[i][1] - d12 * (gx[i][0] / n1) / n1;
-      n2 += v2 * v2;
+      a1[i] = gx[i][1] - d12 * (gx[i][0] / n1) / n1;
+      n2 += a1[i] * a1[i];
     }
-    s_det = fabs(n1 * sqrt(n2));
+    n2 = sqrt(n2);
+    s_det = fabs(n1 * n2);
+    s_g[0] = s_g[1] = s_g[2] = 0.;
+    if (NS == 3 && cfs[BCC_STRESS] != 0.) {
+      // TPZInterpolationSpace::ComputeNormal: axes a0 x a1, turned away from
+      // the volume neighbour (g.bcvec: neighbour centre -> element centre)
+      double a0[3], t[3];
+      for (int i = 0; i < 3; i++) { a0[i] = gx[i][0] / n1; a1[i] /= n2; }
+      t[0] = a0[1] * a1[2] - a0[2] * a1[1];
+      t[1] = -a0[0] * a1[2] + a0[2] * a1[0];
+      t[2] = a0[0] * a1[1] - a0[1] * a1[0];
+      const double *v = g.bcvec + 3 * e;
+      const double sz = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+      double nrm[3], dot = 0.;
+      for (int i = 0; i < 3; i++) { nrm[i] = t[i] / sz; dot += nrm[i] * v[i]; }
+      if (dot < 0.)
+        for (int i = 0; i < 3; i++) nrm[i] = -nrm[i];
+      for (int k = 0; k < 3; k++)
+        s_g[k] = -(cfs[BCC_VAL1 + 3 * k] * nrm[0] + cfs[BCC_VAL1 + 3 * k + 1] * nrm[1] +
+                   cfs[BCC_VAL1 + 3 * k + 2] * nrm[2]);
+    }
   }
   __syncthreads();
   double acc[PER];
@@ -1182,7 +1204,7 @@ __global__ __launch_bounds__(TPB) void k_bc_hp(ElemArgs g, int nsv, const HpTabl
       }
   }
   if (tid < NSUB && s_blk[tid] >= 0)
-    for (int c = 0; c < NS; c++) g.rhs[s_blk[tid] * NS + c] += cfs[BCC_CF + c] * accf;
+    for (int c = 0; c < NS; c++) g.rhs[s_blk[tid] * NS + c] += (cfs[BCC_CF + c] + s_g[c]) * accf;
 }
 
 static ElemArgs make_args(Ctx &c, bool vol) {

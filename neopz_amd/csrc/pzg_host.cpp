@@ -275,16 +275,18 @@ static int bc_setup(Ctx &c, double big) {
   }
   c.bc_vec.clear();
   if (!any_stress) return PZG_OK;
-  if (c.has_dep || c.hp_simplex) {
-    set_error("pzg_mesh_set: stress-field boundary conditions on constrained or high-order "
-              "simplex meshes are not implemented");
+  if (c.has_dep) {
+    set_error("pzg_mesh_set: stress-field boundary conditions on constrained meshes are not "
+              "implemented");
     return PZG_EUNSUP;
   }
   // volume neighbour of every stress-field boundary element: the volume
   // element with a side whose corners are the boundary element's corners
+  // (the element type's own sides: on high-order meshes vol_nsides counts
+  // the virtual sides)
   const int vt = c.vol_etype, ncb = c.bc_ncorner, ncv = c.vol_ncorner;
   std::vector<std::vector<int>> faces;
-  for (int sd = 0; sd < c.vol_nsides; sd++) {
+  for (int sd = 0; sd < etype_ns(vt); sd++) {
     int loc[8];
     const int n = side_corner_list_pub(vt, sd, loc);
     if (n == ncb) faces.emplace_back(loc, loc + n);

@@ -67,10 +67,11 @@ CASES = {
                                      [Z9, A[0], S[0], Z9, Z9, Z9], n=4, p=1),
     "elast_types_tet": lambda: _elast([0, 2, 4, 5, -1, 8], [Z3, [0, 1, 0], Z3, [1.0, 0, 0], Z3, [0.5, 0, 0.5]],
                                       [Z9, A[0], S[0], Z9, Z9, Z9], n=2, p=2, simplex=1),
-    # the boundary mass types on order-3 tetrahedra (hp boundary kernel, 3 states)
-    "elast_types_tet_p3": lambda: _elast([0, 2, -1, 5, 3, 8],
+    # order-3 tetrahedra (hp boundary kernel, 3 states): the boundary mass
+    # types and the stress field
+    "elast_types_tet_p3": lambda: _elast([0, 2, 4, 5, 3, 8],
                                          [Z3, [0, 1, 0], Z3, [1.0, 0, 0], [1.0, 0, 1.0], [0.5, 0, 0.5]],
-                                         [Z9, A[0], Z9, Z9, Z9, Z9], n=2, p=3, simplex=1),
+                                         [Z9, A[0], S[0], Z9, Z9, Z9], n=2, p=3, simplex=1),
     "elast_prestress_hex": lambda: _prestress(_elast([0, 1, -1, 2, -1, 4], [Z3, [0.1, 0.2, 0.3], Z3, Z3, Z3, Z3],
                                                      [Z9, Z9, Z9, A[0], Z9, S[0]], n=3, p=2)),
     "elast_prestress_tet": lambda: _prestress(_elast([0, 1, -1, -1, -1, -1], n=2, p=2, simplex=1)),
