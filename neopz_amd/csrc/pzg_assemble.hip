@@ -1702,24 +1702,51 @@ __global__ __launch_bounds__(256) void k_bsr_gather(int64_t nb, const int64_t *_
   const int64_t b0 = fbia[A], u0 = ubia[A];
   const int n = (int)(fbia[A + 1] - b0), nl = n - 1 - (int)(ubia[A + 1] - u0);
   const int64_t lb = b0 - A - u0;  // first lower block of row A in lowsrc
-  for (int p = lane; p < n * BB; p += 64) {
-    const int j0 = p / (BB * BSR_CHUNK) * BSR_CHUNK, m = min(BSR_CHUNK, n - j0), w = p - j0 * BB;
-    const int kl = w / m, j = j0 + w % m, k = kl / BS, l = kl % BS;
-    double x;
+  int64_t ra[BS];                  // row starts of A's equations in the upper CSR
+  double da[BS];
+#pragma unroll
+  for (int k = 0; k < BS; k++) {
+    ra[k] = ia[A * BS + k];
+    da[k] = SCALED ? dis[A * BS + k] : 1.;
+  }
+  // one lane per block: its indices once, its BB values loaded together; the
+  // value kl of block j sits at chunk offset kl*m + (j - j0), so the lanes of
+  // a chunk write each kl contiguously
+  for (int j = lane; j < n; j += 64) {
+    if (SCALED && j == nl) continue;
+    const int j0 = j / BSR_CHUNK * BSR_CHUNK, m = min(BSR_CHUNK, n - j0);
+    T *out = v + (b0 + j0) * BB + (j - j0);
+    double x[BB];
     if (j == nl) {
-      if (SCALED) continue;
-      x = k <= l ? a[ia[A * BS + k] + (l - k)] : a[ia[A * BS + l] + (k - l)];
+#pragma unroll
+      for (int k = 0; k < BS; k++)
+#pragma unroll
+        for (int l = 0; l < BS; l++) x[k * BS + l] = k <= l ? a[ra[k] + (l - k)] : a[ra[l] + (k - l)];
     } else if (j > nl) {
-      const int t = j - nl - 1;
-      x = a[ia[A * BS + k] + BS + (int64_t)t * BS - k + l];
-      if (SCALED) x = x * dis[A * BS + k] * dis[(int64_t)fbja[b0 + j] * BS + l];
+      const int64_t t = j - nl - 1, B = fbja[b0 + j];
+#pragma unroll
+      for (int k = 0; k < BS; k++)
+#pragma unroll
+        for (int l = 0; l < BS; l++) {
+          x[k * BS + l] = a[ra[k] + BS + t * BS - k + l];
+          if (SCALED) x[k * BS + l] *= da[k] * dis[B * BS + l];
+        }
     } else {
-      const int64_t u = lowsrc[lb + j], B = fbja[b0 + j];
-      const int64_t t = u - ubia[B];
-      x = a[ia[B * BS + l] + BS + t * BS - l + k];
-      if (SCALED) x = x * dis[B * BS + l] * dis[A * BS + k];
+      // the transpose of the upper block (B, A) of row B
+      const int64_t u = lowsrc[lb + j], B = fbja[b0 + j], t = u - ubia[B];
+#pragma unroll
+      for (int l = 0; l < BS; l++) {
+        const int64_t r = ia[B * BS + l] + BS + t * BS - l;
+        const double dl = SCALED ? dis[B * BS + l] : 1.;
+#pragma unroll
+        for (int k = 0; k < BS; k++) {
+          x[k * BS + l] = a[r + k];
+          if (SCALED) x[k * BS + l] *= dl * da[k];
+        }
+      }
     }
-    v[b0 * BB + p] = (T)x;
+#pragma unroll
+    for (int kl = 0; kl < BB; kl++) out[kl * m] = (T)x[kl];
   }
 }
 
