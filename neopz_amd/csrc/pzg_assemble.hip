@@ -397,7 +397,7 @@ __global__ __launch_bounds__(TPB) void k_vol(ElemArgs g, const int32_t *__restri
 // part builds the pair's nstate x nstate block and adds it with AddKel's drop
 // rule; the load vector keeps the quadrature (the body load varies).
 constexpr int AFF_EPB = 4;  // elements per workgroup
-template <int DIM, int NC, int NSD, int NSH, int NS>
+template <int DIM, int NC, int NSD, int NSH, int NS, bool MAT = true>
 __global__ __launch_bounds__(TPB) void k_vol_aff(ElemArgs g, const double *__restrict__ R,
                                                  const int32_t *__restrict__ list, int64_t n) {
   constexpr int NR = DIM * (DIM + 1) / 2;  // R_mn, m <= n
@@ -416,16 +416,18 @@ __global__ __launch_bounds__(TPB) void k_vol_aff(ElemArgs g, const double *__res
   const int nq = T.nq;
   const int64_t e0 = (int64_t)blockIdx.x * AFF_EPB;
   const int nel = (int)(n - e0 < AFF_EPB ? n - e0 : AFF_EPB);
-  for (int t = tid; t < NR * NSH * NSH; t += TPB) (&s_R[0][0][0])[t] = R[t];
+  if (MAT)
+    for (int t = tid; t < NR * NSH * NSH; t += TPB) (&s_R[0][0][0])[t] = R[t];
   for (int t = tid; t < nel * NC * 3; t += TPB) {
     const int el = t / (NC * 3), k = t % (NC * 3);
     const int64_t e = list[e0 + el];
     s_coord[el][k] = g.xyz[3 * g.nodes[e * NC + k / 3] + k % 3];
   }
-  for (int t = tid; t < nel * NSD * NSD; t += TPB) {
-    const int el = t / (NSD * NSD), k = t % (NSD * NSD);
-    s_map[el][k] = g.map[(int64_t)list[e0 + el] * NSD * NSD + k];
-  }
+  if (MAT)
+    for (int t = tid; t < nel * NSD * NSD; t += TPB) {
+      const int el = t / (NSD * NSD), k = t % (NSD * NSD);
+      s_map[el][k] = g.map[(int64_t)list[e0 + el] * NSD * NSD + k];
+    }
   for (int t = tid; t < nel * NSH; t += TPB) {
     const int el = t / NSH, k = t % NSH;
     s_blk[el][k] = g.aidx[g.conn[(int64_t)list[e0 + el] * NSD + T.shape_side[k]]];
@@ -494,7 +496,7 @@ __global__ __launch_bounds__(TPB) void k_vol_aff(ElemArgs g, const double *__res
     for (int i = 0; i < 3; i++) s_f[el][q][i] = f[i];
   }
   __syncthreads();
-  for (int t = tid; t < nel * NSH * NSH; t += TPB) {
+  for (int t = tid; MAT && t < nel * NSH * NSH; t += TPB) {
     const int el = t / (NSH * NSH), ij = t % (NSH * NSH), i = ij / NSH, j = ij % NSH;
     const int64_t bi = s_blk[el][i], bj = s_blk[el][j];
     if (bi < 0 || bj < 0 || bi > bj || (bi == bj && i > j)) continue;
@@ -585,6 +587,83 @@ __global__ __launch_bounds__(TPB) void k_vol_aff(ElemArgs g, const double *__res
       sf -= g.prestress[ci] * sg;
     }
     g.rhs[s_blk[el][i] * g.bs + ci] += sf;
+  }
+}
+
+// Volume elements that are all one affine hexahedron (one Jacobian, one
+// Poisson ratio): K_e = coef_e K_ref with K_ref the element matrix of
+// coefficient 1 (FP64, built on the host from the same reference Gram
+// matrices as k_vol_aff).  Instead of scattering every element's matrix, each
+// stored block (A, B), B >= A, is formed once from the elements that share A
+// and B (the intersection of the two connects' element lists, ascending
+// element order, as AddKel adds them) and written: no read-modify-write and
+// no colours.  Each element term keeps AddKel's |v| < 1e-12 drop rule.
+// One wavefront per block row (grid-stride), one lane per block; the
+// incidence entries pack (element << 5 | local shape).
+constexpr int BLK_TPB = 1024;
+template <int NS, int NE>
+__global__ __launch_bounds__(BLK_TPB) void k_vol_blk(int64_t nb, const int64_t *__restrict__ ia,
+                                                     const int64_t *__restrict__ ubia,
+                                                     const int32_t *__restrict__ ubja,
+                                                     const int64_t *__restrict__ iptr,
+                                                     const int32_t *__restrict__ inc,
+                                                     const double *__restrict__ coefv, double coef0,
+                                                     const double *__restrict__ kref, double *__restrict__ a) {
+  constexpr int NW = BLK_TPB / 64, MAXI = 8;
+  __shared__ double sK[NE * NE];
+  __shared__ int32_t sI[NW][MAXI];
+  for (int t = threadIdx.x; t < NE * NE; t += BLK_TPB) sK[t] = kref[t];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int64_t A = (int64_t)blockIdx.x * NW + w; A < nb; A += (int64_t)gridDim.x * NW) {
+    const int64_t a0 = iptr[A];
+    const int na = (int)(iptr[A + 1] - a0);  // <= MAXI (checked on the host)
+    if (lane < na) sI[w][lane] = inc[a0 + lane];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    int64_t ra[NS];
+#pragma unroll
+    for (int k = 0; k < NS; k++) ra[k] = ia[A * NS + k];
+    const int64_t u0 = ubia[A], nup = ubia[A + 1] - u0;
+    for (int64_t j = lane; j <= nup; j += 64) {
+      const int64_t B = j == 0 ? A : (int64_t)ubja[u0 + j - 1];
+      double acc[NS][NS];
+#pragma unroll
+      for (int k = 0; k < NS; k++)
+#pragma unroll
+        for (int l = 0; l < NS; l++) acc[k][l] = 0.;
+      const int64_t b0 = iptr[B], b1 = iptr[B + 1];
+      int pa = 0;
+      for (int64_t q = b0; q < b1; q++) {
+        const int32_t vb = inc[q];
+        const int eb = vb >> 5;
+        while (pa < na && (sI[w][pa] >> 5) < eb) pa++;
+        if (pa == na) break;
+        const int32_t va = sI[w][pa];
+        if ((va >> 5) != eb) continue;
+        const double cf = coefv ? coefv[eb] : coef0;
+        const double *kr = sK + (va & 31) * NS * NE + (vb & 31) * NS;
+#pragma unroll
+        for (int k = 0; k < NS; k++)
+#pragma unroll
+          for (int l = 0; l < NS; l++) {
+            const double v = cf * kr[k * NE + l];
+            if (!(fabs(v) < 1.e-12)) acc[k][l] += v;  // IsZero drop rule of AddKel
+          }
+      }
+      if (j == 0) {
+#pragma unroll
+        for (int k = 0; k < NS; k++)
+#pragma unroll
+          for (int l = k; l < NS; l++) a[ra[k] + (l - k)] = acc[k][l];
+      } else {
+#pragma unroll
+        for (int k = 0; k < NS; k++)
+#pragma unroll
+          for (int l = 0; l < NS; l++) a[ra[k] + (NS - k) + (j - 1) * NS + l] = acc[k][l];
+      }
+    }
+    __builtin_amdgcn_wave_barrier();  // sI[w] is rewritten by the next row
   }
 }
 
@@ -1246,6 +1325,24 @@ static int run_vol(Ctx &c) {
   // affine elements first in each colour (c.vcolor_aff of them), when the
   // reference Gram matrices of the single rule are on the device
   const bool aff = !c.vcolor_aff.empty() && c.d_vol_R.p && c.aff_on;
+  // identical affine elements: the stiffness by blocks (k_vol_blk, which
+  // writes every stored entry), the load vector by the affine kernel
+  const bool blk = aff && c.blk_asm && NSH * NS <= 81;
+  if (blk) {
+    if (c.blk_grid <= 0) {
+      int per_cu = 0, ncu = 0;
+      PZG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device));
+      PZG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_vol_blk<NS, NSH * NS>, BLK_TPB, 0));
+      c.blk_grid = std::max(1, per_cu) * std::max(1, ncu);
+    }
+    const int64_t rows = (c.nb + BLK_TPB / 64 - 1) / (BLK_TPB / 64);
+    hipLaunchKernelGGL((k_vol_blk<NS, NSH * NS>), dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(rows, c.blk_grid))),
+                       dim3(BLK_TPB), 0, c.stream, c.nb, c.d_ia.as<int64_t>(), c.d_ubia.as<int64_t>(),
+                       c.d_ubja.as<int32_t>(), c.d_blk_iptr.as<int64_t>(), c.d_blk_inc.as<int32_t>(),
+                       g.coef, c.material == PZG_MAT_ELAST3D ? c.E : c.scale, c.d_blk_kref.as<double>(),
+                       c.d_a.as<double>());
+    PZG_HIP(hipGetLastError());
+  }
   for (size_t k = 0; k + 1 < c.vcolor_ptr.size(); k++) {
     int64_t n = c.vcolor_ptr[k + 1] - c.vcolor_ptr[k];
     if (!n) continue;
@@ -1253,8 +1350,13 @@ static int run_vol(Ctx &c) {
     const int32_t *lst = c.d_vcolor_list.as<int32_t>() + c.vcolor_ptr[k];
     int64_t na = aff ? c.vcolor_aff[k] : 0;
     if (na > 0) {
-      hipLaunchKernelGGL((k_vol_aff<DIM, NC, NSD, NSH, NS>), dim3((unsigned)((na + AFF_EPB - 1) / AFF_EPB)),
-                         dim3(TPB), 0, c.stream, g, c.d_vol_R.as<double>(), lst, na);
+      if (blk)
+        hipLaunchKernelGGL((k_vol_aff<DIM, NC, NSD, NSH, NS, false>),
+                           dim3((unsigned)((na + AFF_EPB - 1) / AFF_EPB)), dim3(TPB), 0, c.stream, g,
+                           c.d_vol_R.as<double>(), lst, na);
+      else
+        hipLaunchKernelGGL((k_vol_aff<DIM, NC, NSD, NSH, NS>), dim3((unsigned)((na + AFF_EPB - 1) / AFF_EPB)),
+                           dim3(TPB), 0, c.stream, g, c.d_vol_R.as<double>(), lst, na);
       PZG_HIP(hipGetLastError());
     }
     if (n > na) {
@@ -1562,7 +1664,8 @@ static int run_hp(Ctx &c) {
 }
 
 int launch_assemble(Ctx &c) {
-  fill<double>(c, c.d_a.as<double>(), c.nnz_u, 0.);
+  // (the block assembly of identical elements writes every stored entry)
+  if (!(c.blk_asm && c.aff_on)) fill<double>(c, c.d_a.as<double>(), c.nnz_u, 0.);
   fill<double>(c, c.d_rhs.as<double>(), c.neq, 0.);
   if (c.hp_simplex) return run_hp(c);
   for (const ElemTables &t : c.tvol_all)

@@ -60,7 +60,7 @@ static void free_all(Ctx &c) {
                     &c.d_fa, &c.d_bsr, &c.d_dinvsqrt, &c.d_bjinv, &c.d_xyz, &c.d_vnodes,
                     &c.d_vconn, &c.d_vcoef, &c.d_bnodes, &c.d_bconn, &c.d_bcid, &c.d_bctype,
                     &c.d_bcval, &c.d_bccoef, &c.d_bcvec, &c.d_aidx, &c.d_bumps, &c.d_vmap, &c.d_bmap, &c.d_tvol,
-                    &c.d_tbc, &c.d_vcolor_list, &c.d_vol_R, &c.d_bcolor_list, &c.d_x, &c.d_r, &c.d_z,
+                    &c.d_tbc, &c.d_vcolor_list, &c.d_vol_R, &c.d_blk_iptr, &c.d_blk_inc, &c.d_blk_kref, &c.d_bcolor_list, &c.d_x, &c.d_r, &c.d_z,
                     &c.d_p, &c.d_p2, &c.d_q, &c.d_u, &c.d_f, &c.d_w, &c.d_xs, &c.d_rs,
                     &c.d_zs, &c.d_ps, &c.d_ps2, &c.d_qs, &c.d_bs, &c.d_scal, &c.d_partials,
                     &c.d_counter, &c.d_res2, &c.d_les_th, &c.d_les_m, &c.d_les_v, &c.d_les_g,
@@ -199,6 +199,126 @@ static int expand_hp(Ctx &c) {
   c.vol_side_real.assign(vside, vside + nsv);
   c.hp_simplex = true;
   c.hp_order = P;
+  return PZG_OK;
+}
+
+// Block assembly of identical affine hexahedra (k_vol_blk): when every
+// volume element has the Jacobian of element 0 (within 1e-12), its element
+// matrix is coef_e K_ref, K_ref formed here in FP64 exactly as k_vol_aff forms
+// an element's (P^ab = |det J| Jinv^T R Jinv, then the material's entries
+// with coefficient 1).  Also the element list of every block row (ascending
+// element order, packed element << 5 | local shape).  PZG_NO_BLKASM=1 keeps
+// the element scatter.
+static int blk_setup(Ctx &c, const std::vector<double> &R) {
+  const char *off = std::getenv("PZG_NO_BLKASM");
+  if (off && off[0] == '1') return PZG_OK;
+  const ElemTables &T = c.tvol;
+  const int NC = c.vol_ncorner, NSD = c.vol_nsides, NSH = T.nsh, ns = c.nstate, NE = NSH * ns;
+  if (c.dim != 3 || NC != 8 || NE > 81 || c.nvol <= 0 || c.nvol >= ((int64_t)1 << 26) || c.has_dep ||
+      c.hp_simplex)
+    return PZG_OK;
+  // Jacobian of each element at the first point (the affine elements' constant one)
+  auto jac = [&](int64_t e, double J[3][3]) {
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        double v = 0.;
+        for (int k = 0; k < NC; k++) v += c.xyz[3 * c.vol_nodes[e * NC + k] + i] * T.cdphi[0][j][k];
+        J[i][j] = v;
+      }
+  };
+  int64_t naff = 0;  // every element affine
+  for (int64_t v : c.vcolor_aff) naff += v;
+  if (naff != c.nvol) return PZG_OK;
+  double J0[3][3], sc = 0.;
+  jac(0, J0);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) sc = std::max(sc, std::fabs(J0[i][j]));
+  for (int64_t e = 1; e < c.nvol; e++) {
+    double J[3][3];
+    jac(e, J);
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++)
+        if (std::fabs(J[i][j] - J0[i][j]) > 1e-12 * sc) return PZG_OK;
+  }
+  const double(*j)[3] = J0;
+  const double det = -j[0][2] * j[1][1] * j[2][0] + j[0][1] * j[1][2] * j[2][0] + j[0][2] * j[1][0] * j[2][1] -
+                     j[0][0] * j[1][2] * j[2][1] - j[0][1] * j[1][0] * j[2][2] + j[0][0] * j[1][1] * j[2][2];
+  if (det == 0.) return PZG_OK;
+  double inv[3][3];
+  inv[0][0] = (-j[1][2] * j[2][1] + j[1][1] * j[2][2]) / det;
+  inv[0][1] = (j[0][2] * j[2][1] - j[0][1] * j[2][2]) / det;
+  inv[0][2] = (-j[0][2] * j[1][1] + j[0][1] * j[1][2]) / det;
+  inv[1][0] = (j[1][2] * j[2][0] - j[1][0] * j[2][2]) / det;
+  inv[1][1] = (-j[0][2] * j[2][0] + j[0][0] * j[2][2]) / det;
+  inv[1][2] = (j[0][2] * j[1][0] - j[0][0] * j[1][2]) / det;
+  inv[2][0] = (-j[1][1] * j[2][0] + j[1][0] * j[2][1]) / det;
+  inv[2][1] = (j[0][1] * j[2][0] - j[0][0] * j[2][1]) / det;
+  inv[2][2] = (-j[0][1] * j[1][0] + j[0][0] * j[1][1]) / det;
+  const double adet = std::fabs(det);
+  auto Rv = [&](int m, int n, int i, int jj) {  // R_mn(i, j), stored for m <= n
+    const int lo = std::min(m, n), hi = std::max(m, n), r = lo * 3 - lo * (lo - 1) / 2 + (hi - lo);
+    return m <= n ? R[((size_t)r * NSH + i) * NSH + jj] : R[((size_t)r * NSH + jj) * NSH + i];
+  };
+  std::vector<double> K((size_t)NE * NE, 0.);
+  const double nu = c.nu, den = -1. + nu + 2. * nu * nu;
+  const double C1 = 1. / (2. + 2. * nu), C2 = nu / den, C3 = (nu - 1.) / den;
+  for (int i = 0; i < NSH; i++)
+    for (int jj = 0; jj < NSH; jj++) {
+      double Q[3][3], P[3][3];
+      for (int m = 0; m < 3; m++)
+        for (int b = 0; b < 3; b++) {
+          double v = 0.;
+          for (int n = 0; n < 3; n++) v += Rv(m, n, i, jj) * inv[n][b];
+          Q[m][b] = v;
+        }
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+          double v = 0.;
+          for (int m = 0; m < 3; m++) v += inv[m][a] * Q[m][b];
+          P[a][b] = v * adet;
+        }
+      if (ns == 3) {
+        for (int ci = 0; ci < 3; ci++)
+          for (int cj = 0; cj < 3; cj++) {
+            double v;
+            if (ci == cj) {
+              double tr = 0.;
+              for (int cc = 0; cc < 3; cc++)
+                if (cc != ci) tr += P[cc][cc];
+              v = tr * C1 + P[ci][ci] * C3;
+            } else {
+              v = P[cj][ci] * C1 - P[ci][cj] * C2;
+            }
+            K[(size_t)(3 * i + ci) * NE + 3 * jj + cj] = v;
+          }
+      } else {
+        K[(size_t)i * NE + jj] = P[0][0] + P[1][1] + P[2][2];
+      }
+    }
+  // element lists of the block rows
+  std::vector<int64_t> iptr(c.nb + 1, 0);
+  for (int64_t e = 0; e < c.nvol; e++)
+    for (int sh = 0; sh < NSH; sh++) {
+      const int64_t b = c.aidx[c.vol_conn[e * NSD + T.shape_side[sh]]];
+      if (b >= 0) iptr[b + 1]++;
+    }
+  for (int64_t b = 0; b < c.nb; b++) {
+    if (iptr[b + 1] > 8) return PZG_OK;  // a connect of more than 8 elements
+    iptr[b + 1] += iptr[b];
+  }
+  std::vector<int32_t> inc(std::max<int64_t>(iptr[c.nb], 1));
+  std::vector<int64_t> fill(iptr.begin(), iptr.end() - 1);
+  for (int64_t e = 0; e < c.nvol; e++)
+    for (int sh = 0; sh < NSH; sh++) {
+      const int64_t b = c.aidx[c.vol_conn[e * NSD + T.shape_side[sh]]];
+      if (b >= 0) inc[fill[b]++] = (int32_t)((e << 5) | sh);
+    }
+  hipStream_t s = c.stream;
+  PZG_TRY(dev_upload(c.d_blk_kref, K.data(), K.size(), s));
+  PZG_TRY(dev_upload(c.d_blk_iptr, iptr.data(), iptr.size(), s));
+  PZG_TRY(dev_upload(c.d_blk_inc, inc.data(), inc.size(), s));
+  c.blk_asm = true;
+  c.blk_grid = 0;
   return PZG_OK;
 }
 
@@ -927,6 +1047,7 @@ static int struct_create(Ctx &c) {
   // colour: the assembly integrates them from the rule's reference Gram
   // matrices (k_vol_aff); single-rule quad/hex meshes only
   c.vcolor_aff.clear();
+  c.blk_asm = false;
   c.aff_on = !(std::getenv("PZG_NO_AFFINE") && std::getenv("PZG_NO_AFFINE")[0] == '1');
   if (c.vol_rules.size() == 1 && !c.hp_simplex && c.vol_etype != PZG_TET &&
       ((c.dim == 3 && c.vol_ncorner == 8) || (c.dim == 2 && c.vol_ncorner == 4))) {
@@ -970,6 +1091,7 @@ static int struct_create(Ctx &c) {
             R[((size_t)r * ns + i) * ns + j] = v;
           }
     PZG_TRY(dev_upload(c.d_vol_R, R.data(), R.size(), s));
+    PZG_TRY(blk_setup(c, R));
   }
   c.vcolor_ptr = vptr;
   PZG_TRY(dev_upload(c.d_vcolor_list, vlist.data(), vlist.size(), s));

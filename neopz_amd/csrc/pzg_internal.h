@@ -236,6 +236,12 @@ struct Ctx {
   std::vector<int64_t> vcolor_aff;
   DevBuf d_vol_R;
   bool aff_on = true;
+  // all volume elements one affine hexahedron: stiffness assembled by blocks
+  // (k_vol_blk) from the coefficient-1 element matrix and the connects'
+  // element lists
+  bool blk_asm = false;
+  int blk_grid = 0;
+  DevBuf d_blk_iptr, d_blk_inc, d_blk_kref;
   int64_t nb = 0;  // active blocks
 
   // constrained connects (hanging sides): dependencies per connect, and the
