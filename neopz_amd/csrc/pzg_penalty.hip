@@ -302,25 +302,43 @@ __global__ void k_pen_mark(int64_t npb, const int32_t *__restrict__ pblk, const 
   for (int64_t q = fbia[B]; q < fbia[B + 1]; q++) flag[fbja[q]] = 1;  // symmetric pattern
 }
 
-// r -= K delta on the listed block rows (one thread per row and component);
-// delta is zero off the penalty rows, so this is the exact change of the
-// residual f - K u from u_p += delta_p
+// r -= K delta on the listed block rows; delta is zero off the penalty rows,
+// so this is the exact change of the residual f - K u from u_p += delta_p.
+// Sixteen lanes per row, lane sub taking the row's blocks i = sub mod 16
+// (one value chunk of the block CSR per pass, as the block SpMV reads it).
 template <int BS>
 __global__ void k_pen_rupd(int64_t nr, const int32_t *__restrict__ rows, const int64_t *__restrict__ fbia,
                            const int32_t *__restrict__ fbja, const double *__restrict__ fa,
                            const double *__restrict__ delta, double *__restrict__ r) {
+  constexpr int BB = BS * BS;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nr * BS) return;
-  const int64_t A = rows[t / BS];
-  const int k = (int)(t % BS);
-  const int64_t b0 = fbia[A], n = fbia[A + 1] - b0;
-  double y = 0.;
-  for (int64_t i = 0; i < n; i++) {
-    const int64_t B = fbja[b0 + i];
+  const int64_t gi = t >> 4;
+  const int sub = (int)(t & 15);
+  if (gi >= nr) return;  // whole 16-lane groups
+  const int64_t A = rows[gi], b0 = fbia[A], n = fbia[A + 1] - b0;
+  double acc[BS];
 #pragma unroll
-    for (int l = 0; l < BS; l++) y += fa[bsr_pos<BS * BS>(b0, n, i, k * BS + l)] * delta[B * BS + l];
+  for (int k = 0; k < BS; k++) acc[k] = 0.;
+  for (int64_t i = sub; i < n; i += BSR_CHUNK) {
+    const int64_t j = i - sub, mc = n - j < BSR_CHUNK ? n - j : BSR_CHUNK;
+    const double *vp = fa + (b0 + j) * BB + sub;
+    const int64_t B = fbja[b0 + i];
+    double dl[BS];
+#pragma unroll
+    for (int l = 0; l < BS; l++) dl[l] = delta[B * BS + l];
+#pragma unroll
+    for (int k = 0; k < BS; k++)
+#pragma unroll
+      for (int l = 0; l < BS; l++) acc[k] += vp[(k * BS + l) * mc] * dl[l];
   }
-  r[A * BS + k] -= y;
+  double mine = 0.;
+#pragma unroll
+  for (int k = 0; k < BS; k++) {
+    double s = acc[k];
+    for (int o = BSR_CHUNK / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, BSR_CHUNK);
+    if (sub == k) mine = s;
+  }
+  if (sub < BS) r[A * BS + sub] -= mine;
 }
 
 // ||r||^2 over n entries: per-block partials, then one block sums them
@@ -637,7 +655,7 @@ int pen_solve(Ctx &c, double *r64, double *u64, double stop_abs, bool use_graph,
     // the exact residual: r -= K delta on the rows delta reaches, then its norm
     hipLaunchKernelGGL(k_pen_scatter_d, dim3(g), dim3(TPB), 0, s, np, idx, (const double *)x, u64,
                        c.d_pen_delta.as<double>());
-    const int64_t th = c.pen_narow * c.bs;
+    const int64_t th = c.pen_narow * BSR_CHUNK;
     if (c.bs == 3)
       hipLaunchKernelGGL(k_pen_rupd<3>, dim3((unsigned)((th + TPB - 1) / TPB)), dim3(TPB), 0, s, c.pen_narow,
                          c.d_pen_arow.as<int32_t>(), c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(),

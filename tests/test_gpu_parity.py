@@ -155,7 +155,7 @@ def test_pattern_and_values(name):
     assert np.max(np.abs(rhs - rhs_r)) <= 1e-12 * max(np.max(np.abs(rhs_r)), 1e-300)
 
 
-@pytest.mark.parametrize("name", ["poisson2d_p2", "elast3d_p2_traction", "darcy_tet_hp5"])
+@pytest.mark.parametrize("name", ["poisson2d_p2", "elast3d_p2_traction", "poisson3d_p2", "darcy_tet_hp5"])
 def test_full_csr_matches_oracle(name):
     """TPZSpStructMatrix format: pattern bit-exact with the oracle's
     TPZFYsmpMatrix, values within 1e-12, and the operator of the non-symmetric
