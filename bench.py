@@ -46,8 +46,10 @@ def parse():
     ap.add_argument("--no-baseline", action="store_true", help="skip the FP64 Jacobi-PCG run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle sample")
     ap.add_argument("--precond", default="", choices=["", "multigrid", "block_jacobi", "learned"],
-                    help="preconditioner of the FP32 iterations (default: learned; block_jacobi on the "
-                         "tetrahedral C3, where neither the learned smoother nor the lattice cycle applies)")
+                    help="preconditioner of the FP32 iterations (default: learned on the 3-D elasticity "
+                         "workloads C5 and C2; multigrid on the 2-D C1 and C4, where the learned smoother "
+                         "takes as many iterations as the block-Jacobi-smoothed cycle and costs more per "
+                         "cycle; block_jacobi on the tetrahedral C3, where neither applies)")
     ap.add_argument("--train-n", type=int, default=12, help="elements per direction of the training systems")
     ap.add_argument("--train-seeds", type=int, default=4)
     ap.add_argument("--train-steps", type=int, default=150)
@@ -70,7 +72,8 @@ def parse():
                          "device copy (HBM counter calibration); no solve")
     args = ap.parse_args()
     if not args.precond:
-        args.precond = "block_jacobi" if args.config == "C3" else "learned"
+        args.precond = {"C3": "block_jacobi", "C1": "multigrid", "C4p1": "multigrid",
+                        "C4p2": "multigrid"}.get(args.config, "learned")
     return args
 
 

@@ -97,7 +97,8 @@ class LearnedSmoother:
 def train_on_seeds(ctx, strmat, K, make_coef, seeds, nrhs=4, steps=100, lr=2e-3, setup_seed=0,
                    rounds=1, progress=None):
     """Trains the learned smoother on seeded coefficient fields of one mesh:
-    for each seed, pzg_mesh_set_coef + assembly + pairs of the local problems,
+    for each seed, pzg_mesh_set_coef (skipped when make_coef returns None:
+    the mesh's own coefficients) + assembly + pairs of the local problems,
     then the least-squares fit of G on every pair so far and ``steps`` Adam
     steps of the network on the seed's pairs (``rounds`` passes over the
     seeds; ``progress(k, loss)``, if given, after every 100th seed).  Returns
@@ -109,8 +110,12 @@ def train_on_seeds(ctx, strmat, K, make_coef, seeds, nrhs=4, steps=100, lr=2e-3,
     k = 0
     for _ in range(rounds):
         for s in seeds:
-            coef = np.ascontiguousarray(make_coef(s), np.float64)
-            L.check(ctx.lib.pzg_mesh_set_coef(ctx.h, L.ptr(coef, L.dp)))
+            coef = make_coef(s)
+            if coef is not None:  # (a family without per-element coefficients keeps its mesh's)
+                coef = np.ascontiguousarray(coef, np.float64)
+                if coef.shape != (ctx.sizes()["nvol"],):
+                    raise ValueError("make_coef must return one coefficient per volume element")
+                L.check(ctx.lib.pzg_mesh_set_coef(ctx.h, L.ptr(coef, L.dp)))
             strmat.Assemble(K)
             ls.collect(nrhs=nrhs, seed=1000 + s)
             if first is None:  # the loss of the initial parameters
