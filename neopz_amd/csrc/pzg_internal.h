@@ -363,7 +363,7 @@ struct Ctx {
   double mg_lmax0 = 0.;    // largest eigenvalue of the block-Jacobi scaled S
   int64_t mg_ncoarse = 0;  // unknowns of the coarsest level
   std::vector<MgLevel> mg;
-  DevBuf d_mg_vmap, d_mg_dsq, d_mg_cinv, d_mg_r0, d_mg_d0, d_mg_t0, d_mg_w0, d_mg_lz, d_mg_y0;
+  DevBuf d_mg_vmap, d_mg_vinv, d_mg_dsq, d_mg_cinv, d_mg_r0, d_mg_d0, d_mg_t0, d_mg_w0, d_mg_lz, d_mg_y0;
 
   // post-processing (pzg_post.hip): error-rule tables per rule bucket for the
   // exact-solution order post_porder (-1: not built), the elements of every

@@ -328,38 +328,39 @@ __global__ __launch_bounds__(TPB) void k_mg_finish(int64_t nb, const float *__re
 // ---- set-up kernels --------------------------------------------------------------
 // level 1 stencil from the FP64 full block CSR (values at bsr_pos): block
 // (vmap[v], vmap[w]) for the lattice neighbour w of v in direction s (zero
-// off the lattice or when the row does not hold the column)
+// off the lattice or when the row does not hold the column).  One wavefront
+// per vertex v scans its block row once: vinv maps a block to its vertex (-1
+// for the other connects); the directions found are gathered in a mask so
+// that the zero entries and the found ones are written by disjoint stores.
 template <int BS>
-__global__ void k_mg_l1(Dims D, int64_t nn, const int32_t *__restrict__ vmap,
+__global__ void k_mg_l1(Dims D, int64_t nn, const int32_t *__restrict__ vmap, const int32_t *__restrict__ vinv,
                         const int64_t *__restrict__ fbia, const int32_t *__restrict__ fbja,
                         const double *__restrict__ fa, float *__restrict__ A) {
-  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= nn * 27) return;
-  const int64_t v = x % nn;
-  const int s = (int)(x / nn);
+  constexpr int BB = BS * BS;
+  const int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (v >= nn) return;
   int i, j, k;
   node_ijk(D, v, i, j, k);
-  const int ii = i + s % 3 - 1, jj = j + (s / 3) % 3 - 1, kk = k + s / 9 - 1;
-  int64_t pos = -1, b0 = 0, nr = 0;
-  if (ii >= 0 && jj >= 0 && kk >= 0 && ii < D.n[0] && jj < D.n[1] && kk < D.n[2]) {
-    const int32_t col = vmap[node_id(D, ii, jj, kk)];
-    const int64_t row = vmap[v];
-    b0 = fbia[row];
-    nr = fbia[row + 1] - b0;
-    int64_t lo = b0, hi = b0 + nr - 1;
-    while (lo <= hi) {
-      const int64_t mid = (lo + hi) >> 1;
-      const int32_t cm = fbja[mid];
-      if (cm == col) {
-        pos = mid;
-        break;
-      }
-      if (cm < col) lo = mid + 1;
-      else hi = mid - 1;
-    }
+  const int64_t row = vmap[v];
+  const int64_t b0 = row >= 0 ? fbia[row] : 0, nr = row >= 0 ? fbia[row + 1] - b0 : 0;
+  int found = 0;
+  for (int64_t q = lane; q < nr; q += 64) {
+    const int32_t w = vinv[fbja[b0 + q]];
+    if (w < 0) continue;
+    int ii, jj, kk;
+    node_ijk(D, w, ii, jj, kk);
+    const int di = ii - i, dj = jj - j, dk = kk - k;
+    if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) continue;
+    const int sd = (di + 1) + 3 * (dj + 1) + 9 * (dk + 1);
+    found |= 1 << sd;
+    for (int e = 0; e < BB; e++) A[((int64_t)sd * BB + e) * nn + v] = (float)fa[bsr_pos<BB>(b0, nr, q, e)];
   }
-  for (int e = 0; e < BS * BS; e++)
-    A[((int64_t)s * BS * BS + e) * nn + v] = pos >= 0 ? (float)fa[bsr_pos<BS * BS>(b0, nr, pos - b0, e)] : 0.f;
+  for (int o = 32; o > 0; o >>= 1) found |= __shfl_xor(found, o, 64);
+  for (int t = lane; t < 27 * BB; t += 64) {
+    const int sd = t / BB;
+    if (!((found >> sd) & 1)) A[(int64_t)t * nn + v] = 0.f;
+  }
 }
 
 // Galerkin product A_c = P^T A_f P for trilinear P, one thread per coarse
@@ -883,6 +884,12 @@ int structure_impl(Ctx &c) {
   c.mg_dense = c.mg.back().nn * BS <= MG_MAXDENSE;
   // device arrays
   PZG_TRY(dev_upload(c.d_mg_vmap, vmap.data(), vmap.size(), c.stream));
+  {
+    std::vector<int32_t> vinv(c.nb, -1);  // block -> vertex of the level-1 lattice
+    for (size_t w = 0; w < vmap.size(); w++)
+      if (vmap[w] >= 0) vinv[vmap[w]] = (int32_t)w;
+    PZG_TRY(dev_upload(c.d_mg_vinv, vinv.data(), vinv.size(), c.stream));
+  }
   PZG_TRY(dev_alloc(c.d_mg_dsq, sizeof(float) * nn1 * BS));
   PZG_TRY(dev_alloc(c.d_mg_lz, sizeof(double) * (LZ_ZERO + 1)));
   for (DevBuf *b : {&c.d_mg_r0, &c.d_mg_d0, &c.d_mg_t0, &c.d_mg_w0}) PZG_TRY(dev_alloc(*b, sizeof(float) * c.neq));
@@ -903,9 +910,9 @@ int setup_impl(Ctx &c) {
   const int64_t nn1 = c.mg[0].nn;
   hipLaunchKernelGGL(k_mg_dsq<BS>, dim3(gridx(nn1)), dim3(TPB), 0, c.stream, nn1, c.d_mg_vmap.as<int32_t>(),
                      c.d_dinvsqrt.as<double>(), c.d_mg_dsq.as<float>());
-  hipLaunchKernelGGL(k_mg_l1<BS>, dim3(gridx(27 * nn1)), dim3(TPB), 0, c.stream, dims_of(c.mg[0]), nn1,
-                     c.d_mg_vmap.as<int32_t>(), c.d_fbia.as<int64_t>(), c.d_fbja.as<int32_t>(),
-                     c.d_fa.as<double>(), c.mg[0].A.as<float>());
+  hipLaunchKernelGGL(k_mg_l1<BS>, dim3(gridx(64 * nn1)), dim3(TPB), 0, c.stream, dims_of(c.mg[0]), nn1,
+                     c.d_mg_vmap.as<int32_t>(), c.d_mg_vinv.as<int32_t>(), c.d_fbia.as<int64_t>(),
+                     c.d_fbja.as<int32_t>(), c.d_fa.as<double>(), c.mg[0].A.as<float>());
   PZG_HIP(hipGetLastError());
   for (size_t k = 1; k < c.mg.size(); k++) {
     hipLaunchKernelGGL(k_mg_rap<BS>, dim3(gridx(27 * c.mg[k].nn)), dim3(TPB), 0, c.stream, dims_of(c.mg[k]),
@@ -1057,7 +1064,7 @@ void mg_free(Ctx &c) {
     for (DevBuf *b : {&l.A, &l.minv, &l.x, &l.b, &l.r, &l.d, &l.t}) dev_free(*b);
   c.mg.clear();
   c.mg_struct = false;
-  for (DevBuf *b : {&c.d_mg_vmap, &c.d_mg_dsq, &c.d_mg_cinv, &c.d_mg_r0, &c.d_mg_d0, &c.d_mg_t0, &c.d_mg_w0,
+  for (DevBuf *b : {&c.d_mg_vmap, &c.d_mg_vinv, &c.d_mg_dsq, &c.d_mg_cinv, &c.d_mg_r0, &c.d_mg_d0, &c.d_mg_t0, &c.d_mg_w0,
                     &c.d_mg_lz, &c.d_mg_y0})
     dev_free(*b);
   c.mg_ready = false;
