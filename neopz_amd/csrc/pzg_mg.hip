@@ -373,14 +373,15 @@ __global__ void k_mg_rap(Dims C, Dims F, const float *__restrict__ Af, float *__
   const int64_t nc = (int64_t)C.n[0] * C.n[1] * C.n[2];
   const int64_t nf = (int64_t)F.n[0] * F.n[1] * F.n[2];
   const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= nc * 27) return;
+  // one thread per (coarse node, stencil entry, block entry): the coarse
+  // levels are small, so the block entries spread the work over more threads
+  if (x >= nc * 27 * BS * BS) return;
   const int64_t v = x % nc;
-  const int s = (int)(x / nc);
+  const int s = (int)((x / nc) % 27), e = (int)(x / (nc * 27));
   int N[3];
   node_ijk(C, v, N[0], N[1], N[2]);
   const int o[3] = {s % 3 - 1, (s / 3) % 3 - 1, s / 9 - 1};
-  double acc[BS * BS];
-  for (int e = 0; e < BS * BS; e++) acc[e] = 0.;
+  double acc = 0.;
   bool inside = true;
   for (int d = 0; d < 3; d++) inside &= N[d] + o[d] >= 0 && N[d] + o[d] < C.n[d];
   if (inside) {
@@ -409,10 +410,10 @@ __global__ void k_mg_rap(Dims C, Dims F, const float *__restrict__ Af, float *__
           const int64_t f = pf[0][a] + (int64_t)F.n[0] * (pf[1][b] + (int64_t)F.n[1] * pf[2][c]);
           const int sf = (pg[2][c] + 1) * 9 + (pg[1][b] + 1) * 3 + (pg[0][a] + 1);
           const double w = (double)pw[0][a] * pw[1][b] * pw[2][c];
-          for (int e = 0; e < BS * BS; e++) acc[e] += w * Af[((int64_t)sf * BS * BS + e) * nf + f];
+          acc += w * Af[((int64_t)sf * BS * BS + e) * nf + f];
         }
   }
-  for (int e = 0; e < BS * BS; e++) Ac[((int64_t)s * BS * BS + e) * nc + v] = (float)acc[e];
+  Ac[((int64_t)s * BS * BS + e) * nc + v] = (float)acc;
 }
 
 // inverse of the diagonal block (stencil entry 13), stored [e*nn + v]
@@ -915,7 +916,7 @@ int setup_impl(Ctx &c) {
                      c.d_fbja.as<int32_t>(), c.d_fa.as<double>(), c.mg[0].A.as<float>());
   PZG_HIP(hipGetLastError());
   for (size_t k = 1; k < c.mg.size(); k++) {
-    hipLaunchKernelGGL(k_mg_rap<BS>, dim3(gridx(27 * c.mg[k].nn)), dim3(TPB), 0, c.stream, dims_of(c.mg[k]),
+    hipLaunchKernelGGL(k_mg_rap<BS>, dim3(gridx(27 * BS * BS * c.mg[k].nn)), dim3(TPB), 0, c.stream, dims_of(c.mg[k]),
                        dims_of(c.mg[k - 1]), c.mg[k - 1].A.as<float>(), c.mg[k].A.as<float>());
     PZG_HIP(hipGetLastError());
   }
