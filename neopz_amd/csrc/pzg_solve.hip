@@ -930,6 +930,7 @@ __global__ __launch_bounds__(TPB) void k_fcg_init(int64_t nb, const double *__re
       x[i] = 0.f;
       bb += bi * bi;
     }
+    if (!binv) continue;  // z and p come from the multigrid cycle
     const float *bi = binv + A * BS * BS;
     for (int k = 0; k < BS; k++) {
       float s = 0.f;
@@ -1131,8 +1132,8 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
       PZG_HIP(hipMemcpyAsync(c.d_scal.p, &hs, sizeof(hs), hipMemcpyHostToDevice, c.stream));
       hipLaunchKernelGGL(k_fcg_init<BS>, dim3(gb), dim3(TPB), 0, c.stream, nb, c.d_r.as<double>(),
                          c.d_dinvsqrt.as<double>(), c.d_xs.as<float>(), c.d_rs.as<float>(),
-                         c.d_zs.as<float>(), c.d_ps.as<float>(), c.d_bjinv.as<float>(), sc, part,
-                         cnt);
+                         c.d_zs.as<float>(), c.d_ps.as<float>(),
+                         multigrid ? (const float *)nullptr : c.d_bjinv.as<float>(), sc, part, cnt);
       PZG_HIP(hipGetLastError());
       if (pen_rows) PZG_TRY(pen_zero32(c, c.d_rs.as<float>()));
       if (multigrid)
