@@ -187,8 +187,9 @@ typedef struct {
 typedef struct {
   int32_t solver;        /* PZG_SOLVER_*                                       */
   int32_t precond;       /* PZG_PRECOND_*; PZG_SOLVER_PCG64 takes JACOBI
-                            (EJacobi) or BLOCK_JACOBI (EBlockJacobi; not with
-                            high-order simplex connects, PZG_EUNSUP)           */
+                            (EJacobi) or BLOCK_JACOBI (EBlockJacobi: each
+                            connect's equations, up to 24 on high-order
+                            tetrahedra; PZG_EUNSUP beyond or when filtered)   */
   int64_t max_iter;      /* total inner iterations allowed                     */
   double tol;            /* relative residual target                           */
   double inner_tol;      /* mixed: relative reduction per refinement step      */

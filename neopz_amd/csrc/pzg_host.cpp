@@ -60,7 +60,7 @@ static void free_all(Ctx &c) {
                     &c.d_fa, &c.d_bsr, &c.d_dinvsqrt, &c.d_bjinv, &c.d_xyz, &c.d_vnodes,
                     &c.d_vconn, &c.d_vcoef, &c.d_bnodes, &c.d_bconn, &c.d_bcid, &c.d_bctype,
                     &c.d_bcval, &c.d_bccoef, &c.d_bcvec, &c.d_aidx, &c.d_bumps, &c.d_vmap, &c.d_bmap, &c.d_tvol,
-                    &c.d_tbc, &c.d_vcolor_list, &c.d_vol_R, &c.d_blk_iptr, &c.d_blk_inc, &c.d_blk_kref, &c.d_bcolor_list, &c.d_x, &c.d_r, &c.d_z,
+                    &c.d_tbc, &c.d_vcolor_list, &c.d_vol_R, &c.d_blk_iptr, &c.d_blk_inc, &c.d_blk_kref, &c.d_vbj_pos, &c.d_vbj_m, &c.d_vbj_off, &c.d_vbj_eqb, &c.d_bcolor_list, &c.d_x, &c.d_r, &c.d_z,
                     &c.d_p, &c.d_p2, &c.d_q, &c.d_u, &c.d_f, &c.d_w, &c.d_xs, &c.d_rs,
                     &c.d_zs, &c.d_ps, &c.d_ps2, &c.d_qs, &c.d_bs, &c.d_scal, &c.d_partials,
                     &c.d_counter, &c.d_res2, &c.d_les_th, &c.d_les_m, &c.d_les_v, &c.d_les_g,
@@ -183,6 +183,34 @@ static int expand_hp(Ctx &c) {
   std::vector<int64_t> vconn, bconn;
   PZG_TRY(expand(c.vol_conn, c.vol_nsides, PZG_TET, nsv, vside, vloc, c.nvol, vconn));
   if (c.nbc > 0) PZG_TRY(expand(c.bc_conn, c.bc_nsides, PZG_TRI, nsb, bside, bloc, c.nbc, bconn));
+  // the real connects as the blocks of the connect-block Jacobi
+  {
+    std::vector<int64_t> bp, bo;
+    std::vector<int32_t> bmv, eqb(c.neq, -1);
+    int64_t off = 0;
+    bool ok = true;
+    for (int64_t s = 0; s < c.nconn; s++) {
+      const int m = c.blk_size[s];
+      if (m <= 0) continue;
+      if (m > VBJ_MAX || c.blk_pos[s] + m > c.neq) { ok = false; break; }
+      for (int k = 0; k < m; k++) eqb[c.blk_pos[s] + k] = (int32_t)bp.size();
+      bp.push_back(c.blk_pos[s]);
+      bmv.push_back(m);
+      bo.push_back(off);
+      off += (int64_t)m * m;
+    }
+    for (int64_t i = 0; ok && i < c.neq; i++) ok = eqb[i] >= 0;
+    c.vbj_nblk = 0;
+    if (ok && !bp.empty()) {
+      hipStream_t st = c.stream;
+      PZG_TRY(dev_upload(c.d_vbj_pos, bp.data(), bp.size(), st));
+      PZG_TRY(dev_upload(c.d_vbj_m, bmv.data(), bmv.size(), st));
+      PZG_TRY(dev_upload(c.d_vbj_off, bo.data(), bo.size(), st));
+      PZG_TRY(dev_upload(c.d_vbj_eqb, eqb.data(), eqb.size(), st));
+      c.vbj_nblk = (int64_t)bp.size();
+      c.vbj_ninv = off;
+    }
+  }
   std::vector<int64_t> pos(nvirt + 1);
   std::vector<int32_t> size(nvirt + 1, ns);
   for (int64_t s = 0; s < c.nconn; s++)

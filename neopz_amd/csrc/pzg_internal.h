@@ -144,6 +144,7 @@ __host__ __device__ inline int hp_variant(bool tet, int s, const int64_t *id) {
 // Position of value k (row-major inside the block) of the i-th block of a
 // row whose blocks are [b0, b0 + n).
 constexpr int BSR_CHUNK = 16;
+constexpr int VBJ_MAX = 24;  // largest connect block of the high-order block Jacobi
 template <int BB>
 __host__ __device__ inline int64_t bsr_pos(int64_t b0, int64_t n, int64_t i, int k) {
   const int64_t j = i - i % BSR_CHUNK;
@@ -351,6 +352,10 @@ struct Ctx {
   // sets of the penalized boundary elements with their inverses, the row ->
   // slot gather, CG work vectors and scalars
   bool pen_ready = false, pen_struct = false;  // values of this assembly; structure of this mesh
+  // connect blocks of high-order meshes (consecutive equations of one real
+  // connect, at most VBJ_MAX): starts, sizes, inverse offsets, equation -> block
+  int64_t vbj_nblk = 0, vbj_ninv = 0;
+  DevBuf d_vbj_pos, d_vbj_m, d_vbj_off, d_vbj_eqb;
   int64_t pen_n = 0, pen_nnz = 0, pen_nset = 0, pen_nslot = 0, pen_narow = 0;
   DevBuf d_pen_arow, d_pen_delta;
   DevBuf d_pen_idx, d_pen_pmap, d_pen_ia, d_pen_ja, d_pen_a, d_pen_sptr, d_pen_sidx, d_pen_sset, d_pen_soff,

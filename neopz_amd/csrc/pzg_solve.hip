@@ -315,6 +315,90 @@ __global__ void k_bj_inv64(int64_t nb, const int64_t *__restrict__ fbia, const i
   }
 }
 
+// Connect-block Jacobi on high-order meshes: a connect of several shapes
+// is one block of its consecutive equations [pos, pos + m), m <= VBJ_MAX.
+// Inverse: one wavefront per block, the block (from the upper CSR, whose
+// row i starts with the columns i, i+1, ... of its own connect) and the
+// identity in LDS, Gauss-Jordan with lane r holding row r; a missing entry
+// or a zero pivot sets *fail.
+__global__ __launch_bounds__(256) void k_vbj_inv64(int64_t nblk, const int64_t *__restrict__ bpos,
+                                                   const int32_t *__restrict__ bm,
+                                                   const int64_t *__restrict__ boff,
+                                                   const int64_t *__restrict__ ia,
+                                                   const int64_t *__restrict__ ja,
+                                                   const double *__restrict__ a, double *__restrict__ binv,
+                                                   int *fail) {
+  __shared__ double sA[4][VBJ_MAX][VBJ_MAX], sI[4][VBJ_MAX][VBJ_MAX];
+  const int w = threadIdx.x >> 6, r = threadIdx.x & 63;
+  const int64_t b = (int64_t)blockIdx.x * 4 + w;
+  if (b >= nblk) return;  // whole wavefronts
+  const int64_t p = bpos[b];
+  const int m = bm[b];
+  auto sync = [] {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  };
+  if (r < m)
+    for (int c = 0; c < m; c++) {
+      const int lo = r < c ? r : c, hi = r < c ? c : r;
+      const int64_t q = ia[p + lo] + (hi - lo);
+      if (q >= ia[p + lo + 1] || ja[q] != p + hi) atomicExch(fail, 1);
+      sA[w][r][c] = q < ia[p + lo + 1] ? a[q] : 0.;
+      sI[w][r][c] = r == c ? 1. : 0.;
+    }
+  sync();
+  for (int k = 0; k < m; k++) {
+    const double piv = sA[w][k][k];
+    if (piv == 0.) {
+      if (r == 0) atomicExch(fail, 1);
+      return;
+    }
+    sync();
+    if (r == k)
+      for (int c = 0; c < m; c++) {
+        sA[w][k][c] /= piv;
+        sI[w][k][c] /= piv;
+      }
+    sync();
+    if (r < m && r != k) {
+      const double f = sA[w][r][k];
+      for (int c = 0; c < m; c++) {
+        sA[w][r][c] -= f * sA[w][k][c];
+        sI[w][r][c] -= f * sI[w][k][c];
+      }
+    }
+    sync();
+  }
+  if (r < m)
+    for (int c = 0; c < m; c++) binv[boff[b] + (int64_t)r * m + c] = sI[w][r][c];
+}
+
+// z = Binv r on the blocks (one thread per equation), r.z into partials
+// slot 1 (same grid as k_cg_update64)
+__global__ __launch_bounds__(TPB) void k_vbj_apply64(int64_t n, const int32_t *__restrict__ eqb,
+                                                     const int64_t *__restrict__ bpos,
+                                                     const int32_t *__restrict__ bm,
+                                                     const int64_t *__restrict__ boff,
+                                                     const double *__restrict__ binv,
+                                                     const double *__restrict__ r, double *__restrict__ z,
+                                                     const SolveScalars *sc, double *partials) {
+  if (sc && sc->done) return;
+  double rz = 0.;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t b = eqb[i];
+    const int64_t p = bpos[b];
+    const int m = bm[b];
+    const double *row = binv + boff[b] + (i - p) * m;
+    double s = 0.;
+    for (int c = 0; c < m; c++) s += row[c] * r[p + c];
+    z[i] = s;
+    rz += r[i] * s;
+  }
+  __shared__ double sm[TPB / 64];
+  const double t = block_sum(rz, sm);
+  if (threadIdx.x == 0) partials[1 * MAXBLK + blockIdx.x] = t;
+}
+
 // resid = |r|/|b|; stop tests; beta = (r.z)/rho   (one block)
 __global__ void k_fin_cg64(SolveScalars *sc, const double *partials, int nblk, double *hist) {
   if (sc->done) return;
@@ -704,10 +788,34 @@ int solve_pcg64(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info) {
     set_error("pzg_solve: PCG64 takes the point or the connect-block Jacobi preconditioner");
     return PZG_EINVAL;
   }
-  if (block) {
-    if (c.hp_simplex || c.external_csr) {
-      set_error(c.hp_simplex ? "pzg_solve: connect-block Jacobi with high-order simplex connects not built"
-                             : "pzg_solve: connect-block Jacobi needs the mesh's connects");
+  // high-order meshes: blocks of the real connects (several shapes each)
+  const bool vblock = block && c.hp_simplex;
+  if (vblock) {
+    if (c.filt) {
+      set_error("pzg_solve: connect-block Jacobi of a filtered high-order system not built");
+      return PZG_EUNSUP;
+    }
+    if (c.vbj_nblk == 0) {
+      set_error("pzg_solve: connect blocks of the high-order mesh not recorded");
+      return PZG_ESTATE;
+    }
+    PZG_TRY(dev_alloc(c.d_bjinv64, sizeof(double) * std::max<int64_t>(c.vbj_ninv, 1)));
+    int *fail = reinterpret_cast<int *>(c.d_counter.as<unsigned>() + 8);
+    PZG_HIP(hipMemsetAsync(fail, 0, sizeof(int), c.stream));
+    hipLaunchKernelGGL(k_vbj_inv64, dim3((unsigned)((c.vbj_nblk + 3) / 4)), dim3(256), 0, c.stream, c.vbj_nblk,
+                       c.d_vbj_pos.as<int64_t>(), c.d_vbj_m.as<int32_t>(), c.d_vbj_off.as<int64_t>(),
+                       c.d_ia.as<int64_t>(), c.d_ja.as<int64_t>(), c.d_a.as<double>(), c.d_bjinv64.as<double>(),
+                       fail);
+    int hf = 0;
+    PZG_HIP(hipMemcpyAsync(&hf, fail, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    PZG_HIP(hipStreamSynchronize(c.stream));
+    if (hf) {
+      set_error("pzg_solve: a connect block of K is singular or not stored whole");
+      return PZG_EUNSUP;
+    }
+  } else if (block) {
+    if (c.external_csr) {
+      set_error("pzg_solve: connect-block Jacobi needs the mesh's connects");
       return PZG_EUNSUP;
     }
     PZG_TRY(dev_alloc(c.d_bjinv64, sizeof(double) * c.nb * c.bs * c.bs));
@@ -728,7 +836,16 @@ int solve_pcg64(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info) {
   hipLaunchKernelGGL(k_cg_init64, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_f.as<double>(),
                      c.d_u.as<double>(), c.d_r.as<double>(), c.d_z.as<double>(),
                      c.d_p.as<double>(), dg, sc, part, cnt);
-  if (block) {
+  auto vbj_apply = [&](const SolveScalars *s2) {
+    hipLaunchKernelGGL(k_vbj_apply64, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_vbj_eqb.as<int32_t>(),
+                       c.d_vbj_pos.as<int64_t>(), c.d_vbj_m.as<int32_t>(), c.d_vbj_off.as<int64_t>(),
+                       (const double *)c.d_bjinv64.as<double>(), (const double *)c.d_r.as<double>(),
+                       c.d_z.as<double>(), s2, part);
+  };
+  if (vblock) {
+    vbj_apply(nullptr);
+    hipLaunchKernelGGL(k_bj_rho64, dim3(1), dim3(TPB), 0, c.stream, sc, (const double *)part, gv);
+  } else if (block) {
     DISPATCH_BS(c.bs, hipLaunchKernelGGL(k_bj_apply64<BS>, dim3(gv), dim3(TPB), 0, c.stream, c.nb,
                                          c.d_bjinv64.as<double>(), c.d_r.as<double>(), c.d_z.as<double>(),
                                          (const SolveScalars *)nullptr, part));
@@ -748,7 +865,9 @@ int solve_pcg64(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info) {
                        c.d_r.as<double>(), c.d_z.as<double>(), c.d_p.as<double>(),
                        c.d_q.as<double>(), dg, sc, (double *)nullptr, part,
                        cnt);
-    if (block)
+    if (vblock)
+      vbj_apply(sc);
+    else if (block)
       DISPATCH_BS(c.bs, hipLaunchKernelGGL(k_bj_apply64<BS>, dim3(gv), dim3(TPB), 0, c.stream, c.nb,
                                            c.d_bjinv64.as<double>(), c.d_r.as<double>(), c.d_z.as<double>(),
                                            (const SolveScalars *)sc, part));
