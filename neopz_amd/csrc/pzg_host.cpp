@@ -60,7 +60,7 @@ static void free_all(Ctx &c) {
                     &c.d_fa, &c.d_bsr, &c.d_dinvsqrt, &c.d_bjinv, &c.d_xyz, &c.d_vnodes,
                     &c.d_vconn, &c.d_vcoef, &c.d_bnodes, &c.d_bconn, &c.d_bcid, &c.d_bctype,
                     &c.d_bcval, &c.d_bccoef, &c.d_bcvec, &c.d_aidx, &c.d_bumps, &c.d_vmap, &c.d_bmap, &c.d_tvol,
-                    &c.d_tbc, &c.d_vcolor_list, &c.d_vol_R, &c.d_blk_iptr, &c.d_blk_inc, &c.d_blk_kref, &c.d_vbj_pos, &c.d_vbj_m, &c.d_vbj_off, &c.d_vbj_eqb, &c.d_bcolor_list, &c.d_x, &c.d_r, &c.d_z,
+                    &c.d_tbc, &c.d_vcolor_list, &c.d_vol_R, &c.d_blk_iptr, &c.d_blk_inc, &c.d_blk_kref, &c.d_vbj_pos, &c.d_vbj_m, &c.d_vbj_off, &c.d_vbj_eqb, &c.d_vbjinv64, &c.d_vbjinv32, &c.d_bcolor_list, &c.d_x, &c.d_r, &c.d_z,
                     &c.d_p, &c.d_p2, &c.d_q, &c.d_u, &c.d_f, &c.d_w, &c.d_xs, &c.d_rs,
                     &c.d_zs, &c.d_ps, &c.d_ps2, &c.d_qs, &c.d_bs, &c.d_scal, &c.d_partials,
                     &c.d_counter, &c.d_res2, &c.d_les_th, &c.d_les_m, &c.d_les_v, &c.d_les_g,

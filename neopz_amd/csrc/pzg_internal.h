@@ -355,7 +355,7 @@ struct Ctx {
   // connect blocks of high-order meshes (consecutive equations of one real
   // connect, at most VBJ_MAX): starts, sizes, inverse offsets, equation -> block
   int64_t vbj_nblk = 0, vbj_ninv = 0;
-  DevBuf d_vbj_pos, d_vbj_m, d_vbj_off, d_vbj_eqb;
+  DevBuf d_vbj_pos, d_vbj_m, d_vbj_off, d_vbj_eqb, d_vbjinv64, d_vbjinv32;
   int64_t pen_n = 0, pen_nnz = 0, pen_nset = 0, pen_nslot = 0, pen_narow = 0;
   DevBuf d_pen_arow, d_pen_delta;
   DevBuf d_pen_idx, d_pen_pmap, d_pen_ia, d_pen_ja, d_pen_a, d_pen_sptr, d_pen_sidx, d_pen_sset, d_pen_soff,

@@ -399,6 +399,49 @@ __global__ __launch_bounds__(TPB) void k_vbj_apply64(int64_t n, const int32_t *_
   if (threadIdx.x == 0) partials[1 * MAXBLK + blockIdx.x] = t;
 }
 
+// The connect blocks of the scaled FP32 operator S = D^-1/2 K D^-1/2:
+// S_b^-1 = D^1/2 K_b^-1 D^1/2, from the FP64 inverses (one thread per row)
+__global__ void k_vbj_scale32(int64_t n, const int32_t *__restrict__ eqb, const int64_t *__restrict__ bpos,
+                              const int32_t *__restrict__ bm, const int64_t *__restrict__ boff,
+                              const double *__restrict__ inv64, const double *__restrict__ dis,
+                              float *__restrict__ inv32) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t b = eqb[i];
+    const int64_t p = bpos[b], o = boff[b] + (i - p) * bm[b];
+    for (int c = 0; c < bm[b]; c++) inv32[o + c] = (float)(inv64[o + c] / (dis[i] * dis[p + c]));
+  }
+}
+
+// FP32 FCG with the connect blocks: z = S_b^-1 r (one thread per equation);
+// p = z at a restart; z.q replaces slot 1 of the FCG update's partials
+// (launched on the same grid)
+__global__ __launch_bounds__(TPB) void k_vbj_apply32(int64_t n, const int32_t *__restrict__ eqb,
+                                                     const int64_t *__restrict__ bpos,
+                                                     const int32_t *__restrict__ bm,
+                                                     const int64_t *__restrict__ boff,
+                                                     const float *__restrict__ inv, const float *__restrict__ r,
+                                                     float *__restrict__ z, float *__restrict__ p,
+                                                     const float *__restrict__ q, const SolveScalars *sc,
+                                                     double *partials) {
+  if (sc && sc->done) return;
+  double zq = 0.;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t b = eqb[i];
+    const int64_t s0 = bpos[b];
+    const int m = bm[b];
+    const float *row = inv + boff[b] + (i - s0) * m;
+    float v = 0.f;
+    for (int c = 0; c < m; c++) v += row[c] * r[s0 + c];
+    z[i] = v;
+    if (p) p[i] = v;
+    if (q) zq += (double)v * q[i];
+  }
+  if (!q) return;
+  __shared__ double sm[TPB / 64];
+  const double t = block_sum(zq, sm);
+  if (threadIdx.x == 0) partials[1 * MAXBLK + blockIdx.x] = t;
+}
+
 // resid = |r|/|b|; stop tests; beta = (r.z)/rho   (one block)
 __global__ void k_fin_cg64(SolveScalars *sc, const double *partials, int nblk, double *hist) {
   if (sc->done) return;
@@ -1181,6 +1224,36 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
   };
   PZG_TRY(true_res(true));
   if (rel <= prm.tol) conv = true;
+  // block Jacobi on a high-order mesh: the real connects' blocks (several
+  // shapes each), inverted in FP64 and scaled into the FP32 system
+  const bool vbj = prm.precond == PZG_PRECOND_BLOCK_JACOBI && c.hp_simplex && c.vbj_nblk > 0 && !c.filt;
+  if (vbj && !conv) {
+    PZG_TRY(dev_alloc(c.d_vbjinv64, sizeof(double) * std::max<int64_t>(c.vbj_ninv, 1)));
+    PZG_TRY(dev_alloc(c.d_vbjinv32, sizeof(float) * std::max<int64_t>(c.vbj_ninv, 1)));
+    int *fail = reinterpret_cast<int *>(c.d_counter.as<unsigned>() + 8);
+    PZG_HIP(hipMemsetAsync(fail, 0, sizeof(int), c.stream));
+    hipLaunchKernelGGL(k_vbj_inv64, dim3((unsigned)((c.vbj_nblk + 3) / 4)), dim3(256), 0, c.stream, c.vbj_nblk,
+                       c.d_vbj_pos.as<int64_t>(), c.d_vbj_m.as<int32_t>(), c.d_vbj_off.as<int64_t>(),
+                       c.d_ia.as<int64_t>(), c.d_ja.as<int64_t>(), c.d_a.as<double>(), c.d_vbjinv64.as<double>(),
+                       fail);
+    hipLaunchKernelGGL(k_vbj_scale32, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_vbj_eqb.as<int32_t>(),
+                       c.d_vbj_pos.as<int64_t>(), c.d_vbj_m.as<int32_t>(), c.d_vbj_off.as<int64_t>(),
+                       (const double *)c.d_vbjinv64.as<double>(), c.d_dinvsqrt.as<double>(),
+                       c.d_vbjinv32.as<float>());
+    int hf = 0;
+    PZG_HIP(hipMemcpyAsync(&hf, fail, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    PZG_HIP(hipStreamSynchronize(c.stream));
+    if (hf) {
+      set_error("pzg_solve: a connect block of K is singular or not stored whole");
+      return PZG_EUNSUP;
+    }
+  }
+  auto vbj_apply = [&](float *z, float *p, const float *q) {
+    hipLaunchKernelGGL(k_vbj_apply32, dim3(gb), dim3(TPB), 0, c.stream, n, c.d_vbj_eqb.as<int32_t>(),
+                       c.d_vbj_pos.as<int64_t>(), c.d_vbj_m.as<int32_t>(), c.d_vbj_off.as<int64_t>(),
+                       (const float *)c.d_vbjinv32.as<float>(), (const float *)c.d_rs.as<float>(), z, p, q,
+                       (const SolveScalars *)sc, part);
+  };
   if (multigrid && !conv) {
     // levels and eigenvalue estimates (timed with the solve)
     const bool les = prm.precond == PZG_PRECOND_LEARNED;
@@ -1257,6 +1330,7 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
       if (pen_rows) PZG_TRY(pen_zero32(c, c.d_rs.as<float>()));
       if (multigrid)
         PZG_TRY(mg_apply(c, c.d_rs.as<float>(), c.d_zs.as<float>(), c.d_ps.as<float>(), nullptr, sc, part));
+      if (vbj) vbj_apply(c.d_zs.as<float>(), c.d_ps.as<float>(), nullptr);
       if (use_mf) PZG_TRY(mf_scale(c, c.d_ps.as<float>(), sc));
       restart = false;
     }
@@ -1280,6 +1354,7 @@ static int mixed_impl(Ctx &c, const pzg_solve_params &prm, pzg_solve_info &info)
       if (pen_rows) (void)pen_zero32(c, c.d_rs.as<float>());
       if (multigrid)
         (void)mg_apply(c, c.d_rs.as<float>(), c.d_zs.as<float>(), nullptr, c.d_qs.as<float>(), sc, part);
+      if (vbj) vbj_apply(c.d_zs.as<float>(), nullptr, c.d_qs.as<float>());
       hipLaunchKernelGGL(k_fin_fcg_beta, dim3(1), dim3(TPB), 0, c.stream, sc, (const double *)part, gb);
       hipLaunchKernelGGL(k_p_update32, dim3(gv), dim3(TPB), 0, c.stream, n, c.d_zs.as<float>(),
                          c.d_ps.as<float>(), sc, use_mf ? c.d_mf_dis.as<float>() : nullptr,
